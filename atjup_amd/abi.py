@@ -1,0 +1,111 @@
+"""ctypes mirror of include/atjup_hip.h: field ids, the three context structs and the knob profiles.
+
+A profile is the set of reference environment knobs a configuration runs under, expressed twice:
+as the ``atjup_knobs`` struct the library takes and as the environment string the reference would
+read (``ATJUP_*`` variables, planet/cJupiterModel.h:725-1216 and the shared headers).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import inputs
+
+# ---- field ids (enum atjup_field) ----------------------------------------------------------
+PROG = ("t", "u", "v", "w", "h2o", "h2o_cloud", "h2o_ice", "h2s", "nh3", "nh3_cloud", "nh3_ice",
+        "ch4", "ch4_cloud", "ch4_ice", "nh4sh", "tke", "dis")
+N_PROG = len(PROG)
+INPUTS = ("p_dyn", "p_stat", "p_hydro", "rho_mix", "wall_nue", "nue", "prod", "tke_source",
+          "dis_source", "aux_u", "aux_v", "aux_w", "Q_rad", "Q_precip",
+          "S_precip_h2o", "S_precip_h2o_cloud", "S_precip_h2o_ice",
+          "S_precip_nh3", "S_precip_nh3_cloud", "S_precip_nh3_ice",
+          "S_precip_ch4", "S_precip_ch4_cloud", "S_precip_ch4_ice",
+          "massflux_h2s", "massflux_nh3", "massflux_nh4sh", "fluxlim_nh4sh")
+NLEVEL = 64
+
+FIELD = {n: i for i, n in enumerate(PROG)}
+FIELD.update({n: 20 + i for i, n in enumerate(INPUTS)})
+FIELD.update({n + "n": NLEVEL + i for i, n in enumerate(PROG)})   # tn, un, ..., disn
+F_END = 20 + len(INPUTS)
+
+OK, ERR_ARG, ERR_HIP, ERR_PROFILE, ERR_NOMEM, ERR_STATE, NONFINITE = 0, -1, -2, -3, -4, -5, 1
+PRECISION_FP64, PRECISION_FP32_REF = 0, 1
+
+
+class Grid(C.Structure):
+    _fields_ = [("im", C.c_int), ("jm", C.c_int), ("km", C.c_int),
+                ("r0", C.c_double), ("dr", C.c_double), ("dthe", C.c_double), ("dphi", C.c_double),
+                ("L_atm", C.c_double)]
+
+
+class Consts(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "dt", "re", "pr", "g", "omega", "u_0", "t_ref",
+        "Coriolis", "centrifugal", "buoyancy", "chemical_reaction",
+        "r_mix", "R_mix", "cp_mix", "mue_mix",
+        "sc_h2o", "sc_h2s", "sc_nh3", "sc_nh4sh", "sc_ch4",
+        "rg_nh4sh", "r_p_nh4sh", "re_turb")]
+
+
+class Knobs(C.Structure):
+    _fields_ = [("turb_model", C.c_int), ("press_sweeps", C.c_int), ("press_omega", C.c_double),
+                ("pdyn_cap", C.c_double), ("damp_pdyn", C.c_int), ("rho_pdyn", C.c_double),
+                ("sinthe_min", C.c_double), ("t_diff_scale", C.c_double),
+                ("species_diff_scale", C.c_double), ("rad_coupling", C.c_double),
+                ("precip_coupling", C.c_double), ("turb_coupling", C.c_double),
+                ("tke_max_phys", C.c_double), ("nue_max_phys", C.c_double),
+                ("wall_nue_factor", C.c_double), ("wall_nue_layers", C.c_int),
+                ("satadj", C.c_int), ("rk_scalar_sync", C.c_int), ("dry", C.c_int),
+                ("turb_k_bg_phys", C.c_double)]
+
+
+# Reference environment shared by every profile: the balanced start, vortex, moist convective
+# adjustment, cone projection, continuity init and NH4SH equilibrium are off (they are outside this
+# path), the metric radius is 71492 km, dt 0.005.
+BASE_ENV = {
+    "ATJUP_GEO_BALANCE": "0", "ATJUP_VORTEX": "0", "ATJUP_MCA": "0",
+    "ATJUP_V_MASSBAL_STRIDE": "0", "ATJUP_CONE_PROJ": "0", "ATJUP_CONE_PROJ_STRIDE": "0",
+    "ATJUP_U_INIT_CONT": "0", "ATJUP_NH4SH_EQ": "0", "ATJUP_METRIC_RADIUS": "71492",
+    "ATJUP_DT": "0.005", "ATJUP_BC_PHI_PERIODIC": "1",
+}
+
+TURB_MODELS = {"off": -1, "k_epsilon": 0, "k_omega": 1, "k_omega_SST": 2}
+
+
+def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
+                 grid: inputs.Grid | None = None):
+    """(Grid, Consts, Knobs, env) of a configuration.
+
+    "dyn"  : dynamics only, dry, closure/radiation/precipitation off (the 5-step benchmark case).
+    "full" : k-omega closure, moist, saturation adjustment on, radiation/precipitation as frozen
+             inputs (their modules are outside this path).
+    """
+    g = grid or inputs.Grid()
+    G = Grid(im=g.im, jm=g.jm, km=g.km, r0=g.radius_km / g.L_atm, dr=g.dr, dthe=g.dthe,
+             dphi=g.dphi, L_atm=g.L_atm)
+    Cn = Consts(dt=0.005, re=1000.0, pr=0.69, g=25.92, omega=0.000176, u_0=100.0, t_ref=165.0,
+                Coriolis=1.0, centrifugal=1.0, buoyancy=1.0,
+                chemical_reaction=0.0 if config == "dyn" else 1.0,
+                r_mix=inputs.R_MIX_DENSITY, R_mix=inputs.R_MIX, cp_mix=inputs.CP_MIX,
+                mue_mix=inputs.MUE_MIX, sc_h2o=0.61, sc_h2s=0.94, sc_nh3=0.61, sc_nh4sh=0.7,
+                sc_ch4=0.99, rg_nh4sh=1170.0, r_p_nh4sh=1.0e-5, re_turb=1.0)
+    turb = "off" if config == "dyn" else "k_omega"
+    K = Knobs(turb_model=TURB_MODELS[turb], press_sweeps=sweeps, press_omega=omega,
+              pdyn_cap=0.2, damp_pdyn=1, rho_pdyn=1.0, sinthe_min=0.55, t_diff_scale=0.01,
+              species_diff_scale=0.01, rad_coupling=1.0, precip_coupling=1.0, turb_coupling=1.0,
+              tke_max_phys=1000.0, nue_max_phys=1.0e5, wall_nue_factor=4.0, wall_nue_layers=3,
+              satadj=1, rk_scalar_sync=2, dry=1 if config == "dyn" else 0,
+              turb_k_bg_phys=1.0e-4 if config != "dyn" else 1.0)
+    env = dict(BASE_ENV)
+    env["ATJUP_PRESS_SWEEPS"] = str(sweeps)
+    env["ATJUP_RADIATION"] = "0"
+    env["ATJUP_PRECIP"] = "0"
+    if config == "dyn":
+        env["ATJUP_TURB"] = "0"
+    else:
+        env["ATJUP_TURB_MODEL"] = "k_omega"
+        env["ATJUP_TURB_K_BG"] = "1e-4"
+    return G, Cn, K, env
+
+
+def env_string(env: dict) -> str:
+    return ";".join(f"{k}={v}" for k, v in env.items())

@@ -1,0 +1,182 @@
+/*
+ * atjup_hip.h - C ABI of the MI355X (gfx950) implementation of the ATJUP time step.
+ *
+ * The reference has no FFI for this path: its entry points are C++ member functions of
+ * cJupiterModel and class templates instantiated on it (planet/cJupiterModel.h:671-672,
+ * planet/PressureSolverJup.h, planet/SaturationAdjustmentJup.h, planet/TurbulenceJup.h,
+ * planet/BC_Jup.h), all acting on model-owned Array fields in place. This header is the drop-in
+ * boundary that replaces them: one opaque context that owns the device-resident fields, transfer
+ * calls that accept the reference's Array row layout unchanged, and one operator per reference
+ * entry point with the reference's semantics. Each declaration names the reference code it
+ * replaces.
+ *
+ * Conventions
+ *   - Every call returns int: 0 = ok, < 0 = error (atjup_hip_last_error_string() says why),
+ *     ATJUP_NONFINITE (> 0) = the state holds a non-finite value.
+ *   - No C++ exceptions cross this boundary.
+ *   - One context per HIP stream; calls come from one host thread. State stays on the device
+ *     between calls and is packed only by the download calls.
+ *   - Field layout on the host: [i][j][k], k (phi) fastest, im*jm*km doubles, the reference's
+ *     Array order (lib/Array.h:25-31). The _rows variants take the reference's double*** x.
+ */
+#ifndef ATJUP_HIP_H
+#define ATJUP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATJUP_OK             0
+#define ATJUP_ERR_ARG       -1
+#define ATJUP_ERR_HIP       -2
+#define ATJUP_ERR_PROFILE   -3
+#define ATJUP_ERR_NOMEM     -4
+#define ATJUP_ERR_STATE     -5
+#define ATJUP_NONFINITE      1
+
+#define ATJUP_PRECISION_FP64      0   /* the drop-in: every field and stencil in FP64 */
+#define ATJUP_PRECISION_FP32_REF  1   /* fields and RK4 in FP32; Poisson in FP32 inside FP64 refinement */
+
+/* Field identifiers. 0..16 are the 17 prognostic fields the RK4 integrates
+ * (planet/RungeKutta_Jup_Turb.cpp:425-486); ATJUP_NLEVEL + id is the time-level-n copy of the same
+ * field (tn, un, ... in planet/cJupiterModel.h:1383-1398, 1513-1516). */
+enum atjup_field {
+    ATJUP_F_T = 0, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W,
+    ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE, ATJUP_F_H2S,
+    ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE,
+    ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_NH4SH,
+    ATJUP_F_TKE, ATJUP_F_DIS,
+    ATJUP_N_PROG,                                   /* 17 */
+    /* inputs and diagnostics the path reads or writes */
+    ATJUP_F_P_DYN = 20, ATJUP_F_P_STAT, ATJUP_F_P_HYDRO, ATJUP_F_RHO_MIX, ATJUP_F_WALL_NUE,
+    ATJUP_F_NUE, ATJUP_F_PROD, ATJUP_F_TKE_SOURCE, ATJUP_F_DIS_SOURCE,
+    ATJUP_F_AUX_U, ATJUP_F_AUX_V, ATJUP_F_AUX_W,
+    ATJUP_F_Q_RAD, ATJUP_F_Q_PRECIP,
+    ATJUP_F_S_PRECIP_H2O, ATJUP_F_S_PRECIP_H2O_CLOUD, ATJUP_F_S_PRECIP_H2O_ICE,
+    ATJUP_F_S_PRECIP_NH3, ATJUP_F_S_PRECIP_NH3_CLOUD, ATJUP_F_S_PRECIP_NH3_ICE,
+    ATJUP_F_S_PRECIP_CH4, ATJUP_F_S_PRECIP_CH4_CLOUD, ATJUP_F_S_PRECIP_CH4_ICE,
+    ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH,
+    ATJUP_F_END,
+    ATJUP_NLEVEL = 64                               /* ATJUP_NLEVEL + prognostic id */
+};
+
+/* The grid. rad.z and the.z are cumulative sums from r0 and 0 (lib/Array_1D.cpp:49-60). */
+typedef struct atjup_grid {
+    int im, jm, km;          /* 41, 181, 361 for the reference (planet/cJupiterModel.h:166) */
+    double r0;               /* rad.z[0] = metric radius / L_atm (planet/cJupiterModel.cpp:441-468) */
+    double dr, dthe, dphi;   /* 0.025, 1 deg, 1 deg (planet/cJupiterModel.cpp:160-177) */
+    double L_atm;            /* shell depth [km] */
+} atjup_grid;
+
+/* Physical constants; the reference's config members and mixture constants. */
+typedef struct atjup_consts {
+    double dt, re, pr, g, omega, u_0, t_ref;
+    double Coriolis, centrifugal, buoyancy, chemical_reaction;
+    double r_mix, R_mix, cp_mix, mue_mix;
+    double sc_h2o, sc_h2s, sc_nh3, sc_nh4sh, sc_ch4;
+    double rg_nh4sh, r_p_nh4sh, re_turb;
+} atjup_consts;
+
+/* The knob profile: the reference's environment knobs that select branches on this path, with the
+ * values they take in the supported profile. ctx_create rejects a profile it does not implement. */
+typedef struct atjup_knobs {
+    int    turb_model;          /* ATJUP_TURB/_TURB_MODEL: -1 off, 0 k-epsilon, 1 k-omega, 2 SST */
+    int    press_sweeps;        /* ATJUP_PRESS_SWEEPS (PressureSolver.h:174) */
+    double press_omega;         /* over-relaxation of the red-black sweep; 1 = the reference's GS */
+    double pdyn_cap;            /* ATJUP_PDYN_CAP 0.2 (PressureSolver.h:168) */
+    int    damp_pdyn;           /* ATJUP_DAMP_PDYN, default !GEO_BALANCE (cJupiterModel.cpp:1040) */
+    double rho_pdyn;            /* ATJUP_RHO_PDYN weight, default !GEO_BALANCE (InitValues_Jup.cpp:722) */
+    double sinthe_min;          /* ATJUP_SINTHE_MIN 0.55 (cJupiterModel.h:1121-1128) */
+    double t_diff_scale;        /* ATJUP_T_DIFF_SCALE 0.01 */
+    double species_diff_scale;  /* ATJUP_SPECIES_DIFF_SCALE 0.01 */
+    double rad_coupling;        /* ATJUP_RAD_COUPLING 1 */
+    double precip_coupling;     /* ATJUP_PRECIP_COUPLING 1 */
+    double turb_coupling;       /* ATJUP_TURB_COUPLING 1 */
+    double tke_max_phys;        /* ATJUP_TKE_MAX 1000 m2/s2 */
+    double nue_max_phys;        /* ATJUP_NUE_MAX 1e5 m2/s */
+    double wall_nue_factor;     /* ATJUP_WALL_NUE 4 */
+    int    wall_nue_layers;     /* ATJUP_WALL_NUE_LAYERS 3 */
+    int    satadj;              /* run the saturation adjustment in the physics block */
+    int    rk_scalar_sync;      /* ATJUP_RK_SCALAR_SYNC 2 */
+    int    dry;                 /* species identically zero with zero sources: they are not streamed */
+    double turb_k_bg_phys;      /* ATJUP_TURB_K_BG [m2/s2] */
+} atjup_knobs;
+
+typedef struct atjup_ctx atjup_ctx;
+
+/* ---- context ---------------------------------------------------------------------------- */
+/* Replaces the state cJupiterModel owns (resetArrays, planet/cJupiterModel.cpp:1570-1753) and the
+ * start-up part of Run() that fixes geometry and constants (planet/cJupiterModel.cpp:316-745).
+ * precision: ATJUP_PRECISION_FP64 or ATJUP_PRECISION_FP32_REF. Returns NULL on error. */
+atjup_ctx* atjup_hip_ctx_create(const atjup_grid* grid, const atjup_consts* consts,
+                                const atjup_knobs* knobs, int precision);
+void       atjup_hip_ctx_destroy(atjup_ctx* ctx);
+/* The default knob profile (the reference's defaults with GEO_BALANCE, VORTEX, MCA off). */
+void       atjup_hip_default_knobs(atjup_knobs* knobs);
+
+/* ---- field transfer --------------------------------------------------------------------- */
+/* Contiguous [i][j][k] doubles. The FP32 form converts on the way in and out. */
+int atjup_hip_upload_field(atjup_ctx* ctx, int field, const double* src);
+int atjup_hip_download_field(atjup_ctx* ctx, int field, double* dst);
+/* The reference's own Array storage, one new double[km] per (i, j) row (lib/Array.cpp:43-73). */
+int atjup_hip_upload_field_rows(atjup_ctx* ctx, int field, double* const* const* rows);
+int atjup_hip_download_field_rows(atjup_ctx* ctx, int field, double** const* rows);
+/* The obstacle: SeaMount (1.0 = solid, planet/BC_Jup.h:353-457) as doubles [i][j][k]. Derives
+ * i_topography and the wall eddy viscosity (computeWallViscosity, InitValues_Jup.cpp:84-134). */
+int atjup_hip_upload_mask(atjup_ctx* ctx, const double* seamount);
+/* The friction velocity vel_star (Array_2D [j][k], planet/cJupiterModel.h:1521). */
+int atjup_hip_upload_vel_star(atjup_ctx* ctx, const double* src);
+int atjup_hip_download_vel_star(atjup_ctx* ctx, double* dst);
+/* The per-level buoyancy mean of the last RK4 step (buoy_ref_level[im]). */
+int atjup_hip_download_buoy_ref_level(atjup_ctx* ctx, double* dst);
+
+/* ---- operators -------------------------------------------------------------------------- */
+/* cJupiterModel::RungeKuttaJup (planet/RungeKutta_Jup_Turb.cpp:261-583) with its two per-step
+ * pre-passes computeBuoyancyRefLevel (:47-69) and computeHydrostaticPressure (:119-259) and the
+ * per-cell RHSJup (planet/RHS_Jup_Turb.cpp:31-1457). */
+int atjup_hip_rk4_step(atjup_ctx* ctx);
+/* PressureSolver<cJupiterModel>::run (planet/PressureSolver.h:145-766) with
+ * prepareProjectionBoundaries (planet/cJupiterModel.cpp:1842-1864). sweeps <= 0 takes the
+ * profile's count; omega = 1 is the reference's Gauss-Seidel. */
+int atjup_hip_pressure_solve(atjup_ctx* ctx, int sweeps, double omega);
+/* SaturationAdjustment<cJupiterModel>::run for one gas (planet/SaturationAdjustment.h:255-746):
+ * gas 0 = H2O, 1 = NH3, 2 = CH4, with the reference's per-gas constants. */
+int atjup_hip_satadj(atjup_ctx* ctx, int gas);
+/* Turbulence<cJupiterModel>::run (planet/Turbulence.h:307-1106). */
+int atjup_hip_turbulence_run(atjup_ctx* ctx);
+/* BC_Jup::bcRadius, bcTheta, bcPhi, bcSolidGround in that order (planet/BC_Jup.h:185-658,
+ * planet/BoundaryConditions.h:151-518), as the time loop calls them (cJupiterModel.cpp:1263-1272). */
+int atjup_hip_bc_after_step(atjup_ctx* ctx);
+/* JupiterUtils::damp_wiggles (order 2) / damp_wiggles_ho (order 4), lib/Utils.cpp:221-453.
+ * axes_mask: bit 0 = i, bit 1 = j, bit 2 = k; the passes run k, then j, then i. */
+int atjup_hip_shapiro(atjup_ctx* ctx, int field, int order, double strength, int passes, int axes_mask);
+/* cJupiterModel::computeMixtureDensity (planet/InitValues_Jup.cpp:717-740). */
+int atjup_hip_mixture_density(atjup_ctx* ctx);
+/* FluxLimiter<cJupiterModel>::nh4sh (planet/FluxLimiter.h:67-190). */
+int atjup_hip_flux_limiter(atjup_ctx* ctx);
+/* cJupiterModel::clampNegativeSpecies (planet/FileIO_Jup.cpp:494-548). */
+int atjup_hip_clamp_species(atjup_ctx* ctx);
+/* cJupiterModel::restoreVar(1.0) (planet/cJupiterModel.cpp:1757-1801): the n-level copies. */
+int atjup_hip_swap_nlevel(atjup_ctx* ctx);
+/* ATJUP_RK_SCALAR_SYNC (planet/cJupiterModel.cpp:1118-1140). */
+int atjup_hip_scalar_sync(atjup_ctx* ctx);
+/* One iteration of the loop body (planet/cJupiterModel.cpp:1010-1555), the functions on this path
+ * in the reference's order; iter is iter_n (the physics block runs on even iter and on first). */
+int atjup_hip_step(atjup_ctx* ctx, int iter, int first);
+/* Non-finite census of the prognostic fields (the ATJUP_NANCHECK watch, FileIO_Jup.cpp:326-373):
+ * returns ATJUP_NONFINITE if any prognostic value is NaN or infinite. */
+int atjup_hip_check_finite(atjup_ctx* ctx);
+
+/* ---- housekeeping ----------------------------------------------------------------------- */
+int         atjup_hip_sync(atjup_ctx* ctx);
+/* Use an existing HIP stream (hipStream_t passed as void*); NULL restores the context's own. */
+int         atjup_hip_set_stream(atjup_ctx* ctx, void* stream);
+/* Device address of a field (FP64 or FP32 storage, row pitch atjup_hip_pitch) for zero-copy views. */
+void*       atjup_hip_field_device_ptr(atjup_ctx* ctx, int field);
+int         atjup_hip_pitch(atjup_ctx* ctx);
+const char* atjup_hip_last_error_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATJUP_HIP_H */

@@ -1,0 +1,221 @@
+"""CPU oracle of the ATJUP time step — TEST INFRASTRUCTURE ONLY.
+
+Two checkers live here:
+
+* ``Oracle``: ctypes face of ``libatjup_oracle.so``, the from-scratch C++/OpenMP restatement of the
+  reference path (cpu_ref.cpp), at any grid size;
+* ``Reference``: ctypes face of ``_ref/libatjup_ref_O2.so``, the reference model itself compiled
+  from its own sources (Makefile target ``ref``) behind ref_harness.cpp. One knob profile per
+  process (the reference caches its knobs in function statics), so callers run it in a
+  subprocess (``run_reference``).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may import this package; the
+product path (atjup_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from atjup_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libatjup_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libatjup_ref_O2.so")
+
+_D = C.POINTER(C.c_double)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+_orc = None
+
+
+def _orc_lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} is not built (make -C oracle)")
+        L = C.CDLL(ORACLE_SO)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Consts), C.POINTER(abi.Knobs)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        for n in ("orc_upload_field", "orc_download_field"):
+            getattr(L, n).argtypes = [C.c_void_p, C.c_int, _D]
+        for n in ("orc_upload_mask", "orc_upload_vel_star", "orc_download_vel_star",
+                  "orc_download_buoy_ref_level"):
+            getattr(L, n).argtypes = [C.c_void_p, _D]
+        for n in ("orc_rk4_step", "orc_buoy_ref_level", "orc_hydrostatic", "orc_bc_after_step",
+                  "orc_bc_radius", "orc_bc_theta", "orc_bc_phi", "orc_bc_solid",
+                  "orc_mixture_density", "orc_clamp_species", "orc_swap_nlevel",
+                  "orc_scalar_sync", "orc_turbulence_run", "orc_flux_limiter", "orc_check_finite"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.orc_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.orc_satadj.argtypes = [C.c_void_p, C.c_int]
+        L.orc_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
+        L.orc_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_set_threads.argtypes = [C.c_int]
+        _orc = L
+    return _orc
+
+
+class Oracle:
+    """The CPU restatement, with the same operators as the HIP library."""
+
+    def __init__(self, grid, consts, knobs, threads: int = 0):
+        self.L = _orc_lib()
+        if threads:
+            self.L.orc_set_threads(threads)
+        self.shape = (grid.im, grid.jm, grid.km)
+        self.h = self.L.orc_create(C.byref(grid), C.byref(consts), C.byref(knobs))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _chk(self, rc, what):
+        if rc < 0:
+            raise RuntimeError(f"oracle {what} failed: {rc}")
+        return rc
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._chk(self.L.orc_upload_field(self.h, abi.FIELD[name], _dp(a)), f"upload {name}")
+
+    def get(self, name):
+        a = np.empty(self.shape)
+        self._chk(self.L.orc_download_field(self.h, abi.FIELD[name], _dp(a)), f"download {name}")
+        return a
+
+    def set_mask(self, m):
+        a = np.ascontiguousarray(m, dtype=np.float64)
+        self._chk(self.L.orc_upload_mask(self.h, _dp(a)), "mask")
+
+    def set_vel_star(self, vs):
+        a = np.ascontiguousarray(vs, dtype=np.float64)
+        self._chk(self.L.orc_upload_vel_star(self.h, _dp(a)), "vel_star")
+
+    def buoy_ref_level(self):
+        a = np.empty(self.shape[0])
+        self._chk(self.L.orc_download_buoy_ref_level(self.h, _dp(a)), "brl")
+        return a
+
+    def load_state(self, st):
+        self.set_mask(st.mask)
+        for n, v in st.fields.items():
+            self.set(n, v)
+
+    def call(self, op, *args):
+        return self._chk(getattr(self.L, "orc_" + op)(self.h, *args), op)
+
+
+# ---- the reference, compiled from its own sources -----------------------------------------
+class Reference:
+    """The compiled reference model behind ref_harness.cpp (one profile per process)."""
+
+    def __init__(self, env: dict, quiet: bool = True):
+        if not os.path.exists(REF_SO):
+            raise RuntimeError(f"{REF_SO} is not built (make -C oracle ref)")
+        L = C.CDLL(REF_SO)
+        L.ref_init.argtypes = [C.c_char_p]
+        L.ref_set_field.argtypes = [C.c_char_p, _D]
+        L.ref_get_field.argtypes = [C.c_char_p, _D]
+        L.ref_set_scalar.argtypes = [C.c_char_p, C.c_double]
+        L.ref_get_scalar.argtypes = [C.c_char_p, _D]
+        L.ref_set_mask.argtypes = [_D]
+        L.ref_call.argtypes = [C.c_char_p]
+        L.ref_buoy_ref_level.restype = C.c_double
+        L.ref_buoy_ref_level.argtypes = [C.c_int]
+        L.ref_set_vel_star.argtypes = [_D]
+        L.ref_get_vel_star.argtypes = [_D]
+        L.ref_quiet.argtypes = [C.c_int]
+        self.L = L
+        L.ref_quiet(1 if quiet else 0)
+        if L.ref_init(abi.env_string(env).encode()) != 0:
+            raise RuntimeError("ref_init failed (one profile per process)")
+        im, jm, km = C.c_int(), C.c_int(), C.c_int()
+        L.ref_dims(C.byref(im), C.byref(jm), C.byref(km))
+        self.shape = (im.value, jm.value, km.value)
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if self.L.ref_set_field(name.encode(), _dp(a)) != 0:
+            raise KeyError(name)
+
+    def get(self, name):
+        a = np.empty(self.shape)
+        if self.L.ref_get_field(name.encode(), _dp(a)) != 0:
+            raise KeyError(name)
+        return a
+
+    def set_scalar(self, name, v):
+        if self.L.ref_set_scalar(name.encode(), float(v)) != 0:
+            raise KeyError(name)
+
+    def get_scalar(self, name):
+        v = C.c_double()
+        if self.L.ref_get_scalar(name.encode(), C.byref(v)) != 0:
+            raise KeyError(name)
+        return v.value
+
+    def set_mask(self, m):
+        a = np.ascontiguousarray(m, dtype=np.float64)
+        self.L.ref_set_mask(_dp(a))
+
+    def set_vel_star(self, vs):
+        a = np.ascontiguousarray(vs, dtype=np.float64)
+        self.L.ref_set_vel_star(_dp(a))
+
+    def buoy_ref_level(self):
+        return np.array([self.L.ref_buoy_ref_level(i) for i in range(self.shape[0])])
+
+    def call(self, op):
+        if self.L.ref_call(op.encode()) != 0:
+            raise KeyError(op)
+
+    def load_state(self, st, consts=None):
+        self.set_mask(st.mask)
+        for n, v in st.fields.items():
+            self.set(n, v)
+        if consts is not None:
+            for n, _ in abi.Consts._fields_:
+                try:
+                    self.set_scalar(n, getattr(consts, n))
+                except KeyError:
+                    pass
+
+    def step(self, iter_n, first, satadj=True, turb=False):
+        """The loop body's on-path functions in the reference's order (cJupiterModel.cpp:1010-1555)."""
+        if iter_n % 2 == 0 or first:
+            self.call("pressure")
+            self.call("damp_pdyn")
+            self.call("mixture_density")
+            if satadj:
+                self.call("satadj_h2o")
+                self.call("satadj_nh3")
+                self.call("satadj_ch4")
+            # ATJUP_RK_SCALAR_SYNC = 2: species and t into the n-level copies
+            for n in ("h2o", "h2o_cloud", "h2o_ice", "nh3", "nh3_cloud", "nh3_ice",
+                      "ch4", "ch4_cloud", "ch4_ice", "t"):
+                self.set(n + "n", self.get(n))
+            if turb:
+                self.call("turbulence")
+        self.call("rk4")
+        self.call("bc_all")
+        if turb:
+            self.call("turb_wall")
+        self.call("clamp_species")
+        self.call("restore")
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_SO)

@@ -1,0 +1,1248 @@
+// CPU oracle of the ATJUP time step (test infrastructure; see cpu_ref.h).
+//
+// Every routine restates one reference routine for the knob profile of include/atjup_hip.h and
+// keeps the reference's floating-point expression trees (operand order and association) so that,
+// compiled with -ffp-contract=off against the same libm, it reproduces the compiled reference bit
+// for bit. Fields are contiguous [i][j][k] with k fastest; the grid size is a runtime parameter.
+#include "cpu_ref.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int NP = ATJUP_N_PROG;
+
+struct Geo {
+    double rm, rm2, exp_rm, exp_2_rm;
+    double sinthe, sinthe2, costhe, cotanthe;
+    double inv_rm, inv_rm2, inv_rmsinthe, inv_rm2sinthe, inv_rm2sinthe2, costhe_inv_rm2sinthe;
+    double inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
+};
+
+inline bool nonfinite_bits(double v){
+    std::uint64_t b; std::memcpy(&b, &v, sizeof b);
+    return (b & 0x7FF0000000000000ULL) == 0x7FF0000000000000ULL;
+}
+// RungeKutta_Jup_Turb.cpp:285-290, BC_Jup.h:627-632
+inline double safe_clamp(double v, double lo, double hi){
+    if(nonfinite_bits(v)) return lo;
+    return (v < lo) ? lo : ((v > hi) ? hi : v);
+}
+
+}  // namespace
+
+struct orc_ctx {
+    atjup_grid g;
+    atjup_consts c;
+    atjup_knobs kn;
+    int im, jm, km;
+    size_t n3;
+    std::vector<double> rad, the, sin_the, cos_the, sinthe_rk;
+    std::vector<float> lh;                           // get_layer_height [km]
+    std::vector<std::vector<double>> f;              // by field id; NLEVEL + id for the n level
+    std::vector<std::vector<double>> rhs, acc;       // rhs_* and acc_* of the 17 prognostic fields
+    std::vector<double> mask;                        // SeaMount
+    std::vector<int> itopo;                          // i_topography [j][k]
+    std::vector<double> vel_star;                    // [j][k]
+    std::vector<double> brl;                         // buoy_ref_level [i]
+    double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+
+    size_t id(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
+    std::vector<double>& F(int fid){ return f[fid]; }
+    bool solid(int i, int j, int k) const { return mask[id(i, j, k)] == 1.0; }
+    double p_dyn_to_bar() const { return c.r_mix * c.u_0 * c.u_0 * 1.0e-5; }   // cJupiterModel.h:1193-1195
+    double get_layer_height(int i) const { return (i < 0 || i > im - 1) ? -1.0f : lh[i]; }
+    double layer_thickness_m(int i) const {                                  // cJupiterModel.h:147-150
+        if(i < 0 || i > im - 2) return -1.0;
+        return (double)(lh[i + 1] - lh[i]) * 1.0e3;
+    }
+    int wrap(int kk) const {                                                 // ATPhys.h:132-136
+        const int n = km - 1;
+        return kk < 0 ? kk + n : (kk >= n ? kk - n : kk);
+    }
+    Geo geo(int i, int j) const {                                            // RungeKutta_Jup_Turb.cpp:390-411
+        Geo G;
+        G.rm = rad[i];
+        G.rm2 = G.rm * G.rm;
+        G.exp_rm = 1.0;                                                      // coord_stretching off
+        G.exp_2_rm = G.exp_rm * G.exp_rm;
+        G.sinthe = sinthe_rk[j];
+        G.sinthe2 = G.sinthe * G.sinthe;
+        G.costhe = cos_the[j];
+        G.cotanthe = G.costhe / G.sinthe;
+        G.inv_rm = 1.0 / G.rm;
+        G.inv_rm2 = 1.0 / G.rm2;
+        G.inv_rmsinthe = 1.0 / (G.rm * G.sinthe);
+        G.inv_rm2sinthe = G.inv_rm2 / G.sinthe;
+        G.inv_rm2sinthe2 = G.inv_rm2 / G.sinthe2;
+        G.costhe_inv_rm2sinthe = G.costhe * G.inv_rm2sinthe;
+        G.inv_2dr = 1.0 / (2.0 * g.dr);
+        G.inv_2dthe = 1.0 / (2.0 * g.dthe);
+        G.inv_2dphi = 1.0 / (2.0 * g.dphi);
+        G.inv_dr2 = 1.0 / (g.dr * g.dr);
+        G.inv_dthe2 = 1.0 / (g.dthe * g.dthe);
+        G.inv_dphi2 = 1.0 / (g.dphi * g.dphi);
+        return G;
+    }
+};
+
+namespace {
+
+// ---- computeBuoyancyRefLevel (RungeKutta_Jup_Turb.cpp:47-69) -------------------------------
+void buoy_ref_level(orc_ctx& o){
+    const auto& t = o.f[ATJUP_F_T];
+    const auto& ps = o.f[ATJUP_F_P_STAT];
+    o.brl.assign(o.im, 0.0);
+    #pragma omp parallel for schedule(static)
+    for(int i = 0; i < o.im; i++){
+        double sum = 0.0, wsum = 0.0;
+        for(int j = 0; j < o.jm; j++){
+            const double wgt = o.sin_the[j];
+            for(int k = 0; k < o.km; k++){
+                const size_t q = o.id(i, j, k);
+                if(o.mask[q] == 1.0) continue;
+                if(!(t[q] > 0.0)) continue;
+                const double b = o.c.g * ps[q] / (o.c.r_mix * o.c.R_mix * t[q] * o.c.t_ref);
+                if(!std::isfinite(b)) continue;
+                sum += wgt * b;
+                wsum += wgt;
+            }
+        }
+        o.brl[i] = (wsum > 0.0) ? sum / wsum : 0.0;
+    }
+}
+
+// ---- computeHydrostaticPressure, ATJUP_METRIC_SPLIT + CORIOLIS_SPLIT, no HYDRO_SPLIT, no
+//      p_baro (RungeKutta_Jup_Turb.cpp:119-218) ------------------------------------------------
+void hydrostatic(orc_ctx& o){
+    auto& ph = o.f[ATJUP_F_P_HYDRO];
+    const auto& v = o.f[ATJUP_F_V];
+    const auto& w = o.f[ATJUP_F_W];
+    const int im = o.im;
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 0; j < o.jm; j++){
+        for(int k = 0; k < o.km; k++){
+            int i0 = 0;
+            while(i0 < im && o.solid(i0, j, k)){ ph[o.id(i0, j, k)] = 0.0; i0++; }
+            if(i0 >= im) continue;
+            int i1 = i0;
+            while(i1 + 1 < im && !o.solid(i1 + 1, j, k)) i1++;
+            for(int i = i1 + 1; i < im; i++) ph[o.id(i, j, k)] = 0.0;
+            // nd_cor: ATJUP_ND_COR unset and GEO_BALANCE off -> ATJUP_NONDIM (unset) -> 1
+            const double cor_ = o.c.Coriolis * 1.0 * 2.0 * o.c.omega * std::abs(o.sin_the[j]);
+            auto f_ = [&](int i) -> double {
+                const size_t q = o.id(i, j, k);
+                const double exp_rm_ = 1.0;
+                return 0.0 + ((v[q] * v[q] + w[q] * w[q]) / o.rad[i] + cor_ * w[q]) / exp_rm_;
+            };
+            ph[o.id(i0, j, k)] = 0.0;
+            for(int i = i0 + 1; i <= i1; i++)
+                ph[o.id(i, j, k)] = ph[o.id(i - 1, j, k)] + 0.5 * (f_(i - 1) + f_(i)) * o.g.dr;
+        }
+    }
+}
+
+// ---- RHSJup (RHS_Jup_Turb.cpp:31-1457) for one cell ------------------------------------------
+// The profile: ATJUP_NONDIM unset (nd factors 1), PGRAD_WALL/WRAP 1, ADIABATIC(_RHO) 1, METRIC_SPLIT
+// and CORIOLIS_SPLIT 1, HYDRO_SPLIT 0, CELL_HOLD 0, TURB_BUOY 0, SPECIES_DIVU 0, CHROM 0.
+void rhs_cell(orc_ctx& o, int i, int j, int k, const Geo& G, int turb_sel){
+    const atjup_consts& C = o.c;
+    const atjup_knobs& K = o.kn;
+    const int kw = o.wrap(k - 1), ke = o.wrap(k + 1);
+    auto& F = o.f;
+    const size_t q = o.id(i, j, k);
+    const size_t qip = o.id(i + 1, j, k), qim = o.id(i - 1, j, k);
+    const size_t qjp = o.id(i, j + 1, k), qjm = o.id(i, j - 1, k);
+    const size_t qkp = o.id(i, j, ke), qkm = o.id(i, j, kw);
+
+    const double u_ijk = F[ATJUP_F_U][q], v_ijk = F[ATJUP_F_V][q], w_ijk = F[ATJUP_F_W][q];
+
+    auto pgrad = [&](const std::vector<double>& P, size_t qh, size_t ql, double inv_2d) -> double {
+        const double loc = P[q];
+        double hi = P[qh], lo = P[ql];
+        if(o.mask[qh] == 1.0) hi = loc;
+        if(o.mask[ql] == 1.0) lo = loc;
+        return (hi - lo) * inv_2d;
+    };
+
+    double d1r[NP], d2r[NP], d1t[NP], d2t[NP], d1p[NP], d2p[NP];
+    for(int fi = 0; fi < NP; fi++){
+        const auto& X = F[fi];
+        d1r[fi] = (X[qip] - X[qim]) * G.inv_2dr * G.exp_rm;
+        d2r[fi] = (X[qip] - 2.0 * X[q] + X[qim]) * G.inv_dr2 * G.exp_2_rm;
+        d1t[fi] = (X[qjp] - X[qjm]) * G.inv_2dthe;
+        d2t[fi] = (X[qjp] - 2.0 * X[q] + X[qjm]) * G.inv_dthe2;
+        d1p[fi] = (X[qkp] - X[qkm]) * G.inv_2dphi;
+        d2p[fi] = (X[qkp] - 2.0 * X[q] + X[qkm]) * G.inv_dphi2;
+    }
+    const auto& pd = F[ATJUP_F_P_DYN];
+    const double dpdr = pgrad(pd, qip, qim, G.inv_2dr) * G.exp_rm;
+    const double dpdthe = pgrad(pd, qjp, qjm, G.inv_2dthe);
+    const double dpdphi = pgrad(pd, qkp, qkm, G.inv_2dphi);
+
+    // Neumann k*/dis* at solid faces (RHS_Jup_Turb.cpp:265-303)
+    {
+        const bool sim = o.mask[qim] == 1.0, sip = o.mask[qip] == 1.0;
+        const bool sjm = o.mask[qjm] == 1.0, sjp = o.mask[qjp] == 1.0;
+        const bool skm = o.mask[qkm] == 1.0, skp = o.mask[qkp] == 1.0;
+        for(int fi = ATJUP_F_TKE; fi <= ATJUP_F_DIS; fi++){
+            const auto& X = F[fi];
+            if(sim || sip){
+                const double a = sim ? X[q] : X[qim], b = sip ? X[q] : X[qip];
+                d1r[fi] = (b - a) * G.inv_2dr * G.exp_rm;
+                d2r[fi] = (b - 2.0 * X[q] + a) * G.inv_dr2 * G.exp_2_rm;
+            }
+            if(sjm || sjp){
+                const double a = sjm ? X[q] : X[qjm], b = sjp ? X[q] : X[qjp];
+                d1t[fi] = (b - a) * G.inv_2dthe;
+                d2t[fi] = (b - 2.0 * X[q] + a) * G.inv_dthe2;
+            }
+            if(skm || skp){
+                const double a = skm ? X[q] : X[qkm], b = skp ? X[q] : X[qkp];
+                d1p[fi] = (b - a) * G.inv_2dphi;
+                d2p[fi] = (b - 2.0 * X[q] + a) * G.inv_dphi2;
+            }
+        }
+    }
+
+    const double nd_cor = 1.0, nd_cent = 1.0, nd_buoy = 1.0;
+    const double costhe = G.costhe;
+    const double sinthe_true = std::sqrt(std::max(0.0, 1.0 - costhe * costhe));
+    const double Coriolis_rad = nd_cor * -2.0 * C.omega * sinthe_true * w_ijk;
+    const double Coriolis_the = nd_cor * -2.0 * C.omega * costhe * w_ijk;
+    (void)Coriolis_rad;
+    const double Coriolis_phi = nd_cor * +2.0 * C.omega * (+costhe * v_ijk + sinthe_true * u_ijk);
+    const double centrifugal_rad = nd_cent * C.omega * C.omega * G.rm * sinthe_true * sinthe_true;
+    const double centrifugal_the = nd_cent * C.omega * C.omega * G.rm * sinthe_true * costhe;
+    const double coeff_energy_p = C.u_0 * C.u_0 / (C.cp_mix * C.t_ref);
+
+    const double v_invrm = v_ijk * G.inv_rm;
+    const double w_invrs = w_ijk * G.inv_rmsinthe;
+
+    // adiabatic compression with the local density (RHS_Jup_Turb.cpp:461-482)
+    double dpstatdr;
+    {
+        const auto& ps = F[ATJUP_F_P_STAT];
+        const double to_nd = 1.0 / o.p_dyn_to_bar();
+        dpstatdr = (ps[qip] - ps[qim]) * G.inv_2dr * G.exp_rm * to_nd;
+        const double rho_l = F[ATJUP_F_RHO_MIX][q];
+        if(rho_l > 0.0 && std::isfinite(rho_l)) dpstatdr *= C.r_mix / rho_l;
+    }
+    const double pressure_t = coeff_energy_p
+        * (u_ijk * (dpdr + dpstatdr) + v_invrm * dpdthe + w_invrs * dpdphi);
+
+    auto transport = [&](int fi){ return u_ijk * d1r[fi] + v_invrm * d1t[fi] + w_invrs * d1p[fi]; };
+    const double metric_on = 0.0;
+    const double cor_rad_on = 0.0;
+    const double transport_t = transport(ATJUP_F_T);
+    const double transport_u = u_ijk * d1r[1] + v_invrm * d1t[1] + w_invrs * d1p[1]
+        - metric_on * (v_ijk * v_ijk + w_ijk * w_ijk) * G.inv_rm;
+    const double transport_v = u_ijk * d1r[2] + v_invrm * d1t[2] + w_invrs * d1p[2]
+        + (u_ijk * v_ijk - w_ijk * w_ijk * G.cotanthe) * G.inv_rm;
+    const double transport_w = u_ijk * d1r[3] + v_invrm * d1t[3] + w_invrs * d1p[3]
+        + (w_ijk * u_ijk + v_ijk * w_ijk * G.cotanthe) * G.inv_rm;
+
+    const double two_inv_rm = 2.0 * G.inv_rm;
+    const double v_metric = (1.0 + costhe / (G.sinthe2)) * G.inv_rm2;
+    auto diffusion = [&](int fi){
+        return d2r[fi] + d1r[fi] * two_inv_rm + d2t[fi] * G.inv_rm2
+             + d1t[fi] * G.costhe_inv_rm2sinthe + d2p[fi] * G.inv_rm2sinthe2;
+    };
+    const double diffusion_t = diffusion(ATJUP_F_T);
+    const double diffusion_u = d2r[1] + 2.0 * u_ijk * G.inv_rm2 + d2t[1] * G.inv_rm2
+        + 4.0 * d1r[1] * G.inv_rm + d1t[1] * G.costhe_inv_rm2sinthe
+        + d2p[1] * G.inv_rm2sinthe2;
+    const double diffusion_v = d2r[2] + d1r[2] * two_inv_rm + d2t[2] * G.inv_rm2
+        + d1t[2] * G.costhe_inv_rm2sinthe - v_metric * v_ijk
+        + d2p[2] * G.inv_rm2sinthe2
+        + 2.0 * d1t[1] * G.inv_rm2
+        - d1p[3] * 2.0 * costhe * G.inv_rm2sinthe2;
+    const double diffusion_w = d2r[3] + d1r[3] * two_inv_rm + d2t[3] * G.inv_rm2
+        + d1t[3] * G.costhe_inv_rm2sinthe - v_metric * w_ijk
+        + d2p[3] * G.inv_rm2sinthe2
+        + 2.0 * d1p[1] * G.inv_rm2sinthe
+        + d1p[2] * 2.0 * costhe * G.inv_rm2sinthe2;
+
+    const double pgrad_scale = 1.0;
+    const double dpdr_term = pgrad_scale * dpdr;
+    const double dpdthe_term = pgrad_scale * dpdthe * G.inv_rm;
+    const double dpdphi_term = pgrad_scale * dpdphi * G.inv_rmsinthe;
+    const auto& ph = F[ATJUP_F_P_HYDRO];
+    const double dphdthe_term = pgrad(ph, qjp, qjm, G.inv_2dthe) * G.inv_rm;
+    const double dphdphi_term = pgrad(ph, qkp, qkm, G.inv_2dphi) * G.inv_rmsinthe;
+
+    // radiative heating (RHS_Jup_Turb.cpp:646-676)
+    double radiation_t = 0.0;
+    if(K.rad_coupling != 0.0){
+        const double T_phys = F[ATJUP_F_T][q] * C.t_ref;
+        const double P_phys = F[ATJUP_F_P_STAT][q] * 1.0e5;
+        const double rho_f = F[ATJUP_F_RHO_MIX][q];
+        const double rho = (rho_f > 0.0 && std::isfinite(rho_f))
+                         ? rho_f : ((T_phys > 1.0) ? P_phys / (C.R_mix * T_phys) : 0.0);
+        const double L_rad = o.g.L_atm * 1.0e3;
+        if(rho > 0.0 && C.cp_mix > 0.0){
+            radiation_t = K.rad_coupling * F[ATJUP_F_Q_RAD][q] * L_rad / (rho * C.cp_mix * C.u_0 * C.t_ref);
+            if(!std::isfinite(radiation_t)) radiation_t = 0.0;
+            else if(radiation_t > 0.5) radiation_t = 0.5;
+            else if(radiation_t < -0.5) radiation_t = -0.5;
+        }
+    }
+    // precipitation latent heat with the local density (RHS_Jup_Turb.cpp:701-718)
+    double precip_t = 0.0;
+    if(K.precip_coupling != 0.0){
+        const double L_rad = o.g.L_atm * 1.0e3;
+        const double rf = F[ATJUP_F_RHO_MIX][q];
+        const double rho_p = (rf > 0.0 && std::isfinite(rf)) ? rf : C.r_mix;
+        if(rho_p > 0.0 && C.cp_mix > 0.0){
+            precip_t = K.precip_coupling * F[ATJUP_F_Q_PRECIP][q] * L_rad / (rho_p * C.cp_mix * C.u_0 * C.t_ref);
+            if(!std::isfinite(precip_t)) precip_t = 0.0;
+            else if(precip_t > 0.5) precip_t = 0.5;
+            else if(precip_t < -0.5) precip_t = -0.5;
+        }
+    }
+    const double Pr_t = 0.9;
+    const double nue_c = F[ATJUP_F_NUE][q];
+    const double nue_t = (K.turb_coupling != 0.0 && std::isfinite(nue_c)) ? K.turb_coupling * std::max(0.0, nue_c) : 0.0;
+    const double nue_t_s = nue_t / Pr_t;
+    const double nue_wall = F[ATJUP_F_WALL_NUE][q];
+
+    // k* / dis* sources (RHS_Jup_Turb.cpp:763-998)
+    double diffusion_tke_re = 0.0, diffusion_dis_re = 0.0, tke_src = 0.0, dis_src = 0.0;
+    const bool turb_on = turb_sel >= 0;
+    if(turb_on){
+        const double C_nue = 0.028, dis_min = 1.0e-10, nue_gas_phys = 1.8e-5;
+        const double L_m = o.g.L_atm * 1.0e3;
+        const double nue_max = K.nue_max_phys / (C.u_0 * L_m);
+        const double nue_here = std::isfinite(nue_c) ? std::max(0.0, nue_c) : 0.0;
+        const double tke0 = F[ATJUP_F_TKE][q], dis0 = F[ATJUP_F_DIS][q];
+        const double tke_s = std::max(tke0, dis_min);
+        const double dis_s = std::max(dis0, dis_min);
+        const double dudr_s = d1r[1], dvdr_s = d1r[2], dwdr_s = d1r[3];
+        const double dudthe_s = d1t[1] * G.inv_rm, dvdthe_s = d1t[2] * G.inv_rm, dwdthe_s = d1t[3] * G.inv_rm;
+        const double dudphi_s = d1p[1] * G.inv_rmsinthe, dvdphi_s = d1p[2] * G.inv_rmsinthe, dwdphi_s = d1p[3] * G.inv_rmsinthe;
+        const double dtkedthe_s = d1t[ATJUP_F_TKE] * G.inv_rm, ddisdthe_s = d1t[ATJUP_F_DIS] * G.inv_rm;
+        const double dtkedphi_s = d1p[ATJUP_F_TKE] * G.inv_rmsinthe, ddisdphi_s = d1p[ATJUP_F_DIS] * G.inv_rmsinthe;
+        const double grad_dot = d1r[ATJUP_F_TKE] * d1r[ATJUP_F_DIS] + dtkedthe_s * ddisdthe_s + dtkedphi_s * ddisdphi_s;
+        double cnue = (turb_sel == 0) ? C_nue * tke_s * tke_s / dis_s : tke_s / dis_s;
+        cnue = std::min(cnue, nue_max);
+        const double der = 0.66667 * (dudr_s + dvdthe_s + dwdphi_s);
+        const double P_full = std::max(0.0,
+              (cnue * (2.0 * dudr_s - der) - 0.66667 * tke0) * dudr_s
+            + (cnue * (dudthe_s + dvdr_s)) * dudthe_s
+            + (cnue * (dudphi_s + dwdr_s)) * dudphi_s
+            + (cnue * (2.0 * dvdthe_s - der) - 0.66667 * tke0) * dvdthe_s
+            + (cnue * (dvdr_s + dudthe_s)) * dvdr_s
+            + (cnue * (dvdphi_s + dwdthe_s)) * dvdphi_s
+            + (cnue * (2.0 * dwdphi_s - der) - 0.66667 * tke0) * dwdphi_s
+            + (cnue * (dwdr_s + dudphi_s)) * dwdr_s
+            + (cnue * (dwdthe_s + dvdphi_s)) * dwdthe_s);
+        F[ATJUP_F_PROD][q] = P_full;
+        const double y_mount = (double)o.get_layer_height(o.itopo[(size_t)j * o.km + k]) * 1.0e3;
+        const double dz_layer = std::max(1.0, o.layer_thickness_m(std::max(i - 1, 0)));
+        const double y_phys = std::max((double)o.get_layer_height(i) * 1.0e3 - y_mount, dz_layer);
+        const double y_star = y_phys / L_m;
+        if(turb_sel == 0){
+            const double sig_k = 1.0, sig_w = 1.3, C_eps_1 = 1.35, C_eps_2 = 1.80;
+            diffusion_tke_re = 1.0 / C.re + nue_here / sig_k;
+            diffusion_dis_re = 1.0 / C.re + nue_here / sig_w;
+            const double d_plus = y_phys * o.vel_star[(size_t)j * o.km + k] / nue_gas_phys;
+            const double Re_T = tke_s * tke_s * C.u_0 * L_m / (dis_s * nue_gas_phys);
+            const double f_2 = 1.0 - 0.4 / 1.8 * std::exp(-Re_T * Re_T / 36.0);
+            const double y_star2 = y_star * y_star;
+            const double L_k = -2.0 * tke0 / y_star2;
+            const double L_w = -2.0 * dis0 / y_star2 * std::exp(-0.5 * d_plus);
+            const double P_k = P_full;
+            const double Y_k = dis0;
+            const double P_w = C_eps_1 * dis0 / tke_s * P_k;
+            const double Y_w = C_eps_2 * f_2 * dis0 * dis0 / tke_s;
+            tke_src = P_k - Y_k + L_k / C.re_turb;
+            dis_src = P_w - Y_w + L_w / C.re_turb;
+        } else if(turb_sel == 1){
+            const double sig_k = 0.6, sig_w = 0.5, bet_star = 0.09, gam = 0.52, bet_0 = 0.0708;
+            diffusion_tke_re = 1.0 / C.re + sig_k * nue_here;
+            diffusion_dis_re = 1.0 / C.re + sig_w * nue_here;
+            const double S11 = dudr_s, S22 = dvdthe_s, S33 = dwdphi_s;
+            const double S12 = 0.5 * (dudthe_s + dvdr_s);
+            const double S13 = 0.5 * (dudphi_s + dwdr_s);
+            const double S23 = 0.5 * (dvdphi_s + dwdthe_s);
+            const double S_mag = std::sqrt(2.0 * (S11 * S11 + S22 * S22 + S33 * S33
+                                                + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23)));
+            const double W12 = dudthe_s - dvdr_s;
+            const double W13 = dudphi_s - dwdr_s;
+            const double W23 = dvdphi_s - dwdthe_s;
+            const double Omega = std::sqrt(W12 * W12 + W13 * W13 + W23 * W23);
+            const double chi_w = std::fabs(Omega * Omega * S_mag / std::pow(bet_star * dis_s, 3));
+            const double f_bet = (1.0 + 85.0 * chi_w) / (1.0 + 100.0 * chi_w);
+            const double bet_wc = bet_0 * f_bet;
+            const double sig_d = (grad_dot <= 0.0) ? 0.0 : 0.125;
+            const double P_k = std::min(P_full, 20.0 * bet_star * tke_s * dis_s);
+            const double Y_k = bet_star * tke_s * dis_s;
+            const double P_w = gam * dis_s / tke_s * P_k;
+            const double Y_w = bet_wc * dis_s * dis_s;
+            const double D_w = sig_d / dis_s * grad_dot;
+            tke_src = P_k - Y_k;
+            dis_src = P_w - Y_w + D_w;
+        } else {
+            const double bet_star = 0.09, sig_k1 = 1.176, sig_k2 = 1.0, sig_w1 = 2.0, sig_w2 = 1.168;
+            const double bet1 = 0.0333, bet2 = 0.0368, gam1 = 0.413, gam2 = 0.2;
+            const double nue_air_nd = nue_gas_phys / (C.u_0 * L_m);
+            const double CD_kw = std::max(2.0 * sig_w2 / dis_s * grad_dot, 1.0e-20);
+            const double arg1 = std::min(
+                std::max(std::sqrt(std::max(tke0, 0.0)) / (bet_star * dis_s * y_star),
+                         500.0 * nue_air_nd / (y_star * y_star * dis_s)),
+                4.0 * sig_w2 * tke_s / (CD_kw * y_star * y_star));
+            const double F1 = std::tanh(std::pow(arg1, 4));
+            const double sig_k = F1 * sig_k1 + (1.0 - F1) * sig_k2;
+            const double sig_w = F1 * sig_w1 + (1.0 - F1) * sig_w2;
+            diffusion_tke_re = 1.0 / C.re + nue_here / sig_k;
+            diffusion_dis_re = 1.0 / C.re + nue_here / sig_w;
+            const double P_k = std::min(P_full, 20.0 * bet_star * tke_s * dis_s);
+            const double Y_k = bet_star * tke_s * dis_s;
+            const double P_w = (F1 * gam1 + (1.0 - F1) * gam2) * P_k * dis_s / tke_s;
+            const double Y_w = (F1 * bet1 + (1.0 - F1) * bet2) * dis_s * dis_s;
+            const double D_w = 2.0 * (1.0 - F1) * sig_w2 / dis_s * grad_dot;
+            tke_src = P_k - Y_k;
+            dis_src = P_w - Y_w + D_w;
+        }
+        const double tke_src_max = 20.0 * tke_s * dis_s;
+        const double dis_src_max = 20.0 * dis_s * dis_s;
+        if(!std::isfinite(tke_src)) tke_src = 0.0;
+        if(!std::isfinite(dis_src)) dis_src = 0.0;
+        tke_src = std::max(-tke_src_max, std::min(tke_src_max, tke_src));
+        dis_src = std::max(-dis_src_max, std::min(dis_src_max, dis_src));
+        F[ATJUP_F_TKE_SOURCE][q] = tke_src;
+        F[ATJUP_F_DIS_SOURCE][q] = dis_src;
+    }
+
+    auto& R = o.rhs;
+    R[ATJUP_F_T][q] =
+        + pressure_t
+        - transport_t
+        + diffusion_t * (K.t_diff_scale / (C.re * C.pr) + nue_t_s)
+        + radiation_t
+        + precip_t;
+
+    const double tq = F[ATJUP_F_T][q];
+    const double buoyancy_u = (tq > 0.0)
+        ? -nd_buoy * 1.0 * 1.0 * C.buoyancy
+          * (C.g * F[ATJUP_F_P_STAT][q] / (C.r_mix * C.R_mix * tq * C.t_ref) - o.brl[i])
+        : 0.0;
+    R[ATJUP_F_U][q] =
+        - dpdr_term
+        + buoyancy_u
+        - transport_u
+        + diffusion_u * (1.0 / C.re + nue_t + nue_wall)
+        - cor_rad_on * C.Coriolis * Coriolis_rad
+        + C.centrifugal * centrifugal_rad;
+    R[ATJUP_F_V][q] =
+        - dpdthe_term
+        - dphdthe_term
+        - transport_v
+        + diffusion_v * (1.0 / C.re + nue_t + nue_wall)
+        - C.Coriolis * Coriolis_the
+        + C.centrifugal * centrifugal_the;
+    R[ATJUP_F_W][q] =
+        - dpdphi_term
+        - dphdphi_term
+        - transport_w
+        + diffusion_w * (1.0 / C.re + nue_t + nue_wall)
+        - C.Coriolis * Coriolis_phi;
+
+    // species (RHS_Jup_Turb.cpp:1273-1338)
+    const double sds = K.species_diff_scale, pc = K.precip_coupling, cr = C.chemical_reaction;
+    auto sp = [&](int fi, double sc){ return -transport(fi) + diffusion(fi) * (sds / (sc * C.re) + nue_t_s); };
+    R[ATJUP_F_H2O][q]       = sp(ATJUP_F_H2O, C.sc_h2o)       + pc * F[ATJUP_F_S_PRECIP_H2O][q];
+    R[ATJUP_F_H2O_CLOUD][q] = sp(ATJUP_F_H2O_CLOUD, C.sc_h2o) + pc * F[ATJUP_F_S_PRECIP_H2O_CLOUD][q];
+    R[ATJUP_F_H2O_ICE][q]   = sp(ATJUP_F_H2O_ICE, C.sc_h2o)   + pc * F[ATJUP_F_S_PRECIP_H2O_ICE][q];
+    R[ATJUP_F_H2S][q]       = sp(ATJUP_F_H2S, C.sc_h2s)       + cr * F[ATJUP_F_MASSFLUX_H2S][q];
+    R[ATJUP_F_NH3][q]       = sp(ATJUP_F_NH3, C.sc_nh3)       + cr * F[ATJUP_F_MASSFLUX_NH3][q]
+                                                              + pc * F[ATJUP_F_S_PRECIP_NH3][q];
+    R[ATJUP_F_NH3_CLOUD][q] = sp(ATJUP_F_NH3_CLOUD, C.sc_nh3) + pc * F[ATJUP_F_S_PRECIP_NH3_CLOUD][q];
+    R[ATJUP_F_NH3_ICE][q]   = sp(ATJUP_F_NH3_ICE, C.sc_nh3)   + pc * F[ATJUP_F_S_PRECIP_NH3_ICE][q];
+    R[ATJUP_F_CH4][q]       = sp(ATJUP_F_CH4, C.sc_ch4)       + pc * F[ATJUP_F_S_PRECIP_CH4][q];
+    R[ATJUP_F_CH4_CLOUD][q] = sp(ATJUP_F_CH4_CLOUD, C.sc_ch4) + pc * F[ATJUP_F_S_PRECIP_CH4_CLOUD][q];
+    R[ATJUP_F_CH4_ICE][q]   = sp(ATJUP_F_CH4_ICE, C.sc_ch4)   + pc * F[ATJUP_F_S_PRECIP_CH4_ICE][q];
+    {
+        const double v_stokes = (2.0 / 9.0) * C.r_p_nh4sh * C.r_p_nh4sh
+                              * (C.rg_nh4sh - F[ATJUP_F_RHO_MIX][q]) * C.g / C.mue_mix;
+        R[ATJUP_F_NH4SH][q] = -transport(ATJUP_F_NH4SH)
+            + F[ATJUP_F_FLUXLIM_NH4SH][q]
+            + diffusion(ATJUP_F_NH4SH) * (sds / (C.sc_nh4sh * C.re) + nue_t_s)
+            + cr * F[ATJUP_F_MASSFLUX_NH4SH][q]
+            + (v_stokes / C.u_0) * d1r[ATJUP_F_NH4SH];
+    }
+    if(turb_on){
+        R[ATJUP_F_TKE][q] = -transport(ATJUP_F_TKE) + diffusion(ATJUP_F_TKE) * diffusion_tke_re + tke_src;
+        R[ATJUP_F_DIS][q] = -transport(ATJUP_F_DIS) + diffusion(ATJUP_F_DIS) * diffusion_dis_re + dis_src;
+    } else {
+        R[ATJUP_F_TKE][q] = 0.0;
+        R[ATJUP_F_DIS][q] = 0.0;
+    }
+    F[ATJUP_F_AUX_U][q] = R[ATJUP_F_U][q] + dpdr_term;
+    F[ATJUP_F_AUX_V][q] = R[ATJUP_F_V][q] + dpdthe_term;
+    F[ATJUP_F_AUX_W][q] = R[ATJUP_F_W][q] + dpdphi_term;
+}
+
+// ---- RungeKuttaJup (RungeKutta_Jup_Turb.cpp:261-583) ----------------------------------------
+void rk4(orc_ctx& o){
+    buoy_ref_level(o);
+    hydrostatic(o);
+    const double dt = o.c.dt;
+    const double tke_max_nd = o.kn.tke_max_phys / (o.c.u_0 * o.c.u_0);
+    const double dis_min_nd = 1.0e-10;
+    const int turb_sel = o.kn.turb_model;
+    const int im = o.im, jm = o.jm, km = o.km;
+    for(int stage = 0; stage < 4; stage++){
+        const double c_in = (stage == 0 || stage == 1) ? 0.5 * dt : (stage == 2 ? dt : 0.0);
+        const double wgt = (stage == 0 || stage == 3) ? 1.0 : 2.0;
+        #pragma omp parallel for collapse(2) schedule(static)
+        for(int i = 1; i < im - 1; i++)
+            for(int j = 3; j < jm - 3; j++){
+                const Geo G = o.geo(i, j);
+                for(int k = 0; k < km - 1; k++){
+                    if(o.solid(i, j, k)) continue;
+                    rhs_cell(o, i, j, k, G, turb_sel);
+                }
+            }
+        #pragma omp parallel for collapse(2) schedule(static)
+        for(int i = 1; i < im - 1; i++)
+            for(int j = 3; j < jm - 3; j++)
+                for(int k = 0; k < km - 1; k++){
+                    if(o.solid(i, j, k)) continue;
+                    const size_t q = o.id(i, j, k);
+                    for(int fi = 0; fi < NP; fi++)
+                        o.acc[fi][q] = (stage == 0) ? wgt * o.rhs[fi][q] : o.acc[fi][q] + wgt * o.rhs[fi][q];
+                    if(stage < 3){
+                        for(int fi = 0; fi < ATJUP_F_TKE; fi++)
+                            o.f[fi][q] = o.f[ATJUP_NLEVEL + fi][q] + c_in * o.rhs[fi][q];
+                        o.f[ATJUP_F_TKE][q] = safe_clamp(o.f[ATJUP_NLEVEL + ATJUP_F_TKE][q]
+                            + c_in * o.rhs[ATJUP_F_TKE][q], 0.0, tke_max_nd);
+                        o.f[ATJUP_F_DIS][q] = std::max(dis_min_nd, o.f[ATJUP_NLEVEL + ATJUP_F_DIS][q]
+                            + c_in * o.rhs[ATJUP_F_DIS][q]);
+                    }
+                }
+    }
+    const double one_sixth = 1.0 / 6.0;
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int i = 1; i < im - 1; i++)
+        for(int j = 3; j < jm - 3; j++)
+            for(int k = 0; k < km - 1; k++){
+                if(o.solid(i, j, k)) continue;
+                const size_t q = o.id(i, j, k);
+                for(int fi = 0; fi < 4; fi++)
+                    o.f[fi][q] = o.f[ATJUP_NLEVEL + fi][q] + dt * o.acc[fi][q] * one_sixth;
+                for(int fi = ATJUP_F_H2O; fi < ATJUP_F_NH4SH; fi++)
+                    o.f[fi][q] = std::max(0.0, o.f[ATJUP_NLEVEL + fi][q] + dt * o.acc[fi][q] * one_sixth);
+                {
+                    double nh4sh_new = o.f[ATJUP_NLEVEL + ATJUP_F_NH4SH][q] + dt * o.acc[ATJUP_F_NH4SH][q] * one_sixth;
+                    if(nh4sh_new < 0.0) nh4sh_new = 0.0;
+                    o.f[ATJUP_F_NH4SH][q] = nh4sh_new;
+                }
+                o.f[ATJUP_F_TKE][q] = safe_clamp(o.f[ATJUP_NLEVEL + ATJUP_F_TKE][q]
+                    + dt * o.acc[ATJUP_F_TKE][q] * one_sixth, 0.0, tke_max_nd);
+                o.f[ATJUP_F_DIS][q] = std::max(dis_min_nd, o.f[ATJUP_NLEVEL + ATJUP_F_DIS][q]
+                    + dt * o.acc[ATJUP_F_DIS][q] * one_sixth);
+            }
+}
+
+// ---- PressureSolver<P>::run, PRESS_SRC 1, PRESS_WALL 1, rigid lid, no PRESS_INCR / LINE /
+//      BAROTROPIC (PressureSolver.h:145-766), with prepareProjectionBoundaries
+//      (cJupiterModel.cpp:1842-1864). omega != 1 blends the Gauss-Seidel value (SOR). ---------
+void pressure(orc_ctx& o, int n_sweeps, double omega){
+    const int im = o.im, jm = o.jm, km = o.km;
+    auto& au = o.f[ATJUP_F_AUX_U];
+    auto& av = o.f[ATJUP_F_AUX_V];
+    auto& aw = o.f[ATJUP_F_AUX_W];
+    auto& p = o.f[ATJUP_F_P_DYN];
+    const double c43 = o.c43, c13 = o.c13;
+    // prepareProjectionBoundaries(rigid_lid = true)
+    #pragma omp parallel for collapse(2)
+    for(int j = 1; j < jm - 1; j++)
+        for(int k = 0; k < km - 1; k++){
+            au[o.id(0, j, k)] = 0.0;
+            au[o.id(im - 1, j, k)] = 0.0;
+            av[o.id(0, j, k)] = c43 * av[o.id(1, j, k)] - c13 * av[o.id(2, j, k)];
+            av[o.id(im - 1, j, k)] = c43 * av[o.id(im - 2, j, k)] - c13 * av[o.id(im - 3, j, k)];
+            aw[o.id(0, j, k)] = c43 * aw[o.id(1, j, k)] - c13 * aw[o.id(2, j, k)];
+            aw[o.id(im - 1, j, k)] = c43 * aw[o.id(im - 2, j, k)] - c13 * aw[o.id(im - 3, j, k)];
+        }
+    std::vector<double> sinth(jm);
+    for(int j = 0; j < jm; j++){
+        sinth[j] = o.sin_the[j];
+        if(sinth[j] < o.kn.sinthe_min) sinth[j] = o.kn.sinthe_min;
+        if(sinth[j] == 0.0) sinth[j] = 1.0e-5;
+    }
+    const double inv_2dr = 1.0 / (2.0 * o.g.dr), inv_2dthe = 1.0 / (2.0 * o.g.dthe), inv_2dphi = 1.0 / (2.0 * o.g.dphi);
+    const double inv_dr2 = 1.0 / (o.g.dr * o.g.dr), inv_dthe2 = 1.0 / (o.g.dthe * o.g.dthe), inv_dphi2 = 1.0 / (o.g.dphi * o.g.dphi);
+    const double inv_dthe = 1.0 / o.g.dthe, inv_dphi = 1.0 / o.g.dphi;
+    const double cap = o.kn.pdyn_cap;
+    auto S = [&](const std::vector<double>& A, int i, int j, int k){ return A[o.id(i, j, k)]; };
+    for(int sweep = 0; sweep < n_sweeps; sweep++){
+        for(int colour = 0; colour < 2; colour++){
+            #pragma omp parallel for collapse(2) schedule(static)
+            for(int i = 1; i < im - 1; i++)
+                for(int j = 1; j < jm - 1; j++){
+                    const double rm = o.rad[i];
+                    const double rm2 = rm * rm;
+                    const double exp_rm = 1.0;
+                    const double exp_2_rm = exp_rm * exp_rm;
+                    const double sinthe = sinth[j];
+                    const double sinthe2 = sinthe * sinthe;
+                    const double inv_rm = 1.0 / rm;
+                    const double inv_rm2 = 1.0 / rm2;
+                    const double inv_rmsinthe = 1.0 / (rm * sinthe);
+                    const double inv_rm2sinthe2 = inv_rm2 / sinthe2;
+                    const double num1 = exp_2_rm * inv_dr2;
+                    const double num2 = inv_rm2 * inv_dthe2;
+                    const double num3 = inv_rm2sinthe2 * inv_dphi2;
+                    const double denom = 2.0 * num1 + 2.0 * num2 + 2.0 * num3;
+                    const bool i_in_range = (i < im - 2);
+                    const bool j_inner = (j > 2) && (j < jm - 2);
+                    for(int k = 0; k < km - 1; k++){
+                        if(((i + j + k) & 1) != colour) continue;
+                        const int kw = o.wrap(k - 1), ke = o.wrap(k + 1);
+                        const bool sld = o.solid(i, j, k);
+                        const bool skp = o.solid(i, j, ke), skm = o.solid(i, j, kw);
+                        const bool skp2 = (k < km - 2) ? o.solid(i, j, k + 2) : false;
+                        const bool skm2 = (k > 2) ? o.solid(i, j, k - 2) : false;
+                        double du_dr = 0, dv_dthe = 0, dw_dphi = 0;
+                        bool r_flag = false, the_flag = false, phi_flag = false;
+                        if(i_in_range && sld && !o.solid(i + 1, j, k)){
+                            du_dr = (-3.0 * S(au, i, j, k) + 4.0 * S(au, i + 1, j, k) - S(au, i + 2, j, k)) * inv_2dr;
+                            r_flag = true;
+                        }
+                        if(j_inner){
+                            const bool air_jp1 = !o.solid(i, j + 1, k), air_jm1 = !o.solid(i, j - 1, k);
+                            if(sld && air_jp1 && !o.solid(i, j + 2, k)){
+                                dv_dthe = (-3.0 * S(av, i, j, k) + 4.0 * S(av, i, j + 1, k) - S(av, i, j + 2, k)) * inv_2dthe;
+                                the_flag = true;
+                            }
+                            if(sld && air_jm1 && !o.solid(i, j - 2, k)){
+                                dv_dthe = -(-3.0 * S(av, i, j, k) + 4.0 * S(av, i, j - 1, k) - S(av, i, j - 2, k)) * inv_2dthe;
+                                the_flag = true;
+                            }
+                            if((sld && air_jp1 && o.solid(i, j + 2, k)) || (j == jm - 2 && !sld && o.solid(i, j + 1, k))){
+                                dv_dthe = (S(av, i, j + 1, k) - S(av, i, j, k)) * inv_dthe;
+                                the_flag = true;
+                            }
+                            if((sld && air_jm1 && o.solid(i, j - 2, k)) || (j == 1 && sld && air_jm1)){
+                                dv_dthe = (S(av, i, j - 1, k) - S(av, i, j, k)) * inv_dthe;
+                                the_flag = true;
+                            }
+                        }
+                        if((k > 2) && (k < km - 2)){
+                            const bool air_kp1 = !skp, air_km1 = !skm;
+                            if(sld && air_kp1 && !skp2){
+                                dw_dphi = (-3.0 * S(aw, i, j, k) + 4.0 * S(aw, i, j, k + 1) - S(aw, i, j, k + 2)) * inv_2dphi;
+                                phi_flag = true;
+                            }
+                            if(sld && air_km1 && !skm2){
+                                dw_dphi = -(-3.0 * S(aw, i, j, k) + 4.0 * S(aw, i, j, k - 1) - S(aw, i, j, k - 2)) * inv_2dphi;
+                                phi_flag = true;
+                            }
+                            if((sld && air_kp1 && skp2) || (k == km - 2 && !sld && skp)){
+                                dw_dphi = (S(aw, i, j, k + 1) - S(aw, i, j, k)) * inv_dphi;
+                                phi_flag = true;
+                            }
+                            if((sld && air_km1 && skm2) || (k == 1 && sld && air_km1)){
+                                dw_dphi = (S(aw, i, j, k - 1) - S(aw, i, j, k)) * inv_dphi;
+                                phi_flag = true;
+                            }
+                        }
+                        if(!r_flag) du_dr = (S(au, i + 1, j, k) - S(au, i - 1, j, k)) * inv_2dr;
+                        if(!the_flag) dv_dthe = (S(av, i, j + 1, k) - S(av, i, j - 1, k)) * inv_2dthe;
+                        if(!phi_flag) dw_dphi = (S(aw, i, j, ke) - S(aw, i, j, kw)) * inv_2dphi;
+                        double div_src = du_dr * exp_rm + dv_dthe * inv_rm + dw_dphi * inv_rmsinthe;
+                        double acc = 0.0, den = denom;
+                        if(!sld){
+                            acc = den = 0.0;
+                            if(!o.solid(i + 1, j, k)){ acc += num1 * p[o.id(i + 1, j, k)]; den += num1; }
+                            if(!o.solid(i - 1, j, k)){ acc += num1 * p[o.id(i - 1, j, k)]; den += num1; }
+                            if(!o.solid(i, j + 1, k)){ acc += num2 * p[o.id(i, j + 1, k)]; den += num2; }
+                            if(!o.solid(i, j - 1, k)){ acc += num2 * p[o.id(i, j - 1, k)]; den += num2; }
+                            if(!skp){ acc += num3 * p[o.id(i, j, ke)]; den += num3; }
+                            if(!skm){ acc += num3 * p[o.id(i, j, kw)]; den += num3; }
+                            if(den <= 0.0) continue;
+                        } else {
+                            acc = (p[o.id(i + 1, j, k)] + p[o.id(i - 1, j, k)]) * num1
+                                + (p[o.id(i, j + 1, k)] + p[o.id(i, j - 1, k)]) * num2
+                                + (p[o.id(i, j, ke)] + p[o.id(i, j, kw)]) * num3;
+                        }
+                        const double src_max = den * cap;
+                        if(!std::isfinite(div_src)) div_src = 0.0;
+                        else if(div_src > src_max) div_src = src_max;
+                        else if(div_src < -src_max) div_src = -src_max;
+                        const double p_gs = (acc - div_src) / den;
+                        const size_t q = o.id(i, j, k);
+                        p[q] = (omega == 1.0) ? p_gs : (1.0 - omega) * p[q] + omega * p_gs;
+                    }
+                }
+        }
+        #pragma omp parallel for collapse(2)
+        for(int k = 0; k < km; k++)
+            for(int j = 0; j < jm; j++){
+                p[o.id(0, j, k)] = c43 * p[o.id(1, j, k)] - c13 * p[o.id(2, j, k)];
+                p[o.id(im - 1, j, k)] = c43 * p[o.id(im - 2, j, k)] - c13 * p[o.id(im - 3, j, k)];
+            }
+        #pragma omp parallel for collapse(2)
+        for(int k = 0; k < km; k++)
+            for(int i = 0; i < im; i++){
+                p[o.id(i, 0, k)] = c43 * p[o.id(i, 1, k)] - c13 * p[o.id(i, 2, k)];
+                p[o.id(i, jm - 1, k)] = c43 * p[o.id(i, jm - 2, k)] - c13 * p[o.id(i, jm - 3, k)];
+            }
+        #pragma omp parallel for collapse(2)
+        for(int i = 0; i < im; i++)
+            for(int j = 0; j < jm; j++)
+                p[o.id(i, j, km - 1)] = p[o.id(i, j, 0)];
+    }
+}
+
+// ---- damp_wiggles / damp_wiggles_ho with i_topography, periodic seam (lib/Utils.cpp:221-453) ---
+void shapiro(orc_ctx& o, std::vector<double>& fld, int order, double strength, int passes, int axes){
+    const int im = o.im, jm = o.jm, km = o.km;
+    const double coeff = strength * 0.25;
+    const double c4 = strength / 16.0;
+    auto isurf = [&](int j, int k){ return std::max(o.itopo[(size_t)j * km + k], 0); };
+    auto in_fluid = [&](int i, int j, int k){ return i >= isurf(j, k); };
+    std::vector<double> tmp(o.n3);
+    for(int pass = 0; pass < passes; ++pass){
+        if(axes & 4){
+            tmp = fld;
+            const int n_k = km - 1;
+            #pragma omp parallel for collapse(2) schedule(static)
+            for(int i = 0; i < im; ++i)
+                for(int j = 0; j < jm; ++j){
+                    for(int k = 0; k < n_k; ++k){
+                        if(!in_fluid(i, j, k)) continue;
+                        const size_t q = o.id(i, j, k);
+                        const int km1 = (k == 0) ? km - 2 : k - 1;
+                        const int kp1 = (k == km - 2) ? 0 : k + 1;
+                        if(order == 2){
+                            const double a = in_fluid(i, j, km1) ? tmp[o.id(i, j, km1)] : tmp[q];
+                            const double b = in_fluid(i, j, kp1) ? tmp[o.id(i, j, kp1)] : tmp[q];
+                            fld[q] = tmp[q] + coeff * (a - 2.0 * tmp[q] + b);
+                        } else {
+                            const int km2 = (k - 2 + n_k) % n_k, kp2 = (k + 2) % n_k;
+                            const double c = tmp[q];
+                            if(!in_fluid(i, j, km1) || !in_fluid(i, j, kp1)) continue;
+                            if(in_fluid(i, j, km2) && in_fluid(i, j, kp2)){
+                                const double d4 = tmp[o.id(i, j, kp2)] - 4.0 * tmp[o.id(i, j, kp1)]
+                                                + 6.0 * c - 4.0 * tmp[o.id(i, j, km1)] + tmp[o.id(i, j, km2)];
+                                fld[q] = c - c4 * d4;
+                            } else {
+                                fld[q] = c + coeff * (tmp[o.id(i, j, km1)] - 2.0 * c + tmp[o.id(i, j, kp1)]);
+                            }
+                        }
+                    }
+                    fld[o.id(i, j, km - 1)] = fld[o.id(i, j, 0)];
+                }
+        }
+        if(axes & 2){
+            tmp = fld;
+            #pragma omp parallel for collapse(2) schedule(static)
+            for(int i = 0; i < im; ++i)
+                for(int j = 0; j < jm; ++j){
+                    for(int k = 0; k < km; ++k){
+                        if(!in_fluid(i, j, k)) continue;
+                        const size_t q = o.id(i, j, k);
+                        if(order == 2){
+                            const int jm1 = std::max(j - 1, 0), jp1 = std::min(j + 1, jm - 1);
+                            const double a = in_fluid(i, jm1, k) ? tmp[o.id(i, jm1, k)] : tmp[q];
+                            const double b = in_fluid(i, jp1, k) ? tmp[o.id(i, jp1, k)] : tmp[q];
+                            fld[q] = tmp[q] + coeff * (a - 2.0 * tmp[q] + b);
+                        } else {
+                            const double c = tmp[q];
+                            const bool fm1 = (j - 1 >= 0) && in_fluid(i, j - 1, k);
+                            const bool fp1 = (j + 1 <= jm - 1) && in_fluid(i, j + 1, k);
+                            if(!fm1 || !fp1) continue;
+                            const bool fm2 = (j - 2 >= 0) && in_fluid(i, j - 2, k);
+                            const bool fp2 = (j + 2 <= jm - 1) && in_fluid(i, j + 2, k);
+                            if(fm2 && fp2){
+                                const double d4 = tmp[o.id(i, j + 2, k)] - 4.0 * tmp[o.id(i, j + 1, k)]
+                                                + 6.0 * c - 4.0 * tmp[o.id(i, j - 1, k)] + tmp[o.id(i, j - 2, k)];
+                                fld[q] = c - c4 * d4;
+                            } else {
+                                fld[q] = c + coeff * (tmp[o.id(i, j - 1, k)] - 2.0 * c + tmp[o.id(i, j + 1, k)]);
+                            }
+                        }
+                    }
+                }
+        }
+        if(axes & 1){
+            tmp = fld;
+            #pragma omp parallel for collapse(2) schedule(static)
+            for(int j = 0; j < jm; ++j)
+                for(int k = 0; k < km; ++k){
+                    const int i_surf = isurf(j, k);
+                    for(int i = i_surf; i < im; ++i){
+                        const size_t q = o.id(i, j, k);
+                        if(order == 2){
+                            const int im1 = std::max(i - 1, i_surf), ip1 = std::min(i + 1, im - 1);
+                            fld[q] = tmp[q] + coeff * (tmp[o.id(im1, j, k)] - 2.0 * tmp[q] + tmp[o.id(ip1, j, k)]);
+                        } else {
+                            const double c = tmp[q];
+                            const bool fm1 = (i - 1 >= i_surf), fp1 = (i + 1 <= im - 1);
+                            if(!fm1 || !fp1) continue;
+                            const bool fm2 = (i - 2 >= i_surf), fp2 = (i + 2 <= im - 1);
+                            if(fm2 && fp2){
+                                const double d4 = tmp[o.id(i + 2, j, k)] - 4.0 * tmp[o.id(i + 1, j, k)]
+                                                + 6.0 * c - 4.0 * tmp[o.id(i - 1, j, k)] + tmp[o.id(i - 2, j, k)];
+                                fld[q] = c - c4 * d4;
+                            } else {
+                                fld[q] = c + coeff * (tmp[o.id(i - 1, j, k)] - 2.0 * c + tmp[o.id(i + 1, j, k)]);
+                            }
+                        }
+                    }
+                }
+        }
+    }
+}
+
+// ---- computeMixtureDensity (InitValues_Jup.cpp:717-740) --------------------------------------
+void mixture_density(orc_ctx& o){
+    const auto& t = o.f[ATJUP_F_T];
+    const auto& ps = o.f[ATJUP_F_P_STAT];
+    const auto& pd = o.f[ATJUP_F_P_DYN];
+    auto& rho = o.f[ATJUP_F_RHO_MIX];
+    const double pdyn_w = o.kn.rho_pdyn;
+    const double ptb = o.p_dyn_to_bar();
+    #pragma omp parallel for schedule(static)
+    for(size_t q = 0; q < o.n3; q++){
+        const double T = t[q] * o.c.t_ref;
+        if(!(T > 0.0)){ rho[q] = 0.0; continue; }
+        const double p_pa = (ps[q] + pdyn_w * pd[q] * ptb) * 1.0e5;
+        const double r = p_pa / (o.c.R_mix * T);
+        rho[q] = std::isfinite(r) ? r : 0.0;
+    }
+}
+
+const int kRadius[] = { ATJUP_F_T, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W,
+    ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE,
+    ATJUP_F_H2S, ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_NH4SH,
+    ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH,
+    ATJUP_F_TKE, ATJUP_F_DIS, ATJUP_F_NUE, ATJUP_F_PROD, ATJUP_F_TKE_SOURCE, ATJUP_F_DIS_SOURCE };
+const int kThetaExtrap[] = { ATJUP_F_T, ATJUP_F_U,
+    ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE,
+    ATJUP_F_H2S, ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_NH4SH,
+    ATJUP_F_TKE, ATJUP_F_DIS, ATJUP_F_NUE, ATJUP_F_PROD, ATJUP_F_TKE_SOURCE, ATJUP_F_DIS_SOURCE };
+const int kThetaZero[] = { ATJUP_F_V, ATJUP_F_W,
+    ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH };
+const int kPhi[] = { ATJUP_F_T, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W,
+    ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE,
+    ATJUP_F_H2S, ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_NH4SH,
+    ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH,
+    ATJUP_F_TKE, ATJUP_F_DIS, ATJUP_F_NUE, ATJUP_F_PROD, ATJUP_F_TKE_SOURCE, ATJUP_F_DIS_SOURCE };
+const int kSpecies[] = { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE, ATJUP_F_H2S,
+    ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_NH4SH };
+const int kSolid[] = { ATJUP_F_T, ATJUP_F_P_STAT, ATJUP_F_P_DYN,
+    ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE,
+    ATJUP_F_H2S, ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_NH4SH,
+    ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH };
+
+// ---- BC_Jup::bcRadius: shared bcRadius (BoundaryConditions.h:151-286) with TOP_TAPER 3
+//      (BC_Jup.h:185-263) -------------------------------------------------------------------------
+void bc_radius(orc_ctx& o){
+    const int im = o.im, jm = o.jm, km = o.km, kn = km - 1;
+    auto& V = o.f[ATJUP_F_V];
+    auto& W = o.f[ATJUP_F_W];
+    std::vector<double> zm((size_t)4 * jm, 0.0);
+    #pragma omp parallel for
+    for(int j = 0; j < jm; j++)
+        for(int l = 0; l < 2; l++){
+            const int i = im - 3 + l;
+            double sv = 0.0, sw = 0.0;
+            for(int k = 0; k < kn; k++){ sv += V[o.id(i, j, k)]; sw += W[o.id(i, j, k)]; }
+            zm[(size_t)(2 * l) * jm + j] = sv / kn;
+            zm[(size_t)(2 * l + 1) * jm + j] = sw / kn;
+        }
+    const double c43 = o.c43, c13 = o.c13;
+    #pragma omp parallel for
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++)
+            for(int fid : kRadius){
+                auto& X = o.f[fid];
+                X[o.id(0, j, k)] = c43 * X[o.id(1, j, k)] - c13 * X[o.id(2, j, k)];
+                X[o.id(im - 1, j, k)] = c43 * X[o.id(im - 2, j, k)] - c13 * X[o.id(im - 3, j, k)];
+            }
+    auto& U = o.f[ATJUP_F_U];
+    #pragma omp parallel for
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++){ U[o.id(0, j, k)] = 0.0; U[o.id(im - 1, j, k)] = 0.0; }
+    #pragma omp parallel for
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++){
+            V[o.id(im - 1, j, k)] = 0.0;
+            W[o.id(im - 1, j, k)] = 0.0;
+            V[o.id(im - 2, j, k)] *= (1.0 / 3.0);
+            W[o.id(im - 2, j, k)] *= (1.0 / 3.0);
+            V[o.id(im - 3, j, k)] *= (2.0 / 3.0);
+            W[o.id(im - 3, j, k)] *= (2.0 / 3.0);
+        }
+    #pragma omp parallel for
+    for(int j = 0; j < jm; j++){
+        const double v3 = zm[j], w3 = zm[(size_t)jm + j], v2 = zm[(size_t)2 * jm + j], w2 = zm[(size_t)3 * jm + j];
+        for(int k = 0; k < km; k++){
+            V[o.id(im - 3, j, k)] += (1.0 / 3.0) * v3;
+            W[o.id(im - 3, j, k)] += (1.0 / 3.0) * w3;
+            V[o.id(im - 2, j, k)] += (2.0 / 3.0) * v2;
+            W[o.id(im - 2, j, k)] += (2.0 / 3.0) * w2;
+            V[o.id(im - 1, j, k)] = v2;
+            W[o.id(im - 1, j, k)] = w2;
+        }
+    }
+    #pragma omp parallel for
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++)
+            for(int fid : kSpecies){
+                auto& X = o.f[fid];
+                X[o.id(0, j, k)] = X[o.id(1, j, k)];
+            }
+    #pragma omp parallel for
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++)
+            for(int fid : kSpecies){
+                auto& X = o.f[fid];
+                if(X[o.id(0, j, k)] < 0.0) X[o.id(0, j, k)] = 0.0;
+                if(X[o.id(im - 1, j, k)] < 0.0) X[o.id(im - 1, j, k)] = 0.0;
+            }
+}
+
+// ---- bcTheta with POLE_FILL (2 open rows) and POLE_COPY (BoundaryConditions.h:292-391) --------
+void bc_theta(orc_ctx& o){
+    const int im = o.im, jm = o.jm, km = o.km;
+    const int nfill = 2, jf = nfill + 1;
+    #pragma omp parallel for
+    for(int k = 0; k < km; k++)
+        for(int i = 0; i < im; i++){
+            for(int r = 1; r <= nfill; r++){
+                const double ramp = (double)r / (double)jf;
+                for(int fid : kThetaExtrap){
+                    auto& X = o.f[fid];
+                    X[o.id(i, r, k)] = X[o.id(i, jf, k)];
+                    X[o.id(i, jm - 1 - r, k)] = X[o.id(i, jm - 1 - jf, k)];
+                }
+                for(int fid : kThetaZero){
+                    auto& X = o.f[fid];
+                    X[o.id(i, r, k)] = ramp * X[o.id(i, jf, k)];
+                    X[o.id(i, jm - 1 - r, k)] = ramp * X[o.id(i, jm - 1 - jf, k)];
+                }
+            }
+            for(int fid : kThetaZero){
+                auto& X = o.f[fid];
+                X[o.id(i, 0, k)] = 0.0;
+                X[o.id(i, jm - 1, k)] = 0.0;
+            }
+            for(int fid : kThetaExtrap){
+                auto& X = o.f[fid];
+                X[o.id(i, 0, k)] = X[o.id(i, 1, k)];
+                X[o.id(i, jm - 1, k)] = X[o.id(i, jm - 2, k)];
+            }
+        }
+}
+
+// ---- bcPhi, periodic seam: k = km-1 is the copy of k = 0 (BoundaryConditions.h:398-439) --------
+void bc_phi(orc_ctx& o){
+    const int im = o.im, jm = o.jm, km = o.km;
+    #pragma omp parallel for
+    for(int i = 0; i < im; i++)
+        for(int j = 0; j < jm; j++)
+            for(int fid : kPhi){
+                auto& X = o.f[fid];
+                X[o.id(i, j, km - 1)] = X[o.id(i, j, 0)];
+            }
+}
+
+// ---- BC_Jup::bcSolidGround with MOUNT_T_AMBIENT (BC_Jup.h:460-658) --------------------------
+void bc_solid(orc_ctx& o){
+    const int im = o.im, jm = o.jm, km = o.km;
+    const int ns = (int)(sizeof(kSolid) / sizeof(kSolid[0]));
+    const int n_state = 2;
+    const double c43 = o.c43, c13 = o.c13;
+    auto S = [&](int ii, int jj, int kk){
+        if(ii < 0 || ii >= im || jj < 0 || jj >= jm || kk < 0 || kk >= km) return true;
+        return o.solid(ii, jj, kk);
+    };
+    auto Fl = [&](int ii, int jj, int kk){
+        if(ii < 0 || ii >= im || jj < 0 || jj >= jm || kk < 0 || kk >= km) return false;
+        return !o.solid(ii, jj, kk);
+    };
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++)
+            for(int i = 0; i < im; i++){
+                if(!o.solid(i, j, k)) continue;
+                const size_t q = o.id(i, j, k);
+                o.f[ATJUP_F_U][q] = 0.0;
+                o.f[ATJUP_F_V][q] = 0.0;
+                o.f[ATJUP_F_W][q] = 0.0;
+                auto extrap = [&](int ia, int ja, int ka, int ib, int jb, int kb){
+                    for(int f = 0; f < ns; f++){
+                        auto& X = o.f[kSolid[f]];
+                        X[q] = c43 * X[o.id(ia, ja, ka)] - c13 * X[o.id(ib, jb, kb)];
+                    }
+                };
+                const bool surf = !S(i + 1, j, k) || !S(i - 1, j, k) || !S(i, j + 1, k)
+                               || !S(i, j - 1, k) || !S(i, j, k + 1) || !S(i, j, k - 1);
+                if(!surf){
+                    for(int f = n_state; f < ns; f++) o.f[kSolid[f]][q] = 0.0;
+                    continue;
+                }
+                if(Fl(i + 1, j, k) && Fl(i + 2, j, k)) extrap(i + 1, j, k, i + 2, j, k);
+                else if(Fl(i - 1, j, k) && Fl(i - 2, j, k)) extrap(i - 1, j, k, i - 2, j, k);
+                else if(Fl(i, j + 1, k) && Fl(i, j + 2, k)) extrap(i, j + 1, k, i, j + 2, k);
+                else if(Fl(i, j - 1, k) && Fl(i, j - 2, k)) extrap(i, j - 1, k, i, j - 2, k);
+                else if(Fl(i, j, k + 1) && Fl(i, j, k + 2)) extrap(i, j, k + 1, i, j, k + 2);
+                else if(Fl(i, j, k - 1) && Fl(i, j, k - 2)) extrap(i, j, k - 1, i, j, k - 2);
+                else for(int f = n_state; f < ns; f++) o.f[kSolid[f]][q] = 0.0;
+            }
+    auto& T = o.f[ATJUP_F_T];
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int i = 0; i < im; i++)
+        for(int j = 0; j < jm; j++){
+            double s_ = 0.0; int n_ = 0; bool any_ = false;
+            for(int k = 0; k < km; k++){
+                if(o.solid(i, j, k)){ any_ = true; continue; }
+                s_ += T[o.id(i, j, k)]; n_++;
+            }
+            if(!any_ || n_ == 0) continue;
+            const double t_amb = s_ / n_;
+            for(int k = 0; k < km; k++) if(o.solid(i, j, k)) T[o.id(i, j, k)] = t_amb;
+        }
+    const double tke_max_nd = 1000.0 / (o.c.u_0 * o.c.u_0);
+    const double nue_max = 1.0e5 / (o.c.u_0 * o.g.L_atm * 1.0e3);
+    const double prod_max = 1.0e4, dis_max = 1.0e6;
+    #pragma omp parallel for schedule(static)
+    for(size_t q = 0; q < o.n3; q++){
+        o.f[ATJUP_F_TKE][q] = safe_clamp(o.f[ATJUP_F_TKE][q], 0.0, tke_max_nd);
+        o.f[ATJUP_NLEVEL + ATJUP_F_TKE][q] = safe_clamp(o.f[ATJUP_NLEVEL + ATJUP_F_TKE][q], 0.0, tke_max_nd);
+        o.f[ATJUP_F_DIS][q] = safe_clamp(o.f[ATJUP_F_DIS][q], 1.0e-10, dis_max);
+        o.f[ATJUP_NLEVEL + ATJUP_F_DIS][q] = safe_clamp(o.f[ATJUP_NLEVEL + ATJUP_F_DIS][q], 1.0e-10, dis_max);
+        o.f[ATJUP_F_NUE][q] = safe_clamp(o.f[ATJUP_F_NUE][q], 0.0, nue_max);
+        o.f[ATJUP_F_PROD][q] = safe_clamp(o.f[ATJUP_F_PROD][q], 0.0, prod_max);
+        o.f[ATJUP_F_TKE_SOURCE][q] = safe_clamp(o.f[ATJUP_F_TKE_SOURCE][q], -prod_max, prod_max);
+        o.f[ATJUP_F_DIS_SOURCE][q] = safe_clamp(o.f[ATJUP_F_DIS_SOURCE][q], -prod_max, prod_max);
+    }
+}
+
+// ---- clampNegativeSpecies (FileIO_Jup.cpp:494-548) -------------------------------------------
+void clamp_species(orc_ctx& o){
+    for(int fid : kSpecies){
+        auto& X = o.f[fid];
+        #pragma omp parallel for schedule(static)
+        for(size_t q = 0; q < o.n3; q++) if(!(X[q] >= 0.0)) X[q] = 0.0;
+    }
+}
+
+// ---- restoreVar(1.0) (cJupiterModel.cpp:1757-1801) --------------------------------------------
+void restore(orc_ctx& o){
+    for(int fi = 0; fi < NP; fi++){
+        auto& N = o.f[ATJUP_NLEVEL + fi];
+        const auto& X = o.f[fi];
+        #pragma omp parallel for schedule(static)
+        for(size_t q = 0; q < o.n3; q++) N[q] = 1.0 * X[q];
+    }
+}
+
+// ---- ATJUP_RK_SCALAR_SYNC 2 (cJupiterModel.cpp:1118-1140) -------------------------------------
+void scalar_sync(orc_ctx& o){
+    const int ids[] = { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE, ATJUP_F_NH3, ATJUP_F_NH3_CLOUD,
+                        ATJUP_F_NH3_ICE, ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_T };
+    const int n = (o.kn.rk_scalar_sync >= 2) ? 10 : 9;
+    for(int a = 0; a < n; a++){
+        auto& N = o.f[ATJUP_NLEVEL + ids[a]];
+        const auto& X = o.f[ids[a]];
+        #pragma omp parallel for schedule(static)
+        for(size_t q = 0; q < o.n3; q++) N[q] = X[q];
+    }
+}
+
+void wall_viscosity(orc_ctx& o){                          // InitValues_Jup.cpp:84-134
+    auto& wn = o.f[ATJUP_F_WALL_NUE];
+    std::fill(wn.begin(), wn.end(), 0.0);
+    const double factor = o.kn.wall_nue_factor;
+    const int layers = o.kn.wall_nue_layers;
+    if(factor <= 0.0 || layers <= 0) return;
+    const int im = o.im, jm = o.jm, km = o.km;
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++)
+            for(int i = 0; i < im; i++){
+                if(o.solid(i, j, k)) continue;
+                int d = layers + 1;
+                for(int di = -layers; di <= layers && d > 1; di++){
+                    const int ii = i + di;
+                    if(ii < 0 || ii >= im) continue;
+                    for(int dj = -layers; dj <= layers && d > 1; dj++){
+                        const int jj = j + dj;
+                        if(jj < 0 || jj >= jm) continue;
+                        for(int dk = -layers; dk <= layers; dk++){
+                            const int kk = k + dk;
+                            if(kk < 0 || kk >= km) continue;
+                            if(!o.solid(ii, jj, kk)) continue;
+                            const int ad = std::max(std::abs(di), std::max(std::abs(dj), std::abs(dk)));
+                            if(ad < d) d = ad;
+                            if(d <= 1) break;
+                        }
+                    }
+                }
+                if(d > layers) continue;
+                const double ramp = (double)(layers - d + 1) / (double)layers;
+                wn[o.id(i, j, k)] = factor / o.c.re * ramp;
+            }
+}
+
+bool valid_field(int f){
+    return (f >= 0 && f < ATJUP_N_PROG) || (f >= ATJUP_F_P_DYN && f < ATJUP_F_END)
+        || (f >= ATJUP_NLEVEL && f < ATJUP_NLEVEL + ATJUP_N_PROG);
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_set_threads(int n){
+#ifdef _OPENMP
+    if(n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+orc_ctx* orc_create(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k){
+    if(!g || !c || !k || g->im < 5 || g->jm < 8 || g->km < 6) return nullptr;
+    orc_ctx* o = new orc_ctx();
+    o->g = *g; o->c = *c; o->kn = *k;
+    o->im = g->im; o->jm = g->jm; o->km = g->km;
+    o->n3 = (size_t)o->im * o->jm * o->km;
+    o->rad.resize(o->im);
+    o->the.resize(o->jm);
+    double z = g->r0;
+    o->rad[0] = z;
+    for(int i = 1; i < o->im; i++){ z = z + g->dr; o->rad[i] = z; }
+    z = 0.0;
+    o->the[0] = z;
+    for(int j = 1; j < o->jm; j++){ z = z + g->dthe; o->the[j] = z; }
+    o->sin_the.resize(o->jm); o->cos_the.resize(o->jm); o->sinthe_rk.resize(o->jm);
+    for(int j = 0; j < o->jm; j++){
+        o->sin_the[j] = std::sin(o->the[j]);
+        o->cos_the[j] = std::cos(o->the[j]);
+        o->sinthe_rk[j] = std::max(k->sinthe_min, std::abs(std::sin(o->the[j])));
+    }
+    const float h = (float)(g->L_atm * g->dr);
+    o->lh.resize(o->im);
+    for(int i = 0; i < o->im; i++) o->lh[i] = i * h;
+    o->f.assign(ATJUP_NLEVEL + ATJUP_N_PROG, {});
+    for(int fi = 0; fi < ATJUP_NLEVEL + ATJUP_N_PROG; fi++)
+        if(valid_field(fi)) o->f[fi].assign(o->n3, 0.0);
+    o->f[ATJUP_F_DIS].assign(o->n3, 1.0e-10);
+    o->f[ATJUP_NLEVEL + ATJUP_F_DIS].assign(o->n3, 1.0e-10);
+    o->f[ATJUP_F_RHO_MIX].assign(o->n3, 1.0);
+    o->rhs.assign(NP, std::vector<double>(o->n3, 0.0));
+    o->acc.assign(NP, std::vector<double>(o->n3, 0.0));
+    o->mask.assign(o->n3, 0.0);
+    o->itopo.assign((size_t)o->jm * o->km, 0);
+    o->vel_star.assign((size_t)o->jm * o->km, 0.0);
+    o->brl.assign(o->im, 0.0);
+    return o;
+}
+
+void orc_destroy(orc_ctx* o){ delete o; }
+
+int orc_upload_field(orc_ctx* o, int f, const double* src){
+    if(!o || !src || !valid_field(f)) return ATJUP_ERR_ARG;
+    std::memcpy(o->f[f].data(), src, sizeof(double) * o->n3);
+    return 0;
+}
+int orc_download_field(orc_ctx* o, int f, double* dst){
+    if(!o || !dst || !valid_field(f)) return ATJUP_ERR_ARG;
+    std::memcpy(dst, o->f[f].data(), sizeof(double) * o->n3);
+    return 0;
+}
+int orc_upload_mask(orc_ctx* o, const double* m){
+    if(!o || !m) return ATJUP_ERR_ARG;
+    std::memcpy(o->mask.data(), m, sizeof(double) * o->n3);
+    for(int j = 0; j < o->jm; j++)
+        for(int k = 0; k < o->km; k++){
+            int s = 0;
+            while(s < o->im && o->solid(s, j, k)) ++s;
+            o->itopo[(size_t)j * o->km + k] = s;
+        }
+    wall_viscosity(*o);
+    return 0;
+}
+int orc_upload_vel_star(orc_ctx* o, const double* s){
+    if(!o || !s) return ATJUP_ERR_ARG;
+    std::memcpy(o->vel_star.data(), s, sizeof(double) * o->jm * o->km);
+    return 0;
+}
+int orc_download_vel_star(orc_ctx* o, double* d){
+    if(!o || !d) return ATJUP_ERR_ARG;
+    std::memcpy(d, o->vel_star.data(), sizeof(double) * o->jm * o->km);
+    return 0;
+}
+int orc_download_buoy_ref_level(orc_ctx* o, double* d){
+    if(!o || !d) return ATJUP_ERR_ARG;
+    std::memcpy(d, o->brl.data(), sizeof(double) * o->im);
+    return 0;
+}
+int orc_rk4_step(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; rk4(*o); return 0; }
+int orc_buoy_ref_level(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; buoy_ref_level(*o); return 0; }
+int orc_hydrostatic(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; buoy_ref_level(*o); hydrostatic(*o); return 0; }
+int orc_pressure_solve(orc_ctx* o, int sweeps, double omega){
+    if(!o) return ATJUP_ERR_ARG;
+    pressure(*o, sweeps > 0 ? sweeps : std::max(1, o->kn.press_sweeps), omega);
+    return 0;
+}
+int orc_bc_radius(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; bc_radius(*o); return 0; }
+int orc_bc_theta(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; bc_theta(*o); return 0; }
+int orc_bc_phi(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; bc_phi(*o); return 0; }
+int orc_bc_solid(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; bc_solid(*o); return 0; }
+int orc_bc_after_step(orc_ctx* o){
+    if(!o) return ATJUP_ERR_ARG;
+    bc_radius(*o); bc_theta(*o); bc_phi(*o); bc_solid(*o);
+    return 0;
+}
+int orc_shapiro(orc_ctx* o, int f, int order, double strength, int passes, int axes){
+    if(!o || !valid_field(f) || (order != 2 && order != 4)) return ATJUP_ERR_ARG;
+    shapiro(*o, o->f[f], order, strength, passes, axes);
+    return 0;
+}
+int orc_mixture_density(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; mixture_density(*o); return 0; }
+int orc_clamp_species(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; clamp_species(*o); return 0; }
+int orc_swap_nlevel(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; restore(*o); return 0; }
+int orc_scalar_sync(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; scalar_sync(*o); return 0; }
+int orc_satadj(orc_ctx* o, int gas){ (void)o; (void)gas; return ATJUP_ERR_PROFILE; }
+int orc_turbulence_run(orc_ctx* o){ (void)o; return ATJUP_ERR_PROFILE; }
+int orc_flux_limiter(orc_ctx* o){ (void)o; return ATJUP_ERR_PROFILE; }
+
+// One iteration of the loop body, on-path functions only (cJupiterModel.cpp:1010-1555).
+int orc_step(orc_ctx* o, int iter, int first){
+    if(!o) return ATJUP_ERR_ARG;
+    if(iter % 2 == 0 || first){
+        pressure(*o, std::max(1, o->kn.press_sweeps), o->kn.press_omega);
+        if(o->kn.damp_pdyn) shapiro(*o, o->f[ATJUP_F_P_DYN], 2, 1.0, 1, 7);
+        mixture_density(*o);
+        if(o->kn.rk_scalar_sync >= 1) scalar_sync(*o);
+    }
+    rk4(*o);
+    bc_radius(*o); bc_theta(*o); bc_phi(*o); bc_solid(*o);
+    clamp_species(*o);
+    restore(*o);
+    return 0;
+}
+
+int orc_check_finite(orc_ctx* o){
+    if(!o) return ATJUP_ERR_ARG;
+    for(int fi = 0; fi < NP; fi++)
+        for(double v : o->f[fi]) if(!std::isfinite(v)) return ATJUP_NONFINITE;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,287 @@
+// Test harness around the reference model, compiled against the reference's own headers and
+// linked with its translation units (see oracle/Makefile, target `ref`). It exists only to produce
+// golden vectors and a CPU baseline: it never ships with the library and no product code loads it.
+//
+// Access to the model's private members goes through the friend declaration the model already
+// makes for every specialization of ProcessDiag (planet/cJupiterModel.h:80): the harness defines
+// its own specialization, ProcessDiag<HarnessTag>, and does all member access from there.
+//
+// Exported C ABI (all contiguous buffers are [i][j][k], k fastest, im*jm*km doubles):
+//   ref_init(profile)          set the knob profile ("KEY=VAL;KEY=VAL"), build the model and its
+//                              constants the way cJupiterModel::Run() does before the loop
+//   ref_set_field/ref_get_field(name, buf)
+//   ref_set_scalar/ref_get_scalar(name, value)
+//   ref_set_mask(seamount)     obstacle mask, i_topography and wall_nue (computeWallViscosity)
+//   ref_call(op)               one reference entry point, by name (see the table in ref_call)
+#include "cJupiterModel.h"
+
+#include "ChemistryJup.h"
+#include "PressureSolverJup.h"
+#include "SaturationAdjustmentJup.h"
+#include "BC_Jup.h"
+#include "TurbulenceJup.h"
+
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+struct HarnessTag {};
+
+template<>
+class ProcessDiag<HarnessTag> {
+public:
+    static cJupiterModel* m;
+
+    static std::map<std::string, Array*> fields(){
+        cJupiterModel& M = *m;
+        return {
+            {"t", &M.t}, {"u", &M.u}, {"v", &M.v}, {"w", &M.w},
+            {"tn", &M.tn}, {"un", &M.un}, {"vn", &M.vn}, {"wn", &M.wn},
+            {"h2o", &M.h2o}, {"h2o_cloud", &M.h2o_cloud}, {"h2o_ice", &M.h2o_ice},
+            {"h2s", &M.h2s}, {"nh3", &M.nh3}, {"nh3_cloud", &M.nh3_cloud}, {"nh3_ice", &M.nh3_ice},
+            {"ch4", &M.ch4}, {"ch4_cloud", &M.ch4_cloud}, {"ch4_ice", &M.ch4_ice}, {"nh4sh", &M.nh4sh},
+            {"h2on", &M.h2on}, {"h2o_cloudn", &M.h2o_cloudn}, {"h2o_icen", &M.h2o_icen},
+            {"h2sn", &M.h2sn}, {"nh3n", &M.nh3n}, {"nh3_cloudn", &M.nh3_cloudn}, {"nh3_icen", &M.nh3_icen},
+            {"ch4n", &M.ch4n}, {"ch4_cloudn", &M.ch4_cloudn}, {"ch4_icen", &M.ch4_icen}, {"nh4shn", &M.nh4shn},
+            {"tke", &M.tke}, {"dis", &M.dis}, {"tken", &M.tken}, {"disn", &M.disn},
+            {"nue", &M.nue}, {"prod", &M.prod}, {"tke_source", &M.tke_source}, {"dis_source", &M.dis_source},
+            {"p_dyn", &M.p_dyn}, {"p_dynn", &M.p_dynn}, {"p_stat", &M.p_stat}, {"p_hydro", &M.p_hydro},
+            {"rho_mix", &M.rho_mix}, {"wall_nue", &M.wall_nue}, {"SeaMount", &M.SeaMount},
+            {"aux_u", &M.aux_u}, {"aux_v", &M.aux_v}, {"aux_w", &M.aux_w},
+            {"rhs_t", &M.rhs_t}, {"rhs_u", &M.rhs_u}, {"rhs_v", &M.rhs_v}, {"rhs_w", &M.rhs_w},
+            {"rhs_tke", &M.rhs_tke}, {"rhs_dis", &M.rhs_dis}, {"rhs_h2o", &M.rhs_h2o},
+            {"acc_t", &M.acc_t}, {"acc_u", &M.acc_u}, {"acc_v", &M.acc_v}, {"acc_w", &M.acc_w},
+            {"Q_rad", &M.Q_rad}, {"Q_precip", &M.Q_precip},
+            {"S_precip_h2o", &M.S_precip_h2o}, {"S_precip_h2o_cloud", &M.S_precip_h2o_cloud},
+            {"S_precip_h2o_ice", &M.S_precip_h2o_ice}, {"S_precip_nh3", &M.S_precip_nh3},
+            {"S_precip_nh3_cloud", &M.S_precip_nh3_cloud}, {"S_precip_nh3_ice", &M.S_precip_nh3_ice},
+            {"S_precip_ch4", &M.S_precip_ch4}, {"S_precip_ch4_cloud", &M.S_precip_ch4_cloud},
+            {"S_precip_ch4_ice", &M.S_precip_ch4_ice},
+            {"massflux_h2s", &M.massflux_h2s}, {"massflux_nh3", &M.massflux_nh3},
+            {"massflux_nh4sh", &M.massflux_nh4sh}, {"fluxlim_nh4sh", &M.fluxlim_nh4sh},
+        };
+    }
+
+    static double* scalar(const std::string& n){
+        cJupiterModel& M = *m;
+        static std::map<std::string, double*> s;
+        if(s.empty()){
+            s = {
+                {"Coriolis", &M.Coriolis}, {"centrifugal", &M.centrifugal}, {"buoyancy", &M.buoyancy},
+                {"chemical_reaction", &M.chemical_reaction}, {"L_atm", &M.L_atm}, {"re", &M.re},
+                {"pr", &M.pr}, {"g", &M.g}, {"omega", &M.omega}, {"u_0", &M.u_0}, {"t_ref", &M.t_ref},
+                {"gam", &M.gam}, {"p_ref", &M.p_ref}, {"R_ref", &M.R_ref}, {"dt", &M.dt},
+                {"r_mix", &M.r_mix}, {"R_mix", &M.R_mix}, {"cp_mix", &M.cp_mix}, {"mue_mix", &M.mue_mix},
+                {"sc_h2o", &M.sc_h2o}, {"sc_h2s", &M.sc_h2s}, {"sc_nh3", &M.sc_nh3},
+                {"sc_nh4sh", &M.sc_nh4sh}, {"sc_ch4", &M.sc_ch4}, {"rg_nh4sh", &M.rg_nh4sh},
+                {"r_p_nh4sh", &M.r_p_nh4sh}, {"re_turb", &M.re_turb}, {"t_pole", &M.t_pole},
+                {"t_equator", &M.t_equator}, {"abl_height", &M.abl_height},
+            };
+        }
+        auto it = s.find(n);
+        return it == s.end() ? nullptr : it->second;
+    }
+
+    static void set_mask(const double* src){
+        cJupiterModel& M = *m;
+        const int im = M.im, jm = M.jm, km = M.km;
+        for(int i = 0; i < im; i++) for(int j = 0; j < jm; j++) for(int k = 0; k < km; k++)
+            M.SeaMount.x[i][j][k] = src[((size_t)i * jm + j) * km + k];
+        M.i_topography.assign(jm, std::vector<int>(km, 0));
+        for(int j = 0; j < jm; j++) for(int k = 0; k < km; k++){
+            int s = 0;
+            while(s < im && M.SeaMount.x[s][j][k] == 1.0) ++s;
+            M.i_topography[j][k] = s;
+        }
+        M.computeWallViscosity();
+    }
+
+    static void init(){
+        m = new cJupiterModel();
+        cJupiterModel& M = *m;
+        // Run(): the closure switch (turb_switch, cJupiterModel.cpp:73-78)
+        {
+            const char* e = getenv("ATJUP_TURB");
+            const char* mm = getenv("ATJUP_TURB_MODEL");
+            const std::string s = mm ? std::string(mm) : M.turb_model;
+            M.use_turbulence_model = e ? (atoi(e) != 0) : !(s.empty() || s == "none" || s == "laminar");
+        }
+        JupiterUtils::set_phi_seam_periodic(ATPhys::phi_periodic<cJupiterModel>());
+        // Run(): the metric radius (cJupiterModel.cpp:441-468)
+        {
+            const char* e = getenv("ATJUP_METRIC_RADIUS");
+            const double R = e ? atof(e) : cJupiterModel::R_planet_km;
+            if(R > 0.0 && !M.coord_stretching) M.rad.Coordinates(M.im, R / M.L_atm, M.dr);
+        }
+        M.resetArrays();
+        {
+            const char* e = getenv("ATJUP_DT");
+            M.dt = e ? atof(e) : 0.005;
+        }
+        M.iter_n = 0;
+        M.init_layer_heights();
+        M.i_topography.assign(M.jm, std::vector<int>(M.km, 0));
+        M.computeWallViscosity();
+        // Run(): the mixture constants (cJupiterModel.cpp:590-597)
+        if(M.svp_fit()) M.setSvpFit();
+        if(M.composition()) M.setComposition();
+        ChemistryJup(M).ThermalPropertiesJup();
+        if(M.composition()) M.setCompositionEp();
+    }
+
+    static int call(const std::string& op){
+        cJupiterModel& M = *m;
+        if(op == "rk4")                 M.RungeKuttaJup();
+        else if(op == "buoy_ref_level") M.computeBuoyancyRefLevel();
+        else if(op == "hydrostatic")    { M.computeBuoyancyRefLevel(); M.computeHydrostaticPressure(true); }
+        else if(op == "pressure")       PressureSolverJup(M).run();
+        else if(op == "damp_pdyn")      JupiterUtils::damp_wiggles(M.p_dyn, &M.i_topography, true, true, true);
+        else if(op == "mixture_density") M.computeMixtureDensity();
+        else if(op == "bc_radius")      BC_Jup(M).bcRadius();
+        else if(op == "bc_theta")       BC_Jup(M).bcTheta();
+        else if(op == "bc_phi")         BC_Jup(M).bcPhi();
+        else if(op == "bc_solid")       BC_Jup(M).bcSolidGround();
+        else if(op == "bc_all")         { BC_Jup(M).bcRadius(); BC_Jup(M).bcTheta(); BC_Jup(M).bcPhi(); BC_Jup(M).bcSolidGround(); }
+        else if(op == "clamp_species")  M.clampNegativeSpecies();
+        else if(op == "restore")        M.restoreVar(1.0);
+        else if(op == "flux_limiter")   ChemistryJup(M).FluxLimiterNH4SH();
+        else if(op == "turbulence")     TurbulenceJup(M).run();
+        else if(op == "turb_wall")      TurbulenceJup(M).apply_wall_bc();
+        else if(op == "satadj_h2o")
+            SaturationAdjustmentJup(M).run("H2O", M.t_0_h2o, M.t_00_h2o, M.ep_h2o, M.lv_h2o, M.ls_h2o,
+                M.C_h2o, M.L0_h2o, M.R_h2o, M.del_alf_h2o, M.del_bet_h2o,
+                M.C_h2o_ice, M.L0_h2o_ice, M.del_alf_h2o_ice, M.del_bet_h2o_ice,
+                M.h2o, M.h2o_cloud, M.h2o_ice);
+        else if(op == "satadj_nh3")
+            SaturationAdjustmentJup(M).run("NH3", M.t_0_nh3, M.t_00_nh3, M.ep_nh3, M.lv_nh3, M.ls_nh3,
+                M.C_nh3, M.L0_nh3, M.R_nh3, M.del_alf_nh3, M.del_bet_nh3,
+                M.C_nh3_ice, M.L0_nh3_ice, M.del_alf_nh3_ice, M.del_bet_nh3_ice,
+                M.nh3, M.nh3_cloud, M.nh3_ice);
+        else if(op == "satadj_ch4")
+            SaturationAdjustmentJup(M).run("CH4", M.t_0_ch4, M.t_00_ch4, M.ep_ch4, M.lv_ch4, M.ls_ch4,
+                M.C_ch4, M.L0_ch4, M.R_ch4, M.del_alf_ch4, M.del_bet_ch4,
+                M.C_ch4_ice, M.L0_ch4_ice, M.del_alf_ch4_ice, M.del_bet_ch4_ice,
+                M.ch4, M.ch4_cloud, M.ch4_ice);
+        else if(op == "iter_inc")       M.iter_n++;
+        else return -1;
+        return 0;
+    }
+
+    static double buoy_ref_level(int i){
+        return (i >= 0 && i < (int)m->buoy_ref_level.size()) ? m->buoy_ref_level[i] : 0.0;
+    }
+    static double coord(int axis, int n){
+        if(axis == 0) return m->rad.z[n];
+        if(axis == 1) return m->the.z[n];
+        return m->phi.z[n];
+    }
+    static void dims(int* im, int* jm, int* km){ *im = m->im; *jm = m->jm; *km = m->km; }
+    static double* vel_star_row(int j){ return m->vel_star.y[j]; }
+};
+
+cJupiterModel* ProcessDiag<HarnessTag>::m = nullptr;
+typedef ProcessDiag<HarnessTag> H;
+
+static int g_quiet = 1;
+static int g_saved_stdout = -1;
+static void quiet_begin(){
+    if(!g_quiet) return;
+    fflush(stdout);
+    std::cout.flush();
+    g_saved_stdout = dup(1);
+    const int dn = open("/dev/null", O_WRONLY);
+    dup2(dn, 1);
+    close(dn);
+}
+static void quiet_end(){
+    if(g_saved_stdout < 0) return;
+    fflush(stdout);
+    std::cout.flush();
+    dup2(g_saved_stdout, 1);
+    close(g_saved_stdout);
+    g_saved_stdout = -1;
+}
+
+extern "C" {
+
+void ref_quiet(int on){ g_quiet = on; }
+
+int ref_init(const char* profile){
+    if(H::m) return -1;                       // the knobs are cached function statics: one profile per process
+    std::string p(profile ? profile : "");
+    size_t pos = 0;
+    while(pos < p.size()){
+        size_t end = p.find(';', pos);
+        if(end == std::string::npos) end = p.size();
+        const std::string kv = p.substr(pos, end - pos);
+        const size_t eq = kv.find('=');
+        if(eq != std::string::npos) setenv(kv.substr(0, eq).c_str(), kv.substr(eq + 1).c_str(), 1);
+        pos = end + 1;
+    }
+    quiet_begin();
+    H::init();
+    quiet_end();
+    return 0;
+}
+
+void ref_dims(int* im, int* jm, int* km){ H::dims(im, jm, km); }
+
+int ref_set_field(const char* name, const double* src){
+    auto f = H::fields();
+    auto it = f.find(name);
+    if(it == f.end()) return -1;
+    Array& A = *it->second;
+    for(int i = 0; i < A.im; i++) for(int j = 0; j < A.jm; j++)
+        std::memcpy(A.x[i][j], src + ((size_t)i * A.jm + j) * A.km, sizeof(double) * A.km);
+    return 0;
+}
+
+int ref_get_field(const char* name, double* dst){
+    auto f = H::fields();
+    auto it = f.find(name);
+    if(it == f.end()) return -1;
+    Array& A = *it->second;
+    for(int i = 0; i < A.im; i++) for(int j = 0; j < A.jm; j++)
+        std::memcpy(dst + ((size_t)i * A.jm + j) * A.km, A.x[i][j], sizeof(double) * A.km);
+    return 0;
+}
+
+int ref_set_scalar(const char* name, double v){
+    double* p = H::scalar(name);
+    if(!p) return -1;
+    *p = v;
+    return 0;
+}
+
+int ref_get_scalar(const char* name, double* v){
+    double* p = H::scalar(name);
+    if(!p) return -1;
+    *v = *p;
+    return 0;
+}
+
+void ref_set_mask(const double* seamount){ quiet_begin(); H::set_mask(seamount); quiet_end(); }
+
+int ref_call(const char* op){
+    quiet_begin();
+    const int rc = H::call(op);
+    quiet_end();
+    return rc;
+}
+
+double ref_buoy_ref_level(int i){ return H::buoy_ref_level(i); }
+double ref_coord(int axis, int n){ return H::coord(axis, n); }
+
+void ref_set_vel_star(const double* src){
+    int im, jm, km; H::dims(&im, &jm, &km);
+    for(int j = 0; j < jm; j++) std::memcpy(H::vel_star_row(j), src + (size_t)j * km, sizeof(double) * km);
+}
+void ref_get_vel_star(double* dst){
+    int im, jm, km; H::dims(&im, &jm, &km);
+    for(int j = 0; j < jm; j++) std::memcpy(dst + (size_t)j * km, H::vel_star_row(j), sizeof(double) * km);
+}
+
+}  // extern "C"
