@@ -1,0 +1,475 @@
+// C ABI of libatjup_hip.so (include/atjup_hip.h): context, transfers, operators.
+#include "atjup_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+namespace atjup {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg){ g_err = msg; }
+
+namespace {
+
+bool prog(int f){ return f >= 0 && f < ATJUP_N_PROG; }
+bool input(int f){ return f >= ATJUP_F_P_DYN && f < ATJUP_F_END; }
+bool nlevel(int f){ return f >= ATJUP_NLEVEL && f < ATJUP_NLEVEL + ATJUP_N_PROG; }
+// The time-level-n copy is the live field on this path (restoreVar is the identity, see the
+// context comment), so an n-level id addresses the same storage.
+int storage_of(int f){ return nlevel(f) ? f - ATJUP_NLEVEL : f; }
+
+int alloc(atjup::Buf& b, size_t bytes){
+    if(hipMalloc(&b.p, bytes) != hipSuccess){ set_error("hipMalloc failed"); return ATJUP_ERR_NOMEM; }
+    b.bytes = bytes;
+    if(hipMemset(b.p, 0, bytes) != hipSuccess){ set_error("hipMemset failed"); return ATJUP_ERR_HIP; }
+    return 0;
+}
+void release(atjup::Buf& b){ if(b.p) (void)hipFree(b.p); b.p = nullptr; b.bytes = 0; }
+
+int fill_value(atjup_ctx* c, atjup::Buf& b, double v){
+    const size_t n = c->n3;
+    if(c->precision == ATJUP_PRECISION_FP64){
+        std::vector<double> h(n, v);
+        ATJUP_HIP_CHECK(hipMemcpy(b.p, h.data(), n * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> h(n, (float)v);
+        ATJUP_HIP_CHECK(hipMemcpy(b.p, h.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// Host geometry, built as the reference builds it (lib/Array_1D.cpp:49-60,
+// RungeKutta_Jup_Turb.cpp:299-303 and :390-411, PressureSolver.h:156-162 and :327-353).
+template<typename T>
+void build_tables(atjup_ctx* c, std::vector<GeoIJ<T>>& geo, std::vector<PoisIJ<T>>& pois){
+    const int im = c->im, jm = c->jm;
+    c->rad.resize(im); c->the.resize(jm); c->sin_the.resize(jm); c->cos_the.resize(jm);
+    double z = c->g.r0;
+    c->rad[0] = z;
+    for(int i = 1; i < im; i++){ z = z + c->g.dr; c->rad[i] = z; }
+    z = 0.0;
+    c->the[0] = z;
+    for(int j = 1; j < jm; j++){ z = z + c->g.dthe; c->the[j] = z; }
+    for(int j = 0; j < jm; j++){ c->sin_the[j] = std::sin(c->the[j]); c->cos_the[j] = std::cos(c->the[j]); }
+    const float h = (float)(c->g.L_atm * c->g.dr);
+    c->lh.resize(im);
+    for(int i = 0; i < im; i++) c->lh[i] = i * h;
+    const double omega = c->c.omega;
+    const double inv_dr2 = 1.0 / (c->g.dr * c->g.dr), inv_dthe2 = 1.0 / (c->g.dthe * c->g.dthe);
+    const double inv_dphi2 = 1.0 / (c->g.dphi * c->g.dphi);
+    geo.resize((size_t)im * jm);
+    pois.resize((size_t)im * jm);
+    for(int i = 0; i < im; i++)
+        for(int j = 0; j < jm; j++){
+            const double rm = c->rad[i];
+            const double rm2 = rm * rm;
+            const double sinthe = std::max(c->kn.sinthe_min, std::abs(std::sin(c->the[j])));
+            const double sinthe2 = sinthe * sinthe;
+            const double costhe = c->cos_the[j];
+            const double inv_rm2 = 1.0 / rm2;
+            GeoIJ<T>& G = geo[(size_t)i * jm + j];
+            const double st = std::sqrt(std::max(0.0, 1.0 - costhe * costhe));
+            G.rm = (T)rm;
+            G.inv_rm = (T)(1.0 / rm);
+            G.inv_rm2 = (T)inv_rm2;
+            G.inv_rmsinthe = (T)(1.0 / (rm * sinthe));
+            G.inv_rm2sinthe = (T)(inv_rm2 / sinthe);
+            G.inv_rm2sinthe2 = (T)(inv_rm2 / sinthe2);
+            G.costhe_inv_rm2sinthe = (T)(costhe * (inv_rm2 / sinthe));
+            G.costhe = (T)costhe;
+            G.cotanthe = (T)(costhe / sinthe);
+            G.v_metric = (T)((1.0 + costhe / (sinthe2)) * inv_rm2);
+            G.sinthe_true = (T)st;
+            G.cent_rad = (T)(1.0 * omega * omega * rm * st * st);
+            G.cent_the = (T)(1.0 * omega * omega * rm * st * costhe);
+            G.pad0 = G.pad1 = G.pad2 = (T)0;
+            // Poisson: sinthe from the solver's own table (PressureSolver.h:156-162)
+            double sp = std::sin(c->the[j]);
+            if(sp < c->kn.sinthe_min) sp = c->kn.sinthe_min;
+            if(sp == 0.0) sp = 1.0e-5;
+            const double sp2 = sp * sp;
+            const double exp_2_rm = 1.0 * 1.0;
+            PoisIJ<T>& W = pois[(size_t)i * jm + j];
+            const double num1 = exp_2_rm * inv_dr2;
+            const double num2 = inv_rm2 * inv_dthe2;
+            const double num3 = (inv_rm2 / sp2) * inv_dphi2;
+            W.num1 = (T)num1; W.num2 = (T)num2; W.num3 = (T)num3;
+            W.denom = (T)(2.0 * num1 + 2.0 * num2 + 2.0 * num3);
+            W.inv_rm = (T)(1.0 / rm);
+            W.inv_rmsinthe = (T)(1.0 / (rm * sp));
+            W.pad0 = W.pad1 = (T)0;
+        }
+}
+
+int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, int precision){
+    if(!g || !c || !k){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(g->im < 6 || g->jm < 9 || g->km < 8){ set_error("grid too small"); return ATJUP_ERR_ARG; }
+    if((g->km - 1) % 2 != 0){ set_error("the periodic ring km-1 must be even for a consistent red-black colouring"); return ATJUP_ERR_ARG; }
+    if(precision != ATJUP_PRECISION_FP64 && precision != ATJUP_PRECISION_FP32_REF){ set_error("precision"); return ATJUP_ERR_ARG; }
+    if(k->turb_model < -1 || k->turb_model > 2){ set_error("turb_model"); return ATJUP_ERR_PROFILE; }
+    if(k->rk_scalar_sync != 2){ set_error("only ATJUP_RK_SCALAR_SYNC=2 is implemented"); return ATJUP_ERR_PROFILE; }
+    if(k->dry && k->turb_model != -1){ set_error("the dry profile has the closure off"); return ATJUP_ERR_PROFILE; }
+    if(!(k->pdyn_cap > 0.0)){ set_error("pdyn_cap"); return ATJUP_ERR_PROFILE; }
+    if(!(k->press_omega > 0.0 && k->press_omega < 2.0)){ set_error("press_omega outside (0, 2)"); return ATJUP_ERR_PROFILE; }
+    return 0;
+}
+
+}  // namespace
+
+hipEvent_t timer_begin(atjup_ctx* c){
+    if(!c->timing) return nullptr;
+    hipEvent_t e = nullptr;
+    if(!c->event_pool.empty()){ e = c->event_pool.back(); c->event_pool.pop_back(); }
+    else if(hipEventCreate(&e) != hipSuccess) return nullptr;
+    (void)hipEventRecord(e, c->stream);
+    return e;
+}
+
+void timer_end(atjup_ctx* c, int phase, hipEvent_t begin, int launches){
+    if(!c->timing || !begin) return;
+    hipEvent_t e = timer_begin(c);
+    if(!e) return;
+    c->spans[phase].push_back({ begin, e, launches });
+    if(c->spans[phase].size() > 4096){                  // fold old spans so the pool stays bounded
+        (void)hipEventSynchronize(e);
+        for(auto& sp : c->spans[phase]){
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, sp.a, sp.b);
+            c->t_total[phase] += ms;
+            c->t_launch[phase] += sp.launches;
+            c->event_pool.push_back(sp.a);
+            c->event_pool.push_back(sp.b);
+        }
+        c->spans[phase].clear();
+    }
+}
+
+}  // namespace atjup
+
+using namespace atjup;
+
+extern "C" {
+
+const char* atjup_hip_last_error_string(void){ return g_err.c_str(); }
+
+void atjup_hip_default_knobs(atjup_knobs* k){
+    if(!k) return;
+    std::memset(k, 0, sizeof(*k));
+    k->turb_model = -1;
+    k->press_sweeps = 1;
+    k->press_omega = 1.0;
+    k->pdyn_cap = 0.2;
+    k->damp_pdyn = 1;
+    k->rho_pdyn = 1.0;
+    k->sinthe_min = 0.55;
+    k->t_diff_scale = 0.01;
+    k->species_diff_scale = 0.01;
+    k->rad_coupling = 1.0;
+    k->precip_coupling = 1.0;
+    k->turb_coupling = 1.0;
+    k->tke_max_phys = 1000.0;
+    k->nue_max_phys = 1.0e5;
+    k->wall_nue_factor = 4.0;
+    k->wall_nue_layers = 3;
+    k->satadj = 1;
+    k->rk_scalar_sync = 2;
+    k->dry = 1;
+    k->turb_k_bg_phys = 1.0;
+}
+
+atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, const atjup_knobs* kn, int precision){
+    if(validate(g, cs, kn, precision) != 0) return nullptr;
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1){ set_error("no HIP device"); return nullptr; }
+    atjup_ctx* c = new atjup_ctx();
+    c->g = *g; c->c = *cs; c->kn = *kn; c->precision = precision;
+    c->im = g->im; c->jm = g->jm; c->km = g->km;
+    c->esz = (precision == ATJUP_PRECISION_FP64) ? 8 : 4;
+    const int per128 = 128 / (int)c->esz;
+    c->pitch = (c->km + per128 - 1) / per128 * per128;
+    c->plane = (size_t)c->jm * c->pitch;
+    c->n3 = (size_t)c->im * c->plane;
+    if(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess){
+        set_error("hipStreamCreate failed"); delete c; return nullptr;
+    }
+    c->stream = c->own_stream;
+    const size_t fb = c->n3 * c->esz;
+    int rc = 0;
+    const bool dry = kn->dry != 0;
+    for(int f = 0; f < ATJUP_F_END && !rc; f++){
+        if(!(prog(f) || input(f))) continue;
+        rc = alloc(c->cur[f], fb);
+    }
+    const int np = dry ? 4 : ATJUP_N_PROG;
+    for(int f = 0; f < np && !rc; f++){
+        rc = alloc(c->stg[0][f], fb);
+        if(!rc) rc = alloc(c->stg[1][f], fb);
+        if(!rc) rc = alloc(c->acc[f], fb);
+    }
+    if(!rc && !dry){ rc = alloc(c->live_turb[0], fb); if(!rc) rc = alloc(c->live_turb[1], fb); }
+    if(!rc) rc = alloc(c->scratch, fb);
+    if(!rc) rc = alloc(c->psrc, c->n3 * sizeof(double));
+    if(!rc) rc = alloc(c->mask, c->n3);
+    if(!rc) rc = alloc(c->flags, c->n3);
+    if(!rc) rc = alloc(c->itopo, (size_t)c->jm * c->km * sizeof(int));
+    if(!rc) rc = alloc(c->vel_star, (size_t)c->jm * c->km * sizeof(double));
+    if(!rc) rc = alloc(c->brl, (size_t)c->im * sizeof(double));
+    if(!rc) rc = alloc(c->brl_part, (size_t)c->im * 64 * 2 * sizeof(double));
+    if(!rc) rc = alloc(c->zm, (size_t)4 * c->jm * sizeof(double));
+    if(!rc) rc = alloc(c->nonfinite, 64);
+    if(!rc){
+        // resetArrays: dis = 1e-10, rho_mix = 1 (planet/cJupiterModel.cpp:1719-1720, 1677)
+        rc = fill_value(c, c->cur[ATJUP_F_DIS], 1.0e-10);
+        if(!rc) rc = fill_value(c, c->cur[ATJUP_F_RHO_MIX], 1.0);
+    }
+    if(!rc){
+        std::vector<double> tabs;
+        if(precision == ATJUP_PRECISION_FP64){
+            std::vector<GeoIJ<double>> geo; std::vector<PoisIJ<double>> pois;
+            build_tables<double>(c, geo, pois);
+            rc = alloc(c->geo, geo.size() * sizeof(geo[0]));
+            if(!rc && hipMemcpy(c->geo.p, geo.data(), c->geo.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = ATJUP_ERR_HIP;
+            if(!rc) rc = alloc(c->pois, pois.size() * sizeof(pois[0]));
+            if(!rc && hipMemcpy(c->pois.p, pois.data(), c->pois.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = ATJUP_ERR_HIP;
+        } else {
+            std::vector<GeoIJ<float>> geo; std::vector<PoisIJ<float>> pois32;
+            build_tables<float>(c, geo, pois32);
+            std::vector<GeoIJ<double>> g64; std::vector<PoisIJ<double>> pois;
+            build_tables<double>(c, g64, pois);
+            rc = alloc(c->geo, geo.size() * sizeof(geo[0]));
+            if(!rc && hipMemcpy(c->geo.p, geo.data(), c->geo.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = ATJUP_ERR_HIP;
+            if(!rc) rc = alloc(c->pois, pois.size() * sizeof(pois[0]));
+            if(!rc && hipMemcpy(c->pois.p, pois.data(), c->pois.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = ATJUP_ERR_HIP;
+            if(!rc) rc = alloc(c->pois32, pois32.size() * sizeof(pois32[0]));
+            if(!rc && hipMemcpy(c->pois32.p, pois32.data(), c->pois32.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = ATJUP_ERR_HIP;
+        }
+        if(!rc){
+            tabs.resize((size_t)c->im + 2 * c->jm);
+            for(int i = 0; i < c->im; i++) tabs[i] = c->rad[i];
+            for(int j = 0; j < c->jm; j++){
+                tabs[c->im + j] = c->sin_the[j];
+                // the Coriolis factor of the split column integral, nd_cor = 1 (RungeKutta_Jup_Turb.cpp:185-186)
+                tabs[c->im + c->jm + j] = c->c.Coriolis * 1.0 * 2.0 * c->c.omega * std::abs(c->sin_the[j]);
+            }
+            rc = alloc(c->tabs, tabs.size() * sizeof(double));
+            if(!rc && hipMemcpy(c->tabs.p, tabs.data(), c->tabs.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = ATJUP_ERR_HIP;
+        }
+    }
+    if(rc){ atjup_hip_ctx_destroy(c); return nullptr; }
+    return c;
+}
+
+void atjup_hip_ctx_destroy(atjup_ctx* c){
+    if(!c) return;
+    if(c->stream) (void)hipStreamSynchronize(c->stream);
+    for(auto& b : c->cur) release(b);
+    for(auto& s : c->stg) for(auto& b : s) release(b);
+    for(auto& b : c->acc) release(b);
+    for(auto& b : c->live_turb) release(b);
+    for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
+                          &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
+                          &c->amb_rows, &c->nonfinite })
+        release(*b);
+    for(auto& v : c->spans) for(auto& sp : v){ (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
+    for(auto e : c->event_pool) (void)hipEventDestroy(e);
+    if(c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+static int upload_impl(atjup_ctx* c, int f, const double* src, double* const* const* rows){
+    if(!c || (!src && !rows)){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    const int s = storage_of(f);
+    if(!(prog(s) || input(s))){ set_error("field id"); return ATJUP_ERR_ARG; }
+    if(!c->cur[s].p){ set_error("field not held by this profile"); return ATJUP_ERR_PROFILE; }
+    const size_t row = (size_t)c->km;
+    std::vector<char> host(c->n3 * c->esz, 0);
+    for(int i = 0; i < c->im; i++)
+        for(int j = 0; j < c->jm; j++){
+            const double* r = rows ? rows[i][j] : src + ((size_t)i * c->jm + j) * row;
+            const size_t o = c->idx(i, j, 0);
+            if(c->esz == 8) std::memcpy((double*)host.data() + o, r, row * 8);
+            else { float* d = (float*)host.data() + o; for(size_t k = 0; k < row; k++) d[k] = (float)r[k]; }
+        }
+    if(c->kn.dry && s >= ATJUP_F_H2O && s < ATJUP_F_TKE){
+        // the dry profile never streams the species: they must be, and stay, zero
+        const size_t nb = host.size();
+        for(size_t b = 0; b < nb; b++)
+            if(host[b] != 0){ set_error("the dry profile requires the species to be zero"); return ATJUP_ERR_PROFILE; }
+    }
+    ATJUP_HIP_CHECK(hipMemcpyAsync(c->cur[s].p, host.data(), host.size(), hipMemcpyHostToDevice, c->stream));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if(s == ATJUP_F_TKE || s == ATJUP_F_DIS) c->turb_live_pending = false;
+    return 0;
+}
+
+static int download_impl(atjup_ctx* c, int f, double* dst, double** const* rows){
+    if(!c || (!dst && !rows)){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    const int s = storage_of(f);
+    if(!(prog(s) || input(s))){ set_error("field id"); return ATJUP_ERR_ARG; }
+    if(!c->cur[s].p){ set_error("field not held by this profile"); return ATJUP_ERR_PROFILE; }
+    std::vector<char> host(c->n3 * c->esz);
+    ATJUP_HIP_CHECK(hipMemcpyAsync(host.data(), c->cur[s].p, host.size(), hipMemcpyDeviceToHost, c->stream));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    const size_t row = (size_t)c->km;
+    for(int i = 0; i < c->im; i++)
+        for(int j = 0; j < c->jm; j++){
+            double* r = rows ? rows[i][j] : dst + ((size_t)i * c->jm + j) * row;
+            const size_t o = c->idx(i, j, 0);
+            if(c->esz == 8) std::memcpy(r, (const double*)host.data() + o, row * 8);
+            else { const float* d = (const float*)host.data() + o; for(size_t k = 0; k < row; k++) r[k] = (double)d[k]; }
+        }
+    return 0;
+}
+
+int atjup_hip_upload_field(atjup_ctx* c, int f, const double* src){ return upload_impl(c, f, src, nullptr); }
+int atjup_hip_download_field(atjup_ctx* c, int f, double* dst){ return download_impl(c, f, dst, nullptr); }
+int atjup_hip_upload_field_rows(atjup_ctx* c, int f, double* const* const* rows){ return upload_impl(c, f, nullptr, rows); }
+int atjup_hip_download_field_rows(atjup_ctx* c, int f, double** const* rows){ return download_impl(c, f, nullptr, rows); }
+
+int atjup_hip_upload_mask(atjup_ctx* c, const double* m){
+    if(!c || !m){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    std::vector<uint8_t> host(c->n3, 0);
+    for(int i = 0; i < c->im; i++)
+        for(int j = 0; j < c->jm; j++)
+            for(int k = 0; k < c->km; k++)
+                host[c->idx(i, j, k)] = (m[((size_t)i * c->jm + j) * c->km + k] == 1.0) ? 1 : 0;
+    ATJUP_HIP_CHECK(hipMemcpy(c->mask.p, host.data(), c->n3, hipMemcpyHostToDevice));
+    c->have_mask = true;
+    return launch_mask_setup(c);
+}
+
+int atjup_hip_upload_vel_star(atjup_ctx* c, const double* s){
+    if(!c || !s){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    ATJUP_HIP_CHECK(hipMemcpy(c->vel_star.p, s, (size_t)c->jm * c->km * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+int atjup_hip_download_vel_star(atjup_ctx* c, double* d){
+    if(!c || !d){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    ATJUP_HIP_CHECK(hipMemcpy(d, c->vel_star.p, (size_t)c->jm * c->km * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+int atjup_hip_download_buoy_ref_level(atjup_ctx* c, double* d){
+    if(!c || !d){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    ATJUP_HIP_CHECK(hipMemcpy(d, c->brl.p, (size_t)c->im * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int need_mask(atjup_ctx* c){
+    if(!c){ set_error("null context"); return ATJUP_ERR_ARG; }
+    if(!c->have_mask){ set_error("upload the obstacle mask first (atjup_hip_upload_mask)"); return ATJUP_ERR_STATE; }
+    return 0;
+}
+
+int atjup_hip_rk4_step(atjup_ctx* c){ if(int rc = need_mask(c)) return rc; return launch_rk4(c); }
+int atjup_hip_pressure_solve(atjup_ctx* c, int sweeps, double omega){
+    if(int rc = need_mask(c)) return rc;
+    if(!(omega > 0.0 && omega < 2.0)){ set_error("omega outside (0, 2)"); return ATJUP_ERR_ARG; }
+    hipEvent_t t0 = timer_begin(c);
+    const int rc = launch_pressure(c, sweeps > 0 ? sweeps : std::max(1, c->kn.press_sweeps), omega);
+    timer_end(c, ATJUP_T_PRESSURE, t0, 0);
+    return rc;
+}
+int atjup_hip_satadj(atjup_ctx* c, int gas){
+    if(int rc = need_mask(c)) return rc;
+    if(gas < 0 || gas > 2){ set_error("gas"); return ATJUP_ERR_ARG; }
+    if(c->kn.dry) return 0;        // all species zero: the adjustment is the identity (SaturationAdjustment.h:440)
+    set_error("the saturation adjustment is not built in this library"); return ATJUP_ERR_PROFILE;
+}
+int atjup_hip_turbulence_run(atjup_ctx* c){
+    if(int rc = need_mask(c)) return rc;
+    if(c->kn.turb_model < 0) return 0;
+    set_error("the turbulence closure is not built in this library"); return ATJUP_ERR_PROFILE;
+}
+int atjup_hip_bc_after_step(atjup_ctx* c){ if(int rc = need_mask(c)) return rc; return launch_bc_after_step(c); }
+int atjup_hip_shapiro(atjup_ctx* c, int f, int order, double strength, int passes, int axes){
+    if(int rc = need_mask(c)) return rc;
+    const int s = storage_of(f);
+    if(!(prog(s) || input(s)) || !c->cur[s].p || (order != 2 && order != 4) || passes < 0){
+        set_error("shapiro arguments"); return ATJUP_ERR_ARG;
+    }
+    return launch_shapiro(c, s, order, strength, passes, axes & 7);
+}
+int atjup_hip_mixture_density(atjup_ctx* c){ if(!c) return ATJUP_ERR_ARG; return launch_mixture_density(c); }
+int atjup_hip_flux_limiter(atjup_ctx* c){
+    if(int rc = need_mask(c)) return rc;
+    if(c->kn.dry) return 0;        // nh4sh = 0: the correction is zero
+    set_error("the flux limiter is not built in this library"); return ATJUP_ERR_PROFILE;
+}
+int atjup_hip_clamp_species(atjup_ctx* c){ if(!c) return ATJUP_ERR_ARG; return launch_clamp_species(c); }
+int atjup_hip_swap_nlevel(atjup_ctx* c){ return c ? 0 : ATJUP_ERR_ARG; }
+int atjup_hip_scalar_sync(atjup_ctx* c){ return c ? 0 : ATJUP_ERR_ARG; }
+
+int atjup_hip_step(atjup_ctx* c, int iter, int first){
+    if(int rc = need_mask(c)) return rc;
+    int rc = 0;
+    hipEvent_t t_step = timer_begin(c);
+    if(iter % 2 == 0 || first){
+        hipEvent_t t0 = timer_begin(c);
+        if((rc = launch_pressure(c, std::max(1, c->kn.press_sweeps), c->kn.press_omega))) return rc;
+        timer_end(c, ATJUP_T_PRESSURE, t0, 0);
+        if(c->kn.damp_pdyn && (rc = launch_shapiro(c, ATJUP_F_P_DYN, 2, 1.0, 1, 7))) return rc;
+        if((rc = launch_mixture_density(c))) return rc;
+        for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
+            if((rc = atjup_hip_satadj(c, gas))) return rc;
+        if((rc = atjup_hip_turbulence_run(c))) return rc;
+        if((rc = atjup_hip_flux_limiter(c))) return rc;
+    }
+    if((rc = launch_rk4(c))) return rc;
+    if((rc = launch_bc_after_step(c))) return rc;
+    if((rc = launch_clamp_species(c))) return rc;
+    timer_end(c, ATJUP_T_STEP, t_step, 0);
+    return 0;
+}
+
+int atjup_hip_check_finite(atjup_ctx* c){ if(!c) return ATJUP_ERR_ARG; return launch_check_finite(c); }
+
+int atjup_hip_set_timing(atjup_ctx* c, int enable){
+    if(!c) return ATJUP_ERR_ARG;
+    c->timing = enable != 0;
+    return 0;
+}
+
+int atjup_hip_read_timers(atjup_ctx* c, double* out){
+    if(!c || !out) return ATJUP_ERR_ARG;
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for(int p = 0; p < ATJUP_N_TIMERS; p++){
+        for(auto& sp : c->spans[p]){
+            float ms = 0.0f;
+            ATJUP_HIP_CHECK(hipEventElapsedTime(&ms, sp.a, sp.b));
+            c->t_total[p] += ms;
+            c->t_launch[p] += sp.launches;
+            c->event_pool.push_back(sp.a);
+            c->event_pool.push_back(sp.b);
+        }
+        c->spans[p].clear();
+        out[2 * p] = c->t_total[p];
+        out[2 * p + 1] = c->t_launch[p];
+        c->t_total[p] = 0.0;
+        c->t_launch[p] = 0.0;
+    }
+    return 0;
+}
+
+int atjup_hip_sync(atjup_ctx* c){
+    if(!c) return ATJUP_ERR_ARG;
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int atjup_hip_set_stream(atjup_ctx* c, void* s){
+    if(!c) return ATJUP_ERR_ARG;
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return 0;
+}
+void* atjup_hip_field_device_ptr(atjup_ctx* c, int f){
+    if(!c) return nullptr;
+    const int s = storage_of(f);
+    if(!(prog(s) || input(s))) return nullptr;
+    return c->cur[s].p;
+}
+int atjup_hip_pitch(atjup_ctx* c){ return c ? c->pitch : 0; }
+
+}  // extern "C"

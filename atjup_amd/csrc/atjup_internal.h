@@ -1,0 +1,134 @@
+// Internal declarations shared by the translation units of libatjup_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/atjup_hip.h"
+
+namespace atjup {
+
+void set_error(const std::string& msg);
+
+#define ATJUP_HIP_CHECK(call)                                                                    \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if(e_ != hipSuccess){                                                                    \
+            atjup::set_error(std::string(#call) + ": " + hipGetErrorString(e_));                 \
+            return ATJUP_ERR_HIP;                                                                \
+        }                                                                                        \
+    } while(0)
+
+// Face flags of a cell, built from the obstacle mask (one byte per cell).
+enum : uint8_t {
+    FL_SELF = 1,      // the cell itself is solid (SeaMount == 1)
+    FL_IP = 2,        // (i+1, j, k) solid
+    FL_IM = 4,        // (i-1, j, k) solid
+    FL_JP = 8,        // (i, j+1, k) solid
+    FL_JM = 16,       // (i, j-1, k) solid
+    FL_KP = 32,       // (i, j, ke) solid, ke the periodic east neighbour
+    FL_KM = 64,       // (i, j, kw) solid
+};
+
+// Per-(i, j) metric table of the RHS, computed on the host with the reference's own expressions
+// (RungeKutta_Jup_Turb.cpp:390-411) so that the device reproduces them bit for bit.
+template<typename T>
+struct GeoIJ {
+    T rm, inv_rm, inv_rm2, inv_rmsinthe, inv_rm2sinthe, inv_rm2sinthe2, costhe_inv_rm2sinthe;
+    T costhe, cotanthe, v_metric, sinthe_true, cent_rad, cent_the, pad0, pad1, pad2;
+};
+
+// Per-(i, j) weights of the Poisson stencil (PressureSolver.h:327-353).
+template<typename T>
+struct PoisIJ {
+    T num1, num2, num3, denom, inv_rm, inv_rmsinthe, pad0, pad1;
+};
+
+// A solid cell's treatment in bcSolidGround (BC_Jup.h:517-589): the cell index and a code,
+// 0 = interior (zeroed except t, p_stat), 1..6 = extrapolate from +i,-i,+j,-j,+k,-k, 7 = no pair.
+struct SolidCell {
+    int i, j, k, code;
+};
+
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace atjup
+
+struct atjup_ctx {
+    atjup_grid g;
+    atjup_consts c;
+    atjup_knobs kn;
+    int precision;               // ATJUP_PRECISION_*
+    int im, jm, km, pitch;       // pitch: row length in elements (km rounded up to 128 bytes)
+    size_t plane, n3;            // elements per i-plane (jm * pitch) and per field (im * plane)
+    size_t esz;                  // bytes per element of the field storage (8 or 4)
+    hipStream_t stream = nullptr, own_stream = nullptr;
+
+    // field storage. cur[f]: the live field (for prognostic fields also the time-level-n copy:
+    // restoreVar is the identity on this path); stg[2][f]: the two RK4 stage buffers; acc[f].
+    atjup::Buf cur[ATJUP_F_END];
+    atjup::Buf stg[2][ATJUP_N_PROG];
+    atjup::Buf acc[ATJUP_N_PROG];
+    atjup::Buf live_turb[2];     // tke / dis as Turbulence::run left them, before the RK4 (full profile)
+    bool turb_live_pending = false;
+    atjup::Buf scratch;          // one field, for the filters
+    atjup::Buf psrc;             // Poisson source, FP64 [n3]
+    atjup::Buf mask;             // uint8 SeaMount [n3]
+    atjup::Buf flags;            // uint8 face flags [n3]
+    atjup::Buf itopo;            // int [jm * km]
+    atjup::Buf vel_star;         // double [jm * km]
+    atjup::Buf geo;              // GeoIJ<T> [im * jm]
+    atjup::Buf pois;             // PoisIJ<double> [im * jm]
+    atjup::Buf pois32;           // PoisIJ<float> [im * jm]
+    atjup::Buf tabs;             // double rad[im], sin(the)[jm], Coriolis split factor [jm]
+    atjup::Buf brl;              // double [im]
+    atjup::Buf brl_part;         // double partial sums
+    atjup::Buf zm;               // double [4 * jm], the top-sponge zonal means
+    atjup::Buf solids;           // SolidCell list
+    atjup::Buf amb_rows;         // int2 list of (i, j) rows that hold solid cells
+    atjup::Buf nonfinite;        // int flag
+    int n_solids = 0, n_amb_rows = 0;
+    bool have_mask = false;
+
+    // phase timers (atjup_hip_set_timing): pending event pairs per phase, with launch counts
+    bool timing = false;
+    struct TimedSpan { hipEvent_t a, b; int launches; };
+    std::vector<TimedSpan> spans[ATJUP_N_TIMERS];
+    std::vector<hipEvent_t> event_pool;
+    double t_total[ATJUP_N_TIMERS] = {0};
+    double t_launch[ATJUP_N_TIMERS] = {0};
+
+    // host-side geometry
+    std::vector<double> rad, the, sin_the, cos_the;
+    std::vector<float> lh;
+
+    size_t idx(int i, int j, int k) const { return ((size_t)i * jm + j) * pitch + k; }
+};
+
+namespace atjup {
+
+// launchers (atjup_kernels.hip)
+int launch_buoy_ref_level(atjup_ctx* c);
+int launch_hydrostatic(atjup_ctx* c);
+int launch_rk4(atjup_ctx* c);
+int launch_pressure(atjup_ctx* c, int sweeps, double omega);
+int launch_bc_after_step(atjup_ctx* c);
+int launch_shapiro(atjup_ctx* c, int field, int order, double strength, int passes, int axes);
+int launch_mixture_density(atjup_ctx* c);
+int launch_clamp_species(atjup_ctx* c);
+int launch_mask_setup(atjup_ctx* c);
+int launch_check_finite(atjup_ctx* c);
+int launch_copy_field(atjup_ctx* c, void* dst, const void* src);
+
+// phase timers: begin returns an event recorded now (or nullptr with timing off); end records the
+// closing event and files the span under `phase` with its kernel-launch count.
+hipEvent_t timer_begin(atjup_ctx* c);
+void timer_end(atjup_ctx* c, int phase, hipEvent_t begin, int launches);
+
+}  // namespace atjup

@@ -1,0 +1,1152 @@
+// HIP kernels of the ATJUP time step for gfx950 (MI355X), FP64 drop-in and FP32 form.
+//
+// Layout: every field is one contiguous array [i][j][k] with the k row padded to `pitch`
+// elements (128-byte aligned rows), k (phi) fastest, so a 64-wide wavefront reads one row
+// segment per load. The stencil kernels walk a 1-D grid whose block index is remapped so that
+// each of the 8 XCDs owns a contiguous slab of i-planes: the vertical (i +- 1) and meridional
+// (j +- 1) neighbours a block reads were brought in by blocks of the same XCD and hit its L2.
+//
+// All FP64 kernels keep the reference's expression trees (the file is built with
+// -ffp-contract=off), so that a cell update reproduces the CPU oracle bit for bit whenever its
+// inputs agree.
+#include "atjup_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace atjup {
+
+// ---------------------------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_remap(int b, int nb){
+    // The dispatcher hands block b to XCD b % 8. Give XCD x the contiguous range of logical
+    // blocks [start_x, start_x + count_x) instead, so neighbouring tiles share an L2.
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, n = b >> 3;
+    const int start = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+    return start + n;
+}
+
+template<typename T>
+__device__ __forceinline__ bool finite_bits(T v){ return isfinite(v); }
+
+template<typename T>
+__device__ __forceinline__ T safe_clamp_d(T v, T lo, T hi){
+    if(!isfinite(v)) return lo;                   // exponent-bit test of RungeKutta_Jup_Turb.cpp:285-290
+    return (v < lo) ? lo : ((v > hi) ? hi : v);
+}
+
+template<typename T>
+__device__ __forceinline__ T dmax(T a, T b){ return (a < b) ? b : a; }   // std::max
+template<typename T>
+__device__ __forceinline__ T dmin(T a, T b){ return (b < a) ? b : a; }   // std::min
+
+// ---------------------------------------------------------------------------------------------
+// computeBuoyancyRefLevel (RungeKutta_Jup_Turb.cpp:47-69): per level i the sin(theta)-weighted
+// mean of g p_stat / (r_mix R_mix t t_ref) over fluid cells with t > 0. Two deterministic passes:
+// NCH chunks of (j, k) per level reduce to partial sums in a fixed order, then one thread per
+// level adds its partials in chunk order.
+// ---------------------------------------------------------------------------------------------
+constexpr int BRL_NCH = 32;
+constexpr int BRL_BLOCK = 256;
+
+template<typename T>
+__global__ __launch_bounds__(BRL_BLOCK) void k_brl_partial(const T* __restrict__ t, const T* __restrict__ ps,
+        const uint8_t* __restrict__ flags, const double* __restrict__ sin_the, double* __restrict__ part,
+        int jm, int km, int pitch, double g, double rR, double t_ref){
+    const int i = blockIdx.y, ch = blockIdx.x;
+    const int ncell = jm * km;
+    const int per = (ncell + BRL_NCH - 1) / BRL_NCH;
+    const int c0 = ch * per, c1 = min(ncell, c0 + per);
+    double s = 0.0, w = 0.0;
+    for(int c = c0 + threadIdx.x; c < c1; c += BRL_BLOCK){
+        const int j = c / km, k = c - j * km;
+        const size_t q = ((size_t)i * jm + j) * pitch + k;
+        if(flags[q] & FL_SELF) continue;
+        const double tq = (double)t[q];
+        if(!(tq > 0.0)) continue;
+        const double b = g * (double)ps[q] / (rR * tq * t_ref);
+        if(!isfinite(b)) continue;
+        const double wgt = sin_the[j];
+        s += wgt * b;
+        w += wgt;
+    }
+    __shared__ double sh[2][BRL_BLOCK];
+    sh[0][threadIdx.x] = s;
+    sh[1][threadIdx.x] = w;
+    __syncthreads();
+    for(int o = BRL_BLOCK / 2; o > 0; o >>= 1){
+        if((int)threadIdx.x < o){
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if(threadIdx.x == 0){
+        part[((size_t)i * BRL_NCH + ch) * 2 + 0] = sh[0][0];
+        part[((size_t)i * BRL_NCH + ch) * 2 + 1] = sh[1][0];
+    }
+}
+
+__global__ void k_brl_final(const double* __restrict__ part, double* __restrict__ brl, int im){
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= im) return;
+    double s = 0.0, w = 0.0;
+    for(int ch = 0; ch < BRL_NCH; ch++){
+        s += part[((size_t)i * BRL_NCH + ch) * 2 + 0];
+        w += part[((size_t)i * BRL_NCH + ch) * 2 + 1];
+    }
+    brl[i] = (w > 0.0) ? s / w : 0.0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// computeHydrostaticPressure, METRIC_SPLIT + CORIOLIS_SPLIT without HYDRO_SPLIT and p_baro
+// (RungeKutta_Jup_Turb.cpp:119-218): one thread per (j, k) column, trapezoid scan over the
+// column's fluid run from its foot, zero below and above it.
+// ---------------------------------------------------------------------------------------------
+template<typename T>
+__global__ __launch_bounds__(256) void k_hydrostatic(const T* __restrict__ v, const T* __restrict__ w,
+        const uint8_t* __restrict__ flags, const double* __restrict__ rad, const double* __restrict__ cor_j,
+        T* __restrict__ ph, int im, int jm, int km, int pitch, size_t plane, double dr){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if(k >= km) return;
+    const size_t q0 = (size_t)j * pitch + k;
+    int i0 = 0;
+    while(i0 < im && (flags[q0 + i0 * plane] & FL_SELF)){ ph[q0 + i0 * plane] = T(0); i0++; }
+    if(i0 >= im) return;
+    int i1 = i0;
+    while(i1 + 1 < im && !(flags[q0 + (i1 + 1) * plane] & FL_SELF)) i1++;
+    for(int i = i1 + 1; i < im; i++) ph[q0 + i * plane] = T(0);
+    const T cor_ = (T)cor_j[j];
+    auto f_ = [&](int i) -> T {
+        const T vv = v[q0 + i * plane], ww = w[q0 + i * plane];
+        return T(0) + ((vv * vv + ww * ww) / (T)rad[i] + cor_ * ww) / T(1);
+    };
+    T prev = T(0);
+    ph[q0 + i0 * plane] = prev;
+    T fprev = f_(i0);
+    for(int i = i0 + 1; i <= i1; i++){
+        const T fi = f_(i);
+        prev = prev + T(0.5) * (fprev + fi) * (T)dr;
+        ph[q0 + i * plane] = prev;
+        fprev = fi;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// RK4 stage with the RHS of the four dynamical fields t, u, v, w (RHSJup, RHS_Jup_Turb.cpp:31-1457,
+// under the dry dynamics profile: no species, closure, radiation or precipitation; nue = 0).
+// One thread per cell; the stage accumulation and the next stage input (pass B,
+// RungeKutta_Jup_Turb.cpp:420-490) are fused into the same kernel, double-buffered so that no
+// cell is read after it is written. Stage 3 writes the final assembly (:494-568) in place into
+// the time-level-n buffer and the aux_* fields the next pressure solve reads.
+// ---------------------------------------------------------------------------------------------
+template<typename T>
+struct StageArgs {
+    const T* in[4];
+    T* out[4];
+    const T* nl[4];
+    T* acc[4];
+    const T *pd, *ph, *ps, *rho, *wn;
+    T *aux_u, *aux_v, *aux_w;
+    const uint8_t* flags;
+    const GeoIJ<T>* geo;
+    const double* brl;
+    int im, jm, km, pitch, nkb, njb, nb;
+    size_t plane;
+    int stage, copy_boundary;
+    T c_in, wgt, dt, one_sixth;
+    T inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
+    T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, cbuoy, g, rR, t_ref, kt, visc0;
+};
+
+constexpr int ST_BK = 64, ST_BJ = 4;
+
+template<typename T>
+__global__ __launch_bounds__(ST_BK * ST_BJ) void k_stage_dry(const StageArgs<T> a){
+    const int b = xcd_remap(blockIdx.x, a.nb);
+    const int kb = b % a.nkb;
+    const int rest = b / a.nkb;
+    const int jb = rest % a.njb;
+    const int i = rest / a.njb;
+    const int k = kb * ST_BK + threadIdx.x;
+    const int j = jb * ST_BJ + threadIdx.y;
+    if(k >= a.km || j >= a.jm) return;
+    const size_t q = ((size_t)i * a.jm + j) * a.pitch + k;
+    const uint8_t fl = a.flags[q];
+    const bool interior = i >= 1 && i <= a.im - 2 && j >= 3 && j <= a.jm - 4 && k <= a.km - 2 && !(fl & FL_SELF);
+    if(!interior){
+        if(a.copy_boundary){
+            #pragma unroll
+            for(int f = 0; f < 4; f++) a.out[f][q] = a.in[f][q];
+        }
+        return;
+    }
+    const size_t P = a.plane, R = (size_t)a.pitch;
+    const int kw = (k == 0) ? a.km - 2 : k - 1;
+    const int ke = (k == a.km - 2) ? 0 : k + 1;
+    const size_t qkw = q - k + kw, qke = q - k + ke;
+    const GeoIJ<T> G = a.geo[(size_t)i * a.jm + j];
+
+    T c_[4], d1r[4], d2r[4], d1t[4], d2t[4], d1p[4], d2p[4];
+    #pragma unroll
+    for(int f = 0; f < 4; f++){
+        const T* X = a.in[f];
+        const T xc = X[q], xip = X[q + P], xim = X[q - P], xjp = X[q + R], xjm = X[q - R];
+        const T xkp = X[qke], xkm = X[qkw];
+        c_[f] = xc;
+        d1r[f] = (xip - xim) * a.inv_2dr;
+        d2r[f] = (xip - T(2) * xc + xim) * a.inv_dr2;
+        d1t[f] = (xjp - xjm) * a.inv_2dthe;
+        d2t[f] = (xjp - T(2) * xc + xjm) * a.inv_dthe2;
+        d1p[f] = (xkp - xkm) * a.inv_2dphi;
+        d2p[f] = (xkp - T(2) * xc + xkm) * a.inv_dphi2;
+    }
+    // wall-mirrored pressure gradients (RHS_Jup_Turb.cpp:119-130)
+    const T pdc = a.pd[q], phc = a.ph[q];
+    const T dpdr   = (((fl & FL_IP) ? pdc : a.pd[q + P]) - ((fl & FL_IM) ? pdc : a.pd[q - P])) * a.inv_2dr;
+    const T dpdthe = (((fl & FL_JP) ? pdc : a.pd[q + R]) - ((fl & FL_JM) ? pdc : a.pd[q - R])) * a.inv_2dthe;
+    const T dpdphi = (((fl & FL_KP) ? pdc : a.pd[qke]) - ((fl & FL_KM) ? pdc : a.pd[qkw])) * a.inv_2dphi;
+    const T dphdthe = (((fl & FL_JP) ? phc : a.ph[q + R]) - ((fl & FL_JM) ? phc : a.ph[q - R])) * a.inv_2dthe;
+    const T dphdphi = (((fl & FL_KP) ? phc : a.ph[qke]) - ((fl & FL_KM) ? phc : a.ph[qkw])) * a.inv_2dphi;
+
+    const T tq = c_[0], u = c_[1], v = c_[2], w = c_[3];
+    const T costhe = G.costhe, st = G.sinthe_true;
+    const T Coriolis_the = a.c_m2o * costhe * w;
+    const T Coriolis_phi = a.c_p2o * (+costhe * v + st * u);
+    const T v_invrm = v * G.inv_rm;
+    const T w_invrs = w * G.inv_rmsinthe;
+
+    T dpstatdr = (a.ps[q + P] - a.ps[q - P]) * a.inv_2dr * a.to_nd;
+    {
+        const T rho_l = a.rho[q];
+        if(rho_l > T(0) && isfinite(rho_l)) dpstatdr *= a.r_mix / rho_l;
+    }
+    const T pressure_t = a.cep * (u * (dpdr + dpstatdr) + v_invrm * dpdthe + w_invrs * dpdphi);
+    const T transport_t = u * d1r[0] + v_invrm * d1t[0] + w_invrs * d1p[0];
+    const T transport_u = u * d1r[1] + v_invrm * d1t[1] + w_invrs * d1p[1]
+        - T(0) * (v * v + w * w) * G.inv_rm;
+    const T transport_v = u * d1r[2] + v_invrm * d1t[2] + w_invrs * d1p[2]
+        + (u * v - w * w * G.cotanthe) * G.inv_rm;
+    const T transport_w = u * d1r[3] + v_invrm * d1t[3] + w_invrs * d1p[3]
+        + (w * u + v * w * G.cotanthe) * G.inv_rm;
+    const T two_inv_rm = T(2) * G.inv_rm;
+    const T diffusion_t = d2r[0] + d1r[0] * two_inv_rm + d2t[0] * G.inv_rm2
+        + d1t[0] * G.costhe_inv_rm2sinthe + d2p[0] * G.inv_rm2sinthe2;
+    const T diffusion_u = d2r[1] + T(2) * u * G.inv_rm2 + d2t[1] * G.inv_rm2
+        + T(4) * d1r[1] * G.inv_rm + d1t[1] * G.costhe_inv_rm2sinthe
+        + d2p[1] * G.inv_rm2sinthe2;
+    const T diffusion_v = d2r[2] + d1r[2] * two_inv_rm + d2t[2] * G.inv_rm2
+        + d1t[2] * G.costhe_inv_rm2sinthe - G.v_metric * v
+        + d2p[2] * G.inv_rm2sinthe2
+        + T(2) * d1t[1] * G.inv_rm2
+        - d1p[3] * T(2) * costhe * G.inv_rm2sinthe2;
+    const T diffusion_w = d2r[3] + d1r[3] * two_inv_rm + d2t[3] * G.inv_rm2
+        + d1t[3] * G.costhe_inv_rm2sinthe - G.v_metric * w
+        + d2p[3] * G.inv_rm2sinthe2
+        + T(2) * d1p[1] * G.inv_rm2sinthe
+        + d1p[2] * T(2) * costhe * G.inv_rm2sinthe2;
+    const T dpdthe_term = dpdthe * G.inv_rm;
+    const T dpdphi_term = dpdphi * G.inv_rmsinthe;
+    const T dphdthe_term = dphdthe * G.inv_rm;
+    const T dphdphi_term = dphdphi * G.inv_rmsinthe;
+
+    const T rhs_t = + pressure_t - transport_t + diffusion_t * a.kt;
+    const T buoyancy_u = (tq > T(0))
+        ? a.cbuoy * (a.g * a.ps[q] / (a.rR * tq * a.t_ref) - (T)a.brl[i]) : T(0);
+    const T visc = a.visc0 + a.wn[q];
+    const T rhs_u = - dpdr + buoyancy_u - transport_u + diffusion_u * visc
+        + a.centrifugal * G.cent_rad;
+    const T rhs_v = - dpdthe_term - dphdthe_term - transport_v + diffusion_v * visc
+        - a.Coriolis * Coriolis_the + a.centrifugal * G.cent_the;
+    const T rhs_w = - dpdphi_term - dphdphi_term - transport_w + diffusion_w * visc
+        - a.Coriolis * Coriolis_phi;
+    const T rhs[4] = { rhs_t, rhs_u, rhs_v, rhs_w };
+
+    if(a.stage == 0){
+        #pragma unroll
+        for(int f = 0; f < 4; f++){
+            a.acc[f][q] = a.wgt * rhs[f];
+            a.out[f][q] = c_[f] + a.c_in * rhs[f];       // stage 0 reads the n level itself
+        }
+    } else if(a.stage < 3){
+        #pragma unroll
+        for(int f = 0; f < 4; f++){
+            a.acc[f][q] = a.acc[f][q] + a.wgt * rhs[f];
+            a.out[f][q] = a.nl[f][q] + a.c_in * rhs[f];
+        }
+    } else {
+        #pragma unroll
+        for(int f = 0; f < 4; f++){
+            const T accf = a.acc[f][q] + a.wgt * rhs[f];
+            a.out[f][q] = a.nl[f][q] + a.dt * accf * a.one_sixth;
+        }
+        a.aux_u[q] = rhs_u + dpdr;
+        a.aux_v[q] = rhs_v + dpdthe_term;
+        a.aux_w[q] = rhs_w + dpdphi_term;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Red-black Poisson solve (PressureSolver.h:145-766).
+// ---------------------------------------------------------------------------------------------
+// prepareProjectionBoundaries, rigid lid (cJupiterModel.cpp:1842-1864)
+template<typename T>
+__global__ void k_prep_projection(T* __restrict__ au, T* __restrict__ av, T* __restrict__ aw,
+        int im, int jm, int km, int pitch, size_t plane, double c43, double c13){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y + 1;
+    if(k > km - 2) return;
+    const size_t q = (size_t)j * pitch + k;
+    const size_t t0 = q, t1 = q + plane, t2 = q + 2 * plane;
+    const size_t b0 = q + (size_t)(im - 1) * plane, b1 = q + (size_t)(im - 2) * plane, b2 = q + (size_t)(im - 3) * plane;
+    au[t0] = T(0);
+    au[b0] = T(0);
+    av[t0] = (T)c43 * av[t1] - (T)c13 * av[t2];
+    av[b0] = (T)c43 * av[b1] - (T)c13 * av[b2];
+    aw[t0] = (T)c43 * aw[t1] - (T)c13 * aw[t2];
+    aw[b0] = (T)c43 * aw[b1] - (T)c13 * aw[b2];
+}
+
+// The divergence source of every cell of the sweep domain, computed once per call: it depends
+// only on aux_* and the mask, which the sweeps do not change. Clamped to +-den * PDYN_CAP with the
+// cell's own denominator (PressureSolver.h:370-487).
+template<typename T>
+__global__ __launch_bounds__(256) void k_poisson_source(const T* __restrict__ au, const T* __restrict__ av,
+        const T* __restrict__ aw, const uint8_t* __restrict__ flags, const uint8_t* __restrict__ msk,
+        const PoisIJ<double>* __restrict__ pw, double* __restrict__ src,
+        int im, int jm, int km, int pitch, size_t plane, int nb,
+        double inv_2dr, double inv_2dthe, double inv_2dphi, double inv_dthe, double inv_dphi, double cap){
+    const int b = xcd_remap(blockIdx.x, nb);
+    const int nkb = (km - 1 + 63) / 64;
+    const int kb = b % nkb, rest = b / nkb;
+    const int j = rest % (jm - 2) + 1, i = rest / (jm - 2) + 1;
+    const int k = kb * 64 + threadIdx.x;
+    if(k > km - 2 || i > im - 2) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const size_t P = plane, R = (size_t)pitch;
+    const uint8_t fl = flags[q];
+    const bool sld = fl & FL_SELF;
+    const int kw = (k == 0) ? km - 2 : k - 1, ke = (k == km - 2) ? 0 : k + 1;
+    const bool skp = fl & FL_KP, skm = fl & FL_KM;
+    const bool skp2 = (k < km - 2) ? (msk[q + 2] != 0) : false;
+    const bool skm2 = (k > 2) ? (msk[q - 2] != 0) : false;
+    double du_dr = 0.0, dv_dthe = 0.0, dw_dphi = 0.0;
+    bool r_flag = false, the_flag = false, phi_flag = false;
+    if(i < im - 2 && sld && !(fl & FL_IP)){
+        du_dr = (-3.0 * (double)au[q] + 4.0 * (double)au[q + P] - (double)au[q + 2 * P]) * inv_2dr;
+        r_flag = true;
+    }
+    if(j > 2 && j < jm - 2){
+        const bool air_jp1 = !(fl & FL_JP), air_jm1 = !(fl & FL_JM);
+        const bool sjp2 = msk[q + 2 * R] != 0, sjm2 = msk[q - 2 * R] != 0;
+        if(sld && air_jp1 && !sjp2){
+            dv_dthe = (-3.0 * (double)av[q] + 4.0 * (double)av[q + R] - (double)av[q + 2 * R]) * inv_2dthe;
+            the_flag = true;
+        }
+        if(sld && air_jm1 && !sjm2){
+            dv_dthe = -(-3.0 * (double)av[q] + 4.0 * (double)av[q - R] - (double)av[q - 2 * R]) * inv_2dthe;
+            the_flag = true;
+        }
+        if((sld && air_jp1 && sjp2) || (j == jm - 2 && !sld && (fl & FL_JP))){
+            dv_dthe = ((double)av[q + R] - (double)av[q]) * inv_dthe;
+            the_flag = true;
+        }
+        if((sld && air_jm1 && sjm2) || (j == 1 && sld && air_jm1)){
+            dv_dthe = ((double)av[q - R] - (double)av[q]) * inv_dthe;
+            the_flag = true;
+        }
+    }
+    if(k > 2 && k < km - 2){
+        const bool air_kp1 = !skp, air_km1 = !skm;
+        if(sld && air_kp1 && !skp2){
+            dw_dphi = (-3.0 * (double)aw[q] + 4.0 * (double)aw[q + 1] - (double)aw[q + 2]) * inv_2dphi;
+            phi_flag = true;
+        }
+        if(sld && air_km1 && !skm2){
+            dw_dphi = -(-3.0 * (double)aw[q] + 4.0 * (double)aw[q - 1] - (double)aw[q - 2]) * inv_2dphi;
+            phi_flag = true;
+        }
+        if((sld && air_kp1 && skp2) || (k == km - 2 && !sld && skp)){
+            dw_dphi = ((double)aw[q + 1] - (double)aw[q]) * inv_dphi;
+            phi_flag = true;
+        }
+        if((sld && air_km1 && skm2) || (k == 1 && sld && air_km1)){
+            dw_dphi = ((double)aw[q - 1] - (double)aw[q]) * inv_dphi;
+            phi_flag = true;
+        }
+    }
+    if(!r_flag) du_dr = ((double)au[q + P] - (double)au[q - P]) * inv_2dr;
+    if(!the_flag) dv_dthe = ((double)av[q + R] - (double)av[q - R]) * inv_2dthe;
+    if(!phi_flag) dw_dphi = ((double)aw[q - k + ke] - (double)aw[q - k + kw]) * inv_2dphi;
+    const PoisIJ<double> W = pw[(size_t)i * jm + j];
+    double div_src = du_dr * 1.0 + dv_dthe * W.inv_rm + dw_dphi * W.inv_rmsinthe;
+    double den = W.denom;
+    if(!sld){
+        den = 0.0;
+        if(!(fl & FL_IP)) den += W.num1;
+        if(!(fl & FL_IM)) den += W.num1;
+        if(!(fl & FL_JP)) den += W.num2;
+        if(!(fl & FL_JM)) den += W.num2;
+        if(!skp) den += W.num3;
+        if(!skm) den += W.num3;
+    }
+    const double src_max = den * cap;
+    if(!isfinite(div_src)) div_src = 0.0;
+    else if(div_src > src_max) div_src = src_max;
+    else if(div_src < -src_max) div_src = -src_max;
+    src[q] = div_src;
+}
+
+// One colour pass of a red-black sweep. Thread m of row (i, j) owns k = k0 + 2m, the cells of
+// this colour. The domain boundary planes are the outer condition of the relaxation; the
+// reference rewrites them after every sweep (PressureSolver.h:718-746). Here a plane value is
+// only ever needed by its one interior neighbour, so from the second sweep on it is formed from
+// the interior values it is defined by: a colour-0 reader forms it itself (both sources are
+// unchanged at that point), and the colour-0 thread that owns the second source cell stores it
+// just before updating that cell, for the colour-1 reader. The stored planes are therefore
+// exactly what the reference would hold at every read; all planes are rewritten once at the end.
+template<typename T>
+__global__ __launch_bounds__(64) void k_poisson_sweep(T* __restrict__ p, const double* __restrict__ src,
+        const uint8_t* __restrict__ flags, const PoisIJ<double>* __restrict__ pw,
+        int im, int jm, int km, int pitch, size_t plane, int nb, int colour, int ghost, double omega,
+        double c43, double c13){
+    const int b = xcd_remap(blockIdx.x, nb);
+    const int nmb = ((km - 1) / 2 + 63) / 64;
+    const int mb = b % nmb, rest = b / nmb;
+    const int j = rest % (jm - 2) + 1, i = rest / (jm - 2) + 1;
+    const int m = mb * 64 + threadIdx.x;
+    const int k = ((colour + i + j) & 1) + 2 * m;
+    if(k > km - 2 || i > im - 2) return;
+    const size_t P = plane, R = (size_t)pitch;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const T pc = p[q];
+    const bool gw = ghost && colour == 0;      // ghost writes in this pass
+    const bool gr = ghost && colour == 0;      // on-the-fly ghost reads in this pass
+    if(gw){
+        if(i == 2)      p[q - 2 * P] = (T)((T)c43 * p[q - P] - (T)c13 * pc);
+        if(i == im - 3) p[q + 2 * P] = (T)((T)c43 * p[q + P] - (T)c13 * pc);
+        if(j == 2)      p[q - 2 * R] = (T)((T)c43 * p[q - R] - (T)c13 * pc);
+        if(j == jm - 3) p[q + 2 * R] = (T)((T)c43 * p[q + R] - (T)c13 * pc);
+    }
+    const int kw = (k == 0) ? km - 2 : k - 1, ke = (k == km - 2) ? 0 : k + 1;
+    const T pip = (i == im - 2 && gr) ? (T)((T)c43 * pc - (T)c13 * p[q - P]) : p[q + P];
+    const T pim = (i == 1 && gr)      ? (T)((T)c43 * pc - (T)c13 * p[q + P]) : p[q - P];
+    const T pjp = (j == jm - 2 && gr) ? (T)((T)c43 * pc - (T)c13 * p[q - R]) : p[q + R];
+    const T pjm = (j == 1 && gr)      ? (T)((T)c43 * pc - (T)c13 * p[q + R]) : p[q - R];
+    const T pkp = p[q - k + ke], pkm = p[q - k + kw];
+    const uint8_t fl = flags[q];
+    const PoisIJ<double> W = pw[(size_t)i * jm + j];
+    const T num1 = (T)W.num1, num2 = (T)W.num2, num3 = (T)W.num3;
+    T acc, den;
+    if(!(fl & FL_SELF)){
+        acc = den = T(0);
+        if(!(fl & FL_IP)){ acc += num1 * pip; den += num1; }
+        if(!(fl & FL_IM)){ acc += num1 * pim; den += num1; }
+        if(!(fl & FL_JP)){ acc += num2 * pjp; den += num2; }
+        if(!(fl & FL_JM)){ acc += num2 * pjm; den += num2; }
+        if(!(fl & FL_KP)){ acc += num3 * pkp; den += num3; }
+        if(!(fl & FL_KM)){ acc += num3 * pkm; den += num3; }
+        if(den <= T(0)) return;
+    } else {
+        acc = (pip + pim) * num1 + (pjp + pjm) * num2 + (pkp + pkm) * num3;
+        den = (T)W.denom;
+    }
+    const T p_gs = (acc - (T)src[q]) / den;
+    p[q] = (omega == 1.0) ? p_gs : (T)((T(1) - (T)omega) * pc + (T)omega * p_gs);
+}
+
+// the outer boundary after the last sweep: i planes, then j planes, then the seam copy
+template<typename T>
+__global__ void k_poisson_planes_i(T* __restrict__ p, int im, int jm, int km, int pitch, size_t plane,
+        double c43, double c13){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const size_t q = (size_t)j * pitch + k;
+    p[q] = (T)c43 * p[q + plane] - (T)c13 * p[q + 2 * plane];
+    p[q + (size_t)(im - 1) * plane] = (T)c43 * p[q + (size_t)(im - 2) * plane] - (T)c13 * p[q + (size_t)(im - 3) * plane];
+}
+template<typename T>
+__global__ void k_poisson_planes_j(T* __restrict__ p, int im, int jm, int km, int pitch, size_t plane,
+        double c43, double c13){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(k >= km) return;
+    const size_t q = (size_t)i * plane + k, R = (size_t)pitch;
+    p[q] = (T)c43 * p[q + R] - (T)c13 * p[q + 2 * R];
+    p[q + (size_t)(jm - 1) * R] = (T)c43 * p[q + (size_t)(jm - 2) * R] - (T)c13 * p[q + (size_t)(jm - 3) * R];
+}
+template<typename T>
+__global__ void k_seam_copy(T* __restrict__ p, int im, int jm, int km, int pitch){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(j >= jm) return;
+    const size_t q = ((size_t)i * jm + j) * pitch;
+    p[q + km - 1] = p[q];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shapiro filter passes (lib/Utils.cpp:221-453): each pass reads a snapshot and writes a second
+// buffer; a cell outside the fluid (below i_topography) keeps its value.
+// ---------------------------------------------------------------------------------------------
+template<typename T>
+__global__ __launch_bounds__(256) void k_shapiro_k(const T* __restrict__ in, T* __restrict__ out,
+        const int* __restrict__ itopo, int im, int jm, int km, int pitch, int order, double strength){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t r0 = ((size_t)i * jm + j) * pitch;
+    const int kk = (k == km - 1) ? 0 : k;                  // the seam copy takes the value of k = 0
+    const int* it = itopo + (size_t)j * km;
+    auto fl = [&](int kx){ return i >= max(it[kx], 0); };
+    const T c = in[r0 + kk];
+    T res = c;
+    if(fl(kk)){
+        const int n_k = km - 1;
+        const int km1 = (kk == 0) ? km - 2 : kk - 1, kp1 = (kk == km - 2) ? 0 : kk + 1;
+        const T coeff = (T)(strength * 0.25);
+        if(order == 2){
+            const T a = fl(km1) ? in[r0 + km1] : c;
+            const T b = fl(kp1) ? in[r0 + kp1] : c;
+            res = c + coeff * (a - T(2) * c + b);
+        } else if(fl(km1) && fl(kp1)){
+            const int km2 = (kk - 2 + n_k) % n_k, kp2 = (kk + 2) % n_k;
+            if(fl(km2) && fl(kp2)){
+                const T d4 = in[r0 + kp2] - T(4) * in[r0 + kp1] + T(6) * c - T(4) * in[r0 + km1] + in[r0 + km2];
+                res = c - (T)(strength / 16.0) * d4;
+            } else {
+                res = c + coeff * (in[r0 + km1] - T(2) * c + in[r0 + kp1]);
+            }
+        }
+    }
+    out[r0 + k] = res;
+}
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_shapiro_j(const T* __restrict__ in, T* __restrict__ out,
+        const int* __restrict__ itopo, int im, int jm, int km, int pitch, int order, double strength){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t R = (size_t)pitch;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    auto fl = [&](int jj){ return i >= max(itopo[(size_t)jj * km + k], 0); };
+    const T c = in[q];
+    T res = c;
+    if(fl(j)){
+        const T coeff = (T)(strength * 0.25);
+        if(order == 2){
+            const int jm1 = max(j - 1, 0), jp1 = min(j + 1, jm - 1);
+            const T a = fl(jm1) ? in[q - (size_t)(j - jm1) * R] : c;
+            const T b = fl(jp1) ? in[q + (size_t)(jp1 - j) * R] : c;
+            res = c + coeff * (a - T(2) * c + b);
+        } else {
+            const bool fm1 = (j - 1 >= 0) && fl(j - 1), fp1 = (j + 1 <= jm - 1) && fl(j + 1);
+            if(fm1 && fp1){
+                const bool fm2 = (j - 2 >= 0) && fl(j - 2), fp2 = (j + 2 <= jm - 1) && fl(j + 2);
+                if(fm2 && fp2){
+                    const T d4 = in[q + 2 * R] - T(4) * in[q + R] + T(6) * c - T(4) * in[q - R] + in[q - 2 * R];
+                    res = c - (T)(strength / 16.0) * d4;
+                } else {
+                    res = c + coeff * (in[q - R] - T(2) * c + in[q + R]);
+                }
+            }
+        }
+    }
+    out[q] = res;
+}
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_shapiro_i(const T* __restrict__ in, T* __restrict__ out,
+        const int* __restrict__ itopo, int im, int jm, int km, int pitch, size_t plane, int order, double strength){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const int i_surf = max(itopo[(size_t)j * km + k], 0);
+    const T c = in[q];
+    T res = c;
+    if(i >= i_surf){
+        const T coeff = (T)(strength * 0.25);
+        if(order == 2){
+            const int im1 = max(i - 1, i_surf), ip1 = min(i + 1, im - 1);
+            res = c + coeff * (in[q - (size_t)(i - im1) * plane] - T(2) * c + in[q + (size_t)(ip1 - i) * plane]);
+        } else {
+            const bool fm1 = (i - 1 >= i_surf), fp1 = (i + 1 <= im - 1);
+            if(fm1 && fp1){
+                const bool fm2 = (i - 2 >= i_surf), fp2 = (i + 2 <= im - 1);
+                if(fm2 && fp2){
+                    const T d4 = in[q + 2 * plane] - T(4) * in[q + plane] + T(6) * c - T(4) * in[q - plane] + in[q - 2 * plane];
+                    res = c - (T)(strength / 16.0) * d4;
+                } else {
+                    res = c + coeff * (in[q - plane] - T(2) * c + in[q + plane]);
+                }
+            }
+        }
+    }
+    out[q] = res;
+}
+
+// ---------------------------------------------------------------------------------------------
+// computeMixtureDensity (InitValues_Jup.cpp:717-740), every cell
+// ---------------------------------------------------------------------------------------------
+template<typename T>
+__global__ __launch_bounds__(256) void k_mixture_density(const T* __restrict__ t, const T* __restrict__ ps,
+        const T* __restrict__ pd, T* __restrict__ rho, size_t n, double t_ref, double pdyn_w, double ptb, double R_mix){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= n) return;
+    const double T_ = (double)t[q] * t_ref;
+    if(!(T_ > 0.0)){ rho[q] = T(0); return; }
+    const double p_pa = ((double)ps[q] + pdyn_w * (double)pd[q] * ptb) * 1.0e5;
+    const double r = p_pa / (R_mix * T_);
+    rho[q] = isfinite(r) ? (T)r : T(0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Boundary conditions after the RK4 step
+// ---------------------------------------------------------------------------------------------
+constexpr int MAXBC = 32;
+template<typename T>
+struct BCLists {
+    T* radius[MAXBC]; int n_radius;                 // bc_fields_radius (BC_Jup.h:272-290)
+    T* theta_ex[MAXBC]; int n_theta_ex;             // bc_fields_theta_extrap (:293-312)
+    T* theta_zero[MAXBC]; int n_theta_zero;         // bc_fields_theta_zero (:318-325)
+    T* phi[MAXBC]; int n_phi;                       // bc_fields_phi (:328-350)
+    T* species[MAXBC]; int n_species;               // the wall copy / positivity set (BC_Jup.h:232-262)
+    T* solid[MAXBC]; int n_solid;                   // bcSolidGround scalars (BC_Jup.h:469-487)
+    T *u, *v, *w, *t;
+};
+
+// the zonal means of v, w at i = im-3, im-2 before the top sponge, k = 0..km-2 in order
+template<typename T>
+__global__ void k_bc_zonal_means(const T* __restrict__ v, const T* __restrict__ w, double* __restrict__ zm,
+        int im, int jm, int km, int pitch){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int sel = blockIdx.y;                    // 0..3 = (layer 0 v, layer 0 w, layer 1 v, layer 1 w)
+    if(j >= jm) return;
+    const int l = sel >> 1;
+    const T* X = (sel & 1) ? w : v;
+    const size_t r0 = ((size_t)(im - 3 + l) * jm + j) * pitch;
+    double s = 0.0;
+    for(int k = 0; k < km - 1; k++) s += (double)X[r0 + k];
+    zm[(size_t)sel * jm + j] = s / (km - 1);
+}
+
+// BC_Jup::bcRadius (BC_Jup.h:185-263) with the shared bcRadius (BoundaryConditions.h:151-286),
+// ATJUP defaults: 2-point Neumann, rigid lid on u, TOP_TAPER 3, species wall copy, positivity.
+template<typename T>
+__global__ __launch_bounds__(256) void k_bc_radius(const BCLists<T> L, const double* __restrict__ zm,
+        int im, int jm, int km, int pitch, size_t plane, double c43, double c13){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const size_t q0 = (size_t)j * pitch + k;
+    const size_t qt = q0 + (size_t)(im - 1) * plane;
+    for(int f = 0; f < L.n_radius; f++){
+        T* X = L.radius[f];
+        X[q0] = (T)c43 * X[q0 + plane] - (T)c13 * X[q0 + 2 * plane];
+        X[qt] = (T)c43 * X[qt - plane] - (T)c13 * X[qt - 2 * plane];
+    }
+    L.u[q0] = T(0);
+    L.u[qt] = T(0);
+    const double v3 = zm[j], w3 = zm[(size_t)jm + j], v2 = zm[(size_t)2 * jm + j], w2 = zm[(size_t)3 * jm + j];
+    T* V = L.v; T* W = L.w;
+    V[qt] = T(0); W[qt] = T(0);
+    V[qt - plane] *= (T)(1.0 / 3.0);
+    W[qt - plane] *= (T)(1.0 / 3.0);
+    V[qt - 2 * plane] *= (T)(2.0 / 3.0);
+    W[qt - 2 * plane] *= (T)(2.0 / 3.0);
+    V[qt - 2 * plane] += (T)((1.0 / 3.0) * v3);
+    W[qt - 2 * plane] += (T)((1.0 / 3.0) * w3);
+    V[qt - plane] += (T)((2.0 / 3.0) * v2);
+    W[qt - plane] += (T)((2.0 / 3.0) * w2);
+    V[qt] = (T)v2;
+    W[qt] = (T)w2;
+    for(int f = 0; f < L.n_species; f++){
+        T* X = L.species[f];
+        X[q0] = X[q0 + plane];
+        if(X[q0] < T(0)) X[q0] = T(0);
+        if(X[qt] < T(0)) X[qt] = T(0);
+    }
+}
+
+// bcTheta with POLE_FILL 1 (two open rows) and POLE_COPY 1 (BoundaryConditions.h:292-391)
+template<typename T>
+__global__ __launch_bounds__(256) void k_bc_theta(const BCLists<T> L, int im, int jm, int km, int pitch){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(k >= km) return;
+    const size_t R = (size_t)pitch;
+    const size_t qs = (size_t)i * jm * pitch + k;                 // j = 0
+    const size_t qn = qs + (size_t)(jm - 1) * R;                  // j = jm-1
+    for(int f = 0; f < L.n_theta_ex; f++){
+        T* X = L.theta_ex[f];
+        const T a = X[qs + 3 * R], b = X[qn - 3 * R];
+        X[qs + R] = a; X[qs + 2 * R] = a; X[qs] = a;
+        X[qn - R] = b; X[qn - 2 * R] = b; X[qn] = b;
+    }
+    for(int f = 0; f < L.n_theta_zero; f++){
+        T* X = L.theta_zero[f];
+        const T a = X[qs + 3 * R], b = X[qn - 3 * R];
+        X[qs + R] = (T)(1.0 / 3.0) * a;
+        X[qs + 2 * R] = (T)(2.0 / 3.0) * a;
+        X[qn - R] = (T)(1.0 / 3.0) * b;
+        X[qn - 2 * R] = (T)(2.0 / 3.0) * b;
+        X[qs] = T(0);
+        X[qn] = T(0);
+    }
+}
+
+// bcPhi with the periodic seam: k = km-1 is the copy of k = 0 (BoundaryConditions.h:422-438)
+template<typename T>
+__global__ void k_bc_phi(const BCLists<T> L, int im, int jm, int km, int pitch){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(j >= jm) return;
+    const size_t r0 = ((size_t)i * jm + j) * pitch;
+    for(int f = 0; f < L.n_phi; f++){
+        T* X = L.phi[f];
+        X[r0 + km - 1] = X[r0];
+    }
+}
+
+// bcSolidGround over the precomputed list of solid cells (BC_Jup.h:460-589). The extrapolation
+// reads only fluid cells, which this pass does not write.
+template<typename T>
+__global__ void k_bc_solid(const BCLists<T> L, const SolidCell* __restrict__ cells, int n,
+        int jm, int pitch, size_t plane, double c43, double c13){
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if(s >= n) return;
+    const SolidCell c = cells[s];
+    const size_t q = ((size_t)c.i * jm + c.j) * pitch + c.k;
+    L.u[q] = T(0); L.v[q] = T(0); L.w[q] = T(0);
+    const long R = pitch, P = (long)plane;
+    long da = 0;
+    switch(c.code){
+        case 1: da = P; break;  case 2: da = -P; break;
+        case 3: da = R; break;  case 4: da = -R; break;
+        case 5: da = 1; break;  case 6: da = -1; break;
+        default: da = 0; break;
+    }
+    if(da == 0){
+        for(int f = 2; f < L.n_solid; f++) L.solid[f][q] = T(0);    // t, p_stat keep their values
+        return;
+    }
+    for(int f = 0; f < L.n_solid; f++){
+        T* X = L.solid[f];
+        X[q] = (T)c43 * X[q + da] - (T)c13 * X[q + 2 * da];
+    }
+}
+
+// MOUNT_T_AMBIENT (BC_Jup.h:593-608): the solid cells of a row take the mean t of its fluid cells
+template<typename T>
+__global__ void k_bc_t_ambient(T* __restrict__ t, const uint8_t* __restrict__ msk, const int2* __restrict__ rows,
+        int n, int jm, int km, int pitch){
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if(r >= n) return;
+    const int2 ij = rows[r];
+    const size_t r0 = ((size_t)ij.x * jm + ij.y) * pitch;
+    double s = 0.0; int cnt = 0;
+    for(int k = 0; k < km; k++){
+        if(msk[r0 + k]) continue;
+        s += (double)t[r0 + k]; cnt++;
+    }
+    if(cnt == 0) return;
+    const T t_amb = (T)(s / cnt);
+    for(int k = 0; k < km; k++) if(msk[r0 + k]) t[r0 + k] = t_amb;
+}
+
+// clampNegativeSpecies (FileIO_Jup.cpp:494-548)
+template<typename T>
+__global__ __launch_bounds__(256) void k_clamp_species(T* __restrict__ x, size_t n){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= n) return;
+    const T v = x[q];
+    if(!(v >= T(0))) x[q] = T(0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Obstacle set-up: face flags, i_topography and wall viscosity (computeWallViscosity,
+// InitValues_Jup.cpp:84-134), solid-cell classification for bcSolidGround.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_mask_flags(const uint8_t* __restrict__ msk, uint8_t* __restrict__ flags,
+        int im, int jm, int km, int pitch, size_t plane){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const int kw = (k == 0) ? km - 2 : ((k == km - 1) ? km - 2 : k - 1);
+    const int ke = (k >= km - 2) ? 0 : k + 1;
+    uint8_t f = 0;
+    if(msk[q]) f |= FL_SELF;
+    if(i + 1 < im && msk[q + plane]) f |= FL_IP;
+    if(i > 0 && msk[q - plane]) f |= FL_IM;
+    if(j + 1 < jm && msk[q + pitch]) f |= FL_JP;
+    if(j > 0 && msk[q - pitch]) f |= FL_JM;
+    if(msk[q - k + ke]) f |= FL_KP;
+    if(msk[q - k + kw]) f |= FL_KM;
+    flags[q] = f;
+}
+
+__global__ void k_itopo(const uint8_t* __restrict__ msk, int* __restrict__ itopo, int im, int jm, int km,
+        int pitch, size_t plane){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const size_t q0 = (size_t)j * pitch + k;
+    int s = 0;
+    while(s < im && msk[q0 + s * plane]) ++s;
+    itopo[(size_t)j * km + k] = s;
+}
+
+template<typename T>
+__global__ void k_wall_nue(const uint8_t* __restrict__ msk, T* __restrict__ wn, int im, int jm, int km,
+        int pitch, size_t plane, int layers, double val_per_ramp, double re_factor){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    wn[q] = T(0);
+    if(msk[q]) return;
+    int d = layers + 1;
+    for(int di = -layers; di <= layers && d > 1; di++){
+        const int ii = i + di;
+        if(ii < 0 || ii >= im) continue;
+        for(int dj = -layers; dj <= layers && d > 1; dj++){
+            const int jj = j + dj;
+            if(jj < 0 || jj >= jm) continue;
+            for(int dk = -layers; dk <= layers; dk++){
+                const int kk = k + dk;
+                if(kk < 0 || kk >= km) continue;
+                if(!msk[((size_t)ii * jm + jj) * pitch + kk]) continue;
+                const int ad = max(abs(di), max(abs(dj), abs(dk)));
+                if(ad < d) d = ad;
+                if(d <= 1) break;
+            }
+        }
+    }
+    if(d > layers) return;
+    const double ramp = (double)(layers - d + 1) / (double)layers;
+    wn[q] = (T)(re_factor * ramp);
+}
+
+__global__ void k_solid_codes(const uint8_t* __restrict__ msk, int8_t* __restrict__ code, int im, int jm,
+        int km, int pitch, size_t plane){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    if(!msk[q]){ code[q] = -1; return; }
+    auto inb = [&](int ii, int jj, int kk){ return ii >= 0 && ii < im && jj >= 0 && jj < jm && kk >= 0 && kk < km; };
+    auto M = [&](int ii, int jj, int kk){ return msk[((size_t)ii * jm + jj) * pitch + kk] != 0; };
+    auto solid = [&](int ii, int jj, int kk){ return !inb(ii, jj, kk) || M(ii, jj, kk); };
+    auto fluid = [&](int ii, int jj, int kk){ return inb(ii, jj, kk) && !M(ii, jj, kk); };
+    const bool surf = !solid(i + 1, j, k) || !solid(i - 1, j, k) || !solid(i, j + 1, k)
+                   || !solid(i, j - 1, k) || !solid(i, j, k + 1) || !solid(i, j, k - 1);
+    int8_t c = 0;
+    if(surf){
+        if(fluid(i + 1, j, k) && fluid(i + 2, j, k)) c = 1;
+        else if(fluid(i - 1, j, k) && fluid(i - 2, j, k)) c = 2;
+        else if(fluid(i, j + 1, k) && fluid(i, j + 2, k)) c = 3;
+        else if(fluid(i, j - 1, k) && fluid(i, j - 2, k)) c = 4;
+        else if(fluid(i, j, k + 1) && fluid(i, j, k + 2)) c = 5;
+        else if(fluid(i, j, k - 1) && fluid(i, j, k - 2)) c = 6;
+        else c = 7;
+    }
+    code[q] = c;
+}
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_nonfinite(const T* __restrict__ x, int* __restrict__ flag, size_t n){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= n) return;
+    if(!isfinite(x[q])) atomicOr(flag, 1);
+}
+
+// =============================================================================================
+// host launchers
+// =============================================================================================
+namespace {
+
+inline int nblk(size_t n, int b){ return (int)((n + b - 1) / b); }
+
+template<typename T> T* F(atjup_ctx* c, int f){ return (T*)c->cur[f].p; }
+
+inline int check_launch(){
+    hipError_t e = hipGetLastError();
+    if(e != hipSuccess){ set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return ATJUP_ERR_HIP; }
+    return 0;
+}
+
+template<typename T>
+int do_brl(atjup_ctx* c){
+    hipLaunchKernelGGL(k_brl_partial<T>, dim3(BRL_NCH, c->im), dim3(BRL_BLOCK), 0, c->stream,
+        F<T>(c, ATJUP_F_T), F<T>(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p,
+        (const double*)c->tabs.p + c->im, (double*)c->brl_part.p,
+        c->jm, c->km, c->pitch, c->c.g, c->c.r_mix * c->c.R_mix, c->c.t_ref);
+    hipLaunchKernelGGL(k_brl_final, dim3(1), dim3(64), 0, c->stream,
+        (const double*)c->brl_part.p, (double*)c->brl.p, c->im);
+    return check_launch();
+}
+
+template<typename T>
+int do_hydro(atjup_ctx* c){
+    hipLaunchKernelGGL(k_hydrostatic<T>, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream,
+        F<T>(c, ATJUP_F_V), F<T>(c, ATJUP_F_W), (const uint8_t*)c->flags.p,
+        (const double*)c->tabs.p, (const double*)c->tabs.p + c->im + c->jm,
+        F<T>(c, ATJUP_F_P_HYDRO), c->im, c->jm, c->km, c->pitch, c->plane, c->g.dr);
+    return check_launch();
+}
+
+template<typename T>
+int do_rk4_dry(atjup_ctx* c){
+    StageArgs<T> a{};
+    const double dt = c->c.dt;
+    a.pd = F<T>(c, ATJUP_F_P_DYN); a.ph = F<T>(c, ATJUP_F_P_HYDRO); a.ps = F<T>(c, ATJUP_F_P_STAT);
+    a.rho = F<T>(c, ATJUP_F_RHO_MIX); a.wn = F<T>(c, ATJUP_F_WALL_NUE);
+    a.aux_u = F<T>(c, ATJUP_F_AUX_U); a.aux_v = F<T>(c, ATJUP_F_AUX_V); a.aux_w = F<T>(c, ATJUP_F_AUX_W);
+    a.flags = (const uint8_t*)c->flags.p;
+    a.geo = (const GeoIJ<T>*)c->geo.p;
+    a.brl = (const double*)c->brl.p;
+    a.im = c->im; a.jm = c->jm; a.km = c->km; a.pitch = c->pitch; a.plane = c->plane;
+    a.nkb = (c->km + ST_BK - 1) / ST_BK; a.njb = (c->jm + ST_BJ - 1) / ST_BJ;
+    a.nb = a.nkb * a.njb * c->im;
+    a.dt = (T)dt; a.one_sixth = (T)(1.0 / 6.0);
+    a.inv_2dr = (T)(1.0 / (2.0 * c->g.dr)); a.inv_2dthe = (T)(1.0 / (2.0 * c->g.dthe));
+    a.inv_2dphi = (T)(1.0 / (2.0 * c->g.dphi)); a.inv_dr2 = (T)(1.0 / (c->g.dr * c->g.dr));
+    a.inv_dthe2 = (T)(1.0 / (c->g.dthe * c->g.dthe)); a.inv_dphi2 = (T)(1.0 / (c->g.dphi * c->g.dphi));
+    const atjup_consts& C = c->c;
+    a.cep = (T)(C.u_0 * C.u_0 / (C.cp_mix * C.t_ref));
+    a.to_nd = (T)(1.0 / (C.r_mix * C.u_0 * C.u_0 * 1.0e-5));
+    a.r_mix = (T)C.r_mix;
+    a.c_m2o = (T)(1.0 * -2.0 * C.omega);
+    a.c_p2o = (T)(1.0 * +2.0 * C.omega);
+    a.Coriolis = (T)C.Coriolis; a.centrifugal = (T)C.centrifugal;
+    a.cbuoy = (T)(-1.0 * 1.0 * 1.0 * C.buoyancy);
+    a.g = (T)C.g; a.rR = (T)(C.r_mix * C.R_mix); a.t_ref = (T)C.t_ref;
+    a.kt = (T)(c->kn.t_diff_scale / (C.re * C.pr) + 0.0 / 0.9);
+    a.visc0 = (T)(1.0 / C.re + 0.0);
+    for(int s = 0; s < 4; s++){
+        a.stage = s;
+        a.c_in = (T)((s == 0 || s == 1) ? 0.5 * dt : (s == 2 ? dt : 0.0));
+        a.wgt = (T)((s == 0 || s == 3) ? 1.0 : 2.0);
+        a.copy_boundary = (s <= 1);
+        for(int f = 0; f < 4; f++){
+            T* N = F<T>(c, f);
+            T* X = (T*)c->stg[0][f].p;
+            T* Y = (T*)c->stg[1][f].p;
+            const T* in = (s == 0) ? N : (s == 1 ? X : (s == 2 ? Y : X));
+            T* out = (s == 0) ? X : (s == 1 ? Y : (s == 2 ? X : N));
+            a.in[f] = in; a.out[f] = out; a.nl[f] = N; a.acc[f] = (T*)c->acc[f].p;
+        }
+        hipEvent_t t0 = timer_begin(c);
+        hipLaunchKernelGGL(k_stage_dry<T>, dim3(a.nb), dim3(ST_BK, ST_BJ), 0, c->stream, a);
+        timer_end(c, ATJUP_T_RK_STAGES, t0, 1);
+        if(int rc = check_launch()) return rc;
+    }
+    return 0;
+}
+
+template<typename T>
+int do_pressure(atjup_ctx* c, int sweeps, double omega){
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    T* p = F<T>(c, ATJUP_F_P_DYN);
+    hipLaunchKernelGGL(k_prep_projection<T>, dim3(nblk(c->km - 1, 128), c->jm - 2), dim3(128), 0, c->stream,
+        F<T>(c, ATJUP_F_AUX_U), F<T>(c, ATJUP_F_AUX_V), F<T>(c, ATJUP_F_AUX_W),
+        c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
+    const int nkb = (c->km - 1 + 63) / 64;
+    const int nb_src = nkb * (c->jm - 2) * (c->im - 2);
+    hipLaunchKernelGGL(k_poisson_source<T>, dim3(nb_src), dim3(64), 0, c->stream,
+        F<T>(c, ATJUP_F_AUX_U), F<T>(c, ATJUP_F_AUX_V), F<T>(c, ATJUP_F_AUX_W),
+        (const uint8_t*)c->flags.p, (const uint8_t*)c->mask.p, (const PoisIJ<double>*)c->pois.p,
+        (double*)c->psrc.p, c->im, c->jm, c->km, c->pitch, c->plane, nb_src,
+        1.0 / (2.0 * c->g.dr), 1.0 / (2.0 * c->g.dthe), 1.0 / (2.0 * c->g.dphi),
+        1.0 / c->g.dthe, 1.0 / c->g.dphi, c->kn.pdyn_cap);
+    if(int rc = check_launch()) return rc;
+    const int nmb = ((c->km - 1) / 2 + 63) / 64;
+    const int nb_sw = nmb * (c->jm - 2) * (c->im - 2);
+    hipEvent_t t0 = timer_begin(c);
+    for(int s = 0; s < sweeps; s++){
+        for(int colour = 0; colour < 2; colour++){
+            hipLaunchKernelGGL(k_poisson_sweep<T>, dim3(nb_sw), dim3(64), 0, c->stream,
+                p, (const double*)c->psrc.p, (const uint8_t*)c->flags.p, (const PoisIJ<double>*)c->pois.p,
+                c->im, c->jm, c->km, c->pitch, c->plane, nb_sw, colour, s > 0 ? 1 : 0, omega, c43, c13);
+        }
+    }
+    timer_end(c, ATJUP_T_POISSON_SWEEPS, t0, 2 * sweeps);
+    hipLaunchKernelGGL(k_poisson_planes_i<T>, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream,
+        p, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
+    hipLaunchKernelGGL(k_poisson_planes_j<T>, dim3(nblk(c->km, 128), c->im), dim3(128), 0, c->stream,
+        p, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
+    hipLaunchKernelGGL(k_seam_copy<T>, dim3(nblk(c->jm, 64), c->im), dim3(64), 0, c->stream,
+        p, c->im, c->jm, c->km, c->pitch);
+    return check_launch();
+}
+
+template<typename T>
+int do_shapiro(atjup_ctx* c, int field, int order, double strength, int passes, int axes){
+    dim3 grid(nblk(c->km, 128), c->jm, c->im), blk(128);
+    for(int ps = 0; ps < passes; ps++){
+        if(axes & 4){
+            hipLaunchKernelGGL(k_shapiro_k<T>, grid, blk, 0, c->stream, (const T*)c->cur[field].p, (T*)c->scratch.p,
+                (const int*)c->itopo.p, c->im, c->jm, c->km, c->pitch, order, strength);
+            std::swap(c->cur[field].p, c->scratch.p);
+        }
+        if(axes & 2){
+            hipLaunchKernelGGL(k_shapiro_j<T>, grid, blk, 0, c->stream, (const T*)c->cur[field].p, (T*)c->scratch.p,
+                (const int*)c->itopo.p, c->im, c->jm, c->km, c->pitch, order, strength);
+            std::swap(c->cur[field].p, c->scratch.p);
+        }
+        if(axes & 1){
+            hipLaunchKernelGGL(k_shapiro_i<T>, grid, blk, 0, c->stream, (const T*)c->cur[field].p, (T*)c->scratch.p,
+                (const int*)c->itopo.p, c->im, c->jm, c->km, c->pitch, c->plane, order, strength);
+            std::swap(c->cur[field].p, c->scratch.p);
+        }
+    }
+    return check_launch();
+}
+
+template<typename T>
+int do_mixture(atjup_ctx* c){
+    hipLaunchKernelGGL(k_mixture_density<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream,
+        F<T>(c, ATJUP_F_T), F<T>(c, ATJUP_F_P_STAT), F<T>(c, ATJUP_F_P_DYN), F<T>(c, ATJUP_F_RHO_MIX),
+        c->n3, c->c.t_ref, c->kn.rho_pdyn, c->c.r_mix * c->c.u_0 * c->c.u_0 * 1.0e-5, c->c.R_mix);
+    return check_launch();
+}
+
+template<typename T>
+BCLists<T> bc_lists(atjup_ctx* c){
+    BCLists<T> L{};
+    const bool dry = c->kn.dry != 0;
+    auto add = [&](T** arr, int& n, int f){ arr[n++] = F<T>(c, f); };
+    const int species[] = { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE, ATJUP_F_H2S, ATJUP_F_NH3,
+        ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_NH4SH };
+    const int turbf[] = { ATJUP_F_TKE, ATJUP_F_DIS, ATJUP_F_NUE, ATJUP_F_PROD, ATJUP_F_TKE_SOURCE, ATJUP_F_DIS_SOURCE };
+    const int flux3[] = { ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH };
+    for(int f : { ATJUP_F_T, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W }) add(L.radius, L.n_radius, f);
+    for(int f : { ATJUP_F_T, ATJUP_F_U }) add(L.theta_ex, L.n_theta_ex, f);
+    for(int f : { ATJUP_F_V, ATJUP_F_W }) add(L.theta_zero, L.n_theta_zero, f);
+    for(int f : { ATJUP_F_T, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W }) add(L.phi, L.n_phi, f);
+    for(int f : { ATJUP_F_T, ATJUP_F_P_STAT, ATJUP_F_P_DYN }) add(L.solid, L.n_solid, f);
+    if(!dry){
+        for(int f : species){ add(L.radius, L.n_radius, f); add(L.theta_ex, L.n_theta_ex, f);
+                              add(L.phi, L.n_phi, f); add(L.species, L.n_species, f); add(L.solid, L.n_solid, f); }
+        for(int f : flux3){ add(L.radius, L.n_radius, f); add(L.theta_zero, L.n_theta_zero, f);
+                            add(L.phi, L.n_phi, f); add(L.solid, L.n_solid, f); }
+        add(L.theta_zero, L.n_theta_zero, ATJUP_F_FLUXLIM_NH4SH);
+        add(L.phi, L.n_phi, ATJUP_F_FLUXLIM_NH4SH);
+        add(L.solid, L.n_solid, ATJUP_F_FLUXLIM_NH4SH);
+        for(int f : turbf){ add(L.radius, L.n_radius, f); add(L.theta_ex, L.n_theta_ex, f); add(L.phi, L.n_phi, f); }
+    }
+    L.t = F<T>(c, ATJUP_F_T); L.u = F<T>(c, ATJUP_F_U); L.v = F<T>(c, ATJUP_F_V); L.w = F<T>(c, ATJUP_F_W);
+    return L;
+}
+
+template<typename T>
+int do_bc(atjup_ctx* c){
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    const BCLists<T> L = bc_lists<T>(c);
+    hipLaunchKernelGGL(k_bc_zonal_means<T>, dim3(nblk(c->jm, 64), 4), dim3(64), 0, c->stream,
+        L.v, L.w, (double*)c->zm.p, c->im, c->jm, c->km, c->pitch);
+    hipLaunchKernelGGL(k_bc_radius<T>, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream,
+        L, (const double*)c->zm.p, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
+    hipLaunchKernelGGL(k_bc_theta<T>, dim3(nblk(c->km, 128), c->im), dim3(128), 0, c->stream,
+        L, c->im, c->jm, c->km, c->pitch);
+    hipLaunchKernelGGL(k_bc_phi<T>, dim3(nblk(c->jm, 64), c->im), dim3(64), 0, c->stream,
+        L, c->im, c->jm, c->km, c->pitch);
+    if(c->n_solids > 0)
+        hipLaunchKernelGGL(k_bc_solid<T>, dim3(nblk(c->n_solids, 64)), dim3(64), 0, c->stream,
+            L, (const SolidCell*)c->solids.p, c->n_solids, c->jm, c->pitch, c->plane, c43, c13);
+    if(c->n_amb_rows > 0)
+        hipLaunchKernelGGL(k_bc_t_ambient<T>, dim3(nblk(c->n_amb_rows, 64)), dim3(64), 0, c->stream,
+            L.t, (const uint8_t*)c->mask.p, (const int2*)c->amb_rows.p, c->n_amb_rows, c->jm, c->km, c->pitch);
+    return check_launch();
+}
+
+template<typename T>
+int do_clamp(atjup_ctx* c){
+    if(c->kn.dry) return 0;
+    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++)
+        hipLaunchKernelGGL(k_clamp_species<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream, F<T>(c, f), c->n3);
+    return check_launch();
+}
+
+template<typename T>
+int do_mask_setup(atjup_ctx* c){
+    dim3 grid(nblk(c->km, 128), c->jm, c->im), blk(128);
+    hipLaunchKernelGGL(k_mask_flags, grid, blk, 0, c->stream, (const uint8_t*)c->mask.p, (uint8_t*)c->flags.p,
+        c->im, c->jm, c->km, c->pitch, c->plane);
+    hipLaunchKernelGGL(k_itopo, dim3(nblk(c->km, 128), c->jm), blk, 0, c->stream, (const uint8_t*)c->mask.p,
+        (int*)c->itopo.p, c->im, c->jm, c->km, c->pitch, c->plane);
+    const int layers = c->kn.wall_nue_layers;
+    const double factor = c->kn.wall_nue_factor;
+    if(factor > 0.0 && layers > 0)
+        hipLaunchKernelGGL(k_wall_nue<T>, grid, blk, 0, c->stream, (const uint8_t*)c->mask.p, F<T>(c, ATJUP_F_WALL_NUE),
+            c->im, c->jm, c->km, c->pitch, c->plane, layers, 0.0, factor / c->c.re);
+    else
+        ATJUP_HIP_CHECK(hipMemsetAsync(c->cur[ATJUP_F_WALL_NUE].p, 0, c->cur[ATJUP_F_WALL_NUE].bytes, c->stream));
+    // solid-cell classification, compacted on the host once per mask
+    int8_t* dcode = nullptr;
+    ATJUP_HIP_CHECK(hipMalloc((void**)&dcode, c->n3));
+    hipLaunchKernelGGL(k_solid_codes, grid, blk, 0, c->stream, (const uint8_t*)c->mask.p, dcode,
+        c->im, c->jm, c->km, c->pitch, c->plane);
+    std::vector<int8_t> code(c->n3);
+    ATJUP_HIP_CHECK(hipMemcpyAsync(code.data(), dcode, c->n3, hipMemcpyDeviceToHost, c->stream));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    ATJUP_HIP_CHECK(hipFree(dcode));
+    std::vector<SolidCell> cells;
+    std::vector<int2> rows;
+    for(int i = 0; i < c->im; i++)
+        for(int j = 0; j < c->jm; j++){
+            bool any = false;
+            for(int k = 0; k < c->km; k++){
+                const int8_t cd = code[c->idx(i, j, k)];
+                if(cd < 0) continue;
+                any = true;
+                cells.push_back({ i, j, k, cd == 7 ? 0 : cd });
+            }
+            if(any) rows.push_back(make_int2(i, j));
+        }
+    // code 7 (surface cell without a usable pair) is treated like an interior cell; code 0 too
+    c->n_solids = (int)cells.size();
+    c->n_amb_rows = (int)rows.size();
+    if(c->solids.p){ ATJUP_HIP_CHECK(hipFree(c->solids.p)); c->solids.p = nullptr; }
+    if(c->amb_rows.p){ ATJUP_HIP_CHECK(hipFree(c->amb_rows.p)); c->amb_rows.p = nullptr; }
+    if(!cells.empty()){
+        ATJUP_HIP_CHECK(hipMalloc(&c->solids.p, cells.size() * sizeof(SolidCell)));
+        ATJUP_HIP_CHECK(hipMemcpy(c->solids.p, cells.data(), cells.size() * sizeof(SolidCell), hipMemcpyHostToDevice));
+        ATJUP_HIP_CHECK(hipMalloc(&c->amb_rows.p, rows.size() * sizeof(int2)));
+        ATJUP_HIP_CHECK(hipMemcpy(c->amb_rows.p, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    return check_launch();
+}
+
+template<typename T>
+int do_check_finite(atjup_ctx* c){
+    int* flag = (int*)c->nonfinite.p;
+    ATJUP_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
+    const int nf = c->kn.dry ? 4 : ATJUP_N_PROG;
+    for(int f = 0; f < nf; f++)
+        hipLaunchKernelGGL(k_nonfinite<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream, F<T>(c, f), flag, c->n3);
+    int h = 0;
+    ATJUP_HIP_CHECK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return h ? ATJUP_NONFINITE : 0;
+}
+
+}  // namespace
+
+#define DISPATCH(fn, ...) ((c->precision == ATJUP_PRECISION_FP64) ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+int launch_buoy_ref_level(atjup_ctx* c){ return DISPATCH(do_brl, c); }
+int launch_hydrostatic(atjup_ctx* c){ return DISPATCH(do_hydro, c); }
+int launch_rk4(atjup_ctx* c){
+    if(!c->kn.dry){ set_error("the full-physics RK4 stage is not built in this library"); return ATJUP_ERR_PROFILE; }
+    if(int rc = launch_buoy_ref_level(c)) return rc;
+    if(int rc = launch_hydrostatic(c)) return rc;
+    return DISPATCH(do_rk4_dry, c);
+}
+int launch_pressure(atjup_ctx* c, int sweeps, double omega){ return DISPATCH(do_pressure, c, sweeps, omega); }
+int launch_bc_after_step(atjup_ctx* c){ return DISPATCH(do_bc, c); }
+int launch_shapiro(atjup_ctx* c, int field, int order, double strength, int passes, int axes){
+    return DISPATCH(do_shapiro, c, field, order, strength, passes, axes);
+}
+int launch_mixture_density(atjup_ctx* c){ return DISPATCH(do_mixture, c); }
+int launch_clamp_species(atjup_ctx* c){ return DISPATCH(do_clamp, c); }
+int launch_mask_setup(atjup_ctx* c){ return DISPATCH(do_mask_setup, c); }
+int launch_check_finite(atjup_ctx* c){ return DISPATCH(do_check_finite, c); }
+
+}  // namespace atjup

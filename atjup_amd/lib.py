@@ -1,0 +1,194 @@
+"""ctypes face of libatjup_hip.so, the HIP implementation of the ATJUP time step.
+
+``Context`` owns one device context (atjup_hip_ctx_create) and exposes the operators of
+include/atjup_hip.h under the names of the reference entry points they replace. There is no CPU
+fallback: if the library is missing or no GPU is present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libatjup_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "atjup_hip.h")
+
+_D = C.POINTER(C.c_double)
+_lib = None
+
+
+class AtjupError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the library (build it first with atjup_amd.build.build() or __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AtjupError(f"{LIB_PATH} is not built: run `python -m atjup_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.atjup_hip_ctx_create.restype = C.c_void_p
+    L.atjup_hip_ctx_create.argtypes = [C.POINTER(abi.Grid), C.POINTER(abi.Consts),
+                                       C.POINTER(abi.Knobs), C.c_int]
+    L.atjup_hip_ctx_destroy.argtypes = [C.c_void_p]
+    L.atjup_hip_default_knobs.argtypes = [C.POINTER(abi.Knobs)]
+    for n in ("atjup_hip_upload_field", "atjup_hip_download_field"):
+        getattr(L, n).argtypes = [C.c_void_p, C.c_int, _D]
+    for n in ("atjup_hip_upload_field_rows", "atjup_hip_download_field_rows"):
+        getattr(L, n).argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    for n in ("atjup_hip_upload_mask", "atjup_hip_upload_vel_star", "atjup_hip_download_vel_star",
+              "atjup_hip_download_buoy_ref_level"):
+        getattr(L, n).argtypes = [C.c_void_p, _D]
+    for n in ("atjup_hip_rk4_step", "atjup_hip_turbulence_run", "atjup_hip_bc_after_step",
+              "atjup_hip_mixture_density", "atjup_hip_flux_limiter", "atjup_hip_clamp_species",
+              "atjup_hip_swap_nlevel", "atjup_hip_scalar_sync", "atjup_hip_check_finite",
+              "atjup_hip_sync", "atjup_hip_pitch"):
+        getattr(L, n).argtypes = [C.c_void_p]
+    L.atjup_hip_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    L.atjup_hip_satadj.argtypes = [C.c_void_p, C.c_int]
+    L.atjup_hip_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
+    L.atjup_hip_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.atjup_hip_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+    L.atjup_hip_field_device_ptr.restype = C.c_void_p
+    L.atjup_hip_field_device_ptr.argtypes = [C.c_void_p, C.c_int]
+    L.atjup_hip_last_error_string.restype = C.c_char_p
+    L.atjup_hip_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.atjup_hip_read_timers.argtypes = [C.c_void_p, _D]
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names of every function include/atjup_hip.h declares."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(atjup_hip_[a-z_0-9]+)\s*\(", txt)))
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+class Context:
+    """One device-resident ATJUP state on the current HIP device."""
+
+    def __init__(self, grid, consts, knobs, precision: int = abi.PRECISION_FP64):
+        self.L = load()
+        self.shape = (grid.im, grid.jm, grid.km)
+        self.precision = precision
+        self.h = self.L.atjup_hip_ctx_create(C.byref(grid), C.byref(consts), C.byref(knobs), precision)
+        if not self.h:
+            raise AtjupError("atjup_hip_ctx_create: " + self.error())
+
+    def error(self):
+        return self.L.atjup_hip_last_error_string().decode()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.atjup_hip_ctx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _chk(self, rc, what):
+        if rc < 0:
+            raise AtjupError(f"{what}: {self.error()} ({rc})")
+        return rc
+
+    # ---- transfers ---------------------------------------------------------------------
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        self._chk(self.L.atjup_hip_upload_field(self.h, abi.FIELD[name], _dp(a)), f"upload {name}")
+
+    def get(self, name):
+        a = np.empty(self.shape)
+        self._chk(self.L.atjup_hip_download_field(self.h, abi.FIELD[name], _dp(a)), f"download {name}")
+        return a
+
+    def set_mask(self, m):
+        a = np.ascontiguousarray(m, dtype=np.float64)
+        self._chk(self.L.atjup_hip_upload_mask(self.h, _dp(a)), "upload mask")
+
+    def set_vel_star(self, vs):
+        a = np.ascontiguousarray(vs, dtype=np.float64)
+        self._chk(self.L.atjup_hip_upload_vel_star(self.h, _dp(a)), "upload vel_star")
+
+    def buoy_ref_level(self):
+        a = np.empty(self.shape[0])
+        self._chk(self.L.atjup_hip_download_buoy_ref_level(self.h, _dp(a)), "buoy_ref_level")
+        return a
+
+    def load_state(self, st, skip=()):
+        self.set_mask(st.mask)
+        for n, v in st.fields.items():
+            if n in skip:
+                continue
+            self.set(n, v)
+
+    # ---- operators (names follow the reference entry points) --------------------------
+    def rk4_step(self):
+        self._chk(self.L.atjup_hip_rk4_step(self.h), "rk4_step")
+
+    def pressure_solve(self, sweeps=0, omega=1.0):
+        self._chk(self.L.atjup_hip_pressure_solve(self.h, sweeps, omega), "pressure_solve")
+
+    def satadj(self, gas):
+        self._chk(self.L.atjup_hip_satadj(self.h, gas), "satadj")
+
+    def turbulence_run(self):
+        self._chk(self.L.atjup_hip_turbulence_run(self.h), "turbulence_run")
+
+    def bc_after_step(self):
+        self._chk(self.L.atjup_hip_bc_after_step(self.h), "bc_after_step")
+
+    def shapiro(self, name, order=2, strength=1.0, passes=1, axes=7):
+        self._chk(self.L.atjup_hip_shapiro(self.h, abi.FIELD[name], order, strength, passes, axes), "shapiro")
+
+    def mixture_density(self):
+        self._chk(self.L.atjup_hip_mixture_density(self.h), "mixture_density")
+
+    def flux_limiter(self):
+        self._chk(self.L.atjup_hip_flux_limiter(self.h), "flux_limiter")
+
+    def clamp_species(self):
+        self._chk(self.L.atjup_hip_clamp_species(self.h), "clamp_species")
+
+    def swap_nlevel(self):
+        self._chk(self.L.atjup_hip_swap_nlevel(self.h), "swap_nlevel")
+
+    def step(self, iter_n, first=False):
+        self._chk(self.L.atjup_hip_step(self.h, iter_n, 1 if first else 0), "step")
+
+    def check_finite(self):
+        return self._chk(self.L.atjup_hip_check_finite(self.h), "check_finite") == 0
+
+    def sync(self):
+        self._chk(self.L.atjup_hip_sync(self.h), "sync")
+
+    def set_stream(self, stream_ptr):
+        self._chk(self.L.atjup_hip_set_stream(self.h, C.c_void_p(stream_ptr)), "set_stream")
+
+    def device_ptr(self, name):
+        return self.L.atjup_hip_field_device_ptr(self.h, abi.FIELD[name])
+
+    TIMERS = ("poisson_sweeps", "rk_stages", "pressure", "step")
+
+    def set_timing(self, on=True):
+        self._chk(self.L.atjup_hip_set_timing(self.h, 1 if on else 0), "set_timing")
+
+    def read_timers(self):
+        """{phase: (milliseconds, kernel launches)} since the last read (atjup_hip_read_timers)."""
+        buf = (C.c_double * (2 * len(self.TIMERS)))()
+        self._chk(self.L.atjup_hip_read_timers(self.h, buf), "read_timers")
+        return {n: (buf[2 * i], int(buf[2 * i + 1])) for i, n in enumerate(self.TIMERS)}
+
+    @property
+    def pitch(self):
+        return self.L.atjup_hip_pitch(self.h)

@@ -1,0 +1,211 @@
+"""Benchmark of the ATJUP time step on one MI355X: cell-updates/s of the HIP path.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config dyn] [--precision fp64]
+
+A step is one iteration of the time loop (planet/cJupiterModel.cpp:1010-1555) on the 41 x 181 x 361
+shell: the physics block with the 200-sweep red-black pressure solve on the reference's cadence,
+the RK4 step and the boundary passes. Iterations are numbered in 5-step windows 1..5 with the first
+of each window treated as the first iteration of a run, i.e. three pressure solves per five steps,
+exactly the five-step dynamics case. One cell-update = one interior cell (i 1..39, j 3..177,
+k 0..359: 2,457,000 cells) advanced by one step.
+
+The JSON line carries, besides the throughput, the roofline of the dominant kernel (the red-black
+colour kernel of the pressure solve), measured live with HIP events on the library's stream, and
+the CPU baseline: the reference itself (compiled from its own sources into oracle/_ref) timed on
+this host's cores over a five-step sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s FP32-refined vs FP64 at <=1e-5 rel err; adjoint/forward time"
+INTERIOR_CELLS = 39 * 175 * 360          # RK4 pass A range (RungeKutta_Jup_Turb.cpp:388-412)
+HBM_PEAK_GBS = 8000.0
+
+
+def poisson_launch_bytes(g, esz_p=8, omega=1.0):
+    """Algorithmic HBM bytes of one red-black colour launch with perfect neighbour reuse.
+
+    The sweep domain is i 1..im-2, j 1..jm-2, k 0..km-2 (PressureSolver.h:323-358). One colour pass
+    reads the other colour's values (half the field), its own source (half) and its face flags
+    (1 byte, half the cells) and writes its own half of p; with over-relaxation it also reads its
+    own old value.
+    """
+    n = (g.im - 2) * (g.jm - 2) * (g.km - 1)
+    per_cell = 0.5 * esz_p + 0.5 * 8 + 0.5 * 1 + 0.5 * esz_p
+    if omega != 1.0:
+        per_cell += 0.5 * esz_p
+    return n * per_cell
+
+
+def step_bytes_per_cell_update(esz, sweeps, solves_per_step, n_prog=4, n_in=5):
+    """Algorithmic bytes per cell-update of the whole step (the accounting of the design notes)."""
+    rk = esz * (17 * n_prog + 4 * n_in + 5) + 4 * 1
+    pois = 8 * (4 + 3 * sweeps) * solves_per_step
+    return rk + pois
+
+
+def cpu_baseline(config, sweeps, steps=5):
+    """The reference (oracle/_ref) or, without it, the oracle restatement, on this host."""
+    import oracle
+    from atjup_amd import abi, inputs
+    G, Cn, K, env = abi.make_profile(config, sweeps=sweeps)
+    st = inputs.make_state(config, seed=11)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    if oracle.reference_available():
+        ref = oracle.Reference(env)
+        ref.load_state(st, Cn)
+        ref.call("bc_all")
+        ref.call("restore")
+        t0 = time.perf_counter()
+        for it in range(1, steps + 1):
+            ref.step(it, it == 1, satadj=True, turb=(config != "dyn"))
+        dt = time.perf_counter() - t0
+        kind = "reference"
+    else:
+        orc = oracle.Oracle(G, Cn, K, threads=threads)
+        orc.load_state(st)
+        orc.call("bc_after_step")
+        orc.call("swap_nlevel")
+        t0 = time.perf_counter()
+        for it in range(1, steps + 1):
+            orc.call("step", it, 1 if it == 1 else 0)
+        dt = time.perf_counter() - t0
+        kind = "port"
+    return {"value": INTERIOR_CELLS * steps / dt, "unit": "cell-updates/s", "cores": threads,
+            "kind": kind,
+            "sample": f"{steps} steps (iterations 1..{steps}, {sweeps}-sweep solves on 1, 2, 4) of the "
+                      f"same 41x181x361 case, {dt:.1f} s, OMP_NUM_THREADS={threads}, -O2"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="dyn", choices=["dyn"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--sweeps", type=int, default=200)
+    ap.add_argument("--omega", type=float, default=1.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU layout (phi slabs over xGMI) "
+              "is a design, not an implementation. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from atjup_amd import abi, inputs
+    from atjup_amd.lib import Context
+
+    g = inputs.Grid()
+    G, Cn, K, _ = abi.make_profile(args.config, sweeps=args.sweeps, omega=args.omega)
+    st = inputs.make_state(args.config, seed=11)
+    torch.cuda.init()
+    stream = torch.cuda.current_stream()
+    ctx = Context(G, Cn, K, abi.PRECISION_FP64)
+    ctx.set_stream(stream.cuda_stream)
+    ctx.load_state(st, skip=set(inputs.SPECIES) if args.config == "dyn" else set())
+    ctx.bc_after_step()
+    ctx.swap_nlevel()
+
+    n = 0
+
+    def run(k):
+        nonlocal n
+        for _ in range(k):
+            it = n % 5 + 1
+            ctx.step(it, it == 1)
+            n += 1
+
+    run(args.warmup)
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    ctx.read_timers()                                   # clear
+
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    run(args.steps)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(end)
+    tm = ctx.read_timers()
+    sweep_ms, sweep_launches = tm["poisson_sweeps"]
+    rk_ms, rk_launches = tm["rk_stages"]
+    ok = ctx.check_finite()
+
+    value = INTERIOR_CELLS * args.steps / (ms * 1e-3)
+    launch_ms = sweep_ms / max(sweep_launches, 1)
+    lb = poisson_launch_bytes(g, 8, args.omega)
+    achieved = lb / (launch_ms * 1e-3) / 1e9
+    solves_per_step = 3.0 / 5.0
+    step_b = step_bytes_per_cell_update(8, args.sweeps, solves_per_step)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.config, args.sweeps)
+        except Exception as e:  # the baseline never decides the GPU number
+            cpu = {"value": None, "unit": "cell-updates/s", "cores": 0, "kind": "port", "sample": f"failed: {e}"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 12-jet zonal wind, parabolic T with 110 K clamp, ellipsoid obstacle)",
+        "config": {
+            "workload": "dynamics-only FP64 time step, 41x181x361 shell, RK4 + red-black Poisson "
+                        f"({args.sweeps} sweeps, omega={args.omega}) on iterations 1,2,4 of each 5-step window",
+            "grid": [g.im, g.jm, g.km], "sweeps": args.sweeps, "omega": args.omega,
+            "precision": args.precision, "finite": bool(ok),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_poisson_sweep<double> (one red-black colour pass)",
+            "algorithmic_bytes_per_launch": lb,
+            "avg_launch_ms": launch_ms,
+            "launches": int(sweep_launches),
+        },
+        "phases_ms_per_step": {
+            "poisson_sweeps": sweep_ms / args.steps,
+            "rk4_stage_kernels": rk_ms / args.steps,
+            "rk4_stage_avg_launch_ms": rk_ms / max(rk_launches, 1),
+        },
+        "step_algorithmic_bytes_per_cell_update": step_b,
+        "step_hbm_frac": value * step_b / (HBM_PEAK_GBS * 1e9),
+        "wall_s": wall,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -167,6 +167,22 @@ int atjup_hip_step(atjup_ctx* ctx, int iter, int first);
  * returns ATJUP_NONFINITE if any prognostic value is NaN or infinite. */
 int atjup_hip_check_finite(atjup_ctx* ctx);
 
+/* ---- timers ------------------------------------------------------------------------------ */
+/* Phase timers, the counterpart of the reference's own "time measured" lines
+ * (RungeKutta_Jup_Turb.cpp:579, PressureSolver.h:764, cJupiterModel.cpp:1553). With timing on, a
+ * pair of HIP events is recorded on the context's stream around each timed phase;
+ * atjup_hip_read_timers synchronizes and writes, for each phase p, out[2p] = total milliseconds and
+ * out[2p+1] = kernel launches inside the phase, then clears them. */
+enum atjup_timer {
+    ATJUP_T_POISSON_SWEEPS = 0,   /* the red-black colour kernels of one solve (2 launches per sweep) */
+    ATJUP_T_RK_STAGES,            /* the four fused RK4 stage kernels of one step */
+    ATJUP_T_PRESSURE,             /* the whole pressure solve */
+    ATJUP_T_STEP,                 /* one loop iteration (atjup_hip_step) */
+    ATJUP_N_TIMERS
+};
+int atjup_hip_set_timing(atjup_ctx* ctx, int enable);
+int atjup_hip_read_timers(atjup_ctx* ctx, double* out);
+
 /* ---- housekeeping ----------------------------------------------------------------------- */
 int         atjup_hip_sync(atjup_ctx* ctx);
 /* Use an existing HIP stream (hipStream_t passed as void*); NULL restores the context's own. */

@@ -138,6 +138,7 @@ class Reference:
         L.ref_set_vel_star.argtypes = [_D]
         L.ref_get_vel_star.argtypes = [_D]
         L.ref_quiet.argtypes = [C.c_int]
+        L.ref_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         self.L = L
         L.ref_quiet(1 if quiet else 0)
         if L.ref_init(abi.env_string(env).encode()) != 0:
@@ -195,26 +196,8 @@ class Reference:
 
     def step(self, iter_n, first, satadj=True, turb=False):
         """The loop body's on-path functions in the reference's order (cJupiterModel.cpp:1010-1555)."""
-        if iter_n % 2 == 0 or first:
-            self.call("pressure")
-            self.call("damp_pdyn")
-            self.call("mixture_density")
-            if satadj:
-                self.call("satadj_h2o")
-                self.call("satadj_nh3")
-                self.call("satadj_ch4")
-            # ATJUP_RK_SCALAR_SYNC = 2: species and t into the n-level copies
-            for n in ("h2o", "h2o_cloud", "h2o_ice", "nh3", "nh3_cloud", "nh3_ice",
-                      "ch4", "ch4_cloud", "ch4_ice", "t"):
-                self.set(n + "n", self.get(n))
-            if turb:
-                self.call("turbulence")
-        self.call("rk4")
-        self.call("bc_all")
-        if turb:
-            self.call("turb_wall")
-        self.call("clamp_species")
-        self.call("restore")
+        if self.L.ref_step(iter_n, 1 if first else 0, 1 if satadj else 0, 1 if turb else 0) != 0:
+            raise RuntimeError("ref_step failed")
 
 
 def reference_available() -> bool:

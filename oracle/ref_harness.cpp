@@ -165,6 +165,17 @@ public:
                 M.C_ch4, M.L0_ch4, M.R_ch4, M.del_alf_ch4, M.del_bet_ch4,
                 M.C_ch4_ice, M.L0_ch4_ice, M.del_alf_ch4_ice, M.del_bet_ch4_ice,
                 M.ch4, M.ch4_cloud, M.ch4_ice);
+        else if(op == "scalar_sync"){
+            // ATJUP_RK_SCALAR_SYNC = 2 (cJupiterModel.cpp:1118-1140)
+            #pragma omp parallel for collapse(2) schedule(static)
+            for(int i = 0; i < M.im; i++) for(int j = 0; j < M.jm; j++) for(int k = 0; k < M.km; k++){
+                M.h2on.x[i][j][k] = M.h2o.x[i][j][k];       M.h2o_cloudn.x[i][j][k] = M.h2o_cloud.x[i][j][k];
+                M.h2o_icen.x[i][j][k] = M.h2o_ice.x[i][j][k]; M.nh3n.x[i][j][k] = M.nh3.x[i][j][k];
+                M.nh3_cloudn.x[i][j][k] = M.nh3_cloud.x[i][j][k]; M.nh3_icen.x[i][j][k] = M.nh3_ice.x[i][j][k];
+                M.ch4n.x[i][j][k] = M.ch4.x[i][j][k];       M.ch4_cloudn.x[i][j][k] = M.ch4_cloud.x[i][j][k];
+                M.ch4_icen.x[i][j][k] = M.ch4_ice.x[i][j][k]; M.tn.x[i][j][k] = M.t.x[i][j][k];
+            }
+        }
         else if(op == "iter_inc")       M.iter_n++;
         else return -1;
         return 0;
@@ -274,6 +285,28 @@ int ref_call(const char* op){
 
 double ref_buoy_ref_level(int i){ return H::buoy_ref_level(i); }
 double ref_coord(int axis, int n){ return H::coord(axis, n); }
+
+// The loop body's on-path functions in the reference's order (cJupiterModel.cpp:1010-1555), the
+// functions outside the path (radiation, precipitation, chemistry, MCA, diagnostics) left out.
+int ref_step(int iter, int first, int satadj, int turb){
+    quiet_begin();
+    int rc = 0;
+    if(iter % 2 == 0 || first){
+        rc |= H::call("pressure");
+        rc |= H::call("damp_pdyn");
+        rc |= H::call("mixture_density");
+        if(satadj){ rc |= H::call("satadj_h2o"); rc |= H::call("satadj_nh3"); rc |= H::call("satadj_ch4"); }
+        rc |= H::call("scalar_sync");
+        if(turb) rc |= H::call("turbulence");
+    }
+    rc |= H::call("rk4");
+    rc |= H::call("bc_all");
+    if(turb) rc |= H::call("turb_wall");
+    rc |= H::call("clamp_species");
+    rc |= H::call("restore");
+    quiet_end();
+    return rc;
+}
 
 void ref_set_vel_star(const double* src){
     int im, jm, km; H::dims(&im, &jm, &km);

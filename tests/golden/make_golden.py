@@ -1,0 +1,70 @@
+"""Write the golden fixtures of the dynamics path from the reference itself.
+
+Run in a container that holds the reference sources (the recipe compiles them into oracle/_ref,
+`make -C oracle ref`): it drives the compiled reference model through its own entry points
+(oracle/ref_harness.cpp) on the seeded five-step dynamics case and stores a small, fixed sample of
+its outputs. The fixtures are data only: inputs are regenerated from the seed by
+atjup_amd.inputs, expected outputs are the values the reference produced.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from atjup_amd import abi, inputs  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dyn_reference.npz")
+SEED = 7
+SWEEPS = 20
+STEPS = 3
+FIELDS = ("t", "u", "v", "w", "p_dyn", "p_hydro", "rho_mix", "aux_u", "aux_v", "aux_w")
+
+
+def sample_index(shape, n=4096, seed=99):
+    """Fixed cell sample: random cells plus every boundary class (poles, seam, radial planes)."""
+    rng = np.random.default_rng(seed)
+    im, jm, km = shape
+    idx = [rng.integers(0, im, n), rng.integers(0, jm, n), rng.integers(0, km, n)]
+    extra = []
+    for i in (0, 1, im - 2, im - 1):
+        for j in (0, 1, 2, 3, 90, jm - 4, jm - 3, jm - 2, jm - 1):
+            for k in (0, 1, 180, km - 2, km - 1):
+                extra.append((i, j, k))
+    for i in range(0, 10):                                   # the obstacle and its flanks
+        for j in range(100, 125, 3):
+            for k in range(155, 206, 5):
+                extra.append((i, j, k))
+    e = np.array(extra).T
+    return tuple(np.concatenate([a, b]) for a, b in zip(idx, e))
+
+
+def main():
+    import oracle
+    G, Cn, K, env = abi.make_profile("dyn", sweeps=SWEEPS)
+    st = inputs.make_state("dyn", seed=SEED)
+    ref = oracle.Reference(env)
+    ref.load_state(st, Cn)
+    ref.call("bc_all")
+    ref.call("restore")
+    sel = sample_index(ref.shape)
+    out = {"i": sel[0], "j": sel[1], "k": sel[2], "seed": SEED, "sweeps": SWEEPS, "steps": STEPS}
+    for it in range(1, STEPS + 1):
+        ref.step(it, it == 1, satadj=True, turb=False)
+        for n in FIELDS:
+            a = ref.get(n)
+            out[f"s{it}_{n}"] = a[sel]
+            out[f"s{it}_{n}_levelsum"] = a.sum(axis=(1, 2))
+        out[f"s{it}_brl"] = ref.buoy_ref_level()
+    np.savez_compressed(OUT, **out)
+    print(OUT, os.path.getsize(OUT))
+
+
+if __name__ == "__main__":
+    main()

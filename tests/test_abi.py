@@ -1,0 +1,34 @@
+"""The C-ABI library loads and exports every symbol include/atjup_hip.h declares (no GPU needed)."""
+import ctypes
+import os
+
+import pytest
+
+from atjup_amd import abi, lib
+
+
+def test_header_and_library_agree():
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libatjup_hip.so not built")
+    so = ctypes.CDLL(lib.LIB_PATH)
+    names = lib.exported_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, missing
+
+
+def test_struct_layout_matches_header():
+    # atjup_grid: 3 ints, then 5 doubles, 8-byte aligned
+    assert ctypes.sizeof(abi.Grid) == 16 + 5 * 8
+    assert ctypes.sizeof(abi.Consts) == 23 * 8
+    assert abi.FIELD["p_dyn"] == 20 and abi.FIELD["fluxlim_nh4sh"] == 46
+    assert abi.FIELD["tn"] == 64 and abi.FIELD["disn"] == 80
+
+
+def test_default_knobs_symbol():
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libatjup_hip.so not built")
+    L = lib.load()
+    k = abi.Knobs()
+    L.atjup_hip_default_knobs(ctypes.byref(k))
+    assert k.press_omega == 1.0 and k.sinthe_min == 0.55 and k.rk_scalar_sync == 2
