@@ -58,6 +58,31 @@ class Knobs(C.Structure):
                 ("turb_k_bg_phys", C.c_double)]
 
 
+class GasParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "t_0", "t_00", "ep", "lv", "ls", "C", "L0", "R", "del_alf", "del_bet",
+        "C_i", "L0_i", "del_alf_i", "del_bet_i")]
+
+
+# The per-gas constants the reference's time loop hands the saturation adjustment
+# (planet/cJupiterModel.cpp:1060-1080) after its start-up with the default composition and
+# vapour-pressure fits (setSvpFit, setCompositionEp: planet/InitValues_Jup.cpp:898-950).
+GAS_PARAMS = {
+    0: (273.15, 210.15, 7.651416982890988, 2500900.0, 2833900.0, 25.096, 3148.2, 461.52, 0.0, -0.0087,
+        -1.962499, 2641.4013, 1.629479, -0.003361398),
+    1: (195.5, 140.0, 7.232930612074716, 1372000.0, 1832000.0, 196.077451, 3586.5634, 488.21, -15.978352,
+        0.02837146, 16.086062, 1857.9898, 0.098768, -0.001179989),
+    2: (90.69, 67.36, 6.813102176498581, 511000.0, 511000.0, 1.627, 553.1, 518.28, 1.002, -0.0041,
+        1.627, 553.1, 1.002, -0.0041),
+}
+GAS_FIELDS = {0: ("h2o", "h2o_cloud", "h2o_ice"), 1: ("nh3", "nh3_cloud", "nh3_ice"),
+              2: ("ch4", "ch4_cloud", "ch4_ice")}
+
+
+def gas_params(gas: int) -> GasParams:
+    return GasParams(*GAS_PARAMS[gas])
+
+
 # Reference environment shared by every profile: the balanced start, vortex, moist convective
 # adjustment, cone projection, continuity init and NH4SH equilibrium are off (they are outside this
 # path), the metric radius is 71492 km, dt 0.005.

@@ -373,13 +373,27 @@ int atjup_hip_pressure_solve(atjup_ctx* c, int sweeps, double omega){
     timer_end(c, ATJUP_T_PRESSURE, t0, 0);
     return rc;
 }
-int atjup_hip_satadj(atjup_ctx* c, int gas){
+int atjup_hip_set_gas(atjup_ctx* c, int gas, const atjup_gas_params* p){
+    if(!c || !p){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(gas < 0 || gas > 2){ set_error("gas"); return ATJUP_ERR_ARG; }
+    c->gas[gas] = *p;
+    c->gas_set[gas] = true;
+    return 0;
+}
+int atjup_hip_satadj(atjup_ctx* c, int gas, const atjup_gas_params* p){
     if(int rc = need_mask(c)) return rc;
     if(gas < 0 || gas > 2){ set_error("gas"); return ATJUP_ERR_ARG; }
     if(c->kn.dry) return 0;        // all species zero: the adjustment is the identity (SaturationAdjustment.h:440)
+    if(!p && !c->gas_set[gas]){ set_error("gas constants not set (atjup_hip_set_gas)"); return ATJUP_ERR_STATE; }
+    (void)p;
     set_error("the saturation adjustment is not built in this library"); return ATJUP_ERR_PROFILE;
 }
 int atjup_hip_turbulence_run(atjup_ctx* c){
+    if(int rc = need_mask(c)) return rc;
+    if(c->kn.turb_model < 0) return 0;
+    set_error("the turbulence closure is not built in this library"); return ATJUP_ERR_PROFILE;
+}
+int atjup_hip_turb_wall(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;
     if(c->kn.turb_model < 0) return 0;
     set_error("the turbulence closure is not built in this library"); return ATJUP_ERR_PROFILE;
@@ -414,7 +428,7 @@ int atjup_hip_step(atjup_ctx* c, int iter, int first){
         if(c->kn.damp_pdyn && (rc = launch_shapiro(c, ATJUP_F_P_DYN, 2, 1.0, 1, 7))) return rc;
         if((rc = launch_mixture_density(c))) return rc;
         for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
-            if((rc = atjup_hip_satadj(c, gas))) return rc;
+            if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
         if((rc = atjup_hip_turbulence_run(c))) return rc;
         if((rc = atjup_hip_flux_limiter(c))) return rc;
     }
@@ -457,6 +471,14 @@ int atjup_hip_read_timers(atjup_ctx* c, double* out){
 int atjup_hip_sync(atjup_ctx* c){
     if(!c) return ATJUP_ERR_ARG;
     ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int atjup_hip_device_count(int* count){
+    if(!count) return ATJUP_ERR_ARG;
+    *count = 0;
+    int n = 0;
+    ATJUP_HIP_CHECK(hipGetDeviceCount(&n));
+    *count = n;
     return 0;
 }
 int atjup_hip_set_stream(atjup_ctx* c, void* s){

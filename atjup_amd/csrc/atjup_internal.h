@@ -93,6 +93,8 @@ struct atjup_ctx {
     atjup::Buf solids;           // SolidCell list
     atjup::Buf amb_rows;         // int2 list of (i, j) rows that hold solid cells
     atjup::Buf nonfinite;        // int flag
+    atjup_gas_params gas[3] = {};   // saturation-adjustment constants (atjup_hip_set_gas)
+    bool gas_set[3] = { false, false, false };
     int n_solids = 0, n_amb_rows = 0;
     bool have_mask = false;
 

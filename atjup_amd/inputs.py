@@ -207,26 +207,31 @@ def make_state(config: str = "dyn", seed: int = 1234, g: Grid | None = None,
 
 
 def _moisten(st: State, rng):
-    """Vapour at seeded U(0.5, 1.0) x saturation for H2O, NH3 and NH4SH-forming NH3/H2S; k-omega init."""
+    """Moist state of the full-physics case, seeded.
+
+    Vapour of H2O, NH3 and CH4 at U(0.6, 1.25) x saturation over the liquid (so that part of the
+    shell is supersaturated and the saturation adjustment condenses there), a condensate seed of
+    cloud and ice in a random tenth of the cells of each gas (so that the evaporation and
+    Bergeron branches run), H2S and NH4SH traces, and the k-omega start values.
+    """
+    from .abi import GAS_PARAMS, GAS_FIELDS
     g = st.grid
     f = st.fields
     shp = g.shape
     T = f["t"] * 165.0
     P = f["p_stat"]
     rho = P * 1.0e5 / (R_MIX * T)
-    # Sanchez-Lavega vapour pressures [bar] (planet/ATPhys.h:49-52) with the reference's liquid sets
-    def svp(C, L0, R, da, db):
-        return np.exp(C + (-L0 / T + da * np.log(T) + db * T) / (1e-3 * R))
-    e_h2o = svp(25.096, 3148.2, 461.52, 0.0, -8.7e-3)
-    e_nh3 = svp(27.863, 2016.0, 488.21, -0.888, 0.0)
-    ep_h2o, ep_nh3 = 8.115, 7.671
-    q_h2o = rho * ep_h2o * e_h2o / P
-    q_nh3 = rho * ep_nh3 * e_nh3 / P
-    f["h2o"] = np.minimum(q_h2o, 0.09) * rng.uniform(0.5, 1.0, shp)
-    f["nh3"] = np.minimum(q_nh3, 0.009) * rng.uniform(0.5, 1.0, shp)
-    f["h2s"] = np.minimum(0.5 * q_nh3, 0.0004) * rng.uniform(0.5, 1.0, shp)
-    f["ch4"] = 0.0019 * rho
-    f["nh4sh"] = np.zeros(shp)
+    for gas, (vap, cld, ice) in GAS_FIELDS.items():
+        t_0, t_00, ep, lv, ls, C, L0, R, da, db = GAS_PARAMS[gas][:10]
+        # Sanchez-Lavega vapour pressure over the liquid [bar] (planet/ATPhys.h:49-52)
+        e = np.exp(C + (-L0 / T + da * np.log(T) + db * T) / (1e-3 * R))
+        q_sat = np.minimum(rho * ep * e / P, 0.05)
+        f[vap] = q_sat * rng.uniform(0.6, 1.25, shp)
+        seed = rng.uniform(size=shp) < 0.1
+        f[cld] = np.where(seed, 0.05 * q_sat * rng.uniform(size=shp), 0.0)
+        f[ice] = np.where(seed & (T < t_0), 0.05 * q_sat * rng.uniform(size=shp), 0.0)
+    f["h2s"] = 0.0004 * rng.uniform(0.5, 1.0, shp)
+    f["nh4sh"] = 1.0e-5 * rng.uniform(0.0, 1.0, shp)
     # k = 1e-4 m2/s2 -> 1e-8 in u_0^2; omega = 1e-3 1/s -> 1e-3 * L / u_0 = 1.4
     f["tke"] = np.full(shp, 1.0e-4 / 100.0 ** 2)
     f["dis"] = np.full(shp, 1.0e-3 * 140.0e3 / 100.0)

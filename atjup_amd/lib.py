@@ -45,13 +45,16 @@ def load():
     for n in ("atjup_hip_upload_mask", "atjup_hip_upload_vel_star", "atjup_hip_download_vel_star",
               "atjup_hip_download_buoy_ref_level"):
         getattr(L, n).argtypes = [C.c_void_p, _D]
-    for n in ("atjup_hip_rk4_step", "atjup_hip_turbulence_run", "atjup_hip_bc_after_step",
+    for n in ("atjup_hip_rk4_step", "atjup_hip_turbulence_run", "atjup_hip_turb_wall",
+              "atjup_hip_bc_after_step",
               "atjup_hip_mixture_density", "atjup_hip_flux_limiter", "atjup_hip_clamp_species",
               "atjup_hip_swap_nlevel", "atjup_hip_scalar_sync", "atjup_hip_check_finite",
               "atjup_hip_sync", "atjup_hip_pitch"):
         getattr(L, n).argtypes = [C.c_void_p]
+    L.atjup_hip_device_count.argtypes = [C.POINTER(C.c_int)]
     L.atjup_hip_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
-    L.atjup_hip_satadj.argtypes = [C.c_void_p, C.c_int]
+    L.atjup_hip_satadj.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
+    L.atjup_hip_set_gas.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
     L.atjup_hip_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
     L.atjup_hip_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.atjup_hip_set_stream.argtypes = [C.c_void_p, C.c_void_p]
@@ -62,6 +65,16 @@ def load():
     L.atjup_hip_read_timers.argtypes = [C.c_void_p, _D]
     _lib = L
     return L
+
+
+def device_count() -> int:
+    """HIP devices the library's runtime sees (0 when there is none or the library is missing)."""
+    try:
+        L = load()
+    except (AtjupError, OSError):
+        return 0
+    n = C.c_int(0)
+    return n.value if L.atjup_hip_device_count(C.byref(n)) == 0 else 0
 
 
 def exported_symbols():
@@ -85,6 +98,9 @@ class Context:
         self.h = self.L.atjup_hip_ctx_create(C.byref(grid), C.byref(consts), C.byref(knobs), precision)
         if not self.h:
             raise AtjupError("atjup_hip_ctx_create: " + self.error())
+        if not knobs.dry:
+            for gas in abi.GAS_PARAMS:
+                self.set_gas(gas)
 
     def error(self):
         return self.L.atjup_hip_last_error_string().decode()
@@ -139,11 +155,19 @@ class Context:
     def pressure_solve(self, sweeps=0, omega=1.0):
         self._chk(self.L.atjup_hip_pressure_solve(self.h, sweeps, omega), "pressure_solve")
 
-    def satadj(self, gas):
-        self._chk(self.L.atjup_hip_satadj(self.h, gas), "satadj")
+    def set_gas(self, gas, params=None):
+        gp = params if params is not None else abi.gas_params(gas)
+        self._chk(self.L.atjup_hip_set_gas(self.h, gas, C.byref(gp)), "set_gas")
+
+    def satadj(self, gas, params=None):
+        p = C.byref(params) if params is not None else None
+        self._chk(self.L.atjup_hip_satadj(self.h, gas, p), "satadj")
 
     def turbulence_run(self):
         self._chk(self.L.atjup_hip_turbulence_run(self.h), "turbulence_run")
+
+    def turb_wall(self):
+        self._chk(self.L.atjup_hip_turb_wall(self.h), "turb_wall")
 
     def bc_after_step(self):
         self._chk(self.L.atjup_hip_bc_after_step(self.h), "bc_after_step")

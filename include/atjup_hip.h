@@ -102,6 +102,16 @@ typedef struct atjup_knobs {
     double turb_k_bg_phys;      /* ATJUP_TURB_K_BG [m2/s2] */
 } atjup_knobs;
 
+/* The saturation adjustment's constants of one gas, the arguments of
+ * SaturationAdjustment<P>::run (planet/SaturationAdjustment.h:236-243) as the time loop passes them
+ * (planet/cJupiterModel.cpp:1060-1080): triple point and ice onset [K], ep, latent heats [J/kg],
+ * and the Sanchez-Lavega liquid and ice vapour-pressure sets (planet/ATPhys.h:49-52). */
+typedef struct atjup_gas_params {
+    double t_0, t_00, ep, lv, ls;
+    double C, L0, R, del_alf, del_bet;
+    double C_i, L0_i, del_alf_i, del_bet_i;
+} atjup_gas_params;
+
 typedef struct atjup_ctx atjup_ctx;
 
 /* ---- context ---------------------------------------------------------------------------- */
@@ -139,11 +149,20 @@ int atjup_hip_rk4_step(atjup_ctx* ctx);
  * prepareProjectionBoundaries (planet/cJupiterModel.cpp:1842-1864). sweeps <= 0 takes the
  * profile's count; omega = 1 is the reference's Gauss-Seidel. */
 int atjup_hip_pressure_solve(atjup_ctx* ctx, int sweeps, double omega);
-/* SaturationAdjustment<cJupiterModel>::run for one gas (planet/SaturationAdjustment.h:255-746):
- * gas 0 = H2O, 1 = NH3, 2 = CH4, with the reference's per-gas constants. */
-int atjup_hip_satadj(atjup_ctx* ctx, int gas);
-/* Turbulence<cJupiterModel>::run (planet/Turbulence.h:307-1106). */
+/* SaturationAdjustment<cJupiterModel>::run for one gas (planet/SaturationAdjustment.h:255-746),
+ * gas 0 = H2O (h2o, h2o_cloud, h2o_ice), 1 = NH3, 2 = CH4, with the constants the reference passes
+ * as its 14 double arguments; params = NULL uses the context's set (atjup_hip_set_gas). */
+int atjup_hip_satadj(atjup_ctx* ctx, int gas, const atjup_gas_params* params);
+/* The constants atjup_hip_step hands the adjustment, gas by gas in the reference's order H2O, NH3,
+ * CH4 (planet/cJupiterModel.cpp:1060-1080). Until set, a gas's adjustment fails with
+ * ATJUP_ERR_STATE. */
+int atjup_hip_set_gas(atjup_ctx* ctx, int gas, const atjup_gas_params* params);
+/* Turbulence<cJupiterModel>::run (planet/Turbulence.h:307-1106) followed by the wall-function
+ * omega Run() applies after it (planet/cJupiterModel.cpp:356-374, 1144). */
 int atjup_hip_turbulence_run(atjup_ctx* ctx);
+/* The closure's wall condition held after the step's boundary passes: apply_wall_bc and the
+ * wall-function omega (planet/cJupiterModel.cpp:1294). */
+int atjup_hip_turb_wall(atjup_ctx* ctx);
 /* BC_Jup::bcRadius, bcTheta, bcPhi, bcSolidGround in that order (planet/BC_Jup.h:185-658,
  * planet/BoundaryConditions.h:151-518), as the time loop calls them (cJupiterModel.cpp:1263-1272). */
 int atjup_hip_bc_after_step(atjup_ctx* ctx);
@@ -185,7 +204,11 @@ int atjup_hip_read_timers(atjup_ctx* ctx, double* out);
 
 /* ---- housekeeping ----------------------------------------------------------------------- */
 int         atjup_hip_sync(atjup_ctx* ctx);
-/* Use an existing HIP stream (hipStream_t passed as void*); NULL restores the context's own. */
+/* Number of HIP devices visible to the library's runtime (0 and an error status without one). */
+int         atjup_hip_device_count(int* count);
+/* Use an existing HIP stream (hipStream_t passed as void*); NULL restores the context's own. The
+ * stream must come from the same HIP runtime instance the library is linked against (ROCm's
+ * libamdhip64.so.7); a handle from another copy of the runtime in the process is not valid here. */
 int         atjup_hip_set_stream(atjup_ctx* ctx, void* stream);
 /* Device address of a field (FP64 or FP32 storage, row pitch atjup_hip_pitch) for zero-copy views. */
 void*       atjup_hip_field_device_ptr(atjup_ctx* ctx, int field);

@@ -52,10 +52,12 @@ def _orc_lib():
         for n in ("orc_rk4_step", "orc_buoy_ref_level", "orc_hydrostatic", "orc_bc_after_step",
                   "orc_bc_radius", "orc_bc_theta", "orc_bc_phi", "orc_bc_solid",
                   "orc_mixture_density", "orc_clamp_species", "orc_swap_nlevel",
-                  "orc_scalar_sync", "orc_turbulence_run", "orc_flux_limiter", "orc_check_finite"):
+                  "orc_scalar_sync", "orc_turbulence_run", "orc_turb_wall", "orc_flux_limiter",
+                  "orc_check_finite"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.orc_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_satadj.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_gas.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
         L.orc_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
@@ -74,6 +76,9 @@ class Oracle:
         self.h = self.L.orc_create(C.byref(grid), C.byref(consts), C.byref(knobs))
         if not self.h:
             raise RuntimeError("orc_create failed")
+        for gas in abi.GAS_PARAMS:
+            gp = abi.gas_params(gas)
+            self._chk(self.L.orc_set_gas(self.h, gas, C.byref(gp)), "set_gas")
 
     def close(self):
         if self.h:

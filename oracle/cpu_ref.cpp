@@ -52,6 +52,7 @@ struct orc_ctx {
     std::vector<int> itopo;                          // i_topography [j][k]
     std::vector<double> vel_star;                    // [j][k]
     std::vector<double> brl;                         // buoy_ref_level [i]
+    atjup_gas_params gas[3] = {};                    // saturation-adjustment constants per gas
     double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
 
     size_t id(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
@@ -1098,6 +1099,427 @@ void wall_viscosity(orc_ctx& o){                          // InitValues_Jup.cpp:
             }
 }
 
+// ---- SaturationAdjustment<P>::run for one gas (SaturationAdjustment.h:255-746), ATJUP defaults:
+//      CONSERVE 1, NEWTON 1, EVAP 1, BERGERON 2, DRY_EXIT 1, ICE_ENTRY 0, FREEZE 0, LOCAL_RHO 1 ------
+inline double svp(double T, double C, double L0, double R, double da, double db){   // ATPhys.h:49-52
+    return std::exp(C + (-L0 / T + da * std::log(T) + db * T) / (1e-3 * R));
+}
+
+void satadj(orc_ctx& o, int gas){
+    static const int ids[3][3] = { { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE },
+                                   { ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE },
+                                   { ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE } };
+    const atjup_gas_params P = o.gas[gas];
+    auto& c = o.f[ids[gas][0]];
+    auto& cloud = o.f[ids[gas][1]];
+    auto& ice = o.f[ids[gas][2]];
+    auto& t = o.f[ATJUP_F_T];
+    const auto& ps = o.f[ATJUP_F_P_STAT];
+    const auto& rho = o.f[ATJUP_F_RHO_MIX];
+    const double t_0 = P.t_0, t_00 = P.t_00, ep = P.ep, lv = P.lv, ls = P.ls;
+    const double t_range_inv = 1.0 / (t_0 - t_00);
+    const double Lf = ls - lv;
+    const double cp_mix = o.c.cp_mix, t_ref = o.c.t_ref, r_mix = o.c.r_mix;
+    const int iter_prec_end = 15;
+    const double q_diff_min = 1.0e-3;
+    #pragma omp parallel for schedule(static)
+    for(size_t q = 0; q < o.n3; q++){
+        double t_u = t[q] * t_ref;
+        const double p_u = ps[q];
+        if(!(t_u > 0.0) || !(p_u > 0.0) || o.mask[q] == 1.0) continue;
+        const double rf = rho[q];
+        const double rho_at = (rf > 0.0 && std::isfinite(rf)) ? rf : r_mix;       // cJupiterModel.h:1212-1216
+        if(t_u > t_0){
+            if(ice[q] > 0.0){
+                double dm = ice[q];
+                if(Lf > 0.0){
+                    const double hc = cp_mix * rho_at;
+                    const double cap = hc * (t_u - t_0) / Lf;
+                    if(dm > cap) dm = cap;
+                    t_u -= Lf * dm / hc;
+                    t[q] = t_u / t_ref;
+                }
+                ice[q] -= dm;
+                cloud[q] += dm;
+            }
+        }
+        if(c[q] < 0.0) c[q] = 0.0;
+        if(cloud[q] < 0.0) cloud[q] = 0.0;
+        if(ice[q] < 0.0) ice[q] = 0.0;
+        const double rho_c = rho_at;
+        const double E_Rain_0 = svp(t_u, P.C, P.L0, P.R, P.del_alf, P.del_bet);
+        const double q_Rain_0 = rho_c * ep * E_Rain_0 / p_u;
+        if(q_Rain_0 <= 0.0) continue;
+        double q_entry = q_Rain_0;
+        if(cloud[q] <= 0.0 && ice[q] <= 0.0 && t_u < t_0)
+            q_entry = rho_c * ep * svp(t_u, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
+        if(c[q] <= q_entry){
+            const double q_c0 = cloud[q], q_i0 = ice[q];
+            if(q_c0 <= 0.0 && q_i0 <= 0.0) continue;
+            const double q_Ice_0 = rho_c * ep * svp(t_u, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
+            double q_t0;
+            if(q_c0 > 0.0 && q_i0 > 0.0) q_t0 = (q_c0 * q_Rain_0 + q_i0 * q_Ice_0) / (q_c0 + q_i0);
+            else if(q_i0 <= 0.0)         q_t0 = q_Rain_0;
+            else                         q_t0 = q_Ice_0;
+            if(std::fabs(c[q] - q_t0) <= q_diff_min * q_t0) continue;
+        }
+        double q_v_b = c[q], q_c_b = cloud[q], q_i_b = ice[q];
+        double T = t_u;
+        double q_v_hyp = q_v_b;
+        double q_rain_now = q_Rain_0;
+        for(int itr = 1; itr <= iter_prec_end; itr++){
+            double CND = (T - t_00) * t_range_inv;
+            double DEP = (t_0 - T) * t_range_inv;
+            if(T <= t_00){ CND = 0.0; DEP = 1.0; }
+            if(T >= t_0){ CND = 1.0; DEP = 0.0; }
+            const bool berg = T < t_0 && q_v_b <= q_rain_now && (q_i_b > 0.0 || q_c_b <= 0.0);
+            if(berg){
+                CND = 0.0; DEP = 1.0;
+                if(q_c_b > 0.0){
+                    T -= lv * q_c_b / (cp_mix * rho_c);
+                    q_v_b += q_c_b;
+                    q_c_b = 0.0;
+                }
+            }
+            double d_q_v = q_v_hyp - q_v_b;
+            double d_q_c = -d_q_v * CND;
+            double d_q_i = -d_q_v * DEP;
+            if(d_q_v < -q_v_b){ d_q_v = -q_v_b; d_q_c = -d_q_v * CND; d_q_i = -d_q_v * DEP; }
+            if(q_c_b + d_q_c < 0.0){ const double ex = -(q_c_b + d_q_c); d_q_c = -q_c_b; d_q_i -= ex; }
+            if(q_i_b + d_q_i < 0.0){ const double ex = -(q_i_b + d_q_i); d_q_i = -q_i_b; d_q_c -= ex;
+                                     if(q_c_b + d_q_c < 0.0) d_q_c = -q_c_b; }
+            d_q_v = -(d_q_c + d_q_i);
+            T += (lv * d_q_c + ls * d_q_i) / (cp_mix * rho_c);
+            q_v_b += d_q_v;
+            q_c_b += d_q_c;
+            q_i_b += d_q_i;
+            if(q_v_b < 0.0) q_v_b = 0.0;
+            if(q_c_b < 0.0) q_c_b = 0.0;
+            if(q_i_b < 0.0) q_i_b = 0.0;
+            const double E_Rain = svp(T, P.C, P.L0, P.R, P.del_alf, P.del_bet);
+            const double E_Ice = svp(T, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i);
+            const double q_Rain = rho_c * ep * E_Rain / p_u;
+            const double q_Ice = rho_c * ep * E_Ice / p_u;
+            q_rain_now = q_Rain;
+            if(q_c_b > 0.0 && q_i_b > 0.0) q_v_hyp = (q_c_b * q_Rain + q_i_b * q_Ice) / (q_c_b + q_i_b);
+            else if(q_c_b <= 0.0 && q_i_b <= 0.0 && T < t_0) q_v_hyp = q_Ice;
+            else if(q_i_b == 0.0) q_v_hyp = q_Rain;
+            else                  q_v_hyp = q_Ice;
+            if(T >= t_0){
+                if(q_i_b > 0.0){
+                    double dm = q_i_b;
+                    if(Lf > 0.0){
+                        const double hc = cp_mix * rho_c;
+                        const double cap = hc * (T - t_0) / Lf;
+                        if(dm > cap) dm = cap;
+                        T -= Lf * dm / hc;
+                    }
+                    q_i_b -= dm;
+                    q_c_b += dm;
+                }
+            }
+            const double q_diff = std::fabs(q_v_b - q_v_hyp) / (q_v_hyp + 1e-20);
+            if(q_diff <= q_diff_min) break;
+            if(q_c_b <= 0.0 && q_i_b <= 0.0 && q_v_b < q_v_hyp) break;
+            const double dT = 0.1;
+            const double q_Rain_d = rho_c * ep * svp(T + dT, P.C, P.L0, P.R, P.del_alf, P.del_bet) / p_u;
+            const double q_Ice_d = rho_c * ep * svp(T + dT, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
+            double q_hyp_d;
+            if(q_c_b > 0.0 && q_i_b > 0.0) q_hyp_d = (q_c_b * q_Rain_d + q_i_b * q_Ice_d) / (q_c_b + q_i_b);
+            else if(q_c_b <= 0.0 && q_i_b <= 0.0 && T < t_0) q_hyp_d = q_Ice_d;
+            else if(q_i_b == 0.0) q_hyp_d = q_Rain_d;
+            else                  q_hyp_d = q_Ice_d;
+            double CNDn = (T - t_00) * t_range_inv;
+            if(T <= t_00) CNDn = 0.0;
+            if(T >= t_0) CNDn = 1.0;
+            if(berg) CNDn = 0.0;
+            const double G = (lv * CNDn + ls * (1.0 - CNDn)) / (cp_mix * rho_c) * (q_hyp_d - q_v_hyp) / dT;
+            double w = 1.0 / (1.0 + std::max(G, 0.0));
+            if(w < 0.05) w = 0.05;
+            q_v_hyp = q_v_b + w * (q_v_hyp - q_v_b);
+        }
+        c[q] = q_v_b;
+        cloud[q] = q_c_b;
+        ice[q] = q_i_b;
+        t[q] = T / t_ref;
+    }
+    #pragma omp parallel for schedule(static)
+    for(size_t q = 0; q < o.n3; q++){
+        if(c[q] <= 0.0) c[q] = 0.0;
+        if(cloud[q] <= 0.0) cloud[q] = 0.0;
+        if(ice[q] <= 0.0) ice[q] = 0.0;
+    }
+}
+
+// ---- Turbulence<P>::run, k-omega, TURB_FREE 1, TURB_CLAMP_PASS 1, TURB_CURV 1, no ABL taper
+//      (Turbulence.h:307-1012), then the wall-omega function of Run() (cJupiterModel.cpp:356-374) --
+void turb_wall_omega(orc_ctx& o){
+    const double nue_gas_phys = 1.8e-5, bet_star = 0.09, bet_1 = 0.0333, kappa = 0.41;
+    const double nd_omega = o.g.L_atm * 1.0e3 / o.c.u_0;
+    auto& dis = o.f[ATJUP_F_DIS];
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 0; j < o.jm; j++)
+        for(int k = 0; k < o.km; k++){
+            const int i_w = o.itopo[(size_t)j * o.km + k];
+            if(i_w <= 0 || i_w >= o.im - 1) continue;
+            const double y1 = std::max(o.layer_thickness_m(std::max(i_w - 1, 0)), 1.0);
+            const double y = 0.5 * y1;
+            const double om_vis = 60.0 * nue_gas_phys / (bet_1 * y1 * y1);
+            const double om_log = o.vel_star[(size_t)j * o.km + k] / (std::sqrt(bet_star) * kappa * y);
+            dis[o.id(i_w, j, k)] = std::sqrt(om_vis * om_vis + om_log * om_log) * nd_omega;
+        }
+}
+
+void turb_apply_wall_bc(orc_ctx& o){              // Turbulence.h:378-420, TURB_FREE: i = 0 copy
+    auto& tke = o.f[ATJUP_F_TKE];
+    auto& dis = o.f[ATJUP_F_DIS];
+    auto& nue = o.f[ATJUP_F_NUE];
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 0; j < o.jm; j++)
+        for(int k = 0; k < o.km; k++){
+            tke[o.id(0, j, k)] = tke[o.id(1, j, k)];
+            dis[o.id(0, j, k)] = dis[o.id(1, j, k)];
+            nue[o.id(0, j, k)] = nue[o.id(1, j, k)];
+        }
+}
+
+void turbulence(orc_ctx& o){
+    const int im = o.im, jm = o.jm, km = o.km;
+    auto& tke = o.f[ATJUP_F_TKE];
+    auto& dis = o.f[ATJUP_F_DIS];
+    auto& nue = o.f[ATJUP_F_NUE];
+    auto& prod = o.f[ATJUP_F_PROD];
+    auto& tsrc = o.f[ATJUP_F_TKE_SOURCE];
+    auto& dsrc = o.f[ATJUP_F_DIS_SOURCE];
+    auto& tken = o.f[ATJUP_NLEVEL + ATJUP_F_TKE];
+    auto& disn = o.f[ATJUP_NLEVEL + ATJUP_F_DIS];
+    const auto& U = o.f[ATJUP_F_U];
+    const auto& V = o.f[ATJUP_F_V];
+    const auto& W = o.f[ATJUP_F_W];
+    const double dis_min = 1.0e-10;
+    const double L_m = o.g.L_atm * 1.0e3;
+    const double nue_max = o.kn.nue_max_phys / (o.c.u_0 * L_m);
+    const double k_bg_nd = o.kn.turb_k_bg_phys / (o.c.u_0 * o.c.u_0);
+    // compute_vel_star with TURB_FREE: the calm-air floor everywhere (Turbulence.h:340-342)
+    std::fill(o.vel_star.begin(), o.vel_star.end(), 0.05);
+    // clamp-in pass
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 1; j < jm - 1; j++)
+        for(int k = 0; k < km - 1; k++)
+            for(int i = 1; i < im - 1; i++){
+                if(o.solid(i, j, k)) continue;
+                const size_t q = o.id(i, j, k);
+                tke[q] = std::max(0.0, tke[q]);
+                dis[q] = std::max(dis_min, dis[q]);
+            }
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 1; j < jm - 1; j++)
+        for(int k = 0; k < km - 1; k++){
+            const int kw = o.wrap(k - 1), ke = o.wrap(k + 1);
+            for(int i = 1; i < im - 1; i++){
+                if(o.solid(i, j, k)) continue;
+                const size_t q = o.id(i, j, k);
+                const size_t qip = o.id(i + 1, j, k), qim = o.id(i - 1, j, k);
+                const size_t qjp = o.id(i, j + 1, k), qjm = o.id(i, j - 1, k);
+                const size_t qkp = o.id(i, j, ke), qkm = o.id(i, j, kw);
+                const double rm = o.rad[i];
+                const double exp_rm = 1.0;
+                double sinthe = o.sin_the[j];
+                if(sinthe == 0.0) sinthe = 1.0e-5;
+                const double costhe = o.cos_the[j];
+                const double cotanthe = costhe / sinthe;
+                const double inv_rm = 1.0 / rm;
+                const double rmsinthe = rm * sinthe;
+                const double inv_2dr = 1.0 / (2.0 * o.g.dr);
+                const double inv_2dthe = 1.0 / (2.0 * o.g.dthe);
+                const double inv_2dphi = 1.0 / (2.0 * o.g.dphi);
+                const double inv_rm2dthe = inv_2dthe / rm;
+                const double inv_rmsinthe2dphi = inv_2dphi / rmsinthe;
+                const double dudr = (U[qip] - U[qim]) * inv_2dr * exp_rm;
+                const double dvdr = (V[qip] - V[qim]) * inv_2dr * exp_rm;
+                const double dwdr = (W[qip] - W[qim]) * inv_2dr * exp_rm;
+                const double dudthe = (U[qjp] - U[qjm]) * inv_rm2dthe;
+                const double dvdthe = (V[qjp] - V[qjm]) * inv_rm2dthe;
+                const double dwdthe = (W[qjp] - W[qjm]) * inv_rm2dthe;
+                const double dudphi = (U[qkp] - U[qkm]) * inv_rmsinthe2dphi;
+                const double dvdphi = (V[qkp] - V[qkm]) * inv_rmsinthe2dphi;
+                const double dwdphi = (W[qkp] - W[qkm]) * inv_rmsinthe2dphi;
+                const bool nb = o.solid(i - 1, j, k);
+                const bool njm = o.solid(i, j - 1, k), njp = o.solid(i, j + 1, k);
+                const bool nkm = o.solid(i, j, kw), nkp = o.solid(i, j, ke);
+                const double tke_im1 = nb ? tke[q] : tke[qim];
+                const double dis_im1 = nb ? dis[q] : dis[qim];
+                const double tke_jm1 = njm ? tke[q] : tke[qjm];
+                const double tke_jp1 = njp ? tke[q] : tke[qjp];
+                const double dis_jm1 = njm ? dis[q] : dis[qjm];
+                const double dis_jp1 = njp ? dis[q] : dis[qjp];
+                const double tke_km1 = nkm ? tke[q] : tke[qkm];
+                const double tke_kp1 = nkp ? tke[q] : tke[qkp];
+                const double dis_km1 = nkm ? dis[q] : dis[qkm];
+                const double dis_kp1 = nkp ? dis[q] : dis[qkp];
+                const double dtkedr = (tke[qip] - tke_im1) * inv_2dr * exp_rm;
+                const double ddisdr = (dis[qip] - dis_im1) * inv_2dr * exp_rm;
+                (void)dudthe;
+                double cnue;
+                {
+                    const double dis_here = std::max(dis[q], dis_min);
+                    const double tke_here = std::max(tke[q], 0.0);
+                    cnue = tke_here / dis_here;
+                    cnue = std::min(cnue, nue_max);
+                }
+                const double u_r = U[q] * inv_rm, v_r = V[q] * inv_rm, w_r = W[q] * inv_rm;
+                const double v_cot_r = v_r * cotanthe, w_cot_r = w_r * cotanthe;
+                const double g_rr = dudr, g_rt = dudthe - v_r, g_rp = dudphi - w_r;
+                const double g_tr = dvdr, g_tt = dvdthe + u_r, g_tp = dvdphi - w_cot_r;
+                const double g_pr = dwdr, g_pt = dwdthe, g_pp = dwdphi + u_r + v_cot_r;
+                const double der = 0.66667 * (g_rr + g_tt + g_pp);
+                prod[q] = std::max(0.0,
+                      (cnue * (2.0 * g_rr - der) - 0.66667 * tke[q]) * g_rr
+                    + (cnue * (g_rt + g_tr)) * g_rt
+                    + (cnue * (g_rp + g_pr)) * g_rp
+                    + (cnue * (2.0 * g_tt - der) - 0.66667 * tke[q]) * g_tt
+                    + (cnue * (g_tr + g_rt)) * g_tr
+                    + (cnue * (g_tp + g_pt)) * g_tp
+                    + (cnue * (2.0 * g_pp - der) - 0.66667 * tke[q]) * g_pp
+                    + (cnue * (g_pr + g_rp)) * g_pr
+                    + (cnue * (g_pt + g_tp)) * g_pt);
+                const double W12 = g_rt - g_tr, W13 = g_rp - g_pr, W23 = g_tp - g_pt;
+                const double Omega_mag = std::sqrt(W12 * W12 + W13 * W13 + W23 * W23);
+                // compute_k_omega (Turbulence.h:920-1012)
+                const double bet_star = 0.09, gam = 0.52, C_lim = 0.875, bet_0 = 0.0708;
+                const double erm = 1.0 / (rm + 1.0);
+                const double dtkedr_neu = nb ? (tke[qip] - tke[q]) * (1.0 / (2.0 * o.g.dr)) * erm : dtkedr;
+                const double ddisdr_neu = nb ? (dis[qip] - dis[q]) * (1.0 / (2.0 * o.g.dr)) * erm : ddisdr;
+                const double dtkedthe_neu = (tke_jp1 - tke_jm1) / (rm * 2.0 * o.g.dthe);
+                const double ddisdthe_neu = (dis_jp1 - dis_jm1) / (rm * 2.0 * o.g.dthe);
+                const double dtkedphi_neu = (tke_kp1 - tke_km1) / (rmsinthe * 2.0 * o.g.dphi);
+                const double ddisdphi_neu = (dis_kp1 - dis_km1) / (rmsinthe * 2.0 * o.g.dphi);
+                const double dudr_k = (U[qip] - U[qim]) / (2.0 * o.g.dr) / (rm + 1.0);
+                const double dvdthe_k = (V[qjp] - V[qjm]) / (rm * 2.0 * o.g.dthe);
+                const double dwdphi_k = (W[qkp] - W[qkm]) / (rmsinthe * 2.0 * o.g.dphi);
+                const double dudthe_u = (U[qjp] - U[qjm]) / (rm * 2.0 * o.g.dthe);
+                const double dudphi_u = (U[qkp] - U[qkm]) / (rmsinthe * 2.0 * o.g.dphi);
+                const double dvdr_u = (V[qip] - V[qim]) / (2.0 * o.g.dr) / (rm + 1.0);
+                const double dwdr_u = (W[qip] - W[qim]) / (2.0 * o.g.dr) / (rm + 1.0);
+                const double dvdphi_u = (V[qkp] - V[qkm]) / (rmsinthe * 2.0 * o.g.dphi);
+                const double dwdthe_u = (W[qjp] - W[qjm]) / (rm * 2.0 * o.g.dthe);
+                const double S11 = dudr_k, S22 = dvdthe_k, S33 = dwdphi_k;
+                const double S12 = 0.5 * (dudthe_u + dvdr_u);
+                const double S13 = 0.5 * (dudphi_u + dwdr_u);
+                const double S23 = 0.5 * (dvdphi_u + dwdthe_u);
+                const double S_mag = std::sqrt(2.0 * (S11 * S11 + S22 * S22 + S33 * S33
+                                                    + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23)));
+                const double w_hat = std::max(dis[q], C_lim * S_mag / std::sqrt(bet_star));
+                double sig_d = dtkedr_neu * ddisdr_neu + dtkedthe_neu * ddisdthe_neu + dtkedphi_neu * ddisdphi_neu;
+                sig_d = (sig_d <= 0.0) ? 0.0 : 0.125;
+                const double chi_w = std::fabs(Omega_mag * Omega_mag * S_mag
+                    / std::pow(bet_star * std::max(dis[q], 1.0e-20), 3));
+                const double f_bet = (1.0 + 85.0 * chi_w) / (1.0 + 100.0 * chi_w);
+                const double bet_wc = bet_0 * f_bet;
+                nue[q] = tke[q] / std::max(w_hat, 1.0e-20);
+                const double P_k = std::min(prod[q], 20.0 * bet_star * tke[q] * dis[q]);
+                const double Y_k = bet_star * tke[q] * dis[q];
+                const double P_w = gam * dis[q] / std::max(tke[q], 1.0e-20) * P_k;
+                const double Y_w = bet_wc * dis[q] * dis[q];
+                const double D_w = sig_d / std::max(dis[q], 1.0e-20)
+                    * (dtkedr_neu * ddisdr_neu + dtkedthe_neu * ddisdthe_neu + dtkedphi_neu * ddisdphi_neu);
+                tsrc[q] = P_k - Y_k;
+                dsrc[q] = P_w - Y_w + D_w;
+            }
+        }
+    // clamp-out pass
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 1; j < jm - 1; j++)
+        for(int k = 0; k < km - 1; k++)
+            for(int i = 1; i < im - 1; i++){
+                if(o.solid(i, j, k)) continue;
+                const size_t q = o.id(i, j, k);
+                tke[q] = std::max(k_bg_nd, tke[q]);
+                dis[q] = std::max(dis_min, dis[q]);
+                nue[q] = (nue[q] < 0.0) ? 0.0 : ((nue[q] > nue_max) ? nue_max : nue[q]);
+                const double dsm = 20.0 * dis[q] * dis[q];
+                dsrc[q] = (dsrc[q] < -dsm) ? -dsm : ((dsrc[q] > dsm) ? dsm : dsrc[q]);
+                const double tsm = 20.0 * tke[q] * dis[q];
+                tsrc[q] = (tsrc[q] < -tsm) ? -tsm : ((tsrc[q] > tsm) ? tsm : tsrc[q]);
+            }
+    turb_apply_wall_bc(o);
+    // zero_land_cells
+    std::vector<double>* eight[] = { &tke, &tken, &dis, &disn, &nue, &prod, &tsrc, &dsrc };
+    #pragma omp parallel for schedule(static)
+    for(size_t q = 0; q < o.n3; q++)
+        if(o.mask[q] == 1.0) for(auto* a : eight) (*a)[q] = 0.0;
+    // clamp_nue
+    #pragma omp parallel for schedule(static)
+    for(size_t q = 0; q < o.n3; q++)
+        nue[q] = ((nue[q] < 0.0) ? 0.0 : ((nue[q] > nue_max) ? nue_max : nue[q])) * 1.0;
+    // seam_copy
+    std::vector<double>* seam8[] = { &tke, &dis, &tken, &disn, &nue, &prod, &tsrc, &dsrc };
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int i = 0; i < im; i++)
+        for(int j = 0; j < jm; j++)
+            for(auto* a : seam8) (*a)[o.id(i, j, km - 1)] = (*a)[o.id(i, j, 0)];
+    // fill_lid_corners, TURB_FREE without the separate turbulence BC lists
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 1; j < jm - 1; j++)
+        for(int k = 0; k < km; k++)
+            for(auto* a : eight) (*a)[o.id(im - 1, j, k)] = (*a)[o.id(im - 2, j, k)];
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int i = 0; i < im; i++)
+        for(int k = 0; k < km; k++)
+            for(auto* a : eight){
+                (*a)[o.id(i, 0, k)] = (*a)[o.id(i, 1, k)];
+                (*a)[o.id(i, jm - 1, k)] = (*a)[o.id(i, jm - 2, k)];
+            }
+}
+
+// ---- FluxLimiter<P>::nh4sh, Superbee (FluxLimiter.h:67-190) ----------------------------------
+void flux_limiter(orc_ctx& o){
+    const int im = o.im, jm = o.jm, km = o.km;
+    const auto& N = o.f[ATJUP_F_NH4SH];
+    const auto& U = o.f[ATJUP_F_U];
+    const auto& V = o.f[ATJUP_F_V];
+    const auto& W = o.f[ATJUP_F_W];
+    auto& fl = o.f[ATJUP_F_FLUXLIM_NH4SH];
+    const double dr = o.g.dr, dthe = o.g.dthe, dphi = o.g.dphi, eps = 1.0e-12;
+    auto superbee = [](double r){ return std::max(0.0, std::max(std::min(2.0 * r, 1.0), std::min(r, 2.0))); };
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int k = 0; k < km - 1; k++)
+        for(int j = 1; j < jm - 1; j++)
+            for(int i = 1; i < im - 1; i++){
+                const double q = N[o.id(i, j, k)];
+                const double q_rp = N[o.id(i + 1, j, k)], q_rm = N[o.id(i - 1, j, k)];
+                const double q_tp = N[o.id(i, j + 1, k)], q_tm = N[o.id(i, j - 1, k)];
+                const double q_pp = N[o.id(i, j, o.wrap(k + 1))], q_pm = N[o.id(i, j, o.wrap(k - 1))];
+                const double u = U[o.id(i, j, k)], v = V[o.id(i, j, k)], w = W[o.id(i, j, k)];
+                const double rm = o.rad[i];
+                const double sinthe = std::max(0.4, std::abs(o.sin_the[j]));
+                const double inv_rm = 1.0 / rm;
+                const double inv_rmsinthe = 1.0 / (rm * sinthe);
+                double corr = 0.0;
+                {
+                    const double df = q_rp - q, db = q - q_rm;
+                    const double denom = df + (df >= 0.0 ? eps : -eps);
+                    const double r = (u >= 0.0) ? db / denom
+                        : ((i + 2 < im ? N[o.id(i + 2, j, k)] : q_rp) - q_rp) / denom;
+                    corr += (1.0 - superbee(r)) * std::abs(u) * (df - db) / (2.0 * dr);
+                }
+                {
+                    const double df = q_tp - q, db = q - q_tm;
+                    const double denom = df + (df >= 0.0 ? eps : -eps);
+                    const double q_far = (j + 2 < jm) ? N[o.id(i, j + 2, k)] : (o.c43 * q_tp - o.c13 * q);
+                    const double r = (v >= 0.0) ? db / denom : (q_far - q_tp) / denom;
+                    corr += (1.0 - superbee(r)) * std::abs(v) * inv_rm * (df - db) / (2.0 * dthe);
+                }
+                {
+                    const double df = q_pp - q, db = q - q_pm;
+                    const double denom = df + (df >= 0.0 ? eps : -eps);
+                    const double r = (w >= 0.0) ? db / denom : (N[o.id(i, j, o.wrap(k + 2))] - q_pp) / denom;
+                    corr += (1.0 - superbee(r)) * std::abs(w) * inv_rmsinthe * (df - db) / (2.0 * dphi);
+                }
+                fl[o.id(i, j, k)] = corr;
+            }
+}
+
 bool valid_field(int f){
     return (f >= 0 && f < ATJUP_N_PROG) || (f >= ATJUP_F_P_DYN && f < ATJUP_F_END)
         || (f >= ATJUP_NLEVEL && f < ATJUP_NLEVEL + ATJUP_N_PROG);
@@ -1218,9 +1640,32 @@ int orc_mixture_density(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; mixture_densit
 int orc_clamp_species(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; clamp_species(*o); return 0; }
 int orc_swap_nlevel(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; restore(*o); return 0; }
 int orc_scalar_sync(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; scalar_sync(*o); return 0; }
-int orc_satadj(orc_ctx* o, int gas){ (void)o; (void)gas; return ATJUP_ERR_PROFILE; }
-int orc_turbulence_run(orc_ctx* o){ (void)o; return ATJUP_ERR_PROFILE; }
-int orc_flux_limiter(orc_ctx* o){ (void)o; return ATJUP_ERR_PROFILE; }
+int orc_set_gas(orc_ctx* o, int gas, const atjup_gas_params* p){
+    if(!o || !p || gas < 0 || gas > 2) return ATJUP_ERR_ARG;
+    o->gas[gas] = *p;
+    return 0;
+}
+int orc_satadj(orc_ctx* o, int gas){
+    if(!o || gas < 0 || gas > 2) return ATJUP_ERR_ARG;
+    satadj(*o, gas);
+    return 0;
+}
+int orc_turbulence_run(orc_ctx* o){
+    if(!o) return ATJUP_ERR_ARG;
+    if(o->kn.turb_model < 0) return 0;
+    if(o->kn.turb_model != 1) return ATJUP_ERR_PROFILE;      // k-omega is the supported closure
+    turbulence(*o);
+    turb_wall_omega(*o);
+    return 0;
+}
+int orc_turb_wall(orc_ctx* o){
+    if(!o) return ATJUP_ERR_ARG;
+    if(o->kn.turb_model < 0) return 0;
+    turb_apply_wall_bc(*o);
+    turb_wall_omega(*o);
+    return 0;
+}
+int orc_flux_limiter(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; flux_limiter(*o); return 0; }
 
 // One iteration of the loop body, on-path functions only (cJupiterModel.cpp:1010-1555).
 int orc_step(orc_ctx* o, int iter, int first){
@@ -1229,10 +1674,15 @@ int orc_step(orc_ctx* o, int iter, int first){
         pressure(*o, std::max(1, o->kn.press_sweeps), o->kn.press_omega);
         if(o->kn.damp_pdyn) shapiro(*o, o->f[ATJUP_F_P_DYN], 2, 1.0, 1, 7);
         mixture_density(*o);
+        if(o->kn.satadj) for(int gas = 0; gas < 3; gas++) satadj(*o, gas);
         if(o->kn.rk_scalar_sync >= 1) scalar_sync(*o);
+        if(o->kn.turb_model >= 0){ turbulence(*o); turb_wall_omega(*o); }
+        flux_limiter(*o);
+        shapiro(*o, o->f[ATJUP_F_FLUXLIM_NH4SH], 2, 1.0, 1, 7);
     }
     rk4(*o);
     bc_radius(*o); bc_theta(*o); bc_phi(*o); bc_solid(*o);
+    if(o->kn.turb_model >= 0){ turb_apply_wall_bc(*o); turb_wall_omega(*o); }
     clamp_species(*o);
     restore(*o);
     return 0;

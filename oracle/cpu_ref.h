@@ -34,6 +34,8 @@ int      orc_clamp_species(orc_ctx* o);
 int      orc_swap_nlevel(orc_ctx* o);
 int      orc_scalar_sync(orc_ctx* o);
 int      orc_satadj(orc_ctx* o, int gas);
+int      orc_set_gas(orc_ctx* o, int gas, const atjup_gas_params* p);
+int      orc_turb_wall(orc_ctx* o);
 int      orc_turbulence_run(orc_ctx* o);
 int      orc_flux_limiter(orc_ctx* o);
 int      orc_step(orc_ctx* o, int iter, int first);

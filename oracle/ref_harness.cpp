@@ -80,6 +80,15 @@ public:
                 {"r_p_nh4sh", &M.r_p_nh4sh}, {"re_turb", &M.re_turb}, {"t_pole", &M.t_pole},
                 {"t_equator", &M.t_equator}, {"abl_height", &M.abl_height},
             };
+            // the saturation adjustment's per-gas constants as Run() leaves them (SVP_FIT, COMPOSITION)
+#define GASK(g) \
+            s["t_0_" #g] = &M.t_0_##g; s["t_00_" #g] = &M.t_00_##g; s["ep_" #g] = &M.ep_##g; \
+            s["lv_" #g] = &M.lv_##g; s["ls_" #g] = &M.ls_##g; s["C_" #g] = &M.C_##g; s["L0_" #g] = &M.L0_##g; \
+            s["R_" #g] = &M.R_##g; s["del_alf_" #g] = &M.del_alf_##g; s["del_bet_" #g] = &M.del_bet_##g; \
+            s["C_" #g "_ice"] = &M.C_##g##_ice; s["L0_" #g "_ice"] = &M.L0_##g##_ice; \
+            s["del_alf_" #g "_ice"] = &M.del_alf_##g##_ice; s["del_bet_" #g "_ice"] = &M.del_bet_##g##_ice;
+            GASK(h2o) GASK(nh3) GASK(ch4)
+#undef GASK
         }
         auto it = s.find(n);
         return it == s.end() ? nullptr : it->second;
@@ -132,6 +141,24 @@ public:
         if(M.composition()) M.setCompositionEp();
     }
 
+    // The wall-function omega that Run() applies after every apply_wall_bc(); it is a lambda local
+    // to Run() (planet/cJupiterModel.cpp:356-374), so the harness repeats its few lines here with
+    // ATJUP_TURB_WALL_OMEGA at its default 1 and a k-omega closure.
+    static void wall_omega(cJupiterModel& M){
+        const double nue_gas_phys = 1.8e-5, bet_star = 0.09, bet_1 = 0.0333, kappa = 0.41;
+        const double nd_omega = M.L_atm * 1.0e3 / M.u_0;
+        for(int j = 0; j < M.jm; j++)
+            for(int k = 0; k < M.km; k++){
+                const int i_w = M.surface_index(j, k);
+                if(i_w <= 0 || i_w >= M.im - 1) continue;
+                const double y1 = std::max(M.layer_thickness_m(std::max(i_w - 1, 0)), 1.0);
+                const double y = 0.5 * y1;
+                const double om_vis = 60.0 * nue_gas_phys / (bet_1 * y1 * y1);
+                const double om_log = M.vel_star.y[j][k] / (std::sqrt(bet_star) * kappa * y);
+                M.dis.x[i_w][j][k] = std::sqrt(om_vis * om_vis + om_log * om_log) * nd_omega;
+            }
+    }
+
     static int call(const std::string& op){
         cJupiterModel& M = *m;
         if(op == "rk4")                 M.RungeKuttaJup();
@@ -148,8 +175,9 @@ public:
         else if(op == "clamp_species")  M.clampNegativeSpecies();
         else if(op == "restore")        M.restoreVar(1.0);
         else if(op == "flux_limiter")   ChemistryJup(M).FluxLimiterNH4SH();
-        else if(op == "turbulence")     TurbulenceJup(M).run();
-        else if(op == "turb_wall")      TurbulenceJup(M).apply_wall_bc();
+        else if(op == "fluxlim_damp")   JupiterUtils::damp_wiggles(M.fluxlim_nh4sh, &M.i_topography, true, true, true);
+        else if(op == "turbulence")     { TurbulenceJup(M).run(); wall_omega(M); }
+        else if(op == "turb_wall")      { TurbulenceJup(M).apply_wall_bc(); wall_omega(M); }
         else if(op == "satadj_h2o")
             SaturationAdjustmentJup(M).run("H2O", M.t_0_h2o, M.t_00_h2o, M.ep_h2o, M.lv_h2o, M.ls_h2o,
                 M.C_h2o, M.L0_h2o, M.R_h2o, M.del_alf_h2o, M.del_bet_h2o,
@@ -298,6 +326,8 @@ int ref_step(int iter, int first, int satadj, int turb){
         if(satadj){ rc |= H::call("satadj_h2o"); rc |= H::call("satadj_nh3"); rc |= H::call("satadj_ch4"); }
         rc |= H::call("scalar_sync");
         if(turb) rc |= H::call("turbulence");
+        rc |= H::call("flux_limiter");
+        rc |= H::call("fluxlim_damp");
     }
     rc |= H::call("rk4");
     rc |= H::call("bc_all");

@@ -13,8 +13,5 @@ def pytest_configure(config):
 
 
 def gpu_available() -> bool:
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    from atjup_amd import lib
+    return lib.device_count() > 0

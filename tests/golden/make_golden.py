@@ -2,8 +2,9 @@
 
 Run in a container that holds the reference sources (the recipe compiles them into oracle/_ref,
 `make -C oracle ref`): it drives the compiled reference model through its own entry points
-(oracle/ref_harness.cpp) on the seeded five-step dynamics case and stores a small, fixed sample of
-its outputs. The fixtures are data only: inputs are regenerated from the seed by
+(oracle/ref_harness.cpp) on two seeded cases -- the dry dynamics case (dyn_reference.npz) and the
+moist full-physics case with the k-omega closure, saturation adjustment and flux limiter
+(full_reference.npz) -- and stores a small, fixed sample of its outputs. The fixtures are data only: inputs are regenerated from the seed by
 atjup_amd.inputs, expected outputs are the values the reference produced.
 
     python tests/golden/make_golden.py
@@ -20,11 +21,14 @@ sys.path.insert(0, ROOT)
 
 from atjup_amd import abi, inputs  # noqa: E402
 
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dyn_reference.npz")
-SEED = 7
-SWEEPS = 20
-STEPS = 3
-FIELDS = ("t", "u", "v", "w", "p_dyn", "p_hydro", "rho_mix", "aux_u", "aux_v", "aux_w")
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = {
+    "dyn": dict(out="dyn_reference.npz", seed=7, sweeps=20, steps=3, n=4096,
+                fields=("t", "u", "v", "w", "p_dyn", "p_hydro", "rho_mix", "aux_u", "aux_v", "aux_w")),
+    "full": dict(out="full_reference.npz", seed=13, sweeps=10, steps=2, n=1024,
+                 fields=tuple(abi.PROG) + ("p_dyn", "p_hydro", "rho_mix", "aux_u", "aux_v", "aux_w", "nue",
+                                           "prod", "tke_source", "dis_source", "fluxlim_nh4sh")),
+}
 
 
 def sample_index(shape, n=4096, seed=99):
@@ -45,26 +49,35 @@ def sample_index(shape, n=4096, seed=99):
     return tuple(np.concatenate([a, b]) for a, b in zip(idx, e))
 
 
-def main():
+def main(config):
     import oracle
-    G, Cn, K, env = abi.make_profile("dyn", sweeps=SWEEPS)
-    st = inputs.make_state("dyn", seed=SEED)
+    c = CASES[config]
+    G, Cn, K, env = abi.make_profile(config, sweeps=c["sweeps"])
+    st = inputs.make_state(config, seed=c["seed"])
     ref = oracle.Reference(env)
     ref.load_state(st, Cn)
     ref.call("bc_all")
     ref.call("restore")
-    sel = sample_index(ref.shape)
-    out = {"i": sel[0], "j": sel[1], "k": sel[2], "seed": SEED, "sweeps": SWEEPS, "steps": STEPS}
-    for it in range(1, STEPS + 1):
-        ref.step(it, it == 1, satadj=True, turb=False)
-        for n in FIELDS:
+    sel = sample_index(ref.shape, n=c["n"])
+    out = {"i": sel[0], "j": sel[1], "k": sel[2], "seed": c["seed"], "sweeps": c["sweeps"],
+           "steps": c["steps"]}
+    for it in range(1, c["steps"] + 1):
+        ref.step(it, it == 1, satadj=True, turb=config != "dyn")
+        for n in c["fields"]:
             a = ref.get(n)
             out[f"s{it}_{n}"] = a[sel]
             out[f"s{it}_{n}_levelsum"] = a.sum(axis=(1, 2))
         out[f"s{it}_brl"] = ref.buoy_ref_level()
-    np.savez_compressed(OUT, **out)
-    print(OUT, os.path.getsize(OUT))
+    path = os.path.join(HERE, c["out"])
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path))
 
 
 if __name__ == "__main__":
-    main()
+    # one profile per process: the reference caches its knobs in function statics
+    if len(sys.argv) > 1:
+        main(sys.argv[1])
+    else:
+        import subprocess
+        for cfg in CASES:
+            subprocess.run([sys.executable, os.path.abspath(__file__), cfg], check=True)

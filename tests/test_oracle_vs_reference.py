@@ -49,3 +49,44 @@ print("ALL-EQUAL")
 def test_oracle_bit_exact_against_compiled_reference():
     r = subprocess.run([sys.executable, "-c", SCRIPT % {"root": ROOT}], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+# The full-physics profile: k-omega closure with its wall function, saturation adjustment of the
+# three gases, the NH4SH flux limiter and the 17-field RK4, on a moist seeded state with the
+# ellipsoid obstacle.
+SCRIPT_FULL = r'''
+import sys, numpy as np
+sys.path.insert(0, %(root)r)
+from atjup_amd import abi, inputs
+import oracle
+G, Cn, K, env = abi.make_profile("full", sweeps=8)
+st = inputs.make_state("full", seed=11)
+ref = oracle.Reference(env); ref.load_state(st, Cn)
+orc = oracle.Oracle(G, Cn, K); orc.load_state(st)
+names = list(abi.PROG) + ["p_dyn", "p_hydro", "rho_mix", "aux_u", "aux_v", "aux_w", "nue", "prod",
+                          "tke_source", "dis_source", "fluxlim_nh4sh"]
+def same(tag):
+    for n in names:
+        a, b = ref.get(n), orc.get(n)
+        if not np.array_equal(a, b):
+            print("MISMATCH", tag, n, np.max(np.abs(a - b))); sys.exit(1)
+ref.call("bc_all"); orc.call("bc_after_step"); ref.call("restore"); orc.call("swap_nlevel"); same("bc")
+ref.call("mixture_density"); orc.call("mixture_density")
+before = orc.get("h2o_cloud")
+for gas, nm in enumerate(("h2o", "nh3", "ch4")):
+    ref.call("satadj_" + nm); orc.call("satadj", gas); same("satadj_" + nm)
+if np.count_nonzero(orc.get("h2o_cloud") != before) < 1000: print("satadj left the state alone"); sys.exit(1)
+ref.call("flux_limiter"); orc.call("flux_limiter"); same("fluxlim")
+ref.call("turbulence"); orc.call("turbulence_run"); same("turb")
+if np.count_nonzero(orc.get("prod")) < 1000: print("no production"); sys.exit(1)
+for it in range(1, 3):
+    ref.step(it, it == 1, satadj=True, turb=True); orc.call("step", it, 1 if it == 1 else 0); same("step%%d" %% it)
+print("ALL-EQUAL")
+'''
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_oracle_full_physics_bit_exact_against_compiled_reference():
+    r = subprocess.run([sys.executable, "-c", SCRIPT_FULL % {"root": ROOT}], capture_output=True, text=True,
+                       timeout=900)
+    assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
