@@ -111,6 +111,9 @@ int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, i
     if(k->turb_model < -1 || k->turb_model > 2){ set_error("turb_model"); return ATJUP_ERR_PROFILE; }
     if(k->rk_scalar_sync != 2){ set_error("only ATJUP_RK_SCALAR_SYNC=2 is implemented"); return ATJUP_ERR_PROFILE; }
     if(k->dry && k->turb_model != -1){ set_error("the dry profile has the closure off"); return ATJUP_ERR_PROFILE; }
+    if(k->turb_model != -1 && k->turb_model != 1){
+        set_error("the closure is built for k-omega (turb_model 1) or off (-1)"); return ATJUP_ERR_PROFILE;
+    }
     if(!(k->pdyn_cap > 0.0)){ set_error("pdyn_cap"); return ATJUP_ERR_PROFILE; }
     if(!(k->press_omega > 0.0 && k->press_omega < 2.0)){ set_error("press_omega outside (0, 2)"); return ATJUP_ERR_PROFILE; }
     return 0;
@@ -246,13 +249,17 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
             if(!rc && hipMemcpy(c->pois32.p, pois32.data(), c->pois32.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = ATJUP_ERR_HIP;
         }
         if(!rc){
-            tabs.resize((size_t)c->im + 2 * c->jm);
+            tabs.resize((size_t)2 * c->im + 3 * c->jm);
             for(int i = 0; i < c->im; i++) tabs[i] = c->rad[i];
             for(int j = 0; j < c->jm; j++){
                 tabs[c->im + j] = c->sin_the[j];
                 // the Coriolis factor of the split column integral, nd_cor = 1 (RungeKutta_Jup_Turb.cpp:185-186)
                 tabs[c->im + c->jm + j] = c->c.Coriolis * 1.0 * 2.0 * c->c.omega * std::abs(c->sin_the[j]);
+                tabs[c->im + 2 * c->jm + j] = c->cos_the[j];
             }
+            // layer_thickness_m (cJupiterModel.h:147-150) from the float layer heights
+            for(int i = 0; i < c->im; i++)
+                tabs[c->im + 3 * c->jm + i] = (i > c->im - 2) ? -1.0 : (double)(c->lh[i + 1] - c->lh[i]) * 1.0e3;
             rc = alloc(c->tabs, tabs.size() * sizeof(double));
             if(!rc && hipMemcpy(c->tabs.p, tabs.data(), c->tabs.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = ATJUP_ERR_HIP;
         }
@@ -310,7 +317,9 @@ static int download_impl(atjup_ctx* c, int f, double* dst, double** const* rows)
     if(!(prog(s) || input(s))){ set_error("field id"); return ATJUP_ERR_ARG; }
     if(!c->cur[s].p){ set_error("field not held by this profile"); return ATJUP_ERR_PROFILE; }
     std::vector<char> host(c->n3 * c->esz);
-    ATJUP_HIP_CHECK(hipMemcpyAsync(host.data(), c->cur[s].p, host.size(), hipMemcpyDeviceToHost, c->stream));
+    // the live k*, omega* of a turbulence run that waits for its step; their n-level ids read the n level
+    const void* src_p = nlevel(f) ? c->cur[s].p : live_ptr(c, s);
+    ATJUP_HIP_CHECK(hipMemcpyAsync(host.data(), src_p, host.size(), hipMemcpyDeviceToHost, c->stream));
     ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
     const size_t row = (size_t)c->km;
     for(int i = 0; i < c->im; i++)
@@ -385,18 +394,17 @@ int atjup_hip_satadj(atjup_ctx* c, int gas, const atjup_gas_params* p){
     if(gas < 0 || gas > 2){ set_error("gas"); return ATJUP_ERR_ARG; }
     if(c->kn.dry) return 0;        // all species zero: the adjustment is the identity (SaturationAdjustment.h:440)
     if(!p && !c->gas_set[gas]){ set_error("gas constants not set (atjup_hip_set_gas)"); return ATJUP_ERR_STATE; }
-    (void)p;
-    set_error("the saturation adjustment is not built in this library"); return ATJUP_ERR_PROFILE;
+    return launch_satadj(c, gas, p ? *p : c->gas[gas]);
 }
 int atjup_hip_turbulence_run(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;
     if(c->kn.turb_model < 0) return 0;
-    set_error("the turbulence closure is not built in this library"); return ATJUP_ERR_PROFILE;
+    return launch_turbulence(c);
 }
 int atjup_hip_turb_wall(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;
     if(c->kn.turb_model < 0) return 0;
-    set_error("the turbulence closure is not built in this library"); return ATJUP_ERR_PROFILE;
+    return launch_turb_wall(c);
 }
 int atjup_hip_bc_after_step(atjup_ctx* c){ if(int rc = need_mask(c)) return rc; return launch_bc_after_step(c); }
 int atjup_hip_shapiro(atjup_ctx* c, int f, int order, double strength, int passes, int axes){
@@ -411,10 +419,12 @@ int atjup_hip_mixture_density(atjup_ctx* c){ if(!c) return ATJUP_ERR_ARG; return
 int atjup_hip_flux_limiter(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;
     if(c->kn.dry) return 0;        // nh4sh = 0: the correction is zero
-    set_error("the flux limiter is not built in this library"); return ATJUP_ERR_PROFILE;
+    return launch_flux_limiter(c);
 }
 int atjup_hip_clamp_species(atjup_ctx* c){ if(!c) return ATJUP_ERR_ARG; return launch_clamp_species(c); }
-int atjup_hip_swap_nlevel(atjup_ctx* c){ return c ? 0 : ATJUP_ERR_ARG; }
+// restoreVar: the n level is the live storage, except the closure's live k*, omega* after a
+// turbulence run, which become the n level here
+int atjup_hip_swap_nlevel(atjup_ctx* c){ if(!c) return ATJUP_ERR_ARG; return launch_commit_live(c); }
 int atjup_hip_scalar_sync(atjup_ctx* c){ return c ? 0 : ATJUP_ERR_ARG; }
 
 int atjup_hip_step(atjup_ctx* c, int iter, int first){
@@ -427,14 +437,22 @@ int atjup_hip_step(atjup_ctx* c, int iter, int first){
         timer_end(c, ATJUP_T_PRESSURE, t0, 0);
         if(c->kn.damp_pdyn && (rc = launch_shapiro(c, ATJUP_F_P_DYN, 2, 1.0, 1, 7))) return rc;
         if((rc = launch_mixture_density(c))) return rc;
-        for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
-            if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
-        if((rc = atjup_hip_turbulence_run(c))) return rc;
-        if((rc = atjup_hip_flux_limiter(c))) return rc;
+        if(!c->kn.dry){
+            // the three adjustments, then RK_SCALAR_SYNC 2 (the identity on the shared storage),
+            // the closure with its wall function, the NH4SH limiter and its smoothing
+            // (cJupiterModel.cpp:1060-1161)
+            for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
+                if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
+            if((rc = atjup_hip_turbulence_run(c))) return rc;
+            if((rc = launch_flux_limiter(c))) return rc;
+            if((rc = launch_shapiro(c, ATJUP_F_FLUXLIM_NH4SH, 2, 1.0, 1, 7))) return rc;
+        }
     }
     if((rc = launch_rk4(c))) return rc;
     if((rc = launch_bc_after_step(c))) return rc;
+    if(!c->kn.dry && c->kn.turb_model >= 0 && (rc = launch_turb_wall(c))) return rc;   // cJupiterModel.cpp:1294
     if((rc = launch_clamp_species(c))) return rc;
+    if((rc = launch_commit_live(c))) return rc;
     timer_end(c, ATJUP_T_STEP, t_step, 0);
     return 0;
 }

@@ -86,7 +86,8 @@ struct atjup_ctx {
     atjup::Buf geo;              // GeoIJ<T> [im * jm]
     atjup::Buf pois;             // PoisIJ<double> [im * jm]
     atjup::Buf pois32;           // PoisIJ<float> [im * jm]
-    atjup::Buf tabs;             // double rad[im], sin(the)[jm], Coriolis split factor [jm]
+    atjup::Buf tabs;             // double rad[im], sin(the)[jm], Coriolis split factor [jm], cos(the)[jm],
+                                 // layer thickness [m] [im]
     atjup::Buf brl;              // double [im]
     atjup::Buf brl_part;         // double partial sums
     atjup::Buf zm;               // double [4 * jm], the top-sponge zonal means
@@ -127,6 +128,21 @@ int launch_clamp_species(atjup_ctx* c);
 int launch_mask_setup(atjup_ctx* c);
 int launch_check_finite(atjup_ctx* c);
 int launch_copy_field(atjup_ctx* c, void* dst, const void* src);
+// full-physics profile (atjup_physics.hip)
+int launch_rk4_full(atjup_ctx* c);
+int launch_turbulence(atjup_ctx* c);
+int launch_turb_wall(atjup_ctx* c);
+int launch_turb_sanitize(atjup_ctx* c);
+int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g);
+int launch_flux_limiter(atjup_ctx* c);
+int launch_commit_live(atjup_ctx* c);
+
+// The storage a live field is read from: the closure's live k*, omega* while a turbulence run
+// waits for its RK4 step, the field's own storage otherwise.
+inline void* live_ptr(atjup_ctx* c, int f){
+    if(c->turb_live_pending && (f == ATJUP_F_TKE || f == ATJUP_F_DIS)) return c->live_turb[f - ATJUP_F_TKE].p;
+    return c->cur[f].p;
+}
 
 // phase timers: begin returns an event recorded now (or nullptr with timing off); end records the
 // closing event and files the span under `phase` with its kernel-launch count.

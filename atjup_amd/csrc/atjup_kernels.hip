@@ -9,38 +9,12 @@
 // All FP64 kernels keep the reference's expression trees (the file is built with
 // -ffp-contract=off), so that a cell update reproduces the CPU oracle bit for bit whenever its
 // inputs agree.
-#include "atjup_internal.h"
+#include "atjup_device.h"
 
 #include <algorithm>
 #include <cmath>
 
 namespace atjup {
-
-// ---------------------------------------------------------------------------------------------
-// helpers
-// ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ int xcd_remap(int b, int nb){
-    // The dispatcher hands block b to XCD b % 8. Give XCD x the contiguous range of logical
-    // blocks [start_x, start_x + count_x) instead, so neighbouring tiles share an L2.
-    const int q = nb >> 3, r = nb & 7;
-    const int x = b & 7, n = b >> 3;
-    const int start = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
-    return start + n;
-}
-
-template<typename T>
-__device__ __forceinline__ bool finite_bits(T v){ return isfinite(v); }
-
-template<typename T>
-__device__ __forceinline__ T safe_clamp_d(T v, T lo, T hi){
-    if(!isfinite(v)) return lo;                   // exponent-bit test of RungeKutta_Jup_Turb.cpp:285-290
-    return (v < lo) ? lo : ((v > hi) ? hi : v);
-}
-
-template<typename T>
-__device__ __forceinline__ T dmax(T a, T b){ return (a < b) ? b : a; }   // std::max
-template<typename T>
-__device__ __forceinline__ T dmin(T a, T b){ return (b < a) ? b : a; }   // std::min
 
 // ---------------------------------------------------------------------------------------------
 // computeBuoyancyRefLevel (RungeKutta_Jup_Turb.cpp:47-69): per level i the sin(theta)-weighted
@@ -861,15 +835,7 @@ __global__ __launch_bounds__(256) void k_nonfinite(const T* __restrict__ x, int*
 // =============================================================================================
 namespace {
 
-inline int nblk(size_t n, int b){ return (int)((n + b - 1) / b); }
-
 template<typename T> T* F(atjup_ctx* c, int f){ return (T*)c->cur[f].p; }
-
-inline int check_launch(){
-    hipError_t e = hipGetLastError();
-    if(e != hipSuccess){ set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return ATJUP_ERR_HIP; }
-    return 0;
-}
 
 template<typename T>
 int do_brl(atjup_ctx* c){
@@ -1011,7 +977,7 @@ template<typename T>
 BCLists<T> bc_lists(atjup_ctx* c){
     BCLists<T> L{};
     const bool dry = c->kn.dry != 0;
-    auto add = [&](T** arr, int& n, int f){ arr[n++] = F<T>(c, f); };
+    auto add = [&](T** arr, int& n, int f){ arr[n++] = (T*)live_ptr(c, f); };
     const int species[] = { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE, ATJUP_F_H2S, ATJUP_F_NH3,
         ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_NH4SH };
     const int turbf[] = { ATJUP_F_TKE, ATJUP_F_DIS, ATJUP_F_NUE, ATJUP_F_PROD, ATJUP_F_TKE_SOURCE, ATJUP_F_DIS_SOURCE };
@@ -1053,7 +1019,9 @@ int do_bc(atjup_ctx* c){
     if(c->n_amb_rows > 0)
         hipLaunchKernelGGL(k_bc_t_ambient<T>, dim3(nblk(c->n_amb_rows, 64)), dim3(64), 0, c->stream,
             L.t, (const uint8_t*)c->mask.p, (const int2*)c->amb_rows.p, c->n_amb_rows, c->jm, c->km, c->pitch);
-    return check_launch();
+    if(int rc = check_launch()) return rc;
+    if(!c->kn.dry) return launch_turb_sanitize(c);
+    return 0;
 }
 
 template<typename T>
@@ -1134,9 +1102,9 @@ int do_check_finite(atjup_ctx* c){
 int launch_buoy_ref_level(atjup_ctx* c){ return DISPATCH(do_brl, c); }
 int launch_hydrostatic(atjup_ctx* c){ return DISPATCH(do_hydro, c); }
 int launch_rk4(atjup_ctx* c){
-    if(!c->kn.dry){ set_error("the full-physics RK4 stage is not built in this library"); return ATJUP_ERR_PROFILE; }
     if(int rc = launch_buoy_ref_level(c)) return rc;
     if(int rc = launch_hydrostatic(c)) return rc;
+    if(!c->kn.dry) return launch_rk4_full(c);
     return DISPATCH(do_rk4_dry, c);
 }
 int launch_pressure(atjup_ctx* c, int sweeps, double omega){ return DISPATCH(do_pressure, c, sweeps, omega); }

@@ -1,0 +1,1064 @@
+// HIP kernels of the moist, turbulent time step (the full-physics profile) for gfx950:
+//   * the RK4 stage with the RHS of all 17 prognostic fields (RHSJup, RHS_Jup_Turb.cpp:31-1457,
+//     with the k-omega sources) fused with the stage accumulation and the stage input;
+//   * Turbulence<P>::run with the k-omega closure and the wall-function omega Run() applies;
+//   * SaturationAdjustment<P>::run, one thread per cell and gas;
+//   * FluxLimiter<P>::nh4sh;
+//   * the closure's part of bcSolidGround (the sanitising clamp of the turbulence arrays).
+//
+// Storage conventions are those of atjup_kernels.hip (pitch-padded [i][j][k], k fastest,
+// XCD-aware block order). The live tke / dis the closure produces are held in the context's
+// live_turb buffers until the RK4 step consumes them: the step integrates k* and omega* from the
+// time-level-n copies while its right-hand side reads the closure's values, as the reference's
+// RungeKuttaJup does with tken / disn (RungeKutta_Jup_Turb.cpp:420-490).
+//
+// FP64 kernels keep the reference's expression trees (-ffp-contract=off); the transcendental
+// functions (exp, log, pow) are the device library's, within an ulp of the host's.
+#include "atjup_device.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace atjup {
+
+constexpr int NP = ATJUP_N_PROG;
+
+// =============================================================================================
+// RK4 stage, full physics
+// =============================================================================================
+template<typename T>
+struct StageFullArgs {
+    const T* in[NP];            // stage input (the RHS reads it)
+    T* out[NP];                 // stage output
+    const T* nl[NP];            // time-level-n copy (the update starts from it)
+    T* acc[NP];                 // k1 + 2 k2 + 2 k3 + k4 accumulator
+    const T* bsrc[NP];          // source of the non-updated cells of out (nullptr: none)
+    const T* spr[NP];           // S_precip_* of a species (by field id), nullptr if none
+    const T* mf[NP];            // massflux_* of a species (by field id), nullptr if none
+    T sp_coef[NP];              // species_diff_scale / (sc * re) of a species
+    const T *pd, *ph, *ps, *rho, *wn, *nue, *qrad, *qprec, *fl_nh4sh;
+    T *aux_u, *aux_v, *aux_w, *prod, *tsrc, *dsrc;
+    const uint8_t* flags;
+    const GeoIJ<T>* geo;
+    const double* brl;
+    int im, jm, km, pitch, nkb, njb, nb;
+    size_t plane;
+    int stage, turb;
+    T c_in, wgt, dt, one_sixth;
+    T inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
+    T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, cbuoy, g, rR, t_ref;
+    T t_coef, inv_re, rad_coupling, precip_coupling, turb_coupling, L_rad, R_mix, cp_mix, u_0;
+    T cr, pc, stokes_a, rg_nh4sh, mue_mix, nue_max, tke_max_nd, dis_min;
+};
+
+constexpr int SF_BK = 64, SF_BJ = 4;
+
+template<typename T>
+struct Stencil7 { T c, ip, im, jp, jm, kp, km; };
+
+template<typename T>
+__device__ __forceinline__ Stencil7<T> load7(const T* __restrict__ X, size_t q, size_t P, size_t R,
+                                             size_t qke, size_t qkw){
+    Stencil7<T> s;
+    s.c = X[q]; s.ip = X[q + P]; s.im = X[q - P]; s.jp = X[q + R]; s.jm = X[q - R];
+    s.kp = X[qke]; s.km = X[qkw];
+    return s;
+}
+
+template<typename T>
+__global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_full(const StageFullArgs<T> a){
+    const int b = xcd_remap(blockIdx.x, a.nb);
+    const int kb = b % a.nkb;
+    const int rest = b / a.nkb;
+    const int jb = rest % a.njb;
+    const int i = rest / a.njb;
+    const int k = kb * SF_BK + threadIdx.x;
+    const int j = jb * SF_BJ + threadIdx.y;
+    if(k >= a.km || j >= a.jm) return;
+    const size_t q = ((size_t)i * a.jm + j) * a.pitch + k;
+    const uint8_t fl = a.flags[q];
+    const bool interior = i >= 1 && i <= a.im - 2 && j >= 3 && j <= a.jm - 4 && k <= a.km - 2 && !(fl & FL_SELF);
+    if(!interior){
+        #pragma unroll
+        for(int f = 0; f < NP; f++)
+            if(a.bsrc[f]) a.out[f][q] = a.bsrc[f][q];
+        return;
+    }
+    const size_t P = a.plane, R = (size_t)a.pitch;
+    const int kw = (k == 0) ? a.km - 2 : k - 1;
+    const int ke = (k == a.km - 2) ? 0 : k + 1;
+    const size_t qkw = q - k + kw, qke = q - k + ke;
+    const GeoIJ<T> G = a.geo[(size_t)i * a.jm + j];
+    const int stage = a.stage;
+
+    // the stage update of one field (RungeKutta_Jup_Turb.cpp:420-568)
+    auto update = [&](int f, T rhs, int kind){
+        // kind 0: plain, 1: species (max(0, .) at the end), 2: nh4sh (< 0 -> 0 at the end),
+        // 3: k* (safe_clamp to [0, tke_max]), 4: omega* (max(dis_min, .))
+        T accf;
+        if(stage == 0) accf = a.wgt * rhs;
+        else accf = a.acc[f][q] + a.wgt * rhs;
+        if(stage < 3){
+            a.acc[f][q] = accf;
+            T v = a.nl[f][q] + a.c_in * rhs;
+            if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
+            else if(kind == 4) v = dmax(a.dis_min, v);
+            a.out[f][q] = v;
+        } else {
+            T v = a.nl[f][q] + a.dt * accf * a.one_sixth;
+            if(kind == 1) v = dmax(T(0), v);
+            else if(kind == 2){ if(v < T(0)) v = T(0); }
+            else if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
+            else if(kind == 4) v = dmax(a.dis_min, v);
+            a.out[f][q] = v;
+        }
+    };
+
+    // ---- derivatives of t, u, v, w (RHS_Jup_Turb.cpp:180-254) ----
+    T c_[4], d1r[4], d2r[4], d1t[4], d2t[4], d1p[4], d2p[4];
+    #pragma unroll
+    for(int f = 0; f < 4; f++){
+        const Stencil7<T> s = load7(a.in[f], q, P, R, qke, qkw);
+        c_[f] = s.c;
+        d1r[f] = (s.ip - s.im) * a.inv_2dr;
+        d2r[f] = (s.ip - T(2) * s.c + s.im) * a.inv_dr2;
+        d1t[f] = (s.jp - s.jm) * a.inv_2dthe;
+        d2t[f] = (s.jp - T(2) * s.c + s.jm) * a.inv_dthe2;
+        d1p[f] = (s.kp - s.km) * a.inv_2dphi;
+        d2p[f] = (s.kp - T(2) * s.c + s.km) * a.inv_dphi2;
+    }
+    // wall-mirrored pressure gradients (RHS_Jup_Turb.cpp:119-130)
+    const T pdc = a.pd[q], phc = a.ph[q];
+    const T dpdr   = (((fl & FL_IP) ? pdc : a.pd[q + P]) - ((fl & FL_IM) ? pdc : a.pd[q - P])) * a.inv_2dr;
+    const T dpdthe = (((fl & FL_JP) ? pdc : a.pd[q + R]) - ((fl & FL_JM) ? pdc : a.pd[q - R])) * a.inv_2dthe;
+    const T dpdphi = (((fl & FL_KP) ? pdc : a.pd[qke]) - ((fl & FL_KM) ? pdc : a.pd[qkw])) * a.inv_2dphi;
+    const T dphdthe = (((fl & FL_JP) ? phc : a.ph[q + R]) - ((fl & FL_JM) ? phc : a.ph[q - R])) * a.inv_2dthe;
+    const T dphdphi = (((fl & FL_KP) ? phc : a.ph[qke]) - ((fl & FL_KM) ? phc : a.ph[qkw])) * a.inv_2dphi;
+
+    const T tq = c_[0], u = c_[1], v = c_[2], w = c_[3];
+    const T costhe = G.costhe, st = G.sinthe_true;
+    const T Coriolis_the = a.c_m2o * costhe * w;
+    const T Coriolis_phi = a.c_p2o * (+costhe * v + st * u);
+    const T v_invrm = v * G.inv_rm;
+    const T w_invrs = w * G.inv_rmsinthe;
+    const T two_inv_rm = T(2) * G.inv_rm;
+
+    const T ps_c = a.ps[q];
+    const T rho_f = a.rho[q];
+    T dpstatdr = (a.ps[q + P] - a.ps[q - P]) * a.inv_2dr * a.to_nd;
+    if(rho_f > T(0) && isfinite(rho_f)) dpstatdr *= a.r_mix / rho_f;
+    const T pressure_t = a.cep * (u * (dpdr + dpstatdr) + v_invrm * dpdthe + w_invrs * dpdphi);
+    const T transport_t = u * d1r[0] + v_invrm * d1t[0] + w_invrs * d1p[0];
+    const T transport_u = u * d1r[1] + v_invrm * d1t[1] + w_invrs * d1p[1]
+        - T(0) * (v * v + w * w) * G.inv_rm;
+    const T transport_v = u * d1r[2] + v_invrm * d1t[2] + w_invrs * d1p[2]
+        + (u * v - w * w * G.cotanthe) * G.inv_rm;
+    const T transport_w = u * d1r[3] + v_invrm * d1t[3] + w_invrs * d1p[3]
+        + (w * u + v * w * G.cotanthe) * G.inv_rm;
+    const T diffusion_t = d2r[0] + d1r[0] * two_inv_rm + d2t[0] * G.inv_rm2
+        + d1t[0] * G.costhe_inv_rm2sinthe + d2p[0] * G.inv_rm2sinthe2;
+    const T diffusion_u = d2r[1] + T(2) * u * G.inv_rm2 + d2t[1] * G.inv_rm2
+        + T(4) * d1r[1] * G.inv_rm + d1t[1] * G.costhe_inv_rm2sinthe
+        + d2p[1] * G.inv_rm2sinthe2;
+    const T diffusion_v = d2r[2] + d1r[2] * two_inv_rm + d2t[2] * G.inv_rm2
+        + d1t[2] * G.costhe_inv_rm2sinthe - G.v_metric * v
+        + d2p[2] * G.inv_rm2sinthe2
+        + T(2) * d1t[1] * G.inv_rm2
+        - d1p[3] * T(2) * costhe * G.inv_rm2sinthe2;
+    const T diffusion_w = d2r[3] + d1r[3] * two_inv_rm + d2t[3] * G.inv_rm2
+        + d1t[3] * G.costhe_inv_rm2sinthe - G.v_metric * w
+        + d2p[3] * G.inv_rm2sinthe2
+        + T(2) * d1p[1] * G.inv_rm2sinthe
+        + d1p[2] * T(2) * costhe * G.inv_rm2sinthe2;
+    const T dpdthe_term = dpdthe * G.inv_rm;
+    const T dpdphi_term = dpdphi * G.inv_rmsinthe;
+    const T dphdthe_term = dphdthe * G.inv_rm;
+    const T dphdphi_term = dphdphi * G.inv_rmsinthe;
+
+    // radiative heating and precipitation latent heat, capped at +-0.5 (RHS_Jup_Turb.cpp:646-718)
+    T radiation_t = T(0);
+    if(a.rad_coupling != T(0)){
+        const T T_phys = tq * a.t_ref;
+        const T P_phys = ps_c * T(1.0e5);
+        const T rho = (rho_f > T(0) && isfinite(rho_f)) ? rho_f : ((T_phys > T(1)) ? P_phys / (a.R_mix * T_phys) : T(0));
+        if(rho > T(0) && a.cp_mix > T(0)){
+            radiation_t = a.rad_coupling * a.qrad[q] * a.L_rad / (rho * a.cp_mix * a.u_0 * a.t_ref);
+            if(!isfinite(radiation_t)) radiation_t = T(0);
+            else if(radiation_t > T(0.5)) radiation_t = T(0.5);
+            else if(radiation_t < T(-0.5)) radiation_t = T(-0.5);
+        }
+    }
+    T precip_t = T(0);
+    if(a.precip_coupling != T(0)){
+        const T rho_p = (rho_f > T(0) && isfinite(rho_f)) ? rho_f : a.r_mix;
+        if(rho_p > T(0) && a.cp_mix > T(0)){
+            precip_t = a.precip_coupling * a.qprec[q] * a.L_rad / (rho_p * a.cp_mix * a.u_0 * a.t_ref);
+            if(!isfinite(precip_t)) precip_t = T(0);
+            else if(precip_t > T(0.5)) precip_t = T(0.5);
+            else if(precip_t < T(-0.5)) precip_t = T(-0.5);
+        }
+    }
+    const T nue_c = a.nue[q];
+    const T nue_t = (a.turb_coupling != T(0) && isfinite(nue_c)) ? a.turb_coupling * dmax(T(0), nue_c) : T(0);
+    const T nue_t_s = nue_t / T(0.9);
+    const T visc = a.inv_re + nue_t + a.wn[q];
+
+    const T rhs_t = + pressure_t - transport_t + diffusion_t * (a.t_coef + nue_t_s) + radiation_t + precip_t;
+    const T buoyancy_u = (tq > T(0)) ? a.cbuoy * (a.g * ps_c / (a.rR * tq * a.t_ref) - (T)a.brl[i]) : T(0);
+    const T rhs_u = - dpdr + buoyancy_u - transport_u + diffusion_u * visc + a.centrifugal * G.cent_rad;
+    const T rhs_v = - dpdthe_term - dphdthe_term - transport_v + diffusion_v * visc
+        - a.Coriolis * Coriolis_the + a.centrifugal * G.cent_the;
+    const T rhs_w = - dpdphi_term - dphdphi_term - transport_w + diffusion_w * visc
+        - a.Coriolis * Coriolis_phi;
+    update(ATJUP_F_T, rhs_t, 0);
+    update(ATJUP_F_U, rhs_u, 0);
+    update(ATJUP_F_V, rhs_v, 0);
+    update(ATJUP_F_W, rhs_w, 0);
+    if(stage == 3){
+        a.aux_u[q] = rhs_u + dpdr;
+        a.aux_v[q] = rhs_v + dpdthe_term;
+        a.aux_w[q] = rhs_w + dpdphi_term;
+    }
+
+    // ---- k* and omega* (RHS_Jup_Turb.cpp:265-303 and :763-998, k-omega) ----
+    {
+        const Stencil7<T> K = load7(a.in[ATJUP_F_TKE], q, P, R, qke, qkw);
+        const Stencil7<T> D = load7(a.in[ATJUP_F_DIS], q, P, R, qke, qkw);
+        T d1rK = (K.ip - K.im) * a.inv_2dr, d2rK = (K.ip - T(2) * K.c + K.im) * a.inv_dr2;
+        T d1tK = (K.jp - K.jm) * a.inv_2dthe, d2tK = (K.jp - T(2) * K.c + K.jm) * a.inv_dthe2;
+        T d1pK = (K.kp - K.km) * a.inv_2dphi, d2pK = (K.kp - T(2) * K.c + K.km) * a.inv_dphi2;
+        T d1rD = (D.ip - D.im) * a.inv_2dr, d2rD = (D.ip - T(2) * D.c + D.im) * a.inv_dr2;
+        T d1tD = (D.jp - D.jm) * a.inv_2dthe, d2tD = (D.jp - T(2) * D.c + D.jm) * a.inv_dthe2;
+        T d1pD = (D.kp - D.km) * a.inv_2dphi, d2pD = (D.kp - T(2) * D.c + D.km) * a.inv_dphi2;
+        // Neumann at solid faces: the solid neighbour takes the cell's own value
+        if(fl & (FL_IM | FL_IP)){
+            const T ka = (fl & FL_IM) ? K.c : K.im, kb2 = (fl & FL_IP) ? K.c : K.ip;
+            const T da = (fl & FL_IM) ? D.c : D.im, db2 = (fl & FL_IP) ? D.c : D.ip;
+            d1rK = (kb2 - ka) * a.inv_2dr; d2rK = (kb2 - T(2) * K.c + ka) * a.inv_dr2;
+            d1rD = (db2 - da) * a.inv_2dr; d2rD = (db2 - T(2) * D.c + da) * a.inv_dr2;
+        }
+        if(fl & (FL_JM | FL_JP)){
+            const T ka = (fl & FL_JM) ? K.c : K.jm, kb2 = (fl & FL_JP) ? K.c : K.jp;
+            const T da = (fl & FL_JM) ? D.c : D.jm, db2 = (fl & FL_JP) ? D.c : D.jp;
+            d1tK = (kb2 - ka) * a.inv_2dthe; d2tK = (kb2 - T(2) * K.c + ka) * a.inv_dthe2;
+            d1tD = (db2 - da) * a.inv_2dthe; d2tD = (db2 - T(2) * D.c + da) * a.inv_dthe2;
+        }
+        if(fl & (FL_KM | FL_KP)){
+            const T ka = (fl & FL_KM) ? K.c : K.km, kb2 = (fl & FL_KP) ? K.c : K.kp;
+            const T da = (fl & FL_KM) ? D.c : D.km, db2 = (fl & FL_KP) ? D.c : D.kp;
+            d1pK = (kb2 - ka) * a.inv_2dphi; d2pK = (kb2 - T(2) * K.c + ka) * a.inv_dphi2;
+            d1pD = (db2 - da) * a.inv_2dphi; d2pD = (db2 - T(2) * D.c + da) * a.inv_dphi2;
+        }
+        T rhs_k = T(0), rhs_d = T(0);
+        if(a.turb){
+            const T nue_here = isfinite(nue_c) ? dmax(T(0), nue_c) : T(0);
+            const T tke0 = K.c, dis0 = D.c;
+            const T tke_s = dmax(tke0, a.dis_min);
+            const T dis_s = dmax(dis0, a.dis_min);
+            const T dudr_s = d1r[1], dvdr_s = d1r[2], dwdr_s = d1r[3];
+            const T dudthe_s = d1t[1] * G.inv_rm, dvdthe_s = d1t[2] * G.inv_rm, dwdthe_s = d1t[3] * G.inv_rm;
+            const T dudphi_s = d1p[1] * G.inv_rmsinthe, dvdphi_s = d1p[2] * G.inv_rmsinthe;
+            const T dwdphi_s = d1p[3] * G.inv_rmsinthe;
+            const T dtkedthe_s = d1tK * G.inv_rm, ddisdthe_s = d1tD * G.inv_rm;
+            const T dtkedphi_s = d1pK * G.inv_rmsinthe, ddisdphi_s = d1pD * G.inv_rmsinthe;
+            const T grad_dot = d1rK * d1rD + dtkedthe_s * ddisdthe_s + dtkedphi_s * ddisdphi_s;
+            T cnue = tke_s / dis_s;
+            cnue = dmin(cnue, a.nue_max);
+            const T der = T(0.66667) * (dudr_s + dvdthe_s + dwdphi_s);
+            const T P_full = dmax(T(0),
+                  (cnue * (T(2) * dudr_s - der) - T(0.66667) * tke0) * dudr_s
+                + (cnue * (dudthe_s + dvdr_s)) * dudthe_s
+                + (cnue * (dudphi_s + dwdr_s)) * dudphi_s
+                + (cnue * (T(2) * dvdthe_s - der) - T(0.66667) * tke0) * dvdthe_s
+                + (cnue * (dvdr_s + dudthe_s)) * dvdr_s
+                + (cnue * (dvdphi_s + dwdthe_s)) * dvdphi_s
+                + (cnue * (T(2) * dwdphi_s - der) - T(0.66667) * tke0) * dwdphi_s
+                + (cnue * (dwdr_s + dudphi_s)) * dwdr_s
+                + (cnue * (dwdthe_s + dvdphi_s)) * dwdthe_s);
+            const T sig_k = T(0.6), sig_w = T(0.5), bet_star = T(0.09), gam = T(0.52), bet_0 = T(0.0708);
+            const T diffusion_tke_re = a.inv_re + sig_k * nue_here;
+            const T diffusion_dis_re = a.inv_re + sig_w * nue_here;
+            const T S11 = dudr_s, S22 = dvdthe_s, S33 = dwdphi_s;
+            const T S12 = T(0.5) * (dudthe_s + dvdr_s);
+            const T S13 = T(0.5) * (dudphi_s + dwdr_s);
+            const T S23 = T(0.5) * (dvdphi_s + dwdthe_s);
+            const T S_mag = sqrt(T(2) * (S11 * S11 + S22 * S22 + S33 * S33
+                                         + T(2) * (S12 * S12 + S13 * S13 + S23 * S23)));
+            const T W12 = dudthe_s - dvdr_s;
+            const T W13 = dudphi_s - dwdr_s;
+            const T W23 = dvdphi_s - dwdthe_s;
+            const T Omega = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
+            const T chi_w = fabs(Omega * Omega * S_mag / pow(bet_star * dis_s, T(3)));
+            const T f_bet = (T(1) + T(85) * chi_w) / (T(1) + T(100) * chi_w);
+            const T bet_wc = bet_0 * f_bet;
+            const T sig_d = (grad_dot <= T(0)) ? T(0) : T(0.125);
+            const T P_k = dmin(P_full, T(20) * bet_star * tke_s * dis_s);
+            const T Y_k = bet_star * tke_s * dis_s;
+            const T P_w = gam * dis_s / tke_s * P_k;
+            const T Y_w = bet_wc * dis_s * dis_s;
+            const T D_w = sig_d / dis_s * grad_dot;
+            T tke_src = P_k - Y_k;
+            T dis_src = P_w - Y_w + D_w;
+            const T tke_src_max = T(20) * tke_s * dis_s;
+            const T dis_src_max = T(20) * dis_s * dis_s;
+            if(!isfinite(tke_src)) tke_src = T(0);
+            if(!isfinite(dis_src)) dis_src = T(0);
+            tke_src = dmax(-tke_src_max, dmin(tke_src_max, tke_src));
+            dis_src = dmax(-dis_src_max, dmin(dis_src_max, dis_src));
+            if(stage == 3){
+                a.prod[q] = P_full;
+                a.tsrc[q] = tke_src;
+                a.dsrc[q] = dis_src;
+            }
+            const T transport_k = u * d1rK + v_invrm * d1tK + w_invrs * d1pK;
+            const T transport_d = u * d1rD + v_invrm * d1tD + w_invrs * d1pD;
+            const T diffusion_k = d2rK + d1rK * two_inv_rm + d2tK * G.inv_rm2
+                + d1tK * G.costhe_inv_rm2sinthe + d2pK * G.inv_rm2sinthe2;
+            const T diffusion_d = d2rD + d1rD * two_inv_rm + d2tD * G.inv_rm2
+                + d1tD * G.costhe_inv_rm2sinthe + d2pD * G.inv_rm2sinthe2;
+            rhs_k = -transport_k + diffusion_k * diffusion_tke_re + tke_src;
+            rhs_d = -transport_d + diffusion_d * diffusion_dis_re + dis_src;
+        }
+        update(ATJUP_F_TKE, rhs_k, 3);
+        update(ATJUP_F_DIS, rhs_d, 4);
+    }
+
+    // ---- the eleven species (RHS_Jup_Turb.cpp:1273-1338) ----
+    #pragma unroll 1
+    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++){
+        const Stencil7<T> s = load7(a.in[f], q, P, R, qke, qkw);
+        const T e1r = (s.ip - s.im) * a.inv_2dr, e2r = (s.ip - T(2) * s.c + s.im) * a.inv_dr2;
+        const T e1t = (s.jp - s.jm) * a.inv_2dthe, e2t = (s.jp - T(2) * s.c + s.jm) * a.inv_dthe2;
+        const T e1p = (s.kp - s.km) * a.inv_2dphi, e2p = (s.kp - T(2) * s.c + s.km) * a.inv_dphi2;
+        const T transport = u * e1r + v_invrm * e1t + w_invrs * e1p;
+        const T diffusion = e2r + e1r * two_inv_rm + e2t * G.inv_rm2
+            + e1t * G.costhe_inv_rm2sinthe + e2p * G.inv_rm2sinthe2;
+        T rhs;
+        if(f == ATJUP_F_NH4SH){
+            const T v_stokes = a.stokes_a * (a.rg_nh4sh - rho_f) * a.g / a.mue_mix;
+            rhs = -transport
+                + a.fl_nh4sh[q]
+                + diffusion * (a.sp_coef[f] + nue_t_s)
+                + a.cr * a.mf[f][q]
+                + (v_stokes / a.u_0) * e1r;
+            update(f, rhs, 2);
+        } else {
+            rhs = -transport + diffusion * (a.sp_coef[f] + nue_t_s);
+            if(a.mf[f]) rhs = rhs + a.cr * a.mf[f][q];
+            if(a.spr[f]) rhs = rhs + a.pc * a.spr[f][q];
+            update(f, rhs, 1);
+        }
+    }
+}
+
+// =============================================================================================
+// Turbulence<P>::run, k-omega with TURB_FREE (Turbulence.h:307-1012)
+// =============================================================================================
+// clamp-in pass (interior fluid cells) into the live buffers; every other cell is copied
+template<typename T>
+__global__ __launch_bounds__(256) void k_turb_clamp_in(const T* __restrict__ tke, const T* __restrict__ dis,
+        T* __restrict__ ltke, T* __restrict__ ldis, const uint8_t* __restrict__ flags,
+        int im, int jm, int km, int pitch, T dis_min){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    T kk = tke[q], dd = dis[q];
+    if(i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2 && k <= km - 2 && !(flags[q] & FL_SELF)){
+        kk = dmax(T(0), kk);
+        dd = dmax(dis_min, dd);
+    }
+    ltke[q] = kk;
+    ldis[q] = dd;
+}
+
+struct TurbConsts {
+    double dr, dthe, dphi, nue_max, dis_min;
+};
+
+// compute_sources + compute_k_omega (Turbulence.h:633-1012): nue, prod, tke_source, dis_source of
+// the interior fluid cells from the clamped k*, omega* and the velocity field
+template<typename T>
+__global__ __launch_bounds__(256) void k_turb_sources(const T* __restrict__ tke, const T* __restrict__ dis,
+        const T* __restrict__ U, const T* __restrict__ V, const T* __restrict__ W,
+        T* __restrict__ nue, T* __restrict__ prod, T* __restrict__ tsrc, T* __restrict__ dsrc,
+        const uint8_t* __restrict__ flags, const double* __restrict__ tabs,
+        int im, int jm, int km, int pitch, size_t plane, const TurbConsts tc){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1, i = blockIdx.z + 1;
+    if(k > km - 2 || j > jm - 2 || i > im - 2) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const uint8_t fl = flags[q];
+    if(fl & FL_SELF) return;
+    const size_t P = plane, R = (size_t)pitch;
+    const int kw = (k == 0) ? km - 2 : k - 1;
+    const int ke = (k == km - 2) ? 0 : k + 1;
+    const size_t qip = q + P, qim = q - P, qjp = q + R, qjm = q - R, qkp = q - k + ke, qkm = q - k + kw;
+    const double* rad = tabs;
+    const double* sin_the = tabs + im;
+    const double* cos_the = tabs + im + 2 * jm;
+    const T rm = (T)rad[i];
+    const T exp_rm = T(1);
+    T sinthe = (T)sin_the[j];
+    if(sinthe == T(0)) sinthe = T(1.0e-5);
+    const T costhe = (T)cos_the[j];
+    const T cotanthe = costhe / sinthe;
+    const T inv_rm = T(1) / rm;
+    const T rmsinthe = rm * sinthe;
+    const T dr = (T)tc.dr, dthe = (T)tc.dthe, dphi = (T)tc.dphi;
+    const T inv_2dr = T(1) / (T(2) * dr);
+    const T inv_2dthe = T(1) / (T(2) * dthe);
+    const T inv_2dphi = T(1) / (T(2) * dphi);
+    const T inv_rm2dthe = inv_2dthe / rm;
+    const T inv_rmsinthe2dphi = inv_2dphi / rmsinthe;
+    const T Uq = U[q], Vq = V[q], Wq = W[q];
+    const T Uip = U[qip], Uim = U[qim], Ujp = U[qjp], Ujm = U[qjm], Ukp = U[qkp], Ukm = U[qkm];
+    const T Vip = V[qip], Vim = V[qim], Vjp = V[qjp], Vjm = V[qjm], Vkp = V[qkp], Vkm = V[qkm];
+    const T Wip = W[qip], Wim = W[qim], Wjp = W[qjp], Wjm = W[qjm], Wkp = W[qkp], Wkm = W[qkm];
+    const T dudr = (Uip - Uim) * inv_2dr * exp_rm;
+    const T dvdr = (Vip - Vim) * inv_2dr * exp_rm;
+    const T dwdr = (Wip - Wim) * inv_2dr * exp_rm;
+    const T dudthe = (Ujp - Ujm) * inv_rm2dthe;
+    const T dvdthe = (Vjp - Vjm) * inv_rm2dthe;
+    const T dwdthe = (Wjp - Wjm) * inv_rm2dthe;
+    const T dudphi = (Ukp - Ukm) * inv_rmsinthe2dphi;
+    const T dvdphi = (Vkp - Vkm) * inv_rmsinthe2dphi;
+    const T dwdphi = (Wkp - Wkm) * inv_rmsinthe2dphi;
+    const bool nb = fl & FL_IM;
+    const bool njm = fl & FL_JM, njp = fl & FL_JP, nkm = fl & FL_KM, nkp = fl & FL_KP;
+    const T tq = tke[q], dq = dis[q];
+    const T tke_im1 = nb ? tq : tke[qim];
+    const T dis_im1 = nb ? dq : dis[qim];
+    const T tke_jm1 = njm ? tq : tke[qjm];
+    const T tke_jp1 = njp ? tq : tke[qjp];
+    const T dis_jm1 = njm ? dq : dis[qjm];
+    const T dis_jp1 = njp ? dq : dis[qjp];
+    const T tke_km1 = nkm ? tq : tke[qkm];
+    const T tke_kp1 = nkp ? tq : tke[qkp];
+    const T dis_km1 = nkm ? dq : dis[qkm];
+    const T dis_kp1 = nkp ? dq : dis[qkp];
+    const T tke_ip = tke[qip], dis_ip = dis[qip];
+    const T dtkedr = (tke_ip - tke_im1) * inv_2dr * exp_rm;
+    const T ddisdr = (dis_ip - dis_im1) * inv_2dr * exp_rm;
+    T cnue;
+    {
+        const T dis_here = dmax(dq, (T)tc.dis_min);
+        const T tke_here = dmax(tq, T(0));
+        cnue = tke_here / dis_here;
+        cnue = dmin(cnue, (T)tc.nue_max);
+    }
+    const T u_r = Uq * inv_rm, v_r = Vq * inv_rm, w_r = Wq * inv_rm;
+    const T v_cot_r = v_r * cotanthe, w_cot_r = w_r * cotanthe;
+    const T g_rr = dudr, g_rt = dudthe - v_r, g_rp = dudphi - w_r;
+    const T g_tr = dvdr, g_tt = dvdthe + u_r, g_tp = dvdphi - w_cot_r;
+    const T g_pr = dwdr, g_pt = dwdthe, g_pp = dwdphi + u_r + v_cot_r;
+    const T der = T(0.66667) * (g_rr + g_tt + g_pp);
+    const T pr = dmax(T(0),
+          (cnue * (T(2) * g_rr - der) - T(0.66667) * tq) * g_rr
+        + (cnue * (g_rt + g_tr)) * g_rt
+        + (cnue * (g_rp + g_pr)) * g_rp
+        + (cnue * (T(2) * g_tt - der) - T(0.66667) * tq) * g_tt
+        + (cnue * (g_tr + g_rt)) * g_tr
+        + (cnue * (g_tp + g_pt)) * g_tp
+        + (cnue * (T(2) * g_pp - der) - T(0.66667) * tq) * g_pp
+        + (cnue * (g_pr + g_rp)) * g_pr
+        + (cnue * (g_pt + g_tp)) * g_pt);
+    prod[q] = pr;
+    const T W12 = g_rt - g_tr, W13 = g_rp - g_pr, W23 = g_tp - g_pt;
+    const T Omega_mag = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
+    // compute_k_omega (Turbulence.h:920-1012); exp_rm = 1 / (rm + 1) in its derivatives
+    const T bet_star = T(0.09), gam = T(0.52), C_lim = T(0.875), bet_0 = T(0.0708);
+    const T erm = T(1) / (rm + T(1));
+    const T dtkedr_neu = nb ? (tke_ip - tq) * (T(1) / (T(2) * dr)) * erm : dtkedr;
+    const T ddisdr_neu = nb ? (dis_ip - dq) * (T(1) / (T(2) * dr)) * erm : ddisdr;
+    const T dtkedthe_neu = (tke_jp1 - tke_jm1) / (rm * T(2) * dthe);
+    const T ddisdthe_neu = (dis_jp1 - dis_jm1) / (rm * T(2) * dthe);
+    const T dtkedphi_neu = (tke_kp1 - tke_km1) / (rmsinthe * T(2) * dphi);
+    const T ddisdphi_neu = (dis_kp1 - dis_km1) / (rmsinthe * T(2) * dphi);
+    const T dudr_k = (Uip - Uim) / (T(2) * dr) / (rm + T(1));
+    const T dvdthe_k = (Vjp - Vjm) / (rm * T(2) * dthe);
+    const T dwdphi_k = (Wkp - Wkm) / (rmsinthe * T(2) * dphi);
+    const T dudthe_u = (Ujp - Ujm) / (rm * T(2) * dthe);
+    const T dudphi_u = (Ukp - Ukm) / (rmsinthe * T(2) * dphi);
+    const T dvdr_u = (Vip - Vim) / (T(2) * dr) / (rm + T(1));
+    const T dwdr_u = (Wip - Wim) / (T(2) * dr) / (rm + T(1));
+    const T dvdphi_u = (Vkp - Vkm) / (rmsinthe * T(2) * dphi);
+    const T dwdthe_u = (Wjp - Wjm) / (rm * T(2) * dthe);
+    const T S11 = dudr_k, S22 = dvdthe_k, S33 = dwdphi_k;
+    const T S12 = T(0.5) * (dudthe_u + dvdr_u);
+    const T S13 = T(0.5) * (dudphi_u + dwdr_u);
+    const T S23 = T(0.5) * (dvdphi_u + dwdthe_u);
+    const T S_mag = sqrt(T(2) * (S11 * S11 + S22 * S22 + S33 * S33 + T(2) * (S12 * S12 + S13 * S13 + S23 * S23)));
+    const T w_hat = dmax(dq, C_lim * S_mag / sqrt(bet_star));
+    const T dot = dtkedr_neu * ddisdr_neu + dtkedthe_neu * ddisdthe_neu + dtkedphi_neu * ddisdphi_neu;
+    T sig_d = dot;
+    sig_d = (sig_d <= T(0)) ? T(0) : T(0.125);
+    const T chi_w = fabs(Omega_mag * Omega_mag * S_mag / pow(bet_star * dmax(dq, T(1.0e-20)), T(3)));
+    const T f_bet = (T(1) + T(85) * chi_w) / (T(1) + T(100) * chi_w);
+    const T bet_wc = bet_0 * f_bet;
+    nue[q] = tq / dmax(w_hat, T(1.0e-20));
+    const T P_k = dmin(pr, T(20) * bet_star * tq * dq);
+    const T Y_k = bet_star * tq * dq;
+    const T P_w = gam * dq / dmax(tq, T(1.0e-20)) * P_k;
+    const T Y_w = bet_wc * dq * dq;
+    const T D_w = sig_d / dmax(dq, T(1.0e-20)) * dot;
+    tsrc[q] = P_k - Y_k;
+    dsrc[q] = P_w - Y_w + D_w;
+}
+
+// clamp-out (interior fluid), zero_land_cells and clamp_nue on every cell but the i = 0 plane's
+// wall copy, which k_turb_wall_plane writes first (Turbulence.h:378-420, :623-660, :1040-1106)
+template<typename T>
+__global__ __launch_bounds__(256) void k_turb_post(T* __restrict__ tke, T* __restrict__ dis, T* __restrict__ nue,
+        T* __restrict__ prod, T* __restrict__ tsrc, T* __restrict__ dsrc, const uint8_t* __restrict__ flags,
+        int im, int jm, int km, int pitch, T k_bg, T dis_min, T nue_max){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km || i == 0) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const bool solid = flags[q] & FL_SELF;
+    if(solid){
+        tke[q] = T(0); dis[q] = T(0); nue[q] = T(0); prod[q] = T(0); tsrc[q] = T(0); dsrc[q] = T(0);
+        return;
+    }
+    T n = nue[q];
+    if(i <= im - 2 && j >= 1 && j <= jm - 2 && k <= km - 2){
+        const T kk = dmax(k_bg, tke[q]);
+        const T dd = dmax(dis_min, dis[q]);
+        tke[q] = kk;
+        dis[q] = dd;
+        n = (n < T(0)) ? T(0) : ((n > nue_max) ? nue_max : n);
+        const T dsm = T(20) * dd * dd;
+        const T ds = dsrc[q];
+        dsrc[q] = (ds < -dsm) ? -dsm : ((ds > dsm) ? dsm : ds);
+        const T tsm = T(20) * kk * dd;
+        const T ts = tsrc[q];
+        tsrc[q] = (ts < -tsm) ? -tsm : ((ts > tsm) ? tsm : ts);
+    }
+    nue[q] = ((n < T(0)) ? T(0) : ((n > nue_max) ? nue_max : n)) * T(1);
+}
+
+// the i = 0 plane: apply_wall_bc copies k*, omega*, nue from i = 1 after its clamp-out; then
+// zero_land_cells and clamp_nue of the plane itself
+template<typename T>
+__global__ void k_turb_wall_plane(T* __restrict__ tke, T* __restrict__ dis, T* __restrict__ nue,
+        T* __restrict__ prod, T* __restrict__ tsrc, T* __restrict__ dsrc, const uint8_t* __restrict__ flags,
+        int im, int jm, int km, int pitch, size_t plane, T k_bg, T dis_min, T nue_max, int clamp_src){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const size_t q = (size_t)j * pitch + k, q1 = q + plane;
+    T kk = tke[q1], dd = dis[q1], n = nue[q1];
+    if(clamp_src && 1 <= im - 2 && j >= 1 && j <= jm - 2 && k <= km - 2 && !(flags[q1] & FL_SELF)){
+        kk = dmax(k_bg, kk);
+        dd = dmax(dis_min, dd);
+        n = (n < T(0)) ? T(0) : ((n > nue_max) ? nue_max : n);
+    }
+    if(!clamp_src){ tke[q] = kk; dis[q] = dd; nue[q] = n; return; }
+    if(flags[q] & FL_SELF){
+        tke[q] = T(0); dis[q] = T(0); nue[q] = T(0); prod[q] = T(0); tsrc[q] = T(0); dsrc[q] = T(0);
+        return;
+    }
+    tke[q] = kk;
+    dis[q] = dd;
+    nue[q] = ((n < T(0)) ? T(0) : ((n > nue_max) ? nue_max : n)) * T(1);
+}
+
+template<typename T>
+struct TurbArrays { T* a[6]; };
+
+// seam_copy of the closure's arrays: k = km-1 is the copy of k = 0
+template<typename T>
+__global__ void k_turb_seam(const TurbArrays<T> L, int im, int jm, int km, int pitch){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(j >= jm) return;
+    const size_t r0 = ((size_t)i * jm + j) * pitch;
+    #pragma unroll
+    for(int f = 0; f < 6; f++) L.a[f][r0 + km - 1] = L.a[f][r0];
+}
+
+// fill_lid_corners, first the lid plane i = im-1 (rows 1..jm-2) ...
+template<typename T>
+__global__ void k_turb_lid(const TurbArrays<T> L, int im, int jm, int km, int pitch, size_t plane){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1;
+    if(k >= km || j > jm - 2) return;
+    const size_t q = ((size_t)(im - 1) * jm + j) * pitch + k;
+    #pragma unroll
+    for(int f = 0; f < 6; f++) L.a[f][q] = L.a[f][q - plane];
+}
+
+// ... then the pole rows j = 0 and jm-1 of every level
+template<typename T>
+__global__ void k_turb_pole_rows(const TurbArrays<T> L, int im, int jm, int km, int pitch){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(k >= km) return;
+    const size_t qs = (size_t)i * jm * pitch + k;
+    const size_t qn = qs + (size_t)(jm - 1) * pitch;
+    #pragma unroll
+    for(int f = 0; f < 6; f++){
+        L.a[f][qs] = L.a[f][qs + pitch];
+        L.a[f][qn] = L.a[f][qn - pitch];
+    }
+}
+
+// the wall-function omega* at the first fluid cell of every column with an obstacle
+// (cJupiterModel.cpp:356-374), with the friction velocity of compute_vel_star
+template<typename T>
+__global__ void k_turb_wall_omega(T* __restrict__ dis, const int* __restrict__ itopo, const double* __restrict__ vel_star,
+        const double* __restrict__ lthick, int im, int jm, int km, int pitch, double nd_omega){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const int i_w = itopo[(size_t)j * km + k];
+    if(i_w <= 0 || i_w >= im - 1) return;
+    const double nue_gas_phys = 1.8e-5, bet_star = 0.09, bet_1 = 0.0333, kappa = 0.41;
+    const double y1 = dmax(lthick[max(i_w - 1, 0)], 1.0);
+    const double y = 0.5 * y1;
+    const double om_vis = 60.0 * nue_gas_phys / (bet_1 * y1 * y1);
+    const double om_log = vel_star[(size_t)j * km + k] / (sqrt(bet_star) * kappa * y);
+    dis[((size_t)i_w * jm + j) * pitch + k] = (T)(sqrt(om_vis * om_vis + om_log * om_log) * nd_omega);
+}
+
+__global__ void k_fill_double(double* __restrict__ x, size_t n, double v){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q < n) x[q] = v;
+}
+
+// the closure's share of bcSolidGround: safe_clamp of its arrays on every cell (BC_Jup.h:627-655)
+template<typename T>
+__global__ __launch_bounds__(256) void k_turb_sanitize(T* __restrict__ tke, T* __restrict__ dis, T* __restrict__ nue,
+        T* __restrict__ prod, T* __restrict__ tsrc, T* __restrict__ dsrc, size_t n, T tke_max, T nue_max){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= n) return;
+    const T prod_max = T(1.0e4), dis_max = T(1.0e6);
+    tke[q] = safe_clamp_d(tke[q], T(0), tke_max);
+    dis[q] = safe_clamp_d(dis[q], T(1.0e-10), dis_max);
+    nue[q] = safe_clamp_d(nue[q], T(0), nue_max);
+    prod[q] = safe_clamp_d(prod[q], T(0), prod_max);
+    tsrc[q] = safe_clamp_d(tsrc[q], -prod_max, prod_max);
+    dsrc[q] = safe_clamp_d(dsrc[q], -prod_max, prod_max);
+}
+
+// =============================================================================================
+// SaturationAdjustment<P>::run for one gas (SaturationAdjustment.h:255-746) with the ATJUP
+// defaults CONSERVE 1, NEWTON 1, EVAP 1, BERGERON 2, DRY_EXIT 1, ICE_ENTRY 0, FREEZE 0,
+// LOCAL_RHO 1, and the final positivity pass (:673-685) fused in.
+// =============================================================================================
+struct SatParams {
+    double t_0, t_00, ep, lv, ls, C, L0, R, del_alf, del_bet, C_i, L0_i, del_alf_i, del_bet_i;
+    double cp_mix, t_ref, r_mix;
+};
+
+__device__ __forceinline__ double svp_d(double T, double C, double L0, double R, double da, double db){
+    return exp(C + (-L0 / T + da * log(T) + db * T) / (1e-3 * R));         // ATPhys.h:49-52
+}
+
+__device__ void satadj_cell(double& cv, double& ccl, double& cic, double& tq, double p_u, double rf,
+                            const SatParams& P){
+    const double t_0 = P.t_0, t_00 = P.t_00, ep = P.ep, lv = P.lv, ls = P.ls;
+    const double t_range_inv = 1.0 / (t_0 - t_00);
+    const double Lf = ls - lv;
+    const double cp_mix = P.cp_mix, t_ref = P.t_ref;
+    const int iter_prec_end = 15;
+    const double q_diff_min = 1.0e-3;
+    double t_u = tq * t_ref;
+    if(!(t_u > 0.0) || !(p_u > 0.0)) return;
+    const double rho_at = (rf > 0.0 && isfinite(rf)) ? rf : P.r_mix;
+    if(t_u > t_0){
+        if(cic > 0.0){
+            double dm = cic;
+            if(Lf > 0.0){
+                const double hc = cp_mix * rho_at;
+                const double cap = hc * (t_u - t_0) / Lf;
+                if(dm > cap) dm = cap;
+                t_u -= Lf * dm / hc;
+                tq = t_u / t_ref;
+            }
+            cic -= dm;
+            ccl += dm;
+        }
+    }
+    if(cv < 0.0) cv = 0.0;
+    if(ccl < 0.0) ccl = 0.0;
+    if(cic < 0.0) cic = 0.0;
+    const double rho_c = rho_at;
+    const double E_Rain_0 = svp_d(t_u, P.C, P.L0, P.R, P.del_alf, P.del_bet);
+    const double q_Rain_0 = rho_c * ep * E_Rain_0 / p_u;
+    if(q_Rain_0 <= 0.0) return;
+    double q_entry = q_Rain_0;
+    if(ccl <= 0.0 && cic <= 0.0 && t_u < t_0)
+        q_entry = rho_c * ep * svp_d(t_u, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
+    if(cv <= q_entry){
+        const double q_c0 = ccl, q_i0 = cic;
+        if(q_c0 <= 0.0 && q_i0 <= 0.0) return;
+        const double q_Ice_0 = rho_c * ep * svp_d(t_u, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
+        double q_t0;
+        if(q_c0 > 0.0 && q_i0 > 0.0) q_t0 = (q_c0 * q_Rain_0 + q_i0 * q_Ice_0) / (q_c0 + q_i0);
+        else if(q_i0 <= 0.0)         q_t0 = q_Rain_0;
+        else                         q_t0 = q_Ice_0;
+        if(fabs(cv - q_t0) <= q_diff_min * q_t0) return;
+    }
+    double q_v_b = cv, q_c_b = ccl, q_i_b = cic;
+    double Tt = t_u;
+    double q_v_hyp = q_v_b;
+    double q_rain_now = q_Rain_0;
+    for(int itr = 1; itr <= iter_prec_end; itr++){
+        double CND = (Tt - t_00) * t_range_inv;
+        double DEP = (t_0 - Tt) * t_range_inv;
+        if(Tt <= t_00){ CND = 0.0; DEP = 1.0; }
+        if(Tt >= t_0){ CND = 1.0; DEP = 0.0; }
+        const bool berg = Tt < t_0 && q_v_b <= q_rain_now && (q_i_b > 0.0 || q_c_b <= 0.0);
+        if(berg){
+            CND = 0.0; DEP = 1.0;
+            if(q_c_b > 0.0){
+                Tt -= lv * q_c_b / (cp_mix * rho_c);
+                q_v_b += q_c_b;
+                q_c_b = 0.0;
+            }
+        }
+        double d_q_v = q_v_hyp - q_v_b;
+        double d_q_c = -d_q_v * CND;
+        double d_q_i = -d_q_v * DEP;
+        if(d_q_v < -q_v_b){ d_q_v = -q_v_b; d_q_c = -d_q_v * CND; d_q_i = -d_q_v * DEP; }
+        if(q_c_b + d_q_c < 0.0){ const double ex = -(q_c_b + d_q_c); d_q_c = -q_c_b; d_q_i -= ex; }
+        if(q_i_b + d_q_i < 0.0){ const double ex = -(q_i_b + d_q_i); d_q_i = -q_i_b; d_q_c -= ex;
+                                 if(q_c_b + d_q_c < 0.0) d_q_c = -q_c_b; }
+        d_q_v = -(d_q_c + d_q_i);
+        Tt += (lv * d_q_c + ls * d_q_i) / (cp_mix * rho_c);
+        q_v_b += d_q_v;
+        q_c_b += d_q_c;
+        q_i_b += d_q_i;
+        if(q_v_b < 0.0) q_v_b = 0.0;
+        if(q_c_b < 0.0) q_c_b = 0.0;
+        if(q_i_b < 0.0) q_i_b = 0.0;
+        const double E_Rain = svp_d(Tt, P.C, P.L0, P.R, P.del_alf, P.del_bet);
+        const double E_Ice = svp_d(Tt, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i);
+        const double q_Rain = rho_c * ep * E_Rain / p_u;
+        const double q_Ice = rho_c * ep * E_Ice / p_u;
+        q_rain_now = q_Rain;
+        if(q_c_b > 0.0 && q_i_b > 0.0) q_v_hyp = (q_c_b * q_Rain + q_i_b * q_Ice) / (q_c_b + q_i_b);
+        else if(q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0) q_v_hyp = q_Ice;
+        else if(q_i_b == 0.0) q_v_hyp = q_Rain;
+        else                  q_v_hyp = q_Ice;
+        if(Tt >= t_0){
+            if(q_i_b > 0.0){
+                double dm = q_i_b;
+                if(Lf > 0.0){
+                    const double hc = cp_mix * rho_c;
+                    const double cap = hc * (Tt - t_0) / Lf;
+                    if(dm > cap) dm = cap;
+                    Tt -= Lf * dm / hc;
+                }
+                q_i_b -= dm;
+                q_c_b += dm;
+            }
+        }
+        const double q_diff = fabs(q_v_b - q_v_hyp) / (q_v_hyp + 1e-20);
+        if(q_diff <= q_diff_min) break;
+        if(q_c_b <= 0.0 && q_i_b <= 0.0 && q_v_b < q_v_hyp) break;
+        const double dT = 0.1;
+        const double q_Rain_d = rho_c * ep * svp_d(Tt + dT, P.C, P.L0, P.R, P.del_alf, P.del_bet) / p_u;
+        const double q_Ice_d = rho_c * ep * svp_d(Tt + dT, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
+        double q_hyp_d;
+        if(q_c_b > 0.0 && q_i_b > 0.0) q_hyp_d = (q_c_b * q_Rain_d + q_i_b * q_Ice_d) / (q_c_b + q_i_b);
+        else if(q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0) q_hyp_d = q_Ice_d;
+        else if(q_i_b == 0.0) q_hyp_d = q_Rain_d;
+        else                  q_hyp_d = q_Ice_d;
+        double CNDn = (Tt - t_00) * t_range_inv;
+        if(Tt <= t_00) CNDn = 0.0;
+        if(Tt >= t_0) CNDn = 1.0;
+        if(berg) CNDn = 0.0;
+        const double Gn = (lv * CNDn + ls * (1.0 - CNDn)) / (cp_mix * rho_c) * (q_hyp_d - q_v_hyp) / dT;
+        double wgt = 1.0 / (1.0 + dmax(Gn, 0.0));
+        if(wgt < 0.05) wgt = 0.05;
+        q_v_hyp = q_v_b + wgt * (q_v_hyp - q_v_b);
+    }
+    cv = q_v_b;
+    ccl = q_c_b;
+    cic = q_i_b;
+    tq = Tt / t_ref;
+}
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_satadj(T* __restrict__ c, T* __restrict__ cloud, T* __restrict__ ice,
+        T* __restrict__ t, const T* __restrict__ ps, const T* __restrict__ rho, const uint8_t* __restrict__ flags,
+        int jm, int km, int pitch, const SatParams P){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    double cv = (double)c[q], ccl = (double)cloud[q], cic = (double)ice[q], tq = (double)t[q];
+    const double t0 = tq;
+    if(!(flags[q] & FL_SELF)) satadj_cell(cv, ccl, cic, tq, (double)ps[q], (double)rho[q], P);
+    if(cv <= 0.0) cv = 0.0;
+    if(ccl <= 0.0) ccl = 0.0;
+    if(cic <= 0.0) cic = 0.0;
+    c[q] = (T)cv;
+    cloud[q] = (T)ccl;
+    ice[q] = (T)cic;
+    if(tq != t0 || !(tq == tq)) t[q] = (T)tq;
+}
+
+// =============================================================================================
+// FluxLimiter<P>::nh4sh, Superbee (FluxLimiter.h:67-190), with the periodic seam
+// =============================================================================================
+__device__ __forceinline__ double superbee(double r){
+    return dmax(0.0, dmax(dmin(2.0 * r, 1.0), dmin(r, 2.0)));
+}
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_flux_limiter(const T* __restrict__ N, const T* __restrict__ U,
+        const T* __restrict__ V, const T* __restrict__ W, T* __restrict__ out, const double* __restrict__ tabs,
+        int im, int jm, int km, int pitch, size_t plane, double dr, double dthe, double dphi, double c43, double c13){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1, i = blockIdx.z + 1;
+    if(k > km - 2 || j > jm - 2 || i > im - 2) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const size_t P = plane, R = (size_t)pitch;
+    const int n = km - 1;
+    auto wrap = [n](int kk){ return kk < 0 ? kk + n : (kk >= n ? kk - n : kk); };
+    const size_t row = q - k;
+    const double eps = 1.0e-12;
+    const double qc = N[q];
+    const double q_rp = N[q + P], q_rm = N[q - P];
+    const double q_tp = N[q + R], q_tm = N[q - R];
+    const double q_pp = N[row + wrap(k + 1)], q_pm = N[row + wrap(k - 1)];
+    const double u = U[q], v = V[q], w = W[q];
+    const double rm = tabs[i];
+    const double sinthe = dmax(0.4, fabs(tabs[im + j]));
+    const double inv_rm = 1.0 / rm;
+    const double inv_rmsinthe = 1.0 / (rm * sinthe);
+    double corr = 0.0;
+    {
+        const double df = q_rp - qc, db = qc - q_rm;
+        const double denom = df + (df >= 0.0 ? eps : -eps);
+        const double r = (u >= 0.0) ? db / denom : ((i + 2 < im ? (double)N[q + 2 * P] : q_rp) - q_rp) / denom;
+        corr += (1.0 - superbee(r)) * fabs(u) * (df - db) / (2.0 * dr);
+    }
+    {
+        const double df = q_tp - qc, db = qc - q_tm;
+        const double denom = df + (df >= 0.0 ? eps : -eps);
+        const double q_far = (j + 2 < jm) ? (double)N[q + 2 * R] : (c43 * q_tp - c13 * qc);
+        const double r = (v >= 0.0) ? db / denom : (q_far - q_tp) / denom;
+        corr += (1.0 - superbee(r)) * fabs(v) * inv_rm * (df - db) / (2.0 * dthe);
+    }
+    {
+        const double df = q_pp - qc, db = qc - q_pm;
+        const double denom = df + (df >= 0.0 ? eps : -eps);
+        const double r = (w >= 0.0) ? db / denom : ((double)N[row + wrap(k + 2)] - q_pp) / denom;
+        corr += (1.0 - superbee(r)) * fabs(w) * inv_rmsinthe * (df - db) / (2.0 * dphi);
+    }
+    out[q] = (T)corr;
+}
+
+// =============================================================================================
+// host launchers
+// =============================================================================================
+namespace {
+
+template<typename T> T* F(atjup_ctx* c, int f){ return (T*)c->cur[f].p; }
+template<typename T> T* LIVE(atjup_ctx* c, int f){
+    if(c->turb_live_pending && (f == ATJUP_F_TKE || f == ATJUP_F_DIS))
+        return (T*)c->live_turb[f - ATJUP_F_TKE].p;
+    return (T*)c->cur[f].p;
+}
+
+template<typename T>
+int do_rk4_full(atjup_ctx* c){
+    StageFullArgs<T> a{};
+    const double dt = c->c.dt;
+    const atjup_consts& C = c->c;
+    const atjup_knobs& K = c->kn;
+    a.pd = F<T>(c, ATJUP_F_P_DYN); a.ph = F<T>(c, ATJUP_F_P_HYDRO); a.ps = F<T>(c, ATJUP_F_P_STAT);
+    a.rho = F<T>(c, ATJUP_F_RHO_MIX); a.wn = F<T>(c, ATJUP_F_WALL_NUE); a.nue = F<T>(c, ATJUP_F_NUE);
+    a.qrad = F<T>(c, ATJUP_F_Q_RAD); a.qprec = F<T>(c, ATJUP_F_Q_PRECIP); a.fl_nh4sh = F<T>(c, ATJUP_F_FLUXLIM_NH4SH);
+    a.aux_u = F<T>(c, ATJUP_F_AUX_U); a.aux_v = F<T>(c, ATJUP_F_AUX_V); a.aux_w = F<T>(c, ATJUP_F_AUX_W);
+    a.prod = F<T>(c, ATJUP_F_PROD); a.tsrc = F<T>(c, ATJUP_F_TKE_SOURCE); a.dsrc = F<T>(c, ATJUP_F_DIS_SOURCE);
+    a.flags = (const uint8_t*)c->flags.p;
+    a.geo = (const GeoIJ<T>*)c->geo.p;
+    a.brl = (const double*)c->brl.p;
+    a.im = c->im; a.jm = c->jm; a.km = c->km; a.pitch = c->pitch; a.plane = c->plane;
+    a.nkb = (c->km + SF_BK - 1) / SF_BK; a.njb = (c->jm + SF_BJ - 1) / SF_BJ;
+    a.nb = a.nkb * a.njb * c->im;
+    a.turb = K.turb_model >= 0 ? 1 : 0;
+    a.dt = (T)dt; a.one_sixth = (T)(1.0 / 6.0);
+    a.inv_2dr = (T)(1.0 / (2.0 * c->g.dr)); a.inv_2dthe = (T)(1.0 / (2.0 * c->g.dthe));
+    a.inv_2dphi = (T)(1.0 / (2.0 * c->g.dphi)); a.inv_dr2 = (T)(1.0 / (c->g.dr * c->g.dr));
+    a.inv_dthe2 = (T)(1.0 / (c->g.dthe * c->g.dthe)); a.inv_dphi2 = (T)(1.0 / (c->g.dphi * c->g.dphi));
+    a.cep = (T)(C.u_0 * C.u_0 / (C.cp_mix * C.t_ref));
+    a.to_nd = (T)(1.0 / (C.r_mix * C.u_0 * C.u_0 * 1.0e-5));
+    a.r_mix = (T)C.r_mix;
+    a.c_m2o = (T)(1.0 * -2.0 * C.omega);
+    a.c_p2o = (T)(1.0 * +2.0 * C.omega);
+    a.Coriolis = (T)C.Coriolis; a.centrifugal = (T)C.centrifugal;
+    a.cbuoy = (T)(-1.0 * 1.0 * 1.0 * C.buoyancy);
+    a.g = (T)C.g; a.rR = (T)(C.r_mix * C.R_mix); a.t_ref = (T)C.t_ref;
+    a.t_coef = (T)(K.t_diff_scale / (C.re * C.pr));
+    a.inv_re = (T)(1.0 / C.re);
+    a.rad_coupling = (T)K.rad_coupling; a.precip_coupling = (T)K.precip_coupling; a.turb_coupling = (T)K.turb_coupling;
+    a.L_rad = (T)(c->g.L_atm * 1.0e3); a.R_mix = (T)C.R_mix; a.cp_mix = (T)C.cp_mix; a.u_0 = (T)C.u_0;
+    a.cr = (T)C.chemical_reaction; a.pc = (T)K.precip_coupling;
+    a.stokes_a = (T)((2.0 / 9.0) * C.r_p_nh4sh * C.r_p_nh4sh);
+    a.rg_nh4sh = (T)C.rg_nh4sh; a.mue_mix = (T)C.mue_mix;
+    a.nue_max = (T)(K.nue_max_phys / (C.u_0 * (c->g.L_atm * 1.0e3)));
+    a.tke_max_nd = (T)(K.tke_max_phys / (C.u_0 * C.u_0));
+    a.dis_min = (T)1.0e-10;
+    const double sds = K.species_diff_scale;
+    const double sc[NP] = { 0, 0, 0, 0, C.sc_h2o, C.sc_h2o, C.sc_h2o, C.sc_h2s, C.sc_nh3, C.sc_nh3, C.sc_nh3,
+                            C.sc_ch4, C.sc_ch4, C.sc_ch4, C.sc_nh4sh, 0, 0 };
+    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++) a.sp_coef[f] = (T)(sds / (sc[f] * C.re));
+    const int spr_of[NP] = { -1, -1, -1, -1, ATJUP_F_S_PRECIP_H2O, ATJUP_F_S_PRECIP_H2O_CLOUD, ATJUP_F_S_PRECIP_H2O_ICE,
+                             -1, ATJUP_F_S_PRECIP_NH3, ATJUP_F_S_PRECIP_NH3_CLOUD, ATJUP_F_S_PRECIP_NH3_ICE,
+                             ATJUP_F_S_PRECIP_CH4, ATJUP_F_S_PRECIP_CH4_CLOUD, ATJUP_F_S_PRECIP_CH4_ICE, -1, -1, -1 };
+    const int mf_of[NP] = { -1, -1, -1, -1, -1, -1, -1, ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, -1, -1,
+                            -1, -1, -1, ATJUP_F_MASSFLUX_NH4SH, -1, -1 };
+    for(int f = 0; f < NP; f++){
+        a.spr[f] = spr_of[f] >= 0 ? F<T>(c, spr_of[f]) : nullptr;
+        a.mf[f] = mf_of[f] >= 0 ? F<T>(c, mf_of[f]) : nullptr;
+    }
+    const bool pend = c->turb_live_pending;
+    for(int s = 0; s < 4; s++){
+        a.stage = s;
+        a.c_in = (T)((s == 0 || s == 1) ? 0.5 * dt : (s == 2 ? dt : 0.0));
+        a.wgt = (T)((s == 0 || s == 3) ? 1.0 : 2.0);
+        for(int f = 0; f < NP; f++){
+            T* N = F<T>(c, f);
+            T* X = (T*)c->stg[0][f].p;
+            T* Y = (T*)c->stg[1][f].p;
+            const bool turbf = (f == ATJUP_F_TKE || f == ATJUP_F_DIS);
+            T* L = (pend && turbf) ? (T*)c->live_turb[f - ATJUP_F_TKE].p : N;
+            const T* in = (s == 0) ? L : (s == 1 ? X : (s == 2 ? Y : X));
+            T* out = (s == 0) ? X : (s == 1 ? Y : (s == 2 ? X : N));
+            a.in[f] = in; a.out[f] = out; a.nl[f] = N; a.acc[f] = (T*)c->acc[f].p;
+            // the cells the step does not integrate: stages 0 and 1 carry them into the stage
+            // buffers; the closure's live k*, omega* replace the n level's at the end
+            a.bsrc[f] = (s <= 1) ? in : ((s == 3 && pend && turbf) ? L : nullptr);
+        }
+        hipEvent_t t0 = timer_begin(c);
+        hipLaunchKernelGGL(k_stage_full<T>, dim3(a.nb), dim3(SF_BK, SF_BJ), 0, c->stream, a);
+        timer_end(c, ATJUP_T_RK_STAGES, t0, 1);
+        if(int rc = check_launch()) return rc;
+    }
+    c->turb_live_pending = false;
+    return 0;
+}
+
+template<typename T>
+TurbArrays<T> turb_arrays(atjup_ctx* c, T* tke, T* dis){
+    TurbArrays<T> L{};
+    L.a[0] = tke; L.a[1] = dis; L.a[2] = F<T>(c, ATJUP_F_NUE); L.a[3] = F<T>(c, ATJUP_F_PROD);
+    L.a[4] = F<T>(c, ATJUP_F_TKE_SOURCE); L.a[5] = F<T>(c, ATJUP_F_DIS_SOURCE);
+    return L;
+}
+
+template<typename T>
+int do_turbulence(atjup_ctx* c){
+    const atjup_consts& C = c->c;
+    const double L_m = c->g.L_atm * 1.0e3;
+    const double nue_max = c->kn.nue_max_phys / (C.u_0 * L_m);
+    const double k_bg = c->kn.turb_k_bg_phys / (C.u_0 * C.u_0);
+    const double dis_min = 1.0e-10;
+    // compute_vel_star with TURB_FREE: the calm-air floor 0.05 m/s everywhere (Turbulence.h:340-342)
+    const size_t n2 = (size_t)c->jm * c->km;
+    hipLaunchKernelGGL(k_fill_double, dim3(nblk(n2, 256)), dim3(256), 0, c->stream, (double*)c->vel_star.p, n2, 0.05);
+    T* tke = F<T>(c, ATJUP_F_TKE);
+    T* dis = F<T>(c, ATJUP_F_DIS);
+    T* ltke = (T*)c->live_turb[0].p;
+    T* ldis = (T*)c->live_turb[1].p;
+    if(c->turb_live_pending){
+        // a second run before the step: it starts from the previous run's live values
+        tke = ltke; dis = ldis;
+    }
+    dim3 gfull(nblk(c->km, 128), c->jm, c->im), b128(128);
+    hipLaunchKernelGGL(k_turb_clamp_in<T>, gfull, b128, 0, c->stream, tke, dis, ltke, ldis,
+        (const uint8_t*)c->flags.p, c->im, c->jm, c->km, c->pitch, (T)dis_min);
+    c->turb_live_pending = true;
+    TurbConsts tc{ c->g.dr, c->g.dthe, c->g.dphi, nue_max, dis_min };
+    hipLaunchKernelGGL(k_turb_sources<T>, dim3(nblk(c->km - 1, 128), c->jm - 2, c->im - 2), b128, 0, c->stream,
+        ltke, ldis, F<T>(c, ATJUP_F_U), F<T>(c, ATJUP_F_V), F<T>(c, ATJUP_F_W),
+        F<T>(c, ATJUP_F_NUE), F<T>(c, ATJUP_F_PROD), F<T>(c, ATJUP_F_TKE_SOURCE), F<T>(c, ATJUP_F_DIS_SOURCE),
+        (const uint8_t*)c->flags.p, (const double*)c->tabs.p, c->im, c->jm, c->km, c->pitch, c->plane, tc);
+    const TurbArrays<T> L = turb_arrays<T>(c, ltke, ldis);
+    hipLaunchKernelGGL(k_turb_wall_plane<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream,
+        L.a[0], L.a[1], L.a[2], L.a[3], L.a[4], L.a[5], (const uint8_t*)c->flags.p,
+        c->im, c->jm, c->km, c->pitch, c->plane, (T)k_bg, (T)dis_min, (T)nue_max, 1);
+    hipLaunchKernelGGL(k_turb_post<T>, gfull, b128, 0, c->stream,
+        L.a[0], L.a[1], L.a[2], L.a[3], L.a[4], L.a[5], (const uint8_t*)c->flags.p,
+        c->im, c->jm, c->km, c->pitch, (T)k_bg, (T)dis_min, (T)nue_max);
+    hipLaunchKernelGGL(k_turb_seam<T>, dim3(nblk(c->jm, 64), c->im), dim3(64), 0, c->stream, L, c->im, c->jm, c->km, c->pitch);
+    hipLaunchKernelGGL(k_turb_lid<T>, dim3(nblk(c->km, 128), c->jm - 2), b128, 0, c->stream, L, c->im, c->jm, c->km,
+        c->pitch, c->plane);
+    hipLaunchKernelGGL(k_turb_pole_rows<T>, dim3(nblk(c->km, 128), c->im), b128, 0, c->stream, L, c->im, c->jm, c->km,
+        c->pitch);
+    const double* tabs = (const double*)c->tabs.p;
+    hipLaunchKernelGGL(k_turb_wall_omega<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream, ldis,
+        (const int*)c->itopo.p, (const double*)c->vel_star.p, tabs + c->im + 3 * c->jm,
+        c->im, c->jm, c->km, c->pitch, L_m / C.u_0);
+    return check_launch();
+}
+
+template<typename T>
+int do_turb_wall(atjup_ctx* c){
+    T* tke = LIVE<T>(c, ATJUP_F_TKE);
+    T* dis = LIVE<T>(c, ATJUP_F_DIS);
+    const TurbArrays<T> L = turb_arrays<T>(c, tke, dis);
+    dim3 b128(128);
+    hipLaunchKernelGGL(k_turb_wall_plane<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream,
+        L.a[0], L.a[1], L.a[2], L.a[3], L.a[4], L.a[5], (const uint8_t*)c->flags.p,
+        c->im, c->jm, c->km, c->pitch, c->plane, T(0), T(0), T(0), 0);
+    const double* tabs = (const double*)c->tabs.p;
+    hipLaunchKernelGGL(k_turb_wall_omega<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream, dis,
+        (const int*)c->itopo.p, (const double*)c->vel_star.p, tabs + c->im + 3 * c->jm,
+        c->im, c->jm, c->km, c->pitch, c->g.L_atm * 1.0e3 / c->c.u_0);
+    return check_launch();
+}
+
+template<typename T>
+int do_turb_sanitize(atjup_ctx* c){
+    const double tke_max = 1000.0 / (c->c.u_0 * c->c.u_0);
+    const double nue_max = 1.0e5 / (c->c.u_0 * c->g.L_atm * 1.0e3);
+    hipLaunchKernelGGL(k_turb_sanitize<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream,
+        LIVE<T>(c, ATJUP_F_TKE), LIVE<T>(c, ATJUP_F_DIS), F<T>(c, ATJUP_F_NUE), F<T>(c, ATJUP_F_PROD),
+        F<T>(c, ATJUP_F_TKE_SOURCE), F<T>(c, ATJUP_F_DIS_SOURCE), c->n3, (T)tke_max, (T)nue_max);
+    return check_launch();
+}
+
+template<typename T>
+int do_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){
+    static const int ids[3][3] = { { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE },
+                                   { ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE },
+                                   { ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE } };
+    SatParams P{ g.t_0, g.t_00, g.ep, g.lv, g.ls, g.C, g.L0, g.R, g.del_alf, g.del_bet,
+                 g.C_i, g.L0_i, g.del_alf_i, g.del_bet_i, c->c.cp_mix, c->c.t_ref, c->c.r_mix };
+    hipLaunchKernelGGL(k_satadj<T>, dim3(nblk(c->km, 128), c->jm, c->im), dim3(128), 0, c->stream,
+        F<T>(c, ids[gas][0]), F<T>(c, ids[gas][1]), F<T>(c, ids[gas][2]), F<T>(c, ATJUP_F_T),
+        F<T>(c, ATJUP_F_P_STAT), F<T>(c, ATJUP_F_RHO_MIX), (const uint8_t*)c->flags.p,
+        c->jm, c->km, c->pitch, P);
+    return check_launch();
+}
+
+template<typename T>
+int do_flux_limiter(atjup_ctx* c){
+    hipLaunchKernelGGL(k_flux_limiter<T>, dim3(nblk(c->km - 1, 128), c->jm - 2, c->im - 2), dim3(128), 0, c->stream,
+        F<T>(c, ATJUP_F_NH4SH), F<T>(c, ATJUP_F_U), F<T>(c, ATJUP_F_V), F<T>(c, ATJUP_F_W),
+        F<T>(c, ATJUP_F_FLUXLIM_NH4SH), (const double*)c->tabs.p, c->im, c->jm, c->km, c->pitch, c->plane,
+        c->g.dr, c->g.dthe, c->g.dphi, 4.0 / 3.0, 1.0 / 3.0);
+    return check_launch();
+}
+
+template<typename T>
+int do_commit_live(atjup_ctx* c){
+    if(!c->turb_live_pending) return 0;
+    for(int t = 0; t < 2; t++)
+        ATJUP_HIP_CHECK(hipMemcpyAsync(c->cur[ATJUP_F_TKE + t].p, c->live_turb[t].p, c->cur[ATJUP_F_TKE + t].bytes,
+                                       hipMemcpyDeviceToDevice, c->stream));
+    c->turb_live_pending = false;
+    return 0;
+}
+
+}  // namespace
+
+#define DISPATCH(fn, ...) ((c->precision == ATJUP_PRECISION_FP64) ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+int launch_rk4_full(atjup_ctx* c){ return DISPATCH(do_rk4_full, c); }
+int launch_turbulence(atjup_ctx* c){ return DISPATCH(do_turbulence, c); }
+int launch_turb_wall(atjup_ctx* c){ return DISPATCH(do_turb_wall, c); }
+int launch_turb_sanitize(atjup_ctx* c){ return DISPATCH(do_turb_sanitize, c); }
+int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){ return DISPATCH(do_satadj, c, gas, g); }
+int launch_flux_limiter(atjup_ctx* c){ return DISPATCH(do_flux_limiter, c); }
+int launch_commit_live(atjup_ctx* c){ return DISPATCH(do_commit_live, c); }
+
+}  // namespace atjup

@@ -187,6 +187,9 @@ class Context:
     def swap_nlevel(self):
         self._chk(self.L.atjup_hip_swap_nlevel(self.h), "swap_nlevel")
 
+    def scalar_sync(self):
+        self._chk(self.L.atjup_hip_scalar_sync(self.h), "scalar_sync")
+
     def step(self, iter_n, first=False):
         self._chk(self.L.atjup_hip_step(self.h, iter_n, 1 if first else 0), "step")
 

@@ -30,6 +30,16 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s FP32-refined vs FP64 at <=1e-5 rel err; adjoint/forward time"
 INTERIOR_CELLS = 39 * 175 * 360          # RK4 pass A range (RungeKutta_Jup_Turb.cpp:388-412)
 HBM_PEAK_GBS = 8000.0
+WORKLOAD = {
+    "dyn": "dynamics-only FP64 time step",
+    "full": "full-physics FP64 time step (k-omega closure, saturation adjustment of H2O/NH3/CH4, NH4SH "
+            "flux limiter, 17 prognostic fields)",
+}
+DATA = {
+    "dyn": "synthetic (seeded 12-jet zonal wind, parabolic T with 110 K clamp, ellipsoid obstacle)",
+    "full": "synthetic (seeded 12-jet zonal wind, parabolic T with 110 K clamp, ellipsoid obstacle, vapour at "
+            "seeded 0.6-1.25 x saturation with condensate seeds, k=1e-4 m2/s2, omega=1e-3 1/s)",
+}
 
 
 def poisson_launch_bytes(g, esz_p=8, omega=1.0):
@@ -47,11 +57,24 @@ def poisson_launch_bytes(g, esz_p=8, omega=1.0):
     return n * per_cell
 
 
-def step_bytes_per_cell_update(esz, sweeps, solves_per_step, n_prog=4, n_in=5):
-    """Algorithmic bytes per cell-update of the whole step (the accounting of the design notes)."""
-    rk = esz * (17 * n_prog + 4 * n_in + 5) + 4 * 1
+def step_bytes_per_cell_update(config, esz, sweeps, solves_per_step):
+    """Algorithmic HBM bytes per cell-update of the whole step.
+
+    RK4: every prognostic field is streamed 17 times a step (stage 0: input, accumulator write,
+    output; stages 1-2: input, n level, accumulator read and write, output; stage 3: input, n level,
+    accumulator read, output), every read-only input once a stage, plus the aux / closure outputs
+    and the 1-byte face flags. Pressure solve: per call the source build (4 fields) and 3 field
+    passes per sweep (1.5 per colour: the other colour, the own colour's source and value). Physics
+    block (full profile, per call): mixture density 4, saturation adjustment 3 x 10, closure 25,
+    limiter 5, the two Shapiro passes 12 field streams.
+    """
+    if config == "dyn":
+        n_prog, n_in, n_out, phys = 4, 5, 5, 4 + 6
+    else:
+        n_prog, n_in, n_out, phys = 17, 22, 8, 4 + 6 + 30 + 25 + 5 + 6
+    rk = esz * (17 * n_prog + 4 * n_in + n_out) + 4 * 1
     pois = 8 * (4 + 3 * sweeps) * solves_per_step
-    return rk + pois
+    return rk + pois + esz * phys * solves_per_step
 
 
 def cpu_baseline(config, sweeps, steps=20):
@@ -92,7 +115,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="dyn", choices=["dyn"])
+    ap.add_argument("--config", default="dyn", choices=["dyn", "full"])
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--sweeps", type=int, default=200)
     ap.add_argument("--omega", type=float, default=1.0)
@@ -150,7 +173,7 @@ def main():
     lb = poisson_launch_bytes(g, 8, args.omega)
     achieved = lb / (launch_ms * 1e-3) / 1e9
     solves_per_step = 3.0 / 5.0
-    step_b = step_bytes_per_cell_update(8, args.sweeps, solves_per_step)
+    step_b = step_bytes_per_cell_update(args.config, 8, args.sweeps, solves_per_step)
 
     cpu = None
     if not args.no_cpu:
@@ -171,9 +194,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded 12-jet zonal wind, parabolic T with 110 K clamp, ellipsoid obstacle)",
+        "data": DATA[args.config],
         "config": {
-            "workload": "dynamics-only FP64 time step, 41x181x361 shell, RK4 + red-black Poisson "
+            "workload": WORKLOAD[args.config] + f", 41x181x361 shell, RK4 + red-black Poisson "
                         f"({args.sweeps} sweeps, omega={args.omega}) on iterations 1,2,4 of each 5-step window",
             "grid": [g.im, g.jm, g.km], "sweeps": args.sweeps, "omega": args.omega,
             "precision": args.precision, "finite": bool(ok),

@@ -1,0 +1,139 @@
+"""Parity of the HIP FP64 path with the CPU oracle, full-physics profile.
+
+The moist, turbulent case: k-omega closure with its wall function, the saturation adjustment of
+H2O, NH3 and CH4, the NH4SH flux limiter and the RK4 step of all 17 prognostic fields, on the seeded
+moist state (vapour at 0.6-1.25 x saturation, condensate seeds, NH4SH and H2S traces) with the
+ellipsoid obstacle. The oracle reproduces the compiled reference bit for bit
+(test_oracle_vs_reference.py); here the HIP library is held to it.
+
+Operators without transcendental functions are expected to agree bit for bit. The closure uses
+pow and the adjustment exp and log, which the device library evaluates within an ulp of the host's,
+so their outputs carry ulp-level differences; the bar is the drop-in's 1e-10 relative to each
+field's maximum, with the obstacle mask and the cloud-base index of every column bit-exact.
+"""
+import numpy as np
+import pytest
+
+from atjup_amd import abi, inputs
+from tests.helpers import compare, make_pair, rel_err, small_state
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1.0e-10
+PROG = abi.PROG
+TURB_OUT = ("nue", "prod", "tke_source", "dis_source")
+CLOUDS = ("h2o_cloud", "h2o_ice", "nh3_cloud", "nh3_ice", "ch4_cloud", "ch4_ice")
+
+
+def cloud_base(a):
+    """Per (j, k) column the lowest level with condensate, -1 where there is none."""
+    has = a > 0.0
+    idx = np.argmax(has, axis=0)
+    return np.where(has.any(axis=0), idx, -1)
+
+
+@pytest.fixture(scope="module")
+def pair():
+    st = inputs.make_state("full", seed=7)
+    gpu, orc, st = make_pair("full", sweeps=20, state=st)
+    yield gpu, orc, st
+    gpu.close()
+    orc.close()
+
+
+def test_setup_bit_exact(pair):
+    gpu, orc, _ = pair
+    for n in PROG + ("p_stat", "p_dyn", "wall_nue"):
+        assert np.array_equal(gpu.get(n), orc.get(n)), n
+
+
+def test_mixture_and_saturation_adjustment(pair):
+    gpu, orc, _ = pair
+    gpu.mixture_density()
+    orc.call("mixture_density")
+    assert np.array_equal(gpu.get("rho_mix"), orc.get("rho_mix"))
+    before = orc.get("h2o_cloud")
+    for gas in range(3):
+        gpu.satadj(gas)
+        orc.call("satadj", gas)
+    # RK_SCALAR_SYNC 2 follows the adjustments in the loop body: the n-level copies take the
+    # adjusted species and temperature (the identity on the device, where they share storage)
+    gpu.scalar_sync()
+    orc.call("scalar_sync")
+    assert np.count_nonzero(orc.get("h2o_cloud") != before) > 1000      # the adjustment acted
+    names = ("t",) + tuple(n for g in abi.GAS_FIELDS.values() for n in g)
+    w = compare(gpu, orc, names, TOL, "satadj")
+    print("satadj parity", {n: f"{e:.2e}/{c}" for n, (e, c) in w.items()})
+    for n in CLOUDS:
+        assert np.array_equal(cloud_base(gpu.get(n)), cloud_base(orc.get(n))), n
+
+
+def test_turbulence_run(pair):
+    gpu, orc, _ = pair
+    gpu.turbulence_run()
+    orc.call("turbulence_run")
+    w = compare(gpu, orc, ("tke", "dis") + TURB_OUT, TOL, "turbulence")
+    print("turbulence parity", {n: f"{e:.2e}/{c}" for n, (e, c) in w.items()})
+    # the n-level copies are untouched in the interior until the step integrates from them
+    assert np.count_nonzero(orc.get("prod")) > 1000
+
+
+def test_flux_limiter_bit_exact(pair):
+    gpu, orc, _ = pair
+    gpu.flux_limiter()
+    orc.call("flux_limiter")
+    assert np.array_equal(gpu.get("fluxlim_nh4sh"), orc.get("fluxlim_nh4sh"))
+    assert np.count_nonzero(orc.get("fluxlim_nh4sh")) > 1000
+    gpu.shapiro("fluxlim_nh4sh", 2, 1.0, 1, 7)
+    orc.call("shapiro", abi.FIELD["fluxlim_nh4sh"], 2, 1.0, 1, 7)
+    assert np.array_equal(gpu.get("fluxlim_nh4sh"), orc.get("fluxlim_nh4sh"))
+
+
+def test_rk4_step_after_the_closure(pair):
+    gpu, orc, _ = pair
+    gpu.rk4_step()
+    orc.call("rk4_step")
+    np.testing.assert_allclose(gpu.buoy_ref_level(), orc.buoy_ref_level(), rtol=1e-11, atol=0)
+    w = compare(gpu, orc, PROG + ("aux_u", "aux_v", "aux_w") + TURB_OUT, TOL, "rk4")
+    print("rk4 parity", {n: f"{e:.2e}/{c}" for n, (e, c) in w.items()})
+
+
+def test_boundaries_wall_hold_and_restore(pair):
+    gpu, orc, _ = pair
+    gpu.bc_after_step()
+    orc.call("bc_after_step")
+    gpu.turb_wall()
+    orc.call("turb_wall")
+    gpu.clamp_species()
+    orc.call("clamp_species")
+    gpu.swap_nlevel()
+    orc.call("swap_nlevel")
+    w = compare(gpu, orc, PROG + TURB_OUT, TOL, "bc")
+    print("bc parity", {n: f"{e:.2e}/{c}" for n, (e, c) in w.items()})
+
+
+def test_full_steps():
+    """Five iterations of the loop (three 20-sweep solves) from a fresh moist state."""
+    st = inputs.make_state("full", seed=21)
+    gpu, orc, _ = make_pair("full", sweeps=20, state=st)
+    for it in range(1, 6):
+        gpu.step(it, it == 1)
+        orc.call("step", it, 1 if it == 1 else 0)
+    w = compare(gpu, orc, PROG + ("p_dyn", "p_hydro", "rho_mix") + TURB_OUT, TOL, "5 steps")
+    print("5-step parity", {n: f"{e:.2e}/{c}" for n, (e, c) in w.items()})
+    for n in CLOUDS:
+        assert np.array_equal(cloud_base(gpu.get(n)), cloud_base(orc.get(n))), n
+    assert gpu.check_finite()
+    gpu.close()
+    orc.close()
+
+
+def test_small_grid_full_steps():
+    st = small_state("full", seed=5)
+    gpu, orc, _ = make_pair("full", sweeps=10, state=st)
+    for it in range(1, 4):
+        gpu.step(it, it == 1)
+        orc.call("step", it, 1 if it == 1 else 0)
+    compare(gpu, orc, PROG + ("p_dyn",) + TURB_OUT, TOL, "small grid")
+    gpu.close()
+    orc.close()

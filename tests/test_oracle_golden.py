@@ -57,3 +57,32 @@ def test_planted_mistake_is_caught(gold):
     sel = (gold["i"], gold["j"], gold["k"])
     assert not np.array_equal(o.get("u")[sel], gold["s1_u"])
     o.close()
+
+
+GOLD_FULL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "full_reference.npz")
+
+
+def test_oracle_reproduces_full_physics_reference_bit_for_bit():
+    """The moist case: k-omega closure with its wall function, the three saturation adjustments,
+    the NH4SH flux limiter and the 17-field RK4, two steps of 10-sweep solves."""
+    if not os.path.exists(GOLD_FULL):
+        pytest.skip("golden fixtures absent")
+    gold = np.load(GOLD_FULL, allow_pickle=False)
+    import oracle
+    G, Cn, K, _ = abi.make_profile("full", sweeps=int(gold["sweeps"]))
+    st = inputs.make_state("full", seed=int(gold["seed"]))
+    o = oracle.Oracle(G, Cn, K)
+    o.load_state(st)
+    o.call("bc_after_step")
+    o.call("swap_nlevel")
+    sel = (gold["i"], gold["j"], gold["k"])
+    names = sorted({k[3:] for k in gold.files if k.startswith("s1_") and not k.endswith("_levelsum")} - {"brl"})
+    assert "h2o_cloud" in names and "dis" in names and "fluxlim_nh4sh" in names
+    for it in range(1, int(gold["steps"]) + 1):
+        o.call("step", it, 1 if it == 1 else 0)
+        for n in names:
+            a = o.get(n)
+            np.testing.assert_array_equal(a[sel], gold[f"s{it}_{n}"], err_msg=f"step {it} {n}")
+            np.testing.assert_array_equal(a.sum(axis=(1, 2)), gold[f"s{it}_{n}_levelsum"],
+                                          err_msg=f"step {it} {n} level sums")
+    o.close()
