@@ -28,7 +28,7 @@ FIELD.update({n + "n": NLEVEL + i for i, n in enumerate(PROG)})   # tn, un, ...,
 F_END = 20 + len(INPUTS)
 
 OK, ERR_ARG, ERR_HIP, ERR_PROFILE, ERR_NOMEM, ERR_STATE, NONFINITE = 0, -1, -2, -3, -4, -5, 1
-PRECISION_FP64, PRECISION_FP32_REF = 0, 1
+PRECISION_FP64, PRECISION_FP32_REF, PRECISION_TANGENT = 0, 1, 2
 
 
 class Grid(C.Structure):

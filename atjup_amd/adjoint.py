@@ -1,0 +1,86 @@
+"""Gradients of the FP64 dynamics step through the hand-written adjoint, as a torch.autograd.Function.
+
+The loss of the adjoint benchmark: the cos(latitude)-weighted squared misfit of the zonal-mean
+zonal wind at the top integrated level against a target jet profile,
+
+    L(w0) = sum_j cos(lat_j) (mean_k w_N[i_top, j, k] - target_j)^2,   k over the ring 0..km-2,
+
+after N loop iterations from an initial state whose zonal wind w (the reference's phi component)
+is w0. ``JetMisfit.apply(w0, runner)`` returns L as a torch scalar; its backward runs the
+checkpointed reverse sweep (atjup_hip_adj_backward) and returns dL/dw0.
+
+torch is used on the CPU only: the device work runs in the library's own HIP runtime, and the
+tensors cross as host arrays (the torch wheel's bundled HIP runtime is a different one).
+
+The obstacle is the reference's binary mask, whose derivative with respect to its semi-axes is zero
+almost everywhere; a gradient with respect to the semi-axes needs a differentiable obstacle model
+the reference does not have, so none is offered here.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import abi
+
+try:  # torch is optional for the numpy interface
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+
+class MisfitRunner:
+    """One FP64 context holding the initial state, and the loss definition."""
+
+    def __init__(self, ctx, init_state: dict, target, nsteps=20, every=5, i_top=None):
+        self.ctx = ctx
+        self.init = {n: np.array(v, dtype=np.float64) for n, v in init_state.items()}
+        im, jm, km = ctx.shape
+        self.target = np.asarray(target, dtype=np.float64)
+        self.i_top = im - 2 if i_top is None else i_top
+        self.nsteps, self.every = nsteps, every
+        _, theta, _ = _coords(ctx)
+        self.cos_lat = np.abs(np.sin(theta))          # theta is the colatitude
+        ctx.adj_init()
+
+    def loss_final(self):
+        w = self.ctx.get("w")
+        km = w.shape[2]
+        r = w[self.i_top, :, :-1].mean(axis=1) - self.target
+        lam = np.zeros((4,) + w.shape)
+        lam[3, self.i_top, :, :-1] = (2.0 * self.cos_lat * r / (km - 1))[:, None]
+        return float(np.sum(self.cos_lat * r * r)), lam
+
+    def forward(self, w0):
+        for n, v in self.init.items():
+            self.ctx.set(n, w0 if n == "w" else v)
+        self.ctx.adj_forward(self.nsteps, self.every, 1)
+        loss, self._lam = self.loss_final()
+        return loss
+
+    def backward(self, scale=1.0):
+        return self.ctx.adj_backward(scale * self._lam)[3]
+
+
+def _coords(ctx):
+    """The colatitudes, cumulative sums of dthe from 0 as the reference forms them."""
+    g = ctx.grid
+    the = np.zeros(g.jm)
+    for j in range(1, g.jm):
+        the[j] = the[j - 1] + g.dthe
+    return None, the, None
+
+
+if torch is not None:
+    class JetMisfit(torch.autograd.Function):
+        """L(w0) with the adjoint as its backward (FP64, CPU tensors)."""
+
+        @staticmethod
+        def forward(fctx, w0, runner: MisfitRunner):
+            loss = runner.forward(w0.detach().cpu().numpy().astype(np.float64))
+            fctx.runner = runner
+            return torch.tensor(loss, dtype=torch.float64)
+
+        @staticmethod
+        def backward(fctx, grad_out):
+            g = fctx.runner.backward(float(grad_out))
+            return torch.from_numpy(g), None

@@ -1,0 +1,1233 @@
+// Hand-written adjoint of the FP64 dynamics step (the dry profile): the transposes of the boundary
+// passes, the Shapiro filter on p_dyn, the mixture density, the hydrostatic column integral, the
+// per-level buoyancy mean, the RK4 step with its right-hand side, and the red-black pressure
+// solve, each the exact transpose of the linearised forward operator at the forward state held in
+// the context. Branches (the source clamp, the validity tests, the solid-face mirrors) are frozen
+// at the forward's choice. Adjoint fields live beside the forward fields in the same layout; the
+// scatter of a cell's stencil adjoint into its neighbours uses FP64 atomic adds.
+//
+// Field order and formulas follow the forward kernels of atjup_kernels.hip, which restate
+// RHS_Jup_Turb.cpp:31-1457 (dry: t, u, v, w), RungeKutta_Jup_Turb.cpp:47-583, PressureSolver.h:
+// 145-766, lib/Utils.cpp:221-453, BC_Jup.h:185-658 and BoundaryConditions.h:151-518.
+#include "atjup_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace atjup {
+
+__device__ __forceinline__ void aadd(double* p, double v){ if(v != 0.0) unsafeAtomicAdd(p, v); }
+
+// =============================================================================================
+// boundary passes (bcRadius, bcTheta, bcPhi, bcSolidGround, MOUNT_T_AMBIENT), transposed
+// =============================================================================================
+// MOUNT_T_AMBIENT^T: the solid cells of a row carried the row's fluid mean
+__global__ __launch_bounds__(64) void ka_t_ambient(double* __restrict__ lt, const uint8_t* __restrict__ msk,
+        const int2* __restrict__ rows, int n, int jm, int km, int pitch){
+    const int r = blockIdx.x;
+    if(r >= n) return;
+    const int2 ij = rows[r];
+    const size_t r0 = ((size_t)ij.x * jm + ij.y) * pitch;
+    __shared__ double s_sum;
+    __shared__ int s_cnt;
+    if(threadIdx.x == 0){
+        double s = 0.0; int cnt = 0;
+        for(int k = 0; k < km; k++){
+            if(msk[r0 + k]) s += lt[r0 + k];
+            else cnt++;
+        }
+        s_sum = s; s_cnt = cnt;
+    }
+    __syncthreads();
+    if(s_cnt == 0) return;
+    const double share = s_sum / s_cnt;
+    for(int k = threadIdx.x; k < km; k += 64){
+        if(msk[r0 + k]) lt[r0 + k] = 0.0;
+        else lt[r0 + k] += share;
+    }
+}
+
+// bcSolidGround^T over the solid-cell list: velocities zeroed; t and p_dyn extrapolated from a
+// fluid pair (code 1..6) or, for p_dyn, zeroed (code 0)
+__global__ void ka_bc_solid(double* __restrict__ lu, double* __restrict__ lv, double* __restrict__ lw,
+        double* __restrict__ lt, double* __restrict__ lp, const SolidCell* __restrict__ cells, int n,
+        int jm, int pitch, size_t plane, double c43, double c13){
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if(s >= n) return;
+    const SolidCell c = cells[s];
+    const size_t q = ((size_t)c.i * jm + c.j) * pitch + c.k;
+    lu[q] = 0.0; lv[q] = 0.0; lw[q] = 0.0;
+    const long R = pitch, P = (long)plane;
+    long da = 0;
+    switch(c.code){
+        case 1: da = P; break;  case 2: da = -P; break;
+        case 3: da = R; break;  case 4: da = -R; break;
+        case 5: da = 1; break;  case 6: da = -1; break;
+        default: da = 0; break;
+    }
+    if(da == 0){ lp[q] = 0.0; return; }
+    double* X[2] = { lt, lp };
+    for(int f = 0; f < 2; f++){
+        const double l = X[f][q];
+        X[f][q] = 0.0;
+        aadd(&X[f][q + da], c43 * l);
+        aadd(&X[f][q + 2 * da], -c13 * l);
+    }
+}
+
+// bcPhi^T: the seam meridian was the copy of k = 0
+__global__ void ka_bc_phi(double* __restrict__ a0, double* __restrict__ a1, double* __restrict__ a2,
+        double* __restrict__ a3, int jm, int km, int pitch){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(j >= jm) return;
+    const size_t r0 = ((size_t)i * jm + j) * pitch;
+    double* X[4] = { a0, a1, a2, a3 };
+    for(int f = 0; f < 4; f++){
+        X[f][r0] += X[f][r0 + km - 1];
+        X[f][r0 + km - 1] = 0.0;
+    }
+}
+
+// bcTheta^T: t, u were copied from j = 3 (jm-4) into rows 0..2 (jm-3..jm-1); v, w ramped 1/3, 2/3
+// into rows 1, 2 and zeroed at the pole row
+__global__ void ka_bc_theta(double* __restrict__ lt, double* __restrict__ lu, double* __restrict__ lv,
+        double* __restrict__ lw, int jm, int km, int pitch){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(k >= km) return;
+    const size_t R = (size_t)pitch;
+    const size_t qs = (size_t)i * jm * pitch + k;
+    const size_t qn = qs + (size_t)(jm - 1) * R;
+    double* ex[2] = { lt, lu };
+    for(int f = 0; f < 2; f++){
+        double* X = ex[f];
+        X[qs + 3 * R] += X[qs] + X[qs + R] + X[qs + 2 * R];
+        X[qs] = X[qs + R] = X[qs + 2 * R] = 0.0;
+        X[qn - 3 * R] += X[qn] + X[qn - R] + X[qn - 2 * R];
+        X[qn] = X[qn - R] = X[qn - 2 * R] = 0.0;
+    }
+    double* z[2] = { lv, lw };
+    for(int f = 0; f < 2; f++){
+        double* X = z[f];
+        X[qs + 3 * R] += (1.0 / 3.0) * X[qs + R] + (2.0 / 3.0) * X[qs + 2 * R];
+        X[qs] = X[qs + R] = X[qs + 2 * R] = 0.0;
+        X[qn - 3 * R] += (1.0 / 3.0) * X[qn - R] + (2.0 / 3.0) * X[qn - 2 * R];
+        X[qn] = X[qn - R] = X[qn - 2 * R] = 0.0;
+    }
+}
+
+// bcRadius^T, first the zonal sums the top sponge's means gather their adjoint from:
+// zs[0][j] = sum_k (lV[im-1] + 2/3 lV[im-2]), zs[1][j] = sum_k lV[im-3] / 3, the same for w at 2, 3
+__global__ void ka_bc_zonal_sums(const double* __restrict__ lv, const double* __restrict__ lw, double* __restrict__ zs,
+        int im, int jm, int km, int pitch, size_t plane){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, sel = blockIdx.y;
+    if(j >= jm) return;
+    const double* X = (sel & 2) ? lw : lv;
+    const size_t r1 = ((size_t)(im - 1) * jm + j) * pitch;
+    double s = 0.0;
+    if((sel & 1) == 0){
+        for(int k = 0; k < km; k++) s += X[r1 + k] + (2.0 / 3.0) * X[r1 - plane + k];
+    } else {
+        for(int k = 0; k < km; k++) s += X[r1 - 2 * plane + k] * (1.0 / 3.0);
+    }
+    zs[(size_t)sel * jm + j] = s;
+}
+
+__global__ void ka_bc_radius(double* __restrict__ lt, double* __restrict__ lu, double* __restrict__ lv,
+        double* __restrict__ lw, const double* __restrict__ zs, int im, int jm, int km, int pitch, size_t plane,
+        double c43, double c13){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const size_t q0 = (size_t)j * pitch + k;
+    const size_t qt = q0 + (size_t)(im - 1) * plane;
+    // t: 2-point Neumann at both radial walls
+    lt[q0 + plane] += c43 * lt[q0];
+    lt[q0 + 2 * plane] += -c13 * lt[q0];
+    lt[q0] = 0.0;
+    lt[qt - plane] += c43 * lt[qt];
+    lt[qt - 2 * plane] += -c13 * lt[qt];
+    lt[qt] = 0.0;
+    // u: zero at both walls
+    lu[q0] = 0.0;
+    lu[qt] = 0.0;
+    // v, w: Neumann at the bottom, the top sponge with its zonal means
+    const double inv_n = 1.0 / (km - 1);
+    double* X[2] = { lv, lw };
+    for(int f = 0; f < 2; f++){
+        double* V = X[f];
+        V[q0 + plane] += c43 * V[q0];
+        V[q0 + 2 * plane] += -c13 * V[q0];
+        V[q0] = 0.0;
+        const double s2 = zs[(size_t)(2 * f) * jm + j], s3 = zs[(size_t)(2 * f + 1) * jm + j];
+        const double l2 = V[qt - plane], l3 = V[qt - 2 * plane];
+        V[qt - plane] = (1.0 / 3.0) * l2 + ((k < km - 1) ? s2 * inv_n : 0.0);
+        V[qt - 2 * plane] = (2.0 / 3.0) * l3 + ((k < km - 1) ? s3 * inv_n : 0.0);
+        V[qt] = 0.0;
+    }
+}
+
+// =============================================================================================
+// Shapiro order 2 on one field (k pass, then j, then i), transposed pass by pass
+// =============================================================================================
+__global__ __launch_bounds__(256) void ka_shapiro_i(const double* __restrict__ lo, double* __restrict__ li,
+        const int* __restrict__ itopo, int im, int jm, int km, int pitch, size_t plane, double co){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, n = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)n * jm + j) * pitch + k;
+    const int i_surf = max(itopo[(size_t)j * km + k], 0);
+    if(n < i_surf){ li[q] = lo[q]; return; }
+    double self = 1.0 - 2.0 * co;
+    if(n == i_surf) self += co;                  // the lower neighbour of the first fluid cell is itself
+    if(n == im - 1) self += co;                  // the upper neighbour of the top cell is itself
+    double acc = self * lo[q];
+    if(n + 1 <= im - 1) acc += co * lo[q + plane];
+    if(n - 1 >= i_surf) acc += co * lo[q - plane];
+    li[q] = acc;
+}
+
+__global__ __launch_bounds__(256) void ka_shapiro_j(const double* __restrict__ lo, double* __restrict__ li,
+        const int* __restrict__ itopo, int im, int jm, int km, int pitch, double co){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t R = (size_t)pitch;
+    const size_t q = ((size_t)i * jm + n) * pitch + k;
+    auto fl = [&](int jj){ return i >= max(itopo[(size_t)jj * km + k], 0); };
+    if(!fl(n)){ li[q] = lo[q]; return; }
+    const int jm1 = max(n - 1, 0), jp1 = min(n + 1, jm - 1);
+    double self = 1.0 - 2.0 * co;
+    if(!fl(jm1) || jm1 == n) self += co;
+    if(!fl(jp1) || jp1 == n) self += co;
+    double acc = self * lo[q];
+    if(n + 1 <= jm - 1 && fl(n + 1)) acc += co * lo[q + R];
+    if(n - 1 >= 0 && fl(n - 1)) acc += co * lo[q - R];
+    li[q] = acc;
+}
+
+__global__ __launch_bounds__(256) void ka_shapiro_k(const double* __restrict__ lo, double* __restrict__ li,
+        const int* __restrict__ itopo, int im, int jm, int km, int pitch, double co){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t r0 = ((size_t)i * jm + j) * pitch;
+    if(k == km - 1){ li[r0 + k] = 0.0; return; }      // the seam input is not read
+    const int* it = itopo + (size_t)j * km;
+    auto fl = [&](int kx){ return i >= max(it[kx], 0); };
+    // the output at the seam is the copy of the output at k = 0
+    auto L = [&](int kx){ return lo[r0 + kx] + ((kx == 0) ? lo[r0 + km - 1] : 0.0); };
+    const int km1 = (k == 0) ? km - 2 : k - 1, kp1 = (k == km - 2) ? 0 : k + 1;
+    if(!fl(k)){ li[r0 + k] = L(k); return; }
+    double self = 1.0 - 2.0 * co;
+    if(!fl(km1)) self += co;
+    if(!fl(kp1)) self += co;
+    double acc = self * L(k);
+    if(fl(kp1)) acc += co * L(kp1);
+    if(fl(km1)) acc += co * L(km1);
+    li[r0 + k] = acc;
+}
+
+// =============================================================================================
+// computeMixtureDensity^T and the hydrostatic column integral^T
+// =============================================================================================
+__global__ __launch_bounds__(256) void ka_mixture(const double* __restrict__ t, const double* __restrict__ ps,
+        const double* __restrict__ pd, double* __restrict__ lrho, double* __restrict__ lt, double* __restrict__ lp,
+        size_t n, double t_ref, double pdyn_w, double ptb, double R_mix){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= n) return;
+    const double l = lrho[q];
+    lrho[q] = 0.0;
+    if(l == 0.0) return;
+    const double T_ = t[q] * t_ref;
+    if(!(T_ > 0.0)) return;
+    const double p_pa = (ps[q] + pdyn_w * pd[q] * ptb) * 1.0e5;
+    const double r = p_pa / (R_mix * T_);
+    if(!isfinite(r)) return;
+    lt[q] += l * (-r / t[q]);
+    lp[q] += l * (pdyn_w * ptb * 1.0e5 / (R_mix * T_));
+}
+
+__global__ __launch_bounds__(256) void ka_hydrostatic(const double* __restrict__ v, const double* __restrict__ w,
+        const uint8_t* __restrict__ flags, const double* __restrict__ rad, const double* __restrict__ cor_j,
+        double* __restrict__ lph, double* __restrict__ lv, double* __restrict__ lw,
+        int im, int jm, int km, int pitch, size_t plane, double dr){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const size_t q0 = (size_t)j * pitch + k;
+    int i0 = 0;
+    while(i0 < im && (flags[q0 + i0 * plane] & FL_SELF)) i0++;
+    if(i0 >= im){ for(int i = 0; i < im; i++) lph[q0 + i * plane] = 0.0; return; }
+    int i1 = i0;
+    while(i1 + 1 < im && !(flags[q0 + (i1 + 1) * plane] & FL_SELF)) i1++;
+    const double cor_ = cor_j[j];
+    double G = 0.0, lf_next = 0.0;
+    for(int i = i1; i >= i0; i--){
+        // f(i) receives 0.5 dr G_i from step i and 0.5 dr G_{i+1} from step i+1
+        double lf = lf_next;
+        if(i > i0){
+            G += lph[q0 + i * plane];
+            lf += 0.5 * dr * G;
+            lf_next = 0.5 * dr * G;
+        } else {
+            lf_next = 0.0;
+        }
+        const double vv = v[q0 + i * plane], ww = w[q0 + i * plane];
+        lv[q0 + i * plane] += lf * (2.0 * vv / rad[i]);
+        lw[q0 + i * plane] += lf * (2.0 * ww / rad[i] + cor_);
+    }
+    for(int i = 0; i < im; i++) lph[q0 + i * plane] = 0.0;
+}
+
+// =============================================================================================
+// computeBuoyancyRefLevel^T: the level mean's adjoint (the sum of the cells' -cb * lambda_buoy,
+// gathered by the RHS adjoint into gbrl) spread back onto t with the mean's weights
+// =============================================================================================
+constexpr int ABRL_BLOCK = 256;
+__global__ __launch_bounds__(ABRL_BLOCK) void ka_brl_reduce(const double* __restrict__ t, const double* __restrict__ ps,
+        const uint8_t* __restrict__ flags, const double* __restrict__ sin_the, const double* __restrict__ gbrl,
+        double* __restrict__ out, int jm, int km, int pitch, double g, double rR, double t_ref){
+    const int i = blockIdx.x;
+    double sl = 0.0, sw = 0.0;
+    for(int c = threadIdx.x; c < jm * km; c += ABRL_BLOCK){
+        const int j = c / km, k = c - j * km;
+        const size_t q = ((size_t)i * jm + j) * pitch + k;
+        sl += gbrl[q];
+        if(flags[q] & FL_SELF) continue;
+        const double tq = t[q];
+        if(!(tq > 0.0)) continue;
+        const double b = g * ps[q] / (rR * tq * t_ref);
+        if(!isfinite(b)) continue;
+        sw += sin_the[j];
+    }
+    __shared__ double sh[2][ABRL_BLOCK];
+    sh[0][threadIdx.x] = sl; sh[1][threadIdx.x] = sw;
+    __syncthreads();
+    for(int o = ABRL_BLOCK / 2; o > 0; o >>= 1){
+        if((int)threadIdx.x < o){ sh[0][threadIdx.x] += sh[0][threadIdx.x + o]; sh[1][threadIdx.x] += sh[1][threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if(threadIdx.x == 0){ out[2 * i] = sh[0][0]; out[2 * i + 1] = sh[1][0]; }
+}
+
+__global__ __launch_bounds__(256) void ka_brl_spread(const double* __restrict__ t, const double* __restrict__ ps,
+        const uint8_t* __restrict__ flags, const double* __restrict__ sin_the, const double* __restrict__ red,
+        double* __restrict__ gbrl, double* __restrict__ lt, int jm, int km, int pitch, double g, double rR, double t_ref){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    gbrl[q] = 0.0;
+    const double lbrl = red[2 * i], W = red[2 * i + 1];
+    if(lbrl == 0.0 || !(W > 0.0)) return;
+    if(flags[q] & FL_SELF) return;
+    const double tq = t[q];
+    if(!(tq > 0.0)) return;
+    const double b = g * ps[q] / (rR * tq * t_ref);
+    if(!isfinite(b)) return;
+    lt[q] += lbrl * sin_the[j] / W * (-b / tq);
+}
+
+// =============================================================================================
+// RHS of the dry stage, transposed (RHS_Jup_Turb.cpp:31-1457 as restated by k_stage_dry)
+// =============================================================================================
+struct RhsAdjArgs {
+    const double* y[4];          // the stage input t, u, v, w
+    const double* lk[4];         // the adjoint of the stage's right-hand side (interior cells)
+    const double* la[3];         // the adjoint of aux_u, aux_v, aux_w (stage 3) or nullptr
+    double* g[4];                // the adjoint of the stage input (scatter target)
+    const double *pd, *ph, *ps, *rho, *wn;
+    double *lpd, *lph, *lrho, *gbrl;
+    const uint8_t* flags;
+    const GeoIJ<double>* geo;
+    int im, jm, km, pitch, nkb, njb, nb;
+    size_t plane;
+    double inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
+    double cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, cbuoy, g_, rR, t_ref, kt, visc0;
+};
+
+__global__ __launch_bounds__(256) void ka_rhs_dry(const RhsAdjArgs a){
+    const int b = xcd_remap(blockIdx.x, a.nb);
+    const int kb = b % a.nkb;
+    const int rest = b / a.nkb;
+    const int jb = rest % a.njb;
+    const int i = rest / a.njb;
+    const int k = kb * 64 + threadIdx.x;
+    const int j = jb * 4 + threadIdx.y;
+    if(k >= a.km || j >= a.jm) return;
+    const size_t q = ((size_t)i * a.jm + j) * a.pitch + k;
+    const uint8_t fl = a.flags[q];
+    const bool interior = i >= 1 && i <= a.im - 2 && j >= 3 && j <= a.jm - 4 && k <= a.km - 2 && !(fl & FL_SELF);
+    if(!interior) return;
+    const size_t P = a.plane, R = (size_t)a.pitch;
+    const int kw = (k == 0) ? a.km - 2 : k - 1;
+    const int ke = (k == a.km - 2) ? 0 : k + 1;
+    const size_t qkw = q - k + kw, qke = q - k + ke;
+    const GeoIJ<double> G = a.geo[(size_t)i * a.jm + j];
+
+    double c_[4], d1r[4], d1t[4], d1p[4];
+    for(int f = 0; f < 4; f++){
+        const double* X = a.y[f];
+        c_[f] = X[q];
+        d1r[f] = (X[q + P] - X[q - P]) * a.inv_2dr;
+        d1t[f] = (X[q + R] - X[q - R]) * a.inv_2dthe;
+        d1p[f] = (X[qke] - X[qkw]) * a.inv_2dphi;
+    }
+    const double pdc = a.pd[q];
+    const double dpdr   = (((fl & FL_IP) ? pdc : a.pd[q + P]) - ((fl & FL_IM) ? pdc : a.pd[q - P])) * a.inv_2dr;
+    const double dpdthe = (((fl & FL_JP) ? pdc : a.pd[q + R]) - ((fl & FL_JM) ? pdc : a.pd[q - R])) * a.inv_2dthe;
+    const double dpdphi = (((fl & FL_KP) ? pdc : a.pd[qke]) - ((fl & FL_KM) ? pdc : a.pd[qkw])) * a.inv_2dphi;
+    const double tq = c_[0], u = c_[1], v = c_[2], w = c_[3];
+    const double costhe = G.costhe, st = G.sinthe_true;
+    const double v_invrm = v * G.inv_rm, w_invrs = w * G.inv_rmsinthe;
+    const double S0 = (a.ps[q + P] - a.ps[q - P]) * a.inv_2dr * a.to_nd;
+    const double rho_l = a.rho[q];
+    const bool rvalid = rho_l > 0.0 && isfinite(rho_l);
+    const double dpstatdr = rvalid ? S0 * (a.r_mix / rho_l) : S0;
+    const double visc = a.visc0 + a.wn[q];
+    const double two_inv_rm = 2.0 * G.inv_rm;
+
+    // adjoints of the four right-hand sides (with aux at the last stage)
+    const double Lt = a.lk[0][q];
+    const double lau = a.la[0] ? a.la[0][q] : 0.0, lav = a.la[1] ? a.la[1][q] : 0.0, law = a.la[2] ? a.la[2][q] : 0.0;
+    const double Lu = a.lk[1][q] + lau, Lv = a.lk[2][q] + lav, Lw = a.lk[3][q] + law;
+
+    double L_dpdr = -Lu + lau;
+    double L_dpdthe = (-Lv + lav) * G.inv_rm, L_dphdthe = -Lv * G.inv_rm;
+    double L_dpdphi = (-Lw + law) * G.inv_rmsinthe, L_dphdphi = -Lw * G.inv_rmsinthe;
+    double Lc[4] = { 0.0, 0.0, 0.0, 0.0 };                         // centre-value adjoints
+    double L1r[4], L1t[4], L1p[4], L2r[4], L2t[4], L2p[4];
+    double L_vinv = 0.0, L_winv = 0.0;
+    // transport
+    const double Ltr[4] = { -Lt, -Lu, -Lv, -Lw };
+    for(int f = 0; f < 4; f++){
+        Lc[1] += Ltr[f] * d1r[f];
+        L1r[f] = Ltr[f] * u;
+        L_vinv += Ltr[f] * d1t[f];
+        L1t[f] = Ltr[f] * v_invrm;
+        L_winv += Ltr[f] * d1p[f];
+        L1p[f] = Ltr[f] * w_invrs;
+    }
+    Lc[1] += Ltr[2] * v * G.inv_rm;
+    Lc[2] += Ltr[2] * u * G.inv_rm;
+    Lc[3] += Ltr[2] * (-2.0 * w * G.cotanthe * G.inv_rm);
+    Lc[3] += Ltr[3] * (u + v * G.cotanthe) * G.inv_rm;
+    Lc[1] += Ltr[3] * w * G.inv_rm;
+    Lc[2] += Ltr[3] * w * G.cotanthe * G.inv_rm;
+    // pressure work in the energy equation
+    {
+        const double Pt = a.cep * Lt;
+        Lc[1] += Pt * (dpdr + dpstatdr);
+        L_dpdr += Pt * u;
+        const double L_dpstatdr = Pt * u;
+        L_vinv += Pt * dpdthe;
+        L_dpdthe += Pt * v_invrm;
+        L_winv += Pt * dpdphi;
+        L_dpdphi += Pt * w_invrs;
+        if(rvalid) a.lrho[q] += L_dpstatdr * S0 * a.r_mix * (-1.0 / (rho_l * rho_l));
+    }
+    // diffusion
+    const double Ld[4] = { a.kt * Lt, visc * Lu, visc * Lv, visc * Lw };
+    for(int f = 0; f < 4; f++){
+        L2r[f] = Ld[f];
+        L2t[f] = Ld[f] * G.inv_rm2;
+        L2p[f] = Ld[f] * G.inv_rm2sinthe2;
+        L1t[f] += Ld[f] * G.costhe_inv_rm2sinthe;
+    }
+    L1r[0] += Ld[0] * two_inv_rm;
+    Lc[1] += Ld[1] * 2.0 * G.inv_rm2;
+    L1r[1] += Ld[1] * 4.0 * G.inv_rm;
+    L1r[2] += Ld[2] * two_inv_rm;
+    Lc[2] += -Ld[2] * G.v_metric;
+    L1t[1] += Ld[2] * 2.0 * G.inv_rm2;
+    L1p[3] += -Ld[2] * 2.0 * costhe * G.inv_rm2sinthe2;
+    L1r[3] += Ld[3] * two_inv_rm;
+    Lc[3] += -Ld[3] * G.v_metric;
+    L1p[1] += Ld[3] * 2.0 * G.inv_rm2sinthe;
+    L1p[2] += Ld[3] * 2.0 * costhe * G.inv_rm2sinthe2;
+    // Coriolis
+    Lc[3] += (-a.Coriolis * Lv) * a.c_m2o * costhe;
+    {
+        const double Lcp = -a.Coriolis * Lw;
+        Lc[2] += Lcp * a.c_p2o * costhe;
+        Lc[1] += Lcp * a.c_p2o * st;
+    }
+    Lc[2] += L_vinv * G.inv_rm;
+    Lc[3] += L_winv * G.inv_rmsinthe;
+    // buoyancy anomaly
+    if(tq > 0.0){
+        const double Lb = Lu;
+        Lc[0] += Lb * a.cbuoy * (-a.g_ * a.ps[q] / (a.rR * tq * tq * a.t_ref));
+        a.gbrl[q] += -a.cbuoy * Lb;
+    }
+    // the stencils
+    for(int f = 0; f < 4; f++){
+        double* X = a.g[f];
+        aadd(&X[q + P], a.inv_2dr * L1r[f] + a.inv_dr2 * L2r[f]);
+        aadd(&X[q - P], -a.inv_2dr * L1r[f] + a.inv_dr2 * L2r[f]);
+        aadd(&X[q + R], a.inv_2dthe * L1t[f] + a.inv_dthe2 * L2t[f]);
+        aadd(&X[q - R], -a.inv_2dthe * L1t[f] + a.inv_dthe2 * L2t[f]);
+        aadd(&X[qke], a.inv_2dphi * L1p[f] + a.inv_dphi2 * L2p[f]);
+        aadd(&X[qkw], -a.inv_2dphi * L1p[f] + a.inv_dphi2 * L2p[f]);
+        aadd(&X[q], -2.0 * (a.inv_dr2 * L2r[f] + a.inv_dthe2 * L2t[f] + a.inv_dphi2 * L2p[f]) + Lc[f]);
+    }
+    // the mirrored pressure gradients
+    auto mir = [&](double* L, double coef, uint8_t hi_flag, uint8_t lo_flag, size_t qh, size_t ql){
+        aadd((fl & hi_flag) ? &L[q] : &L[qh], coef);
+        aadd((fl & lo_flag) ? &L[q] : &L[ql], -coef);
+    };
+    mir(a.lpd, a.inv_2dr * L_dpdr, FL_IP, FL_IM, q + P, q - P);
+    mir(a.lpd, a.inv_2dthe * L_dpdthe, FL_JP, FL_JM, q + R, q - R);
+    mir(a.lpd, a.inv_2dphi * L_dpdphi, FL_KP, FL_KM, qke, qkw);
+    mir(a.lph, a.inv_2dthe * L_dphdthe, FL_JP, FL_JM, q + R, q - R);
+    mir(a.lph, a.inv_2dphi * L_dphdphi, FL_KP, FL_KM, qke, qkw);
+}
+
+// the RK4 combination transposed: the adjoint of stage s's right-hand side from the step output's
+// adjoint and the adjoint of the next stage's input; the stage inputs' adjoints gather into the
+// n level's
+__global__ __launch_bounds__(256) void ka_rk_combine(double* __restrict__ lk0, double* __restrict__ lk1,
+        double* __restrict__ lk2, double* __restrict__ lk3, double* __restrict__ g0, double* __restrict__ g1,
+        double* __restrict__ g2, double* __restrict__ g3, const double* __restrict__ lnp0, const double* __restrict__ lnp1,
+        const double* __restrict__ lnp2, const double* __restrict__ lnp3, double* __restrict__ ln0, double* __restrict__ ln1,
+        double* __restrict__ ln2, double* __restrict__ ln3, const uint8_t* __restrict__ flags,
+        int im, int jm, int km, int pitch, double wnp, double cg, int have_g, int write_lk){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const bool interior = i >= 1 && i <= im - 2 && j >= 3 && j <= jm - 4 && k <= km - 2 && !(flags[q] & FL_SELF);
+    double* lk[4] = { lk0, lk1, lk2, lk3 };
+    double* g[4] = { g0, g1, g2, g3 };
+    const double* lnp[4] = { lnp0, lnp1, lnp2, lnp3 };
+    double* ln[4] = { ln0, ln1, ln2, ln3 };
+    for(int f = 0; f < 4; f++){
+        const double gv = have_g ? g[f][q] : 0.0;
+        if(write_lk) lk[f][q] = interior ? (wnp * lnp[f][q] + cg * gv) : 0.0;
+        if(have_g){ ln[f][q] += gv; g[f][q] = 0.0; }
+    }
+}
+
+// =============================================================================================
+// Red-black pressure solve transposed (reference formulation: colour 0, colour 1, then the outer
+// planes i, j and the seam after every sweep). Works on the split storage of the forward solve.
+// =============================================================================================
+// planes^T: seam, then j planes, then i planes; the plane adjoints move into rows 1, 2
+__global__ void ka_rb_seam(double* __restrict__ l0, double* __restrict__ l1, int im, int jm, int hp, int nm){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(j >= jm) return;
+    // the seam cell k = 2 nm has the colour of k = 0: same array, compact index nm -> 0
+    const int c = (i + j) & 1;
+    double* L = c ? l1 : l0;
+    const size_t r = ((size_t)i * jm + j) * hp;
+    L[r] += L[r + nm];
+    L[r + nm] = 0.0;
+}
+__global__ void ka_rb_planes_j(double* __restrict__ l0, double* __restrict__ l1, int im, int jm, int hp, int nm,
+        double c43, double c13){
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y, side = blockIdx.z;
+    if(m > nm) return;
+    const size_t R = (size_t)hp;
+    const int ja = side ? jm - 1 : 0, jb1 = side ? jm - 2 : 1, jb2 = side ? jm - 3 : 2;
+    for(int c = 0; c < 2; c++){
+        // c: colour array of the plane cell (i, ja, m); (i, jb1) is in the other array, (i, jb2) in the same
+        double* same = c ? l1 : l0;
+        double* other = c ? l0 : l1;
+        const size_t base = (size_t)i * jm * hp + m;
+        const double l = same[base + ja * R];
+        same[base + ja * R] = 0.0;
+        other[base + jb1 * R] += c43 * l;
+        same[base + jb2 * R] += -c13 * l;
+    }
+}
+__global__ void ka_rb_planes_i(double* __restrict__ l0, double* __restrict__ l1, int im, int jm, int hp, int nm,
+        double c43, double c13){
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, side = blockIdx.z;
+    if(m > nm) return;
+    const size_t P = (size_t)jm * hp;
+    const int ia = side ? im - 1 : 0, ib1 = side ? im - 2 : 1, ib2 = side ? im - 3 : 2;
+    for(int c = 0; c < 2; c++){
+        double* same = c ? l1 : l0;
+        double* other = c ? l0 : l1;
+        const size_t base = (size_t)j * hp + m;
+        const double l = same[base + ia * P];
+        same[base + ia * P] = 0.0;
+        other[base + ib1 * P] += c43 * l;
+        same[base + ib2 * P] += -c13 * l;
+    }
+}
+
+// One colour's half-sweep transposed, as a gather over the cells it read (the other colour's
+// cells and the outer planes): lambda_n += sum over the sweep cells s that read n of
+// omega w(s -> n) / den_s lambda_s. Launched over every cell of colour `tgt` (= 1 - the swept
+// colour), planes included; the swept colour's own scaling follows in ka_rb_scale.
+__device__ __forceinline__ bool rb_reads(uint8_t fl_s, int dir){
+    // does sweep cell s (flags fl_s) read its neighbour in direction dir (0 +i, 1 -i, 2 +j, 3 -j, 4 +k, 5 -k)
+    if(fl_s & FL_SELF) return true;
+    const uint8_t bit[6] = { FL_IP, FL_IM, FL_JP, FL_JM, FL_KP, FL_KM };
+    return !(fl_s & bit[dir]);
+}
+
+__device__ __forceinline__ double rb_den(uint8_t fl, const PoisIJ<double>& W){
+    if(fl & FL_SELF) return W.denom;
+    double den = 0.0;
+    if(!(fl & FL_IP)) den += W.num1;
+    if(!(fl & FL_IM)) den += W.num1;
+    if(!(fl & FL_JP)) den += W.num2;
+    if(!(fl & FL_JM)) den += W.num2;
+    if(!(fl & FL_KP)) den += W.num3;
+    if(!(fl & FL_KM)) den += W.num3;
+    return den;
+}
+
+__global__ __launch_bounds__(256) void ka_rb_gather(double* __restrict__ lt_arr, const double* __restrict__ ls_arr,
+        const uint8_t* __restrict__ fs_arr, const PoisIJ<double>* __restrict__ pw,
+        int im, int jm, int hp, int nm, int swept, double omega){
+    // target cells: colour tgt = 1 - swept, every i, j (planes included), m < nm
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(m >= nm) return;
+    const int tgt = 1 - swept;
+    const size_t P = (size_t)jm * hp, R = (size_t)hp;
+    const size_t q = ((size_t)i * jm + j) * hp + m;
+    // the target cell's k: 2m + off_t, off_t = (tgt + i + j) & 1; a swept cell in the same row
+    // with compact index ms sits at k = 2 ms + off_s, off_s = 1 - off_t
+    const int off_t = (tgt + i + j) & 1;
+    double acc = 0.0;
+    auto take = [&](int si, int sj, size_t qs, int dir_from_s){
+        if(si < 1 || si > im - 2 || sj < 1 || sj > jm - 2) return;     // not a sweep cell
+        const uint8_t fs = fs_arr[qs];
+        if(!rb_reads(fs, dir_from_s)) return;
+        const PoisIJ<double> W = pw[(size_t)si * jm + sj];
+        const double den = rb_den(fs, W);
+        if(!(den > 0.0)) return;
+        const double wgt = (dir_from_s < 2) ? W.num1 : (dir_from_s < 4 ? W.num2 : W.num3);
+        acc += omega * wgt / den * ls_arr[qs];
+    };
+    // radial and meridional neighbours share the compact index
+    if(i + 1 <= im - 1) take(i + 1, j, q + P, 1);      // s = (i+1) reads its -i neighbour, this cell
+    if(i - 1 >= 0)      take(i - 1, j, q - P, 0);
+    if(j + 1 <= jm - 1) take(i, j + 1, q + R, 3);
+    if(j - 1 >= 0)      take(i, j - 1, q - R, 2);
+    // zonal neighbours on the ring: this cell at k = 2m + off_t; k + 1 and k - 1 (mod 2 nm)
+    if(i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2){
+        const size_t row = ((size_t)i * jm + j) * hp;
+        const int kk = 2 * m + off_t;
+        const int ring = 2 * nm;
+        const int kp = (kk + 1) % ring, kmn = (kk - 1 + ring) % ring;
+        take(i, j, row + (kp >> 1), 5);                // the cell at k+1 reads its -k neighbour
+        take(i, j, row + (kmn >> 1), 4);               // the cell at k-1 reads its +k neighbour
+    }
+    lt_arr[q] += acc;
+}
+
+// the swept colour's own part: lambda_s -> (1 - omega) lambda_s, the source gets -omega lambda_s / den
+__global__ __launch_bounds__(256) void ka_rb_scale(double* __restrict__ ls_arr, double* __restrict__ lsrc,
+        const uint8_t* __restrict__ fs_arr, const PoisIJ<double>* __restrict__ pw,
+        int im, int jm, int hp, int nm, double omega){
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1, i = blockIdx.z + 1;
+    if(m >= nm || j > jm - 2 || i > im - 2) return;
+    const size_t q = ((size_t)i * jm + j) * hp + m;
+    const uint8_t fs = fs_arr[q];
+    const double den = rb_den(fs, pw[(size_t)i * jm + j]);
+    if(!(den > 0.0)) return;                           // the cell was skipped: it keeps its value
+    const double l = ls_arr[q];
+    lsrc[q] += -omega * l / den;
+    ls_arr[q] = (1.0 - omega) * l;
+}
+
+// prepareProjectionBoundaries + the clamped divergence source, transposed: the source adjoint
+// (frozen to zero where the forward clamped) spreads onto aux_u, aux_v, aux_w
+struct SrcAdjArgs {
+    const double *au, *av, *aw;             // forward aux after prepareProjectionBoundaries
+    const double *ls0, *ls1;                // source adjoints, split
+    double *lau, *lav, *law;
+    const uint8_t *flags, *msk;
+    const PoisIJ<double>* pw;
+    int im, jm, km, pitch, hp;
+    size_t plane;
+    double inv_2dr, inv_2dthe, inv_2dphi, inv_dthe, inv_dphi, cap;
+};
+
+__global__ __launch_bounds__(256) void ka_poisson_source(const SrcAdjArgs a){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1, i = blockIdx.z + 1;
+    if(k > a.km - 2 || j > a.jm - 2 || i > a.im - 2) return;
+    const int im = a.im, jm = a.jm, km = a.km;
+    const size_t q = ((size_t)i * jm + j) * a.pitch + k;
+    const size_t P = a.plane, R = (size_t)a.pitch;
+    const double ls = (((i + j + k) & 1) ? a.ls1 : a.ls0)[((size_t)i * jm + j) * a.hp + (k >> 1)];
+    if(ls == 0.0) return;
+    const uint8_t fl = a.flags[q];
+    const bool sld = fl & FL_SELF;
+    const int kw = (k == 0) ? km - 2 : k - 1, ke = (k == km - 2) ? 0 : k + 1;
+    const bool skp = fl & FL_KP, skm = fl & FL_KM;
+    const bool skp2 = (k < km - 2) ? (a.msk[q + 2] != 0) : false;
+    const bool skm2 = (k > 2) ? (a.msk[q - 2] != 0) : false;
+    // forward value, to know whether the clamp held it
+    double du_dr = 0.0, dv_dthe = 0.0, dw_dphi = 0.0;
+    int r_mode = 0, t_mode = 0, p_mode = 0;     // which stencil each derivative took
+    if(i < im - 2 && sld && !(fl & FL_IP)){
+        du_dr = (-3.0 * a.au[q] + 4.0 * a.au[q + P] - a.au[q + 2 * P]) * a.inv_2dr; r_mode = 1;
+    }
+    if(j > 2 && j < jm - 2){
+        const bool air_jp1 = !(fl & FL_JP), air_jm1 = !(fl & FL_JM);
+        const bool sjp2 = a.msk[q + 2 * R] != 0, sjm2 = a.msk[q - 2 * R] != 0;
+        if(sld && air_jp1 && !sjp2){ dv_dthe = (-3.0 * a.av[q] + 4.0 * a.av[q + R] - a.av[q + 2 * R]) * a.inv_2dthe; t_mode = 1; }
+        if(sld && air_jm1 && !sjm2){ dv_dthe = -(-3.0 * a.av[q] + 4.0 * a.av[q - R] - a.av[q - 2 * R]) * a.inv_2dthe; t_mode = 2; }
+        if((sld && air_jp1 && sjp2) || (j == jm - 2 && !sld && (fl & FL_JP))){ dv_dthe = (a.av[q + R] - a.av[q]) * a.inv_dthe; t_mode = 3; }
+        if((sld && air_jm1 && sjm2) || (j == 1 && sld && air_jm1)){ dv_dthe = (a.av[q - R] - a.av[q]) * a.inv_dthe; t_mode = 4; }
+    }
+    if(k > 2 && k < km - 2){
+        const bool air_kp1 = !skp, air_km1 = !skm;
+        if(sld && air_kp1 && !skp2){ dw_dphi = (-3.0 * a.aw[q] + 4.0 * a.aw[q + 1] - a.aw[q + 2]) * a.inv_2dphi; p_mode = 1; }
+        if(sld && air_km1 && !skm2){ dw_dphi = -(-3.0 * a.aw[q] + 4.0 * a.aw[q - 1] - a.aw[q - 2]) * a.inv_2dphi; p_mode = 2; }
+        if((sld && air_kp1 && skp2) || (k == km - 2 && !sld && skp)){ dw_dphi = (a.aw[q + 1] - a.aw[q]) * a.inv_dphi; p_mode = 3; }
+        if((sld && air_km1 && skm2) || (k == 1 && sld && air_km1)){ dw_dphi = (a.aw[q - 1] - a.aw[q]) * a.inv_dphi; p_mode = 4; }
+    }
+    if(!r_mode) du_dr = (a.au[q + P] - a.au[q - P]) * a.inv_2dr;
+    if(!t_mode) dv_dthe = (a.av[q + R] - a.av[q - R]) * a.inv_2dthe;
+    if(!p_mode) dw_dphi = (a.aw[q - k + ke] - a.aw[q - k + kw]) * a.inv_2dphi;
+    const PoisIJ<double> W = a.pw[(size_t)i * jm + j];
+    const double div_src = du_dr * 1.0 + dv_dthe * W.inv_rm + dw_dphi * W.inv_rmsinthe;
+    double den = W.denom;
+    if(!sld){
+        den = 0.0;
+        if(!(fl & FL_IP)) den += W.num1;
+        if(!(fl & FL_IM)) den += W.num1;
+        if(!(fl & FL_JP)) den += W.num2;
+        if(!(fl & FL_JM)) den += W.num2;
+        if(!skp) den += W.num3;
+        if(!skm) den += W.num3;
+    }
+    const double src_max = den * a.cap;
+    if(!isfinite(div_src) || div_src > src_max || div_src < -src_max) return;    // clamped: frozen
+    const double lr = ls * 1.0, lt = ls * W.inv_rm, lp = ls * W.inv_rmsinthe;
+    switch(r_mode){
+        case 1: aadd(&a.lau[q], -3.0 * a.inv_2dr * lr); aadd(&a.lau[q + P], 4.0 * a.inv_2dr * lr);
+                aadd(&a.lau[q + 2 * P], -a.inv_2dr * lr); break;
+        default: aadd(&a.lau[q + P], a.inv_2dr * lr); aadd(&a.lau[q - P], -a.inv_2dr * lr); break;
+    }
+    switch(t_mode){
+        case 1: aadd(&a.lav[q], -3.0 * a.inv_2dthe * lt); aadd(&a.lav[q + R], 4.0 * a.inv_2dthe * lt);
+                aadd(&a.lav[q + 2 * R], -a.inv_2dthe * lt); break;
+        case 2: aadd(&a.lav[q], 3.0 * a.inv_2dthe * lt); aadd(&a.lav[q - R], -4.0 * a.inv_2dthe * lt);
+                aadd(&a.lav[q - 2 * R], a.inv_2dthe * lt); break;
+        case 3: aadd(&a.lav[q + R], a.inv_dthe * lt); aadd(&a.lav[q], -a.inv_dthe * lt); break;
+        case 4: aadd(&a.lav[q - R], a.inv_dthe * lt); aadd(&a.lav[q], -a.inv_dthe * lt); break;
+        default: aadd(&a.lav[q + R], a.inv_2dthe * lt); aadd(&a.lav[q - R], -a.inv_2dthe * lt); break;
+    }
+    switch(p_mode){
+        case 1: aadd(&a.law[q], -3.0 * a.inv_2dphi * lp); aadd(&a.law[q + 1], 4.0 * a.inv_2dphi * lp);
+                aadd(&a.law[q + 2], -a.inv_2dphi * lp); break;
+        case 2: aadd(&a.law[q], 3.0 * a.inv_2dphi * lp); aadd(&a.law[q - 1], -4.0 * a.inv_2dphi * lp);
+                aadd(&a.law[q - 2], a.inv_2dphi * lp); break;
+        case 3: aadd(&a.law[q + 1], a.inv_dphi * lp); aadd(&a.law[q], -a.inv_dphi * lp); break;
+        case 4: aadd(&a.law[q - 1], a.inv_dphi * lp); aadd(&a.law[q], -a.inv_dphi * lp); break;
+        default: aadd(&a.law[q - k + ke], a.inv_2dphi * lp); aadd(&a.law[q - k + kw], -a.inv_2dphi * lp); break;
+    }
+}
+
+// prepareProjectionBoundaries^T: aux_u zeroed at the radial walls, aux_v, aux_w 2-point Neumann
+__global__ void ka_prep_projection(double* __restrict__ lau, double* __restrict__ lav, double* __restrict__ law,
+        int im, int jm, int km, int pitch, size_t plane, double c43, double c13){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y + 1;
+    if(k > km - 2) return;
+    const size_t q = (size_t)j * pitch + k;
+    const size_t t0 = q, t1 = q + plane, t2 = q + 2 * plane;
+    const size_t b0 = q + (size_t)(im - 1) * plane, b1 = q + (size_t)(im - 2) * plane, b2 = q + (size_t)(im - 3) * plane;
+    lau[t0] = 0.0;
+    lau[b0] = 0.0;
+    double* X[2] = { lav, law };
+    for(int f = 0; f < 2; f++){
+        double* A = X[f];
+        A[t1] += c43 * A[t0]; A[t2] += -c13 * A[t0]; A[t0] = 0.0;
+        A[b1] += c43 * A[b0]; A[b2] += -c13 * A[b0]; A[b0] = 0.0;
+    }
+}
+
+// the final planes and seam of the solve, on the pitch layout, transposed (seam, j, i)
+__global__ void ka_seam_field(double* __restrict__ L, int im, int jm, int km, int pitch){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(j >= jm) return;
+    const size_t r = ((size_t)i * jm + j) * pitch;
+    L[r] += L[r + km - 1];
+    L[r + km - 1] = 0.0;
+}
+__global__ void ka_planes_j_field(double* __restrict__ L, int im, int jm, int km, int pitch, size_t plane,
+        double c43, double c13){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if(k >= km) return;
+    const size_t q = (size_t)i * plane + k, R = (size_t)pitch;
+    double l = L[q]; L[q] = 0.0;
+    L[q + R] += c43 * l; L[q + 2 * R] += -c13 * l;
+    const size_t qn = q + (size_t)(jm - 1) * R;
+    l = L[qn]; L[qn] = 0.0;
+    L[qn - R] += c43 * l; L[qn - 2 * R] += -c13 * l;
+}
+__global__ void ka_planes_i_field(double* __restrict__ L, int im, int jm, int km, int pitch, size_t plane,
+        double c43, double c13){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const size_t q = (size_t)j * pitch + k;
+    double l = L[q]; L[q] = 0.0;
+    L[q + plane] += c43 * l; L[q + 2 * plane] += -c13 * l;
+    const size_t qt = q + (size_t)(im - 1) * plane;
+    l = L[qt]; L[qt] = 0.0;
+    L[qt - plane] += c43 * l; L[qt - 2 * plane] += -c13 * l;
+}
+
+// split <-> pitch layout for the adjoint of p (add into the target)
+__global__ void ka_rb_pack_add(const double* __restrict__ L, double* __restrict__ x0, double* __restrict__ x1,
+        int jm, int km, int pitch, int hp){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    double* x = ((i + j + k) & 1) ? x1 : x0;
+    x[((size_t)i * jm + j) * hp + (k >> 1)] = L[((size_t)i * jm + j) * pitch + k];
+}
+__global__ void ka_rb_unpack_add(double* __restrict__ L, const double* __restrict__ x0, const double* __restrict__ x1,
+        int jm, int km, int pitch, int hp){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const double* x = ((i + j + k) & 1) ? x1 : x0;
+    // the compact index of the seam (k = km - 1) is shared with nothing; every (i, j, k) maps once
+    L[((size_t)i * jm + j) * pitch + k] = x[((size_t)i * jm + j) * hp + (k >> 1)];
+}
+
+// the cells the RK4 step integrates (and overwrites aux_* in) lose their adjoint
+__global__ __launch_bounds__(256) void ka_zero_interior(double* __restrict__ a0, double* __restrict__ a1,
+        double* __restrict__ a2, const uint8_t* __restrict__ flags, int im, int jm, int km, int pitch){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const bool interior = i >= 1 && i <= im - 2 && j >= 3 && j <= jm - 4 && k <= km - 2 && !(flags[q] & FL_SELF);
+    if(!interior) return;
+    a0[q] = 0.0; a1[q] = 0.0; a2[q] = 0.0;
+}
+
+__global__ void ka_zero(double* __restrict__ x, size_t n){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q < n) x[q] = 0.0;
+}
+__global__ void ka_copy(double* __restrict__ d, const double* __restrict__ s, size_t n){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q < n) d[q] = s[q];
+}
+
+}  // namespace atjup
+
+// =============================================================================================
+// host side
+// =============================================================================================
+namespace atjup {
+
+enum { AW_G = 0, AW_LK = 4, AW_LNP = 8, AW_Y = 12, AW_NN = 24, AW_AUX = 28, AW_AUXP = 31, AW_GBRL = 34,
+       AW_SCR = 35, AW_PIN = 36, AW_N = 37 };
+// the forward state a step starts from, as checkpoints and snapshots hold it
+static const int kStateFields[] = { ATJUP_F_T, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W, ATJUP_F_P_DYN,
+                                    ATJUP_F_AUX_U, ATJUP_F_AUX_V, ATJUP_F_AUX_W, ATJUP_F_RHO_MIX };
+constexpr int N_STATE = 9;
+
+struct AdjState {
+    Buf lam[ATJUP_F_END];
+    Buf work[AW_N];
+    Buf lsplit, lsrc, red, zs;
+    std::vector<Buf> ckpt;        // N_STATE fields per checkpoint
+    std::vector<Buf> snap;        // N_STATE fields per step of the segment being reversed
+    int nsteps = 0, every = 0, first_iter = 1;
+};
+
+static const int kLamFields[] = { ATJUP_F_T, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W, ATJUP_F_P_DYN, ATJUP_F_P_HYDRO,
+                                  ATJUP_F_RHO_MIX, ATJUP_F_AUX_U, ATJUP_F_AUX_V, ATJUP_F_AUX_W };
+
+static int adj_alloc(Buf& b, size_t bytes){
+    if(hipMalloc(&b.p, bytes) != hipSuccess){ set_error("hipMalloc failed (adjoint)"); return ATJUP_ERR_NOMEM; }
+    b.bytes = bytes;
+    if(hipMemset(b.p, 0, bytes) != hipSuccess){ set_error("hipMemset failed"); return ATJUP_ERR_HIP; }
+    return 0;
+}
+static void adj_release(Buf& b){ if(b.p) (void)hipFree(b.p); b.p = nullptr; b.bytes = 0; }
+
+void adj_free(atjup_ctx* c){
+    if(!c || !c->adj) return;
+    AdjState* A = c->adj;
+    for(auto& b : A->lam) adj_release(b);
+    for(auto& b : A->work) adj_release(b);
+    for(Buf* b : { &A->lsplit, &A->lsrc, &A->red, &A->zs }) adj_release(*b);
+    for(auto& b : A->ckpt) adj_release(b);
+    for(auto& b : A->snap) adj_release(b);
+    delete A;
+    c->adj = nullptr;
+}
+
+static inline double* L(atjup_ctx* c, int f){ return (double*)c->adj->lam[f].p; }
+static inline double* W(atjup_ctx* c, int w){ return (double*)c->adj->work[w].p; }
+static inline double* Fd(atjup_ctx* c, int f){ return (double*)c->cur[f].p; }
+
+static int zero_buf(atjup_ctx* c, Buf& b){
+    const size_t n = b.bytes / sizeof(double);
+    hipLaunchKernelGGL(ka_zero, dim3(nblk(n, 256)), dim3(256), 0, c->stream, (double*)b.p, n);
+    return check_launch();
+}
+static int copy_field(atjup_ctx* c, double* d, const double* s){
+    hipLaunchKernelGGL(ka_copy, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream, d, s, c->n3);
+    return check_launch();
+}
+
+// ---- the transposed pieces --------------------------------------------------------------------
+static int adj_bc(atjup_ctx* c){
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    double *lt = L(c, ATJUP_F_T), *lu = L(c, ATJUP_F_U), *lv = L(c, ATJUP_F_V), *lw = L(c, ATJUP_F_W);
+    double* lp = L(c, ATJUP_F_P_DYN);
+    if(c->n_amb_rows > 0)
+        hipLaunchKernelGGL(ka_t_ambient, dim3(c->n_amb_rows), dim3(64), 0, c->stream, lt,
+            (const uint8_t*)c->mask.p, (const int2*)c->amb_rows.p, c->n_amb_rows, c->jm, c->km, c->pitch);
+    if(c->n_solids > 0)
+        hipLaunchKernelGGL(ka_bc_solid, dim3(nblk(c->n_solids, 64)), dim3(64), 0, c->stream, lu, lv, lw, lt, lp,
+            (const SolidCell*)c->solids.p, c->n_solids, c->jm, c->pitch, c->plane, c43, c13);
+    hipLaunchKernelGGL(ka_bc_phi, dim3(nblk(c->jm, 64), c->im), dim3(64), 0, c->stream, lt, lu, lv, lw,
+        c->jm, c->km, c->pitch);
+    hipLaunchKernelGGL(ka_bc_theta, dim3(nblk(c->km, 128), c->im), dim3(128), 0, c->stream, lt, lu, lv, lw,
+        c->jm, c->km, c->pitch);
+    double* zs = (double*)c->adj->zs.p;
+    hipLaunchKernelGGL(ka_bc_zonal_sums, dim3(nblk(c->jm, 64), 4), dim3(64), 0, c->stream, (const double*)lv,
+        (const double*)lw, zs, c->im, c->jm, c->km, c->pitch, c->plane);
+    hipLaunchKernelGGL(ka_bc_radius, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream, lt, lu, lv, lw,
+        (const double*)zs, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
+    return check_launch();
+}
+
+static int adj_shapiro(atjup_ctx* c, int f){
+    // forward passes k, j, i (strength 1, order 2): transposed i, j, k through the scratch buffer
+    double* a = L(c, f);
+    double* b = W(c, AW_SCR);
+    const double co = 0.25;
+    dim3 g(nblk(c->km, 128), c->jm, c->im), blk(128);
+    hipLaunchKernelGGL(ka_shapiro_i, g, blk, 0, c->stream, (const double*)a, b, (const int*)c->itopo.p,
+        c->im, c->jm, c->km, c->pitch, c->plane, co);
+    hipLaunchKernelGGL(ka_shapiro_j, g, blk, 0, c->stream, (const double*)b, a, (const int*)c->itopo.p,
+        c->im, c->jm, c->km, c->pitch, co);
+    hipLaunchKernelGGL(ka_shapiro_k, g, blk, 0, c->stream, (const double*)a, b, (const int*)c->itopo.p,
+        c->im, c->jm, c->km, c->pitch, co);
+    return copy_field(c, a, b);
+}
+
+static int adj_mixture(atjup_ctx* c){
+    hipLaunchKernelGGL(ka_mixture, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream,
+        (const double*)Fd(c, ATJUP_F_T), (const double*)Fd(c, ATJUP_F_P_STAT), (const double*)Fd(c, ATJUP_F_P_DYN),
+        L(c, ATJUP_F_RHO_MIX), L(c, ATJUP_F_T), L(c, ATJUP_F_P_DYN), c->n3, c->c.t_ref, c->kn.rho_pdyn,
+        c->c.r_mix * c->c.u_0 * c->c.u_0 * 1.0e-5, c->c.R_mix);
+    return check_launch();
+}
+
+// RK4^T at the context's state (the step's n level, p_dyn, rho_mix): records the stage inputs, then
+// runs the four transposed stages; the n level's adjoint replaces the step result's, p_dyn and
+// rho_mix gather theirs, the hydrostatic and level-mean adjoints are spread back onto v, w and t,
+// and aux_* lose their adjoint in the cells the step overwrote.
+static int adj_rk4(atjup_ctx* c){
+    AdjState* A = c->adj;
+    RkBuffers rb{};
+    for(int s = 0; s < 3; s++) for(int f = 0; f < 4; f++) rb.y[s][f] = W(c, AW_Y + 4 * s + f);
+    for(int f = 0; f < 4; f++) rb.nn[f] = W(c, AW_NN + f);
+    for(int f = 0; f < 3; f++) rb.aux[f] = W(c, AW_AUX + f);
+    if(int rc = launch_rk4_dry_record(c, rb)) return rc;
+    const double dt = c->c.dt;
+    const atjup_consts& C = c->c;
+    RhsAdjArgs a{};
+    a.pd = Fd(c, ATJUP_F_P_DYN); a.ph = Fd(c, ATJUP_F_P_HYDRO); a.ps = Fd(c, ATJUP_F_P_STAT);
+    a.rho = Fd(c, ATJUP_F_RHO_MIX); a.wn = Fd(c, ATJUP_F_WALL_NUE);
+    a.lpd = L(c, ATJUP_F_P_DYN); a.lph = L(c, ATJUP_F_P_HYDRO); a.lrho = L(c, ATJUP_F_RHO_MIX); a.gbrl = W(c, AW_GBRL);
+    a.flags = (const uint8_t*)c->flags.p; a.geo = (const GeoIJ<double>*)c->geo.p;
+    a.im = c->im; a.jm = c->jm; a.km = c->km; a.pitch = c->pitch; a.plane = c->plane;
+    a.nkb = (c->km + 63) / 64; a.njb = (c->jm + 3) / 4; a.nb = a.nkb * a.njb * c->im;
+    a.inv_2dr = 1.0 / (2.0 * c->g.dr); a.inv_2dthe = 1.0 / (2.0 * c->g.dthe); a.inv_2dphi = 1.0 / (2.0 * c->g.dphi);
+    a.inv_dr2 = 1.0 / (c->g.dr * c->g.dr); a.inv_dthe2 = 1.0 / (c->g.dthe * c->g.dthe);
+    a.inv_dphi2 = 1.0 / (c->g.dphi * c->g.dphi);
+    a.cep = C.u_0 * C.u_0 / (C.cp_mix * C.t_ref);
+    a.to_nd = 1.0 / (C.r_mix * C.u_0 * C.u_0 * 1.0e-5);
+    a.r_mix = C.r_mix; a.c_m2o = 1.0 * -2.0 * C.omega; a.c_p2o = 1.0 * +2.0 * C.omega;
+    a.Coriolis = C.Coriolis; a.cbuoy = -1.0 * 1.0 * 1.0 * C.buoyancy; a.g_ = C.g; a.rR = C.r_mix * C.R_mix;
+    a.t_ref = C.t_ref; a.kt = c->kn.t_diff_scale / (C.re * C.pr) + 0.0 / 0.9; a.visc0 = 1.0 / C.re + 0.0;
+    for(int f = 0; f < 4; f++){
+        a.lk[f] = W(c, AW_LK + f);
+        a.g[f] = W(c, AW_G + f);
+        if(int rc = copy_field(c, W(c, AW_LNP + f), L(c, f))) return rc;      // lambda of the step result
+    }
+    dim3 gc(nblk(c->km, 128), c->jm, c->im), b128(128);
+    auto combine = [&](double wnp, double cg, int have_g, int write_lk){
+        hipLaunchKernelGGL(ka_rk_combine, gc, b128, 0, c->stream,
+            W(c, AW_LK), W(c, AW_LK + 1), W(c, AW_LK + 2), W(c, AW_LK + 3),
+            W(c, AW_G), W(c, AW_G + 1), W(c, AW_G + 2), W(c, AW_G + 3),
+            (const double*)W(c, AW_LNP), (const double*)W(c, AW_LNP + 1), (const double*)W(c, AW_LNP + 2),
+            (const double*)W(c, AW_LNP + 3), L(c, 0), L(c, 1), L(c, 2), L(c, 3),
+            (const uint8_t*)c->flags.p, c->im, c->jm, c->km, c->pitch, wnp, cg, have_g, write_lk);
+        return check_launch();
+    };
+    const double wts[4] = { dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0 };
+    const double cin[4] = { 0.5 * dt, 0.5 * dt, dt, 0.0 };
+    for(int s = 3; s >= 0; s--){
+        // lk_s = w_s lambda_N' + c_s lambda_{Y_{s+1}}; lambda_N += lambda_{Y_{s+1}}
+        if(int rc = combine(wts[s], cin[s], s < 3 ? 1 : 0, 1)) return rc;
+        for(int f = 0; f < 4; f++) a.y[f] = (s == 0) ? Fd(c, f) : W(c, AW_Y + 4 * (s - 1) + f);
+        for(int f = 0; f < 3; f++) a.la[f] = (s == 3) ? L(c, ATJUP_F_AUX_U + f) : nullptr;
+        hipLaunchKernelGGL(ka_rhs_dry, dim3(a.nb), dim3(64, 4), 0, c->stream, a);
+        if(int rc = check_launch()) return rc;
+    }
+    if(int rc = combine(0.0, 0.0, 1, 0)) return rc;             // lambda_N += lambda_{Y_0}
+    // the level means and the hydrostatic integral, formed once from the n level
+    const double* tabs = (const double*)c->tabs.p;
+    hipLaunchKernelGGL(ka_brl_reduce, dim3(c->im), dim3(ABRL_BLOCK), 0, c->stream, (const double*)Fd(c, ATJUP_F_T),
+        (const double*)Fd(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p, tabs + c->im, (const double*)W(c, AW_GBRL),
+        (double*)A->red.p, c->jm, c->km, c->pitch, C.g, C.r_mix * C.R_mix, C.t_ref);
+    hipLaunchKernelGGL(ka_brl_spread, gc, b128, 0, c->stream, (const double*)Fd(c, ATJUP_F_T),
+        (const double*)Fd(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p, tabs + c->im, (const double*)A->red.p,
+        W(c, AW_GBRL), L(c, ATJUP_F_T), c->jm, c->km, c->pitch, C.g, C.r_mix * C.R_mix, C.t_ref);
+    hipLaunchKernelGGL(ka_hydrostatic, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream,
+        (const double*)Fd(c, ATJUP_F_V), (const double*)Fd(c, ATJUP_F_W), (const uint8_t*)c->flags.p,
+        tabs, tabs + c->im + c->jm, L(c, ATJUP_F_P_HYDRO), L(c, ATJUP_F_V), L(c, ATJUP_F_W),
+        c->im, c->jm, c->km, c->pitch, c->plane, c->g.dr);
+    // aux_* were overwritten in the integrated cells: their earlier values carry no adjoint there
+    hipLaunchKernelGGL(ka_zero_interior, gc, b128, 0, c->stream, L(c, ATJUP_F_AUX_U), L(c, ATJUP_F_AUX_V),
+        L(c, ATJUP_F_AUX_W), (const uint8_t*)c->flags.p, c->im, c->jm, c->km, c->pitch);
+    return check_launch();
+}
+
+// PressureSolver::run^T at the context's state (p_dyn and aux_* before the solve)
+static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
+    AdjState* A = c->adj;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    // the forward solve, for the aux_* after prepareProjectionBoundaries (the clamp test of the
+    // source reads them); p_dyn and aux_* of the context are restored afterwards
+    if(int rc = copy_field(c, W(c, AW_PIN), Fd(c, ATJUP_F_P_DYN))) return rc;
+    for(int f = 0; f < 3; f++)
+        if(int rc = copy_field(c, W(c, AW_AUXP + f), Fd(c, ATJUP_F_AUX_U + f))) return rc;
+    if(int rc = launch_pressure(c, sweeps, omega)) return rc;
+    const int hp = c->rb_hp;
+    const size_t hn = (size_t)c->im * c->jm * hp;
+    const int nm = (c->km - 1) / 2;
+    double* l0 = (double*)A->lsplit.p;
+    double* l1 = l0 + hn;
+    double* s0 = (double*)A->lsrc.p;
+    double* s1 = s0 + hn;
+    const uint8_t* f0 = (const uint8_t*)c->rb_flags.p;
+    const uint8_t* f1 = f0 + hn;
+    const PoisIJ<double>* pw = (const PoisIJ<double>*)c->pois.p;
+    if(int rc = zero_buf(c, A->lsrc)) return rc;
+    dim3 gfull(nblk(c->km, 128), c->jm, c->im), b128(128);
+    hipLaunchKernelGGL(ka_rb_pack_add, gfull, b128, 0, c->stream, (const double*)L(c, ATJUP_F_P_DYN), l0, l1,
+        c->jm, c->km, c->pitch, hp);
+    dim3 gseam(nblk(c->jm, 64), c->im), gpj(nblk(nm + 1, 128), c->im, 2), gpi(nblk(nm + 1, 128), c->jm, 2);
+    dim3 gg(nblk(nm, 128), c->jm, c->im), gs(nblk(nm, 128), c->jm - 2, c->im - 2);
+    for(int s = sweeps - 1; s >= 0; s--){
+        hipLaunchKernelGGL(ka_rb_seam, gseam, dim3(64), 0, c->stream, l0, l1, c->im, c->jm, hp, nm);
+        hipLaunchKernelGGL(ka_rb_planes_j, gpj, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
+        hipLaunchKernelGGL(ka_rb_planes_i, gpi, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
+        // colour 1, then colour 0, transposed
+        hipLaunchKernelGGL(ka_rb_gather, gg, b128, 0, c->stream, l0, (const double*)l1, f1, pw, c->im, c->jm, hp, nm, 1, omega);
+        hipLaunchKernelGGL(ka_rb_scale, gs, b128, 0, c->stream, l1, s1, f1, pw, c->im, c->jm, hp, nm, omega);
+        hipLaunchKernelGGL(ka_rb_gather, gg, b128, 0, c->stream, l1, (const double*)l0, f0, pw, c->im, c->jm, hp, nm, 0, omega);
+        hipLaunchKernelGGL(ka_rb_scale, gs, b128, 0, c->stream, l0, s0, f0, pw, c->im, c->jm, hp, nm, omega);
+        if(int rc = check_launch()) return rc;
+    }
+    hipLaunchKernelGGL(ka_rb_unpack_add, gfull, b128, 0, c->stream, L(c, ATJUP_F_P_DYN), (const double*)l0,
+        (const double*)l1, c->jm, c->km, c->pitch, hp);
+    // the source and prepareProjectionBoundaries
+    SrcAdjArgs sa{};
+    sa.au = Fd(c, ATJUP_F_AUX_U); sa.av = Fd(c, ATJUP_F_AUX_V); sa.aw = Fd(c, ATJUP_F_AUX_W);
+    sa.ls0 = s0; sa.ls1 = s1;
+    sa.lau = L(c, ATJUP_F_AUX_U); sa.lav = L(c, ATJUP_F_AUX_V); sa.law = L(c, ATJUP_F_AUX_W);
+    sa.flags = (const uint8_t*)c->flags.p; sa.msk = (const uint8_t*)c->mask.p; sa.pw = pw;
+    sa.im = c->im; sa.jm = c->jm; sa.km = c->km; sa.pitch = c->pitch; sa.hp = hp; sa.plane = c->plane;
+    sa.inv_2dr = 1.0 / (2.0 * c->g.dr); sa.inv_2dthe = 1.0 / (2.0 * c->g.dthe); sa.inv_2dphi = 1.0 / (2.0 * c->g.dphi);
+    sa.inv_dthe = 1.0 / c->g.dthe; sa.inv_dphi = 1.0 / c->g.dphi; sa.cap = c->kn.pdyn_cap;
+    hipLaunchKernelGGL(ka_poisson_source, dim3(nblk(c->km - 1, 128), c->jm - 2, c->im - 2), b128, 0, c->stream, sa);
+    hipLaunchKernelGGL(ka_prep_projection, dim3(nblk(c->km - 1, 128), c->jm - 2), b128, 0, c->stream,
+        L(c, ATJUP_F_AUX_U), L(c, ATJUP_F_AUX_V), L(c, ATJUP_F_AUX_W), c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
+    // restore the context's state before the solve
+    if(int rc = copy_field(c, Fd(c, ATJUP_F_P_DYN), W(c, AW_PIN))) return rc;
+    for(int f = 0; f < 3; f++)
+        if(int rc = copy_field(c, Fd(c, ATJUP_F_AUX_U + f), W(c, AW_AUXP + f))) return rc;
+    return check_launch();
+}
+
+// one loop iteration transposed. The context holds the state the iteration starts from (t, u, v,
+// w, p_dyn, aux_*, rho_mix); the forward physics block is rerun to reach the states its
+// transposes and the RK4's linearise about. On return the adjoint fields have been carried from
+// the end of the iteration to its start; the context's forward fields are left as the recording
+// left them (the n level and aux_* at the iteration's start, p_dyn and rho_mix after the block).
+static int adj_step(atjup_ctx* c, int iter, int first){
+    const bool phys = (iter % 2 == 0) || first;
+    const int sweeps = std::max(1, c->kn.press_sweeps);
+    if(phys){
+        if(int rc = copy_field(c, W(c, AW_PIN), Fd(c, ATJUP_F_P_DYN))) return rc;
+        for(int f = 0; f < 3; f++)
+            if(int rc = copy_field(c, W(c, AW_AUXP + f), Fd(c, ATJUP_F_AUX_U + f))) return rc;
+        if(int rc = launch_pressure(c, sweeps, c->kn.press_omega)) return rc;
+        if(c->kn.damp_pdyn)
+            if(int rc = launch_shapiro(c, ATJUP_F_P_DYN, 2, 1.0, 1, 7)) return rc;
+        if(int rc = launch_mixture_density(c)) return rc;
+    }
+    if(int rc = adj_bc(c)) return rc;
+    if(int rc = adj_rk4(c)) return rc;
+    if(phys){
+        if(int rc = adj_mixture(c)) return rc;
+        if(c->kn.damp_pdyn)
+            if(int rc = adj_shapiro(c, ATJUP_F_P_DYN)) return rc;
+        // back to the state before the solve for its transpose (adj_pressure keeps its own copy)
+        double* pin = W(c, AW_PIN);
+        if(int rc = copy_field(c, Fd(c, ATJUP_F_P_DYN), pin)) return rc;
+        for(int f = 0; f < 3; f++)
+            if(int rc = copy_field(c, Fd(c, ATJUP_F_AUX_U + f), W(c, AW_AUXP + f))) return rc;
+        if(int rc = adj_pressure(c, sweeps, c->kn.press_omega)) return rc;
+    }
+    return 0;
+}
+
+}  // namespace atjup
+
+using namespace atjup;
+
+namespace {
+
+int need_adj(atjup_ctx* c){
+    if(!c){ set_error("null context"); return ATJUP_ERR_ARG; }
+    if(!c->adj){ set_error("call atjup_hip_adj_init first"); return ATJUP_ERR_STATE; }
+    return 0;
+}
+
+int save_state(atjup_ctx* c, Buf* dst){
+    for(int s = 0; s < N_STATE; s++)
+        if(int rc = copy_field(c, (double*)dst[s].p, Fd(c, kStateFields[s]))) return rc;
+    return 0;
+}
+int load_state(atjup_ctx* c, const Buf* src){
+    for(int s = 0; s < N_STATE; s++)
+        if(int rc = copy_field(c, Fd(c, kStateFields[s]), (const double*)src[s].p)) return rc;
+    return 0;
+}
+
+int host_to_pitch(atjup_ctx* c, double* dev, const double* src){
+    std::vector<double> h(c->n3, 0.0);
+    for(int i = 0; i < c->im; i++)
+        for(int j = 0; j < c->jm; j++)
+            std::memcpy(h.data() + c->idx(i, j, 0), src + ((size_t)i * c->jm + j) * c->km, c->km * sizeof(double));
+    ATJUP_HIP_CHECK(hipMemcpyAsync(dev, h.data(), c->n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int pitch_to_host(atjup_ctx* c, double* dst, const double* dev){
+    std::vector<double> h(c->n3);
+    ATJUP_HIP_CHECK(hipMemcpyAsync(h.data(), dev, c->n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for(int i = 0; i < c->im; i++)
+        for(int j = 0; j < c->jm; j++)
+            std::memcpy(dst + ((size_t)i * c->jm + j) * c->km, h.data() + c->idx(i, j, 0), c->km * sizeof(double));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int atjup_hip_adj_init(atjup_ctx* c){
+    if(!c){ set_error("null context"); return ATJUP_ERR_ARG; }
+    if(c->precision != ATJUP_PRECISION_FP64 || !c->kn.dry){
+        set_error("the adjoint covers the FP64 dynamics profile"); return ATJUP_ERR_PROFILE;
+    }
+    if(c->adj) return 0;
+    c->adj = new AdjState();
+    AdjState* A = c->adj;
+    const size_t fb = c->n3 * sizeof(double);
+    int rc = 0;
+    for(int f : kLamFields) if(!rc) rc = adj_alloc(A->lam[f], fb);
+    for(int w = 0; w < AW_N && !rc; w++) rc = adj_alloc(A->work[w], fb);
+    const size_t hn = (size_t)c->im * c->jm * c->rb_hp;
+    if(!rc) rc = adj_alloc(A->lsplit, 2 * hn * sizeof(double));
+    if(!rc) rc = adj_alloc(A->lsrc, 2 * hn * sizeof(double));
+    if(!rc) rc = adj_alloc(A->red, 2 * (size_t)c->im * sizeof(double));
+    if(!rc) rc = adj_alloc(A->zs, 4 * (size_t)c->jm * sizeof(double));
+    if(rc){ adj_free(c); return rc; }
+    return 0;
+}
+
+int atjup_hip_adj_zero(atjup_ctx* c){
+    if(int rc = need_adj(c)) return rc;
+    for(int f : kLamFields) if(int rc = zero_buf(c, c->adj->lam[f])) return rc;
+    if(int rc = zero_buf(c, c->adj->work[AW_GBRL])) return rc;
+    for(int f = 0; f < 4; f++) if(int rc = zero_buf(c, c->adj->work[AW_G + f])) return rc;
+    return 0;
+}
+
+int atjup_hip_adj_upload(atjup_ctx* c, int f, const double* src){
+    if(int rc = need_adj(c)) return rc;
+    if(f < 0 || f >= ATJUP_F_END || !c->adj->lam[f].p || !src){ set_error("adjoint field id"); return ATJUP_ERR_ARG; }
+    return host_to_pitch(c, (double*)c->adj->lam[f].p, src);
+}
+
+int atjup_hip_adj_download(atjup_ctx* c, int f, double* dst){
+    if(int rc = need_adj(c)) return rc;
+    if(f < 0 || f >= ATJUP_F_END || !c->adj->lam[f].p || !dst){ set_error("adjoint field id"); return ATJUP_ERR_ARG; }
+    return pitch_to_host(c, dst, (const double*)c->adj->lam[f].p);
+}
+
+int atjup_hip_adj_op(atjup_ctx* c, int op, int iarg, double darg){
+    if(int rc = need_adj(c)) return rc;
+    if(!c->have_mask){ set_error("upload the obstacle mask first"); return ATJUP_ERR_STATE; }
+    switch(op){
+        case ATJUP_ADJ_OP_BC:           return adj_bc(c);
+        case ATJUP_ADJ_OP_SHAPIRO_PDYN: return adj_shapiro(c, ATJUP_F_P_DYN);
+        case ATJUP_ADJ_OP_MIXTURE:      return adj_mixture(c);
+        case ATJUP_ADJ_OP_RK4:          return adj_rk4(c);
+        case ATJUP_ADJ_OP_PRESSURE:
+            if(!(darg > 0.0 && darg < 2.0)){ set_error("omega outside (0, 2)"); return ATJUP_ERR_ARG; }
+            return adj_pressure(c, iarg > 0 ? iarg : std::max(1, c->kn.press_sweeps), darg);
+        case ATJUP_ADJ_OP_STEP:         return adj_step(c, iarg, darg != 0.0 ? 1 : 0);
+        default: set_error("adjoint operator"); return ATJUP_ERR_ARG;
+    }
+}
+
+int atjup_hip_adj_forward_checkpointed(atjup_ctx* c, int nsteps, int every, int first_iter){
+    if(int rc = need_adj(c)) return rc;
+    if(nsteps < 1 || every < 1){ set_error("nsteps, every"); return ATJUP_ERR_ARG; }
+    AdjState* A = c->adj;
+    for(auto& b : A->ckpt) adj_release(b);
+    for(auto& b : A->snap) adj_release(b);
+    const int nseg = (nsteps + every - 1) / every;
+    A->ckpt.assign((size_t)nseg * N_STATE, Buf{});
+    A->snap.assign((size_t)every * N_STATE, Buf{});
+    const size_t fb = c->n3 * sizeof(double);
+    for(auto& b : A->ckpt) if(int rc = adj_alloc(b, fb)) return rc;
+    for(auto& b : A->snap) if(int rc = adj_alloc(b, fb)) return rc;
+    A->nsteps = nsteps; A->every = every; A->first_iter = first_iter;
+    for(int n = 0; n < nsteps; n++){
+        if(n % every == 0)
+            if(int rc = save_state(c, &A->ckpt[(size_t)(n / every) * N_STATE])) return rc;
+        if(int rc = atjup_hip_step(c, first_iter + n, n == 0 ? 1 : 0)) return rc;
+    }
+    return 0;
+}
+
+int atjup_hip_adj_backward(atjup_ctx* c, const double* lam_final, double* lam_init){
+    if(int rc = need_adj(c)) return rc;
+    AdjState* A = c->adj;
+    if(A->nsteps < 1){ set_error("run atjup_hip_adj_forward_checkpointed first"); return ATJUP_ERR_STATE; }
+    if(lam_final){
+        if(int rc = atjup_hip_adj_zero(c)) return rc;
+        for(int f = 0; f < 4; f++)
+            if(int rc = host_to_pitch(c, L(c, f), lam_final + (size_t)f * c->im * c->jm * c->km)) return rc;
+    }
+    const int every = A->every, nsteps = A->nsteps;
+    const int nseg = (nsteps + every - 1) / every;
+    for(int seg = nseg - 1; seg >= 0; seg--){
+        const int s0 = seg * every, s1 = std::min(nsteps, s0 + every);
+        if(int rc = load_state(c, &A->ckpt[(size_t)seg * N_STATE])) return rc;
+        for(int n = s0; n < s1; n++){
+            if(int rc = save_state(c, &A->snap[(size_t)(n - s0) * N_STATE])) return rc;
+            if(n + 1 < s1)
+                if(int rc = atjup_hip_step(c, A->first_iter + n, n == 0 ? 1 : 0)) return rc;
+        }
+        for(int n = s1 - 1; n >= s0; n--){
+            if(int rc = load_state(c, &A->snap[(size_t)(n - s0) * N_STATE])) return rc;
+            if(int rc = adj_step(c, A->first_iter + n, n == 0 ? 1 : 0)) return rc;
+        }
+    }
+    if(lam_init)
+        for(int f = 0; f < 4; f++)
+            if(int rc = pitch_to_host(c, lam_init + (size_t)f * c->im * c->jm * c->km, L(c, f))) return rc;
+    return 0;
+}
+
+}  // extern "C"

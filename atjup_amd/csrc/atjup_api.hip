@@ -30,7 +30,11 @@ void release(atjup::Buf& b){ if(b.p) (void)hipFree(b.p); b.p = nullptr; b.bytes 
 
 int fill_value(atjup_ctx* c, atjup::Buf& b, double v){
     const size_t n = c->n3;
-    if(c->precision == ATJUP_PRECISION_FP64){
+    if(c->precision == ATJUP_PRECISION_TANGENT){
+        std::vector<double> h(2 * n, 0.0);
+        for(size_t q = 0; q < n; q++) h[2 * q] = v;
+        ATJUP_HIP_CHECK(hipMemcpy(b.p, h.data(), n * 16, hipMemcpyHostToDevice));
+    } else if(c->precision == ATJUP_PRECISION_FP64){
         std::vector<double> h(n, v);
         ATJUP_HIP_CHECK(hipMemcpy(b.p, h.data(), n * 8, hipMemcpyHostToDevice));
     } else {
@@ -106,8 +110,14 @@ void build_tables(atjup_ctx* c, std::vector<GeoIJ<T>>& geo, std::vector<PoisIJ<T
 int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, int precision){
     if(!g || !c || !k){ set_error("null argument"); return ATJUP_ERR_ARG; }
     if(g->im < 6 || g->jm < 9 || g->km < 8){ set_error("grid too small"); return ATJUP_ERR_ARG; }
+    if(g->km > 1024){ set_error("km above 1024"); return ATJUP_ERR_ARG; }
     if((g->km - 1) % 2 != 0){ set_error("the periodic ring km-1 must be even for a consistent red-black colouring"); return ATJUP_ERR_ARG; }
-    if(precision != ATJUP_PRECISION_FP64 && precision != ATJUP_PRECISION_FP32_REF){ set_error("precision"); return ATJUP_ERR_ARG; }
+    if(precision != ATJUP_PRECISION_FP64 && precision != ATJUP_PRECISION_FP32_REF && precision != ATJUP_PRECISION_TANGENT){
+        set_error("precision"); return ATJUP_ERR_ARG;
+    }
+    if(precision == ATJUP_PRECISION_TANGENT && !k->dry){
+        set_error("the tangent form covers the dynamics profile (dry = 1)"); return ATJUP_ERR_PROFILE;
+    }
     if(k->turb_model < -1 || k->turb_model > 2){ set_error("turb_model"); return ATJUP_ERR_PROFILE; }
     if(k->rk_scalar_sync != 2){ set_error("only ATJUP_RK_SCALAR_SYNC=2 is implemented"); return ATJUP_ERR_PROFILE; }
     if(k->dry && k->turb_model != -1){ set_error("the dry profile has the closure off"); return ATJUP_ERR_PROFILE; }
@@ -189,7 +199,8 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     atjup_ctx* c = new atjup_ctx();
     c->g = *g; c->c = *cs; c->kn = *kn; c->precision = precision;
     c->im = g->im; c->jm = g->jm; c->km = g->km;
-    c->esz = (precision == ATJUP_PRECISION_FP64) ? 8 : 4;
+    c->esz = (precision == ATJUP_PRECISION_FP64) ? 8 : (precision == ATJUP_PRECISION_FP32_REF ? 4 : 16);
+    const size_t asz = (precision == ATJUP_PRECISION_TANGENT) ? 16 : 8;      // FP64 quantities
     const int per128 = 128 / (int)c->esz;
     c->pitch = (c->km + per128 - 1) / per128 * per128;
     c->plane = (size_t)c->jm * c->pitch;
@@ -213,14 +224,22 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     }
     if(!rc && !dry){ rc = alloc(c->live_turb[0], fb); if(!rc) rc = alloc(c->live_turb[1], fb); }
     if(!rc) rc = alloc(c->scratch, fb);
-    if(!rc) rc = alloc(c->psrc, c->n3 * sizeof(double));
+    c->rb_hp = ((c->km + 1) / 2 + 15) / 16 * 16;
+    const size_t hn = (size_t)c->im * c->jm * c->rb_hp;
+    if(!rc) rc = alloc(c->psrc, 2 * hn * asz);
+    if(!rc) rc = alloc(c->rb_p, 2 * hn * asz);
+    if(!rc && precision == ATJUP_PRECISION_FP32_REF){
+        rc = alloc(c->rb_d, 2 * hn * sizeof(float));
+        if(!rc) rc = alloc(c->rb_r, 2 * hn * sizeof(float));
+    }
+    if(!rc) rc = alloc(c->rb_flags, 2 * hn);
     if(!rc) rc = alloc(c->mask, c->n3);
     if(!rc) rc = alloc(c->flags, c->n3);
     if(!rc) rc = alloc(c->itopo, (size_t)c->jm * c->km * sizeof(int));
     if(!rc) rc = alloc(c->vel_star, (size_t)c->jm * c->km * sizeof(double));
-    if(!rc) rc = alloc(c->brl, (size_t)c->im * sizeof(double));
-    if(!rc) rc = alloc(c->brl_part, (size_t)c->im * 64 * 2 * sizeof(double));
-    if(!rc) rc = alloc(c->zm, (size_t)4 * c->jm * sizeof(double));
+    if(!rc) rc = alloc(c->brl, (size_t)c->im * asz);
+    if(!rc) rc = alloc(c->brl_part, (size_t)c->im * 64 * 2 * asz);
+    if(!rc) rc = alloc(c->zm, (size_t)4 * c->jm * asz);
     if(!rc) rc = alloc(c->nonfinite, 64);
     if(!rc){
         // resetArrays: dis = 1e-10, rho_mix = 1 (planet/cJupiterModel.cpp:1719-1720, 1677)
@@ -229,7 +248,7 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     }
     if(!rc){
         std::vector<double> tabs;
-        if(precision == ATJUP_PRECISION_FP64){
+        if(precision != ATJUP_PRECISION_FP32_REF){
             std::vector<GeoIJ<double>> geo; std::vector<PoisIJ<double>> pois;
             build_tables<double>(c, geo, pois);
             rc = alloc(c->geo, geo.size() * sizeof(geo[0]));
@@ -271,16 +290,18 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
 void atjup_hip_ctx_destroy(atjup_ctx* c){
     if(!c) return;
     if(c->stream) (void)hipStreamSynchronize(c->stream);
+    adj_free(c);
     for(auto& b : c->cur) release(b);
     for(auto& s : c->stg) for(auto& b : s) release(b);
     for(auto& b : c->acc) release(b);
     for(auto& b : c->live_turb) release(b);
-    for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
+    for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
                           &c->amb_rows, &c->nonfinite })
         release(*b);
     for(auto& v : c->spans) for(auto& sp : v){ (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
     for(auto e : c->event_pool) (void)hipEventDestroy(e);
+    for(auto& g : c->sweep_graphs) (void)hipGraphExecDestroy((hipGraphExec_t)g.exec);
     if(c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -297,6 +318,7 @@ static int upload_impl(atjup_ctx* c, int f, const double* src, double* const* co
             const double* r = rows ? rows[i][j] : src + ((size_t)i * c->jm + j) * row;
             const size_t o = c->idx(i, j, 0);
             if(c->esz == 8) std::memcpy((double*)host.data() + o, r, row * 8);
+            else if(c->esz == 16){ double* d = (double*)host.data() + 2 * o; for(size_t k = 0; k < row; k++){ d[2 * k] = r[k]; d[2 * k + 1] = 0.0; } }
             else { float* d = (float*)host.data() + o; for(size_t k = 0; k < row; k++) d[k] = (float)r[k]; }
         }
     if(c->kn.dry && s >= ATJUP_F_H2O && s < ATJUP_F_TKE){
@@ -327,12 +349,38 @@ static int download_impl(atjup_ctx* c, int f, double* dst, double** const* rows)
             double* r = rows ? rows[i][j] : dst + ((size_t)i * c->jm + j) * row;
             const size_t o = c->idx(i, j, 0);
             if(c->esz == 8) std::memcpy(r, (const double*)host.data() + o, row * 8);
+            else if(c->esz == 16){ const double* d = (const double*)host.data() + 2 * o; for(size_t k = 0; k < row; k++) r[k] = d[2 * k]; }
             else { const float* d = (const float*)host.data() + o; for(size_t k = 0; k < row; k++) r[k] = (double)d[k]; }
         }
     return 0;
 }
 
 int atjup_hip_upload_field(atjup_ctx* c, int f, const double* src){ return upload_impl(c, f, src, nullptr); }
+
+// the directional-derivative part of a field of the tangent form; the value part is kept
+static int tangent_impl(atjup_ctx* c, int f, const double* src, double* dst){
+    if(!c || (!src && !dst)){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(c->precision != ATJUP_PRECISION_TANGENT){ set_error("not a tangent context"); return ATJUP_ERR_STATE; }
+    const int s = storage_of(f);
+    if(!(prog(s) || input(s)) || !c->cur[s].p){ set_error("field id"); return ATJUP_ERR_ARG; }
+    std::vector<double> host(2 * c->n3);
+    ATJUP_HIP_CHECK(hipMemcpyAsync(host.data(), live_ptr(c, s), c->n3 * 16, hipMemcpyDeviceToHost, c->stream));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for(int i = 0; i < c->im; i++)
+        for(int j = 0; j < c->jm; j++)
+            for(int k = 0; k < c->km; k++){
+                const size_t o = c->idx(i, j, k), h = ((size_t)i * c->jm + j) * c->km + k;
+                if(src) host[2 * o + 1] = src[h];
+                else dst[h] = host[2 * o + 1];
+            }
+    if(src){
+        ATJUP_HIP_CHECK(hipMemcpyAsync(c->cur[s].p, host.data(), c->n3 * 16, hipMemcpyHostToDevice, c->stream));
+        ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+int atjup_hip_upload_tangent(atjup_ctx* c, int f, const double* src){ return tangent_impl(c, f, src, nullptr); }
+int atjup_hip_download_tangent(atjup_ctx* c, int f, double* dst){ return tangent_impl(c, f, nullptr, dst); }
 int atjup_hip_download_field(atjup_ctx* c, int f, double* dst){ return download_impl(c, f, dst, nullptr); }
 int atjup_hip_upload_field_rows(atjup_ctx* c, int f, double* const* const* rows){ return upload_impl(c, f, nullptr, rows); }
 int atjup_hip_download_field_rows(atjup_ctx* c, int f, double** const* rows){ return download_impl(c, f, nullptr, rows); }
@@ -363,6 +411,12 @@ int atjup_hip_download_vel_star(atjup_ctx* c, double* d){
 int atjup_hip_download_buoy_ref_level(atjup_ctx* c, double* d){
     if(!c || !d){ set_error("null argument"); return ATJUP_ERR_ARG; }
     ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if(c->precision == ATJUP_PRECISION_TANGENT){
+        std::vector<double> h(2 * (size_t)c->im);
+        ATJUP_HIP_CHECK(hipMemcpy(h.data(), c->brl.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for(int i = 0; i < c->im; i++) d[i] = h[2 * i];
+        return 0;
+    }
     ATJUP_HIP_CHECK(hipMemcpy(d, c->brl.p, (size_t)c->im * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }

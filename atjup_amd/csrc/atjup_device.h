@@ -2,6 +2,11 @@
 #pragma once
 #include "atjup_internal.h"
 
+// 1: the sweep sequence of a pressure solve is captured once into a HIP graph and replayed
+#ifndef ATJUP_SWEEP_GRAPH
+#define ATJUP_SWEEP_GRAPH 1
+#endif
+
 namespace atjup {
 
 __device__ __forceinline__ int xcd_remap(int b, int nb){
@@ -26,6 +31,64 @@ template<typename T>
 __device__ __forceinline__ T dmin(T a, T b){ return (b < a) ? b : a; }
 
 inline int nblk(size_t n, int b){ return (int)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------------------------
+// Forward-mode derivative: a value with its directional derivative. The dynamics kernels are
+// templates on the storage type; instantiated on Dual they compute the tangent-linear step of the
+// FP64 path alongside the step itself (comparisons and branches follow the value, so the
+// derivative is the one of the branch the forward takes). The adjoint is checked against it.
+// ---------------------------------------------------------------------------------------------
+struct Dual {
+    double v, d;
+    Dual() = default;
+    __host__ __device__ constexpr Dual(double a) : v(a), d(0.0) {}
+    __host__ __device__ constexpr Dual(double a, double b) : v(a), d(b) {}
+    __host__ __device__ explicit operator double() const { return v; }
+    __host__ __device__ explicit operator float() const { return (float)v; }
+};
+__host__ __device__ inline Dual operator+(Dual a, Dual b){ return Dual(a.v + b.v, a.d + b.d); }
+__host__ __device__ inline Dual operator-(Dual a, Dual b){ return Dual(a.v - b.v, a.d - b.d); }
+__host__ __device__ inline Dual operator*(Dual a, Dual b){ return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__host__ __device__ inline Dual operator/(Dual a, Dual b){
+    const double q = a.v / b.v;
+    return Dual(q, (a.d - q * b.d) / b.v);
+}
+__host__ __device__ inline Dual operator-(Dual a){ return Dual(-a.v, -a.d); }
+__host__ __device__ inline Dual operator+(Dual a){ return a; }
+__host__ __device__ inline Dual operator+(Dual a, double b){ return Dual(a.v + b, a.d); }
+__host__ __device__ inline Dual operator+(double a, Dual b){ return Dual(a + b.v, b.d); }
+__host__ __device__ inline Dual operator-(Dual a, double b){ return Dual(a.v - b, a.d); }
+__host__ __device__ inline Dual operator-(double a, Dual b){ return Dual(a - b.v, -b.d); }
+__host__ __device__ inline Dual operator*(Dual a, double b){ return Dual(a.v * b, a.d * b); }
+__host__ __device__ inline Dual operator*(double a, Dual b){ return Dual(a * b.v, a * b.d); }
+__host__ __device__ inline Dual operator/(Dual a, double b){ return Dual(a.v / b, a.d / b); }
+__host__ __device__ inline Dual operator/(double a, Dual b){ const double q = a / b.v; return Dual(q, -q * b.d / b.v); }
+__host__ __device__ inline Dual& operator+=(Dual& a, Dual b){ a = a + b; return a; }
+__host__ __device__ inline Dual& operator-=(Dual& a, Dual b){ a = a - b; return a; }
+__host__ __device__ inline Dual& operator*=(Dual& a, Dual b){ a = a * b; return a; }
+#define ATJUP_DUAL_CMP(op) \
+    __host__ __device__ inline bool operator op(Dual a, Dual b){ return a.v op b.v; } \
+    __host__ __device__ inline bool operator op(Dual a, double b){ return a.v op b; } \
+    __host__ __device__ inline bool operator op(double a, Dual b){ return a op b.v; }
+ATJUP_DUAL_CMP(<) ATJUP_DUAL_CMP(>) ATJUP_DUAL_CMP(<=) ATJUP_DUAL_CMP(>=) ATJUP_DUAL_CMP(==) ATJUP_DUAL_CMP(!=)
+#undef ATJUP_DUAL_CMP
+// keep the library's overloads for the built-in types visible beside the Dual ones
+using ::isfinite;
+using ::sqrt;
+using ::fabs;
+__device__ inline bool isfinite(Dual a){ return ::isfinite(a.v); }
+__device__ inline Dual sqrt(Dual a){ const double r = ::sqrt(a.v); return Dual(r, a.d / (2.0 * r)); }
+__device__ inline Dual fabs(Dual a){ return (a.v < 0.0) ? -a : a; }
+
+// The type a kernel accumulates and forms its FP64 quantities in: FP64 for the FP64 and FP32
+// storage, the derivative-carrying type for the tangent form.
+template<typename T> struct acc_type { using type = double; };
+template<> struct acc_type<Dual> { using type = Dual; };
+template<typename T> using acc_t = typename acc_type<T>::type;
+// Metric tables: FP32 for the FP32 storage, FP64 otherwise.
+template<typename T> struct geo_type { using type = T; };
+template<> struct geo_type<Dual> { using type = double; };
+template<typename T> using geo_t = typename geo_type<T>::type;
 
 inline int check_launch(){
     hipError_t e = hipGetLastError();

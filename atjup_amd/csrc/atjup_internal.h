@@ -60,6 +60,8 @@ struct Buf {
 
 }  // namespace atjup
 
+namespace atjup { struct AdjState; }
+
 struct atjup_ctx {
     atjup_grid g;
     atjup_consts c;
@@ -78,7 +80,14 @@ struct atjup_ctx {
     atjup::Buf live_turb[2];     // tke / dis as Turbulence::run left them, before the RK4 (full profile)
     bool turb_live_pending = false;
     atjup::Buf scratch;          // one field, for the filters
-    atjup::Buf psrc;             // Poisson source, FP64 [n3]
+    atjup::Buf psrc;             // Poisson source, FP64, red-black split [2][im][jm][rb_hp]
+    atjup::Buf rb_p;             // p_dyn during a solve, FP64, red-black split [2][im][jm][rb_hp]
+    atjup::Buf rb_d, rb_r;       // FP32-refined solve: correction and FP64-formed residual, FP32, split
+    int refine_sweeps = 50;      // FP32 sweeps per refinement round
+    struct SweepGraph { int kind, n, ghost_first; double omega; void* exec; };
+    std::vector<SweepGraph> sweep_graphs;   // instantiated sweep sequences (hipGraphExec_t)
+    atjup::Buf rb_flags;         // face flags, red-black split [2][im][jm][rb_hp]
+    int rb_hp = 0;               // compact row length: (km + 1) / 2 rounded up to 128 bytes
     atjup::Buf mask;             // uint8 SeaMount [n3]
     atjup::Buf flags;            // uint8 face flags [n3]
     atjup::Buf itopo;            // int [jm * km]
@@ -106,6 +115,8 @@ struct atjup_ctx {
     std::vector<hipEvent_t> event_pool;
     double t_total[ATJUP_N_TIMERS] = {0};
     double t_launch[ATJUP_N_TIMERS] = {0};
+
+    atjup::AdjState* adj = nullptr;   // the adjoint's buffers (atjup_hip_adj_init)
 
     // host-side geometry
     std::vector<double> rad, the, sin_the, cos_the;
@@ -143,6 +154,12 @@ inline void* live_ptr(atjup_ctx* c, int f){
     if(c->turb_live_pending && (f == ATJUP_F_TKE || f == ATJUP_F_DIS)) return c->live_turb[f - ATJUP_F_TKE].p;
     return c->cur[f].p;
 }
+
+void adj_free(atjup_ctx* c);
+
+// the adjoint's recording RK4 run: the stage inputs Y1..Y3, the step's result and its aux_*
+struct RkBuffers { double* y[3][4]; double* nn[4]; double* aux[3]; };
+int launch_rk4_dry_record(atjup_ctx* c, const RkBuffers& rb);
 
 // phase timers: begin returns an event recorded now (or nullptr with timing off); end records the
 // closing event and files the span under `phase` with its kernel-launch count.

@@ -27,26 +27,27 @@ constexpr int BRL_BLOCK = 256;
 
 template<typename T>
 __global__ __launch_bounds__(BRL_BLOCK) void k_brl_partial(const T* __restrict__ t, const T* __restrict__ ps,
-        const uint8_t* __restrict__ flags, const double* __restrict__ sin_the, double* __restrict__ part,
+        const uint8_t* __restrict__ flags, const double* __restrict__ sin_the, acc_t<T>* __restrict__ part,
         int jm, int km, int pitch, double g, double rR, double t_ref){
+    using A = acc_t<T>;
     const int i = blockIdx.y, ch = blockIdx.x;
     const int ncell = jm * km;
     const int per = (ncell + BRL_NCH - 1) / BRL_NCH;
     const int c0 = ch * per, c1 = min(ncell, c0 + per);
-    double s = 0.0, w = 0.0;
+    A s = 0.0, w = 0.0;
     for(int c = c0 + threadIdx.x; c < c1; c += BRL_BLOCK){
         const int j = c / km, k = c - j * km;
         const size_t q = ((size_t)i * jm + j) * pitch + k;
         if(flags[q] & FL_SELF) continue;
-        const double tq = (double)t[q];
+        const A tq = (A)t[q];
         if(!(tq > 0.0)) continue;
-        const double b = g * (double)ps[q] / (rR * tq * t_ref);
+        const A b = g * (A)ps[q] / (rR * tq * t_ref);
         if(!isfinite(b)) continue;
         const double wgt = sin_the[j];
         s += wgt * b;
         w += wgt;
     }
-    __shared__ double sh[2][BRL_BLOCK];
+    __shared__ A sh[2][BRL_BLOCK];
     sh[0][threadIdx.x] = s;
     sh[1][threadIdx.x] = w;
     __syncthreads();
@@ -63,15 +64,16 @@ __global__ __launch_bounds__(BRL_BLOCK) void k_brl_partial(const T* __restrict__
     }
 }
 
-__global__ void k_brl_final(const double* __restrict__ part, double* __restrict__ brl, int im){
+template<typename A>
+__global__ void k_brl_final(const A* __restrict__ part, A* __restrict__ brl, int im){
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= im) return;
-    double s = 0.0, w = 0.0;
+    A s = 0.0, w = 0.0;
     for(int ch = 0; ch < BRL_NCH; ch++){
         s += part[((size_t)i * BRL_NCH + ch) * 2 + 0];
         w += part[((size_t)i * BRL_NCH + ch) * 2 + 1];
     }
-    brl[i] = (w > 0.0) ? s / w : 0.0;
+    brl[i] = (w > 0.0) ? s / w : A(0.0);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -93,18 +95,20 @@ __global__ __launch_bounds__(256) void k_hydrostatic(const T* __restrict__ v, co
     int i1 = i0;
     while(i1 + 1 < im && !(flags[q0 + (i1 + 1) * plane] & FL_SELF)) i1++;
     for(int i = i1 + 1; i < im; i++) ph[q0 + i * plane] = T(0);
-    const T cor_ = (T)cor_j[j];
-    auto f_ = [&](int i) -> T {
-        const T vv = v[q0 + i * plane], ww = w[q0 + i * plane];
-        return T(0) + ((vv * vv + ww * ww) / (T)rad[i] + cor_ * ww) / T(1);
+    // the column integral is formed in FP64 in both forms
+    using A = acc_t<T>;
+    const double cor_ = cor_j[j];
+    auto f_ = [&](int i) -> A {
+        const A vv = (A)v[q0 + i * plane], ww = (A)w[q0 + i * plane];
+        return 0.0 + ((vv * vv + ww * ww) / rad[i] + cor_ * ww) / 1.0;
     };
-    T prev = T(0);
-    ph[q0 + i0 * plane] = prev;
-    T fprev = f_(i0);
+    A prev = 0.0;
+    ph[q0 + i0 * plane] = (T)prev;
+    A fprev = f_(i0);
     for(int i = i0 + 1; i <= i1; i++){
-        const T fi = f_(i);
-        prev = prev + T(0.5) * (fprev + fi) * (T)dr;
-        ph[q0 + i * plane] = prev;
+        const A fi = f_(i);
+        prev = prev + 0.5 * (fprev + fi) * dr;
+        ph[q0 + i * plane] = (T)prev;
         fprev = fi;
     }
 }
@@ -126,14 +130,15 @@ struct StageArgs {
     const T *pd, *ph, *ps, *rho, *wn;
     T *aux_u, *aux_v, *aux_w;
     const uint8_t* flags;
-    const GeoIJ<T>* geo;
-    const double* brl;
+    const GeoIJ<geo_t<T>>* geo;
+    const acc_t<T>* brl;
     int im, jm, km, pitch, nkb, njb, nb;
     size_t plane;
     int stage, copy_boundary;
     T c_in, wgt, dt, one_sixth;
     T inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
-    T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, cbuoy, g, rR, t_ref, kt, visc0;
+    double cbuoy, g, rR, t_ref_d;          // the buoyancy anomaly is formed in FP64 in both forms
+    T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, t_ref, kt, visc0;
 };
 
 constexpr int ST_BK = 64, ST_BJ = 4;
@@ -162,7 +167,7 @@ __global__ __launch_bounds__(ST_BK * ST_BJ) void k_stage_dry(const StageArgs<T> 
     const int kw = (k == 0) ? a.km - 2 : k - 1;
     const int ke = (k == a.km - 2) ? 0 : k + 1;
     const size_t qkw = q - k + kw, qke = q - k + ke;
-    const GeoIJ<T> G = a.geo[(size_t)i * a.jm + j];
+    const GeoIJ<geo_t<T>> G = a.geo[(size_t)i * a.jm + j];
 
     T c_[4], d1r[4], d2r[4], d1t[4], d2t[4], d1p[4], d2p[4];
     #pragma unroll
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(ST_BK * ST_BJ) void k_stage_dry(const StageArgs<T> 
 
     const T rhs_t = + pressure_t - transport_t + diffusion_t * a.kt;
     const T buoyancy_u = (tq > T(0))
-        ? a.cbuoy * (a.g * a.ps[q] / (a.rR * tq * a.t_ref) - (T)a.brl[i]) : T(0);
+        ? (T)(a.cbuoy * (a.g * (acc_t<T>)a.ps[q] / (a.rR * (acc_t<T>)tq * a.t_ref_d) - a.brl[i])) : T(0);
     const T visc = a.visc0 + a.wn[q];
     const T rhs_u = - dpdr + buoyancy_u - transport_u + diffusion_u * visc
         + a.centrifugal * G.cent_rad;
@@ -287,11 +292,11 @@ __global__ void k_prep_projection(T* __restrict__ au, T* __restrict__ av, T* __r
 // The divergence source of every cell of the sweep domain, computed once per call: it depends
 // only on aux_* and the mask, which the sweeps do not change. Clamped to +-den * PDYN_CAP with the
 // cell's own denominator (PressureSolver.h:370-487).
-template<typename T>
+template<typename T, typename I>
 __global__ __launch_bounds__(256) void k_poisson_source(const T* __restrict__ au, const T* __restrict__ av,
         const T* __restrict__ aw, const uint8_t* __restrict__ flags, const uint8_t* __restrict__ msk,
-        const PoisIJ<double>* __restrict__ pw, double* __restrict__ src,
-        int im, int jm, int km, int pitch, size_t plane, int nb,
+        const PoisIJ<double>* __restrict__ pw, I* __restrict__ src0, I* __restrict__ src1,
+        int im, int jm, int km, int pitch, size_t plane, int nb, int hp,
         double inv_2dr, double inv_2dthe, double inv_2dphi, double inv_dthe, double inv_dphi, double cap){
     const int b = xcd_remap(blockIdx.x, nb);
     const int nkb = (km - 1 + 63) / 64;
@@ -307,56 +312,56 @@ __global__ __launch_bounds__(256) void k_poisson_source(const T* __restrict__ au
     const bool skp = fl & FL_KP, skm = fl & FL_KM;
     const bool skp2 = (k < km - 2) ? (msk[q + 2] != 0) : false;
     const bool skm2 = (k > 2) ? (msk[q - 2] != 0) : false;
-    double du_dr = 0.0, dv_dthe = 0.0, dw_dphi = 0.0;
+    I du_dr = 0.0, dv_dthe = 0.0, dw_dphi = 0.0;
     bool r_flag = false, the_flag = false, phi_flag = false;
     if(i < im - 2 && sld && !(fl & FL_IP)){
-        du_dr = (-3.0 * (double)au[q] + 4.0 * (double)au[q + P] - (double)au[q + 2 * P]) * inv_2dr;
+        du_dr = (-3.0 * (I)au[q] + 4.0 * (I)au[q + P] - (I)au[q + 2 * P]) * inv_2dr;
         r_flag = true;
     }
     if(j > 2 && j < jm - 2){
         const bool air_jp1 = !(fl & FL_JP), air_jm1 = !(fl & FL_JM);
         const bool sjp2 = msk[q + 2 * R] != 0, sjm2 = msk[q - 2 * R] != 0;
         if(sld && air_jp1 && !sjp2){
-            dv_dthe = (-3.0 * (double)av[q] + 4.0 * (double)av[q + R] - (double)av[q + 2 * R]) * inv_2dthe;
+            dv_dthe = (-3.0 * (I)av[q] + 4.0 * (I)av[q + R] - (I)av[q + 2 * R]) * inv_2dthe;
             the_flag = true;
         }
         if(sld && air_jm1 && !sjm2){
-            dv_dthe = -(-3.0 * (double)av[q] + 4.0 * (double)av[q - R] - (double)av[q - 2 * R]) * inv_2dthe;
+            dv_dthe = -(-3.0 * (I)av[q] + 4.0 * (I)av[q - R] - (I)av[q - 2 * R]) * inv_2dthe;
             the_flag = true;
         }
         if((sld && air_jp1 && sjp2) || (j == jm - 2 && !sld && (fl & FL_JP))){
-            dv_dthe = ((double)av[q + R] - (double)av[q]) * inv_dthe;
+            dv_dthe = ((I)av[q + R] - (I)av[q]) * inv_dthe;
             the_flag = true;
         }
         if((sld && air_jm1 && sjm2) || (j == 1 && sld && air_jm1)){
-            dv_dthe = ((double)av[q - R] - (double)av[q]) * inv_dthe;
+            dv_dthe = ((I)av[q - R] - (I)av[q]) * inv_dthe;
             the_flag = true;
         }
     }
     if(k > 2 && k < km - 2){
         const bool air_kp1 = !skp, air_km1 = !skm;
         if(sld && air_kp1 && !skp2){
-            dw_dphi = (-3.0 * (double)aw[q] + 4.0 * (double)aw[q + 1] - (double)aw[q + 2]) * inv_2dphi;
+            dw_dphi = (-3.0 * (I)aw[q] + 4.0 * (I)aw[q + 1] - (I)aw[q + 2]) * inv_2dphi;
             phi_flag = true;
         }
         if(sld && air_km1 && !skm2){
-            dw_dphi = -(-3.0 * (double)aw[q] + 4.0 * (double)aw[q - 1] - (double)aw[q - 2]) * inv_2dphi;
+            dw_dphi = -(-3.0 * (I)aw[q] + 4.0 * (I)aw[q - 1] - (I)aw[q - 2]) * inv_2dphi;
             phi_flag = true;
         }
         if((sld && air_kp1 && skp2) || (k == km - 2 && !sld && skp)){
-            dw_dphi = ((double)aw[q + 1] - (double)aw[q]) * inv_dphi;
+            dw_dphi = ((I)aw[q + 1] - (I)aw[q]) * inv_dphi;
             phi_flag = true;
         }
         if((sld && air_km1 && skm2) || (k == 1 && sld && air_km1)){
-            dw_dphi = ((double)aw[q - 1] - (double)aw[q]) * inv_dphi;
+            dw_dphi = ((I)aw[q - 1] - (I)aw[q]) * inv_dphi;
             phi_flag = true;
         }
     }
-    if(!r_flag) du_dr = ((double)au[q + P] - (double)au[q - P]) * inv_2dr;
-    if(!the_flag) dv_dthe = ((double)av[q + R] - (double)av[q - R]) * inv_2dthe;
-    if(!phi_flag) dw_dphi = ((double)aw[q - k + ke] - (double)aw[q - k + kw]) * inv_2dphi;
+    if(!r_flag) du_dr = ((I)au[q + P] - (I)au[q - P]) * inv_2dr;
+    if(!the_flag) dv_dthe = ((I)av[q + R] - (I)av[q - R]) * inv_2dthe;
+    if(!phi_flag) dw_dphi = ((I)aw[q - k + ke] - (I)aw[q - k + kw]) * inv_2dphi;
     const PoisIJ<double> W = pw[(size_t)i * jm + j];
-    double div_src = du_dr * 1.0 + dv_dthe * W.inv_rm + dw_dphi * W.inv_rmsinthe;
+    I div_src = du_dr * 1.0 + dv_dthe * W.inv_rm + dw_dphi * W.inv_rmsinthe;
     double den = W.denom;
     if(!sld){
         den = 0.0;
@@ -368,68 +373,216 @@ __global__ __launch_bounds__(256) void k_poisson_source(const T* __restrict__ au
         if(!skm) den += W.num3;
     }
     const double src_max = den * cap;
-    if(!isfinite(div_src)) div_src = 0.0;
-    else if(div_src > src_max) div_src = src_max;
-    else if(div_src < -src_max) div_src = -src_max;
-    src[q] = div_src;
+    if(!isfinite(div_src)) div_src = I(0.0);
+    else if(div_src > src_max) div_src = I(src_max);
+    else if(div_src < -src_max) div_src = I(-src_max);
+    I* src = ((i + j + k) & 1) ? src1 : src0;
+    src[((size_t)i * jm + j) * hp + (k >> 1)] = div_src;
 }
 
-// One colour pass of a red-black sweep. Thread m of row (i, j) owns k = k0 + 2m, the cells of
-// this colour. The domain boundary planes are the outer condition of the relaxation; the
-// reference rewrites them after every sweep (PressureSolver.h:718-746). Here a plane value is
-// only ever needed by its one interior neighbour, so from the second sweep on it is formed from
-// the interior values it is defined by: a colour-0 reader forms it itself (both sources are
-// unchanged at that point), and the colour-0 thread that owns the second source cell stores it
-// just before updating that cell, for the colour-1 reader. The stored planes are therefore
-// exactly what the reference would hold at every read; all planes are rewritten once at the end.
-template<typename T>
-__global__ __launch_bounds__(64) void k_poisson_sweep(T* __restrict__ p, const double* __restrict__ src,
-        const uint8_t* __restrict__ flags, const PoisIJ<double>* __restrict__ pw,
-        int im, int jm, int km, int pitch, size_t plane, int nb, int colour, int ghost, double omega,
+// Red-black split storage. The sweep domain's ring of km-1 meridians holds (km-1)/2 cells of each
+// colour per (i, j) row; colour c = (i + j + k) & 1 lives in its own array X_c at compact index
+// m = k >> 1 (row length hp, the seam cell k = km-1 at m = (km-1)/2). A colour pass then reads the
+// other colour's array and its own source and flags with unit stride and writes its own array
+// with whole cache lines. In a row where the own colour sits at k = 2m + off, the phi neighbours
+// are the other colour's m - 1 + off and m + off (periodic on the ring); the i and j neighbours
+// are the other colour's m.
+template<typename T, typename U>
+__global__ void k_rb_pack(const T* __restrict__ p, U* __restrict__ x0, U* __restrict__ x1,
+        int jm, int km, int pitch, int hp){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    U* x = ((i + j + k) & 1) ? x1 : x0;
+    x[((size_t)i * jm + j) * hp + (k >> 1)] = (U)p[((size_t)i * jm + j) * pitch + k];
+}
+template<typename T, typename U>
+__global__ void k_rb_unpack(T* __restrict__ p, const U* __restrict__ x0, const U* __restrict__ x1,
+        int jm, int km, int pitch, int hp){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const U* x = ((i + j + k) & 1) ? x1 : x0;
+    p[((size_t)i * jm + j) * pitch + k] = (T)x[((size_t)i * jm + j) * hp + (k >> 1)];
+}
+__global__ void k_rb_flags(const uint8_t* __restrict__ flags, uint8_t* __restrict__ f0, uint8_t* __restrict__ f1,
+        int jm, int km, int pitch, int hp){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    uint8_t* f = ((i + j + k) & 1) ? f1 : f0;
+    f[((size_t)i * jm + j) * hp + (k >> 1)] = flags[((size_t)i * jm + j) * pitch + k];
+}
+
+// One colour pass of a red-black sweep (PressureSolver.h:323-500) on the split storage. The domain
+// boundary planes are the outer condition of the relaxation; the reference rewrites them after
+// every sweep (PressureSolver.h:718-746). Here a plane value is only ever needed by its one
+// interior neighbour, so from the second sweep on it is formed from the interior values it is
+// defined by: a colour-0 reader forms it itself (both sources are unchanged at that point), and
+// the colour-0 thread that owns the second source cell stores it just before updating that cell,
+// for the colour-1 reader. The stored planes are therefore exactly what the reference would hold
+// at every read; all planes are rewritten once at the end.
+constexpr int RB_BM = 64, RB_BJ = 4, RB_NI = 1;
+
+// A thread may own RB_NI consecutive levels of one (j, m) column of its colour, loading the other
+// colour's column segment i0-1 .. i0+RB_NI once; measured on MI355X, one level per thread (the
+// most waves in flight) is the fastest, 11.5 us per colour pass against 13.1 us with four.
+template<typename T, typename S>
+__global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep(T* __restrict__ x0, T* __restrict__ x1,
+        const S* __restrict__ s0, const S* __restrict__ s1,
+        const uint8_t* __restrict__ f0, const uint8_t* __restrict__ f1, const PoisIJ<double>* __restrict__ pw,
+        int im, int jm, int hp, int nm, int nmb, int njb, int nb, int colour, int ghost, double omega,
         double c43, double c13){
     const int b = xcd_remap(blockIdx.x, nb);
-    const int nmb = ((km - 1) / 2 + 63) / 64;
     const int mb = b % nmb, rest = b / nmb;
-    const int j = rest % (jm - 2) + 1, i = rest / (jm - 2) + 1;
-    const int m = mb * 64 + threadIdx.x;
-    const int k = ((colour + i + j) & 1) + 2 * m;
-    if(k > km - 2 || i > im - 2) return;
-    const size_t P = plane, R = (size_t)pitch;
-    const size_t q = ((size_t)i * jm + j) * pitch + k;
-    const T pc = p[q];
-    const bool gw = ghost && colour == 0;      // ghost writes in this pass
-    const bool gr = ghost && colour == 0;      // on-the-fly ghost reads in this pass
-    if(gw){
-        if(i == 2)      p[q - 2 * P] = (T)((T)c43 * p[q - P] - (T)c13 * pc);
-        if(i == im - 3) p[q + 2 * P] = (T)((T)c43 * p[q + P] - (T)c13 * pc);
-        if(j == 2)      p[q - 2 * R] = (T)((T)c43 * p[q - R] - (T)c13 * pc);
-        if(j == jm - 3) p[q + 2 * R] = (T)((T)c43 * p[q + R] - (T)c13 * pc);
+    const int jb = rest % njb, ib = rest / njb;
+    const int m = mb * RB_BM + threadIdx.x;
+    const int j = jb * RB_BJ + threadIdx.y + 1;
+    const int i0 = ib * RB_NI + 1;
+    if(m >= nm || j > jm - 2 || i0 > im - 2) return;
+    T* __restrict__ own = colour ? x1 : x0;
+    const T* __restrict__ oth = colour ? x0 : x1;
+    const S* __restrict__ src = colour ? s1 : s0;
+    const uint8_t* __restrict__ flg = colour ? f1 : f0;
+    const size_t P = (size_t)jm * hp, R = (size_t)hp;
+    const bool gw = ghost && colour == 0;      // ghost writes and on-the-fly ghost reads in this pass
+    const int n_here = min(RB_NI, im - 2 - i0 + 1);
+    T col[RB_NI + 2];
+    #pragma unroll
+    for(int t = 0; t < RB_NI + 2; t++){
+        const int i = i0 - 1 + t;
+        col[t] = (t <= n_here + 1) ? oth[((size_t)i * jm + j) * hp + m] : T(0);
     }
-    const int kw = (k == 0) ? km - 2 : k - 1, ke = (k == km - 2) ? 0 : k + 1;
-    const T pip = (i == im - 2 && gr) ? (T)((T)c43 * pc - (T)c13 * p[q - P]) : p[q + P];
-    const T pim = (i == 1 && gr)      ? (T)((T)c43 * pc - (T)c13 * p[q + P]) : p[q - P];
-    const T pjp = (j == jm - 2 && gr) ? (T)((T)c43 * pc - (T)c13 * p[q - R]) : p[q + R];
-    const T pjm = (j == 1 && gr)      ? (T)((T)c43 * pc - (T)c13 * p[q + R]) : p[q - R];
-    const T pkp = p[q - k + ke], pkm = p[q - k + kw];
-    const uint8_t fl = flags[q];
+    #pragma unroll
+    for(int t = 0; t < RB_NI; t++){
+        if(t >= n_here) break;
+        const int i = i0 + t;
+        const size_t row = ((size_t)i * jm + j) * hp;
+        const size_t q = row + m;
+        const int off = (colour + i + j) & 1;
+        const int mw = off ? m : (m == 0 ? nm - 1 : m - 1);
+        const int me = off ? (m == nm - 1 ? 0 : m + 1) : m;
+        const bool edge = i == 1 || i == 2 || i == im - 3 || i == im - 2 || j == 1 || j == 2 || j == jm - 3 || j == jm - 2;
+        const bool need_pc = (omega != 1.0) || (gw && edge);
+        const T pc = need_pc ? own[q] : T(0);
+        if(gw){
+            if(i == 2)      own[q - 2 * P] = (T)((T)c43 * col[t] - (T)c13 * pc);
+            if(i == im - 3) own[q + 2 * P] = (T)((T)c43 * col[t + 2] - (T)c13 * pc);
+            if(j == 2)      own[q - 2 * R] = (T)((T)c43 * oth[q - R] - (T)c13 * pc);
+            if(j == jm - 3) own[q + 2 * R] = (T)((T)c43 * oth[q + R] - (T)c13 * pc);
+        }
+        const T pip = (i == im - 2 && gw) ? (T)((T)c43 * pc - (T)c13 * col[t]) : col[t + 2];
+        const T pim = (i == 1 && gw)      ? (T)((T)c43 * pc - (T)c13 * col[t + 2]) : col[t];
+        const T pjp = (j == jm - 2 && gw) ? (T)((T)c43 * pc - (T)c13 * oth[q - R]) : oth[q + R];
+        const T pjm = (j == 1 && gw)      ? (T)((T)c43 * pc - (T)c13 * oth[q + R]) : oth[q - R];
+        const T pkp = oth[row + me], pkm = oth[row + mw];
+        const uint8_t fl = flg[q];
+        const PoisIJ<double> W = pw[(size_t)i * jm + j];
+        const T num1 = (T)W.num1, num2 = (T)W.num2, num3 = (T)W.num3;
+        T acc, den;
+        bool skip = false;
+        if(!(fl & FL_SELF)){
+            acc = den = T(0);
+            if(!(fl & FL_IP)){ acc += num1 * pip; den += num1; }
+            if(!(fl & FL_IM)){ acc += num1 * pim; den += num1; }
+            if(!(fl & FL_JP)){ acc += num2 * pjp; den += num2; }
+            if(!(fl & FL_JM)){ acc += num2 * pjm; den += num2; }
+            if(!(fl & FL_KP)){ acc += num3 * pkp; den += num3; }
+            if(!(fl & FL_KM)){ acc += num3 * pkm; den += num3; }
+            skip = !(den > T(0));
+        } else {
+            acc = (pip + pim) * num1 + (pjp + pjm) * num2 + (pkp + pkm) * num3;
+            den = (T)W.denom;
+        }
+        if(skip) continue;
+        const T p_gs = (acc - (T)src[q]) / den;
+        own[q] = (omega == 1.0) ? p_gs : (T)((T(1) - (T)omega) * pc + (T)omega * p_gs);
+    }
+}
+
+// ---- the FP32-refined solve: FP32 sweeps on the correction inside FP64 refinement ----
+// A red-black sweep is an affine map, so from a start p0 the iterates are p0 + d with d relaxed by
+// the same sweep on zero start and source r0 = src + den p0 - sum(w p0_nb), the cell's FP64
+// Gauss-Seidel residual of p0 (solid cells: all six neighbours with the full denominator, as in
+// the sweep). Each refinement round forms r0 in FP64, relaxes d in FP32 and adds it in FP64; with
+// the outer planes of p0 consistent (c43 p1 - c13 p2) the planes of d start consistent at zero,
+// so the ghost-plane scheme applies from the round's first sweep.
+__global__ __launch_bounds__(256) void k_rb_residual(const double* __restrict__ x0, const double* __restrict__ x1,
+        const double* __restrict__ s0, const double* __restrict__ s1,
+        const uint8_t* __restrict__ f0, const uint8_t* __restrict__ f1, const PoisIJ<double>* __restrict__ pw,
+        float* __restrict__ r0, float* __restrict__ r1, int im, int jm, int hp, int nm){
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1;
+    const int i = (blockIdx.z >> 1) + 1, colour = blockIdx.z & 1;
+    if(m >= nm || j > jm - 2 || i > im - 2) return;
+    const double* own = colour ? x1 : x0;
+    const double* oth = colour ? x0 : x1;
+    const size_t P = (size_t)jm * hp, R = (size_t)hp;
+    const size_t row = ((size_t)i * jm + j) * hp;
+    const size_t q = row + m;
+    const int off = (colour + i + j) & 1;
+    const int mw = off ? m : (m == 0 ? nm - 1 : m - 1);
+    const int me = off ? (m == nm - 1 ? 0 : m + 1) : m;
+    const double pip = oth[q + P], pim = oth[q - P], pjp = oth[q + R], pjm = oth[q - R];
+    const double pkp = oth[row + me], pkm = oth[row + mw];
+    const uint8_t fl = (colour ? f1 : f0)[q];
     const PoisIJ<double> W = pw[(size_t)i * jm + j];
-    const T num1 = (T)W.num1, num2 = (T)W.num2, num3 = (T)W.num3;
-    T acc, den;
+    double acc, den;
     if(!(fl & FL_SELF)){
-        acc = den = T(0);
-        if(!(fl & FL_IP)){ acc += num1 * pip; den += num1; }
-        if(!(fl & FL_IM)){ acc += num1 * pim; den += num1; }
-        if(!(fl & FL_JP)){ acc += num2 * pjp; den += num2; }
-        if(!(fl & FL_JM)){ acc += num2 * pjm; den += num2; }
-        if(!(fl & FL_KP)){ acc += num3 * pkp; den += num3; }
-        if(!(fl & FL_KM)){ acc += num3 * pkm; den += num3; }
-        if(den <= T(0)) return;
+        acc = den = 0.0;
+        if(!(fl & FL_IP)){ acc += W.num1 * pip; den += W.num1; }
+        if(!(fl & FL_IM)){ acc += W.num1 * pim; den += W.num1; }
+        if(!(fl & FL_JP)){ acc += W.num2 * pjp; den += W.num2; }
+        if(!(fl & FL_JM)){ acc += W.num2 * pjm; den += W.num2; }
+        if(!(fl & FL_KP)){ acc += W.num3 * pkp; den += W.num3; }
+        if(!(fl & FL_KM)){ acc += W.num3 * pkm; den += W.num3; }
     } else {
-        acc = (pip + pim) * num1 + (pjp + pjm) * num2 + (pkp + pkm) * num3;
-        den = (T)W.denom;
+        acc = (pip + pim) * W.num1 + (pjp + pjm) * W.num2 + (pkp + pkm) * W.num3;
+        den = W.denom;
     }
-    const T p_gs = (acc - (T)src[q]) / den;
-    p[q] = (omega == 1.0) ? p_gs : (T)((T(1) - (T)omega) * pc + (T)omega * p_gs);
+    (colour ? r1 : r0)[q] = (float)((colour ? s1 : s0)[q] + den * own[q] - acc);
+}
+
+// p0 += d over the sweep domain, both colours; d is cleared for the next round
+__global__ __launch_bounds__(256) void k_rb_add(double* __restrict__ x0, double* __restrict__ x1,
+        float* __restrict__ d0, float* __restrict__ d1, int im, int jm, int hp, int nm){
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1;
+    const int i = (blockIdx.z >> 1) + 1, colour = blockIdx.z & 1;
+    if(m >= nm || j > jm - 2 || i > im - 2) return;
+    const size_t q = ((size_t)i * jm + j) * hp + m;
+    float* d = colour ? d1 : d0;
+    (colour ? x1 : x0)[q] += (double)d[q];
+    d[q] = 0.0f;
+}
+
+__global__ void k_fill_float(float* __restrict__ x, size_t n, float v){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q < n) x[q] = v;
+}
+
+// the outer planes of the split iterate, i planes then j planes (PressureSolver.h:718-746)
+__global__ void k_rb_planes_i(double* __restrict__ x0, double* __restrict__ x1, int im, int jm, int hp, int nm,
+        double c43, double c13){
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, side = blockIdx.z;
+    if(m >= nm) return;
+    const size_t P = (size_t)jm * hp;
+    // plane cell (ia, j, k) has the colour of (ib2, j, k); (ib1, j, k) the other one
+    const int ia = side ? im - 1 : 0, ib1 = side ? im - 2 : 1, ib2 = side ? im - 3 : 2;
+    for(int c = 0; c < 2; c++){
+        double* same = c ? x1 : x0;
+        const double* other = c ? x0 : x1;
+        const size_t base = (size_t)j * hp + m;
+        same[base + ia * P] = c43 * other[base + ib1 * P] - c13 * same[base + ib2 * P];
+    }
+}
+__global__ void k_rb_planes_j(double* __restrict__ x0, double* __restrict__ x1, int im, int jm, int hp, int nm,
+        double c43, double c13){
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y, side = blockIdx.z;
+    if(m >= nm) return;
+    const size_t R = (size_t)hp;
+    const int ja = side ? jm - 1 : 0, jb1 = side ? jm - 2 : 1, jb2 = side ? jm - 3 : 2;
+    for(int c = 0; c < 2; c++){
+        double* same = c ? x1 : x0;
+        const double* other = c ? x0 : x1;
+        const size_t base = (size_t)i * jm * hp + m;
+        same[base + ja * R] = c43 * other[base + jb1 * R] - c13 * same[base + jb2 * R];
+    }
 }
 
 // the outer boundary after the last sweep: i planes, then j planes, then the seam copy
@@ -569,10 +722,11 @@ __global__ __launch_bounds__(256) void k_mixture_density(const T* __restrict__ t
         const T* __restrict__ pd, T* __restrict__ rho, size_t n, double t_ref, double pdyn_w, double ptb, double R_mix){
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(q >= n) return;
-    const double T_ = (double)t[q] * t_ref;
+    using A = acc_t<T>;
+    const A T_ = (A)t[q] * t_ref;
     if(!(T_ > 0.0)){ rho[q] = T(0); return; }
-    const double p_pa = ((double)ps[q] + pdyn_w * (double)pd[q] * ptb) * 1.0e5;
-    const double r = p_pa / (R_mix * T_);
+    const A p_pa = ((A)ps[q] + pdyn_w * (A)pd[q] * ptb) * 1.0e5;
+    const A r = p_pa / (R_mix * T_);
     rho[q] = isfinite(r) ? (T)r : T(0);
 }
 
@@ -593,7 +747,7 @@ struct BCLists {
 
 // the zonal means of v, w at i = im-3, im-2 before the top sponge, k = 0..km-2 in order
 template<typename T>
-__global__ void k_bc_zonal_means(const T* __restrict__ v, const T* __restrict__ w, double* __restrict__ zm,
+__global__ void k_bc_zonal_means(const T* __restrict__ v, const T* __restrict__ w, acc_t<T>* __restrict__ zm,
         int im, int jm, int km, int pitch){
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int sel = blockIdx.y;                    // 0..3 = (layer 0 v, layer 0 w, layer 1 v, layer 1 w)
@@ -601,15 +755,15 @@ __global__ void k_bc_zonal_means(const T* __restrict__ v, const T* __restrict__ 
     const int l = sel >> 1;
     const T* X = (sel & 1) ? w : v;
     const size_t r0 = ((size_t)(im - 3 + l) * jm + j) * pitch;
-    double s = 0.0;
-    for(int k = 0; k < km - 1; k++) s += (double)X[r0 + k];
-    zm[(size_t)sel * jm + j] = s / (km - 1);
+    acc_t<T> s = 0.0;
+    for(int k = 0; k < km - 1; k++) s += (acc_t<T>)X[r0 + k];
+    zm[(size_t)sel * jm + j] = s / (double)(km - 1);
 }
 
 // BC_Jup::bcRadius (BC_Jup.h:185-263) with the shared bcRadius (BoundaryConditions.h:151-286),
 // ATJUP defaults: 2-point Neumann, rigid lid on u, TOP_TAPER 3, species wall copy, positivity.
 template<typename T>
-__global__ __launch_bounds__(256) void k_bc_radius(const BCLists<T> L, const double* __restrict__ zm,
+__global__ __launch_bounds__(256) void k_bc_radius(const BCLists<T> L, const acc_t<T>* __restrict__ zm,
         int im, int jm, int km, int pitch, size_t plane, double c43, double c13){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if(k >= km) return;
@@ -622,7 +776,7 @@ __global__ __launch_bounds__(256) void k_bc_radius(const BCLists<T> L, const dou
     }
     L.u[q0] = T(0);
     L.u[qt] = T(0);
-    const double v3 = zm[j], w3 = zm[(size_t)jm + j], v2 = zm[(size_t)2 * jm + j], w2 = zm[(size_t)3 * jm + j];
+    const acc_t<T> v3 = zm[j], w3 = zm[(size_t)jm + j], v2 = zm[(size_t)2 * jm + j], w2 = zm[(size_t)3 * jm + j];
     T* V = L.v; T* W = L.w;
     V[qt] = T(0); W[qt] = T(0);
     V[qt - plane] *= (T)(1.0 / 3.0);
@@ -709,22 +863,37 @@ __global__ void k_bc_solid(const BCLists<T> L, const SolidCell* __restrict__ cel
     }
 }
 
-// MOUNT_T_AMBIENT (BC_Jup.h:593-608): the solid cells of a row take the mean t of its fluid cells
+// MOUNT_T_AMBIENT (BC_Jup.h:593-608): the solid cells of a row take the mean t of its fluid cells.
+// One block per row: the row is staged in LDS by all lanes, one lane sums it in k order (the
+// reference's order, so the mean is bit-identical), all lanes write the solid cells.
+constexpr int AMB_MAXK = 1024;
 template<typename T>
-__global__ void k_bc_t_ambient(T* __restrict__ t, const uint8_t* __restrict__ msk, const int2* __restrict__ rows,
-        int n, int jm, int km, int pitch){
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_bc_t_ambient(T* __restrict__ t, const uint8_t* __restrict__ msk,
+        const int2* __restrict__ rows, int n, int jm, int km, int pitch){
+    const int r = blockIdx.x;
     if(r >= n) return;
+    using A = acc_t<T>;
+    __shared__ A sv[AMB_MAXK];
+    __shared__ uint8_t sm[AMB_MAXK];
+    __shared__ A s_amb;
+    __shared__ int s_cnt;
     const int2 ij = rows[r];
     const size_t r0 = ((size_t)ij.x * jm + ij.y) * pitch;
-    double s = 0.0; int cnt = 0;
-    for(int k = 0; k < km; k++){
-        if(msk[r0 + k]) continue;
-        s += (double)t[r0 + k]; cnt++;
+    for(int k = threadIdx.x; k < km; k += 64){ sv[k] = (A)t[r0 + k]; sm[k] = msk[r0 + k]; }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        A s = 0.0; int cnt = 0;
+        for(int k = 0; k < km; k++){
+            if(sm[k]) continue;
+            s += sv[k]; cnt++;
+        }
+        s_amb = (cnt > 0) ? s / (double)cnt : A(0.0);
+        s_cnt = cnt;
     }
-    if(cnt == 0) return;
-    const T t_amb = (T)(s / cnt);
-    for(int k = 0; k < km; k++) if(msk[r0 + k]) t[r0 + k] = t_amb;
+    __syncthreads();
+    if(s_cnt == 0) return;
+    const T t_amb = (T)s_amb;
+    for(int k = threadIdx.x; k < km; k += 64) if(sm[k]) t[r0 + k] = t_amb;
 }
 
 // clampNegativeSpecies (FileIO_Jup.cpp:494-548)
@@ -841,10 +1010,10 @@ template<typename T>
 int do_brl(atjup_ctx* c){
     hipLaunchKernelGGL(k_brl_partial<T>, dim3(BRL_NCH, c->im), dim3(BRL_BLOCK), 0, c->stream,
         F<T>(c, ATJUP_F_T), F<T>(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p,
-        (const double*)c->tabs.p + c->im, (double*)c->brl_part.p,
+        (const double*)c->tabs.p + c->im, (acc_t<T>*)c->brl_part.p,
         c->jm, c->km, c->pitch, c->c.g, c->c.r_mix * c->c.R_mix, c->c.t_ref);
-    hipLaunchKernelGGL(k_brl_final, dim3(1), dim3(64), 0, c->stream,
-        (const double*)c->brl_part.p, (double*)c->brl.p, c->im);
+    hipLaunchKernelGGL(k_brl_final<acc_t<T>>, dim3(1), dim3(64), 0, c->stream,
+        (const acc_t<T>*)c->brl_part.p, (acc_t<T>*)c->brl.p, c->im);
     return check_launch();
 }
 
@@ -858,15 +1027,15 @@ int do_hydro(atjup_ctx* c){
 }
 
 template<typename T>
-int do_rk4_dry(atjup_ctx* c){
+int do_rk4_dry(atjup_ctx* c, const RkBuffers* rb = nullptr){
     StageArgs<T> a{};
     const double dt = c->c.dt;
     a.pd = F<T>(c, ATJUP_F_P_DYN); a.ph = F<T>(c, ATJUP_F_P_HYDRO); a.ps = F<T>(c, ATJUP_F_P_STAT);
     a.rho = F<T>(c, ATJUP_F_RHO_MIX); a.wn = F<T>(c, ATJUP_F_WALL_NUE);
     a.aux_u = F<T>(c, ATJUP_F_AUX_U); a.aux_v = F<T>(c, ATJUP_F_AUX_V); a.aux_w = F<T>(c, ATJUP_F_AUX_W);
     a.flags = (const uint8_t*)c->flags.p;
-    a.geo = (const GeoIJ<T>*)c->geo.p;
-    a.brl = (const double*)c->brl.p;
+    a.geo = (const GeoIJ<geo_t<T>>*)c->geo.p;
+    a.brl = (const acc_t<T>*)c->brl.p;
     a.im = c->im; a.jm = c->jm; a.km = c->km; a.pitch = c->pitch; a.plane = c->plane;
     a.nkb = (c->km + ST_BK - 1) / ST_BK; a.njb = (c->jm + ST_BJ - 1) / ST_BJ;
     a.nb = a.nkb * a.njb * c->im;
@@ -881,8 +1050,8 @@ int do_rk4_dry(atjup_ctx* c){
     a.c_m2o = (T)(1.0 * -2.0 * C.omega);
     a.c_p2o = (T)(1.0 * +2.0 * C.omega);
     a.Coriolis = (T)C.Coriolis; a.centrifugal = (T)C.centrifugal;
-    a.cbuoy = (T)(-1.0 * 1.0 * 1.0 * C.buoyancy);
-    a.g = (T)C.g; a.rR = (T)(C.r_mix * C.R_mix); a.t_ref = (T)C.t_ref;
+    a.cbuoy = -1.0 * 1.0 * 1.0 * C.buoyancy;
+    a.g = C.g; a.rR = C.r_mix * C.R_mix; a.t_ref_d = C.t_ref; a.t_ref = (T)C.t_ref;
     a.kt = (T)(c->kn.t_diff_scale / (C.re * C.pr) + 0.0 / 0.9);
     a.visc0 = (T)(1.0 / C.re + 0.0);
     for(int s = 0; s < 4; s++){
@@ -898,12 +1067,68 @@ int do_rk4_dry(atjup_ctx* c){
             T* out = (s == 0) ? X : (s == 1 ? Y : (s == 2 ? X : N));
             a.in[f] = in; a.out[f] = out; a.nl[f] = N; a.acc[f] = (T*)c->acc[f].p;
         }
+        if(rb){
+            // the recording run of the adjoint: every stage input kept (Y1, Y2, Y3), the step's
+            // result in its own buffers, the n level and aux_* of the context left as they were
+            a.copy_boundary = 1;
+            for(int f = 0; f < 4; f++){
+                a.in[f] = (s == 0) ? F<T>(c, f) : (const T*)rb->y[s - 1][f];
+                a.out[f] = (s < 3) ? (T*)rb->y[s][f] : (T*)rb->nn[f];
+            }
+            a.aux_u = (T*)rb->aux[0]; a.aux_v = (T*)rb->aux[1]; a.aux_w = (T*)rb->aux[2];
+        }
         hipEvent_t t0 = timer_begin(c);
         hipLaunchKernelGGL(k_stage_dry<T>, dim3(a.nb), dim3(ST_BK, ST_BJ), 0, c->stream, a);
         timer_end(c, ATJUP_T_RK_STAGES, t0, 1);
         if(int rc = check_launch()) return rc;
     }
     return 0;
+}
+
+// The colour kernels of n sweeps, replayed from a HIP graph captured on first use (one per sweep
+// count, relaxation factor and kind), so that the 2n dependent launches of a solve go to the
+// device as one submission.
+template<typename T, typename S>
+int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* x0, T* x1, const S* s0, const S* s1){
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    const int hp = c->rb_hp;
+    const size_t hn = (size_t)c->im * c->jm * hp;
+    const uint8_t* f0 = (const uint8_t*)c->rb_flags.p;
+    const uint8_t* f1 = f0 + hn;
+    const PoisIJ<double>* pw = (const PoisIJ<double>*)c->pois.p;
+    const int nm = (c->km - 1) / 2;
+    const int nmb = (nm + RB_BM - 1) / RB_BM, njb = (c->jm - 2 + RB_BJ - 1) / RB_BJ;
+    const int nib = (c->im - 2 + RB_NI - 1) / RB_NI;
+    const int nb_sw = nmb * njb * nib;
+    auto launch_all = [&](){
+        for(int s = 0; s < n; s++)
+            for(int colour = 0; colour < 2; colour++)
+                hipLaunchKernelGGL((k_rb_sweep<T, S>), dim3(nb_sw), dim3(RB_BM, RB_BJ), 0, c->stream,
+                    x0, x1, s0, s1, f0, f1, pw, c->im, c->jm, hp, nm, nmb, njb, nb_sw, colour,
+                    (s > 0 || ghost_first) ? 1 : 0, omega, c43, c13);
+    };
+#if ATJUP_SWEEP_GRAPH
+    hipGraphExec_t exec = nullptr;
+    for(auto& g : c->sweep_graphs)
+        if(g.kind == kind && g.n == n && g.ghost_first == ghost_first && g.omega == omega){ exec = (hipGraphExec_t)g.exec; break; }
+    if(!exec){
+        hipGraph_t graph = nullptr;
+        ATJUP_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        launch_all();
+        const hipError_t le = hipGetLastError();
+        ATJUP_HIP_CHECK(hipStreamEndCapture(c->stream, &graph));
+        if(le != hipSuccess){ (void)hipGraphDestroy(graph); set_error(std::string("sweep capture: ") + hipGetErrorString(le)); return ATJUP_ERR_HIP; }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if(ie != hipSuccess){ set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); return ATJUP_ERR_HIP; }
+        c->sweep_graphs.push_back({ kind, n, ghost_first, omega, (void*)exec });
+    }
+    ATJUP_HIP_CHECK(hipGraphLaunch(exec, c->stream));
+    return 0;
+#else
+    launch_all();
+    return check_launch();
+#endif
 }
 
 template<typename T>
@@ -915,24 +1140,59 @@ int do_pressure(atjup_ctx* c, int sweeps, double omega){
         c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
     const int nkb = (c->km - 1 + 63) / 64;
     const int nb_src = nkb * (c->jm - 2) * (c->im - 2);
-    hipLaunchKernelGGL(k_poisson_source<T>, dim3(nb_src), dim3(64), 0, c->stream,
+    const int hp = c->rb_hp;
+    const size_t hn = (size_t)c->im * c->jm * hp;
+    using I = acc_t<T>;                       // the iterate: FP64 in the FP64 and FP32 forms
+    I* s0 = (I*)c->psrc.p;
+    I* s1 = s0 + hn;
+    I* x0 = (I*)c->rb_p.p;
+    I* x1 = x0 + hn;
+    const PoisIJ<double>* pw = (const PoisIJ<double>*)c->pois.p;
+    hipLaunchKernelGGL((k_poisson_source<T, I>), dim3(nb_src), dim3(64), 0, c->stream,
         F<T>(c, ATJUP_F_AUX_U), F<T>(c, ATJUP_F_AUX_V), F<T>(c, ATJUP_F_AUX_W),
-        (const uint8_t*)c->flags.p, (const uint8_t*)c->mask.p, (const PoisIJ<double>*)c->pois.p,
-        (double*)c->psrc.p, c->im, c->jm, c->km, c->pitch, c->plane, nb_src,
+        (const uint8_t*)c->flags.p, (const uint8_t*)c->mask.p, pw,
+        s0, s1, c->im, c->jm, c->km, c->pitch, c->plane, nb_src, hp,
         1.0 / (2.0 * c->g.dr), 1.0 / (2.0 * c->g.dthe), 1.0 / (2.0 * c->g.dphi),
         1.0 / c->g.dthe, 1.0 / c->g.dphi, c->kn.pdyn_cap);
+    dim3 gfull(nblk(c->km, 128), c->jm, c->im), b128(128);
+    hipLaunchKernelGGL((k_rb_pack<T, I>), gfull, b128, 0, c->stream, (const T*)p, x0, x1, c->jm, c->km, c->pitch, hp);
     if(int rc = check_launch()) return rc;
-    const int nmb = ((c->km - 1) / 2 + 63) / 64;
-    const int nb_sw = nmb * (c->jm - 2) * (c->im - 2);
+    const int nm = (c->km - 1) / 2;
+    const bool refined = c->precision == ATJUP_PRECISION_FP32_REF;
+    // the FP64 sweeps: all of them in the drop-in; the first one in the refined form, whose planes
+    // are those of p_dyn at entry (PressureSolver.h:323-358)
+    const int n64 = refined ? 1 : sweeps;
     hipEvent_t t0 = timer_begin(c);
-    for(int s = 0; s < sweeps; s++){
-        for(int colour = 0; colour < 2; colour++){
-            hipLaunchKernelGGL(k_poisson_sweep<T>, dim3(nb_sw), dim3(64), 0, c->stream,
-                p, (const double*)c->psrc.p, (const uint8_t*)c->flags.p, (const PoisIJ<double>*)c->pois.p,
-                c->im, c->jm, c->km, c->pitch, c->plane, nb_sw, colour, s > 0 ? 1 : 0, omega, c43, c13);
+    if(int rc = run_sweeps<I, I>(c, 0, n64, 0, omega, x0, x1, (const I*)s0, (const I*)s1)) return rc;
+    timer_end(c, ATJUP_T_POISSON_SWEEPS, t0, 2 * n64);
+    if constexpr(std::is_same<T, float>::value) if(refined && sweeps > 1){
+        float* d0 = (float*)c->rb_d.p;
+        float* d1 = d0 + hn;
+        float* r0 = (float*)c->rb_r.p;
+        float* r1 = r0 + hn;
+        const uint8_t* f0 = (const uint8_t*)c->rb_flags.p;
+        const uint8_t* f1 = f0 + hn;
+        dim3 gm(nblk(nm, 128), c->jm - 2, 2 * (c->im - 2));
+        hipLaunchKernelGGL(k_fill_float, dim3(nblk(2 * hn, 256)), dim3(256), 0, c->stream, d0, 2 * hn, 0.0f);
+        int left = sweeps - 1;
+        while(left > 0){
+            const int n = std::min(left, c->refine_sweeps);
+            left -= n;
+            hipLaunchKernelGGL(k_rb_planes_i, dim3(nblk(nm, 128), c->jm, 2), b128, 0, c->stream, x0, x1,
+                c->im, c->jm, hp, nm, c43, c13);
+            hipLaunchKernelGGL(k_rb_planes_j, dim3(nblk(nm, 128), c->im, 2), b128, 0, c->stream, x0, x1,
+                c->im, c->jm, hp, nm, c43, c13);
+            hipLaunchKernelGGL(k_rb_residual, gm, b128, 0, c->stream, (const double*)x0, (const double*)x1,
+                (const double*)s0, (const double*)s1, f0, f1, pw, r0, r1, c->im, c->jm, hp, nm);
+            hipEvent_t t1 = timer_begin(c);
+            if(int rc = run_sweeps<float, float>(c, 1, n, 1, omega, d0, d1, (const float*)r0, (const float*)r1)) return rc;
+            timer_end(c, ATJUP_T_POISSON_SWEEPS, t1, 2 * n);
+            hipLaunchKernelGGL(k_rb_add, gm, b128, 0, c->stream, x0, x1, d0, d1, c->im, c->jm, hp, nm);
+            if(int rc = check_launch()) return rc;
         }
     }
-    timer_end(c, ATJUP_T_POISSON_SWEEPS, t0, 2 * sweeps);
+    hipLaunchKernelGGL((k_rb_unpack<T, I>), gfull, b128, 0, c->stream, p, (const I*)x0, (const I*)x1,
+        c->jm, c->km, c->pitch, hp);
     hipLaunchKernelGGL(k_poisson_planes_i<T>, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream,
         p, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
     hipLaunchKernelGGL(k_poisson_planes_j<T>, dim3(nblk(c->km, 128), c->im), dim3(128), 0, c->stream,
@@ -1006,9 +1266,9 @@ int do_bc(atjup_ctx* c){
     const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
     const BCLists<T> L = bc_lists<T>(c);
     hipLaunchKernelGGL(k_bc_zonal_means<T>, dim3(nblk(c->jm, 64), 4), dim3(64), 0, c->stream,
-        L.v, L.w, (double*)c->zm.p, c->im, c->jm, c->km, c->pitch);
+        L.v, L.w, (acc_t<T>*)c->zm.p, c->im, c->jm, c->km, c->pitch);
     hipLaunchKernelGGL(k_bc_radius<T>, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream,
-        L, (const double*)c->zm.p, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
+        L, (const acc_t<T>*)c->zm.p, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
     hipLaunchKernelGGL(k_bc_theta<T>, dim3(nblk(c->km, 128), c->im), dim3(128), 0, c->stream,
         L, c->im, c->jm, c->km, c->pitch);
     hipLaunchKernelGGL(k_bc_phi<T>, dim3(nblk(c->jm, 64), c->im), dim3(64), 0, c->stream,
@@ -1017,7 +1277,7 @@ int do_bc(atjup_ctx* c){
         hipLaunchKernelGGL(k_bc_solid<T>, dim3(nblk(c->n_solids, 64)), dim3(64), 0, c->stream,
             L, (const SolidCell*)c->solids.p, c->n_solids, c->jm, c->pitch, c->plane, c43, c13);
     if(c->n_amb_rows > 0)
-        hipLaunchKernelGGL(k_bc_t_ambient<T>, dim3(nblk(c->n_amb_rows, 64)), dim3(64), 0, c->stream,
+        hipLaunchKernelGGL(k_bc_t_ambient<T>, dim3(c->n_amb_rows), dim3(64), 0, c->stream,
             L.t, (const uint8_t*)c->mask.p, (const int2*)c->amb_rows.p, c->n_amb_rows, c->jm, c->km, c->pitch);
     if(int rc = check_launch()) return rc;
     if(!c->kn.dry) return launch_turb_sanitize(c);
@@ -1037,6 +1297,12 @@ int do_mask_setup(atjup_ctx* c){
     dim3 grid(nblk(c->km, 128), c->jm, c->im), blk(128);
     hipLaunchKernelGGL(k_mask_flags, grid, blk, 0, c->stream, (const uint8_t*)c->mask.p, (uint8_t*)c->flags.p,
         c->im, c->jm, c->km, c->pitch, c->plane);
+    {
+        const size_t hn = (size_t)c->im * c->jm * c->rb_hp;
+        uint8_t* f0 = (uint8_t*)c->rb_flags.p;
+        hipLaunchKernelGGL(k_rb_flags, grid, blk, 0, c->stream, (const uint8_t*)c->flags.p, f0, f0 + hn,
+            c->jm, c->km, c->pitch, c->rb_hp);
+    }
     hipLaunchKernelGGL(k_itopo, dim3(nblk(c->km, 128), c->jm), blk, 0, c->stream, (const uint8_t*)c->mask.p,
         (int*)c->itopo.p, c->im, c->jm, c->km, c->pitch, c->plane);
     const int layers = c->kn.wall_nue_layers;
@@ -1097,7 +1363,8 @@ int do_check_finite(atjup_ctx* c){
 
 }  // namespace
 
-#define DISPATCH(fn, ...) ((c->precision == ATJUP_PRECISION_FP64) ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+#define DISPATCH(fn, ...) ((c->precision == ATJUP_PRECISION_FP64) ? fn<double>(__VA_ARGS__) :          \
+                          (c->precision == ATJUP_PRECISION_FP32_REF) ? fn<float>(__VA_ARGS__) : fn<Dual>(__VA_ARGS__))
 
 int launch_buoy_ref_level(atjup_ctx* c){ return DISPATCH(do_brl, c); }
 int launch_hydrostatic(atjup_ctx* c){ return DISPATCH(do_hydro, c); }
@@ -1115,6 +1382,12 @@ int launch_shapiro(atjup_ctx* c, int field, int order, double strength, int pass
 int launch_mixture_density(atjup_ctx* c){ return DISPATCH(do_mixture, c); }
 int launch_clamp_species(atjup_ctx* c){ return DISPATCH(do_clamp, c); }
 int launch_mask_setup(atjup_ctx* c){ return DISPATCH(do_mask_setup, c); }
+int launch_rk4_dry_record(atjup_ctx* c, const RkBuffers& rb){
+    if(c->precision != ATJUP_PRECISION_FP64 || !c->kn.dry){ set_error("the adjoint covers the FP64 dynamics profile"); return ATJUP_ERR_PROFILE; }
+    if(int rc = launch_buoy_ref_level(c)) return rc;
+    if(int rc = launch_hydrostatic(c)) return rc;
+    return do_rk4_dry<double>(c, &rb);
+}
 int launch_check_finite(atjup_ctx* c){ return DISPATCH(do_check_finite, c); }
 
 }  // namespace atjup

@@ -1,6 +1,7 @@
 // HIP kernels of the moist, turbulent time step (the full-physics profile) for gfx950:
 //   * the RK4 stage with the RHS of all 17 prognostic fields (RHSJup, RHS_Jup_Turb.cpp:31-1457,
-//     with the k-omega sources) fused with the stage accumulation and the stage input;
+//     with the k-omega sources) fused with the stage accumulation and the stage input, as two
+//     kernels per stage: t, u, v, w, k*, omega*, then the eleven species;
 //   * Turbulence<P>::run with the k-omega closure and the wall-function omega Run() applies;
 //   * SaturationAdjustment<P>::run, one thread per cell and gas;
 //   * FluxLimiter<P>::nh4sh;
@@ -40,18 +41,22 @@ struct StageFullArgs {
     T *aux_u, *aux_v, *aux_w, *prod, *tsrc, *dsrc;
     const uint8_t* flags;
     const GeoIJ<T>* geo;
-    const double* brl;
+    const acc_t<T>* brl;
     int im, jm, km, pitch, nkb, njb, nb;
     size_t plane;
     int stage, turb;
     T c_in, wgt, dt, one_sixth;
     T inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
-    T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, cbuoy, g, rR, t_ref;
+    double cbuoy, g, rR, t_ref_d;          // the buoyancy anomaly is formed in FP64 in both forms
+    T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, t_ref;
     T t_coef, inv_re, rad_coupling, precip_coupling, turb_coupling, L_rad, R_mix, cp_mix, u_0;
     T cr, pc, stokes_a, rg_nh4sh, mue_mix, nue_max, tke_max_nd, dis_min;
 };
 
 constexpr int SF_BK = 64, SF_BJ = 4;
+#ifndef SF_MIN_WAVES
+#define SF_MIN_WAVES 1
+#endif
 
 template<typename T>
 struct Stencil7 { T c, ip, im, jp, jm, kp, km; };
@@ -66,12 +71,107 @@ __device__ __forceinline__ Stencil7<T> load7(const T* __restrict__ X, size_t q, 
 }
 
 template<typename T>
-__global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_full(const StageFullArgs<T> a){
+__device__ __forceinline__ T species_rhs(const StageFullArgs<T>& a, int f, size_t q, size_t P, size_t R,
+        size_t qke, size_t qkw, const GeoIJ<T>& G, T u, T v_invrm, T w_invrs, T two_inv_rm, T nue_t_s, T rho_f){
+    const Stencil7<T> s = load7(a.in[f], q, P, R, qke, qkw);
+    const T e1r = (s.ip - s.im) * a.inv_2dr, e2r = (s.ip - T(2) * s.c + s.im) * a.inv_dr2;
+    const T e1t = (s.jp - s.jm) * a.inv_2dthe, e2t = (s.jp - T(2) * s.c + s.jm) * a.inv_dthe2;
+    const T e1p = (s.kp - s.km) * a.inv_2dphi, e2p = (s.kp - T(2) * s.c + s.km) * a.inv_dphi2;
+    const T transport = u * e1r + v_invrm * e1t + w_invrs * e1p;
+    const T diffusion = e2r + e1r * two_inv_rm + e2t * G.inv_rm2
+        + e1t * G.costhe_inv_rm2sinthe + e2p * G.inv_rm2sinthe2;
+    T rhs;
+    if(f == ATJUP_F_NH4SH){
+        const T v_stokes = a.stokes_a * (a.rg_nh4sh - rho_f) * (T)a.g / a.mue_mix;
+        rhs = -transport
+            + a.fl_nh4sh[q]
+            + diffusion * (a.sp_coef[f] + nue_t_s)
+            + a.cr * a.mf[f][q]
+            + (v_stokes / a.u_0) * e1r;
+    } else {
+        rhs = -transport + diffusion * (a.sp_coef[f] + nue_t_s);
+        if(a.mf[f]) rhs = rhs + a.cr * a.mf[f][q];
+        if(a.spr[f]) rhs = rhs + a.pc * a.spr[f][q];
+    }
+    return rhs;
+}
+
+// the stage update of one field (RungeKutta_Jup_Turb.cpp:420-568); kind 0: plain, 1: species
+// (max(0, .) at the end), 2: nh4sh (< 0 -> 0 at the end), 3: k* (safe_clamp to [0, tke_max]),
+// 4: omega* (max(dis_min, .))
+template<typename T>
+__device__ __forceinline__ void stage_update(const StageFullArgs<T>& a, int f, size_t q, T rhs, int kind){
+    T accf;
+    if(a.stage == 0) accf = a.wgt * rhs;
+    else accf = a.acc[f][q] + a.wgt * rhs;
+    if(a.stage < 3){
+        a.acc[f][q] = accf;
+        T v = a.nl[f][q] + a.c_in * rhs;
+        if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
+        else if(kind == 4) v = dmax(a.dis_min, v);
+        a.out[f][q] = v;
+    } else {
+        T v = a.nl[f][q] + a.dt * accf * a.one_sixth;
+        if(kind == 1) v = dmax(T(0), v);
+        else if(kind == 2){ if(v < T(0)) v = T(0); }
+        else if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
+        else if(kind == 4) v = dmax(a.dis_min, v);
+        a.out[f][q] = v;
+    }
+}
+
+// The eleven species (RHS_Jup_Turb.cpp:1273-1338) in a kernel of their own: they need only the
+// centre velocities, the eddy viscosity and the density of the cell, so this kernel runs with few
+// registers and many waves while the dynamics kernel carries the heavy part.
+template<typename T>
+__global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_species(const StageFullArgs<T> a){
     const int b = xcd_remap(blockIdx.x, a.nb);
-    const int kb = b % a.nkb;
-    const int rest = b / a.nkb;
-    const int jb = rest % a.njb;
-    const int i = rest / a.njb;
+    const int i = b % a.im;
+    const int rest = b / a.im;
+    const int kb = rest % a.nkb;
+    const int jb = rest / a.nkb;
+    const int k = kb * SF_BK + threadIdx.x;
+    const int j = jb * SF_BJ + threadIdx.y;
+    if(k >= a.km || j >= a.jm) return;
+    const size_t q = ((size_t)i * a.jm + j) * a.pitch + k;
+    const uint8_t fl = a.flags[q];
+    const bool interior = i >= 1 && i <= a.im - 2 && j >= 3 && j <= a.jm - 4 && k <= a.km - 2 && !(fl & FL_SELF);
+    if(!interior){
+        #pragma unroll
+        for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++)
+            if(a.bsrc[f]) a.out[f][q] = a.bsrc[f][q];
+        return;
+    }
+    const size_t P = a.plane, R = (size_t)a.pitch;
+    const int kw = (k == 0) ? a.km - 2 : k - 1;
+    const int ke = (k == a.km - 2) ? 0 : k + 1;
+    const size_t qkw = q - k + kw, qke = q - k + ke;
+    const GeoIJ<T> G = a.geo[(size_t)i * a.jm + j];
+    const T u = a.in[ATJUP_F_U][q], v = a.in[ATJUP_F_V][q], w = a.in[ATJUP_F_W][q];
+    const T v_invrm = v * G.inv_rm;
+    const T w_invrs = w * G.inv_rmsinthe;
+    const T two_inv_rm = T(2) * G.inv_rm;
+    const T rho_f = a.rho[q];
+    const T nue_c = a.nue[q];
+    const T nue_t = (a.turb_coupling != T(0) && isfinite(nue_c)) ? a.turb_coupling * dmax(T(0), nue_c) : T(0);
+    const T nue_t_s = nue_t / T(0.9);
+    #pragma unroll
+    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++){
+        const T rhs = species_rhs(a, f, q, P, R, qke, qkw, G, u, v_invrm, w_invrs, two_inv_rm, nue_t_s, rho_f);
+        stage_update(a, f, q, rhs, f == ATJUP_F_NH4SH ? 2 : 1);
+    }
+}
+
+template<typename T>
+__global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(const StageFullArgs<T> a){
+    const int b = xcd_remap(blockIdx.x, a.nb);
+    // levels fastest: the blocks resident on an XCD at one time cover whole columns, so the i +- 1
+    // neighbours of the 17 fields a block reads are in flight in the same L2 (4 % faster per stage
+    // on MI355X than k tiles fastest; the 4-field dry stage prefers k tiles fastest)
+    const int i = b % a.im;
+    const int rest = b / a.im;
+    const int kb = rest % a.nkb;
+    const int jb = rest / a.nkb;
     const int k = kb * SF_BK + threadIdx.x;
     const int j = jb * SF_BJ + threadIdx.y;
     if(k >= a.km || j >= a.jm) return;
@@ -81,7 +181,7 @@ __global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_full(const StageFullArg
     if(!interior){
         #pragma unroll
         for(int f = 0; f < NP; f++)
-            if(a.bsrc[f]) a.out[f][q] = a.bsrc[f][q];
+            if((f < ATJUP_F_H2O || f > ATJUP_F_NH4SH) && a.bsrc[f]) a.out[f][q] = a.bsrc[f][q];
         return;
     }
     const size_t P = a.plane, R = (size_t)a.pitch;
@@ -90,29 +190,7 @@ __global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_full(const StageFullArg
     const size_t qkw = q - k + kw, qke = q - k + ke;
     const GeoIJ<T> G = a.geo[(size_t)i * a.jm + j];
     const int stage = a.stage;
-
-    // the stage update of one field (RungeKutta_Jup_Turb.cpp:420-568)
-    auto update = [&](int f, T rhs, int kind){
-        // kind 0: plain, 1: species (max(0, .) at the end), 2: nh4sh (< 0 -> 0 at the end),
-        // 3: k* (safe_clamp to [0, tke_max]), 4: omega* (max(dis_min, .))
-        T accf;
-        if(stage == 0) accf = a.wgt * rhs;
-        else accf = a.acc[f][q] + a.wgt * rhs;
-        if(stage < 3){
-            a.acc[f][q] = accf;
-            T v = a.nl[f][q] + a.c_in * rhs;
-            if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
-            else if(kind == 4) v = dmax(a.dis_min, v);
-            a.out[f][q] = v;
-        } else {
-            T v = a.nl[f][q] + a.dt * accf * a.one_sixth;
-            if(kind == 1) v = dmax(T(0), v);
-            else if(kind == 2){ if(v < T(0)) v = T(0); }
-            else if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
-            else if(kind == 4) v = dmax(a.dis_min, v);
-            a.out[f][q] = v;
-        }
-    };
+    auto update = [&](int f, T rhs, int kind){ stage_update(a, f, q, rhs, kind); };
 
     // ---- derivatives of t, u, v, w (RHS_Jup_Turb.cpp:180-254) ----
     T c_[4], d1r[4], d2r[4], d1t[4], d2t[4], d1p[4], d2p[4];
@@ -204,7 +282,8 @@ __global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_full(const StageFullArg
     const T visc = a.inv_re + nue_t + a.wn[q];
 
     const T rhs_t = + pressure_t - transport_t + diffusion_t * (a.t_coef + nue_t_s) + radiation_t + precip_t;
-    const T buoyancy_u = (tq > T(0)) ? a.cbuoy * (a.g * ps_c / (a.rR * tq * a.t_ref) - (T)a.brl[i]) : T(0);
+    const T buoyancy_u = (tq > T(0))
+        ? (T)(a.cbuoy * (a.g * (double)ps_c / (a.rR * (double)tq * a.t_ref_d) - a.brl[i])) : T(0);
     const T rhs_u = - dpdr + buoyancy_u - transport_u + diffusion_u * visc + a.centrifugal * G.cent_rad;
     const T rhs_v = - dpdthe_term - dphdthe_term - transport_v + diffusion_v * visc
         - a.Coriolis * Coriolis_the + a.centrifugal * G.cent_the;
@@ -323,32 +402,6 @@ __global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_full(const StageFullArg
         update(ATJUP_F_DIS, rhs_d, 4);
     }
 
-    // ---- the eleven species (RHS_Jup_Turb.cpp:1273-1338) ----
-    #pragma unroll 1
-    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++){
-        const Stencil7<T> s = load7(a.in[f], q, P, R, qke, qkw);
-        const T e1r = (s.ip - s.im) * a.inv_2dr, e2r = (s.ip - T(2) * s.c + s.im) * a.inv_dr2;
-        const T e1t = (s.jp - s.jm) * a.inv_2dthe, e2t = (s.jp - T(2) * s.c + s.jm) * a.inv_dthe2;
-        const T e1p = (s.kp - s.km) * a.inv_2dphi, e2p = (s.kp - T(2) * s.c + s.km) * a.inv_dphi2;
-        const T transport = u * e1r + v_invrm * e1t + w_invrs * e1p;
-        const T diffusion = e2r + e1r * two_inv_rm + e2t * G.inv_rm2
-            + e1t * G.costhe_inv_rm2sinthe + e2p * G.inv_rm2sinthe2;
-        T rhs;
-        if(f == ATJUP_F_NH4SH){
-            const T v_stokes = a.stokes_a * (a.rg_nh4sh - rho_f) * a.g / a.mue_mix;
-            rhs = -transport
-                + a.fl_nh4sh[q]
-                + diffusion * (a.sp_coef[f] + nue_t_s)
-                + a.cr * a.mf[f][q]
-                + (v_stokes / a.u_0) * e1r;
-            update(f, rhs, 2);
-        } else {
-            rhs = -transport + diffusion * (a.sp_coef[f] + nue_t_s);
-            if(a.mf[f]) rhs = rhs + a.cr * a.mf[f][q];
-            if(a.spr[f]) rhs = rhs + a.pc * a.spr[f][q];
-            update(f, rhs, 1);
-        }
-    }
 }
 
 // =============================================================================================
@@ -869,7 +922,7 @@ int do_rk4_full(atjup_ctx* c){
     a.prod = F<T>(c, ATJUP_F_PROD); a.tsrc = F<T>(c, ATJUP_F_TKE_SOURCE); a.dsrc = F<T>(c, ATJUP_F_DIS_SOURCE);
     a.flags = (const uint8_t*)c->flags.p;
     a.geo = (const GeoIJ<T>*)c->geo.p;
-    a.brl = (const double*)c->brl.p;
+    a.brl = (const acc_t<T>*)c->brl.p;
     a.im = c->im; a.jm = c->jm; a.km = c->km; a.pitch = c->pitch; a.plane = c->plane;
     a.nkb = (c->km + SF_BK - 1) / SF_BK; a.njb = (c->jm + SF_BJ - 1) / SF_BJ;
     a.nb = a.nkb * a.njb * c->im;
@@ -884,8 +937,8 @@ int do_rk4_full(atjup_ctx* c){
     a.c_m2o = (T)(1.0 * -2.0 * C.omega);
     a.c_p2o = (T)(1.0 * +2.0 * C.omega);
     a.Coriolis = (T)C.Coriolis; a.centrifugal = (T)C.centrifugal;
-    a.cbuoy = (T)(-1.0 * 1.0 * 1.0 * C.buoyancy);
-    a.g = (T)C.g; a.rR = (T)(C.r_mix * C.R_mix); a.t_ref = (T)C.t_ref;
+    a.cbuoy = -1.0 * 1.0 * 1.0 * C.buoyancy;
+    a.g = C.g; a.rR = C.r_mix * C.R_mix; a.t_ref_d = C.t_ref; a.t_ref = (T)C.t_ref;
     a.t_coef = (T)(K.t_diff_scale / (C.re * C.pr));
     a.inv_re = (T)(1.0 / C.re);
     a.rad_coupling = (T)K.rad_coupling; a.precip_coupling = (T)K.precip_coupling; a.turb_coupling = (T)K.turb_coupling;
@@ -929,7 +982,8 @@ int do_rk4_full(atjup_ctx* c){
         }
         hipEvent_t t0 = timer_begin(c);
         hipLaunchKernelGGL(k_stage_full<T>, dim3(a.nb), dim3(SF_BK, SF_BJ), 0, c->stream, a);
-        timer_end(c, ATJUP_T_RK_STAGES, t0, 1);
+        hipLaunchKernelGGL(k_stage_species<T>, dim3(a.nb), dim3(SF_BK, SF_BJ), 0, c->stream, a);
+        timer_end(c, ATJUP_T_RK_STAGES, t0, 2);
         if(int rc = check_launch()) return rc;
     }
     c->turb_live_pending = false;
@@ -1051,7 +1105,10 @@ int do_commit_live(atjup_ctx* c){
 
 }  // namespace
 
-#define DISPATCH(fn, ...) ((c->precision == ATJUP_PRECISION_FP64) ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+// the tangent form covers the dynamics profile only
+#define DISPATCH(fn, ...) ((c->precision == ATJUP_PRECISION_FP64) ? fn<double>(__VA_ARGS__) :          \
+                          (c->precision == ATJUP_PRECISION_FP32_REF) ? fn<float>(__VA_ARGS__) :         \
+                          (set_error("the tangent form covers the dynamics profile"), ATJUP_ERR_PROFILE))
 
 int launch_rk4_full(atjup_ctx* c){ return DISPATCH(do_rk4_full, c); }
 int launch_turbulence(atjup_ctx* c){ return DISPATCH(do_turbulence, c); }
@@ -1059,6 +1116,6 @@ int launch_turb_wall(atjup_ctx* c){ return DISPATCH(do_turb_wall, c); }
 int launch_turb_sanitize(atjup_ctx* c){ return DISPATCH(do_turb_sanitize, c); }
 int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){ return DISPATCH(do_satadj, c, gas, g); }
 int launch_flux_limiter(atjup_ctx* c){ return DISPATCH(do_flux_limiter, c); }
-int launch_commit_live(atjup_ctx* c){ return DISPATCH(do_commit_live, c); }
+int launch_commit_live(atjup_ctx* c){ return c->turb_live_pending ? DISPATCH(do_commit_live, c) : 0; }
 
 }  // namespace atjup

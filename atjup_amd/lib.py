@@ -14,7 +14,8 @@ import numpy as np
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libatjup_hip.so")
+# ATJUP_HIP_LIB: load another build of the same library (a tuning variant) instead of the in-tree one
+LIB_PATH = os.environ.get("ATJUP_HIP_LIB", os.path.join(HERE, "libatjup_hip.so"))
 HEADER = os.path.join(os.path.dirname(HERE), "include", "atjup_hip.h")
 
 _D = C.POINTER(C.c_double)
@@ -52,6 +53,15 @@ def load():
               "atjup_hip_sync", "atjup_hip_pitch"):
         getattr(L, n).argtypes = [C.c_void_p]
     L.atjup_hip_device_count.argtypes = [C.POINTER(C.c_int)]
+    for n in ("atjup_hip_upload_tangent", "atjup_hip_adj_upload"):
+        getattr(L, n).argtypes = [C.c_void_p, C.c_int, _D]
+    for n in ("atjup_hip_download_tangent", "atjup_hip_adj_download"):
+        getattr(L, n).argtypes = [C.c_void_p, C.c_int, _D]
+    for n in ("atjup_hip_adj_init", "atjup_hip_adj_zero"):
+        getattr(L, n).argtypes = [C.c_void_p]
+    L.atjup_hip_adj_op.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+    L.atjup_hip_adj_forward_checkpointed.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.atjup_hip_adj_backward.argtypes = [C.c_void_p, _D, _D]
     L.atjup_hip_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
     L.atjup_hip_satadj.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
     L.atjup_hip_set_gas.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
@@ -93,6 +103,7 @@ class Context:
 
     def __init__(self, grid, consts, knobs, precision: int = abi.PRECISION_FP64):
         self.L = load()
+        self.grid = grid
         self.shape = (grid.im, grid.jm, grid.km)
         self.precision = precision
         self.h = self.L.atjup_hip_ctx_create(C.byref(grid), C.byref(consts), C.byref(knobs), precision)
@@ -204,6 +215,51 @@ class Context:
 
     def device_ptr(self, name):
         return self.L.atjup_hip_field_device_ptr(self.h, abi.FIELD[name])
+
+    # ---- tangent form (PRECISION_TANGENT) ------------------------------------------------
+    def set_tangent(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._chk(self.L.atjup_hip_upload_tangent(self.h, abi.FIELD[name], _dp(a)), f"upload tangent {name}")
+
+    def get_tangent(self, name):
+        a = np.empty(self.shape)
+        self._chk(self.L.atjup_hip_download_tangent(self.h, abi.FIELD[name], _dp(a)), f"download tangent {name}")
+        return a
+
+    # ---- adjoint of the FP64 dynamics step ----------------------------------------------
+    ADJ_OPS = {"bc": 1, "shapiro_pdyn": 2, "mixture": 3, "rk4": 4, "pressure": 5, "step": 6}
+
+    def adj_init(self):
+        self._chk(self.L.atjup_hip_adj_init(self.h), "adj_init")
+
+    def adj_zero(self):
+        self._chk(self.L.atjup_hip_adj_zero(self.h), "adj_zero")
+
+    def adj_set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._chk(self.L.atjup_hip_adj_upload(self.h, abi.FIELD[name], _dp(a)), f"adj upload {name}")
+
+    def adj_get(self, name):
+        a = np.empty(self.shape)
+        self._chk(self.L.atjup_hip_adj_download(self.h, abi.FIELD[name], _dp(a)), f"adj download {name}")
+        return a
+
+    def adj_op(self, op, iarg=0, darg=0.0):
+        self._chk(self.L.atjup_hip_adj_op(self.h, self.ADJ_OPS[op], int(iarg), float(darg)), f"adj_op {op}")
+
+    def adj_forward(self, nsteps, every, first_iter=1):
+        self._chk(self.L.atjup_hip_adj_forward_checkpointed(self.h, nsteps, every, first_iter), "adj_forward")
+
+    def adj_backward(self, lam_final=None):
+        """lam_final: adjoint of the final (t, u, v, w), shape (4, im, jm, km); returns the initial one."""
+        out = np.empty((4,) + self.shape)
+        if lam_final is not None:
+            lf = np.ascontiguousarray(lam_final, dtype=np.float64)
+            assert lf.shape == out.shape
+            self._chk(self.L.atjup_hip_adj_backward(self.h, _dp(lf), _dp(out)), "adj_backward")
+        else:
+            self._chk(self.L.atjup_hip_adj_backward(self.h, None, _dp(out)), "adj_backward")
+        return out
 
     TIMERS = ("poisson_sweeps", "rk_stages", "pressure", "step")
 

@@ -1,6 +1,6 @@
 """Benchmark of the ATJUP time step on one MI355X: cell-updates/s of the HIP path.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config dyn] [--precision fp64]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config full|dyn] [--no-fp32] [--no-cpu]
 
 A step is one iteration of the time loop (planet/cJupiterModel.cpp:1010-1555) on the 41 x 181 x 361
 shell: the physics block with the 200-sweep red-black pressure solve on the reference's cadence,
@@ -77,7 +77,7 @@ def step_bytes_per_cell_update(config, esz, sweeps, solves_per_step):
     return rk + pois + esz * phys * solves_per_step
 
 
-def cpu_baseline(config, sweeps, steps=20):
+def cpu_baseline(config, sweeps, steps=10):
     """The reference (oracle/_ref) or, without it, the oracle restatement, on this host."""
     import oracle
     from atjup_amd import abi, inputs
@@ -115,8 +115,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="dyn", choices=["dyn", "full"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--config", default="full", choices=["dyn", "full"])
+    ap.add_argument("--precision-compare", dest="precision_compare", action="store_true", default=True,
+                    help="also time the FP32-refined form and report its per-field error against FP64 (default)")
+    ap.add_argument("--no-fp32", dest="precision_compare", action="store_false")
     ap.add_argument("--sweeps", type=int, default=200)
     ap.add_argument("--omega", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -138,35 +140,57 @@ def main():
     g = inputs.Grid()
     G, Cn, K, _ = abi.make_profile(args.config, sweeps=args.sweeps, omega=args.omega)
     st = inputs.make_state(args.config, seed=11)
-    ctx = Context(G, Cn, K, abi.PRECISION_FP64)
-    ctx.load_state(st, skip=set(inputs.SPECIES) if args.config == "dyn" else set())
-    ctx.bc_after_step()
-    ctx.swap_nlevel()
 
-    n = 0
+    def timed_run(precision):
+        """warmup + steps iterations from the seeded state; the steps timed between two barriers."""
+        ctx = Context(G, Cn, K, precision)
+        ctx.load_state(st, skip=set(inputs.SPECIES) if args.config == "dyn" else set())
+        ctx.bc_after_step()
+        ctx.swap_nlevel()
+        n = 0
 
-    def run(k):
-        nonlocal n
-        for _ in range(k):
-            it = n % 5 + 1
-            ctx.step(it, it == 1)
-            n += 1
+        def run(k):
+            nonlocal n
+            for _ in range(k):
+                it = n % 5 + 1
+                ctx.step(it, it == 1)
+                n += 1
 
-    run(args.warmup)
-    ctx.sync()
-    ctx.set_timing(True)
-    ctx.read_timers()                                   # clear
+        run(args.warmup)
+        ctx.sync()
+        ctx.set_timing(True)
+        ctx.read_timers()                               # clear
+        ctx.sync()
+        t0 = time.perf_counter()
+        run(args.steps)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        return ctx, wall * 1e3, ctx.read_timers(), ctx.check_finite()
 
-    ctx.sync()
-    t0 = time.perf_counter()
-    run(args.steps)
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    ms = wall * 1e3
-    tm = ctx.read_timers()
+    ctx, ms, tm, ok = timed_run(abi.PRECISION_FP64)
     sweep_ms, sweep_launches = tm["poisson_sweeps"]
     rk_ms, rk_launches = tm["rk_stages"]
-    ok = ctx.check_finite()
+
+    fp32 = None
+    if args.precision_compare:
+        c32, ms32, tm32, ok32 = timed_run(abi.PRECISION_FP32_REF)
+        errs = {}
+        for name in abi.PROG + ("p_dyn",):
+            a, b = c32.get(name), ctx.get(name)
+            d = a - b
+            nb, mb = float(np.linalg.norm(b)), float(np.max(np.abs(b)))
+            errs[name] = {"normwise": float(np.linalg.norm(d)) / nb if nb > 0 else float(np.linalg.norm(d)),
+                          "max_abs": float(np.max(np.abs(d))) / mb if mb > 0 else float(np.max(np.abs(d)))}
+        worst = max(max(e["normwise"], e["max_abs"]) for e in errs.values())
+        fp32 = {
+            "value": INTERIOR_CELLS * args.steps / (ms32 * 1e-3), "unit": "cell-updates/s",
+            "ms_per_step": ms32 / args.steps, "speedup_vs_fp64": ms / ms32, "finite": bool(ok32),
+            "poisson_sweep_avg_launch_ms": tm32["poisson_sweeps"][0] / max(tm32["poisson_sweeps"][1], 1),
+            "rk4_stage_avg_launch_ms": tm32["rk_stages"][0] / max(tm32["rk_stages"][1], 1),
+            "rel_err_vs_fp64_after_steps": args.warmup + args.steps, "worst_rel_err": worst,
+            "within_1e-5": bool(worst <= 1e-5), "rel_err": errs,
+        }
+        c32.close()
 
     value = INTERIOR_CELLS * args.steps / (ms * 1e-3)
     launch_ms = sweep_ms / max(sweep_launches, 1)
@@ -199,7 +223,7 @@ def main():
             "workload": WORKLOAD[args.config] + f", 41x181x361 shell, RK4 + red-black Poisson "
                         f"({args.sweeps} sweeps, omega={args.omega}) on iterations 1,2,4 of each 5-step window",
             "grid": [g.im, g.jm, g.km], "sweeps": args.sweeps, "omega": args.omega,
-            "precision": args.precision, "finite": bool(ok),
+            "precision": "fp64", "finite": bool(ok),
         },
         "roofline": {
             "bound": "hbm",
@@ -222,6 +246,7 @@ def main():
         "step_hbm_frac": value * step_b / (HBM_PEAK_GBS * 1e9),
         "gpu_event_ms_per_step": tm["step"][0] / args.steps,
         "cpu_baseline": cpu,
+        "fp32_refined": fp32,
     }
     print(json.dumps(out))
 

@@ -36,6 +36,9 @@ extern "C" {
 
 #define ATJUP_PRECISION_FP64      0   /* the drop-in: every field and stencil in FP64 */
 #define ATJUP_PRECISION_FP32_REF  1   /* fields and RK4 in FP32; Poisson in FP32 inside FP64 refinement */
+#define ATJUP_PRECISION_TANGENT   2   /* FP64 with a directional derivative carried along every field
+                                         (forward-mode tangent of the dynamics profile; no counterpart
+                                         in the reference: it verifies the adjoint) */
 
 /* Field identifiers. 0..16 are the 17 prognostic fields the RK4 integrates
  * (planet/RungeKutta_Jup_Turb.cpp:425-486); ATJUP_NLEVEL + id is the time-level-n copy of the same
@@ -131,6 +134,11 @@ int atjup_hip_download_field(atjup_ctx* ctx, int field, double* dst);
 /* The reference's own Array storage, one new double[km] per (i, j) row (lib/Array.cpp:43-73). */
 int atjup_hip_upload_field_rows(atjup_ctx* ctx, int field, double* const* const* rows);
 int atjup_hip_download_field_rows(atjup_ctx* ctx, int field, double** const* rows);
+/* The tangent form (ATJUP_PRECISION_TANGENT): the directional derivative carried with a field.
+ * upload_field sets a field's value and clears its derivative; upload_tangent sets the
+ * derivative and keeps the value. No counterpart in the reference (verification of the adjoint). */
+int atjup_hip_upload_tangent(atjup_ctx* ctx, int field, const double* src);
+int atjup_hip_download_tangent(atjup_ctx* ctx, int field, double* dst);
 /* The obstacle: SeaMount (1.0 = solid, planet/BC_Jup.h:353-457) as doubles [i][j][k]. Derives
  * i_topography and the wall eddy viscosity (computeWallViscosity, InitValues_Jup.cpp:84-134). */
 int atjup_hip_upload_mask(atjup_ctx* ctx, const double* seamount);
@@ -201,6 +209,35 @@ enum atjup_timer {
 };
 int atjup_hip_set_timing(atjup_ctx* ctx, int enable);
 int atjup_hip_read_timers(atjup_ctx* ctx, double* out);
+
+/* ---- adjoint of the FP64 dynamics step (dry profile) ---------------------------------------
+ * The hand-written transpose of the linearised step: boundary passes, Shapiro on p_dyn, mixture
+ * density, RK4 with its right-hand side, the hydrostatic integral and level means, and the
+ * red-black pressure solve, branches frozen at the forward's choice. Adjoint fields are
+ * addressed by the forward field ids (t, u, v, w, p_dyn, p_hydro, rho_mix, aux_*). The reference
+ * has no adjoint; the survey of its path lists these entry points for the gradient the
+ * benchmark asks for (zonal-wind misfit at the top level with respect to the initial state). */
+#define ATJUP_ADJ_OP_BC            1   /* bc_after_step^T                                      */
+#define ATJUP_ADJ_OP_SHAPIRO_PDYN  2   /* shapiro(p_dyn, 2, 1, 1, 7)^T                         */
+#define ATJUP_ADJ_OP_MIXTURE       3   /* mixture_density^T                                    */
+#define ATJUP_ADJ_OP_RK4           4   /* rk4_step^T                                           */
+#define ATJUP_ADJ_OP_PRESSURE      5   /* pressure_solve(iarg sweeps, darg omega)^T            */
+#define ATJUP_ADJ_OP_STEP          6   /* step(iarg iter, darg != 0 first)^T                   */
+int atjup_hip_adj_init(atjup_ctx* ctx);
+int atjup_hip_adj_zero(atjup_ctx* ctx);
+int atjup_hip_adj_upload(atjup_ctx* ctx, int field, const double* src);
+int atjup_hip_adj_download(atjup_ctx* ctx, int field, double* dst);
+/* Apply one transposed operator to the adjoint fields, linearised about the context's current
+ * forward state (the state before the forward operator). */
+int atjup_hip_adj_op(atjup_ctx* ctx, int op, int iarg, double darg);
+/* nsteps loop iterations (iter = first_iter + n, the first one treated as a run's first) from the
+ * context's state, keeping a checkpoint of the state every `every` iterations. */
+int atjup_hip_adj_forward_checkpointed(atjup_ctx* ctx, int nsteps, int every, int first_iter);
+/* The reverse sweep: lam_final = the adjoint of the final t, u, v, w ([4][im][jm][km], or NULL to
+ * start from the adjoint fields as uploaded); lam_init receives the adjoint of the initial t, u,
+ * v, w. Each segment is recomputed from its checkpoint. The context's forward fields are left as
+ * the last recomputation left them. */
+int atjup_hip_adj_backward(atjup_ctx* ctx, const double* lam_final, double* lam_init);
 
 /* ---- housekeeping ----------------------------------------------------------------------- */
 int         atjup_hip_sync(atjup_ctx* ctx);

@@ -1,0 +1,196 @@
+"""The hand-written adjoint of the FP64 dynamics step against the tangent-linear step.
+
+The tangent form (PRECISION_TANGENT) runs the same forward kernels on (value, derivative) pairs,
+so it is the exact forward-mode derivative of the drop-in; its values must equal the FP64 path's
+bit for bit, and its derivatives must agree with central finite differences of the FP64 path.
+The adjoint of every operator is then held to the dot-product identity <J dx, ly> = <dx, J^T ly>
+against it, operator by operator and for whole loop iterations with and without the physics
+block, on a small shell with an obstacle; and the gradient of the zonal-jet misfit through the
+checkpointed reverse sweep is held to central differences along seeded directions.
+"""
+import numpy as np
+import pytest
+
+from atjup_amd import abi, inputs
+from tests.helpers import small_state
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("t", "u", "v", "w", "p_dyn", "aux_u", "aux_v", "aux_w", "rho_mix")
+
+
+def _state(seed=3):
+    st = small_state("dyn", seed=seed)
+    rng = np.random.default_rng(seed)
+    g = st.grid
+    # a live pressure field and aux_* so that every term of the step has a derivative
+    st.fields["p_dyn"] = 0.01 * rng.standard_normal(g.shape)
+    for n in ("aux_u", "aux_v", "aux_w"):
+        st.fields[n] = 0.01 * rng.standard_normal(g.shape)
+    for n in ("u", "v"):
+        st.fields[n] = st.fields[n] + 0.001 * rng.standard_normal(g.shape)
+    for n in st.fields:
+        st.fields[n][:, :, -1] = st.fields[n][:, :, 0]
+    return st
+
+
+def _ctx(st, precision, sweeps=6):
+    from atjup_amd.lib import Context
+    G, Cn, K, _ = abi.make_profile("dyn", sweeps=sweeps, grid=st.grid)
+    c = Context(G, Cn, K, precision)
+    c.load_state(st, skip=set(inputs.SPECIES))
+    c.bc_after_step()
+    c.swap_nlevel()
+    c.mixture_density()
+    return c
+
+
+def _snapshot(c):
+    return {n: c.get(n) for n in STATE}
+
+
+def _load(c, snap):
+    for n, v in snap.items():
+        c.set(n, v)
+
+
+def test_tangent_values_are_the_fp64_step_and_derivatives_match_finite_differences():
+    st = _state()
+    c64 = _ctx(st, abi.PRECISION_FP64)
+    ctg = _ctx(st, abi.PRECISION_TANGENT)
+    base = _snapshot(c64)
+    rng = np.random.default_rng(1)
+    dw = rng.standard_normal(st.grid.shape) * 0.01
+    dw[:, :, -1] = dw[:, :, 0]
+    ctg.set_tangent("w", dw)
+    c64.step(1, True)
+    ctg.step(1, True)
+    for n in ("t", "u", "v", "w", "p_dyn"):
+        assert np.array_equal(ctg.get(n), c64.get(n)), n
+    eps = 1e-6
+    outs = {}
+    for sgn in (+1, -1):
+        _load(c64, base)
+        c64.set("w", base["w"] + sgn * eps * dw)
+        c64.step(1, True)
+        outs[sgn] = {n: c64.get(n) for n in ("t", "u", "v", "w", "p_dyn")}
+    checked = 0
+    for n in ("t", "u", "v", "w", "p_dyn"):
+        fd = (outs[1][n] - outs[-1][n]) / (2 * eps)
+        tg = ctg.get_tangent(n)
+        # the rounding of the two forward values bounds what the difference quotient can resolve
+        noise = 4e-16 * np.max(np.abs(outs[1][n])) / (2 * eps)
+        scale = np.max(np.abs(tg))
+        if scale < 1e4 * noise:
+            print(f"tangent vs FD {n}: derivative {scale:.1e} below the difference quotient's noise {noise:.1e}")
+            continue
+        err = np.max(np.abs(tg - fd)) / scale
+        print(f"tangent vs FD {n}: {err:.2e}")
+        assert err < 1e-5, n
+        checked += 1
+    assert checked >= 3
+    c64.close()
+    ctg.close()
+
+
+CASES = [
+    # op, iarg, darg, inputs perturbed, outputs weighted
+    ("bc", 0, 0.0, ("t", "u", "v", "w", "p_dyn"), ("t", "u", "v", "w", "p_dyn")),
+    ("shapiro_pdyn", 0, 0.0, ("p_dyn",), ("p_dyn",)),
+    ("mixture", 0, 0.0, ("t", "p_dyn"), ("rho_mix",)),
+    ("rk4", 0, 0.0, ("t", "u", "v", "w", "p_dyn", "rho_mix", "aux_u", "aux_v", "aux_w"),
+     ("t", "u", "v", "w", "aux_u", "aux_v", "aux_w")),
+    ("pressure", 6, 1.0, ("p_dyn", "aux_u", "aux_v", "aux_w"), ("p_dyn", "aux_u", "aux_v", "aux_w")),
+    ("pressure", 4, 1.5, ("p_dyn", "aux_u", "aux_v", "aux_w"), ("p_dyn", "aux_u", "aux_v", "aux_w")),
+    ("step", 2, 0.0, STATE, STATE),
+    ("step", 3, 0.0, STATE, STATE),
+]
+
+
+def _forward(c, op, iarg, darg):
+    if op == "bc":
+        c.bc_after_step()
+    elif op == "shapiro_pdyn":
+        c.shapiro("p_dyn", 2, 1.0, 1, 7)
+    elif op == "mixture":
+        c.mixture_density()
+    elif op == "rk4":
+        c.rk4_step()
+    elif op == "pressure":
+        c.pressure_solve(iarg, darg)
+    elif op == "step":
+        c.step(iarg, bool(darg))
+
+
+@pytest.mark.parametrize("op,iarg,darg,ins,outs", CASES, ids=[f"{c[0]}-{c[1]}-{c[2]}" for c in CASES])
+def test_dot_product(op, iarg, darg, ins, outs):
+    st = _state(seed=5)
+    rng = np.random.default_rng(7)
+    shp = st.grid.shape
+    c64 = _ctx(st, abi.PRECISION_FP64)
+    ctg = _ctx(st, abi.PRECISION_TANGENT)
+    base = _snapshot(c64)
+    dx = {n: rng.standard_normal(shp) * (1e-3 + np.abs(base[n]).max() * 1e-3) for n in ins}
+    for n, v in dx.items():
+        ctg.set_tangent(n, v)
+    _forward(ctg, op, iarg, darg)
+    ly = {n: rng.standard_normal(shp) for n in outs}
+    lhs = sum(float(np.sum(ctg.get_tangent(n) * ly[n])) for n in outs)
+    # the adjoint, linearised about the same state
+    c64.adj_init()
+    c64.adj_zero()
+    for n, v in ly.items():
+        c64.adj_set(n, v)
+    c64.adj_op(op, iarg, darg)
+    rhs = sum(float(np.sum(dx[n] * c64.adj_get(n))) for n in ins)
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs), 1e-300)
+    print(f"dot product {op}({iarg}, {darg}): <J dx, ly> = {lhs:.15e}, <dx, J^T ly> = {rhs:.15e}, rel {rel:.2e}")
+    assert rel < 1e-10
+    c64.close()
+    ctg.close()
+
+
+def _loss_and_grad_final(c, target, cos_lat, i_top):
+    """Zonal-jet misfit at the top level: sum_j cos(lat_j) (mean_k w[i_top, j, :-1] - target_j)^2."""
+    w = c.get("w")
+    km = w.shape[2]
+    zm = w[i_top, :, :-1].mean(axis=1)
+    r = zm - target
+    loss = float(np.sum(cos_lat * r * r))
+    lam = np.zeros((4,) + w.shape)
+    lam[3, i_top, :, :-1] = (2.0 * cos_lat * r / (km - 1))[:, None]
+    return loss, lam
+
+
+def test_gradient_of_the_jet_misfit_against_central_differences():
+    st = _state(seed=9)
+    g = st.grid
+    _, theta, _ = g.coords()
+    cos_lat = np.abs(np.sin(theta))                 # theta is the colatitude
+    rng = np.random.default_rng(11)
+    target = st.fields["w"][g.im - 2, :, 0] + 0.01 * rng.standard_normal(g.jm)
+    i_top = g.im - 2
+    nsteps, every = 6, 2
+    c = _ctx(st, abi.PRECISION_FP64)
+    init = _snapshot(c)
+    c.adj_init()
+    c.adj_forward(nsteps, every, 1)
+    loss, lam = _loss_and_grad_final(c, target, cos_lat, i_top)
+    grad = c.adj_backward(lam)[3]                   # dL / dw0
+    for d_seed in range(4):
+        d = np.random.default_rng(100 + d_seed).standard_normal(g.shape)
+        d[:, :, -1] = d[:, :, 0]
+        eps = 1e-7
+        vals = []
+        for sgn in (+1, -1):
+            _load(c, init)
+            c.set("w", init["w"] + sgn * eps * d)
+            for it in range(1, nsteps + 1):
+                c.step(it, it == 1)
+            vals.append(_loss_and_grad_final(c, target, cos_lat, i_top)[0])
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        ad = float(np.sum(grad * d))
+        rel = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
+        print(f"direction {d_seed}: FD {fd:.10e}  adjoint {ad:.10e}  rel {rel:.2e}")
+        assert rel < 1e-4
+    c.close()
