@@ -50,12 +50,23 @@ class MisfitRunner:
         lam[3, self.i_top, :, :-1] = (2.0 * self.cos_lat * r / (km - 1))[:, None]
         return float(np.sum(self.cos_lat * r * r)), lam
 
-    def forward(self, w0):
+    def load(self, w0):
         for n, v in self.init.items():
             self.ctx.set(n, w0 if n == "w" else v)
+
+    def forward(self, w0):
+        """L(w0) through the checkpointed forward sweep, ready for backward()."""
+        self.load(w0)
         self.ctx.adj_forward(self.nsteps, self.every, 1)
         loss, self._lam = self.loss_final()
         return loss
+
+    def evaluate(self, w0):
+        """L(w0) by plain forward steps (no checkpoints), for difference quotients."""
+        self.load(w0)
+        for n in range(self.nsteps):
+            self.ctx.step(1 + n, n == 0)
+        return self.loss_final()[0]
 
     def backward(self, scale=1.0):
         return self.ctx.adj_backward(scale * self._lam)[3]

@@ -250,15 +250,17 @@ class Context:
     def adj_forward(self, nsteps, every, first_iter=1):
         self._chk(self.L.atjup_hip_adj_forward_checkpointed(self.h, nsteps, every, first_iter), "adj_forward")
 
-    def adj_backward(self, lam_final=None):
-        """lam_final: adjoint of the final (t, u, v, w), shape (4, im, jm, km); returns the initial one."""
-        out = np.empty((4,) + self.shape)
+    def adj_backward(self, lam_final=None, download=True):
+        """lam_final: adjoint of the final (t, u, v, w), shape (4, im, jm, km), or None to start from
+        the adjoint fields already on the device (adj_set); returns the initial adjoint, or None
+        when download is False (it then stays in the device's adjoint fields, adj_get)."""
+        out = np.empty((4,) + self.shape) if download else None
+        lf = None
         if lam_final is not None:
             lf = np.ascontiguousarray(lam_final, dtype=np.float64)
-            assert lf.shape == out.shape
-            self._chk(self.L.atjup_hip_adj_backward(self.h, _dp(lf), _dp(out)), "adj_backward")
-        else:
-            self._chk(self.L.atjup_hip_adj_backward(self.h, None, _dp(out)), "adj_backward")
+            assert lf.shape == (4,) + self.shape
+        self._chk(self.L.atjup_hip_adj_backward(self.h, None if lf is None else _dp(lf),
+                                                None if out is None else _dp(out)), "adj_backward")
         return out
 
     TIMERS = ("poisson_sweeps", "rk_stages", "pressure", "step")

@@ -110,6 +110,68 @@ def cpu_baseline(config, sweeps, steps=10):
                       f"same 41x181x361 case, {dt:.1f} s, OMP_NUM_THREADS={threads}, -O2"}
 
 
+def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6):
+    """The 20-step jet-misfit gradient on the dynamics profile: the time of one plain forward
+    (steps FP64 steps), of the checkpointed forward and of the reverse sweep, all between library
+    barriers with the adjoint seed already on the device; then dL/dw0 against central differences
+    (eps) along seeded directions."""
+    from atjup_amd import abi, inputs
+    from atjup_amd.adjoint import MisfitRunner
+    from atjup_amd.lib import Context
+
+    g = inputs.Grid()
+    G, Cn, K, _ = abi.make_profile("dyn", sweeps=sweeps, omega=omega)
+    st = inputs.make_state("dyn", seed=11)
+    ctx = Context(G, Cn, K, abi.PRECISION_FP64)
+    ctx.load_state(st, skip=set(inputs.SPECIES))
+    ctx.bc_after_step()
+    ctx.swap_nlevel()
+    ctx.mixture_density()
+    init = {n: ctx.get(n) for n in ("t", "u", "v", "w", "p_dyn", "aux_u", "aux_v", "aux_w", "rho_mix")}
+    target = inputs.zonal_jets(g, np.random.default_rng(12)) / 100.0   # a second seeded 12-jet profile
+    run = MisfitRunner(ctx, init, target, nsteps=steps, every=every)
+    w0 = init["w"]
+
+    def clock(fn):
+        ctx.sync()
+        t0 = time.perf_counter()
+        fn()
+        ctx.sync()
+        return (time.perf_counter() - t0) * 1e3
+
+    run.load(w0)
+    clock(lambda: [ctx.step(1 + n, n == 0) for n in range(steps)])        # warm-up
+    run.load(w0)
+    fwd_ms = clock(lambda: [ctx.step(1 + n, n == 0) for n in range(steps)])
+    run.load(w0)
+    ckpt_ms = clock(lambda: ctx.adj_forward(steps, every, 1))
+    loss, lam = run.loss_final()
+    ctx.adj_zero()
+    for f, name in enumerate(("t", "u", "v", "w")):
+        ctx.adj_set(name, lam[f])
+    bwd_ms = clock(lambda: ctx.adj_backward(None, download=False))
+    grad = ctx.adj_get("w")
+    errs = []
+    for d_seed in range(directions):
+        d = np.random.default_rng(1000 + d_seed).standard_normal(g.shape)
+        d[:, :, -1] = d[:, :, 0]
+        fd = (run.evaluate(w0 + eps * d) - run.evaluate(w0 - eps * d)) / (2 * eps)
+        ad = float(np.sum(grad * d))
+        errs.append(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
+    ctx.close()
+    return {
+        "workload": f"jet misfit at i = {g.im - 2} after {steps} FP64 dynamics steps, 41x181x361, dL/dw0 "
+                    f"({g.im * g.jm * g.km} values), checkpoints every {every} steps",
+        "forward_ms": fwd_ms, "checkpointed_forward_ms": ckpt_ms, "backward_ms": bwd_ms,
+        "backward_over_forward": bwd_ms / fwd_ms,
+        "gradient_over_forward": (ckpt_ms + bwd_ms) / fwd_ms,
+        "loss": loss,
+        "fd_directions": directions, "fd_eps": eps,
+        "fd_max_rel_err": max(errs), "fd_median_rel_err": float(np.median(errs)),
+        "within_1e-4": bool(max(errs) <= 1e-4),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +184,7 @@ def main():
     ap.add_argument("--sweeps", type=int, default=200)
     ap.add_argument("--omega", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -199,6 +262,10 @@ def main():
     solves_per_step = 3.0 / 5.0
     step_b = step_bytes_per_cell_update(args.config, 8, args.sweeps, solves_per_step)
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_leg(args.sweeps, args.omega)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -247,6 +314,7 @@ def main():
         "gpu_event_ms_per_step": tm["step"][0] / args.steps,
         "cpu_baseline": cpu,
         "fp32_refined": fp32,
+        "adjoint": adj,
     }
     print(json.dumps(out))
 
