@@ -551,10 +551,24 @@ __global__ void ka_rb_planes_i(double* __restrict__ l0, double* __restrict__ l1,
     }
 }
 
-// One colour's half-sweep transposed, as a gather over the cells it read (the other colour's
-// cells and the outer planes): lambda_n += sum over the sweep cells s that read n of
-// omega w(s -> n) / den_s lambda_s. Launched over every cell of colour `tgt` (= 1 - the swept
-// colour), planes included; the swept colour's own scaling follows in ka_rb_scale.
+// The sweeps transposed in the form the forward kernel runs them (k_rb_sweep): from the second
+// sweep on, a plane value a sweep cell x reads is formed from x's own old value and the old value
+// of the other-colour cell y behind it (c43 x - c13 y, x at i = 1, y at i = 2 and likewise at
+// the other three planes), and the stored planes are only rewritten after the last sweep (their
+// transpose runs once, before the sweeps). A colour-c pass, cell x of colour c:
+//     x' = (1 - omega) x + (omega / den_x) (sum_d w_d v_d - src_x),
+// v_d the neighbour read in direction d (a stored value, or a formed plane value). With
+// mu_x = omega lambda_x' / den_x (0 where the cell is skipped, den_x = 0):
+//     lambda_x  = (1 - omega) lambda_x' + sum over x's formed planes of c43 w_d mu_x,
+//     lambda_v += w_d mu_x for every stored value v that x read,
+//     lambda_y += -c13 w_d mu_x for the cell y behind each formed plane,
+//     lambda_src_x -= mu_x.
+// In a sweep, colour 0 reads colour 1 before the colour-1 pass and colour 1 reads the updated
+// colour 0, except through formed planes, which hold colour 0's values from before the sweep. So,
+// with mu of colour 1 known, the colour-0 kernel adds the direct reads by colour 1 to lambda_0
+// before applying its own pass, and the formed-plane terms of colour 1 after it; the colour-1
+// kernel applies its own pass and adds the reads by colour 0 (direct and formed), then forms the
+// next sweep's mu of colour 1. Each colour keeps its mu in its own array.
 __device__ __forceinline__ bool rb_reads(uint8_t fl_s, int dir){
     // does sweep cell s (flags fl_s) read its neighbour in direction dir (0 +i, 1 -i, 2 +j, 3 -j, 4 +k, 5 -k)
     if(fl_s & FL_SELF) return true;
@@ -574,59 +588,109 @@ __device__ __forceinline__ double rb_den(uint8_t fl, const PoisIJ<double>& W){
     return den;
 }
 
-__global__ __launch_bounds__(256) void ka_rb_gather(double* __restrict__ lt_arr, const double* __restrict__ ls_arr,
-        const uint8_t* __restrict__ fs_arr, const PoisIJ<double>* __restrict__ pw,
-        int im, int jm, int hp, int nm, int swept, double omega){
-    // target cells: colour tgt = 1 - swept, every i, j (planes included), m < nm
-    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
-    if(m >= nm) return;
-    const int tgt = 1 - swept;
+// Which of a cell's six neighbours read it in a sweep (a sweep cell, reading in the opposite
+// direction): bit d for the neighbour in direction d (0 +i, 1 -i, 2 +j, 3 -j, 4 +k, 5 -k). Fixed
+// by the mask, formed once per transposed solve for both colours.
+constexpr int RBT_BM = 64, RBT_BJ = 4;
+__global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_inmask(const uint8_t* __restrict__ f0, const uint8_t* __restrict__ f1,
+        uint8_t* __restrict__ in0, uint8_t* __restrict__ in1, int im, int jm, int hp, int nm){
+    const int m = blockIdx.x * RBT_BM + threadIdx.x, j = blockIdx.y * RBT_BJ + threadIdx.y;
+    const int i = blockIdx.z >> 1, colour = blockIdx.z & 1;
+    if(m >= nm || j >= jm) return;
+    const uint8_t* fo = colour ? f0 : f1;        // the other colour's flags
     const size_t P = (size_t)jm * hp, R = (size_t)hp;
-    const size_t q = ((size_t)i * jm + j) * hp + m;
-    // the target cell's k: 2m + off_t, off_t = (tgt + i + j) & 1; a swept cell in the same row
-    // with compact index ms sits at k = 2 ms + off_s, off_s = 1 - off_t
-    const int off_t = (tgt + i + j) & 1;
-    double acc = 0.0;
-    auto take = [&](int si, int sj, size_t qs, int dir_from_s){
-        if(si < 1 || si > im - 2 || sj < 1 || sj > jm - 2) return;     // not a sweep cell
-        const uint8_t fs = fs_arr[qs];
-        if(!rb_reads(fs, dir_from_s)) return;
-        const PoisIJ<double> W = pw[(size_t)si * jm + sj];
-        const double den = rb_den(fs, W);
-        if(!(den > 0.0)) return;
-        const double wgt = (dir_from_s < 2) ? W.num1 : (dir_from_s < 4 ? W.num2 : W.num3);
-        acc += omega * wgt / den * ls_arr[qs];
-    };
-    // radial and meridional neighbours share the compact index
-    if(i + 1 <= im - 1) take(i + 1, j, q + P, 1);      // s = (i+1) reads its -i neighbour, this cell
-    if(i - 1 >= 0)      take(i - 1, j, q - P, 0);
-    if(j + 1 <= jm - 1) take(i, j + 1, q + R, 3);
-    if(j - 1 >= 0)      take(i, j - 1, q - R, 2);
-    // zonal neighbours on the ring: this cell at k = 2m + off_t; k + 1 and k - 1 (mod 2 nm)
-    if(i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2){
-        const size_t row = ((size_t)i * jm + j) * hp;
-        const int kk = 2 * m + off_t;
-        const int ring = 2 * nm;
-        const int kp = (kk + 1) % ring, kmn = (kk - 1 + ring) % ring;
-        take(i, j, row + (kp >> 1), 5);                // the cell at k+1 reads its -k neighbour
-        take(i, j, row + (kmn >> 1), 4);               // the cell at k-1 reads its +k neighbour
+    const size_t row = ((size_t)i * jm + j) * hp, q = row + m;
+    auto sweep = [&](int si, int sj){ return si >= 1 && si <= im - 2 && sj >= 1 && sj <= jm - 2; };
+    uint8_t bits = 0;
+    if(i + 1 < im && sweep(i + 1, j) && rb_reads(fo[q + P], 1)) bits |= 1;
+    if(i >= 1 && sweep(i - 1, j) && rb_reads(fo[q - P], 0)) bits |= 2;
+    if(j + 1 < jm && sweep(i, j + 1) && rb_reads(fo[q + R], 3)) bits |= 4;
+    if(j >= 1 && sweep(i, j - 1) && rb_reads(fo[q - R], 2)) bits |= 8;
+    if(sweep(i, j)){
+        const int off = (colour + i + j) & 1, ring = 2 * nm, kk = 2 * m + off;
+        if(rb_reads(fo[row + (((kk + 1) % ring) >> 1)], 5)) bits |= 16;
+        if(rb_reads(fo[row + (((kk - 1 + ring) % ring) >> 1)], 4)) bits |= 32;
     }
-    lt_arr[q] += acc;
+    (colour ? in1 : in0)[q] = bits;
 }
 
-// the swept colour's own part: lambda_s -> (1 - omega) lambda_s, the source gets -omega lambda_s / den
-__global__ __launch_bounds__(256) void ka_rb_scale(double* __restrict__ ls_arr, double* __restrict__ lsrc,
-        const uint8_t* __restrict__ fs_arr, const PoisIJ<double>* __restrict__ pw,
-        int im, int jm, int hp, int nm, double omega){
-    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1, i = blockIdx.z + 1;
-    if(m >= nm || j > jm - 2 || i > im - 2) return;
+// mu of colour 1 for the last sweep: omega lambda / den on the sweep cells, 0 elsewhere
+__global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_mu(const double* __restrict__ lam, double* __restrict__ mu,
+        const uint8_t* __restrict__ fl_arr, const PoisIJ<double>* __restrict__ pw, int im, int jm, int hp, int nm,
+        double omega){
+    const int m = blockIdx.x * RBT_BM + threadIdx.x, j = blockIdx.y * RBT_BJ + threadIdx.y, i = blockIdx.z;
+    if(m >= nm || j >= jm) return;
     const size_t q = ((size_t)i * jm + j) * hp + m;
-    const uint8_t fs = fs_arr[q];
-    const double den = rb_den(fs, pw[(size_t)i * jm + j]);
-    if(!(den > 0.0)) return;                           // the cell was skipped: it keeps its value
-    const double l = ls_arr[q];
-    lsrc[q] += -omega * l / den;
-    ls_arr[q] = (1.0 - omega) * l;
+    double v = 0.0;
+    if(i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2){
+        const double den = rb_den(fl_arr[q], pw[(size_t)i * jm + j]);
+        if(den > 0.0) v = omega * lam[q] / den;
+    }
+    mu[q] = v;
+}
+
+// one colour pass of one sweep, transposed, over every cell of colour c (plane cells take the
+// reads of stored planes, which only the first sweep of a solve makes: ghost = 0)
+__global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restrict__ lam, double* __restrict__ mu,
+        const double* __restrict__ mu_o, double* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
+        const uint8_t* __restrict__ fo_arr, const uint8_t* __restrict__ inm, const PoisIJ<double>* __restrict__ pw,
+        int im, int jm, int hp, int nm, int colour, int ghost, double omega, double c43, double c13){
+    const int m = blockIdx.x * RBT_BM + threadIdx.x, j = blockIdx.y * RBT_BJ + threadIdx.y, i = blockIdx.z;
+    if(m >= nm || j >= jm) return;
+    const bool interior = i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2;
+    if(!interior && ghost) return;
+    const size_t P = (size_t)jm * hp, R = (size_t)hp;
+    const size_t row = ((size_t)i * jm + j) * hp;
+    const size_t q = row + m;
+    // stored-value reads of this cell by the other colour's sweep cells s: sum w mu_s
+    const uint8_t bits = inm[q];
+    double dsum = 0.0;
+    if(bits & 1) dsum += pw[(size_t)(i + 1) * jm + j].num1 * mu_o[q + P];
+    if(bits & 2) dsum += pw[(size_t)(i - 1) * jm + j].num1 * mu_o[q - P];
+    if(bits & 4) dsum += pw[(size_t)i * jm + j + 1].num2 * mu_o[q + R];
+    if(bits & 8) dsum += pw[(size_t)i * jm + j - 1].num2 * mu_o[q - R];
+    if(!interior){
+        lam[q] += dsum;                              // a plane cell, read as stored in the first sweep
+        return;
+    }
+    const PoisIJ<double> W = pw[(size_t)i * jm + j];
+    if(bits & 48){
+        const int off = (colour + i + j) & 1, ring = 2 * nm, kk = 2 * m + off;
+        if(bits & 16) dsum += W.num3 * mu_o[row + (((kk + 1) % ring) >> 1)];
+        if(bits & 32) dsum += W.num3 * mu_o[row + (((kk - 1 + ring) % ring) >> 1)];
+    }
+    // formed-plane reads by the other colour, for which this cell is the one behind the plane
+    double bsum = 0.0;
+    if(ghost){
+        if(i == 2 && rb_reads(fo_arr[q - P], 1))      bsum += -c13 * pw[(size_t)1 * jm + j].num1 * mu_o[q - P];
+        if(i == im - 3 && rb_reads(fo_arr[q + P], 0)) bsum += -c13 * pw[(size_t)(im - 2) * jm + j].num1 * mu_o[q + P];
+        if(j == 2 && rb_reads(fo_arr[q - R], 3))      bsum += -c13 * pw[(size_t)i * jm + 1].num2 * mu_o[q - R];
+        if(j == jm - 3 && rb_reads(fo_arr[q + R], 2)) bsum += -c13 * pw[(size_t)i * jm + jm - 2].num2 * mu_o[q + R];
+    }
+    // this cell's own pass
+    const uint8_t fl = fl_arr[q];
+    const double den = rb_den(fl, W);
+    const bool skip = !(den > 0.0);
+    double t = lam[q];
+    if(colour == 0) t += dsum;                     // colour 1 read colour 0 after its pass
+    double mu_x = 0.0, out = t;
+    if(!skip){
+        mu_x = (colour == 0) ? omega * t / den : mu[q];
+        double self = 0.0;
+        if(ghost){
+            if(i == 1 && rb_reads(fl, 1))      self += c43 * W.num1 * mu_x;
+            if(i == im - 2 && rb_reads(fl, 0)) self += c43 * W.num1 * mu_x;
+            if(j == 1 && rb_reads(fl, 3))      self += c43 * W.num2 * mu_x;
+            if(j == jm - 2 && rb_reads(fl, 2)) self += c43 * W.num2 * mu_x;
+        }
+        out = (1.0 - omega) * t + self;
+        lsrc[q] += -mu_x;
+    }
+    out += bsum;
+    if(colour == 1) out += dsum;                   // colour 0 read colour 1 before colour 1's pass
+    lam[q] = out;
+    // colour 0: its mu for the colour-1 kernel; colour 1: its mu for the previous sweep
+    mu[q] = (colour == 0) ? mu_x : (skip ? 0.0 : omega * out / den);
 }
 
 // prepareProjectionBoundaries + the clamped divergence source, transposed: the source adjoint
@@ -824,11 +888,19 @@ constexpr int N_STATE = 9;
 struct AdjState {
     Buf lam[ATJUP_F_END];
     Buf work[AW_N];
-    Buf lsplit, lsrc, red, zs;
+    Buf lsplit, lsrc, mu, inmask, red, zs;
     std::vector<Buf> ckpt;        // N_STATE fields per checkpoint
     std::vector<Buf> snap;        // N_STATE fields per step of the segment being reversed
+    std::vector<Buf> post;        // N_POST fields per step: the state after its physics block
+    struct SweepGraph { int n; double omega; hipGraphExec_t exec; };
+    std::vector<SweepGraph> graphs;   // the transposed sweep sequences, replayed
     int nsteps = 0, every = 0, first_iter = 1;
 };
+
+// what the physics block of a dynamics-profile iteration changes: p_dyn (solve and smoothing),
+// aux_* (prepareProjectionBoundaries) and rho_mix
+static const int kPostFields[] = { ATJUP_F_P_DYN, ATJUP_F_AUX_U, ATJUP_F_AUX_V, ATJUP_F_AUX_W, ATJUP_F_RHO_MIX };
+constexpr int N_POST = 5;
 
 static const int kLamFields[] = { ATJUP_F_T, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W, ATJUP_F_P_DYN, ATJUP_F_P_HYDRO,
                                   ATJUP_F_RHO_MIX, ATJUP_F_AUX_U, ATJUP_F_AUX_V, ATJUP_F_AUX_W };
@@ -846,9 +918,11 @@ void adj_free(atjup_ctx* c){
     AdjState* A = c->adj;
     for(auto& b : A->lam) adj_release(b);
     for(auto& b : A->work) adj_release(b);
-    for(Buf* b : { &A->lsplit, &A->lsrc, &A->red, &A->zs }) adj_release(*b);
+    for(Buf* b : { &A->lsplit, &A->lsrc, &A->mu, &A->inmask, &A->red, &A->zs }) adj_release(*b);
     for(auto& b : A->ckpt) adj_release(b);
     for(auto& b : A->snap) adj_release(b);
+    for(auto& b : A->post) adj_release(b);
+    for(auto& g : A->graphs) (void)hipGraphExecDestroy(g.exec);
     delete A;
     c->adj = nullptr;
 }
@@ -989,12 +1063,13 @@ static int adj_rk4(atjup_ctx* c){
 static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
     AdjState* A = c->adj;
     const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
-    // the forward solve, for the aux_* after prepareProjectionBoundaries (the clamp test of the
-    // source reads them); p_dyn and aux_* of the context are restored afterwards
+    // the aux_* after prepareProjectionBoundaries, which the clamp test of the source reads (the
+    // solve changes nothing else the transpose depends on: the sweeps are linear); p_dyn and aux_*
+    // of the context are restored afterwards
     if(int rc = copy_field(c, W(c, AW_PIN), Fd(c, ATJUP_F_P_DYN))) return rc;
     for(int f = 0; f < 3; f++)
         if(int rc = copy_field(c, W(c, AW_AUXP + f), Fd(c, ATJUP_F_AUX_U + f))) return rc;
-    if(int rc = launch_pressure(c, sweeps, omega)) return rc;
+    if(int rc = launch_prep_projection(c)) return rc;
     const int hp = c->rb_hp;
     const size_t hn = (size_t)c->im * c->jm * hp;
     const int nm = (c->km - 1) / 2;
@@ -1010,18 +1085,46 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
     hipLaunchKernelGGL(ka_rb_pack_add, gfull, b128, 0, c->stream, (const double*)L(c, ATJUP_F_P_DYN), l0, l1,
         c->jm, c->km, c->pitch, hp);
     dim3 gseam(nblk(c->jm, 64), c->im), gpj(nblk(nm + 1, 128), c->im, 2), gpi(nblk(nm + 1, 128), c->jm, 2);
-    dim3 gg(nblk(nm, 128), c->jm, c->im), gs(nblk(nm, 128), c->jm - 2, c->im - 2);
-    for(int s = sweeps - 1; s >= 0; s--){
+    dim3 gt(nblk(nm, RBT_BM), nblk(c->jm, RBT_BJ), c->im), bt(RBT_BM, RBT_BJ);
+    dim3 gt2(nblk(nm, RBT_BM), nblk(c->jm, RBT_BJ), 2 * c->im);
+    double* mu0 = (double*)A->mu.p;
+    double* mu1 = mu0 + hn;
+    uint8_t* in0 = (uint8_t*)A->inmask.p;
+    uint8_t* in1 = in0 + hn;
+    auto launch_all = [&](){
+        hipLaunchKernelGGL(ka_rbT_inmask, gt2, bt, 0, c->stream, f0, f1, in0, in1, c->im, c->jm, hp, nm);
+        // the planes and the seam written after the last sweep
         hipLaunchKernelGGL(ka_rb_seam, gseam, dim3(64), 0, c->stream, l0, l1, c->im, c->jm, hp, nm);
         hipLaunchKernelGGL(ka_rb_planes_j, gpj, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
         hipLaunchKernelGGL(ka_rb_planes_i, gpi, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
-        // colour 1, then colour 0, transposed
-        hipLaunchKernelGGL(ka_rb_gather, gg, b128, 0, c->stream, l0, (const double*)l1, f1, pw, c->im, c->jm, hp, nm, 1, omega);
-        hipLaunchKernelGGL(ka_rb_scale, gs, b128, 0, c->stream, l1, s1, f1, pw, c->im, c->jm, hp, nm, omega);
-        hipLaunchKernelGGL(ka_rb_gather, gg, b128, 0, c->stream, l1, (const double*)l0, f0, pw, c->im, c->jm, hp, nm, 0, omega);
-        hipLaunchKernelGGL(ka_rb_scale, gs, b128, 0, c->stream, l0, s0, f0, pw, c->im, c->jm, hp, nm, omega);
-        if(int rc = check_launch()) return rc;
+        hipLaunchKernelGGL(ka_rbT_mu, gt, bt, 0, c->stream, (const double*)l1, mu1, f1, pw, c->im, c->jm, hp, nm, omega);
+        for(int s = sweeps - 1; s >= 0; s--){
+            const int ghost = s > 0 ? 1 : 0;         // the drop-in's first sweep reads the stored planes
+            hipLaunchKernelGGL(ka_rbT_pass, gt, bt, 0, c->stream, l0, mu0, (const double*)mu1, s0, f0, f1,
+                (const uint8_t*)in0, pw, c->im, c->jm, hp, nm, 0, ghost, omega, c43, c13);
+            hipLaunchKernelGGL(ka_rbT_pass, gt, bt, 0, c->stream, l1, mu1, (const double*)mu0, s1, f1, f0,
+                (const uint8_t*)in1, pw, c->im, c->jm, hp, nm, 1, ghost, omega, c43, c13);
+        }
+    };
+    // the 2 launches per sweep go to the device as one graph, captured on first use per sweep
+    // count and relaxation factor (the buffers are the adjoint state's own, fixed for its life)
+    hipGraphExec_t exec = nullptr;
+    for(auto& g : A->graphs) if(g.n == sweeps && g.omega == omega){ exec = g.exec; break; }
+    if(!exec){
+        hipGraph_t graph = nullptr;
+        ATJUP_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        launch_all();
+        const hipError_t le = hipGetLastError();
+        ATJUP_HIP_CHECK(hipStreamEndCapture(c->stream, &graph));
+        if(le != hipSuccess){ (void)hipGraphDestroy(graph); set_error(std::string("adjoint sweep capture: ") + hipGetErrorString(le)); return ATJUP_ERR_HIP; }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if(ie != hipSuccess){ set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); return ATJUP_ERR_HIP; }
+        A->graphs.push_back({ sweeps, omega, exec });
     }
+    hipEvent_t t_sw = timer_begin(c);
+    ATJUP_HIP_CHECK(hipGraphLaunch(exec, c->stream));
+    timer_end(c, ATJUP_T_ADJ_SWEEPS, t_sw, 2 * sweeps + 5);
     hipLaunchKernelGGL(ka_rb_unpack_add, gfull, b128, 0, c->stream, L(c, ATJUP_F_P_DYN), (const double*)l0,
         (const double*)l1, c->jm, c->km, c->pitch, hp);
     // the source and prepareProjectionBoundaries
@@ -1048,17 +1151,23 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
 // transposes and the RK4's linearise about. On return the adjoint fields have been carried from
 // the end of the iteration to its start; the context's forward fields are left as the recording
 // left them (the n level and aux_* at the iteration's start, p_dyn and rho_mix after the block).
-static int adj_step(atjup_ctx* c, int iter, int first){
+static int adj_step(atjup_ctx* c, int iter, int first, const Buf* post = nullptr){
     const bool phys = (iter % 2 == 0) || first;
     const int sweeps = std::max(1, c->kn.press_sweeps);
     if(phys){
         if(int rc = copy_field(c, W(c, AW_PIN), Fd(c, ATJUP_F_P_DYN))) return rc;
         for(int f = 0; f < 3; f++)
             if(int rc = copy_field(c, W(c, AW_AUXP + f), Fd(c, ATJUP_F_AUX_U + f))) return rc;
-        if(int rc = launch_pressure(c, sweeps, c->kn.press_omega)) return rc;
-        if(c->kn.damp_pdyn)
-            if(int rc = launch_shapiro(c, ATJUP_F_P_DYN, 2, 1.0, 1, 7)) return rc;
-        if(int rc = launch_mixture_density(c)) return rc;
+        if(post){
+            // the state after the block, as the recomputation recorded it
+            for(int s = 0; s < N_POST; s++)
+                if(int rc = copy_field(c, Fd(c, kPostFields[s]), (const double*)post[s].p)) return rc;
+        } else {
+            if(int rc = launch_pressure(c, sweeps, c->kn.press_omega)) return rc;
+            if(c->kn.damp_pdyn)
+                if(int rc = launch_shapiro(c, ATJUP_F_P_DYN, 2, 1.0, 1, 7)) return rc;
+            if(int rc = launch_mixture_density(c)) return rc;
+        }
     }
     if(int rc = adj_bc(c)) return rc;
     if(int rc = adj_rk4(c)) return rc;
@@ -1137,6 +1246,8 @@ int atjup_hip_adj_init(atjup_ctx* c){
     const size_t hn = (size_t)c->im * c->jm * c->rb_hp;
     if(!rc) rc = adj_alloc(A->lsplit, 2 * hn * sizeof(double));
     if(!rc) rc = adj_alloc(A->lsrc, 2 * hn * sizeof(double));
+    if(!rc) rc = adj_alloc(A->mu, 2 * hn * sizeof(double));
+    if(!rc) rc = adj_alloc(A->inmask, 2 * hn);
     if(!rc) rc = adj_alloc(A->red, 2 * (size_t)c->im * sizeof(double));
     if(!rc) rc = adj_alloc(A->zs, 4 * (size_t)c->jm * sizeof(double));
     if(rc){ adj_free(c); return rc; }
@@ -1187,10 +1298,13 @@ int atjup_hip_adj_forward_checkpointed(atjup_ctx* c, int nsteps, int every, int 
     for(auto& b : A->snap) adj_release(b);
     const int nseg = (nsteps + every - 1) / every;
     A->ckpt.assign((size_t)nseg * N_STATE, Buf{});
+    for(auto& b : A->post) adj_release(b);
     A->snap.assign((size_t)every * N_STATE, Buf{});
+    A->post.assign((size_t)every * N_POST, Buf{});
     const size_t fb = c->n3 * sizeof(double);
     for(auto& b : A->ckpt) if(int rc = adj_alloc(b, fb)) return rc;
     for(auto& b : A->snap) if(int rc = adj_alloc(b, fb)) return rc;
+    for(auto& b : A->post) if(int rc = adj_alloc(b, fb)) return rc;
     A->nsteps = nsteps; A->every = every; A->first_iter = first_iter;
     for(int n = 0; n < nsteps; n++){
         if(n % every == 0)
@@ -1214,14 +1328,29 @@ int atjup_hip_adj_backward(atjup_ctx* c, const double* lam_final, double* lam_in
     for(int seg = nseg - 1; seg >= 0; seg--){
         const int s0 = seg * every, s1 = std::min(nsteps, s0 + every);
         if(int rc = load_state(c, &A->ckpt[(size_t)seg * N_STATE])) return rc;
+        // recompute the segment, keeping each step's start and the state after its physics block
+        hipEvent_t t_rc = timer_begin(c);
         for(int n = s0; n < s1; n++){
+            const int iter = A->first_iter + n, first = n == 0 ? 1 : 0;
             if(int rc = save_state(c, &A->snap[(size_t)(n - s0) * N_STATE])) return rc;
+            const bool phys = (iter % 2 == 0) || first;
+            if(phys){
+                if(int rc = step_block(c, iter, first)) return rc;
+                Buf* post = &A->post[(size_t)(n - s0) * N_POST];
+                for(int s = 0; s < N_POST; s++)
+                    if(int rc = copy_field(c, (double*)post[s].p, Fd(c, kPostFields[s]))) return rc;
+            }
             if(n + 1 < s1)
-                if(int rc = atjup_hip_step(c, A->first_iter + n, n == 0 ? 1 : 0)) return rc;
+                if(int rc = step_rest(c)) return rc;
         }
+        timer_end(c, ATJUP_T_ADJ_RECOMPUTE, t_rc, 0);
         for(int n = s1 - 1; n >= s0; n--){
+            const int iter = A->first_iter + n, first = n == 0 ? 1 : 0;
             if(int rc = load_state(c, &A->snap[(size_t)(n - s0) * N_STATE])) return rc;
-            if(int rc = adj_step(c, A->first_iter + n, n == 0 ? 1 : 0)) return rc;
+            const bool phys = (iter % 2 == 0) || first;
+            hipEvent_t t_as = timer_begin(c);
+            if(int rc = adj_step(c, iter, first, phys ? &A->post[(size_t)(n - s0) * N_POST] : nullptr)) return rc;
+            timer_end(c, ATJUP_T_ADJ_STEP, t_as, 0);
         }
     }
     if(lam_init)

@@ -159,6 +159,39 @@ void timer_end(atjup_ctx* c, int phase, hipEvent_t begin, int launches){
     }
 }
 
+// the physics block of an iteration (cJupiterModel.cpp:1032-1161), on even iterations and the first
+int step_block(atjup_ctx* c, int iter, int first){
+    int rc = 0;
+    if(!(iter % 2 == 0 || first)) return 0;
+    hipEvent_t t0 = timer_begin(c);
+    if((rc = launch_pressure(c, std::max(1, c->kn.press_sweeps), c->kn.press_omega))) return rc;
+    timer_end(c, ATJUP_T_PRESSURE, t0, 0);
+    if(c->kn.damp_pdyn && (rc = launch_shapiro(c, ATJUP_F_P_DYN, 2, 1.0, 1, 7))) return rc;
+    if((rc = launch_mixture_density(c))) return rc;
+    if(!c->kn.dry){
+        // the three adjustments, then RK_SCALAR_SYNC 2 (the identity on the shared storage),
+        // the closure with its wall function, the NH4SH limiter and its smoothing
+        // (cJupiterModel.cpp:1060-1161)
+        for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
+            if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
+        if((rc = atjup_hip_turbulence_run(c))) return rc;
+        if((rc = launch_flux_limiter(c))) return rc;
+        if((rc = launch_shapiro(c, ATJUP_F_FLUXLIM_NH4SH, 2, 1.0, 1, 7))) return rc;
+    }
+    return 0;
+}
+
+// the rest of the iteration: RungeKuttaJup, the boundary passes (cJupiterModel.cpp:1263-1272), the
+// closure's wall hold (:1294), the species clamp and the commit of the live k*, omega*
+int step_rest(atjup_ctx* c){
+    int rc = 0;
+    if((rc = launch_rk4(c))) return rc;
+    if((rc = launch_bc_after_step(c))) return rc;
+    if(!c->kn.dry && c->kn.turb_model >= 0 && (rc = launch_turb_wall(c))) return rc;
+    if((rc = launch_clamp_species(c))) return rc;
+    return launch_commit_live(c);
+}
+
 }  // namespace atjup
 
 using namespace atjup;
@@ -483,30 +516,9 @@ int atjup_hip_scalar_sync(atjup_ctx* c){ return c ? 0 : ATJUP_ERR_ARG; }
 
 int atjup_hip_step(atjup_ctx* c, int iter, int first){
     if(int rc = need_mask(c)) return rc;
-    int rc = 0;
     hipEvent_t t_step = timer_begin(c);
-    if(iter % 2 == 0 || first){
-        hipEvent_t t0 = timer_begin(c);
-        if((rc = launch_pressure(c, std::max(1, c->kn.press_sweeps), c->kn.press_omega))) return rc;
-        timer_end(c, ATJUP_T_PRESSURE, t0, 0);
-        if(c->kn.damp_pdyn && (rc = launch_shapiro(c, ATJUP_F_P_DYN, 2, 1.0, 1, 7))) return rc;
-        if((rc = launch_mixture_density(c))) return rc;
-        if(!c->kn.dry){
-            // the three adjustments, then RK_SCALAR_SYNC 2 (the identity on the shared storage),
-            // the closure with its wall function, the NH4SH limiter and its smoothing
-            // (cJupiterModel.cpp:1060-1161)
-            for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
-                if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
-            if((rc = atjup_hip_turbulence_run(c))) return rc;
-            if((rc = launch_flux_limiter(c))) return rc;
-            if((rc = launch_shapiro(c, ATJUP_F_FLUXLIM_NH4SH, 2, 1.0, 1, 7))) return rc;
-        }
-    }
-    if((rc = launch_rk4(c))) return rc;
-    if((rc = launch_bc_after_step(c))) return rc;
-    if(!c->kn.dry && c->kn.turb_model >= 0 && (rc = launch_turb_wall(c))) return rc;   // cJupiterModel.cpp:1294
-    if((rc = launch_clamp_species(c))) return rc;
-    if((rc = launch_commit_live(c))) return rc;
+    if(int rc = step_block(c, iter, first)) return rc;
+    if(int rc = step_rest(c)) return rc;
     timer_end(c, ATJUP_T_STEP, t_step, 0);
     return 0;
 }

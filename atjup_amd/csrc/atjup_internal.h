@@ -133,6 +133,11 @@ int launch_hydrostatic(atjup_ctx* c);
 int launch_rk4(atjup_ctx* c);
 int launch_pressure(atjup_ctx* c, int sweeps, double omega);
 int launch_bc_after_step(atjup_ctx* c);
+int launch_prep_projection(atjup_ctx* c);          // prepareProjectionBoundaries alone (FP64)
+// one loop iteration in two parts (atjup_hip_step = step_block + step_rest): the physics block of
+// the even and first iterations, then the RK4 step, the boundary passes and the clamps
+int step_block(atjup_ctx* c, int iter, int first);
+int step_rest(atjup_ctx* c);
 int launch_shapiro(atjup_ctx* c, int field, int order, double strength, int passes, int axes);
 int launch_mixture_density(atjup_ctx* c);
 int launch_clamp_species(atjup_ctx* c);

@@ -1375,6 +1375,13 @@ int launch_rk4(atjup_ctx* c){
     return DISPATCH(do_rk4_dry, c);
 }
 int launch_pressure(atjup_ctx* c, int sweeps, double omega){ return DISPATCH(do_pressure, c, sweeps, omega); }
+int launch_prep_projection(atjup_ctx* c){
+    if(c->precision != ATJUP_PRECISION_FP64){ set_error("prepareProjectionBoundaries alone: FP64 only"); return ATJUP_ERR_PROFILE; }
+    hipLaunchKernelGGL(k_prep_projection<double>, dim3(nblk(c->km - 1, 128), c->jm - 2), dim3(128), 0, c->stream,
+        (double*)c->cur[ATJUP_F_AUX_U].p, (double*)c->cur[ATJUP_F_AUX_V].p, (double*)c->cur[ATJUP_F_AUX_W].p,
+        c->im, c->jm, c->km, c->pitch, c->plane, 4.0 / 3.0, 1.0 / 3.0);
+    return check_launch();
+}
 int launch_bc_after_step(atjup_ctx* c){ return DISPATCH(do_bc, c); }
 int launch_shapiro(atjup_ctx* c, int field, int order, double strength, int passes, int axes){
     return DISPATCH(do_shapiro, c, field, order, strength, passes, axes);

@@ -263,7 +263,7 @@ class Context:
                                                 None if out is None else _dp(out)), "adj_backward")
         return out
 
-    TIMERS = ("poisson_sweeps", "rk_stages", "pressure", "step")
+    TIMERS = ("poisson_sweeps", "rk_stages", "pressure", "step", "adj_sweeps", "adj_step", "adj_recompute")
 
     def set_timing(self, on=True):
         self._chk(self.L.atjup_hip_set_timing(self.h, 1 if on else 0), "set_timing")

@@ -29,6 +29,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s FP32-refined vs FP64 at <=1e-5 rel err; adjoint/forward time"
 INTERIOR_CELLS = 39 * 175 * 360          # RK4 pass A range (RungeKutta_Jup_Turb.cpp:388-412)
+
+
+def interior_cells(g):
+    """Cells one step integrates: i 1..im-2, j 3..jm-4, k 0..km-2 (RungeKutta_Jup_Turb.cpp:388-412)."""
+    return (g.im - 2) * (g.jm - 6) * (g.km - 1)
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = {
     "dyn": "dynamics-only FP64 time step",
@@ -110,7 +115,7 @@ def cpu_baseline(config, sweeps, steps=10):
                       f"same 41x181x361 case, {dt:.1f} s, OMP_NUM_THREADS={threads}, -O2"}
 
 
-def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6):
+def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=None):
     """The 20-step jet-misfit gradient on the dynamics profile: the time of one plain forward
     (steps FP64 steps), of the checkpointed forward and of the reverse sweep, all between library
     barriers with the adjoint seed already on the device; then dL/dw0 against central differences
@@ -119,9 +124,9 @@ def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6):
     from atjup_amd.adjoint import MisfitRunner
     from atjup_amd.lib import Context
 
-    g = inputs.Grid()
-    G, Cn, K, _ = abi.make_profile("dyn", sweeps=sweeps, omega=omega)
-    st = inputs.make_state("dyn", seed=11)
+    g = g or inputs.Grid()
+    G, Cn, K, _ = abi.make_profile("dyn", sweeps=sweeps, omega=omega, grid=g)
+    st = inputs.make_state("dyn", seed=11, g=g)
     ctx = Context(G, Cn, K, abi.PRECISION_FP64)
     ctx.load_state(st, skip=set(inputs.SPECIES))
     ctx.bc_after_step()
@@ -149,7 +154,11 @@ def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6):
     ctx.adj_zero()
     for f, name in enumerate(("t", "u", "v", "w")):
         ctx.adj_set(name, lam[f])
+    ctx.set_timing(True)
+    ctx.read_timers()                                                    # clear
     bwd_ms = clock(lambda: ctx.adj_backward(None, download=False))
+    tm = ctx.read_timers()
+    ctx.set_timing(False)
     grad = ctx.adj_get("w")
     errs = []
     for d_seed in range(directions):
@@ -160,10 +169,12 @@ def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6):
         errs.append(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
     ctx.close()
     return {
-        "workload": f"jet misfit at i = {g.im - 2} after {steps} FP64 dynamics steps, 41x181x361, dL/dw0 "
+        "workload": f"jet misfit at i = {g.im - 2} after {steps} FP64 dynamics steps, {g.im}x{g.jm}x{g.km}, dL/dw0 "
                     f"({g.im * g.jm * g.km} values), checkpoints every {every} steps",
         "forward_ms": fwd_ms, "checkpointed_forward_ms": ckpt_ms, "backward_ms": bwd_ms,
         "backward_over_forward": bwd_ms / fwd_ms,
+        "backward_phases_ms": {"recompute": tm["adj_recompute"][0], "adjoint_steps": tm["adj_step"][0],
+                               "adjoint_sweeps": tm["adj_sweeps"][0], "forward_solves": tm["pressure"][0]},
         "gradient_over_forward": (ckpt_ms + bwd_ms) / fwd_ms,
         "loss": loss,
         "fd_directions": directions, "fd_eps": eps,
@@ -185,6 +196,8 @@ def main():
     ap.add_argument("--omega", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
+    ap.add_argument("--grid", default="standard", choices=["standard", "fine"],
+                    help="standard: 41x181x361 (1 deg); fine: 81x361x721 (0.5 deg, no CPU baseline)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -200,9 +213,11 @@ def main():
     from atjup_amd import abi, inputs
     from atjup_amd.lib import Context
 
-    g = inputs.Grid()
-    G, Cn, K, _ = abi.make_profile(args.config, sweeps=args.sweeps, omega=args.omega)
-    st = inputs.make_state(args.config, seed=11)
+    g = inputs.Grid() if args.grid == "standard" else inputs.fine_grid()
+    G, Cn, K, _ = abi.make_profile(args.config, sweeps=args.sweeps, omega=args.omega, grid=g)
+    st = inputs.make_state(args.config, seed=11, g=g)
+    cells = interior_cells(g)
+    fine = args.grid == "fine"
 
     def timed_run(precision):
         """warmup + steps iterations from the seeded state; the steps timed between two barriers."""
@@ -246,7 +261,7 @@ def main():
                           "max_abs": float(np.max(np.abs(d))) / mb if mb > 0 else float(np.max(np.abs(d)))}
         worst = max(max(e["normwise"], e["max_abs"]) for e in errs.values())
         fp32 = {
-            "value": INTERIOR_CELLS * args.steps / (ms32 * 1e-3), "unit": "cell-updates/s",
+            "value": cells * args.steps / (ms32 * 1e-3), "unit": "cell-updates/s",
             "ms_per_step": ms32 / args.steps, "speedup_vs_fp64": ms / ms32, "finite": bool(ok32),
             "poisson_sweep_avg_launch_ms": tm32["poisson_sweeps"][0] / max(tm32["poisson_sweeps"][1], 1),
             "rk4_stage_avg_launch_ms": tm32["rk_stages"][0] / max(tm32["rk_stages"][1], 1),
@@ -255,7 +270,7 @@ def main():
         }
         c32.close()
 
-    value = INTERIOR_CELLS * args.steps / (ms * 1e-3)
+    value = cells * args.steps / (ms * 1e-3)
     launch_ms = sweep_ms / max(sweep_launches, 1)
     lb = poisson_launch_bytes(g, 8, args.omega)
     achieved = lb / (launch_ms * 1e-3) / 1e9
@@ -264,10 +279,11 @@ def main():
 
     adj = None
     if not args.no_adjoint:
-        adj = adjoint_leg(args.sweeps, args.omega)
+        # the fine shell: 50 steps with checkpoints every 10, fewer difference quotients
+        adj = adjoint_leg(args.sweeps, args.omega, g=g, **({"steps": 50, "every": 10, "directions": 4} if fine else {}))
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not fine:
         try:
             cpu = cpu_baseline(args.config, args.sweeps)
         except Exception as e:  # the baseline never decides the GPU number
@@ -287,7 +303,7 @@ def main():
         "dtype": "f64",
         "data": DATA[args.config],
         "config": {
-            "workload": WORKLOAD[args.config] + f", 41x181x361 shell, RK4 + red-black Poisson "
+            "workload": WORKLOAD[args.config] + f", {g.im}x{g.jm}x{g.km} shell, RK4 + red-black Poisson "
                         f"({args.sweeps} sweeps, omega={args.omega}) on iterations 1,2,4 of each 5-step window",
             "grid": [g.im, g.jm, g.km], "sweeps": args.sweeps, "omega": args.omega,
             "precision": "fp64", "finite": bool(ok),

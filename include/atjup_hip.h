@@ -205,6 +205,9 @@ enum atjup_timer {
     ATJUP_T_RK_STAGES,            /* the four fused RK4 stage kernels of one step */
     ATJUP_T_PRESSURE,             /* the whole pressure solve */
     ATJUP_T_STEP,                 /* one loop iteration (atjup_hip_step) */
+    ATJUP_T_ADJ_SWEEPS,           /* the transposed sweeps of one adjoint solve (2 launches per sweep) */
+    ATJUP_T_ADJ_STEP,             /* one transposed loop iteration of atjup_hip_adj_backward */
+    ATJUP_T_ADJ_RECOMPUTE,        /* the forward recomputation of one checkpoint segment */
     ATJUP_N_TIMERS
 };
 int atjup_hip_set_timing(atjup_ctx* ctx, int enable);
