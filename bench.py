@@ -35,6 +35,10 @@ def interior_cells(g):
     """Cells one step integrates: i 1..im-2, j 3..jm-4, k 0..km-2 (RungeKutta_Jup_Turb.cpp:388-412)."""
     return (g.im - 2) * (g.jm - 6) * (g.km - 1)
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the colour pass from two rocprofv3 PMC runs (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, KiB per dispatch, the factors from a device copy of known size), per grid;
+# profiles/step_full_41x181x361_hbm_traffic.md holds the summary and the command
+SWEEP_TRAFFIC_BYTES = {(41, 181, 361): (13872.6 * 2.0 + 10122.1) * 1024}
 WORKLOAD = {
     "dyn": "dynamics-only FP64 time step",
     "full": "full-physics FP64 time step (k-omega closure, saturation adjustment of H2O/NH3/CH4, NH4SH "
@@ -314,8 +318,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_poisson_sweep<double> (one red-black colour pass)",
+            "traffic": SWEEP_TRAFFIC_BYTES.get((g.im, g.jm, g.km)),
+            "kernel": "k_rb_sweep<double, double> (one red-black colour pass)",
             "algorithmic_bytes_per_launch": lb,
             "avg_launch_ms": launch_ms,
             "launches": int(sweep_launches),
