@@ -137,3 +137,22 @@ def test_small_grid_full_steps():
     compare(gpu, orc, PROG + ("p_dyn",) + TURB_OUT, TOL, "small grid")
     gpu.close()
     orc.close()
+
+
+def test_full_grid_steps_with_200_sweep_solves():
+    """Four loop iterations on the 41 x 181 x 361 shell as the benchmark runs them (three
+    200-sweep solves, the physics block on iterations 1, 2 and 4): every prognostic field, p_dyn
+    and the closure's outputs within 1e-10 of the oracle, the cloud-base index of every column of
+    every condensate bit-exact."""
+    st = inputs.make_state("full", seed=11)
+    gpu, orc, _ = make_pair("full", sweeps=200, state=st)
+    for it in range(1, 5):
+        gpu.step(it, it == 1)
+        orc.call("step", it, 1 if it == 1 else 0)
+    worst = compare(gpu, orc, PROG + ("p_dyn",) + TURB_OUT, TOL, "full grid")
+    print("full grid, 4 steps (max relative error, cells not bit-identical):",
+          {n: f"{e:.1e}/{d}" for n, (e, d) in worst.items()})
+    for n in CLOUDS:
+        assert np.array_equal(cloud_base(gpu.get(n)), cloud_base(orc.get(n))), n
+    gpu.close()
+    orc.close()
