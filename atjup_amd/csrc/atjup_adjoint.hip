@@ -341,7 +341,7 @@ struct RhsAdjArgs {
     double cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, cbuoy, g_, rR, t_ref, kt, visc0;
 };
 
-__global__ __launch_bounds__(256) void ka_rhs_dry(const RhsAdjArgs a){
+__global__ __launch_bounds__(256) void ka_rhs_dry(const RhsAdjArgs a, const GeoIJ<double>* __restrict__ geo){
     const int b = xcd_remap(blockIdx.x, a.nb);
     const int kb = b % a.nkb;
     const int rest = b / a.nkb;
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void ka_rhs_dry(const RhsAdjArgs a){
     const int kw = (k == 0) ? a.km - 2 : k - 1;
     const int ke = (k == a.km - 2) ? 0 : k + 1;
     const size_t qkw = q - k + kw, qke = q - k + ke;
-    const GeoIJ<double> G = a.geo[(size_t)i * a.jm + j];
+    const GeoIJ<double> G = geo[wave_uniform(i * a.jm + j)];      // a wavefront is one (i, j) row
 
     double c_[4], d1r[4], d1t[4], d1p[4];
     for(int f = 0; f < 4; f++){
@@ -645,15 +645,16 @@ __global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restric
     // stored-value reads of this cell by the other colour's sweep cells s: sum w mu_s
     const uint8_t bits = inm[q];
     double dsum = 0.0;
-    if(bits & 1) dsum += pw[(size_t)(i + 1) * jm + j].num1 * mu_o[q + P];
-    if(bits & 2) dsum += pw[(size_t)(i - 1) * jm + j].num1 * mu_o[q - P];
-    if(bits & 4) dsum += pw[(size_t)i * jm + j + 1].num2 * mu_o[q + R];
-    if(bits & 8) dsum += pw[(size_t)i * jm + j - 1].num2 * mu_o[q - R];
+    const int ij = wave_uniform(i * jm + j);          // a wavefront is one (i, j) row
+    if(bits & 1) dsum += pw[ij + jm].num1 * mu_o[q + P];
+    if(bits & 2) dsum += pw[ij - jm].num1 * mu_o[q - P];
+    if(bits & 4) dsum += pw[ij + 1].num2 * mu_o[q + R];
+    if(bits & 8) dsum += pw[ij - 1].num2 * mu_o[q - R];
     if(!interior){
         lam[q] += dsum;                              // a plane cell, read as stored in the first sweep
         return;
     }
-    const PoisIJ<double> W = pw[(size_t)i * jm + j];
+    const PoisIJ<double> W = pw[ij];
     if(bits & 48){
         const int off = (colour + i + j) & 1, ring = 2 * nm, kk = 2 * m + off;
         if(bits & 16) dsum += W.num3 * mu_o[row + (((kk + 1) % ring) >> 1)];
@@ -662,10 +663,10 @@ __global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restric
     // formed-plane reads by the other colour, for which this cell is the one behind the plane
     double bsum = 0.0;
     if(ghost){
-        if(i == 2 && rb_reads(fo_arr[q - P], 1))      bsum += -c13 * pw[(size_t)1 * jm + j].num1 * mu_o[q - P];
-        if(i == im - 3 && rb_reads(fo_arr[q + P], 0)) bsum += -c13 * pw[(size_t)(im - 2) * jm + j].num1 * mu_o[q + P];
-        if(j == 2 && rb_reads(fo_arr[q - R], 3))      bsum += -c13 * pw[(size_t)i * jm + 1].num2 * mu_o[q - R];
-        if(j == jm - 3 && rb_reads(fo_arr[q + R], 2)) bsum += -c13 * pw[(size_t)i * jm + jm - 2].num2 * mu_o[q + R];
+        if(i == 2 && rb_reads(fo_arr[q - P], 1))      bsum += -c13 * pw[ij - jm].num1 * mu_o[q - P];
+        if(i == im - 3 && rb_reads(fo_arr[q + P], 0)) bsum += -c13 * pw[ij + jm].num1 * mu_o[q + P];
+        if(j == 2 && rb_reads(fo_arr[q - R], 3))      bsum += -c13 * pw[ij - 1].num2 * mu_o[q - R];
+        if(j == jm - 3 && rb_reads(fo_arr[q + R], 2)) bsum += -c13 * pw[ij + 1].num2 * mu_o[q + R];
     }
     // this cell's own pass
     const uint8_t fl = fl_arr[q];
@@ -1037,7 +1038,7 @@ static int adj_rk4(atjup_ctx* c){
         if(int rc = combine(wts[s], cin[s], s < 3 ? 1 : 0, 1)) return rc;
         for(int f = 0; f < 4; f++) a.y[f] = (s == 0) ? Fd(c, f) : W(c, AW_Y + 4 * (s - 1) + f);
         for(int f = 0; f < 3; f++) a.la[f] = (s == 3) ? L(c, ATJUP_F_AUX_U + f) : nullptr;
-        hipLaunchKernelGGL(ka_rhs_dry, dim3(a.nb), dim3(64, 4), 0, c->stream, a);
+        hipLaunchKernelGGL(ka_rhs_dry, dim3(a.nb), dim3(64, 4), 0, c->stream, a, a.geo);
         if(int rc = check_launch()) return rc;
     }
     if(int rc = combine(0.0, 0.0, 1, 0)) return rc;             // lambda_N += lambda_{Y_0}

@@ -261,6 +261,8 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     const size_t hn = (size_t)c->im * c->jm * c->rb_hp;
     if(!rc) rc = alloc(c->psrc, 2 * hn * asz);
     if(!rc) rc = alloc(c->rb_p, 2 * hn * asz);
+    if(!rc && precision == ATJUP_PRECISION_FP32_REF)
+        for(int f = 0; f < 4 && !rc; f++) rc = alloc(c->lo[f], c->n3 * sizeof(float));
     if(!rc && precision == ATJUP_PRECISION_FP32_REF){
         rc = alloc(c->rb_d, 2 * hn * sizeof(float));
         if(!rc) rc = alloc(c->rb_r, 2 * hn * sizeof(float));
@@ -328,6 +330,7 @@ void atjup_hip_ctx_destroy(atjup_ctx* c){
     for(auto& s : c->stg) for(auto& b : s) release(b);
     for(auto& b : c->acc) release(b);
     for(auto& b : c->live_turb) release(b);
+    for(atjup::Buf* b : { &c->lo[0], &c->lo[1], &c->lo[2], &c->lo[3] }) release(*b);
     for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
                           &c->amb_rows, &c->nonfinite })
@@ -361,6 +364,8 @@ static int upload_impl(atjup_ctx* c, int f, const double* src, double* const* co
             if(host[b] != 0){ set_error("the dry profile requires the species to be zero"); return ATJUP_ERR_PROFILE; }
     }
     ATJUP_HIP_CHECK(hipMemcpyAsync(c->cur[s].p, host.data(), host.size(), hipMemcpyHostToDevice, c->stream));
+    if(s < 4 && c->lo[s].p)                       // an uploaded t, u, v, w starts without a low word
+        ATJUP_HIP_CHECK(hipMemsetAsync(c->lo[s].p, 0, c->lo[s].bytes, c->stream));
     ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
     if(s == ATJUP_F_TKE || s == ATJUP_F_DIS) c->turb_live_pending = false;
     return 0;
@@ -375,6 +380,11 @@ static int download_impl(atjup_ctx* c, int f, double* dst, double** const* rows)
     // the live k*, omega* of a turbulence run that waits for its step; their n-level ids read the n level
     const void* src_p = nlevel(f) ? c->cur[s].p : live_ptr(c, s);
     ATJUP_HIP_CHECK(hipMemcpyAsync(host.data(), src_p, host.size(), hipMemcpyDeviceToHost, c->stream));
+    std::vector<float> lo;
+    if(s < 4 && c->lo[s].p){                       // t, u, v, w of the refined form: hi + lo
+        lo.resize(c->n3);
+        ATJUP_HIP_CHECK(hipMemcpyAsync(lo.data(), c->lo[s].p, c->n3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
     ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
     const size_t row = (size_t)c->km;
     for(int i = 0; i < c->im; i++)
@@ -383,7 +393,11 @@ static int download_impl(atjup_ctx* c, int f, double* dst, double** const* rows)
             const size_t o = c->idx(i, j, 0);
             if(c->esz == 8) std::memcpy(r, (const double*)host.data() + o, row * 8);
             else if(c->esz == 16){ const double* d = (const double*)host.data() + 2 * o; for(size_t k = 0; k < row; k++) r[k] = d[2 * k]; }
-            else { const float* d = (const float*)host.data() + o; for(size_t k = 0; k < row; k++) r[k] = (double)d[k]; }
+            else {
+                const float* d = (const float*)host.data() + o;
+                if(lo.empty()) for(size_t k = 0; k < row; k++) r[k] = (double)d[k];
+                else for(size_t k = 0; k < row; k++) r[k] = (double)d[k] + (double)lo[o + k];
+            }
         }
     return 0;
 }

@@ -18,6 +18,11 @@ __device__ __forceinline__ int xcd_remap(int b, int nb){
     return start + n;
 }
 
+// An index every active lane of a wavefront holds (a per-(i, j) table entry where a wavefront
+// spans one k row), made explicit so that the entry is read once per wavefront through scalar
+// loads instead of once per lane through vector loads.
+__device__ __forceinline__ int wave_uniform(int v){ return __builtin_amdgcn_readfirstlane(v); }
+
 template<typename T>
 __device__ __forceinline__ T safe_clamp_d(T v, T lo, T hi){
     if(!isfinite(v)) return lo;                   // exponent-bit test of RungeKutta_Jup_Turb.cpp:285-290

@@ -139,12 +139,24 @@ struct StageArgs {
     T inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
     double cbuoy, g, rR, t_ref_d;          // the buoyancy anomaly is formed in FP64 in both forms
     T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, t_ref, kt, visc0;
+    float* lo[4];                          // FP32-refined form: the low words of t, u, v, w (else null)
+    double dt_d;
 };
+
+// The FP32-refined form carries t, u, v, w as an FP32 pair hi + lo. The last RK stage adds the
+// step's increment to hi + lo in FP64 and splits the sum again, so that an increment of a few
+// ulps of the field is not rounded the same way step after step (the stencils read hi only).
+__device__ __forceinline__ void split_store(float* hi, float* lo, size_t q, double v){
+    const float h = (float)v;
+    hi[q] = h;
+    lo[q] = (float)(v - (double)h);
+}
 
 constexpr int ST_BK = 64, ST_BJ = 4;
 
 template<typename T>
-__global__ __launch_bounds__(ST_BK * ST_BJ) void k_stage_dry(const StageArgs<T> a){
+__global__ __launch_bounds__(ST_BK * ST_BJ) void k_stage_dry(const StageArgs<T> a,
+        const GeoIJ<geo_t<T>>* __restrict__ geo){
     const int b = xcd_remap(blockIdx.x, a.nb);
     const int kb = b % a.nkb;
     const int rest = b / a.nkb;
@@ -167,7 +179,7 @@ __global__ __launch_bounds__(ST_BK * ST_BJ) void k_stage_dry(const StageArgs<T> 
     const int kw = (k == 0) ? a.km - 2 : k - 1;
     const int ke = (k == a.km - 2) ? 0 : k + 1;
     const size_t qkw = q - k + kw, qke = q - k + ke;
-    const GeoIJ<geo_t<T>> G = a.geo[(size_t)i * a.jm + j];
+    const GeoIJ<geo_t<T>> G = geo[wave_uniform(i * a.jm + j)];     // a wavefront is one (i, j) row
 
     T c_[4], d1r[4], d2r[4], d1t[4], d2t[4], d1p[4], d2p[4];
     #pragma unroll
@@ -260,6 +272,13 @@ __global__ __launch_bounds__(ST_BK * ST_BJ) void k_stage_dry(const StageArgs<T> 
         #pragma unroll
         for(int f = 0; f < 4; f++){
             const T accf = a.acc[f][q] + a.wgt * rhs[f];
+            if constexpr(std::is_same<T, float>::value){
+                if(a.lo[f]){
+                    split_store(a.out[f], a.lo[f], q,
+                                ((double)a.nl[f][q] + (double)a.lo[f][q]) + a.dt_d * (double)accf * (1.0 / 6.0));
+                    continue;
+                }
+            }
             a.out[f][q] = a.nl[f][q] + a.dt * accf * a.one_sixth;
         }
         a.aux_u[q] = rhs_u + dpdr;
@@ -474,7 +493,7 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep(T* __restrict__ x0, 
         const T pjm = (j == 1 && gw)      ? (T)((T)c43 * pc - (T)c13 * oth[q + R]) : oth[q - R];
         const T pkp = oth[row + me], pkm = oth[row + mw];
         const uint8_t fl = flg[q];
-        const PoisIJ<double> W = pw[(size_t)i * jm + j];
+        const PoisIJ<double> W = pw[wave_uniform(i * jm + j)];     // a wavefront is one (i, j) row
         const T num1 = (T)W.num1, num2 = (T)W.num2, num3 = (T)W.num3;
         T acc, den;
         bool skip = false;
@@ -764,7 +783,8 @@ __global__ void k_bc_zonal_means(const T* __restrict__ v, const T* __restrict__ 
 // ATJUP defaults: 2-point Neumann, rigid lid on u, TOP_TAPER 3, species wall copy, positivity.
 template<typename T>
 __global__ __launch_bounds__(256) void k_bc_radius(const BCLists<T> L, const acc_t<T>* __restrict__ zm,
-        int im, int jm, int km, int pitch, size_t plane, double c43, double c13){
+        int im, int jm, int km, int pitch, size_t plane, double c43, double c13,
+        float* __restrict__ lo_v, float* __restrict__ lo_w){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if(k >= km) return;
     const size_t q0 = (size_t)j * pitch + k;
@@ -789,6 +809,10 @@ __global__ __launch_bounds__(256) void k_bc_radius(const BCLists<T> L, const acc
     W[qt - plane] += (T)((2.0 / 3.0) * w2);
     V[qt] = (T)v2;
     W[qt] = (T)w2;
+    if(lo_v){                                      // the sponge's levels carry no low words
+        lo_v[qt - plane] = 0.0f; lo_v[qt - 2 * plane] = 0.0f;
+        lo_w[qt - plane] = 0.0f; lo_w[qt - 2 * plane] = 0.0f;
+    }
     for(int f = 0; f < L.n_species; f++){
         T* X = L.species[f];
         X[q0] = X[q0 + plane];
@@ -1040,6 +1064,8 @@ int do_rk4_dry(atjup_ctx* c, const RkBuffers* rb = nullptr){
     a.nkb = (c->km + ST_BK - 1) / ST_BK; a.njb = (c->jm + ST_BJ - 1) / ST_BJ;
     a.nb = a.nkb * a.njb * c->im;
     a.dt = (T)dt; a.one_sixth = (T)(1.0 / 6.0);
+    a.dt_d = dt;
+    for(int f = 0; f < 4; f++) a.lo[f] = (float*)c->lo[f].p;
     a.inv_2dr = (T)(1.0 / (2.0 * c->g.dr)); a.inv_2dthe = (T)(1.0 / (2.0 * c->g.dthe));
     a.inv_2dphi = (T)(1.0 / (2.0 * c->g.dphi)); a.inv_dr2 = (T)(1.0 / (c->g.dr * c->g.dr));
     a.inv_dthe2 = (T)(1.0 / (c->g.dthe * c->g.dthe)); a.inv_dphi2 = (T)(1.0 / (c->g.dphi * c->g.dphi));
@@ -1078,7 +1104,7 @@ int do_rk4_dry(atjup_ctx* c, const RkBuffers* rb = nullptr){
             a.aux_u = (T*)rb->aux[0]; a.aux_v = (T*)rb->aux[1]; a.aux_w = (T*)rb->aux[2];
         }
         hipEvent_t t0 = timer_begin(c);
-        hipLaunchKernelGGL(k_stage_dry<T>, dim3(a.nb), dim3(ST_BK, ST_BJ), 0, c->stream, a);
+        hipLaunchKernelGGL(k_stage_dry<T>, dim3(a.nb), dim3(ST_BK, ST_BJ), 0, c->stream, a, a.geo);
         timer_end(c, ATJUP_T_RK_STAGES, t0, 1);
         if(int rc = check_launch()) return rc;
     }
@@ -1268,7 +1294,8 @@ int do_bc(atjup_ctx* c){
     hipLaunchKernelGGL(k_bc_zonal_means<T>, dim3(nblk(c->jm, 64), 4), dim3(64), 0, c->stream,
         L.v, L.w, (acc_t<T>*)c->zm.p, c->im, c->jm, c->km, c->pitch);
     hipLaunchKernelGGL(k_bc_radius<T>, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream,
-        L, (const acc_t<T>*)c->zm.p, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
+        L, (const acc_t<T>*)c->zm.p, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13,
+        (float*)c->lo[ATJUP_F_V].p, (float*)c->lo[ATJUP_F_W].p);
     hipLaunchKernelGGL(k_bc_theta<T>, dim3(nblk(c->km, 128), c->im), dim3(128), 0, c->stream,
         L, c->im, c->jm, c->km, c->pitch);
     hipLaunchKernelGGL(k_bc_phi<T>, dim3(nblk(c->jm, 64), c->im), dim3(64), 0, c->stream,

@@ -51,6 +51,8 @@ struct StageFullArgs {
     T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, t_ref;
     T t_coef, inv_re, rad_coupling, precip_coupling, turb_coupling, L_rad, R_mix, cp_mix, u_0;
     T cr, pc, stokes_a, rg_nh4sh, mue_mix, nue_max, tke_max_nd, dis_min;
+    float* lo[4];                          // FP32-refined form: the low words of t, u, v, w (else null)
+    double dt_d;
 };
 
 constexpr int SF_BK = 64, SF_BJ = 4;
@@ -111,6 +113,16 @@ __device__ __forceinline__ void stage_update(const StageFullArgs<T>& a, int f, s
         else if(kind == 4) v = dmax(a.dis_min, v);
         a.out[f][q] = v;
     } else {
+        if constexpr(std::is_same<T, float>::value){
+            // t, u, v, w of the refined form: the increment added to hi + lo in FP64 and split
+            if(f < 4 && a.lo[f]){
+                const double s = ((double)a.nl[f][q] + (double)a.lo[f][q]) + a.dt_d * (double)accf * (1.0 / 6.0);
+                const float h = (float)s;
+                a.out[f][q] = h;
+                a.lo[f][q] = (float)(s - (double)h);
+                return;
+            }
+        }
         T v = a.nl[f][q] + a.dt * accf * a.one_sixth;
         if(kind == 1) v = dmax(T(0), v);
         else if(kind == 2){ if(v < T(0)) v = T(0); }
@@ -124,7 +136,8 @@ __device__ __forceinline__ void stage_update(const StageFullArgs<T>& a, int f, s
 // centre velocities, the eddy viscosity and the density of the cell, so this kernel runs with few
 // registers and many waves while the dynamics kernel carries the heavy part.
 template<typename T>
-__global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_species(const StageFullArgs<T> a){
+__global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_species(const StageFullArgs<T> a,
+        const GeoIJ<T>* __restrict__ geo){
     const int b = xcd_remap(blockIdx.x, a.nb);
     const int i = b % a.im;
     const int rest = b / a.im;
@@ -146,7 +159,7 @@ __global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_species(const StageFull
     const int kw = (k == 0) ? a.km - 2 : k - 1;
     const int ke = (k == a.km - 2) ? 0 : k + 1;
     const size_t qkw = q - k + kw, qke = q - k + ke;
-    const GeoIJ<T> G = a.geo[(size_t)i * a.jm + j];
+    const GeoIJ<T> G = geo[wave_uniform(i * a.jm + j)];     // a wavefront is one (i, j) row
     const T u = a.in[ATJUP_F_U][q], v = a.in[ATJUP_F_V][q], w = a.in[ATJUP_F_W][q];
     const T v_invrm = v * G.inv_rm;
     const T w_invrs = w * G.inv_rmsinthe;
@@ -163,7 +176,8 @@ __global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_species(const StageFull
 }
 
 template<typename T>
-__global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(const StageFullArgs<T> a){
+__global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(const StageFullArgs<T> a,
+        const GeoIJ<T>* __restrict__ geo){
     const int b = xcd_remap(blockIdx.x, a.nb);
     // levels fastest: the blocks resident on an XCD at one time cover whole columns, so the i +- 1
     // neighbours of the 17 fields a block reads are in flight in the same L2 (4 % faster per stage
@@ -188,7 +202,7 @@ __global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(cons
     const int kw = (k == 0) ? a.km - 2 : k - 1;
     const int ke = (k == a.km - 2) ? 0 : k + 1;
     const size_t qkw = q - k + kw, qke = q - k + ke;
-    const GeoIJ<T> G = a.geo[(size_t)i * a.jm + j];
+    const GeoIJ<T> G = geo[wave_uniform(i * a.jm + j)];     // a wavefront is one (i, j) row
     const int stage = a.stage;
     auto update = [&](int f, T rhs, int kind){ stage_update(a, f, q, rhs, kind); };
 
@@ -830,11 +844,12 @@ __device__ void satadj_cell(double& cv, double& ccl, double& cic, double& tq, do
 template<typename T>
 __global__ __launch_bounds__(256) void k_satadj(T* __restrict__ c, T* __restrict__ cloud, T* __restrict__ ice,
         T* __restrict__ t, const T* __restrict__ ps, const T* __restrict__ rho, const uint8_t* __restrict__ flags,
-        int jm, int km, int pitch, const SatParams P){
+        int jm, int km, int pitch, const SatParams P, float* __restrict__ lo_t){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
     if(k >= km) return;
     const size_t q = ((size_t)i * jm + j) * pitch + k;
     double cv = (double)c[q], ccl = (double)cloud[q], cic = (double)ice[q], tq = (double)t[q];
+    if(lo_t) tq += (double)lo_t[q];                // the refined form's t is hi + lo
     const double t0 = tq;
     if(!(flags[q] & FL_SELF)) satadj_cell(cv, ccl, cic, tq, (double)ps[q], (double)rho[q], P);
     if(cv <= 0.0) cv = 0.0;
@@ -843,7 +858,15 @@ __global__ __launch_bounds__(256) void k_satadj(T* __restrict__ c, T* __restrict
     c[q] = (T)cv;
     cloud[q] = (T)ccl;
     ice[q] = (T)cic;
-    if(tq != t0 || !(tq == tq)) t[q] = (T)tq;
+    if(tq != t0 || !(tq == tq)){
+        if(lo_t){
+            const float h = (float)tq;
+            t[q] = h;
+            lo_t[q] = (float)(tq - (double)h);
+        } else {
+            t[q] = (T)tq;
+        }
+    }
 }
 
 // =============================================================================================
@@ -928,6 +951,8 @@ int do_rk4_full(atjup_ctx* c){
     a.nb = a.nkb * a.njb * c->im;
     a.turb = K.turb_model >= 0 ? 1 : 0;
     a.dt = (T)dt; a.one_sixth = (T)(1.0 / 6.0);
+    a.dt_d = dt;
+    for(int f = 0; f < 4; f++) a.lo[f] = (float*)c->lo[f].p;
     a.inv_2dr = (T)(1.0 / (2.0 * c->g.dr)); a.inv_2dthe = (T)(1.0 / (2.0 * c->g.dthe));
     a.inv_2dphi = (T)(1.0 / (2.0 * c->g.dphi)); a.inv_dr2 = (T)(1.0 / (c->g.dr * c->g.dr));
     a.inv_dthe2 = (T)(1.0 / (c->g.dthe * c->g.dthe)); a.inv_dphi2 = (T)(1.0 / (c->g.dphi * c->g.dphi));
@@ -981,8 +1006,8 @@ int do_rk4_full(atjup_ctx* c){
             a.bsrc[f] = (s <= 1) ? in : ((s == 3 && pend && turbf) ? L : nullptr);
         }
         hipEvent_t t0 = timer_begin(c);
-        hipLaunchKernelGGL(k_stage_full<T>, dim3(a.nb), dim3(SF_BK, SF_BJ), 0, c->stream, a);
-        hipLaunchKernelGGL(k_stage_species<T>, dim3(a.nb), dim3(SF_BK, SF_BJ), 0, c->stream, a);
+        hipLaunchKernelGGL(k_stage_full<T>, dim3(a.nb), dim3(SF_BK, SF_BJ), 0, c->stream, a, a.geo);
+        hipLaunchKernelGGL(k_stage_species<T>, dim3(a.nb), dim3(SF_BK, SF_BJ), 0, c->stream, a, a.geo);
         timer_end(c, ATJUP_T_RK_STAGES, t0, 2);
         if(int rc = check_launch()) return rc;
     }
@@ -1080,7 +1105,7 @@ int do_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){
     hipLaunchKernelGGL(k_satadj<T>, dim3(nblk(c->km, 128), c->jm, c->im), dim3(128), 0, c->stream,
         F<T>(c, ids[gas][0]), F<T>(c, ids[gas][1]), F<T>(c, ids[gas][2]), F<T>(c, ATJUP_F_T),
         F<T>(c, ATJUP_F_P_STAT), F<T>(c, ATJUP_F_RHO_MIX), (const uint8_t*)c->flags.p,
-        c->jm, c->km, c->pitch, P);
+        c->jm, c->km, c->pitch, P, (float*)c->lo[ATJUP_F_T].p);
     return check_launch();
 }
 
