@@ -245,17 +245,15 @@ __global__ __launch_bounds__(256) void ka_mixture(const double* __restrict__ t, 
 }
 
 __global__ __launch_bounds__(256) void ka_hydrostatic(const double* __restrict__ v, const double* __restrict__ w,
-        const uint8_t* __restrict__ flags, const double* __restrict__ rad, const double* __restrict__ cor_j,
+        const int2* __restrict__ colr, const double* __restrict__ rad, const double* __restrict__ cor_j,
         double* __restrict__ lph, double* __restrict__ lv, double* __restrict__ lw,
         int im, int jm, int km, int pitch, size_t plane, double dr){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if(k >= km) return;
     const size_t q0 = (size_t)j * pitch + k;
-    int i0 = 0;
-    while(i0 < im && (flags[q0 + i0 * plane] & FL_SELF)) i0++;
+    const int2 run = colr[(size_t)j * km + k];     // the column's fluid run from the bottom
+    const int i0 = run.x, i1 = run.y;
     if(i0 >= im){ for(int i = 0; i < im; i++) lph[q0 + i * plane] = 0.0; return; }
-    int i1 = i0;
-    while(i1 + 1 < im && !(flags[q0 + (i1 + 1) * plane] & FL_SELF)) i1++;
     const double cor_ = cor_j[j];
     double G = 0.0, lf_next = 0.0;
     for(int i = i1; i >= i0; i--){
@@ -1051,7 +1049,7 @@ static int adj_rk4(atjup_ctx* c){
         (const double*)Fd(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p, tabs + c->im, (const double*)A->red.p,
         W(c, AW_GBRL), L(c, ATJUP_F_T), c->jm, c->km, c->pitch, C.g, C.r_mix * C.R_mix, C.t_ref);
     hipLaunchKernelGGL(ka_hydrostatic, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream,
-        (const double*)Fd(c, ATJUP_F_V), (const double*)Fd(c, ATJUP_F_W), (const uint8_t*)c->flags.p,
+        (const double*)Fd(c, ATJUP_F_V), (const double*)Fd(c, ATJUP_F_W), (const int2*)c->colr.p,
         tabs, tabs + c->im + c->jm, L(c, ATJUP_F_P_HYDRO), L(c, ATJUP_F_V), L(c, ATJUP_F_W),
         c->im, c->jm, c->km, c->pitch, c->plane, c->g.dr);
     // aux_* were overwritten in the integrated cells: their earlier values carry no adjoint there

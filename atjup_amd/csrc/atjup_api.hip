@@ -271,6 +271,15 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     if(!rc) rc = alloc(c->mask, c->n3);
     if(!rc) rc = alloc(c->flags, c->n3);
     if(!rc) rc = alloc(c->itopo, (size_t)c->jm * c->km * sizeof(int));
+    if(!rc) rc = alloc(c->colr, (size_t)c->jm * c->km * 2 * sizeof(int));
+    if(!rc){
+        // no obstacle until a mask is uploaded: every column is fluid from 0 to im-1
+        std::vector<int> runs((size_t)c->jm * c->km * 2);
+        for(size_t q = 0; q < runs.size(); q += 2){ runs[q] = 0; runs[q + 1] = c->im - 1; }
+        if(hipMemcpy(c->colr.p, runs.data(), runs.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess){
+            set_error("hipMemcpy failed"); rc = ATJUP_ERR_HIP;
+        }
+    }
     if(!rc) rc = alloc(c->vel_star, (size_t)c->jm * c->km * sizeof(double));
     if(!rc) rc = alloc(c->brl, (size_t)c->im * asz);
     if(!rc) rc = alloc(c->brl_part, (size_t)c->im * 64 * 2 * asz);
@@ -330,7 +339,7 @@ void atjup_hip_ctx_destroy(atjup_ctx* c){
     for(auto& s : c->stg) for(auto& b : s) release(b);
     for(auto& b : c->acc) release(b);
     for(auto& b : c->live_turb) release(b);
-    for(atjup::Buf* b : { &c->lo[0], &c->lo[1], &c->lo[2], &c->lo[3] }) release(*b);
+    for(atjup::Buf* b : { &c->lo[0], &c->lo[1], &c->lo[2], &c->lo[3], &c->colr }) release(*b);
     for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
                           &c->amb_rows, &c->nonfinite })

@@ -83,17 +83,17 @@ __global__ void k_brl_final(const A* __restrict__ part, A* __restrict__ brl, int
 // ---------------------------------------------------------------------------------------------
 template<typename T>
 __global__ __launch_bounds__(256) void k_hydrostatic(const T* __restrict__ v, const T* __restrict__ w,
-        const uint8_t* __restrict__ flags, const double* __restrict__ rad, const double* __restrict__ cor_j,
+        const int2* __restrict__ colr, const double* __restrict__ rad, const double* __restrict__ cor_j,
         T* __restrict__ ph, int im, int jm, int km, int pitch, size_t plane, double dr){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if(k >= km) return;
     const size_t q0 = (size_t)j * pitch + k;
-    int i0 = 0;
-    while(i0 < im && (flags[q0 + i0 * plane] & FL_SELF)){ ph[q0 + i0 * plane] = T(0); i0++; }
+    // the column's fluid run from the bottom, fixed by the mask (k_col_range)
+    const int2 run = colr[(size_t)j * km + k];
+    const int i0 = run.x, i1 = run.y;
+    for(int i = 0; i < i0 && i < im; i++) ph[q0 + i * plane] = T(0);
     if(i0 >= im) return;
-    int i1 = i0;
-    while(i1 + 1 < im && !(flags[q0 + (i1 + 1) * plane] & FL_SELF)) i1++;
     for(int i = i1 + 1; i < im; i++) ph[q0 + i * plane] = T(0);
     // the column integral is formed in FP64 in both forms
     using A = acc_t<T>;
@@ -762,20 +762,29 @@ struct BCLists {
     T* species[MAXBC]; int n_species;               // the wall copy / positivity set (BC_Jup.h:232-262)
     T* solid[MAXBC]; int n_solid;                   // bcSolidGround scalars (BC_Jup.h:469-487)
     T *u, *v, *w, *t;
+    // what bcRadius does to a radius-list field after its Neumann planes: 0 nothing, 1 the rigid
+    // lid (u), 2 / 3 the top sponge (v / w), 4 the species wall copy and positivity
+    uint8_t radius_role[MAXBC];
 };
 
-// the zonal means of v, w at i = im-3, im-2 before the top sponge, k = 0..km-2 in order
+// the zonal means of v, w at i = im-3, im-2 before the top sponge, k = 0..km-2 in order: one
+// wavefront per (layer, field, j) row stages the row in LDS with coalesced loads, then one lane
+// adds it up in the reference's serial order
+constexpr int ZM_MAXK = 1024;
 template<typename T>
-__global__ void k_bc_zonal_means(const T* __restrict__ v, const T* __restrict__ w, acc_t<T>* __restrict__ zm,
-        int im, int jm, int km, int pitch){
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_bc_zonal_means(const T* __restrict__ v, const T* __restrict__ w,
+        acc_t<T>* __restrict__ zm, int im, int jm, int km, int pitch){
+    __shared__ acc_t<T> row[ZM_MAXK];
+    const int j = blockIdx.x;
     const int sel = blockIdx.y;                    // 0..3 = (layer 0 v, layer 0 w, layer 1 v, layer 1 w)
-    if(j >= jm) return;
     const int l = sel >> 1;
     const T* X = (sel & 1) ? w : v;
     const size_t r0 = ((size_t)(im - 3 + l) * jm + j) * pitch;
+    for(int k = threadIdx.x; k < km - 1; k += 64) row[k] = (acc_t<T>)X[r0 + k];
+    __syncthreads();
+    if(threadIdx.x != 0) return;
     acc_t<T> s = 0.0;
-    for(int k = 0; k < km - 1; k++) s += (acc_t<T>)X[r0 + k];
+    for(int k = 0; k < km - 1; k++) s += row[k];
     zm[(size_t)sel * jm + j] = s / (double)(km - 1);
 }
 
@@ -785,36 +794,31 @@ template<typename T>
 __global__ __launch_bounds__(256) void k_bc_radius(const BCLists<T> L, const acc_t<T>* __restrict__ zm,
         int im, int jm, int km, int pitch, size_t plane, double c43, double c13,
         float* __restrict__ lo_v, float* __restrict__ lo_w){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, f = blockIdx.z;
     if(k >= km) return;
     const size_t q0 = (size_t)j * pitch + k;
     const size_t qt = q0 + (size_t)(im - 1) * plane;
-    for(int f = 0; f < L.n_radius; f++){
-        T* X = L.radius[f];
-        X[q0] = (T)c43 * X[q0 + plane] - (T)c13 * X[q0 + 2 * plane];
-        X[qt] = (T)c43 * X[qt - plane] - (T)c13 * X[qt - 2 * plane];
-    }
-    L.u[q0] = T(0);
-    L.u[qt] = T(0);
-    const acc_t<T> v3 = zm[j], w3 = zm[(size_t)jm + j], v2 = zm[(size_t)2 * jm + j], w2 = zm[(size_t)3 * jm + j];
-    T* V = L.v; T* W = L.w;
-    V[qt] = T(0); W[qt] = T(0);
-    V[qt - plane] *= (T)(1.0 / 3.0);
-    W[qt - plane] *= (T)(1.0 / 3.0);
-    V[qt - 2 * plane] *= (T)(2.0 / 3.0);
-    W[qt - 2 * plane] *= (T)(2.0 / 3.0);
-    V[qt - 2 * plane] += (T)((1.0 / 3.0) * v3);
-    W[qt - 2 * plane] += (T)((1.0 / 3.0) * w3);
-    V[qt - plane] += (T)((2.0 / 3.0) * v2);
-    W[qt - plane] += (T)((2.0 / 3.0) * w2);
-    V[qt] = (T)v2;
-    W[qt] = (T)w2;
-    if(lo_v){                                      // the sponge's levels carry no low words
-        lo_v[qt - plane] = 0.0f; lo_v[qt - 2 * plane] = 0.0f;
-        lo_w[qt - plane] = 0.0f; lo_w[qt - 2 * plane] = 0.0f;
-    }
-    for(int f = 0; f < L.n_species; f++){
-        T* X = L.species[f];
+    T* X = L.radius[f];
+    X[q0] = (T)c43 * X[q0 + plane] - (T)c13 * X[q0 + 2 * plane];
+    X[qt] = (T)c43 * X[qt - plane] - (T)c13 * X[qt - 2 * plane];
+    const int role = L.radius_role[f];
+    if(role == 1){                                 // rigid lid
+        X[q0] = T(0);
+        X[qt] = T(0);
+    } else if(role == 2 || role == 3){             // top sponge, zonal-mean preserving
+        const acc_t<T> m3 = zm[(size_t)(role - 2) * jm + j], m2 = zm[(size_t)(role) * jm + j];
+        X[qt] = T(0);
+        X[qt - plane] *= (T)(1.0 / 3.0);
+        X[qt - 2 * plane] *= (T)(2.0 / 3.0);
+        X[qt - 2 * plane] += (T)((1.0 / 3.0) * m3);
+        X[qt - plane] += (T)((2.0 / 3.0) * m2);
+        X[qt] = (T)m2;
+        float* lo = (role == 2) ? lo_v : lo_w;
+        if(lo){                                    // the sponge's levels carry no low words
+            lo[qt - plane] = 0.0f;
+            lo[qt - 2 * plane] = 0.0f;
+        }
+    } else if(role == 4){                          // species: wall copy, positivity
         X[q0] = X[q0 + plane];
         if(X[q0] < T(0)) X[q0] = T(0);
         if(X[qt] < T(0)) X[qt] = T(0);
@@ -824,19 +828,18 @@ __global__ __launch_bounds__(256) void k_bc_radius(const BCLists<T> L, const acc
 // bcTheta with POLE_FILL 1 (two open rows) and POLE_COPY 1 (BoundaryConditions.h:292-391)
 template<typename T>
 __global__ __launch_bounds__(256) void k_bc_theta(const BCLists<T> L, int im, int jm, int km, int pitch){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y, f = blockIdx.z;
     if(k >= km) return;
     const size_t R = (size_t)pitch;
     const size_t qs = (size_t)i * jm * pitch + k;                 // j = 0
     const size_t qn = qs + (size_t)(jm - 1) * R;                  // j = jm-1
-    for(int f = 0; f < L.n_theta_ex; f++){
+    if(f < L.n_theta_ex){
         T* X = L.theta_ex[f];
         const T a = X[qs + 3 * R], b = X[qn - 3 * R];
         X[qs + R] = a; X[qs + 2 * R] = a; X[qs] = a;
         X[qn - R] = b; X[qn - 2 * R] = b; X[qn] = b;
-    }
-    for(int f = 0; f < L.n_theta_zero; f++){
-        T* X = L.theta_zero[f];
+    } else {
+        T* X = L.theta_zero[f - L.n_theta_ex];
         const T a = X[qs + 3 * R], b = X[qn - 3 * R];
         X[qs + R] = (T)(1.0 / 3.0) * a;
         X[qs + 2 * R] = (T)(2.0 / 3.0) * a;
@@ -850,13 +853,11 @@ __global__ __launch_bounds__(256) void k_bc_theta(const BCLists<T> L, int im, in
 // bcPhi with the periodic seam: k = km-1 is the copy of k = 0 (BoundaryConditions.h:422-438)
 template<typename T>
 __global__ void k_bc_phi(const BCLists<T> L, int im, int jm, int km, int pitch){
-    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y, f = blockIdx.z;
     if(j >= jm) return;
     const size_t r0 = ((size_t)i * jm + j) * pitch;
-    for(int f = 0; f < L.n_phi; f++){
-        T* X = L.phi[f];
-        X[r0 + km - 1] = X[r0];
-    }
+    T* X = L.phi[f];
+    X[r0 + km - 1] = X[r0];
 }
 
 // bcSolidGround over the precomputed list of solid cells (BC_Jup.h:460-589). The extrapolation
@@ -922,11 +923,31 @@ __global__ __launch_bounds__(64) void k_bc_t_ambient(T* __restrict__ t, const ui
 
 // clampNegativeSpecies (FileIO_Jup.cpp:494-548)
 template<typename T>
-__global__ __launch_bounds__(256) void k_clamp_species(T* __restrict__ x, size_t n){
+struct SpeciesPtrs { T* x[ATJUP_F_NH4SH - ATJUP_F_H2O + 1]; };
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_clamp_species(const SpeciesPtrs<T> S, size_t n){
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(q >= n) return;
+    T* __restrict__ x = S.x[blockIdx.y];
     const T v = x[q];
     if(!(v >= T(0))) x[q] = T(0);
+}
+
+// the fluid run of every column from the bottom (the levels computeHydrostaticPressure integrates,
+// RungeKutta_Jup_Turb.cpp:119-259): i0 the first fluid level (im when the column is solid), i1
+// the last level of the run that starts there
+__global__ void k_col_range(const uint8_t* __restrict__ flags, int2* __restrict__ colr, int im, int jm, int km,
+        int pitch, size_t plane){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const size_t q0 = (size_t)j * pitch + k;
+    int i0 = 0;
+    while(i0 < im && (flags[q0 + i0 * plane] & FL_SELF)) i0++;
+    int i1 = i0;
+    if(i0 < im)
+        while(i1 + 1 < im && !(flags[q0 + (i1 + 1) * plane] & FL_SELF)) i1++;
+    colr[(size_t)j * km + k] = make_int2(i0, i1);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1044,7 +1065,7 @@ int do_brl(atjup_ctx* c){
 template<typename T>
 int do_hydro(atjup_ctx* c){
     hipLaunchKernelGGL(k_hydrostatic<T>, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream,
-        F<T>(c, ATJUP_F_V), F<T>(c, ATJUP_F_W), (const uint8_t*)c->flags.p,
+        F<T>(c, ATJUP_F_V), F<T>(c, ATJUP_F_W), (const int2*)c->colr.p,
         (const double*)c->tabs.p, (const double*)c->tabs.p + c->im + c->jm,
         F<T>(c, ATJUP_F_P_HYDRO), c->im, c->jm, c->km, c->pitch, c->plane, c->g.dr);
     return check_launch();
@@ -1263,7 +1284,12 @@ template<typename T>
 BCLists<T> bc_lists(atjup_ctx* c){
     BCLists<T> L{};
     const bool dry = c->kn.dry != 0;
-    auto add = [&](T** arr, int& n, int f){ arr[n++] = (T*)live_ptr(c, f); };
+    auto add = [&](T** arr, int& n, int f){
+        if(arr == L.radius)
+            L.radius_role[n] = f == ATJUP_F_U ? 1 : f == ATJUP_F_V ? 2 : f == ATJUP_F_W ? 3
+                             : (f >= ATJUP_F_H2O && f <= ATJUP_F_NH4SH) ? 4 : 0;
+        arr[n++] = (T*)live_ptr(c, f);
+    };
     const int species[] = { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE, ATJUP_F_H2S, ATJUP_F_NH3,
         ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, ATJUP_F_NH4SH };
     const int turbf[] = { ATJUP_F_TKE, ATJUP_F_DIS, ATJUP_F_NUE, ATJUP_F_PROD, ATJUP_F_TKE_SOURCE, ATJUP_F_DIS_SOURCE };
@@ -1291,14 +1317,14 @@ template<typename T>
 int do_bc(atjup_ctx* c){
     const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
     const BCLists<T> L = bc_lists<T>(c);
-    hipLaunchKernelGGL(k_bc_zonal_means<T>, dim3(nblk(c->jm, 64), 4), dim3(64), 0, c->stream,
+    hipLaunchKernelGGL(k_bc_zonal_means<T>, dim3(c->jm, 4), dim3(64), 0, c->stream,
         L.v, L.w, (acc_t<T>*)c->zm.p, c->im, c->jm, c->km, c->pitch);
-    hipLaunchKernelGGL(k_bc_radius<T>, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream,
+    hipLaunchKernelGGL(k_bc_radius<T>, dim3(nblk(c->km, 128), c->jm, L.n_radius), dim3(128), 0, c->stream,
         L, (const acc_t<T>*)c->zm.p, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13,
         (float*)c->lo[ATJUP_F_V].p, (float*)c->lo[ATJUP_F_W].p);
-    hipLaunchKernelGGL(k_bc_theta<T>, dim3(nblk(c->km, 128), c->im), dim3(128), 0, c->stream,
-        L, c->im, c->jm, c->km, c->pitch);
-    hipLaunchKernelGGL(k_bc_phi<T>, dim3(nblk(c->jm, 64), c->im), dim3(64), 0, c->stream,
+    hipLaunchKernelGGL(k_bc_theta<T>, dim3(nblk(c->km, 128), c->im, L.n_theta_ex + L.n_theta_zero), dim3(128), 0,
+        c->stream, L, c->im, c->jm, c->km, c->pitch);
+    hipLaunchKernelGGL(k_bc_phi<T>, dim3(nblk(c->jm, 64), c->im, L.n_phi), dim3(64), 0, c->stream,
         L, c->im, c->jm, c->km, c->pitch);
     if(c->n_solids > 0)
         hipLaunchKernelGGL(k_bc_solid<T>, dim3(nblk(c->n_solids, 64)), dim3(64), 0, c->stream,
@@ -1314,8 +1340,10 @@ int do_bc(atjup_ctx* c){
 template<typename T>
 int do_clamp(atjup_ctx* c){
     if(c->kn.dry) return 0;
-    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++)
-        hipLaunchKernelGGL(k_clamp_species<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream, F<T>(c, f), c->n3);
+    SpeciesPtrs<T> S{};
+    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++) S.x[f - ATJUP_F_H2O] = F<T>(c, f);
+    hipLaunchKernelGGL(k_clamp_species<T>, dim3(nblk(c->n3, 256), ATJUP_F_NH4SH - ATJUP_F_H2O + 1), dim3(256), 0,
+        c->stream, S, c->n3);
     return check_launch();
 }
 
@@ -1330,6 +1358,8 @@ int do_mask_setup(atjup_ctx* c){
         hipLaunchKernelGGL(k_rb_flags, grid, blk, 0, c->stream, (const uint8_t*)c->flags.p, f0, f0 + hn,
             c->jm, c->km, c->pitch, c->rb_hp);
     }
+    hipLaunchKernelGGL(k_col_range, dim3(nblk(c->km, 128), c->jm), blk, 0, c->stream, (const uint8_t*)c->flags.p,
+        (int2*)c->colr.p, c->im, c->jm, c->km, c->pitch, c->plane);
     hipLaunchKernelGGL(k_itopo, dim3(nblk(c->km, 128), c->jm), blk, 0, c->stream, (const uint8_t*)c->mask.p,
         (int*)c->itopo.p, c->im, c->jm, c->km, c->pitch, c->plane);
     const int layers = c->kn.wall_nue_layers;
