@@ -18,6 +18,11 @@ __device__ __forceinline__ int xcd_remap(int b, int nb){
     return start + n;
 }
 
+// the two-element vector of a value type, for loads of two neighbouring cells at once
+template<typename T> struct Pair;
+template<> struct Pair<double> { using V = double2; };
+template<> struct Pair<float>  { using V = float2; };
+
 // An index every active lane of a wavefront holds (a per-(i, j) table entry where a wavefront
 // spans one k row), made explicit so that the entry is read once per wavefront through scalar
 // loads instead of once per lane through vector loads.

@@ -516,6 +516,85 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep(T* __restrict__ x0, 
     }
 }
 
+// The colour pass with each thread owning two consecutive compact cells m, m+1 (m even) of one
+// row: every neighbour row is read as one pair load, so a wavefront covers 128 cells with the
+// vector-memory instructions the one-cell pass spends on 64. The arithmetic of a cell is that
+// pass's. Needs an even ring (nm).
+template<typename T, typename S>
+__device__ __forceinline__ void rb_cell(T* __restrict__ own, const T* __restrict__ oth, const S* __restrict__ src,
+        const uint8_t fl, const PoisIJ<double>& W, size_t q, size_t P, size_t R, int i, int j, int im, int jm,
+        bool gw, double omega, double c43, double c13, T oim, T oip, T ojm, T ojp, T pkp, T pkm){
+    const bool edge = i == 1 || i == 2 || i == im - 3 || i == im - 2 || j == 1 || j == 2 || j == jm - 3 || j == jm - 2;
+    const bool need_pc = (omega != 1.0) || (gw && edge);
+    const T pc = need_pc ? own[q] : T(0);
+    if(gw){
+        if(i == 2)      own[q - 2 * P] = (T)((T)c43 * oim - (T)c13 * pc);
+        if(i == im - 3) own[q + 2 * P] = (T)((T)c43 * oip - (T)c13 * pc);
+        if(j == 2)      own[q - 2 * R] = (T)((T)c43 * ojm - (T)c13 * pc);
+        if(j == jm - 3) own[q + 2 * R] = (T)((T)c43 * ojp - (T)c13 * pc);
+    }
+    const T pip = (i == im - 2 && gw) ? (T)((T)c43 * pc - (T)c13 * oim) : oip;
+    const T pim = (i == 1 && gw)      ? (T)((T)c43 * pc - (T)c13 * oip) : oim;
+    const T pjp = (j == jm - 2 && gw) ? (T)((T)c43 * pc - (T)c13 * ojm) : ojp;
+    const T pjm = (j == 1 && gw)      ? (T)((T)c43 * pc - (T)c13 * ojp) : ojm;
+    const T num1 = (T)W.num1, num2 = (T)W.num2, num3 = (T)W.num3;
+    T acc, den;
+    if(!(fl & FL_SELF)){
+        acc = den = T(0);
+        if(!(fl & FL_IP)){ acc += num1 * pip; den += num1; }
+        if(!(fl & FL_IM)){ acc += num1 * pim; den += num1; }
+        if(!(fl & FL_JP)){ acc += num2 * pjp; den += num2; }
+        if(!(fl & FL_JM)){ acc += num2 * pjm; den += num2; }
+        if(!(fl & FL_KP)){ acc += num3 * pkp; den += num3; }
+        if(!(fl & FL_KM)){ acc += num3 * pkm; den += num3; }
+        if(!(den > T(0))) return;
+    } else {
+        acc = (pip + pim) * num1 + (pjp + pjm) * num2 + (pkp + pkm) * num3;
+        den = (T)W.denom;
+    }
+    const T p_gs = (acc - (T)src[q]) / den;
+    own[q] = (omega == 1.0) ? p_gs : (T)((T(1) - (T)omega) * pc + (T)omega * p_gs);
+}
+
+template<typename T, typename S>
+__global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep2(T* __restrict__ x0, T* __restrict__ x1,
+        const S* __restrict__ s0, const S* __restrict__ s1,
+        const uint8_t* __restrict__ f0, const uint8_t* __restrict__ f1, const PoisIJ<double>* __restrict__ pw,
+        int im, int jm, int hp, int nm, int nmb, int njb, int nb, int colour, int ghost, double omega,
+        double c43, double c13){
+    using V = typename Pair<T>::V;
+    const int b = xcd_remap(blockIdx.x, nb);
+    const int mb = b % nmb, rest = b / nmb;
+    const int jb = rest % njb, ib = rest / njb;
+    const int m = 2 * (mb * RB_BM + threadIdx.x);
+    const int j = jb * RB_BJ + threadIdx.y + 1;
+    const int i = ib + 1;
+    if(m >= nm || j > jm - 2) return;
+    T* __restrict__ own = colour ? x1 : x0;
+    const T* __restrict__ oth = colour ? x0 : x1;
+    const S* __restrict__ src = colour ? s1 : s0;
+    const uint8_t* __restrict__ flg = colour ? f1 : f0;
+    const size_t P = (size_t)jm * hp, R = (size_t)hp;
+    const size_t row = ((size_t)i * jm + j) * hp;
+    const size_t q = row + m;
+    const bool gw = ghost && colour == 0;
+    const V vim = *reinterpret_cast<const V*>(oth + q - P), vip = *reinterpret_cast<const V*>(oth + q + P);
+    const V vjm = *reinterpret_cast<const V*>(oth + q - R), vjp = *reinterpret_cast<const V*>(oth + q + R);
+    const V vc = *reinterpret_cast<const V*>(oth + q);               // the other colour at m, m+1
+    const int off = (colour + i + j) & 1;
+    // the phi neighbours: off = 1: m | m+1 for cell m, m+1 | m+2 for cell m+1; off = 0: m-1 | m, m | m+1
+    const T edge = off ? oth[row + (m + 2 == nm ? 0 : m + 2)] : oth[row + (m == 0 ? nm - 1 : m - 1)];
+    const uint16_t fl2 = *reinterpret_cast<const uint16_t*>(flg + q);
+    const PoisIJ<double> W = pw[wave_uniform(i * jm + j)];
+    T pkm0, pkp0, pkm1, pkp1;
+    if(off){ pkm0 = vc.x; pkp0 = vc.y; pkm1 = vc.y; pkp1 = edge; }
+    else   { pkm0 = edge; pkp0 = vc.x; pkm1 = vc.x; pkp1 = vc.y; }
+    rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 & 0xff), W, q, P, R, i, j, im, jm, gw, omega, c43, c13,
+                  vim.x, vip.x, vjm.x, vjp.x, pkp0, pkm0);
+    rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 >> 8), W, q + 1, P, R, i, j, im, jm, gw, omega, c43, c13,
+                  vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
+}
+
 // ---- the FP32-refined solve: FP32 sweeps on the correction inside FP64 refinement ----
 // A red-black sweep is an affine map, so from a start p0 the iterates are p0 + d with d relaxed by
 // the same sweep on zero start and source r0 = src + den p0 - sum(w p0_nb), the cell's FP64
@@ -1144,15 +1223,34 @@ int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* 
     const uint8_t* f1 = f0 + hn;
     const PoisIJ<double>* pw = (const PoisIJ<double>*)c->pois.p;
     const int nm = (c->km - 1) / 2;
-    const int nmb = (nm + RB_BM - 1) / RB_BM, njb = (c->jm - 2 + RB_BJ - 1) / RB_BJ;
-    const int nib = (c->im - 2 + RB_NI - 1) / RB_NI;
+    const int njb = (c->jm - 2 + RB_BJ - 1) / RB_BJ;
+    // Pairs of cells per thread where the ring is even. They halve the memory instructions per
+    // cell; FP64 takes them only where they keep the lanes as busy as single cells do (measured on
+    // MI355X: 75.7 against 87.5 us per FP64 pass on 81 x 361 x 721, where 180 pairs fill three
+    // wavefronts; 11.5 against 11.2 us on 41 x 181 x 361, where 90 pairs leave 38 of 128 lanes
+    // idle), FP32 takes them always (8.7 against 9.2 us there).
+    const int wf_single = (nm + RB_BM - 1) / RB_BM, wf_pair = (nm / 2 + RB_BM - 1) / RB_BM;
+    const bool pair_type = std::is_same<T, double>::value || std::is_same<T, float>::value;
+    const bool pairs = (nm % 2 == 0) && pair_type &&
+                       (std::is_same<T, float>::value || (double)(nm / 2) / wf_pair >= (double)nm / wf_single);
+    const int nmb = pairs ? (nm / 2 + RB_BM - 1) / RB_BM : (nm + RB_BM - 1) / RB_BM;
+    const int nib = pairs ? c->im - 2 : (c->im - 2 + RB_NI - 1) / RB_NI;
     const int nb_sw = nmb * njb * nib;
     auto launch_all = [&](){
         for(int s = 0; s < n; s++)
-            for(int colour = 0; colour < 2; colour++)
+            for(int colour = 0; colour < 2; colour++){
+                const int gh = (s > 0 || ghost_first) ? 1 : 0;
+                if constexpr(std::is_same<T, double>::value || std::is_same<T, float>::value)
+                    if(pairs){
+                        hipLaunchKernelGGL((k_rb_sweep2<T, S>), dim3(nb_sw), dim3(RB_BM, RB_BJ), 0, c->stream,
+                            x0, x1, s0, s1, f0, f1, pw, c->im, c->jm, hp, nm, nmb, njb, nb_sw, colour,
+                            gh, omega, c43, c13);
+                        continue;
+                    }
                 hipLaunchKernelGGL((k_rb_sweep<T, S>), dim3(nb_sw), dim3(RB_BM, RB_BJ), 0, c->stream,
                     x0, x1, s0, s1, f0, f1, pw, c->im, c->jm, hp, nm, nmb, njb, nb_sw, colour,
-                    (s > 0 || ghost_first) ? 1 : 0, omega, c43, c13);
+                    gh, omega, c43, c13);
+            }
     };
 #if ATJUP_SWEEP_GRAPH
     hipGraphExec_t exec = nullptr;
