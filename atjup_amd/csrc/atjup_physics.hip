@@ -55,7 +55,7 @@ struct StageFullArgs {
     double dt_d;
 };
 
-constexpr int SF_BK = 64, SF_BJ = 4;
+constexpr int SF_BK = 64, SF_BJ = 2;            // 2 rows: 1.7% less stage time than 4 or 8 on MI355X
 #ifndef SF_MIN_WAVES
 #define SF_MIN_WAVES 1
 #endif
