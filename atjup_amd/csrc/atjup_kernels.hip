@@ -595,6 +595,46 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep2(T* __restrict__ x0,
                   vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
 }
 
+// The paired pass with the (row, pair) index flattened over the lanes, so that no lane idles at
+// the end of a row; a wavefront may then span two rows and reads the row's weights per lane.
+template<typename T, typename S>
+__global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __restrict__ x1,
+        const S* __restrict__ s0, const S* __restrict__ s1,
+        const uint8_t* __restrict__ f0, const uint8_t* __restrict__ f1, const PoisIJ<double>* __restrict__ pw,
+        int im, int jm, int hp, int nm, int ntb, int nb, int colour, int ghost, double omega,
+        double c43, double c13){
+    using V = typename Pair<T>::V;
+    const int b = xcd_remap(blockIdx.x, nb);
+    const int tb = b % ntb, i = b / ntb + 1;
+    const int npair = nm / 2;
+    const int t = tb * 256 + threadIdx.x;
+    const int j = t / npair + 1;
+    const int m = 2 * (t - (j - 1) * npair);
+    if(j > jm - 2) return;
+    T* __restrict__ own = colour ? x1 : x0;
+    const T* __restrict__ oth = colour ? x0 : x1;
+    const S* __restrict__ src = colour ? s1 : s0;
+    const uint8_t* __restrict__ flg = colour ? f1 : f0;
+    const size_t P = (size_t)jm * hp, R = (size_t)hp;
+    const size_t row = ((size_t)i * jm + j) * hp;
+    const size_t q = row + m;
+    const bool gw = ghost && colour == 0;
+    const V vim = *reinterpret_cast<const V*>(oth + q - P), vip = *reinterpret_cast<const V*>(oth + q + P);
+    const V vjm = *reinterpret_cast<const V*>(oth + q - R), vjp = *reinterpret_cast<const V*>(oth + q + R);
+    const V vc = *reinterpret_cast<const V*>(oth + q);
+    const int off = (colour + i + j) & 1;
+    const T edge = off ? oth[row + (m + 2 == nm ? 0 : m + 2)] : oth[row + (m == 0 ? nm - 1 : m - 1)];
+    const uint16_t fl2 = *reinterpret_cast<const uint16_t*>(flg + q);
+    const PoisIJ<double> W = pw[(size_t)i * jm + j];
+    T pkm0, pkp0, pkm1, pkp1;
+    if(off){ pkm0 = vc.x; pkp0 = vc.y; pkm1 = vc.y; pkp1 = edge; }
+    else   { pkm0 = edge; pkp0 = vc.x; pkm1 = vc.x; pkp1 = vc.y; }
+    rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 & 0xff), W, q, P, R, i, j, im, jm, gw, omega, c43, c13,
+                  vim.x, vip.x, vjm.x, vjp.x, pkp0, pkm0);
+    rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 >> 8), W, q + 1, P, R, i, j, im, jm, gw, omega, c43, c13,
+                  vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
+}
+
 // ---- the FP32-refined solve: FP32 sweeps on the correction inside FP64 refinement ----
 // A red-black sweep is an affine map, so from a start p0 the iterates are p0 + d with d relaxed by
 // the same sweep on zero start and source r0 = src + den p0 - sum(w p0_nb), the cell's FP64
@@ -1224,15 +1264,19 @@ int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* 
     const PoisIJ<double>* pw = (const PoisIJ<double>*)c->pois.p;
     const int nm = (c->km - 1) / 2;
     const int njb = (c->jm - 2 + RB_BJ - 1) / RB_BJ;
-    // Pairs of cells per thread where the ring is even. They halve the memory instructions per
-    // cell; FP64 takes them only where they keep the lanes as busy as single cells do (measured on
-    // MI355X: 75.7 against 87.5 us per FP64 pass on 81 x 361 x 721, where 180 pairs fill three
-    // wavefronts; 11.5 against 11.2 us on 41 x 181 x 361, where 90 pairs leave 38 of 128 lanes
-    // idle), FP32 takes them always (8.7 against 9.2 us there).
-    const int wf_single = (nm + RB_BM - 1) / RB_BM, wf_pair = (nm / 2 + RB_BM - 1) / RB_BM;
+    // Pairs of cells per thread where the ring is even: they halve the memory instructions per
+    // cell. Row-aligned pairs (one row per wavefront, the row's weights through scalar loads) where
+    // they fill at least 90% of the lanes, else pairs flattened over (row, pair). Measured on
+    // MI355X, one colour pass: 41 x 181 x 361 (90 pairs per row) FP64 10.2 us flattened, 10.9
+    // row-aligned, 11.2 one cell per thread; FP32 7.9 / 8.7 / 9.2; 81 x 361 x 721 (180 pairs)
+    // FP64 75.7 row-aligned, 79.4 flattened, 87.5 one cell.
+    const int wf_pair = (nm / 2 + RB_BM - 1) / RB_BM;
     const bool pair_type = std::is_same<T, double>::value || std::is_same<T, float>::value;
-    const bool pairs = (nm % 2 == 0) && pair_type &&
-                       (std::is_same<T, float>::value || (double)(nm / 2) / wf_pair >= (double)nm / wf_single);
+    const bool pairs_ok = (nm % 2 == 0) && pair_type;
+    const bool pairs_flat = pairs_ok && (double)(nm / 2) / (RB_BM * wf_pair) < 0.9;
+    const bool pairs = pairs_ok && !pairs_flat;
+    const int ntb = ((c->jm - 2) * (nm / 2) + 255) / 256;          // flattened pairs per level
+    const int nb_flat = ntb * (c->im - 2);
     const int nmb = pairs ? (nm / 2 + RB_BM - 1) / RB_BM : (nm + RB_BM - 1) / RB_BM;
     const int nib = pairs ? c->im - 2 : (c->im - 2 + RB_NI - 1) / RB_NI;
     const int nb_sw = nmb * njb * nib;
@@ -1241,7 +1285,12 @@ int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* 
             for(int colour = 0; colour < 2; colour++){
                 const int gh = (s > 0 || ghost_first) ? 1 : 0;
                 if constexpr(std::is_same<T, double>::value || std::is_same<T, float>::value)
-                    if(pairs){
+                    if(pairs_flat){
+                        hipLaunchKernelGGL((k_rb_sweep2f<T, S>), dim3(nb_flat), dim3(256), 0, c->stream,
+                            x0, x1, s0, s1, f0, f1, pw, c->im, c->jm, hp, nm, ntb, nb_flat, colour,
+                            gh, omega, c43, c13);
+                        continue;
+                    } else if(pairs){
                         hipLaunchKernelGGL((k_rb_sweep2<T, S>), dim3(nb_sw), dim3(RB_BM, RB_BJ), 0, c->stream,
                             x0, x1, s0, s1, f0, f1, pw, c->im, c->jm, hp, nm, nmb, njb, nb_sw, colour,
                             gh, omega, c43, c13);

@@ -333,7 +333,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": SWEEP_TRAFFIC_BYTES.get((g.im, g.jm, g.km)),
-            "kernel": "k_rb_sweep<double, double> (one red-black colour pass)",
+            "kernel": "one red-black colour pass of the FP64 solve (k_rb_sweep2f<double, double>, two cells "
+                      "per thread, on the 41x181x361 shell; k_rb_sweep2 on 81x361x721)",
             "algorithmic_bytes_per_launch": lb,
             "avg_launch_ms": launch_ms,
             "launches": int(sweep_launches),

@@ -106,11 +106,12 @@ def test_small_grid_steps_with_sor():
     orc.close()
 
 
-@pytest.mark.parametrize("km,omega", [(257, 1.0), (257, 1.5), (49, 1.0)])
-def test_pressure_solve_both_colour_pass_kernels(km, omega):
-    """The colour pass runs as one cell per thread or, where the ring of compact cells fills whole
-    wavefronts with pairs (256 meridians: 64 pairs), two cells per thread; both agree with the
-    oracle bit for bit, on a shell with an obstacle and an odd sweep count."""
+@pytest.mark.parametrize("km,omega", [(257, 1.0), (257, 1.5), (49, 1.0), (51, 1.0)])
+def test_pressure_solve_all_colour_pass_kernels(km, omega):
+    """The colour pass runs with two cells per thread, one row per wavefront where the pairs fill
+    the wavefronts (256 meridians: 64 pairs), flattened over (row, pair) where they would not (48
+    meridians), and one cell per thread on an odd ring of compact cells (50 meridians: 25); all
+    agree with the oracle bit for bit, on a shell with an obstacle and an odd sweep count."""
     g = inputs.Grid(im=13, jm=33, km=km, the_degree=180.0 / 32, phi_degree=360.0 / (km - 1))
     st = inputs.make_state("dyn", seed=4, g=g, with_obstacle=False)
     st.mask = inputs.ellipsoid_mask(g, j0=20, k0=(km - 1) // 2, a_phi=(km - 1) / 12.0, b_the=3.0, c_r_km=40.0)
