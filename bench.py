@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the colour pass from two rocprofv3 PMC runs (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0, KiB per dispatch, the factors from a device copy of known size), per grid;
 # profiles/step_full_41x181x361_hbm_traffic.md holds the summary and the command
-SWEEP_TRAFFIC_BYTES = {(41, 181, 361): (14089.4 * 2.0 + 10122.1) * 1024}
+SWEEP_TRAFFIC_BYTES = {(41, 181, 361): (13910.6 * 2.0 + 10419.2) * 1024}
 WORKLOAD = {
     "dyn": "dynamics-only FP64 time step",
     "full": "full-physics FP64 time step (k-omega closure, saturation adjustment of H2O/NH3/CH4, NH4SH "
