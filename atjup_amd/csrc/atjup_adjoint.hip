@@ -627,47 +627,42 @@ __global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_mu(const double* __res
     mu[q] = v;
 }
 
-// one colour pass of one sweep, transposed, over every cell of colour c (plane cells take the
-// reads of stored planes, which only the first sweep of a solve makes: ghost = 0)
-__global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restrict__ lam, double* __restrict__ mu,
-        const double* __restrict__ mu_o, double* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
-        const uint8_t* __restrict__ fo_arr, const uint8_t* __restrict__ inm, const PoisIJ<double>* __restrict__ pw,
-        int im, int jm, int hp, int nm, int colour, int ghost, double omega, double c43, double c13){
-    const int m = blockIdx.x * RBT_BM + threadIdx.x, j = blockIdx.y * RBT_BJ + threadIdx.y, i = blockIdx.z;
-    if(m >= nm || j >= jm) return;
+// The values one transposed cell update needs from outside its own arrays: the other colour's mu
+// at the six neighbour positions and its flags at the four radial and meridional ones, and the
+// coupling weights of the (i, j) row and of its four neighbour rows. A caller fills the members
+// the cell's incoming-read bits, position and pass use; the rest stay zero.
+struct RbtNb {
+    double mip, mim, mjp, mjm, mke, mkw;          // mu_o at i+1, i-1, j+1, j-1, k+1, k-1
+    double wip, wim, wjp, wjm;                    // num1 of rows i+1, i-1; num2 of rows j+1, j-1
+    uint8_t fim, fip, fjm, fjp;                   // the other colour's flags at i-1, i+1, j-1, j+1
+};
+
+// one cell of one transposed colour pass (W: the own row's weights, used on sweep cells only)
+__device__ __forceinline__ void rbT_cell(double* __restrict__ lam, double* __restrict__ mu,
+        double* __restrict__ lsrc, size_t q, uint8_t bits, uint8_t fl, const PoisIJ<double>& W, const RbtNb& n,
+        int i, int j, int im, int jm, int colour, int ghost, double omega, double c43, double c13){
     const bool interior = i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2;
-    if(!interior && ghost) return;
-    const size_t P = (size_t)jm * hp, R = (size_t)hp;
-    const size_t row = ((size_t)i * jm + j) * hp;
-    const size_t q = row + m;
     // stored-value reads of this cell by the other colour's sweep cells s: sum w mu_s
-    const uint8_t bits = inm[q];
     double dsum = 0.0;
-    const int ij = wave_uniform(i * jm + j);          // a wavefront is one (i, j) row
-    if(bits & 1) dsum += pw[ij + jm].num1 * mu_o[q + P];
-    if(bits & 2) dsum += pw[ij - jm].num1 * mu_o[q - P];
-    if(bits & 4) dsum += pw[ij + 1].num2 * mu_o[q + R];
-    if(bits & 8) dsum += pw[ij - 1].num2 * mu_o[q - R];
+    if(bits & 1) dsum += n.wip * n.mip;
+    if(bits & 2) dsum += n.wim * n.mim;
+    if(bits & 4) dsum += n.wjp * n.mjp;
+    if(bits & 8) dsum += n.wjm * n.mjm;
     if(!interior){
         lam[q] += dsum;                              // a plane cell, read as stored in the first sweep
         return;
     }
-    const PoisIJ<double> W = pw[ij];
-    if(bits & 48){
-        const int off = (colour + i + j) & 1, ring = 2 * nm, kk = 2 * m + off;
-        if(bits & 16) dsum += W.num3 * mu_o[row + (((kk + 1) % ring) >> 1)];
-        if(bits & 32) dsum += W.num3 * mu_o[row + (((kk - 1 + ring) % ring) >> 1)];
-    }
+    if(bits & 16) dsum += W.num3 * n.mke;
+    if(bits & 32) dsum += W.num3 * n.mkw;
     // formed-plane reads by the other colour, for which this cell is the one behind the plane
     double bsum = 0.0;
     if(ghost){
-        if(i == 2 && rb_reads(fo_arr[q - P], 1))      bsum += -c13 * pw[ij - jm].num1 * mu_o[q - P];
-        if(i == im - 3 && rb_reads(fo_arr[q + P], 0)) bsum += -c13 * pw[ij + jm].num1 * mu_o[q + P];
-        if(j == 2 && rb_reads(fo_arr[q - R], 3))      bsum += -c13 * pw[ij - 1].num2 * mu_o[q - R];
-        if(j == jm - 3 && rb_reads(fo_arr[q + R], 2)) bsum += -c13 * pw[ij + 1].num2 * mu_o[q + R];
+        if(i == 2 && rb_reads(n.fim, 1))      bsum += -c13 * n.wim * n.mim;
+        if(i == im - 3 && rb_reads(n.fip, 0)) bsum += -c13 * n.wip * n.mip;
+        if(j == 2 && rb_reads(n.fjm, 3))      bsum += -c13 * n.wjm * n.mjm;
+        if(j == jm - 3 && rb_reads(n.fjp, 2)) bsum += -c13 * n.wjp * n.mjp;
     }
     // this cell's own pass
-    const uint8_t fl = fl_arr[q];
     const double den = rb_den(fl, W);
     const bool skip = !(den > 0.0);
     double t = lam[q];
@@ -690,6 +685,118 @@ __global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restric
     lam[q] = out;
     // colour 0: its mu for the colour-1 kernel; colour 1: its mu for the previous sweep
     mu[q] = (colour == 0) ? mu_x : (skip ? 0.0 : omega * out / den);
+}
+
+// Which neighbour rows a row of cells needs: the stored-value reads (bits, the union over the
+// row's cells) and, in sweeps that form the outer planes, the cells behind a plane.
+struct RbtNeed {
+    bool ip, im_, jp, jm_;
+};
+__device__ __forceinline__ RbtNeed rbt_need(uint8_t bits, bool interior, int ghost, int i, int j, int im, int jm){
+    const bool g = ghost && interior;
+    return RbtNeed{(bits & 1) || (g && i == im - 3), (bits & 2) || (g && i == 2),
+                   (bits & 4) || (g && j == jm - 3), (bits & 8) || (g && j == 2)};
+}
+
+// one colour pass of one sweep, transposed, over every cell of colour c (plane cells take the
+// reads of stored planes, which only the first sweep of a solve makes: ghost = 0)
+__global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restrict__ lam, double* __restrict__ mu,
+        const double* __restrict__ mu_o, double* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
+        const uint8_t* __restrict__ fo_arr, const uint8_t* __restrict__ inm, const PoisIJ<double>* __restrict__ pw,
+        int im, int jm, int hp, int nm, int colour, int ghost, double omega, double c43, double c13){
+    const int m = blockIdx.x * RBT_BM + threadIdx.x, j = blockIdx.y * RBT_BJ + threadIdx.y, i = blockIdx.z;
+    if(m >= nm || j >= jm) return;
+    const bool interior = i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2;
+    if(!interior && ghost) return;
+    const size_t P = (size_t)jm * hp, R = (size_t)hp;
+    const size_t row = ((size_t)i * jm + j) * hp;
+    const size_t q = row + m;
+    const uint8_t bits = inm[q];
+    const int ij = wave_uniform(i * jm + j);          // a wavefront is one (i, j) row
+    const RbtNeed nd = rbt_need(bits, interior, ghost, i, j, im, jm);
+    RbtNb n = {};
+    if(nd.ip){ n.mip = mu_o[q + P]; n.wip = pw[ij + jm].num1; }
+    if(nd.im_){ n.mim = mu_o[q - P]; n.wim = pw[ij - jm].num1; }
+    if(nd.jp){ n.mjp = mu_o[q + R]; n.wjp = pw[ij + 1].num2; }
+    if(nd.jm_){ n.mjm = mu_o[q - R]; n.wjm = pw[ij - 1].num2; }
+    if(bits & 48){
+        const int off = (colour + i + j) & 1, ring = 2 * nm, kk = 2 * m + off;
+        if(bits & 16) n.mke = mu_o[row + (((kk + 1) % ring) >> 1)];
+        if(bits & 32) n.mkw = mu_o[row + (((kk - 1 + ring) % ring) >> 1)];
+    }
+    if(ghost){
+        if(i == 2) n.fim = fo_arr[q - P];
+        if(i == im - 3) n.fip = fo_arr[q + P];
+        if(j == 2) n.fjm = fo_arr[q - R];
+        if(j == jm - 3) n.fjp = fo_arr[q + R];
+    }
+    PoisIJ<double> W = {};
+    uint8_t fl = 0;
+    if(interior){ W = pw[ij]; fl = fl_arr[q]; }
+    rbT_cell(lam, mu, lsrc, q, bits, fl, W, n, i, j, im, jm, colour, ghost, omega, c43, c13);
+}
+
+// the same pass with two cells m, m+1 (m even) per thread and pair loads, flattened over
+// (row, pair); needs an even ring of compact cells
+__global__ __launch_bounds__(256) void ka_rbT_pass2(double* __restrict__ lam, double* __restrict__ mu,
+        const double* __restrict__ mu_o, double* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
+        const uint8_t* __restrict__ fo_arr, const uint8_t* __restrict__ inm, const PoisIJ<double>* __restrict__ pw,
+        int im, int jm, int hp, int nm, int ntb, int colour, int ghost, double omega, double c43, double c13){
+    const int tb = blockIdx.x % ntb, i = blockIdx.x / ntb;
+    const int npair = nm / 2;
+    const int t = tb * 256 + threadIdx.x;
+    const int j = t / npair;
+    const int m = 2 * (t - j * npair);
+    if(j >= jm) return;
+    const bool interior = i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2;
+    if(!interior && ghost) return;
+    const size_t P = (size_t)jm * hp, R = (size_t)hp;
+    const size_t row = ((size_t)i * jm + j) * hp;
+    const size_t q = row + m;
+    const uint16_t bits2 = *reinterpret_cast<const uint16_t*>(inm + q);
+    const uint8_t b0 = (uint8_t)(bits2 & 0xff), b1 = (uint8_t)(bits2 >> 8);
+    const int ij = i * jm + j;
+    const RbtNeed nd = rbt_need(b0 | b1, interior, ghost, i, j, im, jm);
+    RbtNb n0 = {}, n1 = {};
+    if(nd.ip){
+        const double2 v = *reinterpret_cast<const double2*>(mu_o + q + P);
+        n0.mip = v.x; n1.mip = v.y; n0.wip = n1.wip = pw[ij + jm].num1;
+    }
+    if(nd.im_){
+        const double2 v = *reinterpret_cast<const double2*>(mu_o + q - P);
+        n0.mim = v.x; n1.mim = v.y; n0.wim = n1.wim = pw[ij - jm].num1;
+    }
+    if(nd.jp){
+        const double2 v = *reinterpret_cast<const double2*>(mu_o + q + R);
+        n0.mjp = v.x; n1.mjp = v.y; n0.wjp = n1.wjp = pw[ij + 1].num2;
+    }
+    if(nd.jm_){
+        const double2 v = *reinterpret_cast<const double2*>(mu_o + q - R);
+        n0.mjm = v.x; n1.mjm = v.y; n0.wjm = n1.wjm = pw[ij - 1].num2;
+    }
+    // phi neighbours: the reader at k+1 and at k-1 of each cell (other colour)
+    if((b0 | b1) & 48){
+        const int off = (colour + i + j) & 1;
+        const double2 vc = *reinterpret_cast<const double2*>(mu_o + q);
+        const double edge = off ? mu_o[row + (m + 2 == nm ? 0 : m + 2)] : mu_o[row + (m == 0 ? nm - 1 : m - 1)];
+        if(off){ n0.mke = vc.y; n0.mkw = vc.x; n1.mke = edge; n1.mkw = vc.y; }
+        else   { n0.mke = vc.x; n0.mkw = edge; n1.mke = vc.y; n1.mkw = vc.x; }
+    }
+    if(ghost){
+        auto split = [&](const uint8_t* a, uint8_t& x0, uint8_t& x1){
+            const uint16_t v = *reinterpret_cast<const uint16_t*>(a);
+            x0 = (uint8_t)(v & 0xff); x1 = (uint8_t)(v >> 8);
+        };
+        if(i == 2) split(fo_arr + q - P, n0.fim, n1.fim);
+        if(i == im - 3) split(fo_arr + q + P, n0.fip, n1.fip);
+        if(j == 2) split(fo_arr + q - R, n0.fjm, n1.fjm);
+        if(j == jm - 3) split(fo_arr + q + R, n0.fjp, n1.fjp);
+    }
+    PoisIJ<double> W = {};
+    uint16_t fl2 = 0;
+    if(interior){ W = pw[ij]; fl2 = *reinterpret_cast<const uint16_t*>(fl_arr + q); }
+    rbT_cell(lam, mu, lsrc, q, b0, (uint8_t)(fl2 & 0xff), W, n0, i, j, im, jm, colour, ghost, omega, c43, c13);
+    rbT_cell(lam, mu, lsrc, q + 1, b1, (uint8_t)(fl2 >> 8), W, n1, i, j, im, jm, colour, ghost, omega, c43, c13);
 }
 
 // prepareProjectionBoundaries + the clamped divergence source, transposed: the source adjoint
@@ -1097,12 +1204,25 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
         hipLaunchKernelGGL(ka_rb_planes_j, gpj, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
         hipLaunchKernelGGL(ka_rb_planes_i, gpi, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
         hipLaunchKernelGGL(ka_rbT_mu, gt, bt, 0, c->stream, (const double*)l1, mu1, f1, pw, c->im, c->jm, hp, nm, omega);
+        const bool pairs = nm % 2 == 0;
+        const int ntb = (c->jm * (nm / 2) + 255) / 256;
         for(int s = sweeps - 1; s >= 0; s--){
             const int ghost = s > 0 ? 1 : 0;         // the drop-in's first sweep reads the stored planes
-            hipLaunchKernelGGL(ka_rbT_pass, gt, bt, 0, c->stream, l0, mu0, (const double*)mu1, s0, f0, f1,
-                (const uint8_t*)in0, pw, c->im, c->jm, hp, nm, 0, ghost, omega, c43, c13);
-            hipLaunchKernelGGL(ka_rbT_pass, gt, bt, 0, c->stream, l1, mu1, (const double*)mu0, s1, f1, f0,
-                (const uint8_t*)in1, pw, c->im, c->jm, hp, nm, 1, ghost, omega, c43, c13);
+            for(int colour = 0; colour < 2; colour++){
+                double* lc = colour ? l1 : l0;
+                double* mc = colour ? mu1 : mu0;
+                const double* mo = colour ? mu0 : mu1;
+                double* sc = colour ? s1 : s0;
+                const uint8_t* fc = colour ? f1 : f0;
+                const uint8_t* fo = colour ? f0 : f1;
+                const uint8_t* ic = colour ? in1 : in0;
+                if(pairs)
+                    hipLaunchKernelGGL(ka_rbT_pass2, dim3(ntb * c->im), dim3(256), 0, c->stream, lc, mc, mo, sc, fc, fo,
+                        ic, pw, c->im, c->jm, hp, nm, ntb, colour, ghost, omega, c43, c13);
+                else
+                    hipLaunchKernelGGL(ka_rbT_pass, gt, bt, 0, c->stream, lc, mc, mo, sc, fc, fo,
+                        ic, pw, c->im, c->jm, hp, nm, colour, ghost, omega, c43, c13);
+            }
         }
     };
     // the 2 launches per sweep go to the device as one graph, captured on first use per sweep

@@ -13,8 +13,8 @@ def small_grid():
     return inputs.Grid(im=13, jm=33, km=49, the_degree=180.0 / 32, phi_degree=360.0 / 48)
 
 
-def small_state(config="dyn", seed=3):
-    g = small_grid()
+def small_state(config="dyn", seed=3, g=None):
+    g = g if g is not None else small_grid()
     st = inputs.make_state(config, seed=seed, g=g, with_obstacle=False)
     # a small ellipsoid obstacle that still has flanks in all three directions
     st.mask = inputs.ellipsoid_mask(g, j0=20, k0=24, a_phi=4.0, b_the=3.0, c_r_km=40.0)

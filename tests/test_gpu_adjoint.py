@@ -19,8 +19,8 @@ pytestmark = pytest.mark.gpu
 STATE = ("t", "u", "v", "w", "p_dyn", "aux_u", "aux_v", "aux_w", "rho_mix")
 
 
-def _state(seed=3):
-    st = small_state("dyn", seed=seed)
+def _state(seed=3, g=None):
+    st = small_state("dyn", seed=seed, g=g)
     rng = np.random.default_rng(seed)
     g = st.grid
     # a live pressure field and aux_* so that every term of the step has a derivative
@@ -126,9 +126,28 @@ def _forward(c, op, iarg, darg):
         c.step(iarg, bool(darg))
 
 
+# a ring of 50 meridians: 25 compact cells per colour and row, an odd count, which the transposed
+# colour pass covers one cell per thread (even counts go two cells per thread)
+ODD_RING = [("pressure", 6, 1.0, ("p_dyn", "aux_u", "aux_v", "aux_w"), ("p_dyn", "aux_u", "aux_v", "aux_w")),
+            ("step", 2, 0.0, STATE, STATE)]
+
+
+def _odd_ring_grid():
+    return inputs.Grid(im=13, jm=33, km=51, the_degree=180.0 / 32, phi_degree=360.0 / 50)
+
+
 @pytest.mark.parametrize("op,iarg,darg,ins,outs", CASES, ids=[f"{c[0]}-{c[1]}-{c[2]}" for c in CASES])
 def test_dot_product(op, iarg, darg, ins, outs):
-    st = _state(seed=5)
+    _dot_product(op, iarg, darg, ins, outs, None)
+
+
+@pytest.mark.parametrize("op,iarg,darg,ins,outs", ODD_RING, ids=[f"{c[0]}-{c[1]}-{c[2]}" for c in ODD_RING])
+def test_dot_product_odd_ring(op, iarg, darg, ins, outs):
+    _dot_product(op, iarg, darg, ins, outs, _odd_ring_grid())
+
+
+def _dot_product(op, iarg, darg, ins, outs, g):
+    st = _state(seed=5, g=g)
     rng = np.random.default_rng(7)
     shp = st.grid.shape
     c64 = _ctx(st, abi.PRECISION_FP64)
