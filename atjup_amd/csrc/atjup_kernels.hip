@@ -595,21 +595,24 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep2(T* __restrict__ x0,
                   vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
 }
 
-// The paired pass with the (row, pair) index flattened over the lanes, so that no lane idles at
-// the end of a row; a wavefront may then span two rows and reads the row's weights per lane.
-template<typename T, typename S>
+// The multi-cell pass with the (row, group) index flattened over the lanes, so that no lane idles
+// at the end of a row: each thread owns N consecutive compact cells m..m+N-1 (m a multiple of N),
+// read as N/2 pair loads per neighbour row; a wavefront may span two rows and reads the row's
+// weights per lane. Needs nm % N == 0.
+template<typename T, typename S, int N>
 __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __restrict__ x1,
         const S* __restrict__ s0, const S* __restrict__ s1,
         const uint8_t* __restrict__ f0, const uint8_t* __restrict__ f1, const PoisIJ<double>* __restrict__ pw,
         int im, int jm, int hp, int nm, int ntb, int nb, int colour, int ghost, double omega,
         double c43, double c13){
     using V = typename Pair<T>::V;
+    static_assert(N == 2 || N == 4, "two or four cells per thread");
     const int b = xcd_remap(blockIdx.x, nb);
     const int tb = b % ntb, i = b / ntb + 1;
-    const int npair = nm / 2;
+    const int ngrp = nm / N;
     const int t = tb * 256 + threadIdx.x;
-    const int j = t / npair + 1;
-    const int m = 2 * (t - (j - 1) * npair);
+    const int j = t / ngrp + 1;
+    const int m = N * (t - (j - 1) * ngrp);
     if(j > jm - 2) return;
     T* __restrict__ own = colour ? x1 : x0;
     const T* __restrict__ oth = colour ? x0 : x1;
@@ -619,20 +622,31 @@ __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __res
     const size_t row = ((size_t)i * jm + j) * hp;
     const size_t q = row + m;
     const bool gw = ghost && colour == 0;
-    const V vim = *reinterpret_cast<const V*>(oth + q - P), vip = *reinterpret_cast<const V*>(oth + q + P);
-    const V vjm = *reinterpret_cast<const V*>(oth + q - R), vjp = *reinterpret_cast<const V*>(oth + q + R);
-    const V vc = *reinterpret_cast<const V*>(oth + q);
+    T vim[N], vip[N], vjm[N], vjp[N], vc[N];
+    #pragma unroll
+    for(int h = 0; h < N; h += 2){
+        const V a = *reinterpret_cast<const V*>(oth + q + h - P), bq = *reinterpret_cast<const V*>(oth + q + h + P);
+        const V c = *reinterpret_cast<const V*>(oth + q + h - R), d = *reinterpret_cast<const V*>(oth + q + h + R);
+        const V e = *reinterpret_cast<const V*>(oth + q + h);
+        vim[h] = a.x; vim[h + 1] = a.y; vip[h] = bq.x; vip[h + 1] = bq.y;
+        vjm[h] = c.x; vjm[h + 1] = c.y; vjp[h] = d.x; vjp[h + 1] = d.y;
+        vc[h] = e.x; vc[h + 1] = e.y;
+    }
     const int off = (colour + i + j) & 1;
-    const T edge = off ? oth[row + (m + 2 == nm ? 0 : m + 2)] : oth[row + (m == 0 ? nm - 1 : m - 1)];
-    const uint16_t fl2 = *reinterpret_cast<const uint16_t*>(flg + q);
+    // the phi neighbours: off = 1: cell m+r reads m+r (k-1) and m+r+1 (k+1); off = 0: m+r-1, m+r
+    const T edge = off ? oth[row + (m + N == nm ? 0 : m + N)] : oth[row + (m == 0 ? nm - 1 : m - 1)];
+    uint32_t flw;
+    if constexpr(N == 2) flw = *reinterpret_cast<const uint16_t*>(flg + q);
+    else flw = *reinterpret_cast<const uint32_t*>(flg + q);
     const PoisIJ<double> W = pw[(size_t)i * jm + j];
-    T pkm0, pkp0, pkm1, pkp1;
-    if(off){ pkm0 = vc.x; pkp0 = vc.y; pkm1 = vc.y; pkp1 = edge; }
-    else   { pkm0 = edge; pkp0 = vc.x; pkm1 = vc.x; pkp1 = vc.y; }
-    rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 & 0xff), W, q, P, R, i, j, im, jm, gw, omega, c43, c13,
-                  vim.x, vip.x, vjm.x, vjp.x, pkp0, pkm0);
-    rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 >> 8), W, q + 1, P, R, i, j, im, jm, gw, omega, c43, c13,
-                  vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
+    #pragma unroll
+    for(int r = 0; r < N; r++){
+        T pkm, pkp;
+        if(off){ pkm = vc[r]; pkp = (r + 1 < N) ? vc[r + 1] : edge; }
+        else   { pkm = (r > 0) ? vc[r - 1] : edge; pkp = vc[r]; }
+        rb_cell<T, S>(own, oth, src, (uint8_t)((flw >> (8 * r)) & 0xff), W, q + r, P, R, i, j, im, jm, gw, omega,
+                      c43, c13, vim[r], vip[r], vjm[r], vjp[r], pkp, pkm);
+    }
 }
 
 // ---- the FP32-refined solve: FP32 sweeps on the correction inside FP64 refinement ----
@@ -1277,6 +1291,12 @@ int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* 
     const bool pairs = pairs_ok && !pairs_flat;
     const int ntb = ((c->jm - 2) * (nm / 2) + 255) / 256;          // flattened pairs per level
     const int nb_flat = ntb * (c->im - 2);
+    // FP32: four cells per thread (flattened) where the ring allows; one float4 per neighbour row.
+    // 41 x 181 x 361: 7.4 us against 7.9 us for flattened pairs; FP64 quads (two 16-byte loads per
+    // row) are slower than FP64 pairs (11.8 against 10.4 us).
+    const bool quads = pairs_ok && nm % 4 == 0 && std::is_same<T, float>::value;
+    const int ntb4 = ((c->jm - 2) * (nm / 4) + 255) / 256;
+    const int nb_quad = ntb4 * (c->im - 2);
     const int nmb = pairs ? (nm / 2 + RB_BM - 1) / RB_BM : (nm + RB_BM - 1) / RB_BM;
     const int nib = pairs ? c->im - 2 : (c->im - 2 + RB_NI - 1) / RB_NI;
     const int nb_sw = nmb * njb * nib;
@@ -1285,8 +1305,13 @@ int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* 
             for(int colour = 0; colour < 2; colour++){
                 const int gh = (s > 0 || ghost_first) ? 1 : 0;
                 if constexpr(std::is_same<T, double>::value || std::is_same<T, float>::value)
-                    if(pairs_flat){
-                        hipLaunchKernelGGL((k_rb_sweep2f<T, S>), dim3(nb_flat), dim3(256), 0, c->stream,
+                    if(quads){
+                        hipLaunchKernelGGL((k_rb_sweep2f<T, S, 4>), dim3(nb_quad), dim3(256), 0, c->stream,
+                            x0, x1, s0, s1, f0, f1, pw, c->im, c->jm, hp, nm, ntb4, nb_quad, colour,
+                            gh, omega, c43, c13);
+                        continue;
+                    } else if(pairs_flat){
+                        hipLaunchKernelGGL((k_rb_sweep2f<T, S, 2>), dim3(nb_flat), dim3(256), 0, c->stream,
                             x0, x1, s0, s1, f0, f1, pw, c->im, c->jm, hp, nm, ntb, nb_flat, colour,
                             gh, omega, c43, c13);
                         continue;

@@ -264,26 +264,40 @@ def main():
         cr, _, _, _ = timed_run(abi.PRECISION_FP64, rounded_start=True)
         fields = abi.PROG if args.config == "full" else ("t", "u", "v", "w")
 
+        # a field whose FP64 values all lie below FP32's smallest normal number (a condensate that
+        # has evaporated to ~1e-60) has no FP32 representation to compare: it is reported with its
+        # absolute difference and left out of the worst case
+        fp32_tiny = float(np.finfo(np.float32).tiny)
+
         def rel(c_other):
             out = {}
             for name in fields + ("p_dyn",):
                 a, b = c_other.get(name), ctx.get(name)
                 d = a - b
                 nb, mb = float(np.linalg.norm(b)), float(np.max(np.abs(b)))
-                out[name] = {"normwise": float(np.linalg.norm(d)) / nb if nb > 0 else float(np.linalg.norm(d)),
-                             "max_abs": float(np.max(np.abs(d))) / mb if mb > 0 else float(np.max(np.abs(d)))}
+                if mb < fp32_tiny:
+                    out[name] = {"below_fp32_range": True, "fp64_max_abs": mb,
+                                 "max_abs_diff": float(np.max(np.abs(d)))}
+                    continue
+                out[name] = {"normwise": float(np.linalg.norm(d)) / nb, "max_abs": float(np.max(np.abs(d))) / mb}
             return out
         errs = rel(c32)
         floor = rel(cr)
         cr.close()
-        worst = max(max(e["normwise"], e["max_abs"]) for e in errs.values())
+        compared = {n: e for n, e in errs.items() if "normwise" in e}
+        worst = max(max(e["normwise"], e["max_abs"]) for e in compared.values())
+        dyn_names = [n for n in ("t", "u", "v", "w", "p_dyn") if n in compared]
+        worst_dyn = max(max(compared[n]["normwise"], compared[n]["max_abs"]) for n in dyn_names)
+        worst_norm = max(e["normwise"] for e in compared.values())
         fp32 = {
             "value": cells * args.steps / (ms32 * 1e-3), "unit": "cell-updates/s",
             "ms_per_step": ms32 / args.steps, "speedup_vs_fp64": ms / ms32, "finite": bool(ok32),
             "poisson_sweep_avg_launch_ms": tm32["poisson_sweeps"][0] / max(tm32["poisson_sweeps"][1], 1),
             "rk4_stage_avg_launch_ms": tm32["rk_stages"][0] / max(tm32["rk_stages"][1], 1),
             "rel_err_vs_fp64_after_steps": args.warmup + args.steps, "worst_rel_err": worst,
-            "within_1e-5": bool(worst <= 1e-5), "rel_err": errs,
+            "within_1e-5": bool(worst <= 1e-5),
+            "worst_rel_err_t_u_v_w_p_dyn": worst_dyn, "within_1e-5_t_u_v_w_p_dyn": bool(worst_dyn <= 1e-5),
+            "worst_normwise_rel_err": worst_norm, "rel_err": errs,
             "fp64_from_fp32_rounded_start_rel_err": floor,
         }
         c32.close()
@@ -333,7 +347,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": SWEEP_TRAFFIC_BYTES.get((g.im, g.jm, g.km)),
-            "kernel": "one red-black colour pass of the FP64 solve (k_rb_sweep2f<double, double>, two cells "
+            "kernel": "one red-black colour pass of the FP64 solve (k_rb_sweep2f<double, double, 2>, two cells "
                       "per thread, on the 41x181x361 shell; k_rb_sweep2 on 81x361x721)",
             "algorithmic_bytes_per_launch": lb,
             "avg_launch_ms": launch_ms,

@@ -63,6 +63,28 @@ def test_pressure_solve_refined_matches_fp64_iteration():
     c32.close()
 
 
+@pytest.mark.parametrize("km", [49, 53, 51])
+def test_refined_solve_all_correction_kernels(km):
+    """The FP32 correction sweeps run four cells per thread where the ring of compact cells is a
+    multiple of four (48 meridians: 24 cells), two where it is even (52: 26), one where it is odd
+    (50: 25); each keeps the refined solve within the bar of the FP64 iteration."""
+    g = inputs.Grid(im=13, jm=33, km=km, the_degree=180.0 / 32, phi_degree=360.0 / (km - 1))
+    st = inputs.make_state("full", seed=5, g=g)
+    c64 = _ctx(st, abi.PRECISION_FP64, 200)
+    c32 = _ctx(st, abi.PRECISION_FP32_REF, 200)
+    c32.mixture_density()
+    c32.rk4_step()
+    for n in ("aux_u", "aux_v", "aux_w", "p_dyn"):
+        c64.set(n, c32.get(n))
+    c64.pressure_solve(200, 1.0)
+    c32.pressure_solve(200, 1.0)
+    norm, mx = field_errors(c32.get("p_dyn"), c64.get("p_dyn"))
+    print(f"km {km}: p_dyn after one solve: normwise {norm:.2e}, max-abs {mx:.2e}")
+    assert mx <= 5e-6
+    c64.close()
+    c32.close()
+
+
 # fields without threshold physics: the bar holds max-abs and normwise
 SMOOTH = ("u", "v", "w", "h2s", "ch4", "nh4sh", "tke", "dis", "p_dyn")
 # the saturation adjustment decides per cell whether vapour condenses; a cell within FP32 rounding
