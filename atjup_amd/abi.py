@@ -19,7 +19,8 @@ INPUTS = ("p_dyn", "p_stat", "p_hydro", "rho_mix", "wall_nue", "nue", "prod", "t
           "S_precip_h2o", "S_precip_h2o_cloud", "S_precip_h2o_ice",
           "S_precip_nh3", "S_precip_nh3_cloud", "S_precip_nh3_ice",
           "S_precip_ch4", "S_precip_ch4_cloud", "S_precip_ch4_ice",
-          "massflux_h2s", "massflux_nh3", "massflux_nh4sh", "fluxlim_nh4sh")
+          "massflux_h2s", "massflux_nh3", "massflux_nh4sh", "fluxlim_nh4sh",
+          "epsilon", "radiation")
 NLEVEL = 64
 
 FIELD = {n: i for i, n in enumerate(PROG)}
@@ -55,7 +56,30 @@ class Knobs(C.Structure):
                 ("tke_max_phys", C.c_double), ("nue_max_phys", C.c_double),
                 ("wall_nue_factor", C.c_double), ("wall_nue_layers", C.c_int),
                 ("satadj", C.c_int), ("rk_scalar_sync", C.c_int), ("dry", C.c_int),
-                ("turb_k_bg_phys", C.c_double)]
+                ("turb_k_bg_phys", C.c_double), ("radiation", C.c_int)]
+
+
+class RadParams(C.Structure):
+    """atjup_rad_params: Jupiter's radiation properties (planet/cJupiterModel.h:1109-1118) and the
+    ATJUP_ knobs Radiation<P>::run reads (planet/Radiation.h:270-295)."""
+    _fields_ = [(n, C.c_double) for n in (
+        "F_int", "S_solar", "albedo_bond", "x_H2", "x_He",
+        "cia_strength", "opacity_strength", "solar_strength", "sw_tau_per_bar")] + [
+        ("solar", C.c_int), ("bottom", C.c_int)]
+
+
+def rad_params(**over) -> RadParams:
+    """The defaults of a new context (Jupiter, every knob at its default), with overrides."""
+    d = dict(F_int=5.4, S_solar=50.5, albedo_bond=0.343, x_H2=0.863, x_He=0.134, cia_strength=3.9,
+             opacity_strength=1.0, solar_strength=1.0, sw_tau_per_bar=1.0, solar=1, bottom=0)
+    d.update(over)
+    return RadParams(**d)
+
+
+# the reference environment variable of each radiation knob (planet/Radiation.h:270-295)
+RAD_ENV = {"cia_strength": "ATJUP_CIA_STRENGTH", "opacity_strength": "ATJUP_OPACITY_STRENGTH",
+           "solar": "ATJUP_SOLAR", "solar_strength": "ATJUP_SOLAR_STRENGTH",
+           "sw_tau_per_bar": "ATJUP_SW_TAU_PER_BAR", "bottom": "ATJUP_RAD_BOTTOM"}
 
 
 class GasParams(C.Structure):
@@ -97,12 +121,14 @@ TURB_MODELS = {"off": -1, "k_epsilon": 0, "k_omega": 1, "k_omega_SST": 2}
 
 
 def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
-                 grid: inputs.Grid | None = None):
+                 grid: inputs.Grid | None = None, radiation: bool = False):
     """(Grid, Consts, Knobs, env) of a configuration.
 
     "dyn"  : dynamics only, dry, closure/radiation/precipitation off (the 5-step benchmark case).
     "full" : k-omega closure, moist, saturation adjustment on, radiation/precipitation as frozen
              inputs (their modules are outside this path).
+    radiation: the grey radiation runs in the physics block (ATJUP_RADIATION=1, the reference's
+             default), so Q_rad follows the state instead of entering as a frozen input.
     """
     g = grid or inputs.Grid()
     G = Grid(im=g.im, jm=g.jm, km=g.km, r0=g.radius_km / g.L_atm, dr=g.dr, dthe=g.dthe,
@@ -119,10 +145,10 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
               species_diff_scale=0.01, rad_coupling=1.0, precip_coupling=1.0, turb_coupling=1.0,
               tke_max_phys=1000.0, nue_max_phys=1.0e5, wall_nue_factor=4.0, wall_nue_layers=3,
               satadj=1, rk_scalar_sync=2, dry=1 if config == "dyn" else 0,
-              turb_k_bg_phys=1.0e-4 if config != "dyn" else 1.0)
+              turb_k_bg_phys=1.0e-4 if config != "dyn" else 1.0, radiation=1 if radiation else 0)
     env = dict(BASE_ENV)
     env["ATJUP_PRESS_SWEEPS"] = str(sweeps)
-    env["ATJUP_RADIATION"] = "0"
+    env["ATJUP_RADIATION"] = "1" if radiation else "0"
     env["ATJUP_PRECIP"] = "0"
     if config == "dyn":
         env["ATJUP_TURB"] = "0"

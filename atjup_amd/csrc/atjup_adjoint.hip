@@ -737,7 +737,8 @@ __global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restric
 }
 
 // the same pass with two cells m, m+1 (m even) per thread and pair loads, flattened over
-// (row, pair); needs an even ring of compact cells
+// (row, pair) in 256-thread blocks so that no lane idles at the end of a row; needs an even ring of
+// compact cells
 __global__ __launch_bounds__(256) void ka_rbT_pass2(double* __restrict__ lam, double* __restrict__ mu,
         const double* __restrict__ mu_o, double* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
         const uint8_t* __restrict__ fo_arr, const uint8_t* __restrict__ inm, const PoisIJ<double>* __restrict__ pw,
@@ -1204,7 +1205,13 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
         hipLaunchKernelGGL(ka_rb_planes_j, gpj, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
         hipLaunchKernelGGL(ka_rb_planes_i, gpi, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
         hipLaunchKernelGGL(ka_rbT_mu, gt, bt, 0, c->stream, (const double*)l1, mu1, f1, pw, c->im, c->jm, hp, nm, omega);
-        const bool pairs = nm % 2 == 0;
+        // two cells per thread, flattened over (row, pair), where the ring is even and row-aligned
+        // pairs would leave more than 10% of the lanes idle; else one cell per thread. Measured on
+        // MI355X over the misfit gradients of the bench: 41 x 181 x 361 (90 pairs per row) 81 ms of
+        // transposed sweeps flattened against 84 ms one cell; 81 x 361 x 721 (180 pairs) 1478 ms
+        // one cell against 1592 ms row-aligned pairs and 1729 ms flattened.
+        const int wf_pair = (nm / 2 + RBT_BM - 1) / RBT_BM;
+        const bool pairs = nm % 2 == 0 && (double)(nm / 2) / (RBT_BM * wf_pair) < 0.9;
         const int ntb = (c->jm * (nm / 2) + 255) / 256;
         for(int s = sweeps - 1; s >= 0; s--){
             const int ghost = s > 0 ? 1 : 0;         // the drop-in's first sweep reads the stored planes

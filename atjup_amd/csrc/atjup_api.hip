@@ -126,6 +126,10 @@ int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, i
     }
     if(!(k->pdyn_cap > 0.0)){ set_error("pdyn_cap"); return ATJUP_ERR_PROFILE; }
     if(!(k->press_omega > 0.0 && k->press_omega < 2.0)){ set_error("press_omega outside (0, 2)"); return ATJUP_ERR_PROFILE; }
+    if(k->radiation != 0 && k->radiation != 1){ set_error("radiation must be 0 or 1"); return ATJUP_ERR_PROFILE; }
+    if(k->radiation && precision == ATJUP_PRECISION_TANGENT){
+        set_error("the tangent form has no radiation"); return ATJUP_ERR_PROFILE;
+    }
     return 0;
 }
 
@@ -160,6 +164,35 @@ void timer_end(atjup_ctx* c, int phase, hipEvent_t begin, int launches){
 }
 
 // the physics block of an iteration (cJupiterModel.cpp:1032-1161), on even iterations and the first
+// the radiation's host tables: F_sun_toa per row j (Radiation.h:374-380, the diurnal mean
+// S (1 - A) cos(lat) / pi with colatitude j pi / (jm - 1)) and the layer thickness in metres each
+// layer's heating divides by (Radiation.h:451-452: the top layer takes the one below's; <= 0 -> 1)
+int build_rad_table(atjup_ctx* c){
+    const atjup_rad_params& R = c->radp;
+    const double pi_ = 3.14159265358979323846;
+    std::vector<double> tab((size_t)c->jm + c->im);
+    for(int j = 0; j < c->jm; j++){
+        double f = 0.0;
+        if(R.solar != 0){
+            const double colat = pi_ * (double)j / (double)(c->jm - 1);
+            const double cos_lat = std::max(0.0, std::sin(colat));
+            f = R.solar_strength * R.S_solar * (1.0 - R.albedo_bond) * cos_lat / pi_;
+        }
+        tab[j] = f;
+    }
+    auto ltm = [&](int i){ return (i < 0 || i > c->im - 2) ? -1.0 : (double)(c->lh[i + 1] - c->lh[i]) * 1.0e3; };
+    for(int i = 0; i < c->im; i++){
+        double dz = (i < c->im - 1) ? ltm(i) : ltm(i - 1);
+        if(dz <= 0.0) dz = 1.0;
+        tab[c->jm + i] = dz;
+    }
+    if(!c->rad_tab.p){
+        if(int rc = alloc(c->rad_tab, tab.size() * sizeof(double))) return rc;
+    }
+    ATJUP_HIP_CHECK(hipMemcpy(c->rad_tab.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
 int step_block(atjup_ctx* c, int iter, int first){
     int rc = 0;
     if(!(iter % 2 == 0 || first)) return 0;
@@ -168,6 +201,7 @@ int step_block(atjup_ctx* c, int iter, int first){
     timer_end(c, ATJUP_T_PRESSURE, t0, 0);
     if(c->kn.damp_pdyn && (rc = launch_shapiro(c, ATJUP_F_P_DYN, 2, 1.0, 1, 7))) return rc;
     if((rc = launch_mixture_density(c))) return rc;
+    if(c->kn.radiation && (rc = launch_radiation(c))) return rc;      // cJupiterModel.cpp:1056
     if(!c->kn.dry){
         // the three adjustments, then RK_SCALAR_SYNC 2 (the identity on the shared storage),
         // the closure with its wall function, the NH4SH limiter and its smoothing
@@ -327,6 +361,10 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
             if(!rc && hipMemcpy(c->tabs.p, tabs.data(), c->tabs.bytes, hipMemcpyHostToDevice) != hipSuccess) rc = ATJUP_ERR_HIP;
         }
     }
+    if(!rc){
+        atjup_hip_default_rad_params(&c->radp);
+        rc = build_rad_table(c);
+    }
     if(rc){ atjup_hip_ctx_destroy(c); return nullptr; }
     return c;
 }
@@ -342,7 +380,7 @@ void atjup_hip_ctx_destroy(atjup_ctx* c){
     for(atjup::Buf* b : { &c->lo[0], &c->lo[1], &c->lo[2], &c->lo[3], &c->colr }) release(*b);
     for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
-                          &c->amb_rows, &c->nonfinite })
+                          &c->amb_rows, &c->nonfinite, &c->rad_tab })
         release(*b);
     for(auto& v : c->spans) for(auto& sp : v){ (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
     for(auto e : c->event_pool) (void)hipEventDestroy(e);
@@ -505,6 +543,25 @@ int atjup_hip_satadj(atjup_ctx* c, int gas, const atjup_gas_params* p){
     if(c->kn.dry) return 0;        // all species zero: the adjustment is the identity (SaturationAdjustment.h:440)
     if(!p && !c->gas_set[gas]){ set_error("gas constants not set (atjup_hip_set_gas)"); return ATJUP_ERR_STATE; }
     return launch_satadj(c, gas, p ? *p : c->gas[gas]);
+}
+void atjup_hip_default_rad_params(atjup_rad_params* p){
+    if(!p) return;
+    std::memset(p, 0, sizeof(*p));
+    // Jupiter (planet/cJupiterModel.h:1109-1118) and the knob defaults (planet/Radiation.h:270-295)
+    p->F_int = 5.4; p->S_solar = 50.5; p->albedo_bond = 0.343; p->x_H2 = 0.863; p->x_He = 0.134;
+    p->cia_strength = 3.9; p->opacity_strength = 1.0; p->solar_strength = 1.0; p->sw_tau_per_bar = 1.0;
+    p->solar = 1; p->bottom = 0;
+}
+int atjup_hip_set_radiation(atjup_ctx* c, const atjup_rad_params* p){
+    if(!c || !p){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(p->bottom != 0 && p->bottom != 1){ set_error("radiation bottom must be 0 or 1"); return ATJUP_ERR_ARG; }
+    c->radp = *p;
+    return build_rad_table(c);
+}
+int atjup_hip_radiation(atjup_ctx* c){
+    if(int rc = need_mask(c)) return rc;
+    if((size_t)(c->im + 1) * 64 * sizeof(double) > 65536){ set_error("radiation: im above 1023 levels"); return ATJUP_ERR_PROFILE; }
+    return launch_radiation(c);
 }
 int atjup_hip_turbulence_run(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;

@@ -34,6 +34,14 @@ __device__ __forceinline__ T safe_clamp_d(T v, T lo, T hi){
     return (v < lo) ? lo : ((v > hi) ? hi : v);
 }
 
+// bitwise equality (a NaN equals itself, +0 and -0 differ): a store of `a` over `b` changes nothing
+__device__ __forceinline__ bool same_bits(double a, double b){
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+__device__ __forceinline__ bool same_bits(float a, float b){
+    return __float_as_int(a) == __float_as_int(b);
+}
+
 // std::max / std::min with their exact operand semantics (NaN and signed-zero behaviour included)
 template<typename T>
 __device__ __forceinline__ T dmax(T a, T b){ return (a < b) ? b : a; }

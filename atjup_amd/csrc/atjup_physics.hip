@@ -59,6 +59,11 @@ constexpr int SF_BK = 64, SF_BJ = 2;            // 2 rows: 1.7% less stage time 
 #ifndef SF_MIN_WAVES
 #define SF_MIN_WAVES 1
 #endif
+// species per load group in the species stage kernel: 2 gives 4% less FP32 stage time than 1 on
+// MI355X (FP64 0.5%), 3 no more
+#ifndef SP_GROUP
+#define SP_GROUP 2
+#endif
 
 template<typename T>
 struct Stencil7 { T c, ip, im, jp, jm, kp, km; };
@@ -72,10 +77,28 @@ __device__ __forceinline__ Stencil7<T> load7(const T* __restrict__ X, size_t q, 
     return s;
 }
 
+// what the species RHS and stage update of one species read at a cell, loaded together so that a
+// group of species can have all its loads in flight before its first store
 template<typename T>
-__device__ __forceinline__ T species_rhs(const StageFullArgs<T>& a, int f, size_t q, size_t P, size_t R,
-        size_t qke, size_t qkw, const GeoIJ<T>& G, T u, T v_invrm, T w_invrs, T two_inv_rm, T nue_t_s, T rho_f){
-    const Stencil7<T> s = load7(a.in[f], q, P, R, qke, qkw);
+struct SpeciesLoads { Stencil7<T> s; T mf, spr, fl, acc, nl; };
+
+template<typename T>
+__device__ __forceinline__ SpeciesLoads<T> species_load(const StageFullArgs<T>& a, int f, size_t q, size_t P,
+        size_t R, size_t qke, size_t qkw){
+    SpeciesLoads<T> L;
+    L.s = load7(a.in[f], q, P, R, qke, qkw);
+    L.mf = a.mf[f] ? a.mf[f][q] : T(0);
+    L.spr = (f != ATJUP_F_NH4SH && a.spr[f]) ? a.spr[f][q] : T(0);
+    L.fl = (f == ATJUP_F_NH4SH) ? a.fl_nh4sh[q] : T(0);
+    L.acc = (a.stage == 0) ? T(0) : a.acc[f][q];
+    L.nl = a.nl[f][q];
+    return L;
+}
+
+template<typename T>
+__device__ __forceinline__ T species_rhs(const StageFullArgs<T>& a, int f, const SpeciesLoads<T>& L,
+        const GeoIJ<T>& G, T u, T v_invrm, T w_invrs, T two_inv_rm, T nue_t_s, T rho_f){
+    const Stencil7<T>& s = L.s;
     const T e1r = (s.ip - s.im) * a.inv_2dr, e2r = (s.ip - T(2) * s.c + s.im) * a.inv_dr2;
     const T e1t = (s.jp - s.jm) * a.inv_2dthe, e2t = (s.jp - T(2) * s.c + s.jm) * a.inv_dthe2;
     const T e1p = (s.kp - s.km) * a.inv_2dphi, e2p = (s.kp - T(2) * s.c + s.km) * a.inv_dphi2;
@@ -86,14 +109,14 @@ __device__ __forceinline__ T species_rhs(const StageFullArgs<T>& a, int f, size_
     if(f == ATJUP_F_NH4SH){
         const T v_stokes = a.stokes_a * (a.rg_nh4sh - rho_f) * (T)a.g / a.mue_mix;
         rhs = -transport
-            + a.fl_nh4sh[q]
+            + L.fl
             + diffusion * (a.sp_coef[f] + nue_t_s)
-            + a.cr * a.mf[f][q]
+            + a.cr * L.mf
             + (v_stokes / a.u_0) * e1r;
     } else {
         rhs = -transport + diffusion * (a.sp_coef[f] + nue_t_s);
-        if(a.mf[f]) rhs = rhs + a.cr * a.mf[f][q];
-        if(a.spr[f]) rhs = rhs + a.pc * a.spr[f][q];
+        if(a.mf[f]) rhs = rhs + a.cr * L.mf;
+        if(a.spr[f]) rhs = rhs + a.pc * L.spr;
     }
     return rhs;
 }
@@ -102,13 +125,14 @@ __device__ __forceinline__ T species_rhs(const StageFullArgs<T>& a, int f, size_
 // (max(0, .) at the end), 2: nh4sh (< 0 -> 0 at the end), 3: k* (safe_clamp to [0, tke_max]),
 // 4: omega* (max(dis_min, .))
 template<typename T>
-__device__ __forceinline__ void stage_update(const StageFullArgs<T>& a, int f, size_t q, T rhs, int kind){
+__device__ __forceinline__ void stage_update_v(const StageFullArgs<T>& a, int f, size_t q, T rhs, int kind,
+        T acc_in, T nl_v){
     T accf;
     if(a.stage == 0) accf = a.wgt * rhs;
-    else accf = a.acc[f][q] + a.wgt * rhs;
+    else accf = acc_in + a.wgt * rhs;
     if(a.stage < 3){
         a.acc[f][q] = accf;
-        T v = a.nl[f][q] + a.c_in * rhs;
+        T v = nl_v + a.c_in * rhs;
         if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
         else if(kind == 4) v = dmax(a.dis_min, v);
         a.out[f][q] = v;
@@ -116,20 +140,25 @@ __device__ __forceinline__ void stage_update(const StageFullArgs<T>& a, int f, s
         if constexpr(std::is_same<T, float>::value){
             // t, u, v, w of the refined form: the increment added to hi + lo in FP64 and split
             if(f < 4 && a.lo[f]){
-                const double s = ((double)a.nl[f][q] + (double)a.lo[f][q]) + a.dt_d * (double)accf * (1.0 / 6.0);
+                const double s = ((double)nl_v + (double)a.lo[f][q]) + a.dt_d * (double)accf * (1.0 / 6.0);
                 const float h = (float)s;
                 a.out[f][q] = h;
                 a.lo[f][q] = (float)(s - (double)h);
                 return;
             }
         }
-        T v = a.nl[f][q] + a.dt * accf * a.one_sixth;
+        T v = nl_v + a.dt * accf * a.one_sixth;
         if(kind == 1) v = dmax(T(0), v);
         else if(kind == 2){ if(v < T(0)) v = T(0); }
         else if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
         else if(kind == 4) v = dmax(a.dis_min, v);
         a.out[f][q] = v;
     }
+}
+
+template<typename T>
+__device__ __forceinline__ void stage_update(const StageFullArgs<T>& a, int f, size_t q, T rhs, int kind){
+    stage_update_v(a, f, q, rhs, kind, (a.stage == 0) ? T(0) : a.acc[f][q], a.nl[f][q]);
 }
 
 // The eleven species (RHS_Jup_Turb.cpp:1273-1338) in a kernel of their own: they need only the
@@ -168,10 +197,20 @@ __global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_species(const StageFull
     const T nue_c = a.nue[q];
     const T nue_t = (a.turb_coupling != T(0) && isfinite(nue_c)) ? a.turb_coupling * dmax(T(0), nue_c) : T(0);
     const T nue_t_s = nue_t / T(0.9);
+    // SP_GROUP species at a time: the group's loads, then its updates
     #pragma unroll
-    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++){
-        const T rhs = species_rhs(a, f, q, P, R, qke, qkw, G, u, v_invrm, w_invrs, two_inv_rm, nue_t_s, rho_f);
-        stage_update(a, f, q, rhs, f == ATJUP_F_NH4SH ? 2 : 1);
+    for(int f0 = ATJUP_F_H2O; f0 <= ATJUP_F_NH4SH; f0 += SP_GROUP){
+        SpeciesLoads<T> L[SP_GROUP];
+        #pragma unroll
+        for(int g = 0; g < SP_GROUP; g++)
+            if(f0 + g <= ATJUP_F_NH4SH) L[g] = species_load(a, f0 + g, q, P, R, qke, qkw);
+        #pragma unroll
+        for(int g = 0; g < SP_GROUP; g++){
+            const int f = f0 + g;
+            if(f > ATJUP_F_NH4SH) break;
+            const T rhs = species_rhs(a, f, L[g], G, u, v_invrm, w_invrs, two_inv_rm, nue_t_s, rho_f);
+            stage_update_v(a, f, q, rhs, f == ATJUP_F_NH4SH ? 2 : 1, L[g].acc, L[g].nl);
+        }
     }
 }
 
@@ -219,6 +258,12 @@ __global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(cons
         d1p[f] = (s.kp - s.km) * a.inv_2dphi;
         d2p[f] = (s.kp - T(2) * s.c + s.km) * a.inv_dphi2;
     }
+    const T u = c_[1], v = c_[2], w = c_[3];
+    const T v_invrm = v * G.inv_rm;
+    const T w_invrs = w * G.inv_rmsinthe;
+    const T two_inv_rm = T(2) * G.inv_rm;
+    const T nue_c = a.nue[q];
+
     // wall-mirrored pressure gradients (RHS_Jup_Turb.cpp:119-130)
     const T pdc = a.pd[q], phc = a.ph[q];
     const T dpdr   = (((fl & FL_IP) ? pdc : a.pd[q + P]) - ((fl & FL_IM) ? pdc : a.pd[q - P])) * a.inv_2dr;
@@ -227,13 +272,10 @@ __global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(cons
     const T dphdthe = (((fl & FL_JP) ? phc : a.ph[q + R]) - ((fl & FL_JM) ? phc : a.ph[q - R])) * a.inv_2dthe;
     const T dphdphi = (((fl & FL_KP) ? phc : a.ph[qke]) - ((fl & FL_KM) ? phc : a.ph[qkw])) * a.inv_2dphi;
 
-    const T tq = c_[0], u = c_[1], v = c_[2], w = c_[3];
+    const T tq = c_[0];
     const T costhe = G.costhe, st = G.sinthe_true;
     const T Coriolis_the = a.c_m2o * costhe * w;
     const T Coriolis_phi = a.c_p2o * (+costhe * v + st * u);
-    const T v_invrm = v * G.inv_rm;
-    const T w_invrs = w * G.inv_rmsinthe;
-    const T two_inv_rm = T(2) * G.inv_rm;
 
     const T ps_c = a.ps[q];
     const T rho_f = a.rho[q];
@@ -290,7 +332,6 @@ __global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(cons
             else if(precip_t < T(-0.5)) precip_t = T(-0.5);
         }
     }
-    const T nue_c = a.nue[q];
     const T nue_t = (a.turb_coupling != T(0) && isfinite(nue_c)) ? a.turb_coupling * dmax(T(0), nue_c) : T(0);
     const T nue_t_s = nue_t / T(0.9);
     const T visc = a.inv_re + nue_t + a.wn[q];
@@ -585,21 +626,27 @@ __global__ __launch_bounds__(256) void k_turb_post(T* __restrict__ tke, T* __res
         tke[q] = T(0); dis[q] = T(0); nue[q] = T(0); prod[q] = T(0); tsrc[q] = T(0); dsrc[q] = T(0);
         return;
     }
-    T n = nue[q];
+    // only the values the clamps change are written back
+    const T n0 = nue[q];
+    T n = n0;
     if(i <= im - 2 && j >= 1 && j <= jm - 2 && k <= km - 2){
-        const T kk = dmax(k_bg, tke[q]);
-        const T dd = dmax(dis_min, dis[q]);
-        tke[q] = kk;
-        dis[q] = dd;
+        const T k0 = tke[q], d0 = dis[q];
+        const T kk = dmax(k_bg, k0);
+        const T dd = dmax(dis_min, d0);
+        if(!same_bits(kk, k0)) tke[q] = kk;
+        if(!same_bits(dd, d0)) dis[q] = dd;
         n = (n < T(0)) ? T(0) : ((n > nue_max) ? nue_max : n);
         const T dsm = T(20) * dd * dd;
         const T ds = dsrc[q];
-        dsrc[q] = (ds < -dsm) ? -dsm : ((ds > dsm) ? dsm : ds);
+        const T dsc = (ds < -dsm) ? -dsm : ((ds > dsm) ? dsm : ds);
+        if(!same_bits(dsc, ds)) dsrc[q] = dsc;
         const T tsm = T(20) * kk * dd;
         const T ts = tsrc[q];
-        tsrc[q] = (ts < -tsm) ? -tsm : ((ts > tsm) ? tsm : ts);
+        const T tsc = (ts < -tsm) ? -tsm : ((ts > tsm) ? tsm : ts);
+        if(!same_bits(tsc, ts)) tsrc[q] = tsc;
     }
-    nue[q] = ((n < T(0)) ? T(0) : ((n > nue_max) ? nue_max : n)) * T(1);
+    const T nc = ((n < T(0)) ? T(0) : ((n > nue_max) ? nue_max : n)) * T(1);
+    if(!same_bits(nc, n0)) nue[q] = nc;
 }
 
 // the i = 0 plane: apply_wall_bc copies k*, omega*, nue from i = 1 after its clamp-out; then
@@ -693,12 +740,17 @@ __global__ __launch_bounds__(256) void k_turb_sanitize(T* __restrict__ tke, T* _
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(q >= n) return;
     const T prod_max = T(1.0e4), dis_max = T(1.0e6);
-    tke[q] = safe_clamp_d(tke[q], T(0), tke_max);
-    dis[q] = safe_clamp_d(dis[q], T(1.0e-10), dis_max);
-    nue[q] = safe_clamp_d(nue[q], T(0), nue_max);
-    prod[q] = safe_clamp_d(prod[q], T(0), prod_max);
-    tsrc[q] = safe_clamp_d(tsrc[q], -prod_max, prod_max);
-    dsrc[q] = safe_clamp_d(dsrc[q], -prod_max, prod_max);
+    // only the cells the clamp changes are written back (most are in range)
+    auto clamp = [&](T* x, T lo, T hi){
+        const T v = x[q], c = safe_clamp_d(v, lo, hi);
+        if(!same_bits(c, v)) x[q] = c;
+    };
+    clamp(tke, T(0), tke_max);
+    clamp(dis, T(1.0e-10), dis_max);
+    clamp(nue, T(0), nue_max);
+    clamp(prod, T(0), prod_max);
+    clamp(tsrc, -prod_max, prod_max);
+    clamp(dsrc, -prod_max, prod_max);
 }
 
 // =============================================================================================
@@ -1128,6 +1180,150 @@ int do_commit_live(atjup_ctx* c){
     return 0;
 }
 
+// =============================================================================================
+// Radiation<P>::run (Radiation.h:245-487): grey two-stream radiation of every (j, k) column,
+// i_base = max(i_topography, 0) the deep boundary and i = im-1 the top. Two kernels:
+//   k_rad_layers   one thread per cell: the layer's emission B = sigma T^4, its emissivity
+//                  eps = 1 - exp(-tau) (CIA + CH4/NH3 + capped cloud optical depth) and the
+//                  downward shortwave flux Fsw at its bottom interface, all independent per cell;
+//   k_rad_columns  one thread per column: the two recurrences (downward flux top to bottom, the
+//                  deep boundary, upward flux bottom to top), the layer-centre net flux and the
+//                  heating Q_rad, and the copy into the solid cells below i_base.
+// B and Fsw travel from the first kernel to the second in the Q_rad and radiation arrays, which
+// the second kernel overwrites with the results after reading them; Fd of the column is held in
+// LDS. Columns with i_base >= im-1 are left untouched, as the reference skips them.
+// =============================================================================================
+struct RadConsts {
+    double t_ref, g, cia_coeff, oc, sw_tau_bar, F_int;
+    int bottom;
+};
+constexpr double RAD_SIGMA = 5.670374419e-8;            // Radiation.h:256
+constexpr double RAD_P0 = 1.0e5;                         // Radiation.h:292
+constexpr double RAD_KAPPA_CH4 = 0.003, RAD_KAPPA_NH3 = 1.5;          // Radiation.h:335-338
+constexpr double RAD_KAPPA_CLOUD = 25.0, RAD_KAPPA_ICE = 12.0, RAD_TAU_CLOUD_CAP = 0.4;
+constexpr int RAD_BK = 64;
+
+template<typename T>
+struct RadFields {
+    const T *t, *p_stat, *rho, *ch4, *nh3, *cl[3], *ic[3];
+    const float* lo_t;                     // FP32-refined form: t's low word
+    T *eps, *rad, *qrad;
+};
+
+template<typename T>
+__global__ __launch_bounds__(RAD_BK) void k_rad_layers(const RadFields<T> F, const int* __restrict__ itopo,
+        const double* __restrict__ fsun, int im, int jm, int km, int pitch, const RadConsts C){
+    const int k = blockIdx.x * RAD_BK + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const int i_base = max(itopo[(size_t)j * km + k], 0), i_top = im - 1;
+    if(i_base >= i_top || i < i_base) return;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    double tq = (double)F.t[q];
+    if(F.lo_t) tq += (double)F.lo_t[q];
+    const double Tk = tq * C.t_ref;
+    const double B = RAD_SIGMA * Tk * Tk * Tk * Tk;
+    const double P_lo = (double)F.p_stat[q] * RAD_P0;
+    const double P_hi = (i < i_top) ? (double)F.p_stat[q + (size_t)jm * pitch] * RAD_P0 : 0.0;
+    double dP = P_lo - P_hi;
+    if(dP < 0.0) dP = 0.0;
+    const double P_mean = 0.5 * (P_lo + P_hi);
+    const double dm = dP / C.g;
+    const double tau_cia = (Tk > 0.0) ? C.cia_coeff * (P_mean / Tk) * dP : 0.0;
+    const double rho_c = (double)F.rho[q];
+    const double inv_rho = (rho_c > 0.0 && isfinite(rho_c)) ? 1.0 / rho_c : 0.0;
+    const double q_ch4 = dmax(0.0, (double)F.ch4[q]) * inv_rho;
+    const double q_nh3 = dmax(0.0, (double)F.nh3[q]) * inv_rho;
+    const double tau_gas = C.oc * (RAD_KAPPA_CH4 * q_ch4 + RAD_KAPPA_NH3 * q_nh3) * dm;
+    const double q_liq = (dmax(0.0, (double)F.cl[0][q]) + dmax(0.0, (double)F.cl[1][q])
+                        + dmax(0.0, (double)F.cl[2][q])) * inv_rho;
+    const double q_ice = (dmax(0.0, (double)F.ic[0][q]) + dmax(0.0, (double)F.ic[1][q])
+                        + dmax(0.0, (double)F.ic[2][q])) * inv_rho;
+    double tau_cloud = C.oc * (RAD_KAPPA_CLOUD * q_liq + RAD_KAPPA_ICE * q_ice) * dm;
+    if(tau_cloud > RAD_TAU_CLOUD_CAP) tau_cloud = RAD_TAU_CLOUD_CAP;
+    const double tau = tau_cia + tau_gas + tau_cloud;
+    F.eps[q] = (T)(1.0 - exp(-tau));
+    F.qrad[q] = (T)B;
+    F.rad[q] = (T)(fsun[j] * exp(-C.sw_tau_bar * dmax(0.0, (double)F.p_stat[q])));
+}
+
+template<typename T>
+__global__ __launch_bounds__(RAD_BK) void k_rad_columns(const RadFields<T> F, const int* __restrict__ itopo,
+        const double* __restrict__ fsun, const double* __restrict__ dz_tab, int im, int jm, int km, int pitch,
+        const RadConsts C){
+    extern __shared__ double s_fd[];                    // [(im + 1) * RAD_BK]: Fd per interface
+    const int k = blockIdx.x * RAD_BK + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const int i_base = max(itopo[(size_t)j * km + k], 0), i_top = im - 1;
+    if(i_base >= i_top) return;
+    const size_t P = (size_t)jm * pitch;
+    const size_t q0 = (size_t)j * pitch + k;             // the column's cell at i = 0
+    double* fd = s_fd + threadIdx.x;                     // fd[i * RAD_BK]
+    // downward sweep: Fd[i] = Fd[i+1] (1 - eps_i) + eps_i B_i, Fd[i_top + 1] = 0 (space is cold)
+    double d = 0.0;
+    fd[(i_top + 1) * RAD_BK] = 0.0;
+    for(int i = i_top; i >= i_base; i--){
+        const double e = (double)F.eps[q0 + i * P], b = (double)F.qrad[q0 + i * P];
+        d = d * (1.0 - e) + e * b;
+        fd[i * RAD_BK] = d;
+    }
+    // the deep boundary: net upward F_int plus the shortwave left at depth, or (bottom = 1) no
+    // divergence in the bottom layer
+    double fsw = (double)F.rad[q0 + i_base * P];
+    double u;
+    if(C.bottom == 1) u = dmax(0.0, 2.0 * (double)F.qrad[q0 + i_base * P] - fd[(i_base + 1) * RAD_BK]);
+    else u = fd[i_base * RAD_BK] + C.F_int + fsw;
+    // upward sweep with the layer results
+    const double fsw_top = fsun[j] * exp(-C.sw_tau_bar * dmax(0.0, 0.0));
+    for(int i = i_base; i <= i_top; i++){
+        const size_t q = q0 + i * P;
+        const double e = (double)F.eps[q], b = (double)F.qrad[q];
+        const double u_next = u * (1.0 - e) + e * b;
+        const double net_bot = u - fd[i * RAD_BK];
+        const double net_top = u_next - fd[(i + 1) * RAD_BK];
+        const double dz = dz_tab[i];
+        const double fsw_next = (i < i_top) ? (double)F.rad[q + P] : fsw_top;
+        const double Q_sw = (fsw_next - fsw) / dz;
+        F.rad[q] = (T)(0.5 * (net_bot + net_top));
+        F.qrad[q] = (T)((net_bot - net_top) / dz + Q_sw);
+        u = u_next;
+        fsw = fsw_next;
+    }
+    // the solid cells below take the deepest fluid values, Q_rad 0
+    const T e_base = F.eps[q0 + i_base * P], r_base = F.rad[q0 + i_base * P];
+    for(int i = 0; i < i_base; i++){
+        F.eps[q0 + i * P] = e_base;
+        F.rad[q0 + i * P] = r_base;
+        F.qrad[q0 + i * P] = T(0);
+    }
+}
+
+template<typename T>
+int do_radiation(atjup_ctx* c){
+    const atjup_rad_params& R = c->radp;
+    RadFields<T> Fp;
+    Fp.t = F<T>(c, ATJUP_F_T); Fp.p_stat = F<T>(c, ATJUP_F_P_STAT); Fp.rho = F<T>(c, ATJUP_F_RHO_MIX);
+    Fp.ch4 = F<T>(c, ATJUP_F_CH4); Fp.nh3 = F<T>(c, ATJUP_F_NH3);
+    Fp.cl[0] = F<T>(c, ATJUP_F_H2O_CLOUD); Fp.cl[1] = F<T>(c, ATJUP_F_NH3_CLOUD); Fp.cl[2] = F<T>(c, ATJUP_F_CH4_CLOUD);
+    Fp.ic[0] = F<T>(c, ATJUP_F_H2O_ICE); Fp.ic[1] = F<T>(c, ATJUP_F_NH3_ICE); Fp.ic[2] = F<T>(c, ATJUP_F_CH4_ICE);
+    Fp.lo_t = (const float*)c->lo[ATJUP_F_T].p;
+    Fp.eps = F<T>(c, ATJUP_F_EPSILON); Fp.rad = F<T>(c, ATJUP_F_RADIATION); Fp.qrad = F<T>(c, ATJUP_F_Q_RAD);
+    // Radiation.h:272-280: the collision-pair factor, the CIA coefficient, the opacity calibration
+    const double he_ratio = 0.6, C_cia = 3.5e-6, opac_cal = 0.025;
+    const double comp = R.x_H2 * R.x_H2 + he_ratio * R.x_H2 * R.x_He;
+    RadConsts C{ c->c.t_ref, c->c.g, C_cia * R.cia_strength * comp / c->c.g, R.opacity_strength * opac_cal,
+                 R.sw_tau_per_bar, R.F_int, R.bottom };
+    const double* fsun = (const double*)c->rad_tab.p;
+    const double* dz = fsun + c->jm;
+    hipEvent_t t0 = timer_begin(c);
+    hipLaunchKernelGGL(k_rad_layers<T>, dim3(nblk(c->km, RAD_BK), c->jm, c->im), dim3(RAD_BK), 0, c->stream,
+        Fp, (const int*)c->itopo.p, fsun, c->im, c->jm, c->km, c->pitch, C);
+    const size_t lds = (size_t)(c->im + 1) * RAD_BK * sizeof(double);
+    hipLaunchKernelGGL(k_rad_columns<T>, dim3(nblk(c->km, RAD_BK), c->jm), dim3(RAD_BK), lds, c->stream,
+        Fp, (const int*)c->itopo.p, fsun, dz, c->im, c->jm, c->km, c->pitch, C);
+    timer_end(c, ATJUP_T_RADIATION, t0, 2);
+    return check_launch();
+}
+
 }  // namespace
 
 // the tangent form covers the dynamics profile only
@@ -1141,6 +1337,7 @@ int launch_turb_wall(atjup_ctx* c){ return DISPATCH(do_turb_wall, c); }
 int launch_turb_sanitize(atjup_ctx* c){ return DISPATCH(do_turb_sanitize, c); }
 int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){ return DISPATCH(do_satadj, c, gas, g); }
 int launch_flux_limiter(atjup_ctx* c){ return DISPATCH(do_flux_limiter, c); }
+int launch_radiation(atjup_ctx* c){ return DISPATCH(do_radiation, c); }
 int launch_commit_live(atjup_ctx* c){ return c->turb_live_pending ? DISPATCH(do_commit_live, c) : 0; }
 
 }  // namespace atjup

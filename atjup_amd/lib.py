@@ -50,7 +50,7 @@ def load():
               "atjup_hip_bc_after_step",
               "atjup_hip_mixture_density", "atjup_hip_flux_limiter", "atjup_hip_clamp_species",
               "atjup_hip_swap_nlevel", "atjup_hip_scalar_sync", "atjup_hip_check_finite",
-              "atjup_hip_sync", "atjup_hip_pitch"):
+              "atjup_hip_sync", "atjup_hip_pitch", "atjup_hip_radiation"):
         getattr(L, n).argtypes = [C.c_void_p]
     L.atjup_hip_device_count.argtypes = [C.POINTER(C.c_int)]
     for n in ("atjup_hip_upload_tangent", "atjup_hip_adj_upload"):
@@ -65,6 +65,8 @@ def load():
     L.atjup_hip_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
     L.atjup_hip_satadj.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
     L.atjup_hip_set_gas.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
+    L.atjup_hip_set_radiation.argtypes = [C.c_void_p, C.POINTER(abi.RadParams)]
+    L.atjup_hip_default_rad_params.argtypes = [C.POINTER(abi.RadParams)]
     L.atjup_hip_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
     L.atjup_hip_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.atjup_hip_set_stream.argtypes = [C.c_void_p, C.c_void_p]
@@ -174,6 +176,13 @@ class Context:
         p = C.byref(params) if params is not None else None
         self._chk(self.L.atjup_hip_satadj(self.h, gas, p), "satadj")
 
+    def set_radiation(self, params=None):
+        rp = params if params is not None else abi.rad_params()
+        self._chk(self.L.atjup_hip_set_radiation(self.h, C.byref(rp)), "set_radiation")
+
+    def radiation(self):
+        self._chk(self.L.atjup_hip_radiation(self.h), "radiation")
+
     def turbulence_run(self):
         self._chk(self.L.atjup_hip_turbulence_run(self.h), "turbulence_run")
 
@@ -263,7 +272,8 @@ class Context:
                                                 None if out is None else _dp(out)), "adj_backward")
         return out
 
-    TIMERS = ("poisson_sweeps", "rk_stages", "pressure", "step", "adj_sweeps", "adj_step", "adj_recompute")
+    TIMERS = ("poisson_sweeps", "rk_stages", "pressure", "step", "adj_sweeps", "adj_step", "adj_recompute",
+              "radiation")
 
     def set_timing(self, on=True):
         self._chk(self.L.atjup_hip_set_timing(self.h, 1 if on else 0), "set_timing")

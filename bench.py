@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the colour pass from two rocprofv3 PMC runs (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0, KiB per dispatch, the factors from a device copy of known size), per grid;
 # profiles/step_full_41x181x361_hbm_traffic.md holds the summary and the command
-SWEEP_TRAFFIC_BYTES = {(41, 181, 361): (13910.6 * 2.0 + 10419.2) * 1024}
+SWEEP_TRAFFIC_BYTES = {(41, 181, 361): (13907.4 * 2.0 + 10442.8) * 1024}
 WORKLOAD = {
     "dyn": "dynamics-only FP64 time step",
     "full": "full-physics FP64 time step (k-omega closure, saturation adjustment of H2O/NH3/CH4, NH4SH "
@@ -187,6 +187,29 @@ def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=N
     }
 
 
+# Algorithmic bytes per cell of one radiation call: t, p_stat, rho_mix, CH4, NH3 and the six
+# condensates read once, epsilon, radiation and Q_rad written once (planet/Radiation.h:311-463).
+RAD_BYTES_PER_CELL = (11 + 3) * 8
+
+
+def radiation_leg(ctx, g, calls=20):
+    """Radiation<P>::run on the GPU, the mean over `calls` calls from the library's HIP events."""
+    ctx.radiation()
+    ctx.sync()
+    ctx.set_timing(True)
+    ctx.read_timers()
+    for _ in range(calls):
+        ctx.radiation()
+    ctx.sync()
+    total_ms, launches = ctx.read_timers()["radiation"]
+    ms = total_ms / calls
+    nbytes = RAD_BYTES_PER_CELL * g.im * g.jm * g.km
+    return {"workload": f"grey two-stream radiation of all {g.jm * g.km} columns, {g.im} levels",
+            "ms_per_call": ms, "launches_per_call": launches / calls,
+            "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
+            "hbm_frac": nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,6 +325,10 @@ def main():
         }
         c32.close()
 
+    # the grey radiation (a component beside the path: it runs in the physics block when the
+    # profile turns it on), timed on the state the FP64 run left
+    rad = radiation_leg(ctx, g) if args.config == "full" else None
+
     value = cells * args.steps / (ms * 1e-3)
     launch_ms = sweep_ms / max(sweep_launches, 1)
     lb = poisson_launch_bytes(g, 8, args.omega)
@@ -364,6 +391,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32_refined": fp32,
         "adjoint": adj,
+        "radiation": rad,
     }
     print(json.dumps(out))
 

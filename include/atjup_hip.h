@@ -59,6 +59,7 @@ enum atjup_field {
     ATJUP_F_S_PRECIP_NH3, ATJUP_F_S_PRECIP_NH3_CLOUD, ATJUP_F_S_PRECIP_NH3_ICE,
     ATJUP_F_S_PRECIP_CH4, ATJUP_F_S_PRECIP_CH4_CLOUD, ATJUP_F_S_PRECIP_CH4_ICE,
     ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH,
+    ATJUP_F_EPSILON, ATJUP_F_RADIATION,            /* the radiation's layer emissivity and net flux */
     ATJUP_F_END,
     ATJUP_NLEVEL = 64                               /* ATJUP_NLEVEL + prognostic id */
 };
@@ -103,7 +104,18 @@ typedef struct atjup_knobs {
     int    rk_scalar_sync;      /* ATJUP_RK_SCALAR_SYNC 2 */
     int    dry;                 /* species identically zero with zero sources: they are not streamed */
     double turb_k_bg_phys;      /* ATJUP_TURB_K_BG [m2/s2] */
+    int    radiation;           /* ATJUP_RADIATION: the grey radiation in the physics block (the
+                                   reference's default is 1, cJupiterModel.cpp:52, :1056) */
 } atjup_knobs;
+
+/* The grey two-stream radiation's planet properties (planet/cJupiterModel.h:1109-1118) and its
+ * runtime knobs, read by Radiation<P>::run under the ATJUP_ prefix (planet/Radiation.h:270-295):
+ * CIA_STRENGTH, OPACITY_STRENGTH, SOLAR, SOLAR_STRENGTH, SW_TAU_PER_BAR, RAD_BOTTOM. */
+typedef struct atjup_rad_params {
+    double F_int, S_solar, albedo_bond, x_H2, x_He;
+    double cia_strength, opacity_strength, solar_strength, sw_tau_per_bar;
+    int    solar, bottom;
+} atjup_rad_params;
 
 /* The saturation adjustment's constants of one gas, the arguments of
  * SaturationAdjustment<P>::run (planet/SaturationAdjustment.h:236-243) as the time loop passes them
@@ -165,6 +177,16 @@ int atjup_hip_satadj(atjup_ctx* ctx, int gas, const atjup_gas_params* params);
  * CH4 (planet/cJupiterModel.cpp:1060-1080). Until set, a gas's adjustment fails with
  * ATJUP_ERR_STATE. */
 int atjup_hip_set_gas(atjup_ctx* ctx, int gas, const atjup_gas_params* params);
+/* Radiation<cJupiterModel>::run (planet/Radiation.h:245-487, planet/RadiationJup.h), as the
+ * physics block calls it after computeMixtureDensity (planet/cJupiterModel.cpp:1056): the layer
+ * emissivity (epsilon), the layer-centre net thermal flux (radiation) and the radiative heating
+ * Q_rad of every column, from t, p_stat, rho_mix and the species. The printed column diagnostics
+ * (mean OLR, the tau = 1 level) are not formed. */
+int atjup_hip_radiation(atjup_ctx* ctx);
+/* The properties and knobs atjup_hip_radiation uses; a new context holds Jupiter's values and the
+ * knobs' defaults (atjup_hip_default_rad_params). */
+int atjup_hip_set_radiation(atjup_ctx* ctx, const atjup_rad_params* params);
+void atjup_hip_default_rad_params(atjup_rad_params* params);
 /* Turbulence<cJupiterModel>::run (planet/Turbulence.h:307-1106) followed by the wall-function
  * omega Run() applies after it (planet/cJupiterModel.cpp:356-374, 1144). */
 int atjup_hip_turbulence_run(atjup_ctx* ctx);
@@ -208,6 +230,7 @@ enum atjup_timer {
     ATJUP_T_ADJ_SWEEPS,           /* the transposed sweeps of one adjoint solve (2 launches per sweep) */
     ATJUP_T_ADJ_STEP,             /* one transposed loop iteration of atjup_hip_adj_backward */
     ATJUP_T_ADJ_RECOMPUTE,        /* the forward recomputation of one checkpoint segment */
+    ATJUP_T_RADIATION,            /* the radiation's two kernels (layers, columns) */
     ATJUP_N_TIMERS
 };
 int atjup_hip_set_timing(atjup_ctx* ctx, int enable);

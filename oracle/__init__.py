@@ -53,11 +53,12 @@ def _orc_lib():
                   "orc_bc_radius", "orc_bc_theta", "orc_bc_phi", "orc_bc_solid",
                   "orc_mixture_density", "orc_clamp_species", "orc_swap_nlevel",
                   "orc_scalar_sync", "orc_turbulence_run", "orc_turb_wall", "orc_flux_limiter",
-                  "orc_check_finite"):
+                  "orc_check_finite", "orc_radiation"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.orc_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_satadj.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_gas.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
+        L.orc_set_radiation.argtypes = [C.c_void_p, C.POINTER(abi.RadParams)]
         L.orc_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
@@ -118,6 +119,9 @@ class Oracle:
         self.set_mask(st.mask)
         for n, v in st.fields.items():
             self.set(n, v)
+
+    def set_radiation(self, params):
+        self._chk(self.L.orc_set_radiation(self.h, C.byref(params)), "set_radiation")
 
     def call(self, op, *args):
         return self._chk(getattr(self.L, "orc_" + op)(self.h, *args), op)

@@ -53,6 +53,7 @@ struct orc_ctx {
     std::vector<double> vel_star;                    // [j][k]
     std::vector<double> brl;                         // buoy_ref_level [i]
     atjup_gas_params gas[3] = {};                    // saturation-adjustment constants per gas
+    atjup_rad_params radp = {};                      // radiation properties and knobs
     double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
 
     size_t id(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
@@ -1520,6 +1521,84 @@ void flux_limiter(orc_ctx& o){
             }
 }
 
+// ---- Radiation<P>::run (Radiation.h:245-487): grey two-stream column solve ---------------------
+// For every (j, k): layer emission sigma T^4 and emissivity 1 - exp(-tau) with tau the CIA, CH4/NH3
+// band and capped cloud optical depths (:311-349), the Beer-Lambert shortwave beam from the top
+// (:369-386), the downward thermal sweep from a cold top (:391-393), the deep boundary
+// (:411-414), the upward sweep (:418-419), and per layer the centre net flux and the heating from
+// the flux divergence plus the absorbed shortwave (:434-463); solid cells below the column's base
+// copy its values with zero heating (:467-471). The printed diagnostics are not formed.
+void radiation(orc_ctx& o){
+    const atjup_rad_params& R = o.radp;
+    const int im = o.im, jm = o.jm, km = o.km;
+    const double sigma = 5.670374419e-8, P0 = 1.0e5, he_ratio = 0.6, C_cia = 3.5e-6;
+    const double kappa_ch4 = 0.003, kappa_nh3 = 1.5, kappa_cloud = 25.0, kappa_ice = 12.0;
+    const double tau_cloud_cap = 0.4, opac_cal = 0.025, pi_ = 3.14159265358979323846;
+    const double t_ref = o.c.t_ref, g = o.c.g;
+    const double comp = R.x_H2 * R.x_H2 + he_ratio * R.x_H2 * R.x_He;
+    const double cia_coeff = C_cia * R.cia_strength * comp / g;
+    const std::vector<double>& T_ = o.f[ATJUP_F_T];
+    const std::vector<double>& PS = o.f[ATJUP_F_P_STAT];
+    const std::vector<double>& RHO = o.f[ATJUP_F_RHO_MIX];
+    std::vector<double>& EPS = o.f[ATJUP_F_EPSILON];
+    std::vector<double>& RADF = o.f[ATJUP_F_RADIATION];
+    std::vector<double>& QR = o.f[ATJUP_F_Q_RAD];
+    auto sp = [&](int f, size_t q){ return std::max(0.0, o.f[f][q]); };
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++){
+            const int i_base = std::max(o.itopo[(size_t)j * km + k], 0), i_top = im - 1;
+            if(i_base >= i_top) continue;
+            std::vector<double> B(im, 0.0), eps(im, 0.0), Fd(im + 1, 0.0), Fu(im + 1, 0.0), Fsw(im + 2, 0.0);
+            for(int i = i_base; i <= i_top; i++){
+                const size_t q = o.id(i, j, k);
+                const double T = T_[q] * t_ref;
+                B[i] = sigma * T * T * T * T;
+                const double P_lo = PS[q] * P0;
+                const double P_hi = (i < i_top) ? PS[o.id(i + 1, j, k)] * P0 : 0.0;
+                double dP = P_lo - P_hi;
+                if(dP < 0.0) dP = 0.0;
+                const double P_mean = 0.5 * (P_lo + P_hi);
+                const double dm = dP / g;
+                const double tau_cia = (T > 0.0) ? cia_coeff * (P_mean / T) * dP : 0.0;
+                const double rho_c = RHO[q];
+                const double inv_rho = (rho_c > 0.0 && std::isfinite(rho_c)) ? 1.0 / rho_c : 0.0;
+                const double tau_gas = R.opacity_strength * opac_cal
+                    * (kappa_ch4 * (sp(ATJUP_F_CH4, q) * inv_rho) + kappa_nh3 * (sp(ATJUP_F_NH3, q) * inv_rho)) * dm;
+                const double q_liq = (sp(ATJUP_F_H2O_CLOUD, q) + sp(ATJUP_F_NH3_CLOUD, q) + sp(ATJUP_F_CH4_CLOUD, q)) * inv_rho;
+                const double q_ice = (sp(ATJUP_F_H2O_ICE, q) + sp(ATJUP_F_NH3_ICE, q) + sp(ATJUP_F_CH4_ICE, q)) * inv_rho;
+                double tau_cloud = R.opacity_strength * opac_cal * (kappa_cloud * q_liq + kappa_ice * q_ice) * dm;
+                if(tau_cloud > tau_cloud_cap) tau_cloud = tau_cloud_cap;
+                eps[i] = 1.0 - std::exp(-(tau_cia + tau_gas + tau_cloud));
+            }
+            if(R.solar != 0){
+                const double colat = pi_ * (double)j / (double)(jm - 1);
+                const double F_top = R.solar_strength * R.S_solar * (1.0 - R.albedo_bond)
+                                   * std::max(0.0, std::sin(colat)) / pi_;
+                for(int i = i_base; i <= i_top + 1; i++)
+                    Fsw[i] = F_top * std::exp(-R.sw_tau_per_bar * std::max(0.0, (i <= i_top) ? PS[o.id(i, j, k)] : 0.0));
+            }
+            for(int i = i_top; i >= i_base; i--) Fd[i] = Fd[i + 1] * (1.0 - eps[i]) + eps[i] * B[i];
+            Fu[i_base] = (R.bottom == 1) ? std::max(0.0, 2.0 * B[i_base] - Fd[i_base + 1])
+                                         : Fd[i_base] + R.F_int + Fsw[i_base];
+            for(int i = i_base; i <= i_top; i++) Fu[i + 1] = Fu[i] * (1.0 - eps[i]) + eps[i] * B[i];
+            for(int i = i_base; i <= i_top; i++){
+                const size_t q = o.id(i, j, k);
+                const double lo = Fu[i] - Fd[i], hi = Fu[i + 1] - Fd[i + 1];
+                double dz = o.layer_thickness_m(i < i_top ? i : i - 1);
+                if(dz <= 0.0) dz = 1.0;
+                EPS[q] = eps[i];
+                RADF[q] = 0.5 * (lo + hi);
+                QR[q] = (lo - hi) / dz + (Fsw[i + 1] - Fsw[i]) / dz;
+            }
+            for(int i = 0; i < i_base; i++){
+                EPS[o.id(i, j, k)] = EPS[o.id(i_base, j, k)];
+                RADF[o.id(i, j, k)] = RADF[o.id(i_base, j, k)];
+                QR[o.id(i, j, k)] = 0.0;
+            }
+        }
+}
+
 bool valid_field(int f){
     return (f >= 0 && f < ATJUP_N_PROG) || (f >= ATJUP_F_P_DYN && f < ATJUP_F_END)
         || (f >= ATJUP_NLEVEL && f < ATJUP_NLEVEL + ATJUP_N_PROG);
@@ -1572,6 +1651,8 @@ orc_ctx* orc_create(const atjup_grid* g, const atjup_consts* c, const atjup_knob
     o->itopo.assign((size_t)o->jm * o->km, 0);
     o->vel_star.assign((size_t)o->jm * o->km, 0.0);
     o->brl.assign(o->im, 0.0);
+    // Jupiter (cJupiterModel.h:1109-1118) and the radiation knobs' defaults (Radiation.h:270-295)
+    o->radp = atjup_rad_params{ 5.4, 50.5, 0.343, 0.863, 0.134, 3.9, 1.0, 1.0, 1.0, 1, 0 };
     return o;
 }
 
@@ -1666,6 +1747,12 @@ int orc_turb_wall(orc_ctx* o){
     return 0;
 }
 int orc_flux_limiter(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; flux_limiter(*o); return 0; }
+int orc_set_radiation(orc_ctx* o, const atjup_rad_params* p){
+    if(!o || !p) return ATJUP_ERR_ARG;
+    o->radp = *p;
+    return 0;
+}
+int orc_radiation(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; radiation(*o); return 0; }
 
 // One iteration of the loop body, on-path functions only (cJupiterModel.cpp:1010-1555).
 int orc_step(orc_ctx* o, int iter, int first){
@@ -1674,6 +1761,7 @@ int orc_step(orc_ctx* o, int iter, int first){
         pressure(*o, std::max(1, o->kn.press_sweeps), o->kn.press_omega);
         if(o->kn.damp_pdyn) shapiro(*o, o->f[ATJUP_F_P_DYN], 2, 1.0, 1, 7);
         mixture_density(*o);
+        if(o->kn.radiation) radiation(*o);                    // cJupiterModel.cpp:1056
         if(o->kn.satadj) for(int gas = 0; gas < 3; gas++) satadj(*o, gas);
         if(o->kn.rk_scalar_sync >= 1) scalar_sync(*o);
         if(o->kn.turb_model >= 0){ turbulence(*o); turb_wall_omega(*o); }

@@ -38,6 +38,8 @@ int      orc_set_gas(orc_ctx* o, int gas, const atjup_gas_params* p);
 int      orc_turb_wall(orc_ctx* o);
 int      orc_turbulence_run(orc_ctx* o);
 int      orc_flux_limiter(orc_ctx* o);
+int      orc_radiation(orc_ctx* o);
+int      orc_set_radiation(orc_ctx* o, const atjup_rad_params* p);
 int      orc_step(orc_ctx* o, int iter, int first);
 int      orc_check_finite(orc_ctx* o);
 void     orc_set_threads(int n);

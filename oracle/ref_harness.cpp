@@ -20,6 +20,7 @@
 #include "SaturationAdjustmentJup.h"
 #include "BC_Jup.h"
 #include "TurbulenceJup.h"
+#include "RadiationJup.h"
 
 #include <fcntl.h>
 #include <unistd.h>
@@ -62,6 +63,7 @@ public:
             {"S_precip_ch4_ice", &M.S_precip_ch4_ice},
             {"massflux_h2s", &M.massflux_h2s}, {"massflux_nh3", &M.massflux_nh3},
             {"massflux_nh4sh", &M.massflux_nh4sh}, {"fluxlim_nh4sh", &M.fluxlim_nh4sh},
+            {"epsilon", &M.epsilon}, {"radiation", &M.radiation},
         };
     }
 
@@ -178,6 +180,7 @@ public:
         else if(op == "fluxlim_damp")   JupiterUtils::damp_wiggles(M.fluxlim_nh4sh, &M.i_topography, true, true, true);
         else if(op == "turbulence")     { TurbulenceJup(M).run(); wall_omega(M); }
         else if(op == "turb_wall")      { TurbulenceJup(M).apply_wall_bc(); wall_omega(M); }
+        else if(op == "radiation")      RadiationJup(M).run();
         else if(op == "satadj_h2o")
             SaturationAdjustmentJup(M).run("H2O", M.t_0_h2o, M.t_00_h2o, M.ep_h2o, M.lv_h2o, M.ls_h2o,
                 M.C_h2o, M.L0_h2o, M.R_h2o, M.del_alf_h2o, M.del_bet_h2o,

@@ -32,3 +32,16 @@ def test_default_knobs_symbol():
     k = abi.Knobs()
     L.atjup_hip_default_knobs(ctypes.byref(k))
     assert k.press_omega == 1.0 and k.sinthe_min == 0.55 and k.rk_scalar_sync == 2
+
+
+def test_default_radiation_params_match_mirror():
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libatjup_hip.so not built")
+    L = lib.load()
+    p = abi.RadParams()
+    L.atjup_hip_default_rad_params(ctypes.byref(p))
+    q = abi.rad_params()
+    for name, _ in abi.RadParams._fields_:
+        assert getattr(p, name) == getattr(q, name), name
+    assert ctypes.sizeof(abi.RadParams) == 9 * 8 + 2 * 4
+    assert abi.FIELD["epsilon"] == 47 and abi.FIELD["radiation"] == 48 and abi.F_END == 49

@@ -126,14 +126,16 @@ def _forward(c, op, iarg, darg):
         c.step(iarg, bool(darg))
 
 
-# a ring of 50 meridians: 25 compact cells per colour and row, an odd count, which the transposed
-# colour pass covers one cell per thread (even counts go two cells per thread)
-ODD_RING = [("pressure", 6, 1.0, ("p_dyn", "aux_u", "aux_v", "aux_w"), ("p_dyn", "aux_u", "aux_v", "aux_w")),
-            ("step", 2, 0.0, STATE, STATE)]
+# The transposed colour pass covers a row two cells per thread, flattened over (row, pair), on the
+# 48-meridian shell above (12 pairs per row); one cell per thread where row-aligned pairs would
+# fill the wavefronts (256 meridians: 64 pairs) and on an odd count (50 meridians: 25 cells).
+OTHER_RINGS = [(km, op) for km in (51, 257) for op in (
+    ("pressure", 6, 1.0, ("p_dyn", "aux_u", "aux_v", "aux_w"), ("p_dyn", "aux_u", "aux_v", "aux_w")),
+    ("step", 2, 0.0, STATE, STATE))]
 
 
-def _odd_ring_grid():
-    return inputs.Grid(im=13, jm=33, km=51, the_degree=180.0 / 32, phi_degree=360.0 / 50)
+def _ring_grid(km):
+    return inputs.Grid(im=13, jm=33, km=km, the_degree=180.0 / 32, phi_degree=360.0 / (km - 1))
 
 
 @pytest.mark.parametrize("op,iarg,darg,ins,outs", CASES, ids=[f"{c[0]}-{c[1]}-{c[2]}" for c in CASES])
@@ -141,9 +143,9 @@ def test_dot_product(op, iarg, darg, ins, outs):
     _dot_product(op, iarg, darg, ins, outs, None)
 
 
-@pytest.mark.parametrize("op,iarg,darg,ins,outs", ODD_RING, ids=[f"{c[0]}-{c[1]}-{c[2]}" for c in ODD_RING])
-def test_dot_product_odd_ring(op, iarg, darg, ins, outs):
-    _dot_product(op, iarg, darg, ins, outs, _odd_ring_grid())
+@pytest.mark.parametrize("km,case", OTHER_RINGS, ids=[f"km{k}-{c[0]}-{c[1]}" for k, c in OTHER_RINGS])
+def test_dot_product_other_rings(km, case):
+    _dot_product(*case, _ring_grid(km))
 
 
 def _dot_product(op, iarg, darg, ins, outs, g):

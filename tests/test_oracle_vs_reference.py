@@ -90,3 +90,38 @@ def test_oracle_full_physics_bit_exact_against_compiled_reference():
     r = subprocess.run([sys.executable, "-c", SCRIPT_FULL % {"root": ROOT}], capture_output=True, text=True,
                        timeout=900)
     assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+# Radiation<P>::run on the moist state with the obstacle, at the knob defaults and with every knob
+# moved (the reference reads them from the environment on each call).
+SCRIPT_RAD = r'''
+import os, sys, numpy as np
+sys.path.insert(0, %(root)r)
+from atjup_amd import abi, inputs
+import oracle
+G, Cn, K, env = abi.make_profile("full", sweeps=8)
+st = inputs.make_state("full", seed=11)
+ref = oracle.Reference(env); ref.load_state(st, Cn)
+orc = oracle.Oracle(G, Cn, K); orc.load_state(st)
+ref.call("mixture_density"); orc.call("mixture_density")
+cases = [{}, dict(solar=0, bottom=1, cia_strength=2.5),
+         dict(opacity_strength=3.0, solar_strength=0.5, sw_tau_per_bar=2.0)]
+for over in cases:
+    for v in abi.RAD_ENV.values(): os.environ.pop(v, None)
+    os.environ.update({abi.RAD_ENV[k]: repr(v) for k, v in over.items()})
+    orc.set_radiation(abi.rad_params(**over))
+    ref.call("radiation"); orc.call("radiation")
+    for n in ("epsilon", "radiation", "Q_rad"):
+        a, b = ref.get(n), orc.get(n)
+        if not np.array_equal(a, b):
+            print("MISMATCH", over, n, np.max(np.abs(a - b))); sys.exit(1)
+        if np.count_nonzero(b) < b.size // 2:
+            print("MOSTLY ZERO", over, n); sys.exit(1)
+print("ALL-EQUAL")
+'''
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_radiation_bit_exact_against_compiled_reference():
+    r = subprocess.run([sys.executable, "-c", SCRIPT_RAD % {"root": ROOT}], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
