@@ -20,7 +20,9 @@ INPUTS = ("p_dyn", "p_stat", "p_hydro", "rho_mix", "wall_nue", "nue", "prod", "t
           "S_precip_nh3", "S_precip_nh3_cloud", "S_precip_nh3_ice",
           "S_precip_ch4", "S_precip_ch4_cloud", "S_precip_ch4_ice",
           "massflux_h2s", "massflux_nh3", "massflux_nh4sh", "fluxlim_nh4sh",
-          "epsilon", "radiation")
+          "epsilon", "radiation",
+          "P_rain", "P_snow", "P_graupel", "P_nh3_rain", "P_nh3_snow", "P_nh3_graupel",
+          "P_ch4_rain", "P_ch4_snow", "P_ch4_graupel", "P_nh4sh")
 NLEVEL = 64
 
 FIELD = {n: i for i, n in enumerate(PROG)}
@@ -56,7 +58,22 @@ class Knobs(C.Structure):
                 ("tke_max_phys", C.c_double), ("nue_max_phys", C.c_double),
                 ("wall_nue_factor", C.c_double), ("wall_nue_layers", C.c_int),
                 ("satadj", C.c_int), ("rk_scalar_sync", C.c_int), ("dry", C.c_int),
-                ("turb_k_bg_phys", C.c_double), ("radiation", C.c_int)]
+                ("turb_k_bg_phys", C.c_double), ("radiation", C.c_int), ("precip", C.c_int)]
+
+
+class PrecipParams(C.Structure):
+    """atjup_precip_params: ATJUP_PRECIP_SCALE and the NH4SH crystal band [K]."""
+    _fields_ = [("rate_scale", C.c_double), ("t_0_nh4sh", C.c_double), ("t_00_nh4sh", C.c_double)]
+
+
+def precip_params(**over) -> PrecipParams:
+    d = dict(rate_scale=1.0, t_0_nh4sh=230.0, t_00_nh4sh=200.0)
+    d.update(over)
+    return PrecipParams(**d)
+
+
+# the surface precipitation maps atjup_hip_download_precip_surface returns, in its order
+PRECIP_SURFACE = ("precip_srf_h2o", "precip_srf_nh3", "precip_srf_ch4", "precip_srf_nh4sh", "precip_srf_total")
 
 
 class RadParams(C.Structure):
@@ -121,7 +138,8 @@ TURB_MODELS = {"off": -1, "k_epsilon": 0, "k_omega": 1, "k_omega_SST": 2}
 
 
 def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
-                 grid: inputs.Grid | None = None, radiation: bool = False):
+                 grid: inputs.Grid | None = None, radiation: bool = False, precip: bool = False,
+                 precip_coupling: float = 1.0):
     """(Grid, Consts, Knobs, env) of a configuration.
 
     "dyn"  : dynamics only, dry, closure/radiation/precipitation off (the 5-step benchmark case).
@@ -129,6 +147,8 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
              inputs (their modules are outside this path).
     radiation: the grey radiation runs in the physics block (ATJUP_RADIATION=1, the reference's
              default), so Q_rad follows the state instead of entering as a frozen input.
+    precip:  the precipitation microphysics runs after the saturation adjustments (ATJUP_PRECIP=1,
+             the reference's default); with precip_coupling != 0 it reports S_precip_* rates.
     """
     g = grid or inputs.Grid()
     G = Grid(im=g.im, jm=g.jm, km=g.km, r0=g.radius_km / g.L_atm, dr=g.dr, dthe=g.dthe,
@@ -142,14 +162,16 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
     turb = "off" if config == "dyn" else "k_omega"
     K = Knobs(turb_model=TURB_MODELS[turb], press_sweeps=sweeps, press_omega=omega,
               pdyn_cap=0.2, damp_pdyn=1, rho_pdyn=1.0, sinthe_min=0.55, t_diff_scale=0.01,
-              species_diff_scale=0.01, rad_coupling=1.0, precip_coupling=1.0, turb_coupling=1.0,
+              species_diff_scale=0.01, rad_coupling=1.0, precip_coupling=precip_coupling, turb_coupling=1.0,
               tke_max_phys=1000.0, nue_max_phys=1.0e5, wall_nue_factor=4.0, wall_nue_layers=3,
               satadj=1, rk_scalar_sync=2, dry=1 if config == "dyn" else 0,
-              turb_k_bg_phys=1.0e-4 if config != "dyn" else 1.0, radiation=1 if radiation else 0)
+              turb_k_bg_phys=1.0e-4 if config != "dyn" else 1.0, radiation=1 if radiation else 0,
+              precip=1 if precip else 0)
     env = dict(BASE_ENV)
     env["ATJUP_PRESS_SWEEPS"] = str(sweeps)
     env["ATJUP_RADIATION"] = "1" if radiation else "0"
-    env["ATJUP_PRECIP"] = "0"
+    env["ATJUP_PRECIP"] = "1" if precip else "0"
+    env["ATJUP_PRECIP_COUPLING"] = repr(float(precip_coupling))
     if config == "dyn":
         env["ATJUP_TURB"] = "0"
     else:

@@ -127,6 +127,10 @@ int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, i
     if(!(k->pdyn_cap > 0.0)){ set_error("pdyn_cap"); return ATJUP_ERR_PROFILE; }
     if(!(k->press_omega > 0.0 && k->press_omega < 2.0)){ set_error("press_omega outside (0, 2)"); return ATJUP_ERR_PROFILE; }
     if(k->radiation != 0 && k->radiation != 1){ set_error("radiation must be 0 or 1"); return ATJUP_ERR_PROFILE; }
+    if(k->precip != 0 && k->precip != 1){ set_error("precip must be 0 or 1"); return ATJUP_ERR_PROFILE; }
+    if(k->precip && precision == ATJUP_PRECISION_TANGENT){
+        set_error("the tangent form has no precipitation"); return ATJUP_ERR_PROFILE;
+    }
     if(k->radiation && precision == ATJUP_PRECISION_TANGENT){
         set_error("the tangent form has no radiation"); return ATJUP_ERR_PROFILE;
     }
@@ -208,6 +212,7 @@ int step_block(atjup_ctx* c, int iter, int first){
         // (cJupiterModel.cpp:1060-1161)
         for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
             if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
+        if(c->kn.precip && (rc = atjup_hip_precipitation(c))) return rc;      // cJupiterModel.cpp:1090
         if((rc = atjup_hip_turbulence_run(c))) return rc;
         if((rc = launch_flux_limiter(c))) return rc;
         if((rc = launch_shapiro(c, ATJUP_F_FLUXLIM_NH4SH, 2, 1.0, 1, 7))) return rc;
@@ -297,6 +302,7 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     if(!rc) rc = alloc(c->rb_p, 2 * hn * asz);
     if(!rc && precision == ATJUP_PRECISION_FP32_REF)
         for(int f = 0; f < 4 && !rc; f++) rc = alloc(c->lo[f], c->n3 * sizeof(float));
+    if(!rc && precision == ATJUP_PRECISION_FP32_REF) rc = alloc(c->rad_tmp, 3 * c->n3 * sizeof(double));
     if(!rc && precision == ATJUP_PRECISION_FP32_REF){
         rc = alloc(c->rb_d, 2 * hn * sizeof(float));
         if(!rc) rc = alloc(c->rb_r, 2 * hn * sizeof(float));
@@ -365,6 +371,10 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
         atjup_hip_default_rad_params(&c->radp);
         rc = build_rad_table(c);
     }
+    if(!rc){
+        atjup_hip_default_precip_params(&c->precp);
+        rc = alloc(c->precip_srf, (size_t)5 * c->jm * c->km * sizeof(double));
+    }
     if(rc){ atjup_hip_ctx_destroy(c); return nullptr; }
     return c;
 }
@@ -380,7 +390,7 @@ void atjup_hip_ctx_destroy(atjup_ctx* c){
     for(atjup::Buf* b : { &c->lo[0], &c->lo[1], &c->lo[2], &c->lo[3], &c->colr }) release(*b);
     for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
-                          &c->amb_rows, &c->nonfinite, &c->rad_tab })
+                          &c->amb_rows, &c->nonfinite, &c->rad_tab, &c->rad_tmp, &c->precip_srf })
         release(*b);
     for(auto& v : c->spans) for(auto& sp : v){ (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
     for(auto e : c->event_pool) (void)hipEventDestroy(e);
@@ -562,6 +572,28 @@ int atjup_hip_radiation(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;
     if((size_t)(c->im + 1) * 64 * sizeof(double) > 65536){ set_error("radiation: im above 1023 levels"); return ATJUP_ERR_PROFILE; }
     return launch_radiation(c);
+}
+void atjup_hip_default_precip_params(atjup_precip_params* p){
+    if(!p) return;
+    // ATJUP_PRECIP_SCALE 1 (planet/cJupiterModel.h:1059-1061), the NH4SH band 200-230 K (:194-195)
+    p->rate_scale = 1.0; p->t_0_nh4sh = 230.0; p->t_00_nh4sh = 200.0;
+}
+int atjup_hip_set_precipitation(atjup_ctx* c, const atjup_precip_params* p){
+    if(!c || !p){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    c->precp = *p;
+    return 0;
+}
+int atjup_hip_precipitation(atjup_ctx* c){
+    if(int rc = need_mask(c)) return rc;
+    for(int gas = 0; gas < 3; gas++)
+        if(!c->gas_set[gas]){ set_error("gas constants not set (atjup_hip_set_gas)"); return ATJUP_ERR_STATE; }
+    return launch_precipitation(c);
+}
+int atjup_hip_download_precip_surface(atjup_ctx* c, double* dst){
+    if(!c || !dst){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    ATJUP_HIP_CHECK(hipMemcpy(dst, c->precip_srf.p, c->precip_srf.bytes, hipMemcpyDeviceToHost));
+    return 0;
 }
 int atjup_hip_turbulence_run(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;

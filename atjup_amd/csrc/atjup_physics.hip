@@ -1189,9 +1189,11 @@ int do_commit_live(atjup_ctx* c){
 //   k_rad_columns  one thread per column: the two recurrences (downward flux top to bottom, the
 //                  deep boundary, upward flux bottom to top), the layer-centre net flux and the
 //                  heating Q_rad, and the copy into the solid cells below i_base.
-// B and Fsw travel from the first kernel to the second in the Q_rad and radiation arrays, which
-// the second kernel overwrites with the results after reading them; Fd of the column is held in
-// LDS. Columns with i_base >= im-1 are left untouched, as the reference skips them.
+// eps, B and Fsw travel from the first kernel to the second in FP64: in the FP64 form through the
+// epsilon, Q_rad and radiation arrays themselves (the second kernel overwrites Q_rad and radiation
+// with the results after reading them), in the FP32-refined form through an FP64 buffer of three
+// fields; Fd of the column is held in LDS. Columns with i_base >= im-1 are left untouched, as the
+// reference skips them.
 // =============================================================================================
 struct RadConsts {
     double t_ref, g, cia_coeff, oc, sw_tau_bar, F_int;
@@ -1208,6 +1210,7 @@ struct RadFields {
     const T *t, *p_stat, *rho, *ch4, *nh3, *cl[3], *ic[3];
     const float* lo_t;                     // FP32-refined form: t's low word
     T *eps, *rad, *qrad;
+    double *e64, *b64, *s64;               // eps, B, Fsw between the two kernels
 };
 
 template<typename T>
@@ -1241,9 +1244,9 @@ __global__ __launch_bounds__(RAD_BK) void k_rad_layers(const RadFields<T> F, con
     double tau_cloud = C.oc * (RAD_KAPPA_CLOUD * q_liq + RAD_KAPPA_ICE * q_ice) * dm;
     if(tau_cloud > RAD_TAU_CLOUD_CAP) tau_cloud = RAD_TAU_CLOUD_CAP;
     const double tau = tau_cia + tau_gas + tau_cloud;
-    F.eps[q] = (T)(1.0 - exp(-tau));
-    F.qrad[q] = (T)B;
-    F.rad[q] = (T)(fsun[j] * exp(-C.sw_tau_bar * dmax(0.0, (double)F.p_stat[q])));
+    F.e64[q] = 1.0 - exp(-tau);
+    F.b64[q] = B;
+    F.s64[q] = fsun[j] * exp(-C.sw_tau_bar * dmax(0.0, (double)F.p_stat[q]));
 }
 
 template<typename T>
@@ -1262,27 +1265,28 @@ __global__ __launch_bounds__(RAD_BK) void k_rad_columns(const RadFields<T> F, co
     double d = 0.0;
     fd[(i_top + 1) * RAD_BK] = 0.0;
     for(int i = i_top; i >= i_base; i--){
-        const double e = (double)F.eps[q0 + i * P], b = (double)F.qrad[q0 + i * P];
+        const double e = F.e64[q0 + i * P], b = F.b64[q0 + i * P];
         d = d * (1.0 - e) + e * b;
         fd[i * RAD_BK] = d;
     }
     // the deep boundary: net upward F_int plus the shortwave left at depth, or (bottom = 1) no
     // divergence in the bottom layer
-    double fsw = (double)F.rad[q0 + i_base * P];
+    double fsw = F.s64[q0 + i_base * P];
     double u;
-    if(C.bottom == 1) u = dmax(0.0, 2.0 * (double)F.qrad[q0 + i_base * P] - fd[(i_base + 1) * RAD_BK]);
+    if(C.bottom == 1) u = dmax(0.0, 2.0 * F.b64[q0 + i_base * P] - fd[(i_base + 1) * RAD_BK]);
     else u = fd[i_base * RAD_BK] + C.F_int + fsw;
     // upward sweep with the layer results
     const double fsw_top = fsun[j] * exp(-C.sw_tau_bar * dmax(0.0, 0.0));
     for(int i = i_base; i <= i_top; i++){
         const size_t q = q0 + i * P;
-        const double e = (double)F.eps[q], b = (double)F.qrad[q];
+        const double e = F.e64[q], b = F.b64[q];
         const double u_next = u * (1.0 - e) + e * b;
         const double net_bot = u - fd[i * RAD_BK];
         const double net_top = u_next - fd[(i + 1) * RAD_BK];
         const double dz = dz_tab[i];
-        const double fsw_next = (i < i_top) ? (double)F.rad[q + P] : fsw_top;
+        const double fsw_next = (i < i_top) ? F.s64[q + P] : fsw_top;
         const double Q_sw = (fsw_next - fsw) / dz;
+        if constexpr(!std::is_same<T, double>::value) F.eps[q] = (T)e;
         F.rad[q] = (T)(0.5 * (net_bot + net_top));
         F.qrad[q] = (T)((net_bot - net_top) / dz + Q_sw);
         u = u_next;
@@ -1307,6 +1311,11 @@ int do_radiation(atjup_ctx* c){
     Fp.ic[0] = F<T>(c, ATJUP_F_H2O_ICE); Fp.ic[1] = F<T>(c, ATJUP_F_NH3_ICE); Fp.ic[2] = F<T>(c, ATJUP_F_CH4_ICE);
     Fp.lo_t = (const float*)c->lo[ATJUP_F_T].p;
     Fp.eps = F<T>(c, ATJUP_F_EPSILON); Fp.rad = F<T>(c, ATJUP_F_RADIATION); Fp.qrad = F<T>(c, ATJUP_F_Q_RAD);
+    if constexpr(std::is_same<T, double>::value){
+        Fp.e64 = Fp.eps; Fp.b64 = Fp.qrad; Fp.s64 = Fp.rad;
+    } else {
+        Fp.e64 = (double*)c->rad_tmp.p; Fp.b64 = Fp.e64 + c->n3; Fp.s64 = Fp.b64 + c->n3;
+    }
     // Radiation.h:272-280: the collision-pair factor, the CIA coefficient, the opacity calibration
     const double he_ratio = 0.6, C_cia = 3.5e-6, opac_cal = 0.025;
     const double comp = R.x_H2 * R.x_H2 + he_ratio * R.x_H2 * R.x_He;
@@ -1324,6 +1333,182 @@ int do_radiation(atjup_ctx* c){
     return check_launch();
 }
 
+// =============================================================================================
+// Precipitation<P>::run (Precipitation.h:194-574): one thread per (j, k) column does, in the
+// reference's order per cell,
+//   * the zeroing of Q_precip and the nine S_precip_* rates over the whole column (:199-210);
+//   * the three-category column of H2O, NH3 and CH4 (column(), :273-450): a top-down sweep from
+//     i = im-2 to the column's base with the rain, snow and graupel fluxes arriving from above
+//     in registers, the autoconversion, accretion, riming, snow-to-graupel, melt/freeze handoff
+//     and the capped rain evaporation, the latent heat added into Q_precip gas by gas, the
+//     depletion as rates (coupling on) or in place (off), and the fluxes capped at 1e-4;
+//   * the NH4SH crystal settling flux in its temperature band (sedimentNH4SH, :523-574);
+//   * the surface maps at the column's base (surfaceMap, :470-492).
+// The three gases are independent recurrences down the column, so one thread carries all three.
+// =============================================================================================
+struct PrecipGas {
+    double C, L0, R, da, db, ep, Lv, Ls, t_frz, t_low;
+};
+struct PrecipConsts {
+    PrecipGas gas[3];
+    double t_ref, r_mix, rs, nd, v_settle, t_hi_nh4sh, t_lo_nh4sh;
+    int coupling;
+};
+template<typename T>
+struct PrecipFields {
+    const T *t, *p_stat, *rho, *nh4sh;
+    const float* lo_t;
+    T *vap[3], *cld[3], *ice[3];
+    T *P[3][3];                    // [gas][rain, snow, graupel]
+    T *S[3][3];                    // [gas][vapour, cloud, ice]
+    T *Q, *P_nh4sh;
+    double* srf;                   // [5][jm][km]
+};
+constexpr double PR_C_C_AU = 2.0e-8, PR_C_I_AU = 5.0e-8, PR_Q_C_CRIT = 5.0e-4, PR_C_AC = 1.0e-7;
+constexpr double PR_C_RIM = 1.0e-8, PR_Z_CSG = 2.5e-8, PR_A_EV = 2.76e-3, PR_C_MELT_K = 0.2;
+constexpr double PR_C_FRZ_K = 0.2, PR_P_REF = 1.0e-4, PR_RATIO_MAX = 5.0, PR_V_FALL = 2.0;
+constexpr double PR_P_MAX_FLUX = 1.0e-4, PR_DEP_FRAC = 1.0;          // Precipitation.h:39-85, :277-290
+
+template<typename T>
+__global__ __launch_bounds__(64) void k_precip(const PrecipFields<T> F, const int* __restrict__ itopo,
+        const double* __restrict__ dz_tab, int im, int jm, int km, int pitch, const PrecipConsts C){
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y;
+    if(k >= km) return;
+    const size_t P = (size_t)jm * pitch, q0 = (size_t)j * pitch + k;
+    const int i_base = max(itopo[(size_t)j * km + k], 0), i_top = im - 1;
+    const bool live = i_base < i_top;                   // column() skips the others
+    double Fr[3] = {0.0, 0.0, 0.0}, Fs[3] = {0.0, 0.0, 0.0}, Fg[3] = {0.0, 0.0, 0.0};
+    for(int i = i_top; i >= 0; i--){
+        const size_t q = q0 + (size_t)i * P;
+        double Qp = 0.0, Sv[3] = {0.0, 0.0, 0.0}, Sc[3] = {0.0, 0.0, 0.0}, Si[3] = {0.0, 0.0, 0.0};
+        if(live){
+            const double dz = dmax(0.0, dz_tab[i]);
+            if(i == i_top || i < i_base || dz <= 0.0){
+                // the top's zero fluxes; below the base and across a zero-thickness layer the
+                // fluxes of the cell above (:300-303, :316-321, :444-448)
+                if(i == i_top) for(int g = 0; g < 3; g++){ Fr[g] = 0.0; Fs[g] = 0.0; Fg[g] = 0.0; }
+                for(int g = 0; g < 3; g++){
+                    F.P[g][0][q] = (T)Fr[g]; F.P[g][1][q] = (T)Fs[g]; F.P[g][2][q] = (T)Fg[g];
+                }
+            } else {
+                double tq = (double)F.t[q];
+                if(F.lo_t) tq += (double)F.lo_t[q];
+                const double Tk = tq * C.t_ref;
+                const double p_u = (double)F.p_stat[q];
+                const double rf = (double)F.rho[q];
+                const double rho_at = (rf > 0.0 && isfinite(rf)) ? rf : C.r_mix;
+                for(int g = 0; g < 3; g++){
+                    const PrecipGas& G = C.gas[g];
+                    const double Lf = G.Ls - G.Lv;
+                    const double q_v = dmax(0.0, (double)F.vap[g][q]);
+                    const double q_c = dmax(0.0, (double)F.cld[g][q]);
+                    const double q_i = dmax(0.0, (double)F.ice[g][q]);
+                    const double E_r = exp(G.C + (-G.L0 / Tk + G.da * log(Tk) + G.db * Tk) / (1e-3 * G.R));
+                    const double q_sat = (p_u > E_r) ? rho_at * G.ep * E_r / p_u : q_v;
+                    const double F_r_in = Fr[g], F_s_in = Fs[g], F_g_in = Fg[g];
+                    const double Rain = dmin(PR_RATIO_MAX, F_r_in / PR_P_REF);
+                    const double Snow = dmin(PR_RATIO_MAX, F_s_in / PR_P_REF);
+                    const double Graupel = dmin(PR_RATIO_MAX, F_g_in / PR_P_REF);
+                    const double Rain_79 = (Rain > 0.0) ? pow(Rain, 7.0 / 9.0) : 0.0;
+                    const double Rain_49 = (Rain > 0.0) ? pow(Rain, 4.0 / 9.0) : 0.0;
+                    const bool warm = (Tk >= G.t_frz);
+                    const double S_c_au = (warm && q_c > PR_Q_C_CRIT) ? (C.rs * PR_C_C_AU) * (q_c - PR_Q_C_CRIT) : 0.0;
+                    const double S_i_au = (!warm && Tk >= G.t_low) ? (C.rs * PR_C_I_AU) * q_i : 0.0;
+                    const double S_ac = warm ? (C.rs * PR_C_AC) * q_c * Rain_79 : 0.0;
+                    const double S_s_rim = !warm ? (C.rs * PR_C_RIM) * q_c * Snow : 0.0;
+                    const double S_g_rim = !warm ? (C.rs * PR_C_RIM) * q_c * Graupel : 0.0;
+                    const double S_csg = (!warm && q_c > PR_Q_C_CRIT) ? (C.rs * PR_Z_CSG) * q_c * Snow : 0.0;
+                    const double f_melt = warm ? dmin(1.0, PR_C_MELT_K * (Tk - G.t_frz)) : 0.0;
+                    const double f_frz = warm ? 0.0 : dmin(1.0, PR_C_FRZ_K * (G.t_frz - Tk));
+                    const double F_s_melt = f_melt * F_s_in, F_g_melt = f_melt * F_g_in, F_r_frz = f_frz * F_r_in;
+                    const double F_r_avail = dmax(0.0, F_r_in - F_r_frz + F_s_melt + F_g_melt + (S_c_au + S_ac) * dz);
+                    const double S_ev_raw = (warm && Rain > 0.0 && q_v < q_sat) ? PR_A_EV * (q_sat - q_v) * Rain_49 : 0.0;
+                    const double F_ev_sat = dmax(0.0, q_sat - q_v) * PR_V_FALL;
+                    const double F_ev = dmin(dmin(S_ev_raw * dz, F_r_avail), F_ev_sat);
+                    const double S_ev = F_ev / dz;
+                    Qp += Lf * (S_s_rim + S_g_rim + S_csg) + Lf * (F_r_frz - F_s_melt - F_g_melt) / dz - G.Lv * F_ev / dz;
+                    const double tau = dz / PR_V_FALL;
+                    const double d_c = dmin(PR_DEP_FRAC * q_c, (S_c_au + S_ac + S_s_rim + S_g_rim + S_csg) * tau);
+                    const double d_i = dmin(PR_DEP_FRAC * q_i, S_i_au * tau);
+                    if(C.coupling){
+                        Sc[g] = -dmax(0.0, d_c) / tau * C.nd;
+                        Si[g] = -dmax(0.0, d_i) / tau * C.nd;
+                        Sv[g] = +S_ev * C.nd;
+                    } else {
+                        F.cld[g][q] = (T)(q_c - dmax(0.0, d_c));
+                        F.ice[g][q] = (T)(q_i - dmax(0.0, d_i));
+                        F.vap[g][q] = (T)(q_v + S_ev * tau);
+                    }
+                    const double F_r = F_r_in - F_r_frz + F_s_melt + F_g_melt - F_ev + (S_c_au + S_ac) * dz;
+                    const double F_s = F_s_in - F_s_melt + (S_i_au + S_s_rim - S_csg) * dz;
+                    const double F_g = F_g_in - F_g_melt + F_r_frz + (S_g_rim + S_csg) * dz;
+                    Fr[g] = dmin(PR_P_MAX_FLUX, dmax(0.0, F_r));
+                    Fs[g] = dmin(PR_P_MAX_FLUX, dmax(0.0, F_s));
+                    Fg[g] = dmin(PR_P_MAX_FLUX, dmax(0.0, F_g));
+                    F.P[g][0][q] = (T)Fr[g]; F.P[g][1][q] = (T)Fs[g]; F.P[g][2][q] = (T)Fg[g];
+                }
+            }
+        }
+        F.Q[q] = (T)Qp;
+        for(int g = 0; g < 3; g++){ F.S[g][0][q] = (T)Sv[g]; F.S[g][1][q] = (T)Sc[g]; F.S[g][2][q] = (T)Si[g]; }
+    }
+    // NH4SH crystal settling in its band, the solid cells below taking the base's flux
+    for(int i = i_base; i < im; i++){
+        const size_t q = q0 + (size_t)i * P;
+        double tq = (double)F.t[q];
+        if(F.lo_t) tq += (double)F.lo_t[q];
+        const double Tk = tq * C.t_ref;
+        const double qn = dmax(0.0, (double)F.nh4sh[q]);
+        F.P_nh4sh[q] = (T)((Tk >= C.t_lo_nh4sh && Tk <= C.t_hi_nh4sh) ? qn * C.v_settle : 0.0);
+    }
+    const T base_flux = (i_base < im) ? F.P_nh4sh[q0 + (size_t)i_base * P] : T(0);
+    for(int i = 0; i < i_base && i < im; i++) F.P_nh4sh[q0 + (size_t)i * P] = base_flux;
+    // the surface maps at the base (a fully solid column reads its top cell)
+    const size_t qb = q0 + (size_t)min(i_base, im - 1) * P;
+    double srf[3];
+    for(int g = 0; g < 3; g++) srf[g] = (double)F.P[g][0][qb] + (double)F.P[g][1][qb] + (double)F.P[g][2][qb];
+    const double nh4 = (double)F.P_nh4sh[qb];
+    const size_t n2 = (size_t)jm * km, c2 = (size_t)j * km + k;
+    F.srf[c2] = srf[0];
+    F.srf[n2 + c2] = srf[1];
+    F.srf[2 * n2 + c2] = srf[2];
+    F.srf[3 * n2 + c2] = nh4;
+    F.srf[4 * n2 + c2] = srf[0] + srf[1] + srf[2] + nh4;
+}
+
+template<typename T>
+int do_precipitation(atjup_ctx* c){
+    static const int vap[3] = { ATJUP_F_H2O, ATJUP_F_NH3, ATJUP_F_CH4 };
+    static const int pf[3] = { ATJUP_F_P_RAIN, ATJUP_F_P_NH3_RAIN, ATJUP_F_P_CH4_RAIN };
+    static const int sf[3] = { ATJUP_F_S_PRECIP_H2O, ATJUP_F_S_PRECIP_NH3, ATJUP_F_S_PRECIP_CH4 };
+    PrecipFields<T> Fp;
+    Fp.t = F<T>(c, ATJUP_F_T); Fp.p_stat = F<T>(c, ATJUP_F_P_STAT); Fp.rho = F<T>(c, ATJUP_F_RHO_MIX);
+    Fp.nh4sh = F<T>(c, ATJUP_F_NH4SH); Fp.lo_t = (const float*)c->lo[ATJUP_F_T].p;
+    PrecipConsts C;
+    for(int g = 0; g < 3; g++){
+        Fp.vap[g] = F<T>(c, vap[g]); Fp.cld[g] = F<T>(c, vap[g] + 1); Fp.ice[g] = F<T>(c, vap[g] + 2);
+        for(int r = 0; r < 3; r++){ Fp.P[g][r] = F<T>(c, pf[g] + r); Fp.S[g][r] = F<T>(c, sf[g] + r); }
+        const atjup_gas_params& a = c->gas[g];
+        C.gas[g] = PrecipGas{ a.C, a.L0, a.R, a.del_alf, a.del_bet, a.ep, a.lv, a.ls, a.t_0, a.t_00 };
+    }
+    Fp.Q = F<T>(c, ATJUP_F_Q_PRECIP); Fp.P_nh4sh = F<T>(c, ATJUP_F_P_NH4SH);
+    Fp.srf = (double*)c->precip_srf.p;
+    const atjup_precip_params& pp = c->precp;
+    // Stokes settling of the crystals (Precipitation.h:530-538): r_p 1e-5 m, 1200 kg/m3, mu 9e-6 Pa s
+    const double r_p = 1.0e-5, rho_c = 1200.0, mu = 9.0e-6;
+    C.t_ref = c->c.t_ref; C.r_mix = c->c.r_mix; C.rs = pp.rate_scale;
+    C.nd = (c->g.L_atm * 1.0e3) / c->c.u_0;
+    C.v_settle = (2.0 / 9.0) * r_p * r_p * (rho_c - c->c.r_mix) * c->c.g / mu;
+    C.t_hi_nh4sh = pp.t_0_nh4sh; C.t_lo_nh4sh = pp.t_00_nh4sh;
+    C.coupling = (c->kn.precip_coupling != 0.0) ? 1 : 0;
+    const double* dz = (const double*)c->tabs.p + c->im + 3 * c->jm;    // layer_thickness_m
+    hipEvent_t t0 = timer_begin(c);
+    hipLaunchKernelGGL(k_precip<T>, dim3(nblk(c->km, 64), c->jm), dim3(64), 0, c->stream,
+        Fp, (const int*)c->itopo.p, dz, c->im, c->jm, c->km, c->pitch, C);
+    timer_end(c, ATJUP_T_PRECIPITATION, t0, 1);
+    return check_launch();
+}
+
 }  // namespace
 
 // the tangent form covers the dynamics profile only
@@ -1338,6 +1523,7 @@ int launch_turb_sanitize(atjup_ctx* c){ return DISPATCH(do_turb_sanitize, c); }
 int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){ return DISPATCH(do_satadj, c, gas, g); }
 int launch_flux_limiter(atjup_ctx* c){ return DISPATCH(do_flux_limiter, c); }
 int launch_radiation(atjup_ctx* c){ return DISPATCH(do_radiation, c); }
+int launch_precipitation(atjup_ctx* c){ return DISPATCH(do_precipitation, c); }
 int launch_commit_live(atjup_ctx* c){ return c->turb_live_pending ? DISPATCH(do_commit_live, c) : 0; }
 
 }  // namespace atjup

@@ -50,7 +50,7 @@ def load():
               "atjup_hip_bc_after_step",
               "atjup_hip_mixture_density", "atjup_hip_flux_limiter", "atjup_hip_clamp_species",
               "atjup_hip_swap_nlevel", "atjup_hip_scalar_sync", "atjup_hip_check_finite",
-              "atjup_hip_sync", "atjup_hip_pitch", "atjup_hip_radiation"):
+              "atjup_hip_sync", "atjup_hip_pitch", "atjup_hip_radiation", "atjup_hip_precipitation"):
         getattr(L, n).argtypes = [C.c_void_p]
     L.atjup_hip_device_count.argtypes = [C.POINTER(C.c_int)]
     for n in ("atjup_hip_upload_tangent", "atjup_hip_adj_upload"):
@@ -67,6 +67,9 @@ def load():
     L.atjup_hip_set_gas.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
     L.atjup_hip_set_radiation.argtypes = [C.c_void_p, C.POINTER(abi.RadParams)]
     L.atjup_hip_default_rad_params.argtypes = [C.POINTER(abi.RadParams)]
+    L.atjup_hip_set_precipitation.argtypes = [C.c_void_p, C.POINTER(abi.PrecipParams)]
+    L.atjup_hip_default_precip_params.argtypes = [C.POINTER(abi.PrecipParams)]
+    L.atjup_hip_download_precip_surface.argtypes = [C.c_void_p, _D]
     L.atjup_hip_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
     L.atjup_hip_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.atjup_hip_set_stream.argtypes = [C.c_void_p, C.c_void_p]
@@ -183,6 +186,20 @@ class Context:
     def radiation(self):
         self._chk(self.L.atjup_hip_radiation(self.h), "radiation")
 
+    def set_precipitation(self, params=None):
+        pp = params if params is not None else abi.precip_params()
+        self._chk(self.L.atjup_hip_set_precipitation(self.h, C.byref(pp)), "set_precipitation")
+
+    def precipitation(self):
+        self._chk(self.L.atjup_hip_precipitation(self.h), "precipitation")
+
+    def precip_surface(self):
+        """{name: [jm, km]} of the surface precipitation maps the last call left."""
+        jm, km = self.shape[1], self.shape[2]
+        a = np.empty((5, jm, km))
+        self._chk(self.L.atjup_hip_download_precip_surface(self.h, a.ctypes.data_as(_D)), "precip_surface")
+        return {n: a[i] for i, n in enumerate(abi.PRECIP_SURFACE)}
+
     def turbulence_run(self):
         self._chk(self.L.atjup_hip_turbulence_run(self.h), "turbulence_run")
 
@@ -273,7 +290,7 @@ class Context:
         return out
 
     TIMERS = ("poisson_sweeps", "rk_stages", "pressure", "step", "adj_sweeps", "adj_step", "adj_recompute",
-              "radiation")
+              "radiation", "precipitation")
 
     def set_timing(self, on=True):
         self._chk(self.L.atjup_hip_set_timing(self.h, 1 if on else 0), "set_timing")

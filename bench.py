@@ -210,6 +210,31 @@ def radiation_leg(ctx, g, calls=20):
             "hbm_frac": nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}
 
 
+# Algorithmic bytes per cell of one precipitation call: t, p_stat, rho_mix, the nine H2O/NH3/CH4
+# vapour/cloud/ice fields and NH4SH read once; the nine rain/snow/graupel fluxes, the nine rates,
+# Q_precip and the NH4SH flux written once (planet/Precipitation.h:194-574).
+PRECIP_BYTES_PER_CELL = (13 + 20) * 8
+
+
+def precipitation_leg(ctx, g, calls=20):
+    """Precipitation<P>::run on the GPU, the mean over `calls` calls from the library's HIP events."""
+    ctx.precipitation()
+    ctx.sync()
+    ctx.set_timing(True)
+    ctx.read_timers()
+    for _ in range(calls):
+        ctx.precipitation()
+    ctx.sync()
+    total_ms, launches = ctx.read_timers()["precipitation"]
+    ms = total_ms / calls
+    nbytes = PRECIP_BYTES_PER_CELL * g.im * g.jm * g.km
+    return {"workload": f"three-category precipitation of H2O, NH3, CH4 and NH4SH settling, all {g.jm * g.km} "
+                        f"columns, {g.im} levels",
+            "ms_per_call": ms, "launches_per_call": launches / calls,
+            "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
+            "hbm_frac": nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,9 +350,10 @@ def main():
         }
         c32.close()
 
-    # the grey radiation (a component beside the path: it runs in the physics block when the
-    # profile turns it on), timed on the state the FP64 run left
+    # the grey radiation and the precipitation (components beside the path: they run in the
+    # physics block when the profile turns them on), timed on the state the FP64 run left
     rad = radiation_leg(ctx, g) if args.config == "full" else None
+    precip = precipitation_leg(ctx, g) if args.config == "full" else None
 
     value = cells * args.steps / (ms * 1e-3)
     launch_ms = sweep_ms / max(sweep_launches, 1)
@@ -392,6 +418,7 @@ def main():
         "fp32_refined": fp32,
         "adjoint": adj,
         "radiation": rad,
+        "precipitation": precip,
     }
     print(json.dumps(out))
 

@@ -60,6 +60,9 @@ enum atjup_field {
     ATJUP_F_S_PRECIP_CH4, ATJUP_F_S_PRECIP_CH4_CLOUD, ATJUP_F_S_PRECIP_CH4_ICE,
     ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH,
     ATJUP_F_EPSILON, ATJUP_F_RADIATION,            /* the radiation's layer emissivity and net flux */
+    ATJUP_F_P_RAIN, ATJUP_F_P_SNOW, ATJUP_F_P_GRAUPEL,              /* the precipitation's fluxes */
+    ATJUP_F_P_NH3_RAIN, ATJUP_F_P_NH3_SNOW, ATJUP_F_P_NH3_GRAUPEL,
+    ATJUP_F_P_CH4_RAIN, ATJUP_F_P_CH4_SNOW, ATJUP_F_P_CH4_GRAUPEL, ATJUP_F_P_NH4SH,
     ATJUP_F_END,
     ATJUP_NLEVEL = 64                               /* ATJUP_NLEVEL + prognostic id */
 };
@@ -106,7 +109,18 @@ typedef struct atjup_knobs {
     double turb_k_bg_phys;      /* ATJUP_TURB_K_BG [m2/s2] */
     int    radiation;           /* ATJUP_RADIATION: the grey radiation in the physics block (the
                                    reference's default is 1, cJupiterModel.cpp:52, :1056) */
+    int    precip;              /* ATJUP_PRECIP: the precipitation microphysics after the saturation
+                                   adjustments (default 1, cJupiterModel.cpp:83, :1090); with
+                                   precip_coupling != 0 it reports rates into S_precip_* */
 } atjup_knobs;
+
+/* The precipitation's constants beyond the gases' (which atjup_hip_set_gas supplies):
+ * ATJUP_PRECIP_SCALE on the five rate coefficients (planet/Precipitation.h:111-115,
+ * cJupiterModel.h:1059-1061) and the NH4SH crystal band [t_00_nh4sh, t_0_nh4sh] in K
+ * (planet/cJupiterModel.h:194-195). */
+typedef struct atjup_precip_params {
+    double rate_scale, t_0_nh4sh, t_00_nh4sh;
+} atjup_precip_params;
 
 /* The grey two-stream radiation's planet properties (planet/cJupiterModel.h:1109-1118) and its
  * runtime knobs, read by Radiation<P>::run under the ATJUP_ prefix (planet/Radiation.h:270-295):
@@ -187,6 +201,18 @@ int atjup_hip_radiation(atjup_ctx* ctx);
  * knobs' defaults (atjup_hip_default_rad_params). */
 int atjup_hip_set_radiation(atjup_ctx* ctx, const atjup_rad_params* params);
 void atjup_hip_default_rad_params(atjup_rad_params* params);
+/* Precipitation<cJupiterModel>::run (planet/Precipitation.h:194-574, planet/PrecipitationJup.h), as
+ * the physics block calls it after the saturation adjustments (planet/cJupiterModel.cpp:1090): the
+ * three-category (rain, snow, graupel) column scheme of H2O, NH3 and CH4 with the gases' constants
+ * of atjup_hip_set_gas, the latent heating Q_precip, the moisture rates S_precip_* (coupling on) or
+ * the depleted condensate (coupling off), the NH4SH crystal settling flux and the surface maps. The
+ * printed diagnostics (ATJUP_PRECIP_DIAG) are not formed. */
+int atjup_hip_precipitation(atjup_ctx* ctx);
+int atjup_hip_set_precipitation(atjup_ctx* ctx, const atjup_precip_params* params);
+void atjup_hip_default_precip_params(atjup_precip_params* params);
+/* The surface precipitation maps the last call left (precip_srf_h2o, _nh3, _ch4, _nh4sh, _total,
+ * planet/Precipitation.h:470-492), each [jm][km], in that order: 5 * jm * km doubles. */
+int atjup_hip_download_precip_surface(atjup_ctx* ctx, double* dst);
 /* Turbulence<cJupiterModel>::run (planet/Turbulence.h:307-1106) followed by the wall-function
  * omega Run() applies after it (planet/cJupiterModel.cpp:356-374, 1144). */
 int atjup_hip_turbulence_run(atjup_ctx* ctx);
@@ -231,6 +257,7 @@ enum atjup_timer {
     ATJUP_T_ADJ_STEP,             /* one transposed loop iteration of atjup_hip_adj_backward */
     ATJUP_T_ADJ_RECOMPUTE,        /* the forward recomputation of one checkpoint segment */
     ATJUP_T_RADIATION,            /* the radiation's two kernels (layers, columns) */
+    ATJUP_T_PRECIPITATION,        /* the precipitation's column kernel */
     ATJUP_N_TIMERS
 };
 int atjup_hip_set_timing(atjup_ctx* ctx, int enable);

@@ -53,12 +53,14 @@ def _orc_lib():
                   "orc_bc_radius", "orc_bc_theta", "orc_bc_phi", "orc_bc_solid",
                   "orc_mixture_density", "orc_clamp_species", "orc_swap_nlevel",
                   "orc_scalar_sync", "orc_turbulence_run", "orc_turb_wall", "orc_flux_limiter",
-                  "orc_check_finite", "orc_radiation"):
+                  "orc_check_finite", "orc_radiation", "orc_precipitation"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.orc_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_satadj.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_gas.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
         L.orc_set_radiation.argtypes = [C.c_void_p, C.POINTER(abi.RadParams)]
+        L.orc_set_precipitation.argtypes = [C.c_void_p, C.POINTER(abi.PrecipParams)]
+        L.orc_download_precip_surface.argtypes = [C.c_void_p, _D]
         L.orc_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
@@ -123,6 +125,15 @@ class Oracle:
     def set_radiation(self, params):
         self._chk(self.L.orc_set_radiation(self.h, C.byref(params)), "set_radiation")
 
+    def set_precipitation(self, params):
+        self._chk(self.L.orc_set_precipitation(self.h, C.byref(params)), "set_precipitation")
+
+    def precip_surface(self):
+        jm, km = self.shape[1], self.shape[2]
+        a = np.empty((5, jm, km))
+        self._chk(self.L.orc_download_precip_surface(self.h, _dp(a)), "precip_surface")
+        return {n: a[i] for i, n in enumerate(abi.PRECIP_SURFACE)}
+
     def call(self, op, *args):
         return self._chk(getattr(self.L, "orc_" + op)(self.h, *args), op)
 
@@ -142,6 +153,7 @@ class Reference:
         L.ref_get_scalar.argtypes = [C.c_char_p, _D]
         L.ref_set_mask.argtypes = [_D]
         L.ref_call.argtypes = [C.c_char_p]
+        L.ref_get_field2d.argtypes = [C.c_char_p, _D]
         L.ref_buoy_ref_level.restype = C.c_double
         L.ref_buoy_ref_level.argtypes = [C.c_int]
         L.ref_set_vel_star.argtypes = [_D]
@@ -164,6 +176,12 @@ class Reference:
     def get(self, name):
         a = np.empty(self.shape)
         if self.L.ref_get_field(name.encode(), _dp(a)) != 0:
+            raise KeyError(name)
+        return a
+
+    def get2d(self, name):
+        a = np.empty(self.shape[1:])
+        if self.L.ref_get_field2d(name.encode(), _dp(a)) != 0:
             raise KeyError(name)
         return a
 

@@ -54,6 +54,8 @@ struct orc_ctx {
     std::vector<double> brl;                         // buoy_ref_level [i]
     atjup_gas_params gas[3] = {};                    // saturation-adjustment constants per gas
     atjup_rad_params radp = {};                      // radiation properties and knobs
+    atjup_precip_params precp = { 1.0, 230.0, 200.0 }; // precipitation rate scale and NH4SH band
+    std::vector<double> precip_srf;                  // [5][jm][km] surface maps
     double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
 
     size_t id(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
@@ -1599,6 +1601,137 @@ void radiation(orc_ctx& o){
         }
 }
 
+// ---- Precipitation<P>::run (Precipitation.h:194-574) -----------------------------------------
+// Zero Q_precip and the nine rates (:199-210); the three-category column of H2O, NH3, CH4 in that
+// order over every column (column(), :273-450); the NH4SH crystal settling (:523-574); the surface
+// maps at each column's base (:470-492). The diagnostics print (ATJUP_PRECIP_DIAG) is not formed.
+void precip_column(orc_ctx& o, int gas){
+    static const int vap[3] = { ATJUP_F_H2O, ATJUP_F_NH3, ATJUP_F_CH4 };
+    static const int pf[3] = { ATJUP_F_P_RAIN, ATJUP_F_P_NH3_RAIN, ATJUP_F_P_CH4_RAIN };
+    static const int sf[3] = { ATJUP_F_S_PRECIP_H2O, ATJUP_F_S_PRECIP_NH3, ATJUP_F_S_PRECIP_CH4 };
+    const atjup_gas_params& G = o.gas[gas];
+    const int im = o.im, jm = o.jm, km = o.km;
+    const double t_ref = o.c.t_ref, Lf = G.ls - G.lv, rs = o.precp.rate_scale;
+    const double c_c_au = 2.0e-8, c_i_au = 5.0e-8, q_c_crit = 5.0e-4, c_ac = 1.0e-7, c_rim = 1.0e-8;
+    const double z_csg = 2.5e-8, a_ev = 2.76e-3, c_melt_K = 0.2, c_frz_K = 0.2, P_ref = 1.0e-4;
+    const double ratio_cap = 5.0, v_fall = 2.0, P_max_flux = 1.0e-4, dep_frac = 1.0;
+    const bool coupling = o.kn.precip_coupling != 0.0;
+    std::vector<double>& V = o.f[vap[gas]];
+    std::vector<double>& CL = o.f[vap[gas] + 1];
+    std::vector<double>& IC = o.f[vap[gas] + 2];
+    std::vector<double>& PR = o.f[pf[gas]];
+    std::vector<double>& PS_ = o.f[pf[gas] + 1];
+    std::vector<double>& PG = o.f[pf[gas] + 2];
+    std::vector<double>& SV = o.f[sf[gas]];
+    std::vector<double>& SC = o.f[sf[gas] + 1];
+    std::vector<double>& SI = o.f[sf[gas] + 2];
+    std::vector<double>& Q = o.f[ATJUP_F_Q_PRECIP];
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++){
+            const int i_base = std::max(o.itopo[(size_t)j * km + k], 0), i_top = im - 1;
+            if(i_base >= i_top) continue;
+            const size_t qt = o.id(i_top, j, k);
+            PR[qt] = 0.0; PS_[qt] = 0.0; PG[qt] = 0.0;
+            for(int i = i_top - 1; i >= i_base; i--){
+                const size_t q = o.id(i, j, k), qa = o.id(i + 1, j, k);
+                const double T = o.f[ATJUP_F_T][q] * t_ref;
+                const double p_u = o.f[ATJUP_F_P_STAT][q];
+                const double q_v = std::max(0.0, V[q]), q_c = std::max(0.0, CL[q]), q_i = std::max(0.0, IC[q]);
+                const double E_r = svp(T, G.C, G.L0, G.R, G.del_alf, G.del_bet);
+                const double rf = o.f[ATJUP_F_RHO_MIX][q];
+                const double rho_at = (rf > 0.0 && std::isfinite(rf)) ? rf : o.c.r_mix;
+                const double q_sat = (p_u > E_r) ? rho_at * G.ep * E_r / p_u : q_v;
+                const double dz = std::max(0.0, o.layer_thickness_m(i));
+                if(dz <= 0.0){ PR[q] = PR[qa]; PS_[q] = PS_[qa]; PG[q] = PG[qa]; continue; }
+                const double F_r_in = PR[qa], F_s_in = PS_[qa], F_g_in = PG[qa];
+                const double Rain = std::min(ratio_cap, F_r_in / P_ref);
+                const double Snow = std::min(ratio_cap, F_s_in / P_ref);
+                const double Graupel = std::min(ratio_cap, F_g_in / P_ref);
+                const double Rain_79 = (Rain > 0.0) ? std::pow(Rain, 7.0 / 9.0) : 0.0;
+                const double Rain_49 = (Rain > 0.0) ? std::pow(Rain, 4.0 / 9.0) : 0.0;
+                const bool warm = T >= G.t_0;
+                const double S_c_au = (warm && q_c > q_c_crit) ? (rs * c_c_au) * (q_c - q_c_crit) : 0.0;
+                const double S_i_au = (!warm && T >= G.t_00) ? (rs * c_i_au) * q_i : 0.0;
+                const double S_ac = warm ? (rs * c_ac) * q_c * Rain_79 : 0.0;
+                const double S_s_rim = !warm ? (rs * c_rim) * q_c * Snow : 0.0;
+                const double S_g_rim = !warm ? (rs * c_rim) * q_c * Graupel : 0.0;
+                const double S_csg = (!warm && q_c > q_c_crit) ? (rs * z_csg) * q_c * Snow : 0.0;
+                const double f_melt = warm ? std::min(1.0, c_melt_K * (T - G.t_0)) : 0.0;
+                const double f_frz = warm ? 0.0 : std::min(1.0, c_frz_K * (G.t_0 - T));
+                const double F_s_melt = f_melt * F_s_in, F_g_melt = f_melt * F_g_in, F_r_frz = f_frz * F_r_in;
+                const double F_r_avail = std::max(0.0, F_r_in - F_r_frz + F_s_melt + F_g_melt + (S_c_au + S_ac) * dz);
+                const double S_ev_raw = (warm && Rain > 0.0 && q_v < q_sat) ? a_ev * (q_sat - q_v) * Rain_49 : 0.0;
+                const double F_ev_sat = std::max(0.0, q_sat - q_v) * v_fall;
+                const double F_ev = std::min(std::min(S_ev_raw * dz, F_r_avail), F_ev_sat);
+                const double S_ev = F_ev / dz;
+                Q[q] += Lf * (S_s_rim + S_g_rim + S_csg) + Lf * (F_r_frz - F_s_melt - F_g_melt) / dz - G.lv * F_ev / dz;
+                const double tau = dz / v_fall;
+                const double d_c = std::min(dep_frac * q_c, (S_c_au + S_ac + S_s_rim + S_g_rim + S_csg) * tau);
+                const double d_i = std::min(dep_frac * q_i, S_i_au * tau);
+                if(coupling){
+                    const double nd = (o.g.L_atm * 1.0e3) / o.c.u_0;
+                    SC[q] = -std::max(0.0, d_c) / tau * nd;
+                    SI[q] = -std::max(0.0, d_i) / tau * nd;
+                    SV[q] = +S_ev * nd;
+                } else {
+                    CL[q] = q_c - std::max(0.0, d_c);
+                    IC[q] = q_i - std::max(0.0, d_i);
+                    V[q] = q_v + S_ev * tau;
+                }
+                const double F_r = F_r_in - F_r_frz + F_s_melt + F_g_melt - F_ev + (S_c_au + S_ac) * dz;
+                const double F_s = F_s_in - F_s_melt + (S_i_au + S_s_rim - S_csg) * dz;
+                const double F_g = F_g_in - F_g_melt + F_r_frz + (S_g_rim + S_csg) * dz;
+                PR[q] = std::min(P_max_flux, std::max(0.0, F_r));
+                PS_[q] = std::min(P_max_flux, std::max(0.0, F_s));
+                PG[q] = std::min(P_max_flux, std::max(0.0, F_g));
+            }
+            const size_t qb = o.id(i_base, j, k);
+            for(int i = 0; i < i_base; i++){
+                const size_t q = o.id(i, j, k);
+                PR[q] = PR[qb]; PS_[q] = PS_[qb]; PG[q] = PG[qb];
+            }
+        }
+}
+
+void precipitation(orc_ctx& o){
+    static const int zero_me[] = { ATJUP_F_Q_PRECIP, ATJUP_F_S_PRECIP_H2O, ATJUP_F_S_PRECIP_H2O_CLOUD,
+        ATJUP_F_S_PRECIP_H2O_ICE, ATJUP_F_S_PRECIP_NH3, ATJUP_F_S_PRECIP_NH3_CLOUD, ATJUP_F_S_PRECIP_NH3_ICE,
+        ATJUP_F_S_PRECIP_CH4, ATJUP_F_S_PRECIP_CH4_CLOUD, ATJUP_F_S_PRECIP_CH4_ICE };
+    for(int f : zero_me) std::fill(o.f[f].begin(), o.f[f].end(), 0.0);
+    for(int gas = 0; gas < 3; gas++) precip_column(o, gas);
+    // NH4SH crystals: Stokes settling in [t_00_nh4sh, t_0_nh4sh], the base's flux below the base
+    const int im = o.im, jm = o.jm, km = o.km;
+    const double r_p = 1.0e-5, rho_c = 1200.0, mu = 9.0e-6;
+    const double v_settle = (2.0 / 9.0) * r_p * r_p * (rho_c - o.c.r_mix) * o.c.g / mu;
+    std::vector<double>& PN = o.f[ATJUP_F_P_NH4SH];
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++){
+            const int i_base = std::max(o.itopo[(size_t)j * km + k], 0);
+            for(int i = i_base; i < im; i++){
+                const double T = o.f[ATJUP_F_T][o.id(i, j, k)] * o.c.t_ref;
+                const double qn = std::max(0.0, o.f[ATJUP_F_NH4SH][o.id(i, j, k)]);
+                PN[o.id(i, j, k)] = (T >= o.precp.t_00_nh4sh && T <= o.precp.t_0_nh4sh) ? qn * v_settle : 0.0;
+            }
+            const double base_flux = (i_base < im) ? PN[o.id(i_base, j, k)] : 0.0;
+            for(int i = 0; i < i_base && i < im; i++) PN[o.id(i, j, k)] = base_flux;
+        }
+    // surface maps at the base
+    const size_t n2 = (size_t)jm * km;
+    o.precip_srf.assign(5 * n2, 0.0);
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++){
+            const int ib = std::min(std::max(o.itopo[(size_t)j * km + k], 0), im - 1);
+            const size_t q = o.id(ib, j, k), c2 = (size_t)j * km + k;
+            const double h2o = o.f[ATJUP_F_P_RAIN][q] + o.f[ATJUP_F_P_SNOW][q] + o.f[ATJUP_F_P_GRAUPEL][q];
+            const double nh3 = o.f[ATJUP_F_P_NH3_RAIN][q] + o.f[ATJUP_F_P_NH3_SNOW][q] + o.f[ATJUP_F_P_NH3_GRAUPEL][q];
+            const double ch4 = o.f[ATJUP_F_P_CH4_RAIN][q] + o.f[ATJUP_F_P_CH4_SNOW][q] + o.f[ATJUP_F_P_CH4_GRAUPEL][q];
+            const double nh4 = PN[q];
+            o.precip_srf[c2] = h2o; o.precip_srf[n2 + c2] = nh3; o.precip_srf[2 * n2 + c2] = ch4;
+            o.precip_srf[3 * n2 + c2] = nh4; o.precip_srf[4 * n2 + c2] = h2o + nh3 + ch4 + nh4;
+        }
+}
+
 bool valid_field(int f){
     return (f >= 0 && f < ATJUP_N_PROG) || (f >= ATJUP_F_P_DYN && f < ATJUP_F_END)
         || (f >= ATJUP_NLEVEL && f < ATJUP_NLEVEL + ATJUP_N_PROG);
@@ -1753,6 +1886,18 @@ int orc_set_radiation(orc_ctx* o, const atjup_rad_params* p){
     return 0;
 }
 int orc_radiation(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; radiation(*o); return 0; }
+int orc_set_precipitation(orc_ctx* o, const atjup_precip_params* p){
+    if(!o || !p) return ATJUP_ERR_ARG;
+    o->precp = *p;
+    return 0;
+}
+int orc_precipitation(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; precipitation(*o); return 0; }
+int orc_download_precip_surface(orc_ctx* o, double* dst){
+    if(!o || !dst) return ATJUP_ERR_ARG;
+    if(o->precip_srf.empty()) o->precip_srf.assign((size_t)5 * o->jm * o->km, 0.0);
+    std::memcpy(dst, o->precip_srf.data(), o->precip_srf.size() * sizeof(double));
+    return 0;
+}
 
 // One iteration of the loop body, on-path functions only (cJupiterModel.cpp:1010-1555).
 int orc_step(orc_ctx* o, int iter, int first){
@@ -1763,6 +1908,7 @@ int orc_step(orc_ctx* o, int iter, int first){
         mixture_density(*o);
         if(o->kn.radiation) radiation(*o);                    // cJupiterModel.cpp:1056
         if(o->kn.satadj) for(int gas = 0; gas < 3; gas++) satadj(*o, gas);
+        if(o->kn.precip) precipitation(*o);                   // cJupiterModel.cpp:1090
         if(o->kn.rk_scalar_sync >= 1) scalar_sync(*o);
         if(o->kn.turb_model >= 0){ turbulence(*o); turb_wall_omega(*o); }
         flux_limiter(*o);

@@ -21,6 +21,7 @@
 #include "BC_Jup.h"
 #include "TurbulenceJup.h"
 #include "RadiationJup.h"
+#include "PrecipitationJup.h"
 
 #include <fcntl.h>
 #include <unistd.h>
@@ -64,6 +65,19 @@ public:
             {"massflux_h2s", &M.massflux_h2s}, {"massflux_nh3", &M.massflux_nh3},
             {"massflux_nh4sh", &M.massflux_nh4sh}, {"fluxlim_nh4sh", &M.fluxlim_nh4sh},
             {"epsilon", &M.epsilon}, {"radiation", &M.radiation},
+            {"P_rain", &M.P_rain}, {"P_snow", &M.P_snow}, {"P_graupel", &M.P_graupel},
+            {"P_nh3_rain", &M.P_nh3_rain}, {"P_nh3_snow", &M.P_nh3_snow}, {"P_nh3_graupel", &M.P_nh3_graupel},
+            {"P_ch4_rain", &M.P_ch4_rain}, {"P_ch4_snow", &M.P_ch4_snow}, {"P_ch4_graupel", &M.P_ch4_graupel},
+            {"P_nh4sh", &M.P_nh4sh},
+        };
+    }
+
+    static std::map<std::string, Array_2D*> fields2d(){
+        cJupiterModel& M = *m;
+        return {
+            {"precip_srf_h2o", &M.precip_srf_h2o}, {"precip_srf_nh3", &M.precip_srf_nh3},
+            {"precip_srf_ch4", &M.precip_srf_ch4}, {"precip_srf_nh4sh", &M.precip_srf_nh4sh},
+            {"precip_srf_total", &M.precip_srf_total},
         };
     }
 
@@ -181,6 +195,7 @@ public:
         else if(op == "turbulence")     { TurbulenceJup(M).run(); wall_omega(M); }
         else if(op == "turb_wall")      { TurbulenceJup(M).apply_wall_bc(); wall_omega(M); }
         else if(op == "radiation")      RadiationJup(M).run();
+        else if(op == "precipitation")  PrecipitationJup(M).run();
         else if(op == "satadj_h2o")
             SaturationAdjustmentJup(M).run("H2O", M.t_0_h2o, M.t_00_h2o, M.ep_h2o, M.lv_h2o, M.ls_h2o,
                 M.C_h2o, M.L0_h2o, M.R_h2o, M.del_alf_h2o, M.del_bet_h2o,
@@ -288,6 +303,16 @@ int ref_get_field(const char* name, double* dst){
     Array& A = *it->second;
     for(int i = 0; i < A.im; i++) for(int j = 0; j < A.jm; j++)
         std::memcpy(dst + ((size_t)i * A.jm + j) * A.km, A.x[i][j], sizeof(double) * A.km);
+    return 0;
+}
+
+int ref_get_field2d(const char* name, double* dst){
+    auto f = H::fields2d();
+    auto it = f.find(name);
+    if(it == f.end()) return -1;
+    Array_2D& A = *it->second;
+    int im, jm, km; H::dims(&im, &jm, &km);
+    for(int j = 0; j < jm; j++) std::memcpy(dst + (size_t)j * km, A.y[j], sizeof(double) * km);
     return 0;
 }
 

@@ -44,4 +44,16 @@ def test_default_radiation_params_match_mirror():
     for name, _ in abi.RadParams._fields_:
         assert getattr(p, name) == getattr(q, name), name
     assert ctypes.sizeof(abi.RadParams) == 9 * 8 + 2 * 4
-    assert abi.FIELD["epsilon"] == 47 and abi.FIELD["radiation"] == 48 and abi.F_END == 49
+    assert abi.FIELD["epsilon"] == 47 and abi.FIELD["radiation"] == 48
+    assert abi.FIELD["P_rain"] == 49 and abi.FIELD["P_nh4sh"] == 58 and abi.F_END == 59 < abi.NLEVEL
+
+
+def test_default_precipitation_params_match_mirror():
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libatjup_hip.so not built")
+    L = lib.load()
+    p = abi.PrecipParams()
+    L.atjup_hip_default_precip_params(ctypes.byref(p))
+    q = abi.precip_params()
+    for name, _ in abi.PrecipParams._fields_:
+        assert getattr(p, name) == getattr(q, name), name

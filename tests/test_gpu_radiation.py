@@ -54,20 +54,27 @@ def test_radiation_against_oracle(pair, over):
 
 
 def test_radiation_fp32_refined_form():
+    """The FP32-refined form's radiation (FP32 fields in, FP64 column arithmetic) against FP64 on
+    the same state, next to the FP64 radiation of the state rounded to FP32: Q_rad is a flux
+    divergence over thin layers, so the rounding of t and p_stat to FP32 alone moves it by about
+    1e-5 of its maximum; the form must stay within twice that floor and within 1e-6 on the
+    emissivity and the net flux."""
     st = inputs.make_state("full", seed=7)
+    rounded = inputs.State(grid=st.grid, mask=st.mask,
+                           fields={n: v.astype(np.float32).astype(np.float64) for n, v in st.fields.items()})
     g64, orc, _ = make_pair("full", sweeps=20, state=st)
     g32, orc2, _ = make_pair("full", sweeps=20, state=st, precision=abi.PRECISION_FP32_REF)
-    orc2.close()
-    for c in (g64, g32):
+    gr, orc3, _ = make_pair("full", sweeps=20, state=rounded)
+    for c in (g64, g32, gr):
         c.mixture_density()
         c.radiation()
     for n in OUT:
         e = rel_err(g32.get(n), g64.get(n))
-        print(f"FP32-refined {n}: {e:.2e}")
-        assert e < 1.0e-5, n
-    g64.close()
-    g32.close()
-    orc.close()
+        floor = rel_err(gr.get(n), g64.get(n))
+        print(f"FP32-refined {n}: {e:.2e} (FP64 from the rounded state: {floor:.2e})")
+        assert e <= max(2.0 * floor, 1.0e-6), n
+    for c in (g64, g32, gr, orc, orc2, orc3):
+        c.close()
 
 
 def test_step_with_radiation_against_oracle():

@@ -125,3 +125,56 @@ print("ALL-EQUAL")
 def test_radiation_bit_exact_against_compiled_reference():
     r = subprocess.run([sys.executable, "-c", SCRIPT_RAD % {"root": ROOT}], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+# Precipitation<P>::run on the moist state with the obstacle: the rain/snow/graupel fluxes of the
+# three gases, the NH4SH settling flux, Q_precip, the nine rates or (coupling off) the depleted
+# condensate, and the surface maps. The reference caches PRECIP_COUPLING and PRECIP_SCALE, so each
+# knob setting runs in its own process.
+SCRIPT_PRECIP = r'''
+import os, sys, numpy as np
+sys.path.insert(0, %(root)r)
+from atjup_amd import abi, inputs
+import oracle
+coupling, scale, inversion = %(coupling)r, %(scale)r, %(inversion)r
+os.environ["ATJUP_PRECIP_SCALE"] = repr(scale)
+G, Cn, K, env = abi.make_profile("full", sweeps=8, precip=True, precip_coupling=coupling)
+st = inputs.make_state("full", seed=11)
+if inversion:
+    # the four lowest levels below freezing in a band of latitudes (350 -> 245 K and colder),
+    # under the warm layers where the H2O rain forms: rain falls into freezing layers (the freeze
+    # handoff to graupel, riming)
+    st.fields["t"][:4, 40:90, :] *= 0.7
+ref = oracle.Reference(env); ref.load_state(st, Cn)
+orc = oracle.Oracle(G, Cn, K); orc.load_state(st)
+orc.set_precipitation(abi.precip_params(rate_scale=scale))
+ref.call("mixture_density"); orc.call("mixture_density")
+ref.call("precipitation"); orc.call("precipitation")
+names = ["P_rain", "P_snow", "P_graupel", "P_nh3_rain", "P_nh3_snow", "P_nh3_graupel", "P_ch4_rain",
+         "P_ch4_snow", "P_ch4_graupel", "P_nh4sh", "Q_precip"] + [n for n in abi.INPUTS if n.startswith("S_precip")] \
+        + list(inputs.SPECIES)
+for n in names:
+    a, b = ref.get(n), orc.get(n)
+    if not np.array_equal(a, b):
+        print("MISMATCH", n, np.max(np.abs(a - b))); sys.exit(1)
+srf = orc.precip_surface()
+for n in abi.PRECIP_SURFACE:
+    if not np.array_equal(ref.get2d(n), srf[n]):
+        print("MISMATCH", n); sys.exit(1)
+nz = {n: int(np.count_nonzero(orc.get(n))) for n in ("P_rain", "P_snow", "P_graupel", "P_nh3_rain", "P_nh4sh", "Q_precip")}
+print("nonzero", nz)
+if min(nz["P_rain"], nz["P_snow"], nz["Q_precip"], nz["P_nh4sh"]) == 0: print("NOTHING FELL"); sys.exit(1)
+if inversion and nz["P_graupel"] == 0: print("NO GRAUPEL"); sys.exit(1)
+print("ALL-EQUAL")
+'''
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("coupling,scale,inversion", [(1.0, 1.0, False), (0.0, 3.0, False), (1.0, 1.0, True)],
+                         ids=["rates", "in-place-scale3", "inversion"])
+def test_precipitation_bit_exact_against_compiled_reference(coupling, scale, inversion):
+    r = subprocess.run([sys.executable, "-c", SCRIPT_PRECIP % {"root": ROOT, "coupling": coupling, "scale": scale,
+                                                                 "inversion": inversion}],
+                       capture_output=True, text=True, timeout=600)
+    print(r.stdout[-500:])
+    assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
