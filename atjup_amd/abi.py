@@ -22,7 +22,8 @@ INPUTS = ("p_dyn", "p_stat", "p_hydro", "rho_mix", "wall_nue", "nue", "prod", "t
           "massflux_h2s", "massflux_nh3", "massflux_nh4sh", "fluxlim_nh4sh",
           "epsilon", "radiation",
           "P_rain", "P_snow", "P_graupel", "P_nh3_rain", "P_nh3_snow", "P_nh3_graupel",
-          "P_ch4_rain", "P_ch4_snow", "P_ch4_graupel", "P_nh4sh")
+          "P_ch4_rain", "P_ch4_snow", "P_ch4_graupel", "P_nh4sh",
+          "difflux_h2s", "difflux_nh3", "thermalmassflux")
 NLEVEL = 64
 
 FIELD = {n: i for i, n in enumerate(PROG)}
@@ -182,3 +183,15 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
 
 def env_string(env: dict) -> str:
     return ";".join(f"{k}={v}" for k, v in env.items())
+
+
+# The restart file's arrays in the reference's order (planet/FileIO_Jup.cpp:281-305): the fixed list
+# (n-level copies by their reference names) and, in JUP2 files, the carried list, then vel_star.
+RESTART_FIXED = ("t", "u", "v", "w", "tn", "un", "vn", "wn",
+                 "h2o", "h2o_cloud", "h2o_ice", "h2on", "h2o_cloudn", "h2o_icen", "h2s", "h2sn",
+                 "nh3", "nh3_cloud", "nh3_ice", "nh3n", "nh3_cloudn", "nh3_icen",
+                 "ch4", "ch4_cloud", "ch4_ice", "ch4n", "ch4_cloudn", "ch4_icen", "nh4sh", "nh4shn",
+                 "p_dyn", "p_dynn", "p_stat", "tke", "dis", "tken", "disn", "nue")
+RESTART_CARRIED = ("aux_u", "aux_v", "aux_w", "difflux_h2s", "difflux_nh3", "massflux_h2s", "massflux_nh3",
+                   "fluxlim_nh4sh", "thermalmassflux", "Q_precip", "Q_rad", "rho_mix",
+                   "S_precip_h2o", "S_precip_h2o_cloud", "S_precip_h2o_ice", "S_precip_nh3", "S_precip_nh3_ice")

@@ -1,6 +1,8 @@
 // C ABI of libatjup_hip.so (include/atjup_hip.h): context, transfers, operators.
 #include "atjup_internal.h"
 
+#include <cstdio>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -488,6 +490,119 @@ int atjup_hip_download_tangent(atjup_ctx* c, int f, double* dst){ return tangent
 int atjup_hip_download_field(atjup_ctx* c, int f, double* dst){ return download_impl(c, f, dst, nullptr); }
 int atjup_hip_upload_field_rows(atjup_ctx* c, int f, double* const* const* rows){ return upload_impl(c, f, nullptr, rows); }
 int atjup_hip_download_field_rows(atjup_ctx* c, int f, double** const* rows){ return download_impl(c, f, nullptr, rows); }
+
+// ---- the restart file (planet/FileIO_Jup.cpp:281-305, 584-684) ---------------------------------
+namespace {
+constexpr int RS_PDYNN = -1;                  // p_dynn: the n-level of p_dyn, the live field here
+constexpr int NL = ATJUP_NLEVEL;
+// restart_arrays() (FileIO_Jup.cpp:281-293)
+const int kRestartFixed[38] = {
+    ATJUP_F_T, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W, NL + ATJUP_F_T, NL + ATJUP_F_U, NL + ATJUP_F_V, NL + ATJUP_F_W,
+    ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE, NL + ATJUP_F_H2O, NL + ATJUP_F_H2O_CLOUD, NL + ATJUP_F_H2O_ICE,
+    ATJUP_F_H2S, NL + ATJUP_F_H2S,
+    ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, NL + ATJUP_F_NH3, NL + ATJUP_F_NH3_CLOUD, NL + ATJUP_F_NH3_ICE,
+    ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE, NL + ATJUP_F_CH4, NL + ATJUP_F_CH4_CLOUD, NL + ATJUP_F_CH4_ICE,
+    ATJUP_F_NH4SH, NL + ATJUP_F_NH4SH,
+    ATJUP_F_P_DYN, RS_PDYNN, ATJUP_F_P_STAT,
+    ATJUP_F_TKE, ATJUP_F_DIS, NL + ATJUP_F_TKE, NL + ATJUP_F_DIS, ATJUP_F_NUE };
+// restart_arrays_carried() (FileIO_Jup.cpp:297-305)
+const int kRestartCarried[17] = {
+    ATJUP_F_AUX_U, ATJUP_F_AUX_V, ATJUP_F_AUX_W, ATJUP_F_DIFFLUX_H2S, ATJUP_F_DIFFLUX_NH3,
+    ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_FLUXLIM_NH4SH, ATJUP_F_THERMALMASSFLUX,
+    ATJUP_F_Q_PRECIP, ATJUP_F_Q_RAD, ATJUP_F_RHO_MIX,
+    ATJUP_F_S_PRECIP_H2O, ATJUP_F_S_PRECIP_H2O_CLOUD, ATJUP_F_S_PRECIP_H2O_ICE, ATJUP_F_S_PRECIP_NH3,
+    ATJUP_F_S_PRECIP_NH3_ICE };
+constexpr int32_t MAGIC_JUP1 = 0x4A555031, MAGIC_JUP2 = 0x4A555032;
+
+struct FileCloser { FILE* f; ~FileCloser(){ if(f) fclose(f); } };
+}  // namespace
+
+int atjup_hip_save_state(atjup_ctx* c, const char* path, int iter, int carried){
+    if(!c || !path){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    FILE* f = fopen(path, "wb");
+    if(!f){ set_error(std::string("save_state: cannot open ") + path); return ATJUP_ERR_STATE; }
+    FileCloser fc{f};
+    const int32_t hdr[5] = { carried ? MAGIC_JUP2 : MAGIC_JUP1, c->im, c->jm, c->km, iter };
+    bool ok = fwrite(hdr, sizeof(hdr), 1, f) == 1;
+    const size_t n = (size_t)c->im * c->jm * c->km;
+    std::vector<double> buf(n);
+    auto put = [&](int id){
+        if(!ok) return;
+        if(int rc = download_impl(c, id == RS_PDYNN ? ATJUP_F_P_DYN : id, buf.data(), nullptr)){ ok = false; (void)rc; return; }
+        ok = fwrite(buf.data(), sizeof(double), n, f) == n;
+    };
+    for(int id : kRestartFixed) put(id);
+    if(carried){
+        for(int id : kRestartCarried) put(id);
+        if(ok){
+            std::vector<double> vs((size_t)c->jm * c->km);
+            if(atjup_hip_download_vel_star(c, vs.data()) != 0) ok = false;
+            else ok = fwrite(vs.data(), sizeof(double), vs.size(), f) == vs.size();
+        }
+    }
+    if(!ok){ set_error(std::string("save_state: failed while writing ") + path); return ATJUP_ERR_STATE; }
+    if(fflush(f) != 0){ set_error(std::string("save_state: failed while writing ") + path); return ATJUP_ERR_STATE; }
+    return 0;
+}
+
+int atjup_hip_load_state(atjup_ctx* c, const char* path, int* iter, int* carried_out){
+    if(!c || !path){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    FILE* f = fopen(path, "rb");
+    if(!f){ set_error(std::string("load_state: no file ") + path); return ATJUP_ERR_STATE; }
+    FileCloser fc{f};
+    int32_t hdr[5];
+    if(fread(hdr, sizeof(hdr), 1, f) != 1 || (hdr[0] != MAGIC_JUP1 && hdr[0] != MAGIC_JUP2)
+       || hdr[1] != c->im || hdr[2] != c->jm || hdr[3] != c->km){
+        set_error(std::string("load_state: bad header / grid mismatch in ") + path); return ATJUP_ERR_STATE;
+    }
+    const bool jup2 = hdr[0] == MAGIC_JUP2;
+    const size_t n = (size_t)c->im * c->jm * c->km, nb = n * sizeof(double);
+    const size_t n_arr = 38 + (jup2 ? 17 : 0);
+    const size_t need = sizeof(hdr) + n_arr * nb + (jup2 ? (size_t)c->jm * c->km * sizeof(double) : 0);
+    // nothing is loaded from a short file or from one whose n-level copies differ from their fields
+    // (one storage holds both on this path): both are checked before the first upload
+    if(fseek(f, 0, SEEK_END) != 0 || (size_t)ftell(f) < need){
+        set_error(std::string("load_state: truncated file ") + path); return ATJUP_ERR_STATE;
+    }
+    std::vector<double> buf(n), base(n);
+    auto read_at = [&](size_t a, double* dst){
+        return fseek(f, (long)(sizeof(hdr) + a * nb), SEEK_SET) == 0 && fread(dst, sizeof(double), n, f) == n;
+    };
+    for(size_t a = 0; a < 38; a++){
+        const int id = kRestartFixed[a];
+        const bool nlev = id == RS_PDYNN || (id >= NL && id < NL + ATJUP_N_PROG);
+        if(!nlev) continue;
+        const int want = id == RS_PDYNN ? ATJUP_F_P_DYN : id - NL;
+        size_t b = 0;
+        while(b < 38 && kRestartFixed[b] != want) b++;
+        if(!read_at(a, buf.data()) || !read_at(b, base.data())){
+            set_error(std::string("load_state: read error in ") + path); return ATJUP_ERR_STATE;
+        }
+        if(std::memcmp(buf.data(), base.data(), nb) != 0){
+            set_error(std::string("load_state: n-level arrays differ from their fields in ") + path);
+            return ATJUP_ERR_STATE;
+        }
+    }
+    if(fseek(f, (long)sizeof(hdr), SEEK_SET) != 0){ set_error("load_state: seek failed"); return ATJUP_ERR_STATE; }
+    for(size_t a = 0; a < n_arr; a++){
+        const int id = a < 38 ? kRestartFixed[a] : kRestartCarried[a - 38];
+        if(fread(buf.data(), sizeof(double), n, f) != n){
+            set_error(std::string("load_state: read error in ") + path); return ATJUP_ERR_STATE;
+        }
+        if(id == RS_PDYNN || (id >= NL && id < NL + ATJUP_N_PROG)) continue;
+        if(int rc = upload_impl(c, id, buf.data(), nullptr)) return rc;
+    }
+    if(jup2){
+        std::vector<double> vs((size_t)c->jm * c->km);
+        if(fread(vs.data(), sizeof(double), vs.size(), f) != vs.size()){
+            set_error(std::string("load_state: read error in ") + path); return ATJUP_ERR_STATE;
+        }
+        if(int rc = atjup_hip_upload_vel_star(c, vs.data())) return rc;
+    }
+    if(iter) *iter = hdr[4];
+    if(carried_out) *carried_out = jup2 ? 1 : 0;
+    return 0;
+}
 
 int atjup_hip_upload_mask(atjup_ctx* c, const double* m){
     if(!c || !m){ set_error("null argument"); return ATJUP_ERR_ARG; }

@@ -70,6 +70,8 @@ def load():
     L.atjup_hip_set_precipitation.argtypes = [C.c_void_p, C.POINTER(abi.PrecipParams)]
     L.atjup_hip_default_precip_params.argtypes = [C.POINTER(abi.PrecipParams)]
     L.atjup_hip_download_precip_surface.argtypes = [C.c_void_p, _D]
+    L.atjup_hip_save_state.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
+    L.atjup_hip_load_state.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.atjup_hip_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
     L.atjup_hip_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.atjup_hip_set_stream.argtypes = [C.c_void_p, C.c_void_p]
@@ -152,6 +154,11 @@ class Context:
         a = np.ascontiguousarray(vs, dtype=np.float64)
         self._chk(self.L.atjup_hip_upload_vel_star(self.h, _dp(a)), "upload vel_star")
 
+    def get_vel_star(self):
+        a = np.empty(self.shape[1:])
+        self._chk(self.L.atjup_hip_download_vel_star(self.h, _dp(a)), "download vel_star")
+        return a
+
     def buoy_ref_level(self):
         a = np.empty(self.shape[0])
         self._chk(self.L.atjup_hip_download_buoy_ref_level(self.h, _dp(a)), "buoy_ref_level")
@@ -199,6 +206,17 @@ class Context:
         a = np.empty((5, jm, km))
         self._chk(self.L.atjup_hip_download_precip_surface(self.h, a.ctypes.data_as(_D)), "precip_surface")
         return {n: a[i] for i, n in enumerate(abi.PRECIP_SURFACE)}
+
+    def save_restart(self, path, iteration, carried=True):
+        """cJupiterModel::save_state: the reference's JUP2 (or, carried=False, JUP1) file."""
+        self._chk(self.L.atjup_hip_save_state(self.h, str(path).encode(), int(iteration), 1 if carried else 0),
+                  "save_state")
+
+    def load_restart(self, path):
+        """cJupiterModel::load_state: returns (iteration, carried)."""
+        it, car = C.c_int(0), C.c_int(0)
+        self._chk(self.L.atjup_hip_load_state(self.h, str(path).encode(), C.byref(it), C.byref(car)), "load_state")
+        return it.value, bool(car.value)
 
     def turbulence_run(self):
         self._chk(self.L.atjup_hip_turbulence_run(self.h), "turbulence_run")

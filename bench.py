@@ -235,6 +235,26 @@ def precipitation_leg(ctx, g, calls=20):
             "hbm_frac": nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}
 
 
+def restart_leg(ctx, g):
+    """save_state then load_state of the device-resident state (JUP2, 55 arrays and vel_star), in a
+    temporary directory on the host's disk: the D2H/H2D transfers and the file I/O together."""
+    import tempfile
+    nbytes = 20 + (38 + 17) * g.im * g.jm * g.km * 8 + g.jm * g.km * 8
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "jup_restart_0.bin")
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.save_restart(path, 0)
+        t1 = time.perf_counter()
+        ctx.load_restart(path)
+        ctx.sync()
+        t2 = time.perf_counter()
+        size = os.path.getsize(path)
+    return {"workload": f"JUP2 restart file of the {g.im}x{g.jm}x{g.km} state ({size / 1e9:.2f} GB)",
+            "bytes": size, "expected_bytes": nbytes, "save_s": t1 - t0, "load_s": t2 - t1,
+            "save_GBs": size / (t1 - t0) / 1e9, "load_GBs": size / (t2 - t1) / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +268,7 @@ def main():
     ap.add_argument("--omega", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
+    ap.add_argument("--no-restart", action="store_true", help="skip the restart-file leg")
     ap.add_argument("--grid", default="standard", choices=["standard", "fine"],
                     help="standard: 41x181x361 (1 deg); fine: 81x361x721 (0.5 deg, no CPU baseline)")
     args = ap.parse_args()
@@ -354,6 +375,12 @@ def main():
     # physics block when the profile turns them on), timed on the state the FP64 run left
     rad = radiation_leg(ctx, g) if args.config == "full" else None
     precip = precipitation_leg(ctx, g) if args.config == "full" else None
+    rst = None
+    if not args.no_restart:
+        try:
+            rst = restart_leg(ctx, g)
+        except Exception as e:  # disk space on the host never decides the GPU number
+            rst = {"workload": "JUP2 restart file", "failed": str(e)}
 
     value = cells * args.steps / (ms * 1e-3)
     launch_ms = sweep_ms / max(sweep_launches, 1)
@@ -419,6 +446,7 @@ def main():
         "adjoint": adj,
         "radiation": rad,
         "precipitation": precip,
+        "restart": rst,
     }
     print(json.dumps(out))
 

@@ -63,6 +63,7 @@ enum atjup_field {
     ATJUP_F_P_RAIN, ATJUP_F_P_SNOW, ATJUP_F_P_GRAUPEL,              /* the precipitation's fluxes */
     ATJUP_F_P_NH3_RAIN, ATJUP_F_P_NH3_SNOW, ATJUP_F_P_NH3_GRAUPEL,
     ATJUP_F_P_CH4_RAIN, ATJUP_F_P_CH4_SNOW, ATJUP_F_P_CH4_GRAUPEL, ATJUP_F_P_NH4SH,
+    ATJUP_F_DIFFLUX_H2S, ATJUP_F_DIFFLUX_NH3, ATJUP_F_THERMALMASSFLUX,   /* the chemistry's carried fields */
     ATJUP_F_END,
     ATJUP_NLEVEL = 64                               /* ATJUP_NLEVEL + prognostic id */
 };
@@ -213,6 +214,18 @@ void atjup_hip_default_precip_params(atjup_precip_params* params);
 /* The surface precipitation maps the last call left (precip_srf_h2o, _nh3, _ch4, _nh4sh, _total,
  * planet/Precipitation.h:470-492), each [jm][km], in that order: 5 * jm * km doubles. */
 int atjup_hip_download_precip_surface(atjup_ctx* ctx, double* dst);
+/* cJupiterModel::save_state / load_state (planet/FileIO_Jup.cpp:584-684): the binary restart file,
+ * byte for byte the reference's layout: int32 {magic, im, jm, km, iter}, then the 38 fixed 3D arrays
+ * of restart_arrays() (:281-293), in JUP2 files the 17 carried ones of restart_arrays_carried()
+ * (:297-305) and vel_star, each [i][j][k] (or [j][k]) FP64. carried = 1 writes "JUP2" (the
+ * reference's default ATJUP_RESTART_CARRIED=1), 0 writes "JUP1". The n-level copies are written
+ * from the live fields, as the reference writes them after restoreVar. The FP32-refined form
+ * writes its values widened to FP64 (t, u, v, w as hi + lo).
+ * load_state reads either magic, rejects a file of another grid or a truncated one
+ * (ATJUP_ERR_STATE), and rejects n-level arrays that differ from their fields: this path keeps one
+ * copy. iter receives the header's iteration; carried_out whether the file held the carried set. */
+int atjup_hip_save_state(atjup_ctx* ctx, const char* path, int iter, int carried);
+int atjup_hip_load_state(atjup_ctx* ctx, const char* path, int* iter, int* carried_out);
 /* Turbulence<cJupiterModel>::run (planet/Turbulence.h:307-1106) followed by the wall-function
  * omega Run() applies after it (planet/cJupiterModel.cpp:356-374, 1144). */
 int atjup_hip_turbulence_run(atjup_ctx* ctx);

@@ -154,6 +154,8 @@ class Reference:
         L.ref_set_mask.argtypes = [_D]
         L.ref_call.argtypes = [C.c_char_p]
         L.ref_get_field2d.argtypes = [C.c_char_p, _D]
+        L.ref_save_state.argtypes = [C.c_char_p, C.c_int]
+        L.ref_load_state.argtypes = [C.c_char_p, C.c_int]
         L.ref_buoy_ref_level.restype = C.c_double
         L.ref_buoy_ref_level.argtypes = [C.c_int]
         L.ref_set_vel_star.argtypes = [_D]
@@ -179,6 +181,13 @@ class Reference:
             raise KeyError(name)
         return a
 
+    def save_state(self, directory, iteration):
+        """The reference's save_state: writes directory/jup_restart_<iteration>.bin."""
+        self.L.ref_save_state(str(directory).encode(), int(iteration))
+
+    def load_state_file(self, directory, iteration):
+        return self.L.ref_load_state(str(directory).encode(), int(iteration)) == 0
+
     def get2d(self, name):
         a = np.empty(self.shape[1:])
         if self.L.ref_get_field2d(name.encode(), _dp(a)) != 0:
@@ -202,6 +211,11 @@ class Reference:
     def set_vel_star(self, vs):
         a = np.ascontiguousarray(vs, dtype=np.float64)
         self.L.ref_set_vel_star(_dp(a))
+
+    def get_vel_star(self):
+        a = np.empty(self.shape[1:])
+        self.L.ref_get_vel_star(_dp(a))
+        return a
 
     def buoy_ref_level(self):
         return np.array([self.L.ref_buoy_ref_level(i) for i in range(self.shape[0])])

@@ -69,8 +69,13 @@ public:
             {"P_nh3_rain", &M.P_nh3_rain}, {"P_nh3_snow", &M.P_nh3_snow}, {"P_nh3_graupel", &M.P_nh3_graupel},
             {"P_ch4_rain", &M.P_ch4_rain}, {"P_ch4_snow", &M.P_ch4_snow}, {"P_ch4_graupel", &M.P_ch4_graupel},
             {"P_nh4sh", &M.P_nh4sh},
+            {"difflux_h2s", &M.difflux_h2s}, {"difflux_nh3", &M.difflux_nh3},
+            {"thermalmassflux", &M.thermalmassflux},
         };
     }
+
+    static void save_state(const char* dir, int iter){ m->output_path = dir; m->save_state(iter); }
+    static bool load_state(const char* dir, int iter){ m->output_path = dir; return m->load_state(iter); }
 
     static std::map<std::string, Array_2D*> fields2d(){
         cJupiterModel& M = *m;
@@ -328,6 +333,20 @@ int ref_get_scalar(const char* name, double* v){
     if(!p) return -1;
     *v = *p;
     return 0;
+}
+
+// cJupiterModel::save_state / load_state (FileIO_Jup.cpp:584-684) with output_path = dir
+int ref_save_state(const char* dir, int iter){
+    quiet_begin();
+    H::save_state(dir, iter);
+    quiet_end();
+    return 0;
+}
+int ref_load_state(const char* dir, int iter){
+    quiet_begin();
+    const bool ok = H::load_state(dir, iter);
+    quiet_end();
+    return ok ? 0 : -1;
 }
 
 void ref_set_mask(const double* seamount){ quiet_begin(); H::set_mask(seamount); quiet_end(); }

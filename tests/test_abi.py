@@ -45,7 +45,8 @@ def test_default_radiation_params_match_mirror():
         assert getattr(p, name) == getattr(q, name), name
     assert ctypes.sizeof(abi.RadParams) == 9 * 8 + 2 * 4
     assert abi.FIELD["epsilon"] == 47 and abi.FIELD["radiation"] == 48
-    assert abi.FIELD["P_rain"] == 49 and abi.FIELD["P_nh4sh"] == 58 and abi.F_END == 59 < abi.NLEVEL
+    assert abi.FIELD["P_rain"] == 49 and abi.FIELD["P_nh4sh"] == 58
+    assert abi.FIELD["thermalmassflux"] == 61 and abi.F_END == 62 < abi.NLEVEL
 
 
 def test_default_precipitation_params_match_mirror():

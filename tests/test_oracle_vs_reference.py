@@ -178,3 +178,54 @@ def test_precipitation_bit_exact_against_compiled_reference(coupling, scale, inv
                        capture_output=True, text=True, timeout=600)
     print(r.stdout[-500:])
     assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+# The restart file: the numpy restatement of the format (oracle/restart.py) writes the reference's
+# save_state bytes exactly and reads them back, and the reference's load_state reads what it
+# writes; JUP2 (the default) and JUP1 (ATJUP_RESTART_CARRIED=0, cached by the reference: a process
+# of its own). The carried arrays and vel_star hold seeded values so that every slot is exercised.
+SCRIPT_RESTART = r'''
+import os, sys, filecmp, tempfile, numpy as np
+sys.path.insert(0, %(root)r)
+from atjup_amd import abi, inputs
+import oracle
+from oracle import restart
+carried = %(carried)r
+os.environ["ATJUP_RESTART_CARRIED"] = "1" if carried else "0"
+G, Cn, K, env = abi.make_profile("full", sweeps=8)
+st = inputs.make_state("full", seed=11)
+ref = oracle.Reference(env); ref.load_state(st, Cn)
+ref.call("restore")
+rng = np.random.default_rng(4)
+for n in abi.RESTART_CARRIED:
+    ref.set(n, rng.standard_normal(st.grid.shape))
+ref.set_vel_star(rng.standard_normal(st.grid.shape[1:]))
+with tempfile.TemporaryDirectory() as d:
+    ref.save_state(d, 17)
+    theirs = os.path.join(d, "jup_restart_17.bin")
+    fields = {n: ref.get(n) for n in abi.RESTART_FIXED + abi.RESTART_CARRIED}
+    mine = os.path.join(d, "mine.bin")
+    restart.write(mine, fields, ref.get_vel_star(), 17, carried=carried)
+    if not filecmp.cmp(theirs, mine, shallow=False): print("BYTES DIFFER"); sys.exit(1)
+    it, car, got, vs = restart.read(theirs, st.grid.shape)
+    if it != 17 or car != carried: print("HEADER"); sys.exit(1)
+    for n in got:
+        if not np.array_equal(got[n], fields[n]): print("READ", n); sys.exit(1)
+    # the reference loads a file the restatement wrote, with new values
+    new = {n: v + 1.0 for n, v in fields.items()}
+    restart.write(os.path.join(d, "jup_restart_18.bin"), new, ref.get_vel_star() + 1.0, 18, carried=carried)
+    os.remove(theirs); os.remove(mine)
+    if not ref.load_state_file(d, 18): print("LOAD FAILED"); sys.exit(1)
+    names = abi.RESTART_FIXED + (abi.RESTART_CARRIED if carried else ())
+    for n in names:
+        if not np.array_equal(ref.get(n), new[n]): print("LOADED", n); sys.exit(1)
+print("ALL-EQUAL")
+'''
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("carried", [True, False], ids=["JUP2", "JUP1"])
+def test_restart_format_against_compiled_reference(carried):
+    r = subprocess.run([sys.executable, "-c", SCRIPT_RESTART % {"root": ROOT, "carried": carried}],
+                       capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
