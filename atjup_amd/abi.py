@@ -23,7 +23,7 @@ INPUTS = ("p_dyn", "p_stat", "p_hydro", "rho_mix", "wall_nue", "nue", "prod", "t
           "epsilon", "radiation",
           "P_rain", "P_snow", "P_graupel", "P_nh3_rain", "P_nh3_snow", "P_nh3_graupel",
           "P_ch4_rain", "P_ch4_snow", "P_ch4_graupel", "P_nh4sh",
-          "difflux_h2s", "difflux_nh3", "thermalmassflux")
+          "difflux_h2s", "difflux_nh3", "thermalmassflux", "difflux_nh4sh")
 NLEVEL = 64
 
 FIELD = {n: i for i, n in enumerate(PROG)}
@@ -59,7 +59,33 @@ class Knobs(C.Structure):
                 ("tke_max_phys", C.c_double), ("nue_max_phys", C.c_double),
                 ("wall_nue_factor", C.c_double), ("wall_nue_layers", C.c_int),
                 ("satadj", C.c_int), ("rk_scalar_sync", C.c_int), ("dry", C.c_int),
-                ("turb_k_bg_phys", C.c_double), ("radiation", C.c_int), ("precip", C.c_int)]
+                ("turb_k_bg_phys", C.c_double), ("radiation", C.c_int), ("precip", C.c_int),
+                ("chemistry", C.c_int)]
+
+
+CHEM_DOUBLES = ("m_nh3", "m_h2s", "m_nh4sh", "m_h2", "m_he", "m_h2o", "X_h2", "X_he", "X_h2o", "X_h2s",
+                "X_nh3", "X_nh4sh", "mue_nh3", "mue_h2s", "mue_nh4sh", "rg_nh3", "rg_h2s", "k_mix",
+                "cp_nh3", "cp_h2s", "cp_nh4sh", "k_nh3", "k_h2s", "k_nh4sh", "dh_nh4sh", "t_0_nh4sh",
+                "t_00_nh4sh")
+
+
+class ChemParams(C.Structure):
+    """atjup_chem_params: the chemistry's constants (planet/cJupiterModel.h:172-401) and knobs."""
+    _fields_ = [(n, C.c_double) for n in CHEM_DOUBLES] + [
+        ("gate_zero", C.c_int), ("molar_conc", C.c_int), ("enthalpy", C.c_int)]
+
+
+# the values the reference's start-up leaves (read from the compiled reference's members)
+CHEM_DEFAULTS = dict(zip(CHEM_DOUBLES, (
+    17.03052, 34.08088, 51.1114, 2.016, 4.0026, 18.01588, 0.864, 0.136, 0.0017, 7.7e-05, 0.0002, 0.0,
+    9.2e-06, 1.3e-05, 0.0, 817.0, 949.0, 0.12478002943205087, 2190.0, 2240.0, 2000.0, 0.02102, 0.013,
+    0.0, 1768700.0, 230.0, 200.0)), gate_zero=1, molar_conc=0, enthalpy=0)
+
+
+def chem_params(**over) -> ChemParams:
+    d = dict(CHEM_DEFAULTS)
+    d.update(over)
+    return ChemParams(**d)
 
 
 class PrecipParams(C.Structure):
@@ -140,7 +166,7 @@ TURB_MODELS = {"off": -1, "k_epsilon": 0, "k_omega": 1, "k_omega_SST": 2}
 
 def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
                  grid: inputs.Grid | None = None, radiation: bool = False, precip: bool = False,
-                 precip_coupling: float = 1.0):
+                 precip_coupling: float = 1.0, chemistry: bool = False):
     """(Grid, Consts, Knobs, env) of a configuration.
 
     "dyn"  : dynamics only, dry, closure/radiation/precipitation off (the 5-step benchmark case).
@@ -150,6 +176,8 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
              default), so Q_rad follows the state instead of entering as a frozen input.
     precip:  the precipitation microphysics runs after the saturation adjustments (ATJUP_PRECIP=1,
              the reference's default); with precip_coupling != 0 it reports S_precip_* rates.
+    chemistry: the physics block's DiffMassFluxJup / ChemMassRateJup and their smoothing run in the
+             step (the reference always runs them), so massflux_* follow the state.
     """
     g = grid or inputs.Grid()
     G = Grid(im=g.im, jm=g.jm, km=g.km, r0=g.radius_km / g.L_atm, dr=g.dr, dthe=g.dthe,
@@ -167,7 +195,7 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
               tke_max_phys=1000.0, nue_max_phys=1.0e5, wall_nue_factor=4.0, wall_nue_layers=3,
               satadj=1, rk_scalar_sync=2, dry=1 if config == "dyn" else 0,
               turb_k_bg_phys=1.0e-4 if config != "dyn" else 1.0, radiation=1 if radiation else 0,
-              precip=1 if precip else 0)
+              precip=1 if precip else 0, chemistry=1 if chemistry else 0)
     env = dict(BASE_ENV)
     env["ATJUP_PRESS_SWEEPS"] = str(sweeps)
     env["ATJUP_RADIATION"] = "1" if radiation else "0"

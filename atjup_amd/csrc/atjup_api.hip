@@ -130,6 +130,10 @@ int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, i
     if(!(k->press_omega > 0.0 && k->press_omega < 2.0)){ set_error("press_omega outside (0, 2)"); return ATJUP_ERR_PROFILE; }
     if(k->radiation != 0 && k->radiation != 1){ set_error("radiation must be 0 or 1"); return ATJUP_ERR_PROFILE; }
     if(k->precip != 0 && k->precip != 1){ set_error("precip must be 0 or 1"); return ATJUP_ERR_PROFILE; }
+    if(k->chemistry != 0 && k->chemistry != 1){ set_error("chemistry must be 0 or 1"); return ATJUP_ERR_PROFILE; }
+    if(k->chemistry && precision == ATJUP_PRECISION_TANGENT){
+        set_error("the tangent form has no chemistry"); return ATJUP_ERR_PROFILE;
+    }
     if(k->precip && precision == ATJUP_PRECISION_TANGENT){
         set_error("the tangent form has no precipitation"); return ATJUP_ERR_PROFILE;
     }
@@ -216,8 +220,12 @@ int step_block(atjup_ctx* c, int iter, int first){
             if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
         if(c->kn.precip && (rc = atjup_hip_precipitation(c))) return rc;      // cJupiterModel.cpp:1090
         if((rc = atjup_hip_turbulence_run(c))) return rc;
-        if((rc = launch_flux_limiter(c))) return rc;
-        if((rc = launch_shapiro(c, ATJUP_F_FLUXLIM_NH4SH, 2, 1.0, 1, 7))) return rc;
+        if(c->kn.chemistry){
+            if((rc = atjup_hip_chemistry(c))) return rc;                // :1146-1161, limiter included
+        } else {
+            if((rc = launch_flux_limiter(c))) return rc;
+            if((rc = launch_shapiro(c, ATJUP_F_FLUXLIM_NH4SH, 2, 1.0, 1, 7))) return rc;
+        }
     }
     return 0;
 }
@@ -377,6 +385,10 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
         atjup_hip_default_precip_params(&c->precp);
         rc = alloc(c->precip_srf, (size_t)5 * c->jm * c->km * sizeof(double));
     }
+    if(!rc){
+        atjup_hip_default_chem_params(&c->chemp);
+        rc = alloc(c->chem_w, 3 * c->n3 * sizeof(double));
+    }
     if(rc){ atjup_hip_ctx_destroy(c); return nullptr; }
     return c;
 }
@@ -392,7 +404,7 @@ void atjup_hip_ctx_destroy(atjup_ctx* c){
     for(atjup::Buf* b : { &c->lo[0], &c->lo[1], &c->lo[2], &c->lo[3], &c->colr }) release(*b);
     for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
-                          &c->amb_rows, &c->nonfinite, &c->rad_tab, &c->rad_tmp, &c->precip_srf })
+                          &c->amb_rows, &c->nonfinite, &c->rad_tab, &c->rad_tmp, &c->precip_srf, &c->chem_w })
         release(*b);
     for(auto& v : c->spans) for(auto& sp : v){ (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
     for(auto e : c->event_pool) (void)hipEventDestroy(e);
@@ -709,6 +721,33 @@ int atjup_hip_download_precip_surface(atjup_ctx* c, double* dst){
     ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
     ATJUP_HIP_CHECK(hipMemcpy(dst, c->precip_srf.p, c->precip_srf.bytes, hipMemcpyDeviceToHost));
     return 0;
+}
+void atjup_hip_default_chem_params(atjup_chem_params* p){
+    if(!p) return;
+    // planet/cJupiterModel.h:172-401 as the start-up leaves them (k_mix the mixture conductivity of
+    // the default composition), GATE_ZERO 1, MOLAR_CONC 0, ENTHALPY 0 (planet/ChemistryJup.h)
+    *p = atjup_chem_params{ 17.03052, 34.08088, 51.1114, 2.016, 4.0026, 18.01588,
+                            0.864, 0.136, 0.0017, 7.7e-05, 0.0002, 0.0,
+                            9.2e-06, 1.3e-05, 0.0, 817.0, 949.0, 0.12478002943205087,
+                            2190.0, 2240.0, 2000.0, 0.02102, 0.013, 0.0, 1768700.0, 230.0, 200.0,
+                            1, 0, 0 };
+}
+int atjup_hip_set_chemistry(atjup_ctx* c, const atjup_chem_params* p){
+    if(!c || !p){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    c->chemp = *p;
+    return 0;
+}
+int atjup_hip_chemistry(atjup_ctx* c){
+    if(int rc = need_mask(c)) return rc;
+    if(c->precision == ATJUP_PRECISION_TANGENT){ set_error("the tangent form has no chemistry"); return ATJUP_ERR_PROFILE; }
+    static const int diff[4] = { ATJUP_F_DIFFLUX_H2S, ATJUP_F_DIFFLUX_NH3, ATJUP_F_DIFFLUX_NH4SH, ATJUP_F_THERMALMASSFLUX };
+    static const int mass[4] = { ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH };
+    int rc = launch_chem_kernel(c, 0);
+    for(int f : diff) if(!rc) rc = launch_shapiro(c, f, 2, 1.0, 1, 7);
+    if(!rc) rc = launch_chem_kernel(c, 1);
+    if(!rc) rc = launch_flux_limiter(c);
+    for(int f : mass) if(!rc) rc = launch_shapiro(c, f, 2, 1.0, 1, 7);
+    return rc;
 }
 int atjup_hip_turbulence_run(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;

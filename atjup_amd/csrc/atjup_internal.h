@@ -110,6 +110,8 @@ struct atjup_ctx {
     atjup::Buf rad_tmp;             // FP32-refined form: the radiation's FP64 eps, B, Fsw [3][n3]
     atjup_precip_params precp = {}; // the precipitation's constants (atjup_hip_set_precipitation)
     atjup::Buf precip_srf;          // double [5][jm][km]: the surface precipitation maps
+    atjup_chem_params chemp = {};   // the chemistry's constants and knobs (atjup_hip_set_chemistry)
+    atjup::Buf chem_w;              // double [3][n3]: the reaction rates w_nh3, w_h2s, w_nh4sh
     atjup_gas_params gas[3] = {};   // saturation-adjustment constants (atjup_hip_set_gas)
     bool gas_set[3] = { false, false, false };
     int n_solids = 0, n_amb_rows = 0;
@@ -159,6 +161,7 @@ int launch_turb_sanitize(atjup_ctx* c);
 int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g);
 int launch_radiation(atjup_ctx* c);
 int launch_precipitation(atjup_ctx* c);
+int launch_chem_kernel(atjup_ctx* c, int which);      // 0: DiffMassFluxJup, 1: ChemMassRateJup
 int build_rad_table(atjup_ctx* c);
 int launch_flux_limiter(atjup_ctx* c);
 int launch_commit_live(atjup_ctx* c);

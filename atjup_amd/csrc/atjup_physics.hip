@@ -1509,6 +1509,216 @@ int do_precipitation(atjup_ctx* c){
     return check_launch();
 }
 
+// =============================================================================================
+// ChemistryJup (ChemistryJup.h): DiffMassFluxJup (:375-540) and ChemMassRateJup (:59-213).
+// k_diffmassflux: per cell of i 1..im-2, j 1..jm-2, k 0..km-2 (the periodic seam), the first
+//   derivatives of t, H2S, NH3, NH4SH (central, 0 across two solid neighbours; at k = 0 the
+//   one-sided phi form of derivative_1_order_boundary), the multicomponent fluxes j_*, the
+//   thermo-diffusion jT_*, the Fickian difflux_* = D laplacian and thermalmassflux, which reads
+//   the reaction rates w_* the previous call left; solid cells and t <= 0 get zeros.
+// k_chemmassrate: the polar bands j < 3 and j > jm-4 zeroed; per cell of i 1..im-2, j 3..jm-4,
+//   k 0..km-2 the NH4SH formation rate law inside [t_00_nh4sh, t_0_nh4sh] (zero outside with
+//   GATE_ZERO, else the last rate), a solid cell with six solid neighbours zeroed, and
+//   massflux_* = w_* + difflux_*.
+// =============================================================================================
+struct ChemConsts {
+    double D_nh3, D_h2s, D_nh4sh, M_mix, L_nh3, L_h2s, L_nh4sh, LT_nh3, LT_h2s, LT_nh4sh;
+    double m_nh3, m_h2s, m_nh4sh, cp_nh3, cp_h2s, cp_nh4sh, k_nh3, k_h2s, k_nh4sh, dh_nh4sh;
+    double r_mix, t_ref, keq, t_0, t_00;
+    double dr_2, dthe_2, dphi_2, dr2, dthe2, dphi2;
+    int gate_zero, molar_conc, enthalpy;
+};
+constexpr double CHEM_FLOOR = 0.4;                 // polar_divisor_floor, SINTHE_TRACK 0 (ATPhys.h:104-108)
+
+template<typename T>
+struct ChemFields {
+    const T *t, *h2s, *nh3, *nh4sh, *rho;
+    const float* lo_t;
+    T *dfl_h2s, *dfl_nh3, *dfl_nh4sh, *tmf, *mf_h2s, *mf_nh3, *mf_nh4sh;
+    double *w_nh3, *w_h2s, *w_nh4sh;
+};
+
+// derivative_1_order (ChemistryJup.h:653-727) at a fluid cell: central unless both neighbours of
+// the direction are solid (then 0)
+__device__ __forceinline__ double chem_d1(double cm, double cp, bool sm, bool sp, double two_d){
+    return (sm && sp) ? 0.0 : (cp - cm) / two_d;
+}
+
+template<typename T>
+__global__ __launch_bounds__(64) void k_diffmassflux(const ChemFields<T> F, const uint8_t* __restrict__ flags,
+        const double* __restrict__ tabs, int im, int jm, int km, int pitch, const ChemConsts C){
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y + 1, i = blockIdx.z + 1;
+    if(k > km - 2) return;
+    const size_t P = (size_t)jm * pitch, R = (size_t)pitch;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    const size_t row = q - k;
+    const int kw = (k == 0) ? km - 2 : k - 1, ke = (k == km - 2) ? 0 : k + 1;
+    const uint8_t fl = flags[q];
+    double tq = (double)F.t[q];
+    if(F.lo_t) tq += (double)F.lo_t[q];
+    if(!(tq > 0.0) || (fl & FL_SELF)){
+        F.dfl_h2s[q] = T(0); F.dfl_nh3[q] = T(0); F.dfl_nh4sh[q] = T(0); F.tmf[q] = T(0);
+        return;
+    }
+    const double rm = tabs[i];
+    const double sin_t = tabs[im + j], cos_t = tabs[im + 2 * jm + j];
+    const double sinthe = dmax(sin_t, CHEM_FLOOR);
+    const double rmsinthe = rm * sinthe;
+    const bool sim = fl & FL_IM, sip = fl & FL_IP, sjm = fl & FL_JM, sjp = fl & FL_JP;
+    const bool skm = fl & FL_KM, skp = fl & FL_KP;
+    auto ld = [&](const T* X, size_t qq){
+        double v = (double)X[qq];
+        if(X == F.t && F.lo_t) v += (double)F.lo_t[qq];
+        return v;
+    };
+    // first derivatives of a field: r, theta central (0 across two solid neighbours); phi central,
+    // one-sided at k = 0 (derivative_1_order_boundary, ChemistryJup.h:729-750)
+    auto d3 = [&](const T* X, double& dr, double& dth, double& dph){
+        const double c = ld(X, q);
+        dr = chem_d1(ld(X, q - P), ld(X, q + P), sim, sip, C.dr_2);
+        dth = chem_d1(ld(X, q - R), ld(X, q + R), sjm, sjp, C.dthe_2);
+        if(k == 0) dph = (-3.0 * c + 4.0 * ld(X, row + 1) - ld(X, row + 2)) / C.dphi_2;
+        else dph = chem_d1(ld(X, row + kw), ld(X, row + ke), skm, skp, C.dphi_2);
+    };
+    double dtdr, dtdthe, dtdphi, dnh3dr, dnh3dthe, dnh3dphi, dh2sdr, dh2sdthe, dh2sdphi, dnhdr, dnhdthe, dnhdphi;
+    d3(F.t, dtdr, dtdthe, dtdphi);
+    d3(F.h2s, dh2sdr, dh2sdthe, dh2sdphi);
+    d3(F.nh3, dnh3dr, dnh3dthe, dnh3dphi);
+    d3(F.nh4sh, dnhdr, dnhdthe, dnhdphi);
+    const double dt_div = dtdr + fabs(dtdthe) / rm + dtdphi / rmsinthe;
+    const double rf = (double)F.rho[q];
+    const double rho_c = (rf > 0.0 && isfinite(rf)) ? rf : C.r_mix;
+    const double sc = rho_c / C.r_mix;
+    const double jT_nh3 = (C.LT_nh3 * sc) / tq * dt_div;
+    const double jT_h2s = (C.LT_h2s * sc) / tq * dt_div;
+    const double jT_nh4sh = (C.LT_nh4sh * sc) / tq * dt_div;
+    const double dnh3_div = dnh3dr + fabs(dnh3dthe) / rm + dnh3dphi / rmsinthe;
+    const double dh2s_div = dh2sdr + fabs(dh2sdthe) / rm + dh2sdphi / rmsinthe;
+    const double dnh4sh_div = dnhdr + fabs(dnhdthe) / rm + dnhdphi / rmsinthe;
+    const double L_nh3_c = C.L_nh3 * sc, L_h2s_c = C.L_h2s * sc, L_nh4sh_c = C.L_nh4sh * sc;
+    const double nh3 = (double)F.nh3[q], h2s = (double)F.h2s[q], nh4sh = (double)F.nh4sh[q];
+    const double cM_nh3 = L_nh3_c * nh3 / rho_c;
+    const double cM_h2s = L_h2s_c * h2s / rho_c;
+    const double cM_nh4sh = L_nh4sh_c * nh4sh / rho_c;
+    const double M = C.M_mix;
+    const double j_nh3 = C.m_nh3 / M * (L_nh3_c * dnh3_div - M / C.m_nh3 * cM_nh3 * dnh3_div
+        - M / C.m_h2s * cM_nh3 * dh2s_div - M / C.m_nh4sh * cM_nh3 * dnh4sh_div) - jT_nh3;
+    const double j_h2s = C.m_h2s / M * (L_h2s_c * dh2s_div - M / C.m_nh3 * cM_h2s * dnh3_div
+        - M / C.m_h2s * cM_h2s * dh2s_div - M / C.m_nh4sh * cM_h2s * dnh4sh_div) - jT_h2s;
+    const double j_nh4sh = C.m_nh4sh / M * (L_nh4sh_c * dnh4sh_div - M / C.m_nh3 * cM_nh4sh * dnh3_div
+        - M / C.m_h2s * cM_nh4sh * dh2s_div - M / C.m_nh4sh * cM_nh4sh * dnh4sh_div) - jT_nh4sh;
+    // laplacian_spherical (ChemistryJup.h:752-776)
+    auto lap = [&](const T* X){
+        const double c = ld(X, q);
+        const double rmsin = rm * dmax(sin_t, CHEM_FLOOR);
+        const double d2r = (ld(X, q + P) - 2.0 * c + ld(X, q - P)) / C.dr2 * 1.0;
+        const double d2t = (ld(X, q + R) - 2.0 * c + ld(X, q - R)) / C.dthe2;
+        const double d2p = (ld(X, row + ke) - 2.0 * c + ld(X, row + kw)) / C.dphi2;
+        const double d1r = (ld(X, q + P) - ld(X, q - P)) / C.dr_2 * 1.0;
+        const double d1t = (ld(X, q + R) - ld(X, q - R)) / C.dthe_2;
+        return d2r + 2.0 / rm * d1r + d2t / (rm * rm) + cos_t / (rm * rmsin) * d1t + d2p / (rmsin * rmsin);
+    };
+    F.dfl_h2s[q] = (T)(C.D_h2s * lap(F.h2s));
+    F.dfl_nh3[q] = (T)(C.D_nh3 * lap(F.nh3));
+    F.dfl_nh4sh[q] = (T)(C.D_nh4sh * lap(F.nh4sh));
+    const double wt = C.enthalpy ? F.w_nh4sh[q] * C.dh_nh4sh
+        : tq * (F.w_nh3[q] * C.k_nh3 + F.w_h2s[q] * C.k_h2s + F.w_nh4sh[q] * C.k_nh4sh);
+    F.tmf[q] = (T)((j_nh3 * C.cp_nh3 + j_h2s * C.cp_h2s + j_nh4sh * C.cp_nh4sh) * dt_div + wt);
+}
+
+template<typename T>
+__global__ __launch_bounds__(64) void k_chemmassrate(const ChemFields<T> F, const uint8_t* __restrict__ msk,
+        int im, int jm, int km, int pitch, const ChemConsts C){
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= km) return;
+    const size_t P = (size_t)jm * pitch, R = (size_t)pitch;
+    const size_t q = ((size_t)i * jm + j) * pitch + k;
+    if(j < 3 || j > jm - 4){
+        F.w_nh3[q] = 0.0; F.w_h2s[q] = 0.0; F.w_nh4sh[q] = 0.0;
+        F.mf_nh3[q] = T(0); F.mf_h2s[q] = T(0); F.mf_nh4sh[q] = T(0);
+        return;
+    }
+    if(i < 1 || i > im - 2 || k > km - 2) return;
+    const size_t row = q - k;
+    const int kw = (k == 0) ? km - 2 : k - 1, ke = (k == km - 2) ? 0 : k + 1;
+    if(msk[q] && msk[q - P] && msk[q + P] && msk[q - R] && msk[q + R] && msk[row + kw] && msk[row + ke]){
+        F.w_nh3[q] = 0.0; F.w_h2s[q] = 0.0; F.w_nh4sh[q] = 0.0;
+        F.mf_nh3[q] = T(0); F.mf_h2s[q] = T(0); F.mf_nh4sh[q] = T(0);
+        return;
+    }
+    double tq = (double)F.t[q];
+    if(F.lo_t) tq += (double)F.lo_t[q];
+    const double t_u = tq * C.t_ref;
+    const double kf = 15000.0 * pow(t_u, -0.5) * exp(-3020.0 / t_u);        // react_rate_const (:648-651)
+    const double kb = kf / C.keq;
+    double w_nh3 = F.w_nh3[q], w_h2s = F.w_h2s[q], w_nh4sh = F.w_nh4sh[q];
+    if(t_u <= C.t_0 && t_u >= C.t_00){
+        const double nh3 = (double)F.nh3[q], h2s = (double)F.h2s[q], nh4sh = (double)F.nh4sh[q];
+        double denom;
+        if(C.molar_conc){
+            double rho = (double)F.rho[q];
+            if(!(rho > 0.0)) rho = C.r_mix;
+            denom = rho;
+        } else {
+            const double sum_c = nh3 / C.m_nh3 + h2s / C.m_h2s + nh4sh / C.m_nh4sh;
+            denom = (sum_c == 0.0) ? 0.0 : C.r_mix / sum_c;
+        }
+        const double c_nh3 = nh3 / C.m_nh3 * denom, c_h2s = h2s / C.m_h2s * denom, c_nh4sh = nh4sh / C.m_nh4sh * denom;
+        const double R_diff = kf * c_nh3 * c_h2s - kb * c_nh4sh;
+        w_nh3 = -C.m_nh3 * R_diff;
+        w_h2s = -C.m_h2s * R_diff;
+        w_nh4sh = C.m_nh4sh * R_diff;
+    } else if(C.gate_zero){
+        w_nh3 = 0.0; w_h2s = 0.0; w_nh4sh = 0.0;
+    }
+    F.w_nh3[q] = w_nh3; F.w_h2s[q] = w_h2s; F.w_nh4sh[q] = w_nh4sh;
+    F.mf_h2s[q] = (T)(w_h2s + (double)F.dfl_h2s[q]);
+    F.mf_nh3[q] = (T)(w_nh3 + (double)F.dfl_nh3[q]);
+    F.mf_nh4sh[q] = (T)(w_nh4sh + (double)F.dfl_nh4sh[q]);
+}
+
+template<typename T>
+int do_chem_kernels(atjup_ctx* c, int which){
+    const atjup_chem_params& p = c->chemp;
+    ChemFields<T> Fp;
+    Fp.t = F<T>(c, ATJUP_F_T); Fp.h2s = F<T>(c, ATJUP_F_H2S); Fp.nh3 = F<T>(c, ATJUP_F_NH3);
+    Fp.nh4sh = F<T>(c, ATJUP_F_NH4SH); Fp.rho = F<T>(c, ATJUP_F_RHO_MIX);
+    Fp.lo_t = (const float*)c->lo[ATJUP_F_T].p;
+    Fp.dfl_h2s = F<T>(c, ATJUP_F_DIFFLUX_H2S); Fp.dfl_nh3 = F<T>(c, ATJUP_F_DIFFLUX_NH3);
+    Fp.dfl_nh4sh = F<T>(c, ATJUP_F_DIFFLUX_NH4SH); Fp.tmf = F<T>(c, ATJUP_F_THERMALMASSFLUX);
+    Fp.mf_h2s = F<T>(c, ATJUP_F_MASSFLUX_H2S); Fp.mf_nh3 = F<T>(c, ATJUP_F_MASSFLUX_NH3);
+    Fp.mf_nh4sh = F<T>(c, ATJUP_F_MASSFLUX_NH4SH);
+    Fp.w_nh3 = (double*)c->chem_w.p; Fp.w_h2s = Fp.w_nh3 + c->n3; Fp.w_nh4sh = Fp.w_h2s + c->n3;
+    // the constants of DiffMassFluxJup (:383-400) and ChemMassRateJup (:66-69)
+    ChemConsts C;
+    const double DT_nh3 = p.mue_nh3 / (p.rg_nh3 * c->c.sc_nh3);
+    const double DT_h2s = p.mue_h2s / (p.rg_h2s * c->c.sc_h2s);
+    const double DT_nh4sh = p.mue_nh4sh / (c->c.rg_nh4sh * c->c.sc_nh4sh);
+    C.D_nh3 = DT_nh3; C.D_h2s = DT_h2s; C.D_nh4sh = DT_nh4sh;
+    C.M_mix = p.X_h2 * p.m_h2 + p.X_he * p.m_he + p.X_h2o * p.m_h2o + p.X_h2s * p.m_h2s
+            + p.X_nh3 * p.m_nh3 + p.X_nh4sh * p.m_nh4sh;
+    const double rc = c->c.r_mix * c->c.cp_mix;
+    C.L_nh3 = rc * DT_nh3 / p.k_mix; C.L_h2s = rc * DT_h2s / p.k_mix; C.L_nh4sh = rc * DT_nh4sh / p.k_mix;
+    C.LT_nh3 = C.L_nh3; C.LT_h2s = C.L_h2s; C.LT_nh4sh = C.L_nh4sh;
+    C.m_nh3 = p.m_nh3; C.m_h2s = p.m_h2s; C.m_nh4sh = p.m_nh4sh;
+    C.cp_nh3 = p.cp_nh3; C.cp_h2s = p.cp_h2s; C.cp_nh4sh = p.cp_nh4sh;
+    C.k_nh3 = p.k_nh3; C.k_h2s = p.k_h2s; C.k_nh4sh = p.k_nh4sh; C.dh_nh4sh = p.dh_nh4sh;
+    C.r_mix = c->c.r_mix; C.t_ref = c->c.t_ref; C.keq = p.m_nh4sh / (p.m_nh3 * p.m_h2s);
+    C.t_0 = p.t_0_nh4sh; C.t_00 = p.t_00_nh4sh;
+    C.dr_2 = 2.0 * c->g.dr; C.dthe_2 = 2.0 * c->g.dthe; C.dphi_2 = 2.0 * c->g.dphi;
+    C.dr2 = c->g.dr * c->g.dr; C.dthe2 = c->g.dthe * c->g.dthe; C.dphi2 = c->g.dphi * c->g.dphi;
+    C.gate_zero = p.gate_zero; C.molar_conc = p.molar_conc; C.enthalpy = p.enthalpy;
+    hipEvent_t t0 = timer_begin(c);
+    if(which == 0)
+        hipLaunchKernelGGL(k_diffmassflux<T>, dim3(nblk(c->km - 1, 64), c->jm - 2, c->im - 2), dim3(64), 0, c->stream,
+            Fp, (const uint8_t*)c->flags.p, (const double*)c->tabs.p, c->im, c->jm, c->km, c->pitch, C);
+    else
+        hipLaunchKernelGGL(k_chemmassrate<T>, dim3(nblk(c->km, 64), c->jm, c->im), dim3(64), 0, c->stream,
+            Fp, (const uint8_t*)c->mask.p, c->im, c->jm, c->km, c->pitch, C);
+    timer_end(c, ATJUP_T_CHEMISTRY, t0, 1);
+    return check_launch();
+}
+
 }  // namespace
 
 // the tangent form covers the dynamics profile only
@@ -1524,6 +1734,7 @@ int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){ return DISP
 int launch_flux_limiter(atjup_ctx* c){ return DISPATCH(do_flux_limiter, c); }
 int launch_radiation(atjup_ctx* c){ return DISPATCH(do_radiation, c); }
 int launch_precipitation(atjup_ctx* c){ return DISPATCH(do_precipitation, c); }
+int launch_chem_kernel(atjup_ctx* c, int which){ return DISPATCH(do_chem_kernels, c, which); }
 int launch_commit_live(atjup_ctx* c){ return c->turb_live_pending ? DISPATCH(do_commit_live, c) : 0; }
 
 }  // namespace atjup

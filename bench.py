@@ -235,6 +235,24 @@ def precipitation_leg(ctx, g, calls=20):
             "hbm_frac": nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}
 
 
+def chemistry_leg(ctx, g, calls=20):
+    """The physics block's chemistry sequence (two kernels, eight smoothing passes and the flux
+    limiter): wall time per call with the device synchronised, and the two kernels' HIP-event time."""
+    ctx.chemistry()
+    ctx.sync()
+    ctx.set_timing(True)
+    ctx.read_timers()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        ctx.chemistry()
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / calls
+    k_ms, launches = ctx.read_timers()["chemistry"]
+    return {"workload": f"DiffMassFluxJup, ChemMassRateJup, the NH4SH limiter and eight Shapiro passes, "
+                        f"{g.im}x{g.jm}x{g.km}",
+            "ms_per_call": wall * 1e3, "kernels_ms_per_call": k_ms / calls, "kernel_launches_per_call": launches / calls}
+
+
 def restart_leg(ctx, g):
     """save_state then load_state of the device-resident state (JUP2, 55 arrays and vel_star), in a
     temporary directory on the host's disk: the D2H/H2D transfers and the file I/O together."""
@@ -375,6 +393,7 @@ def main():
     # physics block when the profile turns them on), timed on the state the FP64 run left
     rad = radiation_leg(ctx, g) if args.config == "full" else None
     precip = precipitation_leg(ctx, g) if args.config == "full" else None
+    chem = chemistry_leg(ctx, g) if args.config == "full" else None
     rst = None
     if not args.no_restart:
         try:
@@ -446,6 +465,7 @@ def main():
         "adjoint": adj,
         "radiation": rad,
         "precipitation": precip,
+        "chemistry": chem,
         "restart": rst,
     }
     print(json.dumps(out))

@@ -64,6 +64,7 @@ enum atjup_field {
     ATJUP_F_P_NH3_RAIN, ATJUP_F_P_NH3_SNOW, ATJUP_F_P_NH3_GRAUPEL,
     ATJUP_F_P_CH4_RAIN, ATJUP_F_P_CH4_SNOW, ATJUP_F_P_CH4_GRAUPEL, ATJUP_F_P_NH4SH,
     ATJUP_F_DIFFLUX_H2S, ATJUP_F_DIFFLUX_NH3, ATJUP_F_THERMALMASSFLUX,   /* the chemistry's carried fields */
+    ATJUP_F_DIFFLUX_NH4SH,
     ATJUP_F_END,
     ATJUP_NLEVEL = 64                               /* ATJUP_NLEVEL + prognostic id */
 };
@@ -113,7 +114,24 @@ typedef struct atjup_knobs {
     int    precip;              /* ATJUP_PRECIP: the precipitation microphysics after the saturation
                                    adjustments (default 1, cJupiterModel.cpp:83, :1090); with
                                    precip_coupling != 0 it reports rates into S_precip_* */
+    int    chemistry;           /* the physics block's chemistry (DiffMassFluxJup, ChemMassRateJup and
+                                   their smoothing, cJupiterModel.cpp:1146-1161, which the reference
+                                   always runs) in the step; 0: massflux_* enter as frozen inputs */
 } atjup_knobs;
+
+/* The chemistry's constants as the reference's start-up leaves them (planet/cJupiterModel.h:172-401:
+ * molar masses [kg/kmol], mole fractions, dynamic viscosities, densities, the mixture conductivity,
+ * specific heats, conductivities, the NH4SH reaction enthalpy, the NH4SH band [K]) and its knobs
+ * ATJUP_CHEM_GATE_ZERO (1), ATJUP_CHEM_MOLAR_CONC (0), ATJUP_CHEM_ENTHALPY (0)
+ * (planet/ChemistryJup.h:54-57, :123-126). ATJUP_NH4SH_EQ must be 0: the equilibrium
+ * adjustment is not built. */
+typedef struct atjup_chem_params {
+    double m_nh3, m_h2s, m_nh4sh, m_h2, m_he, m_h2o;
+    double X_h2, X_he, X_h2o, X_h2s, X_nh3, X_nh4sh;
+    double mue_nh3, mue_h2s, mue_nh4sh, rg_nh3, rg_h2s, k_mix;
+    double cp_nh3, cp_h2s, cp_nh4sh, k_nh3, k_h2s, k_nh4sh, dh_nh4sh, t_0_nh4sh, t_00_nh4sh;
+    int    gate_zero, molar_conc, enthalpy;
+} atjup_chem_params;
 
 /* The precipitation's constants beyond the gases' (which atjup_hip_set_gas supplies):
  * ATJUP_PRECIP_SCALE on the five rate coefficients (planet/Precipitation.h:111-115,
@@ -214,6 +232,14 @@ void atjup_hip_default_precip_params(atjup_precip_params* params);
 /* The surface precipitation maps the last call left (precip_srf_h2o, _nh3, _ch4, _nh4sh, _total,
  * planet/Precipitation.h:470-492), each [jm][km], in that order: 5 * jm * km doubles. */
 int atjup_hip_download_precip_surface(atjup_ctx* ctx, double* dst);
+/* The physics block's chemistry in the reference's order (planet/cJupiterModel.cpp:1146-1161):
+ * ChemistryJup::DiffMassFluxJup (planet/ChemistryJup.h:375-540), damp_wiggles of difflux_h2s,
+ * difflux_nh3, difflux_nh4sh and thermalmassflux, ChemMassRateJup (:59-213), FluxLimiterNH4SH,
+ * damp_wiggles of massflux_h2s, massflux_nh3, massflux_nh4sh and fluxlim_nh4sh. The reaction
+ * rates w_* persist in the context between calls, as the reference's arrays do. */
+int atjup_hip_chemistry(atjup_ctx* ctx);
+int atjup_hip_set_chemistry(atjup_ctx* ctx, const atjup_chem_params* params);
+void atjup_hip_default_chem_params(atjup_chem_params* params);
 /* cJupiterModel::save_state / load_state (planet/FileIO_Jup.cpp:584-684): the binary restart file,
  * byte for byte the reference's layout: int32 {magic, im, jm, km, iter}, then the 38 fixed 3D arrays
  * of restart_arrays() (:281-293), in JUP2 files the 17 carried ones of restart_arrays_carried()
@@ -271,6 +297,7 @@ enum atjup_timer {
     ATJUP_T_ADJ_RECOMPUTE,        /* the forward recomputation of one checkpoint segment */
     ATJUP_T_RADIATION,            /* the radiation's two kernels (layers, columns) */
     ATJUP_T_PRECIPITATION,        /* the precipitation's column kernel */
+    ATJUP_T_CHEMISTRY,            /* the chemistry's two kernels (the smoothing passes not included) */
     ATJUP_N_TIMERS
 };
 int atjup_hip_set_timing(atjup_ctx* ctx, int enable);

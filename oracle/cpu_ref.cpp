@@ -56,6 +56,12 @@ struct orc_ctx {
     atjup_rad_params radp = {};                      // radiation properties and knobs
     atjup_precip_params precp = { 1.0, 230.0, 200.0 }; // precipitation rate scale and NH4SH band
     std::vector<double> precip_srf;                  // [5][jm][km] surface maps
+    atjup_chem_params chemp = { 17.03052, 34.08088, 51.1114, 2.016, 4.0026, 18.01588,
+                                0.864, 0.136, 0.0017, 7.7e-05, 0.0002, 0.0,
+                                9.2e-06, 1.3e-05, 0.0, 817.0, 949.0, 0.12478002943205087,
+                                2190.0, 2240.0, 2000.0, 0.02102, 0.013, 0.0, 1768700.0, 230.0, 200.0,
+                                1, 0, 0 };             // the chemistry's constants (cJupiterModel.h:172-401)
+    std::vector<double> w_nh3, w_h2s, w_nh4sh;       // the reaction rates, persistent between calls
     double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
 
     size_t id(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
@@ -1732,6 +1738,159 @@ void precipitation(orc_ctx& o){
         }
 }
 
+// ---- ChemistryJup::DiffMassFluxJup (ChemistryJup.h:375-540) ----------------------------------
+// derivative_1_order at a fluid cell (:653-727): central unless both neighbours are solid
+inline double chem_d1(double cm, double cp, bool sm, bool sp, double two_d){
+    return (sm && sp) ? 0.0 : (cp - cm) / two_d;
+}
+
+void diffmassflux(orc_ctx& o){
+    const atjup_chem_params& p = o.chemp;
+    const int im = o.im, jm = o.jm, km = o.km;
+    const double DT_nh3 = p.mue_nh3 / (p.rg_nh3 * o.c.sc_nh3);
+    const double DT_h2s = p.mue_h2s / (p.rg_h2s * o.c.sc_h2s);
+    const double DT_nh4sh = p.mue_nh4sh / (o.c.rg_nh4sh * o.c.sc_nh4sh);
+    const double M_mix = p.X_h2 * p.m_h2 + p.X_he * p.m_he + p.X_h2o * p.m_h2o + p.X_h2s * p.m_h2s
+                       + p.X_nh3 * p.m_nh3 + p.X_nh4sh * p.m_nh4sh;
+    const double L_nh3 = o.c.r_mix * o.c.cp_mix * DT_nh3 / p.k_mix;
+    const double L_h2s = o.c.r_mix * o.c.cp_mix * DT_h2s / p.k_mix;
+    const double L_nh4sh = o.c.r_mix * o.c.cp_mix * DT_nh4sh / p.k_mix;
+    const double floor_ = 0.4;
+    std::vector<double>& T_ = o.f[ATJUP_F_T];
+    std::vector<double>* X[3] = { &o.f[ATJUP_F_H2S], &o.f[ATJUP_F_NH3], &o.f[ATJUP_F_NH4SH] };
+    #pragma omp parallel for schedule(static)
+    for(int k = 0; k < km - 1; k++)
+        for(int j = 1; j < jm - 1; j++){
+            const double sinthe = std::max(std::sin(o.the[j]), floor_);
+            for(int i = 1; i < im - 1; i++){
+                const double rm = o.rad[i], rmsinthe = rm * sinthe;
+                const size_t q = o.id(i, j, k);
+                if(!(T_[q] > 0.0) || o.mask[q] == 1.0){
+                    o.f[ATJUP_F_DIFFLUX_H2S][q] = 0.0; o.f[ATJUP_F_DIFFLUX_NH3][q] = 0.0;
+                    o.f[ATJUP_F_DIFFLUX_NH4SH][q] = 0.0; o.f[ATJUP_F_THERMALMASSFLUX][q] = 0.0;
+                    continue;
+                }
+                const int kw = o.wrap(k - 1), ke = o.wrap(k + 1);
+                auto solid = [&](int ii, int jj, int kk){ return o.mask[o.id(ii, jj, kk)] == 1.0; };
+                auto d3 = [&](const std::vector<double>& c, double& dr, double& dth, double& dph){
+                    dr = chem_d1(c[o.id(i - 1, j, k)], c[o.id(i + 1, j, k)], solid(i - 1, j, k), solid(i + 1, j, k), 2.0 * o.g.dr);
+                    dth = chem_d1(c[o.id(i, j - 1, k)], c[o.id(i, j + 1, k)], solid(i, j - 1, k), solid(i, j + 1, k), 2.0 * o.g.dthe);
+                    if(k == 0) dph = (-3.0 * c[o.id(i, j, 0)] + 4.0 * c[o.id(i, j, 1)] - c[o.id(i, j, 2)]) / (2.0 * o.g.dphi);
+                    else dph = chem_d1(c[o.id(i, j, kw)], c[o.id(i, j, ke)], solid(i, j, kw), solid(i, j, ke), 2.0 * o.g.dphi);
+                };
+                double dt[3], dh2s[3], dnh3[3], dnh4[3];
+                d3(T_, dt[0], dt[1], dt[2]);
+                d3(*X[0], dh2s[0], dh2s[1], dh2s[2]);
+                d3(*X[1], dnh3[0], dnh3[1], dnh3[2]);
+                d3(*X[2], dnh4[0], dnh4[1], dnh4[2]);
+                const double dt_div = dt[0] + std::abs(dt[1]) / rm + dt[2] / rmsinthe;
+                const double rf = o.f[ATJUP_F_RHO_MIX][q];
+                const double rho_c = (rf > 0.0 && std::isfinite(rf)) ? rf : o.c.r_mix;
+                const double sc = rho_c / o.c.r_mix;
+                const double jT_nh3 = (L_nh3 * sc) / T_[q] * dt_div;
+                const double jT_h2s = (L_h2s * sc) / T_[q] * dt_div;
+                const double jT_nh4sh = (L_nh4sh * sc) / T_[q] * dt_div;
+                const double dnh3_div = dnh3[0] + std::abs(dnh3[1]) / rm + dnh3[2] / rmsinthe;
+                const double dh2s_div = dh2s[0] + std::abs(dh2s[1]) / rm + dh2s[2] / rmsinthe;
+                const double dnh4sh_div = dnh4[0] + std::abs(dnh4[1]) / rm + dnh4[2] / rmsinthe;
+                const double Lc_nh3 = L_nh3 * sc, Lc_h2s = L_h2s * sc, Lc_nh4sh = L_nh4sh * sc;
+                const double cM_nh3 = Lc_nh3 * (*X[1])[q] / rho_c;
+                const double cM_h2s = Lc_h2s * (*X[0])[q] / rho_c;
+                const double cM_nh4sh = Lc_nh4sh * (*X[2])[q] / rho_c;
+                const double j_nh3 = p.m_nh3 / M_mix * (Lc_nh3 * dnh3_div - M_mix / p.m_nh3 * cM_nh3 * dnh3_div
+                    - M_mix / p.m_h2s * cM_nh3 * dh2s_div - M_mix / p.m_nh4sh * cM_nh3 * dnh4sh_div) - jT_nh3;
+                const double j_h2s = p.m_h2s / M_mix * (Lc_h2s * dh2s_div - M_mix / p.m_nh3 * cM_h2s * dnh3_div
+                    - M_mix / p.m_h2s * cM_h2s * dh2s_div - M_mix / p.m_nh4sh * cM_h2s * dnh4sh_div) - jT_h2s;
+                const double j_nh4sh = p.m_nh4sh / M_mix * (Lc_nh4sh * dnh4sh_div - M_mix / p.m_nh3 * cM_nh4sh * dnh3_div
+                    - M_mix / p.m_h2s * cM_nh4sh * dh2s_div - M_mix / p.m_nh4sh * cM_nh4sh * dnh4sh_div) - jT_nh4sh;
+                // laplacian_spherical (:752-776)
+                auto lap = [&](const std::vector<double>& c){
+                    const double rmsin = rm * std::max(std::sin(o.the[j]), floor_);
+                    const double d2r = (c[o.id(i + 1, j, k)] - 2.0 * c[q] + c[o.id(i - 1, j, k)]) / (o.g.dr * o.g.dr) * 1.0;
+                    const double d2t = (c[o.id(i, j + 1, k)] - 2.0 * c[q] + c[o.id(i, j - 1, k)]) / (o.g.dthe * o.g.dthe);
+                    const double d2p = (c[o.id(i, j, ke)] - 2.0 * c[q] + c[o.id(i, j, kw)]) / (o.g.dphi * o.g.dphi);
+                    const double d1r = (c[o.id(i + 1, j, k)] - c[o.id(i - 1, j, k)]) / (2.0 * o.g.dr) * 1.0;
+                    const double d1t = (c[o.id(i, j + 1, k)] - c[o.id(i, j - 1, k)]) / (2.0 * o.g.dthe);
+                    return d2r + 2.0 / rm * d1r + d2t / (rm * rm) + std::cos(o.the[j]) / (rm * rmsin) * d1t
+                         + d2p / (rmsin * rmsin);
+                };
+                o.f[ATJUP_F_DIFFLUX_H2S][q] = DT_h2s * lap(*X[0]);
+                o.f[ATJUP_F_DIFFLUX_NH3][q] = DT_nh3 * lap(*X[1]);
+                o.f[ATJUP_F_DIFFLUX_NH4SH][q] = DT_nh4sh * lap(*X[2]);
+                o.f[ATJUP_F_THERMALMASSFLUX][q] = (j_nh3 * p.cp_nh3 + j_h2s * p.cp_h2s + j_nh4sh * p.cp_nh4sh) * dt_div
+                    + (p.enthalpy ? o.w_nh4sh[q] * p.dh_nh4sh
+                                  : T_[q] * (o.w_nh3[q] * p.k_nh3 + o.w_h2s[q] * p.k_h2s + o.w_nh4sh[q] * p.k_nh4sh));
+            }
+        }
+}
+
+// ---- ChemistryJup::ChemMassRateJup (ChemistryJup.h:59-213), ATJUP_NH4SH_EQ 0 -------------------
+void chemmassrate(orc_ctx& o){
+    const atjup_chem_params& p = o.chemp;
+    const int im = o.im, jm = o.jm, km = o.km;
+    const double keq = p.m_nh4sh / (p.m_nh3 * p.m_h2s), A = 15000.0, B = -0.5, T_d = 3020.0;
+    std::vector<double>& MF_h2s = o.f[ATJUP_F_MASSFLUX_H2S];
+    std::vector<double>& MF_nh3 = o.f[ATJUP_F_MASSFLUX_NH3];
+    std::vector<double>& MF_nh4sh = o.f[ATJUP_F_MASSFLUX_NH4SH];
+    for(int i = 0; i < im; i++)
+        for(int k = 0; k < km; k++)
+            for(int j = 0; j < jm; j++){
+                if(j >= 3 && j < jm - 3) continue;
+                const size_t q = o.id(i, j, k);
+                o.w_nh3[q] = o.w_h2s[q] = o.w_nh4sh[q] = 0.0;
+                MF_nh3[q] = MF_h2s[q] = MF_nh4sh[q] = 0.0;
+            }
+    #pragma omp parallel for schedule(static)
+    for(int k = 0; k < km - 1; k++)
+        for(int j = 3; j < jm - 3; j++)
+            for(int i = 1; i < im - 1; i++){
+                const size_t q = o.id(i, j, k);
+                const int kw = o.wrap(k - 1), ke = o.wrap(k + 1);
+                if(o.mask[q] == 1.0 && o.mask[o.id(i - 1, j, k)] == 1.0 && o.mask[o.id(i + 1, j, k)] == 1.0
+                   && o.mask[o.id(i, j - 1, k)] == 1.0 && o.mask[o.id(i, j + 1, k)] == 1.0
+                   && o.mask[o.id(i, j, kw)] == 1.0 && o.mask[o.id(i, j, ke)] == 1.0){
+                    o.w_nh3[q] = o.w_h2s[q] = o.w_nh4sh[q] = 0.0;
+                    MF_nh3[q] = MF_h2s[q] = MF_nh4sh[q] = 0.0;
+                    continue;
+                }
+                const double t_u = o.f[ATJUP_F_T][q] * o.c.t_ref;
+                const double kf = A * std::pow(t_u, B) * std::exp(-T_d / t_u);
+                const double kb = kf / keq;
+                if(t_u <= p.t_0_nh4sh && t_u >= p.t_00_nh4sh){
+                    const double nh3 = o.f[ATJUP_F_NH3][q], h2s = o.f[ATJUP_F_H2S][q], nh4sh = o.f[ATJUP_F_NH4SH][q];
+                    double denom;
+                    if(p.molar_conc){
+                        double rho = o.f[ATJUP_F_RHO_MIX][q];
+                        if(!(rho > 0.0)) rho = o.c.r_mix;
+                        denom = rho;
+                    } else {
+                        const double sum_c = nh3 / p.m_nh3 + h2s / p.m_h2s + nh4sh / p.m_nh4sh;
+                        denom = (sum_c == 0.0) ? 0.0 : o.c.r_mix / sum_c;
+                    }
+                    const double R_diff = kf * (nh3 / p.m_nh3 * denom) * (h2s / p.m_h2s * denom) - kb * (nh4sh / p.m_nh4sh * denom);
+                    o.w_nh3[q] = -p.m_nh3 * R_diff;
+                    o.w_h2s[q] = -p.m_h2s * R_diff;
+                    o.w_nh4sh[q] = p.m_nh4sh * R_diff;
+                } else if(p.gate_zero){
+                    o.w_nh3[q] = o.w_h2s[q] = o.w_nh4sh[q] = 0.0;
+                }
+                MF_h2s[q] = o.w_h2s[q] + o.f[ATJUP_F_DIFFLUX_H2S][q];
+                MF_nh3[q] = o.w_nh3[q] + o.f[ATJUP_F_DIFFLUX_NH3][q];
+                MF_nh4sh[q] = o.w_nh4sh[q] + o.f[ATJUP_F_DIFFLUX_NH4SH][q];
+            }
+}
+
+// the physics block's chemistry in the reference's order (cJupiterModel.cpp:1146-1161)
+void chemistry(orc_ctx& o){
+    diffmassflux(o);
+    for(int f : { ATJUP_F_DIFFLUX_H2S, ATJUP_F_DIFFLUX_NH3, ATJUP_F_DIFFLUX_NH4SH, ATJUP_F_THERMALMASSFLUX })
+        shapiro(o, o.f[f], 2, 1.0, 1, 7);
+    chemmassrate(o);
+    flux_limiter(o);
+    for(int f : { ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH })
+        shapiro(o, o.f[f], 2, 1.0, 1, 7);
+}
+
 bool valid_field(int f){
     return (f >= 0 && f < ATJUP_N_PROG) || (f >= ATJUP_F_P_DYN && f < ATJUP_F_END)
         || (f >= ATJUP_NLEVEL && f < ATJUP_NLEVEL + ATJUP_N_PROG);
@@ -1784,6 +1943,7 @@ orc_ctx* orc_create(const atjup_grid* g, const atjup_consts* c, const atjup_knob
     o->itopo.assign((size_t)o->jm * o->km, 0);
     o->vel_star.assign((size_t)o->jm * o->km, 0.0);
     o->brl.assign(o->im, 0.0);
+    o->w_nh3.assign(o->n3, 0.0); o->w_h2s.assign(o->n3, 0.0); o->w_nh4sh.assign(o->n3, 0.0);
     // Jupiter (cJupiterModel.h:1109-1118) and the radiation knobs' defaults (Radiation.h:270-295)
     o->radp = atjup_rad_params{ 5.4, 50.5, 0.343, 0.863, 0.134, 3.9, 1.0, 1.0, 1.0, 1, 0 };
     return o;
@@ -1886,6 +2046,12 @@ int orc_set_radiation(orc_ctx* o, const atjup_rad_params* p){
     return 0;
 }
 int orc_radiation(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; radiation(*o); return 0; }
+int orc_set_chemistry(orc_ctx* o, const atjup_chem_params* p){
+    if(!o || !p) return ATJUP_ERR_ARG;
+    o->chemp = *p;
+    return 0;
+}
+int orc_chemistry(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; chemistry(*o); return 0; }
 int orc_set_precipitation(orc_ctx* o, const atjup_precip_params* p){
     if(!o || !p) return ATJUP_ERR_ARG;
     o->precp = *p;
@@ -1911,8 +2077,11 @@ int orc_step(orc_ctx* o, int iter, int first){
         if(o->kn.precip) precipitation(*o);                   // cJupiterModel.cpp:1090
         if(o->kn.rk_scalar_sync >= 1) scalar_sync(*o);
         if(o->kn.turb_model >= 0){ turbulence(*o); turb_wall_omega(*o); }
-        flux_limiter(*o);
-        shapiro(*o, o->f[ATJUP_F_FLUXLIM_NH4SH], 2, 1.0, 1, 7);
+        if(o->kn.chemistry) chemistry(*o);                    // cJupiterModel.cpp:1146-1161
+        else {
+            flux_limiter(*o);
+            shapiro(*o, o->f[ATJUP_F_FLUXLIM_NH4SH], 2, 1.0, 1, 7);
+        }
     }
     rk4(*o);
     bc_radius(*o); bc_theta(*o); bc_phi(*o); bc_solid(*o);

@@ -40,6 +40,8 @@ int      orc_turbulence_run(orc_ctx* o);
 int      orc_flux_limiter(orc_ctx* o);
 int      orc_radiation(orc_ctx* o);
 int      orc_set_radiation(orc_ctx* o, const atjup_rad_params* p);
+int      orc_chemistry(orc_ctx* o);
+int      orc_set_chemistry(orc_ctx* o, const atjup_chem_params* p);
 int      orc_precipitation(orc_ctx* o);
 int      orc_set_precipitation(orc_ctx* o, const atjup_precip_params* p);
 int      orc_download_precip_surface(orc_ctx* o, double* dst);

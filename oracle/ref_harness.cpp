@@ -70,7 +70,8 @@ public:
             {"P_ch4_rain", &M.P_ch4_rain}, {"P_ch4_snow", &M.P_ch4_snow}, {"P_ch4_graupel", &M.P_ch4_graupel},
             {"P_nh4sh", &M.P_nh4sh},
             {"difflux_h2s", &M.difflux_h2s}, {"difflux_nh3", &M.difflux_nh3},
-            {"thermalmassflux", &M.thermalmassflux},
+            {"thermalmassflux", &M.thermalmassflux}, {"difflux_nh4sh", &M.difflux_nh4sh},
+            {"w_nh3", &M.w_nh3}, {"w_h2s", &M.w_h2s}, {"w_nh4sh", &M.w_nh4sh},
         };
     }
 
@@ -100,6 +101,15 @@ public:
                 {"sc_nh4sh", &M.sc_nh4sh}, {"sc_ch4", &M.sc_ch4}, {"rg_nh4sh", &M.rg_nh4sh},
                 {"r_p_nh4sh", &M.r_p_nh4sh}, {"re_turb", &M.re_turb}, {"t_pole", &M.t_pole},
                 {"t_equator", &M.t_equator}, {"abl_height", &M.abl_height},
+                // the chemistry's constants (ChemistryJup.h:59-545) as Run() leaves them
+                {"m_nh3", &M.m_nh3}, {"m_h2s", &M.m_h2s}, {"m_nh4sh", &M.m_nh4sh}, {"m_h2", &M.m_h2},
+                {"m_he", &M.m_he}, {"m_h2o", &M.m_h2o}, {"X_h2", &M.X_h2}, {"X_he", &M.X_he},
+                {"X_h2o", &M.X_h2o}, {"X_h2s", &M.X_h2s}, {"X_nh3", &M.X_nh3}, {"X_nh4sh", &M.X_nh4sh},
+                {"mue_nh3", &M.mue_nh3}, {"mue_h2s", &M.mue_h2s}, {"mue_nh4sh", &M.mue_nh4sh},
+                {"rg_nh3", &M.rg_nh3}, {"rg_h2s", &M.rg_h2s}, {"k_mix", &M.k_mix},
+                {"cp_nh3", &M.cp_nh3}, {"cp_h2s", &M.cp_h2s}, {"cp_nh4sh", &M.cp_nh4sh},
+                {"k_nh3", &M.k_nh3}, {"k_h2s", &M.k_h2s}, {"k_nh4sh", &M.k_nh4sh}, {"dh_nh4sh", &M.dh_nh4sh},
+                {"t_0_nh4sh", &M.t_0_nh4sh}, {"t_00_nh4sh", &M.t_00_nh4sh},
             };
             // the saturation adjustment's per-gas constants as Run() leaves them (SVP_FIT, COMPOSITION)
 #define GASK(g) \
@@ -201,6 +211,22 @@ public:
         else if(op == "turb_wall")      { TurbulenceJup(M).apply_wall_bc(); wall_omega(M); }
         else if(op == "radiation")      RadiationJup(M).run();
         else if(op == "precipitation")  PrecipitationJup(M).run();
+        else if(op == "chemistry"){
+            // the physics block's chemistry (cJupiterModel.cpp:1146-1161)
+            ChemistryJup(M).DiffMassFluxJup();
+            JupiterUtils::damp_wiggles(M.difflux_h2s,     &M.i_topography, true, true, true);
+            JupiterUtils::damp_wiggles(M.difflux_nh3,     &M.i_topography, true, true, true);
+            JupiterUtils::damp_wiggles(M.difflux_nh4sh,   &M.i_topography, true, true, true);
+            JupiterUtils::damp_wiggles(M.thermalmassflux, &M.i_topography, true, true, true);
+            ChemistryJup(M).ChemMassRateJup();
+            ChemistryJup(M).FluxLimiterNH4SH();
+            JupiterUtils::damp_wiggles(M.massflux_h2s,   &M.i_topography, true, true, true);
+            JupiterUtils::damp_wiggles(M.massflux_nh3,   &M.i_topography, true, true, true);
+            JupiterUtils::damp_wiggles(M.massflux_nh4sh, &M.i_topography, true, true, true);
+            JupiterUtils::damp_wiggles(M.fluxlim_nh4sh,  &M.i_topography, true, true, true);
+        }
+        else if(op == "diffmassflux")   ChemistryJup(M).DiffMassFluxJup();
+        else if(op == "chemmassrate")   ChemistryJup(M).ChemMassRateJup();
         else if(op == "satadj_h2o")
             SaturationAdjustmentJup(M).run("H2O", M.t_0_h2o, M.t_00_h2o, M.ep_h2o, M.lv_h2o, M.ls_h2o,
                 M.C_h2o, M.L0_h2o, M.R_h2o, M.del_alf_h2o, M.del_bet_h2o,

@@ -46,7 +46,18 @@ def test_default_radiation_params_match_mirror():
     assert ctypes.sizeof(abi.RadParams) == 9 * 8 + 2 * 4
     assert abi.FIELD["epsilon"] == 47 and abi.FIELD["radiation"] == 48
     assert abi.FIELD["P_rain"] == 49 and abi.FIELD["P_nh4sh"] == 58
-    assert abi.FIELD["thermalmassflux"] == 61 and abi.F_END == 62 < abi.NLEVEL
+    assert abi.FIELD["thermalmassflux"] == 61 and abi.FIELD["difflux_nh4sh"] == 62 and abi.F_END == 63 < abi.NLEVEL
+
+
+def test_default_chemistry_params_match_mirror():
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libatjup_hip.so not built")
+    L = lib.load()
+    p = abi.ChemParams()
+    L.atjup_hip_default_chem_params(ctypes.byref(p))
+    q = abi.chem_params()
+    for name, _ in abi.ChemParams._fields_:
+        assert getattr(p, name) == getattr(q, name), name
 
 
 def test_default_precipitation_params_match_mirror():

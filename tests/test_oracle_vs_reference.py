@@ -229,3 +229,49 @@ def test_restart_format_against_compiled_reference(carried):
     r = subprocess.run([sys.executable, "-c", SCRIPT_RESTART % {"root": ROOT, "carried": carried}],
                        capture_output=True, text=True, timeout=900)
     assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+# The physics block's chemistry (DiffMassFluxJup, its smoothing, ChemMassRateJup, the flux limiter
+# and the massflux smoothing), called twice so that thermalmassflux reads the rates the first call
+# left; at the knob defaults and with ATJUP_CHEM_GATE_ZERO=0 + ATJUP_CHEM_MOLAR_CONC=1 (both cached
+# by the reference: a process of their own).
+SCRIPT_CHEM = r'''
+import os, sys, numpy as np
+sys.path.insert(0, %(root)r)
+from atjup_amd import abi, inputs
+import oracle
+gate, molar = %(gate)r, %(molar)r
+os.environ["ATJUP_CHEM_GATE_ZERO"] = str(gate)
+os.environ["ATJUP_CHEM_MOLAR_CONC"] = str(molar)
+G, Cn, K, env = abi.make_profile("full", sweeps=8)
+st = inputs.make_state("full", seed=11)
+ref = oracle.Reference(env); ref.load_state(st, Cn)
+orc = oracle.Oracle(G, Cn, K); orc.load_state(st)
+orc.set_chemistry(abi.chem_params(gate_zero=gate, molar_conc=molar))
+ref.call("mixture_density"); orc.call("mixture_density")
+names = ["difflux_h2s", "difflux_nh3", "difflux_nh4sh", "thermalmassflux", "massflux_h2s", "massflux_nh3",
+         "massflux_nh4sh", "fluxlim_nh4sh"]
+for call in (1, 2):
+    ref.call("chemistry"); orc.call("chemistry")
+    for n in names:
+        a, b = ref.get(n), orc.get(n)
+        if not np.array_equal(a, b):
+            print("MISMATCH", call, n, np.max(np.abs(a - b)), np.count_nonzero(a != b)); sys.exit(1)
+    # a temperature change between the calls moves cells across the NH4SH band
+    if call == 1:
+        t = ref.get("t") * 1.03
+        ref.set("t", t); orc.set("t", t)
+nz = {n: int(np.count_nonzero(orc.get(n))) for n in names}
+print("nonzero", nz)
+if min(nz["massflux_nh4sh"], nz["thermalmassflux"], nz["difflux_h2s"]) == 0: print("EMPTY"); sys.exit(1)
+print("ALL-EQUAL")
+'''
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("gate,molar", [(1, 0), (0, 1)], ids=["defaults", "no-gate-molar"])
+def test_chemistry_bit_exact_against_compiled_reference(gate, molar):
+    r = subprocess.run([sys.executable, "-c", SCRIPT_CHEM % {"root": ROOT, "gate": gate, "molar": molar}],
+                       capture_output=True, text=True, timeout=900)
+    print(r.stdout[-600:])
+    assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
