@@ -1334,8 +1334,8 @@ int do_radiation(atjup_ctx* c){
 }
 
 // =============================================================================================
-// Precipitation<P>::run (Precipitation.h:194-574): one thread per (j, k) column does, in the
-// reference's order per cell,
+// Precipitation<P>::run (Precipitation.h:194-574), per (j, k) column, in the reference's order per
+// cell:
 //   * the zeroing of Q_precip and the nine S_precip_* rates over the whole column (:199-210);
 //   * the three-category column of H2O, NH3 and CH4 (column(), :273-450): a top-down sweep from
 //     i = im-2 to the column's base with the rain, snow and graupel fluxes arriving from above
@@ -1344,7 +1344,11 @@ int do_radiation(atjup_ctx* c){
 //     depletion as rates (coupling on) or in place (off), and the fluxes capped at 1e-4;
 //   * the NH4SH crystal settling flux in its temperature band (sedimentNH4SH, :523-574);
 //   * the surface maps at the column's base (surfaceMap, :470-492).
-// The three gases are independent recurrences down the column, so one thread carries all three.
+// The three gases are independent recurrences down the column, so a workgroup is three waves,
+// one gas each, over the same 64 columns of the flattened (j, k) index. Per level each wave puts
+// its latent-heat term in LDS and, after one barrier, wave 0 adds the three in the reference's
+// gas order (0 + H2O + NH3 + CH4) into Q_precip; double-buffered LDS needs one barrier a level.
+// Each wave loads the next level's inputs before it works on the current one.
 // =============================================================================================
 struct PrecipGas {
     double C, L0, R, da, db, ep, Lv, Ls, t_frz, t_low;
@@ -1368,112 +1372,149 @@ constexpr double PR_C_C_AU = 2.0e-8, PR_C_I_AU = 5.0e-8, PR_Q_C_CRIT = 5.0e-4, P
 constexpr double PR_C_RIM = 1.0e-8, PR_Z_CSG = 2.5e-8, PR_A_EV = 2.76e-3, PR_C_MELT_K = 0.2;
 constexpr double PR_C_FRZ_K = 0.2, PR_P_REF = 1.0e-4, PR_RATIO_MAX = 5.0, PR_V_FALL = 2.0;
 constexpr double PR_P_MAX_FLUX = 1.0e-4, PR_DEP_FRAC = 1.0;          // Precipitation.h:39-85, :277-290
+constexpr int PR_BK = 64;                                             // columns per workgroup
+
+struct PrecipIn { double tq, p_u, rf, q_v, q_c, q_i; };
 
 template<typename T>
-__global__ __launch_bounds__(64) void k_precip(const PrecipFields<T> F, const int* __restrict__ itopo,
+__device__ __forceinline__ PrecipIn precip_load(const PrecipFields<T>& F, int g, size_t q){
+    PrecipIn r;
+    r.tq = (double)F.t[q];
+    if(F.lo_t) r.tq += (double)F.lo_t[q];
+    r.p_u = (double)F.p_stat[q];
+    r.rf = (double)F.rho[q];
+    r.q_v = (double)F.vap[g][q];
+    r.q_c = (double)F.cld[g][q];
+    r.q_i = (double)F.ice[g][q];
+    return r;
+}
+
+// the NH4SH settling flux of one cell (sedimentNH4SH, Precipitation.h:540-556)
+template<typename T>
+__device__ __forceinline__ T nh4sh_flux(const PrecipFields<T>& F, size_t q, const PrecipConsts& C){
+    double tq = (double)F.t[q];
+    if(F.lo_t) tq += (double)F.lo_t[q];
+    const double Tk = tq * C.t_ref;
+    const double qn = dmax(0.0, (double)F.nh4sh[q]);
+    return (T)((Tk >= C.t_lo_nh4sh && Tk <= C.t_hi_nh4sh) ? qn * C.v_settle : 0.0);
+}
+
+template<typename T>
+__global__ __launch_bounds__(3 * PR_BK) __attribute__((amdgpu_waves_per_eu(3))) void k_precip(const PrecipFields<T> F, const int* __restrict__ itopo,
         const double* __restrict__ dz_tab, int im, int jm, int km, int pitch, const PrecipConsts C){
-    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y;
-    if(k >= km) return;
+    __shared__ double s_q[2][3][PR_BK];
+    __shared__ double s_srf[3][PR_BK];
+    const int lane = threadIdx.x % PR_BK, g = wave_uniform(threadIdx.x / PR_BK);
+    const int n2 = jm * km, col_raw = blockIdx.x * PR_BK + lane;
+    const bool valid = col_raw < n2;                  // every lane reaches every barrier
+    const int col = valid ? col_raw : n2 - 1;
+    const int j = col / km, k = col - j * km;
     const size_t P = (size_t)jm * pitch, q0 = (size_t)j * pitch + k;
-    const int i_base = max(itopo[(size_t)j * km + k], 0), i_top = im - 1;
+    const int i_base = max(itopo[col], 0), i_top = im - 1;
     const bool live = i_base < i_top;                   // column() skips the others
-    double Fr[3] = {0.0, 0.0, 0.0}, Fs[3] = {0.0, 0.0, 0.0}, Fg[3] = {0.0, 0.0, 0.0};
+    const PrecipGas G = C.gas[g];
+    const double Lf = G.Ls - G.Lv;
+    T* const Pr = F.P[g][0]; T* const Ps = F.P[g][1]; T* const Pg = F.P[g][2];
+    double Fr = 0.0, Fs = 0.0, Fg = 0.0;
+    PrecipIn nxt = precip_load(F, g, q0 + (size_t)i_top * P);
     for(int i = i_top; i >= 0; i--){
         const size_t q = q0 + (size_t)i * P;
-        double Qp = 0.0, Sv[3] = {0.0, 0.0, 0.0}, Sc[3] = {0.0, 0.0, 0.0}, Si[3] = {0.0, 0.0, 0.0};
-        if(live){
+        const PrecipIn in = nxt;
+        if(i > 0) nxt = precip_load(F, g, q - P);
+        double Qp = 0.0, Sv = 0.0, Sc = 0.0, Si = 0.0;
+        if(valid && live){
             const double dz = dmax(0.0, dz_tab[i]);
             if(i == i_top || i < i_base || dz <= 0.0){
                 // the top's zero fluxes; below the base and across a zero-thickness layer the
                 // fluxes of the cell above (:300-303, :316-321, :444-448)
-                if(i == i_top) for(int g = 0; g < 3; g++){ Fr[g] = 0.0; Fs[g] = 0.0; Fg[g] = 0.0; }
-                for(int g = 0; g < 3; g++){
-                    F.P[g][0][q] = (T)Fr[g]; F.P[g][1][q] = (T)Fs[g]; F.P[g][2][q] = (T)Fg[g];
-                }
+                if(i == i_top){ Fr = 0.0; Fs = 0.0; Fg = 0.0; }
+                Pr[q] = (T)Fr; Ps[q] = (T)Fs; Pg[q] = (T)Fg;
             } else {
-                double tq = (double)F.t[q];
-                if(F.lo_t) tq += (double)F.lo_t[q];
-                const double Tk = tq * C.t_ref;
-                const double p_u = (double)F.p_stat[q];
-                const double rf = (double)F.rho[q];
-                const double rho_at = (rf > 0.0 && isfinite(rf)) ? rf : C.r_mix;
-                for(int g = 0; g < 3; g++){
-                    const PrecipGas& G = C.gas[g];
-                    const double Lf = G.Ls - G.Lv;
-                    const double q_v = dmax(0.0, (double)F.vap[g][q]);
-                    const double q_c = dmax(0.0, (double)F.cld[g][q]);
-                    const double q_i = dmax(0.0, (double)F.ice[g][q]);
-                    const double E_r = exp(G.C + (-G.L0 / Tk + G.da * log(Tk) + G.db * Tk) / (1e-3 * G.R));
-                    const double q_sat = (p_u > E_r) ? rho_at * G.ep * E_r / p_u : q_v;
-                    const double F_r_in = Fr[g], F_s_in = Fs[g], F_g_in = Fg[g];
-                    const double Rain = dmin(PR_RATIO_MAX, F_r_in / PR_P_REF);
-                    const double Snow = dmin(PR_RATIO_MAX, F_s_in / PR_P_REF);
-                    const double Graupel = dmin(PR_RATIO_MAX, F_g_in / PR_P_REF);
-                    const double Rain_79 = (Rain > 0.0) ? pow(Rain, 7.0 / 9.0) : 0.0;
-                    const double Rain_49 = (Rain > 0.0) ? pow(Rain, 4.0 / 9.0) : 0.0;
-                    const bool warm = (Tk >= G.t_frz);
-                    const double S_c_au = (warm && q_c > PR_Q_C_CRIT) ? (C.rs * PR_C_C_AU) * (q_c - PR_Q_C_CRIT) : 0.0;
-                    const double S_i_au = (!warm && Tk >= G.t_low) ? (C.rs * PR_C_I_AU) * q_i : 0.0;
-                    const double S_ac = warm ? (C.rs * PR_C_AC) * q_c * Rain_79 : 0.0;
-                    const double S_s_rim = !warm ? (C.rs * PR_C_RIM) * q_c * Snow : 0.0;
-                    const double S_g_rim = !warm ? (C.rs * PR_C_RIM) * q_c * Graupel : 0.0;
-                    const double S_csg = (!warm && q_c > PR_Q_C_CRIT) ? (C.rs * PR_Z_CSG) * q_c * Snow : 0.0;
-                    const double f_melt = warm ? dmin(1.0, PR_C_MELT_K * (Tk - G.t_frz)) : 0.0;
-                    const double f_frz = warm ? 0.0 : dmin(1.0, PR_C_FRZ_K * (G.t_frz - Tk));
-                    const double F_s_melt = f_melt * F_s_in, F_g_melt = f_melt * F_g_in, F_r_frz = f_frz * F_r_in;
-                    const double F_r_avail = dmax(0.0, F_r_in - F_r_frz + F_s_melt + F_g_melt + (S_c_au + S_ac) * dz);
-                    const double S_ev_raw = (warm && Rain > 0.0 && q_v < q_sat) ? PR_A_EV * (q_sat - q_v) * Rain_49 : 0.0;
-                    const double F_ev_sat = dmax(0.0, q_sat - q_v) * PR_V_FALL;
-                    const double F_ev = dmin(dmin(S_ev_raw * dz, F_r_avail), F_ev_sat);
-                    const double S_ev = F_ev / dz;
-                    Qp += Lf * (S_s_rim + S_g_rim + S_csg) + Lf * (F_r_frz - F_s_melt - F_g_melt) / dz - G.Lv * F_ev / dz;
-                    const double tau = dz / PR_V_FALL;
-                    const double d_c = dmin(PR_DEP_FRAC * q_c, (S_c_au + S_ac + S_s_rim + S_g_rim + S_csg) * tau);
-                    const double d_i = dmin(PR_DEP_FRAC * q_i, S_i_au * tau);
-                    if(C.coupling){
-                        Sc[g] = -dmax(0.0, d_c) / tau * C.nd;
-                        Si[g] = -dmax(0.0, d_i) / tau * C.nd;
-                        Sv[g] = +S_ev * C.nd;
-                    } else {
-                        F.cld[g][q] = (T)(q_c - dmax(0.0, d_c));
-                        F.ice[g][q] = (T)(q_i - dmax(0.0, d_i));
-                        F.vap[g][q] = (T)(q_v + S_ev * tau);
-                    }
-                    const double F_r = F_r_in - F_r_frz + F_s_melt + F_g_melt - F_ev + (S_c_au + S_ac) * dz;
-                    const double F_s = F_s_in - F_s_melt + (S_i_au + S_s_rim - S_csg) * dz;
-                    const double F_g = F_g_in - F_g_melt + F_r_frz + (S_g_rim + S_csg) * dz;
-                    Fr[g] = dmin(PR_P_MAX_FLUX, dmax(0.0, F_r));
-                    Fs[g] = dmin(PR_P_MAX_FLUX, dmax(0.0, F_s));
-                    Fg[g] = dmin(PR_P_MAX_FLUX, dmax(0.0, F_g));
-                    F.P[g][0][q] = (T)Fr[g]; F.P[g][1][q] = (T)Fs[g]; F.P[g][2][q] = (T)Fg[g];
+                const double Tk = in.tq * C.t_ref;
+                const double p_u = in.p_u;
+                const double rho_at = (in.rf > 0.0 && isfinite(in.rf)) ? in.rf : C.r_mix;
+                const double q_v = dmax(0.0, in.q_v);
+                const double q_c = dmax(0.0, in.q_c);
+                const double q_i = dmax(0.0, in.q_i);
+                const double E_r = exp(G.C + (-G.L0 / Tk + G.da * log(Tk) + G.db * Tk) / (1e-3 * G.R));
+                const double q_sat = (p_u > E_r) ? rho_at * G.ep * E_r / p_u : q_v;
+                const double F_r_in = Fr, F_s_in = Fs, F_g_in = Fg;
+                const double Rain = dmin(PR_RATIO_MAX, F_r_in / PR_P_REF);
+                const double Snow = dmin(PR_RATIO_MAX, F_s_in / PR_P_REF);
+                const double Graupel = dmin(PR_RATIO_MAX, F_g_in / PR_P_REF);
+                const double Rain_79 = (Rain > 0.0) ? pow(Rain, 7.0 / 9.0) : 0.0;
+                const double Rain_49 = (Rain > 0.0) ? pow(Rain, 4.0 / 9.0) : 0.0;
+                const bool warm = (Tk >= G.t_frz);
+                const double S_c_au = (warm && q_c > PR_Q_C_CRIT) ? (C.rs * PR_C_C_AU) * (q_c - PR_Q_C_CRIT) : 0.0;
+                const double S_i_au = (!warm && Tk >= G.t_low) ? (C.rs * PR_C_I_AU) * q_i : 0.0;
+                const double S_ac = warm ? (C.rs * PR_C_AC) * q_c * Rain_79 : 0.0;
+                const double S_s_rim = !warm ? (C.rs * PR_C_RIM) * q_c * Snow : 0.0;
+                const double S_g_rim = !warm ? (C.rs * PR_C_RIM) * q_c * Graupel : 0.0;
+                const double S_csg = (!warm && q_c > PR_Q_C_CRIT) ? (C.rs * PR_Z_CSG) * q_c * Snow : 0.0;
+                const double f_melt = warm ? dmin(1.0, PR_C_MELT_K * (Tk - G.t_frz)) : 0.0;
+                const double f_frz = warm ? 0.0 : dmin(1.0, PR_C_FRZ_K * (G.t_frz - Tk));
+                const double F_s_melt = f_melt * F_s_in, F_g_melt = f_melt * F_g_in, F_r_frz = f_frz * F_r_in;
+                const double F_r_avail = dmax(0.0, F_r_in - F_r_frz + F_s_melt + F_g_melt + (S_c_au + S_ac) * dz);
+                const double S_ev_raw = (warm && Rain > 0.0 && q_v < q_sat) ? PR_A_EV * (q_sat - q_v) * Rain_49 : 0.0;
+                const double F_ev_sat = dmax(0.0, q_sat - q_v) * PR_V_FALL;
+                const double F_ev = dmin(dmin(S_ev_raw * dz, F_r_avail), F_ev_sat);
+                const double S_ev = F_ev / dz;
+                Qp = Lf * (S_s_rim + S_g_rim + S_csg) + Lf * (F_r_frz - F_s_melt - F_g_melt) / dz - G.Lv * F_ev / dz;
+                const double tau = dz / PR_V_FALL;
+                const double d_c = dmin(PR_DEP_FRAC * q_c, (S_c_au + S_ac + S_s_rim + S_g_rim + S_csg) * tau);
+                const double d_i = dmin(PR_DEP_FRAC * q_i, S_i_au * tau);
+                if(C.coupling){
+                    Sc = -dmax(0.0, d_c) / tau * C.nd;
+                    Si = -dmax(0.0, d_i) / tau * C.nd;
+                    Sv = +S_ev * C.nd;
+                } else {
+                    F.cld[g][q] = (T)(q_c - dmax(0.0, d_c));
+                    F.ice[g][q] = (T)(q_i - dmax(0.0, d_i));
+                    F.vap[g][q] = (T)(q_v + S_ev * tau);
                 }
+                const double F_r = F_r_in - F_r_frz + F_s_melt + F_g_melt - F_ev + (S_c_au + S_ac) * dz;
+                const double F_s = F_s_in - F_s_melt + (S_i_au + S_s_rim - S_csg) * dz;
+                const double F_g = F_g_in - F_g_melt + F_r_frz + (S_g_rim + S_csg) * dz;
+                Fr = dmin(PR_P_MAX_FLUX, dmax(0.0, F_r));
+                Fs = dmin(PR_P_MAX_FLUX, dmax(0.0, F_s));
+                Fg = dmin(PR_P_MAX_FLUX, dmax(0.0, F_g));
+                Pr[q] = (T)Fr; Ps[q] = (T)Fs; Pg[q] = (T)Fg;
             }
         }
-        F.Q[q] = (T)Qp;
-        for(int g = 0; g < 3; g++){ F.S[g][0][q] = (T)Sv[g]; F.S[g][1][q] = (T)Sc[g]; F.S[g][2][q] = (T)Si[g]; }
+        if(valid){ F.S[g][0][q] = (T)Sv; F.S[g][1][q] = (T)Sc; F.S[g][2][q] = (T)Si; }
+        s_q[i & 1][g][lane] = Qp;
+        __syncthreads();
+        if(g == 0 && valid){
+            double s = 0.0;
+            s += s_q[i & 1][0][lane];
+            s += s_q[i & 1][1][lane];
+            s += s_q[i & 1][2][lane];
+            F.Q[q] = (T)s;
+        }
     }
-    // NH4SH crystal settling in its band, the solid cells below taking the base's flux
-    for(int i = i_base; i < im; i++){
-        const size_t q = q0 + (size_t)i * P;
-        double tq = (double)F.t[q];
-        if(F.lo_t) tq += (double)F.lo_t[q];
-        const double Tk = tq * C.t_ref;
-        const double qn = dmax(0.0, (double)F.nh4sh[q]);
-        F.P_nh4sh[q] = (T)((Tk >= C.t_lo_nh4sh && Tk <= C.t_hi_nh4sh) ? qn * C.v_settle : 0.0);
-    }
-    const T base_flux = (i_base < im) ? F.P_nh4sh[q0 + (size_t)i_base * P] : T(0);
-    for(int i = 0; i < i_base && i < im; i++) F.P_nh4sh[q0 + (size_t)i * P] = base_flux;
-    // the surface maps at the base (a fully solid column reads its top cell)
+    // NH4SH crystal settling in its band, the solid cells below taking the base's flux; the
+    // three waves take every third level
+    const T base_flux = (i_base < im) ? nh4sh_flux(F, q0 + (size_t)i_base * P, C) : T(0);
+    if(valid)
+        for(int i = g; i < im; i += 3){
+            const size_t q = q0 + (size_t)i * P;
+            F.P_nh4sh[q] = (i >= i_base) ? nh4sh_flux(F, q, C) : base_flux;
+        }
+    // the surface maps at the base (a fully solid column reads its top cell); each wave reads
+    // back its own gas's fluxes
     const size_t qb = q0 + (size_t)min(i_base, im - 1) * P;
-    double srf[3];
-    for(int g = 0; g < 3; g++) srf[g] = (double)F.P[g][0][qb] + (double)F.P[g][1][qb] + (double)F.P[g][2][qb];
-    const double nh4 = (double)F.P_nh4sh[qb];
-    const size_t n2 = (size_t)jm * km, c2 = (size_t)j * km + k;
-    F.srf[c2] = srf[0];
-    F.srf[n2 + c2] = srf[1];
-    F.srf[2 * n2 + c2] = srf[2];
-    F.srf[3 * n2 + c2] = nh4;
-    F.srf[4 * n2 + c2] = srf[0] + srf[1] + srf[2] + nh4;
+    s_srf[g][lane] = (double)Pr[qb] + (double)Ps[qb] + (double)Pg[qb];
+    __syncthreads();
+    if(g == 0 && valid){
+        const double srf0 = s_srf[0][lane], srf1 = s_srf[1][lane], srf2 = s_srf[2][lane];
+        const double nh4 = (double)base_flux;
+        const size_t c2 = (size_t)col;
+        F.srf[c2] = srf0;
+        F.srf[(size_t)n2 + c2] = srf1;
+        F.srf[2 * (size_t)n2 + c2] = srf2;
+        F.srf[3 * (size_t)n2 + c2] = nh4;
+        F.srf[4 * (size_t)n2 + c2] = srf0 + srf1 + srf2 + nh4;
+    }
 }
 
 template<typename T>
@@ -1503,7 +1544,7 @@ int do_precipitation(atjup_ctx* c){
     C.coupling = (c->kn.precip_coupling != 0.0) ? 1 : 0;
     const double* dz = (const double*)c->tabs.p + c->im + 3 * c->jm;    // layer_thickness_m
     hipEvent_t t0 = timer_begin(c);
-    hipLaunchKernelGGL(k_precip<T>, dim3(nblk(c->km, 64), c->jm), dim3(64), 0, c->stream,
+    hipLaunchKernelGGL(k_precip<T>, dim3(nblk(c->jm * c->km, PR_BK)), dim3(3 * PR_BK), 0, c->stream,
         Fp, (const int*)c->itopo.p, dz, c->im, c->jm, c->km, c->pitch, C);
     timer_end(c, ATJUP_T_PRECIPITATION, t0, 1);
     return check_launch();
