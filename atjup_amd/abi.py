@@ -60,7 +60,7 @@ class Knobs(C.Structure):
                 ("wall_nue_factor", C.c_double), ("wall_nue_layers", C.c_int),
                 ("satadj", C.c_int), ("rk_scalar_sync", C.c_int), ("dry", C.c_int),
                 ("turb_k_bg_phys", C.c_double), ("radiation", C.c_int), ("precip", C.c_int),
-                ("chemistry", C.c_int)]
+                ("chemistry", C.c_int), ("mca", C.c_int)]
 
 
 CHEM_DOUBLES = ("m_nh3", "m_h2s", "m_nh4sh", "m_h2", "m_he", "m_h2o", "X_h2", "X_he", "X_h2o", "X_h2s",
@@ -86,6 +86,25 @@ def chem_params(**over) -> ChemParams:
     d = dict(CHEM_DEFAULTS)
     d.update(over)
     return ChemParams(**d)
+
+
+class MCAParams(C.Structure):
+    """atjup_mca_params: ATJUP_MCA_RELAX / _LEDOUX / _LOADING / _PASSES, ep_h2s and the NH4SH heat."""
+    _fields_ = [("relax", C.c_double), ("ep_h2s", C.c_double), ("l_nh4sh", C.c_double),
+                ("ledoux", C.c_int), ("loading", C.c_int), ("max_pass", C.c_int)]
+
+
+# the knobs' defaults (planet/ConvectiveAdjustment.h:409-413) and ep_h2s = m_h2s / M_mix as the
+# reference's start-up leaves it (setCompositionEp, planet/InitValues_Jup.cpp:940-946)
+MCA_DEFAULTS = dict(relax=1.0, ep_h2s=14.474287352261996, l_nh4sh=0.0, ledoux=1, loading=1, max_pass=64)
+MCA_ENV = {"relax": "ATJUP_MCA_RELAX", "ledoux": "ATJUP_MCA_LEDOUX", "loading": "ATJUP_MCA_LOADING",
+           "max_pass": "ATJUP_MCA_PASSES"}
+
+
+def mca_params(**over) -> MCAParams:
+    d = dict(MCA_DEFAULTS)
+    d.update(over)
+    return MCAParams(**d)
 
 
 class PrecipParams(C.Structure):
@@ -166,7 +185,7 @@ TURB_MODELS = {"off": -1, "k_epsilon": 0, "k_omega": 1, "k_omega_SST": 2}
 
 def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
                  grid: inputs.Grid | None = None, radiation: bool = False, precip: bool = False,
-                 precip_coupling: float = 1.0, chemistry: bool = False):
+                 precip_coupling: float = 1.0, chemistry: bool = False, mca: int = 0):
     """(Grid, Consts, Knobs, env) of a configuration.
 
     "dyn"  : dynamics only, dry, closure/radiation/precipitation off (the 5-step benchmark case).
@@ -178,6 +197,8 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
              the reference's default); with precip_coupling != 0 it reports S_precip_* rates.
     chemistry: the physics block's DiffMassFluxJup / ChemMassRateJup and their smoothing run in the
              step (the reference always runs them), so massflux_* follow the state.
+    mca:     the moist convective adjustment every mca iterations after the step (ATJUP_MCA, the
+             reference's default 1); 0 off.
     """
     g = grid or inputs.Grid()
     G = Grid(im=g.im, jm=g.jm, km=g.km, r0=g.radius_km / g.L_atm, dr=g.dr, dthe=g.dthe,
@@ -195,12 +216,13 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
               tke_max_phys=1000.0, nue_max_phys=1.0e5, wall_nue_factor=4.0, wall_nue_layers=3,
               satadj=1, rk_scalar_sync=2, dry=1 if config == "dyn" else 0,
               turb_k_bg_phys=1.0e-4 if config != "dyn" else 1.0, radiation=1 if radiation else 0,
-              precip=1 if precip else 0, chemistry=1 if chemistry else 0)
+              precip=1 if precip else 0, chemistry=1 if chemistry else 0, mca=int(mca))
     env = dict(BASE_ENV)
     env["ATJUP_PRESS_SWEEPS"] = str(sweeps)
     env["ATJUP_RADIATION"] = "1" if radiation else "0"
     env["ATJUP_PRECIP"] = "1" if precip else "0"
     env["ATJUP_PRECIP_COUPLING"] = repr(float(precip_coupling))
+    env["ATJUP_MCA"] = str(int(mca))
     if config == "dyn":
         env["ATJUP_TURB"] = "0"
     else:

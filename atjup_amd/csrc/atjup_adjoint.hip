@@ -1467,7 +1467,7 @@ int atjup_hip_adj_backward(atjup_ctx* c, const double* lam_final, double* lam_in
                     if(int rc = copy_field(c, (double*)post[s].p, Fd(c, kPostFields[s]))) return rc;
             }
             if(n + 1 < s1)
-                if(int rc = step_rest(c)) return rc;
+                if(int rc = step_rest(c, iter)) return rc;
         }
         timer_end(c, ATJUP_T_ADJ_RECOMPUTE, t_rc, 0);
         for(int n = s1 - 1; n >= s0; n--){

@@ -131,6 +131,10 @@ int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, i
     if(k->radiation != 0 && k->radiation != 1){ set_error("radiation must be 0 or 1"); return ATJUP_ERR_PROFILE; }
     if(k->precip != 0 && k->precip != 1){ set_error("precip must be 0 or 1"); return ATJUP_ERR_PROFILE; }
     if(k->chemistry != 0 && k->chemistry != 1){ set_error("chemistry must be 0 or 1"); return ATJUP_ERR_PROFILE; }
+    if(k->mca < 0){ set_error("mca (the adjustment's stride) must be >= 0"); return ATJUP_ERR_PROFILE; }
+    if(k->mca > 0 && (k->dry || precision == ATJUP_PRECISION_TANGENT)){
+        set_error("the moist convective adjustment needs the moist profile (dry = 0)"); return ATJUP_ERR_PROFILE;
+    }
     if(k->chemistry && precision == ATJUP_PRECISION_TANGENT){
         set_error("the tangent form has no chemistry"); return ATJUP_ERR_PROFILE;
     }
@@ -231,12 +235,14 @@ int step_block(atjup_ctx* c, int iter, int first){
 }
 
 // the rest of the iteration: RungeKuttaJup, the boundary passes (cJupiterModel.cpp:1263-1272), the
-// closure's wall hold (:1294), the species clamp and the commit of the live k*, omega*
-int step_rest(atjup_ctx* c){
+// closure's wall hold (:1294), the moist convective adjustment every kn.mca iterations (:1301),
+// the species clamp and the commit of the live k*, omega*
+int step_rest(atjup_ctx* c, int iter){
     int rc = 0;
     if((rc = launch_rk4(c))) return rc;
     if((rc = launch_bc_after_step(c))) return rc;
     if(!c->kn.dry && c->kn.turb_model >= 0 && (rc = launch_turb_wall(c))) return rc;
+    if(c->kn.mca > 0 && iter % c->kn.mca == 0 && (rc = atjup_hip_mca(c))) return rc;   // :1301-1322
     if((rc = launch_clamp_species(c))) return rc;
     return launch_commit_live(c);
 }
@@ -387,6 +393,7 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     }
     if(!rc){
         atjup_hip_default_chem_params(&c->chemp);
+        atjup_hip_default_mca_params(&c->mcap);
         rc = alloc(c->chem_w, 3 * c->n3 * sizeof(double));
     }
     if(rc){ atjup_hip_ctx_destroy(c); return nullptr; }
@@ -404,7 +411,7 @@ void atjup_hip_ctx_destroy(atjup_ctx* c){
     for(atjup::Buf* b : { &c->lo[0], &c->lo[1], &c->lo[2], &c->lo[3], &c->colr }) release(*b);
     for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
-                          &c->amb_rows, &c->nonfinite, &c->rad_tab, &c->rad_tmp, &c->precip_srf, &c->chem_w })
+                          &c->amb_rows, &c->nonfinite, &c->rad_tab, &c->rad_tmp, &c->precip_srf, &c->chem_w, &c->mca_ws })
         release(*b);
     for(auto& v : c->spans) for(auto& sp : v){ (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
     for(auto e : c->event_pool) (void)hipEventDestroy(e);
@@ -749,6 +756,38 @@ int atjup_hip_chemistry(atjup_ctx* c){
     for(int f : mass) if(!rc) rc = launch_shapiro(c, f, 2, 1.0, 1, 7);
     return rc;
 }
+void atjup_hip_default_mca_params(atjup_mca_params* p){
+    if(!p) return;
+    // ConvectiveAdjustment.h:409-413 defaults; ep_h2s = m_h2s / M_mix of the default composition
+    // (InitValues_Jup.cpp:940-946); no NH4SH heat of formation with ATJUP_NH4SH_EQ 0
+    *p = atjup_mca_params{ 1.0, 14.474287352261996, 0.0, 1, 1, 64 };
+}
+int atjup_hip_set_mca(atjup_ctx* c, const atjup_mca_params* p){
+    if(!c || !p){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(!std::isfinite(p->relax) || !std::isfinite(p->ep_h2s) || !std::isfinite(p->l_nh4sh)){
+        set_error("non-finite adjustment parameter"); return ATJUP_ERR_ARG;
+    }
+    c->mcap = *p;
+    return 0;
+}
+int atjup_hip_mca(atjup_ctx* c){
+    if(int rc = need_mask(c)) return rc;
+    if(c->precision == ATJUP_PRECISION_TANGENT || c->kn.dry){
+        set_error("the moist convective adjustment needs the moist profile"); return ATJUP_ERR_PROFILE;
+    }
+    for(int gas = 0; gas < 2; gas++)
+        if(!c->gas_set[gas]){ set_error("gas constants not set (atjup_hip_set_gas)"); return ATJUP_ERR_STATE; }
+    if(!c->mca_ws.p){
+        // layer heights [m] as double(get_layer_height(i)) * 1e3 (ConvectiveAdjustment.h:584), then
+        // the column workspace
+        const size_t n = (size_t)c->im * c->jm * c->km;
+        if(int rc = alloc(c->mca_ws, (c->im + 8 * n) * sizeof(double) + n)) return rc;
+        std::vector<double> z(c->im);
+        for(int i = 0; i < c->im; i++) z[i] = (double)c->lh[i] * 1.0e3;
+        ATJUP_HIP_CHECK(hipMemcpy(c->mca_ws.p, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return launch_mca(c);
+}
 int atjup_hip_turbulence_run(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;
     if(c->kn.turb_model < 0) return 0;
@@ -784,7 +823,7 @@ int atjup_hip_step(atjup_ctx* c, int iter, int first){
     if(int rc = need_mask(c)) return rc;
     hipEvent_t t_step = timer_begin(c);
     if(int rc = step_block(c, iter, first)) return rc;
-    if(int rc = step_rest(c)) return rc;
+    if(int rc = step_rest(c, iter)) return rc;
     timer_end(c, ATJUP_T_STEP, t_step, 0);
     return 0;
 }

@@ -112,6 +112,9 @@ struct atjup_ctx {
     atjup::Buf precip_srf;          // double [5][jm][km]: the surface precipitation maps
     atjup_chem_params chemp = {};   // the chemistry's constants and knobs (atjup_hip_set_chemistry)
     atjup::Buf chem_w;              // double [3][n3]: the reaction rates w_nh3, w_h2s, w_nh4sh
+    atjup_mca_params mcap = {};     // the moist convective adjustment's knobs (atjup_hip_set_mca)
+    atjup::Buf mca_ws;              // its layer heights [im] (m), then the column workspace
+                                    // double [8][im][jm * km] and the flags uint8 [im][jm * km]
     atjup_gas_params gas[3] = {};   // saturation-adjustment constants (atjup_hip_set_gas)
     bool gas_set[3] = { false, false, false };
     int n_solids = 0, n_amb_rows = 0;
@@ -146,7 +149,7 @@ int launch_prep_projection(atjup_ctx* c);          // prepareProjectionBoundarie
 // one loop iteration in two parts (atjup_hip_step = step_block + step_rest): the physics block of
 // the even and first iterations, then the RK4 step, the boundary passes and the clamps
 int step_block(atjup_ctx* c, int iter, int first);
-int step_rest(atjup_ctx* c);
+int step_rest(atjup_ctx* c, int iter);
 int launch_shapiro(atjup_ctx* c, int field, int order, double strength, int passes, int axes);
 int launch_mixture_density(atjup_ctx* c);
 int launch_clamp_species(atjup_ctx* c);
@@ -162,6 +165,7 @@ int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g);
 int launch_radiation(atjup_ctx* c);
 int launch_precipitation(atjup_ctx* c);
 int launch_chem_kernel(atjup_ctx* c, int which);      // 0: DiffMassFluxJup, 1: ChemMassRateJup
+int launch_mca(atjup_ctx* c);
 int build_rad_table(atjup_ctx* c);
 int launch_flux_limiter(atjup_ctx* c);
 int launch_commit_live(atjup_ctx* c);

@@ -1760,6 +1760,300 @@ int do_chem_kernels(atjup_ctx* c, int which){
     return check_launch();
 }
 
+// =============================================================================================
+// MoistConvectiveAdjustment<P>::run (ConvectiveAdjustment.h:403-849) with the gases H2O, NH3 and
+// the tracers H2S, NH4SH (cJupiterModel.cpp:1301-1322); SMOOTH 0, PHASE 0, INTERIOR 1, cp(T) off.
+// One thread per (j, k) column of the flattened index. A column's working state (h_L, the total
+// H2O / NH3, the tracers per unit mass, each cell's own equilibrium temperature and condensate,
+// and the touched / equilibrium-known / census bits) lives in a workspace laid out [var][i][column]
+// so that a wavefront's 64 columns read one level as one contiguous line; p, rho, z and the layer
+// weights are read from the fields at each use, and the unmixed state h_L0 / q_t0 that MCA_RELAX
+// blends with is formed again from the fields, which stay untouched until the write-back.
+// Per column, as the reference: the parcel census of every adjacent pair, sweeps that mix each
+// unstable segment (grown while its ends stay unstable) and settle its cells by the Illinois
+// equilibrium solve, the optional relaxation, and the write-back of the touched cells.
+// =============================================================================================
+struct McaGas { double t_0, ep, lv, ls, C, L0, R, da, db, Ci, L0i, dai, dbi; };
+struct McaConsts {
+    McaGas gas[2];
+    double cp, grav, t_ref, r_mix, relax, ep_h2s, l_nh4sh;
+    int ledoux, loading, max_pass;
+};
+template<typename T>
+struct McaFields {
+    T *t, *v[2], *cl[2], *ice[2], *tr[2];        // tr: H2S, NH4SH
+    float* lo_t;
+    const T *p_stat, *rho;
+};
+enum { MV_HL = 0, MV_QT = 1, MV_QR = 3, MV_TEQ = 5, MV_CEQ = 6, MCA_NV = 8 };
+enum { MF_TOUCHED = 1, MF_EQ_OK = 2, MF_CEN_U = 4 };
+constexpr double MCA_TOL_K = 1.0e-7, MCA_TOL_T = 1.0e-10;            // ConvectiveAdjustment.h:416-418
+constexpr int MCA_N_ITER = 60;
+
+__device__ __forceinline__ double mca_qsat(const McaGas& s, double T, double p){
+    const double E = (T >= s.t_0) ? svp_d(T, s.C, s.L0, s.R, s.da, s.db) : svp_d(T, s.Ci, s.L0i, s.R, s.dai, s.dbi);
+    const double q = s.ep * E / p;
+    return isfinite(q) ? q : 0.0;
+}
+__device__ __forceinline__ double mca_lat(const McaGas& s, double T){ return T >= s.t_0 ? s.lv : s.ls; }
+
+// the equilibrium temperature of (h_L, q_t) at (p, z): cp T + g z - sum L(T) max(0, q_t - q_sat(T))
+// = h_L, bracketed by the all-vapour value and T_lo + sum L q_t / cp, by Illinois regula falsi
+// (ConvectiveAdjustment.h:453-478)
+__device__ bool mca_equil(const McaConsts& K, double hL, double qt0, double qt1, double p, double z,
+                          double& T, double& c0, double& c1){
+    const double T_lo = (hL - K.grav * z) / K.cp;
+    if(!(T_lo > 0.0) || !(p > 0.0)) return false;
+    if(!(qt0 > mca_qsat(K.gas[0], T_lo, p)) && !(qt1 > mca_qsat(K.gas[1], T_lo, p))){
+        T = T_lo; c0 = 0.0; c1 = 0.0;
+        return true;
+    }
+    double span = 0.0;
+    span += dmax(K.gas[0].lv, K.gas[0].ls) * dmax(0.0, qt0);
+    span += dmax(K.gas[1].lv, K.gas[1].ls) * dmax(0.0, qt1);
+    auto fof = [&](double TT){
+        double f = K.cp * TT + K.grav * z - hL;
+        f -= mca_lat(K.gas[0], TT) * dmax(0.0, qt0 - mca_qsat(K.gas[0], TT, p));
+        f -= mca_lat(K.gas[1], TT) * dmax(0.0, qt1 - mca_qsat(K.gas[1], TT, p));
+        return f;
+    };
+    double a = T_lo, b = T_lo + span / K.cp + 1.0e-9;
+    double fa = fof(a), fb = fof(b);
+    int side = 0;
+    for(int it = 0; it < MCA_N_ITER && b - a > MCA_TOL_T; it++){
+        double c = (fb - fa != 0.0) ? b - fb * (b - a) / (fb - fa) : 0.5 * (a + b);
+        if(!(c > a && c < b)) c = 0.5 * (a + b);
+        const double fc = fof(c);
+        if(fc < 0.0){ a = c; fa = fc; if(side == -1) fb *= 0.5; side = -1; }
+        else        { b = c; fb = fc; if(side == +1) fa *= 0.5; side = +1; }
+        if(fabs(fc) <= K.cp * MCA_TOL_T){ a = b = c; break; }
+    }
+    T = b;                                      // the side where the condensate is not overcounted
+    c0 = dmax(0.0, qt0 - mca_qsat(K.gas[0], T, p));
+    c1 = dmax(0.0, qt1 - mca_qsat(K.gas[1], T, p));
+    return true;
+}
+
+template<typename T>
+struct McaColumn {
+    const McaFields<T>& F;
+    const McaConsts& K;
+    const double* ztab;
+    double* ws;
+    uint8_t* wf;
+    size_t VS, P, q0;
+    int n2, col, i0, i1;
+
+    __device__ double& W(int var, int i) const { return ws[(size_t)var * VS + (size_t)i * n2 + col]; }
+    __device__ uint8_t& FL(int i) const { return wf[(size_t)i * n2 + col]; }
+    __device__ double Tk(int i) const {
+        const size_t q = q0 + (size_t)i * P;
+        double tq = (double)F.t[q];
+        if(F.lo_t) tq += (double)F.lo_t[q];
+        return tq * K.t_ref;
+    }
+    __device__ double pr(int i) const { return (double)F.p_stat[q0 + (size_t)i * P]; }
+    __device__ double rh(int i) const {                  // rho_at (cJupiterModel.h:1212-1216)
+        const double rf = (double)F.rho[q0 + (size_t)i * P];
+        return (rf > 0.0 && isfinite(rf)) ? rf : K.r_mix;
+    }
+    __device__ double wgt(int i) const {                 // rho dz, face-centred (:598-601)
+        const double lo = (i > i0) ? 0.5 * (ztab[i - 1] + ztab[i]) : ztab[i0];
+        const double hi = (i < i1) ? 0.5 * (ztab[i] + ztab[i + 1]) : ztab[i1];
+        return rh(i) * (hi - lo);
+    }
+    // h_L, q_t and the tracers of a cell from the fields (:602-616)
+    __device__ void init(int i, double& h, double* qt, double* qr) const {
+        const size_t q = q0 + (size_t)i * P;
+        const double rho = rh(i);
+        h = K.cp * Tk(i) + K.grav * ztab[i];
+        for(int g = 0; g < 2; g++){
+            const double qv = dmax(0.0, (double)F.v[g][q]) / rho;
+            const double qc = dmax(0.0, (double)F.cl[g][q]) / rho;
+            const double qi = dmax(0.0, (double)F.ice[g][q]) / rho;
+            qt[g] = qv + qc + qi;
+            h -= K.gas[g].lv * qc + K.gas[g].ls * qi;
+        }
+        qr[0] = (double)F.tr[0][q] / rho;
+        h -= 0.0 * qr[0];
+        qr[1] = (double)F.tr[1][q] / rho;
+        h -= K.l_nh4sh * qr[1];
+    }
+    __device__ double trE(int i) const {
+        double e = 0.0;
+        e += 0.0 * W(MV_QR, i);
+        e += K.l_nh4sh * W(MV_QR + 1, i);
+        return e;
+    }
+    // the virtual-temperature factor, composition only with LEDOUX (:636-647)
+    __device__ double tv(double TT, double v0, double v1, double c0, double c1, int i) const {
+        if(!K.ledoux) return TT;
+        double f = 1.0;
+        f += v0 * (1.0 / K.gas[0].ep - 1.0);
+        if(K.loading) f -= c0;
+        f += v1 * (1.0 / K.gas[1].ep - 1.0);
+        if(K.loading) f -= c1;
+        if(K.ep_h2s > 0.0) f += W(MV_QR, i) * (1.0 / K.ep_h2s - 1.0);
+        if(K.loading) f -= W(MV_QR + 1, i);
+        return TT * f;
+    }
+    // the parcel from a lifted to b against b in its own equilibrium (:651-673)
+    __device__ bool unstable(int a, int b) const {
+        double Tp, pc0, pc1;
+        if(!mca_equil(K, W(MV_HL, a) + trE(a), W(MV_QT, a), W(MV_QT + 1, a), pr(b), ztab[b], Tp, pc0, pc1)) return false;
+        if(!(FL(b) & MF_EQ_OK)){
+            double Te, e0, e1;
+            if(!mca_equil(K, W(MV_HL, b) + trE(b), W(MV_QT, b), W(MV_QT + 1, b), pr(b), ztab[b], Te, e0, e1)) return false;
+            W(MV_TEQ, b) = Te; W(MV_CEQ, b) = e0; W(MV_CEQ + 1, b) = e1;
+            FL(b) |= MF_EQ_OK;
+        }
+        const double ec0 = W(MV_CEQ, b), ec1 = W(MV_CEQ + 1, b);
+        const double pv0 = W(MV_QT, a) - pc0, pv1 = W(MV_QT + 1, a) - pc1;
+        const double ev0 = W(MV_QT, b) - ec0, ev1 = W(MV_QT + 1, b) - ec1;
+        return tv(Tp, pv0, pv1, pc0, pc1, a) - tv(W(MV_TEQ, b), ev0, ev1, ec0, ec1, b) > MCA_TOL_K;
+    }
+    // a cell to its equilibrium from h_L, q_t (:674-690); the vapour and the liquid / ice split
+    // follow from (q_t, condensate, T) at the write-back
+    __device__ bool settle(int q) const {
+        double TT, c0, c1;
+        if(!mca_equil(K, W(MV_HL, q) + trE(q), W(MV_QT, q), W(MV_QT + 1, q), pr(q), ztab[q], TT, c0, c1)) return false;
+        W(MV_TEQ, q) = TT; W(MV_CEQ, q) = c0; W(MV_CEQ + 1, q) = c1;
+        FL(q) |= MF_TOUCHED | MF_EQ_OK;
+        return true;
+    }
+};
+
+template<typename T>
+__global__ __launch_bounds__(64) void k_mca(const McaFields<T> F, const uint8_t* __restrict__ mask,
+        const double* __restrict__ ztab, double* __restrict__ ws, uint8_t* __restrict__ wf,
+        int im, int jm, int km, int pitch, const McaConsts K){
+    const int n2 = jm * km, col = blockIdx.x * 64 + threadIdx.x;
+    if(col >= n2) return;
+    const int j = col / km, k = col - j * km;
+    const size_t P = (size_t)jm * pitch, q0 = (size_t)j * pitch + k;
+    int i0 = 0;
+    while(i0 < im && mask[q0 + (size_t)i0 * P] == 1) i0++;
+    int i1 = i0;
+    while(i1 + 1 < im && mask[q0 + (size_t)(i1 + 1) * P] != 1) i1++;
+    if(i0 == 0) i0 = 1;                                  // MCA_INTERIOR (:566-575)
+    if(i1 == im - 1) i1 = im - 2;
+    if(i1 - i0 < 1) return;
+    const McaColumn<T> C{ F, K, ztab, ws, wf, (size_t)im * n2, P, q0, n2, col, i0, i1 };
+    for(int i = i0; i <= i1; i++){                       // a non-physical column is left alone
+        const double Tq = C.Tk(i), p = C.pr(i), rho = C.rh(i);
+        if(!(Tq > 0.0) || !(p > 0.0) || !(rho > 0.0) || !isfinite(Tq + p + rho)) return;
+    }
+    for(int i = i0; i <= i1; i++){
+        if(!(C.wgt(i) > 0.0)) return;
+        double h, qt[2], qr[2];
+        C.init(i, h, qt, qr);
+        C.W(MV_HL, i) = h; C.W(MV_QT, i) = qt[0]; C.W(MV_QT + 1, i) = qt[1];
+        C.W(MV_QR, i) = qr[0]; C.W(MV_QR + 1, i) = qr[1];
+        C.FL(i) = 0;
+    }
+    for(int i = i0; i < i1; i++) if(C.unstable(i, i + 1)) C.FL(i) |= MF_CEN_U;     // the census (:700-706)
+    const int max_pass = K.max_pass > 0 ? K.max_pass : 64;
+    int passes = 0;
+    bool changed = true;
+    while(changed && passes < max_pass){                 // the sweeps (:716-756)
+        changed = false; passes++;
+        int i = i0;
+        while(i < i1){
+            const bool uns = (passes == 1 && !(C.FL(i) & MF_TOUCHED) && !(C.FL(i + 1) & MF_TOUCHED))
+                ? (C.FL(i) & MF_CEN_U) != 0 : C.unstable(i, i + 1);
+            if(!uns){ i++; continue; }
+            int a = i, b = i + 1;
+            for(;;){
+                double sw = 0.0, sh = 0.0, sq0 = 0.0, sq1 = 0.0, sr0 = 0.0, sr1 = 0.0;
+                for(int q = a; q <= b; q++){
+                    const double w = C.wgt(q);
+                    sw += w; sh += w * C.W(MV_HL, q);
+                    sq0 += w * C.W(MV_QT, q); sq1 += w * C.W(MV_QT + 1, q);
+                    sr0 += w * C.W(MV_QR, q); sr1 += w * C.W(MV_QR + 1, q);
+                }
+                for(int q = a; q <= b; q++){
+                    C.W(MV_HL, q) = sh / sw;
+                    C.W(MV_QT, q) = sq0 / sw; C.W(MV_QT + 1, q) = sq1 / sw;
+                    C.W(MV_QR, q) = sr0 / sw; C.W(MV_QR + 1, q) = sr1 / sw;
+                    if(!C.settle(q)) return;             // the column keeps its state
+                }
+                bool ext = false;
+                if(a > i0 && C.unstable(a - 1, a)){ a--; ext = true; }
+                if(b < i1 && C.unstable(b, b + 1)){ b++; ext = true; }
+                if(!ext) break;
+            }
+            changed = true;
+            i = b;
+        }
+    }
+    if(K.relax < 1.0){                                   // MCA_RELAX (:800-811)
+        const double relax = dmax(0.0, K.relax);
+        for(int q = i0; q <= i1; q++){
+            if(!(C.FL(q) & MF_TOUCHED)) continue;
+            double h0, qt0[2], qr0[2];
+            C.init(q, h0, qt0, qr0);
+            C.W(MV_HL, q) = h0 + relax * (C.W(MV_HL, q) - h0);
+            C.W(MV_QT, q) = qt0[0] + relax * (C.W(MV_QT, q) - qt0[0]);
+            C.W(MV_QT + 1, q) = qt0[1] + relax * (C.W(MV_QT + 1, q) - qt0[1]);
+            C.W(MV_QR, q) = qr0[0] + relax * (C.W(MV_QR, q) - qr0[0]);
+            C.W(MV_QR + 1, q) = qr0[1] + relax * (C.W(MV_QR + 1, q) - qr0[1]);
+            if(!C.settle(q)) return;
+        }
+    }
+    for(int q = i0; q <= i1; q++){                       // the write-back (:822-833)
+        if(!(C.FL(q) & MF_TOUCHED)) continue;
+        const size_t c = q0 + (size_t)q * P;
+        const double rho = C.rh(q), TT = C.W(MV_TEQ, q);
+        const double tq = TT / K.t_ref;
+        if(F.lo_t){
+            const float h = (float)tq;
+            F.t[c] = h;
+            F.lo_t[c] = (float)(tq - (double)h);
+        } else {
+            F.t[c] = (T)tq;
+        }
+        for(int g = 0; g < 2; g++){
+            const double ce = C.W(MV_CEQ + g, q), qv = C.W(MV_QT + g, q) - ce;
+            const bool liquid = TT >= K.gas[g].t_0;
+            F.v[g][c] = (T)(qv * rho);
+            F.cl[g][c] = (T)((liquid ? ce : 0.0) * rho);
+            F.ice[g][c] = (T)((liquid ? 0.0 : ce) * rho);
+        }
+        F.tr[0][c] = (T)(C.W(MV_QR, q) * rho);
+        F.tr[1][c] = (T)(C.W(MV_QR + 1, q) * rho);
+    }
+}
+
+template<typename T>
+int do_mca(atjup_ctx* c){
+    static const int gv[2] = { ATJUP_F_H2O, ATJUP_F_NH3 };
+    McaFields<T> Fm;
+    Fm.t = F<T>(c, ATJUP_F_T); Fm.lo_t = (float*)c->lo[ATJUP_F_T].p;
+    Fm.p_stat = F<T>(c, ATJUP_F_P_STAT); Fm.rho = F<T>(c, ATJUP_F_RHO_MIX);
+    Fm.tr[0] = F<T>(c, ATJUP_F_H2S); Fm.tr[1] = F<T>(c, ATJUP_F_NH4SH);
+    McaConsts K;
+    for(int g = 0; g < 2; g++){
+        Fm.v[g] = F<T>(c, gv[g]); Fm.cl[g] = F<T>(c, gv[g] + 1); Fm.ice[g] = F<T>(c, gv[g] + 2);
+        const atjup_gas_params& a = c->gas[g];
+        K.gas[g] = McaGas{ a.t_0, a.ep, a.lv, a.ls, a.C, a.L0, a.R, a.del_alf, a.del_bet,
+                           a.C_i, a.L0_i, a.del_alf_i, a.del_bet_i };
+    }
+    const atjup_mca_params& mp = c->mcap;
+    K.cp = c->c.cp_mix; K.grav = c->c.g; K.t_ref = c->c.t_ref; K.r_mix = c->c.r_mix;
+    K.relax = std::min(1.0, std::max(0.0, mp.relax));
+    K.ep_h2s = mp.ep_h2s; K.l_nh4sh = mp.l_nh4sh;
+    K.ledoux = mp.ledoux != 0; K.loading = mp.loading != 0; K.max_pass = mp.max_pass;
+    const size_t n = (size_t)c->im * c->jm * c->km;
+    double* ztab = (double*)c->mca_ws.p;
+    double* ws = ztab + c->im;
+    uint8_t* wf = (uint8_t*)(ws + MCA_NV * n);
+    hipEvent_t t0 = timer_begin(c);
+    hipLaunchKernelGGL(k_mca<T>, dim3(nblk(c->jm * c->km, 64)), dim3(64), 0, c->stream,
+        Fm, (const uint8_t*)c->mask.p, (const double*)ztab, ws, wf, c->im, c->jm, c->km, c->pitch, K);
+    timer_end(c, ATJUP_T_MCA, t0, 1);
+    return check_launch();
+}
+
 }  // namespace
 
 // the tangent form covers the dynamics profile only
@@ -1776,6 +2070,7 @@ int launch_flux_limiter(atjup_ctx* c){ return DISPATCH(do_flux_limiter, c); }
 int launch_radiation(atjup_ctx* c){ return DISPATCH(do_radiation, c); }
 int launch_precipitation(atjup_ctx* c){ return DISPATCH(do_precipitation, c); }
 int launch_chem_kernel(atjup_ctx* c, int which){ return DISPATCH(do_chem_kernels, c, which); }
+int launch_mca(atjup_ctx* c){ return DISPATCH(do_mca, c); }
 int launch_commit_live(atjup_ctx* c){ return c->turb_live_pending ? DISPATCH(do_commit_live, c) : 0; }
 
 }  // namespace atjup

@@ -51,7 +51,7 @@ def load():
               "atjup_hip_mixture_density", "atjup_hip_flux_limiter", "atjup_hip_clamp_species",
               "atjup_hip_swap_nlevel", "atjup_hip_scalar_sync", "atjup_hip_check_finite",
               "atjup_hip_sync", "atjup_hip_pitch", "atjup_hip_radiation", "atjup_hip_precipitation",
-              "atjup_hip_chemistry"):
+              "atjup_hip_chemistry", "atjup_hip_mca"):
         getattr(L, n).argtypes = [C.c_void_p]
     L.atjup_hip_device_count.argtypes = [C.POINTER(C.c_int)]
     for n in ("atjup_hip_upload_tangent", "atjup_hip_adj_upload"):
@@ -73,6 +73,8 @@ def load():
     L.atjup_hip_download_precip_surface.argtypes = [C.c_void_p, _D]
     L.atjup_hip_set_chemistry.argtypes = [C.c_void_p, C.POINTER(abi.ChemParams)]
     L.atjup_hip_default_chem_params.argtypes = [C.POINTER(abi.ChemParams)]
+    L.atjup_hip_set_mca.argtypes = [C.c_void_p, C.POINTER(abi.MCAParams)]
+    L.atjup_hip_default_mca_params.argtypes = [C.POINTER(abi.MCAParams)]
     L.atjup_hip_save_state.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
     L.atjup_hip_load_state.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.atjup_hip_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
@@ -217,6 +219,13 @@ class Context:
     def chemistry(self):
         self._chk(self.L.atjup_hip_chemistry(self.h), "chemistry")
 
+    def set_mca(self, params=None):
+        mp = params if params is not None else abi.mca_params()
+        self._chk(self.L.atjup_hip_set_mca(self.h, C.byref(mp)), "set_mca")
+
+    def mca(self):
+        self._chk(self.L.atjup_hip_mca(self.h), "mca")
+
     def save_restart(self, path, iteration, carried=True):
         """cJupiterModel::save_state: the reference's JUP2 (or, carried=False, JUP1) file."""
         self._chk(self.L.atjup_hip_save_state(self.h, str(path).encode(), int(iteration), 1 if carried else 0),
@@ -318,7 +327,7 @@ class Context:
         return out
 
     TIMERS = ("poisson_sweeps", "rk_stages", "pressure", "step", "adj_sweeps", "adj_step", "adj_recompute",
-              "radiation", "precipitation", "chemistry")
+              "radiation", "precipitation", "chemistry", "mca")
 
     def set_timing(self, on=True):
         self._chk(self.L.atjup_hip_set_timing(self.h, 1 if on else 0), "set_timing")

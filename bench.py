@@ -253,6 +253,24 @@ def chemistry_leg(ctx, g, calls=20):
             "ms_per_call": wall * 1e3, "kernels_ms_per_call": k_ms / calls, "kernel_launches_per_call": launches / calls}
 
 
+def mca_leg(ctx, g, calls=5):
+    """The moist convective adjustment (one thread per column): HIP-event time of the first call on
+    the stepped state, where nearly every column of the seeded state is unstable and is mixed, then
+    per call on the adjusted state, where the parcel census finds few unstable pairs."""
+    ctx.sync()
+    ctx.set_timing(True)
+    ctx.read_timers()
+    ctx.mca()
+    first_ms, _ = ctx.read_timers()["mca"]
+    for _ in range(calls):
+        ctx.mca()
+    again_ms, launches = ctx.read_timers()["mca"]
+    return {"workload": f"moist convective adjustment (H2O, NH3; H2S, NH4SH tracers) of all {g.jm * g.km} "
+                        f"columns, {g.im} levels",
+            "first_call_ms": first_ms, "adjusted_state_ms_per_call": again_ms / calls,
+            "launches_per_call": launches / calls}
+
+
 def restart_leg(ctx, g):
     """save_state then load_state of the device-resident state (JUP2, 55 arrays and vel_star), in a
     temporary directory on the host's disk: the D2H/H2D transfers and the file I/O together."""
@@ -394,6 +412,7 @@ def main():
     rad = radiation_leg(ctx, g) if args.config == "full" else None
     precip = precipitation_leg(ctx, g) if args.config == "full" else None
     chem = chemistry_leg(ctx, g) if args.config == "full" else None
+    mca = mca_leg(ctx, g) if args.config == "full" else None
     rst = None
     if not args.no_restart:
         try:
@@ -466,6 +485,7 @@ def main():
         "radiation": rad,
         "precipitation": precip,
         "chemistry": chem,
+        "mca": mca,
         "restart": rst,
     }
     print(json.dumps(out))

@@ -117,6 +117,9 @@ typedef struct atjup_knobs {
     int    chemistry;           /* the physics block's chemistry (DiffMassFluxJup, ChemMassRateJup and
                                    their smoothing, cJupiterModel.cpp:1146-1161, which the reference
                                    always runs) in the step; 0: massflux_* enter as frozen inputs */
+    int    mca;                 /* ATJUP_MCA=<N>: the moist convective adjustment every N iterations,
+                                   after the closure's wall hold (cJupiterModel.cpp:115-125, :1301-1322;
+                                   the reference's default is 1); 0 off */
 } atjup_knobs;
 
 /* The chemistry's constants as the reference's start-up leaves them (planet/cJupiterModel.h:172-401:
@@ -132,6 +135,18 @@ typedef struct atjup_chem_params {
     double cp_nh3, cp_h2s, cp_nh4sh, k_nh3, k_h2s, k_nh4sh, dh_nh4sh, t_0_nh4sh, t_00_nh4sh;
     int    gate_zero, molar_conc, enthalpy;
 } atjup_chem_params;
+
+/* The moist convective adjustment's knobs and the constants beyond the gases' (H2O and NH3 take
+ * atjup_hip_set_gas's values): ATJUP_MCA_LEDOUX (1), ATJUP_MCA_LOADING (1), ATJUP_MCA_RELAX (1, in
+ * [0, 1]), ATJUP_MCA_PASSES (64) (planet/ConvectiveAdjustment.h:409-413), the H2S tracer's ep_h2s
+ * (InitValues_Jup.cpp:945) and the NH4SH tracer's heat of formation (0 unless ATJUP_NH4SH_EQ,
+ * planet/cJupiterModel.cpp:1316). The supported settings are ATJUP_MCA_SMOOTH 0, ATJUP_MCA_PHASE 0,
+ * ATJUP_MCA_INTERIOR 1, ATJUP_CP_H2 and ATJUP_CP_GAS 0 and the chromophore off (ATJUP_CHROM 0), which
+ * leaves that tracer zero everywhere. */
+typedef struct atjup_mca_params {
+    double relax, ep_h2s, l_nh4sh;
+    int    ledoux, loading, max_pass;
+} atjup_mca_params;
 
 /* The precipitation's constants beyond the gases' (which atjup_hip_set_gas supplies):
  * ATJUP_PRECIP_SCALE on the five rate coefficients (planet/Precipitation.h:111-115,
@@ -240,6 +255,17 @@ int atjup_hip_download_precip_surface(atjup_ctx* ctx, double* dst);
 int atjup_hip_chemistry(atjup_ctx* ctx);
 int atjup_hip_set_chemistry(atjup_ctx* ctx, const atjup_chem_params* params);
 void atjup_hip_default_chem_params(atjup_chem_params* params);
+/* MoistConvectiveAdjustment<cJupiterModel>::run (planet/ConvectiveAdjustment.h:403-849) with the gases
+ * H2O and NH3 and the tracers H2S and NH4SH (planet/cJupiterModel.cpp:1301-1322): per (j, k) column
+ * of fluid cells (the radial boundary planes left out) the parcel test of every adjacent pair
+ * (Ledoux: virtual temperature with the condensate loading), then sweeps that mix h_L, the total
+ * H2O / NH3 and the tracers over each unstable segment (grown while its ends stay unstable) and
+ * bring its cells to equilibrium, until a sweep finds nothing or the sweep cap; the touched cells
+ * get t, the vapour, cloud and ice of H2O and NH3, H2S and NH4SH. A column with a non-physical
+ * state, or whose equilibrium solve fails, keeps its state. The printed census is not formed. */
+int atjup_hip_mca(atjup_ctx* ctx);
+int atjup_hip_set_mca(atjup_ctx* ctx, const atjup_mca_params* params);
+void atjup_hip_default_mca_params(atjup_mca_params* params);
 /* cJupiterModel::save_state / load_state (planet/FileIO_Jup.cpp:584-684): the binary restart file,
  * byte for byte the reference's layout: int32 {magic, im, jm, km, iter}, then the 38 fixed 3D arrays
  * of restart_arrays() (:281-293), in JUP2 files the 17 carried ones of restart_arrays_carried()
@@ -298,6 +324,7 @@ enum atjup_timer {
     ATJUP_T_RADIATION,            /* the radiation's two kernels (layers, columns) */
     ATJUP_T_PRECIPITATION,        /* the precipitation's column kernel */
     ATJUP_T_CHEMISTRY,            /* the chemistry's two kernels (the smoothing passes not included) */
+    ATJUP_T_MCA,                  /* the moist convective adjustment's column kernel */
     ATJUP_N_TIMERS
 };
 int atjup_hip_set_timing(atjup_ctx* ctx, int enable);

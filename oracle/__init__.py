@@ -53,7 +53,8 @@ def _orc_lib():
                   "orc_bc_radius", "orc_bc_theta", "orc_bc_phi", "orc_bc_solid",
                   "orc_mixture_density", "orc_clamp_species", "orc_swap_nlevel",
                   "orc_scalar_sync", "orc_turbulence_run", "orc_turb_wall", "orc_flux_limiter",
-                  "orc_check_finite", "orc_radiation", "orc_precipitation", "orc_chemistry"):
+                  "orc_check_finite", "orc_radiation", "orc_precipitation", "orc_chemistry",
+                  "orc_mca"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.orc_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_satadj.argtypes = [C.c_void_p, C.c_int]
@@ -61,6 +62,7 @@ def _orc_lib():
         L.orc_set_radiation.argtypes = [C.c_void_p, C.POINTER(abi.RadParams)]
         L.orc_set_precipitation.argtypes = [C.c_void_p, C.POINTER(abi.PrecipParams)]
         L.orc_set_chemistry.argtypes = [C.c_void_p, C.POINTER(abi.ChemParams)]
+        L.orc_set_mca.argtypes = [C.c_void_p, C.POINTER(abi.MCAParams)]
         L.orc_download_precip_surface.argtypes = [C.c_void_p, _D]
         L.orc_shapiro.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -128,6 +130,9 @@ class Oracle:
 
     def set_chemistry(self, params):
         self._chk(self.L.orc_set_chemistry(self.h, C.byref(params)), "set_chemistry")
+
+    def set_mca(self, params):
+        self._chk(self.L.orc_set_mca(self.h, C.byref(params)), "set_mca")
 
     def set_precipitation(self, params):
         self._chk(self.L.orc_set_precipitation(self.h, C.byref(params)), "set_precipitation")

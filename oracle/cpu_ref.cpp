@@ -62,6 +62,7 @@ struct orc_ctx {
                                 2190.0, 2240.0, 2000.0, 0.02102, 0.013, 0.0, 1768700.0, 230.0, 200.0,
                                 1, 0, 0 };             // the chemistry's constants (cJupiterModel.h:172-401)
     std::vector<double> w_nh3, w_h2s, w_nh4sh;       // the reaction rates, persistent between calls
+    atjup_mca_params mcap = { 1.0, 14.474287352261996, 0.0, 1, 1, 64 };   // moist convective adjustment knobs
     double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
 
     size_t id(int i, int j, int k) const { return ((size_t)i * jm + j) * km + k; }
@@ -1891,6 +1892,220 @@ void chemistry(orc_ctx& o){
         shapiro(o, o.f[f], 2, 1.0, 1, 7);
 }
 
+// ---- MoistConvectiveAdjustment<P>::run (ConvectiveAdjustment.h:403-849) --------------------------
+// Gases H2O and NH3 (gas[0], gas[1]); tracers H2S (ep_h2s, a vapour, no heat) and NH4SH (ep 0, a
+// condensate, heat l_nh4sh); the chromophore tracer is zero with ATJUP_CHROM 0 and changes nothing.
+// Settings: SMOOTH 0, PHASE 0, INTERIOR 1, cp(T) off (H(T) = cp_mix T). The census counters and
+// the conservation diagnostics are print-only and not formed.
+namespace mca_detail {
+constexpr int NG = 2, NT = 2;
+constexpr double TOL_K = 1.0e-7, TOL_T = 1.0e-10;      // :416-418
+constexpr int N_ITER = 60;
+
+struct Col {
+    const orc_ctx& o;
+    double cp, grav, ep_tr[NT], L_tr[NT];
+    bool cond_tr[NT];
+    // q_sat over the phase T selects, without the liquid/ice split (:430-445)
+    double qsat(int g, double T, double p) const {
+        const atjup_gas_params& s = o.gas[g];
+        const double E = (T >= s.t_0) ? svp(T, s.C, s.L0, s.R, s.del_alf, s.del_bet)
+                                      : svp(T, s.C_i, s.L0_i, s.R, s.del_alf_i, s.del_bet_i);
+        const double q = s.ep * E / p;
+        return std::isfinite(q) ? q : 0.0;
+    }
+    double lat(int g, double T) const { return T >= o.gas[g].t_0 ? o.gas[g].lv : o.gas[g].ls; }
+    // the equilibrium temperature of (h_L, q_t) at (p, z) by Illinois regula falsi (:453-478)
+    bool equil(double hL, const double* qt, double p, double z, double& T, double* cond) const {
+        const double T_lo = (hL - grav * z) / cp;
+        if(!(T_lo > 0.0) || !(p > 0.0)) return false;
+        bool any = false;
+        for(int g = 0; g < NG; g++) if(qt[g] > qsat(g, T_lo, p)){ any = true; break; }
+        if(!any){ T = T_lo; for(int g = 0; g < NG; g++) cond[g] = 0.0; return true; }
+        double span = 0.0;
+        for(int g = 0; g < NG; g++) span += std::max(o.gas[g].lv, o.gas[g].ls) * std::max(0.0, qt[g]);
+        auto fof = [&](double TT){
+            double f = cp * TT + grav * z - hL;
+            for(int g = 0; g < NG; g++) f -= lat(g, TT) * std::max(0.0, qt[g] - qsat(g, TT, p));
+            return f;
+        };
+        double a = T_lo, b = T_lo + span / cp + 1.0e-9;
+        double fa = fof(a), fb = fof(b);
+        int side = 0;
+        for(int it = 0; it < N_ITER && b - a > TOL_T; it++){
+            double c = (fb - fa != 0.0) ? b - fb * (b - a) / (fb - fa) : 0.5 * (a + b);
+            if(!(c > a && c < b)) c = 0.5 * (a + b);
+            const double fc = fof(c);
+            if(fc < 0.0){ a = c; fa = fc; if(side == -1) fb *= 0.5; side = -1; }
+            else        { b = c; fb = fc; if(side == +1) fa *= 0.5; side = +1; }
+            if(std::fabs(fc) <= cp * TOL_T){ a = b = c; break; }
+        }
+        T = b;
+        for(int g = 0; g < NG; g++) cond[g] = std::max(0.0, qt[g] - qsat(g, T, p));
+        return true;
+    }
+};
+}  // namespace mca_detail
+
+void mca(orc_ctx& o){
+    using namespace mca_detail;
+    const atjup_mca_params& P = o.mcap;
+    const int im = o.im, jm = o.jm, km = o.km;
+    const double relax = std::min(1.0, std::max(0.0, P.relax));
+    const int max_pass = P.max_pass > 0 ? P.max_pass : 64;
+    const bool ledoux = P.ledoux != 0, loading = P.loading != 0;
+    static const int gf[NG][3] = { { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE },
+                                   { ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE } };
+    static const int tf[NT] = { ATJUP_F_H2S, ATJUP_F_NH4SH };
+    Col C{ o, o.c.cp_mix, o.c.g, { P.ep_h2s, 0.0 }, { 0.0, P.l_nh4sh }, { false, true } };
+    #pragma omp parallel for collapse(2) schedule(dynamic, 8)
+    for(int j = 0; j < jm; j++)
+        for(int k = 0; k < km; k++){
+            int i0 = 0;
+            while(i0 < im && o.mask[o.id(i0, j, k)] == 1.0) i0++;
+            int i1 = i0;
+            while(i1 + 1 < im && o.mask[o.id(i1 + 1, j, k)] != 1.0) i1++;
+            if(i0 == 0) i0 = 1;                               // MCA_INTERIOR (:566-575)
+            if(i1 == im - 1) i1 = im - 2;
+            if(i1 - i0 < 1) continue;
+            std::vector<double> T(im), T0(im), p(im), z(im), rho(im), w(im), hL(im);
+            std::vector<double> qt(im * NG), qv(im * NG), qc(im * NG), qi(im * NG), qr(im * NT);
+            std::vector<double> Teq(im), ceq(im * NG);
+            std::vector<char> touched(im, 0), eq_ok(im, 0), cen_u(im, 0);
+            bool ok = true;
+            for(int i = i0; i <= i1 && ok; i++){
+                const size_t q = o.id(i, j, k);
+                T[i] = T0[i] = o.f[ATJUP_F_T][q] * o.c.t_ref;
+                p[i] = o.f[ATJUP_F_P_STAT][q];
+                z[i] = o.get_layer_height(i) * 1.0e3;
+                const double rf = o.f[ATJUP_F_RHO_MIX][q];
+                rho[i] = (rf > 0.0 && std::isfinite(rf)) ? rf : o.c.r_mix;
+                if(!(T[i] > 0.0) || !(p[i] > 0.0) || !(rho[i] > 0.0) || !std::isfinite(T[i] + p[i] + rho[i])) ok = false;
+            }
+            if(!ok) continue;
+            for(int i = i0; i <= i1 && ok; i++){
+                const size_t q = o.id(i, j, k);
+                const double lo = (i > i0) ? 0.5 * (z[i - 1] + z[i]) : z[i0];
+                const double hi = (i < i1) ? 0.5 * (z[i] + z[i + 1]) : z[i1];
+                w[i] = rho[i] * (hi - lo);
+                if(!(w[i] > 0.0)) ok = false;
+                double h = C.cp * T[i] + C.grav * z[i];
+                for(int g = 0; g < NG; g++){
+                    const int x = i * NG + g;
+                    qv[x] = std::max(0.0, o.f[gf[g][0]][q]) / rho[i];
+                    qc[x] = std::max(0.0, o.f[gf[g][1]][q]) / rho[i];
+                    qi[x] = std::max(0.0, o.f[gf[g][2]][q]) / rho[i];
+                    qt[x] = qv[x] + qc[x] + qi[x];
+                    h -= o.gas[g].lv * qc[x] + o.gas[g].ls * qi[x];
+                }
+                for(int r = 0; r < NT; r++){
+                    qr[i * NT + r] = o.f[tf[r]][q] / rho[i];
+                    h -= C.L_tr[r] * qr[i * NT + r];
+                }
+                hL[i] = h;
+            }
+            if(!ok) continue;
+            auto trE = [&](int i){ double e = 0.0; for(int r = 0; r < NT; r++) e += C.L_tr[r] * qr[i * NT + r]; return e; };
+            auto tv = [&](double TT, const double* v_, const double* cn, int i){
+                if(!ledoux) return TT;
+                double f = 1.0;
+                for(int g = 0; g < NG; g++){ f += v_[g] * (1.0 / o.gas[g].ep - 1.0); if(loading) f -= cn[g]; }
+                for(int r = 0; r < NT; r++){
+                    const double qq = qr[i * NT + r];
+                    if(C.ep_tr[r] > 0.0) f += qq * (1.0 / C.ep_tr[r] - 1.0);
+                    else if(C.cond_tr[r] && loading) f -= qq;
+                }
+                return TT * f;
+            };
+            double pc[NG], pv[NG], ec[NG], ev[NG];
+            // the parcel from a lifted to b against b in its own equilibrium (:651-673)
+            auto unstable = [&](int a, int b) -> bool {
+                double Tp;
+                if(!C.equil(hL[a] + trE(a), &qt[a * NG], p[b], z[b], Tp, pc)) return false;
+                if(!eq_ok[b]){
+                    if(!C.equil(hL[b] + trE(b), &qt[b * NG], p[b], z[b], Teq[b], &ceq[b * NG])) return false;
+                    eq_ok[b] = 1;
+                }
+                for(int g = 0; g < NG; g++){
+                    ec[g] = ceq[b * NG + g];
+                    pv[g] = qt[a * NG + g] - pc[g];
+                    ev[g] = qt[b * NG + g] - ec[g];
+                }
+                return tv(Tp, pv, pc, a) - tv(Teq[b], ev, ec, b) > TOL_K;
+            };
+            auto settle = [&](int q) -> bool {                 // (:674-690)
+                double TT;
+                if(!C.equil(hL[q] + trE(q), &qt[q * NG], p[q], z[q], TT, pc)) return false;
+                T[q] = TT; Teq[q] = TT; eq_ok[q] = 1;
+                for(int g = 0; g < NG; g++){
+                    const int x = q * NG + g;
+                    ceq[x] = pc[g];
+                    qv[x] = qt[x] - pc[g];
+                    if(TT >= o.gas[g].t_0){ qc[x] = pc[g]; qi[x] = 0.0; } else { qi[x] = pc[g]; qc[x] = 0.0; }
+                }
+                touched[q] = 1;
+                return true;
+            };
+            for(int i = i0; i < i1; i++) cen_u[i] = unstable(i, i + 1);      // the census (:700-706)
+            const std::vector<double> hL0(hL), qt0(qt), qr0(qr);
+            int passes = 0; long long layers = 0; bool changed = true, fail = false;
+            while(changed && passes < max_pass && !fail){                       // (:716-756)
+                changed = false; passes++;
+                int i = i0;
+                while(i < i1 && !fail){
+                    const bool uns = (passes == 1 && !touched[i] && !touched[i + 1]) ? (bool)cen_u[i] : unstable(i, i + 1);
+                    if(!uns){ i++; continue; }
+                    int a = i, b = i + 1;
+                    for(;;){
+                        double sw = 0.0, sh = 0.0, sq[NG] = { 0.0, 0.0 }, sr[NT] = { 0.0, 0.0 };
+                        for(int q = a; q <= b; q++){
+                            sw += w[q]; sh += w[q] * hL[q];
+                            for(int g = 0; g < NG; g++) sq[g] += w[q] * qt[q * NG + g];
+                            for(int r = 0; r < NT; r++) sr[r] += w[q] * qr[q * NT + r];
+                        }
+                        for(int q = a; q <= b; q++){
+                            hL[q] = sh / sw;
+                            for(int g = 0; g < NG; g++) qt[q * NG + g] = sq[g] / sw;
+                            for(int r = 0; r < NT; r++) qr[q * NT + r] = sr[r] / sw;
+                            if(!settle(q)){ fail = true; break; }
+                        }
+                        if(fail) break;
+                        bool ext = false;
+                        if(a > i0 && unstable(a - 1, a)){ a--; ext = true; }
+                        if(b < i1 && unstable(b, b + 1)){ b++; ext = true; }
+                        if(!ext) break;
+                    }
+                    layers += (b - a + 1);
+                    changed = true;
+                    i = b;
+                }
+            }
+            if(fail || layers == 0) continue;                 // the column keeps its state
+            if(relax < 1.0){                                  // MCA_RELAX (:800-811)
+                for(int q = i0; q <= i1; q++){
+                    if(!touched[q]) continue;
+                    hL[q] = hL0[q] + relax * (hL[q] - hL0[q]);
+                    for(int g = 0; g < NG; g++){ const int x = q * NG + g; qt[x] = qt0[x] + relax * (qt[x] - qt0[x]); }
+                    for(int r = 0; r < NT; r++){ const int x = q * NT + r; qr[x] = qr0[x] + relax * (qr[x] - qr0[x]); }
+                    if(!settle(q)){ fail = true; break; }
+                }
+                if(fail) continue;
+            }
+            for(int q = i0; q <= i1; q++){                    // (:822-833)
+                if(!touched[q]) continue;
+                const size_t c = o.id(q, j, k);
+                o.f[ATJUP_F_T][c] = T[q] / o.c.t_ref;
+                for(int g = 0; g < NG; g++){
+                    const int x = q * NG + g;
+                    o.f[gf[g][0]][c] = qv[x] * rho[q];
+                    o.f[gf[g][1]][c] = qc[x] * rho[q];
+                    o.f[gf[g][2]][c] = qi[x] * rho[q];
+                }
+                for(int r = 0; r < NT; r++) o.f[tf[r]][c] = qr[q * NT + r] * rho[q];
+            }
+        }
+}
+
 bool valid_field(int f){
     return (f >= 0 && f < ATJUP_N_PROG) || (f >= ATJUP_F_P_DYN && f < ATJUP_F_END)
         || (f >= ATJUP_NLEVEL && f < ATJUP_NLEVEL + ATJUP_N_PROG);
@@ -2052,6 +2267,12 @@ int orc_set_chemistry(orc_ctx* o, const atjup_chem_params* p){
     return 0;
 }
 int orc_chemistry(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; chemistry(*o); return 0; }
+int orc_set_mca(orc_ctx* o, const atjup_mca_params* p){
+    if(!o || !p) return ATJUP_ERR_ARG;
+    o->mcap = *p;
+    return 0;
+}
+int orc_mca(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; mca(*o); return 0; }
 int orc_set_precipitation(orc_ctx* o, const atjup_precip_params* p){
     if(!o || !p) return ATJUP_ERR_ARG;
     o->precp = *p;
@@ -2086,6 +2307,7 @@ int orc_step(orc_ctx* o, int iter, int first){
     rk4(*o);
     bc_radius(*o); bc_theta(*o); bc_phi(*o); bc_solid(*o);
     if(o->kn.turb_model >= 0){ turb_apply_wall_bc(*o); turb_wall_omega(*o); }
+    if(o->kn.mca > 0 && iter % o->kn.mca == 0) mca(*o);  // cJupiterModel.cpp:1301-1322
     clamp_species(*o);
     restore(*o);
     return 0;

@@ -42,6 +42,8 @@ int      orc_radiation(orc_ctx* o);
 int      orc_set_radiation(orc_ctx* o, const atjup_rad_params* p);
 int      orc_chemistry(orc_ctx* o);
 int      orc_set_chemistry(orc_ctx* o, const atjup_chem_params* p);
+int      orc_mca(orc_ctx* o);
+int      orc_set_mca(orc_ctx* o, const atjup_mca_params* p);
 int      orc_precipitation(orc_ctx* o);
 int      orc_set_precipitation(orc_ctx* o, const atjup_precip_params* p);
 int      orc_download_precip_surface(orc_ctx* o, double* dst);

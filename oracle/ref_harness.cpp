@@ -22,6 +22,7 @@
 #include "TurbulenceJup.h"
 #include "RadiationJup.h"
 #include "PrecipitationJup.h"
+#include "ConvectiveAdjustmentJup.h"
 
 #include <fcntl.h>
 #include <unistd.h>
@@ -94,7 +95,7 @@ public:
             s = {
                 {"Coriolis", &M.Coriolis}, {"centrifugal", &M.centrifugal}, {"buoyancy", &M.buoyancy},
                 {"chemical_reaction", &M.chemical_reaction}, {"L_atm", &M.L_atm}, {"re", &M.re},
-                {"pr", &M.pr}, {"g", &M.g}, {"omega", &M.omega}, {"u_0", &M.u_0}, {"t_ref", &M.t_ref},
+                {"pr", &M.pr}, {"g", &M.g}, {"omega", &M.omega}, {"u_0", &M.u_0}, {"t_ref", &M.t_ref}, {"ep_h2s", &M.ep_h2s},
                 {"gam", &M.gam}, {"p_ref", &M.p_ref}, {"R_ref", &M.R_ref}, {"dt", &M.dt},
                 {"r_mix", &M.r_mix}, {"R_mix", &M.R_mix}, {"cp_mix", &M.cp_mix}, {"mue_mix", &M.mue_mix},
                 {"sc_h2o", &M.sc_h2o}, {"sc_h2s", &M.sc_h2s}, {"sc_nh3", &M.sc_nh3},
@@ -224,6 +225,22 @@ public:
             JupiterUtils::damp_wiggles(M.massflux_nh3,   &M.i_topography, true, true, true);
             JupiterUtils::damp_wiggles(M.massflux_nh4sh, &M.i_topography, true, true, true);
             JupiterUtils::damp_wiggles(M.fluxlim_nh4sh,  &M.i_topography, true, true, true);
+        }
+        else if(op == "mca"){
+            // the moist convective adjustment as the time loop builds it (cJupiterModel.cpp:1301-1320)
+            typedef MoistConvectiveAdjustmentJup::Gas Gas_;
+            typedef MoistConvectiveAdjustmentJup::Tracer Tr_;
+            std::vector<Gas_> gases = {
+                { "H2O", &M.h2o, &M.h2o_cloud, &M.h2o_ice, M.ep_h2o, M.lv_h2o, M.ls_h2o, M.t_0_h2o,
+                  M.C_h2o, M.L0_h2o, M.R_h2o, M.del_alf_h2o, M.del_bet_h2o, M.C_h2o_ice, M.L0_h2o_ice,
+                  M.del_alf_h2o_ice, M.del_bet_h2o_ice, M.t_00_h2o },
+                { "NH3", &M.nh3, &M.nh3_cloud, &M.nh3_ice, M.ep_nh3, M.lv_nh3, M.ls_nh3, M.t_0_nh3,
+                  M.C_nh3, M.L0_nh3, M.R_nh3, M.del_alf_nh3, M.del_bet_nh3, M.C_nh3_ice, M.L0_nh3_ice,
+                  M.del_alf_nh3_ice, M.del_bet_nh3_ice, M.t_00_nh3 } };
+            const double L_nh4sh_ = ChemistryJup::nh4sh_eq() ? ChemistryJup::nh4sh_formation_heat() : 0.0;
+            std::vector<Tr_> tracers = { { "H2S", &M.h2s, M.ep_h2s, false, 0.0 }, { "NH4SH", &M.nh4sh, 0.0, true, L_nh4sh_ },
+                                         { "chrom", &M.chrom, 0.0, false, 0.0 } };
+            MoistConvectiveAdjustmentJup(M, gases, tracers).run();
         }
         else if(op == "diffmassflux")   ChemistryJup(M).DiffMassFluxJup();
         else if(op == "chemmassrate")   ChemistryJup(M).ChemMassRateJup();
@@ -405,6 +422,9 @@ int ref_step(int iter, int first, int satadj, int turb){
     rc |= H::call("rk4");
     rc |= H::call("bc_all");
     if(turb) rc |= H::call("turb_wall");
+    const char* e = getenv("ATJUP_MCA");                 // mca_stride() (cJupiterModel.cpp:122-125)
+    const int mca = e ? std::max(0, atoi(e)) : 1;
+    if(mca > 0 && iter % mca == 0) rc |= H::call("mca");
     rc |= H::call("clamp_species");
     rc |= H::call("restore");
     quiet_end();

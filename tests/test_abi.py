@@ -60,6 +60,15 @@ def test_default_chemistry_params_match_mirror():
         assert getattr(p, name) == getattr(q, name), name
 
 
+def test_default_mca_params_match_mirror():
+    L = lib.load()
+    p = abi.MCAParams()
+    L.atjup_hip_default_mca_params(ctypes.byref(p))
+    q = abi.mca_params()
+    for name, _ in abi.MCAParams._fields_:
+        assert getattr(p, name) == getattr(q, name), name
+
+
 def test_default_precipitation_params_match_mirror():
     if not os.path.exists(lib.LIB_PATH):
         pytest.skip("libatjup_hip.so not built")

@@ -275,3 +275,57 @@ def test_chemistry_bit_exact_against_compiled_reference(gate, molar):
                        capture_output=True, text=True, timeout=900)
     print(r.stdout[-600:])
     assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+# MoistConvectiveAdjustment<P>::run (planet/ConvectiveAdjustment.h:403-849) on the moist state with a
+# warm lower band and a cold layer aloft: a first call that mixes nearly every column, a second on
+# the adjusted state, and (defaults) two steps with the adjustment in the loop. The reference caches
+# its knobs on the first call, so each knob set runs in a process of its own.
+SCRIPT_MCA = r'''
+import os, sys, numpy as np
+sys.path.insert(0, %(root)r)
+from atjup_amd import abi, inputs
+import oracle
+over = %(over)r
+for k, v in over.items(): os.environ[abi.MCA_ENV[k]] = str(v)
+G, Cn, K, env = abi.make_profile("full", sweeps=8, mca=1 if not over else 0)
+st = inputs.make_state("full", seed=11)
+t = st.fields["t"].copy()
+t[2:7, 60:100, 100:200] *= 1.25
+t[20:24, 100:140, 200:300] *= 0.8
+st.fields["t"] = t
+ref = oracle.Reference(env); ref.load_state(st, Cn)
+orc = oracle.Oracle(G, Cn, K); orc.load_state(st)
+if over: orc.set_mca(abi.mca_params(**over))     # the defaults case runs on the oracle's own defaults
+if abs(ref.get_scalar("ep_h2s") - abi.MCA_DEFAULTS["ep_h2s"]) > 0: print("ep_h2s", ref.get_scalar("ep_h2s")); sys.exit(1)
+ref.call("mixture_density"); orc.call("mixture_density")
+names = ["t", "h2o", "h2o_cloud", "h2o_ice", "nh3", "nh3_cloud", "nh3_ice", "h2s", "nh4sh"]
+for call in (1, 2):
+    before = orc.get("t").copy()
+    ref.call("mca"); orc.call("mca")
+    for n in names:
+        a, b = ref.get(n), orc.get(n)
+        if not np.array_equal(a, b):
+            print("MISMATCH", call, n, np.max(np.abs(a - b)), np.count_nonzero(a != b)); sys.exit(1)
+    cols = int((orc.get("t") != before).any(axis=0).sum())
+    print("call", call, "columns changed", cols)
+    if call == 1 and cols < 1000: print("TOO FEW"); sys.exit(1)
+if not over:
+    for it in range(1, 3):
+        ref.step(it, it == 1, satadj=True, turb=True); orc.call("step", it, 1 if it == 1 else 0)
+        for n in list(abi.PROG) + ["p_dyn"]:
+            a, b = ref.get(n), orc.get(n)
+            if not np.array_equal(a, b):
+                print("MISMATCH step", it, n, np.max(np.abs(a - b))); sys.exit(1)
+print("ALL-EQUAL")
+'''
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("over", [{}, {"ledoux": 0}, {"loading": 0, "relax": 0.5}, {"max_pass": 2}],
+                         ids=["defaults", "no-ledoux", "no-loading-relax", "two-sweeps"])
+def test_mca_bit_exact_against_compiled_reference(over):
+    r = subprocess.run([sys.executable, "-c", SCRIPT_MCA % {"root": ROOT, "over": over}],
+                       capture_output=True, text=True, timeout=900)
+    print(r.stdout[-600:])
+    assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
