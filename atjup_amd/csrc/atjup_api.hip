@@ -781,7 +781,8 @@ int atjup_hip_mca(atjup_ctx* c){
         // layer heights [m] as double(get_layer_height(i)) * 1e3 (ConvectiveAdjustment.h:584), then
         // the column workspace
         const size_t n = (size_t)c->im * c->jm * c->km;
-        if(int rc = alloc(c->mca_ws, (c->im + 8 * n) * sizeof(double) + n)) return rc;
+        const size_t flags = (2 * n + 7) / 8 * 8;       // touched / equilibrium bits, census bits
+        if(int rc = alloc(c->mca_ws, (c->im + 8 * n) * sizeof(double) + flags + (size_t)c->jm * c->km * 8)) return rc;
         std::vector<double> z(c->im);
         for(int i = 0; i < c->im; i++) z[i] = (double)c->lh[i] * 1.0e3;
         ATJUP_HIP_CHECK(hipMemcpy(c->mca_ws.p, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));

@@ -1786,7 +1786,8 @@ struct McaFields {
     const T *p_stat, *rho;
 };
 enum { MV_HL = 0, MV_QT = 1, MV_QR = 3, MV_TEQ = 5, MV_CEQ = 6, MCA_NV = 8 };
-enum { MF_TOUCHED = 1, MF_EQ_OK = 2, MF_CEN_U = 4 };
+enum { MF_TOUCHED = 1, MF_EQ_OK = 2 };
+enum { MC_UNSTABLE = 1, MC_VALID = 2 };                              // the per-pair test memo
 constexpr double MCA_TOL_K = 1.0e-7, MCA_TOL_T = 1.0e-10;            // ConvectiveAdjustment.h:416-418
 constexpr int MCA_N_ITER = 60;
 
@@ -1886,16 +1887,19 @@ struct McaColumn {
         return e;
     }
     // the virtual-temperature factor, composition only with LEDOUX (:636-647)
-    __device__ double tv(double TT, double v0, double v1, double c0, double c1, int i) const {
+    __device__ double tv_q(double TT, double v0, double v1, double c0, double c1, double r0, double r1) const {
         if(!K.ledoux) return TT;
         double f = 1.0;
         f += v0 * (1.0 / K.gas[0].ep - 1.0);
         if(K.loading) f -= c0;
         f += v1 * (1.0 / K.gas[1].ep - 1.0);
         if(K.loading) f -= c1;
-        if(K.ep_h2s > 0.0) f += W(MV_QR, i) * (1.0 / K.ep_h2s - 1.0);
-        if(K.loading) f -= W(MV_QR + 1, i);
+        if(K.ep_h2s > 0.0) f += r0 * (1.0 / K.ep_h2s - 1.0);
+        if(K.loading) f -= r1;
         return TT * f;
+    }
+    __device__ double tv(double TT, double v0, double v1, double c0, double c1, int i) const {
+        return tv_q(TT, v0, v1, c0, c1, W(MV_QR, i), W(MV_QR + 1, i));
     }
     // the parcel from a lifted to b against b in its own equilibrium (:651-673)
     __device__ bool unstable(int a, int b) const {
@@ -1923,10 +1927,11 @@ struct McaColumn {
     }
 };
 
+// the column's fluid range with the radial boundary planes left out (:553-575), or i1 < i0 + 1
+// when the column is skipped: too short, or a non-physical state (:578-600)
 template<typename T>
-__global__ __launch_bounds__(64) void k_mca(const McaFields<T> F, const uint8_t* __restrict__ mask,
-        const double* __restrict__ ztab, double* __restrict__ ws, uint8_t* __restrict__ wf,
-        int im, int jm, int km, int pitch, const McaConsts K){
+__global__ __launch_bounds__(64) void k_mca_columns(const McaFields<T> F, const uint8_t* __restrict__ mask,
+        const double* __restrict__ ztab, int2* __restrict__ range, int im, int jm, int km, int pitch, const McaConsts K){
     const int n2 = jm * km, col = blockIdx.x * 64 + threadIdx.x;
     if(col >= n2) return;
     const int j = col / km, k = col - j * km;
@@ -1935,23 +1940,99 @@ __global__ __launch_bounds__(64) void k_mca(const McaFields<T> F, const uint8_t*
     while(i0 < im && mask[q0 + (size_t)i0 * P] == 1) i0++;
     int i1 = i0;
     while(i1 + 1 < im && mask[q0 + (size_t)(i1 + 1) * P] != 1) i1++;
-    if(i0 == 0) i0 = 1;                                  // MCA_INTERIOR (:566-575)
+    if(i0 == 0) i0 = 1;                                  // MCA_INTERIOR
     if(i1 == im - 1) i1 = im - 2;
+    bool ok = i1 - i0 >= 1;
+    if(ok){
+        const McaColumn<T> C{ F, K, ztab, nullptr, nullptr, 0, P, q0, n2, col, i0, i1 };
+        for(int i = i0; i <= i1 && ok; i++){
+            const double Tq = C.Tk(i), p = C.pr(i), rho = C.rh(i);
+            if(!(Tq > 0.0) || !(p > 0.0) || !(rho > 0.0) || !isfinite(Tq + p + rho)) ok = false;
+        }
+        for(int i = i0; i <= i1 && ok; i++) if(!(C.wgt(i) > 0.0)) ok = false;
+    }
+    range[col] = ok ? make_int2(i0, i1) : make_int2(1, 0);
+}
+
+// per adjacent pair (i, i+1) of a kept column: h_L, q_t and the tracers of cell i (and of i1 by the
+// last pair) into the workspace, and the census of the pair (:700-706), which solves the parcel
+// from i lifted to i+1 and cell i+1's own equilibrium; the pairs of a column are independent
+// because the census reads the unmixed state. Cell i+1's equilibrium is written by its pair only.
+template<typename T>
+__global__ __launch_bounds__(64) void k_mca_census(const McaFields<T> F, const double* __restrict__ ztab,
+        const int2* __restrict__ range, double* __restrict__ ws, uint8_t* __restrict__ wf, uint8_t* __restrict__ cen,
+        int im, int jm, int km, int pitch, const McaConsts K){
+    const int n2 = jm * km, col = blockIdx.x * 64 + threadIdx.x, i = blockIdx.y + 1;
+    if(col >= n2) return;
+    const int2 r = range[col];
+    if(i < r.x || i >= r.y) return;
+    const int j = col / km, k = col - j * km;
+    const size_t P = (size_t)jm * pitch, q0 = (size_t)j * pitch + k;
+    const McaColumn<T> C{ F, K, ztab, ws, wf, (size_t)im * n2, P, q0, n2, col, r.x, r.y };
+    double ha, qta[2], qra[2], hb, qtb[2], qrb[2];
+    C.init(i, ha, qta, qra);
+    C.init(i + 1, hb, qtb, qrb);
+    C.W(MV_HL, i) = ha; C.W(MV_QT, i) = qta[0]; C.W(MV_QT + 1, i) = qta[1];
+    C.W(MV_QR, i) = qra[0]; C.W(MV_QR + 1, i) = qra[1];
+    if(i + 1 == r.y){
+        C.W(MV_HL, i + 1) = hb; C.W(MV_QT, i + 1) = qtb[0]; C.W(MV_QT + 1, i + 1) = qtb[1];
+        C.W(MV_QR, i + 1) = qrb[0]; C.W(MV_QR + 1, i + 1) = qrb[1];
+    }
+    if(i == r.x) C.FL(i) = 0;
+    uint8_t eq = 0, u = 0;
+    double Tp, pc0, pc1;
+    const double pb = C.pr(i + 1), zb = ztab[i + 1];
+    const double tra = 0.0 + 0.0 * qra[0] + K.l_nh4sh * qra[1];
+    const double trb = 0.0 + 0.0 * qrb[0] + K.l_nh4sh * qrb[1];
+    if(mca_equil(K, ha + tra, qta[0], qta[1], pb, zb, Tp, pc0, pc1)){
+        double Te, e0, e1;
+        if(mca_equil(K, hb + trb, qtb[0], qtb[1], pb, zb, Te, e0, e1)){
+            C.W(MV_TEQ, i + 1) = Te; C.W(MV_CEQ, i + 1) = e0; C.W(MV_CEQ + 1, i + 1) = e1;
+            eq = MF_EQ_OK;
+            const double pv0 = qta[0] - pc0, pv1 = qta[1] - pc1, ev0 = qtb[0] - e0, ev1 = qtb[1] - e1;
+            u = (C.tv_q(Tp, pv0, pv1, pc0, pc1, qra[0], qra[1]) - C.tv_q(Te, ev0, ev1, e0, e1, qrb[0], qrb[1])
+                 > MCA_TOL_K) ? 1 : 0;
+        }
+    }
+    C.FL(i + 1) = eq;
+    cen[(size_t)i * n2 + col] = MC_VALID | u;
+}
+
+// the sweeps over a column whose census found an unstable pair (:716-756), the relaxation and the
+// write-back; a column with none is left as it is, as in the reference. The reference's first
+// sweep takes the census result for a pair of untouched cells and tests the others again; later
+// sweeps test every pair again. A test whose two cells are unchanged since it last ran gives the
+// same answer, so each pair's result is kept until one of its cells is settled.
+template<typename T>
+__global__ __launch_bounds__(64) void k_mca(const McaFields<T> F, const double* __restrict__ ztab,
+        const int2* __restrict__ range, double* __restrict__ ws, uint8_t* __restrict__ wf,
+        uint8_t* __restrict__ cen, int im, int jm, int km, int pitch, const McaConsts K){
+    const int n2 = jm * km, col = blockIdx.x * 64 + threadIdx.x;
+    if(col >= n2) return;
+    const int2 r = range[col];
+    const int i0 = r.x, i1 = r.y;
     if(i1 - i0 < 1) return;
+    bool any = false;
+    for(int i = i0; i < i1; i++) any |= (cen[(size_t)i * n2 + col] & MC_UNSTABLE) != 0;
+    if(!any) return;
+    const int j = col / km, k = col - j * km;
+    const size_t P = (size_t)jm * pitch, q0 = (size_t)j * pitch + k;
     const McaColumn<T> C{ F, K, ztab, ws, wf, (size_t)im * n2, P, q0, n2, col, i0, i1 };
-    for(int i = i0; i <= i1; i++){                       // a non-physical column is left alone
-        const double Tq = C.Tk(i), p = C.pr(i), rho = C.rh(i);
-        if(!(Tq > 0.0) || !(p > 0.0) || !(rho > 0.0) || !isfinite(Tq + p + rho)) return;
-    }
-    for(int i = i0; i <= i1; i++){
-        if(!(C.wgt(i) > 0.0)) return;
-        double h, qt[2], qr[2];
-        C.init(i, h, qt, qr);
-        C.W(MV_HL, i) = h; C.W(MV_QT, i) = qt[0]; C.W(MV_QT + 1, i) = qt[1];
-        C.W(MV_QR, i) = qr[0]; C.W(MV_QR + 1, i) = qr[1];
-        C.FL(i) = 0;
-    }
-    for(int i = i0; i < i1; i++) if(C.unstable(i, i + 1)) C.FL(i) |= MF_CEN_U;     // the census (:700-706)
+    // a pair's parcel test depends only on its two cells, so its last result stands until one of
+    // them is settled again: the census results seed the memo, settle() clears the two pairs
+    auto uns_at = [&](int i) -> bool {
+        uint8_t& m = cen[(size_t)i * n2 + col];
+        if(m & MC_VALID) return (m & MC_UNSTABLE) != 0;
+        const bool u = C.unstable(i, i + 1);
+        m = MC_VALID | (u ? MC_UNSTABLE : 0);
+        return u;
+    };
+    auto settle_at = [&](int q) -> bool {
+        if(!C.settle(q)) return false;
+        if(q > i0) cen[(size_t)(q - 1) * n2 + col] = 0;
+        if(q < i1) cen[(size_t)q * n2 + col] = 0;
+        return true;
+    };
     const int max_pass = K.max_pass > 0 ? K.max_pass : 64;
     int passes = 0;
     bool changed = true;
@@ -1959,8 +2040,7 @@ __global__ __launch_bounds__(64) void k_mca(const McaFields<T> F, const uint8_t*
         changed = false; passes++;
         int i = i0;
         while(i < i1){
-            const bool uns = (passes == 1 && !(C.FL(i) & MF_TOUCHED) && !(C.FL(i + 1) & MF_TOUCHED))
-                ? (C.FL(i) & MF_CEN_U) != 0 : C.unstable(i, i + 1);
+            const bool uns = uns_at(i);
             if(!uns){ i++; continue; }
             int a = i, b = i + 1;
             for(;;){
@@ -1975,11 +2055,11 @@ __global__ __launch_bounds__(64) void k_mca(const McaFields<T> F, const uint8_t*
                     C.W(MV_HL, q) = sh / sw;
                     C.W(MV_QT, q) = sq0 / sw; C.W(MV_QT + 1, q) = sq1 / sw;
                     C.W(MV_QR, q) = sr0 / sw; C.W(MV_QR + 1, q) = sr1 / sw;
-                    if(!C.settle(q)) return;             // the column keeps its state
+                    if(!settle_at(q)) return;            // the column keeps its state
                 }
                 bool ext = false;
-                if(a > i0 && C.unstable(a - 1, a)){ a--; ext = true; }
-                if(b < i1 && C.unstable(b, b + 1)){ b++; ext = true; }
+                if(a > i0 && uns_at(a - 1)){ a--; ext = true; }
+                if(b < i1 && uns_at(b)){ b++; ext = true; }
                 if(!ext) break;
             }
             changed = true;
@@ -2043,14 +2123,21 @@ int do_mca(atjup_ctx* c){
     K.relax = std::min(1.0, std::max(0.0, mp.relax));
     K.ep_h2s = mp.ep_h2s; K.l_nh4sh = mp.l_nh4sh;
     K.ledoux = mp.ledoux != 0; K.loading = mp.loading != 0; K.max_pass = mp.max_pass;
-    const size_t n = (size_t)c->im * c->jm * c->km;
+    const int n2 = c->jm * c->km;
+    const size_t n = (size_t)c->im * n2;
     double* ztab = (double*)c->mca_ws.p;
     double* ws = ztab + c->im;
     uint8_t* wf = (uint8_t*)(ws + MCA_NV * n);
+    uint8_t* cen = wf + n;
+    int2* range = (int2*)(wf + (2 * n + 7) / 8 * 8);     // 8-byte aligned after the two flag planes
     hipEvent_t t0 = timer_begin(c);
-    hipLaunchKernelGGL(k_mca<T>, dim3(nblk(c->jm * c->km, 64)), dim3(64), 0, c->stream,
-        Fm, (const uint8_t*)c->mask.p, (const double*)ztab, ws, wf, c->im, c->jm, c->km, c->pitch, K);
-    timer_end(c, ATJUP_T_MCA, t0, 1);
+    hipLaunchKernelGGL(k_mca_columns<T>, dim3(nblk(n2, 64)), dim3(64), 0, c->stream,
+        Fm, (const uint8_t*)c->mask.p, (const double*)ztab, range, c->im, c->jm, c->km, c->pitch, K);
+    hipLaunchKernelGGL(k_mca_census<T>, dim3(nblk(n2, 64), c->im - 2), dim3(64), 0, c->stream,
+        Fm, (const double*)ztab, (const int2*)range, ws, wf, cen, c->im, c->jm, c->km, c->pitch, K);
+    hipLaunchKernelGGL(k_mca<T>, dim3(nblk(n2, 64)), dim3(64), 0, c->stream,
+        Fm, (const double*)ztab, (const int2*)range, ws, wf, cen, c->im, c->jm, c->km, c->pitch, K);
+    timer_end(c, ATJUP_T_MCA, t0, 3);
     return check_launch();
 }
 
