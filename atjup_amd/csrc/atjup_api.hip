@@ -777,6 +777,7 @@ int atjup_hip_mca(atjup_ctx* c){
     }
     for(int gas = 0; gas < 2; gas++)
         if(!c->gas_set[gas]){ set_error("gas constants not set (atjup_hip_set_gas)"); return ATJUP_ERR_STATE; }
+    if(c->im > 128){ set_error("the moist convective adjustment takes at most 128 levels"); return ATJUP_ERR_PROFILE; }
     if(!c->mca_ws.p){
         // layer heights [m] as double(get_layer_height(i)) * 1e3 (ConvectiveAdjustment.h:584), then
         // the column workspace

@@ -1801,7 +1801,7 @@ __device__ __forceinline__ double mca_lat(const McaGas& s, double T){ return T >
 // the equilibrium temperature of (h_L, q_t) at (p, z): cp T + g z - sum L(T) max(0, q_t - q_sat(T))
 // = h_L, bracketed by the all-vapour value and T_lo + sum L q_t / cp, by Illinois regula falsi
 // (ConvectiveAdjustment.h:453-478)
-__device__ bool mca_equil(const McaConsts& K, double hL, double qt0, double qt1, double p, double z,
+__device__ __forceinline__ bool mca_equil(const McaConsts& K, double hL, double qt0, double qt1, double p, double z,
                           double& T, double& c0, double& c1){
     const double T_lo = (hL - K.grav * z) / K.cp;
     if(!(T_lo > 0.0) || !(p > 0.0)) return false;
@@ -1999,91 +1999,146 @@ __global__ __launch_bounds__(64) void k_mca_census(const McaFields<T> F, const d
 }
 
 // the sweeps over a column whose census found an unstable pair (:716-756), the relaxation and the
-// write-back; a column with none is left as it is, as in the reference. The reference's first
-// sweep takes the census result for a pair of untouched cells and tests the others again; later
-// sweeps test every pair again. A test whose two cells are unchanged since it last ran gives the
-// same answer, so each pair's result is kept until one of its cells is settled.
+// write-back, one wavefront per column; a column with no unstable pair is left as it is, as in the
+// reference. The column's state moves to LDS. The control flow (sweeps, segments, their growth) is
+// the reference's and runs uniformly in the wavefront; each parcel test runs on one lane (the two
+// end tests of a grown segment on two lanes side by side), the segment's sums run on one lane in
+// the reference's order, and the cells of a mixed segment are settled in parallel, one per lane.
+// The reference's first sweep takes the census result for a pair of untouched cells and tests the
+// others again, later sweeps test every pair again; a test whose two cells are unchanged since it
+// last ran gives the same answer, so each pair's result is kept until one of its cells is settled.
+constexpr int MCA_MAXI = 128;                      // levels a column may have (the shells: 41, 81)
+
 template<typename T>
 __global__ __launch_bounds__(64) void k_mca(const McaFields<T> F, const double* __restrict__ ztab,
         const int2* __restrict__ range, double* __restrict__ ws, uint8_t* __restrict__ wf,
-        uint8_t* __restrict__ cen, int im, int jm, int km, int pitch, const McaConsts K){
-    const int n2 = jm * km, col = blockIdx.x * 64 + threadIdx.x;
-    if(col >= n2) return;
+        const uint8_t* __restrict__ cen, int im, int jm, int km, int pitch, const McaConsts K){
+    __shared__ double s_h[MCA_MAXI], s_q0[MCA_MAXI], s_q1[MCA_MAXI], s_r0[MCA_MAXI], s_r1[MCA_MAXI];
+    __shared__ double s_te[MCA_MAXI], s_c0[MCA_MAXI], s_c1[MCA_MAXI], s_w[MCA_MAXI], s_p[MCA_MAXI], s_z[MCA_MAXI];
+    __shared__ double s_mix[5];
+    __shared__ uint8_t s_fl[MCA_MAXI], s_memo[MCA_MAXI];
+    __shared__ int s_bc[4];
+    const int n2 = jm * km, col = blockIdx.x, lane = threadIdx.x;
     const int2 r = range[col];
     const int i0 = r.x, i1 = r.y;
     if(i1 - i0 < 1) return;
     bool any = false;
-    for(int i = i0; i < i1; i++) any |= (cen[(size_t)i * n2 + col] & MC_UNSTABLE) != 0;
-    if(!any) return;
+    for(int i = i0 + lane; i < i1; i += 64) any |= (cen[(size_t)i * n2 + col] & MC_UNSTABLE) != 0;
+    if(!__any(any)) return;
     const int j = col / km, k = col - j * km;
     const size_t P = (size_t)jm * pitch, q0 = (size_t)j * pitch + k;
     const McaColumn<T> C{ F, K, ztab, ws, wf, (size_t)im * n2, P, q0, n2, col, i0, i1 };
-    // a pair's parcel test depends only on its two cells, so its last result stands until one of
-    // them is settled again: the census results seed the memo, settle() clears the two pairs
-    auto uns_at = [&](int i) -> bool {
-        uint8_t& m = cen[(size_t)i * n2 + col];
+    for(int i = i0 + lane; i <= i1; i += 64){
+        s_h[i] = C.W(MV_HL, i); s_q0[i] = C.W(MV_QT, i); s_q1[i] = C.W(MV_QT + 1, i);
+        s_r0[i] = C.W(MV_QR, i); s_r1[i] = C.W(MV_QR + 1, i);
+        s_te[i] = C.W(MV_TEQ, i); s_c0[i] = C.W(MV_CEQ, i); s_c1[i] = C.W(MV_CEQ + 1, i);
+        s_w[i] = C.wgt(i); s_p[i] = C.pr(i); s_z[i] = ztab[i];
+        s_fl[i] = C.FL(i);
+        s_memo[i] = (i < i1) ? cen[(size_t)i * n2 + col] : 0;
+    }
+    __syncthreads();
+    auto trE = [&](int i) __attribute__((always_inline)) { double e = 0.0; e += 0.0 * s_r0[i]; e += K.l_nh4sh * s_r1[i]; return e; };
+    // the parcel from a lifted to b against b in its own equilibrium (:651-673), on one lane
+    auto unstable = [&](int a, int b) __attribute__((always_inline)) -> bool {
+        double Tp, pc0, pc1;
+        if(!mca_equil(K, s_h[a] + trE(a), s_q0[a], s_q1[a], s_p[b], s_z[b], Tp, pc0, pc1)) return false;
+        if(!(s_fl[b] & MF_EQ_OK)){
+            double Te, e0, e1;
+            if(!mca_equil(K, s_h[b] + trE(b), s_q0[b], s_q1[b], s_p[b], s_z[b], Te, e0, e1)) return false;
+            s_te[b] = Te; s_c0[b] = e0; s_c1[b] = e1;
+            s_fl[b] |= MF_EQ_OK;
+        }
+        const double ec0 = s_c0[b], ec1 = s_c1[b];
+        const double pv0 = s_q0[a] - pc0, pv1 = s_q1[a] - pc1, ev0 = s_q0[b] - ec0, ev1 = s_q1[b] - ec1;
+        return C.tv_q(Tp, pv0, pv1, pc0, pc1, s_r0[a], s_r1[a]) - C.tv_q(s_te[b], ev0, ev1, ec0, ec1, s_r0[b], s_r1[b])
+               > MCA_TOL_K;
+    };
+    auto test = [&](int i) __attribute__((always_inline)) -> bool {
+        const uint8_t m = s_memo[i];
         if(m & MC_VALID) return (m & MC_UNSTABLE) != 0;
-        const bool u = C.unstable(i, i + 1);
-        m = MC_VALID | (u ? MC_UNSTABLE : 0);
+        const bool u = unstable(i, i + 1);
+        s_memo[i] = MC_VALID | (u ? MC_UNSTABLE : 0);
         return u;
     };
-    auto settle_at = [&](int q) -> bool {
-        if(!C.settle(q)) return false;
-        if(q > i0) cen[(size_t)(q - 1) * n2 + col] = 0;
-        if(q < i1) cen[(size_t)q * n2 + col] = 0;
+    // a cell to its equilibrium (:674-690); returns false when the solve has no positive temperature
+    auto settle = [&](int q) __attribute__((always_inline)) -> bool {
+        double TT, c0, c1;
+        if(!mca_equil(K, s_h[q] + trE(q), s_q0[q], s_q1[q], s_p[q], s_z[q], TT, c0, c1)) return false;
+        s_te[q] = TT; s_c0[q] = c0; s_c1[q] = c1;
+        s_fl[q] |= MF_TOUCHED | MF_EQ_OK;
         return true;
     };
     const int max_pass = K.max_pass > 0 ? K.max_pass : 64;
     int passes = 0;
     bool changed = true;
-    while(changed && passes < max_pass){                 // the sweeps (:716-756)
+    while(changed && passes < max_pass){
         changed = false; passes++;
         int i = i0;
         while(i < i1){
-            const bool uns = uns_at(i);
-            if(!uns){ i++; continue; }
+            if(!(s_memo[i] & MC_VALID)){
+                if(lane == 0) (void)test(i);
+                __syncthreads();
+            }
+            if(!(s_memo[i] & MC_UNSTABLE)){ i++; continue; }
             int a = i, b = i + 1;
             for(;;){
-                double sw = 0.0, sh = 0.0, sq0 = 0.0, sq1 = 0.0, sr0 = 0.0, sr1 = 0.0;
-                for(int q = a; q <= b; q++){
-                    const double w = C.wgt(q);
-                    sw += w; sh += w * C.W(MV_HL, q);
-                    sq0 += w * C.W(MV_QT, q); sq1 += w * C.W(MV_QT + 1, q);
-                    sr0 += w * C.W(MV_QR, q); sr1 += w * C.W(MV_QR + 1, q);
+                if(lane == 0){                              // the sums in the reference's order
+                    double sw = 0.0, sh = 0.0, sq0 = 0.0, sq1 = 0.0, sr0 = 0.0, sr1 = 0.0;
+                    for(int q = a; q <= b; q++){
+                        const double w = s_w[q];
+                        sw += w; sh += w * s_h[q];
+                        sq0 += w * s_q0[q]; sq1 += w * s_q1[q];
+                        sr0 += w * s_r0[q]; sr1 += w * s_r1[q];
+                    }
+                    s_mix[0] = sh / sw; s_mix[1] = sq0 / sw; s_mix[2] = sq1 / sw;
+                    s_mix[3] = sr0 / sw; s_mix[4] = sr1 / sw;
+                    s_bc[1] = 0;
                 }
-                for(int q = a; q <= b; q++){
-                    C.W(MV_HL, q) = sh / sw;
-                    C.W(MV_QT, q) = sq0 / sw; C.W(MV_QT + 1, q) = sq1 / sw;
-                    C.W(MV_QR, q) = sr0 / sw; C.W(MV_QR + 1, q) = sr1 / sw;
-                    if(!settle_at(q)) return;            // the column keeps its state
+                __syncthreads();
+                for(int q = a + lane; q <= b; q += 64){
+                    s_h[q] = s_mix[0]; s_q0[q] = s_mix[1]; s_q1[q] = s_mix[2];
+                    s_r0[q] = s_mix[3]; s_r1[q] = s_mix[4];
+                    if(!settle(q)) s_bc[1] = 1;
+                    if(q > i0) s_memo[q - 1] = 0;
+                    if(q < i1) s_memo[q] = 0;
                 }
+                __syncthreads();
+                if(s_bc[1]) return;                         // the column keeps its state
+                if(lane == 0) s_bc[2] = (a > i0 && test(a - 1)) ? 1 : 0;
+                if(lane == 1) s_bc[3] = (b < i1 && test(b)) ? 1 : 0;
+                __syncthreads();
                 bool ext = false;
-                if(a > i0 && uns_at(a - 1)){ a--; ext = true; }
-                if(b < i1 && uns_at(b)){ b++; ext = true; }
+                if(s_bc[2]){ a--; ext = true; }
+                if(s_bc[3]){ b++; ext = true; }
+                __syncthreads();
                 if(!ext) break;
             }
             changed = true;
             i = b;
         }
     }
-    if(K.relax < 1.0){                                   // MCA_RELAX (:800-811)
+    if(K.relax < 1.0){                                       // MCA_RELAX (:800-811)
         const double relax = dmax(0.0, K.relax);
-        for(int q = i0; q <= i1; q++){
-            if(!(C.FL(q) & MF_TOUCHED)) continue;
+        if(lane == 0) s_bc[1] = 0;
+        __syncthreads();
+        for(int q = i0 + lane; q <= i1; q += 64){
+            if(!(s_fl[q] & MF_TOUCHED)) continue;
             double h0, qt0[2], qr0[2];
             C.init(q, h0, qt0, qr0);
-            C.W(MV_HL, q) = h0 + relax * (C.W(MV_HL, q) - h0);
-            C.W(MV_QT, q) = qt0[0] + relax * (C.W(MV_QT, q) - qt0[0]);
-            C.W(MV_QT + 1, q) = qt0[1] + relax * (C.W(MV_QT + 1, q) - qt0[1]);
-            C.W(MV_QR, q) = qr0[0] + relax * (C.W(MV_QR, q) - qr0[0]);
-            C.W(MV_QR + 1, q) = qr0[1] + relax * (C.W(MV_QR + 1, q) - qr0[1]);
-            if(!C.settle(q)) return;
+            s_h[q] = h0 + relax * (s_h[q] - h0);
+            s_q0[q] = qt0[0] + relax * (s_q0[q] - qt0[0]);
+            s_q1[q] = qt0[1] + relax * (s_q1[q] - qt0[1]);
+            s_r0[q] = qr0[0] + relax * (s_r0[q] - qr0[0]);
+            s_r1[q] = qr0[1] + relax * (s_r1[q] - qr0[1]);
+            if(!settle(q)) s_bc[1] = 1;
         }
+        __syncthreads();
+        if(s_bc[1]) return;
     }
-    for(int q = i0; q <= i1; q++){                       // the write-back (:822-833)
-        if(!(C.FL(q) & MF_TOUCHED)) continue;
+    for(int q = i0 + lane; q <= i1; q += 64){                // the write-back (:822-833)
+        if(!(s_fl[q] & MF_TOUCHED)) continue;
         const size_t c = q0 + (size_t)q * P;
-        const double rho = C.rh(q), TT = C.W(MV_TEQ, q);
+        const double rho = C.rh(q), TT = s_te[q];
         const double tq = TT / K.t_ref;
         if(F.lo_t){
             const float h = (float)tq;
@@ -2092,15 +2147,16 @@ __global__ __launch_bounds__(64) void k_mca(const McaFields<T> F, const double* 
         } else {
             F.t[c] = (T)tq;
         }
+        const double ce[2] = { s_c0[q], s_c1[q] }, qt[2] = { s_q0[q], s_q1[q] };
         for(int g = 0; g < 2; g++){
-            const double ce = C.W(MV_CEQ + g, q), qv = C.W(MV_QT + g, q) - ce;
+            const double qv = qt[g] - ce[g];
             const bool liquid = TT >= K.gas[g].t_0;
             F.v[g][c] = (T)(qv * rho);
-            F.cl[g][c] = (T)((liquid ? ce : 0.0) * rho);
-            F.ice[g][c] = (T)((liquid ? 0.0 : ce) * rho);
+            F.cl[g][c] = (T)((liquid ? ce[g] : 0.0) * rho);
+            F.ice[g][c] = (T)((liquid ? 0.0 : ce[g]) * rho);
         }
-        F.tr[0][c] = (T)(C.W(MV_QR, q) * rho);
-        F.tr[1][c] = (T)(C.W(MV_QR + 1, q) * rho);
+        F.tr[0][c] = (T)(s_r0[q] * rho);
+        F.tr[1][c] = (T)(s_r1[q] * rho);
     }
 }
 
@@ -2135,8 +2191,8 @@ int do_mca(atjup_ctx* c){
         Fm, (const uint8_t*)c->mask.p, (const double*)ztab, range, c->im, c->jm, c->km, c->pitch, K);
     hipLaunchKernelGGL(k_mca_census<T>, dim3(nblk(n2, 64), c->im - 2), dim3(64), 0, c->stream,
         Fm, (const double*)ztab, (const int2*)range, ws, wf, cen, c->im, c->jm, c->km, c->pitch, K);
-    hipLaunchKernelGGL(k_mca<T>, dim3(nblk(n2, 64)), dim3(64), 0, c->stream,
-        Fm, (const double*)ztab, (const int2*)range, ws, wf, cen, c->im, c->jm, c->km, c->pitch, K);
+    hipLaunchKernelGGL(k_mca<T>, dim3(n2), dim3(64), 0, c->stream,
+        Fm, (const double*)ztab, (const int2*)range, ws, wf, (const uint8_t*)cen, c->im, c->jm, c->km, c->pitch, K);
     timer_end(c, ATJUP_T_MCA, t0, 3);
     return check_launch();
 }
