@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libatjup_hip.so")
-SOURCES = ["atjup_kernels.hip", "atjup_physics.hip", "atjup_adjoint.hip", "atjup_api.hip"]
+SOURCES = ["atjup_kernels.hip", "atjup_physics.hip", "atjup_adjoint.hip", "atjup_report.hip", "atjup_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: the FP64 kernels keep the reference's expression trees (no FMA contraction),

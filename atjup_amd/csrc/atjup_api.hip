@@ -394,6 +394,7 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     if(!rc){
         atjup_hip_default_chem_params(&c->chemp);
         atjup_hip_default_mca_params(&c->mcap);
+        rc = alloc(c->report, report_bytes(c->im, c->jm, c->km));
         rc = alloc(c->chem_w, 3 * c->n3 * sizeof(double));
     }
     if(rc){ atjup_hip_ctx_destroy(c); return nullptr; }
@@ -411,7 +412,7 @@ void atjup_hip_ctx_destroy(atjup_ctx* c){
     for(atjup::Buf* b : { &c->lo[0], &c->lo[1], &c->lo[2], &c->lo[3], &c->colr }) release(*b);
     for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
-                          &c->amb_rows, &c->nonfinite, &c->rad_tab, &c->rad_tmp, &c->precip_srf, &c->chem_w, &c->mca_ws })
+                          &c->amb_rows, &c->nonfinite, &c->rad_tab, &c->rad_tmp, &c->precip_srf, &c->chem_w, &c->mca_ws, &c->report })
         release(*b);
     for(auto& v : c->spans) for(auto& sp : v){ (void)hipEventDestroy(sp.a); (void)hipEventDestroy(sp.b); }
     for(auto e : c->event_pool) (void)hipEventDestroy(e);

@@ -112,6 +112,7 @@ struct atjup_ctx {
     atjup::Buf precip_srf;          // double [5][jm][km]: the surface precipitation maps
     atjup_chem_params chemp = {};   // the chemistry's constants and knobs (atjup_hip_set_chemistry)
     atjup::Buf chem_w;              // double [3][n3]: the reaction rates w_nh3, w_h2s, w_nh4sh
+    atjup::Buf report;              // the reporting reductions' partial results and slice staging
     atjup_mca_params mcap = {};     // the moist convective adjustment's knobs (atjup_hip_set_mca)
     atjup::Buf mca_ws;              // its layer heights [im] (m), then the column workspace
                                     // double [8][im][jm * km] and the flags uint8 [im][jm * km]
@@ -166,6 +167,7 @@ int launch_radiation(atjup_ctx* c);
 int launch_precipitation(atjup_ctx* c);
 int launch_chem_kernel(atjup_ctx* c, int which);      // 0: DiffMassFluxJup, 1: ChemMassRateJup
 int launch_mca(atjup_ctx* c);
+size_t report_bytes(int im, int jm, int km);          // the size of atjup_ctx::report
 int build_rad_table(atjup_ctx* c);
 int launch_flux_limiter(atjup_ctx* c);
 int launch_commit_live(atjup_ctx* c);

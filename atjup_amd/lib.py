@@ -74,6 +74,10 @@ def load():
     L.atjup_hip_set_chemistry.argtypes = [C.c_void_p, C.POINTER(abi.ChemParams)]
     L.atjup_hip_default_chem_params.argtypes = [C.POINTER(abi.ChemParams)]
     L.atjup_hip_set_mca.argtypes = [C.c_void_p, C.POINTER(abi.MCAParams)]
+    for n in ("atjup_hip_search_minmax_3d", "atjup_hip_search_minmax_2d"):
+        getattr(L, n).argtypes = [C.c_void_p, C.c_int, _D, C.POINTER(C.c_int)]
+    L.atjup_hip_continuity_residual.argtypes = [C.c_void_p, _D, C.POINTER(C.c_int)]
+    L.atjup_hip_download_slice.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _D]
     L.atjup_hip_default_mca_params.argtypes = [C.POINTER(abi.MCAParams)]
     L.atjup_hip_save_state.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
     L.atjup_hip_load_state.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -225,6 +229,31 @@ class Context:
 
     def mca(self):
         self._chk(self.L.atjup_hip_mca(self.h), "mca")
+
+    def search_minmax_3d(self, name):
+        """(max, (i, j, k), min, (i, j, k)) of a field, as Reporting::searchMinMax_3D finds them."""
+        out, loc = (C.c_double * 2)(), (C.c_int * 6)()
+        self._chk(self.L.atjup_hip_search_minmax_3d(self.h, abi.FIELD[name], out, loc), "search_minmax_3d")
+        return out[0], tuple(loc[0:3]), out[1], tuple(loc[3:6])
+
+    def search_minmax_2d(self, map_id):
+        """(max, (j, k), min, (j, k)) of surface map 0-4 or vel_star (5), as searchMinMax_2D."""
+        out, loc = (C.c_double * 2)(), (C.c_int * 4)()
+        self._chk(self.L.atjup_hip_search_minmax_2d(self.h, int(map_id), out, loc), "search_minmax_2d")
+        return out[0], tuple(loc[0:2]), out[1], tuple(loc[2:4])
+
+    def continuity_residual(self):
+        """(max |residual|, (i, j, k)) of steadyQuery's continuity check."""
+        v, loc = C.c_double(), (C.c_int * 3)()
+        self._chk(self.L.atjup_hip_continuity_residual(self.h, C.byref(v), loc), "continuity_residual")
+        return v.value, tuple(loc)
+
+    def slice(self, name, axis, index):
+        im, jm, km = self.shape
+        shape = [(jm, km), (im, km), (im, jm)][axis]
+        a = np.empty(shape, dtype=np.float64)
+        self._chk(self.L.atjup_hip_download_slice(self.h, abi.FIELD[name], axis, index, a.ctypes.data_as(_D)), "slice")
+        return a
 
     def save_restart(self, path, iteration, carried=True):
         """cJupiterModel::save_state: the reference's JUP2 (or, carried=False, JUP1) file."""

@@ -271,6 +271,24 @@ def mca_leg(ctx, g, calls=5):
             "launches_per_call": launches / calls}
 
 
+def reporting_leg(ctx, g, calls=20):
+    """searchMinMax_3D of t and steadyQuery's continuity residual on the device: wall time per call
+    including the few-byte result download and the stream synchronisation."""
+    ctx.search_minmax_3d("t"); ctx.continuity_residual()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        ctx.search_minmax_3d("t")
+    mm = (time.perf_counter() - t0) / calls
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        ctx.continuity_residual()
+    cr = (time.perf_counter() - t0) / calls
+    field = g.im * g.jm * g.km * 8
+    return {"workload": f"searchMinMax_3D of t and the continuity residual of u, v, w, {g.im}x{g.jm}x{g.km}",
+            "minmax_ms_per_call": mm * 1e3, "minmax_GBs": field / mm / 1e9,
+            "residual_ms_per_call": cr * 1e3, "residual_GBs": 3 * field / cr / 1e9}
+
+
 def restart_leg(ctx, g):
     """save_state then load_state of the device-resident state (JUP2, 55 arrays and vel_star), in a
     temporary directory on the host's disk: the D2H/H2D transfers and the file I/O together."""
@@ -413,6 +431,7 @@ def main():
     precip = precipitation_leg(ctx, g) if args.config == "full" else None
     chem = chemistry_leg(ctx, g) if args.config == "full" else None
     mca = mca_leg(ctx, g) if args.config == "full" else None
+    report = reporting_leg(ctx, g)
     rst = None
     if not args.no_restart:
         try:
@@ -486,6 +505,7 @@ def main():
         "precipitation": precip,
         "chemistry": chem,
         "mca": mca,
+        "reporting": report,
         "restart": rst,
     }
     print(json.dumps(out))

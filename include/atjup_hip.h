@@ -264,6 +264,28 @@ void atjup_hip_default_chem_params(atjup_chem_params* params);
  * get t, the vapour, cloud and ice of H2O and NH3, H2S and NH4SH. A column with a non-physical
  * state, or whose equilibrium solve fails, keeps its state. The printed census is not formed. */
 int atjup_hip_mca(atjup_ctx* ctx);
+/* Reporting<cJupiterModel>::searchMinMax_3D (planet/Reporting.h:261-297) on the device: the
+ * largest and smallest value of a field and their cells, the first cell of the reference's j, k, i
+ * loops on ties (a NaN is never taken, unless it is the loops' starting cell (0, 0, 0), which then
+ * stands for both). out[0] max, out[1] min (the raw values: the caller applies the printout's
+ * coefficient and conversion); loc = {i, j, k} of the max, then of the min. FP32-refined fields
+ * t, u, v, w are read as hi + lo. */
+int atjup_hip_search_minmax_3d(atjup_ctx* ctx, int field, double* out, int* loc);
+/* searchMinMax_2D (planet/Reporting.h:299-330) over map 0-4 (the surface precipitation maps in the
+ * order of atjup_hip_download_precip_surface) or 5 (vel_star): the starting element (0, 0) and the
+ * interior j in [1, jm-2], k in [1, km-2]; loc = {j, k} of the max, then of the min. */
+int atjup_hip_search_minmax_2d(atjup_ctx* ctx, int map, double* out, int* loc);
+/* The continuity residual of steadyQuery (planet/Reporting.h:372-398): the largest |du/dr + 2u/r +
+ * dv/dtheta / r + cos(theta) v / (r sin(theta)) + dw/dphi / (r sin(theta))| over the interior and its
+ * cell, the last cell of the i, j, k loops on ties (ATJUP_COSTHE_ABS 0). With no finite residual the
+ * value is 0 and loc is {-1, -1, -1} (the reference keeps its previous cell). The per-field changes
+ * of steadyQuery need the n-level copies, which this path commits in place (restoreVar is the
+ * identity here), so they are not formed. */
+int atjup_hip_continuity_residual(atjup_ctx* ctx, double* value, int* loc);
+/* One plane of a field for the writers (planet/ParaView_Jup.cpp): axis 0 the level i = index
+ * ([jm][km]), axis 1 the latitude j = index ([im][km]), axis 2 the longitude k = index ([im][jm]),
+ * as FP64, gathered on the device so that only the plane crosses PCIe. */
+int atjup_hip_download_slice(atjup_ctx* ctx, int field, int axis, int index, double* dst);
 int atjup_hip_set_mca(atjup_ctx* ctx, const atjup_mca_params* params);
 void atjup_hip_default_mca_params(atjup_mca_params* params);
 /* cJupiterModel::save_state / load_state (planet/FileIO_Jup.cpp:584-684): the binary restart file,

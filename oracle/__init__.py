@@ -207,6 +207,13 @@ class Reference:
         if self.L.ref_set_scalar(name.encode(), float(v)) != 0:
             raise KeyError(name)
 
+    def get_res(self):
+        """The continuity residual's cell (i_res, j_res, k_res) the last steady_query left."""
+        a = (C.c_int * 3)()
+        self.L.ref_get_res.argtypes = [C.POINTER(C.c_int)]
+        self.L.ref_get_res(a)
+        return tuple(a)
+
     def get_scalar(self, name):
         v = C.c_double()
         if self.L.ref_get_scalar(name.encode(), C.byref(v)) != 0:

@@ -23,6 +23,7 @@
 #include "RadiationJup.h"
 #include "PrecipitationJup.h"
 #include "ConvectiveAdjustmentJup.h"
+#include "Reporting.h"
 
 #include <fcntl.h>
 #include <unistd.h>
@@ -87,6 +88,8 @@ public:
             {"precip_srf_total", &M.precip_srf_total},
         };
     }
+
+    static void res(int* ijk){ ijk[0] = m->i_res; ijk[1] = m->j_res; ijk[2] = m->k_res; }
 
     static double* scalar(const std::string& n){
         cJupiterModel& M = *m;
@@ -242,6 +245,7 @@ public:
                                          { "chrom", &M.chrom, 0.0, false, 0.0 } };
             MoistConvectiveAdjustmentJup(M, gases, tracers).run();
         }
+        else if(op == "steady_query")   Reporting<cJupiterModel>(M).steadyQuery();   // prints; i/j/k_res kept
         else if(op == "diffmassflux")   ChemistryJup(M).DiffMassFluxJup();
         else if(op == "chemmassrate")   ChemistryJup(M).ChemMassRateJup();
         else if(op == "satadj_h2o")
@@ -430,6 +434,9 @@ int ref_step(int iter, int first, int satadj, int turb){
     quiet_end();
     return rc;
 }
+
+// the continuity residual's cell as steadyQuery leaves it (cJupiterModel.h:121)
+void ref_get_res(int* ijk){ H::res(ijk); }
 
 void ref_set_vel_star(const double* src){
     int im, jm, km; H::dims(&im, &jm, &km);
