@@ -357,8 +357,8 @@ int atjup_hip_read_timers(atjup_ctx* ctx, double* out);
  * density, RK4 with its right-hand side, the hydrostatic integral and level means, and the
  * red-black pressure solve, branches frozen at the forward's choice. Adjoint fields are
  * addressed by the forward field ids (t, u, v, w, p_dyn, p_hydro, rho_mix, aux_*). The reference
- * has no adjoint; the survey of its path lists these entry points for the gradient the
- * benchmark asks for (zonal-wind misfit at the top level with respect to the initial state). */
+ * has no adjoint: these entry points have no counterpart there and serve the gradient of a
+ * zonal-wind misfit at the top level with respect to the initial state. */
 #define ATJUP_ADJ_OP_BC            1   /* bc_after_step^T                                      */
 #define ATJUP_ADJ_OP_SHAPIRO_PDYN  2   /* shapiro(p_dyn, 2, 1, 1, 7)^T                         */
 #define ATJUP_ADJ_OP_MIXTURE       3   /* mixture_density^T                                    */
