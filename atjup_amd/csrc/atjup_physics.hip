@@ -1607,25 +1607,30 @@ __global__ __launch_bounds__(64) void k_diffmassflux(const ChemFields<T> F, cons
     const double rmsinthe = rm * sinthe;
     const bool sim = fl & FL_IM, sip = fl & FL_IP, sjm = fl & FL_JM, sjp = fl & FL_JP;
     const bool skm = fl & FL_KM, skp = fl & FL_KP;
-    auto ld = [&](const T* X, size_t qq){
-        double v = (double)X[qq];
-        if(X == F.t && F.lo_t) v += (double)F.lo_t[qq];
-        return v;
+    // the seven-point values of the four fields, loaded once before any store
+    struct Pts { double c, rp, rm, tp, tm, pw, pe, k1, k2; };
+    auto pts = [&](const T* X, const float* L){
+        auto ld = [&](size_t qq){ double v = (double)X[qq]; if(L) v += (double)L[qq]; return v; };
+        Pts p;
+        p.c = ld(q); p.rp = ld(q + P); p.rm = ld(q - P); p.tp = ld(q + R); p.tm = ld(q - R);
+        p.pw = ld(row + kw); p.pe = ld(row + ke);
+        p.k1 = (k == 0) ? ld(row + 1) : 0.0; p.k2 = (k == 0) ? ld(row + 2) : 0.0;
+        return p;
     };
+    const Pts Pt = pts(F.t, F.lo_t), Ph = pts(F.h2s, nullptr), Pn = pts(F.nh3, nullptr), P4 = pts(F.nh4sh, nullptr);
     // first derivatives of a field: r, theta central (0 across two solid neighbours); phi central,
     // one-sided at k = 0 (derivative_1_order_boundary, ChemistryJup.h:729-750)
-    auto d3 = [&](const T* X, double& dr, double& dth, double& dph){
-        const double c = ld(X, q);
-        dr = chem_d1(ld(X, q - P), ld(X, q + P), sim, sip, C.dr_2);
-        dth = chem_d1(ld(X, q - R), ld(X, q + R), sjm, sjp, C.dthe_2);
-        if(k == 0) dph = (-3.0 * c + 4.0 * ld(X, row + 1) - ld(X, row + 2)) / C.dphi_2;
-        else dph = chem_d1(ld(X, row + kw), ld(X, row + ke), skm, skp, C.dphi_2);
+    auto d3 = [&](const Pts& X, double& dr, double& dth, double& dph){
+        dr = chem_d1(X.rm, X.rp, sim, sip, C.dr_2);
+        dth = chem_d1(X.tm, X.tp, sjm, sjp, C.dthe_2);
+        if(k == 0) dph = (-3.0 * X.c + 4.0 * X.k1 - X.k2) / C.dphi_2;
+        else dph = chem_d1(X.pw, X.pe, skm, skp, C.dphi_2);
     };
     double dtdr, dtdthe, dtdphi, dnh3dr, dnh3dthe, dnh3dphi, dh2sdr, dh2sdthe, dh2sdphi, dnhdr, dnhdthe, dnhdphi;
-    d3(F.t, dtdr, dtdthe, dtdphi);
-    d3(F.h2s, dh2sdr, dh2sdthe, dh2sdphi);
-    d3(F.nh3, dnh3dr, dnh3dthe, dnh3dphi);
-    d3(F.nh4sh, dnhdr, dnhdthe, dnhdphi);
+    d3(Pt, dtdr, dtdthe, dtdphi);
+    d3(Ph, dh2sdr, dh2sdthe, dh2sdphi);
+    d3(Pn, dnh3dr, dnh3dthe, dnh3dphi);
+    d3(P4, dnhdr, dnhdthe, dnhdphi);
     const double dt_div = dtdr + fabs(dtdthe) / rm + dtdphi / rmsinthe;
     const double rf = (double)F.rho[q];
     const double rho_c = (rf > 0.0 && isfinite(rf)) ? rf : C.r_mix;
@@ -1637,7 +1642,7 @@ __global__ __launch_bounds__(64) void k_diffmassflux(const ChemFields<T> F, cons
     const double dh2s_div = dh2sdr + fabs(dh2sdthe) / rm + dh2sdphi / rmsinthe;
     const double dnh4sh_div = dnhdr + fabs(dnhdthe) / rm + dnhdphi / rmsinthe;
     const double L_nh3_c = C.L_nh3 * sc, L_h2s_c = C.L_h2s * sc, L_nh4sh_c = C.L_nh4sh * sc;
-    const double nh3 = (double)F.nh3[q], h2s = (double)F.h2s[q], nh4sh = (double)F.nh4sh[q];
+    const double nh3 = Pn.c, h2s = Ph.c, nh4sh = P4.c;
     const double cM_nh3 = L_nh3_c * nh3 / rho_c;
     const double cM_h2s = L_h2s_c * h2s / rho_c;
     const double cM_nh4sh = L_nh4sh_c * nh4sh / rho_c;
@@ -1649,21 +1654,22 @@ __global__ __launch_bounds__(64) void k_diffmassflux(const ChemFields<T> F, cons
     const double j_nh4sh = C.m_nh4sh / M * (L_nh4sh_c * dnh4sh_div - M / C.m_nh3 * cM_nh4sh * dnh3_div
         - M / C.m_h2s * cM_nh4sh * dh2s_div - M / C.m_nh4sh * cM_nh4sh * dnh4sh_div) - jT_nh4sh;
     // laplacian_spherical (ChemistryJup.h:752-776)
-    auto lap = [&](const T* X){
-        const double c = ld(X, q);
+    auto lap = [&](const Pts& X){
+        const double c = X.c;
         const double rmsin = rm * dmax(sin_t, CHEM_FLOOR);
-        const double d2r = (ld(X, q + P) - 2.0 * c + ld(X, q - P)) / C.dr2 * 1.0;
-        const double d2t = (ld(X, q + R) - 2.0 * c + ld(X, q - R)) / C.dthe2;
-        const double d2p = (ld(X, row + ke) - 2.0 * c + ld(X, row + kw)) / C.dphi2;
-        const double d1r = (ld(X, q + P) - ld(X, q - P)) / C.dr_2 * 1.0;
-        const double d1t = (ld(X, q + R) - ld(X, q - R)) / C.dthe_2;
+        const double d2r = (X.rp - 2.0 * c + X.rm) / C.dr2 * 1.0;
+        const double d2t = (X.tp - 2.0 * c + X.tm) / C.dthe2;
+        const double d2p = (X.pe - 2.0 * c + X.pw) / C.dphi2;
+        const double d1r = (X.rp - X.rm) / C.dr_2 * 1.0;
+        const double d1t = (X.tp - X.tm) / C.dthe_2;
         return d2r + 2.0 / rm * d1r + d2t / (rm * rm) + cos_t / (rm * rmsin) * d1t + d2p / (rmsin * rmsin);
     };
-    F.dfl_h2s[q] = (T)(C.D_h2s * lap(F.h2s));
-    F.dfl_nh3[q] = (T)(C.D_nh3 * lap(F.nh3));
-    F.dfl_nh4sh[q] = (T)(C.D_nh4sh * lap(F.nh4sh));
+    const double lap_h2s = lap(Ph), lap_nh3 = lap(Pn), lap_nh4sh = lap(P4);
     const double wt = C.enthalpy ? F.w_nh4sh[q] * C.dh_nh4sh
         : tq * (F.w_nh3[q] * C.k_nh3 + F.w_h2s[q] * C.k_h2s + F.w_nh4sh[q] * C.k_nh4sh);
+    F.dfl_h2s[q] = (T)(C.D_h2s * lap_h2s);
+    F.dfl_nh3[q] = (T)(C.D_nh3 * lap_nh3);
+    F.dfl_nh4sh[q] = (T)(C.D_nh4sh * lap_nh4sh);
     F.tmf[q] = (T)((j_nh3 * C.cp_nh3 + j_h2s * C.cp_h2s + j_nh4sh * C.cp_nh4sh) * dt_div + wt);
 }
 
