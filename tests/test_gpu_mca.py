@@ -101,3 +101,25 @@ def test_mca_fp32_refined_form():
         assert e <= max(2.0 * floor, 1.0e-6), n
     for c in (g64, g32, gr, o1, o2, o3):
         c.close()
+
+
+def test_mca_deep_columns_against_oracle():
+    """81 levels with a strongly unstable lower part: segments longer than a wavefront's 64 lanes,
+    so a lane settles two cells of a segment."""
+    from tests.helpers import small_state
+    g = inputs.Grid(im=81, jm=33, km=49, the_degree=180.0 / 32, phi_degree=360.0 / 48)
+    st = small_state("full", seed=5, g=g)
+    t = st.fields["t"].copy()
+    t[1:60] *= np.linspace(1.6, 1.0, 59)[:, None, None]
+    st.fields["t"] = t
+    gpu, orc, _ = make_pair("full", sweeps=20, state=st)
+    gpu.mixture_density(); orc.call("mixture_density")
+    before = orc.get("t").copy()
+    gpu.mca(); orc.call("mca")
+    cells = (orc.get("t") != before).sum(axis=0)
+    assert (cells > 64).sum() > 100
+    worst = {n: rel_err(gpu.get(n), orc.get(n)) for n in FIELDS}
+    print("mca deep columns", int(cells.max()), "cells at most;", {n: f"{e:.1e}" for n, e in worst.items()})
+    for n, e in worst.items():
+        assert e <= TOL, (n, e)
+    gpu.close(); orc.close()
