@@ -40,6 +40,11 @@ class MisfitRunner:
         self.nsteps, self.every = nsteps, every
         _, theta, _ = _coords(ctx)
         self.cos_lat = np.abs(np.sin(theta))          # theta is the colatitude
+        # The checkpoints and the adjoint seed of a forward live in the context until the next
+        # forward overwrites them; each forward gets a new generation number so that a backward
+        # belonging to an earlier forward is refused instead of returning another one's gradient.
+        self.generation = 0
+        self._lam = None
         ctx.adj_init()
 
     def loss_final(self):
@@ -56,20 +61,33 @@ class MisfitRunner:
 
     def forward(self, w0):
         """L(w0) through the checkpointed forward sweep, ready for backward()."""
+        self.generation += 1
+        self._lam = None
         self.load(w0)
         self.ctx.adj_forward(self.nsteps, self.every, 1)
         loss, self._lam = self.loss_final()
         return loss
 
     def evaluate(self, w0):
-        """L(w0) by plain forward steps (no checkpoints), for difference quotients."""
+        """L(w0) by plain forward steps (no checkpoints), for difference quotients. The context's
+        state then no longer belongs to the last forward(): its backward is refused."""
+        self.generation += 1
+        self._lam = None
         self.load(w0)
         for n in range(self.nsteps):
             self.ctx.step(1 + n, n == 0)
         return self.loss_final()[0]
 
-    def backward(self, scale=1.0):
-        return self.ctx.adj_backward(scale * self._lam)[3]
+    def backward(self, scale=1.0, generation=None):
+        """dL/dw0 of the last forward(). generation: the number forward() had when the caller
+        ran it; a backward for any other forward raises (the checkpoints are gone)."""
+        if self._lam is None or (generation is not None and generation != self.generation):
+            raise RuntimeError("MisfitRunner.backward: the checkpoints of that forward were replaced by a later "
+                               "forward or evaluate on the same runner; run the forward again (one runner "
+                               "holds one forward at a time)")
+        g = self.ctx.adj_backward(scale * self._lam)[3]
+        self._lam = None          # the reverse sweep consumed the checkpoints' state
+        return g
 
 
 def _coords(ctx):
@@ -89,9 +107,10 @@ if torch is not None:
         def forward(fctx, w0, runner: MisfitRunner):
             loss = runner.forward(w0.detach().cpu().numpy().astype(np.float64))
             fctx.runner = runner
+            fctx.generation = runner.generation
             return torch.tensor(loss, dtype=torch.float64)
 
         @staticmethod
         def backward(fctx, grad_out):
-            g = fctx.runner.backward(float(grad_out))
+            g = fctx.runner.backward(float(grad_out), generation=fctx.generation)
             return torch.from_numpy(g), None

@@ -999,7 +999,9 @@ struct AdjState {
     std::vector<Buf> ckpt;        // N_STATE fields per checkpoint
     std::vector<Buf> snap;        // N_STATE fields per step of the segment being reversed
     std::vector<Buf> post;        // N_POST fields per step: the state after its physics block
-    struct SweepGraph { int n; double omega; hipGraphExec_t exec; };
+    // a captured transposed sweep sequence, replayed only on the buffers and geometry it was
+    // captured with (its kernels' arguments are baked in)
+    struct SweepGraph { int n; double omega; hipGraphExec_t exec; const void* ptrs[8]; int dims[4]; };
     std::vector<SweepGraph> graphs;   // the transposed sweep sequences, replayed
     int nsteps = 0, every = 0, first_iter = 1;
 };
@@ -1234,8 +1236,12 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
     };
     // the 2 launches per sweep go to the device as one graph, captured on first use per sweep
     // count and relaxation factor (the buffers are the adjoint state's own, fixed for its life)
+    const void* ptrs[8] = { l0, l1, s0, s1, mu0, in0, f0, pw };
+    const int dims[4] = { c->im, c->jm, hp, nm };
     hipGraphExec_t exec = nullptr;
-    for(auto& g : A->graphs) if(g.n == sweeps && g.omega == omega){ exec = g.exec; break; }
+    AdjState::SweepGraph* slot = nullptr;
+    for(auto& g : A->graphs) if(g.n == sweeps && g.omega == omega){ slot = &g; break; }
+    if(slot && std::equal(ptrs, ptrs + 8, slot->ptrs) && std::equal(dims, dims + 4, slot->dims)) exec = slot->exec;
     if(!exec){
         hipGraph_t graph = nullptr;
         ATJUP_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1246,7 +1252,16 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
         const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if(ie != hipSuccess){ set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); return ATJUP_ERR_HIP; }
-        A->graphs.push_back({ sweeps, omega, exec });
+        AdjState::SweepGraph g{ sweeps, omega, exec, {}, {} };
+        std::copy(ptrs, ptrs + 8, g.ptrs);
+        std::copy(dims, dims + 4, g.dims);
+        if(slot){
+            ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+            (void)hipGraphExecDestroy(slot->exec);
+            *slot = g;
+        } else {
+            A->graphs.push_back(g);
+        }
     }
     hipEvent_t t_sw = timer_begin(c);
     ATJUP_HIP_CHECK(hipGraphLaunch(exec, c->stream));

@@ -135,6 +135,9 @@ int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, i
     if(k->mca > 0 && (k->dry || precision == ATJUP_PRECISION_TANGENT)){
         set_error("the moist convective adjustment needs the moist profile (dry = 0)"); return ATJUP_ERR_PROFILE;
     }
+    if(k->chemistry && k->dry){
+        set_error("the chemistry needs the moist profile (dry = 0)"); return ATJUP_ERR_PROFILE;
+    }
     if(k->chemistry && precision == ATJUP_PRECISION_TANGENT){
         set_error("the tangent form has no chemistry"); return ATJUP_ERR_PROFILE;
     }
@@ -395,7 +398,8 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
         atjup_hip_default_chem_params(&c->chemp);
         atjup_hip_default_mca_params(&c->mcap);
         rc = alloc(c->report, report_bytes(c->im, c->jm, c->km));
-        rc = alloc(c->chem_w, 3 * c->n3 * sizeof(double));
+        // the reaction rates persist between chemistry calls; a dry context runs no chemistry
+        if(!rc && !dry) rc = alloc(c->chem_w, 3 * c->n3 * sizeof(double));
     }
     if(rc){ atjup_hip_ctx_destroy(c); return nullptr; }
     return c;
@@ -748,6 +752,7 @@ int atjup_hip_set_chemistry(atjup_ctx* c, const atjup_chem_params* p){
 int atjup_hip_chemistry(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;
     if(c->precision == ATJUP_PRECISION_TANGENT){ set_error("the tangent form has no chemistry"); return ATJUP_ERR_PROFILE; }
+    if(c->kn.dry || !c->chem_w.p){ set_error("the chemistry needs the moist profile (dry = 0)"); return ATJUP_ERR_PROFILE; }
     static const int diff[4] = { ATJUP_F_DIFFLUX_H2S, ATJUP_F_DIFFLUX_NH3, ATJUP_F_DIFFLUX_NH4SH, ATJUP_F_THERMALMASSFLUX };
     static const int mass[4] = { ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH };
     int rc = launch_chem_kernel(c, 0);

@@ -84,7 +84,13 @@ struct atjup_ctx {
     atjup::Buf rb_p;             // p_dyn during a solve, FP64, red-black split [2][im][jm][rb_hp]
     atjup::Buf rb_d, rb_r;       // FP32-refined solve: correction and FP64-formed residual, FP32, split
     int refine_sweeps = 50;      // FP32 sweeps per refinement round
-    struct SweepGraph { int kind, n, ghost_first; double omega; void* exec; };
+    // A captured sweep sequence bakes in its kernels' arguments: the buffers (iterate, source, face
+    // flags and weights of both colours) and the launch geometry. It is replayed only when all of
+    // them equal the request's; any difference re-captures it.
+    struct SweepGraph {
+        int kind, n, ghost_first; double omega; void* exec;
+        const void* ptrs[7]; int dims[4];
+    };
     std::vector<SweepGraph> sweep_graphs;   // instantiated sweep sequences (hipGraphExec_t)
     atjup::Buf rb_flags;         // face flags, red-black split [2][im][jm][rb_hp]
     atjup::Buf lo[4];            // FP32-refined form: low words of t, u, v, w (the value is hi + lo)

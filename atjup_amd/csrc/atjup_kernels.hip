@@ -1327,9 +1327,15 @@ int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* 
             }
     };
 #if ATJUP_SWEEP_GRAPH
+    // the graph's key: everything its launches take as arguments besides the colour and the sweep
+    const void* ptrs[7] = { x0, x1, s0, s1, f0, f1, pw };
+    const int dims[4] = { c->im, c->jm, hp, nm };
     hipGraphExec_t exec = nullptr;
+    atjup_ctx::SweepGraph* slot = nullptr;
     for(auto& g : c->sweep_graphs)
-        if(g.kind == kind && g.n == n && g.ghost_first == ghost_first && g.omega == omega){ exec = (hipGraphExec_t)g.exec; break; }
+        if(g.kind == kind && g.n == n && g.ghost_first == ghost_first && g.omega == omega){ slot = &g; break; }
+    if(slot && std::equal(ptrs, ptrs + 7, slot->ptrs) && std::equal(dims, dims + 4, slot->dims))
+        exec = (hipGraphExec_t)slot->exec;
     if(!exec){
         hipGraph_t graph = nullptr;
         ATJUP_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1340,7 +1346,18 @@ int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* 
         const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if(ie != hipSuccess){ set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); return ATJUP_ERR_HIP; }
-        c->sweep_graphs.push_back({ kind, n, ghost_first, omega, (void*)exec });
+        atjup_ctx::SweepGraph g{ kind, n, ghost_first, omega, (void*)exec, {}, {} };
+        std::copy(ptrs, ptrs + 7, g.ptrs);
+        std::copy(dims, dims + 4, g.dims);
+        if(slot){
+            // the same sequence on other buffers: the old executable is no longer replayed (the
+            // stream is ordered, so a replay still queued has been submitted before this point)
+            ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+            (void)hipGraphExecDestroy((hipGraphExec_t)slot->exec);
+            *slot = g;
+        } else {
+            c->sweep_graphs.push_back(g);
+        }
     }
     ATJUP_HIP_CHECK(hipGraphLaunch(exec, c->stream));
     return 0;

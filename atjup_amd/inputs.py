@@ -171,11 +171,14 @@ def zonal_jets(g: Grid, rng, n_jets=12, w_max=150.0):
 
 
 def make_state(config: str = "dyn", seed: int = 1234, g: Grid | None = None,
-               with_obstacle: bool = True, u_0=100.0) -> State:
+               with_obstacle: bool = True, u_0=100.0, moisture: str = "seeded") -> State:
     """The seeded initial state of one configuration.
 
     config "dyn": dry dynamics (species zero, k = 0, dis = 1e-10 as resetArrays leaves them).
-    config "full": moist with the k-omega closure, see ``_moisten``.
+    config "full": moist with the k-omega closure; moisture "seeded" (``_moisten``: vapour at
+    0.6-1.25 x saturation and condensate seeds, so that every branch of the adjustment runs) or
+    "subsaturated" (``_moisten_subsaturated``: vapour at 0.5-1.0 x saturation, no condensate, the
+    full-physics benchmark case).
     """
     g = g or Grid()
     rng = np.random.default_rng(seed)
@@ -202,8 +205,47 @@ def make_state(config: str = "dyn", seed: int = 1234, g: Grid | None = None,
         f[n][:, :, g.km - 1] = f[n][:, :, 0]
     st = State(grid=g, fields=f, mask=mask)
     if config == "full":
-        _moisten(st, rng)
+        if moisture == "seeded":
+            _moisten(st, rng)
+        elif moisture == "subsaturated":
+            _moisten_subsaturated(st, rng)
+        else:
+            raise ValueError(f"moisture recipe {moisture!r}")
     return st
+
+
+def _saturation_mixing(gas, T, P, rho):
+    """q_sat over the liquid [kg/m3] of one gas, capped at 0.05, from the Sanchez-Lavega fit."""
+    from .abi import GAS_PARAMS
+    t_0, t_00, ep, lv, ls, C, L0, R, da, db = GAS_PARAMS[gas][:10]
+    e = np.exp(C + (-L0 / T + da * np.log(T) + db * T) / (1e-3 * R))   # [bar] (planet/ATPhys.h:49-52)
+    return np.minimum(rho * ep * e / P, 0.05)
+
+
+def _moisten_subsaturated(st: State, rng):
+    """The full-physics benchmark's moist state: H2O, NH3 and CH4 vapour at seeded U(0.5, 1.0) x
+    saturation over the liquid, no condensate (clouds form only where the flow or the radiation
+    brings a cell to saturation), H2S and NH4SH traces, and the k-omega start values."""
+    from .abi import GAS_FIELDS
+    g = st.grid
+    f = st.fields
+    shp = g.shape
+    T = f["t"] * 165.0
+    P = f["p_stat"]
+    rho = P * 1.0e5 / (R_MIX * T)
+    for gas, (vap, cld, ice) in GAS_FIELDS.items():
+        f[vap] = _saturation_mixing(gas, T, P, rho) * rng.uniform(0.5, 1.0, shp)
+        f[cld] = np.zeros(shp)
+        f[ice] = np.zeros(shp)
+    f["h2s"] = 0.0004 * rng.uniform(0.5, 1.0, shp)
+    f["nh4sh"] = 1.0e-5 * rng.uniform(0.0, 1.0, shp)
+    f["tke"] = np.full(shp, 1.0e-4 / 100.0 ** 2)
+    f["dis"] = np.full(shp, 1.0e-3 * 140.0e3 / 100.0)
+    solid = st.mask == 1.0
+    for s in SPECIES + ("tke",):
+        f[s][solid] = 0.0
+    for n in f:
+        f[n][:, :, g.km - 1] = f[n][:, :, 0]
 
 
 def _moisten(st: State, rng):

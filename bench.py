@@ -9,10 +9,15 @@ of each window treated as the first iteration of a run, i.e. three pressure solv
 exactly the five-step dynamics case. One cell-update = one interior cell (i 1..39, j 3..177,
 k 0..359: 2,457,000 cells) advanced by one step.
 
+The default workload (--config full) is the full-physics case: the k-omega closure, the saturation
+adjustments, the grey radiation, the three-category precipitation and the chemistry sequence in the
+physics block, from vapour at 0.5-1.0 x saturation. --config dyn is the dry dynamics case.
+
 The JSON line carries, besides the throughput, the roofline of the dominant kernel (the red-black
 colour kernel of the pressure solve), measured live with HIP events on the library's stream, and
-the CPU baseline: the reference itself (compiled from its own sources into oracle/_ref) timed on
-this host's cores over a five-step sample of the same workload.
+the CPU baseline: the reference itself (compiled from its own sources into oracle/_ref, at -O2 and
+at the -O0 of its own Makefile) timed on this host's cores over a sample of the same workload, in
+child processes with the OpenMP thread count set and reported.
 """
 from __future__ import annotations
 
@@ -41,14 +46,18 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_TRAFFIC_BYTES = {(41, 181, 361): (13907.4 * 2.0 + 10442.8) * 1024}
 WORKLOAD = {
     "dyn": "dynamics-only FP64 time step",
-    "full": "full-physics FP64 time step (k-omega closure, saturation adjustment of H2O/NH3/CH4, NH4SH "
-            "flux limiter, 17 prognostic fields)",
+    "full": "full-physics FP64 time step (k-omega closure, saturation adjustment of H2O/NH3/CH4, grey "
+            "radiation, three-category precipitation, chemistry sequence with the NH4SH flux limiter, "
+            "17 prognostic fields)",
 }
 DATA = {
     "dyn": "synthetic (seeded 12-jet zonal wind, parabolic T with 110 K clamp, ellipsoid obstacle)",
-    "full": "synthetic (seeded 12-jet zonal wind, parabolic T with 110 K clamp, ellipsoid obstacle, vapour at "
-            "seeded 0.6-1.25 x saturation with condensate seeds, k=1e-4 m2/s2, omega=1e-3 1/s)",
+    "full": "synthetic (seeded 12-jet zonal wind, parabolic T with 110 K clamp, ellipsoid obstacle, H2O/NH3/CH4 "
+            "vapour at seeded 0.5-1.0 x saturation, no condensate, k=1e-4 m2/s2, omega=1e-3 1/s)",
 }
+# the physics the profile of each workload runs in its physics block
+PHYSICS = {"dyn": {}, "full": {"radiation": True, "precip": True, "chemistry": True}}
+MOISTURE = {"dyn": "seeded", "full": "subsaturated"}
 
 
 def poisson_launch_bytes(g, esz_p=8, omega=1.0):
@@ -66,7 +75,7 @@ def poisson_launch_bytes(g, esz_p=8, omega=1.0):
     return n * per_cell
 
 
-def step_bytes_per_cell_update(config, esz, sweeps, solves_per_step):
+def step_bytes_per_cell_update(config, esz, sweeps, solves_per_step, physics=None):
     """Algorithmic HBM bytes per cell-update of the whole step.
 
     RK4: every prognostic field is streamed 17 times a step (stage 0: input, accumulator write,
@@ -74,37 +83,68 @@ def step_bytes_per_cell_update(config, esz, sweeps, solves_per_step):
     accumulator read, output), every read-only input once a stage, plus the aux / closure outputs
     and the 1-byte face flags. Pressure solve: per call the source build (4 fields) and 3 field
     passes per sweep (1.5 per colour: the other colour, the own colour's source and value). Physics
-    block (full profile, per call): mixture density 4, saturation adjustment 3 x 10, closure 25,
-    limiter 5, the two Shapiro passes 12 field streams.
+    block per call, in field streams: mixture density 4, the Shapiro pass on p_dyn 6 (3 axes, a read
+    and a write each); full profile also the saturation adjustment 3 x 10, the closure 25, the
+    radiation 14, the precipitation 33, and the chemistry sequence 76 (its two kernels 10 + 13, the
+    limiter 5, eight Shapiro passes 48); without the chemistry the limiter 5 and its Shapiro pass 6.
     """
+    physics = physics or {}
     if config == "dyn":
         n_prog, n_in, n_out, phys = 4, 5, 5, 4 + 6
     else:
-        n_prog, n_in, n_out, phys = 17, 22, 8, 4 + 6 + 30 + 25 + 5 + 6
+        n_prog, n_in, n_out = 17, 22, 8
+        phys = 4 + 6 + 30 + 25
+        phys += 14 if physics.get("radiation") else 0
+        phys += 33 if physics.get("precip") else 0
+        phys += 76 if physics.get("chemistry") else 5 + 6
     rk = esz * (17 * n_prog + 4 * n_in + n_out) + 4 * 1
     pois = 8 * (4 + 3 * sweeps) * solves_per_step
     return rk + pois + esz * phys * solves_per_step
 
 
-def cpu_baseline(config, sweeps, steps=10):
-    """The reference (oracle/_ref) or, without it, the oracle restatement, on this host."""
+def cpu_threads():
+    """The host threads the CPU legs use: ATJUP_CPU_THREADS, else the 16 CPUs a one-GPU box grants a
+    job (os.cpu_count() shows the whole host there), else fewer if the host has fewer."""
+    env = os.environ.get("ATJUP_CPU_THREADS")
+    return int(env) if env else min(16, os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg_child(config, sweeps, steps, opt, threads):
+    """One CPU-baseline sample in this (child) process: the reference compiled at -<opt>, or the
+    oracle restatement where the reference is not built, over iterations 1..steps of the workload
+    (the same physics and inputs as the GPU run), with `threads` OpenMP threads. Prints one JSON line."""
     import oracle
     from atjup_amd import abi, inputs
-    G, Cn, K, env = abi.make_profile(config, sweeps=sweeps)
-    st = inputs.make_state(config, seed=11)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    if oracle.reference_available():
-        ref = oracle.Reference(env)
+    G, Cn, K, env = abi.make_profile(config, sweeps=sweeps, **PHYSICS[config])
+    st = inputs.make_state(config, seed=11, moisture=MOISTURE[config])
+    ph = PHYSICS[config]
+    if oracle.reference_available(opt):
+        ref = oracle.Reference(env, opt=opt)
+        ref.set_threads(threads)
+        used = ref.max_threads()
         ref.load_state(st, Cn)
         ref.call("bc_all")
         ref.call("restore")
         t0 = time.perf_counter()
         for it in range(1, steps + 1):
-            ref.step(it, it == 1, satadj=True, turb=(config != "dyn"))
+            ref.step_physics(it, it == 1, satadj=True, turb=(config != "dyn"), radiation=ph.get("radiation", False),
+                             precip=ph.get("precip", False), chemistry=ph.get("chemistry", False))
         dt = time.perf_counter() - t0
         kind = "reference"
     else:
         orc = oracle.Oracle(G, Cn, K, threads=threads)
+        used = threads
         orc.load_state(st)
         orc.call("bc_after_step")
         orc.call("swap_nlevel")
@@ -113,10 +153,45 @@ def cpu_baseline(config, sweeps, steps=10):
             orc.call("step", it, 1 if it == 1 else 0)
         dt = time.perf_counter() - t0
         kind = "port"
-    return {"value": INTERIOR_CELLS * steps / dt, "unit": "cell-updates/s", "cores": threads,
-            "kind": kind,
-            "sample": f"{steps} steps (iterations 1..{steps}, {sweeps}-sweep solves on 1, 2, 4) of the "
-                      f"same 41x181x361 case, {dt:.1f} s, OMP_NUM_THREADS={threads}, -O2"}
+    print(json.dumps({"seconds": dt, "steps": steps, "threads": used, "kind": kind, "opt": opt}), flush=True)
+
+
+def cpu_baseline(config, sweeps, steps_o2=10, steps_o0=5):
+    """The reference on this host's cores at -O2 (the value) and at -O0 (its own Makefile's level),
+    each in a child process of its own (the reference holds one knob profile per process)."""
+    import subprocess
+    threads = cpu_threads()
+    legs = {}
+    for opt, steps in (("O2", steps_o2), ("O0", steps_o0)):
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", opt, "--cpu-steps", str(steps),
+               "--cpu-threads", str(threads), "--config", config, "--sweeps", str(sweeps)]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=600)
+        if r.returncode != 0:
+            legs[opt] = {"failed": (r.stderr or r.stdout)[-400:]}
+            continue
+        legs[opt] = json.loads(r.stdout.strip().splitlines()[-1])
+    o2 = legs["O2"]
+    if "seconds" not in o2:
+        return {"value": None, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+                "sample": f"failed: {o2.get('failed')}"}
+    solves = sorted(it for it in range(1, o2["steps"] + 1) if it % 2 == 0 or it == 1)
+    out = {
+        "value": INTERIOR_CELLS * o2["steps"] / o2["seconds"], "unit": "cell-updates/s",
+        "cores": o2["threads"], "kind": o2["kind"],
+        "sample": f"{o2['steps']} steps (iterations 1..{o2['steps']}, {sweeps}-sweep solves on iterations "
+                  f"{','.join(map(str, solves))}) of the same 41x181x361 workload, {o2['seconds']:.1f} s, "
+                  f"{o2['threads']} OpenMP threads (omp_set_num_threads), g++ -O2 -fopenmp",
+        "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+    }
+    o0 = legs.get("O0", {})
+    if "seconds" in o0:
+        out["O0"] = {"value": INTERIOR_CELLS * o0["steps"] / o0["seconds"], "unit": "cell-updates/s",
+                     "cores": o0["threads"], "kind": o0["kind"],
+                     "sample": f"{o0['steps']} steps of the same workload, {o0['seconds']:.1f} s, g++ -O0 -fopenmp "
+                               f"(the reference Makefile's flags)"}
+    else:
+        out["O0"] = o0
+    return out
 
 
 def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=None):
@@ -325,7 +400,15 @@ def main():
     ap.add_argument("--no-restart", action="store_true", help="skip the restart-file leg")
     ap.add_argument("--grid", default="standard", choices=["standard", "fine"],
                     help="standard: 41x181x361 (1 deg); fine: 81x361x721 (0.5 deg, no CPU baseline)")
+    ap.add_argument("--cpu-leg", dest="cpu_leg", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-steps", dest="cpu_steps", type=int, default=10, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-threads", dest="cpu_threads", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+
+    if args.cpu_leg:
+        # a child of cpu_baseline(): the CPU sample only, no GPU
+        cpu_leg_child(args.config, args.sweeps, args.cpu_steps, args.cpu_leg, args.cpu_threads or cpu_threads())
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -341,14 +424,15 @@ def main():
     from atjup_amd.lib import Context
 
     g = inputs.Grid() if args.grid == "standard" else inputs.fine_grid()
-    G, Cn, K, _ = abi.make_profile(args.config, sweeps=args.sweeps, omega=args.omega, grid=g)
-    st = inputs.make_state(args.config, seed=11, g=g)
+    physics = PHYSICS[args.config]
+    G, Cn, K, _ = abi.make_profile(args.config, sweeps=args.sweeps, omega=args.omega, grid=g, **physics)
+    st = inputs.make_state(args.config, seed=11, g=g, moisture=MOISTURE[args.config])
     cells = interior_cells(g)
     fine = args.grid == "fine"
 
-    def timed_run(precision, rounded_start=False):
+    def timed_run(precision, rounded_start=False, knobs=None):
         """warmup + steps iterations from the seeded state; the steps timed between two barriers."""
-        ctx = Context(G, Cn, K, precision)
+        ctx = Context(G, Cn, knobs or K, precision)
         s0 = st
         if rounded_start:
             s0 = inputs.State(grid=st.grid, mask=st.mask,
@@ -439,12 +523,25 @@ def main():
         except Exception as e:  # disk space on the host never decides the GPU number
             rst = {"workload": "JUP2 restart file", "failed": str(e)}
 
+    # the same step with the radiation, precipitation and chemistry left out of the physics block
+    # (massflux_*, Q_rad, Q_precip, S_precip_* frozen): the dynamics, closure and adjustments alone
+    closure = None
+    if args.config == "full":
+        _, _, Kc, _ = abi.make_profile("full", sweeps=args.sweeps, omega=args.omega, grid=g)
+        cc, msc, _, okc = timed_run(abi.PRECISION_FP64, knobs=Kc)
+        cc.close()
+        closure = {"workload": "the same FP64 step with radiation, precipitation and chemistry off (their outputs "
+                               "frozen inputs)", "value": cells * args.steps / (msc * 1e-3),
+                   "unit": "cell-updates/s", "ms_per_step": msc / args.steps, "finite": bool(okc),
+                   "step_algorithmic_bytes_per_cell_update":
+                       step_bytes_per_cell_update("full", 8, args.sweeps, 3.0 / 5.0, {})}
+
     value = cells * args.steps / (ms * 1e-3)
     launch_ms = sweep_ms / max(sweep_launches, 1)
     lb = poisson_launch_bytes(g, 8, args.omega)
     achieved = lb / (launch_ms * 1e-3) / 1e9
     solves_per_step = 3.0 / 5.0
-    step_b = step_bytes_per_cell_update(args.config, 8, args.sweeps, solves_per_step)
+    step_b = step_bytes_per_cell_update(args.config, 8, args.sweeps, solves_per_step, physics)
 
     adj = None
     if not args.no_adjoint:
@@ -494,11 +591,16 @@ def main():
             "poisson_sweeps": sweep_ms / args.steps,
             "rk4_stage_kernels": rk_ms / args.steps,
             "rk4_stage_avg_launch_ms": rk_ms / max(rk_launches, 1),
+            "pressure_solve": tm["pressure"][0] / args.steps,
+            "radiation": tm["radiation"][0] / args.steps,
+            "precipitation": tm["precipitation"][0] / args.steps,
+            "chemistry_kernels": tm["chemistry"][0] / args.steps,
         },
         "step_algorithmic_bytes_per_cell_update": step_b,
         "step_hbm_frac": value * step_b / (HBM_PEAK_GBS * 1e9),
         "gpu_event_ms_per_step": tm["step"][0] / args.steps,
         "cpu_baseline": cpu,
+        "dynamics_closure_only": closure,
         "fp32_refined": fp32,
         "adjoint": adj,
         "radiation": rad,

@@ -25,6 +25,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libatjup_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libatjup_ref_O2.so")
 
+
+def ref_so(opt: str = "O2") -> str:
+    """The compiled reference at optimisation level opt ("O2" or "O0")."""
+    return os.path.join(HERE, "_ref", f"libatjup_ref_{opt}.so")
+
 _D = C.POINTER(C.c_double)
 
 
@@ -57,6 +62,7 @@ def _orc_lib():
                   "orc_mca"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.orc_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.orc_rb_colour_pass.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
         L.orc_satadj.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_gas.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.GasParams)]
         L.orc_set_radiation.argtypes = [C.c_void_p, C.POINTER(abi.RadParams)]
@@ -151,10 +157,13 @@ class Oracle:
 class Reference:
     """The compiled reference model behind ref_harness.cpp (one profile per process)."""
 
-    def __init__(self, env: dict, quiet: bool = True):
-        if not os.path.exists(REF_SO):
-            raise RuntimeError(f"{REF_SO} is not built (make -C oracle ref)")
-        L = C.CDLL(REF_SO)
+    def __init__(self, env: dict, quiet: bool = True, opt: str = "O2"):
+        """opt: the optimisation level the reference was compiled at ("O2", or "O0", the level of
+        the reference's own Makefile); each lives in its own library under _ref/."""
+        so = ref_so(opt)
+        if not os.path.exists(so):
+            raise RuntimeError(f"{so} is not built (make -C oracle ref OPT=-{opt})")
+        L = C.CDLL(so)
         L.ref_init.argtypes = [C.c_char_p]
         L.ref_set_field.argtypes = [C.c_char_p, _D]
         L.ref_get_field.argtypes = [C.c_char_p, _D]
@@ -256,6 +265,24 @@ class Reference:
         if self.L.ref_step(iter_n, 1 if first else 0, 1 if satadj else 0, 1 if turb else 0) != 0:
             raise RuntimeError("ref_step failed")
 
+    def step_physics(self, iter_n, first, satadj=True, turb=False, radiation=False, precip=False,
+                     chemistry=False, nh4sh_eq=False):
+        """The loop body with the physics block's radiation, precipitation, chemistry sequence and
+        NH4SH equilibrium as the reference runs them (cJupiterModel.cpp:1032-1161)."""
+        flags = ((1 if satadj else 0) | (2 if turb else 0) | (4 if radiation else 0) | (8 if precip else 0)
+                 | (16 if chemistry else 0) | (32 if nh4sh_eq else 0))
+        self.L.ref_step_physics.argtypes = [C.c_int, C.c_int, C.c_int]
+        if self.L.ref_step_physics(iter_n, 1 if first else 0, flags) != 0:
+            raise RuntimeError("ref_step_physics failed")
 
-def reference_available() -> bool:
-    return os.path.exists(REF_SO)
+    def set_threads(self, n: int):
+        """OpenMP threads of the reference's loops."""
+        self.L.ref_set_threads.argtypes = [C.c_int]
+        self.L.ref_set_threads(int(n))
+
+    def max_threads(self) -> int:
+        return int(self.L.ref_max_threads())
+
+
+def reference_available(opt: str = "O2") -> bool:
+    return os.path.exists(ref_so(opt))

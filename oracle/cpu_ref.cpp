@@ -566,14 +566,13 @@ void rk4(orc_ctx& o){
 // ---- PressureSolver<P>::run, PRESS_SRC 1, PRESS_WALL 1, rigid lid, no PRESS_INCR / LINE /
 //      BAROTROPIC (PressureSolver.h:145-766), with prepareProjectionBoundaries
 //      (cJupiterModel.cpp:1842-1864). omega != 1 blends the Gauss-Seidel value (SOR). ---------
-void pressure(orc_ctx& o, int n_sweeps, double omega){
+// prepareProjectionBoundaries(rigid_lid = true) (cJupiterModel.cpp:1842-1864); idempotent
+void prep_projection(orc_ctx& o){
     const int im = o.im, jm = o.jm, km = o.km;
     auto& au = o.f[ATJUP_F_AUX_U];
     auto& av = o.f[ATJUP_F_AUX_V];
     auto& aw = o.f[ATJUP_F_AUX_W];
-    auto& p = o.f[ATJUP_F_P_DYN];
     const double c43 = o.c43, c13 = o.c13;
-    // prepareProjectionBoundaries(rigid_lid = true)
     #pragma omp parallel for collapse(2)
     for(int j = 1; j < jm - 1; j++)
         for(int k = 0; k < km - 1; k++){
@@ -584,6 +583,17 @@ void pressure(orc_ctx& o, int n_sweeps, double omega){
             aw[o.id(0, j, k)] = c43 * aw[o.id(1, j, k)] - c13 * aw[o.id(2, j, k)];
             aw[o.id(im - 1, j, k)] = c43 * aw[o.id(im - 2, j, k)] - c13 * aw[o.id(im - 3, j, k)];
         }
+}
+
+// One colour of a red-black sweep (PressureSolver.h:323-495): every interior cell with
+// (i + j + k) & 1 == colour takes its Gauss-Seidel value from the current neighbours, blended with
+// its old value when omega != 1. The outer planes are not touched.
+void rb_colour_pass(orc_ctx& o, int colour, double omega){
+    const int im = o.im, jm = o.jm, km = o.km;
+    auto& au = o.f[ATJUP_F_AUX_U];
+    auto& av = o.f[ATJUP_F_AUX_V];
+    auto& aw = o.f[ATJUP_F_AUX_W];
+    auto& p = o.f[ATJUP_F_P_DYN];
     std::vector<double> sinth(jm);
     for(int j = 0; j < jm; j++){
         sinth[j] = o.sin_the[j];
@@ -595,8 +605,8 @@ void pressure(orc_ctx& o, int n_sweeps, double omega){
     const double inv_dthe = 1.0 / o.g.dthe, inv_dphi = 1.0 / o.g.dphi;
     const double cap = o.kn.pdyn_cap;
     auto S = [&](const std::vector<double>& A, int i, int j, int k){ return A[o.id(i, j, k)]; };
-    for(int sweep = 0; sweep < n_sweeps; sweep++){
-        for(int colour = 0; colour < 2; colour++){
+    {
+        {
             #pragma omp parallel for collapse(2) schedule(static)
             for(int i = 1; i < im - 1; i++)
                 for(int j = 1; j < jm - 1; j++){
@@ -696,6 +706,16 @@ void pressure(orc_ctx& o, int n_sweeps, double omega){
                     }
                 }
         }
+    }
+}
+
+// the outer 2-point Neumann planes i = 0, im-1 and j = 0, jm-1 and the seam copy, after every
+// sweep (PressureSolver.h:718-746)
+void rb_planes(orc_ctx& o){
+    const int im = o.im, jm = o.jm, km = o.km;
+    auto& p = o.f[ATJUP_F_P_DYN];
+    const double c43 = o.c43, c13 = o.c13;
+    {
         #pragma omp parallel for collapse(2)
         for(int k = 0; k < km; k++)
             for(int j = 0; j < jm; j++){
@@ -712,6 +732,15 @@ void pressure(orc_ctx& o, int n_sweeps, double omega){
         for(int i = 0; i < im; i++)
             for(int j = 0; j < jm; j++)
                 p[o.id(i, j, km - 1)] = p[o.id(i, j, 0)];
+    }
+}
+
+void pressure(orc_ctx& o, int n_sweeps, double omega){
+    prep_projection(o);
+    for(int sweep = 0; sweep < n_sweeps; sweep++){
+        rb_colour_pass(o, 0, omega);
+        rb_colour_pass(o, 1, omega);
+        rb_planes(o);
     }
 }
 
@@ -2209,6 +2238,15 @@ int orc_hydrostatic(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; buoy_ref_level(*o)
 int orc_pressure_solve(orc_ctx* o, int sweeps, double omega){
     if(!o) return ATJUP_ERR_ARG;
     pressure(*o, sweeps > 0 ? sweeps : std::max(1, o->kn.press_sweeps), omega);
+    return 0;
+}
+// One colour pass of the sweep alone (after prepareProjectionBoundaries, which is idempotent), then
+// the outer planes and the seam if `planes`: a sweep is pass 0, pass 1 with planes.
+int orc_rb_colour_pass(orc_ctx* o, int colour, double omega, int planes){
+    if(!o || (colour != 0 && colour != 1)) return ATJUP_ERR_ARG;
+    prep_projection(*o);
+    rb_colour_pass(*o, colour, omega);
+    if(planes) rb_planes(*o);
     return 0;
 }
 int orc_bc_radius(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; bc_radius(*o); return 0; }

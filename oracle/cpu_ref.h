@@ -23,6 +23,7 @@ int      orc_rk4_step(orc_ctx* o);
 int      orc_buoy_ref_level(orc_ctx* o);
 int      orc_hydrostatic(orc_ctx* o);
 int      orc_pressure_solve(orc_ctx* o, int sweeps, double omega);
+int      orc_rb_colour_pass(orc_ctx* o, int colour, double omega, int planes);
 int      orc_bc_after_step(orc_ctx* o);
 int      orc_bc_radius(orc_ctx* o);
 int      orc_bc_theta(orc_ctx* o);

@@ -26,6 +26,7 @@
 #include "Reporting.h"
 
 #include <fcntl.h>
+#include <omp.h>
 #include <unistd.h>
 
 #include <map>
@@ -264,7 +265,9 @@ public:
                 M.C_ch4_ice, M.L0_ch4_ice, M.del_alf_ch4_ice, M.del_bet_ch4_ice,
                 M.ch4, M.ch4_cloud, M.ch4_ice);
         else if(op == "scalar_sync"){
-            // ATJUP_RK_SCALAR_SYNC = 2 (cJupiterModel.cpp:1118-1140)
+            // ATJUP_RK_SCALAR_SYNC = 2 (cJupiterModel.cpp:1118-1140); H2S and NH4SH join it when
+            // the NH4SH equilibrium adjustment runs (:1120)
+            const bool eq = ChemistryJup::nh4sh_eq();
             #pragma omp parallel for collapse(2) schedule(static)
             for(int i = 0; i < M.im; i++) for(int j = 0; j < M.jm; j++) for(int k = 0; k < M.km; k++){
                 M.h2on.x[i][j][k] = M.h2o.x[i][j][k];       M.h2o_cloudn.x[i][j][k] = M.h2o_cloud.x[i][j][k];
@@ -272,8 +275,10 @@ public:
                 M.nh3_cloudn.x[i][j][k] = M.nh3_cloud.x[i][j][k]; M.nh3_icen.x[i][j][k] = M.nh3_ice.x[i][j][k];
                 M.ch4n.x[i][j][k] = M.ch4.x[i][j][k];       M.ch4_cloudn.x[i][j][k] = M.ch4_cloud.x[i][j][k];
                 M.ch4_icen.x[i][j][k] = M.ch4_ice.x[i][j][k]; M.tn.x[i][j][k] = M.t.x[i][j][k];
+                if(eq){ M.h2sn.x[i][j][k] = M.h2s.x[i][j][k]; M.nh4shn.x[i][j][k] = M.nh4sh.x[i][j][k]; }
             }
         }
+        else if(op == "nh4sh_eq")       ChemistryJup(M).EquilibriumNH4SH("loop");   // cJupiterModel.cpp:1084
         else if(op == "iter_inc")       M.iter_n++;
         else return -1;
         return 0;
@@ -434,6 +439,49 @@ int ref_step(int iter, int first, int satadj, int turb){
     quiet_end();
     return rc;
 }
+
+// The loop body with the whole physics block as the reference runs it (cJupiterModel.cpp:1032-1161):
+// flags bit 0 the saturation adjustments, bit 1 the closure, bit 2 the radiation, bit 3 the
+// precipitation, bit 4 the chemistry sequence (DiffMassFluxJup, its smoothing, ChemMassRateJup, the
+// NH4SH limiter and the smoothing of the mass fluxes, which the reference always runs; without it
+// only the limiter and its smoothing run, massflux_* frozen), bit 5 the NH4SH equilibrium
+// adjustment (its knob ATJUP_NH4SH_EQ must be on in the profile). Then the rest of the iteration
+// as ref_step.
+int ref_step_physics(int iter, int first, int flags){
+    quiet_begin();
+    int rc = 0;
+    const bool turb = flags & 2;
+    if(iter % 2 == 0 || first){
+        rc |= H::call("pressure");
+        rc |= H::call("damp_pdyn");
+        rc |= H::call("mixture_density");
+        if(flags & 4) rc |= H::call("radiation");
+        if(flags & 1){ rc |= H::call("satadj_h2o"); rc |= H::call("satadj_nh3"); rc |= H::call("satadj_ch4"); }
+        if(flags & 32){
+            if(!ChemistryJup::nh4sh_eq()) rc |= 1;
+            else rc |= H::call("nh4sh_eq");
+        }
+        if(flags & 8) rc |= H::call("precipitation");
+        rc |= H::call("scalar_sync");
+        if(turb) rc |= H::call("turbulence");
+        if(flags & 16) rc |= H::call("chemistry");
+        else { rc |= H::call("flux_limiter"); rc |= H::call("fluxlim_damp"); }
+    }
+    rc |= H::call("rk4");
+    rc |= H::call("bc_all");
+    if(turb) rc |= H::call("turb_wall");
+    const char* e = getenv("ATJUP_MCA");
+    const int mca = e ? std::max(0, atoi(e)) : 1;
+    if(mca > 0 && iter % mca == 0) rc |= H::call("mca");
+    rc |= H::call("clamp_species");
+    rc |= H::call("restore");
+    quiet_end();
+    return rc;
+}
+
+// The OpenMP thread count of the reference's parallel loops (the CPU baseline states it)
+void ref_set_threads(int n){ if(n > 0) omp_set_num_threads(n); }
+int ref_max_threads(void){ return omp_get_max_threads(); }
 
 // the continuity residual's cell as steadyQuery leaves it (cJupiterModel.h:121)
 void ref_get_res(int* ijk){ H::res(ijk); }

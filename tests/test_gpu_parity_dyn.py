@@ -95,6 +95,23 @@ def test_five_steps_full_grid():
     orc.close()
 
 
+def test_config1_as_stated_omega_1_5():
+    """The dynamics case as specified: 41 x 181 x 361 with the ellipsoid obstacle, five steps with
+    200-sweep red-black solves at omega = 1.5 on iterations 1, 2 and 4, against the oracle (whose
+    blend is pinned by test_oracle_math), every field within 1e-10 and the mask bit-exact."""
+    gpu, orc, st = make_pair("dyn", sweeps=200, omega=1.5)
+    assert np.array_equal(gpu.get("wall_nue"), orc.get("wall_nue"))
+    for it in range(1, 6):
+        print(f"config 1 step {it}", flush=True)
+        gpu.step(it, it == 1)
+        orc.call("step", it, 1 if it == 1 else 0)
+    w = compare(gpu, orc, DYN_FIELDS, TOL, "config 1")
+    print("config-1 parity", {k: f"{v[0]:.2e}/{v[1]}" for k, v in w.items()})
+    assert gpu.check_finite()
+    gpu.close()
+    orc.close()
+
+
 def test_small_grid_steps_with_sor():
     st = small_state("dyn")
     gpu, orc, _ = make_pair("dyn", sweeps=30, omega=1.5, state=st)

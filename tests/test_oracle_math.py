@@ -145,3 +145,68 @@ def test_sor_relaxes_the_vertical_mode_faster_than_gauss_seidel():
         o.close()
     assert spread[1.0] < 0.1 * 2.0
     assert spread[1.5] < 0.1 * spread[1.0]
+
+
+def _rb_state(seed=5):
+    """The small shell with its obstacle, random p_dyn and aux_* (the sources are O(1), so the clamp
+    at +-0.2 den cuts some of them)."""
+    st = small_state()
+    g = st.grid
+    rng = np.random.default_rng(seed)
+    o = _orc(st, sweeps=1)
+    for n in ("p_dyn", "aux_u", "aux_v", "aux_w"):
+        a = rng.standard_normal(g.shape) * (1.0 if n == "p_dyn" else 0.02)
+        a[:, :, -1] = a[:, :, 0]
+        o.set(n, a)
+    return o, g
+
+
+def _colour_mask(g, colour):
+    i, j, k = np.meshgrid(np.arange(g.im), np.arange(g.jm), np.arange(g.km), indexing="ij")
+    return ((i + j + k) % 2 == colour) & (i >= 1) & (i <= g.im - 2) & (j >= 1) & (j <= g.jm - 2) & (k <= g.km - 2)
+
+
+def test_colour_passes_compose_to_the_pinned_sweep():
+    """A Gauss-Seidel sweep is colour pass 0, colour pass 1, then the outer planes: the two passes
+    alone reproduce pressure_solve(1, 1.0), which equals the compiled reference bit for bit
+    (test_oracle_vs_reference), and at omega = 1.5 they reproduce pressure_solve(1, 1.5)."""
+    for omega in (1.0, 1.5):
+        o, g = _rb_state()
+        p0 = o.get("p_dyn")
+        o.call("pressure_solve", 1, omega)
+        sweep = o.get("p_dyn")
+        o.set("p_dyn", p0)
+        o.call("rb_colour_pass", 0, omega, 0)
+        o.call("rb_colour_pass", 1, omega, 1)
+        assert np.array_equal(o.get("p_dyn"), sweep), omega
+        o.close()
+
+
+@pytest.mark.parametrize("omega", [0.6, 1.3, 1.5, 1.9])
+@pytest.mark.parametrize("colour", [0, 1])
+def test_sor_blend_is_the_gauss_seidel_value_relaxed(colour, omega):
+    """Over-relaxation as a closed form of the pinned Gauss-Seidel pass: on the same input, a colour
+    pass at omega leaves every cell of the other colour and every boundary cell as it was and sets
+    each cell of its colour to (1 - omega) p_old + omega p_GS, where p_GS is that cell after the
+    omega = 1 pass. Colour 1 is checked on the state after a colour-0 pass, as a sweep reaches it.
+    A blend with another weight (an effective omega of 1.3 for 1.5), applied to a neighbour's or an
+    updated old value, or to the other colour fails this."""
+    o, g = _rb_state(seed=11 + colour)
+    if colour == 1:
+        o.call("rb_colour_pass", 0, omega, 0)
+    p_old = o.get("p_dyn")
+    o.call("rb_colour_pass", colour, 1.0, 0)
+    p_gs = o.get("p_dyn")
+    o.set("p_dyn", p_old)
+    o.call("rb_colour_pass", colour, omega, 0)
+    p_sor = o.get("p_dyn")
+    o.close()
+    mine = _colour_mask(g, colour)
+    assert np.array_equal(p_sor[~mine], p_old[~mine])
+    assert np.count_nonzero(p_gs[mine] != p_old[mine]) > 0.9 * np.count_nonzero(mine)
+    expect = (1.0 - omega) * p_old[mine] + omega * p_gs[mine]
+    scale = np.maximum(np.abs(p_old[mine]), np.abs(p_gs[mine]))
+    assert np.all(np.abs(p_sor[mine] - expect) <= 4.0 * np.finfo(float).eps * scale)
+    # and not the blend with the weight of a neighbouring omega
+    wrong = (1.0 - (omega - 0.2)) * p_old[mine] + (omega - 0.2) * p_gs[mine]
+    assert np.max(np.abs(p_sor[mine] - wrong)) > 1e3 * np.finfo(float).eps * np.max(scale)
