@@ -400,6 +400,8 @@ def main():
     ap.add_argument("--no-restart", action="store_true", help="skip the restart-file leg")
     ap.add_argument("--grid", default="standard", choices=["standard", "fine"],
                     help="standard: 41x181x361 (1 deg); fine: 81x361x721 (0.5 deg, no CPU baseline)")
+    ap.add_argument("--headline-only", dest="headline_only", action="store_true",
+                    help="time the headline FP64 step only (no FP32 form, adjoint, CPU baseline or component legs)")
     ap.add_argument("--cpu-leg", dest="cpu_leg", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-steps", dest="cpu_steps", type=int, default=10, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", dest="cpu_threads", type=int, default=0, help=argparse.SUPPRESS)
@@ -409,6 +411,9 @@ def main():
         # a child of cpu_baseline(): the CPU sample only, no GPU
         cpu_leg_child(args.config, args.sweeps, args.cpu_steps, args.cpu_leg, args.cpu_threads or cpu_threads())
         return
+    if args.headline_only:
+        args.precision_compare = False
+        args.no_cpu = args.no_adjoint = args.no_restart = True
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -511,11 +516,12 @@ def main():
 
     # the grey radiation and the precipitation (components beside the path: they run in the
     # physics block when the profile turns them on), timed on the state the FP64 run left
-    rad = radiation_leg(ctx, g) if args.config == "full" else None
-    precip = precipitation_leg(ctx, g) if args.config == "full" else None
-    chem = chemistry_leg(ctx, g) if args.config == "full" else None
-    mca = mca_leg(ctx, g) if args.config == "full" else None
-    report = reporting_leg(ctx, g)
+    legs = args.config == "full" and not args.headline_only
+    rad = radiation_leg(ctx, g) if legs else None
+    precip = precipitation_leg(ctx, g) if legs else None
+    chem = chemistry_leg(ctx, g) if legs else None
+    mca = mca_leg(ctx, g) if legs else None
+    report = reporting_leg(ctx, g) if not args.headline_only else None
     rst = None
     if not args.no_restart:
         try:
@@ -526,7 +532,7 @@ def main():
     # the same step with the radiation, precipitation and chemistry left out of the physics block
     # (massflux_*, Q_rad, Q_precip, S_precip_* frozen): the dynamics, closure and adjustments alone
     closure = None
-    if args.config == "full":
+    if legs:
         _, _, Kc, _ = abi.make_profile("full", sweeps=args.sweeps, omega=args.omega, grid=g)
         cc, msc, _, okc = timed_run(abi.PRECISION_FP64, knobs=Kc)
         cc.close()
