@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <mutex>
 
 namespace atjup {
 namespace {
@@ -221,6 +222,34 @@ size_t report_bytes(int im, int jm, int km){
     return (RP_BLOCKS + 1) * sizeof(Part) + 2 * sizeof(double) + slice * sizeof(double);
 }
 
+// ---- the allocation-free search (atjup_hip_search_minmax_3d_noalloc) ---------------------------
+// Its partial results live in this file-scope device array instead of a context's report buffer:
+// RP_BLOCKS partials, the combined result and the starting element. Each GPU holds its own copy
+// for the life of the process; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device (the array's name is a host-side stand-in, not a device address).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (RP_BLOCKS + 1) * sizeof(Part) + 2 * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the device address of `scratch` on the current device, looked up once per device
+static int noalloc_scratch(void** out){
+    int dev = -1;
+    ATJUP_HIP_CHECK(hipGetDevice(&dev));
+    if(dev < 0 || dev >= NOALLOC_MAX_DEVICES){ set_error("device index above 63"); return ATJUP_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if(!g_scratch_ptr[dev]){
+        void* p = nullptr;
+        ATJUP_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if(!p){ set_error("hipGetSymbolAddress returned no address for the scratch array"); return ATJUP_ERR_HIP; }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 }  // namespace atjup
 
 using namespace atjup;
@@ -240,6 +269,46 @@ int atjup_hip_search_minmax_3d(atjup_ctx* c, int field, double* out, int* loc){
         loc[3 * s + 0] = (int)(o[s] % c->im);
         loc[3 * s + 2] = (int)((o[s] / c->im) % c->km);
         loc[3 * s + 1] = (int)(o[s] / ((long long)c->im * c->km));
+    }
+    return 0;
+}
+
+// searchMinMax_3D over a caller's device field with no context and no allocation: the partial
+// results go to the file-scope scratch array. Calls that overlap on one GPU share that array and
+// race; each call writes every partial it reads, so nothing carries over between calls.
+int atjup_hip_search_minmax_3d_noalloc(const double* field, int im, int jm, int km, int pitch,
+                                       void* stream, double* out, int* loc){
+    if(!field || !out || !loc){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(im < 1 || jm < 1 || km < 1 || pitch < km){ set_error("grid: im, jm, km >= 1 and pitch >= km"); return ATJUP_ERR_ARG; }
+    constexpr size_t MAX_CELLS = (size_t)1 << 36;
+    const size_t cells = (size_t)im * jm * pitch;
+    if(im > 4096 || jm > 65536 || pitch > 65536 || cells > MAX_CELLS){
+        set_error("search_minmax_3d_noalloc: the field exceeds the maximum (im <= 4096, jm <= 65536, "
+                  "pitch <= 65536, im * jm * pitch <= 2^36)");
+        return ATJUP_ERR_ARG;
+    }
+    void* s = nullptr;
+    if(int rc = noalloc_scratch(&s)) return rc;
+    Part* parts = (Part*)s;
+    Part* res = parts + RP_BLOCKS;
+    double* first = (double*)(res + 1);
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = (int)std::min<size_t>(RP_BLOCKS, (cells + RP_BK - 1) / RP_BK);
+    hipLaunchKernelGGL(k_minmax3d<double>, dim3(nb), dim3(RP_BK), 0, st, field, (const float*)nullptr,
+        im, jm, km, pitch, parts);
+    hipLaunchKernelGGL(k_first<double>, dim3(1), dim3(1), 0, st, field, (const float*)nullptr, first);
+    hipLaunchKernelGGL(k_reduce_parts<0>, dim3(1), dim3(RP_BK), 0, st, (const Part*)parts, nb,
+        (const double*)first, res);
+    if(int rc = check_launch()) return rc;
+    Part r;
+    ATJUP_HIP_CHECK(hipMemcpyAsync(&r, res, sizeof(Part), hipMemcpyDeviceToHost, st));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(st));
+    const long long o[2] = { r.mx.o, r.mn.o };
+    out[0] = r.mx.v; out[1] = r.mn.v;
+    for(int q = 0; q < 2; q++){                              // position (j * km + k) * im + i
+        loc[3 * q + 0] = (int)(o[q] % im);
+        loc[3 * q + 2] = (int)((o[q] / im) % km);
+        loc[3 * q + 1] = (int)(o[q] / ((long long)im * km));
     }
     return 0;
 }

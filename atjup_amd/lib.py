@@ -77,6 +77,8 @@ def load():
     for n in ("atjup_hip_search_minmax_3d", "atjup_hip_search_minmax_2d"):
         getattr(L, n).argtypes = [C.c_void_p, C.c_int, _D, C.POINTER(C.c_int)]
     L.atjup_hip_continuity_residual.argtypes = [C.c_void_p, _D, C.POINTER(C.c_int)]
+    L.atjup_hip_search_minmax_3d_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                     C.c_void_p, _D, C.POINTER(C.c_int)]
     L.atjup_hip_download_slice.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _D]
     L.atjup_hip_default_mca_params.argtypes = [C.POINTER(abi.MCAParams)]
     L.atjup_hip_save_state.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]

@@ -271,6 +271,17 @@ int atjup_hip_mca(atjup_ctx* ctx);
  * coefficient and conversion); loc = {i, j, k} of the max, then of the min. FP32-refined fields
  * t, u, v, w are read as hi + lo. */
 int atjup_hip_search_minmax_3d(atjup_ctx* ctx, int field, double* out, int* loc);
+/* The search of atjup_hip_search_minmax_3d, allocating nothing. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc). field is a caller's FP64 device array [im][jm][pitch] (the library's layout, row
+ * pitch >= km, the elements km..pitch-1 of a row ignored); the launches go to `stream` (a hipStream_t
+ * of the library's HIP runtime, NULL the null stream), which is synchronized before out and loc
+ * (as in atjup_hip_search_minmax_3d) are written. The partial results live in a fixed device array
+ * of the library (about 64 KiB per GPU, held until the process ends); the field may hold at most
+ * im <= 4096, jm <= 65536, pitch <= 65536 and 2^36 elements in all, larger fields are refused with
+ * ATJUP_ERR_ARG. Calls that overlap on one GPU share that array and race. */
+int atjup_hip_search_minmax_3d_noalloc(const double* field, int im, int jm, int km, int pitch,
+                                       void* stream, double* out, int* loc);
 /* searchMinMax_2D (planet/Reporting.h:299-330) over map 0-4 (the surface precipitation maps in the
  * order of atjup_hip_download_precip_surface) or 5 (vel_star): the starting element (0, 0) and the
  * interior j in [1, jm-2], k in [1, km-2]; loc = {j, k} of the max, then of the min. */
