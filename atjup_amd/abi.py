@@ -7,6 +7,7 @@ read (``ATJUP_*`` variables, planet/cJupiterModel.h:725-1216 and the shared head
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 from . import inputs
 
@@ -60,7 +61,7 @@ class Knobs(C.Structure):
                 ("wall_nue_factor", C.c_double), ("wall_nue_layers", C.c_int),
                 ("satadj", C.c_int), ("rk_scalar_sync", C.c_int), ("dry", C.c_int),
                 ("turb_k_bg_phys", C.c_double), ("radiation", C.c_int), ("precip", C.c_int),
-                ("chemistry", C.c_int), ("mca", C.c_int)]
+                ("chemistry", C.c_int), ("mca", C.c_int), ("nh4sh_eq", C.c_int)]
 
 
 CHEM_DOUBLES = ("m_nh3", "m_h2s", "m_nh4sh", "m_h2", "m_he", "m_h2o", "X_h2", "X_he", "X_h2o", "X_h2s",
@@ -170,22 +171,26 @@ def gas_params(gas: int) -> GasParams:
     return GasParams(*GAS_PARAMS[gas])
 
 
-# Reference environment shared by every profile: the balanced start, vortex, moist convective
-# adjustment, cone projection, continuity init and NH4SH equilibrium are off (they are outside this
-# path), the metric radius is 71492 km, dt 0.005.
+# Reference environment shared by every profile: the balanced start, vortex, cone projection and
+# continuity init are off (they are outside this path), the metric radius is 71492 km, dt 0.005.
+# make_profile sets the moist convective adjustment and the NH4SH equilibrium per profile.
 BASE_ENV = {
     "ATJUP_GEO_BALANCE": "0", "ATJUP_VORTEX": "0", "ATJUP_MCA": "0",
     "ATJUP_V_MASSBAL_STRIDE": "0", "ATJUP_CONE_PROJ": "0", "ATJUP_CONE_PROJ_STRIDE": "0",
     "ATJUP_U_INIT_CONT": "0", "ATJUP_NH4SH_EQ": "0", "ATJUP_METRIC_RADIUS": "71492",
     "ATJUP_DT": "0.005", "ATJUP_BC_PHI_PERIODIC": "1",
 }
+# the NH4SH heat of formation the equilibrium adjustment and the moist convective adjustment use:
+# the van't Hoff enthalpy of the Lewis K_p, 4705 ln(10) R* / m_nh4sh [J/kg] (ChemistryJup.h:282-283)
+NH4SH_FORMATION_HEAT = 4705.0 * math.log(10.0) * 8314.46 / 51.1114
 
 TURB_MODELS = {"off": -1, "k_epsilon": 0, "k_omega": 1, "k_omega_SST": 2}
 
 
 def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
                  grid: inputs.Grid | None = None, radiation: bool = False, precip: bool = False,
-                 precip_coupling: float = 1.0, chemistry: bool = False, mca: int = 0):
+                 precip_coupling: float = 1.0, chemistry: bool = False, mca: int = 0,
+                 turb: str = "k_omega", nh4sh_eq: bool = False):
     """(Grid, Consts, Knobs, env) of a configuration.
 
     "dyn"  : dynamics only, dry, closure/radiation/precipitation off (the 5-step benchmark case).
@@ -199,7 +204,13 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
              step (the reference always runs them), so massflux_* follow the state.
     mca:     the moist convective adjustment every mca iterations after the step (ATJUP_MCA, the
              reference's default 1); 0 off.
+    turb:    the closure of the "full" profile (ATJUP_TURB_MODEL): "k_omega" (the benchmark case),
+             "k_omega_SST" (the reference's default) or "k_epsilon".
+    nh4sh_eq: the NH4SH equilibrium adjustment after the saturation adjustments (ATJUP_NH4SH_EQ,
+             the reference's default 1), which also zeroes the NH4SH rate law of the chemistry.
     """
+    if config != "dyn" and turb not in ("k_omega", "k_omega_SST", "k_epsilon"):
+        raise ValueError(f"closure {turb!r}")
     g = grid or inputs.Grid()
     G = Grid(im=g.im, jm=g.jm, km=g.km, r0=g.radius_km / g.L_atm, dr=g.dr, dthe=g.dthe,
              dphi=g.dphi, L_atm=g.L_atm)
@@ -208,25 +219,30 @@ def make_profile(config: str = "dyn", sweeps: int = 200, omega: float = 1.0,
                 chemical_reaction=0.0 if config == "dyn" else 1.0,
                 r_mix=inputs.R_MIX_DENSITY, R_mix=inputs.R_MIX, cp_mix=inputs.CP_MIX,
                 mue_mix=inputs.MUE_MIX, sc_h2o=0.61, sc_h2s=0.94, sc_nh3=0.61, sc_nh4sh=0.7,
-                sc_ch4=0.99, rg_nh4sh=1170.0, r_p_nh4sh=1.0e-5, re_turb=1.0)
-    turb = "off" if config == "dyn" else "k_omega"
+                sc_ch4=0.99, rg_nh4sh=1170.0, r_p_nh4sh=1.0e-5,
+                # Turbulence<P>::init sets the model's re_turb = vel_star_ref z_0 / nue_gas of its
+                # default constructor (Turbulence.h:245-251, :286); only the k-epsilon sources read it
+                re_turb=0.4 * 0.1 / 1.8e-5)
+    turb = "off" if config == "dyn" else turb
     K = Knobs(turb_model=TURB_MODELS[turb], press_sweeps=sweeps, press_omega=omega,
               pdyn_cap=0.2, damp_pdyn=1, rho_pdyn=1.0, sinthe_min=0.55, t_diff_scale=0.01,
               species_diff_scale=0.01, rad_coupling=1.0, precip_coupling=precip_coupling, turb_coupling=1.0,
               tke_max_phys=1000.0, nue_max_phys=1.0e5, wall_nue_factor=4.0, wall_nue_layers=3,
               satadj=1, rk_scalar_sync=2, dry=1 if config == "dyn" else 0,
               turb_k_bg_phys=1.0e-4 if config != "dyn" else 1.0, radiation=1 if radiation else 0,
-              precip=1 if precip else 0, chemistry=1 if chemistry else 0, mca=int(mca))
+              precip=1 if precip else 0, chemistry=1 if chemistry else 0, mca=int(mca),
+              nh4sh_eq=1 if nh4sh_eq else 0)
     env = dict(BASE_ENV)
     env["ATJUP_PRESS_SWEEPS"] = str(sweeps)
     env["ATJUP_RADIATION"] = "1" if radiation else "0"
     env["ATJUP_PRECIP"] = "1" if precip else "0"
     env["ATJUP_PRECIP_COUPLING"] = repr(float(precip_coupling))
     env["ATJUP_MCA"] = str(int(mca))
+    env["ATJUP_NH4SH_EQ"] = "1" if nh4sh_eq else "0"
     if config == "dyn":
         env["ATJUP_TURB"] = "0"
     else:
-        env["ATJUP_TURB_MODEL"] = "k_omega"
+        env["ATJUP_TURB_MODEL"] = turb
         env["ATJUP_TURB_K_BG"] = "1e-4"
     return G, Cn, K, env
 

@@ -123,9 +123,6 @@ int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, i
     if(k->turb_model < -1 || k->turb_model > 2){ set_error("turb_model"); return ATJUP_ERR_PROFILE; }
     if(k->rk_scalar_sync != 2){ set_error("only ATJUP_RK_SCALAR_SYNC=2 is implemented"); return ATJUP_ERR_PROFILE; }
     if(k->dry && k->turb_model != -1){ set_error("the dry profile has the closure off"); return ATJUP_ERR_PROFILE; }
-    if(k->turb_model != -1 && k->turb_model != 1){
-        set_error("the closure is built for k-omega (turb_model 1) or off (-1)"); return ATJUP_ERR_PROFILE;
-    }
     if(!(k->pdyn_cap > 0.0)){ set_error("pdyn_cap"); return ATJUP_ERR_PROFILE; }
     if(!(k->press_omega > 0.0 && k->press_omega < 2.0)){ set_error("press_omega outside (0, 2)"); return ATJUP_ERR_PROFILE; }
     if(k->radiation != 0 && k->radiation != 1){ set_error("radiation must be 0 or 1"); return ATJUP_ERR_PROFILE; }
@@ -134,6 +131,10 @@ int validate(const atjup_grid* g, const atjup_consts* c, const atjup_knobs* k, i
     if(k->mca < 0){ set_error("mca (the adjustment's stride) must be >= 0"); return ATJUP_ERR_PROFILE; }
     if(k->mca > 0 && (k->dry || precision == ATJUP_PRECISION_TANGENT)){
         set_error("the moist convective adjustment needs the moist profile (dry = 0)"); return ATJUP_ERR_PROFILE;
+    }
+    if(k->nh4sh_eq != 0 && k->nh4sh_eq != 1){ set_error("nh4sh_eq must be 0 or 1"); return ATJUP_ERR_PROFILE; }
+    if(k->nh4sh_eq && (k->dry || precision == ATJUP_PRECISION_TANGENT)){
+        set_error("the NH4SH equilibrium needs the moist profile (dry = 0)"); return ATJUP_ERR_PROFILE;
     }
     if(k->chemistry && k->dry){
         set_error("the chemistry needs the moist profile (dry = 0)"); return ATJUP_ERR_PROFILE;
@@ -225,6 +226,7 @@ int step_block(atjup_ctx* c, int iter, int first){
         // (cJupiterModel.cpp:1060-1161)
         for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
             if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
+        if(c->kn.nh4sh_eq && (rc = launch_nh4sh_eq(c))) return rc;           // cJupiterModel.cpp:1084
         if(c->kn.precip && (rc = atjup_hip_precipitation(c))) return rc;      // cJupiterModel.cpp:1090
         if((rc = atjup_hip_turbulence_run(c))) return rc;
         if(c->kn.chemistry){
@@ -397,9 +399,12 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     if(!rc){
         atjup_hip_default_chem_params(&c->chemp);
         atjup_hip_default_mca_params(&c->mcap);
+        // with the NH4SH equilibrium the moist convective adjustment carries its heat of formation
+        // (cJupiterModel.cpp:1316)
+        if(kn->nh4sh_eq) c->mcap.l_nh4sh = 4705.0 * std::log(10.0) * 8314.46 / 51.1114;
         rc = alloc(c->report, report_bytes(c->im, c->jm, c->km));
         // the reaction rates persist between chemistry calls; a dry context runs no chemistry
-        if(!rc && !dry) rc = alloc(c->chem_w, 3 * c->n3 * sizeof(double));
+        if(!rc && !dry) rc = alloc(c->chem_w, 3 * c->n3 * c->esz);
     }
     if(rc){ atjup_hip_ctx_destroy(c); return nullptr; }
     return c;
@@ -795,6 +800,13 @@ int atjup_hip_mca(atjup_ctx* c){
         ATJUP_HIP_CHECK(hipMemcpy(c->mca_ws.p, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     return launch_mca(c);
+}
+int atjup_hip_nh4sh_equilibrium(atjup_ctx* c){
+    if(int rc = need_mask(c)) return rc;
+    if(c->precision == ATJUP_PRECISION_TANGENT || c->kn.dry){
+        set_error("the NH4SH equilibrium needs the moist profile"); return ATJUP_ERR_PROFILE;
+    }
+    return launch_nh4sh_eq(c);
 }
 int atjup_hip_turbulence_run(atjup_ctx* c){
     if(int rc = need_mask(c)) return rc;

@@ -173,6 +173,7 @@ int launch_radiation(atjup_ctx* c);
 int launch_precipitation(atjup_ctx* c);
 int launch_chem_kernel(atjup_ctx* c, int which);      // 0: DiffMassFluxJup, 1: ChemMassRateJup
 int launch_mca(atjup_ctx* c);
+int launch_nh4sh_eq(atjup_ctx* c);
 size_t report_bytes(int im, int jm, int km);          // the size of atjup_ctx::report
 int build_rad_table(atjup_ctx* c);
 int launch_flux_limiter(atjup_ctx* c);

@@ -885,7 +885,7 @@ __global__ __launch_bounds__(256) void k_mixture_density(const T* __restrict__ t
 // ---------------------------------------------------------------------------------------------
 // Boundary conditions after the RK4 step
 // ---------------------------------------------------------------------------------------------
-constexpr int MAXBC = 32;
+constexpr int MAXBC = 40;           // the longest list (phi) holds 32 fields in the moist profile
 template<typename T>
 struct BCLists {
     T* radius[MAXBC]; int n_radius;                 // bc_fields_radius (BC_Jup.h:272-290)
@@ -1497,6 +1497,24 @@ BCLists<T> bc_lists(atjup_ctx* c){
         add(L.phi, L.n_phi, ATJUP_F_FLUXLIM_NH4SH);
         add(L.solid, L.n_solid, ATJUP_F_FLUXLIM_NH4SH);
         for(int f : turbf){ add(L.radius, L.n_radius, f); add(L.theta_ex, L.n_theta_ex, f); add(L.phi, L.n_phi, f); }
+        // the chemistry's carried fields, which its kernels write on the interior only: the
+        // diffusive fluxes, thermalmassflux and the reaction rates (held in chem_w, in the fields'
+        // precision); the boundary values these passes leave reach the next call's smoothing
+        for(int f : { ATJUP_F_DIFFLUX_H2S, ATJUP_F_DIFFLUX_NH3, ATJUP_F_DIFFLUX_NH4SH }){
+            add(L.radius, L.n_radius, f); add(L.theta_zero, L.n_theta_zero, f);
+            add(L.phi, L.n_phi, f); add(L.solid, L.n_solid, f);
+        }
+        add(L.radius, L.n_radius, ATJUP_F_THERMALMASSFLUX); add(L.theta_ex, L.n_theta_ex, ATJUP_F_THERMALMASSFLUX);
+        add(L.phi, L.n_phi, ATJUP_F_THERMALMASSFLUX); add(L.solid, L.n_solid, ATJUP_F_THERMALMASSFLUX);
+        if(c->chem_w.p)
+            for(int r = 0; r < 3; r++){
+                T* w = (T*)c->chem_w.p + (size_t)r * c->n3;
+                L.radius_role[L.n_radius] = 0;
+                L.radius[L.n_radius++] = w;
+                L.theta_ex[L.n_theta_ex++] = w;
+                L.phi[L.n_phi++] = w;
+                L.solid[L.n_solid++] = w;
+            }
     }
     L.t = F<T>(c, ATJUP_F_T); L.u = F<T>(c, ATJUP_F_U); L.v = F<T>(c, ATJUP_F_V); L.w = F<T>(c, ATJUP_F_W);
     return L;

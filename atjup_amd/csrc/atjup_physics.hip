@@ -53,6 +53,14 @@ struct StageFullArgs {
     T cr, pc, stokes_a, rg_nh4sh, mue_mix, nue_max, tke_max_nd, dis_min;
     float* lo[4];                          // FP32-refined form: the low words of t, u, v, w (else null)
     double dt_d;
+    // the wall distance of the SST and k-epsilon sources: i_topography, the layer thickness [m] of
+    // each layer and the float layer spacing L_atm * dr [km] the layer heights are multiples of;
+    // the friction velocity and re_turb of the k-epsilon damping
+    const int* itopo;
+    const double* lthick;
+    const double* vel_star;
+    float lh_h;
+    T re_turb;
 };
 
 constexpr int SF_BK = 64, SF_BJ = 2;            // 2 rows: 1.7% less stage time than 4 or 8 on MI355X
@@ -214,7 +222,8 @@ __global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_species(const StageFull
     }
 }
 
-template<typename T>
+// TM: the closure (-1 off, 0 k-epsilon, 1 k-omega, 2 SST), one instantiation each
+template<typename T, int TM>
 __global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(const StageFullArgs<T> a,
         const GeoIJ<T>* __restrict__ geo){
     const int b = xcd_remap(blockIdx.x, a.nb);
@@ -384,7 +393,7 @@ __global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(cons
             d1pD = (db2 - da) * a.inv_2dphi; d2pD = (db2 - T(2) * D.c + da) * a.inv_dphi2;
         }
         T rhs_k = T(0), rhs_d = T(0);
-        if(a.turb){
+        if constexpr(TM >= 0){
             const T nue_here = isfinite(nue_c) ? dmax(T(0), nue_c) : T(0);
             const T tke0 = K.c, dis0 = D.c;
             const T tke_s = dmax(tke0, a.dis_min);
@@ -396,7 +405,7 @@ __global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(cons
             const T dtkedthe_s = d1tK * G.inv_rm, ddisdthe_s = d1tD * G.inv_rm;
             const T dtkedphi_s = d1pK * G.inv_rmsinthe, ddisdphi_s = d1pD * G.inv_rmsinthe;
             const T grad_dot = d1rK * d1rD + dtkedthe_s * ddisdthe_s + dtkedphi_s * ddisdphi_s;
-            T cnue = tke_s / dis_s;
+            T cnue = (TM == 0) ? T(0.028) * tke_s * tke_s / dis_s : tke_s / dis_s;
             cnue = dmin(cnue, a.nue_max);
             const T der = T(0.66667) * (dudr_s + dvdthe_s + dwdphi_s);
             const T P_full = dmax(T(0),
@@ -409,30 +418,83 @@ __global__ __launch_bounds__(SF_BK * SF_BJ, SF_MIN_WAVES) void k_stage_full(cons
                 + (cnue * (T(2) * dwdphi_s - der) - T(0.66667) * tke0) * dwdphi_s
                 + (cnue * (dwdr_s + dudphi_s)) * dwdr_s
                 + (cnue * (dwdthe_s + dvdphi_s)) * dwdthe_s);
-            const T sig_k = T(0.6), sig_w = T(0.5), bet_star = T(0.09), gam = T(0.52), bet_0 = T(0.0708);
-            const T diffusion_tke_re = a.inv_re + sig_k * nue_here;
-            const T diffusion_dis_re = a.inv_re + sig_w * nue_here;
-            const T S11 = dudr_s, S22 = dvdthe_s, S33 = dwdphi_s;
-            const T S12 = T(0.5) * (dudthe_s + dvdr_s);
-            const T S13 = T(0.5) * (dudphi_s + dwdr_s);
-            const T S23 = T(0.5) * (dvdphi_s + dwdthe_s);
-            const T S_mag = sqrt(T(2) * (S11 * S11 + S22 * S22 + S33 * S33
-                                         + T(2) * (S12 * S12 + S13 * S13 + S23 * S23)));
-            const T W12 = dudthe_s - dvdr_s;
-            const T W13 = dudphi_s - dwdr_s;
-            const T W23 = dvdphi_s - dwdthe_s;
-            const T Omega = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
-            const T chi_w = fabs(Omega * Omega * S_mag / pow(bet_star * dis_s, T(3)));
-            const T f_bet = (T(1) + T(85) * chi_w) / (T(1) + T(100) * chi_w);
-            const T bet_wc = bet_0 * f_bet;
-            const T sig_d = (grad_dot <= T(0)) ? T(0) : T(0.125);
-            const T P_k = dmin(P_full, T(20) * bet_star * tke_s * dis_s);
-            const T Y_k = bet_star * tke_s * dis_s;
-            const T P_w = gam * dis_s / tke_s * P_k;
-            const T Y_w = bet_wc * dis_s * dis_s;
-            const T D_w = sig_d / dis_s * grad_dot;
-            T tke_src = P_k - Y_k;
-            T dis_src = P_w - Y_w + D_w;
+            T diffusion_tke_re, diffusion_dis_re, tke_src, dis_src;
+            if constexpr(TM == 1){
+                // k-omega, Wilcox 1988/2006
+                const T sig_k = T(0.6), sig_w = T(0.5), bet_star = T(0.09), gam = T(0.52), bet_0 = T(0.0708);
+                diffusion_tke_re = a.inv_re + sig_k * nue_here;
+                diffusion_dis_re = a.inv_re + sig_w * nue_here;
+                const T S11 = dudr_s, S22 = dvdthe_s, S33 = dwdphi_s;
+                const T S12 = T(0.5) * (dudthe_s + dvdr_s);
+                const T S13 = T(0.5) * (dudphi_s + dwdr_s);
+                const T S23 = T(0.5) * (dvdphi_s + dwdthe_s);
+                const T S_mag = sqrt(T(2) * (S11 * S11 + S22 * S22 + S33 * S33
+                                             + T(2) * (S12 * S12 + S13 * S13 + S23 * S23)));
+                const T W12 = dudthe_s - dvdr_s;
+                const T W13 = dudphi_s - dwdr_s;
+                const T W23 = dvdphi_s - dwdthe_s;
+                const T Omega = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
+                const T chi_w = fabs(Omega * Omega * S_mag / pow(bet_star * dis_s, T(3)));
+                const T f_bet = (T(1) + T(85) * chi_w) / (T(1) + T(100) * chi_w);
+                const T bet_wc = bet_0 * f_bet;
+                const T sig_d = (grad_dot <= T(0)) ? T(0) : T(0.125);
+                const T P_k = dmin(P_full, T(20) * bet_star * tke_s * dis_s);
+                const T Y_k = bet_star * tke_s * dis_s;
+                const T P_w = gam * dis_s / tke_s * P_k;
+                const T Y_w = bet_wc * dis_s * dis_s;
+                const T D_w = sig_d / dis_s * grad_dot;
+                tke_src = P_k - Y_k;
+                dis_src = P_w - Y_w + D_w;
+            } else {
+                // the wall distance above the local obstacle surface, floored at one layer: the
+                // layer heights are the float multiples i * (L_atm dr) of the reference
+                const T L_m = a.L_rad;
+                const double y_mount = (double)((float)a.itopo[(size_t)j * a.km + k] * a.lh_h) * 1.0e3;
+                const double dz_layer = dmax(1.0, a.lthick[i >= 1 ? i - 1 : 0]);
+                const T y_phys = (T)dmax((double)((float)i * a.lh_h) * 1.0e3 - y_mount, dz_layer);
+                const T y_star = y_phys / L_m;
+                if constexpr(TM == 0){
+                    // k-epsilon, Chien 1982, with its near-wall damping
+                    const T sig_k = T(1.0), sig_w = T(1.3), C_eps_1 = T(1.35), C_eps_2 = T(1.80);
+                    const T nue_gas_phys = T(1.8e-5);
+                    diffusion_tke_re = a.inv_re + nue_here / sig_k;
+                    diffusion_dis_re = a.inv_re + nue_here / sig_w;
+                    const T d_plus = y_phys * (T)a.vel_star[(size_t)j * a.km + k] / nue_gas_phys;
+                    const T Re_T = tke_s * tke_s * a.u_0 * L_m / (dis_s * nue_gas_phys);
+                    const T f_2 = T(1) - T(0.4) / T(1.8) * exp(-Re_T * Re_T / T(36));
+                    const T y_star2 = y_star * y_star;
+                    const T L_k = T(-2) * tke0 / y_star2;
+                    const T L_w = T(-2) * dis0 / y_star2 * exp(T(-0.5) * d_plus);
+                    const T P_k = P_full;
+                    const T Y_k = dis0;
+                    const T P_w = C_eps_1 * dis0 / tke_s * P_k;
+                    const T Y_w = C_eps_2 * f_2 * dis0 * dis0 / tke_s;
+                    tke_src = P_k - Y_k + L_k / a.re_turb;
+                    dis_src = P_w - Y_w + L_w / a.re_turb;
+                } else {
+                    // k-omega SST, Menter 1994, with its F1 blending of the coefficients
+                    const T bet_star = T(0.09), sig_k1 = T(1.176), sig_k2 = T(1.0), sig_w1 = T(2.0);
+                    const T sig_w2 = T(1.168), bet1 = T(0.0333), bet2 = T(0.0368), gam1 = T(0.413), gam2 = T(0.2);
+                    const T nue_air_nd = T(1.8e-5) / (a.u_0 * L_m);
+                    const T CD_kw = dmax(T(2) * sig_w2 / dis_s * grad_dot, T(1.0e-20));
+                    const T arg1 = dmin(
+                        dmax(sqrt(dmax(tke0, T(0))) / (bet_star * dis_s * y_star),
+                             T(500) * nue_air_nd / (y_star * y_star * dis_s)),
+                        T(4) * sig_w2 * tke_s / (CD_kw * y_star * y_star));
+                    const T F1 = tanh(pow(arg1, T(4)));
+                    const T sig_k = F1 * sig_k1 + (T(1) - F1) * sig_k2;
+                    const T sig_w = F1 * sig_w1 + (T(1) - F1) * sig_w2;
+                    diffusion_tke_re = a.inv_re + nue_here / sig_k;
+                    diffusion_dis_re = a.inv_re + nue_here / sig_w;
+                    const T P_k = dmin(P_full, T(20) * bet_star * tke_s * dis_s);
+                    const T Y_k = bet_star * tke_s * dis_s;
+                    const T P_w = (F1 * gam1 + (T(1) - F1) * gam2) * P_k * dis_s / tke_s;
+                    const T Y_w = (F1 * bet1 + (T(1) - F1) * bet2) * dis_s * dis_s;
+                    const T D_w = T(2) * (T(1) - F1) * sig_w2 / dis_s * grad_dot;
+                    tke_src = P_k - Y_k;
+                    dis_src = P_w - Y_w + D_w;
+                }
+            }
             const T tke_src_max = T(20) * tke_s * dis_s;
             const T dis_src_max = T(20) * dis_s * dis_s;
             if(!isfinite(tke_src)) tke_src = T(0);
@@ -481,15 +543,19 @@ __global__ __launch_bounds__(256) void k_turb_clamp_in(const T* __restrict__ tke
 
 struct TurbConsts {
     double dr, dthe, dphi, nue_max, dis_min;
+    double u_0, L_m;           // velocity scale [m/s], shell depth [m]
+    double re_turb;            // the closure's vel_star_ref z_0 / nue_gas (Turbulence.h:245-251)
 };
 
-// compute_sources + compute_k_omega (Turbulence.h:633-1012): nue, prod, tke_source, dis_source of
-// the interior fluid cells from the clamped k*, omega* and the velocity field
-template<typename T>
+// compute_sources + compute_k_omega / compute_k_epsilon / compute_k_omega_SST (Turbulence.h:633-1106),
+// TM the model (0 k-epsilon, 1 k-omega, 2 SST): nue, prod, tke_source, dis_source of the interior
+// fluid cells from the clamped k*, omega* and the velocity field. With TURB_FREE the wall distance
+// of k-epsilon and SST is free_wall_dist() = 1e20 m.
+template<typename T, int TM>
 __global__ __launch_bounds__(256) void k_turb_sources(const T* __restrict__ tke, const T* __restrict__ dis,
         const T* __restrict__ U, const T* __restrict__ V, const T* __restrict__ W,
         T* __restrict__ nue, T* __restrict__ prod, T* __restrict__ tsrc, T* __restrict__ dsrc,
-        const uint8_t* __restrict__ flags, const double* __restrict__ tabs,
+        const uint8_t* __restrict__ flags, const double* __restrict__ tabs, const double* __restrict__ vel_star,
         int im, int jm, int km, int pitch, size_t plane, const TurbConsts tc){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1, i = blockIdx.z + 1;
     if(k > km - 2 || j > jm - 2 || i > im - 2) return;
@@ -550,7 +616,7 @@ __global__ __launch_bounds__(256) void k_turb_sources(const T* __restrict__ tke,
     {
         const T dis_here = dmax(dq, (T)tc.dis_min);
         const T tke_here = dmax(tq, T(0));
-        cnue = tke_here / dis_here;
+        cnue = (TM == 0) ? T(0.028) * tke_here * tke_here / dis_here : tke_here / dis_here;
         cnue = dmin(cnue, (T)tc.nue_max);
     }
     const T u_r = Uq * inv_rm, v_r = Vq * inv_rm, w_r = Wq * inv_rm;
@@ -572,7 +638,33 @@ __global__ __launch_bounds__(256) void k_turb_sources(const T* __restrict__ tke,
     prod[q] = pr;
     const T W12 = g_rt - g_tr, W13 = g_rp - g_pr, W23 = g_tp - g_pt;
     const T Omega_mag = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
-    // compute_k_omega (Turbulence.h:920-1012); exp_rm = 1 / (rm + 1) in its derivatives
+    if constexpr(TM == 0){
+        // compute_k_epsilon (Turbulence.h:886-918), Chien's damping in its free-flow form
+        const T C_nue = T(0.028), C_eps_1 = T(1.35), C_eps_2 = T(1.80), nue_air = T(1.8e-5);
+        const T dis_min = (T)tc.dis_min, L_m = (T)tc.L_m;
+        const T y_phys = T(1.0e20);
+        const T y_star = y_phys / L_m;
+        T nu = C_nue * tq * tq / dmax(dq, dis_min);
+        const T d_plus = y_phys * (T)vel_star[(size_t)j * km + k] / nue_air;
+        const T f_nue = T(1) - exp(T(-0.0115) * d_plus);
+        const T Re_T = tq * tq * (T)tc.u_0 * L_m / dmax(dq * nue_air, dis_min * nue_air);
+        const T f_2 = T(1) - T(0.4) / T(1.8) * exp(-Re_T * Re_T / T(36));
+        nu *= f_nue;
+        nue[q] = nu;
+        const T y_star2 = y_star * y_star;
+        const T L_k = T(-2) * tq / y_star2;
+        const T L_w = T(-2) * dq / y_star2 * exp(-d_plus / T(2));
+        const T P_k = pr;
+        const T Y_k = dq;
+        const T tke_safe = dmax(tq, dis_min);
+        const T P_w = C_eps_1 * dq / tke_safe * P_k;
+        const T Y_w = C_eps_2 * f_2 * dq * dq / tke_safe;
+        tsrc[q] = P_k - Y_k + L_k / (T)tc.re_turb;
+        dsrc[q] = P_w - Y_w + L_w / (T)tc.re_turb;
+        return;
+    }
+    // compute_k_omega (Turbulence.h:920-1012); exp_rm = 1 / (rm + 1) in its derivatives, which
+    // compute_k_omega_SST shares
     const T bet_star = T(0.09), gam = T(0.52), C_lim = T(0.875), bet_0 = T(0.0708);
     const T erm = T(1) / (rm + T(1));
     const T dtkedr_neu = nb ? (tke_ip - tq) * (T(1) / (T(2) * dr)) * erm : dtkedr;
@@ -581,6 +673,34 @@ __global__ __launch_bounds__(256) void k_turb_sources(const T* __restrict__ tke,
     const T ddisdthe_neu = (dis_jp1 - dis_jm1) / (rm * T(2) * dthe);
     const T dtkedphi_neu = (tke_kp1 - tke_km1) / (rmsinthe * T(2) * dphi);
     const T ddisdphi_neu = (dis_kp1 - dis_km1) / (rmsinthe * T(2) * dphi);
+    if constexpr(TM == 2){
+        // compute_k_omega_SST (Turbulence.h:1021-1106), Menter 1994
+        const T a1 = T(0.31), bet1 = T(0.0333), bet2 = T(0.0368), gam1 = T(0.413), gam2 = T(0.2);
+        const T sig_w2 = T(1.168);
+        const T L_m = (T)tc.L_m;
+        const T nue_air_nd = T(1.8e-5) / ((T)tc.u_0 * L_m);
+        const T y_star = T(1.0e20) / L_m;
+        const T grad_dot = dtkedr_neu * ddisdr_neu + dtkedthe_neu * ddisdthe_neu + dtkedphi_neu * ddisdphi_neu;
+        const T CD_kw = dmax(T(2) * sig_w2 / dmax(dq, T(1.0e-20)) * grad_dot, T(1.0e-20));
+        const T dis_safe = dmax(dq, T(1.0e-20));
+        const T arg1 = dmin(
+            dmax(sqrt(tq) / (bet_star * dis_safe * y_star), T(500) * nue_air_nd / (y_star * y_star * dis_safe)),
+            T(4) * sig_w2 * tq / (CD_kw * y_star * y_star));
+        const T arg2 = dmax(T(2) * sqrt(tq) / (bet_star * dis_safe * y_star),
+                            T(500) * nue_air_nd / (y_star * y_star * dis_safe));
+        const T F1 = tanh(pow(arg1, T(4)));
+        const T F2 = tanh(pow(arg2, T(2)));
+        nue[q] = a1 * tq / dmax(a1 * dis_safe, Omega_mag * F2);
+        const T tke_safe = dmax(tq, T(1.0e-20));
+        const T P_k = dmin(pr, T(20) * bet_star * tke_safe * dis_safe);
+        const T Y_k = bet_star * tke_safe * dis_safe;
+        const T P_w = (F1 * gam1 + (T(1) - F1) * gam2) * P_k * dis_safe / tke_safe;
+        const T Y_w = (F1 * bet1 + (T(1) - F1) * bet2) * dis_safe * dis_safe;
+        const T D_w = T(2) * (T(1) - F1) * sig_w2 / dis_safe * grad_dot;
+        tsrc[q] = P_k - Y_k;
+        dsrc[q] = P_w - Y_w + D_w;
+        return;
+    }
     const T dudr_k = (Uip - Uim) / (T(2) * dr) / (rm + T(1));
     const T dvdthe_k = (Vjp - Vjm) / (rm * T(2) * dthe);
     const T dwdphi_k = (Wkp - Wkm) / (rmsinthe * T(2) * dphi);
@@ -1002,6 +1122,11 @@ int do_rk4_full(atjup_ctx* c){
     a.nkb = (c->km + SF_BK - 1) / SF_BK; a.njb = (c->jm + SF_BJ - 1) / SF_BJ;
     a.nb = a.nkb * a.njb * c->im;
     a.turb = K.turb_model >= 0 ? 1 : 0;
+    a.itopo = (const int*)c->itopo.p;
+    a.lthick = (const double*)c->tabs.p + c->im + 3 * c->jm;
+    a.vel_star = (const double*)c->vel_star.p;
+    a.lh_h = (float)(c->g.L_atm * c->g.dr);
+    a.re_turb = (T)C.re_turb;
     a.dt = (T)dt; a.one_sixth = (T)(1.0 / 6.0);
     a.dt_d = dt;
     for(int f = 0; f < 4; f++) a.lo[f] = (float*)c->lo[f].p;
@@ -1058,7 +1183,13 @@ int do_rk4_full(atjup_ctx* c){
             a.bsrc[f] = (s <= 1) ? in : ((s == 3 && pend && turbf) ? L : nullptr);
         }
         hipEvent_t t0 = timer_begin(c);
-        hipLaunchKernelGGL(k_stage_full<T>, dim3(a.nb), dim3(SF_BK, SF_BJ), 0, c->stream, a, a.geo);
+        const dim3 gs(a.nb), bs(SF_BK, SF_BJ);
+        switch(K.turb_model){
+            case 0:  hipLaunchKernelGGL((k_stage_full<T, 0>), gs, bs, 0, c->stream, a, a.geo); break;
+            case 1:  hipLaunchKernelGGL((k_stage_full<T, 1>), gs, bs, 0, c->stream, a, a.geo); break;
+            case 2:  hipLaunchKernelGGL((k_stage_full<T, 2>), gs, bs, 0, c->stream, a, a.geo); break;
+            default: hipLaunchKernelGGL((k_stage_full<T, -1>), gs, bs, 0, c->stream, a, a.geo); break;
+        }
         hipLaunchKernelGGL(k_stage_species<T>, dim3(a.nb), dim3(SF_BK, SF_BJ), 0, c->stream, a, a.geo);
         timer_end(c, ATJUP_T_RK_STAGES, t0, 2);
         if(int rc = check_launch()) return rc;
@@ -1097,11 +1228,20 @@ int do_turbulence(atjup_ctx* c){
     hipLaunchKernelGGL(k_turb_clamp_in<T>, gfull, b128, 0, c->stream, tke, dis, ltke, ldis,
         (const uint8_t*)c->flags.p, c->im, c->jm, c->km, c->pitch, (T)dis_min);
     c->turb_live_pending = true;
-    TurbConsts tc{ c->g.dr, c->g.dthe, c->g.dphi, nue_max, dis_min };
-    hipLaunchKernelGGL(k_turb_sources<T>, dim3(nblk(c->km - 1, 128), c->jm - 2, c->im - 2), b128, 0, c->stream,
-        ltke, ldis, F<T>(c, ATJUP_F_U), F<T>(c, ATJUP_F_V), F<T>(c, ATJUP_F_W),
-        F<T>(c, ATJUP_F_NUE), F<T>(c, ATJUP_F_PROD), F<T>(c, ATJUP_F_TKE_SOURCE), F<T>(c, ATJUP_F_DIS_SOURCE),
-        (const uint8_t*)c->flags.p, (const double*)c->tabs.p, c->im, c->jm, c->km, c->pitch, c->plane, tc);
+    TurbConsts tc{ c->g.dr, c->g.dthe, c->g.dphi, nue_max, dis_min, C.u_0, L_m, 0.4 * 0.1 / 1.8e-5 };
+    {
+        const dim3 gs(nblk(c->km - 1, 128), c->jm - 2, c->im - 2);
+        auto launch = [&](auto kern){
+            hipLaunchKernelGGL(kern, gs, b128, 0, c->stream,
+                (const T*)ltke, (const T*)ldis, (const T*)F<T>(c, ATJUP_F_U), (const T*)F<T>(c, ATJUP_F_V),
+                (const T*)F<T>(c, ATJUP_F_W), F<T>(c, ATJUP_F_NUE), F<T>(c, ATJUP_F_PROD),
+                F<T>(c, ATJUP_F_TKE_SOURCE), F<T>(c, ATJUP_F_DIS_SOURCE), (const uint8_t*)c->flags.p,
+                (const double*)c->tabs.p, (const double*)c->vel_star.p, c->im, c->jm, c->km, c->pitch, c->plane, tc);
+        };
+        if(c->kn.turb_model == 0) launch(k_turb_sources<T, 0>);
+        else if(c->kn.turb_model == 2) launch(k_turb_sources<T, 2>);
+        else launch(k_turb_sources<T, 1>);
+    }
     const TurbArrays<T> L = turb_arrays<T>(c, ltke, ldis);
     hipLaunchKernelGGL(k_turb_wall_plane<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream,
         L.a[0], L.a[1], L.a[2], L.a[3], L.a[4], L.a[5], (const uint8_t*)c->flags.p,
@@ -1115,9 +1255,10 @@ int do_turbulence(atjup_ctx* c){
     hipLaunchKernelGGL(k_turb_pole_rows<T>, dim3(nblk(c->km, 128), c->im), b128, 0, c->stream, L, c->im, c->jm, c->km,
         c->pitch);
     const double* tabs = (const double*)c->tabs.p;
-    hipLaunchKernelGGL(k_turb_wall_omega<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream, ldis,
-        (const int*)c->itopo.p, (const double*)c->vel_star.p, tabs + c->im + 3 * c->jm,
-        c->im, c->jm, c->km, c->pitch, L_m / C.u_0);
+    if(c->kn.turb_model != 0)            // the wall function leaves k-epsilon alone (cJupiterModel.cpp:356-360)
+        hipLaunchKernelGGL(k_turb_wall_omega<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream, ldis,
+            (const int*)c->itopo.p, (const double*)c->vel_star.p, tabs + c->im + 3 * c->jm,
+            c->im, c->jm, c->km, c->pitch, L_m / C.u_0);
     return check_launch();
 }
 
@@ -1131,9 +1272,10 @@ int do_turb_wall(atjup_ctx* c){
         L.a[0], L.a[1], L.a[2], L.a[3], L.a[4], L.a[5], (const uint8_t*)c->flags.p,
         c->im, c->jm, c->km, c->pitch, c->plane, T(0), T(0), T(0), 0);
     const double* tabs = (const double*)c->tabs.p;
-    hipLaunchKernelGGL(k_turb_wall_omega<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream, dis,
-        (const int*)c->itopo.p, (const double*)c->vel_star.p, tabs + c->im + 3 * c->jm,
-        c->im, c->jm, c->km, c->pitch, c->g.L_atm * 1.0e3 / c->c.u_0);
+    if(c->kn.turb_model != 0)
+        hipLaunchKernelGGL(k_turb_wall_omega<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream, dis,
+            (const int*)c->itopo.p, (const double*)c->vel_star.p, tabs + c->im + 3 * c->jm,
+            c->im, c->jm, c->km, c->pitch, c->g.L_atm * 1.0e3 / c->c.u_0);
     return check_launch();
 }
 
@@ -1167,6 +1309,76 @@ int do_flux_limiter(atjup_ctx* c){
         F<T>(c, ATJUP_F_NH4SH), F<T>(c, ATJUP_F_U), F<T>(c, ATJUP_F_V), F<T>(c, ATJUP_F_W),
         F<T>(c, ATJUP_F_FLUXLIM_NH4SH), (const double*)c->tabs.p, c->im, c->jm, c->km, c->pitch, c->plane,
         c->g.dr, c->g.dthe, c->g.dphi, 4.0 / 3.0, 1.0 / 3.0);
+    return check_launch();
+}
+
+// =============================================================================================
+// ChemistryJup::EquilibriumNH4SH("loop") (ChemistryJup.h:293-370): every fluid cell brought to the
+// Lewis equilibrium p_NH3 p_H2S = K_p(T) = 10^(14.82 - 4705 / T) atm^2, the heat of formation
+// moving T with the extent x [kmol/m3] of NH4SH formed (x > 0) or decomposed (x < 0, at most all of
+// it); x by bisection (at most 80 halvings, to 1e-15 of the largest of a, b, s) on the monotone
+// f(x) = (a - x)(b - x)(R* T(x))^2 - K_p(T(x)). One thread per cell (the cells are independent); the
+// arithmetic is FP64 in both forms, the FP32-refined form's t read and written as hi + lo.
+// =============================================================================================
+struct EqConsts { double m_nh3, m_h2s, m_nh4sh, t_ref, r_mix, cp_mix, L_f; };
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_nh4sh_eq(T* __restrict__ t, float* __restrict__ lo_t, T* __restrict__ nh3,
+        T* __restrict__ h2s, T* __restrict__ nh4sh, const T* __restrict__ rho_mix, const uint8_t* __restrict__ msk,
+        size_t n, int km, int pitch, const EqConsts C){
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= n || (int)(q % (size_t)pitch) >= km || msk[q]) return;
+    const double R_univ = 8314.46, atm = 101325.0;
+    double tq = (double)t[q];
+    if(lo_t) tq += (double)lo_t[q];
+    const double T0 = tq * C.t_ref;
+    if(!(T0 > 0.0)) return;
+    double rho = (double)rho_mix[q];
+    if(!(rho > 0.0) || !isfinite(rho)) rho = C.r_mix;
+    const double nh3_0 = (double)nh3[q], h2s_0 = (double)h2s[q], nh4sh_0 = (double)nh4sh[q];
+    const double a = dmax(0.0, nh3_0) / C.m_nh3;
+    const double b = dmax(0.0, h2s_0) / C.m_h2s;
+    const double s = dmax(0.0, nh4sh_0) / C.m_nh4sh;
+    const double heat = C.L_f * C.m_nh4sh / (C.cp_mix * rho);
+    auto f = [&](double x){
+        const double Tx = T0 + heat * x, RT = R_univ * Tx;
+        return (a - x) * (b - x) * RT * RT - pow(10.0, 14.82 - 4705.0 / Tx) * atm * atm;
+    };
+    const double f0 = f(0.0);
+    double lo, hi;
+    if(f0 > 0.0){ lo = 0.0; hi = dmin(a, b); }
+    else if(s > 0.0){
+        if(f(-s) <= 0.0){ lo = -s; hi = -s; }
+        else { lo = -s; hi = 0.0; }
+    } else return;
+    if(hi > lo){
+        const double tol = 1.0e-15 * dmax(a, dmax(b, s));
+        for(int it = 0; it < 80; it++){
+            const double xm = 0.5 * (lo + hi);
+            if(f(xm) > 0.0) lo = xm; else hi = xm;
+            if(hi - lo <= tol) break;
+        }
+    }
+    const double x = (f0 > 0.0) ? lo : hi;
+    if(x == 0.0) return;
+    nh3[q] = (T)dmax(0.0, nh3_0 - C.m_nh3 * x);
+    h2s[q] = (T)dmax(0.0, h2s_0 - C.m_h2s * x);
+    nh4sh[q] = (T)dmax(0.0, nh4sh_0 + C.m_nh4sh * x);
+    const double tn = (T0 + heat * x) / C.t_ref;
+    const T th = (T)tn;
+    t[q] = th;
+    if(lo_t) lo_t[q] = (float)(tn - (double)th);
+}
+
+template<typename T>
+int do_nh4sh_eq(atjup_ctx* c){
+    const atjup_chem_params& p = c->chemp;
+    const EqConsts C{ p.m_nh3, p.m_h2s, p.m_nh4sh, c->c.t_ref, c->c.r_mix, c->c.cp_mix,
+                      4705.0 * std::log(10.0) * 8314.46 / 51.1114 };
+    hipLaunchKernelGGL(k_nh4sh_eq<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream,
+        F<T>(c, ATJUP_F_T), (float*)c->lo[ATJUP_F_T].p, F<T>(c, ATJUP_F_NH3), F<T>(c, ATJUP_F_H2S),
+        F<T>(c, ATJUP_F_NH4SH), (const T*)F<T>(c, ATJUP_F_RHO_MIX), (const uint8_t*)c->mask.p,
+        c->n3, c->km, c->pitch, C);
     return check_launch();
 }
 
@@ -1567,7 +1779,7 @@ struct ChemConsts {
     double m_nh3, m_h2s, m_nh4sh, cp_nh3, cp_h2s, cp_nh4sh, k_nh3, k_h2s, k_nh4sh, dh_nh4sh;
     double r_mix, t_ref, keq, t_0, t_00;
     double dr_2, dthe_2, dphi_2, dr2, dthe2, dphi2;
-    int gate_zero, molar_conc, enthalpy;
+    int gate_zero, molar_conc, enthalpy, eq_on;
 };
 constexpr double CHEM_FLOOR = 0.4;                 // polar_divisor_floor, SINTHE_TRACK 0 (ATPhys.h:104-108)
 
@@ -1576,7 +1788,8 @@ struct ChemFields {
     const T *t, *h2s, *nh3, *nh4sh, *rho;
     const float* lo_t;
     T *dfl_h2s, *dfl_nh3, *dfl_nh4sh, *tmf, *mf_h2s, *mf_nh3, *mf_nh4sh;
-    double *w_nh3, *w_h2s, *w_nh4sh;
+    T *w_nh3, *w_h2s, *w_nh4sh;           // the reaction rates, in the fields' precision (the
+                                          // boundary passes treat them with the fields)
 };
 
 // derivative_1_order (ChemistryJup.h:653-727) at a fluid cell: central unless both neighbours of
@@ -1681,7 +1894,7 @@ __global__ __launch_bounds__(64) void k_chemmassrate(const ChemFields<T> F, cons
     const size_t P = (size_t)jm * pitch, R = (size_t)pitch;
     const size_t q = ((size_t)i * jm + j) * pitch + k;
     if(j < 3 || j > jm - 4){
-        F.w_nh3[q] = 0.0; F.w_h2s[q] = 0.0; F.w_nh4sh[q] = 0.0;
+        F.w_nh3[q] = T(0); F.w_h2s[q] = T(0); F.w_nh4sh[q] = T(0);
         F.mf_nh3[q] = T(0); F.mf_h2s[q] = T(0); F.mf_nh4sh[q] = T(0);
         return;
     }
@@ -1689,7 +1902,7 @@ __global__ __launch_bounds__(64) void k_chemmassrate(const ChemFields<T> F, cons
     const size_t row = q - k;
     const int kw = (k == 0) ? km - 2 : k - 1, ke = (k == km - 2) ? 0 : k + 1;
     if(msk[q] && msk[q - P] && msk[q + P] && msk[q - R] && msk[q + R] && msk[row + kw] && msk[row + ke]){
-        F.w_nh3[q] = 0.0; F.w_h2s[q] = 0.0; F.w_nh4sh[q] = 0.0;
+        F.w_nh3[q] = T(0); F.w_h2s[q] = T(0); F.w_nh4sh[q] = T(0);
         F.mf_nh3[q] = T(0); F.mf_h2s[q] = T(0); F.mf_nh4sh[q] = T(0);
         return;
     }
@@ -1699,7 +1912,8 @@ __global__ __launch_bounds__(64) void k_chemmassrate(const ChemFields<T> F, cons
     const double kf = 15000.0 * pow(t_u, -0.5) * exp(-3020.0 / t_u);        // react_rate_const (:648-651)
     const double kb = kf / C.keq;
     double w_nh3 = F.w_nh3[q], w_h2s = F.w_h2s[q], w_nh4sh = F.w_nh4sh[q];
-    if(t_u <= C.t_0 && t_u >= C.t_00){
+    // with the NH4SH equilibrium adjustment the rate law contributes nothing (ChemistryJup.h:133-135)
+    if(!C.eq_on && t_u <= C.t_0 && t_u >= C.t_00){
         const double nh3 = (double)F.nh3[q], h2s = (double)F.h2s[q], nh4sh = (double)F.nh4sh[q];
         double denom;
         if(C.molar_conc){
@@ -1715,10 +1929,10 @@ __global__ __launch_bounds__(64) void k_chemmassrate(const ChemFields<T> F, cons
         w_nh3 = -C.m_nh3 * R_diff;
         w_h2s = -C.m_h2s * R_diff;
         w_nh4sh = C.m_nh4sh * R_diff;
-    } else if(C.gate_zero){
+    } else if(C.gate_zero || C.eq_on){
         w_nh3 = 0.0; w_h2s = 0.0; w_nh4sh = 0.0;
     }
-    F.w_nh3[q] = w_nh3; F.w_h2s[q] = w_h2s; F.w_nh4sh[q] = w_nh4sh;
+    F.w_nh3[q] = (T)w_nh3; F.w_h2s[q] = (T)w_h2s; F.w_nh4sh[q] = (T)w_nh4sh;
     F.mf_h2s[q] = (T)(w_h2s + (double)F.dfl_h2s[q]);
     F.mf_nh3[q] = (T)(w_nh3 + (double)F.dfl_nh3[q]);
     F.mf_nh4sh[q] = (T)(w_nh4sh + (double)F.dfl_nh4sh[q]);
@@ -1735,7 +1949,7 @@ int do_chem_kernels(atjup_ctx* c, int which){
     Fp.dfl_nh4sh = F<T>(c, ATJUP_F_DIFFLUX_NH4SH); Fp.tmf = F<T>(c, ATJUP_F_THERMALMASSFLUX);
     Fp.mf_h2s = F<T>(c, ATJUP_F_MASSFLUX_H2S); Fp.mf_nh3 = F<T>(c, ATJUP_F_MASSFLUX_NH3);
     Fp.mf_nh4sh = F<T>(c, ATJUP_F_MASSFLUX_NH4SH);
-    Fp.w_nh3 = (double*)c->chem_w.p; Fp.w_h2s = Fp.w_nh3 + c->n3; Fp.w_nh4sh = Fp.w_h2s + c->n3;
+    Fp.w_nh3 = (T*)c->chem_w.p; Fp.w_h2s = Fp.w_nh3 + c->n3; Fp.w_nh4sh = Fp.w_h2s + c->n3;
     // the constants of DiffMassFluxJup (:383-400) and ChemMassRateJup (:66-69)
     ChemConsts C;
     const double DT_nh3 = p.mue_nh3 / (p.rg_nh3 * c->c.sc_nh3);
@@ -1755,6 +1969,7 @@ int do_chem_kernels(atjup_ctx* c, int which){
     C.dr_2 = 2.0 * c->g.dr; C.dthe_2 = 2.0 * c->g.dthe; C.dphi_2 = 2.0 * c->g.dphi;
     C.dr2 = c->g.dr * c->g.dr; C.dthe2 = c->g.dthe * c->g.dthe; C.dphi2 = c->g.dphi * c->g.dphi;
     C.gate_zero = p.gate_zero; C.molar_conc = p.molar_conc; C.enthalpy = p.enthalpy;
+    C.eq_on = c->kn.nh4sh_eq != 0;
     hipEvent_t t0 = timer_begin(c);
     if(which == 0)
         hipLaunchKernelGGL(k_diffmassflux<T>, dim3(nblk(c->km - 1, 64), c->jm - 2, c->im - 2), dim3(64), 0, c->stream,
@@ -2220,6 +2435,7 @@ int launch_radiation(atjup_ctx* c){ return DISPATCH(do_radiation, c); }
 int launch_precipitation(atjup_ctx* c){ return DISPATCH(do_precipitation, c); }
 int launch_chem_kernel(atjup_ctx* c, int which){ return DISPATCH(do_chem_kernels, c, which); }
 int launch_mca(atjup_ctx* c){ return DISPATCH(do_mca, c); }
+int launch_nh4sh_eq(atjup_ctx* c){ return DISPATCH(do_nh4sh_eq, c); }
 int launch_commit_live(atjup_ctx* c){ return c->turb_live_pending ? DISPATCH(do_commit_live, c) : 0; }
 
 }  // namespace atjup

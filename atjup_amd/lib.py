@@ -51,7 +51,7 @@ def load():
               "atjup_hip_mixture_density", "atjup_hip_flux_limiter", "atjup_hip_clamp_species",
               "atjup_hip_swap_nlevel", "atjup_hip_scalar_sync", "atjup_hip_check_finite",
               "atjup_hip_sync", "atjup_hip_pitch", "atjup_hip_radiation", "atjup_hip_precipitation",
-              "atjup_hip_chemistry", "atjup_hip_mca"):
+              "atjup_hip_chemistry", "atjup_hip_mca", "atjup_hip_nh4sh_equilibrium"):
         getattr(L, n).argtypes = [C.c_void_p]
     L.atjup_hip_device_count.argtypes = [C.POINTER(C.c_int)]
     for n in ("atjup_hip_upload_tangent", "atjup_hip_adj_upload"):
@@ -231,6 +231,10 @@ class Context:
 
     def mca(self):
         self._chk(self.L.atjup_hip_mca(self.h), "mca")
+
+    def nh4sh_equilibrium(self):
+        """ChemistryJup::EquilibriumNH4SH("loop") on every fluid cell."""
+        self._chk(self.L.atjup_hip_nh4sh_equilibrium(self.h), "nh4sh_equilibrium")
 
     def search_minmax_3d(self, name):
         """(max, (i, j, k), min, (i, j, k)) of a field, as Reporting::searchMinMax_3D finds them."""

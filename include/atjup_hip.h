@@ -120,14 +120,19 @@ typedef struct atjup_knobs {
     int    mca;                 /* ATJUP_MCA=<N>: the moist convective adjustment every N iterations,
                                    after the closure's wall hold (cJupiterModel.cpp:115-125, :1301-1322;
                                    the reference's default is 1); 0 off */
+    int    nh4sh_eq;            /* ATJUP_NH4SH_EQ: the NH4SH equilibrium adjustment after the saturation
+                                   adjustments (ChemistryJup::EquilibriumNH4SH, ChemistryJup.h:285-370,
+                                   cJupiterModel.cpp:1084; the reference's default is 1): the chemistry's
+                                   rate law then contributes nothing, and the moist convective
+                                   adjustment carries the NH4SH heat of formation */
 } atjup_knobs;
 
 /* The chemistry's constants as the reference's start-up leaves them (planet/cJupiterModel.h:172-401:
  * molar masses [kg/kmol], mole fractions, dynamic viscosities, densities, the mixture conductivity,
  * specific heats, conductivities, the NH4SH reaction enthalpy, the NH4SH band [K]) and its knobs
  * ATJUP_CHEM_GATE_ZERO (1), ATJUP_CHEM_MOLAR_CONC (0), ATJUP_CHEM_ENTHALPY (0)
- * (planet/ChemistryJup.h:54-57, :123-126). ATJUP_NH4SH_EQ must be 0: the equilibrium
- * adjustment is not built. */
+ * (planet/ChemistryJup.h:54-57, :123-126). The molar masses m_nh3, m_h2s, m_nh4sh also serve the
+ * NH4SH equilibrium adjustment (atjup_knobs.nh4sh_eq). */
 typedef struct atjup_chem_params {
     double m_nh3, m_h2s, m_nh4sh, m_h2, m_he, m_h2o;
     double X_h2, X_he, X_h2o, X_h2s, X_nh3, X_nh4sh;
@@ -140,7 +145,8 @@ typedef struct atjup_chem_params {
  * atjup_hip_set_gas's values): ATJUP_MCA_LEDOUX (1), ATJUP_MCA_LOADING (1), ATJUP_MCA_RELAX (1, in
  * [0, 1]), ATJUP_MCA_PASSES (64) (planet/ConvectiveAdjustment.h:409-413), the H2S tracer's ep_h2s
  * (InitValues_Jup.cpp:945) and the NH4SH tracer's heat of formation (0 unless ATJUP_NH4SH_EQ,
- * planet/cJupiterModel.cpp:1316). The supported settings are ATJUP_MCA_SMOOTH 0, ATJUP_MCA_PHASE 0,
+ * planet/cJupiterModel.cpp:1316; a context created with atjup_knobs.nh4sh_eq starts with the
+ * equilibrium's 4705 ln(10) R* / m_nh4sh J/kg). The supported settings are ATJUP_MCA_SMOOTH 0, ATJUP_MCA_PHASE 0,
  * ATJUP_MCA_INTERIOR 1, ATJUP_CP_H2 and ATJUP_CP_GAS 0 and the chromophore off (ATJUP_CHROM 0), which
  * leaves that tracer zero everywhere. */
 typedef struct atjup_mca_params {
@@ -311,6 +317,12 @@ void atjup_hip_default_mca_params(atjup_mca_params* params);
  * copy. iter receives the header's iteration; carried_out whether the file held the carried set. */
 int atjup_hip_save_state(atjup_ctx* ctx, const char* path, int iter, int carried);
 int atjup_hip_load_state(atjup_ctx* ctx, const char* path, int* iter, int* carried_out);
+/* ChemistryJup::EquilibriumNH4SH("loop") (planet/ChemistryJup.h:293-370): every fluid cell brought
+ * to the Lewis NH4SH equilibrium p_NH3 p_H2S = K_p(T), NH3, H2S and NH4SH exchanged and t moved by
+ * the heat of formation (the chemistry's molar masses of atjup_hip_set_chemistry). atjup_hip_step
+ * runs it after the saturation adjustments when atjup_knobs.nh4sh_eq is set
+ * (planet/cJupiterModel.cpp:1084). The printed census is not formed. */
+int atjup_hip_nh4sh_equilibrium(atjup_ctx* ctx);
 /* Turbulence<cJupiterModel>::run (planet/Turbulence.h:307-1106) followed by the wall-function
  * omega Run() applies after it (planet/cJupiterModel.cpp:356-374, 1144). */
 int atjup_hip_turbulence_run(atjup_ctx* ctx);

@@ -59,7 +59,7 @@ def _orc_lib():
                   "orc_mixture_density", "orc_clamp_species", "orc_swap_nlevel",
                   "orc_scalar_sync", "orc_turbulence_run", "orc_turb_wall", "orc_flux_limiter",
                   "orc_check_finite", "orc_radiation", "orc_precipitation", "orc_chemistry",
-                  "orc_mca"):
+                  "orc_mca", "orc_nh4sh_eq"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.orc_pressure_solve.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_rb_colour_pass.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]

@@ -36,6 +36,10 @@ inline double safe_clamp(double v, double lo, double hi){
     return (v < lo) ? lo : ((v > hi) ? hi : v);
 }
 
+// J per kg NH4SH formed: the van't Hoff enthalpy of the Lewis K_p, 4705 ln(10) R* / 51.1114
+// (ChemistryJup.h:282-283)
+inline double nh4sh_formation_heat(){ return 4705.0 * std::log(10.0) * 8314.46 / 51.1114; }
+
 }  // namespace
 
 struct orc_ctx {
@@ -886,6 +890,32 @@ const int kSolid[] = { ATJUP_F_T, ATJUP_F_P_STAT, ATJUP_F_P_DYN,
     ATJUP_F_H2S, ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_NH4SH,
     ATJUP_F_MASSFLUX_H2S, ATJUP_F_MASSFLUX_NH3, ATJUP_F_MASSFLUX_NH4SH, ATJUP_F_FLUXLIM_NH4SH };
 
+// The fields each boundary pass treats, as the reference's lists name them (BC_Jup.h:272-350,
+// :469-487): the field ids above, and the chemistry's carried fields, which DiffMassFluxJup and
+// ChemMassRateJup write on the interior only, so that the values these passes leave on the
+// boundary rows reach the next call's smoothing and thermal flux: the reaction rates w_* (held
+// beside the fields), the diffusive fluxes and thermalmassflux. (j_*, jT_* and the force
+// diagnostics are on the lists too but are never read across calls.)
+struct BCSets { std::vector<std::vector<double>*> radius, theta_ex, theta_zero, phi, solid; };
+BCSets bc_sets(orc_ctx& o){
+    BCSets s;
+    for(int f : kRadius) s.radius.push_back(&o.f[f]);
+    for(int f : kThetaExtrap) s.theta_ex.push_back(&o.f[f]);
+    for(int f : kThetaZero) s.theta_zero.push_back(&o.f[f]);
+    for(int f : kPhi) s.phi.push_back(&o.f[f]);
+    for(int f : kSolid) s.solid.push_back(&o.f[f]);
+    for(std::vector<double>* w : { &o.w_h2s, &o.w_nh3, &o.w_nh4sh }){
+        s.radius.push_back(w); s.theta_ex.push_back(w); s.phi.push_back(w); s.solid.push_back(w);
+    }
+    for(int f : { ATJUP_F_DIFFLUX_H2S, ATJUP_F_DIFFLUX_NH3, ATJUP_F_DIFFLUX_NH4SH }){
+        s.radius.push_back(&o.f[f]); s.theta_zero.push_back(&o.f[f]); s.phi.push_back(&o.f[f]);
+        s.solid.push_back(&o.f[f]);
+    }
+    std::vector<double>* th = &o.f[ATJUP_F_THERMALMASSFLUX];
+    s.radius.push_back(th); s.theta_ex.push_back(th); s.phi.push_back(th); s.solid.push_back(th);
+    return s;
+}
+
 // ---- BC_Jup::bcRadius: shared bcRadius (BoundaryConditions.h:151-286) with TOP_TAPER 3
 //      (BC_Jup.h:185-263) -------------------------------------------------------------------------
 void bc_radius(orc_ctx& o){
@@ -903,11 +933,12 @@ void bc_radius(orc_ctx& o){
             zm[(size_t)(2 * l + 1) * jm + j] = sw / kn;
         }
     const double c43 = o.c43, c13 = o.c13;
+    const BCSets sets = bc_sets(o);
     #pragma omp parallel for
     for(int j = 0; j < jm; j++)
         for(int k = 0; k < km; k++)
-            for(int fid : kRadius){
-                auto& X = o.f[fid];
+            for(std::vector<double>* Xp : sets.radius){
+                auto& X = *Xp;
                 X[o.id(0, j, k)] = c43 * X[o.id(1, j, k)] - c13 * X[o.id(2, j, k)];
                 X[o.id(im - 1, j, k)] = c43 * X[o.id(im - 2, j, k)] - c13 * X[o.id(im - 3, j, k)];
             }
@@ -958,29 +989,30 @@ void bc_radius(orc_ctx& o){
 void bc_theta(orc_ctx& o){
     const int im = o.im, jm = o.jm, km = o.km;
     const int nfill = 2, jf = nfill + 1;
+    const BCSets sets = bc_sets(o);
     #pragma omp parallel for
     for(int k = 0; k < km; k++)
         for(int i = 0; i < im; i++){
             for(int r = 1; r <= nfill; r++){
                 const double ramp = (double)r / (double)jf;
-                for(int fid : kThetaExtrap){
-                    auto& X = o.f[fid];
+                for(std::vector<double>* Xp : sets.theta_ex){
+                    auto& X = *Xp;
                     X[o.id(i, r, k)] = X[o.id(i, jf, k)];
                     X[o.id(i, jm - 1 - r, k)] = X[o.id(i, jm - 1 - jf, k)];
                 }
-                for(int fid : kThetaZero){
-                    auto& X = o.f[fid];
+                for(std::vector<double>* Xp : sets.theta_zero){
+                    auto& X = *Xp;
                     X[o.id(i, r, k)] = ramp * X[o.id(i, jf, k)];
                     X[o.id(i, jm - 1 - r, k)] = ramp * X[o.id(i, jm - 1 - jf, k)];
                 }
             }
-            for(int fid : kThetaZero){
-                auto& X = o.f[fid];
+            for(std::vector<double>* Xp : sets.theta_zero){
+                auto& X = *Xp;
                 X[o.id(i, 0, k)] = 0.0;
                 X[o.id(i, jm - 1, k)] = 0.0;
             }
-            for(int fid : kThetaExtrap){
-                auto& X = o.f[fid];
+            for(std::vector<double>* Xp : sets.theta_ex){
+                auto& X = *Xp;
                 X[o.id(i, 0, k)] = X[o.id(i, 1, k)];
                 X[o.id(i, jm - 1, k)] = X[o.id(i, jm - 2, k)];
             }
@@ -990,11 +1022,12 @@ void bc_theta(orc_ctx& o){
 // ---- bcPhi, periodic seam: k = km-1 is the copy of k = 0 (BoundaryConditions.h:398-439) --------
 void bc_phi(orc_ctx& o){
     const int im = o.im, jm = o.jm, km = o.km;
+    const BCSets sets = bc_sets(o);
     #pragma omp parallel for
     for(int i = 0; i < im; i++)
         for(int j = 0; j < jm; j++)
-            for(int fid : kPhi){
-                auto& X = o.f[fid];
+            for(std::vector<double>* Xp : sets.phi){
+                auto& X = *Xp;
                 X[o.id(i, j, km - 1)] = X[o.id(i, j, 0)];
             }
 }
@@ -1002,7 +1035,8 @@ void bc_phi(orc_ctx& o){
 // ---- BC_Jup::bcSolidGround with MOUNT_T_AMBIENT (BC_Jup.h:460-658) --------------------------
 void bc_solid(orc_ctx& o){
     const int im = o.im, jm = o.jm, km = o.km;
-    const int ns = (int)(sizeof(kSolid) / sizeof(kSolid[0]));
+    const BCSets sets = bc_sets(o);
+    const int ns = (int)sets.solid.size();
     const int n_state = 2;
     const double c43 = o.c43, c13 = o.c13;
     auto S = [&](int ii, int jj, int kk){
@@ -1024,14 +1058,14 @@ void bc_solid(orc_ctx& o){
                 o.f[ATJUP_F_W][q] = 0.0;
                 auto extrap = [&](int ia, int ja, int ka, int ib, int jb, int kb){
                     for(int f = 0; f < ns; f++){
-                        auto& X = o.f[kSolid[f]];
+                        auto& X = *sets.solid[f];
                         X[q] = c43 * X[o.id(ia, ja, ka)] - c13 * X[o.id(ib, jb, kb)];
                     }
                 };
                 const bool surf = !S(i + 1, j, k) || !S(i - 1, j, k) || !S(i, j + 1, k)
                                || !S(i, j - 1, k) || !S(i, j, k + 1) || !S(i, j, k - 1);
                 if(!surf){
-                    for(int f = n_state; f < ns; f++) o.f[kSolid[f]][q] = 0.0;
+                    for(int f = n_state; f < ns; f++) (*sets.solid[f])[q] = 0.0;
                     continue;
                 }
                 if(Fl(i + 1, j, k) && Fl(i + 2, j, k)) extrap(i + 1, j, k, i + 2, j, k);
@@ -1040,7 +1074,7 @@ void bc_solid(orc_ctx& o){
                 else if(Fl(i, j - 1, k) && Fl(i, j - 2, k)) extrap(i, j - 1, k, i, j - 2, k);
                 else if(Fl(i, j, k + 1) && Fl(i, j, k + 2)) extrap(i, j, k + 1, i, j, k + 2);
                 else if(Fl(i, j, k - 1) && Fl(i, j, k - 2)) extrap(i, j, k - 1, i, j, k - 2);
-                else for(int f = n_state; f < ns; f++) o.f[kSolid[f]][q] = 0.0;
+                else for(int f = n_state; f < ns; f++) (*sets.solid[f])[q] = 0.0;
             }
     auto& T = o.f[ATJUP_F_T];
     #pragma omp parallel for collapse(2) schedule(static)
@@ -1101,6 +1135,14 @@ void scalar_sync(orc_ctx& o){
         #pragma omp parallel for schedule(static)
         for(size_t q = 0; q < o.n3; q++) N[q] = X[q];
     }
+    // H2S and NH4SH join the sync when the equilibrium adjustment writes them (cJupiterModel.cpp:1120)
+    if(o.kn.nh4sh_eq)
+        for(int f : { ATJUP_F_H2S, ATJUP_F_NH4SH }){
+            auto& N = o.f[ATJUP_NLEVEL + f];
+            const auto& X = o.f[f];
+            #pragma omp parallel for schedule(static)
+            for(size_t q = 0; q < o.n3; q++) N[q] = X[q];
+        }
 }
 
 void wall_viscosity(orc_ctx& o){                          // InitValues_Jup.cpp:84-134
@@ -1293,6 +1335,7 @@ void satadj(orc_ctx& o, int gas){
 // ---- Turbulence<P>::run, k-omega, TURB_FREE 1, TURB_CLAMP_PASS 1, TURB_CURV 1, no ABL taper
 //      (Turbulence.h:307-1012), then the wall-omega function of Run() (cJupiterModel.cpp:356-374) --
 void turb_wall_omega(orc_ctx& o){
+    if(o.kn.turb_model == 0) return;              // k-epsilon keeps its wall value (cJupiterModel.cpp:356-360)
     const double nue_gas_phys = 1.8e-5, bet_star = 0.09, bet_1 = 0.0333, kappa = 0.41;
     const double nd_omega = o.g.L_atm * 1.0e3 / o.c.u_0;
     auto& dis = o.f[ATJUP_F_DIS];
@@ -1339,6 +1382,12 @@ void turbulence(orc_ctx& o){
     const double L_m = o.g.L_atm * 1.0e3;
     const double nue_max = o.kn.nue_max_phys / (o.c.u_0 * L_m);
     const double k_bg_nd = o.kn.turb_k_bg_phys / (o.c.u_0 * o.c.u_0);
+    // the model (0 k-epsilon, 1 k-omega, 2 SST) and the class constants: C_mu, the H2/He viscosity
+    // nue_air and re_turb = vel_star_ref z_0 / nue_gas of the default constructor (Turbulence.h:192,
+    // :245-251)
+    const int model = o.kn.turb_model;
+    const double C_nue = 0.028, nue_air = 1.8e-5;
+    const double re_turb_cls = 0.4 * 0.1 / 1.8e-5;
     // compute_vel_star with TURB_FREE: the calm-air floor everywhere (Turbulence.h:340-342)
     std::fill(o.vel_star.begin(), o.vel_star.end(), 0.05);
     // clamp-in pass
@@ -1403,7 +1452,7 @@ void turbulence(orc_ctx& o){
                 {
                     const double dis_here = std::max(dis[q], dis_min);
                     const double tke_here = std::max(tke[q], 0.0);
-                    cnue = tke_here / dis_here;
+                    cnue = (model == 0) ? C_nue * tke_here * tke_here / dis_here : tke_here / dis_here;
                     cnue = std::min(cnue, nue_max);
                 }
                 const double u_r = U[q] * inv_rm, v_r = V[q] * inv_rm, w_r = W[q] * inv_rm;
@@ -1424,7 +1473,33 @@ void turbulence(orc_ctx& o){
                     + (cnue * (g_pt + g_tp)) * g_pt);
                 const double W12 = g_rt - g_tr, W13 = g_rp - g_pr, W23 = g_tp - g_pt;
                 const double Omega_mag = std::sqrt(W12 * W12 + W13 * W13 + W23 * W23);
-                // compute_k_omega (Turbulence.h:920-1012)
+                if(model == 0){
+                    // compute_k_epsilon (Turbulence.h:886-918) with TURB_FREE: the wall distance is
+                    // free_wall_dist() = 1e20 m, so the Chien damping takes its free-flow form
+                    const double C_eps_1 = 1.35, C_eps_2 = 1.80;
+                    const double y_phys = 1.0e20;
+                    const double y_star = y_phys / L_m;
+                    nue[q] = C_nue * tke[q] * tke[q] / std::max(dis[q], dis_min);
+                    const double d_plus = y_phys * o.vel_star[(size_t)j * km + k] / nue_air;
+                    const double f_nue = 1.0 - std::exp(-0.0115 * d_plus);
+                    const double Re_T = tke[q] * tke[q] * o.c.u_0 * L_m
+                                      / std::max(dis[q] * nue_air, dis_min * nue_air);
+                    const double f_2 = 1.0 - 0.4 / 1.8 * std::exp(-Re_T * Re_T / 36.0);
+                    nue[q] *= f_nue;
+                    const double y_star2 = y_star * y_star;
+                    const double L_k = -2.0 * tke[q] / y_star2;
+                    const double L_w = -2.0 * dis[q] / y_star2 * std::exp(-d_plus / 2.0);
+                    const double P_k = prod[q];
+                    const double Y_k = dis[q];
+                    const double tke_safe = std::max(tke[q], dis_min);
+                    const double P_w = C_eps_1 * dis[q] / tke_safe * P_k;
+                    const double Y_w = C_eps_2 * f_2 * dis[q] * dis[q] / tke_safe;
+                    tsrc[q] = P_k - Y_k + L_k / re_turb_cls;
+                    dsrc[q] = P_w - Y_w + L_w / re_turb_cls;
+                    continue;
+                }
+                // compute_k_omega (Turbulence.h:920-1012); compute_k_omega_SST shares its
+                // Neumann-aware gradients with the unconditional 1 / (rm + 1)
                 const double bet_star = 0.09, gam = 0.52, C_lim = 0.875, bet_0 = 0.0708;
                 const double erm = 1.0 / (rm + 1.0);
                 const double dtkedr_neu = nb ? (tke[qip] - tke[q]) * (1.0 / (2.0 * o.g.dr)) * erm : dtkedr;
@@ -1433,6 +1508,38 @@ void turbulence(orc_ctx& o){
                 const double ddisdthe_neu = (dis_jp1 - dis_jm1) / (rm * 2.0 * o.g.dthe);
                 const double dtkedphi_neu = (tke_kp1 - tke_km1) / (rmsinthe * 2.0 * o.g.dphi);
                 const double ddisdphi_neu = (dis_kp1 - dis_km1) / (rmsinthe * 2.0 * o.g.dphi);
+                if(model == 2){
+                    // compute_k_omega_SST (Turbulence.h:1021-1106), Menter 1994, TURB_FREE wall distance
+                    const double a1 = 0.31, bet1 = 0.0333, bet2 = 0.0368, gam1 = 0.413, gam2 = 0.2;
+                    const double sig_w2 = 1.168;
+                    const double nue_air_nd = nue_air / (o.c.u_0 * L_m);
+                    const double y_phys = 1.0e20;
+                    const double y_star = y_phys / L_m;
+                    const double grad_dot = dtkedr_neu * ddisdr_neu
+                                          + dtkedthe_neu * ddisdthe_neu
+                                          + dtkedphi_neu * ddisdphi_neu;
+                    const double CD_kw = std::max(2.0 * sig_w2 / std::max(dis[q], 1.0e-20) * grad_dot, 1.0e-20);
+                    const double dis_safe = std::max(dis[q], 1.0e-20);
+                    const double arg1 = std::min(
+                        std::max(std::sqrt(tke[q]) / (bet_star * dis_safe * y_star),
+                                 500.0 * nue_air_nd / (y_star * y_star * dis_safe)),
+                        4.0 * sig_w2 * tke[q] / (CD_kw * y_star * y_star));
+                    const double arg2 = std::max(
+                        2.0 * std::sqrt(tke[q]) / (bet_star * dis_safe * y_star),
+                        500.0 * nue_air_nd / (y_star * y_star * dis_safe));
+                    const double F1 = std::tanh(std::pow(arg1, 4));
+                    const double F2 = std::tanh(std::pow(arg2, 2));
+                    nue[q] = a1 * tke[q] / std::max(a1 * dis_safe, Omega_mag * F2);
+                    const double tke_safe = std::max(tke[q], 1.0e-20);
+                    const double P_k = std::min(prod[q], 20.0 * bet_star * tke_safe * dis_safe);
+                    const double Y_k = bet_star * tke_safe * dis_safe;
+                    const double P_w = (F1 * gam1 + (1.0 - F1) * gam2) * P_k * dis_safe / tke_safe;
+                    const double Y_w = (F1 * bet1 + (1.0 - F1) * bet2) * dis_safe * dis_safe;
+                    const double D_w = 2.0 * (1.0 - F1) * sig_w2 / dis_safe * grad_dot;
+                    tsrc[q] = P_k - Y_k;
+                    dsrc[q] = P_w - Y_w + D_w;
+                    continue;
+                }
                 const double dudr_k = (U[qip] - U[qim]) / (2.0 * o.g.dr) / (rm + 1.0);
                 const double dvdthe_k = (V[qjp] - V[qjm]) / (rm * 2.0 * o.g.dthe);
                 const double dwdphi_k = (W[qkp] - W[qkm]) / (rmsinthe * 2.0 * o.g.dphi);
@@ -1859,6 +1966,7 @@ void chemmassrate(orc_ctx& o){
     const atjup_chem_params& p = o.chemp;
     const int im = o.im, jm = o.jm, km = o.km;
     const double keq = p.m_nh4sh / (p.m_nh3 * p.m_h2s), A = 15000.0, B = -0.5, T_d = 3020.0;
+    const bool eq_on = o.kn.nh4sh_eq != 0;
     std::vector<double>& MF_h2s = o.f[ATJUP_F_MASSFLUX_H2S];
     std::vector<double>& MF_nh3 = o.f[ATJUP_F_MASSFLUX_NH3];
     std::vector<double>& MF_nh4sh = o.f[ATJUP_F_MASSFLUX_NH4SH];
@@ -1886,7 +1994,8 @@ void chemmassrate(orc_ctx& o){
                 const double t_u = o.f[ATJUP_F_T][q] * o.c.t_ref;
                 const double kf = A * std::pow(t_u, B) * std::exp(-T_d / t_u);
                 const double kb = kf / keq;
-                if(t_u <= p.t_0_nh4sh && t_u >= p.t_00_nh4sh){
+                // with the equilibrium adjustment the rate law contributes nothing (ChemistryJup.h:133-135)
+                if(!eq_on && t_u <= p.t_0_nh4sh && t_u >= p.t_00_nh4sh){
                     const double nh3 = o.f[ATJUP_F_NH3][q], h2s = o.f[ATJUP_F_H2S][q], nh4sh = o.f[ATJUP_F_NH4SH][q];
                     double denom;
                     if(p.molar_conc){
@@ -1901,13 +2010,64 @@ void chemmassrate(orc_ctx& o){
                     o.w_nh3[q] = -p.m_nh3 * R_diff;
                     o.w_h2s[q] = -p.m_h2s * R_diff;
                     o.w_nh4sh[q] = p.m_nh4sh * R_diff;
-                } else if(p.gate_zero){
+                } else if(p.gate_zero || eq_on){
                     o.w_nh3[q] = o.w_h2s[q] = o.w_nh4sh[q] = 0.0;
                 }
                 MF_h2s[q] = o.w_h2s[q] + o.f[ATJUP_F_DIFFLUX_H2S][q];
                 MF_nh3[q] = o.w_nh3[q] + o.f[ATJUP_F_DIFFLUX_NH3][q];
                 MF_nh4sh[q] = o.w_nh4sh[q] + o.f[ATJUP_F_DIFFLUX_NH4SH][q];
             }
+}
+
+// ---- ChemistryJup::EquilibriumNH4SH("loop") (ChemistryJup.h:293-370): every fluid cell brought to
+//      the Lewis equilibrium p_NH3 p_H2S = K_p(T), K_p = 10^(14.82 - 4705 / T) atm^2, with the heat of
+//      formation moving T; the extent x [kmol/m3] found by bisection on the monotone
+//      f(x) = (a - x)(b - x)(R* T(x))^2 - K_p(T(x)). The printed census is not formed. ----------------
+void nh4sh_eq(orc_ctx& o){
+    const atjup_chem_params& p = o.chemp;
+    const double R_univ = 8314.46, atm = 101325.0;
+    const double L_f = nh4sh_formation_heat();
+    auto Kp = [&](double T){ return std::pow(10.0, 14.82 - 4705.0 / T) * atm * atm; };
+    auto& t = o.f[ATJUP_F_T];
+    auto& nh3 = o.f[ATJUP_F_NH3];
+    auto& h2s = o.f[ATJUP_F_H2S];
+    auto& nh4sh = o.f[ATJUP_F_NH4SH];
+    const auto& rho_mix = o.f[ATJUP_F_RHO_MIX];
+    #pragma omp parallel for schedule(static)
+    for(size_t q = 0; q < o.n3; q++){
+        if(o.mask[q] == 1.0) continue;
+        const double T0 = t[q] * o.c.t_ref;
+        if(!(T0 > 0.0)) continue;
+        double rho = rho_mix[q];
+        if(!(rho > 0.0) || !std::isfinite(rho)) rho = o.c.r_mix;
+        const double a = std::max(0.0, nh3[q]) / p.m_nh3;
+        const double b = std::max(0.0, h2s[q]) / p.m_h2s;
+        const double s = std::max(0.0, nh4sh[q]) / p.m_nh4sh;
+        const double heat = L_f * p.m_nh4sh / (o.c.cp_mix * rho);
+        auto f = [&](double x){
+            const double T = T0 + heat * x, RT = R_univ * T;
+            return (a - x) * (b - x) * RT * RT - Kp(T);
+        };
+        const double f0 = f(0.0);
+        double lo, hi;
+        if(f0 > 0.0){ lo = 0.0; hi = std::min(a, b); }
+        else if(s > 0.0){
+            if(f(-s) <= 0.0){ lo = hi = -s; }
+            else { lo = -s; hi = 0.0; }
+        } else continue;
+        if(hi > lo)
+            for(int it = 0; it < 80; it++){
+                const double xm = 0.5 * (lo + hi);
+                if(f(xm) > 0.0) lo = xm; else hi = xm;
+                if(hi - lo <= 1.0e-15 * std::max(a, std::max(b, s))) break;
+            }
+        const double x = (f0 > 0.0) ? lo : hi;
+        if(x == 0.0) continue;
+        nh3[q] = std::max(0.0, nh3[q] - p.m_nh3 * x);
+        h2s[q] = std::max(0.0, h2s[q] - p.m_h2s * x);
+        nh4sh[q] = std::max(0.0, nh4sh[q] + p.m_nh4sh * x);
+        t[q] = (T0 + heat * x) / o.c.t_ref;
+    }
 }
 
 // the physics block's chemistry in the reference's order (cJupiterModel.cpp:1146-1161)
@@ -2190,6 +2350,9 @@ orc_ctx* orc_create(const atjup_grid* g, const atjup_consts* c, const atjup_knob
     o->w_nh3.assign(o->n3, 0.0); o->w_h2s.assign(o->n3, 0.0); o->w_nh4sh.assign(o->n3, 0.0);
     // Jupiter (cJupiterModel.h:1109-1118) and the radiation knobs' defaults (Radiation.h:270-295)
     o->radp = atjup_rad_params{ 5.4, 50.5, 0.343, 0.863, 0.134, 3.9, 1.0, 1.0, 1.0, 1, 0 };
+    // with the NH4SH equilibrium the moist convective adjustment carries its heat of formation
+    // (cJupiterModel.cpp:1316)
+    if(k->nh4sh_eq) o->mcap.l_nh4sh = nh4sh_formation_heat();
     return o;
 }
 
@@ -2280,7 +2443,7 @@ int orc_satadj(orc_ctx* o, int gas){
 int orc_turbulence_run(orc_ctx* o){
     if(!o) return ATJUP_ERR_ARG;
     if(o->kn.turb_model < 0) return 0;
-    if(o->kn.turb_model != 1) return ATJUP_ERR_PROFILE;      // k-omega is the supported closure
+    if(o->kn.turb_model > 2) return ATJUP_ERR_PROFILE;
     turbulence(*o);
     turb_wall_omega(*o);
     return 0;
@@ -2305,6 +2468,7 @@ int orc_set_chemistry(orc_ctx* o, const atjup_chem_params* p){
     return 0;
 }
 int orc_chemistry(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; chemistry(*o); return 0; }
+int orc_nh4sh_eq(orc_ctx* o){ if(!o) return ATJUP_ERR_ARG; nh4sh_eq(*o); return 0; }
 int orc_set_mca(orc_ctx* o, const atjup_mca_params* p){
     if(!o || !p) return ATJUP_ERR_ARG;
     o->mcap = *p;
@@ -2333,6 +2497,7 @@ int orc_step(orc_ctx* o, int iter, int first){
         mixture_density(*o);
         if(o->kn.radiation) radiation(*o);                    // cJupiterModel.cpp:1056
         if(o->kn.satadj) for(int gas = 0; gas < 3; gas++) satadj(*o, gas);
+        if(o->kn.nh4sh_eq) nh4sh_eq(*o);                      // cJupiterModel.cpp:1084
         if(o->kn.precip) precipitation(*o);                   // cJupiterModel.cpp:1090
         if(o->kn.rk_scalar_sync >= 1) scalar_sync(*o);
         if(o->kn.turb_model >= 0){ turbulence(*o); turb_wall_omega(*o); }

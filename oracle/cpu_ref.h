@@ -42,6 +42,7 @@ int      orc_flux_limiter(orc_ctx* o);
 int      orc_radiation(orc_ctx* o);
 int      orc_set_radiation(orc_ctx* o, const atjup_rad_params* p);
 int      orc_chemistry(orc_ctx* o);
+int      orc_nh4sh_eq(orc_ctx* o);
 int      orc_set_chemistry(orc_ctx* o, const atjup_chem_params* p);
 int      orc_mca(orc_ctx* o);
 int      orc_set_mca(orc_ctx* o, const atjup_mca_params* p);

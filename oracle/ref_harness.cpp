@@ -181,6 +181,9 @@ public:
     // to Run() (planet/cJupiterModel.cpp:356-374), so the harness repeats its few lines here with
     // ATJUP_TURB_WALL_OMEGA at its default 1 and a k-omega closure.
     static void wall_omega(cJupiterModel& M){
+        // the lambda leaves a k-epsilon closure alone (cJupiterModel.cpp:356-360)
+        const char* e = getenv("ATJUP_TURB_MODEL");
+        if((e ? std::string(e) : M.turb_model) == "k_epsilon") return;
         const double nue_gas_phys = 1.8e-5, bet_star = 0.09, bet_1 = 0.0333, kappa = 0.41;
         const double nd_omega = M.L_atm * 1.0e3 / M.u_0;
         for(int j = 0; j < M.jm; j++)

@@ -24,11 +24,13 @@ def small_state(config="dyn", seed=3, g=None):
     return st
 
 
-def make_pair(config="dyn", sweeps=20, omega=1.0, state=None, precision=abi.PRECISION_FP64):
+def make_pair(config="dyn", sweeps=20, omega=1.0, state=None, precision=abi.PRECISION_FP64, **profile):
+    """The HIP context and the oracle on the same state; `profile` goes to abi.make_profile (the
+    closure, the physics-block components)."""
     from atjup_amd.lib import Context
     import oracle
     st = state if state is not None else inputs.make_state(config, seed=7)
-    G, Cn, K, env = abi.make_profile(config, sweeps=sweeps, omega=omega, grid=st.grid)
+    G, Cn, K, env = abi.make_profile(config, sweeps=sweeps, omega=omega, grid=st.grid, **profile)
     gpu = Context(G, Cn, K, precision)
     orc = oracle.Oracle(G, Cn, K)
     skip = set(inputs.SPECIES) if config == "dyn" else set()

@@ -69,6 +69,61 @@ def test_default_mca_params_match_mirror():
         assert getattr(p, name) == getattr(q, name), name
 
 
+def _create(G, Cn, K, precision=abi.PRECISION_FP64):
+    L = lib.load()
+    h = L.atjup_hip_ctx_create(ctypes.byref(G), ctypes.byref(Cn), ctypes.byref(K), precision)
+    msg = L.atjup_hip_last_error_string().decode()
+    if h:
+        L.atjup_hip_ctx_destroy(h)
+    return h, msg
+
+
+@pytest.mark.parametrize("change,expect", [
+    (dict(turb_model=3), "turb_model"),
+    (dict(rk_scalar_sync=0), "RK_SCALAR_SYNC"),
+    (dict(press_omega=2.0), "press_omega"),
+    (dict(radiation=2), "radiation"),
+    (dict(mca=-1), "mca"),
+])
+def test_ctx_create_refuses_profiles_it_does_not_implement(change, expect):
+    """A knob profile outside the implemented set is refused before any device work, with the knob
+    named in atjup_hip_last_error_string() (ATJUP_ERR_PROFILE: atjup_hip_ctx_create returns NULL)."""
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libatjup_hip.so not built")
+    G, Cn, K, _ = abi.make_profile("full", sweeps=5)
+    for k, v in change.items():
+        setattr(K, k, v)
+    h, msg = _create(G, Cn, K)
+    assert not h and expect in msg, msg
+
+
+def test_ctx_create_refuses_the_dry_profile_with_chemistry_or_a_closure():
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libatjup_hip.so not built")
+    G, Cn, K, _ = abi.make_profile("dyn", sweeps=5)
+    K.chemistry = 1
+    h, msg = _create(G, Cn, K)
+    assert not h and "moist profile" in msg, msg
+    G, Cn, K, _ = abi.make_profile("dyn", sweeps=5)
+    K.turb_model = 2
+    h, msg = _create(G, Cn, K)
+    assert not h and "closure off" in msg, msg
+    G, Cn, K, _ = abi.make_profile("full", sweeps=5)
+    h, msg = _create(G, Cn, K, abi.PRECISION_TANGENT)
+    assert not h and "tangent" in msg, msg
+
+
+@pytest.mark.parametrize("model", ["k_omega_SST", "k_epsilon", "k_omega"])
+def test_every_closure_passes_validation(model):
+    """The three closures of the reference are accepted: validation passes, and without a GPU the
+    creation stops at the device check (with one, it succeeds)."""
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libatjup_hip.so not built")
+    G, Cn, K, _ = abi.make_profile("full", sweeps=5, turb=model)
+    h, msg = _create(G, Cn, K)
+    assert h or "no HIP device" in msg, msg
+
+
 def test_default_precipitation_params_match_mirror():
     if not os.path.exists(lib.LIB_PATH):
         pytest.skip("libatjup_hip.so not built")

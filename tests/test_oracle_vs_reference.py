@@ -92,6 +92,89 @@ def test_oracle_full_physics_bit_exact_against_compiled_reference():
     assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
 
 
+# The other two closures the reference offers (ATJUP_TURB_MODEL): k-omega SST, its default, and
+# k-epsilon. Turbulence<P>::run with its wall hold, then two full-physics steps, whose right-hand
+# side carries each model's own transport coefficients and sources (the SST blending reads the wall
+# distance above the obstacle).
+SCRIPT_TURB = r'''
+import sys, numpy as np
+sys.path.insert(0, %(root)r)
+from atjup_amd import abi, inputs
+import oracle
+G, Cn, K, env = abi.make_profile("full", sweeps=8, turb=%(model)r)
+st = inputs.make_state("full", seed=13)
+ref = oracle.Reference(env); ref.load_state(st, Cn)
+orc = oracle.Oracle(G, Cn, K); orc.load_state(st)
+names = list(abi.PROG) + ["p_dyn", "p_hydro", "rho_mix", "aux_u", "aux_v", "aux_w", "nue", "prod",
+                          "tke_source", "dis_source"]
+def same(tag):
+    for n in names:
+        a, b = ref.get(n), orc.get(n)
+        if not np.array_equal(a, b):
+            d = np.abs(a - b); print("MISMATCH", tag, n, np.max(d), np.count_nonzero(a != b)); sys.exit(1)
+ref.call("bc_all"); orc.call("bc_after_step"); ref.call("restore"); orc.call("swap_nlevel"); same("bc")
+ref.call("mixture_density"); orc.call("mixture_density")
+ref.call("turbulence"); orc.call("turbulence_run"); same("turb")
+if np.count_nonzero(orc.get("prod")) < 1000: print("no production"); sys.exit(1)
+nue0 = orc.get("nue")
+ref.call("turb_wall"); orc.call("turb_wall"); same("wall")
+for it in range(1, 3):
+    ref.step(it, it == 1, satadj=True, turb=True); orc.call("step", it, 1 if it == 1 else 0); same("step%%d" %% it)
+print("ALL-EQUAL", float(np.max(nue0)))
+'''
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("model", ["k_omega_SST", "k_epsilon"])
+def test_other_closures_bit_exact_against_compiled_reference(model):
+    r = subprocess.run([sys.executable, "-c", SCRIPT_TURB % {"root": ROOT, "model": model}], capture_output=True,
+                       text=True, timeout=900)
+    assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+    print(model, r.stdout.strip().splitlines()[-1])
+
+
+# The NH4SH equilibrium adjustment (ATJUP_NH4SH_EQ, the reference's default): EquilibriumNH4SH("loop")
+# on the moist state, where cells both form and decompose NH4SH, then two iterations of the loop
+# body with the whole physics block (radiation, adjustments, equilibrium, precipitation, closure,
+# chemistry with its rate law switched off), which also checks the H2S / NH4SH scalar sync.
+SCRIPT_EQ = r'''
+import sys, numpy as np
+sys.path.insert(0, %(root)r)
+from atjup_amd import abi, inputs
+import oracle
+G, Cn, K, env = abi.make_profile("full", sweeps=8, radiation=True, precip=True, chemistry=True, nh4sh_eq=True)
+st = inputs.make_state("full", seed=17)
+st.fields["nh4sh"] *= 30.0
+ref = oracle.Reference(env); ref.load_state(st, Cn)
+orc = oracle.Oracle(G, Cn, K); orc.load_state(st)
+names = list(abi.PROG) + ["p_dyn", "rho_mix", "massflux_h2s", "massflux_nh3", "massflux_nh4sh", "Q_rad", "Q_precip"]
+def same(tag, ns=names):
+    for n in ns:
+        a, b = ref.get(n), orc.get(n)
+        if not np.array_equal(a, b):
+            d = np.abs(a - b); print("MISMATCH", tag, n, np.max(d), np.count_nonzero(a != b)); sys.exit(1)
+ref.call("bc_all"); orc.call("bc_after_step"); ref.call("restore"); orc.call("swap_nlevel"); same("bc")
+ref.call("mixture_density"); orc.call("mixture_density")
+s0 = orc.get("nh4sh")
+ref.call("nh4sh_eq"); orc.call("nh4sh_eq"); same("eq", ["t", "nh3", "h2s", "nh4sh"])
+s1 = orc.get("nh4sh")
+formed, decomposed = int(np.count_nonzero(s1 > s0)), int(np.count_nonzero(s1 < s0))
+if formed < 100 or decomposed < 100: print("one-sided", formed, decomposed); sys.exit(1)
+ref.call("chemistry"); orc.call("chemistry"); same("chem", ["massflux_h2s", "massflux_nh3", "massflux_nh4sh"])
+for it in range(1, 3):
+    ref.step_physics(it, it == 1, satadj=True, turb=True, radiation=True, precip=True, chemistry=True, nh4sh_eq=True)
+    orc.call("step", it, 1 if it == 1 else 0); same("step%%d" %% it)
+print("ALL-EQUAL", formed, decomposed)
+'''
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_nh4sh_equilibrium_bit_exact_against_compiled_reference():
+    r = subprocess.run([sys.executable, "-c", SCRIPT_EQ % {"root": ROOT}], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0 and "ALL-EQUAL" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+    print(r.stdout.strip().splitlines()[-1])
+
+
 # Radiation<P>::run on the moist state with the obstacle, at the knob defaults and with every knob
 # moved (the reference reads them from the environment on each call).
 SCRIPT_RAD = r'''
