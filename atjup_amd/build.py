@@ -47,5 +47,31 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     return out
 
 
+def build_pyatjup(verbose: bool = False) -> str:
+    """The Cython binding atjup_amd/pyatjup*.so: pyatjup.pyx -> C++ (Cython) -> g++, linked against the
+    in-tree libatjup_hip.so (found next to it at run time through an $ORIGIN rpath)."""
+    import sysconfig
+    build_dir = os.path.join(HERE, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    cpp = os.path.join(build_dir, "pyatjup.cpp")
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = [sys.executable, "-m", "cython", "--cplus", "-3", "-I", inc, os.path.join(HERE, "pyatjup.pyx"), "-o", cpp]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("cython failed:\n" + r.stdout.decode(errors="replace"))
+    out = os.path.join(HERE, "pyatjup" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-shared", "-fPIC", "-std=c++17", "-w", "-I", sysconfig.get_paths()["include"], "-I", inc,
+           cpp, "-L", HERE, "-latjup_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed on pyatjup:\n" + r.stdout.decode(errors="replace"))
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_pyatjup(verbose="-v" in sys.argv))

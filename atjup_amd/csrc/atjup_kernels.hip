@@ -1190,7 +1190,7 @@ int do_brl(atjup_ctx* c){
         F<T>(c, ATJUP_F_T), F<T>(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p,
         (const double*)c->tabs.p + c->im, (acc_t<T>*)c->brl_part.p,
         c->jm, c->km, c->pitch, c->c.g, c->c.r_mix * c->c.R_mix, c->c.t_ref);
-    hipLaunchKernelGGL(k_brl_final<acc_t<T>>, dim3(1), dim3(64), 0, c->stream,
+    hipLaunchKernelGGL(k_brl_final<acc_t<T>>, dim3((c->im + 63) / 64), dim3(64), 0, c->stream,
         (const acc_t<T>*)c->brl_part.p, (acc_t<T>*)c->brl.p, c->im);
     return check_launch();
 }

@@ -362,3 +362,32 @@ int atjup_hip_download_slice(atjup_ctx* c, int field, int axis, int index, doubl
 }
 
 }  // extern "C"
+
+// The device math library's FP64 exp, log, pow and tanh, the transcendental functions the time step's
+// kernels call, evaluated elementwise so that their distance from the host's libm can be measured.
+__global__ void k_libm_probe(int fn, const double* __restrict__ x, const double* __restrict__ y,
+                             double* __restrict__ out, long n){
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= n) return;
+    const double a = x[q];
+    out[q] = fn == 0 ? exp(a) : fn == 1 ? log(a) : fn == 2 ? pow(a, y[q]) : tanh(a);
+}
+
+extern "C" int atjup_hip_libm_probe(int fn, const double* x, const double* y, double* out, long n){
+    if(!x || !out || (fn == 2 && !y)){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(fn < 0 || fn > 3 || n < 1 || n > (1L << 26)){ set_error("fn 0-3 (exp, log, pow, tanh), 1 <= n <= 2^26"); return ATJUP_ERR_ARG; }
+    double* d = nullptr;
+    const size_t b = (size_t)n * sizeof(double);
+    ATJUP_HIP_CHECK(hipMalloc(&d, 3 * b));
+    int rc = 0;
+    if(hipMemcpy(d, x, b, hipMemcpyHostToDevice) != hipSuccess ||
+       (fn == 2 && hipMemcpy(d + n, y, b, hipMemcpyHostToDevice) != hipSuccess)) rc = ATJUP_ERR_HIP;
+    if(!rc){
+        hipLaunchKernelGGL(k_libm_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, fn, d, d + n, d + 2 * n, n);
+        if(hipGetLastError() != hipSuccess || hipMemcpy(out, d + 2 * n, b, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = ATJUP_ERR_HIP;
+    }
+    if(rc) set_error("libm_probe: HIP call failed");
+    hipFree(d);
+    return rc;
+}

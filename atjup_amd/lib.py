@@ -91,6 +91,7 @@ def load():
     L.atjup_hip_last_error_string.restype = C.c_char_p
     L.atjup_hip_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.atjup_hip_read_timers.argtypes = [C.c_void_p, _D]
+    L.atjup_hip_libm_probe.argtypes = [C.c_int, _D, _D, _D, C.c_long]
     _lib = L
     return L
 

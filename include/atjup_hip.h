@@ -416,6 +416,10 @@ int         atjup_hip_set_stream(atjup_ctx* ctx, void* stream);
 void*       atjup_hip_field_device_ptr(atjup_ctx* ctx, int field);
 int         atjup_hip_pitch(atjup_ctx* ctx);
 const char* atjup_hip_last_error_string(void);
+/* Diagnostic, no counterpart in the reference: the device math library's FP64 exp (fn 0), log (1),
+ * pow(x, y) (2) or tanh (3) of n host values x (and y), written to the host array out; for measuring
+ * the device functions the time step calls against a host libm, in ulps. 1 <= n <= 2^26. */
+int         atjup_hip_libm_probe(int fn, const double* x, const double* y, double* out, long n);
 
 #ifdef __cplusplus
 }

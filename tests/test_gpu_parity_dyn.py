@@ -147,3 +147,19 @@ def test_pressure_solve_all_colour_pass_kernels(km, omega):
     assert np.array_equal(a, b), f"max diff {np.max(np.abs(a - b)):.3e}"
     gpu.close()
     orc.close()
+
+
+def test_rk4_step_with_more_levels_than_a_wavefront():
+    """81 levels (the 81 x 361 x 721 shell's radial count) on a narrow shell: the per-level
+    buoyancy reference means of every level, those above the first 64 included, and the step."""
+    g = inputs.Grid(im=81, jm=17, km=25, the_degree=180.0 / 16, phi_degree=360.0 / 24)
+    st = inputs.make_state("dyn", seed=6, g=g, with_obstacle=False)
+    gpu, orc, _ = make_pair("dyn", sweeps=7, state=st)
+    gpu.mixture_density()
+    orc.call("mixture_density")
+    gpu.rk4_step()
+    orc.call("rk4_step")
+    np.testing.assert_allclose(gpu.buoy_ref_level(), orc.buoy_ref_level(), rtol=1e-11, atol=0)
+    compare(gpu, orc, ("t", "u", "v", "w", "aux_u", "aux_v", "aux_w"), TOL, "81 levels")
+    gpu.close()
+    orc.close()
