@@ -133,3 +133,22 @@ def test_default_precipitation_params_match_mirror():
     q = abi.precip_params()
     for name, _ in abi.PrecipParams._fields_:
         assert getattr(p, name) == getattr(q, name), name
+
+
+REF_DIR = os.environ.get("ATJUP_REFERENCE", "/root/reference")
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF_DIR, "planet")),
+                    reason="the reference sources are not present (the check compiles against them)")
+def test_reference_side_adapter_compiles_and_links():
+    """include/atjup_hip_step.hpp, HipStep<cJupiterModel> with every member function instantiated,
+    against the reference's own headers (plus the friend line the adapter asks for) and objects,
+    linked with libatjup_hip.so (oracle/Makefile, target `adapter`)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["make", "-C", os.path.join(root, "oracle"), "adapter", f"REF={REF_DIR}", "-j8"],
+                       capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    out = subprocess.run([os.path.join(root, "oracle", "_ref", "hip_step_check")], capture_output=True, text=True,
+                         timeout=60)
+    assert out.returncode == 0 and "linked with libatjup_hip.so" in out.stdout
