@@ -277,13 +277,19 @@ __global__ __launch_bounds__(256) void ka_hydrostatic(const double* __restrict__
 // computeBuoyancyRefLevel^T: the level mean's adjoint (the sum of the cells' -cb * lambda_buoy,
 // gathered by the RHS adjoint into gbrl) spread back onto t with the mean's weights
 // =============================================================================================
+// Two passes: ABRL_NCH chunks of (j, k) per level reduce to partial sums, then one thread per
+// level adds its partials in chunk order.
 constexpr int ABRL_BLOCK = 256;
+constexpr int ABRL_NCH = 32;
 __global__ __launch_bounds__(ABRL_BLOCK) void ka_brl_reduce(const double* __restrict__ t, const double* __restrict__ ps,
         const uint8_t* __restrict__ flags, const double* __restrict__ sin_the, const double* __restrict__ gbrl,
-        double* __restrict__ out, int jm, int km, int pitch, double g, double rR, double t_ref){
-    const int i = blockIdx.x;
+        double* __restrict__ part, int jm, int km, int pitch, double g, double rR, double t_ref){
+    const int i = blockIdx.y, ch = blockIdx.x;
+    const int ncell = jm * km;
+    const int per = (ncell + ABRL_NCH - 1) / ABRL_NCH;
+    const int c0 = ch * per, c1 = min(ncell, c0 + per);
     double sl = 0.0, sw = 0.0;
-    for(int c = threadIdx.x; c < jm * km; c += ABRL_BLOCK){
+    for(int c = c0 + threadIdx.x; c < c1; c += ABRL_BLOCK){
         const int j = c / km, k = c - j * km;
         const size_t q = ((size_t)i * jm + j) * pitch + k;
         sl += gbrl[q];
@@ -301,7 +307,22 @@ __global__ __launch_bounds__(ABRL_BLOCK) void ka_brl_reduce(const double* __rest
         if((int)threadIdx.x < o){ sh[0][threadIdx.x] += sh[0][threadIdx.x + o]; sh[1][threadIdx.x] += sh[1][threadIdx.x + o]; }
         __syncthreads();
     }
-    if(threadIdx.x == 0){ out[2 * i] = sh[0][0]; out[2 * i + 1] = sh[1][0]; }
+    if(threadIdx.x == 0){
+        part[((size_t)i * ABRL_NCH + ch) * 2 + 0] = sh[0][0];
+        part[((size_t)i * ABRL_NCH + ch) * 2 + 1] = sh[1][0];
+    }
+}
+
+__global__ void ka_brl_final(const double* __restrict__ part, double* __restrict__ out, int im){
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= im) return;
+    double sl = 0.0, sw = 0.0;
+    for(int ch = 0; ch < ABRL_NCH; ch++){
+        sl += part[((size_t)i * ABRL_NCH + ch) * 2 + 0];
+        sw += part[((size_t)i * ABRL_NCH + ch) * 2 + 1];
+    }
+    out[2 * i] = sl;
+    out[2 * i + 1] = sw;
 }
 
 __global__ __launch_bounds__(256) void ka_brl_spread(const double* __restrict__ t, const double* __restrict__ ps,
@@ -998,12 +1019,16 @@ struct AdjState {
     Buf lsplit, lsrc, mu, inmask, red, zs;
     std::vector<Buf> ckpt;        // N_STATE fields per checkpoint
     std::vector<Buf> snap;        // N_STATE fields per step of the segment being reversed
-    std::vector<Buf> post;        // N_POST fields per step: the state after its physics block
+    std::vector<Buf> post;        // N_POST fields per physics-block step of the whole run: the state
+                                  // after its block (the pressure solve's result), kept from the forward
+    std::vector<int> post_slot;   // step -> its slot in post, -1 for steps without the block
     // a captured transposed sweep sequence, replayed only on the buffers and geometry it was
     // captured with (its kernels' arguments are baked in)
     struct SweepGraph { int n; double omega; hipGraphExec_t exec; const void* ptrs[8]; int dims[4]; };
     std::vector<SweepGraph> graphs;   // the transposed sweep sequences, replayed
+    Buf brl_part;                 // per-level partial sums of the level-mean transpose
     int nsteps = 0, every = 0, first_iter = 1;
+    bool last_in_snap = false;    // snap holds the last segment's step states, as the forward left them
 };
 
 // what the physics block of a dynamics-profile iteration changes: p_dyn (solve and smoothing),
@@ -1027,7 +1052,7 @@ void adj_free(atjup_ctx* c){
     AdjState* A = c->adj;
     for(auto& b : A->lam) adj_release(b);
     for(auto& b : A->work) adj_release(b);
-    for(Buf* b : { &A->lsplit, &A->lsrc, &A->mu, &A->inmask, &A->red, &A->zs }) adj_release(*b);
+    for(Buf* b : { &A->lsplit, &A->lsrc, &A->mu, &A->inmask, &A->red, &A->zs, &A->brl_part }) adj_release(*b);
     for(auto& b : A->ckpt) adj_release(b);
     for(auto& b : A->snap) adj_release(b);
     for(auto& b : A->post) adj_release(b);
@@ -1152,9 +1177,12 @@ static int adj_rk4(atjup_ctx* c){
     if(int rc = combine(0.0, 0.0, 1, 0)) return rc;             // lambda_N += lambda_{Y_0}
     // the level means and the hydrostatic integral, formed once from the n level
     const double* tabs = (const double*)c->tabs.p;
-    hipLaunchKernelGGL(ka_brl_reduce, dim3(c->im), dim3(ABRL_BLOCK), 0, c->stream, (const double*)Fd(c, ATJUP_F_T),
-        (const double*)Fd(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p, tabs + c->im, (const double*)W(c, AW_GBRL),
-        (double*)A->red.p, c->jm, c->km, c->pitch, C.g, C.r_mix * C.R_mix, C.t_ref);
+    hipLaunchKernelGGL(ka_brl_reduce, dim3(ABRL_NCH, c->im), dim3(ABRL_BLOCK), 0, c->stream,
+        (const double*)Fd(c, ATJUP_F_T), (const double*)Fd(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p,
+        tabs + c->im, (const double*)W(c, AW_GBRL), (double*)A->brl_part.p, c->jm, c->km, c->pitch, C.g,
+        C.r_mix * C.R_mix, C.t_ref);
+    hipLaunchKernelGGL(ka_brl_final, dim3(nblk(c->im, 64)), dim3(64), 0, c->stream, (const double*)A->brl_part.p,
+        (double*)A->red.p, c->im);
     hipLaunchKernelGGL(ka_brl_spread, gc, b128, 0, c->stream, (const double*)Fd(c, ATJUP_F_T),
         (const double*)Fd(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p, tabs + c->im, (const double*)A->red.p,
         W(c, AW_GBRL), L(c, ATJUP_F_T), c->jm, c->km, c->pitch, C.g, C.r_mix * C.R_mix, C.t_ref);
@@ -1390,6 +1418,7 @@ int atjup_hip_adj_init(atjup_ctx* c){
     if(!rc) rc = adj_alloc(A->mu, 2 * hn * sizeof(double));
     if(!rc) rc = adj_alloc(A->inmask, 2 * hn);
     if(!rc) rc = adj_alloc(A->red, 2 * (size_t)c->im * sizeof(double));
+    if(!rc) rc = adj_alloc(A->brl_part, 2 * (size_t)c->im * ABRL_NCH * sizeof(double));
     if(!rc) rc = adj_alloc(A->zs, 4 * (size_t)c->jm * sizeof(double));
     if(rc){ adj_free(c); return rc; }
     return 0;
@@ -1434,24 +1463,52 @@ int atjup_hip_adj_op(atjup_ctx* c, int op, int iarg, double darg){
 int atjup_hip_adj_forward_checkpointed(atjup_ctx* c, int nsteps, int every, int first_iter){
     if(int rc = need_adj(c)) return rc;
     if(nsteps < 1 || every < 1){ set_error("nsteps, every"); return ATJUP_ERR_ARG; }
+    if(!c->have_mask){ set_error("upload the obstacle mask first (atjup_hip_upload_mask)"); return ATJUP_ERR_STATE; }
     AdjState* A = c->adj;
-    for(auto& b : A->ckpt) adj_release(b);
-    for(auto& b : A->snap) adj_release(b);
+    A->nsteps = 0;
+    A->last_in_snap = false;
+    // Storage: the state every `every` steps (the checkpoints), the state after each physics block
+    // (the pressure solve's result: the recomputation of a segment then runs no solve), and the
+    // step states of the last segment (its reverse sweep then needs no recomputation). On the
+    // 41 x 181 x 361 shell, 20 steps with checkpoints every 5: 36 + 55 + 45 fields of 21.6 MB.
+    // Buffers are kept between calls of the same shape.
     const int nseg = (nsteps + every - 1) / every;
-    A->ckpt.assign((size_t)nseg * N_STATE, Buf{});
-    for(auto& b : A->post) adj_release(b);
-    A->snap.assign((size_t)every * N_STATE, Buf{});
-    A->post.assign((size_t)every * N_POST, Buf{});
-    const size_t fb = c->n3 * sizeof(double);
-    for(auto& b : A->ckpt) if(int rc = adj_alloc(b, fb)) return rc;
-    for(auto& b : A->snap) if(int rc = adj_alloc(b, fb)) return rc;
-    for(auto& b : A->post) if(int rc = adj_alloc(b, fb)) return rc;
-    A->nsteps = nsteps; A->every = every; A->first_iter = first_iter;
+    const int s_last = (nseg - 1) * every;
+    A->post_slot.assign(nsteps, -1);
+    int nphys = 0;
     for(int n = 0; n < nsteps; n++){
+        const int iter = first_iter + n;
+        if(iter % 2 == 0 || n == 0) A->post_slot[n] = nphys++;
+    }
+    const size_t fb = c->n3 * sizeof(double);
+    auto ensure = [&](std::vector<Buf>& v, size_t count) -> int {
+        if(v.size() == count && (count == 0 || v[0].bytes == fb)) return 0;
+        for(auto& b : v) adj_release(b);
+        v.assign(count, Buf{});
+        for(auto& b : v) if(int rc = adj_alloc(b, fb)) return rc;
+        return 0;
+    };
+    if(int rc = ensure(A->ckpt, (size_t)nseg * N_STATE)) return rc;
+    if(int rc = ensure(A->snap, (size_t)every * N_STATE)) return rc;
+    if(int rc = ensure(A->post, (size_t)nphys * N_POST)) return rc;
+    for(int n = 0; n < nsteps; n++){
+        const int iter = first_iter + n, first = n == 0 ? 1 : 0;
         if(n % every == 0)
             if(int rc = save_state(c, &A->ckpt[(size_t)(n / every) * N_STATE])) return rc;
-        if(int rc = atjup_hip_step(c, first_iter + n, n == 0 ? 1 : 0)) return rc;
+        if(n >= s_last)
+            if(int rc = save_state(c, &A->snap[(size_t)(n - s_last) * N_STATE])) return rc;
+        hipEvent_t t_step = timer_begin(c);
+        if(int rc = step_block(c, iter, first)) return rc;
+        if(A->post_slot[n] >= 0){
+            Buf* post = &A->post[(size_t)A->post_slot[n] * N_POST];
+            for(int s = 0; s < N_POST; s++)
+                if(int rc = copy_field(c, (double*)post[s].p, Fd(c, kPostFields[s]))) return rc;
+        }
+        if(int rc = step_rest(c, iter)) return rc;
+        timer_end(c, ATJUP_T_STEP, t_step, 0);
     }
+    A->nsteps = nsteps; A->every = every; A->first_iter = first_iter;
+    A->last_in_snap = true;
     return 0;
 }
 
@@ -1468,29 +1525,31 @@ int atjup_hip_adj_backward(atjup_ctx* c, const double* lam_final, double* lam_in
     const int nseg = (nsteps + every - 1) / every;
     for(int seg = nseg - 1; seg >= 0; seg--){
         const int s0 = seg * every, s1 = std::min(nsteps, s0 + every);
-        if(int rc = load_state(c, &A->ckpt[(size_t)seg * N_STATE])) return rc;
-        // recompute the segment, keeping each step's start and the state after its physics block
-        hipEvent_t t_rc = timer_begin(c);
-        for(int n = s0; n < s1; n++){
-            const int iter = A->first_iter + n, first = n == 0 ? 1 : 0;
-            if(int rc = save_state(c, &A->snap[(size_t)(n - s0) * N_STATE])) return rc;
-            const bool phys = (iter % 2 == 0) || first;
-            if(phys){
-                if(int rc = step_block(c, iter, first)) return rc;
-                Buf* post = &A->post[(size_t)(n - s0) * N_POST];
-                for(int s = 0; s < N_POST; s++)
-                    if(int rc = copy_field(c, (double*)post[s].p, Fd(c, kPostFields[s]))) return rc;
-            }
-            if(n + 1 < s1)
+        if(!(seg == nseg - 1 && A->last_in_snap)){
+            // recompute the segment from its checkpoint, keeping each step's start; the physics
+            // blocks' results come from the forward's store, so no pressure solve runs here
+            if(int rc = load_state(c, &A->ckpt[(size_t)seg * N_STATE])) return rc;
+            hipEvent_t t_rc = timer_begin(c);
+            for(int n = s0; n < s1; n++){
+                const int iter = A->first_iter + n;
+                if(int rc = save_state(c, &A->snap[(size_t)(n - s0) * N_STATE])) return rc;
+                if(n + 1 == s1) break;
+                if(A->post_slot[n] >= 0){
+                    const Buf* post = &A->post[(size_t)A->post_slot[n] * N_POST];
+                    for(int s = 0; s < N_POST; s++)
+                        if(int rc = copy_field(c, Fd(c, kPostFields[s]), (const double*)post[s].p)) return rc;
+                }
                 if(int rc = step_rest(c, iter)) return rc;
+            }
+            timer_end(c, ATJUP_T_ADJ_RECOMPUTE, t_rc, 0);
         }
-        timer_end(c, ATJUP_T_ADJ_RECOMPUTE, t_rc, 0);
+        A->last_in_snap = false;        // the snapshots now belong to this segment
         for(int n = s1 - 1; n >= s0; n--){
             const int iter = A->first_iter + n, first = n == 0 ? 1 : 0;
             if(int rc = load_state(c, &A->snap[(size_t)(n - s0) * N_STATE])) return rc;
-            const bool phys = (iter % 2 == 0) || first;
+            const int slot = A->post_slot[n];
             hipEvent_t t_as = timer_begin(c);
-            if(int rc = adj_step(c, iter, first, phys ? &A->post[(size_t)(n - s0) * N_POST] : nullptr)) return rc;
+            if(int rc = adj_step(c, iter, first, slot >= 0 ? &A->post[(size_t)slot * N_POST] : nullptr)) return rc;
             timer_end(c, ATJUP_T_ADJ_STEP, t_as, 0);
         }
     }

@@ -259,8 +259,9 @@ def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=N
         "gradient_over_forward": (ckpt_ms + bwd_ms) / fwd_ms,
         "loss": loss,
         "fd_directions": directions, "fd_eps": eps,
-        "fd_max_rel_err": max(errs), "fd_median_rel_err": float(np.median(errs)),
-        "within_1e-4": bool(max(errs) <= 1e-4),
+        "fd_max_rel_err": max(errs) if errs else None,
+        "fd_median_rel_err": float(np.median(errs)) if errs else None,
+        "within_1e-4": bool(max(errs) <= 1e-4) if errs else None,
     }
 
 
@@ -404,6 +405,8 @@ def main():
                     help="standard: 41x181x361 (1 deg); fine: 81x361x721 (0.5 deg, no CPU baseline)")
     ap.add_argument("--headline-only", dest="headline_only", action="store_true",
                     help="time the headline FP64 step only (no FP32 form, adjoint, CPU baseline or component legs)")
+    ap.add_argument("--adjoint-only", dest="adjoint_only", action="store_true",
+                    help="time the adjoint leg alone, without difference quotients (for kernel traces)")
     ap.add_argument("--cpu-leg", dest="cpu_leg", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-steps", dest="cpu_steps", type=int, default=10, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", dest="cpu_threads", type=int, default=0, help=argparse.SUPPRESS)
@@ -416,6 +419,12 @@ def main():
     if args.headline_only:
         args.precision_compare = False
         args.no_cpu = args.no_adjoint = args.no_restart = True
+    if args.adjoint_only:
+        from atjup_amd import inputs
+        g = inputs.Grid() if args.grid == "standard" else inputs.fine_grid()
+        kw = {"steps": 50, "every": 10} if args.grid == "fine" else {}
+        print(json.dumps({"adjoint": adjoint_leg(args.sweeps, args.omega, g=g, directions=0, **kw)}), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
