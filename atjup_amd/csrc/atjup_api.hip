@@ -105,7 +105,12 @@ void build_tables(atjup_ctx* c, std::vector<GeoIJ<T>>& geo, std::vector<PoisIJ<T
             W.denom = (T)(2.0 * num1 + 2.0 * num2 + 2.0 * num3);
             W.inv_rm = (T)(1.0 / rm);
             W.inv_rmsinthe = (T)(1.0 / (rm * sp));
-            W.pad0 = W.pad1 = (T)0;
+            double dd = 0.0;
+            for(double nv : { num1, num1, num2, num2, num3, num3 }) dd += nv;
+            float df = 0.0f;
+            for(double nv : { num1, num1, num2, num2, num3, num3 }) df += (float)nv;
+            W.den_fluid = (T)dd;
+            W.den_fluid32 = (T)df;
         }
 }
 

@@ -44,7 +44,11 @@ struct GeoIJ {
 // Per-(i, j) weights of the Poisson stencil (PressureSolver.h:327-353).
 template<typename T>
 struct PoisIJ {
-    T num1, num2, num3, denom, inv_rm, inv_rmsinthe, pad0, pad1;
+    T num1, num2, num3, denom, inv_rm, inv_rmsinthe;
+    // The denominator a fluid cell with six fluid neighbours accumulates in the sweep, 0 + num1 +
+    // num1 + num2 + num2 + num3 + num3 in that order, in FP64 (den_fluid) and in FP32 (den_fluid32,
+    // each term rounded to FP32 first): the sweep's fast path takes it instead of summing it per cell.
+    T den_fluid, den_fluid32;
 };
 
 // A solid cell's treatment in bcSolidGround (BC_Jup.h:517-589): the cell index and a code,

@@ -556,6 +556,57 @@ __device__ __forceinline__ void rb_cell(T* __restrict__ own, const T* __restrict
     own[q] = (omega == 1.0) ? p_gs : (T)((T(1) - (T)omega) * pc + (T)omega * p_gs);
 }
 
+// rb_cell where the pass writes no ghost plane at this cell and reads none on the fly (the pass
+// is not a ghost pass, or the cell is at least three rows from every outer plane): the neighbour
+// values are the stored ones. A fluid cell with six fluid neighbours sums the same six terms in the
+// same order from 0 and takes the row's precomputed denominator of that sum.
+template<typename T, typename S>
+__device__ __forceinline__ void rb_cell_plain(T* __restrict__ own, const S* __restrict__ src, const uint8_t fl,
+        const PoisIJ<double>& W, size_t q, double omega, T pim, T pip, T pjm, T pjp, T pkp, T pkm){
+    const T num1 = (T)W.num1, num2 = (T)W.num2, num3 = (T)W.num3;
+    T acc, den;
+    if((fl & 0x7f) == 0){
+        acc = T(0) + num1 * pip;
+        acc += num1 * pim;
+        acc += num2 * pjp;
+        acc += num2 * pjm;
+        acc += num3 * pkp;
+        acc += num3 * pkm;
+        den = std::is_same<T, float>::value ? (T)W.den_fluid32 : (T)W.den_fluid;
+    } else if(!(fl & FL_SELF)){
+        acc = den = T(0);
+        if(!(fl & FL_IP)){ acc += num1 * pip; den += num1; }
+        if(!(fl & FL_IM)){ acc += num1 * pim; den += num1; }
+        if(!(fl & FL_JP)){ acc += num2 * pjp; den += num2; }
+        if(!(fl & FL_JM)){ acc += num2 * pjm; den += num2; }
+        if(!(fl & FL_KP)){ acc += num3 * pkp; den += num3; }
+        if(!(fl & FL_KM)){ acc += num3 * pkm; den += num3; }
+        if(!(den > T(0))) return;
+    } else {
+        acc = (pip + pim) * num1 + (pjp + pjm) * num2 + (pkp + pkm) * num3;
+        den = (T)W.denom;
+    }
+    const T p_gs = (acc - (T)src[q]) / den;
+    own[q] = (omega == 1.0) ? p_gs : (T)((T(1) - (T)omega) * own[q] + (T)omega * p_gs);
+}
+
+// t / d and t % d for 0 <= t < 2^22 and 1 <= d < 2^22 through a float reciprocal of d: the
+// quotient it gives is off by at most one, which the remainder's range corrects.
+__device__ __forceinline__ int div_small(int t, int d, float inv_d, int& rem){
+    int qt = (int)((float)t * inv_d);
+    int r = t - qt * d;
+    if(r < 0){ qt -= 1; r += d; }
+    else if(r >= d){ qt += 1; r -= d; }
+    rem = r;
+    return qt;
+}
+
+// Does a colour pass write or read ghost planes at row (i, j)? Only the ghost pass (colour 0 of a
+// sweep after the first) does, and only within two rows of an outer plane.
+__device__ __forceinline__ bool rb_ghost_row(bool gw, int i, int j, int im, int jm){
+    return gw && (i <= 2 || i >= im - 3 || j <= 2 || j >= jm - 3);
+}
+
 template<typename T, typename S>
 __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep2(T* __restrict__ x0, T* __restrict__ x1,
         const S* __restrict__ s0, const S* __restrict__ s1,
@@ -589,10 +640,15 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep2(T* __restrict__ x0,
     T pkm0, pkp0, pkm1, pkp1;
     if(off){ pkm0 = vc.x; pkp0 = vc.y; pkm1 = vc.y; pkp1 = edge; }
     else   { pkm0 = edge; pkp0 = vc.x; pkm1 = vc.x; pkp1 = vc.y; }
-    rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 & 0xff), W, q, P, R, i, j, im, jm, gw, omega, c43, c13,
-                  vim.x, vip.x, vjm.x, vjp.x, pkp0, pkm0);
-    rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 >> 8), W, q + 1, P, R, i, j, im, jm, gw, omega, c43, c13,
-                  vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
+    if(rb_ghost_row(gw, i, j, im, jm)){
+        rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 & 0xff), W, q, P, R, i, j, im, jm, gw, omega, c43, c13,
+                      vim.x, vip.x, vjm.x, vjp.x, pkp0, pkm0);
+        rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 >> 8), W, q + 1, P, R, i, j, im, jm, gw, omega, c43, c13,
+                      vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
+    } else {
+        rb_cell_plain<T, S>(own, src, (uint8_t)(fl2 & 0xff), W, q, omega, vim.x, vip.x, vjm.x, vjp.x, pkp0, pkm0);
+        rb_cell_plain<T, S>(own, src, (uint8_t)(fl2 >> 8), W, q + 1, omega, vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
+    }
 }
 
 // The multi-cell pass with the (row, group) index flattened over the lanes, so that no lane idles
@@ -604,15 +660,16 @@ __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __res
         const S* __restrict__ s0, const S* __restrict__ s1,
         const uint8_t* __restrict__ f0, const uint8_t* __restrict__ f1, const PoisIJ<double>* __restrict__ pw,
         int im, int jm, int hp, int nm, int ntb, int nb, int colour, int ghost, double omega,
-        double c43, double c13){
+        double c43, double c13, float inv_ngrp){
     using V = typename Pair<T>::V;
     static_assert(N == 2 || N == 4, "two or four cells per thread");
     const int b = xcd_remap(blockIdx.x, nb);
     const int tb = b % ntb, i = b / ntb + 1;
     const int ngrp = nm / N;
     const int t = tb * 256 + threadIdx.x;
-    const int j = t / ngrp + 1;
-    const int m = N * (t - (j - 1) * ngrp);
+    int mg;
+    const int j = div_small(t, ngrp, inv_ngrp, mg) + 1;
+    const int m = N * mg;
     if(j > jm - 2) return;
     T* __restrict__ own = colour ? x1 : x0;
     const T* __restrict__ oth = colour ? x0 : x1;
@@ -639,13 +696,18 @@ __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __res
     if constexpr(N == 2) flw = *reinterpret_cast<const uint16_t*>(flg + q);
     else flw = *reinterpret_cast<const uint32_t*>(flg + q);
     const PoisIJ<double> W = pw[(size_t)i * jm + j];
+    const bool ghost_row = rb_ghost_row(gw, i, j, im, jm);
     #pragma unroll
     for(int r = 0; r < N; r++){
         T pkm, pkp;
         if(off){ pkm = vc[r]; pkp = (r + 1 < N) ? vc[r + 1] : edge; }
         else   { pkm = (r > 0) ? vc[r - 1] : edge; pkp = vc[r]; }
-        rb_cell<T, S>(own, oth, src, (uint8_t)((flw >> (8 * r)) & 0xff), W, q + r, P, R, i, j, im, jm, gw, omega,
-                      c43, c13, vim[r], vip[r], vjm[r], vjp[r], pkp, pkm);
+        const uint8_t fl = (uint8_t)((flw >> (8 * r)) & 0xff);
+        if(ghost_row)
+            rb_cell<T, S>(own, oth, src, fl, W, q + r, P, R, i, j, im, jm, gw, omega, c43, c13,
+                          vim[r], vip[r], vjm[r], vjp[r], pkp, pkm);
+        else
+            rb_cell_plain<T, S>(own, src, fl, W, q + r, omega, vim[r], vip[r], vjm[r], vjp[r], pkp, pkm);
     }
 }
 
@@ -1287,14 +1349,17 @@ int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* 
     const int wf_pair = (nm / 2 + RB_BM - 1) / RB_BM;
     const bool pair_type = std::is_same<T, double>::value || std::is_same<T, float>::value;
     const bool pairs_ok = (nm % 2 == 0) && pair_type;
-    const bool pairs_flat = pairs_ok && (double)(nm / 2) / (RB_BM * wf_pair) < 0.9;
+    // the flattened passes find a thread's row through a float reciprocal (div_small): fewer than
+    // 2^22 groups per level
+    const bool flat_ok = (long long)(c->jm - 2) * (nm / 2) + 255 < (1LL << 22);
+    const bool pairs_flat = pairs_ok && flat_ok && (double)(nm / 2) / (RB_BM * wf_pair) < 0.9;
     const bool pairs = pairs_ok && !pairs_flat;
     const int ntb = ((c->jm - 2) * (nm / 2) + 255) / 256;          // flattened pairs per level
     const int nb_flat = ntb * (c->im - 2);
     // FP32: four cells per thread (flattened) where the ring allows; one float4 per neighbour row.
     // 41 x 181 x 361: 7.4 us against 7.9 us for flattened pairs; FP64 quads (two 16-byte loads per
     // row) are slower than FP64 pairs (11.8 against 10.4 us).
-    const bool quads = pairs_ok && nm % 4 == 0 && std::is_same<T, float>::value;
+    const bool quads = pairs_ok && flat_ok && nm % 4 == 0 && std::is_same<T, float>::value;
     const int ntb4 = ((c->jm - 2) * (nm / 4) + 255) / 256;
     const int nb_quad = ntb4 * (c->im - 2);
     const int nmb = pairs ? (nm / 2 + RB_BM - 1) / RB_BM : (nm + RB_BM - 1) / RB_BM;
@@ -1308,12 +1373,12 @@ int run_sweeps(atjup_ctx* c, int kind, int n, int ghost_first, double omega, T* 
                     if(quads){
                         hipLaunchKernelGGL((k_rb_sweep2f<T, S, 4>), dim3(nb_quad), dim3(256), 0, c->stream,
                             x0, x1, s0, s1, f0, f1, pw, c->im, c->jm, hp, nm, ntb4, nb_quad, colour,
-                            gh, omega, c43, c13);
+                            gh, omega, c43, c13, 1.0f / (float)(nm / 4));
                         continue;
                     } else if(pairs_flat){
                         hipLaunchKernelGGL((k_rb_sweep2f<T, S, 2>), dim3(nb_flat), dim3(256), 0, c->stream,
                             x0, x1, s0, s1, f0, f1, pw, c->im, c->jm, hp, nm, ntb, nb_flat, colour,
-                            gh, omega, c43, c13);
+                            gh, omega, c43, c13, 1.0f / (float)(nm / 2));
                         continue;
                     } else if(pairs){
                         hipLaunchKernelGGL((k_rb_sweep2<T, S>), dim3(nb_sw), dim3(RB_BM, RB_BJ), 0, c->stream,
