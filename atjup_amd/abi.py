@@ -113,6 +113,24 @@ class PrecipParams(C.Structure):
     _fields_ = [("rate_scale", C.c_double), ("t_0_nh4sh", C.c_double), ("t_00_nh4sh", C.c_double)]
 
 
+class Obstacle(C.Structure):
+    """atjup_obstacle: the smooth drag region standing in for the obstacle in gradients with
+    respect to its semi-axes (no counterpart in the reference)."""
+    _fields_ = [("j0", C.c_double), ("k0", C.c_double), ("a_phi_deg", C.c_double), ("b_the_deg", C.c_double),
+                ("c_r_km", C.c_double), ("width", C.c_double), ("tau", C.c_double)]
+
+
+def obstacle(g: inputs.Grid, a_phi_deg=20.0, b_the_deg=10.0, c_r_km=30.0, width=0.25, tau=0.05,
+             j0=None, k0=None) -> Obstacle:
+    """The drag region of the benchmark's obstacle (22 S, 20 deg x 10 deg x 30 km, centred as
+    inputs.ellipsoid_mask centres it): chi = 1 / (1 + exp(-(1 - s) / width)) and a drag time tau
+    of ten time steps."""
+    j0 = round(112 * (g.jm - 1) / 180) if j0 is None else j0
+    k0 = round(180 * (g.km - 1) / 360) if k0 is None else k0
+    return Obstacle(j0=float(j0), k0=float(k0), a_phi_deg=a_phi_deg, b_the_deg=b_the_deg, c_r_km=c_r_km,
+                    width=width, tau=tau)
+
+
 def precip_params(**over) -> PrecipParams:
     d = dict(rate_scale=1.0, t_0_nh4sh=230.0, t_00_nh4sh=200.0)
     d.update(over)

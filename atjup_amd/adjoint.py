@@ -12,9 +12,12 @@ checkpointed reverse sweep (atjup_hip_adj_backward) and returns dL/dw0.
 torch is used on the CPU only: the device work runs in the library's own HIP runtime, and the
 tensors cross as host arrays (the torch wheel's bundled HIP runtime is a different one).
 
-The obstacle is the reference's binary mask, whose derivative with respect to its semi-axes is zero
-almost everywhere; a gradient with respect to the semi-axes needs a differentiable obstacle model
-the reference does not have, so none is offered here.
+The reference's obstacle is a binary mask, whose derivative with respect to its semi-axes is zero
+almost everywhere. For gradients with respect to the semi-axes the runner takes a smooth drag region
+instead (abi.Obstacle, atjup_hip_set_smooth_obstacle: -chi u / tau in the momentum equations with
+chi a logistic function of the ellipsoid's normalised distance), with an all-fluid binary mask; the
+forward that difference quotients differentiate is then that model, and ``JetMisfit.apply(w0,
+runner, axes)`` returns L with dL/dw0 and dL/d(a_phi_deg, b_the_deg, c_r_km) as its gradients.
 """
 from __future__ import annotations
 
@@ -31,8 +34,12 @@ except Exception:  # pragma: no cover
 class MisfitRunner:
     """One FP64 context holding the initial state, and the loss definition."""
 
-    def __init__(self, ctx, init_state: dict, target, nsteps=20, every=5, i_top=None):
+    def __init__(self, ctx, init_state: dict, target, nsteps=20, every=5, i_top=None, obstacle=None):
         self.ctx = ctx
+        # the smooth drag region (abi.Obstacle) whose semi-axes set_axes() changes, or None
+        self.obstacle = obstacle
+        if obstacle is not None:
+            ctx.set_smooth_obstacle(obstacle)
         self.init = {n: np.array(v, dtype=np.float64) for n, v in init_state.items()}
         im, jm, km = ctx.shape
         self.target = np.asarray(target, dtype=np.float64)
@@ -45,6 +52,7 @@ class MisfitRunner:
         # belonging to an earlier forward is refused instead of returning another one's gradient.
         self.generation = 0
         self._lam = None
+        self.axes_grad = None
         ctx.adj_init()
 
     def loss_final(self):
@@ -54,6 +62,14 @@ class MisfitRunner:
         lam = np.zeros((4,) + w.shape)
         lam[3, self.i_top, :, :-1] = (2.0 * self.cos_lat * r / (km - 1))[:, None]
         return float(np.sum(self.cos_lat * r * r)), lam
+
+    def set_axes(self, axes):
+        """The drag region's semi-axes (a_phi_deg, b_the_deg, c_r_km) for the next forward."""
+        if self.obstacle is None:
+            raise RuntimeError("MisfitRunner: no smooth drag region (pass obstacle=abi.obstacle(...))")
+        o = self.obstacle
+        o.a_phi_deg, o.b_the_deg, o.c_r_km = (float(x) for x in axes)
+        self.ctx.set_smooth_obstacle(o)
 
     def load(self, w0):
         for n, v in self.init.items():
@@ -86,6 +102,8 @@ class MisfitRunner:
                                "forward or evaluate on the same runner; run the forward again (one runner "
                                "holds one forward at a time)")
         g = self.ctx.adj_backward(scale * self._lam)[3]
+        # dL/d(a_phi_deg, b_the_deg, c_r_km) of the drag region, from the same reverse sweep
+        self.axes_grad = self.ctx.adj_obstacle_gradient() if self.obstacle is not None else None
         self._lam = None          # the reverse sweep consumed the checkpoints' state
         return g
 
@@ -101,16 +119,22 @@ def _coords(ctx):
 
 if torch is not None:
     class JetMisfit(torch.autograd.Function):
-        """L(w0) with the adjoint as its backward (FP64, CPU tensors)."""
+        """L(w0[, axes]) with the adjoint as its backward (FP64, CPU tensors). axes: the drag
+        region's (a_phi_deg, b_the_deg, c_r_km) as a tensor of 3, for a runner built with an
+        obstacle; its gradient comes from the same reverse sweep."""
 
         @staticmethod
-        def forward(fctx, w0, runner: MisfitRunner):
+        def forward(fctx, w0, runner: MisfitRunner, axes=None):
+            if axes is not None:
+                runner.set_axes(axes.detach().cpu().numpy().astype(np.float64))
             loss = runner.forward(w0.detach().cpu().numpy().astype(np.float64))
             fctx.runner = runner
             fctx.generation = runner.generation
+            fctx.has_axes = axes is not None
             return torch.tensor(loss, dtype=torch.float64)
 
         @staticmethod
         def backward(fctx, grad_out):
             g = fctx.runner.backward(float(grad_out), generation=fctx.generation)
-            return torch.from_numpy(g), None
+            ga = torch.from_numpy(fctx.runner.axes_grad.copy()) if fctx.has_axes else None
+            return torch.from_numpy(g), None, ga

@@ -343,6 +343,88 @@ __global__ __launch_bounds__(256) void ka_brl_spread(const double* __restrict__ 
 }
 
 // =============================================================================================
+// The smooth drag region standing in for the obstacle in gradients with respect to its
+// semi-axes: chi = 1 / (1 + exp(-(1 - s) / width)), s the ellipsoid's normalised squared distance
+// in index space, and dchi/d(a_phi_deg, b_the_deg, c_r_km) for the gradient's reduction.
+// =============================================================================================
+constexpr int OBST_BLOCKS = 1024;
+
+struct ObstGeom {
+    double j0, k0;
+    double a, b, c;              // semi-axes in cells (phi, theta) and levels (r)
+    double da, db, dc;           // d(a, b, c) / d(a_phi_deg, b_the_deg, c_r_km)
+    double width;
+    int im, jm, km, pitch;
+};
+
+// chi at (i, j, k) and, through ds, the derivatives of s with respect to a, b, c (cells)
+__device__ __forceinline__ double obst_chi(const ObstGeom& o, int i, int j, int k, double* ds){
+    const int kk = (k == o.km - 1) ? 0 : k;                  // the seam is the copy of k = 0
+    const double xk = (kk - o.k0) / o.a, xj = (j - o.j0) / o.b, xi = i / o.c;
+    const double s = xk * xk + xj * xj + xi * xi;
+    if(ds){
+        ds[0] = -2.0 * xk * xk / o.a;
+        ds[1] = -2.0 * xj * xj / o.b;
+        ds[2] = -2.0 * xi * xi / o.c;
+    }
+    const double x = (1.0 - s) / o.width;
+    return (x < -36.8) ? 0.0 : 1.0 / (1.0 + exp(-x));       // below ~1e-16 it is taken as 0
+}
+
+__global__ __launch_bounds__(256) void k_obstacle_chi(double* __restrict__ chi, const ObstGeom o){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= o.km) return;
+    chi[((size_t)i * o.jm + j) * o.pitch + k] = obst_chi(o, i, j, k, nullptr);
+}
+
+// sum over cells of dL/dchi * dchi/dp, p = the three semi-axes in their own units; per-block
+// partial sums, added on the host in block order
+__global__ __launch_bounds__(256) void ka_obstacle_grad(const double* __restrict__ gchi, double* __restrict__ part,
+        const ObstGeom o){
+    double acc[3] = { 0.0, 0.0, 0.0 };
+    const size_t n = (size_t)o.im * o.jm * o.km;
+    for(size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x){
+        const int k = (int)(c % o.km);
+        const size_t r = c / o.km;
+        const int j = (int)(r % o.jm), i = (int)(r / o.jm);
+        const double g = gchi[((size_t)i * o.jm + j) * o.pitch + k];
+        if(g == 0.0) continue;
+        double ds[3];
+        const double x = obst_chi(o, i, j, k, ds);
+        if(x == 0.0) continue;
+        const double dchi_ds = -x * (1.0 - x) / o.width;
+        acc[0] += g * dchi_ds * ds[0] * o.da;
+        acc[1] += g * dchi_ds * ds[1] * o.db;
+        acc[2] += g * dchi_ds * ds[2] * o.dc;
+    }
+    __shared__ double sh[3][256];
+    for(int p = 0; p < 3; p++) sh[p][threadIdx.x] = acc[p];
+    __syncthreads();
+    for(int s = 128; s > 0; s >>= 1){
+        if((int)threadIdx.x < s)
+            for(int p = 0; p < 3; p++) sh[p][threadIdx.x] += sh[p][threadIdx.x + s];
+        __syncthreads();
+    }
+    if(threadIdx.x == 0)
+        for(int p = 0; p < 3; p++) part[(size_t)blockIdx.x * 3 + p] = sh[p][0];
+}
+
+static ObstGeom obst_geom(const atjup_ctx* c){
+    const double deg = M_PI / 180.0;
+    ObstGeom o{};
+    o.j0 = c->obst.j0; o.k0 = c->obst.k0;
+    o.da = deg / c->g.dphi;
+    o.db = deg / c->g.dthe;
+    o.dc = 1.0 / (c->g.L_atm * c->g.dr);
+    o.a = c->obst.a_phi_deg * o.da;
+    o.b = c->obst.b_the_deg * o.db;
+    o.c = c->obst.c_r_km * o.dc;
+    o.width = c->obst.width;
+    o.im = c->im; o.jm = c->jm; o.km = c->km; o.pitch = c->pitch;
+    return o;
+}
+
+// =============================================================================================
 // RHS of the dry stage, transposed (RHS_Jup_Turb.cpp:31-1457 as restated by k_stage_dry)
 // =============================================================================================
 struct RhsAdjArgs {
@@ -358,6 +440,9 @@ struct RhsAdjArgs {
     size_t plane;
     double inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
     double cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, cbuoy, g_, rR, t_ref, kt, visc0;
+    const double* chi;           // the smooth drag region's indicator (else null)
+    double inv_tau;
+    double* gchi;                // dL/dchi, accumulated over stages and steps
 };
 
 __global__ __launch_bounds__(256) void ka_rhs_dry(const RhsAdjArgs a, const GeoIJ<double>* __restrict__ geo){
@@ -412,6 +497,14 @@ __global__ __launch_bounds__(256) void ka_rhs_dry(const RhsAdjArgs a, const GeoI
     double Lc[4] = { 0.0, 0.0, 0.0, 0.0 };                         // centre-value adjoints
     double L1r[4], L1t[4], L1p[4], L2r[4], L2t[4], L2p[4];
     double L_vinv = 0.0, L_winv = 0.0;
+    if(a.chi){
+        // the drag region -chi / tau (u, v, w): its transpose, and dL/dchi of this stage
+        const double pen = a.chi[q] * a.inv_tau;
+        Lc[1] += -pen * Lu;
+        Lc[2] += -pen * Lv;
+        Lc[3] += -pen * Lw;
+        a.gchi[q] += -(u * Lu + v * Lv + w * Lw) * a.inv_tau;
+    }
     // transport
     const double Ltr[4] = { -Lt, -Lu, -Lv, -Lw };
     for(int f = 0; f < 4; f++){
@@ -1027,6 +1120,8 @@ struct AdjState {
     struct SweepGraph { int n; double omega; hipGraphExec_t exec; const void* ptrs[8]; int dims[4]; };
     std::vector<SweepGraph> graphs;   // the transposed sweep sequences, replayed
     Buf brl_part;                 // per-level partial sums of the level-mean transpose
+    Buf gchi;                     // dL/dchi of the drag region, [n3]
+    Buf obst_part;                // per-block partial sums of dL/d(semi-axes)
     int nsteps = 0, every = 0, first_iter = 1;
     bool last_in_snap = false;    // snap holds the last segment's step states, as the forward left them
 };
@@ -1052,7 +1147,8 @@ void adj_free(atjup_ctx* c){
     AdjState* A = c->adj;
     for(auto& b : A->lam) adj_release(b);
     for(auto& b : A->work) adj_release(b);
-    for(Buf* b : { &A->lsplit, &A->lsrc, &A->mu, &A->inmask, &A->red, &A->zs, &A->brl_part }) adj_release(*b);
+    for(Buf* b : { &A->lsplit, &A->lsrc, &A->mu, &A->inmask, &A->red, &A->zs, &A->brl_part, &A->gchi, &A->obst_part })
+        adj_release(*b);
     for(auto& b : A->ckpt) adj_release(b);
     for(auto& b : A->snap) adj_release(b);
     for(auto& b : A->post) adj_release(b);
@@ -1149,6 +1245,9 @@ static int adj_rk4(atjup_ctx* c){
     a.r_mix = C.r_mix; a.c_m2o = 1.0 * -2.0 * C.omega; a.c_p2o = 1.0 * +2.0 * C.omega;
     a.Coriolis = C.Coriolis; a.cbuoy = -1.0 * 1.0 * 1.0 * C.buoyancy; a.g_ = C.g; a.rR = C.r_mix * C.R_mix;
     a.t_ref = C.t_ref; a.kt = c->kn.t_diff_scale / (C.re * C.pr) + 0.0 / 0.9; a.visc0 = 1.0 / C.re + 0.0;
+    a.chi = c->obst_on ? (const double*)c->chi.p : nullptr;
+    a.inv_tau = c->obst_on ? 1.0 / c->obst.tau : 0.0;
+    a.gchi = (double*)A->gchi.p;
     for(int f = 0; f < 4; f++){
         a.lk[f] = W(c, AW_LK + f);
         a.g[f] = W(c, AW_G + f);
@@ -1419,6 +1518,8 @@ int atjup_hip_adj_init(atjup_ctx* c){
     if(!rc) rc = adj_alloc(A->inmask, 2 * hn);
     if(!rc) rc = adj_alloc(A->red, 2 * (size_t)c->im * sizeof(double));
     if(!rc) rc = adj_alloc(A->brl_part, 2 * (size_t)c->im * ABRL_NCH * sizeof(double));
+    if(!rc) rc = adj_alloc(A->gchi, fb);
+    if(!rc) rc = adj_alloc(A->obst_part, 3 * (size_t)OBST_BLOCKS * sizeof(double));
     if(!rc) rc = adj_alloc(A->zs, 4 * (size_t)c->jm * sizeof(double));
     if(rc){ adj_free(c); return rc; }
     return 0;
@@ -1429,6 +1530,47 @@ int atjup_hip_adj_zero(atjup_ctx* c){
     for(int f : kLamFields) if(int rc = zero_buf(c, c->adj->lam[f])) return rc;
     if(int rc = zero_buf(c, c->adj->work[AW_GBRL])) return rc;
     for(int f = 0; f < 4; f++) if(int rc = zero_buf(c, c->adj->work[AW_G + f])) return rc;
+    return zero_buf(c, c->adj->gchi);
+}
+
+int atjup_hip_set_smooth_obstacle(atjup_ctx* c, const atjup_obstacle* p){
+    if(!c){ set_error("null context"); return ATJUP_ERR_ARG; }
+    if(!p){ c->obst_on = false; return 0; }
+    if(!c->kn.dry){ set_error("the smooth drag region belongs to the dynamics profile"); return ATJUP_ERR_PROFILE; }
+    const double v[5] = { p->a_phi_deg, p->b_the_deg, p->c_r_km, p->width, p->tau };
+    for(double x : v)
+        if(!(x > 0.0) || !std::isfinite(x)){ set_error("semi-axes, width and tau must be positive"); return ATJUP_ERR_ARG; }
+    if(!std::isfinite(p->j0) || !std::isfinite(p->k0)){ set_error("centre"); return ATJUP_ERR_ARG; }
+    if(!c->chi.p){
+        const size_t bytes = c->n3 * sizeof(double);
+        if(hipMalloc(&c->chi.p, bytes) != hipSuccess){ c->chi.p = nullptr; set_error("hipMalloc failed"); return ATJUP_ERR_NOMEM; }
+        c->chi.bytes = bytes;
+        ATJUP_HIP_CHECK(hipMemsetAsync(c->chi.p, 0, bytes, c->stream));
+    }
+    c->obst = *p;
+    const ObstGeom o = obst_geom(c);
+    hipLaunchKernelGGL(k_obstacle_chi, dim3(nblk(c->km, 256), c->jm, c->im), dim3(256), 0, c->stream,
+                       (double*)c->chi.p, o);
+    if(int rc = check_launch()) return rc;
+    c->obst_on = true;
+    return 0;
+}
+
+int atjup_hip_adj_obstacle_gradient(atjup_ctx* c, double* out){
+    if(int rc = need_adj(c)) return rc;
+    if(!out){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(!c->obst_on){ set_error("no smooth drag region is set (atjup_hip_set_smooth_obstacle)"); return ATJUP_ERR_STATE; }
+    AdjState* A = c->adj;
+    const ObstGeom o = obst_geom(c);
+    hipLaunchKernelGGL(ka_obstacle_grad, dim3(OBST_BLOCKS), dim3(256), 0, c->stream, (const double*)A->gchi.p,
+                       (double*)A->obst_part.p, o);
+    if(int rc = check_launch()) return rc;
+    std::vector<double> h(3 * (size_t)OBST_BLOCKS);
+    ATJUP_HIP_CHECK(hipMemcpyAsync(h.data(), A->obst_part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    out[0] = out[1] = out[2] = 0.0;
+    for(int b = 0; b < OBST_BLOCKS; b++)
+        for(int p = 0; p < 3; p++) out[p] += h[(size_t)b * 3 + p];
     return 0;
 }
 

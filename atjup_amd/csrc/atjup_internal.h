@@ -123,6 +123,9 @@ struct atjup_ctx {
     atjup_chem_params chemp = {};   // the chemistry's constants and knobs (atjup_hip_set_chemistry)
     atjup::Buf chem_w;              // double [3][n3]: the reaction rates w_nh3, w_h2s, w_nh4sh
     atjup::Buf report;              // the reporting reductions' partial results and slice staging
+    atjup_obstacle obst = {};       // the smooth drag region (atjup_hip_set_smooth_obstacle)
+    bool obst_on = false;
+    atjup::Buf chi;                 // double [n3]: its indicator chi, when set
     atjup_mca_params mcap = {};     // the moist convective adjustment's knobs (atjup_hip_set_mca)
     atjup::Buf mca_ws;              // its layer heights [im] (m), then the column workspace
                                     // double [8][im][jm * km] and the flags uint8 [im][jm * km]

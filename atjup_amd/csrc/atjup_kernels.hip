@@ -141,6 +141,8 @@ struct StageArgs {
     T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, t_ref, kt, visc0;
     float* lo[4];                          // FP32-refined form: the low words of t, u, v, w (else null)
     double dt_d;
+    const double* chi;                     // the smooth drag region's indicator (else null)
+    double inv_tau;                        // its inverse drag time
 };
 
 // The FP32-refined form carries t, u, v, w as an FP32 pair hi + lo. The last RK stage adds the
@@ -248,12 +250,19 @@ __global__ __launch_bounds__(ST_BK * ST_BJ) void k_stage_dry(const StageArgs<T> 
     const T buoyancy_u = (tq > T(0))
         ? (T)(a.cbuoy * (a.g * (acc_t<T>)a.ps[q] / (a.rR * (acc_t<T>)tq * a.t_ref_d) - a.brl[i])) : T(0);
     const T visc = a.visc0 + a.wn[q];
-    const T rhs_u = - dpdr + buoyancy_u - transport_u + diffusion_u * visc
+    T rhs_u = - dpdr + buoyancy_u - transport_u + diffusion_u * visc
         + a.centrifugal * G.cent_rad;
-    const T rhs_v = - dpdthe_term - dphdthe_term - transport_v + diffusion_v * visc
+    T rhs_v = - dpdthe_term - dphdthe_term - transport_v + diffusion_v * visc
         - a.Coriolis * Coriolis_the + a.centrifugal * G.cent_the;
-    const T rhs_w = - dpdphi_term - dphdphi_term - transport_w + diffusion_w * visc
+    T rhs_w = - dpdphi_term - dphdphi_term - transport_w + diffusion_w * visc
         - a.Coriolis * Coriolis_phi;
+    if(a.chi){
+        // the smooth drag region, -chi / tau on each velocity component
+        const T pen = (T)(a.chi[q] * a.inv_tau);
+        rhs_u = rhs_u - pen * u;
+        rhs_v = rhs_v - pen * v;
+        rhs_w = rhs_w - pen * w;
+    }
     const T rhs[4] = { rhs_t, rhs_u, rhs_v, rhs_w };
 
     if(a.stage == 0){
@@ -1296,6 +1305,8 @@ int do_rk4_dry(atjup_ctx* c, const RkBuffers* rb = nullptr){
     a.g = C.g; a.rR = C.r_mix * C.R_mix; a.t_ref_d = C.t_ref; a.t_ref = (T)C.t_ref;
     a.kt = (T)(c->kn.t_diff_scale / (C.re * C.pr) + 0.0 / 0.9);
     a.visc0 = (T)(1.0 / C.re + 0.0);
+    a.chi = c->obst_on ? (const double*)c->chi.p : nullptr;
+    a.inv_tau = c->obst_on ? 1.0 / c->obst.tau : 0.0;
     for(int s = 0; s < 4; s++){
         a.stage = s;
         a.c_in = (T)((s == 0 || s == 1) ? 0.5 * dt : (s == 2 ? dt : 0.0));

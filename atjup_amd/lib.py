@@ -92,6 +92,8 @@ def load():
     L.atjup_hip_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.atjup_hip_read_timers.argtypes = [C.c_void_p, _D]
     L.atjup_hip_libm_probe.argtypes = [C.c_int, _D, _D, _D, C.c_long]
+    L.atjup_hip_set_smooth_obstacle.argtypes = [C.c_void_p, C.POINTER(abi.Obstacle)]
+    L.atjup_hip_adj_obstacle_gradient.argtypes = [C.c_void_p, _D]
     _lib = L
     return L
 
@@ -345,6 +347,17 @@ class Context:
 
     def adj_op(self, op, iarg=0, darg=0.0):
         self._chk(self.L.atjup_hip_adj_op(self.h, self.ADJ_OPS[op], int(iarg), float(darg)), f"adj_op {op}")
+
+    def set_smooth_obstacle(self, obst=None):
+        """The smooth drag region (abi.Obstacle), or None to remove it."""
+        self._chk(self.L.atjup_hip_set_smooth_obstacle(self.h, None if obst is None else C.byref(obst)),
+                  "set_smooth_obstacle")
+
+    def adj_obstacle_gradient(self):
+        """dL/d(a_phi_deg, b_the_deg, c_r_km) of the last adj_backward."""
+        out = np.zeros(3)
+        self._chk(self.L.atjup_hip_adj_obstacle_gradient(self.h, _dp(out)), "adj_obstacle_gradient")
+        return out
 
     def adj_forward(self, nsteps, every, first_iter=1):
         self._chk(self.L.atjup_hip_adj_forward_checkpointed(self.h, nsteps, every, first_iter), "adj_forward")

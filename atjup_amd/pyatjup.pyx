@@ -32,6 +32,10 @@ cdef extern from "atjup_hip.h":
         pass
     ctypedef struct atjup_gas_params:
         pass
+    ctypedef struct atjup_obstacle:
+        pass
+    int atjup_hip_set_smooth_obstacle(atjup_ctx* ctx, const atjup_obstacle* params)
+    int atjup_hip_adj_obstacle_gradient(atjup_ctx* ctx, double* out)
     ctypedef struct atjup_ctx:
         pass
     atjup_ctx* atjup_hip_ctx_create(const atjup_grid* grid, const atjup_consts* consts,
@@ -115,6 +119,7 @@ cdef class Jupiter:
     cdef public int iteration           # iterations run on the device-resident state
     cdef readonly int generation        # changes of the device state; a backward must match its forward's
     cdef object _adj_init               # the adjoint's initial state (adjoint_initial_state)
+    cdef object _obst                   # the smooth drag region (abi.Obstacle) or None
     cdef int _fwd_gen                   # generation right after the last adjoint_forward
 
     def __cinit__(self):
@@ -135,6 +140,7 @@ cdef class Jupiter:
                             g=25.92, omega=0.000176, u_0=100.0, t_ref=165.0, sweeps=int(sweeps))
         self.state = None
         self._adj_init = None
+        self._obst = None
         self.iteration = 0
 
     # ---- the reference binding's calls -------------------------------------------------------
@@ -339,14 +345,45 @@ cdef class Jupiter:
         _err(atjup_hip_adj_backward(self._ctx, _ptr(lf), _ptr(out)), "adj_backward")
         return out
 
+    def set_smooth_obstacle(self, a_phi_deg=20.0, b_the_deg=10.0, c_r_km=30.0, width=0.25, tau=0.05,
+                            j0=None, k0=None):
+        """The smooth drag region (no counterpart in the reference, whose obstacle is the binary
+        SeaMount mask): -chi u / tau in the momentum equations with chi = 1 / (1 + exp(-(1 - s) /
+        width)), s the ellipsoid's normalised squared distance; for gradients with respect to its
+        semi-axes (adjoint_obstacle_gradient). The centre defaults to the benchmark obstacle's."""
+        self._check_alive(create=True)
+        o = abi.obstacle(self._grid, a_phi_deg, b_the_deg, c_r_km, width=width, tau=tau, j0=j0, k0=k0)
+        cdef size_t ao = ctypes.addressof(o)
+        self.generation += 1
+        _err(atjup_hip_set_smooth_obstacle(self._ctx, <atjup_obstacle*>ao), "set_smooth_obstacle")
+        self._obst = o
 
-def jet_misfit(Jupiter jup, w0, target, int nsteps=20, int every=5, i_top=None):
+    def set_obstacle_axes(self, axes):
+        """The drag region's semi-axes (a_phi_deg, b_the_deg, c_r_km), its other parameters kept."""
+        if self._obst is None:
+            raise AtjupError("no smooth drag region is set (set_smooth_obstacle)")
+        o = self._obst
+        a, b, c = (float(x) for x in axes)
+        self.set_smooth_obstacle(a, b, c, o.width, o.tau, o.j0, o.k0)
+
+    def adjoint_obstacle_gradient(self):
+        """dL/d(a_phi_deg, b_the_deg, c_r_km) of the loss the last adjoint_backward carried back."""
+        self._check_alive()
+        out = np.zeros(3)
+        _err(atjup_hip_adj_obstacle_gradient(self._ctx, _ptr(out)), "adj_obstacle_gradient")
+        return out
+
+
+def jet_misfit(Jupiter jup, w0, target, int nsteps=20, int every=5, i_top=None, axes=None):
     """L(w0) and its seed for the reverse sweep: sets the zonal wind w to w0 on the model's initial
-    state, runs the checkpointed forward and forms the misfit at level i_top (default im-2)."""
+    state (and, with axes, the drag region's semi-axes), runs the checkpointed forward and forms the
+    misfit at level i_top (default im-2)."""
     im, jm, km = jup.shape
     i_top = im - 2 if i_top is None else int(i_top)
     if jup.profile != "dyn" or jup.precision_name != "fp64":
         raise AtjupError("the adjoint covers the FP64 dynamics profile: Jupiter(profile='dyn', precision='fp64')")
+    if axes is not None:
+        jup.set_obstacle_axes(axes)
     for n, v in jup.adjoint_initial_state().items():
         jup.set_field(n, w0 if n == "w" else v)
     jup.adjoint_forward(nsteps, every)
@@ -372,10 +409,13 @@ try:
         same model raises, as does a second backward of one forward."""
 
         @staticmethod
-        def forward(fctx, w0, Jupiter jup, target, nsteps=20, every=5):
-            loss, lam = jet_misfit(jup, w0.detach().cpu().numpy().astype(np.float64), target, nsteps, every)
+        def forward(fctx, w0, Jupiter jup, target, nsteps=20, every=5, axes=None, i_top=None):
+            ax = None if axes is None else axes.detach().cpu().numpy().astype(np.float64)
+            loss, lam = jet_misfit(jup, w0.detach().cpu().numpy().astype(np.float64), target, nsteps, every,
+                                   i_top=i_top, axes=ax)
             fctx.jup, fctx.lam = jup, lam
             fctx.gen = jup.generation
+            fctx.has_axes = axes is not None
             return torch.tensor(loss, dtype=torch.float64)
 
         @staticmethod
@@ -384,6 +424,7 @@ try:
                 raise RuntimeError("JetMisfit.backward: the model's state changed after this forward (a later "
                                    "forward replaced its checkpoints); run the forward again")
             g = fctx.jup.adjoint_backward(float(grad_out) * fctx.lam)[3]
-            return torch.from_numpy(g), None, None, None, None
+            ga = torch.from_numpy(fctx.jup.adjoint_obstacle_gradient()) if fctx.has_axes else None
+            return torch.from_numpy(g), None, None, None, None, ga, None
 except ImportError:  # torch is optional for the rest of the binding
     JetMisfit = None

@@ -227,6 +227,10 @@ def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=N
 
     run.load(w0)
     clock(lambda: [ctx.step(1 + n, n == 0) for n in range(steps)])        # warm-up
+    # warm-up of the gradient as well (its checkpoint buffers are allocated on the first call and
+    # kept for calls of the same shape), as the forward's warm-up above
+    run.forward(w0)
+    run.backward()
     run.load(w0)
     fwd_ms = clock(lambda: [ctx.step(1 + n, n == 0) for n in range(steps)])
     run.load(w0)
@@ -262,6 +266,60 @@ def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=N
         "fd_max_rel_err": max(errs) if errs else None,
         "fd_median_rel_err": float(np.median(errs)) if errs else None,
         "within_1e-4": bool(max(errs) <= 1e-4) if errs else None,
+    }
+
+
+def semi_axes_leg(sweeps, omega, steps=20, every=5, rel_eps=1.0e-4, g=None, i_top=None):
+    """The misfit's gradient with respect to the obstacle's semi-axes (a_phi_deg, b_the_deg,
+    c_r_km = 20, 10, 30): the binary obstacle replaced by the smooth drag region of the same
+    ellipsoid (all-fluid mask), dL/daxes from one checkpointed forward and reverse sweep, against
+    central differences of that forward (eps = rel_eps x the semi-axis). i_top: the misfit's level
+    (default im - 2, the benchmark's top level)."""
+    from atjup_amd import abi, inputs
+    from atjup_amd.adjoint import MisfitRunner
+    from atjup_amd.lib import Context
+
+    g = g or inputs.Grid()
+    G, Cn, K, _ = abi.make_profile("dyn", sweeps=sweeps, omega=omega, grid=g)
+    st = inputs.make_state("dyn", seed=11, g=g, with_obstacle=False)
+    ctx = Context(G, Cn, K, abi.PRECISION_FP64)
+    ctx.load_state(st, skip=set(inputs.SPECIES))
+    ctx.bc_after_step()
+    ctx.swap_nlevel()
+    ctx.mixture_density()
+    init = {n: ctx.get(n) for n in ("t", "u", "v", "w", "p_dyn", "aux_u", "aux_v", "aux_w", "rho_mix")}
+    target = inputs.zonal_jets(g, np.random.default_rng(12)) / 100.0
+    axes0 = np.array([20.0, 10.0, 30.0])
+    if i_top is not None:
+        target = init["w"][i_top, :, :-1].mean(axis=1) * 0.9
+    run = MisfitRunner(ctx, init, target, nsteps=steps, every=every, obstacle=abi.obstacle(g, *axes0), i_top=i_top)
+    w0 = init["w"]
+    ctx.sync()
+    t0 = time.perf_counter()
+    loss = run.forward(w0)
+    run.backward()
+    ctx.sync()
+    grad_ms = (time.perf_counter() - t0) * 1e3
+    ga = run.axes_grad.copy()
+    fds, rels = [], []
+    for p in range(3):
+        eps = rel_eps * axes0[p]
+        vals = []
+        for sgn in (+1, -1):
+            ax = axes0.copy()
+            ax[p] += sgn * eps
+            run.set_axes(ax)
+            vals.append(run.evaluate(w0))
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        fds.append(fd)
+        rels.append(abs(fd - ga[p]) / max(abs(fd), abs(ga[p]), 1e-300))
+    ctx.close()
+    return {
+        "workload": f"jet misfit at i = {run.i_top} after {steps} FP64 dynamics steps, {g.im}x{g.jm}x{g.km}, "
+                    f"dL/d(a_phi_deg, b_the_deg, c_r_km) of the smooth drag region (20 deg x 10 deg x 30 km, width "
+                    f"0.25, tau 0.05) in place of the binary obstacle, checkpoints every {every} steps",
+        "loss": loss, "gradient_ms": grad_ms, "dL_daxes": ga.tolist(), "fd": fds, "fd_rel_eps": rel_eps,
+        "fd_rel_err": rels, "within_1e-4": bool(max(rels) <= 1e-4),
     }
 
 
@@ -423,7 +481,11 @@ def main():
         from atjup_amd import inputs
         g = inputs.Grid() if args.grid == "standard" else inputs.fine_grid()
         kw = {"steps": 50, "every": 10} if args.grid == "fine" else {}
-        print(json.dumps({"adjoint": adjoint_leg(args.sweeps, args.omega, g=g, directions=0, **kw)}), flush=True)
+        out = {"adjoint": adjoint_leg(args.sweeps, args.omega, g=g, directions=0, **kw)}
+        if args.grid == "standard":
+            out["semi_axes"] = semi_axes_leg(args.sweeps, args.omega, g=g)
+            out["semi_axes_level3"] = semi_axes_leg(args.sweeps, args.omega, g=g, i_top=3)
+        print(json.dumps(out), flush=True)
         return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -564,6 +626,11 @@ def main():
     if not args.no_adjoint:
         # the fine shell: 50 steps with checkpoints every 10, fewer difference quotients
         adj = adjoint_leg(args.sweeps, args.omega, g=g, **({"steps": 50, "every": 10, "directions": 4} if fine else {}))
+        if not fine:
+            # the benchmark's top-level misfit, and one at level 3, inside the drag region's height,
+            # where the loss responds to the semi-axes well above the rounding of its difference quotients
+            adj["semi_axes"] = semi_axes_leg(args.sweeps, args.omega, g=g)
+            adj["semi_axes_level3"] = semi_axes_leg(args.sweeps, args.omega, g=g, i_top=3)
 
     cpu = None
     if not args.no_cpu and not fine:

@@ -400,9 +400,34 @@ int atjup_hip_adj_op(atjup_ctx* ctx, int op, int iarg, double darg);
 int atjup_hip_adj_forward_checkpointed(atjup_ctx* ctx, int nsteps, int every, int first_iter);
 /* The reverse sweep: lam_final = the adjoint of the final t, u, v, w ([4][im][jm][km], or NULL to
  * start from the adjoint fields as uploaded); lam_init receives the adjoint of the initial t, u,
- * v, w. Each segment is recomputed from its checkpoint. The context's forward fields are left as
- * the last recomputation left them. */
+ * v, w. The step states come from the checkpoints (every `every` iterations) by recomputation;
+ * the state after each physics block (the pressure solve's result) and the last segment's step
+ * states are kept from the checkpointed forward. The context's forward fields are left as the
+ * last recomputation left them. */
 int atjup_hip_adj_backward(atjup_ctx* ctx, const double* lam_final, double* lam_init);
+
+/* A smooth obstacle for gradients with respect to its semi-axes. No counterpart in the reference,
+ * whose obstacle is the binary SeaMount mask (planet/BC_Jup.h:353-457): a loss depends on the
+ * binary mask's semi-axes only through cells switching between fluid and solid, so its derivative
+ * is zero almost everywhere. In its place the dynamics profile can carry a drag region: the
+ * momentum right-hand sides of u, v, w gain -chi u / tau (and v, w alike), where
+ *     chi = 1 / (1 + exp(-(1 - s) / width)),
+ *     s = ((k - k0) / a)^2 + ((j - j0) / b)^2 + (i / c)^2,
+ * a = a_phi_deg / dphi and b = b_the_deg / dtheta in cells, c = c_r_km / (L_atm dr) in levels, the
+ * seam k = km - 1 taking the value of k = 0 (chi < 1e-16 is taken as 0). The binary obstacle is
+ * independent of it (upload an all-fluid mask to use the drag region alone); the drop-in's parity
+ * path never sets one. Dynamics profile only (FP64, FP32-refined and tangent forms). */
+typedef struct atjup_obstacle {
+    double j0, k0;                         /* centre: latitude and longitude index */
+    double a_phi_deg, b_the_deg, c_r_km;   /* semi-axes */
+    double width;                          /* smoothing width of chi in s */
+    double tau;                            /* drag time scale, nondimensional */
+} atjup_obstacle;
+/* Set (params) or remove (NULL) the drag region. */
+int atjup_hip_set_smooth_obstacle(atjup_ctx* ctx, const atjup_obstacle* params);
+/* dL/d(a_phi_deg, b_the_deg, c_r_km) of the loss whose final-state adjoint the last
+ * atjup_hip_adj_backward carried back (FP64 dynamics profile with the drag region set); out[3]. */
+int atjup_hip_adj_obstacle_gradient(atjup_ctx* ctx, double* out);
 
 /* ---- housekeeping ----------------------------------------------------------------------- */
 int         atjup_hip_sync(atjup_ctx* ctx);

@@ -254,3 +254,54 @@ def test_jet_misfit_gradient_on_the_full_shell():
         print(f"full shell, direction {d_seed}: loss {float(loss):.10e} FD {fd:.10e} adjoint {ad:.10e} rel {rel:.2e}")
         assert rel < 1e-4
     c.close()
+
+
+def test_semi_axis_gradient_of_the_smooth_drag_region():
+    """dL/d(a_phi_deg, b_the_deg, c_r_km) of the jet misfit with the smooth drag region in place of
+    the binary obstacle (all-fluid mask), through JetMisfit with the axes as a tensor, against
+    central differences of the same forward; dL/dw0 of that forward against central differences
+    along a seeded direction. The drag region has no counterpart in the reference (its obstacle
+    is binary); the forward checked here is the one the library computes with the region set."""
+    torch = pytest.importorskip("torch")
+    from atjup_amd.adjoint import JetMisfit, MisfitRunner
+    st = _state(seed=9)
+    g = st.grid
+    st.mask = np.zeros(g.shape)
+    c = _ctx(st, abi.PRECISION_FP64)
+    init = _snapshot(c)
+    rng = np.random.default_rng(11)
+    target = st.fields["w"][g.im - 2, :, 0] + 0.01 * rng.standard_normal(g.jm)
+    axes0 = np.array([30.0, 17.0, 60.0])
+    obst = abi.obstacle(g, *axes0, width=0.25, tau=0.05, j0=20, k0=24)
+    # the misfit at level 3, inside the drag region's height (c = 60 km is five levels here), where
+    # the loss responds to the semi-axes well above the rounding of its difference quotients
+    target = st.fields["w"][3, :, 0] + 0.01 * rng.standard_normal(g.jm)
+    run = MisfitRunner(c, init, target, nsteps=6, every=2, obstacle=obst, i_top=3)
+    w0 = torch.from_numpy(init["w"].copy()).requires_grad_(True)
+    axes = torch.from_numpy(axes0.copy()).requires_grad_(True)
+    loss = JetMisfit.apply(w0, run, axes)
+    loss.backward()
+    ga, gw = axes.grad.numpy(), w0.grad.numpy()
+    assert np.all(np.isfinite(ga)) and np.abs(ga).max() > 0
+    for p in range(3):
+        eps = 1.0e-4 * axes0[p]
+        vals = []
+        for sgn in (+1, -1):
+            ax = axes0.copy()
+            ax[p] += sgn * eps
+            run.set_axes(ax)
+            vals.append(run.evaluate(init["w"]))
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        rel = abs(fd - ga[p]) / max(abs(fd), abs(ga[p]), 1e-300)
+        print(f"semi-axis {p}: FD {fd:.10e} adjoint {ga[p]:.10e} rel {rel:.2e}", flush=True)
+        assert rel < 1e-4
+    run.set_axes(axes0)
+    d = np.random.default_rng(100).standard_normal(g.shape)
+    d[:, :, -1] = d[:, :, 0]
+    eps = 1e-7
+    fd = (run.evaluate(init["w"] + eps * d) - run.evaluate(init["w"] - eps * d)) / (2 * eps)
+    ad = float(np.sum(gw * d))
+    rel = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
+    print(f"w0 direction: FD {fd:.10e} adjoint {ad:.10e} rel {rel:.2e}")
+    assert rel < 1e-4
+    c.close()

@@ -99,3 +99,32 @@ def test_jet_misfit_gradient_through_the_binding():
         a.backward()
     b.backward()
     jup.close()
+
+
+@pytest.mark.gpu
+def test_semi_axis_gradient_through_the_binding():
+    """JetMisfit with the drag region's semi-axes as a tensor input (all-fluid binary mask): the
+    axes' gradient against central differences of the same forward."""
+    torch = pytest.importorskip("torch")
+    st = small_state("dyn")
+    st.mask = np.zeros(st.grid.shape)
+    jup = pyatjup.Jupiter(profile="dyn", grid=st.grid, sweeps=20)
+    jup.state = st
+    jup.set_smooth_obstacle(30.0, 17.0, 60.0, width=0.25, tau=0.05, j0=20, k0=24)
+    target = st.fields["w"][3, :, :-1].mean(axis=1) * 0.9
+    w0 = torch.from_numpy(st.fields["w"].copy()).requires_grad_(True)
+    axes0 = np.array([30.0, 17.0, 60.0])
+    axes = torch.from_numpy(axes0.copy()).requires_grad_(True)
+    loss = pyatjup.JetMisfit.apply(w0, jup, target, 6, 2, axes, 3)
+    loss.backward()
+    ga = axes.grad.numpy()
+    for p in range(3):
+        eps = 1.0e-4 * axes0[p]
+        lp, _ = pyatjup.jet_misfit(jup, st.fields["w"], target, 6, 2, i_top=3,
+                                   axes=axes0 + eps * np.eye(3)[p])
+        lm, _ = pyatjup.jet_misfit(jup, st.fields["w"], target, 6, 2, i_top=3,
+                                   axes=axes0 - eps * np.eye(3)[p])
+        fd = (lp - lm) / (2 * eps)
+        print("semi-axis", p, "FD", fd, "adjoint", ga[p])
+        assert abs(fd - ga[p]) <= 1e-4 * max(abs(fd), abs(ga[p]))
+    jup.close()
