@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the colour pass from two rocprofv3 PMC runs (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0, KiB per dispatch, the factors from a device copy of known size), per grid;
 # profiles/step_full_41x181x361_hbm_traffic.md holds the summary and the command
-SWEEP_TRAFFIC_BYTES = {(41, 181, 361): (13907.4 * 2.0 + 10442.8) * 1024}
+SWEEP_TRAFFIC_BYTES = {(41, 181, 361): (13915.8 * 2.0 + 11615.6) * 1024}
 WORKLOAD = {
     "dyn": "dynamics-only FP64 time step",
     "full": "full-physics FP64 time step (k-omega closure, saturation adjustment of H2O/NH3/CH4, NH4SH "
