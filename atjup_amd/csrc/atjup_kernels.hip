@@ -673,7 +673,8 @@ __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __res
     using V = typename Pair<T>::V;
     static_assert(N == 2 || N == 4, "two or four cells per thread");
     const int b = xcd_remap(blockIdx.x, nb);
-    const int tb = b % ntb, i = b / ntb + 1;
+    // levels fastest: a group's neighbours at i +- 1 are read by the workgroups dispatched beside it
+    const int i = b % (im - 2) + 1, tb = b / (im - 2);
     const int ngrp = nm / N;
     const int t = tb * 256 + threadIdx.x;
     int mg;
@@ -706,6 +707,35 @@ __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __res
     else flw = *reinterpret_cast<const uint32_t*>(flg + q);
     const PoisIJ<double> W = pw[(size_t)i * jm + j];
     const bool ghost_row = rb_ghost_row(gw, i, j, im, jm);
+    constexpr uint32_t all_fluid_mask = (N == 2) ? 0x7f7fu : 0x7f7f7f7fu;
+    if(!ghost_row && (flw & all_fluid_mask) == 0 && omega == 1.0){
+        // every cell of the group fluid with fluid neighbours: the sources read and the results
+        // written as pairs
+        const T num1 = (T)W.num1, num2 = (T)W.num2, num3 = (T)W.num3;
+        const T den = std::is_same<T, float>::value ? (T)W.den_fluid32 : (T)W.den_fluid;
+        #pragma unroll
+        for(int h = 0; h < N; h += 2){
+            const typename Pair<S>::V sv = *reinterpret_cast<const typename Pair<S>::V*>(src + q + h);
+            T res[2];
+            #pragma unroll
+            for(int e = 0; e < 2; e++){
+                const int r = h + e;
+                T pkm, pkp;
+                if(off){ pkm = vc[r]; pkp = (r + 1 < N) ? vc[r + 1] : edge; }
+                else   { pkm = (r > 0) ? vc[r - 1] : edge; pkp = vc[r]; }
+                T acc = T(0) + num1 * vip[r];
+                acc += num1 * vim[r];
+                acc += num2 * vjp[r];
+                acc += num2 * vjm[r];
+                acc += num3 * pkp;
+                acc += num3 * pkm;
+                res[e] = (acc - (T)(e ? sv.y : sv.x)) / den;
+            }
+            V o; o.x = res[0]; o.y = res[1];
+            *reinterpret_cast<V*>(own + q + h) = o;
+        }
+        return;
+    }
     #pragma unroll
     for(int r = 0; r < N; r++){
         T pkm, pkp;
