@@ -857,7 +857,10 @@ __global__ __launch_bounds__(256) void ka_rbT_pass2(double* __restrict__ lam, do
         const double* __restrict__ mu_o, double* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
         const uint8_t* __restrict__ fo_arr, const uint8_t* __restrict__ inm, const PoisIJ<double>* __restrict__ pw,
         int im, int jm, int hp, int nm, int ntb, int colour, int ghost, double omega, double c43, double c13){
-    const int tb = blockIdx.x % ntb, i = blockIdx.x / ntb;
+    // levels fastest, each XCD a contiguous range: a group's neighbours at i +- 1 are read by the
+    // workgroups dispatched beside it
+    const int b = xcd_remap(blockIdx.x, ntb * im);
+    const int i = b % im, tb = b / im;
     const int npair = nm / 2;
     const int t = tb * 256 + threadIdx.x;
     const int j = t / npair;
@@ -871,6 +874,57 @@ __global__ __launch_bounds__(256) void ka_rbT_pass2(double* __restrict__ lam, do
     const uint16_t bits2 = *reinterpret_cast<const uint16_t*>(inm + q);
     const uint8_t b0 = (uint8_t)(bits2 & 0xff), b1 = (uint8_t)(bits2 >> 8);
     const int ij = i * jm + j;
+    if(bits2 == 0x3f3f && i >= 3 && i <= im - 4 && j >= 3 && j <= jm - 4){
+        // both cells fluid with six fluid neighbours that all read them, away from the rows that
+        // form or sit behind the outer planes: rbT_cell without its branches, the pair at once
+        const uint16_t fl2 = *reinterpret_cast<const uint16_t*>(fl_arr + q);
+        if(fl2 == 0){
+            const PoisIJ<double> W = pw[ij];
+            const double wip = pw[ij + jm].num1, wim = pw[ij - jm].num1;
+            const double wjp = pw[ij + 1].num2, wjm = pw[ij - 1].num2;
+            const double2 vip = *reinterpret_cast<const double2*>(mu_o + q + P);
+            const double2 vim = *reinterpret_cast<const double2*>(mu_o + q - P);
+            const double2 vjp = *reinterpret_cast<const double2*>(mu_o + q + R);
+            const double2 vjm = *reinterpret_cast<const double2*>(mu_o + q - R);
+            const double2 vc = *reinterpret_cast<const double2*>(mu_o + q);
+            const int off = (colour + i + j) & 1;
+            const double edge = off ? mu_o[row + (m + 2 == nm ? 0 : m + 2)] : mu_o[row + (m == 0 ? nm - 1 : m - 1)];
+            double mke[2], mkw[2];
+            if(off){ mke[0] = vc.y; mkw[0] = vc.x; mke[1] = edge; mkw[1] = vc.y; }
+            else   { mke[0] = vc.x; mkw[0] = edge; mke[1] = vc.y; mkw[1] = vc.x; }
+            const double mip[2] = { vip.x, vip.y }, mim[2] = { vim.x, vim.y };
+            const double mjp[2] = { vjp.x, vjp.y }, mjm[2] = { vjm.x, vjm.y };
+            const double den = W.den_fluid;
+            const double2 lv = *reinterpret_cast<const double2*>(lam + q);
+            const double2 sv = *reinterpret_cast<const double2*>(lsrc + q);
+            double2 mv = colour ? *reinterpret_cast<const double2*>(mu + q) : make_double2(0.0, 0.0);
+            const double lin[2] = { lv.x, lv.y }, sin_[2] = { sv.x, sv.y };
+            const double min_[2] = { mv.x, mv.y };
+            double lout[2], sout[2], mout[2];
+            #pragma unroll
+            for(int e = 0; e < 2; e++){
+                double dsum = 0.0;
+                dsum += wip * mip[e];
+                dsum += wim * mim[e];
+                dsum += wjp * mjp[e];
+                dsum += wjm * mjm[e];
+                dsum += W.num3 * mke[e];
+                dsum += W.num3 * mkw[e];
+                double t = lin[e];
+                if(colour == 0) t += dsum;
+                const double mu_x = (colour == 0) ? omega * t / den : min_[e];
+                double out = (1.0 - omega) * t + 0.0;
+                sout[e] = sin_[e] + -mu_x;
+                if(colour == 1) out += dsum;
+                lout[e] = out;
+                mout[e] = (colour == 0) ? mu_x : omega * out / den;
+            }
+            *reinterpret_cast<double2*>(lam + q) = make_double2(lout[0], lout[1]);
+            *reinterpret_cast<double2*>(lsrc + q) = make_double2(sout[0], sout[1]);
+            *reinterpret_cast<double2*>(mu + q) = make_double2(mout[0], mout[1]);
+            return;
+        }
+    }
     const RbtNeed nd = rbt_need(b0 | b1, interior, ghost, i, j, im, jm);
     RbtNb n0 = {}, n1 = {};
     if(nd.ip){
