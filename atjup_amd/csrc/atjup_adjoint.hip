@@ -445,23 +445,29 @@ struct RhsAdjArgs {
     double* gchi;                // dL/dchi, accumulated over stages and steps
 };
 
-__global__ __launch_bounds__(256) void ka_rhs_dry(const RhsAdjArgs a, const GeoIJ<double>* __restrict__ geo){
-    const int b = xcd_remap(blockIdx.x, a.nb);
-    const int kb = b % a.nkb;
-    const int rest = b / a.nkb;
-    const int jb = rest % a.njb;
-    const int i = rest / a.njb;
-    const int k = kb * 64 + threadIdx.x;
-    const int j = jb * 4 + threadIdx.y;
-    if(k >= a.km || j >= a.jm) return;
-    const size_t q = ((size_t)i * a.jm + j) * a.pitch + k;
-    const uint8_t fl = a.flags[q];
-    const bool interior = i >= 1 && i <= a.im - 2 && j >= 3 && j <= a.jm - 4 && k <= a.km - 2 && !(fl & FL_SELF);
-    if(!interior) return;
+// The scatter targets of one cell's stencil adjoints: its own cell and its six neighbours. The
+// workgroup's (j, k) tile with a one-cell rim accumulates in LDS whatever lands in it (the cell,
+// its j and k neighbours); the neighbours at i +- 1 and across the periodic seam go straight to
+// global memory. The tile is added to global memory once, after every cell of the workgroup.
+constexpr int RD_BK = 64, RD_BJ = 4;
+constexpr int RD_NF = 6;                     // the scatter targets: g[0..3], lpd, lph
+struct RhsTile { double v[RD_NF][RD_BJ + 2][RD_BK + 2]; };
+
+__device__ __forceinline__ void rhs_put(RhsTile& s, double* X, int f, size_t qg, int ty, int tx, double v){
+    if(v == 0.0) return;
+    if(ty >= 0) atomicAdd(&s.v[f][ty][tx], v);
+    else unsafeAtomicAdd(X + qg, v);
+}
+
+__device__ __forceinline__ void rhs_dry_cell(const RhsAdjArgs& a, const GeoIJ<double>* __restrict__ geo,
+        RhsTile& tile, int i, int j, int k, size_t q, uint8_t fl){
     const size_t P = a.plane, R = (size_t)a.pitch;
     const int kw = (k == 0) ? a.km - 2 : k - 1;
     const int ke = (k == a.km - 2) ? 0 : k + 1;
     const size_t qkw = q - k + kw, qke = q - k + ke;
+    // tile positions (row, column) of the cell and its in-tile neighbours; -1: global
+    const int ty = (int)threadIdx.y + 1, tx = (int)threadIdx.x + 1;
+    const int tx_e = (ke == k + 1) ? tx + 1 : -1, tx_w = (kw == k - 1) ? tx - 1 : -1;
     const GeoIJ<double> G = geo[wave_uniform(i * a.jm + j)];      // a wavefront is one (i, j) row
 
     double c_[4], d1r[4], d1t[4], d1p[4];
@@ -567,27 +573,65 @@ __global__ __launch_bounds__(256) void ka_rhs_dry(const RhsAdjArgs a, const GeoI
         Lc[0] += Lb * a.cbuoy * (-a.g_ * a.ps[q] / (a.rR * tq * tq * a.t_ref));
         a.gbrl[q] += -a.cbuoy * Lb;
     }
-    // the stencils
+    // the stencils: the i +- 1 targets straight to global memory, the rest through the tile
     for(int f = 0; f < 4; f++){
         double* X = a.g[f];
-        aadd(&X[q + P], a.inv_2dr * L1r[f] + a.inv_dr2 * L2r[f]);
-        aadd(&X[q - P], -a.inv_2dr * L1r[f] + a.inv_dr2 * L2r[f]);
-        aadd(&X[q + R], a.inv_2dthe * L1t[f] + a.inv_dthe2 * L2t[f]);
-        aadd(&X[q - R], -a.inv_2dthe * L1t[f] + a.inv_dthe2 * L2t[f]);
-        aadd(&X[qke], a.inv_2dphi * L1p[f] + a.inv_dphi2 * L2p[f]);
-        aadd(&X[qkw], -a.inv_2dphi * L1p[f] + a.inv_dphi2 * L2p[f]);
-        aadd(&X[q], -2.0 * (a.inv_dr2 * L2r[f] + a.inv_dthe2 * L2t[f] + a.inv_dphi2 * L2p[f]) + Lc[f]);
+        rhs_put(tile, X, f, q + P, -1, 0, a.inv_2dr * L1r[f] + a.inv_dr2 * L2r[f]);
+        rhs_put(tile, X, f, q - P, -1, 0, -a.inv_2dr * L1r[f] + a.inv_dr2 * L2r[f]);
+        rhs_put(tile, X, f, q + R, ty + 1, tx, a.inv_2dthe * L1t[f] + a.inv_dthe2 * L2t[f]);
+        rhs_put(tile, X, f, q - R, ty - 1, tx, -a.inv_2dthe * L1t[f] + a.inv_dthe2 * L2t[f]);
+        rhs_put(tile, X, f, qke, tx_e >= 0 ? ty : -1, tx_e, a.inv_2dphi * L1p[f] + a.inv_dphi2 * L2p[f]);
+        rhs_put(tile, X, f, qkw, tx_w >= 0 ? ty : -1, tx_w, -a.inv_2dphi * L1p[f] + a.inv_dphi2 * L2p[f]);
+        rhs_put(tile, X, f, q, ty, tx, -2.0 * (a.inv_dr2 * L2r[f] + a.inv_dthe2 * L2t[f] + a.inv_dphi2 * L2p[f]) + Lc[f]);
     }
-    // the mirrored pressure gradients
-    auto mir = [&](double* L, double coef, uint8_t hi_flag, uint8_t lo_flag, size_t qh, size_t ql){
-        aadd((fl & hi_flag) ? &L[q] : &L[qh], coef);
-        aadd((fl & lo_flag) ? &L[q] : &L[ql], -coef);
+    // the mirrored pressure gradients (a face towards a solid cell reads the cell itself)
+    auto mir = [&](double* L, int f, double coef, uint8_t hi_flag, uint8_t lo_flag, size_t qh, size_t ql,
+                   int hy, int hx, int ly, int lx){
+        if(fl & hi_flag) rhs_put(tile, L, f, q, ty, tx, coef);
+        else rhs_put(tile, L, f, qh, hy, hx, coef);
+        if(fl & lo_flag) rhs_put(tile, L, f, q, ty, tx, -coef);
+        else rhs_put(tile, L, f, ql, ly, lx, -coef);
     };
-    mir(a.lpd, a.inv_2dr * L_dpdr, FL_IP, FL_IM, q + P, q - P);
-    mir(a.lpd, a.inv_2dthe * L_dpdthe, FL_JP, FL_JM, q + R, q - R);
-    mir(a.lpd, a.inv_2dphi * L_dpdphi, FL_KP, FL_KM, qke, qkw);
-    mir(a.lph, a.inv_2dthe * L_dphdthe, FL_JP, FL_JM, q + R, q - R);
-    mir(a.lph, a.inv_2dphi * L_dphdphi, FL_KP, FL_KM, qke, qkw);
+    const int ey = tx_e >= 0 ? ty : -1, wy = tx_w >= 0 ? ty : -1;
+    mir(a.lpd, 4, a.inv_2dr * L_dpdr, FL_IP, FL_IM, q + P, q - P, -1, 0, -1, 0);
+    mir(a.lpd, 4, a.inv_2dthe * L_dpdthe, FL_JP, FL_JM, q + R, q - R, ty + 1, tx, ty - 1, tx);
+    mir(a.lpd, 4, a.inv_2dphi * L_dpdphi, FL_KP, FL_KM, qke, qkw, ey, tx_e, wy, tx_w);
+    mir(a.lph, 5, a.inv_2dthe * L_dphdthe, FL_JP, FL_JM, q + R, q - R, ty + 1, tx, ty - 1, tx);
+    mir(a.lph, 5, a.inv_2dphi * L_dphdphi, FL_KP, FL_KM, qke, qkw, ey, tx_e, wy, tx_w);
+}
+
+__global__ __launch_bounds__(RD_BK * RD_BJ) void ka_rhs_dry(const RhsAdjArgs a, const GeoIJ<double>* __restrict__ geo){
+    __shared__ RhsTile tile;
+    const int b = xcd_remap(blockIdx.x, a.nb);
+    const int kb = b % a.nkb;
+    const int rest = b / a.nkb;
+    const int jb = rest % a.njb;
+    const int i = rest / a.njb;
+    const int k = kb * RD_BK + threadIdx.x;
+    const int j = jb * RD_BJ + threadIdx.y;
+    const int tid = threadIdx.y * RD_BK + threadIdx.x;
+    double* tv = &tile.v[0][0][0];
+    constexpr int NT = RD_NF * (RD_BJ + 2) * (RD_BK + 2);
+    for(int c = tid; c < NT; c += RD_BK * RD_BJ) tv[c] = 0.0;
+    __syncthreads();
+    if(k < a.km && j < a.jm){
+        const size_t q = ((size_t)i * a.jm + j) * a.pitch + k;
+        const uint8_t fl = a.flags[q];
+        const bool interior = i >= 1 && i <= a.im - 2 && j >= 3 && j <= a.jm - 4 && k <= a.km - 2 && !(fl & FL_SELF);
+        if(interior) rhs_dry_cell(a, geo, tile, i, j, k, q, fl);
+    }
+    __syncthreads();
+    // the tile and its rim into global memory, one add per touched cell
+    constexpr int NC = (RD_BJ + 2) * (RD_BK + 2);
+    for(int c = tid; c < RD_NF * NC; c += RD_BK * RD_BJ){
+        const double v = tv[c];
+        if(v == 0.0) continue;
+        const int f = c / NC, r = c - f * NC;
+        const int jj = jb * RD_BJ + r / (RD_BK + 2) - 1, kk = kb * RD_BK + r % (RD_BK + 2) - 1;
+        if(jj < 0 || jj >= a.jm || kk < 0 || kk >= a.km) continue;      // holds no contribution
+        double* X = f < 4 ? a.g[f] : (f == 4 ? a.lpd : a.lph);
+        unsafeAtomicAdd(X + ((size_t)i * a.jm + jj) * a.pitch + kk, v);
+    }
 }
 
 // the RK4 combination transposed: the adjoint of stage s's right-hand side from the step output's
