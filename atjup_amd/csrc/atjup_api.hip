@@ -326,8 +326,10 @@ atjup_ctx* atjup_hip_ctx_create(const atjup_grid* g, const atjup_consts* cs, con
     const size_t hn = (size_t)c->im * c->jm * c->rb_hp;
     if(!rc) rc = alloc(c->psrc, 2 * hn * asz);
     if(!rc) rc = alloc(c->rb_p, 2 * hn * asz);
-    if(!rc && precision == ATJUP_PRECISION_FP32_REF)
-        for(int f = 0; f < 4 && !rc; f++) rc = alloc(c->lo[f], c->n3 * sizeof(float));
+    if(!rc && precision == ATJUP_PRECISION_FP32_REF){
+        const int n_lo = c->kn.dry ? 4 : ATJUP_F_TKE;               // t, u, v, w (+ the species)
+        for(int f = 0; f < n_lo && !rc; f++) rc = alloc(c->lo[f], c->n3 * sizeof(float));
+    }
     if(!rc && precision == ATJUP_PRECISION_FP32_REF) rc = alloc(c->rad_tmp, 3 * c->n3 * sizeof(double));
     if(!rc && precision == ATJUP_PRECISION_FP32_REF){
         rc = alloc(c->rb_d, 2 * hn * sizeof(float));
@@ -423,7 +425,8 @@ void atjup_hip_ctx_destroy(atjup_ctx* c){
     for(auto& s : c->stg) for(auto& b : s) release(b);
     for(auto& b : c->acc) release(b);
     for(auto& b : c->live_turb) release(b);
-    for(atjup::Buf* b : { &c->lo[0], &c->lo[1], &c->lo[2], &c->lo[3], &c->colr }) release(*b);
+    for(auto& b : c->lo) release(b);
+    release(c->colr);
     for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
                           &c->amb_rows, &c->nonfinite, &c->rad_tab, &c->rad_tmp, &c->precip_srf, &c->chem_w, &c->mca_ws, &c->report,
@@ -458,7 +461,7 @@ static int upload_impl(atjup_ctx* c, int f, const double* src, double* const* co
             if(host[b] != 0){ set_error("the dry profile requires the species to be zero"); return ATJUP_ERR_PROFILE; }
     }
     ATJUP_HIP_CHECK(hipMemcpyAsync(c->cur[s].p, host.data(), host.size(), hipMemcpyHostToDevice, c->stream));
-    if(s < 4 && c->lo[s].p)                       // an uploaded t, u, v, w starts without a low word
+    if(s < ATJUP_N_PROG && c->lo[s].p)            // an uploaded field starts without a low word
         ATJUP_HIP_CHECK(hipMemsetAsync(c->lo[s].p, 0, c->lo[s].bytes, c->stream));
     ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
     if(s == ATJUP_F_TKE || s == ATJUP_F_DIS) c->turb_live_pending = false;
@@ -475,7 +478,7 @@ static int download_impl(atjup_ctx* c, int f, double* dst, double** const* rows)
     const void* src_p = nlevel(f) ? c->cur[s].p : live_ptr(c, s);
     ATJUP_HIP_CHECK(hipMemcpyAsync(host.data(), src_p, host.size(), hipMemcpyDeviceToHost, c->stream));
     std::vector<float> lo;
-    if(s < 4 && c->lo[s].p){                       // t, u, v, w of the refined form: hi + lo
+    if(s < ATJUP_N_PROG && c->lo[s].p){            // a paired field of the refined form: hi + lo
         lo.resize(c->n3);
         ATJUP_HIP_CHECK(hipMemcpyAsync(lo.data(), c->lo[s].p, c->n3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     }

@@ -9,6 +9,24 @@
 
 namespace atjup {
 
+// The FP32-refined form carries t, u, v, w and the species as FP32 pairs hi + lo; lo is null for
+// the other forms and fields. A value written through pair_put is split so that hi + lo holds it
+// to about 2^-48 of its size.
+template<typename T>
+__device__ __forceinline__ double pair_get(const T* hi, const float* lo, size_t q){
+    return lo ? (double)hi[q] + (double)lo[q] : (double)hi[q];
+}
+template<typename T>
+__device__ __forceinline__ void pair_put(T* hi, float* lo, size_t q, double v){
+    if(lo){
+        const float h = (float)v;
+        hi[q] = (T)h;
+        lo[q] = (float)(v - (double)h);
+    } else {
+        hi[q] = (T)v;
+    }
+}
+
 __device__ __forceinline__ int xcd_remap(int b, int nb){
     // The dispatcher hands block b to XCD b % 8. Give XCD x the contiguous range of logical
     // blocks [start_x, start_x + count_x) instead, so neighbouring tiles share an L2.

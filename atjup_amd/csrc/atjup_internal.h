@@ -97,7 +97,8 @@ struct atjup_ctx {
     };
     std::vector<SweepGraph> sweep_graphs;   // instantiated sweep sequences (hipGraphExec_t)
     atjup::Buf rb_flags;         // face flags, red-black split [2][im][jm][rb_hp]
-    atjup::Buf lo[4];            // FP32-refined form: low words of t, u, v, w (the value is hi + lo)
+    atjup::Buf lo[ATJUP_N_PROG]; // FP32-refined form: low words of t, u, v, w and, in the moist
+                                 // profiles, the eleven species (the value is hi + lo)
     atjup::Buf colr;             // int2 (i0, i1) per (j, k): the column's fluid run from the bottom
     int rb_hp = 0;               // compact row length: (km + 1) / 2 rounded up to 128 bytes
     atjup::Buf mask;             // uint8 SeaMount [n3]

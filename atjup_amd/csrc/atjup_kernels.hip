@@ -1157,13 +1157,21 @@ __global__ __launch_bounds__(64) void k_bc_t_ambient(T* __restrict__ t, const ui
 
 // clampNegativeSpecies (FileIO_Jup.cpp:494-548)
 template<typename T>
-struct SpeciesPtrs { T* x[ATJUP_F_NH4SH - ATJUP_F_H2O + 1]; };
+struct SpeciesPtrs {
+    T* x[ATJUP_F_NH4SH - ATJUP_F_H2O + 1];
+    float* lo[ATJUP_F_NH4SH - ATJUP_F_H2O + 1];     // the refined form's low words (else null)
+};
 
 template<typename T>
 __global__ __launch_bounds__(256) void k_clamp_species(const SpeciesPtrs<T> S, size_t n){
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(q >= n) return;
     T* __restrict__ x = S.x[blockIdx.y];
+    float* __restrict__ lo = S.lo[blockIdx.y];
+    if(lo){
+        if(!((double)x[q] + (double)lo[q] >= 0.0)){ x[q] = T(0); lo[q] = 0.0f; }
+        return;
+    }
     const T v = x[q];
     if(!(v >= T(0))) x[q] = T(0);
 }
@@ -1564,6 +1572,10 @@ int do_shapiro(atjup_ctx* c, int field, int order, double strength, int passes, 
             std::swap(c->cur[field].p, c->scratch.p);
         }
     }
+    // the refined form's pair: the filtered value is the new hi, so its low word starts over
+    if(passes > 0 && axes && field < ATJUP_N_PROG && c->lo[field].p)
+        hipLaunchKernelGGL(k_fill_float, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream,
+            (float*)c->lo[field].p, c->n3, 0.0f);
     return check_launch();
 }
 
@@ -1654,7 +1666,10 @@ template<typename T>
 int do_clamp(atjup_ctx* c){
     if(c->kn.dry) return 0;
     SpeciesPtrs<T> S{};
-    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++) S.x[f - ATJUP_F_H2O] = F<T>(c, f);
+    for(int f = ATJUP_F_H2O; f <= ATJUP_F_NH4SH; f++){
+        S.x[f - ATJUP_F_H2O] = F<T>(c, f);
+        S.lo[f - ATJUP_F_H2O] = (float*)c->lo[f].p;
+    }
     hipLaunchKernelGGL(k_clamp_species<T>, dim3(nblk(c->n3, 256), ATJUP_F_NH4SH - ATJUP_F_H2O + 1), dim3(256), 0,
         c->stream, S, c->n3);
     return check_launch();

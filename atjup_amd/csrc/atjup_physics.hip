@@ -51,7 +51,8 @@ struct StageFullArgs {
     T cep, to_nd, r_mix, c_m2o, c_p2o, Coriolis, centrifugal, t_ref;
     T t_coef, inv_re, rad_coupling, precip_coupling, turb_coupling, L_rad, R_mix, cp_mix, u_0;
     T cr, pc, stokes_a, rg_nh4sh, mue_mix, nue_max, tke_max_nd, dis_min;
-    float* lo[4];                          // FP32-refined form: the low words of t, u, v, w (else null)
+    float* lo[ATJUP_N_PROG];               // FP32-refined form: the low words of t, u, v, w and the
+                                           // species (else null)
     double dt_d;
     // the wall distance of the SST and k-epsilon sources: i_topography, the layer thickness [m] of
     // each layer and the float layer spacing L_atm * dr [km] the layer heights are multiples of;
@@ -146,9 +147,12 @@ __device__ __forceinline__ void stage_update_v(const StageFullArgs<T>& a, int f,
         a.out[f][q] = v;
     } else {
         if constexpr(std::is_same<T, float>::value){
-            // t, u, v, w of the refined form: the increment added to hi + lo in FP64 and split
-            if(f < 4 && a.lo[f]){
-                const double s = ((double)nl_v + (double)a.lo[f][q]) + a.dt_d * (double)accf * (1.0 / 6.0);
+            // the paired fields of the refined form (t, u, v, w, the species): the increment added
+            // to hi + lo in FP64, the species' clamps applied to that sum, and the sum split again
+            if(a.lo[f]){
+                double s = ((double)nl_v + (double)a.lo[f][q]) + a.dt_d * (double)accf * (1.0 / 6.0);
+                if(kind == 1) s = dmax(0.0, s);
+                else if(kind == 2){ if(s < 0.0) s = 0.0; }
                 const float h = (float)s;
                 a.out[f][q] = h;
                 a.lo[f][q] = (float)(s - (double)h);
@@ -1016,20 +1020,29 @@ __device__ void satadj_cell(double& cv, double& ccl, double& cic, double& tq, do
 template<typename T>
 __global__ __launch_bounds__(256) void k_satadj(T* __restrict__ c, T* __restrict__ cloud, T* __restrict__ ice,
         T* __restrict__ t, const T* __restrict__ ps, const T* __restrict__ rho, const uint8_t* __restrict__ flags,
-        int jm, int km, int pitch, const SatParams P, float* __restrict__ lo_t){
+        int jm, int km, int pitch, const SatParams P, float* __restrict__ lo_t, float* __restrict__ lo_c,
+        float* __restrict__ lo_cl, float* __restrict__ lo_ic){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
     if(k >= km) return;
     const size_t q = ((size_t)i * jm + j) * pitch + k;
-    double cv = (double)c[q], ccl = (double)cloud[q], cic = (double)ice[q], tq = (double)t[q];
-    if(lo_t) tq += (double)lo_t[q];                // the refined form's t is hi + lo
+    // the refined form's t and species are hi + lo
+    double cv = pair_get(c, lo_c, q), ccl = pair_get(cloud, lo_cl, q), cic = pair_get(ice, lo_ic, q);
+    double tq = pair_get(t, lo_t, q);
+    const double cv0 = cv, ccl0 = ccl, cic0 = cic;
     const double t0 = tq;
     if(!(flags[q] & FL_SELF)) satadj_cell(cv, ccl, cic, tq, (double)ps[q], (double)rho[q], P);
     if(cv <= 0.0) cv = 0.0;
     if(ccl <= 0.0) ccl = 0.0;
     if(cic <= 0.0) cic = 0.0;
-    c[q] = (T)cv;
-    cloud[q] = (T)ccl;
-    ice[q] = (T)cic;
+    if(!lo_c){
+        c[q] = (T)cv;
+        cloud[q] = (T)ccl;
+        ice[q] = (T)cic;
+    } else {                                       // pairs: rewritten where the value changed
+        if(!(cv == cv0)) pair_put(c, lo_c, q, cv);
+        if(!(ccl == ccl0)) pair_put(cloud, lo_cl, q, ccl);
+        if(!(cic == cic0)) pair_put(ice, lo_ic, q, cic);
+    }
     if(tq != t0 || !(tq == tq)){
         if(lo_t){
             const float h = (float)tq;
@@ -1129,7 +1142,7 @@ int do_rk4_full(atjup_ctx* c){
     a.re_turb = (T)C.re_turb;
     a.dt = (T)dt; a.one_sixth = (T)(1.0 / 6.0);
     a.dt_d = dt;
-    for(int f = 0; f < 4; f++) a.lo[f] = (float*)c->lo[f].p;
+    for(int f = 0; f < ATJUP_N_PROG; f++) a.lo[f] = (float*)c->lo[f].p;
     a.inv_2dr = (T)(1.0 / (2.0 * c->g.dr)); a.inv_2dthe = (T)(1.0 / (2.0 * c->g.dthe));
     a.inv_2dphi = (T)(1.0 / (2.0 * c->g.dphi)); a.inv_dr2 = (T)(1.0 / (c->g.dr * c->g.dr));
     a.inv_dthe2 = (T)(1.0 / (c->g.dthe * c->g.dthe)); a.inv_dphi2 = (T)(1.0 / (c->g.dphi * c->g.dphi));
@@ -1299,7 +1312,8 @@ int do_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){
     hipLaunchKernelGGL(k_satadj<T>, dim3(nblk(c->km, 128), c->jm, c->im), dim3(128), 0, c->stream,
         F<T>(c, ids[gas][0]), F<T>(c, ids[gas][1]), F<T>(c, ids[gas][2]), F<T>(c, ATJUP_F_T),
         F<T>(c, ATJUP_F_P_STAT), F<T>(c, ATJUP_F_RHO_MIX), (const uint8_t*)c->flags.p,
-        c->jm, c->km, c->pitch, P, (float*)c->lo[ATJUP_F_T].p);
+        c->jm, c->km, c->pitch, P, (float*)c->lo[ATJUP_F_T].p, (float*)c->lo[ids[gas][0]].p,
+        (float*)c->lo[ids[gas][1]].p, (float*)c->lo[ids[gas][2]].p);
     return check_launch();
 }
 
@@ -1325,17 +1339,18 @@ struct EqConsts { double m_nh3, m_h2s, m_nh4sh, t_ref, r_mix, cp_mix, L_f; };
 template<typename T>
 __global__ __launch_bounds__(256) void k_nh4sh_eq(T* __restrict__ t, float* __restrict__ lo_t, T* __restrict__ nh3,
         T* __restrict__ h2s, T* __restrict__ nh4sh, const T* __restrict__ rho_mix, const uint8_t* __restrict__ msk,
-        size_t n, int km, int pitch, const EqConsts C){
+        size_t n, int km, int pitch, const EqConsts C, float* __restrict__ lo_nh3, float* __restrict__ lo_h2s,
+        float* __restrict__ lo_nh4sh){
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(q >= n || (int)(q % (size_t)pitch) >= km || msk[q]) return;
     const double R_univ = 8314.46, atm = 101325.0;
-    double tq = (double)t[q];
-    if(lo_t) tq += (double)lo_t[q];
+    const double tq = pair_get(t, lo_t, q);              // the refined form's pairs: hi + lo
     const double T0 = tq * C.t_ref;
     if(!(T0 > 0.0)) return;
     double rho = (double)rho_mix[q];
     if(!(rho > 0.0) || !isfinite(rho)) rho = C.r_mix;
-    const double nh3_0 = (double)nh3[q], h2s_0 = (double)h2s[q], nh4sh_0 = (double)nh4sh[q];
+    const double nh3_0 = pair_get(nh3, lo_nh3, q), h2s_0 = pair_get(h2s, lo_h2s, q);
+    const double nh4sh_0 = pair_get(nh4sh, lo_nh4sh, q);
     const double a = dmax(0.0, nh3_0) / C.m_nh3;
     const double b = dmax(0.0, h2s_0) / C.m_h2s;
     const double s = dmax(0.0, nh4sh_0) / C.m_nh4sh;
@@ -1361,13 +1376,10 @@ __global__ __launch_bounds__(256) void k_nh4sh_eq(T* __restrict__ t, float* __re
     }
     const double x = (f0 > 0.0) ? lo : hi;
     if(x == 0.0) return;
-    nh3[q] = (T)dmax(0.0, nh3_0 - C.m_nh3 * x);
-    h2s[q] = (T)dmax(0.0, h2s_0 - C.m_h2s * x);
-    nh4sh[q] = (T)dmax(0.0, nh4sh_0 + C.m_nh4sh * x);
-    const double tn = (T0 + heat * x) / C.t_ref;
-    const T th = (T)tn;
-    t[q] = th;
-    if(lo_t) lo_t[q] = (float)(tn - (double)th);
+    pair_put(nh3, lo_nh3, q, dmax(0.0, nh3_0 - C.m_nh3 * x));
+    pair_put(h2s, lo_h2s, q, dmax(0.0, h2s_0 - C.m_h2s * x));
+    pair_put(nh4sh, lo_nh4sh, q, dmax(0.0, nh4sh_0 + C.m_nh4sh * x));
+    pair_put(t, lo_t, q, (T0 + heat * x) / C.t_ref);
 }
 
 template<typename T>
@@ -1378,7 +1390,8 @@ int do_nh4sh_eq(atjup_ctx* c){
     hipLaunchKernelGGL(k_nh4sh_eq<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream,
         F<T>(c, ATJUP_F_T), (float*)c->lo[ATJUP_F_T].p, F<T>(c, ATJUP_F_NH3), F<T>(c, ATJUP_F_H2S),
         F<T>(c, ATJUP_F_NH4SH), (const T*)F<T>(c, ATJUP_F_RHO_MIX), (const uint8_t*)c->mask.p,
-        c->n3, c->km, c->pitch, C);
+        c->n3, c->km, c->pitch, C, (float*)c->lo[ATJUP_F_NH3].p, (float*)c->lo[ATJUP_F_H2S].p,
+        (float*)c->lo[ATJUP_F_NH4SH].p);
     return check_launch();
 }
 
@@ -1575,6 +1588,7 @@ struct PrecipFields {
     const T *t, *p_stat, *rho, *nh4sh;
     const float* lo_t;
     T *vap[3], *cld[3], *ice[3];
+    float *lo_vap[3], *lo_cld[3], *lo_ice[3];   // the refined form's low words (else null)
     T *P[3][3];                    // [gas][rain, snow, graupel]
     T *S[3][3];                    // [gas][vapour, cloud, ice]
     T *Q, *P_nh4sh;
@@ -1595,9 +1609,9 @@ __device__ __forceinline__ PrecipIn precip_load(const PrecipFields<T>& F, int g,
     if(F.lo_t) r.tq += (double)F.lo_t[q];
     r.p_u = (double)F.p_stat[q];
     r.rf = (double)F.rho[q];
-    r.q_v = (double)F.vap[g][q];
-    r.q_c = (double)F.cld[g][q];
-    r.q_i = (double)F.ice[g][q];
+    r.q_v = pair_get(F.vap[g], F.lo_vap[g], q);
+    r.q_c = pair_get(F.cld[g], F.lo_cld[g], q);
+    r.q_i = pair_get(F.ice[g], F.lo_ice[g], q);
     return r;
 }
 
@@ -1680,9 +1694,9 @@ __global__ __launch_bounds__(3 * PR_BK) __attribute__((amdgpu_waves_per_eu(3))) 
                     Si = -dmax(0.0, d_i) / tau * C.nd;
                     Sv = +S_ev * C.nd;
                 } else {
-                    F.cld[g][q] = (T)(q_c - dmax(0.0, d_c));
-                    F.ice[g][q] = (T)(q_i - dmax(0.0, d_i));
-                    F.vap[g][q] = (T)(q_v + S_ev * tau);
+                    pair_put(F.cld[g], F.lo_cld[g], q, q_c - dmax(0.0, d_c));
+                    pair_put(F.ice[g], F.lo_ice[g], q, q_i - dmax(0.0, d_i));
+                    pair_put(F.vap[g], F.lo_vap[g], q, q_v + S_ev * tau);
                 }
                 const double F_r = F_r_in - F_r_frz + F_s_melt + F_g_melt - F_ev + (S_c_au + S_ac) * dz;
                 const double F_s = F_s_in - F_s_melt + (S_i_au + S_s_rim - S_csg) * dz;
@@ -1740,6 +1754,8 @@ int do_precipitation(atjup_ctx* c){
     PrecipConsts C;
     for(int g = 0; g < 3; g++){
         Fp.vap[g] = F<T>(c, vap[g]); Fp.cld[g] = F<T>(c, vap[g] + 1); Fp.ice[g] = F<T>(c, vap[g] + 2);
+        Fp.lo_vap[g] = (float*)c->lo[vap[g]].p; Fp.lo_cld[g] = (float*)c->lo[vap[g] + 1].p;
+        Fp.lo_ice[g] = (float*)c->lo[vap[g] + 2].p;
         for(int r = 0; r < 3; r++){ Fp.P[g][r] = F<T>(c, pf[g] + r); Fp.S[g][r] = F<T>(c, sf[g] + r); }
         const atjup_gas_params& a = c->gas[g];
         C.gas[g] = PrecipGas{ a.C, a.L0, a.R, a.del_alf, a.del_bet, a.ep, a.lv, a.ls, a.t_0, a.t_00 };
@@ -2004,6 +2020,7 @@ template<typename T>
 struct McaFields {
     T *t, *v[2], *cl[2], *ice[2], *tr[2];        // tr: H2S, NH4SH
     float* lo_t;
+    float *lo_v[2], *lo_cl[2], *lo_ice[2], *lo_tr[2];   // the refined form's low words (else null)
     const T *p_stat, *rho;
 };
 enum { MV_HL = 0, MV_QT = 1, MV_QR = 3, MV_TEQ = 5, MV_CEQ = 6, MCA_NV = 8 };
@@ -2372,12 +2389,12 @@ __global__ __launch_bounds__(64) void k_mca(const McaFields<T> F, const double* 
         for(int g = 0; g < 2; g++){
             const double qv = qt[g] - ce[g];
             const bool liquid = TT >= K.gas[g].t_0;
-            F.v[g][c] = (T)(qv * rho);
-            F.cl[g][c] = (T)((liquid ? ce[g] : 0.0) * rho);
-            F.ice[g][c] = (T)((liquid ? 0.0 : ce[g]) * rho);
+            pair_put(F.v[g], F.lo_v[g], c, qv * rho);
+            pair_put(F.cl[g], F.lo_cl[g], c, (liquid ? ce[g] : 0.0) * rho);
+            pair_put(F.ice[g], F.lo_ice[g], c, (liquid ? 0.0 : ce[g]) * rho);
         }
-        F.tr[0][c] = (T)(s_r0[q] * rho);
-        F.tr[1][c] = (T)(s_r1[q] * rho);
+        pair_put(F.tr[0], F.lo_tr[0], c, s_r0[q] * rho);
+        pair_put(F.tr[1], F.lo_tr[1], c, s_r1[q] * rho);
     }
 }
 
@@ -2388,9 +2405,12 @@ int do_mca(atjup_ctx* c){
     Fm.t = F<T>(c, ATJUP_F_T); Fm.lo_t = (float*)c->lo[ATJUP_F_T].p;
     Fm.p_stat = F<T>(c, ATJUP_F_P_STAT); Fm.rho = F<T>(c, ATJUP_F_RHO_MIX);
     Fm.tr[0] = F<T>(c, ATJUP_F_H2S); Fm.tr[1] = F<T>(c, ATJUP_F_NH4SH);
+    Fm.lo_tr[0] = (float*)c->lo[ATJUP_F_H2S].p; Fm.lo_tr[1] = (float*)c->lo[ATJUP_F_NH4SH].p;
     McaConsts K;
     for(int g = 0; g < 2; g++){
         Fm.v[g] = F<T>(c, gv[g]); Fm.cl[g] = F<T>(c, gv[g] + 1); Fm.ice[g] = F<T>(c, gv[g] + 2);
+        Fm.lo_v[g] = (float*)c->lo[gv[g]].p; Fm.lo_cl[g] = (float*)c->lo[gv[g] + 1].p;
+        Fm.lo_ice[g] = (float*)c->lo[gv[g] + 2].p;
         const atjup_gas_params& a = c->gas[g];
         K.gas[g] = McaGas{ a.t_0, a.ep, a.lv, a.ls, a.C, a.L0, a.R, a.del_alf, a.del_bet,
                            a.C_i, a.L0_i, a.del_alf_i, a.del_bet_i };

@@ -178,7 +178,7 @@ double* slice_of(atjup_ctx* c){ return first_of(c) + 2; }
 template<typename T>
 int do_minmax3d(atjup_ctx* c, int field){
     const T* f = (const T*)c->cur[field].p;
-    const float* lo = field < 4 ? (const float*)c->lo[field].p : nullptr;
+    const float* lo = field < ATJUP_N_PROG ? (const float*)c->lo[field].p : nullptr;
     hipLaunchKernelGGL(k_minmax3d<T>, dim3(RP_BLOCKS), dim3(RP_BK), 0, c->stream, f, lo, c->im, c->jm, c->km, c->pitch, parts_of(c));
     hipLaunchKernelGGL(k_first<T>, dim3(1), dim3(1), 0, c->stream, f, lo, first_of(c));
     hipLaunchKernelGGL(k_reduce_parts<0>, dim3(1), dim3(RP_BK), 0, c->stream, (const Part*)parts_of(c), RP_BLOCKS,
@@ -201,7 +201,7 @@ int do_continuity(atjup_ctx* c){
 
 template<typename T>
 int do_slice(atjup_ctx* c, int field, int axis, int index, int n){
-    const float* lo = field < 4 ? (const float*)c->lo[field].p : nullptr;
+    const float* lo = field < ATJUP_N_PROG ? (const float*)c->lo[field].p : nullptr;
     hipLaunchKernelGGL(k_slice<T>, dim3(nblk(n, RP_BK)), dim3(RP_BK), 0, c->stream, (const T*)c->cur[field].p, lo,
         axis, index, c->im, c->jm, c->km, c->pitch, slice_of(c));
     return check_launch();

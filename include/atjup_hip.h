@@ -35,7 +35,8 @@ extern "C" {
 #define ATJUP_NONFINITE      1
 
 #define ATJUP_PRECISION_FP64      0   /* the drop-in: every field and stencil in FP64 */
-#define ATJUP_PRECISION_FP32_REF  1   /* fields and RK4 in FP32; Poisson in FP32 inside FP64 refinement */
+#define ATJUP_PRECISION_FP32_REF  1   /* fields and RK4 in FP32 (t, u, v, w and the species as FP32
+                                         pairs hi + lo); Poisson in FP32 inside FP64 refinement */
 #define ATJUP_PRECISION_TANGENT   2   /* FP64 with a directional derivative carried along every field
                                          (forward-mode tangent of the dynamics profile; no counterpart
                                          in the reference: it verifies the adjoint) */
@@ -275,7 +276,7 @@ int atjup_hip_mca(atjup_ctx* ctx);
  * loops on ties (a NaN is never taken, unless it is the loops' starting cell (0, 0, 0), which then
  * stands for both). out[0] max, out[1] min (the raw values: the caller applies the printout's
  * coefficient and conversion); loc = {i, j, k} of the max, then of the min. FP32-refined fields
- * t, u, v, w are read as hi + lo. */
+ * carried as pairs (t, u, v, w and the species) are read as hi + lo. */
 int atjup_hip_search_minmax_3d(atjup_ctx* ctx, int field, double* out, int* loc);
 /* The search of atjup_hip_search_minmax_3d, allocating nothing. No counterpart in the reference: it
  * is for callers that cannot allocate device memory (tight serving loops, harnesses that forbid
@@ -311,7 +312,7 @@ void atjup_hip_default_mca_params(atjup_mca_params* params);
  * (:297-305) and vel_star, each [i][j][k] (or [j][k]) FP64. carried = 1 writes "JUP2" (the
  * reference's default ATJUP_RESTART_CARRIED=1), 0 writes "JUP1". The n-level copies are written
  * from the live fields, as the reference writes them after restoreVar. The FP32-refined form
- * writes its values widened to FP64 (t, u, v, w as hi + lo).
+ * writes its values widened to FP64 (t, u, v, w and the species as hi + lo).
  * load_state reads either magic, rejects a file of another grid or a truncated one
  * (ATJUP_ERR_STATE), and rejects n-level arrays that differ from their fields: this path keeps one
  * copy. iter receives the header's iteration; carried_out whether the file held the carried set. */

@@ -172,9 +172,9 @@ def test_500_full_physics_steps():
     precipitation, chemistry, NH4SH equilibrium; vapour at 0.5-1.0 x saturation): per prognostic
     field and p_dyn, the normwise error against FP64 beside its floor, the FP64 path from the
     FP32-rounded start. A field passes at <= 1e-5, or at <= 2x its floor where the floor exceeds
-    5e-6. The three condensates that form on this state miss it (3 to 26x their floors, the form
-    stores them as single FP32 words); for them the test holds a regression bound of 50x the floor.
-    Fields zero in both paths are skipped."""
+    5e-6. The three condensates that form on this state miss it (2.3 to 4x their floors: the FP32
+    right-hand side moves the saturation border by more than the start's rounding does); for them
+    the test holds a regression bound of 6x the floor. Fields zero in both paths are skipped."""
     from atjup_amd.lib import Context
     g = inputs.Grid()
     G, Cn, K, _ = abi.make_profile("full", sweeps=200, grid=g, radiation=True, precip=True, chemistry=True,
@@ -210,7 +210,7 @@ def test_500_full_physics_steps():
         ok = e <= bar
         print(f"{f}: {e:.2e} (floor {fl:.2e}, bar {bar:.1e}) {'pass' if ok else 'MISS'}")
         if f in FORMING_CONDENSATES:
-            assert e <= 50.0 * fl, f
+            assert e <= 6.0 * fl, f
         elif not ok:
             failures.append(f)
     for c in ctxs.values():
