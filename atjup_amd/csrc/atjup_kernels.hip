@@ -967,6 +967,86 @@ __global__ __launch_bounds__(256) void k_shapiro_i(const T* __restrict__ in, T* 
     out[q] = res;
 }
 
+// The order-2 pass of all three directions in one kernel: the phi pass, then the theta pass on its
+// result, then the radial pass on that, each cell's arithmetic the one of k_shapiro_k, _j, _i in
+// sequence (so the result is bit-identical to the three launches). A thread owns SH_L levels
+// [i0, i0 + SH_L) of a (j, k) column: it forms the phi-filtered values of rows j-1, j, j+1 at those
+// levels and the two beside them, the theta-filtered values of row j from those, and the radial
+// results from three consecutive theta-filtered values. Every load is unconditional and in bounds
+// (the branches select among loaded values), so all of a thread's loads are in flight at once. The
+// field is read once and written once instead of three times each.
+constexpr int SH_BK = 64, SH_BJ = 4;
+#ifndef SH_L
+#define SH_L 4                                      // levels per thread
+#endif
+
+template<typename T>
+__global__ __launch_bounds__(SH_BK * SH_BJ) void k_shapiro2_fused(const T* __restrict__ in, T* __restrict__ out,
+        const int* __restrict__ itopo, int im, int jm, int km, int pitch, size_t plane, int nkb, int njb, int nch,
+        T coeff){
+    const int nb = nkb * njb * nch;
+    // level ranges fastest, each XCD a contiguous range of blocks: the workgroups that share a
+    // tile's outer levels and rows run beside each other on one L2
+    const int b = xcd_remap(blockIdx.x, nb);
+    const int ch = b % nch, rest = b / nch;
+    const int kb = rest % nkb, jb = rest / nkb;
+    const int k = kb * SH_BK + threadIdx.x, j = jb * SH_BJ + threadIdx.y;
+    if(k >= km || j >= jm) return;
+    const int i0 = ch * SH_L;
+    const int kk = (k == km - 1) ? 0 : k;                  // the seam copy takes the value of k = 0
+    const int km1 = (kk == 0) ? km - 2 : kk - 1, kp1 = (kk == km - 2) ? 0 : kk + 1;
+    const int jm1 = max(j - 1, 0), jp1 = min(j + 1, jm - 1);
+    auto surf = [&](int jj, int kx){ return max(itopo[(size_t)jj * km + kx], 0); };
+    const int jr[3] = { jm1, j, jp1 };
+    int sc[3], sm[3], sp[3];
+    #pragma unroll
+    for(int r = 0; r < 3; r++){ sc[r] = surf(jr[r], kk); sm[r] = surf(jr[r], km1); sp[r] = surf(jr[r], kp1); }
+    const int tm = surf(jm1, k), tc = surf(j, k), tp = surf(jp1, k);
+    const T two = T(2);
+    // theta-filtered values of row j at levels i0 - 1 .. i0 + SH_L (clamped into the column; the
+    // clamped ones are never used)
+    T th[SH_L + 2];
+    #pragma unroll
+    for(int l = 0; l < SH_L + 2; l++){
+        const int i = min(max(i0 - 1 + l, 0), im - 1);
+        T ph[3];
+        #pragma unroll
+        for(int r = 0; r < 3; r++){                        // phi pass of row jr[r] (k_shapiro_k)
+            const size_t r0 = ((size_t)i * jm + jr[r]) * pitch;
+            const T c = in[r0 + kk], am = in[r0 + km1], ap = in[r0 + kp1];
+            if(i < sc[r]) ph[r] = c;
+            else {
+                const T a = (i >= sm[r]) ? am : c;
+                const T bb = (i >= sp[r]) ? ap : c;
+                ph[r] = c + coeff * (a - two * c + bb);
+            }
+        }
+        const T c = ph[1];                                 // theta pass (k_shapiro_j)
+        if(i < tc) th[l] = c;
+        else {
+            const T a = (i >= tm) ? ph[0] : c;
+            const T bb = (i >= tp) ? ph[2] : c;
+            th[l] = c + coeff * (a - two * c + bb);
+        }
+    }
+    // radial pass (k_shapiro_i, order 2)
+    const int i_surf = tc;
+    const size_t col = (size_t)j * pitch + k;
+    #pragma unroll
+    for(int l = 1; l <= SH_L; l++){
+        const int i = i0 - 1 + l;
+        if(i >= im) break;
+        const T cur = th[l];
+        T res = cur;
+        if(i >= i_surf){
+            const T lo = (i - 1 >= i_surf) ? th[l - 1] : cur;
+            const T hi = (i + 1 <= im - 1) ? th[l + 1] : cur;
+            res = cur + coeff * (lo - two * cur + hi);
+        }
+        out[col + (size_t)i * plane] = res;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // computeMixtureDensity (InitValues_Jup.cpp:717-740), every cell
 // ---------------------------------------------------------------------------------------------
@@ -1556,6 +1636,14 @@ template<typename T>
 int do_shapiro(atjup_ctx* c, int field, int order, double strength, int passes, int axes){
     dim3 grid(nblk(c->km, 128), c->jm, c->im), blk(128);
     for(int ps = 0; ps < passes; ps++){
+        if(order == 2 && axes == 7){                 // the three directions in one launch
+            const int nkb = nblk(c->km, SH_BK), njb = nblk(c->jm, SH_BJ), nch = nblk(c->im, SH_L);
+            hipLaunchKernelGGL(k_shapiro2_fused<T>, dim3(nkb * njb * nch), dim3(SH_BK, SH_BJ), 0, c->stream,
+                (const T*)c->cur[field].p, (T*)c->scratch.p, (const int*)c->itopo.p, c->im, c->jm, c->km,
+                c->pitch, c->plane, nkb, njb, nch, (T)(strength * 0.25));
+            std::swap(c->cur[field].p, c->scratch.p);
+            continue;
+        }
         if(axes & 4){
             hipLaunchKernelGGL(k_shapiro_k<T>, grid, blk, 0, c->stream, (const T*)c->cur[field].p, (T*)c->scratch.p,
                 (const int*)c->itopo.p, c->im, c->jm, c->km, c->pitch, order, strength);

@@ -1355,26 +1355,42 @@ __global__ __launch_bounds__(256) void k_nh4sh_eq(T* __restrict__ t, float* __re
     const double b = dmax(0.0, h2s_0) / C.m_h2s;
     const double s = dmax(0.0, nh4sh_0) / C.m_nh4sh;
     const double heat = C.L_f * C.m_nh4sh / (C.cp_mix * rho);
-    auto f = [&](double x){
+    // The bisection needs only the sign of f(x) = A - pow(10, y) atm^2, y = 14.82 - 4705 / T(x):
+    // 1 for f > 0, 0 for f <= 0, 2 unordered (NaN). The sign is first taken from exp(y ln 10) in
+    // place of pow(10, y), which differ by less than 4e-14 of the value for |y| < 100 and 1.3e-13
+    // for |y| < 300 (the rounding of y ln 10, about 2 |y ln 10| 2^-53, and the two functions' own
+    // ulps; below y = -307 the values are subnormal and lose that bound, so y < -300, T below
+    // 15 K, is left out); where f is farther from zero than 1e-12 + 4e-15 |y| of its larger term
+    // the sign is the one f has with pow, otherwise (and for every non-finite case) f is formed with
+    // pow as written. So the decisions, and the result, are those of the bisection on f itself at
+    // a fraction of its pow calls: the halvings far from the root, about 40 of 50, take exp.
+    constexpr double LN10 = 2.302585092994045684;
+    auto f_sign = [&](double x) -> int {
         const double Tx = T0 + heat * x, RT = R_univ * Tx;
-        return (a - x) * (b - x) * RT * RT - pow(10.0, 14.82 - 4705.0 / Tx) * atm * atm;
+        const double A = (a - x) * (b - x) * RT * RT;
+        const double y = 14.82 - 4705.0 / Tx;
+        const double Pc = exp(y * LN10) * atm * atm;
+        const double d = A - Pc;
+        if(y > -300.0 && fabs(d) > (1.0e-12 + 4.0e-15 * fabs(y)) * dmax(fabs(A), Pc)) return d > 0.0 ? 1 : 0;
+        const double fe = A - pow(10.0, y) * atm * atm;
+        return (fe > 0.0) ? 1 : ((fe <= 0.0) ? 0 : 2);
     };
-    const double f0 = f(0.0);
+    const int s0 = f_sign(0.0);
     double lo, hi;
-    if(f0 > 0.0){ lo = 0.0; hi = dmin(a, b); }
+    if(s0 == 1){ lo = 0.0; hi = dmin(a, b); }
     else if(s > 0.0){
-        if(f(-s) <= 0.0){ lo = -s; hi = -s; }
+        if(f_sign(-s) == 0){ lo = -s; hi = -s; }
         else { lo = -s; hi = 0.0; }
     } else return;
     if(hi > lo){
         const double tol = 1.0e-15 * dmax(a, dmax(b, s));
         for(int it = 0; it < 80; it++){
             const double xm = 0.5 * (lo + hi);
-            if(f(xm) > 0.0) lo = xm; else hi = xm;
+            if(f_sign(xm) == 1) lo = xm; else hi = xm;
             if(hi - lo <= tol) break;
         }
     }
-    const double x = (f0 > 0.0) ? lo : hi;
+    const double x = (s0 == 1) ? lo : hi;
     if(x == 0.0) return;
     pair_put(nh3, lo_nh3, q, dmax(0.0, nh3_0 - C.m_nh3 * x));
     pair_put(h2s, lo_h2s, q, dmax(0.0, h2s_0 - C.m_h2s * x));

@@ -82,6 +82,23 @@ def test_shapiro_bit_exact(pair, order):
         assert np.array_equal(gpu.get(n), orc.get(n)), (n, order)
 
 
+def test_three_direction_filter_equals_the_directional_passes(pair):
+    """The order-2 filter over all three directions runs as one kernel; it must equal the phi, theta
+    and radial passes applied one after the other, bit for bit, on the obstacle's grid."""
+    gpu, _, st = pair
+    x = gpu.get("t")
+    try:
+        gpu.shapiro("t", 2, 0.7, 1, 7)
+        fused = gpu.get("t")
+        gpu.set("t", x)
+        for axes in (4, 2, 1):
+            gpu.shapiro("t", 2, 0.7, 1, axes)
+        assert np.array_equal(fused, gpu.get("t"))
+        assert not np.array_equal(fused, x)
+    finally:
+        gpu.set("t", x)
+
+
 def test_five_steps_full_grid():
     """Five time steps of the 41 x 181 x 361 dynamics case (200 sweeps, omega = 1)."""
     gpu, orc, _ = make_pair("dyn", sweeps=200)
