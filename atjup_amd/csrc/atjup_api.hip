@@ -229,8 +229,12 @@ int step_block(atjup_ctx* c, int iter, int first){
         // the three adjustments, then RK_SCALAR_SYNC 2 (the identity on the shared storage),
         // the closure with its wall function, the NH4SH limiter and its smoothing
         // (cJupiterModel.cpp:1060-1161)
-        for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
-            if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
+        if(c->kn.satadj && c->gas_set[0] && c->gas_set[1] && c->gas_set[2] && c->have_mask){
+            if((rc = launch_satadj3(c))) return rc;          // the three in one pass
+        } else {
+            for(int gas = 0; gas < 3 && c->kn.satadj; gas++)
+                if((rc = atjup_hip_satadj(c, gas, nullptr))) return rc;
+        }
         if(c->kn.nh4sh_eq && (rc = launch_nh4sh_eq(c))) return rc;           // cJupiterModel.cpp:1084
         if(c->kn.precip && (rc = atjup_hip_precipitation(c))) return rc;      // cJupiterModel.cpp:1090
         if((rc = atjup_hip_turbulence_run(c))) return rc;

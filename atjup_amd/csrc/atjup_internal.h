@@ -177,6 +177,7 @@ int launch_turbulence(atjup_ctx* c);
 int launch_turb_wall(atjup_ctx* c);
 int launch_turb_sanitize(atjup_ctx* c);
 int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g);
+int launch_satadj3(atjup_ctx* c);                     // the three gases in one pass (constants set)
 int launch_radiation(atjup_ctx* c);
 int launch_precipitation(atjup_ctx* c);
 int launch_chem_kernel(atjup_ctx* c, int which);      // 0: DiffMassFluxJup, 1: ChemMassRateJup

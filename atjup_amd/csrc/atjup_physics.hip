@@ -1034,10 +1034,10 @@ __global__ __launch_bounds__(256) void k_satadj(T* __restrict__ c, T* __restrict
     if(cv <= 0.0) cv = 0.0;
     if(ccl <= 0.0) ccl = 0.0;
     if(cic <= 0.0) cic = 0.0;
-    if(!lo_c){
-        c[q] = (T)cv;
-        cloud[q] = (T)ccl;
-        ice[q] = (T)cic;
+    if(!lo_c){                                     // stored where the bits changed (cv0 is c[q] widened)
+        if(!same_bits((T)cv, (T)cv0)) c[q] = (T)cv;
+        if(!same_bits((T)ccl, (T)ccl0)) cloud[q] = (T)ccl;
+        if(!same_bits((T)cic, (T)cic0)) ice[q] = (T)cic;
     } else {                                       // pairs: rewritten where the value changed
         if(!(cv == cv0)) pair_put(c, lo_c, q, cv);
         if(!(ccl == ccl0)) pair_put(cloud, lo_cl, q, ccl);
@@ -1051,6 +1051,66 @@ __global__ __launch_bounds__(256) void k_satadj(T* __restrict__ c, T* __restrict
         } else {
             t[q] = (T)tq;
         }
+    }
+}
+
+// The three adjustments of the physics block (H2O, NH3, CH4 in that order) in one pass: a cell's
+// adjustments depend on no other cell, so each thread runs the three in sequence on its cell with
+// t, p_stat, rho and the flag read once. What a gas's adjustment reads of t is what the per-gas
+// kernel would have stored before it (the pair split again in the refined form), so the stored
+// results are those of three k_satadj launches.
+template<typename T>
+struct Sat3Args {
+    SatParams P[3];
+    T* sp[3][3];                 // [gas][vapour, cloud, ice]
+    float* lo[3][3];             // their low words in the refined form (else null)
+    T* t; float* lo_t;
+    const T* ps; const T* rho; const uint8_t* flags;
+    int jm, km, pitch;
+};
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_satadj3(const Sat3Args<T> A){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= A.km) return;
+    const size_t q = ((size_t)i * A.jm + j) * A.pitch + k;
+    const bool solid = A.flags[q] & FL_SELF;
+    const double ps = (double)A.ps[q], rho = (double)A.rho[q];
+    double tq = pair_get(A.t, A.lo_t, q);
+    bool t_dirty = false;
+    float th = 0.0f, tl = 0.0f;                   // the refined form's stored pair of t
+    #pragma unroll 1
+    for(int g = 0; g < 3; g++){
+        T* c = A.sp[g][0]; T* cloud = A.sp[g][1]; T* ice = A.sp[g][2];
+        float* lo_c = A.lo[g][0]; float* lo_cl = A.lo[g][1]; float* lo_ic = A.lo[g][2];
+        double cv = pair_get(c, lo_c, q), ccl = pair_get(cloud, lo_cl, q), cic = pair_get(ice, lo_ic, q);
+        const double cv0 = cv, ccl0 = ccl, cic0 = cic;
+        const double t0 = tq;
+        if(!solid) satadj_cell(cv, ccl, cic, tq, ps, rho, A.P[g]);
+        if(cv <= 0.0) cv = 0.0;
+        if(ccl <= 0.0) ccl = 0.0;
+        if(cic <= 0.0) cic = 0.0;
+        if(!lo_c){
+            if(!same_bits((T)cv, (T)cv0)) c[q] = (T)cv;
+            if(!same_bits((T)ccl, (T)ccl0)) cloud[q] = (T)ccl;
+            if(!same_bits((T)cic, (T)cic0)) ice[q] = (T)cic;
+        } else {
+            if(!(cv == cv0)) pair_put(c, lo_c, q, cv);
+            if(!(ccl == ccl0)) pair_put(cloud, lo_cl, q, ccl);
+            if(!(cic == cic0)) pair_put(ice, lo_ic, q, cic);
+        }
+        if(tq != t0 || !(tq == tq)){
+            t_dirty = true;
+            if(A.lo_t){                           // stored as a pair: the next gas reads hi + lo
+                th = (float)tq;
+                tl = (float)(tq - (double)th);
+                tq = (double)th + (double)tl;
+            }
+        }
+    }
+    if(t_dirty){
+        if(A.lo_t){ A.t[q] = (T)th; A.lo_t[q] = tl; }
+        else A.t[q] = (T)tq;
     }
 }
 
@@ -1314,6 +1374,28 @@ int do_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){
         F<T>(c, ATJUP_F_P_STAT), F<T>(c, ATJUP_F_RHO_MIX), (const uint8_t*)c->flags.p,
         c->jm, c->km, c->pitch, P, (float*)c->lo[ATJUP_F_T].p, (float*)c->lo[ids[gas][0]].p,
         (float*)c->lo[ids[gas][1]].p, (float*)c->lo[ids[gas][2]].p);
+    return check_launch();
+}
+
+template<typename T>
+int do_satadj3(atjup_ctx* c){
+    static const int ids[3][3] = { { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE },
+                                   { ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE },
+                                   { ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE } };
+    Sat3Args<T> A;
+    for(int gas = 0; gas < 3; gas++){
+        const atjup_gas_params& g = c->gas[gas];
+        A.P[gas] = SatParams{ g.t_0, g.t_00, g.ep, g.lv, g.ls, g.C, g.L0, g.R, g.del_alf, g.del_bet,
+                              g.C_i, g.L0_i, g.del_alf_i, g.del_bet_i, c->c.cp_mix, c->c.t_ref, c->c.r_mix };
+        for(int s = 0; s < 3; s++){
+            A.sp[gas][s] = F<T>(c, ids[gas][s]);
+            A.lo[gas][s] = (float*)c->lo[ids[gas][s]].p;
+        }
+    }
+    A.t = F<T>(c, ATJUP_F_T); A.lo_t = (float*)c->lo[ATJUP_F_T].p;
+    A.ps = F<T>(c, ATJUP_F_P_STAT); A.rho = F<T>(c, ATJUP_F_RHO_MIX); A.flags = (const uint8_t*)c->flags.p;
+    A.jm = c->jm; A.km = c->km; A.pitch = c->pitch;
+    hipLaunchKernelGGL(k_satadj3<T>, dim3(nblk(c->km, 128), c->jm, c->im), dim3(128), 0, c->stream, A);
     return check_launch();
 }
 
@@ -2466,6 +2548,7 @@ int launch_turbulence(atjup_ctx* c){ return DISPATCH(do_turbulence, c); }
 int launch_turb_wall(atjup_ctx* c){ return DISPATCH(do_turb_wall, c); }
 int launch_turb_sanitize(atjup_ctx* c){ return DISPATCH(do_turb_sanitize, c); }
 int launch_satadj(atjup_ctx* c, int gas, const atjup_gas_params& g){ return DISPATCH(do_satadj, c, gas, g); }
+int launch_satadj3(atjup_ctx* c){ return DISPATCH(do_satadj3, c); }
 int launch_flux_limiter(atjup_ctx* c){ return DISPATCH(do_flux_limiter, c); }
 int launch_radiation(atjup_ctx* c){ return DISPATCH(do_radiation, c); }
 int launch_precipitation(atjup_ctx* c){ return DISPATCH(do_precipitation, c); }
