@@ -502,6 +502,7 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep(T* __restrict__ x0, 
         const T pjm = (j == 1 && gw)      ? (T)((T)c43 * pc - (T)c13 * oth[q + R]) : oth[q - R];
         const T pkp = oth[row + me], pkm = oth[row + mw];
         const uint8_t fl = flg[q];
+        const S sq = src[q];        // issued with the other loads, ahead of the flag tests
         const PoisIJ<double> W = pw[wave_uniform(i * jm + j)];     // a wavefront is one (i, j) row
         const T num1 = (T)W.num1, num2 = (T)W.num2, num3 = (T)W.num3;
         T acc, den;
@@ -520,7 +521,7 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep(T* __restrict__ x0, 
             den = (T)W.denom;
         }
         if(skip) continue;
-        const T p_gs = (acc - (T)src[q]) / den;
+        const T p_gs = (acc - (T)sq) / den;
         own[q] = (omega == 1.0) ? p_gs : (T)((T(1) - (T)omega) * pc + (T)omega * p_gs);
     }
 }
@@ -530,7 +531,7 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep(T* __restrict__ x0, 
 // vector-memory instructions the one-cell pass spends on 64. The arithmetic of a cell is that
 // pass's. Needs an even ring (nm).
 template<typename T, typename S>
-__device__ __forceinline__ void rb_cell(T* __restrict__ own, const T* __restrict__ oth, const S* __restrict__ src,
+__device__ __forceinline__ void rb_cell(T* __restrict__ own, const S sq,
         const uint8_t fl, const PoisIJ<double>& W, size_t q, size_t P, size_t R, int i, int j, int im, int jm,
         bool gw, double omega, double c43, double c13, T oim, T oip, T ojm, T ojp, T pkp, T pkm){
     const bool edge = i == 1 || i == 2 || i == im - 3 || i == im - 2 || j == 1 || j == 2 || j == jm - 3 || j == jm - 2;
@@ -561,7 +562,7 @@ __device__ __forceinline__ void rb_cell(T* __restrict__ own, const T* __restrict
         acc = (pip + pim) * num1 + (pjp + pjm) * num2 + (pkp + pkm) * num3;
         den = (T)W.denom;
     }
-    const T p_gs = (acc - (T)src[q]) / den;
+    const T p_gs = (acc - (T)sq) / den;
     own[q] = (omega == 1.0) ? p_gs : (T)((T(1) - (T)omega) * pc + (T)omega * p_gs);
 }
 
@@ -570,7 +571,7 @@ __device__ __forceinline__ void rb_cell(T* __restrict__ own, const T* __restrict
 // values are the stored ones. A fluid cell with six fluid neighbours sums the same six terms in the
 // same order from 0 and takes the row's precomputed denominator of that sum.
 template<typename T, typename S>
-__device__ __forceinline__ void rb_cell_plain(T* __restrict__ own, const S* __restrict__ src, const uint8_t fl,
+__device__ __forceinline__ void rb_cell_plain(T* __restrict__ own, const S sq, const uint8_t fl,
         const PoisIJ<double>& W, size_t q, double omega, T pim, T pip, T pjm, T pjp, T pkp, T pkm){
     const T num1 = (T)W.num1, num2 = (T)W.num2, num3 = (T)W.num3;
     T acc, den;
@@ -595,7 +596,7 @@ __device__ __forceinline__ void rb_cell_plain(T* __restrict__ own, const S* __re
         acc = (pip + pim) * num1 + (pjp + pjm) * num2 + (pkp + pkm) * num3;
         den = (T)W.denom;
     }
-    const T p_gs = (acc - (T)src[q]) / den;
+    const T p_gs = (acc - (T)sq) / den;
     own[q] = (omega == 1.0) ? p_gs : (T)((T(1) - (T)omega) * own[q] + (T)omega * p_gs);
 }
 
@@ -641,6 +642,9 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep2(T* __restrict__ x0,
     const V vim = *reinterpret_cast<const V*>(oth + q - P), vip = *reinterpret_cast<const V*>(oth + q + P);
     const V vjm = *reinterpret_cast<const V*>(oth + q - R), vjp = *reinterpret_cast<const V*>(oth + q + R);
     const V vc = *reinterpret_cast<const V*>(oth + q);               // the other colour at m, m+1
+    // the own colour's sources, loaded with the neighbours: every path reads them, and issuing them
+    // here saves the second round trip to memory a load after the flag tests would cost
+    const typename Pair<S>::V sv = *reinterpret_cast<const typename Pair<S>::V*>(src + q);
     const int off = (colour + i + j) & 1;
     // the phi neighbours: off = 1: m | m+1 for cell m, m+1 | m+2 for cell m+1; off = 0: m-1 | m, m | m+1
     const T edge = off ? oth[row + (m + 2 == nm ? 0 : m + 2)] : oth[row + (m == 0 ? nm - 1 : m - 1)];
@@ -650,13 +654,13 @@ __global__ __launch_bounds__(RB_BM * RB_BJ) void k_rb_sweep2(T* __restrict__ x0,
     if(off){ pkm0 = vc.x; pkp0 = vc.y; pkm1 = vc.y; pkp1 = edge; }
     else   { pkm0 = edge; pkp0 = vc.x; pkm1 = vc.x; pkp1 = vc.y; }
     if(rb_ghost_row(gw, i, j, im, jm)){
-        rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 & 0xff), W, q, P, R, i, j, im, jm, gw, omega, c43, c13,
+        rb_cell<T, S>(own, sv.x, (uint8_t)(fl2 & 0xff), W, q, P, R, i, j, im, jm, gw, omega, c43, c13,
                       vim.x, vip.x, vjm.x, vjp.x, pkp0, pkm0);
-        rb_cell<T, S>(own, oth, src, (uint8_t)(fl2 >> 8), W, q + 1, P, R, i, j, im, jm, gw, omega, c43, c13,
+        rb_cell<T, S>(own, sv.y, (uint8_t)(fl2 >> 8), W, q + 1, P, R, i, j, im, jm, gw, omega, c43, c13,
                       vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
     } else {
-        rb_cell_plain<T, S>(own, src, (uint8_t)(fl2 & 0xff), W, q, omega, vim.x, vip.x, vjm.x, vjp.x, pkp0, pkm0);
-        rb_cell_plain<T, S>(own, src, (uint8_t)(fl2 >> 8), W, q + 1, omega, vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
+        rb_cell_plain<T, S>(own, sv.x, (uint8_t)(fl2 & 0xff), W, q, omega, vim.x, vip.x, vjm.x, vjp.x, pkp0, pkm0);
+        rb_cell_plain<T, S>(own, sv.y, (uint8_t)(fl2 >> 8), W, q + 1, omega, vim.y, vip.y, vjm.y, vjp.y, pkp1, pkm1);
     }
 }
 
@@ -690,14 +694,17 @@ __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __res
     const size_t q = row + m;
     const bool gw = ghost && colour == 0;
     T vim[N], vip[N], vjm[N], vjp[N], vc[N];
+    S sv[N];        // the own colour's sources, loaded with the neighbours (one round trip to memory)
     #pragma unroll
     for(int h = 0; h < N; h += 2){
         const V a = *reinterpret_cast<const V*>(oth + q + h - P), bq = *reinterpret_cast<const V*>(oth + q + h + P);
         const V c = *reinterpret_cast<const V*>(oth + q + h - R), d = *reinterpret_cast<const V*>(oth + q + h + R);
         const V e = *reinterpret_cast<const V*>(oth + q + h);
+        const typename Pair<S>::V f = *reinterpret_cast<const typename Pair<S>::V*>(src + q + h);
         vim[h] = a.x; vim[h + 1] = a.y; vip[h] = bq.x; vip[h + 1] = bq.y;
         vjm[h] = c.x; vjm[h + 1] = c.y; vjp[h] = d.x; vjp[h + 1] = d.y;
         vc[h] = e.x; vc[h + 1] = e.y;
+        sv[h] = f.x; sv[h + 1] = f.y;
     }
     const int off = (colour + i + j) & 1;
     // the phi neighbours: off = 1: cell m+r reads m+r (k-1) and m+r+1 (k+1); off = 0: m+r-1, m+r
@@ -715,7 +722,6 @@ __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __res
         const T den = std::is_same<T, float>::value ? (T)W.den_fluid32 : (T)W.den_fluid;
         #pragma unroll
         for(int h = 0; h < N; h += 2){
-            const typename Pair<S>::V sv = *reinterpret_cast<const typename Pair<S>::V*>(src + q + h);
             T res[2];
             #pragma unroll
             for(int e = 0; e < 2; e++){
@@ -729,7 +735,7 @@ __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __res
                 acc += num2 * vjm[r];
                 acc += num3 * pkp;
                 acc += num3 * pkm;
-                res[e] = (acc - (T)(e ? sv.y : sv.x)) / den;
+                res[e] = (acc - (T)sv[r]) / den;
             }
             V o; o.x = res[0]; o.y = res[1];
             *reinterpret_cast<V*>(own + q + h) = o;
@@ -743,10 +749,10 @@ __global__ __launch_bounds__(256) void k_rb_sweep2f(T* __restrict__ x0, T* __res
         else   { pkm = (r > 0) ? vc[r - 1] : edge; pkp = vc[r]; }
         const uint8_t fl = (uint8_t)((flw >> (8 * r)) & 0xff);
         if(ghost_row)
-            rb_cell<T, S>(own, oth, src, fl, W, q + r, P, R, i, j, im, jm, gw, omega, c43, c13,
+            rb_cell<T, S>(own, sv[r], fl, W, q + r, P, R, i, j, im, jm, gw, omega, c43, c13,
                           vim[r], vip[r], vjm[r], vjp[r], pkp, pkm);
         else
-            rb_cell_plain<T, S>(own, src, fl, W, q + r, omega, vim[r], vip[r], vjm[r], vjp[r], pkp, pkm);
+            rb_cell_plain<T, S>(own, sv[r], fl, W, q + r, omega, vim[r], vip[r], vjm[r], vjp[r], pkp, pkm);
     }
 }
 
