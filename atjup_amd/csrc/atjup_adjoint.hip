@@ -14,16 +14,20 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
+#include <vector>
 
 namespace atjup {
 
-__device__ __forceinline__ void aadd(double* p, double v){ if(v != 0.0) unsafeAtomicAdd(p, v); }
+template<typename T>
+__device__ __forceinline__ void aadd(T* p, double v){ if(v != 0.0) unsafeAtomicAdd(p, (T)v); }
 
 // =============================================================================================
 // boundary passes (bcRadius, bcTheta, bcPhi, bcSolidGround, MOUNT_T_AMBIENT), transposed
 // =============================================================================================
 // MOUNT_T_AMBIENT^T: the solid cells of a row carried the row's fluid mean
-__global__ __launch_bounds__(64) void ka_t_ambient(double* __restrict__ lt, const uint8_t* __restrict__ msk,
+template<typename T>
+__global__ __launch_bounds__(64) void ka_t_ambient(T* __restrict__ lt, const uint8_t* __restrict__ msk,
         const int2* __restrict__ rows, int n, int jm, int km, int pitch){
     const int r = blockIdx.x;
     if(r >= n) return;
@@ -50,8 +54,9 @@ __global__ __launch_bounds__(64) void ka_t_ambient(double* __restrict__ lt, cons
 
 // bcSolidGround^T over the solid-cell list: velocities zeroed; t and p_dyn extrapolated from a
 // fluid pair (code 1..6) or, for p_dyn, zeroed (code 0)
-__global__ void ka_bc_solid(double* __restrict__ lu, double* __restrict__ lv, double* __restrict__ lw,
-        double* __restrict__ lt, double* __restrict__ lp, const SolidCell* __restrict__ cells, int n,
+template<typename T>
+__global__ void ka_bc_solid(T* __restrict__ lu, T* __restrict__ lv, T* __restrict__ lw,
+        T* __restrict__ lt, T* __restrict__ lp, const SolidCell* __restrict__ cells, int n,
         int jm, int pitch, size_t plane, double c43, double c13){
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if(s >= n) return;
@@ -67,7 +72,7 @@ __global__ void ka_bc_solid(double* __restrict__ lu, double* __restrict__ lv, do
         default: da = 0; break;
     }
     if(da == 0){ lp[q] = 0.0; return; }
-    double* X[2] = { lt, lp };
+    T* X[2] = { lt, lp };
     for(int f = 0; f < 2; f++){
         const double l = X[f][q];
         X[f][q] = 0.0;
@@ -77,12 +82,13 @@ __global__ void ka_bc_solid(double* __restrict__ lu, double* __restrict__ lv, do
 }
 
 // bcPhi^T: the seam meridian was the copy of k = 0
-__global__ void ka_bc_phi(double* __restrict__ a0, double* __restrict__ a1, double* __restrict__ a2,
-        double* __restrict__ a3, int jm, int km, int pitch){
+template<typename T>
+__global__ void ka_bc_phi(T* __restrict__ a0, T* __restrict__ a1, T* __restrict__ a2,
+        T* __restrict__ a3, int jm, int km, int pitch){
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if(j >= jm) return;
     const size_t r0 = ((size_t)i * jm + j) * pitch;
-    double* X[4] = { a0, a1, a2, a3 };
+    T* X[4] = { a0, a1, a2, a3 };
     for(int f = 0; f < 4; f++){
         X[f][r0] += X[f][r0 + km - 1];
         X[f][r0 + km - 1] = 0.0;
@@ -91,24 +97,25 @@ __global__ void ka_bc_phi(double* __restrict__ a0, double* __restrict__ a1, doub
 
 // bcTheta^T: t, u were copied from j = 3 (jm-4) into rows 0..2 (jm-3..jm-1); v, w ramped 1/3, 2/3
 // into rows 1, 2 and zeroed at the pole row
-__global__ void ka_bc_theta(double* __restrict__ lt, double* __restrict__ lu, double* __restrict__ lv,
-        double* __restrict__ lw, int jm, int km, int pitch){
+template<typename T>
+__global__ void ka_bc_theta(T* __restrict__ lt, T* __restrict__ lu, T* __restrict__ lv,
+        T* __restrict__ lw, int jm, int km, int pitch){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if(k >= km) return;
     const size_t R = (size_t)pitch;
     const size_t qs = (size_t)i * jm * pitch + k;
     const size_t qn = qs + (size_t)(jm - 1) * R;
-    double* ex[2] = { lt, lu };
+    T* ex[2] = { lt, lu };
     for(int f = 0; f < 2; f++){
-        double* X = ex[f];
+        T* X = ex[f];
         X[qs + 3 * R] += X[qs] + X[qs + R] + X[qs + 2 * R];
         X[qs] = X[qs + R] = X[qs + 2 * R] = 0.0;
         X[qn - 3 * R] += X[qn] + X[qn - R] + X[qn - 2 * R];
         X[qn] = X[qn - R] = X[qn - 2 * R] = 0.0;
     }
-    double* z[2] = { lv, lw };
+    T* z[2] = { lv, lw };
     for(int f = 0; f < 2; f++){
-        double* X = z[f];
+        T* X = z[f];
         X[qs + 3 * R] += (1.0 / 3.0) * X[qs + R] + (2.0 / 3.0) * X[qs + 2 * R];
         X[qs] = X[qs + R] = X[qs + 2 * R] = 0.0;
         X[qn - 3 * R] += (1.0 / 3.0) * X[qn - R] + (2.0 / 3.0) * X[qn - 2 * R];
@@ -118,11 +125,12 @@ __global__ void ka_bc_theta(double* __restrict__ lt, double* __restrict__ lu, do
 
 // bcRadius^T, first the zonal sums the top sponge's means gather their adjoint from:
 // zs[0][j] = sum_k (lV[im-1] + 2/3 lV[im-2]), zs[1][j] = sum_k lV[im-3] / 3, the same for w at 2, 3
-__global__ void ka_bc_zonal_sums(const double* __restrict__ lv, const double* __restrict__ lw, double* __restrict__ zs,
+template<typename T>
+__global__ void ka_bc_zonal_sums(const T* __restrict__ lv, const T* __restrict__ lw, double* __restrict__ zs,
         int im, int jm, int km, int pitch, size_t plane){
     const int j = blockIdx.x * blockDim.x + threadIdx.x, sel = blockIdx.y;
     if(j >= jm) return;
-    const double* X = (sel & 2) ? lw : lv;
+    const T* X = (sel & 2) ? lw : lv;
     const size_t r1 = ((size_t)(im - 1) * jm + j) * pitch;
     double s = 0.0;
     if((sel & 1) == 0){
@@ -133,8 +141,9 @@ __global__ void ka_bc_zonal_sums(const double* __restrict__ lv, const double* __
     zs[(size_t)sel * jm + j] = s;
 }
 
-__global__ void ka_bc_radius(double* __restrict__ lt, double* __restrict__ lu, double* __restrict__ lv,
-        double* __restrict__ lw, const double* __restrict__ zs, int im, int jm, int km, int pitch, size_t plane,
+template<typename T>
+__global__ void ka_bc_radius(T* __restrict__ lt, T* __restrict__ lu, T* __restrict__ lv,
+        T* __restrict__ lw, const double* __restrict__ zs, int im, int jm, int km, int pitch, size_t plane,
         double c43, double c13){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if(k >= km) return;
@@ -152,9 +161,9 @@ __global__ void ka_bc_radius(double* __restrict__ lt, double* __restrict__ lu, d
     lu[qt] = 0.0;
     // v, w: Neumann at the bottom, the top sponge with its zonal means
     const double inv_n = 1.0 / (km - 1);
-    double* X[2] = { lv, lw };
+    T* X[2] = { lv, lw };
     for(int f = 0; f < 2; f++){
-        double* V = X[f];
+        T* V = X[f];
         V[q0 + plane] += c43 * V[q0];
         V[q0 + 2 * plane] += -c13 * V[q0];
         V[q0] = 0.0;
@@ -169,7 +178,8 @@ __global__ void ka_bc_radius(double* __restrict__ lt, double* __restrict__ lu, d
 // =============================================================================================
 // Shapiro order 2 on one field (k pass, then j, then i), transposed pass by pass
 // =============================================================================================
-__global__ __launch_bounds__(256) void ka_shapiro_i(const double* __restrict__ lo, double* __restrict__ li,
+template<typename T>
+__global__ __launch_bounds__(256) void ka_shapiro_i(const T* __restrict__ lo, T* __restrict__ li,
         const int* __restrict__ itopo, int im, int jm, int km, int pitch, size_t plane, double co){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, n = blockIdx.z;
     if(k >= km) return;
@@ -185,7 +195,8 @@ __global__ __launch_bounds__(256) void ka_shapiro_i(const double* __restrict__ l
     li[q] = acc;
 }
 
-__global__ __launch_bounds__(256) void ka_shapiro_j(const double* __restrict__ lo, double* __restrict__ li,
+template<typename T>
+__global__ __launch_bounds__(256) void ka_shapiro_j(const T* __restrict__ lo, T* __restrict__ li,
         const int* __restrict__ itopo, int im, int jm, int km, int pitch, double co){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y, i = blockIdx.z;
     if(k >= km) return;
@@ -203,7 +214,8 @@ __global__ __launch_bounds__(256) void ka_shapiro_j(const double* __restrict__ l
     li[q] = acc;
 }
 
-__global__ __launch_bounds__(256) void ka_shapiro_k(const double* __restrict__ lo, double* __restrict__ li,
+template<typename T>
+__global__ __launch_bounds__(256) void ka_shapiro_k(const T* __restrict__ lo, T* __restrict__ li,
         const int* __restrict__ itopo, int im, int jm, int km, int pitch, double co){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
     if(k >= km) return;
@@ -227,8 +239,9 @@ __global__ __launch_bounds__(256) void ka_shapiro_k(const double* __restrict__ l
 // =============================================================================================
 // computeMixtureDensity^T and the hydrostatic column integral^T
 // =============================================================================================
-__global__ __launch_bounds__(256) void ka_mixture(const double* __restrict__ t, const double* __restrict__ ps,
-        const double* __restrict__ pd, double* __restrict__ lrho, double* __restrict__ lt, double* __restrict__ lp,
+template<typename T>
+__global__ __launch_bounds__(256) void ka_mixture(const T* __restrict__ t, const T* __restrict__ ps,
+        const T* __restrict__ pd, T* __restrict__ lrho, T* __restrict__ lt, T* __restrict__ lp,
         size_t n, double t_ref, double pdyn_w, double ptb, double R_mix){
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(q >= n) return;
@@ -237,16 +250,17 @@ __global__ __launch_bounds__(256) void ka_mixture(const double* __restrict__ t, 
     if(l == 0.0) return;
     const double T_ = t[q] * t_ref;
     if(!(T_ > 0.0)) return;
-    const double p_pa = (ps[q] + pdyn_w * pd[q] * ptb) * 1.0e5;
+    const double p_pa = ((double)ps[q] + pdyn_w * (double)pd[q] * ptb) * 1.0e5;
     const double r = p_pa / (R_mix * T_);
     if(!isfinite(r)) return;
-    lt[q] += l * (-r / t[q]);
+    lt[q] += l * (-r / (double)t[q]);
     lp[q] += l * (pdyn_w * ptb * 1.0e5 / (R_mix * T_));
 }
 
-__global__ __launch_bounds__(256) void ka_hydrostatic(const double* __restrict__ v, const double* __restrict__ w,
+template<typename T>
+__global__ __launch_bounds__(256) void ka_hydrostatic(const T* __restrict__ v, const T* __restrict__ w,
         const int2* __restrict__ colr, const double* __restrict__ rad, const double* __restrict__ cor_j,
-        double* __restrict__ lph, double* __restrict__ lv, double* __restrict__ lw,
+        T* __restrict__ lph, T* __restrict__ lv, T* __restrict__ lw,
         int im, int jm, int km, int pitch, size_t plane, double dr){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if(k >= km) return;
@@ -266,7 +280,7 @@ __global__ __launch_bounds__(256) void ka_hydrostatic(const double* __restrict__
         } else {
             lf_next = 0.0;
         }
-        const double vv = v[q0 + i * plane], ww = w[q0 + i * plane];
+        const double vv = (double)v[q0 + i * plane], ww = (double)w[q0 + i * plane];
         lv[q0 + i * plane] += lf * (2.0 * vv / rad[i]);
         lw[q0 + i * plane] += lf * (2.0 * ww / rad[i] + cor_);
     }
@@ -281,8 +295,9 @@ __global__ __launch_bounds__(256) void ka_hydrostatic(const double* __restrict__
 // level adds its partials in chunk order.
 constexpr int ABRL_BLOCK = 256;
 constexpr int ABRL_NCH = 32;
-__global__ __launch_bounds__(ABRL_BLOCK) void ka_brl_reduce(const double* __restrict__ t, const double* __restrict__ ps,
-        const uint8_t* __restrict__ flags, const double* __restrict__ sin_the, const double* __restrict__ gbrl,
+template<typename T>
+__global__ __launch_bounds__(ABRL_BLOCK) void ka_brl_reduce(const T* __restrict__ t, const T* __restrict__ ps,
+        const uint8_t* __restrict__ flags, const double* __restrict__ sin_the, const T* __restrict__ gbrl,
         double* __restrict__ part, int jm, int km, int pitch, double g, double rR, double t_ref){
     const int i = blockIdx.y, ch = blockIdx.x;
     const int ncell = jm * km;
@@ -294,9 +309,9 @@ __global__ __launch_bounds__(ABRL_BLOCK) void ka_brl_reduce(const double* __rest
         const size_t q = ((size_t)i * jm + j) * pitch + k;
         sl += gbrl[q];
         if(flags[q] & FL_SELF) continue;
-        const double tq = t[q];
+        const double tq = (double)t[q];
         if(!(tq > 0.0)) continue;
-        const double b = g * ps[q] / (rR * tq * t_ref);
+        const double b = g * (double)ps[q] / (rR * tq * t_ref);
         if(!isfinite(b)) continue;
         sw += sin_the[j];
     }
@@ -325,9 +340,10 @@ __global__ void ka_brl_final(const double* __restrict__ part, double* __restrict
     out[2 * i + 1] = sw;
 }
 
-__global__ __launch_bounds__(256) void ka_brl_spread(const double* __restrict__ t, const double* __restrict__ ps,
+template<typename T>
+__global__ __launch_bounds__(256) void ka_brl_spread(const T* __restrict__ t, const T* __restrict__ ps,
         const uint8_t* __restrict__ flags, const double* __restrict__ sin_the, const double* __restrict__ red,
-        double* __restrict__ gbrl, double* __restrict__ lt, int jm, int km, int pitch, double g, double rR, double t_ref){
+        T* __restrict__ gbrl, T* __restrict__ lt, int jm, int km, int pitch, double g, double rR, double t_ref){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
     if(k >= km) return;
     const size_t q = ((size_t)i * jm + j) * pitch + k;
@@ -335,9 +351,9 @@ __global__ __launch_bounds__(256) void ka_brl_spread(const double* __restrict__ 
     const double lbrl = red[2 * i], W = red[2 * i + 1];
     if(lbrl == 0.0 || !(W > 0.0)) return;
     if(flags[q] & FL_SELF) return;
-    const double tq = t[q];
+    const double tq = (double)t[q];
     if(!(tq > 0.0)) return;
-    const double b = g * ps[q] / (rR * tq * t_ref);
+    const double b = g * (double)ps[q] / (rR * tq * t_ref);
     if(!isfinite(b)) return;
     lt[q] += lbrl * sin_the[j] / W * (-b / tq);
 }
@@ -427,15 +443,16 @@ static ObstGeom obst_geom(const atjup_ctx* c){
 // =============================================================================================
 // RHS of the dry stage, transposed (RHS_Jup_Turb.cpp:31-1457 as restated by k_stage_dry)
 // =============================================================================================
+template<typename T>
 struct RhsAdjArgs {
-    const double* y[4];          // the stage input t, u, v, w
-    const double* lk[4];         // the adjoint of the stage's right-hand side (interior cells)
-    const double* la[3];         // the adjoint of aux_u, aux_v, aux_w (stage 3) or nullptr
-    double* g[4];                // the adjoint of the stage input (scatter target)
-    const double *pd, *ph, *ps, *rho, *wn;
-    double *lpd, *lph, *lrho, *gbrl;
+    const T* y[4];               // the stage input t, u, v, w
+    const T* lk[4];              // the adjoint of the stage's right-hand side (interior cells)
+    const T* la[3];              // the adjoint of aux_u, aux_v, aux_w (stage 3) or nullptr
+    T* g[4];                     // the adjoint of the stage input (scatter target)
+    const T *pd, *ph, *ps, *rho, *wn;
+    T *lpd, *lph, *lrho, *gbrl;
     const uint8_t* flags;
-    const GeoIJ<double>* geo;
+    const GeoIJ<geo_t<T>>* geo;
     int im, jm, km, pitch, nkb, njb, nb;
     size_t plane;
     double inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
@@ -453,13 +470,15 @@ constexpr int RD_BK = 64, RD_BJ = 4;
 constexpr int RD_NF = 6;                     // the scatter targets: g[0..3], lpd, lph
 struct RhsTile { double v[RD_NF][RD_BJ + 2][RD_BK + 2]; };
 
-__device__ __forceinline__ void rhs_put(RhsTile& s, double* X, int f, size_t qg, int ty, int tx, double v){
+template<typename T>
+__device__ __forceinline__ void rhs_put(RhsTile& s, T* X, int f, size_t qg, int ty, int tx, double v){
     if(v == 0.0) return;
     if(ty >= 0) atomicAdd(&s.v[f][ty][tx], v);
-    else unsafeAtomicAdd(X + qg, v);
+    else unsafeAtomicAdd(X + qg, (T)v);
 }
 
-__device__ __forceinline__ void rhs_dry_cell(const RhsAdjArgs& a, const GeoIJ<double>* __restrict__ geo,
+template<typename T>
+__device__ __forceinline__ void rhs_dry_cell(const RhsAdjArgs<T>& a, const GeoIJ<geo_t<T>>* __restrict__ geo,
         RhsTile& tile, int i, int j, int k, size_t q, uint8_t fl){
     const size_t P = a.plane, R = (size_t)a.pitch;
     const int kw = (k == 0) ? a.km - 2 : k - 1;
@@ -468,34 +487,35 @@ __device__ __forceinline__ void rhs_dry_cell(const RhsAdjArgs& a, const GeoIJ<do
     // tile positions (row, column) of the cell and its in-tile neighbours; -1: global
     const int ty = (int)threadIdx.y + 1, tx = (int)threadIdx.x + 1;
     const int tx_e = (ke == k + 1) ? tx + 1 : -1, tx_w = (kw == k - 1) ? tx - 1 : -1;
-    const GeoIJ<double> G = geo[wave_uniform(i * a.jm + j)];      // a wavefront is one (i, j) row
+    const GeoIJ<geo_t<T>> G = geo[wave_uniform(i * a.jm + j)];    // a wavefront is one (i, j) row
 
     double c_[4], d1r[4], d1t[4], d1p[4];
     for(int f = 0; f < 4; f++){
-        const double* X = a.y[f];
+        const T* X = a.y[f];
         c_[f] = X[q];
-        d1r[f] = (X[q + P] - X[q - P]) * a.inv_2dr;
-        d1t[f] = (X[q + R] - X[q - R]) * a.inv_2dthe;
-        d1p[f] = (X[qke] - X[qkw]) * a.inv_2dphi;
+        d1r[f] = ((double)X[q + P] - (double)X[q - P]) * a.inv_2dr;
+        d1t[f] = ((double)X[q + R] - (double)X[q - R]) * a.inv_2dthe;
+        d1p[f] = ((double)X[qke] - (double)X[qkw]) * a.inv_2dphi;
     }
     const double pdc = a.pd[q];
-    const double dpdr   = (((fl & FL_IP) ? pdc : a.pd[q + P]) - ((fl & FL_IM) ? pdc : a.pd[q - P])) * a.inv_2dr;
-    const double dpdthe = (((fl & FL_JP) ? pdc : a.pd[q + R]) - ((fl & FL_JM) ? pdc : a.pd[q - R])) * a.inv_2dthe;
-    const double dpdphi = (((fl & FL_KP) ? pdc : a.pd[qke]) - ((fl & FL_KM) ? pdc : a.pd[qkw])) * a.inv_2dphi;
+    const double dpdr   = (((fl & FL_IP) ? pdc : (double)a.pd[q + P]) - ((fl & FL_IM) ? pdc : (double)a.pd[q - P])) * a.inv_2dr;
+    const double dpdthe = (((fl & FL_JP) ? pdc : (double)a.pd[q + R]) - ((fl & FL_JM) ? pdc : (double)a.pd[q - R])) * a.inv_2dthe;
+    const double dpdphi = (((fl & FL_KP) ? pdc : (double)a.pd[qke]) - ((fl & FL_KM) ? pdc : (double)a.pd[qkw])) * a.inv_2dphi;
     const double tq = c_[0], u = c_[1], v = c_[2], w = c_[3];
     const double costhe = G.costhe, st = G.sinthe_true;
     const double v_invrm = v * G.inv_rm, w_invrs = w * G.inv_rmsinthe;
-    const double S0 = (a.ps[q + P] - a.ps[q - P]) * a.inv_2dr * a.to_nd;
+    const double S0 = ((double)a.ps[q + P] - (double)a.ps[q - P]) * a.inv_2dr * a.to_nd;
     const double rho_l = a.rho[q];
     const bool rvalid = rho_l > 0.0 && isfinite(rho_l);
     const double dpstatdr = rvalid ? S0 * (a.r_mix / rho_l) : S0;
-    const double visc = a.visc0 + a.wn[q];
+    const double visc = a.visc0 + (double)a.wn[q];
     const double two_inv_rm = 2.0 * G.inv_rm;
 
     // adjoints of the four right-hand sides (with aux at the last stage)
     const double Lt = a.lk[0][q];
-    const double lau = a.la[0] ? a.la[0][q] : 0.0, lav = a.la[1] ? a.la[1][q] : 0.0, law = a.la[2] ? a.la[2][q] : 0.0;
-    const double Lu = a.lk[1][q] + lau, Lv = a.lk[2][q] + lav, Lw = a.lk[3][q] + law;
+    const double lau = a.la[0] ? (double)a.la[0][q] : 0.0, lav = a.la[1] ? (double)a.la[1][q] : 0.0;
+    const double law = a.la[2] ? (double)a.la[2][q] : 0.0;
+    const double Lu = (double)a.lk[1][q] + lau, Lv = (double)a.lk[2][q] + lav, Lw = (double)a.lk[3][q] + law;
 
     double L_dpdr = -Lu + lau;
     double L_dpdthe = (-Lv + lav) * G.inv_rm, L_dphdthe = -Lv * G.inv_rm;
@@ -570,12 +590,12 @@ __device__ __forceinline__ void rhs_dry_cell(const RhsAdjArgs& a, const GeoIJ<do
     // buoyancy anomaly
     if(tq > 0.0){
         const double Lb = Lu;
-        Lc[0] += Lb * a.cbuoy * (-a.g_ * a.ps[q] / (a.rR * tq * tq * a.t_ref));
+        Lc[0] += Lb * a.cbuoy * (-a.g_ * (double)a.ps[q] / (a.rR * tq * tq * a.t_ref));
         a.gbrl[q] += -a.cbuoy * Lb;
     }
     // the stencils: the i +- 1 targets straight to global memory, the rest through the tile
     for(int f = 0; f < 4; f++){
-        double* X = a.g[f];
+        T* X = a.g[f];
         rhs_put(tile, X, f, q + P, -1, 0, a.inv_2dr * L1r[f] + a.inv_dr2 * L2r[f]);
         rhs_put(tile, X, f, q - P, -1, 0, -a.inv_2dr * L1r[f] + a.inv_dr2 * L2r[f]);
         rhs_put(tile, X, f, q + R, ty + 1, tx, a.inv_2dthe * L1t[f] + a.inv_dthe2 * L2t[f]);
@@ -585,7 +605,7 @@ __device__ __forceinline__ void rhs_dry_cell(const RhsAdjArgs& a, const GeoIJ<do
         rhs_put(tile, X, f, q, ty, tx, -2.0 * (a.inv_dr2 * L2r[f] + a.inv_dthe2 * L2t[f] + a.inv_dphi2 * L2p[f]) + Lc[f]);
     }
     // the mirrored pressure gradients (a face towards a solid cell reads the cell itself)
-    auto mir = [&](double* L, int f, double coef, uint8_t hi_flag, uint8_t lo_flag, size_t qh, size_t ql,
+    auto mir = [&](T* L, int f, double coef, uint8_t hi_flag, uint8_t lo_flag, size_t qh, size_t ql,
                    int hy, int hx, int ly, int lx){
         if(fl & hi_flag) rhs_put(tile, L, f, q, ty, tx, coef);
         else rhs_put(tile, L, f, qh, hy, hx, coef);
@@ -600,7 +620,8 @@ __device__ __forceinline__ void rhs_dry_cell(const RhsAdjArgs& a, const GeoIJ<do
     mir(a.lph, 5, a.inv_2dphi * L_dphdphi, FL_KP, FL_KM, qke, qkw, ey, tx_e, wy, tx_w);
 }
 
-__global__ __launch_bounds__(RD_BK * RD_BJ) void ka_rhs_dry(const RhsAdjArgs a, const GeoIJ<double>* __restrict__ geo){
+template<typename T>
+__global__ __launch_bounds__(RD_BK * RD_BJ) void ka_rhs_dry(const RhsAdjArgs<T> a, const GeoIJ<geo_t<T>>* __restrict__ geo){
     __shared__ RhsTile tile;
     const int b = xcd_remap(blockIdx.x, a.nb);
     const int kb = b % a.nkb;
@@ -629,31 +650,32 @@ __global__ __launch_bounds__(RD_BK * RD_BJ) void ka_rhs_dry(const RhsAdjArgs a, 
         const int f = c / NC, r = c - f * NC;
         const int jj = jb * RD_BJ + r / (RD_BK + 2) - 1, kk = kb * RD_BK + r % (RD_BK + 2) - 1;
         if(jj < 0 || jj >= a.jm || kk < 0 || kk >= a.km) continue;      // holds no contribution
-        double* X = f < 4 ? a.g[f] : (f == 4 ? a.lpd : a.lph);
-        unsafeAtomicAdd(X + ((size_t)i * a.jm + jj) * a.pitch + kk, v);
+        T* X = f < 4 ? a.g[f] : (f == 4 ? a.lpd : a.lph);
+        unsafeAtomicAdd(X + ((size_t)i * a.jm + jj) * a.pitch + kk, (T)v);
     }
 }
 
 // the RK4 combination transposed: the adjoint of stage s's right-hand side from the step output's
 // adjoint and the adjoint of the next stage's input; the stage inputs' adjoints gather into the
 // n level's
-__global__ __launch_bounds__(256) void ka_rk_combine(double* __restrict__ lk0, double* __restrict__ lk1,
-        double* __restrict__ lk2, double* __restrict__ lk3, double* __restrict__ g0, double* __restrict__ g1,
-        double* __restrict__ g2, double* __restrict__ g3, const double* __restrict__ lnp0, const double* __restrict__ lnp1,
-        const double* __restrict__ lnp2, const double* __restrict__ lnp3, double* __restrict__ ln0, double* __restrict__ ln1,
-        double* __restrict__ ln2, double* __restrict__ ln3, const uint8_t* __restrict__ flags,
+template<typename T>
+__global__ __launch_bounds__(256) void ka_rk_combine(T* __restrict__ lk0, T* __restrict__ lk1,
+        T* __restrict__ lk2, T* __restrict__ lk3, T* __restrict__ g0, T* __restrict__ g1,
+        T* __restrict__ g2, T* __restrict__ g3, const T* __restrict__ lnp0, const T* __restrict__ lnp1,
+        const T* __restrict__ lnp2, const T* __restrict__ lnp3, T* __restrict__ ln0, T* __restrict__ ln1,
+        T* __restrict__ ln2, T* __restrict__ ln3, const uint8_t* __restrict__ flags,
         int im, int jm, int km, int pitch, double wnp, double cg, int have_g, int write_lk){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
     if(k >= km) return;
     const size_t q = ((size_t)i * jm + j) * pitch + k;
     const bool interior = i >= 1 && i <= im - 2 && j >= 3 && j <= jm - 4 && k <= km - 2 && !(flags[q] & FL_SELF);
-    double* lk[4] = { lk0, lk1, lk2, lk3 };
-    double* g[4] = { g0, g1, g2, g3 };
-    const double* lnp[4] = { lnp0, lnp1, lnp2, lnp3 };
-    double* ln[4] = { ln0, ln1, ln2, ln3 };
+    T* lk[4] = { lk0, lk1, lk2, lk3 };
+    T* g[4] = { g0, g1, g2, g3 };
+    const T* lnp[4] = { lnp0, lnp1, lnp2, lnp3 };
+    T* ln[4] = { ln0, ln1, ln2, ln3 };
     for(int f = 0; f < 4; f++){
-        const double gv = have_g ? g[f][q] : 0.0;
-        if(write_lk) lk[f][q] = interior ? (wnp * lnp[f][q] + cg * gv) : 0.0;
+        const double gv = have_g ? (double)g[f][q] : 0.0;
+        if(write_lk) lk[f][q] = interior ? (wnp * (double)lnp[f][q] + cg * gv) : 0.0;
         if(have_g){ ln[f][q] += gv; g[f][q] = 0.0; }
     }
 }
@@ -771,15 +793,16 @@ __global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_inmask(const uint8_t* 
 }
 
 // mu of colour 1 for the last sweep: omega lambda / den on the sweep cells, 0 elsewhere
-__global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_mu(const double* __restrict__ lam, double* __restrict__ mu,
+template<typename TS>
+__global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_mu(const TS* __restrict__ lam, TS* __restrict__ mu,
         const uint8_t* __restrict__ fl_arr, const PoisIJ<double>* __restrict__ pw, int im, int jm, int hp, int nm,
-        double omega){
+        TS omega){
     const int m = blockIdx.x * RBT_BM + threadIdx.x, j = blockIdx.y * RBT_BJ + threadIdx.y, i = blockIdx.z;
     if(m >= nm || j >= jm) return;
     const size_t q = ((size_t)i * jm + j) * hp + m;
-    double v = 0.0;
+    TS v = 0.0;
     if(i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2){
-        const double den = rb_den(fl_arr[q], pw[(size_t)i * jm + j]);
+        const TS den = rb_den(fl_arr[q], pw[(size_t)i * jm + j]);
         if(den > 0.0) v = omega * lam[q] / den;
     }
     mu[q] = v;
@@ -789,19 +812,21 @@ __global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_mu(const double* __res
 // at the six neighbour positions and its flags at the four radial and meridional ones, and the
 // coupling weights of the (i, j) row and of its four neighbour rows. A caller fills the members
 // the cell's incoming-read bits, position and pass use; the rest stay zero.
+template<typename TS>
 struct RbtNb {
-    double mip, mim, mjp, mjm, mke, mkw;          // mu_o at i+1, i-1, j+1, j-1, k+1, k-1
-    double wip, wim, wjp, wjm;                    // num1 of rows i+1, i-1; num2 of rows j+1, j-1
+    TS mip, mim, mjp, mjm, mke, mkw;          // mu_o at i+1, i-1, j+1, j-1, k+1, k-1
+    TS wip, wim, wjp, wjm;                    // num1 of rows i+1, i-1; num2 of rows j+1, j-1
     uint8_t fim, fip, fjm, fjp;                   // the other colour's flags at i-1, i+1, j-1, j+1
 };
 
 // one cell of one transposed colour pass (W: the own row's weights, used on sweep cells only)
-__device__ __forceinline__ void rbT_cell(double* __restrict__ lam, double* __restrict__ mu,
-        double* __restrict__ lsrc, size_t q, uint8_t bits, uint8_t fl, const PoisIJ<double>& W, const RbtNb& n,
-        int i, int j, int im, int jm, int colour, int ghost, double omega, double c43, double c13){
+template<typename TS>
+__device__ __forceinline__ void rbT_cell(TS* __restrict__ lam, TS* __restrict__ mu,
+        TS* __restrict__ lsrc, size_t q, uint8_t bits, uint8_t fl, const PoisIJ<double>& W, const RbtNb<TS>& n,
+        int i, int j, int im, int jm, int colour, int ghost, TS omega, TS c43, TS c13){
     const bool interior = i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2;
     // stored-value reads of this cell by the other colour's sweep cells s: sum w mu_s
-    double dsum = 0.0;
+    TS dsum = 0.0;
     if(bits & 1) dsum += n.wip * n.mip;
     if(bits & 2) dsum += n.wim * n.mim;
     if(bits & 4) dsum += n.wjp * n.mjp;
@@ -813,7 +838,7 @@ __device__ __forceinline__ void rbT_cell(double* __restrict__ lam, double* __res
     if(bits & 16) dsum += W.num3 * n.mke;
     if(bits & 32) dsum += W.num3 * n.mkw;
     // formed-plane reads by the other colour, for which this cell is the one behind the plane
-    double bsum = 0.0;
+    TS bsum = 0.0;
     if(ghost){
         if(i == 2 && rb_reads(n.fim, 1))      bsum += -c13 * n.wim * n.mim;
         if(i == im - 3 && rb_reads(n.fip, 0)) bsum += -c13 * n.wip * n.mip;
@@ -821,14 +846,14 @@ __device__ __forceinline__ void rbT_cell(double* __restrict__ lam, double* __res
         if(j == jm - 3 && rb_reads(n.fjp, 2)) bsum += -c13 * n.wjp * n.mjp;
     }
     // this cell's own pass
-    const double den = rb_den(fl, W);
+    const TS den = rb_den(fl, W);
     const bool skip = !(den > 0.0);
-    double t = lam[q];
+    TS t = lam[q];
     if(colour == 0) t += dsum;                     // colour 1 read colour 0 after its pass
-    double mu_x = 0.0, out = t;
+    TS mu_x = 0.0, out = t;
     if(!skip){
         mu_x = (colour == 0) ? omega * t / den : mu[q];
-        double self = 0.0;
+        TS self = 0.0;
         if(ghost){
             if(i == 1 && rb_reads(fl, 1))      self += c43 * W.num1 * mu_x;
             if(i == im - 2 && rb_reads(fl, 0)) self += c43 * W.num1 * mu_x;
@@ -858,10 +883,11 @@ __device__ __forceinline__ RbtNeed rbt_need(uint8_t bits, bool interior, int gho
 
 // one colour pass of one sweep, transposed, over every cell of colour c (plane cells take the
 // reads of stored planes, which only the first sweep of a solve makes: ghost = 0)
-__global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restrict__ lam, double* __restrict__ mu,
-        const double* __restrict__ mu_o, double* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
+template<typename TS>
+__global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(TS* __restrict__ lam, TS* __restrict__ mu,
+        const TS* __restrict__ mu_o, TS* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
         const uint8_t* __restrict__ fo_arr, const uint8_t* __restrict__ inm, const PoisIJ<double>* __restrict__ pw,
-        int im, int jm, int hp, int nm, int colour, int ghost, double omega, double c43, double c13){
+        int im, int jm, int hp, int nm, int colour, int ghost, TS omega, TS c43, TS c13){
     const int m = blockIdx.x * RBT_BM + threadIdx.x, j = blockIdx.y * RBT_BJ + threadIdx.y, i = blockIdx.z;
     if(m >= nm || j >= jm) return;
     const bool interior = i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2;
@@ -872,7 +898,7 @@ __global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restric
     const uint8_t bits = inm[q];
     const int ij = wave_uniform(i * jm + j);          // a wavefront is one (i, j) row
     const RbtNeed nd = rbt_need(bits, interior, ghost, i, j, im, jm);
-    RbtNb n = {};
+    RbtNb<TS> n = {};
     if(nd.ip){ n.mip = mu_o[q + P]; n.wip = pw[ij + jm].num1; }
     if(nd.im_){ n.mim = mu_o[q - P]; n.wim = pw[ij - jm].num1; }
     if(nd.jp){ n.mjp = mu_o[q + R]; n.wjp = pw[ij + 1].num2; }
@@ -897,10 +923,12 @@ __global__ __launch_bounds__(RBT_BM * RBT_BJ) void ka_rbT_pass(double* __restric
 // the same pass with two cells m, m+1 (m even) per thread and pair loads, flattened over
 // (row, pair) in 256-thread blocks so that no lane idles at the end of a row; needs an even ring of
 // compact cells
-__global__ __launch_bounds__(256) void ka_rbT_pass2(double* __restrict__ lam, double* __restrict__ mu,
-        const double* __restrict__ mu_o, double* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
+template<typename TS>
+__global__ __launch_bounds__(256) void ka_rbT_pass2(TS* __restrict__ lam, TS* __restrict__ mu,
+        const TS* __restrict__ mu_o, TS* __restrict__ lsrc, const uint8_t* __restrict__ fl_arr,
         const uint8_t* __restrict__ fo_arr, const uint8_t* __restrict__ inm, const PoisIJ<double>* __restrict__ pw,
-        int im, int jm, int hp, int nm, int ntb, int colour, int ghost, double omega, double c43, double c13){
+        int im, int jm, int hp, int nm, int ntb, int colour, int ghost, TS omega, TS c43, TS c13){
+    using V2 = typename Pair<TS>::V;
     // levels fastest, each XCD a contiguous range: a group's neighbours at i +- 1 are read by the
     // workgroups dispatched beside it
     const int b = xcd_remap(blockIdx.x, ntb * im);
@@ -924,74 +952,74 @@ __global__ __launch_bounds__(256) void ka_rbT_pass2(double* __restrict__ lam, do
         const uint16_t fl2 = *reinterpret_cast<const uint16_t*>(fl_arr + q);
         if(fl2 == 0){
             const PoisIJ<double> W = pw[ij];
-            const double wip = pw[ij + jm].num1, wim = pw[ij - jm].num1;
-            const double wjp = pw[ij + 1].num2, wjm = pw[ij - 1].num2;
-            const double2 vip = *reinterpret_cast<const double2*>(mu_o + q + P);
-            const double2 vim = *reinterpret_cast<const double2*>(mu_o + q - P);
-            const double2 vjp = *reinterpret_cast<const double2*>(mu_o + q + R);
-            const double2 vjm = *reinterpret_cast<const double2*>(mu_o + q - R);
-            const double2 vc = *reinterpret_cast<const double2*>(mu_o + q);
+            const TS wip = pw[ij + jm].num1, wim = pw[ij - jm].num1;
+            const TS wjp = pw[ij + 1].num2, wjm = pw[ij - 1].num2;
+            const V2 vip = *reinterpret_cast<const V2*>(mu_o + q + P);
+            const V2 vim = *reinterpret_cast<const V2*>(mu_o + q - P);
+            const V2 vjp = *reinterpret_cast<const V2*>(mu_o + q + R);
+            const V2 vjm = *reinterpret_cast<const V2*>(mu_o + q - R);
+            const V2 vc = *reinterpret_cast<const V2*>(mu_o + q);
             const int off = (colour + i + j) & 1;
-            const double edge = off ? mu_o[row + (m + 2 == nm ? 0 : m + 2)] : mu_o[row + (m == 0 ? nm - 1 : m - 1)];
-            double mke[2], mkw[2];
+            const TS edge = off ? mu_o[row + (m + 2 == nm ? 0 : m + 2)] : mu_o[row + (m == 0 ? nm - 1 : m - 1)];
+            TS mke[2], mkw[2];
             if(off){ mke[0] = vc.y; mkw[0] = vc.x; mke[1] = edge; mkw[1] = vc.y; }
             else   { mke[0] = vc.x; mkw[0] = edge; mke[1] = vc.y; mkw[1] = vc.x; }
-            const double mip[2] = { vip.x, vip.y }, mim[2] = { vim.x, vim.y };
-            const double mjp[2] = { vjp.x, vjp.y }, mjm[2] = { vjm.x, vjm.y };
-            const double den = W.den_fluid;
-            const double2 lv = *reinterpret_cast<const double2*>(lam + q);
-            const double2 sv = *reinterpret_cast<const double2*>(lsrc + q);
-            double2 mv = colour ? *reinterpret_cast<const double2*>(mu + q) : make_double2(0.0, 0.0);
-            const double lin[2] = { lv.x, lv.y }, sin_[2] = { sv.x, sv.y };
-            const double min_[2] = { mv.x, mv.y };
-            double lout[2], sout[2], mout[2];
+            const TS mip[2] = { vip.x, vip.y }, mim[2] = { vim.x, vim.y };
+            const TS mjp[2] = { vjp.x, vjp.y }, mjm[2] = { vjm.x, vjm.y };
+            const TS den = W.den_fluid;
+            const V2 lv = *reinterpret_cast<const V2*>(lam + q);
+            const V2 sv = *reinterpret_cast<const V2*>(lsrc + q);
+            V2 mv = colour ? *reinterpret_cast<const V2*>(mu + q) : V2{TS(0), TS(0)};
+            const TS lin[2] = { lv.x, lv.y }, sin_[2] = { sv.x, sv.y };
+            const TS min_[2] = { mv.x, mv.y };
+            TS lout[2], sout[2], mout[2];
             #pragma unroll
             for(int e = 0; e < 2; e++){
-                double dsum = 0.0;
+                TS dsum = 0.0;
                 dsum += wip * mip[e];
                 dsum += wim * mim[e];
                 dsum += wjp * mjp[e];
                 dsum += wjm * mjm[e];
                 dsum += W.num3 * mke[e];
                 dsum += W.num3 * mkw[e];
-                double t = lin[e];
+                TS t = lin[e];
                 if(colour == 0) t += dsum;
-                const double mu_x = (colour == 0) ? omega * t / den : min_[e];
-                double out = (1.0 - omega) * t + 0.0;
+                const TS mu_x = (colour == 0) ? omega * t / den : min_[e];
+                TS out = (1.0 - omega) * t + 0.0;
                 sout[e] = sin_[e] + -mu_x;
                 if(colour == 1) out += dsum;
                 lout[e] = out;
                 mout[e] = (colour == 0) ? mu_x : omega * out / den;
             }
-            *reinterpret_cast<double2*>(lam + q) = make_double2(lout[0], lout[1]);
-            *reinterpret_cast<double2*>(lsrc + q) = make_double2(sout[0], sout[1]);
-            *reinterpret_cast<double2*>(mu + q) = make_double2(mout[0], mout[1]);
+            *reinterpret_cast<V2*>(lam + q) = V2{lout[0], lout[1]};
+            *reinterpret_cast<V2*>(lsrc + q) = V2{sout[0], sout[1]};
+            *reinterpret_cast<V2*>(mu + q) = V2{mout[0], mout[1]};
             return;
         }
     }
     const RbtNeed nd = rbt_need(b0 | b1, interior, ghost, i, j, im, jm);
-    RbtNb n0 = {}, n1 = {};
+    RbtNb<TS> n0 = {}, n1 = {};
     if(nd.ip){
-        const double2 v = *reinterpret_cast<const double2*>(mu_o + q + P);
+        const V2 v = *reinterpret_cast<const V2*>(mu_o + q + P);
         n0.mip = v.x; n1.mip = v.y; n0.wip = n1.wip = pw[ij + jm].num1;
     }
     if(nd.im_){
-        const double2 v = *reinterpret_cast<const double2*>(mu_o + q - P);
+        const V2 v = *reinterpret_cast<const V2*>(mu_o + q - P);
         n0.mim = v.x; n1.mim = v.y; n0.wim = n1.wim = pw[ij - jm].num1;
     }
     if(nd.jp){
-        const double2 v = *reinterpret_cast<const double2*>(mu_o + q + R);
+        const V2 v = *reinterpret_cast<const V2*>(mu_o + q + R);
         n0.mjp = v.x; n1.mjp = v.y; n0.wjp = n1.wjp = pw[ij + 1].num2;
     }
     if(nd.jm_){
-        const double2 v = *reinterpret_cast<const double2*>(mu_o + q - R);
+        const V2 v = *reinterpret_cast<const V2*>(mu_o + q - R);
         n0.mjm = v.x; n1.mjm = v.y; n0.wjm = n1.wjm = pw[ij - 1].num2;
     }
     // phi neighbours: the reader at k+1 and at k-1 of each cell (other colour)
     if((b0 | b1) & 48){
         const int off = (colour + i + j) & 1;
-        const double2 vc = *reinterpret_cast<const double2*>(mu_o + q);
-        const double edge = off ? mu_o[row + (m + 2 == nm ? 0 : m + 2)] : mu_o[row + (m == 0 ? nm - 1 : m - 1)];
+        const V2 vc = *reinterpret_cast<const V2*>(mu_o + q);
+        const TS edge = off ? mu_o[row + (m + 2 == nm ? 0 : m + 2)] : mu_o[row + (m == 0 ? nm - 1 : m - 1)];
         if(off){ n0.mke = vc.y; n0.mkw = vc.x; n1.mke = edge; n1.mkw = vc.y; }
         else   { n0.mke = vc.x; n0.mkw = edge; n1.mke = vc.y; n1.mkw = vc.x; }
     }
@@ -1012,12 +1040,70 @@ __global__ __launch_bounds__(256) void ka_rbT_pass2(double* __restrict__ lam, do
     rbT_cell(lam, mu, lsrc, q + 1, b1, (uint8_t)(fl2 >> 8), W, n1, i, j, im, jm, colour, ghost, omega, c43, c13);
 }
 
+// ---- the FP32-refined form's rounds, transposed ----------------------------------------------
+// A round of the refined solve forms the outer planes of the FP64 iterate x from its interior,
+// the FP64 Gauss-Seidel residual r = src + den x - sum(w x_nb) of every sweep cell, relaxes the
+// correction d from zero with n FP32 sweeps on the source r, and adds d to x on the sweep cells.
+// Transposed, in reverse: the correction's adjoint is the iterate's on the sweep cells (x += d),
+// the FP32 sweeps run transposed on it and leave the adjoint of r in their source adjoint, r's
+// transpose adds into the FP64 adjoints of x and of the source, and the planes' transposes move
+// the plane cells' adjoint into rows 1 and 2.
+
+// x += d, transposed: lambda_d = lambda_x on the sweep cells (0 elsewhere); the round's lambda_r
+// starts at zero
+__global__ __launch_bounds__(128) void ka_rb_addT(const double* __restrict__ lx0, const double* __restrict__ lx1,
+        float* __restrict__ ld0, float* __restrict__ ld1, float* __restrict__ lr0, float* __restrict__ lr1,
+        int im, int jm, int hp, int nm){
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const int i = blockIdx.z >> 1, colour = blockIdx.z & 1;
+    if(m >= nm) return;
+    const size_t q = ((size_t)i * jm + j) * hp + m;
+    const bool sweep = i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2;
+    (colour ? ld1 : ld0)[q] = sweep ? (float)(colour ? lx1 : lx0)[q] : 0.0f;
+    (colour ? lr1 : lr0)[q] = 0.0f;
+}
+
+// r = src + den x - sum(w x_nb) over the sweep cells, transposed and gathered per cell y of either
+// colour (plane cells included): lambda_x(y) += den_y lambda_r(y) - sum over the other colour's
+// sweep cells z that read y of w_z lambda_r(z) (w_z the reader row's weight of that direction, as
+// the residual applies it; the incoming-read bits of y list the readers); lambda_src(y) += lambda_r(y)
+__global__ __launch_bounds__(128) void ka_rb_resT(double* __restrict__ lx0, double* __restrict__ lx1,
+        double* __restrict__ ls0, double* __restrict__ ls1, const float* __restrict__ lr0, const float* __restrict__ lr1,
+        const uint8_t* __restrict__ f0, const uint8_t* __restrict__ f1, const uint8_t* __restrict__ in0,
+        const uint8_t* __restrict__ in1, const PoisIJ<double>* __restrict__ pw, int im, int jm, int hp, int nm){
+    const int m = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const int i = blockIdx.z >> 1, colour = blockIdx.z & 1;
+    if(m >= nm) return;
+    const size_t P = (size_t)jm * hp, R = (size_t)hp;
+    const size_t row = ((size_t)i * jm + j) * hp, q = row + m;
+    const float* lro = colour ? lr0 : lr1;              // the readers are of the other colour
+    const uint8_t bits = (colour ? in1 : in0)[q];
+    double g = 0.0;
+    if(bits & 1) g -= pw[(size_t)(i + 1) * jm + j].num1 * (double)lro[q + P];
+    if(bits & 2) g -= pw[(size_t)(i - 1) * jm + j].num1 * (double)lro[q - P];
+    if(bits & 4) g -= pw[(size_t)i * jm + j + 1].num2 * (double)lro[q + R];
+    if(bits & 8) g -= pw[(size_t)i * jm + j - 1].num2 * (double)lro[q - R];
+    if(bits & 48){
+        const int off = (colour + i + j) & 1, ring = 2 * nm, kk = 2 * m + off;
+        const double w3 = pw[(size_t)i * jm + j].num3;
+        if(bits & 16) g -= w3 * (double)lro[row + (((kk + 1) % ring) >> 1)];
+        if(bits & 32) g -= w3 * (double)lro[row + (((kk - 1 + ring) % ring) >> 1)];
+    }
+    if(i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2){
+        const double lr = (double)(colour ? lr1 : lr0)[q];
+        g += rb_den((colour ? f1 : f0)[q], pw[(size_t)i * jm + j]) * lr;
+        (colour ? ls1 : ls0)[q] += lr;
+    }
+    (colour ? lx1 : lx0)[q] += g;
+}
+
 // prepareProjectionBoundaries + the clamped divergence source, transposed: the source adjoint
 // (frozen to zero where the forward clamped) spreads onto aux_u, aux_v, aux_w
+template<typename T>
 struct SrcAdjArgs {
-    const double *au, *av, *aw;             // forward aux after prepareProjectionBoundaries
-    const double *ls0, *ls1;                // source adjoints, split
-    double *lau, *lav, *law;
+    const T *au, *av, *aw;                  // forward aux after prepareProjectionBoundaries
+    const double *ls0, *ls1;                // source adjoints, split (the source is FP64 in both forms)
+    T *lau, *lav, *law;
     const uint8_t *flags, *msk;
     const PoisIJ<double>* pw;
     int im, jm, km, pitch, hp;
@@ -1025,7 +1111,8 @@ struct SrcAdjArgs {
     double inv_2dr, inv_2dthe, inv_2dphi, inv_dthe, inv_dphi, cap;
 };
 
-__global__ __launch_bounds__(256) void ka_poisson_source(const SrcAdjArgs a){
+template<typename T>
+__global__ __launch_bounds__(256) void ka_poisson_source(const SrcAdjArgs<T> a){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1, i = blockIdx.z + 1;
     if(k > a.km - 2 || j > a.jm - 2 || i > a.im - 2) return;
     const int im = a.im, jm = a.jm, km = a.km;
@@ -1043,26 +1130,26 @@ __global__ __launch_bounds__(256) void ka_poisson_source(const SrcAdjArgs a){
     double du_dr = 0.0, dv_dthe = 0.0, dw_dphi = 0.0;
     int r_mode = 0, t_mode = 0, p_mode = 0;     // which stencil each derivative took
     if(i < im - 2 && sld && !(fl & FL_IP)){
-        du_dr = (-3.0 * a.au[q] + 4.0 * a.au[q + P] - a.au[q + 2 * P]) * a.inv_2dr; r_mode = 1;
+        du_dr = (-3.0 * (double)a.au[q] + 4.0 * (double)a.au[q + P] - (double)a.au[q + 2 * P]) * a.inv_2dr; r_mode = 1;
     }
     if(j > 2 && j < jm - 2){
         const bool air_jp1 = !(fl & FL_JP), air_jm1 = !(fl & FL_JM);
         const bool sjp2 = a.msk[q + 2 * R] != 0, sjm2 = a.msk[q - 2 * R] != 0;
-        if(sld && air_jp1 && !sjp2){ dv_dthe = (-3.0 * a.av[q] + 4.0 * a.av[q + R] - a.av[q + 2 * R]) * a.inv_2dthe; t_mode = 1; }
-        if(sld && air_jm1 && !sjm2){ dv_dthe = -(-3.0 * a.av[q] + 4.0 * a.av[q - R] - a.av[q - 2 * R]) * a.inv_2dthe; t_mode = 2; }
-        if((sld && air_jp1 && sjp2) || (j == jm - 2 && !sld && (fl & FL_JP))){ dv_dthe = (a.av[q + R] - a.av[q]) * a.inv_dthe; t_mode = 3; }
-        if((sld && air_jm1 && sjm2) || (j == 1 && sld && air_jm1)){ dv_dthe = (a.av[q - R] - a.av[q]) * a.inv_dthe; t_mode = 4; }
+        if(sld && air_jp1 && !sjp2){ dv_dthe = (-3.0 * (double)a.av[q] + 4.0 * (double)a.av[q + R] - (double)a.av[q + 2 * R]) * a.inv_2dthe; t_mode = 1; }
+        if(sld && air_jm1 && !sjm2){ dv_dthe = -(-3.0 * (double)a.av[q] + 4.0 * (double)a.av[q - R] - (double)a.av[q - 2 * R]) * a.inv_2dthe; t_mode = 2; }
+        if((sld && air_jp1 && sjp2) || (j == jm - 2 && !sld && (fl & FL_JP))){ dv_dthe = ((double)a.av[q + R] - (double)a.av[q]) * a.inv_dthe; t_mode = 3; }
+        if((sld && air_jm1 && sjm2) || (j == 1 && sld && air_jm1)){ dv_dthe = ((double)a.av[q - R] - (double)a.av[q]) * a.inv_dthe; t_mode = 4; }
     }
     if(k > 2 && k < km - 2){
         const bool air_kp1 = !skp, air_km1 = !skm;
-        if(sld && air_kp1 && !skp2){ dw_dphi = (-3.0 * a.aw[q] + 4.0 * a.aw[q + 1] - a.aw[q + 2]) * a.inv_2dphi; p_mode = 1; }
-        if(sld && air_km1 && !skm2){ dw_dphi = -(-3.0 * a.aw[q] + 4.0 * a.aw[q - 1] - a.aw[q - 2]) * a.inv_2dphi; p_mode = 2; }
-        if((sld && air_kp1 && skp2) || (k == km - 2 && !sld && skp)){ dw_dphi = (a.aw[q + 1] - a.aw[q]) * a.inv_dphi; p_mode = 3; }
-        if((sld && air_km1 && skm2) || (k == 1 && sld && air_km1)){ dw_dphi = (a.aw[q - 1] - a.aw[q]) * a.inv_dphi; p_mode = 4; }
+        if(sld && air_kp1 && !skp2){ dw_dphi = (-3.0 * (double)a.aw[q] + 4.0 * (double)a.aw[q + 1] - (double)a.aw[q + 2]) * a.inv_2dphi; p_mode = 1; }
+        if(sld && air_km1 && !skm2){ dw_dphi = -(-3.0 * (double)a.aw[q] + 4.0 * (double)a.aw[q - 1] - (double)a.aw[q - 2]) * a.inv_2dphi; p_mode = 2; }
+        if((sld && air_kp1 && skp2) || (k == km - 2 && !sld && skp)){ dw_dphi = ((double)a.aw[q + 1] - (double)a.aw[q]) * a.inv_dphi; p_mode = 3; }
+        if((sld && air_km1 && skm2) || (k == 1 && sld && air_km1)){ dw_dphi = ((double)a.aw[q - 1] - (double)a.aw[q]) * a.inv_dphi; p_mode = 4; }
     }
-    if(!r_mode) du_dr = (a.au[q + P] - a.au[q - P]) * a.inv_2dr;
-    if(!t_mode) dv_dthe = (a.av[q + R] - a.av[q - R]) * a.inv_2dthe;
-    if(!p_mode) dw_dphi = (a.aw[q - k + ke] - a.aw[q - k + kw]) * a.inv_2dphi;
+    if(!r_mode) du_dr = ((double)a.au[q + P] - (double)a.au[q - P]) * a.inv_2dr;
+    if(!t_mode) dv_dthe = ((double)a.av[q + R] - (double)a.av[q - R]) * a.inv_2dthe;
+    if(!p_mode) dw_dphi = ((double)a.aw[q - k + ke] - (double)a.aw[q - k + kw]) * a.inv_2dphi;
     const PoisIJ<double> W = a.pw[(size_t)i * jm + j];
     const double div_src = du_dr * 1.0 + dv_dthe * W.inv_rm + dw_dphi * W.inv_rmsinthe;
     double den = W.denom;
@@ -1104,7 +1191,8 @@ __global__ __launch_bounds__(256) void ka_poisson_source(const SrcAdjArgs a){
 }
 
 // prepareProjectionBoundaries^T: aux_u zeroed at the radial walls, aux_v, aux_w 2-point Neumann
-__global__ void ka_prep_projection(double* __restrict__ lau, double* __restrict__ lav, double* __restrict__ law,
+template<typename T>
+__global__ void ka_prep_projection(T* __restrict__ lau, T* __restrict__ lav, T* __restrict__ law,
         int im, int jm, int km, int pitch, size_t plane, double c43, double c13){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y + 1;
@@ -1114,65 +1202,37 @@ __global__ void ka_prep_projection(double* __restrict__ lau, double* __restrict_
     const size_t b0 = q + (size_t)(im - 1) * plane, b1 = q + (size_t)(im - 2) * plane, b2 = q + (size_t)(im - 3) * plane;
     lau[t0] = 0.0;
     lau[b0] = 0.0;
-    double* X[2] = { lav, law };
+    T* X[2] = { lav, law };
     for(int f = 0; f < 2; f++){
-        double* A = X[f];
+        T* A = X[f];
         A[t1] += c43 * A[t0]; A[t2] += -c13 * A[t0]; A[t0] = 0.0;
         A[b1] += c43 * A[b0]; A[b2] += -c13 * A[b0]; A[b0] = 0.0;
     }
 }
 
-// the final planes and seam of the solve, on the pitch layout, transposed (seam, j, i)
-__global__ void ka_seam_field(double* __restrict__ L, int im, int jm, int km, int pitch){
-    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-    if(j >= jm) return;
-    const size_t r = ((size_t)i * jm + j) * pitch;
-    L[r] += L[r + km - 1];
-    L[r + km - 1] = 0.0;
-}
-__global__ void ka_planes_j_field(double* __restrict__ L, int im, int jm, int km, int pitch, size_t plane,
-        double c43, double c13){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-    if(k >= km) return;
-    const size_t q = (size_t)i * plane + k, R = (size_t)pitch;
-    double l = L[q]; L[q] = 0.0;
-    L[q + R] += c43 * l; L[q + 2 * R] += -c13 * l;
-    const size_t qn = q + (size_t)(jm - 1) * R;
-    l = L[qn]; L[qn] = 0.0;
-    L[qn - R] += c43 * l; L[qn - 2 * R] += -c13 * l;
-}
-__global__ void ka_planes_i_field(double* __restrict__ L, int im, int jm, int km, int pitch, size_t plane,
-        double c43, double c13){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if(k >= km) return;
-    const size_t q = (size_t)j * pitch + k;
-    double l = L[q]; L[q] = 0.0;
-    L[q + plane] += c43 * l; L[q + 2 * plane] += -c13 * l;
-    const size_t qt = q + (size_t)(im - 1) * plane;
-    l = L[qt]; L[qt] = 0.0;
-    L[qt - plane] += c43 * l; L[qt - 2 * plane] += -c13 * l;
-}
-
 // split <-> pitch layout for the adjoint of p (add into the target)
-__global__ void ka_rb_pack_add(const double* __restrict__ L, double* __restrict__ x0, double* __restrict__ x1,
+template<typename T>
+__global__ void ka_rb_pack_add(const T* __restrict__ L, double* __restrict__ x0, double* __restrict__ x1,
         int jm, int km, int pitch, int hp){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
     if(k >= km) return;
     double* x = ((i + j + k) & 1) ? x1 : x0;
     x[((size_t)i * jm + j) * hp + (k >> 1)] = L[((size_t)i * jm + j) * pitch + k];
 }
-__global__ void ka_rb_unpack_add(double* __restrict__ L, const double* __restrict__ x0, const double* __restrict__ x1,
+template<typename T>
+__global__ void ka_rb_unpack_add(T* __restrict__ L, const double* __restrict__ x0, const double* __restrict__ x1,
         int jm, int km, int pitch, int hp){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
     if(k >= km) return;
     const double* x = ((i + j + k) & 1) ? x1 : x0;
     // the compact index of the seam (k = km - 1) is shared with nothing; every (i, j, k) maps once
-    L[((size_t)i * jm + j) * pitch + k] = x[((size_t)i * jm + j) * hp + (k >> 1)];
+    L[((size_t)i * jm + j) * pitch + k] = (T)x[((size_t)i * jm + j) * hp + (k >> 1)];
 }
 
 // the cells the RK4 step integrates (and overwrites aux_* in) lose their adjoint
-__global__ __launch_bounds__(256) void ka_zero_interior(double* __restrict__ a0, double* __restrict__ a1,
-        double* __restrict__ a2, const uint8_t* __restrict__ flags, int im, int jm, int km, int pitch){
+template<typename T>
+__global__ __launch_bounds__(256) void ka_zero_interior(T* __restrict__ a0, T* __restrict__ a1,
+        T* __restrict__ a2, const uint8_t* __restrict__ flags, int im, int jm, int km, int pitch){
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
     if(k >= km) return;
     const size_t q = ((size_t)i * jm + j) * pitch + k;
@@ -1181,11 +1241,13 @@ __global__ __launch_bounds__(256) void ka_zero_interior(double* __restrict__ a0,
     a0[q] = 0.0; a1[q] = 0.0; a2[q] = 0.0;
 }
 
-__global__ void ka_zero(double* __restrict__ x, size_t n){
+template<typename T>
+__global__ void ka_zero(T* __restrict__ x, size_t n){
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(q < n) x[q] = 0.0;
+    if(q < n) x[q] = T(0);
 }
-__global__ void ka_copy(double* __restrict__ d, const double* __restrict__ s, size_t n){
+template<typename T>
+__global__ void ka_copy(T* __restrict__ d, const T* __restrict__ s, size_t n){
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(q < n) d[q] = s[q];
 }
@@ -1199,24 +1261,30 @@ namespace atjup {
 
 enum { AW_G = 0, AW_LK = 4, AW_LNP = 8, AW_Y = 12, AW_NN = 24, AW_AUX = 28, AW_AUXP = 31, AW_GBRL = 34,
        AW_SCR = 35, AW_PIN = 36, AW_N = 37 };
-// the forward state a step starts from, as checkpoints and snapshots hold it
+// the forward state a step starts from, as checkpoints and snapshots hold it; the FP32-refined
+// form adds the low words of its paired fields t, u, v, w
 static const int kStateFields[] = { ATJUP_F_T, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W, ATJUP_F_P_DYN,
                                     ATJUP_F_AUX_U, ATJUP_F_AUX_V, ATJUP_F_AUX_W, ATJUP_F_RHO_MIX };
-constexpr int N_STATE = 9;
+constexpr int N_STATE_FIELDS = 9, N_PAIRED = 4;
+static int n_state(const atjup_ctx* c){
+    return N_STATE_FIELDS + (c->precision == ATJUP_PRECISION_FP32_REF ? N_PAIRED : 0);
+}
 
 struct AdjState {
     Buf lam[ATJUP_F_END];
     Buf work[AW_N];
     Buf lsplit, lsrc, mu, inmask, red, zs;
-    std::vector<Buf> ckpt;        // N_STATE fields per checkpoint
-    std::vector<Buf> snap;        // N_STATE fields per step of the segment being reversed
+    Buf ld32, lr32, mu32;         // the refined form's rounds: the correction's adjoint, the round's
+                                  // source adjoint and the transposed FP32 sweeps' scaled adjoints
+    std::vector<Buf> ckpt;        // n_state fields per checkpoint
+    std::vector<Buf> snap;        // n_state fields per step of the segment being reversed
     std::vector<Buf> post;        // N_POST fields per physics-block step of the whole run: the state
                                   // after its block (the pressure solve's result), kept from the forward
     std::vector<int> post_slot;   // step -> its slot in post, -1 for steps without the block
-    // a captured transposed sweep sequence, replayed only on the buffers and geometry it was
-    // captured with (its kernels' arguments are baked in)
-    struct SweepGraph { int n; double omega; hipGraphExec_t exec; const void* ptrs[8]; int dims[4]; };
-    std::vector<SweepGraph> graphs;   // the transposed sweep sequences, replayed
+    // a captured transposed solve, replayed only on the buffers and geometry it was captured with
+    // (its kernels' arguments are baked in); kind 0: FP64 sweeps, 1: the refined form's rounds
+    struct SweepGraph { int kind; int n; double omega; int nr; hipGraphExec_t exec; const void* ptrs[12]; int dims[4]; };
+    std::vector<SweepGraph> graphs;
     Buf brl_part;                 // per-level partial sums of the level-mean transpose
     Buf gchi;                     // dL/dchi of the drag region, [n3]
     Buf obst_part;                // per-block partial sums of dL/d(semi-axes)
@@ -1245,7 +1313,8 @@ void adj_free(atjup_ctx* c){
     AdjState* A = c->adj;
     for(auto& b : A->lam) adj_release(b);
     for(auto& b : A->work) adj_release(b);
-    for(Buf* b : { &A->lsplit, &A->lsrc, &A->mu, &A->inmask, &A->red, &A->zs, &A->brl_part, &A->gchi, &A->obst_part })
+    for(Buf* b : { &A->lsplit, &A->lsrc, &A->mu, &A->inmask, &A->red, &A->zs, &A->brl_part, &A->gchi, &A->obst_part,
+                   &A->ld32, &A->lr32, &A->mu32 })
         adj_release(*b);
     for(auto& b : A->ckpt) adj_release(b);
     for(auto& b : A->snap) adj_release(b);
@@ -1255,62 +1324,67 @@ void adj_free(atjup_ctx* c){
     c->adj = nullptr;
 }
 
-static inline double* L(atjup_ctx* c, int f){ return (double*)c->adj->lam[f].p; }
-static inline double* W(atjup_ctx* c, int w){ return (double*)c->adj->work[w].p; }
-static inline double* Fd(atjup_ctx* c, int f){ return (double*)c->cur[f].p; }
+// the adjoint fields, work fields and forward fields in the context's storage type T
+template<typename T> static inline T* L(atjup_ctx* c, int f){ return (T*)c->adj->lam[f].p; }
+template<typename T> static inline T* W(atjup_ctx* c, int w){ return (T*)c->adj->work[w].p; }
+template<typename T> static inline T* Fd(atjup_ctx* c, int f){ return (T*)c->cur[f].p; }
 
 static int zero_buf(atjup_ctx* c, Buf& b){
-    const size_t n = b.bytes / sizeof(double);
-    hipLaunchKernelGGL(ka_zero, dim3(nblk(n, 256)), dim3(256), 0, c->stream, (double*)b.p, n);
+    const size_t n = b.bytes / sizeof(uint32_t);            // every adjoint buffer is a multiple of 4 bytes
+    hipLaunchKernelGGL(ka_zero<uint32_t>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, (uint32_t*)b.p, n);
     return check_launch();
 }
-static int copy_field(atjup_ctx* c, double* d, const double* s){
-    hipLaunchKernelGGL(ka_copy, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream, d, s, c->n3);
+template<typename T>
+static int copy_field(atjup_ctx* c, T* d, const T* s){
+    hipLaunchKernelGGL(ka_copy<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream, d, s, c->n3);
     return check_launch();
 }
 
 // ---- the transposed pieces --------------------------------------------------------------------
+template<typename T>
 static int adj_bc(atjup_ctx* c){
     const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
-    double *lt = L(c, ATJUP_F_T), *lu = L(c, ATJUP_F_U), *lv = L(c, ATJUP_F_V), *lw = L(c, ATJUP_F_W);
-    double* lp = L(c, ATJUP_F_P_DYN);
+    T *lt = L<T>(c, ATJUP_F_T), *lu = L<T>(c, ATJUP_F_U), *lv = L<T>(c, ATJUP_F_V), *lw = L<T>(c, ATJUP_F_W);
+    T* lp = L<T>(c, ATJUP_F_P_DYN);
     if(c->n_amb_rows > 0)
-        hipLaunchKernelGGL(ka_t_ambient, dim3(c->n_amb_rows), dim3(64), 0, c->stream, lt,
+        hipLaunchKernelGGL(ka_t_ambient<T>, dim3(c->n_amb_rows), dim3(64), 0, c->stream, lt,
             (const uint8_t*)c->mask.p, (const int2*)c->amb_rows.p, c->n_amb_rows, c->jm, c->km, c->pitch);
     if(c->n_solids > 0)
-        hipLaunchKernelGGL(ka_bc_solid, dim3(nblk(c->n_solids, 64)), dim3(64), 0, c->stream, lu, lv, lw, lt, lp,
+        hipLaunchKernelGGL(ka_bc_solid<T>, dim3(nblk(c->n_solids, 64)), dim3(64), 0, c->stream, lu, lv, lw, lt, lp,
             (const SolidCell*)c->solids.p, c->n_solids, c->jm, c->pitch, c->plane, c43, c13);
-    hipLaunchKernelGGL(ka_bc_phi, dim3(nblk(c->jm, 64), c->im), dim3(64), 0, c->stream, lt, lu, lv, lw,
+    hipLaunchKernelGGL(ka_bc_phi<T>, dim3(nblk(c->jm, 64), c->im), dim3(64), 0, c->stream, lt, lu, lv, lw,
         c->jm, c->km, c->pitch);
-    hipLaunchKernelGGL(ka_bc_theta, dim3(nblk(c->km, 128), c->im), dim3(128), 0, c->stream, lt, lu, lv, lw,
+    hipLaunchKernelGGL(ka_bc_theta<T>, dim3(nblk(c->km, 128), c->im), dim3(128), 0, c->stream, lt, lu, lv, lw,
         c->jm, c->km, c->pitch);
     double* zs = (double*)c->adj->zs.p;
-    hipLaunchKernelGGL(ka_bc_zonal_sums, dim3(nblk(c->jm, 64), 4), dim3(64), 0, c->stream, (const double*)lv,
-        (const double*)lw, zs, c->im, c->jm, c->km, c->pitch, c->plane);
-    hipLaunchKernelGGL(ka_bc_radius, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream, lt, lu, lv, lw,
+    hipLaunchKernelGGL(ka_bc_zonal_sums<T>, dim3(nblk(c->jm, 64), 4), dim3(64), 0, c->stream, (const T*)lv,
+        (const T*)lw, zs, c->im, c->jm, c->km, c->pitch, c->plane);
+    hipLaunchKernelGGL(ka_bc_radius<T>, dim3(nblk(c->km, 128), c->jm), dim3(128), 0, c->stream, lt, lu, lv, lw,
         (const double*)zs, c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
     return check_launch();
 }
 
+template<typename T>
 static int adj_shapiro(atjup_ctx* c, int f){
     // forward passes k, j, i (strength 1, order 2): transposed i, j, k through the scratch buffer
-    double* a = L(c, f);
-    double* b = W(c, AW_SCR);
+    T* a = L<T>(c, f);
+    T* b = W<T>(c, AW_SCR);
     const double co = 0.25;
     dim3 g(nblk(c->km, 128), c->jm, c->im), blk(128);
-    hipLaunchKernelGGL(ka_shapiro_i, g, blk, 0, c->stream, (const double*)a, b, (const int*)c->itopo.p,
+    hipLaunchKernelGGL(ka_shapiro_i<T>, g, blk, 0, c->stream, (const T*)a, b, (const int*)c->itopo.p,
         c->im, c->jm, c->km, c->pitch, c->plane, co);
-    hipLaunchKernelGGL(ka_shapiro_j, g, blk, 0, c->stream, (const double*)b, a, (const int*)c->itopo.p,
+    hipLaunchKernelGGL(ka_shapiro_j<T>, g, blk, 0, c->stream, (const T*)b, a, (const int*)c->itopo.p,
         c->im, c->jm, c->km, c->pitch, co);
-    hipLaunchKernelGGL(ka_shapiro_k, g, blk, 0, c->stream, (const double*)a, b, (const int*)c->itopo.p,
+    hipLaunchKernelGGL(ka_shapiro_k<T>, g, blk, 0, c->stream, (const T*)a, b, (const int*)c->itopo.p,
         c->im, c->jm, c->km, c->pitch, co);
-    return copy_field(c, a, b);
+    return copy_field<T>(c, a, b);
 }
 
+template<typename T>
 static int adj_mixture(atjup_ctx* c){
-    hipLaunchKernelGGL(ka_mixture, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream,
-        (const double*)Fd(c, ATJUP_F_T), (const double*)Fd(c, ATJUP_F_P_STAT), (const double*)Fd(c, ATJUP_F_P_DYN),
-        L(c, ATJUP_F_RHO_MIX), L(c, ATJUP_F_T), L(c, ATJUP_F_P_DYN), c->n3, c->c.t_ref, c->kn.rho_pdyn,
+    hipLaunchKernelGGL(ka_mixture<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream,
+        (const T*)Fd<T>(c, ATJUP_F_T), (const T*)Fd<T>(c, ATJUP_F_P_STAT), (const T*)Fd<T>(c, ATJUP_F_P_DYN),
+        L<T>(c, ATJUP_F_RHO_MIX), L<T>(c, ATJUP_F_T), L<T>(c, ATJUP_F_P_DYN), c->n3, c->c.t_ref, c->kn.rho_pdyn,
         c->c.r_mix * c->c.u_0 * c->c.u_0 * 1.0e-5, c->c.R_mix);
     return check_launch();
 }
@@ -1319,20 +1393,22 @@ static int adj_mixture(atjup_ctx* c){
 // runs the four transposed stages; the n level's adjoint replaces the step result's, p_dyn and
 // rho_mix gather theirs, the hydrostatic and level-mean adjoints are spread back onto v, w and t,
 // and aux_* lose their adjoint in the cells the step overwrote.
+template<typename T>
 static int adj_rk4(atjup_ctx* c){
     AdjState* A = c->adj;
     RkBuffers rb{};
-    for(int s = 0; s < 3; s++) for(int f = 0; f < 4; f++) rb.y[s][f] = W(c, AW_Y + 4 * s + f);
-    for(int f = 0; f < 4; f++) rb.nn[f] = W(c, AW_NN + f);
-    for(int f = 0; f < 3; f++) rb.aux[f] = W(c, AW_AUX + f);
+    for(int s = 0; s < 3; s++) for(int f = 0; f < 4; f++) rb.y[s][f] = (double*)W<T>(c, AW_Y + 4 * s + f);
+    for(int f = 0; f < 4; f++) rb.nn[f] = (double*)W<T>(c, AW_NN + f);
+    for(int f = 0; f < 3; f++) rb.aux[f] = (double*)W<T>(c, AW_AUX + f);
     if(int rc = launch_rk4_dry_record(c, rb)) return rc;
     const double dt = c->c.dt;
     const atjup_consts& C = c->c;
-    RhsAdjArgs a{};
-    a.pd = Fd(c, ATJUP_F_P_DYN); a.ph = Fd(c, ATJUP_F_P_HYDRO); a.ps = Fd(c, ATJUP_F_P_STAT);
-    a.rho = Fd(c, ATJUP_F_RHO_MIX); a.wn = Fd(c, ATJUP_F_WALL_NUE);
-    a.lpd = L(c, ATJUP_F_P_DYN); a.lph = L(c, ATJUP_F_P_HYDRO); a.lrho = L(c, ATJUP_F_RHO_MIX); a.gbrl = W(c, AW_GBRL);
-    a.flags = (const uint8_t*)c->flags.p; a.geo = (const GeoIJ<double>*)c->geo.p;
+    RhsAdjArgs<T> a{};
+    a.pd = Fd<T>(c, ATJUP_F_P_DYN); a.ph = Fd<T>(c, ATJUP_F_P_HYDRO); a.ps = Fd<T>(c, ATJUP_F_P_STAT);
+    a.rho = Fd<T>(c, ATJUP_F_RHO_MIX); a.wn = Fd<T>(c, ATJUP_F_WALL_NUE);
+    a.lpd = L<T>(c, ATJUP_F_P_DYN); a.lph = L<T>(c, ATJUP_F_P_HYDRO); a.lrho = L<T>(c, ATJUP_F_RHO_MIX);
+    a.gbrl = W<T>(c, AW_GBRL);
+    a.flags = (const uint8_t*)c->flags.p; a.geo = (const GeoIJ<geo_t<T>>*)c->geo.p;
     a.im = c->im; a.jm = c->jm; a.km = c->km; a.pitch = c->pitch; a.plane = c->plane;
     a.nkb = (c->km + 63) / 64; a.njb = (c->jm + 3) / 4; a.nb = a.nkb * a.njb * c->im;
     a.inv_2dr = 1.0 / (2.0 * c->g.dr); a.inv_2dthe = 1.0 / (2.0 * c->g.dthe); a.inv_2dphi = 1.0 / (2.0 * c->g.dphi);
@@ -1347,17 +1423,17 @@ static int adj_rk4(atjup_ctx* c){
     a.inv_tau = c->obst_on ? 1.0 / c->obst.tau : 0.0;
     a.gchi = (double*)A->gchi.p;
     for(int f = 0; f < 4; f++){
-        a.lk[f] = W(c, AW_LK + f);
-        a.g[f] = W(c, AW_G + f);
-        if(int rc = copy_field(c, W(c, AW_LNP + f), L(c, f))) return rc;      // lambda of the step result
+        a.lk[f] = W<T>(c, AW_LK + f);
+        a.g[f] = W<T>(c, AW_G + f);
+        if(int rc = copy_field<T>(c, W<T>(c, AW_LNP + f), L<T>(c, f))) return rc;      // lambda of the step result
     }
     dim3 gc(nblk(c->km, 128), c->jm, c->im), b128(128);
     auto combine = [&](double wnp, double cg, int have_g, int write_lk){
-        hipLaunchKernelGGL(ka_rk_combine, gc, b128, 0, c->stream,
-            W(c, AW_LK), W(c, AW_LK + 1), W(c, AW_LK + 2), W(c, AW_LK + 3),
-            W(c, AW_G), W(c, AW_G + 1), W(c, AW_G + 2), W(c, AW_G + 3),
-            (const double*)W(c, AW_LNP), (const double*)W(c, AW_LNP + 1), (const double*)W(c, AW_LNP + 2),
-            (const double*)W(c, AW_LNP + 3), L(c, 0), L(c, 1), L(c, 2), L(c, 3),
+        hipLaunchKernelGGL(ka_rk_combine<T>, gc, b128, 0, c->stream,
+            W<T>(c, AW_LK), W<T>(c, AW_LK + 1), W<T>(c, AW_LK + 2), W<T>(c, AW_LK + 3),
+            W<T>(c, AW_G), W<T>(c, AW_G + 1), W<T>(c, AW_G + 2), W<T>(c, AW_G + 3),
+            (const T*)W<T>(c, AW_LNP), (const T*)W<T>(c, AW_LNP + 1), (const T*)W<T>(c, AW_LNP + 2),
+            (const T*)W<T>(c, AW_LNP + 3), L<T>(c, 0), L<T>(c, 1), L<T>(c, 2), L<T>(c, 3),
             (const uint8_t*)c->flags.p, c->im, c->jm, c->km, c->pitch, wnp, cg, have_g, write_lk);
         return check_launch();
     };
@@ -1366,43 +1442,47 @@ static int adj_rk4(atjup_ctx* c){
     for(int s = 3; s >= 0; s--){
         // lk_s = w_s lambda_N' + c_s lambda_{Y_{s+1}}; lambda_N += lambda_{Y_{s+1}}
         if(int rc = combine(wts[s], cin[s], s < 3 ? 1 : 0, 1)) return rc;
-        for(int f = 0; f < 4; f++) a.y[f] = (s == 0) ? Fd(c, f) : W(c, AW_Y + 4 * (s - 1) + f);
-        for(int f = 0; f < 3; f++) a.la[f] = (s == 3) ? L(c, ATJUP_F_AUX_U + f) : nullptr;
-        hipLaunchKernelGGL(ka_rhs_dry, dim3(a.nb), dim3(64, 4), 0, c->stream, a, a.geo);
+        for(int f = 0; f < 4; f++) a.y[f] = (s == 0) ? Fd<T>(c, f) : W<T>(c, AW_Y + 4 * (s - 1) + f);
+        for(int f = 0; f < 3; f++) a.la[f] = (s == 3) ? L<T>(c, ATJUP_F_AUX_U + f) : nullptr;
+        hipLaunchKernelGGL(ka_rhs_dry<T>, dim3(a.nb), dim3(64, 4), 0, c->stream, a, a.geo);
         if(int rc = check_launch()) return rc;
     }
     if(int rc = combine(0.0, 0.0, 1, 0)) return rc;             // lambda_N += lambda_{Y_0}
     // the level means and the hydrostatic integral, formed once from the n level
     const double* tabs = (const double*)c->tabs.p;
-    hipLaunchKernelGGL(ka_brl_reduce, dim3(ABRL_NCH, c->im), dim3(ABRL_BLOCK), 0, c->stream,
-        (const double*)Fd(c, ATJUP_F_T), (const double*)Fd(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p,
-        tabs + c->im, (const double*)W(c, AW_GBRL), (double*)A->brl_part.p, c->jm, c->km, c->pitch, C.g,
+    hipLaunchKernelGGL(ka_brl_reduce<T>, dim3(ABRL_NCH, c->im), dim3(ABRL_BLOCK), 0, c->stream,
+        (const T*)Fd<T>(c, ATJUP_F_T), (const T*)Fd<T>(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p,
+        tabs + c->im, (const T*)W<T>(c, AW_GBRL), (double*)A->brl_part.p, c->jm, c->km, c->pitch, C.g,
         C.r_mix * C.R_mix, C.t_ref);
     hipLaunchKernelGGL(ka_brl_final, dim3(nblk(c->im, 64)), dim3(64), 0, c->stream, (const double*)A->brl_part.p,
         (double*)A->red.p, c->im);
-    hipLaunchKernelGGL(ka_brl_spread, gc, b128, 0, c->stream, (const double*)Fd(c, ATJUP_F_T),
-        (const double*)Fd(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p, tabs + c->im, (const double*)A->red.p,
-        W(c, AW_GBRL), L(c, ATJUP_F_T), c->jm, c->km, c->pitch, C.g, C.r_mix * C.R_mix, C.t_ref);
-    hipLaunchKernelGGL(ka_hydrostatic, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream,
-        (const double*)Fd(c, ATJUP_F_V), (const double*)Fd(c, ATJUP_F_W), (const int2*)c->colr.p,
-        tabs, tabs + c->im + c->jm, L(c, ATJUP_F_P_HYDRO), L(c, ATJUP_F_V), L(c, ATJUP_F_W),
+    hipLaunchKernelGGL(ka_brl_spread<T>, gc, b128, 0, c->stream, (const T*)Fd<T>(c, ATJUP_F_T),
+        (const T*)Fd<T>(c, ATJUP_F_P_STAT), (const uint8_t*)c->flags.p, tabs + c->im, (const double*)A->red.p,
+        W<T>(c, AW_GBRL), L<T>(c, ATJUP_F_T), c->jm, c->km, c->pitch, C.g, C.r_mix * C.R_mix, C.t_ref);
+    hipLaunchKernelGGL(ka_hydrostatic<T>, dim3(nblk(c->km, 128), c->jm), b128, 0, c->stream,
+        (const T*)Fd<T>(c, ATJUP_F_V), (const T*)Fd<T>(c, ATJUP_F_W), (const int2*)c->colr.p,
+        tabs, tabs + c->im + c->jm, L<T>(c, ATJUP_F_P_HYDRO), L<T>(c, ATJUP_F_V), L<T>(c, ATJUP_F_W),
         c->im, c->jm, c->km, c->pitch, c->plane, c->g.dr);
     // aux_* were overwritten in the integrated cells: their earlier values carry no adjoint there
-    hipLaunchKernelGGL(ka_zero_interior, gc, b128, 0, c->stream, L(c, ATJUP_F_AUX_U), L(c, ATJUP_F_AUX_V),
-        L(c, ATJUP_F_AUX_W), (const uint8_t*)c->flags.p, c->im, c->jm, c->km, c->pitch);
+    hipLaunchKernelGGL(ka_zero_interior<T>, gc, b128, 0, c->stream, L<T>(c, ATJUP_F_AUX_U), L<T>(c, ATJUP_F_AUX_V),
+        L<T>(c, ATJUP_F_AUX_W), (const uint8_t*)c->flags.p, c->im, c->jm, c->km, c->pitch);
     return check_launch();
 }
 
-// PressureSolver::run^T at the context's state (p_dyn and aux_* before the solve)
+// PressureSolver::run^T at the context's state (p_dyn and aux_* before the solve). The FP64 form
+// transposes its sweeps; the FP32-refined form transposes what it runs: the first sweep in FP64,
+// then rounds of FP32 sweeps on the correction inside the FP64 refinement (the rounds' FP32
+// sweeps run transposed in FP32, the residuals and corrections transposed in FP64).
+template<typename T>
 static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
     AdjState* A = c->adj;
     const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
     // the aux_* after prepareProjectionBoundaries, which the clamp test of the source reads (the
     // solve changes nothing else the transpose depends on: the sweeps are linear); p_dyn and aux_*
     // of the context are restored afterwards
-    if(int rc = copy_field(c, W(c, AW_PIN), Fd(c, ATJUP_F_P_DYN))) return rc;
+    if(int rc = copy_field<T>(c, W<T>(c, AW_PIN), Fd<T>(c, ATJUP_F_P_DYN))) return rc;
     for(int f = 0; f < 3; f++)
-        if(int rc = copy_field(c, W(c, AW_AUXP + f), Fd(c, ATJUP_F_AUX_U + f))) return rc;
+        if(int rc = copy_field<T>(c, W<T>(c, AW_AUXP + f), Fd<T>(c, ATJUP_F_AUX_U + f))) return rc;
     if(int rc = launch_prep_projection(c)) return rc;
     const int hp = c->rb_hp;
     const size_t hn = (size_t)c->im * c->jm * hp;
@@ -1416,57 +1496,91 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
     const PoisIJ<double>* pw = (const PoisIJ<double>*)c->pois.p;
     if(int rc = zero_buf(c, A->lsrc)) return rc;
     dim3 gfull(nblk(c->km, 128), c->jm, c->im), b128(128);
-    hipLaunchKernelGGL(ka_rb_pack_add, gfull, b128, 0, c->stream, (const double*)L(c, ATJUP_F_P_DYN), l0, l1,
+    hipLaunchKernelGGL(ka_rb_pack_add<T>, gfull, b128, 0, c->stream, (const T*)L<T>(c, ATJUP_F_P_DYN), l0, l1,
         c->jm, c->km, c->pitch, hp);
     dim3 gseam(nblk(c->jm, 64), c->im), gpj(nblk(nm + 1, 128), c->im, 2), gpi(nblk(nm + 1, 128), c->jm, 2);
     dim3 gt(nblk(nm, RBT_BM), nblk(c->jm, RBT_BJ), c->im), bt(RBT_BM, RBT_BJ);
     dim3 gt2(nblk(nm, RBT_BM), nblk(c->jm, RBT_BJ), 2 * c->im);
-    double* mu0 = (double*)A->mu.p;
-    double* mu1 = mu0 + hn;
+    dim3 gcol(nblk(nm, 128), c->jm, 2 * c->im);
     uint8_t* in0 = (uint8_t*)A->inmask.p;
     uint8_t* in1 = in0 + hn;
-    auto launch_all = [&](){
-        hipLaunchKernelGGL(ka_rbT_inmask, gt2, bt, 0, c->stream, f0, f1, in0, in1, c->im, c->jm, hp, nm);
-        // the planes and the seam written after the last sweep
-        hipLaunchKernelGGL(ka_rb_seam, gseam, dim3(64), 0, c->stream, l0, l1, c->im, c->jm, hp, nm);
-        hipLaunchKernelGGL(ka_rb_planes_j, gpj, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
-        hipLaunchKernelGGL(ka_rb_planes_i, gpi, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
-        hipLaunchKernelGGL(ka_rbT_mu, gt, bt, 0, c->stream, (const double*)l1, mu1, f1, pw, c->im, c->jm, hp, nm, omega);
-        // two cells per thread, flattened over (row, pair), where the ring is even and row-aligned
-        // pairs would leave more than 10% of the lanes idle; else one cell per thread. Measured on
-        // MI355X over the misfit gradients of the bench: 41 x 181 x 361 (90 pairs per row) 81 ms of
-        // transposed sweeps flattened against 84 ms one cell; 81 x 361 x 721 (180 pairs) 1478 ms
-        // one cell against 1592 ms row-aligned pairs and 1729 ms flattened.
-        const int wf_pair = (nm / 2 + RBT_BM - 1) / RBT_BM;
-        const bool pairs = nm % 2 == 0 && (double)(nm / 2) / (RBT_BM * wf_pair) < 0.9;
-        const int ntb = (c->jm * (nm / 2) + 255) / 256;
-        for(int s = sweeps - 1; s >= 0; s--){
-            const int ghost = s > 0 ? 1 : 0;         // the drop-in's first sweep reads the stored planes
+    // two cells per thread, flattened over (row, pair), where the ring is even and row-aligned
+    // pairs would leave more than 10% of the lanes idle; else one cell per thread. Measured on
+    // MI355X over the misfit gradients of the bench: 41 x 181 x 361 (90 pairs per row) 81 ms of
+    // transposed sweeps flattened against 84 ms one cell; 81 x 361 x 721 (180 pairs) 1478 ms
+    // one cell against 1592 ms row-aligned pairs and 1729 ms flattened.
+    const int wf_pair = (nm / 2 + RBT_BM - 1) / RBT_BM;
+    const bool pairs = nm % 2 == 0 && (double)(nm / 2) / (RBT_BM * wf_pair) < 0.9;
+    const int ntb = (c->jm * (nm / 2) + 255) / 256;
+    // n sweeps transposed on the split adjoint la (its mu and source adjoint beside it), in the
+    // sweeps' own precision; the forward's first sweep reads the stored planes unless all of them
+    // form the planes (the refined form's correction sweeps)
+    auto tsweeps = [&](auto* la0, auto* mua0, auto* lsa0, int n, bool all_ghost){
+        using S = typename std::remove_pointer<decltype(la0)>::type;
+        S* la1 = la0 + hn;
+        S* mua1 = mua0 + hn;
+        S* lsa1 = lsa0 + hn;
+        hipLaunchKernelGGL(ka_rbT_mu<S>, gt, bt, 0, c->stream, (const S*)la1, mua1, f1, pw, c->im, c->jm, hp, nm, (S)omega);
+        for(int s = n - 1; s >= 0; s--){
+            const int ghost = (all_ghost || s > 0) ? 1 : 0;
             for(int colour = 0; colour < 2; colour++){
-                double* lc = colour ? l1 : l0;
-                double* mc = colour ? mu1 : mu0;
-                const double* mo = colour ? mu0 : mu1;
-                double* sc = colour ? s1 : s0;
+                S* lc = colour ? la1 : la0;
+                S* mc = colour ? mua1 : mua0;
+                const S* mo = colour ? mua0 : mua1;
+                S* sc = colour ? lsa1 : lsa0;
                 const uint8_t* fc = colour ? f1 : f0;
                 const uint8_t* fo = colour ? f0 : f1;
                 const uint8_t* ic = colour ? in1 : in0;
                 if(pairs)
-                    hipLaunchKernelGGL(ka_rbT_pass2, dim3(ntb * c->im), dim3(256), 0, c->stream, lc, mc, mo, sc, fc, fo,
-                        ic, pw, c->im, c->jm, hp, nm, ntb, colour, ghost, omega, c43, c13);
+                    hipLaunchKernelGGL(ka_rbT_pass2<S>, dim3(ntb * c->im), dim3(256), 0, c->stream, lc, mc, mo, sc, fc, fo,
+                        ic, pw, c->im, c->jm, hp, nm, ntb, colour, ghost, (S)omega, (S)c43, (S)c13);
                 else
-                    hipLaunchKernelGGL(ka_rbT_pass, gt, bt, 0, c->stream, lc, mc, mo, sc, fc, fo,
-                        ic, pw, c->im, c->jm, hp, nm, colour, ghost, omega, c43, c13);
+                    hipLaunchKernelGGL(ka_rbT_pass<S>, gt, bt, 0, c->stream, lc, mc, mo, sc, fc, fo,
+                        ic, pw, c->im, c->jm, hp, nm, colour, ghost, (S)omega, (S)c43, (S)c13);
             }
         }
     };
-    // the 2 launches per sweep go to the device as one graph, captured on first use per sweep
-    // count and relaxation factor (the buffers are the adjoint state's own, fixed for its life)
-    const void* ptrs[8] = { l0, l1, s0, s1, mu0, in0, f0, pw };
+    auto planes_T = [&](){      // k_rb_planes_i then k_rb_planes_j, transposed
+        hipLaunchKernelGGL(ka_rb_planes_j, gpj, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
+        hipLaunchKernelGGL(ka_rb_planes_i, gpi, b128, 0, c->stream, l0, l1, c->im, c->jm, hp, nm, c43, c13);
+    };
+    const bool refined = std::is_same<T, float>::value && sweeps > 1;
+    std::vector<int> rounds;                      // the refined form's correction sweeps per round
+    if(refined)
+        for(int left = sweeps - 1; left > 0;){ const int n = std::min(left, c->refine_sweeps); rounds.push_back(n); left -= n; }
+    float* ld0 = (float*)A->ld32.p;
+    float* lr0 = (float*)A->lr32.p;
+    float* m32 = (float*)A->mu32.p;
+    if(refined && (!ld0 || !lr0 || !m32)){ set_error("adjoint: the refined form's buffers are missing"); return ATJUP_ERR_STATE; }
+    auto launch_all = [&](){
+        hipLaunchKernelGGL(ka_rbT_inmask, gt2, bt, 0, c->stream, f0, f1, in0, in1, c->im, c->jm, hp, nm);
+        // the planes and the seam written after the last sweep
+        hipLaunchKernelGGL(ka_rb_seam, gseam, dim3(64), 0, c->stream, l0, l1, c->im, c->jm, hp, nm);
+        planes_T();
+        if(!refined){
+            tsweeps(l0, (double*)A->mu.p, s0, sweeps, false);
+            return;
+        }
+        for(int r = (int)rounds.size() - 1; r >= 0; r--){
+            hipLaunchKernelGGL(ka_rb_addT, gcol, b128, 0, c->stream, (const double*)l0, (const double*)l1,
+                ld0, ld0 + hn, lr0, lr0 + hn, c->im, c->jm, hp, nm);
+            tsweeps(ld0, m32, lr0, rounds[r], true);
+            hipLaunchKernelGGL(ka_rb_resT, gcol, b128, 0, c->stream, l0, l1, s0, s1, (const float*)lr0,
+                (const float*)(lr0 + hn), f0, f1, (const uint8_t*)in0, (const uint8_t*)in1, pw, c->im, c->jm, hp, nm);
+            planes_T();
+        }
+        tsweeps(l0, (double*)A->mu.p, s0, 1, false);          // the first sweep, FP64
+    };
+    // the transposed solve goes to the device as one graph, captured on first use per kind, sweep
+    // count, relaxation factor and round length, and replayed only on the buffers it was captured with
+    const int kind = refined ? 1 : 0;
+    const void* ptrs[12] = { l0, s0, A->mu.p, in0, f0, pw, ld0, lr0, m32, nullptr, nullptr, nullptr };
     const int dims[4] = { c->im, c->jm, hp, nm };
+    const int nr = refined ? c->refine_sweeps : 0;
     hipGraphExec_t exec = nullptr;
     AdjState::SweepGraph* slot = nullptr;
-    for(auto& g : A->graphs) if(g.n == sweeps && g.omega == omega){ slot = &g; break; }
-    if(slot && std::equal(ptrs, ptrs + 8, slot->ptrs) && std::equal(dims, dims + 4, slot->dims)) exec = slot->exec;
+    for(auto& g : A->graphs) if(g.kind == kind && g.n == sweeps && g.omega == omega && g.nr == nr){ slot = &g; break; }
+    if(slot && std::equal(ptrs, ptrs + 12, slot->ptrs) && std::equal(dims, dims + 4, slot->dims)) exec = slot->exec;
     if(!exec){
         hipGraph_t graph = nullptr;
         ATJUP_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1477,8 +1591,8 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
         const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if(ie != hipSuccess){ set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); return ATJUP_ERR_HIP; }
-        AdjState::SweepGraph g{ sweeps, omega, exec, {}, {} };
-        std::copy(ptrs, ptrs + 8, g.ptrs);
+        AdjState::SweepGraph g{ kind, sweeps, omega, nr, exec, {}, {} };
+        std::copy(ptrs, ptrs + 12, g.ptrs);
         std::copy(dims, dims + 4, g.dims);
         if(slot){
             ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1491,24 +1605,24 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
     hipEvent_t t_sw = timer_begin(c);
     ATJUP_HIP_CHECK(hipGraphLaunch(exec, c->stream));
     timer_end(c, ATJUP_T_ADJ_SWEEPS, t_sw, 2 * sweeps + 5);
-    hipLaunchKernelGGL(ka_rb_unpack_add, gfull, b128, 0, c->stream, L(c, ATJUP_F_P_DYN), (const double*)l0,
+    hipLaunchKernelGGL(ka_rb_unpack_add<T>, gfull, b128, 0, c->stream, L<T>(c, ATJUP_F_P_DYN), (const double*)l0,
         (const double*)l1, c->jm, c->km, c->pitch, hp);
     // the source and prepareProjectionBoundaries
-    SrcAdjArgs sa{};
-    sa.au = Fd(c, ATJUP_F_AUX_U); sa.av = Fd(c, ATJUP_F_AUX_V); sa.aw = Fd(c, ATJUP_F_AUX_W);
+    SrcAdjArgs<T> sa{};
+    sa.au = Fd<T>(c, ATJUP_F_AUX_U); sa.av = Fd<T>(c, ATJUP_F_AUX_V); sa.aw = Fd<T>(c, ATJUP_F_AUX_W);
     sa.ls0 = s0; sa.ls1 = s1;
-    sa.lau = L(c, ATJUP_F_AUX_U); sa.lav = L(c, ATJUP_F_AUX_V); sa.law = L(c, ATJUP_F_AUX_W);
+    sa.lau = L<T>(c, ATJUP_F_AUX_U); sa.lav = L<T>(c, ATJUP_F_AUX_V); sa.law = L<T>(c, ATJUP_F_AUX_W);
     sa.flags = (const uint8_t*)c->flags.p; sa.msk = (const uint8_t*)c->mask.p; sa.pw = pw;
     sa.im = c->im; sa.jm = c->jm; sa.km = c->km; sa.pitch = c->pitch; sa.hp = hp; sa.plane = c->plane;
     sa.inv_2dr = 1.0 / (2.0 * c->g.dr); sa.inv_2dthe = 1.0 / (2.0 * c->g.dthe); sa.inv_2dphi = 1.0 / (2.0 * c->g.dphi);
     sa.inv_dthe = 1.0 / c->g.dthe; sa.inv_dphi = 1.0 / c->g.dphi; sa.cap = c->kn.pdyn_cap;
-    hipLaunchKernelGGL(ka_poisson_source, dim3(nblk(c->km - 1, 128), c->jm - 2, c->im - 2), b128, 0, c->stream, sa);
-    hipLaunchKernelGGL(ka_prep_projection, dim3(nblk(c->km - 1, 128), c->jm - 2), b128, 0, c->stream,
-        L(c, ATJUP_F_AUX_U), L(c, ATJUP_F_AUX_V), L(c, ATJUP_F_AUX_W), c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
+    hipLaunchKernelGGL(ka_poisson_source<T>, dim3(nblk(c->km - 1, 128), c->jm - 2, c->im - 2), b128, 0, c->stream, sa);
+    hipLaunchKernelGGL(ka_prep_projection<T>, dim3(nblk(c->km - 1, 128), c->jm - 2), b128, 0, c->stream,
+        L<T>(c, ATJUP_F_AUX_U), L<T>(c, ATJUP_F_AUX_V), L<T>(c, ATJUP_F_AUX_W), c->im, c->jm, c->km, c->pitch, c->plane, c43, c13);
     // restore the context's state before the solve
-    if(int rc = copy_field(c, Fd(c, ATJUP_F_P_DYN), W(c, AW_PIN))) return rc;
+    if(int rc = copy_field<T>(c, Fd<T>(c, ATJUP_F_P_DYN), W<T>(c, AW_PIN))) return rc;
     for(int f = 0; f < 3; f++)
-        if(int rc = copy_field(c, Fd(c, ATJUP_F_AUX_U + f), W(c, AW_AUXP + f))) return rc;
+        if(int rc = copy_field<T>(c, Fd<T>(c, ATJUP_F_AUX_U + f), W<T>(c, AW_AUXP + f))) return rc;
     return check_launch();
 }
 
@@ -1517,17 +1631,18 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
 // transposes and the RK4's linearise about. On return the adjoint fields have been carried from
 // the end of the iteration to its start; the context's forward fields are left as the recording
 // left them (the n level and aux_* at the iteration's start, p_dyn and rho_mix after the block).
+template<typename T>
 static int adj_step(atjup_ctx* c, int iter, int first, const Buf* post = nullptr){
     const bool phys = (iter % 2 == 0) || first;
     const int sweeps = std::max(1, c->kn.press_sweeps);
     if(phys){
-        if(int rc = copy_field(c, W(c, AW_PIN), Fd(c, ATJUP_F_P_DYN))) return rc;
+        if(int rc = copy_field<T>(c, W<T>(c, AW_PIN), Fd<T>(c, ATJUP_F_P_DYN))) return rc;
         for(int f = 0; f < 3; f++)
-            if(int rc = copy_field(c, W(c, AW_AUXP + f), Fd(c, ATJUP_F_AUX_U + f))) return rc;
+            if(int rc = copy_field<T>(c, W<T>(c, AW_AUXP + f), Fd<T>(c, ATJUP_F_AUX_U + f))) return rc;
         if(post){
             // the state after the block, as the recomputation recorded it
             for(int s = 0; s < N_POST; s++)
-                if(int rc = copy_field(c, Fd(c, kPostFields[s]), (const double*)post[s].p)) return rc;
+                if(int rc = copy_field<T>(c, Fd<T>(c, kPostFields[s]), (const T*)post[s].p)) return rc;
         } else {
             if(int rc = launch_pressure(c, sweeps, c->kn.press_omega)) return rc;
             if(c->kn.damp_pdyn)
@@ -1535,18 +1650,18 @@ static int adj_step(atjup_ctx* c, int iter, int first, const Buf* post = nullptr
             if(int rc = launch_mixture_density(c)) return rc;
         }
     }
-    if(int rc = adj_bc(c)) return rc;
-    if(int rc = adj_rk4(c)) return rc;
+    if(int rc = adj_bc<T>(c)) return rc;
+    if(int rc = adj_rk4<T>(c)) return rc;
     if(phys){
-        if(int rc = adj_mixture(c)) return rc;
+        if(int rc = adj_mixture<T>(c)) return rc;
         if(c->kn.damp_pdyn)
-            if(int rc = adj_shapiro(c, ATJUP_F_P_DYN)) return rc;
+            if(int rc = adj_shapiro<T>(c, ATJUP_F_P_DYN)) return rc;
         // back to the state before the solve for its transpose (adj_pressure keeps its own copy)
-        double* pin = W(c, AW_PIN);
-        if(int rc = copy_field(c, Fd(c, ATJUP_F_P_DYN), pin)) return rc;
+        T* pin = W<T>(c, AW_PIN);
+        if(int rc = copy_field<T>(c, Fd<T>(c, ATJUP_F_P_DYN), pin)) return rc;
         for(int f = 0; f < 3; f++)
-            if(int rc = copy_field(c, Fd(c, ATJUP_F_AUX_U + f), W(c, AW_AUXP + f))) return rc;
-        if(int rc = adj_pressure(c, sweeps, c->kn.press_omega)) return rc;
+            if(int rc = copy_field<T>(c, Fd<T>(c, ATJUP_F_AUX_U + f), W<T>(c, AW_AUXP + f))) return rc;
+        if(int rc = adj_pressure<T>(c, sweeps, c->kn.press_omega)) return rc;
     }
     return 0;
 }
@@ -1563,34 +1678,88 @@ int need_adj(atjup_ctx* c){
     return 0;
 }
 
-int save_state(atjup_ctx* c, Buf* dst){
-    for(int s = 0; s < N_STATE; s++)
-        if(int rc = copy_field(c, (double*)dst[s].p, Fd(c, kStateFields[s]))) return rc;
+bool fp32(const atjup_ctx* c){ return c->precision == ATJUP_PRECISION_FP32_REF; }
+
+// the forward state a step starts from: the nine fields and, in the refined form, the low words of
+// the paired fields (all of one field's size: n3 doubles, or n3 floats in the refined form)
+template<typename T>
+int state_copy(atjup_ctx* c, Buf* ckpt, bool save){
+    for(int s = 0; s < N_STATE_FIELDS; s++){
+        T* f = Fd<T>(c, kStateFields[s]);
+        T* b = (T*)ckpt[s].p;
+        if(int rc = save ? copy_field<T>(c, b, f) : copy_field<T>(c, f, b)) return rc;
+    }
+    for(int s = N_STATE_FIELDS; s < n_state(c); s++){
+        float* f = (float*)c->lo[ATJUP_F_T + (s - N_STATE_FIELDS)].p;
+        float* b = (float*)ckpt[s].p;
+        if(!f){ set_error("adjoint: a paired field has no low word"); return ATJUP_ERR_STATE; }
+        if(int rc = save ? copy_field<float>(c, b, f) : copy_field<float>(c, f, b)) return rc;
+    }
     return 0;
 }
+int save_state(atjup_ctx* c, Buf* dst){
+    return fp32(c) ? state_copy<float>(c, dst, true) : state_copy<double>(c, dst, true);
+}
 int load_state(atjup_ctx* c, const Buf* src){
-    for(int s = 0; s < N_STATE; s++)
-        if(int rc = copy_field(c, Fd(c, kStateFields[s]), (const double*)src[s].p)) return rc;
+    return fp32(c) ? state_copy<float>(c, const_cast<Buf*>(src), false) : state_copy<double>(c, const_cast<Buf*>(src), false);
+}
+int post_copy(atjup_ctx* c, Buf* post, bool save){
+    for(int s = 0; s < N_POST; s++){
+        int rc;
+        if(fp32(c)){
+            float* f = Fd<float>(c, kPostFields[s]);
+            rc = save ? copy_field<float>(c, (float*)post[s].p, f) : copy_field<float>(c, f, (const float*)post[s].p);
+        } else {
+            double* f = Fd<double>(c, kPostFields[s]);
+            rc = save ? copy_field<double>(c, (double*)post[s].p, f) : copy_field<double>(c, f, (const double*)post[s].p);
+        }
+        if(rc) return rc;
+    }
     return 0;
 }
 
-int host_to_pitch(atjup_ctx* c, double* dev, const double* src){
+// host (i, j, k) doubles <-> a device field of the context's storage type in the pitch layout
+int host_to_pitch(atjup_ctx* c, void* dev, const double* src){
+    const size_t esz = fp32(c) ? sizeof(float) : sizeof(double);
     std::vector<double> h(c->n3, 0.0);
     for(int i = 0; i < c->im; i++)
         for(int j = 0; j < c->jm; j++)
             std::memcpy(h.data() + c->idx(i, j, 0), src + ((size_t)i * c->jm + j) * c->km, c->km * sizeof(double));
-    ATJUP_HIP_CHECK(hipMemcpyAsync(dev, h.data(), c->n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<float> hf;
+    const void* up = h.data();
+    if(esz == sizeof(float)){ hf.assign(h.begin(), h.end()); up = hf.data(); }
+    ATJUP_HIP_CHECK(hipMemcpyAsync(dev, up, c->n3 * esz, hipMemcpyHostToDevice, c->stream));
     ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
-int pitch_to_host(atjup_ctx* c, double* dst, const double* dev){
+int pitch_to_host(atjup_ctx* c, double* dst, const void* dev){
+    const size_t esz = fp32(c) ? sizeof(float) : sizeof(double);
     std::vector<double> h(c->n3);
-    ATJUP_HIP_CHECK(hipMemcpyAsync(h.data(), dev, c->n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if(esz == sizeof(float)){
+        std::vector<float> hf(c->n3);
+        ATJUP_HIP_CHECK(hipMemcpyAsync(hf.data(), dev, c->n3 * esz, hipMemcpyDeviceToHost, c->stream));
+        ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+        for(size_t q = 0; q < c->n3; q++) h[q] = (double)hf[q];
+    } else {
+        ATJUP_HIP_CHECK(hipMemcpyAsync(h.data(), dev, c->n3 * esz, hipMemcpyDeviceToHost, c->stream));
+        ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
     for(int i = 0; i < c->im; i++)
         for(int j = 0; j < c->jm; j++)
             std::memcpy(dst + ((size_t)i * c->jm + j) * c->km, h.data() + c->idx(i, j, 0), c->km * sizeof(double));
     return 0;
+}
+
+// the transposed pieces in the context's precision
+int adj_bc_any(atjup_ctx* c){ return fp32(c) ? adj_bc<float>(c) : adj_bc<double>(c); }
+int adj_rk4_any(atjup_ctx* c){ return fp32(c) ? adj_rk4<float>(c) : adj_rk4<double>(c); }
+int adj_mixture_any(atjup_ctx* c){ return fp32(c) ? adj_mixture<float>(c) : adj_mixture<double>(c); }
+int adj_shapiro_any(atjup_ctx* c, int f){ return fp32(c) ? adj_shapiro<float>(c, f) : adj_shapiro<double>(c, f); }
+int adj_pressure_any(atjup_ctx* c, int sweeps, double omega){
+    return fp32(c) ? adj_pressure<float>(c, sweeps, omega) : adj_pressure<double>(c, sweeps, omega);
+}
+int adj_step_any(atjup_ctx* c, int iter, int first, const Buf* post = nullptr){
+    return fp32(c) ? adj_step<float>(c, iter, first, post) : adj_step<double>(c, iter, first, post);
 }
 
 }  // namespace
@@ -1599,13 +1768,13 @@ extern "C" {
 
 int atjup_hip_adj_init(atjup_ctx* c){
     if(!c){ set_error("null context"); return ATJUP_ERR_ARG; }
-    if(c->precision != ATJUP_PRECISION_FP64 || !c->kn.dry){
-        set_error("the adjoint covers the FP64 dynamics profile"); return ATJUP_ERR_PROFILE;
+    if((c->precision != ATJUP_PRECISION_FP64 && c->precision != ATJUP_PRECISION_FP32_REF) || !c->kn.dry){
+        set_error("the adjoint covers the dynamics profile in the FP64 and FP32-refined forms"); return ATJUP_ERR_PROFILE;
     }
     if(c->adj) return 0;
     c->adj = new AdjState();
     AdjState* A = c->adj;
-    const size_t fb = c->n3 * sizeof(double);
+    const size_t fb = c->n3 * (fp32(c) ? sizeof(float) : sizeof(double));
     int rc = 0;
     for(int f : kLamFields) if(!rc) rc = adj_alloc(A->lam[f], fb);
     for(int w = 0; w < AW_N && !rc; w++) rc = adj_alloc(A->work[w], fb);
@@ -1616,9 +1785,14 @@ int atjup_hip_adj_init(atjup_ctx* c){
     if(!rc) rc = adj_alloc(A->inmask, 2 * hn);
     if(!rc) rc = adj_alloc(A->red, 2 * (size_t)c->im * sizeof(double));
     if(!rc) rc = adj_alloc(A->brl_part, 2 * (size_t)c->im * ABRL_NCH * sizeof(double));
-    if(!rc) rc = adj_alloc(A->gchi, fb);
+    if(!rc) rc = adj_alloc(A->gchi, c->n3 * sizeof(double));
     if(!rc) rc = adj_alloc(A->obst_part, 3 * (size_t)OBST_BLOCKS * sizeof(double));
     if(!rc) rc = adj_alloc(A->zs, 4 * (size_t)c->jm * sizeof(double));
+    if(fp32(c)){
+        if(!rc) rc = adj_alloc(A->ld32, 2 * hn * sizeof(float));
+        if(!rc) rc = adj_alloc(A->lr32, 2 * hn * sizeof(float));
+        if(!rc) rc = adj_alloc(A->mu32, 2 * hn * sizeof(float));
+    }
     if(rc){ adj_free(c); return rc; }
     return 0;
 }
@@ -1675,27 +1849,27 @@ int atjup_hip_adj_obstacle_gradient(atjup_ctx* c, double* out){
 int atjup_hip_adj_upload(atjup_ctx* c, int f, const double* src){
     if(int rc = need_adj(c)) return rc;
     if(f < 0 || f >= ATJUP_F_END || !c->adj->lam[f].p || !src){ set_error("adjoint field id"); return ATJUP_ERR_ARG; }
-    return host_to_pitch(c, (double*)c->adj->lam[f].p, src);
+    return host_to_pitch(c, c->adj->lam[f].p, src);
 }
 
 int atjup_hip_adj_download(atjup_ctx* c, int f, double* dst){
     if(int rc = need_adj(c)) return rc;
     if(f < 0 || f >= ATJUP_F_END || !c->adj->lam[f].p || !dst){ set_error("adjoint field id"); return ATJUP_ERR_ARG; }
-    return pitch_to_host(c, dst, (const double*)c->adj->lam[f].p);
+    return pitch_to_host(c, dst, c->adj->lam[f].p);
 }
 
 int atjup_hip_adj_op(atjup_ctx* c, int op, int iarg, double darg){
     if(int rc = need_adj(c)) return rc;
     if(!c->have_mask){ set_error("upload the obstacle mask first"); return ATJUP_ERR_STATE; }
     switch(op){
-        case ATJUP_ADJ_OP_BC:           return adj_bc(c);
-        case ATJUP_ADJ_OP_SHAPIRO_PDYN: return adj_shapiro(c, ATJUP_F_P_DYN);
-        case ATJUP_ADJ_OP_MIXTURE:      return adj_mixture(c);
-        case ATJUP_ADJ_OP_RK4:          return adj_rk4(c);
+        case ATJUP_ADJ_OP_BC:           return adj_bc_any(c);
+        case ATJUP_ADJ_OP_SHAPIRO_PDYN: return adj_shapiro_any(c, ATJUP_F_P_DYN);
+        case ATJUP_ADJ_OP_MIXTURE:      return adj_mixture_any(c);
+        case ATJUP_ADJ_OP_RK4:          return adj_rk4_any(c);
         case ATJUP_ADJ_OP_PRESSURE:
             if(!(darg > 0.0 && darg < 2.0)){ set_error("omega outside (0, 2)"); return ATJUP_ERR_ARG; }
-            return adj_pressure(c, iarg > 0 ? iarg : std::max(1, c->kn.press_sweeps), darg);
-        case ATJUP_ADJ_OP_STEP:         return adj_step(c, iarg, darg != 0.0 ? 1 : 0);
+            return adj_pressure_any(c, iarg > 0 ? iarg : std::max(1, c->kn.press_sweeps), darg);
+        case ATJUP_ADJ_OP_STEP:         return adj_step_any(c, iarg, darg != 0.0 ? 1 : 0);
         default: set_error("adjoint operator"); return ATJUP_ERR_ARG;
     }
 }
@@ -1710,17 +1884,19 @@ int atjup_hip_adj_forward_checkpointed(atjup_ctx* c, int nsteps, int every, int 
     // Storage: the state every `every` steps (the checkpoints), the state after each physics block
     // (the pressure solve's result: the recomputation of a segment then runs no solve), and the
     // step states of the last segment (its reverse sweep then needs no recomputation). On the
-    // 41 x 181 x 361 shell, 20 steps with checkpoints every 5: 36 + 55 + 45 fields of 21.6 MB.
-    // Buffers are kept between calls of the same shape.
+    // 41 x 181 x 361 shell, 20 steps with checkpoints every 5, FP64: 36 + 55 + 45 fields of 21.6 MB
+    // (the refined form: 52 + 55 + 65 fields of 11.4 MB). Buffers are kept between calls of the
+    // same shape.
     const int nseg = (nsteps + every - 1) / every;
     const int s_last = (nseg - 1) * every;
+    const int ns = n_state(c);
     A->post_slot.assign(nsteps, -1);
     int nphys = 0;
     for(int n = 0; n < nsteps; n++){
         const int iter = first_iter + n;
         if(iter % 2 == 0 || n == 0) A->post_slot[n] = nphys++;
     }
-    const size_t fb = c->n3 * sizeof(double);
+    const size_t fb = c->n3 * (fp32(c) ? sizeof(float) : sizeof(double));
     auto ensure = [&](std::vector<Buf>& v, size_t count) -> int {
         if(v.size() == count && (count == 0 || v[0].bytes == fb)) return 0;
         for(auto& b : v) adj_release(b);
@@ -1728,22 +1904,19 @@ int atjup_hip_adj_forward_checkpointed(atjup_ctx* c, int nsteps, int every, int 
         for(auto& b : v) if(int rc = adj_alloc(b, fb)) return rc;
         return 0;
     };
-    if(int rc = ensure(A->ckpt, (size_t)nseg * N_STATE)) return rc;
-    if(int rc = ensure(A->snap, (size_t)every * N_STATE)) return rc;
+    if(int rc = ensure(A->ckpt, (size_t)nseg * ns)) return rc;
+    if(int rc = ensure(A->snap, (size_t)every * ns)) return rc;
     if(int rc = ensure(A->post, (size_t)nphys * N_POST)) return rc;
     for(int n = 0; n < nsteps; n++){
         const int iter = first_iter + n, first = n == 0 ? 1 : 0;
         if(n % every == 0)
-            if(int rc = save_state(c, &A->ckpt[(size_t)(n / every) * N_STATE])) return rc;
+            if(int rc = save_state(c, &A->ckpt[(size_t)(n / every) * ns])) return rc;
         if(n >= s_last)
-            if(int rc = save_state(c, &A->snap[(size_t)(n - s_last) * N_STATE])) return rc;
+            if(int rc = save_state(c, &A->snap[(size_t)(n - s_last) * ns])) return rc;
         hipEvent_t t_step = timer_begin(c);
         if(int rc = step_block(c, iter, first)) return rc;
-        if(A->post_slot[n] >= 0){
-            Buf* post = &A->post[(size_t)A->post_slot[n] * N_POST];
-            for(int s = 0; s < N_POST; s++)
-                if(int rc = copy_field(c, (double*)post[s].p, Fd(c, kPostFields[s]))) return rc;
-        }
+        if(A->post_slot[n] >= 0)
+            if(int rc = post_copy(c, &A->post[(size_t)A->post_slot[n] * N_POST], true)) return rc;
         if(int rc = step_rest(c, iter)) return rc;
         timer_end(c, ATJUP_T_STEP, t_step, 0);
     }
@@ -1759,26 +1932,24 @@ int atjup_hip_adj_backward(atjup_ctx* c, const double* lam_final, double* lam_in
     if(lam_final){
         if(int rc = atjup_hip_adj_zero(c)) return rc;
         for(int f = 0; f < 4; f++)
-            if(int rc = host_to_pitch(c, L(c, f), lam_final + (size_t)f * c->im * c->jm * c->km)) return rc;
+            if(int rc = host_to_pitch(c, c->adj->lam[f].p, lam_final + (size_t)f * c->im * c->jm * c->km)) return rc;
     }
     const int every = A->every, nsteps = A->nsteps;
     const int nseg = (nsteps + every - 1) / every;
+    const int ns = n_state(c);
     for(int seg = nseg - 1; seg >= 0; seg--){
         const int s0 = seg * every, s1 = std::min(nsteps, s0 + every);
         if(!(seg == nseg - 1 && A->last_in_snap)){
             // recompute the segment from its checkpoint, keeping each step's start; the physics
             // blocks' results come from the forward's store, so no pressure solve runs here
-            if(int rc = load_state(c, &A->ckpt[(size_t)seg * N_STATE])) return rc;
+            if(int rc = load_state(c, &A->ckpt[(size_t)seg * ns])) return rc;
             hipEvent_t t_rc = timer_begin(c);
             for(int n = s0; n < s1; n++){
                 const int iter = A->first_iter + n;
-                if(int rc = save_state(c, &A->snap[(size_t)(n - s0) * N_STATE])) return rc;
+                if(int rc = save_state(c, &A->snap[(size_t)(n - s0) * ns])) return rc;
                 if(n + 1 == s1) break;
-                if(A->post_slot[n] >= 0){
-                    const Buf* post = &A->post[(size_t)A->post_slot[n] * N_POST];
-                    for(int s = 0; s < N_POST; s++)
-                        if(int rc = copy_field(c, Fd(c, kPostFields[s]), (const double*)post[s].p)) return rc;
-                }
+                if(A->post_slot[n] >= 0)
+                    if(int rc = post_copy(c, &A->post[(size_t)A->post_slot[n] * N_POST], false)) return rc;
                 if(int rc = step_rest(c, iter)) return rc;
             }
             timer_end(c, ATJUP_T_ADJ_RECOMPUTE, t_rc, 0);
@@ -1786,16 +1957,16 @@ int atjup_hip_adj_backward(atjup_ctx* c, const double* lam_final, double* lam_in
         A->last_in_snap = false;        // the snapshots now belong to this segment
         for(int n = s1 - 1; n >= s0; n--){
             const int iter = A->first_iter + n, first = n == 0 ? 1 : 0;
-            if(int rc = load_state(c, &A->snap[(size_t)(n - s0) * N_STATE])) return rc;
+            if(int rc = load_state(c, &A->snap[(size_t)(n - s0) * ns])) return rc;
             const int slot = A->post_slot[n];
             hipEvent_t t_as = timer_begin(c);
-            if(int rc = adj_step(c, iter, first, slot >= 0 ? &A->post[(size_t)slot * N_POST] : nullptr)) return rc;
+            if(int rc = adj_step_any(c, iter, first, slot >= 0 ? &A->post[(size_t)slot * N_POST] : nullptr)) return rc;
             timer_end(c, ATJUP_T_ADJ_STEP, t_as, 0);
         }
     }
     if(lam_init)
         for(int f = 0; f < 4; f++)
-            if(int rc = pitch_to_host(c, lam_init + (size_t)f * c->im * c->jm * c->km, L(c, f))) return rc;
+            if(int rc = pitch_to_host(c, lam_init + (size_t)f * c->im * c->jm * c->km, c->adj->lam[f].p)) return rc;
     return 0;
 }
 

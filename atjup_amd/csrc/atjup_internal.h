@@ -197,7 +197,8 @@ inline void* live_ptr(atjup_ctx* c, int f){
 
 void adj_free(atjup_ctx* c);
 
-// the adjoint's recording RK4 run: the stage inputs Y1..Y3, the step's result and its aux_*
+// the adjoint's recording RK4 run: the stage inputs Y1..Y3, the step's result and its aux_*, in
+// fields of the context's storage type (FP64 or FP32; the pointers are carried untyped)
 struct RkBuffers { double* y[3][4]; double* nn[4]; double* aux[3]; };
 int launch_rk4_dry_record(atjup_ctx* c, const RkBuffers& rb);
 

@@ -1446,8 +1446,10 @@ int do_rk4_dry(atjup_ctx* c, const RkBuffers* rb = nullptr){
         }
         if(rb){
             // the recording run of the adjoint: every stage input kept (Y1, Y2, Y3), the step's
-            // result in its own buffers, the n level and aux_* of the context left as they were
+            // result in its own buffers, the n level, its low words and aux_* of the context left
+            // as they were (the RK buffers hold T values; RkBuffers carries them untyped)
             a.copy_boundary = 1;
+            for(int f = 0; f < 4; f++) a.lo[f] = nullptr;
             for(int f = 0; f < 4; f++){
                 a.in[f] = (s == 0) ? F<T>(c, f) : (const T*)rb->y[s - 1][f];
                 a.out[f] = (s < 3) ? (T*)rb->y[s][f] : (T*)rb->nn[f];
@@ -1854,12 +1856,16 @@ int launch_rk4(atjup_ctx* c){
     return DISPATCH(do_rk4_dry, c);
 }
 int launch_pressure(atjup_ctx* c, int sweeps, double omega){ return DISPATCH(do_pressure, c, sweeps, omega); }
-int launch_prep_projection(atjup_ctx* c){
-    if(c->precision != ATJUP_PRECISION_FP64){ set_error("prepareProjectionBoundaries alone: FP64 only"); return ATJUP_ERR_PROFILE; }
-    hipLaunchKernelGGL(k_prep_projection<double>, dim3(nblk(c->km - 1, 128), c->jm - 2), dim3(128), 0, c->stream,
-        (double*)c->cur[ATJUP_F_AUX_U].p, (double*)c->cur[ATJUP_F_AUX_V].p, (double*)c->cur[ATJUP_F_AUX_W].p,
+template<typename T>
+int do_prep_projection(atjup_ctx* c){
+    hipLaunchKernelGGL(k_prep_projection<T>, dim3(nblk(c->km - 1, 128), c->jm - 2), dim3(128), 0, c->stream,
+        (T*)c->cur[ATJUP_F_AUX_U].p, (T*)c->cur[ATJUP_F_AUX_V].p, (T*)c->cur[ATJUP_F_AUX_W].p,
         c->im, c->jm, c->km, c->pitch, c->plane, 4.0 / 3.0, 1.0 / 3.0);
     return check_launch();
+}
+int launch_prep_projection(atjup_ctx* c){
+    if(c->precision == ATJUP_PRECISION_TANGENT){ set_error("prepareProjectionBoundaries alone: FP64 and FP32 forms"); return ATJUP_ERR_PROFILE; }
+    return (c->precision == ATJUP_PRECISION_FP64) ? do_prep_projection<double>(c) : do_prep_projection<float>(c);
 }
 int launch_bc_after_step(atjup_ctx* c){ return DISPATCH(do_bc, c); }
 int launch_shapiro(atjup_ctx* c, int field, int order, double strength, int passes, int axes){
@@ -1869,10 +1875,12 @@ int launch_mixture_density(atjup_ctx* c){ return DISPATCH(do_mixture, c); }
 int launch_clamp_species(atjup_ctx* c){ return DISPATCH(do_clamp, c); }
 int launch_mask_setup(atjup_ctx* c){ return DISPATCH(do_mask_setup, c); }
 int launch_rk4_dry_record(atjup_ctx* c, const RkBuffers& rb){
-    if(c->precision != ATJUP_PRECISION_FP64 || !c->kn.dry){ set_error("the adjoint covers the FP64 dynamics profile"); return ATJUP_ERR_PROFILE; }
+    if(!c->kn.dry || (c->precision != ATJUP_PRECISION_FP64 && c->precision != ATJUP_PRECISION_FP32_REF)){
+        set_error("the adjoint covers the dynamics profile in the FP64 and FP32-refined forms"); return ATJUP_ERR_PROFILE;
+    }
     if(int rc = launch_buoy_ref_level(c)) return rc;
     if(int rc = launch_hydrostatic(c)) return rc;
-    return do_rk4_dry<double>(c, &rb);
+    return (c->precision == ATJUP_PRECISION_FP64) ? do_rk4_dry<double>(c, &rb) : do_rk4_dry<float>(c, &rb);
 }
 int launch_check_finite(atjup_ctx* c){ return DISPATCH(do_check_finite, c); }
 

@@ -196,19 +196,23 @@ def cpu_baseline(config, sweeps, steps_o2=10, steps_o0=5):
     return out
 
 
-def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=None):
+def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=None, precision=None, ref=None):
     """The 20-step jet-misfit gradient on the dynamics profile: the time of one plain forward
-    (steps FP64 steps), of the checkpointed forward and of the reverse sweep, all between library
+    (steps steps), of the checkpointed forward and of the reverse sweep, all between library
     barriers with the adjoint seed already on the device; then dL/dw0 against central differences
-    (eps) along seeded directions."""
+    (eps) of the FP64 forward along seeded directions. precision: the form of the forward and of
+    its adjoint (FP64 by default; the FP32-refined form's adjoint keeps its fields in FP32). ref:
+    the FP64 leg's result, whose gradient and difference quotients (the FP64 reference gradient)
+    the FP32-refined leg is compared with instead of forming its own."""
     from atjup_amd import abi, inputs
     from atjup_amd.adjoint import MisfitRunner
     from atjup_amd.lib import Context
 
+    precision = abi.PRECISION_FP64 if precision is None else precision
     g = g or inputs.Grid()
     G, Cn, K, _ = abi.make_profile("dyn", sweeps=sweeps, omega=omega, grid=g)
     st = inputs.make_state("dyn", seed=11, g=g)
-    ctx = Context(G, Cn, K, abi.PRECISION_FP64)
+    ctx = Context(G, Cn, K, precision)
     ctx.load_state(st, skip=set(inputs.SPECIES))
     ctx.bc_after_step()
     ctx.swap_nlevel()
@@ -245,17 +249,21 @@ def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=N
     tm = ctx.read_timers()
     ctx.set_timing(False)
     grad = ctx.adj_get("w")
-    errs = []
-    for d_seed in range(directions):
+    errs, fds = [], []
+    for d_seed in range(directions if ref is None else len(ref["_fds"])):
         d = np.random.default_rng(1000 + d_seed).standard_normal(g.shape)
         d[:, :, -1] = d[:, :, 0]
-        fd = (run.evaluate(w0 + eps * d) - run.evaluate(w0 - eps * d)) / (2 * eps)
+        fd = ((run.evaluate(w0 + eps * d) - run.evaluate(w0 - eps * d)) / (2 * eps)) if ref is None else ref["_fds"][d_seed]
+        fds.append(fd)
         ad = float(np.sum(grad * d))
         errs.append(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
     ctx.close()
-    return {
-        "workload": f"jet misfit at i = {g.im - 2} after {steps} FP64 dynamics steps, {g.im}x{g.jm}x{g.km}, dL/dw0 "
-                    f"({g.im * g.jm * g.km} values), checkpoints every {every} steps",
+    form = "FP64" if precision == abi.PRECISION_FP64 else "FP32-refined"
+    out = {
+        "workload": f"jet misfit at i = {g.im - 2} after {steps} {form} dynamics steps, {g.im}x{g.jm}x{g.km}, dL/dw0 "
+                    f"({g.im * g.jm * g.km} values), checkpoints every {every} steps"
+                    + ("" if ref is None else ", the adjoint in FP32 (the solve's FP32 correction sweeps "
+                                              "transposed inside the FP64 refinement)"),
         "forward_ms": fwd_ms, "checkpointed_forward_ms": ckpt_ms, "backward_ms": bwd_ms,
         "backward_over_forward": bwd_ms / fwd_ms,
         "backward_phases_ms": {"recompute": tm["adj_recompute"][0], "adjoint_steps": tm["adj_step"][0],
@@ -266,7 +274,21 @@ def adjoint_leg(sweeps, omega, steps=20, every=5, directions=16, eps=1.0e-6, g=N
         "fd_max_rel_err": max(errs) if errs else None,
         "fd_median_rel_err": float(np.median(errs)) if errs else None,
         "within_1e-4": bool(max(errs) <= 1e-4) if errs else None,
+        "_grad": grad, "_fds": fds,
     }
+    if ref is not None:
+        # the FP32 gradient against the FP64 one (the FP64 leg's start is not rounded to FP32,
+        # so this includes the rounding of the initial state)
+        out["fd_of"] = "the FP64 forward (the FP64 leg's difference quotients)"
+        out["vs_fp64_gradient_normwise"] = float(np.linalg.norm(grad - ref["_grad"]) / np.linalg.norm(ref["_grad"]))
+        out["gradient_over_fp64_forward"] = (ckpt_ms + bwd_ms) / ref["forward_ms"]
+        out["gradient_speedup_vs_fp64"] = (ref["checkpointed_forward_ms"] + ref["backward_ms"]) / (ckpt_ms + bwd_ms)
+    return out
+
+
+def _public(leg):
+    """A leg's result without its private arrays."""
+    return {k: v for k, v in leg.items() if not k.startswith("_")} if leg else leg
 
 
 def semi_axes_leg(sweeps, omega, steps=20, every=5, rel_eps=1.0e-4, g=None, i_top=None):
@@ -463,6 +485,8 @@ def main():
                     help="standard: 41x181x361 (1 deg); fine: 81x361x721 (0.5 deg, no CPU baseline)")
     ap.add_argument("--headline-only", dest="headline_only", action="store_true",
                     help="time the headline FP64 step only (no FP32 form, adjoint, CPU baseline or component legs)")
+    ap.add_argument("--fp32-only", dest="fp32_only", action="store_true",
+                    help="time the FP32-refined step alone (for kernel traces); prints its own short JSON line")
     ap.add_argument("--adjoint-only", dest="adjoint_only", action="store_true",
                     help="time the adjoint leg alone, without difference quotients (for kernel traces)")
     ap.add_argument("--cpu-leg", dest="cpu_leg", default=None, help=argparse.SUPPRESS)
@@ -478,10 +502,13 @@ def main():
         args.precision_compare = False
         args.no_cpu = args.no_adjoint = args.no_restart = True
     if args.adjoint_only:
-        from atjup_amd import inputs
+        from atjup_amd import abi, inputs
         g = inputs.Grid() if args.grid == "standard" else inputs.fine_grid()
         kw = {"steps": 50, "every": 10} if args.grid == "fine" else {}
-        out = {"adjoint": adjoint_leg(args.sweeps, args.omega, g=g, directions=0, **kw)}
+        a64 = adjoint_leg(args.sweeps, args.omega, g=g, directions=0, **kw)
+        out = {"adjoint": _public(a64),
+               "adjoint_fp32": _public(adjoint_leg(args.sweeps, args.omega, g=g, directions=0,
+                                                   precision=abi.PRECISION_FP32_REF, ref=a64, **kw))}
         if args.grid == "standard":
             out["semi_axes"] = semi_axes_leg(args.sweeps, args.omega, g=g)
             out["semi_axes_level3"] = semi_axes_leg(args.sweeps, args.omega, g=g, i_top=3)
@@ -538,6 +565,15 @@ def main():
         wall = time.perf_counter() - t0
         return ctx, wall * 1e3, ctx.read_timers(), ctx.check_finite()
 
+    if args.fp32_only:
+        c32, ms32, tm32, ok32 = timed_run(abi.PRECISION_FP32_REF)
+        print(json.dumps({"workload": WORKLOAD[args.config] + f", FP32-refined, {g.im}x{g.jm}x{g.km}",
+                          "ms_per_step": ms32 / args.steps, "finite": bool(ok32),
+                          "phases_ms_per_step": {k: v[0] / args.steps for k, v in tm32.items()},
+                          "poisson_sweep_avg_launch_ms": tm32["poisson_sweeps"][0] / max(tm32["poisson_sweeps"][1], 1)}),
+              flush=True)
+        c32.close()
+        return
     ctx, ms, tm, ok = timed_run(abi.PRECISION_FP64)
     sweep_ms, sweep_launches = tm["poisson_sweeps"]
     rk_ms, rk_launches = tm["rk_stages"]
@@ -625,7 +661,12 @@ def main():
     adj = None
     if not args.no_adjoint:
         # the fine shell: 50 steps with checkpoints every 10, fewer difference quotients
-        adj = adjoint_leg(args.sweeps, args.omega, g=g, **({"steps": 50, "every": 10, "directions": 4} if fine else {}))
+        kw = {"steps": 50, "every": 10, "directions": 4} if fine else {}
+        adj = adjoint_leg(args.sweeps, args.omega, g=g, **kw)
+        # the FP32-refined forward and its FP32 adjoint on the same misfit, against the FP64 gradient
+        adj["fp32"] = _public(adjoint_leg(args.sweeps, args.omega, g=g, precision=abi.PRECISION_FP32_REF, ref=adj,
+                                          **kw))
+        adj = _public(adj)
         if not fine:
             # the benchmark's top-level misfit, and one at level 3, inside the drag region's height,
             # where the loss responds to the semi-axes well above the rounding of its difference quotients
