@@ -1254,6 +1254,10 @@ __global__ void ka_copy(T* __restrict__ d, const T* __restrict__ s, size_t n){
 
 }  // namespace atjup
 
+#ifndef ATJUP_RBT32_PAIRS
+#define ATJUP_RBT32_PAIRS 1
+#endif
+
 // =============================================================================================
 // host side
 // =============================================================================================
@@ -1510,13 +1514,15 @@ static int adj_pressure(atjup_ctx* c, int sweeps, double omega){
     // transposed sweeps flattened against 84 ms one cell; 81 x 361 x 721 (180 pairs) 1478 ms
     // one cell against 1592 ms row-aligned pairs and 1729 ms flattened.
     const int wf_pair = (nm / 2 + RBT_BM - 1) / RBT_BM;
-    const bool pairs = nm % 2 == 0 && (double)(nm / 2) / (RBT_BM * wf_pair) < 0.9;
+    const bool pairs64 = nm % 2 == 0 && (double)(nm / 2) / (RBT_BM * wf_pair) < 0.9;
     const int ntb = (c->jm * (nm / 2) + 255) / 256;
     // n sweeps transposed on the split adjoint la (its mu and source adjoint beside it), in the
     // sweeps' own precision; the forward's first sweep reads the stored planes unless all of them
     // form the planes (the refined form's correction sweeps)
     auto tsweeps = [&](auto* la0, auto* mua0, auto* lsa0, int n, bool all_ghost){
         using S = typename std::remove_pointer<decltype(la0)>::type;
+        // FP32: pairs wherever the ring is even (ATJUP_RBT32_PAIRS 0: the FP64 rule)
+        const bool pairs = pairs64 || (ATJUP_RBT32_PAIRS && std::is_same<S, float>::value && nm % 2 == 0);
         S* la1 = la0 + hn;
         S* mua1 = mua0 + hn;
         S* lsa1 = lsa0 + hn;
