@@ -1,4 +1,4 @@
-"""Gradients of the FP64 dynamics step through the hand-written adjoint, as a torch.autograd.Function.
+"""Gradients of the dynamics step through the hand-written adjoint, as a torch.autograd.Function.
 
 The loss of the adjoint benchmark: the cos(latitude)-weighted squared misfit of the zonal-mean
 zonal wind at the top integrated level against a target jet profile,
@@ -32,7 +32,8 @@ except Exception:  # pragma: no cover
 
 
 class MisfitRunner:
-    """One FP64 context holding the initial state, and the loss definition."""
+    """One context holding the initial state, and the loss definition. An FP64 context gives the
+    FP64 adjoint; an FP32-refined one the adjoint of its refined forward with FP32 adjoint fields."""
 
     def __init__(self, ctx, init_state: dict, target, nsteps=20, every=5, i_top=None, obstacle=None):
         self.ctx = ctx
@@ -119,7 +120,7 @@ def _coords(ctx):
 
 if torch is not None:
     class JetMisfit(torch.autograd.Function):
-        """L(w0[, axes]) with the adjoint as its backward (FP64, CPU tensors). axes: the drag
+        """L(w0[, axes]) with the adjoint as its backward (FP64 CPU tensors). axes: the drag
         region's (a_phi_deg, b_the_deg, c_r_km) as a tensor of 3, for a runner built with an
         obstacle; its gradient comes from the same reverse sweep."""
 

@@ -7,7 +7,8 @@ the configuration's scalar properties by their reference names (nm, L_atm, re, p
 t_ref, Coriolis, centrifugal, buoyancy, chemical_reaction, turb_model, ...), and the context-manager
 protocol that frees the model deterministically -- and adds what the reference binding has no way to
 do: field access (``get_field`` / ``set_field`` on the device-resident state), single iterations
-(``step``) and the hand-written adjoint of the FP64 dynamics step. ``JetMisfit`` is a
+(``step``) and the hand-written adjoint of the dynamics step (FP64, or FP32 in the FP32-refined form).
+``JetMisfit`` is a
 ``torch.autograd.Function`` over it: L(w0), the cos(latitude)-weighted squared misfit of the zonal-mean
 zonal wind at the top integrated level against a target jet profile after N iterations, whose backward
 is the checkpointed reverse sweep of the library.
@@ -302,7 +303,7 @@ cdef class Jupiter:
         self.generation += 1
         _err(atjup_hip_step(self._ctx, self.iteration, 1 if self.iteration == 1 else 0), "step")
 
-    # ---- the adjoint of the FP64 dynamics step -----------------------------------------------
+    # ---- the adjoint of the dynamics step (the model's precision) ------------------------------
     def adjoint_initial_state(self):
         """The state the misfit's iterations start from, as the reference's run starts (the
         boundary passes, restoreVar, the mixture density) from ``state``: the fields the adjoint
@@ -380,8 +381,8 @@ def jet_misfit(Jupiter jup, w0, target, int nsteps=20, int every=5, i_top=None, 
     misfit at level i_top (default im-2)."""
     im, jm, km = jup.shape
     i_top = im - 2 if i_top is None else int(i_top)
-    if jup.profile != "dyn" or jup.precision_name != "fp64":
-        raise AtjupError("the adjoint covers the FP64 dynamics profile: Jupiter(profile='dyn', precision='fp64')")
+    if jup.profile != "dyn":
+        raise AtjupError("the adjoint covers the dynamics profile: Jupiter(profile='dyn')")
     if axes is not None:
         jup.set_obstacle_axes(axes)
     for n, v in jup.adjoint_initial_state().items():

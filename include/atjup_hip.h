@@ -376,13 +376,17 @@ enum atjup_timer {
 int atjup_hip_set_timing(atjup_ctx* ctx, int enable);
 int atjup_hip_read_timers(atjup_ctx* ctx, double* out);
 
-/* ---- adjoint of the FP64 dynamics step (dry profile) ---------------------------------------
+/* ---- adjoint of the dynamics step (dry profile), FP64 and FP32-refined -----------------------
  * The hand-written transpose of the linearised step: boundary passes, Shapiro on p_dyn, mixture
  * density, RK4 with its right-hand side, the hydrostatic integral and level means, and the
  * red-black pressure solve, branches frozen at the forward's choice. Adjoint fields are
- * addressed by the forward field ids (t, u, v, w, p_dyn, p_hydro, rho_mix, aux_*). The reference
- * has no adjoint: these entry points have no counterpart there and serve the gradient of a
- * zonal-wind misfit at the top level with respect to the initial state. */
+ * addressed by the forward field ids (t, u, v, w, p_dyn, p_hydro, rho_mix, aux_*) and held in the
+ * context's precision: an ATJUP_PRECISION_FP32_REF context transposes what its refined forward
+ * runs, with its adjoint fields in FP32 (the first sweep of each solve transposed in FP64, the
+ * rounds of FP32 correction sweeps transposed in FP32 inside the FP64 residual and correction
+ * transposes; checkpoints carry the paired fields' low words). Host arrays stay FP64. The
+ * reference has no adjoint: these entry points have no counterpart there and serve the gradient
+ * of a zonal-wind misfit at the top level with respect to the initial state. */
 #define ATJUP_ADJ_OP_BC            1   /* bc_after_step^T                                      */
 #define ATJUP_ADJ_OP_SHAPIRO_PDYN  2   /* shapiro(p_dyn, 2, 1, 1, 7)^T                         */
 #define ATJUP_ADJ_OP_MIXTURE       3   /* mixture_density^T                                    */
@@ -427,7 +431,7 @@ typedef struct atjup_obstacle {
 /* Set (params) or remove (NULL) the drag region. */
 int atjup_hip_set_smooth_obstacle(atjup_ctx* ctx, const atjup_obstacle* params);
 /* dL/d(a_phi_deg, b_the_deg, c_r_km) of the loss whose final-state adjoint the last
- * atjup_hip_adj_backward carried back (FP64 dynamics profile with the drag region set); out[3]. */
+ * atjup_hip_adj_backward carried back (dynamics profile with the drag region set); out[3]. */
 int atjup_hip_adj_obstacle_gradient(atjup_ctx* ctx, double* out);
 
 /* ---- housekeeping ----------------------------------------------------------------------- */
