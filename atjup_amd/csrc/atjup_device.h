@@ -2,6 +2,8 @@
 #pragma once
 #include "atjup_internal.h"
 
+#include <type_traits>
+
 // 1: the sweep sequence of a pressure solve is captured once into a HIP graph and replayed
 #ifndef ATJUP_SWEEP_GRAPH
 #define ATJUP_SWEEP_GRAPH 1
@@ -12,9 +14,18 @@ namespace atjup {
 // The FP32-refined form carries t, u, v, w and the species as FP32 pairs hi + lo; lo is null for
 // the other forms and fields. A value written through pair_put is split so that hi + lo holds it
 // to about 2^-48 of its size.
+// In the FP32 storage the low word is loaded without a branch (from the value's own array where
+// the field has none, and then not added): a load under a branch makes the wavefront wait at the
+// branch's end for every load it has in flight, which serialises a kernel's loads field by field.
 template<typename T>
 __device__ __forceinline__ double pair_get(const T* hi, const float* lo, size_t q){
-    return lo ? (double)hi[q] + (double)lo[q] : (double)hi[q];
+    if constexpr(std::is_same<T, float>::value){
+        const float h = hi[q];
+        const float l = (lo ? lo : hi)[q];
+        return (double)h + (lo ? (double)l : 0.0);
+    } else {
+        return lo ? (double)hi[q] + (double)lo[q] : (double)hi[q];
+    }
 }
 template<typename T>
 __device__ __forceinline__ void pair_put(T* hi, float* lo, size_t q, double v){

@@ -282,11 +282,16 @@ __global__ __launch_bounds__(ST_BK * ST_BJ) void k_stage_dry(const StageArgs<T> 
         for(int f = 0; f < 4; f++){
             const T accf = a.acc[f][q] + a.wgt * rhs[f];
             if constexpr(std::is_same<T, float>::value){
+                // the low word loaded without a branch (from the n level where there is none), so
+                // that the wavefront does not wait at the branch for its loads in flight
+                const T nlv = a.nl[f][q];
+                const float lov = (a.lo[f] ? a.lo[f] : (const float*)a.nl[f])[q];
                 if(a.lo[f]){
-                    split_store(a.out[f], a.lo[f], q,
-                                ((double)a.nl[f][q] + (double)a.lo[f][q]) + a.dt_d * (double)accf * (1.0 / 6.0));
+                    split_store(a.out[f], a.lo[f], q, ((double)nlv + (double)lov) + a.dt_d * (double)accf * (1.0 / 6.0));
                     continue;
                 }
+                a.out[f][q] = nlv + a.dt * accf * a.one_sixth;
+                continue;
             }
             a.out[f][q] = a.nl[f][q] + a.dt * accf * a.one_sixth;
         }

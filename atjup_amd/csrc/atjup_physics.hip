@@ -89,7 +89,20 @@ __device__ __forceinline__ Stencil7<T> load7(const T* __restrict__ X, size_t q, 
 // what the species RHS and stage update of one species read at a cell, loaded together so that a
 // group of species can have all its loads in flight before its first store
 template<typename T>
-struct SpeciesLoads { Stencil7<T> s; T mf, spr, fl, acc, nl; };
+struct SpeciesLoads { Stencil7<T> s; T mf, spr, fl, acc, nl; float lo; };
+
+// The low word of a paired field for the last stage's FP64 update (FP32 storage): loaded without a
+// branch, from the n level itself in the other stages and for unpaired fields (a value the stage
+// reads anyway), so that the wavefront does not wait at a branch for its loads in flight.
+template<typename T>
+__device__ __forceinline__ float stage_lo(const StageFullArgs<T>& a, int f, size_t q){
+    if constexpr(std::is_same<T, float>::value){
+        const float* lp = (a.stage == 3 && a.lo[f]) ? a.lo[f] : (const float*)a.nl[f];
+        return lp[q];
+    } else {
+        return 0.0f;
+    }
+}
 
 template<typename T>
 __device__ __forceinline__ SpeciesLoads<T> species_load(const StageFullArgs<T>& a, int f, size_t q, size_t P,
@@ -101,6 +114,7 @@ __device__ __forceinline__ SpeciesLoads<T> species_load(const StageFullArgs<T>& 
     L.fl = (f == ATJUP_F_NH4SH) ? a.fl_nh4sh[q] : T(0);
     L.acc = (a.stage == 0) ? T(0) : a.acc[f][q];
     L.nl = a.nl[f][q];
+    L.lo = stage_lo(a, f, q);
     return L;
 }
 
@@ -135,7 +149,7 @@ __device__ __forceinline__ T species_rhs(const StageFullArgs<T>& a, int f, const
 // 4: omega* (max(dis_min, .))
 template<typename T>
 __device__ __forceinline__ void stage_update_v(const StageFullArgs<T>& a, int f, size_t q, T rhs, int kind,
-        T acc_in, T nl_v){
+        T acc_in, T nl_v, float lo_v){
     T accf;
     if(a.stage == 0) accf = a.wgt * rhs;
     else accf = acc_in + a.wgt * rhs;
@@ -150,7 +164,7 @@ __device__ __forceinline__ void stage_update_v(const StageFullArgs<T>& a, int f,
             // the paired fields of the refined form (t, u, v, w, the species): the increment added
             // to hi + lo in FP64, the species' clamps applied to that sum, and the sum split again
             if(a.lo[f]){
-                double s = ((double)nl_v + (double)a.lo[f][q]) + a.dt_d * (double)accf * (1.0 / 6.0);
+                double s = ((double)nl_v + (double)lo_v) + a.dt_d * (double)accf * (1.0 / 6.0);
                 if(kind == 1) s = dmax(0.0, s);
                 else if(kind == 2){ if(s < 0.0) s = 0.0; }
                 const float h = (float)s;
@@ -170,7 +184,7 @@ __device__ __forceinline__ void stage_update_v(const StageFullArgs<T>& a, int f,
 
 template<typename T>
 __device__ __forceinline__ void stage_update(const StageFullArgs<T>& a, int f, size_t q, T rhs, int kind){
-    stage_update_v(a, f, q, rhs, kind, (a.stage == 0) ? T(0) : a.acc[f][q], a.nl[f][q]);
+    stage_update_v(a, f, q, rhs, kind, (a.stage == 0) ? T(0) : a.acc[f][q], a.nl[f][q], stage_lo(a, f, q));
 }
 
 // The eleven species (RHS_Jup_Turb.cpp:1273-1338) in a kernel of their own: they need only the
@@ -221,7 +235,7 @@ __global__ __launch_bounds__(SF_BK * SF_BJ) void k_stage_species(const StageFull
             const int f = f0 + g;
             if(f > ATJUP_F_NH4SH) break;
             const T rhs = species_rhs(a, f, L[g], G, u, v_invrm, w_invrs, two_inv_rm, nue_t_s, rho_f);
-            stage_update_v(a, f, q, rhs, f == ATJUP_F_NH4SH ? 2 : 1, L[g].acc, L[g].nl);
+            stage_update_v(a, f, q, rhs, f == ATJUP_F_NH4SH ? 2 : 1, L[g].acc, L[g].nl, L[g].lo);
         }
     }
 }
@@ -1544,8 +1558,7 @@ __global__ __launch_bounds__(RAD_BK) void k_rad_layers(const RadFields<T> F, con
     const int i_base = max(itopo[(size_t)j * km + k], 0), i_top = im - 1;
     if(i_base >= i_top || i < i_base) return;
     const size_t q = ((size_t)i * jm + j) * pitch + k;
-    double tq = (double)F.t[q];
-    if(F.lo_t) tq += (double)F.lo_t[q];
+    double tq = pair_get(F.t, F.lo_t, q);
     const double Tk = tq * C.t_ref;
     const double B = RAD_SIGMA * Tk * Tk * Tk * Tk;
     const double P_lo = (double)F.p_stat[q] * RAD_P0;
@@ -1700,24 +1713,44 @@ constexpr int PR_BK = 64;                                             // columns
 
 struct PrecipIn { double tq, p_u, rf, q_v, q_c, q_i; };
 
+// A level's inputs as loaded (the storage type and, in the FP32-refined form, the pairs' low words),
+// widened only when the level is worked on: the next level's loads stay in flight meanwhile.
 template<typename T>
-__device__ __forceinline__ PrecipIn precip_load(const PrecipFields<T>& F, int g, size_t q){
-    PrecipIn r;
-    r.tq = (double)F.t[q];
-    if(F.lo_t) r.tq += (double)F.lo_t[q];
-    r.p_u = (double)F.p_stat[q];
-    r.rf = (double)F.rho[q];
-    r.q_v = pair_get(F.vap[g], F.lo_vap[g], q);
-    r.q_c = pair_get(F.cld[g], F.lo_cld[g], q);
-    r.q_i = pair_get(F.ice[g], F.lo_ice[g], q);
+struct PrecipRaw { T t, p, rho, v, c, i; float lt, lv, lc, li; };
+
+template<typename T>
+__device__ __forceinline__ PrecipRaw<T> precip_load(const PrecipFields<T>& F, int g, size_t q){
+    PrecipRaw<T> r;
+    r.t = F.t[q]; r.p = F.p_stat[q]; r.rho = F.rho[q];
+    r.v = F.vap[g][q]; r.c = F.cld[g][q]; r.i = F.ice[g][q];
+    if constexpr(std::is_same<T, float>::value){
+        // the low words without a branch (from the value's own array where a field has none)
+        r.lt = (F.lo_t ? F.lo_t : F.t)[q];
+        r.lv = (F.lo_vap[g] ? F.lo_vap[g] : F.vap[g])[q];
+        r.lc = (F.lo_cld[g] ? F.lo_cld[g] : F.cld[g])[q];
+        r.li = (F.lo_ice[g] ? F.lo_ice[g] : F.ice[g])[q];
+    } else {
+        r.lt = r.lv = r.lc = r.li = 0.0f;
+    }
     return r;
+}
+
+template<typename T>
+__device__ __forceinline__ PrecipIn precip_widen(const PrecipFields<T>& F, int g, const PrecipRaw<T>& r){
+    PrecipIn w;
+    w.tq = F.lo_t ? (double)r.t + (double)r.lt : (double)r.t;
+    w.p_u = (double)r.p;
+    w.rf = (double)r.rho;
+    w.q_v = F.lo_vap[g] ? (double)r.v + (double)r.lv : (double)r.v;
+    w.q_c = F.lo_cld[g] ? (double)r.c + (double)r.lc : (double)r.c;
+    w.q_i = F.lo_ice[g] ? (double)r.i + (double)r.li : (double)r.i;
+    return w;
 }
 
 // the NH4SH settling flux of one cell (sedimentNH4SH, Precipitation.h:540-556)
 template<typename T>
 __device__ __forceinline__ T nh4sh_flux(const PrecipFields<T>& F, size_t q, const PrecipConsts& C){
-    double tq = (double)F.t[q];
-    if(F.lo_t) tq += (double)F.lo_t[q];
+    double tq = pair_get(F.t, F.lo_t, q);
     const double Tk = tq * C.t_ref;
     const double qn = dmax(0.0, (double)F.nh4sh[q]);
     return (T)((Tk >= C.t_lo_nh4sh && Tk <= C.t_hi_nh4sh) ? qn * C.v_settle : 0.0);
@@ -1740,10 +1773,10 @@ __global__ __launch_bounds__(3 * PR_BK) __attribute__((amdgpu_waves_per_eu(3))) 
     const double Lf = G.Ls - G.Lv;
     T* const Pr = F.P[g][0]; T* const Ps = F.P[g][1]; T* const Pg = F.P[g][2];
     double Fr = 0.0, Fs = 0.0, Fg = 0.0;
-    PrecipIn nxt = precip_load(F, g, q0 + (size_t)i_top * P);
+    PrecipRaw<T> nxt = precip_load(F, g, q0 + (size_t)i_top * P);
     for(int i = i_top; i >= 0; i--){
         const size_t q = q0 + (size_t)i * P;
-        const PrecipIn in = nxt;
+        const PrecipIn in = precip_widen(F, g, nxt);
         if(i > 0) nxt = precip_load(F, g, q - P);
         double Qp = 0.0, Sv = 0.0, Sc = 0.0, Si = 0.0;
         if(valid && live){
@@ -1922,8 +1955,7 @@ __global__ __launch_bounds__(64) void k_diffmassflux(const ChemFields<T> F, cons
     const size_t row = q - k;
     const int kw = (k == 0) ? km - 2 : k - 1, ke = (k == km - 2) ? 0 : k + 1;
     const uint8_t fl = flags[q];
-    double tq = (double)F.t[q];
-    if(F.lo_t) tq += (double)F.lo_t[q];
+    double tq = pair_get(F.t, F.lo_t, q);
     if(!(tq > 0.0) || (fl & FL_SELF)){
         F.dfl_h2s[q] = T(0); F.dfl_nh3[q] = T(0); F.dfl_nh4sh[q] = T(0); F.tmf[q] = T(0);
         return;
@@ -2020,8 +2052,7 @@ __global__ __launch_bounds__(64) void k_chemmassrate(const ChemFields<T> F, cons
         F.mf_nh3[q] = T(0); F.mf_h2s[q] = T(0); F.mf_nh4sh[q] = T(0);
         return;
     }
-    double tq = (double)F.t[q];
-    if(F.lo_t) tq += (double)F.lo_t[q];
+    double tq = pair_get(F.t, F.lo_t, q);
     const double t_u = tq * C.t_ref;
     const double kf = 15000.0 * pow(t_u, -0.5) * exp(-3020.0 / t_u);        // react_rate_const (:648-651)
     const double kb = kf / C.keq;
@@ -2185,8 +2216,7 @@ struct McaColumn {
     __device__ uint8_t& FL(int i) const { return wf[(size_t)i * n2 + col]; }
     __device__ double Tk(int i) const {
         const size_t q = q0 + (size_t)i * P;
-        double tq = (double)F.t[q];
-        if(F.lo_t) tq += (double)F.lo_t[q];
+        double tq = pair_get(F.t, F.lo_t, q);
         return tq * K.t_ref;
     }
     __device__ double pr(int i) const { return (double)F.p_stat[q0 + (size_t)i * P]; }
