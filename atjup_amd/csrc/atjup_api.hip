@@ -245,6 +245,15 @@ int step_block(atjup_ctx* c, int iter, int first){
             if((rc = launch_shapiro(c, ATJUP_F_FLUXLIM_NH4SH, 2, 1.0, 1, 7))) return rc;
         }
     }
+    // steadyQuery's n level of t and the condensables (NH4SH with the equilibrium) is the state
+    // RK_SCALAR_SYNC 2 copies after the adjustments (cJupiterModel.cpp:1118-1140); the closure and
+    // the chemistry after it change none of them
+    if(c->steady_on){
+        unsigned m = 0;
+        for(int e = 4; e <= 10; e++) m |= 1u << e;           // t, h2o, h2o_cloud, h2o_ice, nh3, nh3_cloud, nh3_ice
+        if(c->kn.nh4sh_eq) m |= 1u << 11;
+        if((rc = steady_mark(c, m))) return rc;
+    }
     return 0;
 }
 
@@ -430,6 +439,7 @@ void atjup_hip_ctx_destroy(atjup_ctx* c){
     for(auto& b : c->acc) release(b);
     for(auto& b : c->live_turb) release(b);
     for(auto& b : c->lo) release(b);
+    release(c->steady);
     release(c->colr);
     for(atjup::Buf* b : { &c->scratch, &c->psrc, &c->rb_p, &c->rb_d, &c->rb_r, &c->rb_flags, &c->mask, &c->flags, &c->itopo, &c->vel_star, &c->geo,
                           &c->pois, &c->pois32, &c->tabs, &c->brl, &c->brl_part, &c->zm, &c->solids,
@@ -855,10 +865,21 @@ int atjup_hip_scalar_sync(atjup_ctx* c){ return c ? 0 : ATJUP_ERR_ARG; }
 int atjup_hip_step(atjup_ctx* c, int iter, int first){
     if(int rc = need_mask(c)) return rc;
     hipEvent_t t_step = timer_begin(c);
-    if(int rc = step_block(c, iter, first)) return rc;
+    if(int rc = atjup_hip_step_block(c, iter, first)) return rc;
     if(int rc = step_rest(c, iter)) return rc;
     timer_end(c, ATJUP_T_STEP, t_step, 0);
     return 0;
+}
+int atjup_hip_step_block(atjup_ctx* c, int iter, int first){
+    if(int rc = need_mask(c)) return rc;
+    // the iteration starts: steadyQuery's n level is the state now (restoreVar of the last one)
+    if(c->steady_on)
+        if(int rc = steady_mark(c, (1u << ATJUP_N_STEADY) - 1)) return rc;
+    return step_block(c, iter, first);
+}
+int atjup_hip_step_rest(atjup_ctx* c, int iter){
+    if(int rc = need_mask(c)) return rc;
+    return step_rest(c, iter);
 }
 
 int atjup_hip_check_finite(atjup_ctx* c){ if(!c) return ATJUP_ERR_ARG; return launch_check_finite(c); }

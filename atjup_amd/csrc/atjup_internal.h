@@ -100,6 +100,8 @@ struct atjup_ctx {
     atjup::Buf lo[ATJUP_N_PROG]; // FP32-refined form: low words of t, u, v, w and, in the moist
                                  // profiles, the eleven species (the value is hi + lo)
     atjup::Buf colr;             // int2 (i0, i1) per (j, k): the column's fluid run from the bottom
+    atjup::Buf steady;           // steadyQuery tracking: 12 FP64 n-level copies [12][n3], then 12 results
+    bool steady_on = false;
     int rb_hp = 0;               // compact row length: (km + 1) / 2 rounded up to 128 bytes
     atjup::Buf mask;             // uint8 SeaMount [n3]
     atjup::Buf flags;            // uint8 face flags [n3]
@@ -183,7 +185,9 @@ int launch_precipitation(atjup_ctx* c);
 int launch_chem_kernel(atjup_ctx* c, int which);      // 0: DiffMassFluxJup, 1: ChemMassRateJup
 int launch_mca(atjup_ctx* c);
 int launch_nh4sh_eq(atjup_ctx* c);
-size_t report_bytes(int im, int jm, int km);          // the size of atjup_ctx::report
+size_t report_bytes(int im, int jm, int km);
+// steadyQuery's n-level copies (bit e: entry e of the report's field order) into the tracking buffer
+int steady_mark(atjup_ctx* c, unsigned entries);          // the size of atjup_ctx::report
 int build_rad_table(atjup_ctx* c);
 int launch_flux_limiter(atjup_ctx* c);
 int launch_commit_live(atjup_ctx* c);

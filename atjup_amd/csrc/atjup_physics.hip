@@ -147,6 +147,18 @@ __device__ __forceinline__ T species_rhs(const StageFullArgs<T>& a, int f, const
 // the stage update of one field (RungeKutta_Jup_Turb.cpp:420-568); kind 0: plain, 1: species
 // (max(0, .) at the end), 2: nh4sh (< 0 -> 0 at the end), 3: k* (safe_clamp to [0, tke_max]),
 // 4: omega* (max(dis_min, .))
+// a stage kernel's stores of the accumulator and the stage output: non-temporal (streamed past the
+// caches; the next stage reads them after the whole grid has gone through), 1-2% less stage time
+// on MI355X (ATJUP_NT_STAGE 0: plain stores)
+#ifndef ATJUP_NT_STAGE
+#define ATJUP_NT_STAGE 1
+#endif
+template<typename T>
+__device__ __forceinline__ void st_field(T* p, T v){
+    if constexpr(ATJUP_NT_STAGE) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 template<typename T>
 __device__ __forceinline__ void stage_update_v(const StageFullArgs<T>& a, int f, size_t q, T rhs, int kind,
         T acc_in, T nl_v, float lo_v){
@@ -154,11 +166,11 @@ __device__ __forceinline__ void stage_update_v(const StageFullArgs<T>& a, int f,
     if(a.stage == 0) accf = a.wgt * rhs;
     else accf = acc_in + a.wgt * rhs;
     if(a.stage < 3){
-        a.acc[f][q] = accf;
+        st_field(&a.acc[f][q], accf);
         T v = nl_v + a.c_in * rhs;
         if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
         else if(kind == 4) v = dmax(a.dis_min, v);
-        a.out[f][q] = v;
+        st_field(&a.out[f][q], v);
     } else {
         if constexpr(std::is_same<T, float>::value){
             // the paired fields of the refined form (t, u, v, w, the species): the increment added
@@ -168,7 +180,7 @@ __device__ __forceinline__ void stage_update_v(const StageFullArgs<T>& a, int f,
                 if(kind == 1) s = dmax(0.0, s);
                 else if(kind == 2){ if(s < 0.0) s = 0.0; }
                 const float h = (float)s;
-                a.out[f][q] = h;
+                st_field(&a.out[f][q], h);
                 a.lo[f][q] = (float)(s - (double)h);
                 return;
             }
@@ -178,7 +190,7 @@ __device__ __forceinline__ void stage_update_v(const StageFullArgs<T>& a, int f,
         else if(kind == 2){ if(v < T(0)) v = T(0); }
         else if(kind == 3) v = safe_clamp_d(v, T(0), a.tke_max_nd);
         else if(kind == 4) v = dmax(a.dis_min, v);
-        a.out[f][q] = v;
+        st_field(&a.out[f][q], v);
     }
 }
 

@@ -148,6 +148,40 @@ __global__ __launch_bounds__(RP_BK) void k_reduce_parts(const Part* __restrict__
     }
 }
 
+// steadyQuery's per-field change (Reporting.h:399-410): |f - f_n| over every cell, the last cell of
+// the i, j, k loops on ties (>= from 0), position (i * jm + j) * km + k; NaNs never taken
+template<typename T>
+__global__ __launch_bounds__(RP_BK) void k_steady(const T* __restrict__ f, const float* __restrict__ lo,
+        const double* __restrict__ fn, int im, int jm, int km, int pitch, Part* __restrict__ parts){
+    Cand best{ -1.0, -1 }, unused = no_min();
+    const size_t plane = (size_t)jm * pitch, n = (size_t)im * plane;
+    for(size_t q = (size_t)blockIdx.x * RP_BK + threadIdx.x; q < n; q += (size_t)gridDim.x * RP_BK){
+        const int i = (int)(q / plane);
+        const size_t r = q - (size_t)i * plane;
+        const int j = (int)(r / pitch), k = (int)(r - (size_t)j * pitch);
+        if(k >= km) continue;
+        double v = (double)f[q];
+        if(lo) v += (double)lo[q];
+        const double d = fabs(v - fn[q]);
+        if(!(d >= 0.0)) continue;
+        const Cand c{ d, ((long long)i * jm + j) * km + k };
+        if(better_last(c, best)) best = c;
+    }
+    block_reduce<1>(best, unused);
+    if(threadIdx.x == 0) parts[blockIdx.x] = Part{ best, unused };
+}
+
+// a field's value (hi + lo in the refined form) as the n-level copy of the report
+template<typename T>
+__global__ __launch_bounds__(256) void k_snap(const T* __restrict__ f, const float* __restrict__ lo,
+        double* __restrict__ out, size_t n){
+    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if(q >= n) return;
+    double v = (double)f[q];
+    if(lo) v += (double)lo[q];
+    out[q] = v;
+}
+
 template<typename T>
 __global__ void k_first(const T* __restrict__ f, const float* __restrict__ lo, double* __restrict__ out){
     double v = (double)f[0];
@@ -207,6 +241,37 @@ int do_slice(atjup_ctx* c, int field, int axis, int index, int n){
     return check_launch();
 }
 
+// the report's field order (Reporting.h:356-369)
+const int kSteadyFields[ATJUP_N_STEADY] = { ATJUP_F_P_DYN, ATJUP_F_U, ATJUP_F_V, ATJUP_F_W, ATJUP_F_T, ATJUP_F_H2O,
+    ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE, ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE, ATJUP_F_NH4SH };
+double* steady_copy(atjup_ctx* c, int e){ return (double*)c->steady.p + (size_t)e * c->n3; }
+Part* steady_results(atjup_ctx* c){ return (Part*)(steady_copy(c, ATJUP_N_STEADY)); }
+
+template<typename T>
+int do_steady_mark(atjup_ctx* c, unsigned entries){
+    for(int e = 0; e < ATJUP_N_STEADY; e++){
+        if(!(entries & (1u << e))) continue;
+        const int f = kSteadyFields[e];
+        const float* lo = f < ATJUP_N_PROG ? (const float*)c->lo[f].p : nullptr;
+        hipLaunchKernelGGL(k_snap<T>, dim3(nblk(c->n3, 256)), dim3(256), 0, c->stream, (const T*)c->cur[f].p, lo,
+            steady_copy(c, e), c->n3);
+    }
+    return check_launch();
+}
+
+template<typename T>
+int do_steady(atjup_ctx* c){
+    for(int e = 0; e < ATJUP_N_STEADY; e++){
+        const int f = kSteadyFields[e];
+        const float* lo = f < ATJUP_N_PROG ? (const float*)c->lo[f].p : nullptr;
+        hipLaunchKernelGGL(k_steady<T>, dim3(RP_BLOCKS), dim3(RP_BK), 0, c->stream, (const T*)c->cur[f].p, lo,
+            (const double*)steady_copy(c, e), c->im, c->jm, c->km, c->pitch, parts_of(c));
+        hipLaunchKernelGGL(k_reduce_parts<1>, dim3(1), dim3(RP_BK), 0, c->stream, (const Part*)parts_of(c), RP_BLOCKS,
+            (const double*)nullptr, steady_results(c) + e);
+    }
+    return check_launch();
+}
+
 #define RDISPATCH(fn, ...) ((c->precision == ATJUP_PRECISION_FP32_REF) ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
 
 int fetch_result(atjup_ctx* c, Part& r){
@@ -220,6 +285,11 @@ int fetch_result(atjup_ctx* c, Part& r){
 size_t report_bytes(int im, int jm, int km){
     const size_t slice = std::max((size_t)jm * km, std::max((size_t)im * km, (size_t)im * jm));
     return (RP_BLOCKS + 1) * sizeof(Part) + 2 * sizeof(double) + slice * sizeof(double);
+}
+
+int steady_mark(atjup_ctx* c, unsigned entries){
+    if(!c->steady_on || !c->steady.p) return 0;
+    return RDISPATCH(do_steady_mark, c, entries);
 }
 
 // ---- the allocation-free search (atjup_hip_search_minmax_3d_noalloc) ---------------------------
@@ -330,6 +400,36 @@ int atjup_hip_search_minmax_2d(atjup_ctx* c, int map, double* out, int* loc){
         const long long p = o[s] < 0 ? 0 : o[s];
         loc[2 * s + 0] = (int)(p / c->km);
         loc[2 * s + 1] = (int)(p % c->km);
+    }
+    return 0;
+}
+
+int atjup_hip_steady_track(atjup_ctx* c, int on){
+    if(!c){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(c->precision == ATJUP_PRECISION_TANGENT){ set_error("the tangent form has no reporting"); return ATJUP_ERR_PROFILE; }
+    if(on && !c->steady.p){
+        const size_t bytes = (size_t)ATJUP_N_STEADY * c->n3 * sizeof(double) + ATJUP_N_STEADY * sizeof(Part);
+        if(hipMalloc(&c->steady.p, bytes) != hipSuccess){ c->steady.p = nullptr; set_error("hipMalloc failed (steadyQuery tracking)"); return ATJUP_ERR_NOMEM; }
+        c->steady.bytes = bytes;
+    }
+    c->steady_on = on != 0;
+    // the n level as it stands: the copies a query before the next iteration differences against
+    return on ? steady_mark(c, (1u << ATJUP_N_STEADY) - 1) : 0;
+}
+
+int atjup_hip_steady_changes(atjup_ctx* c, double* values, int* cells){
+    if(!c || !values || !cells){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(!c->steady_on || !c->steady.p){ set_error("steadyQuery tracking is off (atjup_hip_steady_track)"); return ATJUP_ERR_STATE; }
+    if(int rc = RDISPATCH(do_steady, c)) return rc;
+    Part r[ATJUP_N_STEADY];
+    ATJUP_HIP_CHECK(hipMemcpyAsync(r, steady_results(c), sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    ATJUP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for(int e = 0; e < ATJUP_N_STEADY; e++){
+        const long long o = r[e].mx.o < 0 ? 0 : r[e].mx.o;   // position (i * jm + j) * km + k
+        values[e] = r[e].mx.o < 0 ? 0.0 : r[e].mx.v;
+        cells[3 * e + 2] = (int)(o % c->km);
+        cells[3 * e + 1] = (int)((o / c->km) % c->jm);
+        cells[3 * e + 0] = (int)(o / ((long long)c->km * c->jm));
     }
     return 0;
 }

@@ -77,6 +77,10 @@ def load():
     for n in ("atjup_hip_search_minmax_3d", "atjup_hip_search_minmax_2d"):
         getattr(L, n).argtypes = [C.c_void_p, C.c_int, _D, C.POINTER(C.c_int)]
     L.atjup_hip_continuity_residual.argtypes = [C.c_void_p, _D, C.POINTER(C.c_int)]
+    L.atjup_hip_steady_track.argtypes = [C.c_void_p, C.c_int]
+    L.atjup_hip_steady_changes.argtypes = [C.c_void_p, _D, C.POINTER(C.c_int)]
+    L.atjup_hip_step_block.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.atjup_hip_step_rest.argtypes = [C.c_void_p, C.c_int]
     L.atjup_hip_search_minmax_3d_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                                      C.c_void_p, _D, C.POINTER(C.c_int)]
     L.atjup_hip_download_slice.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _D]
@@ -257,6 +261,18 @@ class Context:
         self._chk(self.L.atjup_hip_continuity_residual(self.h, C.byref(v), loc), "continuity_residual")
         return v.value, tuple(loc)
 
+    STEADY_FIELDS = ("p_dyn", "u", "v", "w", "t", "h2o", "h2o_cloud", "h2o_ice", "nh3", "nh3_cloud", "nh3_ice", "nh4sh")
+
+    def steady_track(self, on=True):
+        """Keep (or stop keeping) the n-level copies steadyQuery's per-field changes difference against."""
+        self._chk(self.L.atjup_hip_steady_track(self.h, 1 if on else 0), "steady_track")
+
+    def steady_changes(self):
+        """steadyQuery's per-field changes: [(name, max |f - f_n|, (i, j, k))] in the report's order."""
+        vals, cells = (C.c_double * 12)(), (C.c_int * 36)()
+        self._chk(self.L.atjup_hip_steady_changes(self.h, vals, cells), "steady_changes")
+        return [(n, vals[e], tuple(cells[3 * e:3 * e + 3])) for e, n in enumerate(self.STEADY_FIELDS)]
+
     def slice(self, name, axis, index):
         im, jm, km = self.shape
         shape = [(jm, km), (im, km), (im, jm)][axis]
@@ -304,6 +320,13 @@ class Context:
 
     def step(self, iter_n, first=False):
         self._chk(self.L.atjup_hip_step(self.h, iter_n, 1 if first else 0), "step")
+
+    def step_block(self, iter_n, first=False):
+        """The iteration's physics block (step = step_block + step_rest)."""
+        self._chk(self.L.atjup_hip_step_block(self.h, iter_n, 1 if first else 0), "step_block")
+
+    def step_rest(self, iter_n):
+        self._chk(self.L.atjup_hip_step_rest(self.h, iter_n), "step_rest")
 
     def check_finite(self):
         return self._chk(self.L.atjup_hip_check_finite(self.h), "check_finite") == 0

@@ -296,10 +296,22 @@ int atjup_hip_search_minmax_2d(atjup_ctx* ctx, int map, double* out, int* loc);
 /* The continuity residual of steadyQuery (planet/Reporting.h:372-398): the largest |du/dr + 2u/r +
  * dv/dtheta / r + cos(theta) v / (r sin(theta)) + dw/dphi / (r sin(theta))| over the interior and its
  * cell, the last cell of the i, j, k loops on ties (ATJUP_COSTHE_ABS 0). With no finite residual the
- * value is 0 and loc is {-1, -1, -1} (the reference keeps its previous cell). The per-field changes
- * of steadyQuery need the n-level copies, which this path commits in place (restoreVar is the
- * identity here), so they are not formed. */
+ * value is 0 and loc is {-1, -1, -1} (the reference keeps its previous cell). */
 int atjup_hip_continuity_residual(atjup_ctx* ctx, double* value, int* loc);
+/* steadyQuery's per-field changes (planet/Reporting.h:346-410): max |f - f_n| over every cell with
+ * its cell (the last of the i, j, k loops on ties, >= from 0; NaNs never taken), for the twelve
+ * fields of the report in its order: p_dyn, u, v, w, t, h2o, h2o_cloud, h2o_ice, nh3, nh3_cloud,
+ * nh3_ice, nh4sh. This path commits the n level in place (restoreVar is the identity here), so
+ * with tracking on the library keeps the n-level copies the report differences against: the
+ * state at the start of each iteration, and for t and the six condensables (NH4SH with the
+ * equilibrium on) the state RK_SCALAR_SYNC copies after the physics block's adjustments
+ * (planet/cJupiterModel.cpp:1118-1140), as the reference's f_n hold them when steadyQuery runs
+ * before restoreVar (:1378). Tracking costs twelve FP64 fields of device memory and their copies
+ * per iteration. values[12], cells[12][3] = {i, j, k}; with no finite change the entry is 0 at
+ * (0, 0, 0), the reference's starting values. */
+#define ATJUP_N_STEADY 12
+int atjup_hip_steady_track(atjup_ctx* ctx, int on);
+int atjup_hip_steady_changes(atjup_ctx* ctx, double* values, int* cells);
 /* One plane of a field for the writers (planet/ParaView_Jup.cpp): axis 0 the level i = index
  * ([jm][km]), axis 1 the latitude j = index ([im][km]), axis 2 the longitude k = index ([im][jm]),
  * as FP64, gathered on the device so that only the plane crosses PCIe. */
@@ -349,6 +361,12 @@ int atjup_hip_scalar_sync(atjup_ctx* ctx);
 /* One iteration of the loop body (planet/cJupiterModel.cpp:1010-1555), the functions on this path
  * in the reference's order; iter is iter_n (the physics block runs on even iter and on first). */
 int atjup_hip_step(atjup_ctx* ctx, int iter, int first);
+/* The same iteration in two halves, for hosts that work between them: the physics block
+ * (cJupiterModel.cpp:1032-1161; nothing on odd iterations other than the first), then RK4, the
+ * boundary passes, the closure's wall hold, the moist convective adjustment, the species clamp
+ * and restoreVar (:1165-1555). atjup_hip_step is the two in a row. */
+int atjup_hip_step_block(atjup_ctx* ctx, int iter, int first);
+int atjup_hip_step_rest(atjup_ctx* ctx, int iter);
 /* Non-finite census of the prognostic fields (the ATJUP_NANCHECK watch, FileIO_Jup.cpp:326-373):
  * returns ATJUP_NONFINITE if any prognostic value is NaN or infinite. */
 int atjup_hip_check_finite(atjup_ctx* ctx);

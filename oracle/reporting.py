@@ -8,6 +8,8 @@ search_minmax_2d: searchMinMax_2D (:299-330), starting from (0, 0), the interior
   k in [1, km-2].
 continuity_residual: steadyQuery's residual (:372-398), loops i, j, k over the interior with
   `fabs(r) >= minimum` from 0: the last cell on ties; NaNs never taken.
+steady_changes: steadyQuery's per-field changes (:399-410), |f - f_n| over every cell, loops i, j, k
+  with `d >= val` from 0: the last cell on ties; NaNs never taken; (0.0, (0, 0, 0)) when none is.
 """
 import math
 
@@ -85,3 +87,22 @@ def continuity_residual(u, v, w, rad, the, dr, dthe, dphi):
     i, r = divmod(p, (jm - 2) * (km - 2))
     j, k = divmod(r, km - 2)
     return float(a[p]), (i + 1, j + 1, k + 1)
+
+
+STEADY_FIELDS = ("p_dyn", "u", "v", "w", "t", "h2o", "h2o_cloud", "h2o_ice", "nh3", "nh3_cloud", "nh3_ice", "nh4sh")
+
+
+def steady_changes(fields, fields_n):
+    """[(name, max |f - f_n|, (i, j, k))] in the report's order (Reporting.h:356-369, :399-410)."""
+    out = []
+    for name in STEADY_FIELDS:
+        f, fn = fields[name], fields_n[name]
+        d = np.abs(f - fn).ravel()                      # the i, j, k loop order of a C array
+        p = _first_extreme(d, False, np.max)
+        if p is None:
+            out.append((name, 0.0, (0, 0, 0)))
+            continue
+        i, r = divmod(p, f.shape[1] * f.shape[2])
+        j, k = divmod(r, f.shape[2])
+        out.append((name, float(d[p]), (int(i), int(j), int(k))))
+    return out

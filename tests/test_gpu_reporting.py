@@ -62,3 +62,33 @@ def test_reporting_ties_and_nans():
     gpu.set_vel_star(vs)
     _same(gpu.search_minmax_2d(5), reporting.search_minmax_2d(vs))
     gpu.close(); orc.close()
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_FP32_REF], ids=["fp64", "fp32-refined"])
+def test_steady_changes_against_oracle(precision):
+    """steadyQuery's per-field changes after a physics iteration and after a plain one: the
+    library's tracked n level (u, v, w, p_dyn from the iteration's start; t and the condensables
+    from after the physics block's adjustments, as RK_SCALAR_SYNC leaves them) against the
+    restatement on the fields downloaded at those points; values and cells exact."""
+    gpu, orc, st = make_pair("full", sweeps=20, precision=precision)
+    gpu.steady_track(True)
+    names = reporting.STEADY_FIELDS
+    synced = ("t", "h2o", "h2o_cloud", "h2o_ice", "nh3", "nh3_cloud", "nh3_ice")
+    for it, first in ((1, True), (2, False), (3, False)):
+        start = {n: gpu.get(n) for n in names}
+        gpu.step_block(it, first)
+        physics = it % 2 == 0 or first
+        nlevel = dict(start)
+        if physics:
+            nlevel.update({n: gpu.get(n) for n in synced})
+        gpu.step_rest(it)
+        now = {n: gpu.get(n) for n in names}
+        got = gpu.steady_changes()
+        want = reporting.steady_changes(now, nlevel)
+        for (n, v, c), (n2, v2, c2) in zip(got, want):
+            print(f"iteration {it} {n}: {v:.6e} {c} (restatement {v2:.6e} {c2})")
+            assert n == n2 and v == v2 and tuple(c) == tuple(c2), (it, n)
+        assert any(v > 0.0 for _, v, _ in got)
+    gpu.steady_track(False)
+    gpu.close()
+    orc.close()

@@ -124,3 +124,90 @@ def test_continuity_residual_cell_against_compiled_reference():
     r = subprocess.run([sys.executable, "-c", SCRIPT_RES % {"root": ROOT}], capture_output=True, text=True, timeout=900)
     print(r.stdout[-400:])
     assert r.returncode == 0 and "SAME" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+def _loops_steady(f, fn):
+    """The reference's loops literally (Reporting.h:399-410): i, j, k with d >= val from 0."""
+    val, cell = 0.0, (0, 0, 0)
+    im, jm, km = f.shape
+    for i in range(im):
+        for j in range(jm):
+            for k in range(km):
+                d = abs(f[i, j, k] - fn[i, j, k])
+                if d >= val:
+                    val, cell = d, (i, j, k)
+    return val, cell
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_steady_changes_against_the_loops(seed):
+    rng = np.random.default_rng(seed)
+    shape = (4, 5, 6)
+    fields, fields_n = {}, {}
+    for e, name in enumerate(reporting.STEADY_FIELDS):
+        f = rng.integers(-3, 4, shape).astype(np.float64)       # many ties
+        fn = rng.integers(-3, 4, shape).astype(np.float64)
+        if seed == 1 and e % 3 == 0:
+            fn = f.copy()                                         # no change: the last cell, 0
+        if seed == 2:
+            f[rng.random(shape) < 0.2] = np.nan                   # NaNs are never taken
+            if e == 5:
+                f[:] = np.nan                                     # nothing finite: 0 at (0, 0, 0)
+        fields[name], fields_n[name] = f, fn
+    for name, val, cell in reporting.steady_changes(fields, fields_n):
+        lv, lc = _loops_steady(fields[name], fields_n[name])
+        assert (val, cell) == (lv, lc), name
+
+
+SCRIPT_STEADY = r"""
+import sys, numpy as np
+sys.path.insert(0, %(root)r)
+from atjup_amd import abi, inputs
+import oracle
+from oracle import reporting
+G, Cn, K, env = abi.make_profile("full", sweeps=8)
+st = inputs.make_state("full", seed=11)
+ref = oracle.Reference(env); ref.load_state(st, Cn)
+ref.step(1, True, satadj=True, turb=True)
+# iteration 2 (a physics iteration) as the harness's step runs it, steadyQuery before restoreVar
+for op in ("pressure", "damp_pdyn", "mixture_density", "satadj_h2o", "satadj_nh3", "satadj_ch4", "scalar_sync",
+           "turbulence", "flux_limiter", "fluxlim_damp", "rk4", "bc_all", "turb_wall", "clamp_species"):
+    ref.call(op)
+names = reporting.STEADY_FIELDS
+f = {n: ref.get(n) for n in names}
+fn = {n: ref.get(n + "n") for n in names}
+ref.L.ref_quiet(0)                      # the harness otherwise silences the reference's output
+ref.call("steady_query")
+ref.L.ref_quiet(1)
+ref.call("restore")                     # (quiet again: flushes the report)
+im = G.im
+for name, val, (i, j, k) in reporting.steady_changes(f, fn):
+    lat = (90 - j, "N") if j <= 90 else (j - 90, "S")
+    lon = (k, "E") if k <= 180 else (360 - k, "W")
+    level = i * int(G.L_atm) // (im - 1)
+    print("ORACLE %%s|%%.6f|%%d%%s|%%d%%s|%%d" %% (name, val, lat[0], lat[1], lon[0], lon[1], level))
+"""
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_steady_changes_against_compiled_reference():
+    """The restatement on the compiled reference's fields and n-level copies after a physics
+    iteration (the n level of t and the condensables refreshed by RK_SCALAR_SYNC, of u, v, w and
+    p_dyn by the last restoreVar) against the twelve lines the reference's own steadyQuery prints:
+    value to its six printed decimals, and the printed cell (latitude, longitude, level)."""
+    import re
+    r = subprocess.run([sys.executable, "-c", SCRIPT_STEADY % {"root": ROOT}], capture_output=True, text=True,
+                       timeout=900)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = r.stdout
+    report = out
+    pat = re.compile(r"^(.{36}) = \s*(\S+)\s+(\d+)°([NS])\s*(\d+)°([EW])\s+(-?\d+)km", re.M)
+    printed = [m.groups() for m in pat.finditer(report)]
+    oracle_lines = [l.split(" ", 1)[1].split("|") for l in out.splitlines() if l.startswith("ORACLE ")]
+    # the report prints the continuity residual first, then the twelve fields in the table's order
+    fields = printed[-12:]
+    assert len(fields) == 12 and len(oracle_lines) == 12, report
+    for (name, val, lat, lon, level), p in zip(oracle_lines, fields):
+        print(name, val, lat, lon, level, "| printed", p[1:])
+        assert val == p[1], name
+        assert lat == p[2] + p[3] and lon == p[4] + p[5] and level == p[6], name
