@@ -856,6 +856,55 @@ int atjup_hip_flux_limiter(atjup_ctx* c){
     if(c->kn.dry) return 0;        // nh4sh = 0: the correction is zero
     return launch_flux_limiter(c);
 }
+// The frozen-branch linearisations: nin host arrays [im][jm][km] staged into the field layout on
+// the device, the kernel, nout arrays staged back. Scratch buffers are allocated per call.
+typedef int (*lin_launch)(atjup_ctx* c, int gas, int adjoint, const double* in, double* out);
+static int lin_flux(atjup_ctx* c, int, int adjoint, const double* in, double* out){
+    return launch_flux_limiter_lin(c, adjoint, in, out);
+}
+static int lin_call(atjup_ctx* c, int nin, int nout, const double* in, double* out, int gas, int adjoint,
+                    lin_launch launch){
+    if(!c || !in || !out){ set_error("null argument"); return ATJUP_ERR_ARG; }
+    if(int rc = need_mask(c)) return rc;
+    if(c->precision != ATJUP_PRECISION_FP64 || c->kn.dry){
+        set_error("the linearised operators need an FP64 context of a moist profile"); return ATJUP_ERR_PROFILE; }
+    const size_t n3 = c->n3, nh = (size_t)c->im * c->jm * c->km;
+    double *din = nullptr, *dout = nullptr;
+    if(hipMalloc((void**)&din, nin * n3 * 8) != hipSuccess || hipMalloc((void**)&dout, nout * n3 * 8) != hipSuccess){
+        if(din) (void)hipFree(din);
+        set_error("hipMalloc failed"); return ATJUP_ERR_NOMEM;
+    }
+    int rc = 0;
+    auto hip = [&](hipError_t e, const char* what){
+        if(!rc && e != hipSuccess){ set_error(std::string(what) + ": " + hipGetErrorString(e)); rc = ATJUP_ERR_HIP; }
+    };
+    hip(hipMemsetAsync(din, 0, nin * n3 * 8, c->stream), "hipMemsetAsync");
+    hip(hipMemsetAsync(dout, 0, nout * n3 * 8, c->stream), "hipMemsetAsync");
+    for(int a = 0; a < nin && !rc; a++)
+        hip(hipMemcpy2DAsync(din + a * n3, (size_t)c->pitch * 8, in + a * nh, (size_t)c->km * 8, (size_t)c->km * 8,
+                             (size_t)c->im * c->jm, hipMemcpyHostToDevice, c->stream), "hipMemcpy2DAsync");
+    if(!rc) rc = launch(c, gas, adjoint, din, dout);
+    for(int a = 0; a < nout && !rc; a++)
+        hip(hipMemcpy2DAsync(out + a * nh, (size_t)c->km * 8, dout + a * n3, (size_t)c->pitch * 8, (size_t)c->km * 8,
+                             (size_t)c->im * c->jm, hipMemcpyDeviceToHost, c->stream), "hipMemcpy2DAsync");
+    hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return rc;
+}
+static int satadj_lin(atjup_ctx* c, int gas, int adjoint, const double* in, double* out){
+    if(c && (gas < 0 || gas > 2)){ set_error("gas"); return ATJUP_ERR_ARG; }
+    if(c && !c->gas_set[gas]){ set_error("gas constants not set (atjup_hip_set_gas)"); return ATJUP_ERR_STATE; }
+    return lin_call(c, adjoint ? 4 : 6, adjoint ? 6 : 4, in, out, gas, adjoint, launch_satadj_lin);
+}
+int atjup_hip_satadj_tangent(atjup_ctx* c, int gas, const double* dx, double* dy){ return satadj_lin(c, gas, 0, dx, dy); }
+int atjup_hip_satadj_adjoint(atjup_ctx* c, int gas, const double* ly, double* lx){ return satadj_lin(c, gas, 1, ly, lx); }
+static int flux_limiter_lin(atjup_ctx* c, int adjoint, const double* in, double* out){
+    if(c && c->km - 1 < 4){ set_error("the linearised limiter needs at least 4 distinct longitudes"); return ATJUP_ERR_ARG; }
+    return lin_call(c, adjoint ? 1 : 4, adjoint ? 4 : 1, in, out, 0, adjoint, lin_flux);
+}
+int atjup_hip_flux_limiter_tangent(atjup_ctx* c, const double* dx, double* dy){ return flux_limiter_lin(c, 0, dx, dy); }
+int atjup_hip_flux_limiter_adjoint(atjup_ctx* c, const double* ly, double* lx){ return flux_limiter_lin(c, 1, ly, lx); }
 int atjup_hip_clamp_species(atjup_ctx* c){ if(!c) return ATJUP_ERR_ARG; return launch_clamp_species(c); }
 // restoreVar: the n level is the live storage, except the closure's live k*, omega* after a
 // turbulence run, which become the n level here

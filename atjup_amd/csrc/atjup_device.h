@@ -126,6 +126,11 @@ using ::fabs;
 __device__ inline bool isfinite(Dual a){ return ::isfinite(a.v); }
 __device__ inline Dual sqrt(Dual a){ const double r = ::sqrt(a.v); return Dual(r, a.d / (2.0 * r)); }
 __device__ inline Dual fabs(Dual a){ return (a.v < 0.0) ? -a : a; }
+// exp and log for code templated on double or Dual (the double forms are the library's own)
+__device__ __forceinline__ double dual_exp(double a){ return exp(a); }
+__device__ __forceinline__ double dual_log(double a){ return log(a); }
+__device__ inline Dual dual_exp(Dual a){ const double e = ::exp(a.v); return Dual(e, e * a.d); }
+__device__ inline Dual dual_log(Dual a){ return Dual(::log(a.v), a.d / a.v); }
 
 // The type a kernel accumulates and forms its FP64 quantities in: FP64 for the FP64 and FP32
 // storage, the derivative-carrying type for the tangent form.

@@ -190,6 +190,9 @@ size_t report_bytes(int im, int jm, int km);
 int steady_mark(atjup_ctx* c, unsigned entries);          // the size of atjup_ctx::report
 int build_rad_table(atjup_ctx* c);
 int launch_flux_limiter(atjup_ctx* c);
+// frozen-branch tangent (adjoint = 0) or adjoint (1) of one gas's adjustment / of the limiter, FP64
+int launch_satadj_lin(atjup_ctx* c, int gas, int adjoint, const double* in, double* out);
+int launch_flux_limiter_lin(atjup_ctx* c, int adjoint, const double* in, double* out);
 int launch_commit_live(atjup_ctx* c);
 
 // The storage a live field is read from: the closure's live k*, omega* while a turbulence run

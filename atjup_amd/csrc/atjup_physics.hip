@@ -913,27 +913,30 @@ struct SatParams {
     double cp_mix, t_ref, r_mix;
 };
 
-__device__ __forceinline__ double svp_d(double T, double C, double L0, double R, double da, double db){
-    return exp(C + (-L0 / T + da * log(T) + db * T) / (1e-3 * R));         // ATPhys.h:49-52
+// D: double for the adjustment itself, Dual for its frozen-branch linearisation (comparisons
+// and the iteration's exits follow the values, so the derivative is that of the branches taken)
+template<typename D>
+__device__ __forceinline__ D svp_d(D T, double C, double L0, double R, double da, double db){
+    return dual_exp(C + (-L0 / T + da * dual_log(T) + db * T) / (1e-3 * R));      // ATPhys.h:49-52
 }
 
-__device__ void satadj_cell(double& cv, double& ccl, double& cic, double& tq, double p_u, double rf,
-                            const SatParams& P){
+template<typename D>
+__device__ void satadj_cell(D& cv, D& ccl, D& cic, D& tq, D p_u, D rf, const SatParams& P){
     const double t_0 = P.t_0, t_00 = P.t_00, ep = P.ep, lv = P.lv, ls = P.ls;
     const double t_range_inv = 1.0 / (t_0 - t_00);
     const double Lf = ls - lv;
     const double cp_mix = P.cp_mix, t_ref = P.t_ref;
     const int iter_prec_end = 15;
     const double q_diff_min = 1.0e-3;
-    double t_u = tq * t_ref;
+    D t_u = tq * t_ref;
     if(!(t_u > 0.0) || !(p_u > 0.0)) return;
-    const double rho_at = (rf > 0.0 && isfinite(rf)) ? rf : P.r_mix;
+    const D rho_at = (rf > 0.0 && isfinite(rf)) ? rf : D(P.r_mix);
     if(t_u > t_0){
         if(cic > 0.0){
-            double dm = cic;
+            D dm = cic;
             if(Lf > 0.0){
-                const double hc = cp_mix * rho_at;
-                const double cap = hc * (t_u - t_0) / Lf;
+                const D hc = cp_mix * rho_at;
+                const D cap = hc * (t_u - t_0) / Lf;
                 if(dm > cap) dm = cap;
                 t_u -= Lf * dm / hc;
                 tq = t_u / t_ref;
@@ -945,30 +948,30 @@ __device__ void satadj_cell(double& cv, double& ccl, double& cic, double& tq, do
     if(cv < 0.0) cv = 0.0;
     if(ccl < 0.0) ccl = 0.0;
     if(cic < 0.0) cic = 0.0;
-    const double rho_c = rho_at;
-    const double E_Rain_0 = svp_d(t_u, P.C, P.L0, P.R, P.del_alf, P.del_bet);
-    const double q_Rain_0 = rho_c * ep * E_Rain_0 / p_u;
+    const D rho_c = rho_at;
+    const D E_Rain_0 = svp_d(t_u, P.C, P.L0, P.R, P.del_alf, P.del_bet);
+    const D q_Rain_0 = rho_c * ep * E_Rain_0 / p_u;
     if(q_Rain_0 <= 0.0) return;
-    double q_entry = q_Rain_0;
+    D q_entry = q_Rain_0;
     if(ccl <= 0.0 && cic <= 0.0 && t_u < t_0)
         q_entry = rho_c * ep * svp_d(t_u, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
     if(cv <= q_entry){
-        const double q_c0 = ccl, q_i0 = cic;
+        const D q_c0 = ccl, q_i0 = cic;
         if(q_c0 <= 0.0 && q_i0 <= 0.0) return;
-        const double q_Ice_0 = rho_c * ep * svp_d(t_u, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
-        double q_t0;
+        const D q_Ice_0 = rho_c * ep * svp_d(t_u, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
+        D q_t0;
         if(q_c0 > 0.0 && q_i0 > 0.0) q_t0 = (q_c0 * q_Rain_0 + q_i0 * q_Ice_0) / (q_c0 + q_i0);
         else if(q_i0 <= 0.0)         q_t0 = q_Rain_0;
         else                         q_t0 = q_Ice_0;
         if(fabs(cv - q_t0) <= q_diff_min * q_t0) return;
     }
-    double q_v_b = cv, q_c_b = ccl, q_i_b = cic;
-    double Tt = t_u;
-    double q_v_hyp = q_v_b;
-    double q_rain_now = q_Rain_0;
+    D q_v_b = cv, q_c_b = ccl, q_i_b = cic;
+    D Tt = t_u;
+    D q_v_hyp = q_v_b;
+    D q_rain_now = q_Rain_0;
     for(int itr = 1; itr <= iter_prec_end; itr++){
-        double CND = (Tt - t_00) * t_range_inv;
-        double DEP = (t_0 - Tt) * t_range_inv;
+        D CND = (Tt - t_00) * t_range_inv;
+        D DEP = (t_0 - Tt) * t_range_inv;
         if(Tt <= t_00){ CND = 0.0; DEP = 1.0; }
         if(Tt >= t_0){ CND = 1.0; DEP = 0.0; }
         const bool berg = Tt < t_0 && q_v_b <= q_rain_now && (q_i_b > 0.0 || q_c_b <= 0.0);
@@ -980,12 +983,12 @@ __device__ void satadj_cell(double& cv, double& ccl, double& cic, double& tq, do
                 q_c_b = 0.0;
             }
         }
-        double d_q_v = q_v_hyp - q_v_b;
-        double d_q_c = -d_q_v * CND;
-        double d_q_i = -d_q_v * DEP;
+        D d_q_v = q_v_hyp - q_v_b;
+        D d_q_c = -d_q_v * CND;
+        D d_q_i = -d_q_v * DEP;
         if(d_q_v < -q_v_b){ d_q_v = -q_v_b; d_q_c = -d_q_v * CND; d_q_i = -d_q_v * DEP; }
-        if(q_c_b + d_q_c < 0.0){ const double ex = -(q_c_b + d_q_c); d_q_c = -q_c_b; d_q_i -= ex; }
-        if(q_i_b + d_q_i < 0.0){ const double ex = -(q_i_b + d_q_i); d_q_i = -q_i_b; d_q_c -= ex;
+        if(q_c_b + d_q_c < 0.0){ const D ex = -(q_c_b + d_q_c); d_q_c = -q_c_b; d_q_i -= ex; }
+        if(q_i_b + d_q_i < 0.0){ const D ex = -(q_i_b + d_q_i); d_q_i = -q_i_b; d_q_c -= ex;
                                  if(q_c_b + d_q_c < 0.0) d_q_c = -q_c_b; }
         d_q_v = -(d_q_c + d_q_i);
         Tt += (lv * d_q_c + ls * d_q_i) / (cp_mix * rho_c);
@@ -995,10 +998,10 @@ __device__ void satadj_cell(double& cv, double& ccl, double& cic, double& tq, do
         if(q_v_b < 0.0) q_v_b = 0.0;
         if(q_c_b < 0.0) q_c_b = 0.0;
         if(q_i_b < 0.0) q_i_b = 0.0;
-        const double E_Rain = svp_d(Tt, P.C, P.L0, P.R, P.del_alf, P.del_bet);
-        const double E_Ice = svp_d(Tt, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i);
-        const double q_Rain = rho_c * ep * E_Rain / p_u;
-        const double q_Ice = rho_c * ep * E_Ice / p_u;
+        const D E_Rain = svp_d(Tt, P.C, P.L0, P.R, P.del_alf, P.del_bet);
+        const D E_Ice = svp_d(Tt, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i);
+        const D q_Rain = rho_c * ep * E_Rain / p_u;
+        const D q_Ice = rho_c * ep * E_Ice / p_u;
         q_rain_now = q_Rain;
         if(q_c_b > 0.0 && q_i_b > 0.0) q_v_hyp = (q_c_b * q_Rain + q_i_b * q_Ice) / (q_c_b + q_i_b);
         else if(q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0) q_v_hyp = q_Ice;
@@ -1006,10 +1009,10 @@ __device__ void satadj_cell(double& cv, double& ccl, double& cic, double& tq, do
         else                  q_v_hyp = q_Ice;
         if(Tt >= t_0){
             if(q_i_b > 0.0){
-                double dm = q_i_b;
+                D dm = q_i_b;
                 if(Lf > 0.0){
-                    const double hc = cp_mix * rho_c;
-                    const double cap = hc * (Tt - t_0) / Lf;
+                    const D hc = cp_mix * rho_c;
+                    const D cap = hc * (Tt - t_0) / Lf;
                     if(dm > cap) dm = cap;
                     Tt -= Lf * dm / hc;
                 }
@@ -1017,23 +1020,23 @@ __device__ void satadj_cell(double& cv, double& ccl, double& cic, double& tq, do
                 q_c_b += dm;
             }
         }
-        const double q_diff = fabs(q_v_b - q_v_hyp) / (q_v_hyp + 1e-20);
+        const D q_diff = fabs(q_v_b - q_v_hyp) / (q_v_hyp + 1e-20);
         if(q_diff <= q_diff_min) break;
         if(q_c_b <= 0.0 && q_i_b <= 0.0 && q_v_b < q_v_hyp) break;
         const double dT = 0.1;
-        const double q_Rain_d = rho_c * ep * svp_d(Tt + dT, P.C, P.L0, P.R, P.del_alf, P.del_bet) / p_u;
-        const double q_Ice_d = rho_c * ep * svp_d(Tt + dT, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
-        double q_hyp_d;
+        const D q_Rain_d = rho_c * ep * svp_d(Tt + dT, P.C, P.L0, P.R, P.del_alf, P.del_bet) / p_u;
+        const D q_Ice_d = rho_c * ep * svp_d(Tt + dT, P.C_i, P.L0_i, P.R, P.del_alf_i, P.del_bet_i) / p_u;
+        D q_hyp_d;
         if(q_c_b > 0.0 && q_i_b > 0.0) q_hyp_d = (q_c_b * q_Rain_d + q_i_b * q_Ice_d) / (q_c_b + q_i_b);
         else if(q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0) q_hyp_d = q_Ice_d;
         else if(q_i_b == 0.0) q_hyp_d = q_Rain_d;
         else                  q_hyp_d = q_Ice_d;
-        double CNDn = (Tt - t_00) * t_range_inv;
+        D CNDn = (Tt - t_00) * t_range_inv;
         if(Tt <= t_00) CNDn = 0.0;
         if(Tt >= t_0) CNDn = 1.0;
         if(berg) CNDn = 0.0;
-        const double Gn = (lv * CNDn + ls * (1.0 - CNDn)) / (cp_mix * rho_c) * (q_hyp_d - q_v_hyp) / dT;
-        double wgt = 1.0 / (1.0 + dmax(Gn, 0.0));
+        const D Gn = (lv * CNDn + ls * (1.0 - CNDn)) / (cp_mix * rho_c) * (q_hyp_d - q_v_hyp) / dT;
+        D wgt = 1.0 / (1.0 + dmax(Gn, D(0.0)));
         if(wgt < 0.05) wgt = 0.05;
         q_v_hyp = q_v_b + wgt * (q_v_hyp - q_v_b);
     }
@@ -1143,8 +1146,51 @@ __global__ __launch_bounds__(256) void k_satadj3(const Sat3Args<T> A){
 // =============================================================================================
 // FluxLimiter<P>::nh4sh, Superbee (FluxLimiter.h:67-190), with the periodic seam
 // =============================================================================================
-__device__ __forceinline__ double superbee(double r){
-    return dmax(0.0, dmax(dmin(2.0 * r, 1.0), dmin(r, 2.0)));
+template<typename D>
+__device__ __forceinline__ D superbee(D r){
+    return dmax(D(0.0), dmax(dmin(2.0 * r, D(1.0)), dmin(r, D(2.0))));
+}
+
+// The Superbee correction of one interior cell (i, j, k) at storage index q; n_at(index) gives
+// NH4SH at a storage index and u, v, w are the cell's velocities. D: double for the limiter itself,
+// Dual for its frozen-branch linearisation (the limiter's choices follow the values).
+template<typename D, typename NA>
+__device__ __forceinline__ D flux_corr(const NA& n_at, D u, D v, D w, size_t q, int i, int j, int k,
+        int im, int jm, int km, size_t P, size_t R, const double* __restrict__ tabs,
+        double dr, double dthe, double dphi, double c43, double c13){
+    const int n = km - 1;
+    auto wrap = [n](int kk){ return kk < 0 ? kk + n : (kk >= n ? kk - n : kk); };
+    const size_t row = q - k;
+    const double eps = 1.0e-12;
+    const D qc = n_at(q);
+    const D q_rp = n_at(q + P), q_rm = n_at(q - P);
+    const D q_tp = n_at(q + R), q_tm = n_at(q - R);
+    const D q_pp = n_at(row + wrap(k + 1)), q_pm = n_at(row + wrap(k - 1));
+    const double rm = tabs[i];
+    const double sinthe = dmax(0.4, fabs(tabs[im + j]));
+    const double inv_rm = 1.0 / rm;
+    const double inv_rmsinthe = 1.0 / (rm * sinthe);
+    D corr = 0.0;
+    {
+        const D df = q_rp - qc, db = qc - q_rm;
+        const D denom = df + (df >= 0.0 ? eps : -eps);
+        const D r = (u >= 0.0) ? db / denom : ((i + 2 < im ? n_at(q + 2 * P) : q_rp) - q_rp) / denom;
+        corr += (1.0 - superbee(r)) * fabs(u) * (df - db) / (2.0 * dr);
+    }
+    {
+        const D df = q_tp - qc, db = qc - q_tm;
+        const D denom = df + (df >= 0.0 ? eps : -eps);
+        const D q_far = (j + 2 < jm) ? n_at(q + 2 * R) : (c43 * q_tp - c13 * qc);
+        const D r = (v >= 0.0) ? db / denom : (q_far - q_tp) / denom;
+        corr += (1.0 - superbee(r)) * fabs(v) * inv_rm * (df - db) / (2.0 * dthe);
+    }
+    {
+        const D df = q_pp - qc, db = qc - q_pm;
+        const D denom = df + (df >= 0.0 ? eps : -eps);
+        const D r = (w >= 0.0) ? db / denom : (n_at(row + wrap(k + 2)) - q_pp) / denom;
+        corr += (1.0 - superbee(r)) * fabs(w) * inv_rmsinthe * (df - db) / (2.0 * dphi);
+    }
+    return corr;
 }
 
 template<typename T>
@@ -1154,41 +1200,138 @@ __global__ __launch_bounds__(256) void k_flux_limiter(const T* __restrict__ N, c
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + 1, i = blockIdx.z + 1;
     if(k > km - 2 || j > jm - 2 || i > im - 2) return;
     const size_t q = ((size_t)i * jm + j) * pitch + k;
-    const size_t P = plane, R = (size_t)pitch;
-    const int n = km - 1;
-    auto wrap = [n](int kk){ return kk < 0 ? kk + n : (kk >= n ? kk - n : kk); };
-    const size_t row = q - k;
-    const double eps = 1.0e-12;
-    const double qc = N[q];
-    const double q_rp = N[q + P], q_rm = N[q - P];
-    const double q_tp = N[q + R], q_tm = N[q - R];
-    const double q_pp = N[row + wrap(k + 1)], q_pm = N[row + wrap(k - 1)];
-    const double u = U[q], v = V[q], w = W[q];
-    const double rm = tabs[i];
-    const double sinthe = dmax(0.4, fabs(tabs[im + j]));
-    const double inv_rm = 1.0 / rm;
-    const double inv_rmsinthe = 1.0 / (rm * sinthe);
-    double corr = 0.0;
-    {
-        const double df = q_rp - qc, db = qc - q_rm;
-        const double denom = df + (df >= 0.0 ? eps : -eps);
-        const double r = (u >= 0.0) ? db / denom : ((i + 2 < im ? (double)N[q + 2 * P] : q_rp) - q_rp) / denom;
-        corr += (1.0 - superbee(r)) * fabs(u) * (df - db) / (2.0 * dr);
+    auto n_at = [N](size_t x){ return (double)N[x]; };
+    out[q] = (T)flux_corr<double>(n_at, (double)U[q], (double)V[q], (double)W[q], q, i, j, k, im, jm, km,
+                                  plane, (size_t)pitch, tabs, dr, dthe, dphi, c43, c13);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Frozen-branch linearisations of the saturation adjustment and the NH4SH flux limiter (FP64
+// contexts): the tangent runs the operator on Dual numbers along a direction, the adjoint applies
+// the transpose of the same Jacobian. The branches, clamps and the adjustment's iteration exits
+// are those the forward takes at the context's state.
+// ---------------------------------------------------------------------------------------------
+// the adjustment's six inputs and four outputs at a cell: vapour, cloud, ice, t, then p_stat, rho_mix
+struct SatLin {
+    const double* x[6];          // the state the adjustment would start from
+    const uint8_t* flags;
+    int jm, km, pitch;
+};
+
+__device__ __forceinline__ void satadj_dual(const SatLin& L, size_t q, const Dual in[6], Dual out[4], const SatParams& P){
+    Dual cv = in[0], ccl = in[1], cic = in[2], tq = in[3];
+    if(!(L.flags[q] & FL_SELF)) satadj_cell(cv, ccl, cic, tq, in[4], in[5], P);
+    if(cv <= 0.0) cv = Dual(0.0);
+    if(ccl <= 0.0) ccl = Dual(0.0);
+    if(cic <= 0.0) cic = Dual(0.0);
+    out[0] = cv; out[1] = ccl; out[2] = cic; out[3] = tq;
+}
+
+__global__ __launch_bounds__(128) void kt_satadj(const SatLin L, const SatParams P,
+        const double* __restrict__ dx, double* __restrict__ dy, size_t n3){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= L.km) return;
+    const size_t q = ((size_t)i * L.jm + j) * L.pitch + k;
+    Dual in[6], out[4];
+    #pragma unroll
+    for(int a = 0; a < 6; a++) in[a] = Dual(L.x[a][q], dx[a * n3 + q]);
+    satadj_dual(L, q, in, out, P);
+    #pragma unroll
+    for(int r = 0; r < 4; r++) dy[r * n3 + q] = out[r].d;
+}
+
+// lx = J^T ly per cell, the Jacobian's six columns from six Dual runs
+__global__ __launch_bounds__(128) void ka_satadj(const SatLin L, const SatParams P,
+        const double* __restrict__ ly, double* __restrict__ lx, size_t n3){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= L.km) return;
+    const size_t q = ((size_t)i * L.jm + j) * L.pitch + k;
+    double lam[4];
+    #pragma unroll
+    for(int r = 0; r < 4; r++) lam[r] = ly[r * n3 + q];
+    #pragma unroll 1
+    for(int a = 0; a < 6; a++){
+        Dual in[6], out[4];
+        #pragma unroll
+        for(int b = 0; b < 6; b++) in[b] = Dual(L.x[b][q], b == a ? 1.0 : 0.0);
+        satadj_dual(L, q, in, out, P);
+        double acc = 0.0;
+        #pragma unroll
+        for(int r = 0; r < 4; r++) acc += out[r].d * lam[r];
+        lx[a * n3 + q] = acc;
     }
-    {
-        const double df = q_tp - qc, db = qc - q_tm;
-        const double denom = df + (df >= 0.0 ? eps : -eps);
-        const double q_far = (j + 2 < jm) ? (double)N[q + 2 * R] : (c43 * q_tp - c13 * qc);
-        const double r = (v >= 0.0) ? db / denom : (q_far - q_tp) / denom;
-        corr += (1.0 - superbee(r)) * fabs(v) * inv_rm * (df - db) / (2.0 * dthe);
+}
+
+struct FlxLin {
+    const double *N, *U, *V, *W;
+    const double* tabs;
+    int im, jm, km, pitch;
+    size_t plane;
+    double dr, dthe, dphi;
+};
+
+__device__ __forceinline__ bool flx_interior(const FlxLin& L, int i, int j, int k){
+    return i >= 1 && i <= L.im - 2 && j >= 1 && j <= L.jm - 2 && k >= 0 && k <= L.km - 2;
+}
+
+// d(fluxlim_nh4sh) along (dN, dU, dV, dW) [4][n3]; cells the limiter does not write get 0
+__global__ __launch_bounds__(128) void kt_flux_limiter(const FlxLin L, const double* __restrict__ dx,
+        double* __restrict__ dy, size_t n3){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= L.km) return;
+    const size_t q = ((size_t)i * L.jm + j) * L.pitch + k;
+    double d = 0.0;
+    if(flx_interior(L, i, j, k)){
+        const double* N = L.N;
+        auto n_at = [N, dx](size_t x){ return Dual(N[x], dx[x]); };
+        d = flux_corr<Dual>(n_at, Dual(L.U[q], dx[n3 + q]), Dual(L.V[q], dx[2 * n3 + q]), Dual(L.W[q], dx[3 * n3 + q]),
+                            q, i, j, k, L.im, L.jm, L.km, L.plane, (size_t)L.pitch, L.tabs,
+                            L.dr, L.dthe, L.dphi, 4.0 / 3.0, 1.0 / 3.0).d;
     }
-    {
-        const double df = q_pp - qc, db = qc - q_pm;
-        const double denom = df + (df >= 0.0 ? eps : -eps);
-        const double r = (w >= 0.0) ? db / denom : ((double)N[row + wrap(k + 2)] - q_pp) / denom;
-        corr += (1.0 - superbee(r)) * fabs(w) * inv_rmsinthe * (df - db) / (2.0 * dphi);
+    dy[q] = d;
+}
+
+// (lN, lU, lV, lW) = J^T ly, gathered: NH4SH at x is read by the corrections of x itself, of its
+// radial and meridional neighbours at -2 .. +1 and of its zonal neighbours at -2 .. +1 (periodic
+// over the km - 1 distinct columns); each of those corrections is differentiated with respect to
+// the value at x. The velocities enter only the own cell's correction.
+__global__ __launch_bounds__(128) void ka_flux_limiter(const FlxLin L, const double* __restrict__ ly,
+        double* __restrict__ lx, size_t n3){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if(k >= L.km) return;
+    const size_t q = ((size_t)i * L.jm + j) * L.pitch + k;
+    const int n = L.km - 1;
+    const double* N = L.N;
+    double lN = 0.0, lU = 0.0, lV = 0.0, lW = 0.0;
+    if(k < n){
+        const int off[10][3] = { { 0, 0, 0 }, { -1, 0, 0 }, { 1, 0, 0 }, { -2, 0, 0 }, { 0, -1, 0 }, { 0, 1, 0 },
+                                 { 0, -2, 0 }, { 0, 0, -1 }, { 0, 0, 1 }, { 0, 0, -2 } };
+        #pragma unroll 1
+        for(int c = 0; c < 10; c++){
+            const int yi = i + off[c][0], yj = j + off[c][1];
+            int yk = k + off[c][2];
+            yk = yk < 0 ? yk + n : (yk >= n ? yk - n : yk);
+            if(!flx_interior(L, yi, yj, yk)) continue;
+            const size_t qy = ((size_t)yi * L.jm + yj) * L.pitch + yk;
+            auto n_at = [N, q](size_t x){ return Dual(N[x], x == q ? 1.0 : 0.0); };
+            const Dual cy = flux_corr<Dual>(n_at, Dual(L.U[qy]), Dual(L.V[qy]), Dual(L.W[qy]), qy, yi, yj, yk,
+                                            L.im, L.jm, L.km, L.plane, (size_t)L.pitch, L.tabs,
+                                            L.dr, L.dthe, L.dphi, 4.0 / 3.0, 1.0 / 3.0);
+            lN += cy.d * ly[qy];
+        }
     }
-    out[q] = (T)corr;
+    if(flx_interior(L, i, j, k)){
+        const double l = ly[q];
+        auto n_at = [N](size_t x){ return Dual(N[x]); };
+        #pragma unroll 1
+        for(int a = 0; a < 3; a++){
+            const Dual cu = flux_corr<Dual>(n_at, Dual(L.U[q], a == 0 ? 1.0 : 0.0), Dual(L.V[q], a == 1 ? 1.0 : 0.0),
+                                            Dual(L.W[q], a == 2 ? 1.0 : 0.0), q, i, j, k, L.im, L.jm, L.km, L.plane,
+                                            (size_t)L.pitch, L.tabs, L.dr, L.dthe, L.dphi, 4.0 / 3.0, 1.0 / 3.0);
+            (a == 0 ? lU : a == 1 ? lV : lW) = cu.d * l;
+        }
+    }
+    lx[q] = lN; lx[n3 + q] = lU; lx[2 * n3 + q] = lV; lx[3 * n3 + q] = lW;
 }
 
 // =============================================================================================
@@ -1431,6 +1574,39 @@ int do_flux_limiter(atjup_ctx* c){
         F<T>(c, ATJUP_F_NH4SH), F<T>(c, ATJUP_F_U), F<T>(c, ATJUP_F_V), F<T>(c, ATJUP_F_W),
         F<T>(c, ATJUP_F_FLUXLIM_NH4SH), (const double*)c->tabs.p, c->im, c->jm, c->km, c->pitch, c->plane,
         c->g.dr, c->g.dthe, c->g.dphi, 4.0 / 3.0, 1.0 / 3.0);
+    return check_launch();
+}
+
+// the frozen-branch linearisations (FP64 contexts): in and out are [n][n3] device arrays in the
+// field layout; tangent: in = the direction, out = the derivative; adjoint: in = the outputs'
+// adjoints, out = the inputs' adjoints
+int do_satadj_lin(atjup_ctx* c, int gas, int adjoint, const double* in, double* out){
+    static const int ids[3][3] = { { ATJUP_F_H2O, ATJUP_F_H2O_CLOUD, ATJUP_F_H2O_ICE },
+                                   { ATJUP_F_NH3, ATJUP_F_NH3_CLOUD, ATJUP_F_NH3_ICE },
+                                   { ATJUP_F_CH4, ATJUP_F_CH4_CLOUD, ATJUP_F_CH4_ICE } };
+    const atjup_gas_params& g = c->gas[gas];
+    const SatParams P{ g.t_0, g.t_00, g.ep, g.lv, g.ls, g.C, g.L0, g.R, g.del_alf, g.del_bet,
+                       g.C_i, g.L0_i, g.del_alf_i, g.del_bet_i, c->c.cp_mix, c->c.t_ref, c->c.r_mix };
+    SatLin L;
+    for(int s = 0; s < 3; s++) L.x[s] = F<double>(c, ids[gas][s]);
+    L.x[3] = F<double>(c, ATJUP_F_T); L.x[4] = F<double>(c, ATJUP_F_P_STAT); L.x[5] = F<double>(c, ATJUP_F_RHO_MIX);
+    L.flags = (const uint8_t*)c->flags.p;
+    L.jm = c->jm; L.km = c->km; L.pitch = c->pitch;
+    const dim3 grid(nblk(c->km, 128), c->jm, c->im);
+    if(adjoint) hipLaunchKernelGGL(ka_satadj, grid, dim3(128), 0, c->stream, L, P, in, out, c->n3);
+    else        hipLaunchKernelGGL(kt_satadj, grid, dim3(128), 0, c->stream, L, P, in, out, c->n3);
+    return check_launch();
+}
+
+int do_flux_limiter_lin(atjup_ctx* c, int adjoint, const double* in, double* out){
+    FlxLin L;
+    L.N = F<double>(c, ATJUP_F_NH4SH); L.U = F<double>(c, ATJUP_F_U); L.V = F<double>(c, ATJUP_F_V); L.W = F<double>(c, ATJUP_F_W);
+    L.tabs = (const double*)c->tabs.p;
+    L.im = c->im; L.jm = c->jm; L.km = c->km; L.pitch = c->pitch; L.plane = c->plane;
+    L.dr = c->g.dr; L.dthe = c->g.dthe; L.dphi = c->g.dphi;
+    const dim3 grid(nblk(c->km, 128), c->jm, c->im);
+    if(adjoint) hipLaunchKernelGGL(ka_flux_limiter, grid, dim3(128), 0, c->stream, L, in, out, c->n3);
+    else        hipLaunchKernelGGL(kt_flux_limiter, grid, dim3(128), 0, c->stream, L, in, out, c->n3);
     return check_launch();
 }
 
@@ -2598,5 +2774,13 @@ int launch_chem_kernel(atjup_ctx* c, int which){ return DISPATCH(do_chem_kernels
 int launch_mca(atjup_ctx* c){ return DISPATCH(do_mca, c); }
 int launch_nh4sh_eq(atjup_ctx* c){ return DISPATCH(do_nh4sh_eq, c); }
 int launch_commit_live(atjup_ctx* c){ return c->turb_live_pending ? DISPATCH(do_commit_live, c) : 0; }
+int launch_satadj_lin(atjup_ctx* c, int gas, int adjoint, const double* in, double* out){
+    if(c->precision != ATJUP_PRECISION_FP64){ set_error("the linearised adjustment needs an FP64 context"); return ATJUP_ERR_PROFILE; }
+    return do_satadj_lin(c, gas, adjoint, in, out);
+}
+int launch_flux_limiter_lin(atjup_ctx* c, int adjoint, const double* in, double* out){
+    if(c->precision != ATJUP_PRECISION_FP64){ set_error("the linearised limiter needs an FP64 context"); return ATJUP_ERR_PROFILE; }
+    return do_flux_limiter_lin(c, adjoint, in, out);
+}
 
 }  // namespace atjup

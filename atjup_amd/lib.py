@@ -98,6 +98,10 @@ def load():
     L.atjup_hip_libm_probe.argtypes = [C.c_int, _D, _D, _D, C.c_long]
     L.atjup_hip_set_smooth_obstacle.argtypes = [C.c_void_p, C.POINTER(abi.Obstacle)]
     L.atjup_hip_adj_obstacle_gradient.argtypes = [C.c_void_p, _D]
+    for n in ("atjup_hip_satadj_tangent", "atjup_hip_satadj_adjoint"):
+        getattr(L, n).argtypes = [C.c_void_p, C.c_int, _D, _D]
+    for n in ("atjup_hip_flux_limiter_tangent", "atjup_hip_flux_limiter_adjoint"):
+        getattr(L, n).argtypes = [C.c_void_p, _D, _D]
     _lib = L
     return L
 
@@ -381,6 +385,30 @@ class Context:
         out = np.zeros(3)
         self._chk(self.L.atjup_hip_adj_obstacle_gradient(self.h, _dp(out)), "adj_obstacle_gradient")
         return out
+
+    def _lin(self, fn, what, nout, arrays, *args):
+        src = np.ascontiguousarray(np.stack([np.asarray(a, dtype=np.float64) for a in arrays]))
+        assert src.shape[1:] == self.shape, (src.shape, self.shape)
+        out = np.zeros((nout,) + self.shape)
+        self._chk(fn(self.h, *args, _dp(src), _dp(out)), what)
+        return list(out)
+
+    def satadj_tangent(self, gas, dx):
+        """Frozen-branch tangent of one gas's saturation adjustment at the current state:
+        dx = d(vapour, cloud, ice, t, p_stat, rho_mix) -> d(vapour, cloud, ice, t)."""
+        return self._lin(self.L.atjup_hip_satadj_tangent, "satadj_tangent", 4, dx, gas)
+
+    def satadj_adjoint(self, gas, ly):
+        """Its transpose: the outputs' adjoints (vapour, cloud, ice, t) -> the inputs' six."""
+        return self._lin(self.L.atjup_hip_satadj_adjoint, "satadj_adjoint", 6, ly, gas)
+
+    def flux_limiter_tangent(self, dx):
+        """Frozen-branch tangent of the NH4SH flux limiter: d(nh4sh, u, v, w) -> d fluxlim_nh4sh."""
+        return self._lin(self.L.atjup_hip_flux_limiter_tangent, "flux_limiter_tangent", 1, dx)[0]
+
+    def flux_limiter_adjoint(self, ly):
+        """Its transpose: the adjoint of fluxlim_nh4sh -> those of (nh4sh, u, v, w)."""
+        return self._lin(self.L.atjup_hip_flux_limiter_adjoint, "flux_limiter_adjoint", 4, [ly])
 
     def adj_forward(self, nsteps, every, first_iter=1):
         self._chk(self.L.atjup_hip_adj_forward_checkpointed(self.h, nsteps, every, first_iter), "adj_forward")

@@ -452,6 +452,22 @@ int atjup_hip_set_smooth_obstacle(atjup_ctx* ctx, const atjup_obstacle* params);
  * atjup_hip_adj_backward carried back (dynamics profile with the drag region set); out[3]. */
 int atjup_hip_adj_obstacle_gradient(atjup_ctx* ctx, double* out);
 
+/* Frozen-branch linearisations of two moist operators at the context's current state (FP64
+ * context of a moist profile; every array [n][im][jm][km] on the host). The branches, clamps and
+ * iteration exits are those the forward takes at that state; the tangent is the operator run on
+ * (value, derivative) pairs along a direction, the adjoint the transpose of the same Jacobian.
+ * SaturationAdjustment<cJupiterModel>::run for one gas (planet/SaturationAdjustment.h:255-746,
+ * its final positivity pass included; constants of atjup_hip_set_gas): inputs (vapour, cloud,
+ * ice, t, p_stat, rho_mix), outputs (vapour, cloud, ice, t).
+ *   tangent: dx[6] -> dy[4];  adjoint: ly[4] (adjoints of the outputs) -> lx[6]. */
+int atjup_hip_satadj_tangent(atjup_ctx* ctx, int gas, const double* dx, double* dy);
+int atjup_hip_satadj_adjoint(atjup_ctx* ctx, int gas, const double* ly, double* lx);
+/* FluxLimiter<cJupiterModel>::nh4sh (planet/FluxLimiter.h:89-190): inputs (nh4sh, u, v, w),
+ * output fluxlim_nh4sh on the interior cells the limiter writes (0 elsewhere).
+ *   tangent: dx[4] -> dy[1];  adjoint: ly[1] -> lx[4]. */
+int atjup_hip_flux_limiter_tangent(atjup_ctx* ctx, const double* dx, double* dy);
+int atjup_hip_flux_limiter_adjoint(atjup_ctx* ctx, const double* ly, double* lx);
+
 /* ---- housekeeping ----------------------------------------------------------------------- */
 int         atjup_hip_sync(atjup_ctx* ctx);
 /* Number of HIP devices visible to the library's runtime (0 and an error status without one). */

@@ -1,0 +1,170 @@
+"""Frozen-branch tangent and adjoint of two moist operators on a moist state: the saturation
+adjustment of each gas (planet/SaturationAdjustment.h:255-746, with its final positivity pass) and
+the NH4SH flux limiter (planet/FluxLimiter.h:89-190).
+
+Per operator, three checks:
+* the adjoint is the transpose of the tangent: <J dx, ly> = <dx, J^T ly> for seeded dx and ly,
+  to 1e-12 of the sum of the absolute terms (the two sides sum the same per-cell derivatives in
+  different orders);
+* the tangent is the derivative of the forward as run: central differences of the FP64 forward
+  operator (the library's own kernel) along dx, with a relative step of 1e-7 on every input,
+  agree with the tangent to 1e-5 relative in every cell whose branches the step does not flip
+  (an iteration exit, a clamp, a limiter regime); such flips are counted and must stay below
+  0.5% of the cells (at most 16 of 21,021 per output occur). The directions keep exact zeros at
+  zero, and NH4SH is given a seeded non-uniform profile: a species at zero and one at a tiny
+  positive amount take different branches, a velocity at zero is the kink of |u|, and a uniform
+  NH4SH field sits on the limiter's kink df = 0, where no derivative exists;
+* the pieces the forward leaves alone stay alone: a solid cell's adjustment is the identity (up
+  to the positivity clamp), and the limiter's derivative is zero off the interior cells it writes.
+A plausible mistake (a wrong derivative of exp or log, a dropped term of the quotient rule, a
+transposed index in the gathered adjoint) fails the second or the first check in most cells.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from atjup_amd import abi, inputs
+from helpers import small_state
+
+pytestmark = pytest.mark.gpu
+
+SAT_IN = {0: ("h2o", "h2o_cloud", "h2o_ice"), 1: ("nh3", "nh3_cloud", "nh3_ice"), 2: ("ch4", "ch4_cloud", "ch4_ice")}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from atjup_amd.lib import Context
+    st = small_state("full", seed=5)
+    G, Cn, K, _ = abi.make_profile("full", sweeps=8, grid=st.grid)
+    c = Context(G, Cn, K, abi.PRECISION_FP64)
+    c.load_state(st)
+    c.bc_after_step()
+    c.swap_nlevel()
+    c.mixture_density()
+    yield c, st
+    c.close()
+
+
+def _dot_rel(lhs_terms, rhs_terms):
+    lhs = sum(float(np.sum(a)) for a in lhs_terms)
+    rhs = sum(float(np.sum(a)) for a in rhs_terms)
+    scale = sum(float(np.sum(np.abs(a))) for a in lhs_terms) + sum(float(np.sum(np.abs(a))) for a in rhs_terms)
+    return abs(lhs - rhs) / max(scale, 1e-300), lhs, rhs
+
+
+def _fd_check(tan, fd, tag, tol=1e-5, max_flip=0.005):
+    """Cells where |fd - tan| exceeds tol x (|tan| + 1e-6 x the field's largest |tan|): branch flips."""
+    bad, total = 0, 0
+    for r, (t, f) in enumerate(zip(tan, fd)):
+        scale = float(np.max(np.abs(t)))
+        err = np.abs(f - t) / (np.abs(t) + 1e-6 * scale + 1e-300)
+        bad += int(np.count_nonzero(err > tol))
+        total += t.size
+        good = err <= tol
+        print(f"{tag} output {r}: {int(np.count_nonzero(~good))} of {t.size} cells off by > {tol:g} "
+              f"(branch flips), worst elsewhere {float(np.max(err[good])) if good.any() else 0.0:.2e}")
+    assert bad <= max_flip * total, (tag, bad, total)
+
+
+@pytest.mark.parametrize("gas", [0, 1, 2])
+def test_satadj_tangent_adjoint_and_differences(ctx, gas):
+    c, st = ctx
+    names = SAT_IN[gas] + ("t", "p_stat", "rho_mix")
+    x0 = [c.get(n) for n in names]
+    rng = np.random.default_rng(100 + gas)
+    # a relative direction: each input perturbed in proportion to its own size, so that a species
+    # at exactly zero stays zero (a zero and a tiny positive amount take different branches: the
+    # entry test against ice or liquid saturation, the Bergeron switch)
+    dx = [rng.standard_normal(a.shape) * np.abs(a) for a in x0]
+    ly = [rng.standard_normal(x0[0].shape) for _ in range(4)]
+    dy = c.satadj_tangent(gas, dx)
+    lx = c.satadj_adjoint(gas, ly)
+    rel, lhs, rhs = _dot_rel([a * b for a, b in zip(dy, ly)], [a * b for a, b in zip(dx, lx)])
+    print(f"gas {gas}: <J dx, ly> = {lhs:.15e}, <dx, J^T ly> = {rhs:.15e}, relative {rel:.2e}")
+    assert rel < 1e-12
+
+    # central differences of the forward adjustment
+    eps = 1e-7
+    outs = []
+    for sgn in (+1, -1):
+        for n, a, d in zip(names, x0, dx):
+            c.set(n, a + sgn * eps * d)
+        c.satadj(gas)
+        outs.append([c.get(n) for n in SAT_IN[gas] + ("t",)])
+    for n, a in zip(names, x0):
+        c.set(n, a)
+    fd = [(p - m) / (2 * eps) for p, m in zip(*outs)]
+    _fd_check(dy, fd, f"satadj gas {gas}")
+
+    # a solid cell: the identity, except the positivity clamp of the species
+    solid = st.mask == 1.0
+    assert solid.any()
+    for r in range(3):
+        expect = np.where(x0[r] <= 0.0, 0.0, dx[r])
+        assert np.array_equal(dy[r][solid], expect[solid])
+    assert np.array_equal(dy[3][solid], dx[3][solid])
+
+
+def test_flux_limiter_tangent_adjoint_and_differences(ctx):
+    c, st = ctx
+    names = ("nh4sh", "u", "v", "w")
+    g = st.grid
+    rng = np.random.default_rng(7)
+    # NH4SH as a seeded positive field: where it is uniform (the trace of the seeded state) the
+    # limiter sits on its kink df = 0 (the sign of the 1e-12 guard), where no derivative exists
+    nh4sh0 = c.get("nh4sh")
+    c.set("nh4sh", 1e-4 * (1.0 + rng.uniform(0.0, 1.0, g.shape)))
+    x0 = [c.get(n) for n in names]
+    # in proportion to each input, so that a velocity at exactly zero (a solid cell's), the kink of
+    # |u|, stays zero
+    dx = [rng.standard_normal(a.shape) * np.abs(a) for a in x0]
+    ly = rng.standard_normal(x0[0].shape)
+    dy = c.flux_limiter_tangent(dx)
+    lx = c.flux_limiter_adjoint(ly)
+    rel, lhs, rhs = _dot_rel([dy * ly], [a * b for a, b in zip(dx, lx)])
+    print(f"flux limiter: <J dx, ly> = {lhs:.15e}, <dx, J^T ly> = {rhs:.15e}, relative {rel:.2e}")
+    assert rel < 1e-12
+    # zero off the interior cells it writes; the seam column never enters
+    inner = np.zeros(g.shape, bool)
+    inner[1:-1, 1:-1, :-1] = True
+    assert not np.any(dy[~inner])
+    assert not np.any(lx[0][:, :, -1])
+
+    eps = 1e-7
+    base_out = c.get("fluxlim_nh4sh")
+    outs = []
+    for sgn in (+1, -1):
+        for n, a, d in zip(names, x0, dx):
+            c.set(n, a + sgn * eps * d)
+        c.flux_limiter()
+        outs.append(c.get("fluxlim_nh4sh"))
+    for n, a in zip(names, x0):
+        c.set(n, a)
+    c.set("fluxlim_nh4sh", base_out)
+    c.set("nh4sh", nh4sh0)
+    fd = (outs[0] - outs[1]) / (2 * eps)
+    _fd_check([dy[inner]], [fd[inner]], "flux limiter")
+
+
+def test_refused_outside_fp64_moist_contexts():
+    """The linearisations are FP64 and moist only: an FP32-refined context and the dry profile are
+    refused with the profile error, before anything runs."""
+    from atjup_amd.lib import Context
+    st = small_state("full", seed=5)
+    G, Cn, K, _ = abi.make_profile("full", sweeps=8, grid=st.grid)
+    c = Context(G, Cn, K, abi.PRECISION_FP32_REF)
+    c.load_state(st)
+    z = [np.zeros(st.grid.shape)] * 6
+    with pytest.raises(RuntimeError, match="FP64 context"):
+        c.satadj_tangent(0, z)
+    with pytest.raises(RuntimeError, match="FP64 context"):
+        c.flux_limiter_adjoint(z[0])
+    c.close()
+    std = small_state("dyn", seed=5)
+    G, Cn, K, _ = abi.make_profile("dyn", sweeps=8, grid=std.grid)
+    d = Context(G, Cn, K, abi.PRECISION_FP64)
+    d.load_state(std, skip=set(inputs.SPECIES))
+    with pytest.raises(RuntimeError, match="moist profile"):
+        d.flux_limiter_tangent(z[:4])
+    d.close()
