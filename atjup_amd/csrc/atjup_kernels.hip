@@ -983,37 +983,60 @@ __global__ __launch_bounds__(256) void k_shapiro_i(const T* __restrict__ in, T* 
 // sequence (so the result is bit-identical to the three launches). A thread owns SH_L levels
 // [i0, i0 + SH_L) of a (j, k) column: it forms the phi-filtered values of rows j-1, j, j+1 at those
 // levels and the two beside them, the theta-filtered values of row j from those, and the radial
-// results from three consecutive theta-filtered values. Every load is unconditional and in bounds
-// (the branches select among loaded values), so all of a thread's loads are in flight at once. The
+// results from three consecutive theta-filtered values. The workgroup first stages the raw tile it
+// reads (SH_L + 2 levels, SH_BJ + 2 rows, SH_BK + 2 columns, each clamped or wrapped as the cells
+// read it) in LDS with coalesced loads, about nine per thread instead of 54 scattered ones. The
 // field is read once and written once instead of three times each.
 constexpr int SH_BK = 64, SH_BJ = 4;
 #ifndef SH_L
 #define SH_L 4                                      // levels per thread
 #endif
+constexpr int SH_TL = SH_L + 2, SH_TJ = SH_BJ + 2, SH_TK = SH_BK + 2;
 
 template<typename T>
 __global__ __launch_bounds__(SH_BK * SH_BJ) void k_shapiro2_fused(const T* __restrict__ in, T* __restrict__ out,
         const int* __restrict__ itopo, int im, int jm, int km, int pitch, size_t plane, int nkb, int njb, int nch,
         T coeff){
+    __shared__ T tile[SH_TL][SH_TJ][SH_TK];
     const int nb = nkb * njb * nch;
     // level ranges fastest, each XCD a contiguous range of blocks: the workgroups that share a
     // tile's outer levels and rows run beside each other on one L2
     const int b = xcd_remap(blockIdx.x, nb);
     const int ch = b % nch, rest = b / nch;
     const int kb = rest % nkb, jb = rest / nkb;
-    const int k = kb * SH_BK + threadIdx.x, j = jb * SH_BJ + threadIdx.y;
-    if(k >= km || j >= jm) return;
-    const int i0 = ch * SH_L;
-    const int kk = (k == km - 1) ? 0 : k;                  // the seam copy takes the value of k = 0
+    const int k0 = kb * SH_BK, j0 = jb * SH_BJ, i0 = ch * SH_L;
+    const int n = km - 1;                                  // distinct columns of the ring
+    // tile slot (l, r, s): level clamp(i0 - 1 + l), row clamp(j0 - 1 + r), column wrap(k0 - 1 + s)
+    // over the ring (the seam k = km - 1 reads column 0 and its neighbours km - 2 and 1, as the
+    // column km - 1 + 1 = km wraps to 1)
+    // the surface levels first (the masks of the cells the thread filters), so that their loads and
+    // the tile's are in flight together
+    const int k = k0 + threadIdx.x, j = j0 + threadIdx.y;
+    const bool own = k < km && j < jm;
+    const int kc = own ? k : 0, jc = own ? j : 0;
+    const int kk = (kc == km - 1) ? 0 : kc;                // the seam copy takes the value of k = 0
     const int km1 = (kk == 0) ? km - 2 : kk - 1, kp1 = (kk == km - 2) ? 0 : kk + 1;
-    const int jm1 = max(j - 1, 0), jp1 = min(j + 1, jm - 1);
+    const int jm1 = max(jc - 1, 0), jp1 = min(jc + 1, jm - 1);
     auto surf = [&](int jj, int kx){ return max(itopo[(size_t)jj * km + kx], 0); };
-    const int jr[3] = { jm1, j, jp1 };
+    const int jr[3] = { jm1, jc, jp1 };
     int sc[3], sm[3], sp[3];
     #pragma unroll
     for(int r = 0; r < 3; r++){ sc[r] = surf(jr[r], kk); sm[r] = surf(jr[r], km1); sp[r] = surf(jr[r], kp1); }
-    const int tm = surf(jm1, k), tc = surf(j, k), tp = surf(jp1, k);
+    const int tm = surf(jm1, kc), tc = surf(jc, kc), tp = surf(jp1, kc);
+    const int tid = threadIdx.y * SH_BK + threadIdx.x;
+    for(int e = tid; e < SH_TL * SH_TJ * SH_TK; e += SH_BK * SH_BJ){
+        const int s = e % SH_TK, rr = (e / SH_TK) % SH_TJ, l = e / (SH_TK * SH_TJ);
+        const int i = min(max(i0 - 1 + l, 0), im - 1);
+        const int jj = min(max(j0 - 1 + rr, 0), jm - 1);
+        const int kx = ((k0 - 1 + s) % n + n) % n;
+        tile[l][rr][s] = in[((size_t)i * jm + jj) * pitch + kx];
+    }
+    __syncthreads();
+    if(!own) return;
     const T two = T(2);
+    // the thread's column sits at slot threadIdx.x + 1 (its kk), with km1 and kp1 beside it; its
+    // rows j - 1, j, j + 1 at slots threadIdx.y .. threadIdx.y + 2
+    const int sk = threadIdx.x + 1, sj = threadIdx.y;
     // theta-filtered values of row j at levels i0 - 1 .. i0 + SH_L (clamped into the column; the
     // clamped ones are never used)
     T th[SH_L + 2];
@@ -1021,24 +1044,20 @@ __global__ __launch_bounds__(SH_BK * SH_BJ) void k_shapiro2_fused(const T* __res
     for(int l = 0; l < SH_L + 2; l++){
         const int i = min(max(i0 - 1 + l, 0), im - 1);
         T ph[3];
+        // the filtered values are formed unconditionally and selected
         #pragma unroll
         for(int r = 0; r < 3; r++){                        // phi pass of row jr[r] (k_shapiro_k)
-            const size_t r0 = ((size_t)i * jm + jr[r]) * pitch;
-            const T c = in[r0 + kk], am = in[r0 + km1], ap = in[r0 + kp1];
-            if(i < sc[r]) ph[r] = c;
-            else {
-                const T a = (i >= sm[r]) ? am : c;
-                const T bb = (i >= sp[r]) ? ap : c;
-                ph[r] = c + coeff * (a - two * c + bb);
-            }
+            const T c = tile[l][sj + r][sk], am = tile[l][sj + r][sk - 1], ap = tile[l][sj + r][sk + 1];
+            const T a = (i >= sm[r]) ? am : c;
+            const T bb = (i >= sp[r]) ? ap : c;
+            const T f = c + coeff * (a - two * c + bb);
+            ph[r] = (i < sc[r]) ? c : f;
         }
         const T c = ph[1];                                 // theta pass (k_shapiro_j)
-        if(i < tc) th[l] = c;
-        else {
-            const T a = (i >= tm) ? ph[0] : c;
-            const T bb = (i >= tp) ? ph[2] : c;
-            th[l] = c + coeff * (a - two * c + bb);
-        }
+        const T a = (i >= tm) ? ph[0] : c;
+        const T bb = (i >= tp) ? ph[2] : c;
+        const T f = c + coeff * (a - two * c + bb);
+        th[l] = (i < tc) ? c : f;
     }
     // radial pass (k_shapiro_i, order 2)
     const int i_surf = tc;
@@ -1048,13 +1067,10 @@ __global__ __launch_bounds__(SH_BK * SH_BJ) void k_shapiro2_fused(const T* __res
         const int i = i0 - 1 + l;
         if(i >= im) break;
         const T cur = th[l];
-        T res = cur;
-        if(i >= i_surf){
-            const T lo = (i - 1 >= i_surf) ? th[l - 1] : cur;
-            const T hi = (i + 1 <= im - 1) ? th[l + 1] : cur;
-            res = cur + coeff * (lo - two * cur + hi);
-        }
-        out[col + (size_t)i * plane] = res;
+        const T lo = (i - 1 >= i_surf) ? th[l - 1] : cur;
+        const T hi = (i + 1 <= im - 1) ? th[l + 1] : cur;
+        const T f = cur + coeff * (lo - two * cur + hi);
+        out[col + (size_t)i * plane] = (i >= i_surf) ? f : cur;
     }
 }
 
@@ -1232,8 +1248,17 @@ __global__ __launch_bounds__(64) void k_bc_t_ambient(T* __restrict__ t, const ui
     for(int k = threadIdx.x; k < km; k += 64){ sv[k] = (A)t[r0 + k]; sm[k] = msk[r0 + k]; }
     __syncthreads();
     if(threadIdx.x == 0){
+        // the sum in the reference's order; the LDS reads issued eight at a time ahead of the adds
         A s = 0.0; int cnt = 0;
-        for(int k = 0; k < km; k++){
+        int k = 0;
+        for(; k + 8 <= km; k += 8){
+            A v[8]; uint8_t m[8];
+            #pragma unroll
+            for(int u = 0; u < 8; u++){ v[u] = sv[k + u]; m[u] = sm[k + u]; }
+            #pragma unroll
+            for(int u = 0; u < 8; u++) if(!m[u]){ s += v[u]; cnt++; }
+        }
+        for(; k < km; k++){
             if(sm[k]) continue;
             s += sv[k]; cnt++;
         }
