@@ -105,11 +105,20 @@ __global__ __launch_bounds__(256) void k_hydrostatic(const T* __restrict__ v, co
     A prev = 0.0;
     ph[q0 + i0 * plane] = (T)prev;
     A fprev = f_(i0);
-    for(int i = i0 + 1; i <= i1; i++){
-        const A fi = f_(i);
-        prev = prev + 0.5 * (fprev + fi) * dr;
-        ph[q0 + i * plane] = (T)prev;
-        fprev = fi;
+    // the integrands of eight levels formed ahead of the in-order sum (their loads in flight
+    // together; a level past i1 reads i1 again and is not used)
+    for(int base = i0 + 1; base <= i1; base += 8){
+        A fv[8];
+        #pragma unroll
+        for(int u = 0; u < 8; u++) fv[u] = f_(min(base + u, i1));
+        #pragma unroll
+        for(int u = 0; u < 8; u++){
+            const int i = base + u;
+            if(i > i1) break;
+            prev = prev + 0.5 * (fprev + fv[u]) * dr;
+            ph[q0 + i * plane] = (T)prev;
+            fprev = fv[u];
+        }
     }
 }
 
