@@ -168,3 +168,64 @@ def test_refused_outside_fp64_moist_contexts():
     with pytest.raises(RuntimeError, match="moist profile"):
         d.flux_limiter_tangent(z[:4])
     d.close()
+
+
+def test_three_gas_chain_is_the_physics_block_adjustment(ctx):
+    """The physics block adjusts H2O, NH3 and CH4 in turn on a shared t (the fused three-gas kernel
+    stores what three per-gas launches store). Chained, the per-gas tangents (each at the state the
+    previous gas left) are the derivative of that sequence, against central differences of the
+    three forward launches; the adjoints chained in reverse (t's adjoint handed back from gas to
+    gas, those of p_stat and rho_mix summed) are their transpose."""
+    c, _ = ctx
+    species = SAT_IN[0] + SAT_IN[1] + SAT_IN[2]
+    names = species + ("t", "p_stat", "rho_mix")
+    x0 = {n: c.get(n) for n in names}
+    rng = np.random.default_rng(31)
+    dx = {n: rng.standard_normal(a.shape) * np.abs(a) for n, a in x0.items()}
+    ly = {n: rng.standard_normal(x0["t"].shape) for n in species + ("t",)}
+
+    def load(state):
+        for n, a in state.items():
+            c.set(n, a)
+
+    # the linearisation points: the state before each gas
+    states = [dict(x0)]
+    load(x0)
+    for gas in range(2):
+        c.satadj(gas)
+        states.append({n: c.get(n) for n in names})
+    # tangent chain
+    dy, dt = {}, dx["t"]
+    for gas in range(3):
+        load(states[gas])
+        v, cl, ic = SAT_IN[gas]
+        out = c.satadj_tangent(gas, [dx[v], dx[cl], dx[ic], dt, dx["p_stat"], dx["rho_mix"]])
+        dy[v], dy[cl], dy[ic] = out[0], out[1], out[2]
+        dt = out[3]
+    dy["t"] = dt
+    # adjoint chain, in reverse
+    lx = {"p_stat": 0.0, "rho_mix": 0.0}
+    lt = ly["t"]
+    for gas in (2, 1, 0):
+        load(states[gas])
+        v, cl, ic = SAT_IN[gas]
+        back = c.satadj_adjoint(gas, [ly[v], ly[cl], ly[ic], lt])
+        lx[v], lx[cl], lx[ic] = back[0], back[1], back[2]
+        lt = back[3]
+        lx["p_stat"] = lx["p_stat"] + back[4]
+        lx["rho_mix"] = lx["rho_mix"] + back[5]
+    lx["t"] = lt
+    rel, lhs, rhs = _dot_rel([dy[n] * ly[n] for n in ly], [dx[n] * lx[n] for n in names])
+    print(f"three-gas chain: <J dx, ly> = {lhs:.15e}, <dx, J^T ly> = {rhs:.15e}, relative {rel:.2e}")
+    assert rel < 1e-12
+    # central differences of the three forward launches
+    eps = 1e-7
+    outs = []
+    for sgn in (+1, -1):
+        load({n: x0[n] + sgn * eps * dx[n] for n in names})
+        for gas in range(3):
+            c.satadj(gas)
+        outs.append({n: c.get(n) for n in species + ("t",)})
+    load(x0)
+    fd = [(outs[0][n] - outs[1][n]) / (2 * eps) for n in species + ("t",)]
+    _fd_check([dy[n] for n in species + ("t",)], fd, "three-gas chain")
