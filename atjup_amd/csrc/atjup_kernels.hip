@@ -35,17 +35,21 @@ __global__ __launch_bounds__(BRL_BLOCK) void k_brl_partial(const T* __restrict__
     const int per = (ncell + BRL_NCH - 1) / BRL_NCH;
     const int c0 = ch * per, c1 = min(ncell, c0 + per);
     A s = 0.0, w = 0.0;
+    // every load unconditional and the skipped cells' terms left out by selection, so that a
+    // thread's loads are in flight together; the sums run in the same order as a skipping loop
     for(int c = c0 + threadIdx.x; c < c1; c += BRL_BLOCK){
         const int j = c / km, k = c - j * km;
         const size_t q = ((size_t)i * jm + j) * pitch + k;
-        if(flags[q] & FL_SELF) continue;
+        const bool solid = flags[q] & FL_SELF;
         const A tq = (A)t[q];
-        if(!(tq > 0.0)) continue;
-        const A b = g * (A)ps[q] / (rR * tq * t_ref);
-        if(!isfinite(b)) continue;
+        const A pq = (A)ps[q];
         const double wgt = sin_the[j];
-        s += wgt * b;
-        w += wgt;
+        const bool warm = tq > 0.0;
+        const A b = g * pq / (rR * (warm ? tq : A(1.0)) * t_ref);
+        if(!solid && warm && isfinite(b)){
+            s += wgt * b;
+            w += wgt;
+        }
     }
     __shared__ A sh[2][BRL_BLOCK];
     sh[0][threadIdx.x] = s;
