@@ -1786,12 +1786,24 @@ __global__ __launch_bounds__(RAD_BK) void k_rad_columns(const RadFields<T> F, co
     const size_t q0 = (size_t)j * pitch + k;             // the column's cell at i = 0
     double* fd = s_fd + threadIdx.x;                     // fd[i * RAD_BK]
     // downward sweep: Fd[i] = Fd[i+1] (1 - eps_i) + eps_i B_i, Fd[i_top + 1] = 0 (space is cold)
+    // (both sweeps load eight levels ahead of their in-order recurrences, so that the loads are in
+    // flight together; a level read early is one no earlier level writes)
     double d = 0.0;
     fd[(i_top + 1) * RAD_BK] = 0.0;
-    for(int i = i_top; i >= i_base; i--){
-        const double e = F.e64[q0 + i * P], b = F.b64[q0 + i * P];
-        d = d * (1.0 - e) + e * b;
-        fd[i * RAD_BK] = d;
+    for(int i0 = i_top; i0 >= i_base; i0 -= 8){
+        double ev[8], bv[8];
+        #pragma unroll
+        for(int u = 0; u < 8; u++){
+            const int i = max(i0 - u, i_base);
+            ev[u] = F.e64[q0 + i * P]; bv[u] = F.b64[q0 + i * P];
+        }
+        #pragma unroll
+        for(int u = 0; u < 8; u++){
+            const int i = i0 - u;
+            if(i < i_base) break;
+            d = d * (1.0 - ev[u]) + ev[u] * bv[u];
+            fd[i * RAD_BK] = d;
+        }
     }
     // the deep boundary: net upward F_int plus the shortwave left at depth, or (bottom = 1) no
     // divergence in the bottom layer
@@ -1801,20 +1813,32 @@ __global__ __launch_bounds__(RAD_BK) void k_rad_columns(const RadFields<T> F, co
     else u = fd[i_base * RAD_BK] + C.F_int + fsw;
     // upward sweep with the layer results
     const double fsw_top = fsun[j] * exp(-C.sw_tau_bar * dmax(0.0, 0.0));
-    for(int i = i_base; i <= i_top; i++){
-        const size_t q = q0 + i * P;
-        const double e = F.e64[q], b = F.b64[q];
-        const double u_next = u * (1.0 - e) + e * b;
-        const double net_bot = u - fd[i * RAD_BK];
-        const double net_top = u_next - fd[(i + 1) * RAD_BK];
-        const double dz = dz_tab[i];
-        const double fsw_next = (i < i_top) ? F.s64[q + P] : fsw_top;
-        const double Q_sw = (fsw_next - fsw) / dz;
-        if constexpr(!std::is_same<T, double>::value) F.eps[q] = (T)e;
-        F.rad[q] = (T)(0.5 * (net_bot + net_top));
-        F.qrad[q] = (T)((net_bot - net_top) / dz + Q_sw);
-        u = u_next;
-        fsw = fsw_next;
+    for(int i0 = i_base; i0 <= i_top; i0 += 8){
+        double ev[8], bv[8], sv[8];
+        #pragma unroll
+        for(int v = 0; v < 8; v++){
+            const int i = min(i0 + v, i_top);
+            ev[v] = F.e64[q0 + i * P]; bv[v] = F.b64[q0 + i * P];
+            sv[v] = F.s64[q0 + min(i + 1, i_top) * P];
+        }
+        #pragma unroll
+        for(int v = 0; v < 8; v++){
+            const int i = i0 + v;
+            if(i > i_top) break;
+            const size_t q = q0 + i * P;
+            const double e = ev[v], b = bv[v];
+            const double u_next = u * (1.0 - e) + e * b;
+            const double net_bot = u - fd[i * RAD_BK];
+            const double net_top = u_next - fd[(i + 1) * RAD_BK];
+            const double dz = dz_tab[i];
+            const double fsw_next = (i < i_top) ? sv[v] : fsw_top;
+            const double Q_sw = (fsw_next - fsw) / dz;
+            if constexpr(!std::is_same<T, double>::value) F.eps[q] = (T)e;
+            F.rad[q] = (T)(0.5 * (net_bot + net_top));
+            F.qrad[q] = (T)((net_bot - net_top) / dz + Q_sw);
+            u = u_next;
+            fsw = fsw_next;
+        }
     }
     // the solid cells below take the deepest fluid values, Q_rad 0
     const T e_base = F.eps[q0 + i_base * P], r_base = F.rad[q0 + i_base * P];
